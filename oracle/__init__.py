@@ -1,0 +1,164 @@
+"""ctypes front end of the FP64 CPU oracle (liboracle.so).
+
+Test infrastructure only: imported by tests/, the smoke check and bench.py's
+CPU-baseline leg as the checker.  The product package never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Box(C.Structure):
+    _fields_ = [("kind", C.c_int), ("xbox", C.c_double), ("ybox", C.c_double), ("zbox", C.c_double),
+                ("alpha", C.c_double), ("beta", C.c_double), ("gamma", C.c_double)]
+
+
+class MpoleSys(C.Structure):
+    _fields_ = [("n", C.c_int), ("x", _dp), ("y", _dp), ("z", _dp), ("pole", _dp),
+                ("polaxe", _ip), ("zaxis", _ip), ("xaxis", _ip), ("yaxis", _ip),
+                ("use", _ip), ("molcule", _ip),
+                ("n12", _ip), ("i12", _ip), ("n13", _ip), ("i13", _ip),
+                ("n14", _ip), ("i14", _ip), ("n15", _ip), ("i15", _ip),
+                ("m2scale", C.c_double), ("m3scale", C.c_double), ("m4scale", C.c_double),
+                ("m5scale", C.c_double), ("f", C.c_double), ("off2", C.c_double), ("box", Box),
+                ("pentype", C.c_int), ("pcore", _dp), ("pval0", _dp), ("palpha", _dp), ("mono0", _dp),
+                ("use_chgflx", C.c_int),
+                ("nbond", C.c_int), ("ibnd", _ip), ("bl", _dp), ("bflx", _dp),
+                ("nangle", C.c_int), ("iang", _ip), ("anat", _dp), ("aflx", _dp), ("abflx", _dp),
+                ("balist", _ip)]
+
+
+class MpoleOut(C.Structure):
+    _fields_ = [("em", C.c_double), ("einter", C.c_double), ("nem", C.c_long),
+                ("aem", _dp), ("dem", _dp), ("vir", _dp), ("rpole", _dp), ("pole_out", _dp)]
+
+
+ENERGY, ANALYSIS, GRADIENT, VIRIAL = 0, 1, 2, 3
+BOX_NONE, BOX_ORTHO, BOX_MONO, BOX_TRI, BOX_OCTA, BOX_DODECA = range(6)
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"oracle library missing: build it with `make -C {HERE}`")
+        _lib = C.CDLL(LIB)
+        _lib.orc_empole.argtypes = [C.POINTER(MpoleSys), C.c_int, C.POINTER(MpoleOut)]
+        _lib.orc_empole.restype = C.c_int
+        _lib.orc_image.argtypes = [C.POINTER(Box), _dp, _dp, _dp]
+        _lib.orc_pair_mpole.argtypes = [C.c_double] * 6 + [_dp, _dp, _dp]
+    return _lib
+
+
+def box_of(fx) -> Box:
+    """Box from a fixture's reference globals (boxes.h flags)."""
+    if not int(fx["use_bounds"]):
+        kind = BOX_NONE
+    elif int(fx["octahedron"]):
+        kind = BOX_OCTA
+    elif int(fx["dodecadron"]):
+        kind = BOX_DODECA
+    elif int(fx["triclinic"]):
+        kind = BOX_TRI
+    elif int(fx["monoclinic"]):
+        kind = BOX_MONO
+    else:
+        kind = BOX_ORTHO
+    return Box(kind, float(fx["xbox"]), float(fx["ybox"]), float(fx["zbox"]),
+               float(fx["alpha"]), float(fx["beta"]), float(fx["gamma"]))
+
+
+class _Keep:
+    """Holds the numpy arrays a ctypes struct points into."""
+
+    def __init__(self):
+        self.arrays = []
+
+    def d(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self.arrays.append(a)
+        return a.ctypes.data_as(_dp)
+
+    def i(self, a):
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        self.arrays.append(a)
+        return a.ctypes.data_as(_ip)
+
+
+def mpole_sys(fx, keep: _Keep, x=None, yaxis=None) -> MpoleSys:
+    """MpoleSys over a fixture dict (keys as written by oracle/ref_dump.cpp)."""
+    n = int(fx["n"])
+    xyz = (fx["x"], fx["y"], fx["z"]) if x is None else (x[:, 0], x[:, 1], x[:, 2])
+    cutoff = float(fx["mpolecut"])
+    s = MpoleSys()
+    s.n = n
+    s.x, s.y, s.z = (keep.d(v) for v in xyz)
+    s.pole = keep.d(fx["pole"])
+    s.polaxe, s.zaxis, s.xaxis = keep.i(fx["polaxe"]), keep.i(fx["zaxis"]), keep.i(fx["xaxis"])
+    ya = np.array(fx["yaxis"] if yaxis is None else yaxis, dtype=np.int32)
+    s.yaxis = keep.i(ya)
+    s.use, s.molcule = keep.i(fx["use"]), keep.i(fx["molcule"])
+    for a in ("12", "13", "14", "15"):
+        setattr(s, "n" + a, keep.i(fx["n" + a]))
+        setattr(s, "i" + a, keep.i(fx["i" + a] if len(fx["i" + a]) else np.zeros(1)))
+    s.m2scale, s.m3scale = float(fx["m2scale"]), float(fx["m3scale"])
+    s.m4scale, s.m5scale = float(fx["m4scale"]), float(fx["m5scale"])
+    s.f = float(fx["electric"]) / float(fx["dielec"])
+    s.off2 = cutoff * cutoff
+    s.box = box_of(fx)
+    s.pentype = int(fx["pentype"])
+    s.pcore, s.pval0 = keep.d(fx["pcore"]), keep.d(fx["pval0"])
+    s.palpha, s.mono0 = keep.d(fx["palpha"]), keep.d(fx["mono0"])
+    s.use_chgflx = int(fx["use_chgflx"])
+    s.nbond = int(fx["nbond"])
+    s.ibnd, s.bl = keep.i(fx["ibnd"]), keep.d(fx["bl"])
+    s.bflx = keep.d(fx["bflx"] if len(fx["bflx"]) else np.zeros(1))
+    s.nangle = int(fx["nangle"])
+    s.iang, s.anat = keep.i(fx["iang"] if len(fx["iang"]) else np.zeros(1)), keep.d(
+        fx["anat"] if len(fx["anat"]) else np.zeros(1))
+    s.aflx = keep.d(fx["aflx"] if len(fx["aflx"]) else np.zeros(1))
+    s.abflx = keep.d(fx["abflx"] if len(fx["abflx"]) else np.zeros(1))
+    s.balist = keep.i(fx["balist"] if len(fx["balist"]) else np.zeros(1))
+    return s
+
+
+def empole(fx, mode: int = VIRIAL, x=None) -> dict:
+    """Run the oracle's empole on a fixture; returns em/einter/nem/aem/dem/vir/rpole."""
+    keep = _Keep()
+    s = mpole_sys(fx, keep, x=x)
+    n = s.n
+    aem = np.zeros(n)
+    dem = np.zeros(3 * n)
+    vir = np.zeros(9)
+    rpole = np.zeros(13 * n)
+    o = MpoleOut()
+    o.aem, o.dem = aem.ctypes.data_as(_dp), dem.ctypes.data_as(_dp)
+    o.vir, o.rpole = vir.ctypes.data_as(_dp), rpole.ctypes.data_as(_dp)
+    o.pole_out = None
+    rc = lib().orc_empole(C.byref(s), mode, C.byref(o))
+    if rc != 0:
+        raise RuntimeError(f"orc_empole failed ({rc})")
+    return {"em": o.em, "einter": o.einter, "nem": o.nem, "aem": aem, "dem": dem.reshape(n, 3),
+            "vir": vir.reshape(3, 3), "rpole": rpole.reshape(n, 13)}
+
+
+def pair_mpole(r, mscale, f, mi, mk):
+    """One undamped multipole pair: (e, frc[3], trqi[3], trqk[3], vir[6])."""
+    out = np.zeros(16)
+    mi = np.ascontiguousarray(mi, dtype=np.float64)
+    mk = np.ascontiguousarray(mk, dtype=np.float64)
+    r = np.asarray(r, dtype=np.float64)
+    lib().orc_pair_mpole(float(r @ r), float(r[0]), float(r[1]), float(r[2]), float(mscale), float(f),
+                         mi.ctypes.data_as(_dp), mk.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+    return out

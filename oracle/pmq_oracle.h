@@ -1,0 +1,92 @@
+/*
+ * pmq_oracle.h -- FP64 CPU restatement of the polarizable-electrostatics path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the GPU path is
+ * compared against; it is never linked into, loaded by or called from the
+ * product library (polmdqc_amd/libpmq.so).  Only tests/, the smoke check and
+ * bench.py's CPU-baseline leg may use it.
+ *
+ * Part 1 restates the reference's multipole energy (src/md/empole.cpp:96-357
+ * and what it calls: rotpole.cpp, chkpole.cpp, torque.cpp, image.cpp,
+ * alterchg.cpp, dcflux.cpp, include/md/pairMpole.h, damping.h) on plain
+ * arrays with the reference's conventions (1-based frame atoms, 0 = none,
+ * signed yaxis for chirality, use[] of length n+1).
+ *
+ * Part 2 (pmq_oracle_ewald.c) adds what the reference declares but does not
+ * compute: Ewald real space, PME, polarization (direct field, mutual PCG,
+ * energy and gradient) and the HIPPO terms.
+ */
+#ifndef PMQ_ORACLE_H
+#define PMQ_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* periodic box, boxes.h:46-67 conventions */
+enum { ORC_BOX_NONE = 0, ORC_BOX_ORTHO, ORC_BOX_MONO, ORC_BOX_TRI, ORC_BOX_OCTA, ORC_BOX_DODECA };
+
+typedef struct {
+    int kind;
+    double xbox, ybox, zbox;
+    double alpha, beta, gamma; /* degrees */
+} orc_box;
+
+/* frame types, kmulti.h:23-31 */
+enum { ORC_FRAME_NONE = 0, ORC_FRAME_ZTHENX, ORC_FRAME_ZONLY, ORC_FRAME_BISECTOR,
+       ORC_FRAME_ZBISECT, ORC_FRAME_THREEFOLD };
+
+typedef struct {
+    int n;
+    const double *x, *y, *z;
+    const double *pole;        /* n*13 local-frame multipoles (kmpole units) */
+    const int *polaxe;         /* n */
+    const int *zaxis, *xaxis;  /* n, 1-based, 0 = none */
+    int *yaxis;                /* n, 1-based signed; chkpole may flip the sign */
+    const int *use;            /* n+1, use[0] = 0 */
+    const int *molcule;        /* n */
+    const int *n12, *i12, *n13, *i13, *n14, *i14, *n15, *i15; /* counts and flat lists */
+    double m2scale, m3scale, m4scale, m5scale;
+    double f;                  /* electric / dielec */
+    double off2;               /* squared cutoff */
+    orc_box box;
+    int pentype;               /* 0 none, 1 Gordon1, 2 Gordon2 (mplpot.h:25) */
+    const double *pcore, *pval0, *palpha, *mono0; /* n each (charge penetration) */
+    int use_chgflx;
+    int nbond;  const int *ibnd; const double *bl, *bflx;          /* 2*nbond, nbond, nbond */
+    int nangle; const int *iang; const double *anat, *aflx, *abflx; /* 3*nangle, nangle, 2*, 2* */
+    const int *balist;         /* 2*nangle: bond indices of the angle's two bonds */
+} orc_mpole_sys;
+
+/* calculation modes, calcMode.h:35-49 */
+enum { ORC_ENERGY = 0, ORC_ANALYSIS = 1, ORC_GRADIENT = 2, ORC_VIRIAL = 3 };
+
+typedef struct {
+    double em;
+    double einter;
+    long nem;
+    double *aem;    /* n, analysis mode (may be NULL) */
+    double *dem;    /* 3n AoS, gradient modes (may be NULL) */
+    double *vir;    /* 9, row-major (may be NULL) */
+    double *rpole;  /* 13n global-frame multipoles (may be NULL) */
+    double *pole_out; /* 13n local multipoles after chkpole/charge flux (may be NULL) */
+} orc_mpole_out;
+
+/* empole_a<Mode, PenType, use_cf>: empole.cpp:96-357 */
+int orc_empole(orc_mpole_sys *s, int mode, orc_mpole_out *out);
+
+/* minimum image of one displacement, image.cpp:37-124 */
+void orc_image(const orc_box *b, double *xr, double *yr, double *zr);
+
+/* rotpole(Mpole) for all atoms: rotpole.cpp:22-49 (after chkpole) */
+void orc_rotpole(const orc_mpole_sys *s, const double *pole, double *rpole);
+
+/* one pair of the undamped point-multipole interaction, pairMpole.h:21-175;
+ * out = {e, frc[3], trqi[3], trqk[3], vir[6]} */
+void orc_pair_mpole(double r2, double xr, double yr, double zr, double mscale, double f,
+                    const double *mi, const double *mk, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
