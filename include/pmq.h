@@ -1,0 +1,182 @@
+/*
+ * pmq.h -- C ABI of the MI355X polarizable-electrostatics library (libpmq.so).
+ *
+ * The reference has no FFI: its hot path is a set of argument-less C++
+ * templates over global arrays, `energy<CalcMode>()` (src/md/energy.cpp:31)
+ * calling `empole<CalcMode>()` (src/md/empole.cpp:50) and, by Tinker
+ * convention, `epolar/erepel/edisp<CalcMode>()` (energy.cpp:267,279,280).
+ * This header is the boundary those bodies cross: plain pointers and sizes,
+ * no C++ or torch types.  A host shim that keeps the reference's globals
+ * (polmdqc_amd/csrc/host/) fills these structs from them and maps the return
+ * codes onto the reference's fatal() (fatal.cpp:21-28).
+ *
+ * Conventions (identical to the reference's globals):
+ *   - atoms are 0-based in arrays; frame atoms (zaxis/xaxis/yaxis) are 1-based
+ *     with 0 meaning "none" and the sign of yaxis carrying chirality
+ *     (mpole.h:30-41, chkpole.cpp:44-89);
+ *   - pole is n*13 per atom in the local frame with kmpole's units
+ *     (dipoles x bohr, quadrupoles x bohr^2/3, kmpole.cpp:543-552);
+ *   - use[] has n+1 entries with use[0] = 0 (active.cpp:41,51);
+ *   - gradients are AoS double[3n] in kcal/mol/A, virial double[9] row-major.
+ *
+ * Every function returns 0 on success or a negative PMQ_ERR_* code, and never
+ * exits; pmq_last_error() describes the last failure.  One context per host
+ * thread; calls on one context are not re-entrant; no caller pointer is kept
+ * past a call.
+ */
+#ifndef PMQ_H
+#define PMQ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PMQ_ABI_VERSION 1
+
+#if defined(__GNUC__) || defined(__clang__)
+#define PMQ_API __attribute__((visibility("default")))
+#else
+#define PMQ_API
+#endif
+
+enum {
+    PMQ_OK = 0,
+    PMQ_ERR_ARG = -1,         /* bad argument or size */
+    PMQ_ERR_HIP = -2,         /* a HIP runtime call failed */
+    PMQ_ERR_STATE = -3,       /* call out of order (e.g. compute before set_params) */
+    PMQ_ERR_UNSUPPORTED = -4, /* option not built */
+    PMQ_ERR_NAN = -5,         /* non-finite total energy (energy.cpp:329-332) */
+    PMQ_ERR_NOCONV = -6       /* induced dipoles did not converge in politer steps */
+};
+
+/* CalcMode, include/md/calcMode.h:18-49 */
+enum { PMQ_ENERGY = 0, PMQ_ANALYSIS = 1, PMQ_GRADIENT = 2, PMQ_VIRIAL = 3 };
+
+/* energy terms (bit mask) */
+enum {
+    PMQ_TERM_MPOLE = 1u << 0, /* empole: permanent multipoles */
+    PMQ_TERM_POLAR = 1u << 1, /* epolar: induced dipoles (mutual, PCG) */
+    PMQ_TERM_REPEL = 1u << 2, /* erepel: HIPPO Pauli repulsion */
+    PMQ_TERM_DISP = 1u << 3   /* edisp:  HIPPO damped dispersion */
+};
+
+/* arithmetic of the pair/PME kernels */
+enum {
+    PMQ_PLAIN_FP64 = 0, /* FP64 throughout (parity form) */
+    PMQ_MIXED = 1       /* FP32 pair/PME math, int64 fixed-point forces, FP64 reductions */
+};
+
+/* periodic box, boxes.h:46-67 */
+enum { PMQ_BOX_NONE = 0, PMQ_BOX_ORTHO, PMQ_BOX_MONO, PMQ_BOX_TRI, PMQ_BOX_OCTA, PMQ_BOX_DODECA };
+
+/* local frame types, kmulti.h:23-31 */
+enum { PMQ_FRAME_NONE = 0, PMQ_FRAME_ZTHENX, PMQ_FRAME_ZONLY, PMQ_FRAME_BISECTOR, PMQ_FRAME_ZBISECT,
+       PMQ_FRAME_THREEFOLD };
+
+/* charge penetration, mplpot.h:25 */
+enum { PMQ_PEN_NONE = 0, PMQ_PEN_GORDON1 = 1, PMQ_PEN_GORDON2 = 2 };
+
+typedef struct pmq_ctx pmq_ctx;
+
+typedef struct {
+    int device; /* HIP device ordinal */
+} pmq_config;
+
+typedef struct {
+    int kind;                  /* PMQ_BOX_* */
+    double xbox, ybox, zbox;   /* A */
+    double alpha, beta, gamma; /* degrees */
+} pmq_box;
+
+/* Static topology: replaces what mechanic() leaves in couple.h, mpole.h,
+ * usage.h, molcul.h, polgrp.h, bndstr.h and angbnd.h. */
+typedef struct {
+    int n;
+    const int *polaxe;                 /* n, PMQ_FRAME_* */
+    const int *zaxis, *xaxis, *yaxis;  /* n, 1-based, signed yaxis */
+    const int *use;                    /* n+1, or NULL = every atom active */
+    const int *molcule;                /* n, or NULL = one molecule */
+    /* 1-2 .. 1-5 connectivity: counts n1x[n] and flat 0-based lists */
+    const int *n12, *i12, *n13, *i13, *n14, *i14, *n15, *i15;
+    /* polarization groups (polgrp.h:28-39), counts and flat lists, may be NULL */
+    const int *np11, *ip11, *np12, *ip12, *np13, *ip13, *np14, *ip14;
+    /* charge-flux geometry (bndstr.h, angbnd.h, atmlst.h) */
+    int nbond;
+    const int *ibnd;   /* 2*nbond */
+    int nangle;
+    const int *iang;   /* 3*nangle */
+    const int *balist; /* 2*nangle */
+} pmq_topology;
+
+/* Parameters: what kmpole/kpolar/kchgflx/kewald/cutoffs assign. */
+typedef struct {
+    const double *pole; /* n*13 */
+    double m2scale, m3scale, m4scale, m5scale;
+    double electric, dielec;
+    /* charge penetration (chgpen.h) */
+    int pentype;
+    const double *pcore, *pval0, *palpha; /* n each */
+    /* charge flux (cflux.h) */
+    int use_chgflx;
+    const double *bl, *bflx;            /* nbond */
+    const double *anat, *aflx, *abflx;  /* nangle, 2*nangle, 2*nangle */
+    /* polarization (polar.h, polpot.h) */
+    const double *polarity, *thole, *pdamp; /* n each */
+    double p2scale, p3scale, p4scale, p5scale;
+    double p2iscale, p3iscale, p4iscale, p5iscale;
+    double d1scale, d2scale, d3scale, d4scale;
+    double u1scale, u2scale, u3scale, u4scale;
+    double poleps; /* rms induced-dipole change, Debye */
+    int politer;
+    /* method and cutoffs (limits.h, ewald.h, pme.h) */
+    int use_ewald;
+    double mpolecut; /* non-Ewald multipole cutoff, A */
+    double ewaldcut; /* Ewald real-space cutoff, A */
+    double aewald;   /* Ewald coefficient, 1/A */
+    int nfft1, nfft2, nfft3, bsorder;
+} pmq_params;
+
+typedef struct {
+    /* energies, kcal/mol */
+    double em, ep, er, edsp, esum;
+    double einter;     /* intermolecular part of em (analysis) */
+    long long nem;     /* number of nonzero multipole pair energies (analysis) */
+    double vir[9];     /* row-major */
+    int pcg_iters;     /* iterations the induced-dipole solve took */
+    /* optional outputs (NULL = not wanted) */
+    double *dem;   /* 3n gradient of em */
+    double *dep;   /* 3n gradient of ep */
+    double *desum; /* 3n total gradient */
+    double *aem;   /* n per-atom em partition (analysis) */
+    double *rpole; /* 13n global-frame multipoles */
+    double *uind, *uinp; /* 3n induced dipoles (d and p fields) */
+} pmq_result;
+
+PMQ_API int pmq_create(const pmq_config *cfg, pmq_ctx **ctx);
+PMQ_API int pmq_destroy(pmq_ctx *ctx);
+PMQ_API const char *pmq_last_error(const pmq_ctx *ctx);
+PMQ_API int pmq_abi_version(void);
+
+/* Static state, called after mechanic() and whenever topology changes. */
+PMQ_API int pmq_set_topology(pmq_ctx *ctx, const pmq_topology *top);
+PMQ_API int pmq_set_params(pmq_ctx *ctx, const pmq_params *prm);
+
+/* One full evaluation from host coordinates into host outputs:
+ * upload x,y,z and box, compute `terms` in `mode` with `precision`, fetch. */
+PMQ_API int pmq_eval(pmq_ctx *ctx, int mode, unsigned terms, int precision,
+             const double *x, const double *y, const double *z, const pmq_box *box, pmq_result *res);
+
+/* The same evaluation split so coordinates can stay resident in HBM:
+ * pmq_set_coords uploads, pmq_compute only enqueues device work on the
+ * context's stream (asynchronous), pmq_fetch waits and copies results out. */
+PMQ_API int pmq_set_coords(pmq_ctx *ctx, const double *x, const double *y, const double *z, const pmq_box *box);
+PMQ_API int pmq_compute(pmq_ctx *ctx, int mode, unsigned terms, int precision);
+PMQ_API int pmq_fetch(pmq_ctx *ctx, pmq_result *res);
+PMQ_API int pmq_sync(pmq_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

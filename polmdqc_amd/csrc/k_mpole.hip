@@ -1,0 +1,826 @@
+// k_mpole.hip -- permanent-multipole kernels: local->global frames, the
+// block-tile pair kernel, excluded-pair corrections, torque->force and charge
+// flux.  Plain FP64 form; the reference's arithmetic is FP64 throughout
+// (precision.h:16).
+//
+//   k_rotpole      chkpole.cpp:44-89 + rotmat/rotsite (rotpole.cpp:102-425),
+//                  charge-flux monopole shift (alterchg.cpp:50-53), one
+//                  thread per atom, writes the sorted SoA multipoles
+//   k_mpole_tiles  the i<k loop of empole_a (empole.cpp:198-345) over 64x64
+//                  block tiles: every pair gets scale 1 here
+//   k_mpole_excl   adds (scale-1) x pair for the 1-2..1-5 pairs whose scale
+//                  is not 1 (empole.cpp:224-235: later classes override)
+//   k_torque       torque.cpp:42-397
+//   k_cf_delta / k_cf_grad   alterchg.cpp:36-169 / dcflux.cpp:31-225
+#include "pair_mpole.h"
+#include "pmq_internal.h"
+
+namespace pmq {
+
+__global__ void k_zero(double *__restrict__ p, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+int launch_zero(pmq_ctx *c, double *p, size_t n)
+{
+    if (!n) return PMQ_OK;
+    const int tb = 256;
+    const int gb = (int)std::min<size_t>((n + tb - 1) / tb, 4096);
+    hipLaunchKernelGGL(k_zero, dim3(gb), dim3(tb), 0, c->stream, p, n);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// local -> global frame
+struct RotArgs {
+    int n, npad;
+    const int *perm;
+    const double *x, *y, *z;
+    const double *pole;               // 13n local
+    const int *polaxe, *zaxis, *xaxis, *yaxis;
+    int cp, cf;
+    const double *pcore, *pval0, *palpha, *pdelta;
+    double *smp;                      // 10 * npad sorted
+    double *scp;                      // 3 * npad sorted: core, alpha, charge (monopole)
+    double *rpole13;                  // 13n caller order or null
+};
+
+__device__ __forceinline__ void dvec(const RotArgs &a, int from, int to, double *d)
+{
+    d[0] = a.x[to] - a.x[from];
+    d[1] = a.y[to] - a.y[from];
+    d[2] = a.z[to] - a.z[from];
+}
+
+__device__ __forceinline__ double dot3(const double *u, const double *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; }
+
+__global__ void k_rotpole(RotArgs a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.npad) return;
+    const int i = s < a.n ? a.perm[s] : -1;
+    if (i < 0) {
+        for (int c = 0; c < 10; c++) a.smp[c * a.npad + s] = 0.0;
+        for (int c = 0; c < 3; c++) a.scp[c * a.npad + s] = 0.0;
+        return;
+    }
+    double p[13];
+    for (int c = 0; c < 13; c++) p[c] = a.pole[13 * i + c];
+    double val = a.cp ? a.pval0[i] : 0.0;
+    if (a.cf) {
+        // alterchg.cpp:50-53: pole[0] = mono0 + delta, pval = pval0 + delta
+        // (the caller passes mono0 in pole[0])
+        p[0] += a.pdelta[i];
+        val += a.pdelta[i];
+    }
+    const int frame = a.polaxe[i];
+    const int iz = a.zaxis[i] - 1, ix = a.xaxis[i] - 1, ky = a.yaxis[i], iy = abs(ky) - 1;
+    // chirality check at Z-then-X sites with a y-axis atom (chkpole.cpp:60-87)
+    if (frame == PMQ_FRAME_ZTHENX && ky != 0) {
+        double ad[3], bd[3], cd[3];
+        dvec(a, iy, i, ad);
+        dvec(a, iy, iz, bd);
+        dvec(a, iy, ix, cd);
+        const double c1 = bd[1] * cd[2] - bd[2] * cd[1];
+        const double c2 = cd[1] * ad[2] - cd[2] * ad[1];
+        const double c3 = ad[1] * bd[2] - ad[2] * bd[1];
+        const double vol = ad[0] * c1 + bd[0] * c2 + cd[0] * c3;
+        if ((ky < 0 && vol > 0.0) || (ky > 0 && vol < 0.0)) {
+            p[2] = -p[2]; p[5] = -p[5]; p[7] = -p[7]; p[9] = -p[9]; p[11] = -p[11];
+        }
+    }
+    // rotation matrix rows: ax (x axis), ay, az
+    double ax[3] = {1, 0, 0}, ay[3], az[3] = {0, 0, 1}, d[3], r, dot;
+    bool planar = false;
+    if (frame == PMQ_FRAME_ZONLY) {
+        dvec(a, i, iz, d);
+        r = sqrt(dot3(d, d));
+        for (int j = 0; j < 3; j++) az[j] = d[j] / r;
+        double v[3] = {1, 0, 0};
+        dot = az[0];
+        if (fabs(dot) > 0.707) { v[0] = 0; v[1] = 1; dot = az[1]; }
+        for (int j = 0; j < 3; j++) v[j] -= dot * az[j];
+        r = sqrt(dot3(v, v));
+        for (int j = 0; j < 3; j++) ax[j] = v[j] / r;
+    } else if (frame == PMQ_FRAME_ZTHENX) {
+        dvec(a, i, iz, d);
+        r = sqrt(dot3(d, d));
+        for (int j = 0; j < 3; j++) az[j] = d[j] / r;
+        dvec(a, i, ix, d);
+        dot = dot3(d, az);
+        for (int j = 0; j < 3; j++) d[j] -= dot * az[j];
+        r = sqrt(dot3(d, d));
+        for (int j = 0; j < 3; j++) ax[j] = d[j] / r;
+    } else if (frame == PMQ_FRAME_BISECTOR) {
+        double u1[3], u2[3];
+        dvec(a, i, iz, u1);
+        dvec(a, i, ix, u2);
+        const double r1 = sqrt(dot3(u1, u1)), r2 = sqrt(dot3(u2, u2));
+        for (int j = 0; j < 3; j++) { u1[j] /= r1; u2[j] /= r2; d[j] = u1[j] + u2[j]; }
+        r = sqrt(dot3(d, d));
+        for (int j = 0; j < 3; j++) az[j] = d[j] / r;
+        dot = dot3(u2, az);
+        for (int j = 0; j < 3; j++) d[j] = u2[j] - dot * az[j];
+        r = sqrt(dot3(d, d));
+        for (int j = 0; j < 3; j++) ax[j] = d[j] / r;
+    } else if (frame == PMQ_FRAME_ZBISECT) {
+        double u1[3], u2[3];
+        dvec(a, i, iz, d);
+        r = sqrt(dot3(d, d));
+        for (int j = 0; j < 3; j++) az[j] = d[j] / r;
+        dvec(a, i, ix, u1);
+        dvec(a, i, iy, u2);
+        const double r1 = sqrt(dot3(u1, u1)), r2 = sqrt(dot3(u2, u2));
+        for (int j = 0; j < 3; j++) { u1[j] /= r1; u2[j] /= r2; d[j] = u1[j] + u2[j]; }
+        r = sqrt(dot3(d, d));
+        for (int j = 0; j < 3; j++) d[j] /= r;
+        dot = dot3(d, az);
+        if (180.0 - 57.29577951308232088 * acos(dot) < 0.0) {
+            planar = true;
+            d[0] = u1[1] * u2[2] - u1[2] * u2[1];
+            d[1] = u1[2] * u2[0] - u1[0] * u2[2];
+            d[2] = u1[0] * u2[1] - u1[1] * u2[0];
+            dot = dot3(d, az);
+            if (dot < 0.0) { for (int j = 0; j < 3; j++) d[j] = -d[j]; dot = -dot; }
+        }
+        for (int j = 0; j < 3; j++) d[j] -= dot * az[j];
+        r = sqrt(dot3(d, d));
+        for (int j = 0; j < 3; j++) ax[j] = d[j] / r;
+    } else if (frame == PMQ_FRAME_THREEFOLD) {
+        double u1[3], u2[3], u3[3];
+        dvec(a, i, iz, u1);
+        dvec(a, i, ix, u2);
+        dvec(a, i, iy, u3);
+        const double r1 = sqrt(dot3(u1, u1)), r2 = sqrt(dot3(u2, u2)), r3 = sqrt(dot3(u3, u3));
+        for (int j = 0; j < 3; j++) { u1[j] /= r1; u2[j] /= r2; u3[j] /= r3; d[j] = u1[j] + u2[j] + u3[j]; }
+        r = sqrt(dot3(d, d));
+        for (int j = 0; j < 3; j++) az[j] = d[j] / r;
+        dot = dot3(u1, az);
+        for (int j = 0; j < 3; j++) d[j] = u1[j] - dot * az[j];
+        r = sqrt(dot3(d, d));
+        for (int j = 0; j < 3; j++) ax[j] = d[j] / r;
+    }
+    ay[0] = ax[2] * az[1] - ax[1] * az[2];
+    ay[1] = ax[0] * az[2] - ax[2] * az[0];
+    ay[2] = ax[1] * az[0] - ax[0] * az[1];
+    if (planar && frame == PMQ_FRAME_ZBISECT) {
+        p[1] = 0; p[6] = 0; p[10] = 0;
+        p[4] = 0.5 * (p[4] + p[8]);
+        p[8] = p[4];
+    }
+    const double *A[3] = {ax, ay, az};
+    double g[13];
+    g[0] = p[0];
+    for (int h = 0; h < 3; h++) g[1 + h] = p[1] * ax[h] + p[2] * ay[h] + p[3] * az[h];
+    for (int h = 0; h < 3; h++)
+        for (int k = h; k < 3; k++) {
+            double v = 0.0;
+            for (int l = 0; l < 3; l++) {
+                const double al = A[l][h];
+                double t = 0.0;
+                for (int m = 0; m < 3; m++) t += A[m][k] * p[4 + 3 * l + m];
+                v += al * t;
+            }
+            g[4 + 3 * h + k] = v;
+            g[4 + 3 * k + h] = v;
+        }
+    const int np = a.npad;
+    a.smp[0 * np + s] = a.cp ? val : g[0];
+    a.smp[1 * np + s] = g[1];
+    a.smp[2 * np + s] = g[2];
+    a.smp[3 * np + s] = g[3];
+    a.smp[4 * np + s] = g[4];
+    a.smp[5 * np + s] = g[5];
+    a.smp[6 * np + s] = g[6];
+    a.smp[7 * np + s] = g[8];
+    a.smp[8 * np + s] = g[9];
+    a.smp[9 * np + s] = g[12];
+    a.scp[0 * np + s] = a.cp ? a.pcore[i] : 0.0;
+    a.scp[1 * np + s] = a.cp ? a.palpha[i] : 0.0;
+    a.scp[2 * np + s] = g[0];
+    if (a.rpole13)
+        for (int c = 0; c < 13; c++) a.rpole13[13 * i + c] = g[c];
+}
+
+int launch_rotpole(pmq_ctx *c, bool want_rpole13)
+{
+    RotArgs a;
+    a.n = c->n;
+    a.npad = c->npad;
+    a.perm = c->perm.p;
+    a.x = c->x.p; a.y = c->y.p; a.z = c->z.p;
+    a.pole = c->pole.p;
+    a.polaxe = c->polaxe.p; a.zaxis = c->zaxis.p; a.xaxis = c->xaxis.p; a.yaxis = c->yaxis.p;
+    a.cp = c->prm.pentype != PMQ_PEN_NONE;
+    a.cf = c->prm.use_chgflx;
+    a.pcore = c->pcore.p; a.pval0 = c->pval0.p; a.palpha = c->palpha.p; a.pdelta = c->pdelta.p;
+    a.smp = c->smp.p;
+    a.scp = c->scp.p;
+    a.rpole13 = want_rpole13 ? c->rpole13.p : nullptr;
+    const int tb = 128;
+    hipLaunchKernelGGL(k_rotpole, dim3((c->npad + tb - 1) / tb), dim3(tb), 0, c->stream, a);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// wave-level sums
+template <class T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+struct MpoleArgs {
+    int n, npad;
+    const int2 *tiles;
+    const int *ntiles;
+    const double *sx, *sy, *sz, *smp, *scp;
+    const int *sflag, *smol, *perm, *iperm;
+    DevBox box;
+    double off2, f;
+    double *dem, *te, *pot, *aem;
+    Totals *tot;
+    // exclusion corrections
+    const ExclPair *excl;
+    int nexcl;
+    double mscale[6];
+};
+
+template <int PEN, bool CF, bool ANA, bool GRAD, bool VIR>
+__global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
+{
+    using R = double;
+    constexpr bool CP = PEN != 0;
+    __shared__ R jx[PMQ_BLOCK], jy[PMQ_BLOCK], jz[PMQ_BLOCK];
+    __shared__ R jm[10][PMQ_BLOCK];
+    __shared__ R jc[2][PMQ_BLOCK];
+    __shared__ int jflag[PMQ_BLOCK], jmol[PMQ_BLOCK];
+    __shared__ R acc[8][PMQ_BLOCK]; // f(3), t(3), pot, aem
+    const int lane = threadIdx.x;
+    const int ntiles = *a.ntiles;
+    const int np = a.npad;
+
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 IJ = a.tiles[t];
+        const int si = IJ.x * PMQ_BLOCK + lane, sj = IJ.y * PMQ_BLOCK + lane;
+        __syncthreads();
+        jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
+        for (int c = 0; c < 10; c++) jm[c][lane] = a.smp[c * np + sj];
+        if (CP) { jc[0][lane] = a.scp[sj]; jc[1][lane] = a.scp[np + sj]; }
+        jflag[lane] = a.sflag[sj];
+        if (ANA) jmol[lane] = a.smol[sj];
+        for (int c = 0; c < 8; c++) acc[c][lane] = 0.0;
+        const R xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
+        const int fi = a.sflag[si];
+        const int moli = ANA ? a.smol[si] : 0;
+        Mp<R> mi;
+        mi.c = a.smp[0 * np + si]; mi.dx = a.smp[1 * np + si]; mi.dy = a.smp[2 * np + si]; mi.dz = a.smp[3 * np + si];
+        mi.qxx = a.smp[4 * np + si]; mi.qxy = a.smp[5 * np + si]; mi.qxz = a.smp[6 * np + si];
+        mi.qyy = a.smp[7 * np + si]; mi.qyz = a.smp[8 * np + si]; mi.qzz = a.smp[9 * np + si];
+        const R corei = CP ? a.scp[si] : R(0), alphai = CP ? a.scp[np + si] : R(0);
+        __syncthreads();
+
+        // which j of the tile does this lane's i interact with
+        unsigned long long mask = 0ull;
+        if (fi & 1) {
+            const int jstart = IJ.x == IJ.y ? lane + 1 : 0;
+            for (int jj = jstart; jj < PMQ_BLOCK; jj++) {
+                const int fj = jflag[jj];
+                if (!(fj & 1) || !((fi | fj) & 2)) continue;
+                R xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
+                image<R>(a.box, xr, yr, zr);
+                if (xr * xr + yr * yr + zr * zr <= a.off2) mask |= 1ull << jj;
+            }
+        }
+
+        R e = 0, fxi = 0, fyi = 0, fzi = 0, txi = 0, tyi = 0, tzi = 0, poti = 0, aemi = 0, ein = 0;
+        R vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
+        unsigned nem = 0;
+        while (mask) {
+            const int jj = __ffsll((long long)mask) - 1;
+            mask &= mask - 1;
+            R xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
+            image<R>(a.box, xr, yr, zr);
+            const R r2 = xr * xr + yr * yr + zr * zr;
+            Mp<R> mk;
+            mk.c = jm[0][jj]; mk.dx = jm[1][jj]; mk.dy = jm[2][jj]; mk.dz = jm[3][jj];
+            mk.qxx = jm[4][jj]; mk.qxy = jm[5][jj]; mk.qxz = jm[6][jj];
+            mk.qyy = jm[7][jj]; mk.qyz = jm[8][jj]; mk.qzz = jm[9][jj];
+            PairOut<R> o;
+            if (CP) {
+                pair_cp<R, GRAD, (PEN == 0 ? 1 : PEN)>(r2, xr, yr, zr, a.f, corei, alphai, mi, jc[0][jj], jc[1][jj], mk, o);
+            } else {
+                R Rs[6];
+                rr_series<R>(r2, a.f, Rs);
+                pair_series<R, GRAD>(Rs, mi, mk, xr, yr, zr, o);
+            }
+            e += o.e;
+            if (ANA) {
+                if (o.e != R(0)) nem++;
+                aemi += R(0.5) * o.e;
+                atomicAdd(&acc[7][jj], R(0.5) * o.e);
+                if (moli != jmol[jj]) ein += o.e;
+            }
+            if (GRAD) {
+                fxi += o.fx; fyi += o.fy; fzi += o.fz;
+                txi += o.tix; tyi += o.tiy; tzi += o.tiz;
+                atomicAdd(&acc[0][jj], -o.fx);
+                atomicAdd(&acc[1][jj], -o.fy);
+                atomicAdd(&acc[2][jj], -o.fz);
+                atomicAdd(&acc[3][jj], o.tkx);
+                atomicAdd(&acc[4][jj], o.tky);
+                atomicAdd(&acc[5][jj], o.tkz);
+                if (CF) { poti += o.phii; atomicAdd(&acc[6][jj], o.phik); }
+                if (VIR) {
+                    vxx -= xr * o.fx;
+                    vxy -= R(0.5) * (yr * o.fx + xr * o.fy);
+                    vxz -= R(0.5) * (zr * o.fx + xr * o.fz);
+                    vyy -= yr * o.fy;
+                    vyz -= R(0.5) * (zr * o.fy + yr * o.fz);
+                    vzz -= zr * o.fz;
+                }
+            }
+        }
+        __syncthreads();
+        if (GRAD || ANA) {
+            if (fi & 1) {
+                const int p = a.perm[si];
+                if (GRAD) {
+                    atomicAdd(&a.dem[3 * p + 0], fxi); atomicAdd(&a.dem[3 * p + 1], fyi); atomicAdd(&a.dem[3 * p + 2], fzi);
+                    atomicAdd(&a.te[3 * p + 0], txi); atomicAdd(&a.te[3 * p + 1], tyi); atomicAdd(&a.te[3 * p + 2], tzi);
+                    if (CF) atomicAdd(&a.pot[p], poti);
+                }
+                if (ANA) atomicAdd(&a.aem[p], aemi);
+            }
+            if (jflag[lane] & 1) {
+                const int p = a.perm[sj];
+                if (GRAD) {
+                    for (int c = 0; c < 3; c++) atomicAdd(&a.dem[3 * p + c], acc[c][lane]);
+                    for (int c = 0; c < 3; c++) atomicAdd(&a.te[3 * p + c], acc[3 + c][lane]);
+                    if (CF) atomicAdd(&a.pot[p], acc[6][lane]);
+                }
+                if (ANA) atomicAdd(&a.aem[p], acc[7][lane]);
+            }
+        }
+        e = wave_sum(e);
+        if (VIR) {
+            vxx = wave_sum(vxx); vxy = wave_sum(vxy); vxz = wave_sum(vxz);
+            vyy = wave_sum(vyy); vyz = wave_sum(vyz); vzz = wave_sum(vzz);
+        }
+        if (ANA) { ein = wave_sum(ein); nem = wave_sum(nem); }
+        if (lane == 0) {
+            atomicAdd(&a.tot->em, e);
+            if (VIR) {
+                atomicAdd(&a.tot->vir[0], vxx); atomicAdd(&a.tot->vir[1], vxy); atomicAdd(&a.tot->vir[2], vxz);
+                atomicAdd(&a.tot->vir[3], vxy); atomicAdd(&a.tot->vir[4], vyy); atomicAdd(&a.tot->vir[5], vyz);
+                atomicAdd(&a.tot->vir[6], vxz); atomicAdd(&a.tot->vir[7], vyz); atomicAdd(&a.tot->vir[8], vzz);
+            }
+            if (ANA) {
+                atomicAdd(&a.tot->einter, ein);
+                atomicAdd(&a.tot->nem, (unsigned long long)nem);
+            }
+        }
+    }
+}
+
+// excluded / scaled pairs: add (scale - 1) times the pair (the tile kernel
+// counted it at scale 1); a pair whose scale is 0 is not counted in nem
+template <int PEN, bool CF, bool ANA, bool GRAD, bool VIR>
+__global__ __launch_bounds__(64) void k_mpole_excl(MpoleArgs a)
+{
+    using R = double;
+    constexpr bool CP = PEN != 0;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nexcl) return;
+    const ExclPair q = a.excl[t];
+    const R sc = R(a.mscale[q.cls]);
+    if (sc == R(1)) return;
+    const int si = a.iperm[q.i], sk = a.iperm[q.k];
+    const int fi = a.sflag[si], fk = a.sflag[sk];
+    if (!((fi | fk) & 2)) return;
+    R xr = a.sx[sk] - a.sx[si], yr = a.sy[sk] - a.sy[si], zr = a.sz[sk] - a.sz[si];
+    image<R>(a.box, xr, yr, zr);
+    const R r2 = xr * xr + yr * yr + zr * zr;
+    if (r2 > a.off2) return;
+    const int np = a.npad;
+    Mp<R> mi, mk;
+    const double *m = a.smp;
+    mi.c = m[si]; mi.dx = m[np + si]; mi.dy = m[2 * np + si]; mi.dz = m[3 * np + si];
+    mi.qxx = m[4 * np + si]; mi.qxy = m[5 * np + si]; mi.qxz = m[6 * np + si];
+    mi.qyy = m[7 * np + si]; mi.qyz = m[8 * np + si]; mi.qzz = m[9 * np + si];
+    mk.c = m[sk]; mk.dx = m[np + sk]; mk.dy = m[2 * np + sk]; mk.dz = m[3 * np + sk];
+    mk.qxx = m[4 * np + sk]; mk.qxy = m[5 * np + sk]; mk.qxz = m[6 * np + sk];
+    mk.qyy = m[7 * np + sk]; mk.qyz = m[8 * np + sk]; mk.qzz = m[9 * np + sk];
+    const R sf = (sc - R(1)) * R(a.f);
+    PairOut<R> o;
+    if (CP) {
+        pair_cp<R, GRAD, (PEN == 0 ? 1 : PEN)>(r2, xr, yr, zr, sf, a.scp[si], a.scp[np + si], mi, a.scp[sk],
+                                               a.scp[np + sk], mk, o);
+    } else {
+        R Rs[6];
+        rr_series<R>(r2, sf, Rs);
+        pair_series<R, GRAD>(Rs, mi, mk, xr, yr, zr, o);
+    }
+    atomicAdd(&a.tot->em, o.e);
+    if (ANA) {
+        if (sc == R(0) && o.e != R(0)) atomicAdd(&a.tot->nem, (unsigned long long)-1ll);
+        atomicAdd(&a.aem[q.i], R(0.5) * o.e);
+        atomicAdd(&a.aem[q.k], R(0.5) * o.e);
+        if (a.smol[si] != a.smol[sk]) atomicAdd(&a.tot->einter, o.e);
+    }
+    if (GRAD) {
+        atomicAdd(&a.dem[3 * q.i + 0], o.fx); atomicAdd(&a.dem[3 * q.i + 1], o.fy); atomicAdd(&a.dem[3 * q.i + 2], o.fz);
+        atomicAdd(&a.dem[3 * q.k + 0], -o.fx); atomicAdd(&a.dem[3 * q.k + 1], -o.fy); atomicAdd(&a.dem[3 * q.k + 2], -o.fz);
+        atomicAdd(&a.te[3 * q.i + 0], o.tix); atomicAdd(&a.te[3 * q.i + 1], o.tiy); atomicAdd(&a.te[3 * q.i + 2], o.tiz);
+        atomicAdd(&a.te[3 * q.k + 0], o.tkx); atomicAdd(&a.te[3 * q.k + 1], o.tky); atomicAdd(&a.te[3 * q.k + 2], o.tkz);
+        if (CF) { atomicAdd(&a.pot[q.i], o.phii); atomicAdd(&a.pot[q.k], o.phik); }
+        if (VIR) {
+            const R vxy = -R(0.5) * (yr * o.fx + xr * o.fy);
+            const R vxz = -R(0.5) * (zr * o.fx + xr * o.fz);
+            const R vyz = -R(0.5) * (zr * o.fy + yr * o.fz);
+            atomicAdd(&a.tot->vir[0], -xr * o.fx); atomicAdd(&a.tot->vir[4], -yr * o.fy);
+            atomicAdd(&a.tot->vir[8], -zr * o.fz);
+            atomicAdd(&a.tot->vir[1], vxy); atomicAdd(&a.tot->vir[3], vxy);
+            atomicAdd(&a.tot->vir[2], vxz); atomicAdd(&a.tot->vir[6], vxz);
+            atomicAdd(&a.tot->vir[5], vyz); atomicAdd(&a.tot->vir[7], vyz);
+        }
+    }
+}
+
+template <int PEN, bool CF, bool ANA, bool GRAD, bool VIR>
+static int launch_mpole_t(pmq_ctx *c, const MpoleArgs &a)
+{
+    const int grid = std::max(1, std::min(c->ntiles_cap, 16384));
+    hipLaunchKernelGGL((k_mpole_tiles<PEN, CF, ANA, GRAD, VIR>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
+    if (a.nexcl > 0)
+        hipLaunchKernelGGL((k_mpole_excl<PEN, CF, ANA, GRAD, VIR>), dim3((a.nexcl + 63) / 64), dim3(64), 0,
+                           c->stream, a);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+template <int PEN, bool CF>
+static int launch_mpole_mode(pmq_ctx *c, const MpoleArgs &a, int mode)
+{
+    switch (mode) {
+    case PMQ_ENERGY: return launch_mpole_t<PEN, CF, false, false, false>(c, a);
+    case PMQ_ANALYSIS: return launch_mpole_t<PEN, CF, true, false, false>(c, a);
+    case PMQ_GRADIENT: return launch_mpole_t<PEN, CF, false, true, false>(c, a);
+    default: return launch_mpole_t<PEN, CF, false, true, true>(c, a);
+    }
+}
+
+int launch_mpole_direct(pmq_ctx *c, int mode, double off2)
+{
+    MpoleArgs a;
+    a.n = c->n;
+    a.npad = c->npad;
+    a.tiles = c->tiles.p;
+    a.ntiles = c->ntiles_d.p;
+    a.sx = c->sx.p; a.sy = c->sy.p; a.sz = c->sz.p;
+    a.smp = c->smp.p; a.scp = c->scp.p;
+    a.sflag = c->sflag.p; a.smol = c->smol.p; a.perm = c->perm.p; a.iperm = c->iperm.p;
+    a.box = c->box;
+    a.off2 = off2;
+    a.f = c->prm.electric / c->prm.dielec;
+    a.dem = c->dem.p; a.te = c->te.p; a.pot = c->pot.p; a.aem = c->aem.p;
+    a.tot = c->totals.p;
+    a.excl = c->excl.p;
+    a.nexcl = (int)c->h_excl.size();
+    a.mscale[0] = a.mscale[1] = 1.0;
+    a.mscale[2] = c->prm.m2scale; a.mscale[3] = c->prm.m3scale;
+    a.mscale[4] = c->prm.m4scale; a.mscale[5] = c->prm.m5scale;
+    const int pen = c->prm.pentype;
+    const bool cf = pen != PMQ_PEN_NONE && c->prm.use_chgflx;
+    if (pen == PMQ_PEN_NONE) return launch_mpole_mode<0, false>(c, a, mode);
+    if (pen == PMQ_PEN_GORDON1) return cf ? launch_mpole_mode<1, true>(c, a, mode) : launch_mpole_mode<1, false>(c, a, mode);
+    return cf ? launch_mpole_mode<2, true>(c, a, mode) : launch_mpole_mode<2, false>(c, a, mode);
+}
+
+// ---------------------------------------------------------------------------
+// torque -> forces on the frame atoms (torque.cpp:42-397)
+struct TorqueArgs {
+    int n;
+    const double *x, *y, *z, *te;
+    const int *polaxe, *zaxis, *xaxis, *yaxis;
+    double *de;
+    Totals *tot;
+    int vir;
+};
+
+__device__ __forceinline__ void cross(const double *a, const double *b, double *c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ void normalize(double *v)
+{
+    const double l = sqrt(dot3(v, v));
+    v[0] /= l; v[1] /= l; v[2] /= l;
+}
+
+__global__ __launch_bounds__(128) void k_torque(TorqueArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double W[6] = {0, 0, 0, 0, 0, 0};
+    if (i < a.n && a.polaxe[i] != PMQ_FRAME_NONE) {
+        const int frame = a.polaxe[i];
+        const double t[3] = {a.te[3 * i], a.te[3 * i + 1], a.te[3 * i + 2]};
+        const int ia = a.zaxis[i] - 1, ic = a.xaxis[i] - 1, id = abs(a.yaxis[i]) - 1;
+        double u[3] = {a.x[ia] - a.x[i], a.y[ia] - a.y[i], a.z[ia] - a.z[i]}, v[3], w[3];
+        const double usiz = sqrt(dot3(u, u));
+        double vsiz;
+        if (frame != PMQ_FRAME_ZONLY) {
+            v[0] = a.x[ic] - a.x[i]; v[1] = a.y[ic] - a.y[i]; v[2] = a.z[ic] - a.z[i];
+            vsiz = sqrt(dot3(v, v));
+        } else {
+            v[0] = 1; v[1] = 0; v[2] = 0;
+            vsiz = 1;
+            if (fabs(u[0] / usiz) > 0.866) { v[0] = 0; v[1] = 1; }
+        }
+        if (frame == PMQ_FRAME_ZBISECT || frame == PMQ_FRAME_THREEFOLD) {
+            w[0] = a.x[id] - a.x[i]; w[1] = a.y[id] - a.y[i]; w[2] = a.z[id] - a.z[i];
+        } else {
+            cross(u, v, w);
+        }
+        const double wsiz = sqrt(dot3(w, w));
+        for (int j = 0; j < 3; j++) { u[j] /= usiz; v[j] /= vsiz; w[j] /= wsiz; }
+        double fz[3] = {0, 0, 0}, fx[3] = {0, 0, 0}, fy[3] = {0, 0, 0};
+        const double dphidu = -dot3(t, u), dphidv = -dot3(t, v), dphidw = -dot3(t, w);
+        double uv[3], uw[3], vw[3];
+        cross(v, u, uv); normalize(uv);
+        cross(w, u, uw); normalize(uw);
+        cross(w, v, vw); normalize(vw);
+        const double uvcos = dot3(u, v), uvsin = sqrt(1.0 - uvcos * uvcos);
+        if (frame == PMQ_FRAME_ZONLY) {
+            for (int j = 0; j < 3; j++) fz[j] = uv[j] * dphidv / (usiz * uvsin) + uw[j] * dphidw / usiz;
+        } else if (frame == PMQ_FRAME_ZTHENX) {
+            for (int j = 0; j < 3; j++) {
+                fz[j] = uv[j] * dphidv / (usiz * uvsin) + uw[j] * dphidw / usiz;
+                fx[j] = -uv[j] * dphidu / (vsiz * uvsin);
+            }
+        } else if (frame == PMQ_FRAME_BISECTOR) {
+            for (int j = 0; j < 3; j++) {
+                fz[j] = uv[j] * dphidv / (usiz * uvsin) + 0.5 * uw[j] * dphidw / usiz;
+                fx[j] = -uv[j] * dphidu / (vsiz * uvsin) + 0.5 * vw[j] * dphidw / vsiz;
+            }
+        } else if (frame == PMQ_FRAME_ZBISECT) {
+            double r[3], s[3];
+            for (int j = 0; j < 3; j++) r[j] = v[j] + w[j];
+            normalize(r);
+            cross(u, r, s);
+            normalize(s);
+            const double dphidr = -dot3(t, r), dphids = -dot3(t, s);
+            double ur[3], us[3];
+            cross(r, u, ur); normalize(ur);
+            cross(s, u, us); normalize(us);
+            const double urcos = dot3(u, r), ursin = sqrt(1.0 - urcos * urcos);
+            const double vscos = dot3(v, s), vssin = sqrt(1.0 - vscos * vscos);
+            const double wscos = dot3(w, s), wssin = sqrt(1.0 - wscos * wscos);
+            double t1[3], t2[3];
+            for (int j = 0; j < 3; j++) { t1[j] = v[j] - s[j] * vscos; t2[j] = w[j] - s[j] * wscos; }
+            normalize(t1);
+            normalize(t2);
+            const double c1 = dot3(u, t1), c2 = dot3(u, t2);
+            const double ut1sin = sqrt(1.0 - c1 * c1), ut2sin = sqrt(1.0 - c2 * c2);
+            for (int j = 0; j < 3; j++) {
+                fz[j] = ur[j] * dphidr / (usiz * ursin) + us[j] * dphids / usiz;
+                fx[j] = (vssin * s[j] - vscos * t1[j]) * dphidu / (vsiz * (ut1sin + ut2sin));
+                fy[j] = (wssin * s[j] - wscos * t2[j]) * dphidu / (wsiz * (ut1sin + ut2sin));
+            }
+        } else if (frame == PMQ_FRAME_THREEFOLD) {
+            double p[3];
+            for (int j = 0; j < 3; j++) p[j] = u[j] + v[j] + w[j];
+            const double psiz = sqrt(dot3(p, p));
+            for (int j = 0; j < 3; j++) p[j] /= psiz;
+            const double cs[3] = {dot3(u, p), dot3(v, p), dot3(w, p)};
+            // axis pairs: (u+v) about w -> fy, (v+w) about u -> fz, (u+w) about v -> fx
+            const double *ax[3] = {w, u, v};
+            const double *pa[3] = {u, v, u};
+            const double *pb[3] = {v, w, w};
+            const double sz[3] = {wsiz, usiz, vsiz};
+            const double pc[3] = {cs[2], cs[0], cs[1]};
+            double *dst[3] = {fy, fz, fx};
+            for (int q = 0; q < 3; q++) {
+                double r[3], del[3], eps[3];
+                for (int j = 0; j < 3; j++) r[j] = pa[q][j] + pb[q][j];
+                normalize(r);
+                const double c = dot3(r, ax[q]), sn = sqrt(1.0 - c * c);
+                const double dr = -dot3(t, r);
+                cross(r, ax[q], del);
+                normalize(del);
+                const double dd = -dot3(t, del);
+                cross(del, ax[q], eps);
+                for (int j = 0; j < 3; j++)
+                    dst[q][j] = del[j] * dr / (sz[q] * sn) + eps[j] * dd * pc[q] / (sz[q] * psiz);
+            }
+        }
+        for (int j = 0; j < 3; j++) {
+            atomicAdd(&a.de[3 * ia + j], fz[j]);
+            if (frame != PMQ_FRAME_ZONLY) atomicAdd(&a.de[3 * ic + j], fx[j]);
+            if (frame == PMQ_FRAME_ZBISECT || frame == PMQ_FRAME_THREEFOLD) atomicAdd(&a.de[3 * id + j], fy[j]);
+            atomicAdd(&a.de[3 * i + j], -(fz[j] + fx[j] + fy[j]));
+        }
+        if (a.vir) {
+            const int jz = ia >= 0 ? ia : i, jx = ic >= 0 ? ic : i, jy = id >= 0 ? id : i;
+            const double rz[3] = {a.x[jz] - a.x[i], a.y[jz] - a.y[i], a.z[jz] - a.z[i]};
+            const double rx[3] = {a.x[jx] - a.x[i], a.y[jx] - a.y[i], a.z[jx] - a.z[i]};
+            const double ry[3] = {a.x[jy] - a.x[i], a.y[jy] - a.y[i], a.z[jy] - a.z[i]};
+            double M[3][3];
+            for (int p = 0; p < 3; p++)
+                for (int q = 0; q < 3; q++) M[p][q] = rx[p] * fx[q] + ry[p] * fy[q] + rz[p] * fz[q];
+            W[0] = M[0][0];
+            W[1] = 0.5 * (M[1][0] + M[0][1]);
+            W[2] = 0.5 * (M[2][0] + M[0][2]);
+            W[3] = M[1][1];
+            W[4] = 0.5 * (M[2][1] + M[1][2]);
+            W[5] = M[2][2];
+        }
+    }
+    if (a.vir) {
+        __shared__ double red[6][128];
+        for (int k = 0; k < 6; k++) red[k][threadIdx.x] = W[k];
+        __syncthreads();
+        for (int w = 64; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w)
+                for (int k = 0; k < 6; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            atomicAdd(&a.tot->vir[0], red[0][0]);
+            atomicAdd(&a.tot->vir[1], red[1][0]); atomicAdd(&a.tot->vir[3], red[1][0]);
+            atomicAdd(&a.tot->vir[2], red[2][0]); atomicAdd(&a.tot->vir[6], red[2][0]);
+            atomicAdd(&a.tot->vir[4], red[3][0]);
+            atomicAdd(&a.tot->vir[5], red[4][0]); atomicAdd(&a.tot->vir[7], red[4][0]);
+            atomicAdd(&a.tot->vir[8], red[5][0]);
+        }
+    }
+}
+
+int launch_torque(pmq_ctx *c, int mode)
+{
+    TorqueArgs a;
+    a.n = c->n;
+    a.x = c->x.p; a.y = c->y.p; a.z = c->z.p; a.te = c->te.p;
+    a.polaxe = c->polaxe.p; a.zaxis = c->zaxis.p; a.xaxis = c->xaxis.p; a.yaxis = c->yaxis.p;
+    a.de = c->dem.p;
+    a.tot = c->totals.p;
+    a.vir = mode == PMQ_VIRIAL;
+    hipLaunchKernelGGL(k_torque, dim3((c->n + 127) / 128), dim3(128), 0, c->stream, a);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// charge flux
+struct CfArgs {
+    int n, nbond, nangle;
+    const double *x, *y, *z;
+    const int *ibnd, *iang, *balist;
+    const double *bl, *bflx, *anat, *aflx, *abflx;
+    double *pdelta;
+    const double *pot;
+    double *de;
+    Totals *tot;
+    int vir;
+};
+
+__global__ void k_cf_delta(CfArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const double radian = 57.29577951308232088;
+    if (t < a.nbond) {
+        const int ia = a.ibnd[2 * t], ib = a.ibnd[2 * t + 1];
+        const double d[3] = {a.x[ia] - a.x[ib], a.y[ia] - a.y[ib], a.z[ia] - a.z[ib]};
+        const double dq = a.bflx[t] * (sqrt(dot3(d, d)) - a.bl[t]);
+        atomicAdd(&a.pdelta[ia], -dq);
+        atomicAdd(&a.pdelta[ib], dq);
+    } else if (t < a.nbond + a.nangle) {
+        const int g = t - a.nbond;
+        const int ia = a.iang[3 * g], ib = a.iang[3 * g + 1], ic = a.iang[3 * g + 2];
+        const double ab[3] = {a.x[ia] - a.x[ib], a.y[ia] - a.y[ib], a.z[ia] - a.z[ib]};
+        const double cb[3] = {a.x[ic] - a.x[ib], a.y[ic] - a.y[ib], a.z[ic] - a.z[ib]};
+        const double rab = sqrt(dot3(ab, ab)), rcb = sqrt(dot3(cb, cb));
+        if (rab > 0.0 && rcb > 0.0) {
+            double cs = dot3(ab, cb) / (rab * rcb);
+            cs = fmin(1.0, fmax(-1.0, cs));
+            const double dth = (radian * acos(cs) - a.anat[g]) / radian;
+            const double dq1 = a.abflx[2 * g] * (rcb - a.bl[a.balist[2 * g + 1]]) + a.aflx[2 * g] * dth;
+            const double dq2 = a.abflx[2 * g + 1] * (rab - a.bl[a.balist[2 * g]]) + a.aflx[2 * g + 1] * dth;
+            atomicAdd(&a.pdelta[ia], dq1);
+            atomicAdd(&a.pdelta[ib], -(dq1 + dq2));
+            atomicAdd(&a.pdelta[ic], dq2);
+        }
+    }
+}
+
+__device__ __forceinline__ void cf_add(const CfArgs &a, int i, const double *f, double *W)
+{
+    atomicAdd(&a.de[3 * i], f[0]);
+    atomicAdd(&a.de[3 * i + 1], f[1]);
+    atomicAdd(&a.de[3 * i + 2], f[2]);
+    // dcflux.cpp:204-223 builds the virial from absolute coordinates
+    const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
+    W[0] += xi * f[0]; W[1] += yi * f[0]; W[2] += zi * f[0];
+    W[3] += yi * f[1]; W[4] += zi * f[1]; W[5] += zi * f[2];
+}
+
+__global__ __launch_bounds__(128) void k_cf_grad(CfArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double W[6] = {0, 0, 0, 0, 0, 0};
+    if (t < a.nbond) {
+        const int ia = a.ibnd[2 * t], ib = a.ibnd[2 * t + 1];
+        const double d[3] = {a.x[ia] - a.x[ib], a.y[ia] - a.y[ib], a.z[ia] - a.z[ib]};
+        const double pb = a.bflx[t] * (a.pot[ib] - a.pot[ia]) / sqrt(dot3(d, d));
+        const double fa[3] = {pb * d[0], pb * d[1], pb * d[2]};
+        const double fb[3] = {-fa[0], -fa[1], -fa[2]};
+        cf_add(a, ia, fa, W);
+        cf_add(a, ib, fb, W);
+    } else if (t < a.nbond + a.nangle) {
+        const int g = t - a.nbond;
+        const int ia = a.iang[3 * g], ib = a.iang[3 * g + 1], ic = a.iang[3 * g + 2];
+        const double ab[3] = {a.x[ia] - a.x[ib], a.y[ia] - a.y[ib], a.z[ia] - a.z[ib]};
+        const double cb[3] = {a.x[ic] - a.x[ib], a.y[ic] - a.y[ib], a.z[ic] - a.z[ib]};
+        const double rab2 = dot3(ab, ab), rcb2 = dot3(cb, cb);
+        const double rab = sqrt(rab2), rcb = sqrt(rcb2);
+        if (rab > 0.0 && rcb > 0.0) {
+            const double dpa = a.pot[ia] - a.pot[ib], dpc = a.pot[ic] - a.pot[ib];
+            const double pb1 = dpa * a.abflx[2 * g], pb2 = dpc * a.abflx[2 * g + 1];
+            const double rabc = rab * rcb, dp = dot3(ab, cb);
+            const double dra3 = dp / (rab2 * rab * rcb), drc3 = dp / (rab * rcb2 * rcb);
+            const double fterm = -rabc / sqrt(rab2 * rcb2 - dp * dp) * (dpa * a.aflx[2 * g] + dpc * a.aflx[2 * g + 1]);
+            double fa[3], fc[3], fb[3];
+            for (int j = 0; j < 3; j++) {
+                const double fa1 = pb2 * ab[j] / rab, fc1 = pb1 * cb[j] / rcb;
+                const double fa2 = fterm * (cb[j] / rabc - ab[j] * dra3);
+                const double fc2 = fterm * (ab[j] / rabc - cb[j] * drc3);
+                fa[j] = fa1 + fa2;
+                fc[j] = fc1 + fc2;
+                fb[j] = -fa1 - fc1 - fa2 - fc2;
+            }
+            cf_add(a, ia, fa, W);
+            cf_add(a, ib, fb, W);
+            cf_add(a, ic, fc, W);
+        }
+    }
+    if (a.vir) {
+        __shared__ double red[6][128];
+        for (int k = 0; k < 6; k++) red[k][threadIdx.x] = W[k];
+        __syncthreads();
+        for (int w = 64; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w)
+                for (int k = 0; k < 6; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            atomicAdd(&a.tot->vir[0], red[0][0]);
+            atomicAdd(&a.tot->vir[1], red[1][0]); atomicAdd(&a.tot->vir[3], red[1][0]);
+            atomicAdd(&a.tot->vir[2], red[2][0]); atomicAdd(&a.tot->vir[6], red[2][0]);
+            atomicAdd(&a.tot->vir[4], red[3][0]);
+            atomicAdd(&a.tot->vir[5], red[4][0]); atomicAdd(&a.tot->vir[7], red[4][0]);
+            atomicAdd(&a.tot->vir[8], red[5][0]);
+        }
+    }
+}
+
+static CfArgs cf_args(pmq_ctx *c, int mode)
+{
+    CfArgs a;
+    a.n = c->n; a.nbond = c->nbond; a.nangle = c->nangle;
+    a.x = c->x.p; a.y = c->y.p; a.z = c->z.p;
+    a.ibnd = c->ibnd.p; a.iang = c->iang.p; a.balist = c->balist.p;
+    a.bl = c->bl.p; a.bflx = c->bflx.p; a.anat = c->anat.p; a.aflx = c->aflx.p; a.abflx = c->abflx.p;
+    a.pdelta = c->pdelta.p;
+    a.pot = c->pot.p;
+    a.de = c->dem.p;
+    a.tot = c->totals.p;
+    a.vir = mode == PMQ_VIRIAL;
+    return a;
+}
+
+int launch_chgflx_delta(pmq_ctx *c)
+{
+    int rc = launch_zero(c, c->pdelta.p, c->n);
+    if (rc) return rc;
+    const int nt = c->nbond + c->nangle;
+    if (nt > 0) hipLaunchKernelGGL(k_cf_delta, dim3((nt + 127) / 128), dim3(128), 0, c->stream, cf_args(c, 0));
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+int launch_chgflx_grad(pmq_ctx *c, int mode)
+{
+    const int nt = c->nbond + c->nangle;
+    if (nt > 0) hipLaunchKernelGGL(k_cf_grad, dim3((nt + 127) / 128), dim3(128), 0, c->stream, cf_args(c, mode));
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+} // namespace pmq

@@ -1,0 +1,335 @@
+// k_neighbors.hip -- spatial sort and the 64x64 block-tile list.
+//
+// Every evaluation: (1) key each atom by the Morton index of its cell
+// (cells about 4 A wide in the box's fractional frame, or in the bounding box
+// of a non-periodic system) with the atom index in the low 32 bits, so the
+// sort is total and deterministic; (2) radix-sort the keys; (3) gather the
+// permutation; (4) bound each 64-atom block; (5) keep every block pair
+// (I <= J) whose bounds come within the cutoff, in (I, J) order.
+//
+// A block pair is kept when the minimum-image gap between the two bounding
+// boxes is within the cutoff.  For orthogonal boxes and free clusters the
+// gap is per axis; for monoclinic and triclinic boxes it is a sphere test
+// over the 27 nearest lattice images; truncated-octahedron and dodecahedron
+// boxes keep every pair (their image() is not a lattice minimum image).
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "pmq_internal.h"
+
+namespace pmq {
+
+DevBox make_box(const pmq_box *b)
+{
+    DevBox d{};
+    d.kind = b ? b->kind : PMQ_BOX_NONE;
+    if (!b || b->kind == PMQ_BOX_NONE) return d;
+    const double radian = 57.29577951308232088;
+    d.xbox = b->xbox;
+    d.ybox = b->ybox;
+    d.zbox = b->zbox;
+    d.beta_sin = 1; d.beta_cos = 0; d.gamma_sin = 1; d.gamma_cos = 0; d.beta_term = 0; d.gamma_term = 1;
+    if (b->kind == PMQ_BOX_TRI) {
+        const double ac = cos(b->alpha / radian);
+        d.beta_sin = sin(b->beta / radian);
+        d.beta_cos = cos(b->beta / radian);
+        d.gamma_sin = sin(b->gamma / radian);
+        d.gamma_cos = cos(b->gamma / radian);
+        d.beta_term = (ac - d.beta_cos * d.gamma_cos) / d.gamma_sin;
+        d.gamma_term = sqrt(d.beta_sin * d.beta_sin - d.beta_term * d.beta_term);
+    } else if (b->kind == PMQ_BOX_MONO) {
+        d.beta_sin = sin(b->beta / radian);
+        d.beta_cos = cos(b->beta / radian);
+        d.gamma_term = d.beta_sin;
+    }
+    // lattice vectors as columns (lattice.cpp), reciprocal rows
+    const double ar[3] = {d.xbox, 0, 0};
+    const double br[3] = {d.ybox * d.gamma_cos, d.ybox * d.gamma_sin, 0};
+    const double cr[3] = {d.zbox * d.beta_cos, d.zbox * d.beta_term, d.zbox * d.gamma_term};
+    for (int a = 0; a < 3; a++) {
+        d.lvec[a][0] = ar[a];
+        d.lvec[a][1] = br[a];
+        d.lvec[a][2] = cr[a];
+    }
+    const double vol = ar[0] * (br[1] * cr[2] - br[2] * cr[1]) - ar[1] * (br[0] * cr[2] - br[2] * cr[0])
+                     + ar[2] * (br[0] * cr[1] - br[1] * cr[0]);
+    // recip rows = (b x c, c x a, a x b) / vol
+    d.recip[0][0] = (br[1] * cr[2] - cr[1] * br[2]) / vol;
+    d.recip[0][1] = (br[2] * cr[0] - cr[2] * br[0]) / vol;
+    d.recip[0][2] = (br[0] * cr[1] - cr[0] * br[1]) / vol;
+    d.recip[1][0] = (cr[1] * ar[2] - ar[1] * cr[2]) / vol;
+    d.recip[1][1] = (cr[2] * ar[0] - ar[2] * cr[0]) / vol;
+    d.recip[1][2] = (cr[0] * ar[1] - ar[0] * cr[1]) / vol;
+    d.recip[2][0] = (ar[1] * br[2] - br[1] * ar[2]) / vol;
+    d.recip[2][1] = (ar[2] * br[0] - br[2] * ar[0]) / vol;
+    d.recip[2][2] = (ar[0] * br[1] - br[0] * ar[1]) / vol;
+    return d;
+}
+
+__device__ __forceinline__ unsigned spread3(unsigned v)
+{
+    v &= 0x3ff;
+    v = (v | (v << 16)) & 0x030000ff;
+    v = (v | (v << 8)) & 0x0300f00f;
+    v = (v | (v << 4)) & 0x030c30c3;
+    v = (v | (v << 2)) & 0x09249249;
+    return v;
+}
+
+// bounding box of a free cluster (one workgroup)
+__global__ __launch_bounds__(1024) void k_bbox(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                               const double *__restrict__ z, double *__restrict__ out)
+{
+    __shared__ double s[6][1024];
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double v[3] = {x[i], y[i], z[i]};
+        for (int a = 0; a < 3; a++) { lo[a] = fmin(lo[a], v[a]); hi[a] = fmax(hi[a], v[a]); }
+    }
+    for (int a = 0; a < 3; a++) { s[a][threadIdx.x] = lo[a]; s[3 + a][threadIdx.x] = hi[a]; }
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int a = 0; a < 3; a++) {
+                s[a][threadIdx.x] = fmin(s[a][threadIdx.x], s[a][threadIdx.x + w]);
+                s[3 + a][threadIdx.x] = fmax(s[3 + a][threadIdx.x], s[3 + a][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
+}
+
+// fractional coordinate in [0,1) of atom i for the cell grid
+__device__ __forceinline__ void frac_of(const DevBox &b, const double *bbox, double px, double py, double pz,
+                                        double *f)
+{
+    if (b.kind == PMQ_BOX_NONE) {
+        const double p[3] = {px, py, pz};
+        for (int a = 0; a < 3; a++) {
+            const double ext = fmax(bbox[3 + a] - bbox[a], 1e-6);
+            f[a] = fmin(fmax((p[a] - bbox[a]) / ext, 0.0), 0.999999);
+        }
+    } else {
+        for (int a = 0; a < 3; a++) {
+            double v = b.recip[a][0] * px + b.recip[a][1] * py + b.recip[a][2] * pz;
+            v -= floor(v);
+            f[a] = v >= 1.0 ? 0.0 : v;
+        }
+    }
+}
+
+__global__ void k_cell_keys(int n, DevBox b, const double *__restrict__ bbox, int ncx, int ncy, int ncz,
+                            const double *__restrict__ x, const double *__restrict__ y,
+                            const double *__restrict__ z, unsigned long long *__restrict__ keys)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double f[3];
+    frac_of(b, bbox, x[i], y[i], z[i], f);
+    const unsigned cx = min((int)(f[0] * ncx), ncx - 1);
+    const unsigned cy = min((int)(f[1] * ncy), ncy - 1);
+    const unsigned cz = min((int)(f[2] * ncz), ncz - 1);
+    const unsigned m = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
+    keys[i] = ((unsigned long long)m << 32) | (unsigned)i;
+}
+
+// permutation, sorted positions and per-atom flags (bit0 valid, bit1 active)
+__global__ void k_gather(int n, int npad, const unsigned long long *__restrict__ skeys, const double *__restrict__ x,
+                         const double *__restrict__ y, const double *__restrict__ z, const int *__restrict__ usef,
+                         const int *__restrict__ mol, int *__restrict__ perm, int *__restrict__ iperm,
+                         double *__restrict__ sx, double *__restrict__ sy, double *__restrict__ sz,
+                         int *__restrict__ sflag, int *__restrict__ smol)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= npad) return;
+    if (s < n) {
+        const int i = (int)(skeys[s] & 0xffffffffu);
+        perm[s] = i;
+        iperm[i] = s;
+        sx[s] = x[i];
+        sy[s] = y[i];
+        sz[s] = z[i];
+        sflag[s] = 1 | (usef[i] ? 2 : 0);
+        smol[s] = mol[i];
+    } else {
+        perm[s] = -1;
+        sx[s] = sy[s] = sz[s] = 0.0;
+        sflag[s] = 0;
+        smol[s] = -1;
+    }
+}
+
+// axis-aligned bounds of each block in the (wrapped) fractional-cell frame,
+// expressed as a Cartesian center and half extent
+__global__ void k_block_bounds(int n, int nblk, DevBox b, const double *__restrict__ bbox,
+                               const double *__restrict__ sx, const double *__restrict__ sy,
+                               const double *__restrict__ sz, double *__restrict__ blk)
+{
+    const int I = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int s = I * PMQ_BLOCK + lane;
+    __shared__ double lo[3][PMQ_BLOCK], hi[3][PMQ_BLOCK];
+    double p[3] = {0, 0, 0};
+    bool valid = s < n;
+    if (valid) {
+        p[0] = sx[s]; p[1] = sy[s]; p[2] = sz[s];
+        if (b.kind != PMQ_BOX_NONE) {
+            // wrap into the primary cell so the block is compact
+            double f[3];
+            for (int a = 0; a < 3; a++) {
+                f[a] = b.recip[a][0] * p[0] + b.recip[a][1] * p[1] + b.recip[a][2] * p[2];
+                f[a] -= floor(f[a]);
+            }
+            for (int a = 0; a < 3; a++) p[a] = b.lvec[a][0] * f[0] + b.lvec[a][1] * f[1] + b.lvec[a][2] * f[2];
+        }
+    }
+    for (int a = 0; a < 3; a++) {
+        lo[a][lane] = valid ? p[a] : 1e300;
+        hi[a][lane] = valid ? p[a] : -1e300;
+    }
+    __syncthreads();
+    for (int w = PMQ_BLOCK / 2; w > 0; w >>= 1) {
+        if (lane < w)
+            for (int a = 0; a < 3; a++) {
+                lo[a][lane] = fmin(lo[a][lane], lo[a][lane + w]);
+                hi[a][lane] = fmax(hi[a][lane], hi[a][lane + w]);
+            }
+        __syncthreads();
+    }
+    if (lane < 3) {
+        blk[6 * I + lane] = 0.5 * (lo[lane][0] + hi[lane][0]);
+        blk[6 * I + 3 + lane] = 0.5 * (hi[lane][0] - lo[lane][0]);
+    }
+}
+
+__device__ __forceinline__ void tri_index(long long t, int nb, int &I, int &J)
+{
+    // t enumerates I <= J row by row: row I has nb - I entries
+    // solve for I with the quadratic, then fix rounding
+    const double dn = 2.0 * nb + 1.0;
+    int i = (int)floor((dn - sqrt(dn * dn - 8.0 * (double)t)) * 0.5);
+    if (i < 0) i = 0;
+    while (i > 0 && (long long)i * nb - (long long)i * (i - 1) / 2 > t) i--;
+    while ((long long)(i + 1) * nb - (long long)(i + 1) * i / 2 <= t) i++;
+    I = i;
+    J = (int)(t - ((long long)i * nb - (long long)i * (i - 1) / 2)) + i;
+}
+
+__global__ void k_tile_flags(int nblk, long long npairs, DevBox b, double cut, const double *__restrict__ blk,
+                             int *__restrict__ flag)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs) return;
+    int I, J;
+    tri_index(t, nblk, I, J);
+    bool keep = true;
+    const double *bi = blk + 6 * I, *bj = blk + 6 * J;
+    double d[3] = {bj[0] - bi[0], bj[1] - bi[1], bj[2] - bi[2]};
+    if (b.kind == PMQ_BOX_NONE || b.kind == PMQ_BOX_ORTHO) {
+        if (b.kind == PMQ_BOX_ORTHO) {
+            const double L[3] = {b.xbox, b.ybox, b.zbox};
+            for (int a = 0; a < 3; a++) d[a] -= L[a] * rint(d[a] / L[a]);
+        }
+        double g2 = 0.0;
+        for (int a = 0; a < 3; a++) {
+            const double g = fmax(0.0, fabs(d[a]) - bi[3 + a] - bj[3 + a]);
+            g2 += g * g;
+        }
+        keep = g2 <= cut * cut * (1.0 + 1e-12) + 1e-12;
+    } else if (b.kind == PMQ_BOX_MONO || b.kind == PMQ_BOX_TRI) {
+        // fractional wrap of the center offset, then the nearest of 27 images
+        double f[3];
+        for (int a = 0; a < 3; a++) {
+            f[a] = b.recip[a][0] * d[0] + b.recip[a][1] * d[1] + b.recip[a][2] * d[2];
+            f[a] -= rint(f[a]);
+        }
+        double best = 1e300;
+        for (int u = -1; u <= 1; u++)
+            for (int v = -1; v <= 1; v++)
+                for (int w = -1; w <= 1; w++) {
+                    const double g[3] = {f[0] + u, f[1] + v, f[2] + w};
+                    double r2 = 0.0;
+                    for (int a = 0; a < 3; a++) {
+                        const double c = b.lvec[a][0] * g[0] + b.lvec[a][1] * g[1] + b.lvec[a][2] * g[2];
+                        r2 += c * c;
+                    }
+                    best = fmin(best, r2);
+                }
+        const double ri = sqrt(bi[3] * bi[3] + bi[4] * bi[4] + bi[5] * bi[5]);
+        const double rj = sqrt(bj[3] * bj[3] + bj[4] * bj[4] + bj[5] * bj[5]);
+        keep = sqrt(best) - ri - rj <= cut * (1.0 + 1e-12) + 1e-9;
+    }
+    flag[t] = keep ? 1 : 0;
+}
+
+__global__ void k_tile_write(int nblk, long long npairs, const int *__restrict__ flag, const int *__restrict__ scan,
+                             int2 *__restrict__ tiles)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs || !flag[t]) return;
+    int I, J;
+    tri_index(t, nblk, I, J);
+    tiles[scan[t]] = make_int2(I, J);
+}
+
+__global__ void k_tile_count(long long npairs, const int *__restrict__ flag, const int *__restrict__ scan,
+                             int *__restrict__ count)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) count[0] = npairs ? scan[npairs - 1] + flag[npairs - 1] : 0;
+}
+
+int build_neighbors(pmq_ctx *c, double cutoff)
+{
+    const int n = c->n, npad = c->npad, nblk = c->nblk;
+    hipStream_t st = c->stream;
+    const DevBox &b = c->box;
+    if (b.kind == PMQ_BOX_NONE) {
+        hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, st, n, c->x.p, c->y.p, c->z.p, c->bbox.p);
+    }
+    // cells about 4 A wide (at most 1024 per axis)
+    int nc[3];
+    if (b.kind == PMQ_BOX_NONE) {
+        // cell counts from the host-side estimate of the extent
+        for (int a = 0; a < 3; a++) nc[a] = 64;
+    } else {
+        const double len[3] = {b.xbox, b.ybox, b.zbox};
+        for (int a = 0; a < 3; a++) nc[a] = std::max(1, std::min(1024, (int)(len[a] / 4.0)));
+    }
+    const int tb = 256;
+    hipLaunchKernelGGL(k_cell_keys, dim3((n + tb - 1) / tb), dim3(tb), 0, st, n, b, c->bbox.p, nc[0], nc[1], nc[2],
+                       c->x.p, c->y.p, c->z.p, c->keys.p);
+    size_t bytes = c->scratch_sort_bytes;
+    PMQ_HIP(rocprim::radix_sort_keys(c->scratch_sort.p, bytes, c->keys.p, c->keys_sorted.p, (size_t)n, 0, 64, st));
+    hipLaunchKernelGGL(k_gather, dim3((npad + tb - 1) / tb), dim3(tb), 0, st, n, npad, c->keys_sorted.p, c->x.p,
+                       c->y.p, c->z.p, c->usef.p, c->molcule.p, c->perm.p, c->iperm.p, c->sx.p, c->sy.p, c->sz.p,
+                       c->sflag.p, c->smol.p);
+    hipLaunchKernelGGL(k_block_bounds, dim3(nblk), dim3(PMQ_BLOCK), 0, st, n, nblk, b, c->bbox.p, c->sx.p, c->sy.p,
+                       c->sz.p, c->blk.p);
+    const long long npairs = (long long)nblk * (nblk + 1) / 2;
+    const int gb = (int)((npairs + tb - 1) / tb);
+    hipLaunchKernelGGL(k_tile_flags, dim3(gb), dim3(tb), 0, st, nblk, npairs, b, cutoff, c->blk.p, c->tileflag.p);
+    bytes = c->scratch_sort_bytes;
+    PMQ_HIP(rocprim::exclusive_scan(c->scratch_sort.p, bytes, c->tileflag.p, c->tilescan.p, 0, (size_t)npairs,
+                                    rocprim::plus<int>(), st));
+    hipLaunchKernelGGL(k_tile_write, dim3(gb), dim3(tb), 0, st, nblk, npairs, c->tileflag.p, c->tilescan.p,
+                       c->tiles.p);
+    hipLaunchKernelGGL(k_tile_count, dim3(1), dim3(1), 0, st, npairs, c->tileflag.p, c->tilescan.p, c->ntiles_d.p);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+// scratch sizes of the two device-wide primitives for n atoms / nblk blocks
+size_t neighbor_scratch_bytes(int n, int nblk)
+{
+    size_t a = 0, s = 0;
+    const long long npairs = (long long)nblk * (nblk + 1) / 2;
+    (void)rocprim::radix_sort_keys(nullptr, a, (unsigned long long *)nullptr, (unsigned long long *)nullptr,
+                                   (size_t)n, 0, 64);
+    (void)rocprim::exclusive_scan(nullptr, s, (int *)nullptr, (int *)nullptr, 0, (size_t)npairs,
+                                  rocprim::plus<int>());
+    return std::max(a, s);
+}
+
+} // namespace pmq

@@ -1,0 +1,392 @@
+// pmq_api.cpp -- the C ABI of include/pmq.h: context, uploads, evaluation
+// sequence and result download.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+#include "pmq_internal.h"
+
+namespace pmq {
+size_t neighbor_scratch_bytes(int n, int nblk);
+int polar_setup(pmq_ctx *c);
+int run_polar(pmq_ctx *c, int mode, int precision);
+} // namespace pmq
+
+using namespace pmq;
+
+#define CHECK_CTX(c)                                                                               \
+    do {                                                                                           \
+        if (!(c)) return PMQ_ERR_ARG;                                                              \
+    } while (0)
+
+template <class T>
+static int upload(pmq_ctx *c, DBuf<T> &d, const T *h, size_t count)
+{
+    PMQ_HIP(d.alloc(count));
+    if (count && h) PMQ_HIP(hipMemcpyAsync(d.p, h, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return PMQ_OK;
+}
+
+extern "C" {
+
+int pmq_abi_version(void) { return PMQ_ABI_VERSION; }
+
+int pmq_create(const pmq_config *cfg, pmq_ctx **out)
+{
+    if (!out) return PMQ_ERR_ARG;
+    *out = nullptr;
+    pmq_ctx *c = new pmq_ctx();
+    c->device = cfg ? cfg->device : 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->device || c->device < 0) {
+        delete c;
+        return PMQ_ERR_HIP;
+    }
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return PMQ_ERR_HIP;
+    }
+    if (c->totals.alloc(1) != hipSuccess || c->bbox.alloc(6) != hipSuccess || c->ntiles_d.alloc(1) != hipSuccess) {
+        delete c;
+        return PMQ_ERR_HIP;
+    }
+    *out = c;
+    return PMQ_OK;
+}
+
+int pmq_destroy(pmq_ctx *c)
+{
+    if (!c) return PMQ_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PMQ_OK;
+}
+
+const char *pmq_last_error(const pmq_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
+{
+    CHECK_CTX(c);
+    if (!t || t->n <= 0 || !t->polaxe || !t->zaxis || !t->xaxis || !t->yaxis || !t->n12 || !t->n13 || !t->n14 ||
+        !t->n15) {
+        c->err = "pmq_set_topology: missing arrays or n <= 0";
+        return PMQ_ERR_ARG;
+    }
+    PMQ_HIP(hipSetDevice(c->device));
+    const int n = t->n;
+    c->n = n;
+    c->nblk = (n + PMQ_BLOCK - 1) / PMQ_BLOCK;
+    c->npad = c->nblk * PMQ_BLOCK;
+    c->h_polaxe.assign(t->polaxe, t->polaxe + n);
+    c->h_zaxis.assign(t->zaxis, t->zaxis + n);
+    c->h_xaxis.assign(t->xaxis, t->xaxis + n);
+    c->h_yaxis.assign(t->yaxis, t->yaxis + n);
+    c->h_molcule.assign(n, 0);
+    if (t->molcule) c->h_molcule.assign(t->molcule, t->molcule + n);
+    for (int i = 0; i < n; i++) {
+        const int fr[3] = {t->zaxis[i], t->xaxis[i], std::abs(t->yaxis[i])};
+        if (t->polaxe[i] < PMQ_FRAME_NONE || t->polaxe[i] > PMQ_FRAME_THREEFOLD) {
+            c->err = "pmq_set_topology: bad frame type";
+            return PMQ_ERR_ARG;
+        }
+        for (int k = 0; k < 3; k++)
+            if (fr[k] < 0 || fr[k] > n) {
+                c->err = "pmq_set_topology: frame atom out of range";
+                return PMQ_ERR_ARG;
+            }
+        // the frame atoms each frame type reads (rotpole.cpp:102-345, torque.cpp:140-160)
+        const int f = t->polaxe[i];
+        const int need = f == PMQ_FRAME_NONE ? 0 : f == PMQ_FRAME_ZONLY ? 1
+                       : (f == PMQ_FRAME_ZTHENX || f == PMQ_FRAME_BISECTOR) ? 2 : 3;
+        if ((need >= 1 && fr[0] == 0) || (need >= 2 && fr[1] == 0) || (need >= 3 && fr[2] == 0)) {
+            c->err = "pmq_set_topology: frame type needs a frame atom that is missing";
+            return PMQ_ERR_ARG;
+        }
+    }
+    if (t->nbond < 0 || t->nangle < 0 || (t->nbond && !t->ibnd) || (t->nangle && !t->iang)) {
+        c->err = "pmq_set_topology: bad bond/angle lists";
+        return PMQ_ERR_ARG;
+    }
+    for (long long j = 0; j < 2LL * t->nbond; j++)
+        if (t->ibnd[j] < 0 || t->ibnd[j] >= n) {
+            c->err = "pmq_set_topology: bond atom out of range";
+            return PMQ_ERR_ARG;
+        }
+    for (long long j = 0; j < 3LL * t->nangle; j++)
+        if (t->iang[j] < 0 || t->iang[j] >= n) {
+            c->err = "pmq_set_topology: angle atom out of range";
+            return PMQ_ERR_ARG;
+        }
+    if (t->balist)
+        for (long long j = 0; j < 2LL * t->nangle; j++)
+            if (t->balist[j] < 0 || t->balist[j] >= std::max(t->nbond, 1)) {
+                c->err = "pmq_set_topology: angle bond index out of range";
+                return PMQ_ERR_ARG;
+            }
+    // an atom takes part when it or one of its frame atoms is active (empole.cpp:221,242-243)
+    c->h_usef.assign(n, 1);
+    if (t->use) {
+        for (int i = 0; i < n; i++) {
+            const int iz = t->zaxis[i], ix = t->xaxis[i], iy = std::abs(t->yaxis[i]);
+            c->h_usef[i] = (t->use[i + 1] || t->use[iz] || t->use[ix] || t->use[iy]) ? 1 : 0;
+        }
+    }
+    // scaled pairs i < k; a later connectivity class overrides an earlier one
+    // (empole.cpp:224-235 assigns 1-2, 1-3, 1-4, 1-5 in that order)
+    c->h_excl.clear();
+    {
+        const int *cnt[4] = {t->n12, t->n13, t->n14, t->n15};
+        const int *lst[4] = {t->i12, t->i13, t->i14, t->i15};
+        std::vector<long long> off[4];
+        for (int q = 0; q < 4; q++) {
+            off[q].assign(n + 1, 0);
+            for (int i = 0; i < n; i++) off[q][i + 1] = off[q][i] + cnt[q][i];
+        }
+        std::map<int, int> cls;
+        for (int i = 0; i < n; i++) {
+            cls.clear();
+            for (int q = 0; q < 4; q++)
+                for (long long j = off[q][i]; j < off[q][i + 1]; j++) {
+                    const int k = lst[q][j];
+                    if (k < 0 || k >= n) {
+                        c->err = "pmq_set_topology: connectivity atom out of range";
+                        return PMQ_ERR_ARG;
+                    }
+                    cls[k] = q + 2;
+                }
+            for (auto &kv : cls)
+                if (kv.first > i) c->h_excl.push_back({i, kv.first, kv.second, 0});
+        }
+    }
+    int rc;
+    if ((rc = upload(c, c->polaxe, t->polaxe, n))) return rc;
+    if ((rc = upload(c, c->zaxis, t->zaxis, n))) return rc;
+    if ((rc = upload(c, c->xaxis, t->xaxis, n))) return rc;
+    if ((rc = upload(c, c->yaxis, t->yaxis, n))) return rc;
+    if ((rc = upload(c, c->molcule, c->h_molcule.data(), n))) return rc;
+    if ((rc = upload(c, c->usef, c->h_usef.data(), n))) return rc;
+    if ((rc = upload(c, c->excl, c->h_excl.data(), c->h_excl.size()))) return rc;
+    c->nbond = t->nbond;
+    c->nangle = t->nangle;
+    if ((rc = upload(c, c->ibnd, t->ibnd, 2 * (size_t)t->nbond))) return rc;
+    if ((rc = upload(c, c->iang, t->iang, 3 * (size_t)t->nangle))) return rc;
+    if ((rc = upload(c, c->balist, t->balist, t->balist ? 2 * (size_t)t->nangle : 0))) return rc;
+
+    // per-eval buffers sized for this topology
+    const int npad = c->npad, nblk = c->nblk;
+    const long long npairs = (long long)nblk * (nblk + 1) / 2;
+    PMQ_HIP(c->x.alloc(n)); PMQ_HIP(c->y.alloc(n)); PMQ_HIP(c->z.alloc(n));
+    PMQ_HIP(c->pdelta.alloc(n));
+    PMQ_HIP(c->rpole13.alloc(13 * (size_t)n));
+    PMQ_HIP(c->perm.alloc(npad)); PMQ_HIP(c->iperm.alloc(n));
+    PMQ_HIP(c->sx.alloc(npad)); PMQ_HIP(c->sy.alloc(npad)); PMQ_HIP(c->sz.alloc(npad));
+    PMQ_HIP(c->smp.alloc(10 * (size_t)npad)); PMQ_HIP(c->scp.alloc(3 * (size_t)npad));
+    PMQ_HIP(c->sflag.alloc(npad)); PMQ_HIP(c->smol.alloc(npad));
+    PMQ_HIP(c->keys.alloc(n)); PMQ_HIP(c->keys_sorted.alloc(n));
+    PMQ_HIP(c->blk.alloc(6 * (size_t)nblk));
+    PMQ_HIP(c->tileflag.alloc(npairs)); PMQ_HIP(c->tilescan.alloc(npairs));
+    PMQ_HIP(c->tiles.alloc(npairs));
+    c->ntiles_cap = (int)std::min<long long>(npairs, 1 << 30);
+    c->scratch_sort_bytes = neighbor_scratch_bytes(n, nblk);
+    PMQ_HIP(c->scratch_sort.alloc(c->scratch_sort_bytes));
+    PMQ_HIP(c->dem.alloc(3 * (size_t)n)); PMQ_HIP(c->te.alloc(3 * (size_t)n));
+    PMQ_HIP(c->pot.alloc(n)); PMQ_HIP(c->aem.alloc(n));
+    PMQ_HIP(hipStreamSynchronize(c->stream));
+    c->have_top = true;
+    c->have_prm = false;
+    c->have_xyz = false;
+    return PMQ_OK;
+}
+
+int pmq_set_params(pmq_ctx *c, const pmq_params *p)
+{
+    CHECK_CTX(c);
+    if (!c->have_top) {
+        c->err = "pmq_set_params: call pmq_set_topology first";
+        return PMQ_ERR_STATE;
+    }
+    if (!p || !p->pole) {
+        c->err = "pmq_set_params: pole is required";
+        return PMQ_ERR_ARG;
+    }
+    if (p->pentype < PMQ_PEN_NONE || p->pentype > PMQ_PEN_GORDON2) {
+        c->err = "pmq_set_params: bad pentype";
+        return PMQ_ERR_ARG;
+    }
+    if (p->pentype != PMQ_PEN_NONE && (!p->pcore || !p->pval0 || !p->palpha)) {
+        c->err = "pmq_set_params: charge penetration needs pcore, pval0, palpha";
+        return PMQ_ERR_ARG;
+    }
+    if (p->use_chgflx && (c->nbond + c->nangle > 0) &&
+        (!p->bl || !p->bflx || (c->nangle && (!p->anat || !p->aflx || !p->abflx)))) {
+        c->err = "pmq_set_params: charge flux needs bl, bflx, anat, aflx, abflx";
+        return PMQ_ERR_ARG;
+    }
+    if (p->use_chgflx && c->nangle > 0 && c->balist.n < 2 * (size_t)c->nangle) {
+        c->err = "pmq_set_params: charge flux over angles needs the topology's balist";
+        return PMQ_ERR_ARG;
+    }
+    if (p->electric == 0.0 || p->dielec == 0.0) {
+        c->err = "pmq_set_params: electric and dielec must be nonzero";
+        return PMQ_ERR_ARG;
+    }
+    PMQ_HIP(hipSetDevice(c->device));
+    const int n = c->n;
+    int rc;
+    if ((rc = upload(c, c->pole, p->pole, 13 * (size_t)n))) return rc;
+    const bool cp = p->pentype != PMQ_PEN_NONE;
+    if ((rc = upload(c, c->pcore, cp ? p->pcore : nullptr, n))) return rc;
+    if ((rc = upload(c, c->pval0, cp ? p->pval0 : nullptr, n))) return rc;
+    if ((rc = upload(c, c->palpha, cp ? p->palpha : nullptr, n))) return rc;
+    if (p->use_chgflx) {
+        if ((rc = upload(c, c->bl, p->bl, c->nbond))) return rc;
+        if ((rc = upload(c, c->bflx, p->bflx, c->nbond))) return rc;
+        if ((rc = upload(c, c->anat, p->anat, c->nangle))) return rc;
+        if ((rc = upload(c, c->aflx, p->aflx, 2 * (size_t)c->nangle))) return rc;
+        if ((rc = upload(c, c->abflx, p->abflx, 2 * (size_t)c->nangle))) return rc;
+    }
+    c->prm = *p;
+    c->prm.pole = nullptr;
+    c->prm.pcore = c->prm.pval0 = c->prm.palpha = nullptr;
+    c->prm.bl = c->prm.bflx = c->prm.anat = c->prm.aflx = c->prm.abflx = nullptr;
+    c->prm.polarity = c->prm.thole = c->prm.pdamp = nullptr;
+    PMQ_HIP(hipStreamSynchronize(c->stream));
+    c->have_prm = true;
+    return PMQ_OK;
+}
+
+int pmq_set_coords(pmq_ctx *c, const double *x, const double *y, const double *z, const pmq_box *box)
+{
+    CHECK_CTX(c);
+    if (!c->have_top) {
+        c->err = "pmq_set_coords: call pmq_set_topology first";
+        return PMQ_ERR_STATE;
+    }
+    if (!x || !y || !z) {
+        c->err = "pmq_set_coords: null coordinates";
+        return PMQ_ERR_ARG;
+    }
+    if (box && (box->kind < PMQ_BOX_NONE || box->kind > PMQ_BOX_DODECA)) {
+        c->err = "pmq_set_coords: bad box kind";
+        return PMQ_ERR_ARG;
+    }
+    if (box && box->kind != PMQ_BOX_NONE && !(box->xbox > 0 && box->ybox > 0 && box->zbox > 0)) {
+        c->err = "pmq_set_coords: box lengths must be positive";
+        return PMQ_ERR_ARG;
+    }
+    PMQ_HIP(hipSetDevice(c->device));
+    const size_t bytes = c->n * sizeof(double);
+    PMQ_HIP(hipMemcpyAsync(c->x.p, x, bytes, hipMemcpyHostToDevice, c->stream));
+    PMQ_HIP(hipMemcpyAsync(c->y.p, y, bytes, hipMemcpyHostToDevice, c->stream));
+    PMQ_HIP(hipMemcpyAsync(c->z.p, z, bytes, hipMemcpyHostToDevice, c->stream));
+    c->box = make_box(box);
+    PMQ_HIP(hipStreamSynchronize(c->stream));
+    c->have_xyz = true;
+    return PMQ_OK;
+}
+
+int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
+{
+    CHECK_CTX(c);
+    if (!c->have_top || !c->have_prm || !c->have_xyz) {
+        c->err = "pmq_compute: topology, parameters and coordinates must be set first";
+        return PMQ_ERR_STATE;
+    }
+    if (mode < PMQ_ENERGY || mode > PMQ_VIRIAL) {
+        c->err = "pmq_compute: bad mode";
+        return PMQ_ERR_ARG;
+    }
+    if (precision != PMQ_PLAIN_FP64) {
+        c->err = "pmq_compute: only the plain FP64 form is built";
+        return PMQ_ERR_UNSUPPORTED;
+    }
+    if (terms & ~(unsigned)PMQ_TERM_MPOLE) {
+        c->err = "pmq_compute: only the multipole term is built";
+        return PMQ_ERR_UNSUPPORTED;
+    }
+    if (c->prm.use_ewald) {
+        c->err = "pmq_compute: Ewald summation is not built";
+        return PMQ_ERR_UNSUPPORTED;
+    }
+    PMQ_HIP(hipSetDevice(c->device));
+    const int n = c->n;
+    int rc;
+    if ((rc = launch_zero(c, (double *)c->totals.p, sizeof(Totals) / sizeof(double)))) return rc;
+    const bool grad = mode == PMQ_GRADIENT || mode == PMQ_VIRIAL;
+    if (grad) {
+        if ((rc = launch_zero(c, c->dem.p, 3 * (size_t)n))) return rc;
+        if ((rc = launch_zero(c, c->te.p, 3 * (size_t)n))) return rc;
+        if ((rc = launch_zero(c, c->pot.p, n))) return rc;
+    }
+    if (mode == PMQ_ANALYSIS && (rc = launch_zero(c, c->aem.p, n))) return rc;
+    if (terms & PMQ_TERM_MPOLE) {
+        const double cut = c->box.kind == PMQ_BOX_NONE ? std::min(c->prm.mpolecut, 1e12) : c->prm.mpolecut;
+        const bool cf = c->prm.pentype != PMQ_PEN_NONE && c->prm.use_chgflx;
+        if (c->prm.use_chgflx && (rc = launch_chgflx_delta(c))) return rc;
+        if ((rc = build_neighbors(c, cut))) return rc;
+        if ((rc = launch_rotpole(c, true))) return rc;
+        if ((rc = launch_mpole_direct(c, mode, cut * cut))) return rc;
+        if (grad) {
+            if ((rc = launch_torque(c, mode))) return rc;
+            if (cf && (rc = launch_chgflx_grad(c, mode))) return rc;
+        }
+    }
+    c->last_mode = mode;
+    c->last_terms = terms;
+    c->last_precision = precision;
+    return PMQ_OK;
+}
+
+int pmq_sync(pmq_ctx *c)
+{
+    CHECK_CTX(c);
+    PMQ_HIP(hipStreamSynchronize(c->stream));
+    return PMQ_OK;
+}
+
+int pmq_fetch(pmq_ctx *c, pmq_result *r)
+{
+    CHECK_CTX(c);
+    if (!r) return PMQ_ERR_ARG;
+    PMQ_HIP(hipSetDevice(c->device));
+    PMQ_HIP(hipStreamSynchronize(c->stream));
+    Totals t;
+    PMQ_HIP(hipMemcpy(&t, c->totals.p, sizeof t, hipMemcpyDeviceToHost));
+    const int n = c->n;
+    const bool grad = c->last_mode == PMQ_GRADIENT || c->last_mode == PMQ_VIRIAL;
+    r->em = t.em;
+    r->ep = t.ep;
+    r->er = t.er;
+    r->edsp = t.edsp;
+    r->esum = t.em + t.ep + t.er + t.edsp;
+    r->einter = t.einter;
+    r->nem = (long long)t.nem;
+    for (int k = 0; k < 9; k++) r->vir[k] = c->last_mode == PMQ_VIRIAL ? t.vir[k] : 0.0;
+    r->pcg_iters = t.pcg_iters;
+    if (r->dem && grad) PMQ_HIP(hipMemcpy(r->dem, c->dem.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    if (r->desum && grad) PMQ_HIP(hipMemcpy(r->desum, c->dem.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    if (r->aem && c->last_mode == PMQ_ANALYSIS)
+        PMQ_HIP(hipMemcpy(r->aem, c->aem.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (r->rpole) PMQ_HIP(hipMemcpy(r->rpole, c->rpole13.p, 13 * n * sizeof(double), hipMemcpyDeviceToHost));
+    if (!std::isfinite(r->esum)) {
+        c->err = "pmq_fetch: non-finite total energy";
+        return PMQ_ERR_NAN;
+    }
+    return PMQ_OK;
+}
+
+int pmq_eval(pmq_ctx *c, int mode, unsigned terms, int precision, const double *x, const double *y, const double *z,
+             const pmq_box *box, pmq_result *res)
+{
+    int rc = pmq_set_coords(c, x, y, z, box);
+    if (rc) return rc;
+    if ((rc = pmq_compute(c, mode, terms, precision))) return rc;
+    return pmq_fetch(c, res);
+}
+
+} // extern "C"

@@ -1,0 +1,218 @@
+// pmq_internal.h -- shared definitions of the HIP library behind include/pmq.h.
+//
+// Data layout in HBM (one context = one system on one GPU):
+//   * coordinates x,y,z are kept in the caller's atom order (double[n] each);
+//   * every eval sorts atoms spatially (cells in Morton order, stable by atom
+//     index) and the pair kernels work on "sorted" SoA copies: positions,
+//     the 10 rotated multipole components, and per-atom flags, padded to a
+//     multiple of 64 (one wavefront per 64-atom block);
+//   * pair work is a list of (I,J) block tiles, I <= J, whose bounding boxes
+//     come within the cutoff; each tile is one wavefront;
+//   * forces, torques and per-atom potentials are accumulated in the
+//     caller's atom order, so the frame (torque) and charge-flux kernels
+//     index frame atoms directly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pmq.h"
+
+#define PMQ_BLOCK 64
+
+namespace pmq {
+
+// ---------------------------------------------------------------------------
+// periodic box as the kernels see it (lattice constants of lattice.cpp)
+struct DevBox {
+    int kind;
+    double xbox, ybox, zbox;
+    double beta_sin, beta_cos, gamma_sin, gamma_cos, beta_term, gamma_term;
+    double recip[3][3]; // fractional = recip * cartesian (rows a*, b*, c*)
+    double lvec[3][3];  // cartesian = lvec * fractional (columns a, b, c)
+};
+
+DevBox make_box(const pmq_box *b);
+
+// minimum image of one displacement, the reference's image() (image.cpp:37-124)
+template <class R>
+__device__ __host__ inline void image_wrap(R &v, R cell)
+{
+    const R half = R(0.5) * cell;
+    while (fabs(v) > half) v -= copysign(cell, v);
+}
+
+template <class R>
+__device__ __host__ inline void image(const DevBox &b, R &x, R &y, R &z)
+{
+    switch (b.kind) {
+    case PMQ_BOX_ORTHO:
+        image_wrap<R>(x, R(b.xbox)); image_wrap<R>(y, R(b.ybox)); image_wrap<R>(z, R(b.zbox));
+        break;
+    case PMQ_BOX_MONO:
+        z = z / R(b.beta_sin);
+        x = x - z * R(b.beta_cos);
+        image_wrap<R>(x, R(b.xbox)); image_wrap<R>(y, R(b.ybox)); image_wrap<R>(z, R(b.zbox));
+        x = x + z * R(b.beta_cos);
+        z = z * R(b.beta_sin);
+        break;
+    case PMQ_BOX_TRI:
+        z = z / R(b.gamma_term);
+        y = (y - z * R(b.beta_term)) / R(b.gamma_sin);
+        x = x - y * R(b.gamma_cos) - z * R(b.beta_cos);
+        image_wrap<R>(x, R(b.xbox)); image_wrap<R>(y, R(b.ybox)); image_wrap<R>(z, R(b.zbox));
+        x = x + y * R(b.gamma_cos) + z * R(b.beta_cos);
+        y = y * R(b.gamma_sin) + z * R(b.beta_term);
+        z = z * R(b.gamma_term);
+        break;
+    case PMQ_BOX_OCTA:
+        image_wrap<R>(x, R(b.xbox)); image_wrap<R>(y, R(b.ybox)); image_wrap<R>(z, R(b.zbox));
+        if (fabs(x) + fabs(y) + fabs(z) > R(0.75) * R(b.xbox)) {
+            x -= copysign(R(0.5) * R(b.xbox), x);
+            y -= copysign(R(0.5) * R(b.ybox), y);
+            z -= copysign(R(0.5) * R(b.zbox), z);
+        }
+        break;
+    case PMQ_BOX_DODECA: {
+        const R root2 = R(1.41421356237309504880);
+        image_wrap<R>(x, R(b.xbox)); image_wrap<R>(y, R(b.ybox));
+        z = z - root2 * R(b.zbox) * round(z / (R(b.zbox) * root2));
+        const R corr = R(0.5) * R(b.xbox) * R((int)(fabs(x / R(b.xbox)) + fabs(y / R(b.ybox)) + fabs(root2 * z / R(b.zbox))));
+        x -= copysign(corr, x);
+        y -= copysign(corr, y);
+        z -= copysign(corr, z) * root2;
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// device buffers
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    hipError_t alloc(size_t count)
+    {
+        if (count <= n && p) return hipSuccess;
+        release();
+        n = count;
+        return hipMalloc((void **)&p, (count ? count : 1) * sizeof(T));
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// energy / virial / count accumulators (one per context, in device memory)
+struct Totals {
+    double em, ep, er, edsp;
+    double einter;
+    unsigned long long nem;
+    double vir[9];
+    // fixed-point accumulators of the mixed path
+    long long em_fx, ep_fx;
+    long long vir_fx[9];
+    int pcg_iters;
+    int pad;
+};
+
+// scale classes of an excluded pair (connectivity 1-2 .. 1-5)
+struct ExclPair {
+    int i, k;   // original atom indices, i < k
+    int cls;    // 2..5
+    int pad;
+};
+
+} // namespace pmq
+
+// ---------------------------------------------------------------------------
+struct pmq_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // sizes
+    int n = 0;      // atoms
+    int npad = 0;   // atoms padded to a whole number of 64-atom blocks
+    int nblk = 0;   // blocks
+    bool have_top = false, have_prm = false, have_xyz = false;
+
+    // ---- topology (host copies kept for the exclusion list)
+    std::vector<int> h_polaxe, h_zaxis, h_xaxis, h_yaxis, h_molcule, h_usef;
+    std::vector<pmq::ExclPair> h_excl;
+    int nbond = 0, nangle = 0;
+
+    // ---- device: static per topology / params
+    pmq::DBuf<int> polaxe, zaxis, xaxis, yaxis, molcule, usef;
+    pmq::DBuf<pmq::ExclPair> excl;
+    pmq::DBuf<double> pole;          // 13n local
+    pmq::DBuf<double> pcore, pval0, palpha;
+    pmq::DBuf<int> ibnd, iang, balist;
+    pmq::DBuf<double> bl, bflx, anat, aflx, abflx;
+    pmq_params prm{};                // scalars only (pointers cleared)
+
+    // ---- device: per eval
+    pmq::DBuf<double> x, y, z;
+    pmq::DevBox box{};
+    pmq::DBuf<double> pdelta;        // charge-flux monopole shift
+    pmq::DBuf<double> rpole13;       // 13n global-frame multipoles, caller order
+    // sorted SoA
+    pmq::DBuf<int> perm, iperm;      // sorted -> original, original -> sorted
+    pmq::DBuf<double> sx, sy, sz;    // npad each
+    pmq::DBuf<double> smp;           // 10 * npad rotated multipoles (charge = valence if CP)
+    pmq::DBuf<double> scp;           // 3 * npad: core, alpha (CP), spare
+    pmq::DBuf<int> sflag;            // npad: bit0 valid, bit1 active(use)
+    pmq::DBuf<int> smol;             // npad molecule ids
+    // neighbor structure
+    pmq::DBuf<unsigned long long> keys, keys_sorted;
+    pmq::DBuf<double> blk;           // 6 * nblk: center xyz, half extent xyz
+    pmq::DBuf<int> tileflag, tilescan;
+    pmq::DBuf<int2> tiles;
+    pmq::DBuf<int> ntiles_d;
+    pmq::DBuf<double> bbox;          // 6: min xyz, max xyz (non-periodic systems)
+    pmq::DBuf<unsigned char> scratch_sort;
+    size_t scratch_sort_bytes = 0;
+    int ntiles = 0;
+    int ntiles_cap = 0;
+    // outputs
+    pmq::DBuf<double> dem, te, pot, aem;
+    pmq::DBuf<pmq::Totals> totals;
+
+    int last_mode = PMQ_ENERGY;
+    unsigned last_terms = 0;
+    int last_precision = PMQ_PLAIN_FP64;
+};
+
+// ---------------------------------------------------------------------------
+// launchers (defined in the .hip files)
+namespace pmq {
+int build_neighbors(pmq_ctx *c, double cutoff);
+int launch_rotpole(pmq_ctx *c, bool want_rpole13);
+int launch_chgflx_delta(pmq_ctx *c);
+int launch_mpole_direct(pmq_ctx *c, int mode, double off2);
+int launch_torque(pmq_ctx *c, int mode);
+int launch_chgflx_grad(pmq_ctx *c, int mode);
+int launch_zero(pmq_ctx *c, double *p, size_t n);
+} // namespace pmq
+
+#define PMQ_HIP(call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) {                                                                    \
+            c->err = std::string(#call) + ": " + hipGetErrorString(e_);                            \
+            return PMQ_ERR_HIP;                                                                    \
+        }                                                                                          \
+    } while (0)

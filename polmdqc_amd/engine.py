@@ -1,0 +1,190 @@
+"""Python front end over the C ABI: one Engine = one pmq context on one GPU.
+
+Systems are described by a dict whose keys are the reference's global arrays
+(``x, y, z, pole, polaxe, zaxis, xaxis, yaxis, use, molcule, n12, i12, ...,
+m2scale, electric, dielec, pentype, ...``), i.e. what ``mechanic()`` leaves
+in the modules of include/md/*.h.  Arrays are numpy; connectivity lists are
+flat (counts ``n1x`` plus concatenated 0-based lists ``i1x``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class PmqError(RuntimeError):
+    pass
+
+
+def box_from(sys: dict) -> L.Box:
+    """Box struct from the reference's box globals (boxes.h flags)."""
+    if not int(sys.get("use_bounds", 0)):
+        kind = L.BOX_NONE
+    elif int(sys.get("octahedron", 0)):
+        kind = L.BOX_OCTA
+    elif int(sys.get("dodecadron", 0)):
+        kind = L.BOX_DODECA
+    elif int(sys.get("triclinic", 0)):
+        kind = L.BOX_TRI
+    elif int(sys.get("monoclinic", 0)):
+        kind = L.BOX_MONO
+    else:
+        kind = L.BOX_ORTHO
+    return L.Box(kind, float(sys.get("xbox", 0)), float(sys.get("ybox", 0)), float(sys.get("zbox", 0)),
+                 float(sys.get("alpha", 90)), float(sys.get("beta", 90)), float(sys.get("gamma", 90)))
+
+
+class Engine:
+    def __init__(self, device: int = 0):
+        self._L = L.lib()
+        self._ctx = C.c_void_p()
+        cfg = L.Config(device)
+        rc = self._L.pmq_create(C.byref(cfg), C.byref(self._ctx))
+        if rc != 0:
+            raise PmqError(f"pmq_create failed: {L.ERRORS.get(rc, rc)}")
+        self._keep = []
+        self.n = 0
+
+    # ------------------------------------------------------------------
+    def close(self):
+        if self._ctx:
+            self._L.pmq_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self._L.pmq_last_error(self._ctx).decode(errors="replace")
+            raise PmqError(f"{what}: {L.ERRORS.get(rc, rc)}: {msg}")
+
+    def _d(self, a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self._keep.append(a)
+        return a.ctypes.data_as(_dp)
+
+    def _i(self, a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        if a.size == 0:
+            a = np.zeros(1, dtype=np.int32)
+        self._keep.append(a)
+        return a.ctypes.data_as(_ip)
+
+    # ------------------------------------------------------------------
+    def set_system(self, s: dict):
+        """Upload topology and parameters of a system dict."""
+        self._keep = []
+        n = int(s["n"])
+        t = L.Topology()
+        t.n = n
+        t.polaxe, t.zaxis, t.xaxis, t.yaxis = (self._i(s[k]) for k in ("polaxe", "zaxis", "xaxis", "yaxis"))
+        t.use = self._i(s.get("use"))
+        t.molcule = self._i(s.get("molcule"))
+        for a in ("12", "13", "14", "15"):
+            setattr(t, "n" + a, self._i(s["n" + a]))
+            setattr(t, "i" + a, self._i(s["i" + a]))
+        for a in ("11", "12", "13", "14"):
+            if "np" + a in s:
+                setattr(t, "np" + a, self._i(s["np" + a]))
+                setattr(t, "ip" + a, self._i(s["ip" + a]))
+        t.nbond = int(s.get("nbond", 0))
+        t.ibnd = self._i(s.get("ibnd", np.zeros(0)))
+        t.nangle = int(s.get("nangle", 0))
+        t.iang = self._i(s.get("iang", np.zeros(0)))
+        bal = s.get("balist")
+        t.balist = self._i(bal) if bal is not None and len(bal) else None
+        self._check(self._L.pmq_set_topology(self._ctx, C.byref(t)), "pmq_set_topology")
+
+        p = L.Params()
+        p.pole = self._d(s["pole"])
+        p.m2scale, p.m3scale = float(s["m2scale"]), float(s["m3scale"])
+        p.m4scale, p.m5scale = float(s["m4scale"]), float(s["m5scale"])
+        p.electric, p.dielec = float(s["electric"]), float(s["dielec"])
+        p.pentype = int(s.get("pentype", 0))
+        if p.pentype:
+            p.pcore, p.pval0, p.palpha = self._d(s["pcore"]), self._d(s["pval0"]), self._d(s["palpha"])
+        p.use_chgflx = int(s.get("use_chgflx", 0))
+        if p.use_chgflx:
+            pole = np.array(s["pole"], dtype=np.float64).reshape(n, 13).copy()
+            pole[:, 0] = s["mono0"]
+            p.pole = self._d(pole)
+            for k in ("bl", "bflx", "anat", "aflx", "abflx"):
+                setattr(p, k, self._d(s[k]))
+        for k in ("polarity", "thole", "pdamp"):
+            if k in s:
+                setattr(p, k, self._d(s[k]))
+        for k in ("p2scale", "p3scale", "p4scale", "p5scale", "p2iscale", "p3iscale", "p4iscale", "p5iscale",
+                  "d1scale", "d2scale", "d3scale", "d4scale", "u1scale", "u2scale", "u3scale", "u4scale",
+                  "poleps"):
+            if k in s:
+                setattr(p, k, float(s[k]))
+        p.politer = int(s.get("politer", 100))
+        p.use_ewald = int(s.get("use_ewald", 0))
+        p.mpolecut = float(s.get("mpolecut", 1e12))
+        p.ewaldcut = float(s.get("ewaldcut", 7.0))
+        p.aewald = float(s.get("aewald", 0.0))
+        p.nfft1, p.nfft2, p.nfft3 = int(s.get("nfft1", 0)), int(s.get("nfft2", 0)), int(s.get("nfft3", 0))
+        p.bsorder = int(s.get("bsorder", 5))
+        self._check(self._L.pmq_set_params(self._ctx, C.byref(p)), "pmq_set_params")
+        self.n = n
+        self.box = box_from(s)
+        self.set_coords(np.stack([s["x"], s["y"], s["z"]], 1))
+
+    def set_coords(self, xyz: np.ndarray, box: L.Box | None = None):
+        xyz = np.asarray(xyz, dtype=np.float64)
+        if box is not None:
+            self.box = box
+        x, y, z = (np.ascontiguousarray(xyz[:, a]) for a in range(3))
+        self._check(self._L.pmq_set_coords(self._ctx, x.ctypes.data_as(_dp), y.ctypes.data_as(_dp),
+                                           z.ctypes.data_as(_dp), C.byref(self.box)), "pmq_set_coords")
+
+    def compute(self, mode: int = L.VIRIAL, terms: int = L.TERM_MPOLE, precision: int = L.PLAIN_FP64):
+        """Enqueue one evaluation on the device (asynchronous)."""
+        self._check(self._L.pmq_compute(self._ctx, mode, terms, precision), "pmq_compute")
+
+    def sync(self):
+        self._check(self._L.pmq_sync(self._ctx), "pmq_sync")
+
+    def fetch(self, want_rpole: bool = False) -> dict:
+        n = self.n
+        dem = np.zeros(3 * n)
+        aem = np.zeros(n)
+        rpole = np.zeros(13 * n) if want_rpole else None
+        uind = np.zeros(3 * n)
+        uinp = np.zeros(3 * n)
+        dep = np.zeros(3 * n)
+        desum = np.zeros(3 * n)
+        r = L.Result()
+        r.dem, r.aem = dem.ctypes.data_as(_dp), aem.ctypes.data_as(_dp)
+        r.dep, r.desum = dep.ctypes.data_as(_dp), desum.ctypes.data_as(_dp)
+        r.rpole = rpole.ctypes.data_as(_dp) if want_rpole else None
+        r.uind, r.uinp = uind.ctypes.data_as(_dp), uinp.ctypes.data_as(_dp)
+        self._check(self._L.pmq_fetch(self._ctx, C.byref(r)), "pmq_fetch")
+        out = {"em": r.em, "ep": r.ep, "er": r.er, "edsp": r.edsp, "esum": r.esum, "einter": r.einter,
+               "nem": int(r.nem), "vir": np.array(r.vir[:]).reshape(3, 3), "pcg_iters": r.pcg_iters,
+               "dem": dem.reshape(n, 3), "dep": dep.reshape(n, 3), "desum": desum.reshape(n, 3), "aem": aem,
+               "uind": uind.reshape(n, 3), "uinp": uinp.reshape(n, 3)}
+        if want_rpole:
+            out["rpole"] = rpole.reshape(n, 13)
+        return out
+
+    def eval(self, mode: int = L.VIRIAL, terms: int = L.TERM_MPOLE, precision: int = L.PLAIN_FP64,
+             xyz: np.ndarray | None = None, want_rpole: bool = False) -> dict:
+        if xyz is not None:
+            self.set_coords(xyz)
+        self.compute(mode, terms, precision)
+        return self.fetch(want_rpole)
