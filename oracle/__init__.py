@@ -162,3 +162,64 @@ def pair_mpole(r, mscale, f, mi, mk):
     lib().orc_pair_mpole(float(r @ r), float(r[0]), float(r[1]), float(r[2]), float(mscale), float(f),
                          mi.ctypes.data_as(_dp), mk.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
     return out
+
+
+class ElecSys(C.Structure):
+    _fields_ = [("mp", C.POINTER(MpoleSys)), ("use_ewald", C.c_int), ("aewald", C.c_double), ("cut", C.c_double),
+                ("nfft1", C.c_int), ("nfft2", C.c_int), ("nfft3", C.c_int), ("bsorder", C.c_int),
+                ("use_polar", C.c_int), ("polarity", _dp), ("thole", _dp), ("pdamp", _dp),
+                ("np11", _ip), ("ip11", _ip), ("np12", _ip), ("ip12", _ip),
+                ("np13", _ip), ("ip13", _ip), ("np14", _ip), ("ip14", _ip)] + \
+               [(k, C.c_double) for k in ("p2scale", "p3scale", "p4scale", "p5scale", "p2iscale", "p3iscale",
+                                          "p4iscale", "p5iscale", "d1scale", "d2scale", "d3scale", "d4scale",
+                                          "u1scale", "u2scale", "u3scale", "u4scale", "poleps")] + \
+               [("politer", C.c_int), ("solver", C.c_int)]
+
+
+class ElecOut(C.Structure):
+    _fields_ = [("em", C.c_double), ("ep", C.c_double), ("em_real", C.c_double), ("em_recip", C.c_double),
+                ("em_self", C.c_double), ("iters", C.c_int), ("dem", _dp), ("dep", _dp), ("uind", _dp),
+                ("uinp", _dp)]
+
+
+def elec(fx, want_grad: bool = True, solver: int = 0, x=None, poleps: float | None = None,
+         polar: bool | None = None) -> dict:
+    """Ewald/PME multipoles and AMOEBA polarization on a system dict."""
+    L = lib()
+    if not hasattr(L, "_elec_ready"):
+        L.orc_elec.argtypes = [C.POINTER(ElecSys), C.c_int, C.POINTER(ElecOut)]
+        L.orc_elec.restype = C.c_int
+        L._elec_ready = True
+    keep = _Keep()
+    mp = mpole_sys(fx, keep, x=x)
+    s = ElecSys()
+    s.mp = C.pointer(mp)
+    s.use_ewald = int(fx.get("use_ewald", 0))
+    s.aewald = float(fx.get("aewald", 0.0))
+    s.cut = float(fx["ewaldcut"]) if s.use_ewald else float(fx["mpolecut"])
+    s.nfft1, s.nfft2, s.nfft3 = int(fx.get("nfft1", 0)), int(fx.get("nfft2", 0)), int(fx.get("nfft3", 0))
+    s.bsorder = int(fx.get("bsorder", 5))
+    s.use_polar = int(fx.get("use_polar", 1)) if polar is None else int(polar)
+    n = int(fx["n"])
+    s.polarity, s.thole, s.pdamp = keep.d(fx["polarity"]), keep.d(fx["thole"]), keep.d(fx["pdamp"])
+    for a in ("11", "12", "13", "14"):
+        cnt = fx.get("np" + a, np.zeros(n, np.int32))
+        lst = fx.get("ip" + a, np.zeros(0, np.int32))
+        setattr(s, "np" + a, keep.i(cnt))
+        setattr(s, "ip" + a, keep.i(lst if len(lst) else np.zeros(1)))
+    for k in ("p2scale", "p3scale", "p4scale", "p5scale", "p2iscale", "p3iscale", "p4iscale", "p5iscale",
+              "d1scale", "d2scale", "d3scale", "d4scale", "u1scale", "u2scale", "u3scale", "u4scale"):
+        setattr(s, k, float(fx[k]))
+    s.poleps = float(fx.get("poleps", 1e-6)) if poleps is None else poleps
+    s.politer = int(fx.get("politer", 100))
+    s.solver = solver
+    o = ElecOut()
+    dem, dep, ud, up = (np.zeros(3 * n) for _ in range(4))
+    o.dem, o.dep = dem.ctypes.data_as(_dp), dep.ctypes.data_as(_dp)
+    o.uind, o.uinp = ud.ctypes.data_as(_dp), up.ctypes.data_as(_dp)
+    rc = L.orc_elec(C.byref(s), 1 if want_grad else 0, C.byref(o))
+    if rc != 0:
+        raise RuntimeError(f"orc_elec failed ({rc})")
+    return {"em": o.em, "ep": o.ep, "em_real": o.em_real, "em_recip": o.em_recip, "em_self": o.em_self,
+            "iters": o.iters, "dem": dem.reshape(n, 3), "dep": dep.reshape(n, 3), "uind": ud.reshape(n, 3),
+            "uinp": up.reshape(n, 3)}

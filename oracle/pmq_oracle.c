@@ -384,7 +384,7 @@ void orc_image(const orc_box *b, double *xr, double *yr, double *zr)
  * Local frames: chkpole.cpp:44-89, rotmat rotpole.cpp:102-345,
  * rotsite rotpole.cpp:356-425
  * ---------------------------------------------------------------------- */
-static void chkpole_all(orc_mpole_sys *s, double *pole)
+void orc_chkpole(orc_mpole_sys *s, double *pole)
 {
     for (int i = 0; i < s->n; i++) {
         const int k = s->yaxis[i];
@@ -555,7 +555,7 @@ static void unitv(double *v, double *len)
     v[0] /= *len; v[1] /= *len; v[2] /= *len;
 }
 
-static void torque_all(const orc_mpole_sys *s, const double *te, double *de, double *vir)
+void orc_torque(const orc_mpole_sys *s, const double *te, double *de, double *vir)
 {
     for (int i = 0; i < s->n; i++) {
         const int frame = s->polaxe[i];
@@ -828,7 +828,7 @@ int orc_empole(orc_mpole_sys *s, int mode, orc_mpole_out *out)
         }
         free(pdelta);
     }
-    chkpole_all(s, pole);
+    orc_chkpole(s, pole);
     orc_rotpole(s, pole, rpole);
 
     double em = 0.0, einter = 0.0;
@@ -902,7 +902,7 @@ int orc_empole(orc_mpole_sys *s, int mode, orc_mpole_out *out)
         for (int j = o15[i]; j < o15[i + 1]; j++) mscale[s->i15[j]] = 1.0;
     }
     if (do_g && dem) {
-        torque_all(s, te, dem, do_v ? vloc : NULL);
+        orc_torque(s, te, dem, do_v ? vloc : NULL);
         if (cf) charge_flux_grad(s, pot, dem, do_v ? vloc : NULL);
     }
     out->em = em;

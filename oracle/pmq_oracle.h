@@ -86,6 +86,43 @@ void orc_rotpole(const orc_mpole_sys *s, const double *pole, double *rpole);
 void orc_pair_mpole(double r2, double xr, double yr, double zr, double mscale, double f,
                     const double *mi, const double *mk, double *out);
 
+/* chkpole (chkpole.cpp:24-89): flips pole rows and yaxis signs in place */
+void orc_chkpole(orc_mpole_sys *s, double *pole);
+
+/* torque -> gradient on frame atoms, torque.cpp:42-397 (vir may be NULL) */
+void orc_torque(const orc_mpole_sys *s, const double *te, double *de, double *vir);
+
+/* ---------------------------------------------------------------------
+ * Part 2: Ewald multipoles and AMOEBA polarization (pmq_oracle_elec.c)
+ * ------------------------------------------------------------------- */
+typedef struct {
+    orc_mpole_sys *mp;      /* multipoles, frames, connectivity, box, f */
+    int use_ewald;
+    double aewald;          /* Ewald coefficient, 1/A */
+    double cut;             /* real-space cutoff, A (ewaldcut, or mpolecut without Ewald) */
+    int nfft1, nfft2, nfft3, bsorder;
+    int use_polar;
+    const double *polarity, *thole, *pdamp; /* n each */
+    const int *np11, *ip11, *np12, *ip12, *np13, *ip13, *np14, *ip14; /* polarization groups */
+    double p2scale, p3scale, p4scale, p5scale;
+    double p2iscale, p3iscale, p4iscale, p5iscale;
+    double d1scale, d2scale, d3scale, d4scale;
+    double u1scale, u2scale, u3scale, u4scale;
+    double poleps;
+    int politer;
+    int solver;             /* 0 preconditioned CG, 1 dense direct solve */
+} orc_elec_sys;
+
+typedef struct {
+    double em, ep;
+    double em_real, em_recip, em_self;
+    int iters;              /* CG iterations (negative if not converged) */
+    double *dem, *dep;      /* 3n gradients (may be NULL) */
+    double *uind, *uinp;    /* 3n induced dipoles (may be NULL) */
+} orc_elec_out;
+
+int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,1051 @@
+/*
+ * pmq_oracle_elec.c -- FP64 CPU restatement of the electrostatics the
+ * reference declares but does not compute: Ewald-summed permanent
+ * multipoles (real space, smooth PME reciprocal space, self energy) and
+ * mutual induced-dipole polarization (direct field, preconditioned CG,
+ * energy and analytic gradient), for AMOEBA (Thole damping).
+ *
+ * TEST INFRASTRUCTURE ONLY (see pmq_oracle.h).
+ *
+ * Nothing here can be compared with the reference, which computes none of
+ * it (empole.cpp:53-60 Ewald branches are empty; epolar is commented out at
+ * energy.cpp:267).  The reference contributes the state these terms read:
+ * polarizabilities and Thole factors (kpolar.cpp:461-537), polarization
+ * groups (kpolar.cpp:592-853), p/d/u scale factors (polpot.h:47-66), Ewald
+ * coefficient and grid (kewald.cpp:40-349), PCG defaults (kpolar.cpp:80-103).
+ * The algorithms restated are the published ones the reference cites:
+ *   - smooth PME for multipoles: Essmann et al., J. Chem. Phys. 103, 8577
+ *     (1995); Sagui, Pedersen and Darden, J. Chem. Phys. 120, 73 (2004);
+ *   - AMOEBA polarization with Thole damping and mutual induction:
+ *     Ren and Ponder, J. Phys. Chem. B 107, 5933 (2003).
+ * Every piece is pinned by tests/test_oracle_elec.py: direct lattice sums,
+ * independence of the Ewald coefficient, a dense linear solve, central
+ * finite differences of the energy, and the variational property.
+ *
+ * Energy definitions used (f = electric/dielec):
+ *   E_mpole = real space [f (B_n - (1-m) rr_n) series] + reciprocal + self,
+ *     self = -f a/sqrt(pi) sum_i [c^2 + (2a^2/3) d.d + (8a^4/5) q:q]
+ *     (q in the reference's traceless x1/3 convention);
+ *   E_pol = -(f/2) sum_i mu_d,i . E^p_i, with mu_d solving
+ *     mu_d = alpha (E^d + T mu_d) and mu_p = alpha (E^p + T mu_p),
+ *     E^d/E^p the permanent field with d/p exclusion scaling and T the
+ *     u-scaled dipole field operator, both Thole damped and Ewald summed.
+ */
+#include "pmq_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SQRTPI 1.77245385090551602730
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+static inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* ---------------------------------------------------------------------
+ * generic multipole pair (same algebra as pair_series in pmq_oracle.c;
+ * restated here so this file stands alone)
+ * ------------------------------------------------------------------- */
+typedef struct {
+    double e, frc[3], trqi[3], trqk[3];
+} gpair;
+
+static inline void qmul(const double *q, const double *v, double *y)
+{
+    y[0] = q[0] * v[0] + q[1] * v[1] + q[2] * v[2];
+    y[1] = q[1] * v[0] + q[3] * v[1] + q[4] * v[2];
+    y[2] = q[2] * v[0] + q[4] * v[1] + q[5] * v[2];
+}
+static inline void cross3(const double *a, const double *b, double *c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static void gpair_eval(const double *R, const double *mi, const double *mk, const double *r, int do_g, gpair *o)
+{
+    const double ci = mi[0], ck = mk[0];
+    const double *di = mi + 1, *dk = mk + 1, *qi = mi + 4, *qk = mk + 4;
+    double qirv[3], qkrv[3];
+    qmul(qi, r, qirv);
+    qmul(qk, r, qkrv);
+    const double dir = dot3(di, r), dkr = dot3(dk, r), qir = dot3(qirv, r), qkr = dot3(qkrv, r);
+    const double dik = dot3(di, dk), qik = dot3(qirv, qkrv), diqk = dot3(di, qkrv), dkqi = dot3(dk, qirv);
+    const double qiqk = 2.0 * (qi[1] * qk[1] + qi[2] * qk[2] + qi[4] * qk[4]) + qi[0] * qk[0] + qi[3] * qk[3] + qi[5] * qk[5];
+    const double t1 = ci * ck, t2 = ck * dir - ci * dkr + dik;
+    const double t3 = ci * qkr + ck * qir - dir * dkr + 2.0 * (dkqi - diqk + qiqk);
+    const double t4 = dir * qkr - dkr * qir - 4.0 * qik, t5 = qir * qkr;
+    o->e = t1 * R[0] + t2 * R[1] + t3 * R[2] + t4 * R[3] + t5 * R[4];
+    if (!do_g) return;
+    const double de = t1 * R[1] + t2 * R[2] + t3 * R[3] + t4 * R[4] + t5 * R[5];
+    const double g1 = -ck * R[1] + dkr * R[2] - qkr * R[3];
+    const double g2 = ci * R[1] + dir * R[2] + qir * R[3];
+    const double g3 = 2.0 * R[2];
+    const double g4 = 2.0 * (-ck * R[2] + dkr * R[3] - qkr * R[4]);
+    const double g5 = 2.0 * (-ci * R[2] - dir * R[3] - qir * R[4]);
+    const double g6 = 4.0 * R[3];
+    double qkdi[3], qidk[3], qiqkr[3], qkqir[3];
+    qmul(qk, di, qkdi);
+    qmul(qi, dk, qidk);
+    qmul(qi, qkrv, qiqkr);
+    qmul(qk, qirv, qkqir);
+    for (int a = 0; a < 3; a++)
+        o->frc[a] = de * r[a] + g1 * di[a] + g2 * dk[a] + g3 * (qkdi[a] - qidk[a]) + g4 * qirv[a] + g5 * qkrv[a]
+                  + g6 * (qiqkr[a] + qkqir[a]);
+    double dixdk[3], dixr[3], dkxr[3], rxqir[3], rxqkr[3], qkrxqir[3], rxqiqkr[3], rxqkqir[3], rxqkdi[3], rxqidk[3];
+    double dixqkr[3], dkxqir[3];
+    cross3(di, dk, dixdk); cross3(di, r, dixr); cross3(dk, r, dkxr);
+    cross3(r, qirv, rxqir); cross3(r, qkrv, rxqkr); cross3(qkrv, qirv, qkrxqir);
+    cross3(r, qiqkr, rxqiqkr); cross3(r, qkqir, rxqkqir); cross3(r, qkdi, rxqkdi); cross3(r, qidk, rxqidk);
+    cross3(di, qkrv, dixqkr); cross3(dk, qirv, dkxqir);
+    const double Qi[3][3] = {{qi[0], qi[1], qi[2]}, {qi[1], qi[3], qi[4]}, {qi[2], qi[4], qi[5]}};
+    const double Qk[3][3] = {{qk[0], qk[1], qk[2]}, {qk[1], qk[3], qk[4]}, {qk[2], qk[4], qk[5]}};
+    double A[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) A[a][b] = Qi[a][0] * Qk[0][b] + Qi[a][1] * Qk[1][b] + Qi[a][2] * Qk[2][b];
+    const double eA[3] = {A[1][2] - A[2][1], A[2][0] - A[0][2], A[0][1] - A[1][0]};
+    for (int a = 0; a < 3; a++) {
+        const double dq = dixqkr[a] + dkxqir[a] - 2.0 * eA[a];
+        o->trqi[a] = -R[1] * dixdk[a] + g1 * dixr[a] + g3 * (dq + rxqidk[a]) - g4 * rxqir[a] - g6 * (rxqiqkr[a] + qkrxqir[a]);
+        o->trqk[a] = R[1] * dixdk[a] + g2 * dkxr[a] - g3 * (dq + rxqkdi[a]) - g5 * rxqkr[a] - g6 * (rxqkqir[a] - qkrxqir[a]);
+    }
+}
+
+/* ---------------------------------------------------------------------
+ * radial series
+ * ------------------------------------------------------------------- */
+/* undamped rr1..rr11 (with the (2n-1)!! factors) */
+static void series_rr(double r2, double *rr)
+{
+    const double rinv = 1.0 / sqrt(r2), r2i = rinv * rinv;
+    rr[0] = rinv;
+    for (int j = 1; j < 6; j++) rr[j] = (2 * j - 1) * rr[j - 1] * r2i;
+}
+/* Ewald-screened B_0..B_5: B_0 = erfc(a r)/r, B_n = ((2n-1) B_{n-1} + (2a^2)^n e^{-a^2 r^2}/(a sqrt(pi)))/r^2 */
+static void series_ewald(double r2, double a, double *bn)
+{
+    const double r = sqrt(r2);
+    bn[0] = erfc(a * r) / r;
+    const double ex = exp(-a * a * r2);
+    double an = 1.0 / (SQRTPI * a);
+    for (int j = 1; j < 6; j++) {
+        an *= 2.0 * a * a;
+        bn[j] = ((2 * j - 1) * bn[j - 1] + an * ex) / r2;
+    }
+}
+/* Thole damping factors lambda_3,5,7,9 (index 1..4; index 0 unused = 1):
+ * u = r/(pd_i pd_k), D = a u^3, lambda_3 = 1 - e^{-D}, and each higher
+ * order makes d(lambda_n/r^n)/dr = -n lambda_{n+2}/r^{n+1} hold. */
+static void thole_lambda(double r, double pdi, double pdk, double pgamma, double *lam)
+{
+    for (int j = 0; j < 6; j++) lam[j] = 1.0;
+    const double damp = pdi * pdk;
+    if (damp == 0.0) return;
+    const double u = r / damp;
+    const double D = pgamma * u * u * u;
+    if (D >= 50.0) return;
+    const double e = exp(-D);
+    lam[1] = 1.0 - e;
+    lam[2] = 1.0 - (1.0 + D) * e;
+    lam[3] = 1.0 - (1.0 + D + 0.6 * D * D) * e;
+    lam[4] = 1.0 - (1.0 + D + (18.0 / 35.0) * D * D + (9.0 / 35.0) * D * D * D) * e;
+    lam[5] = 1.0; /* unused by dipole interactions */
+}
+
+/* ---------------------------------------------------------------------
+ * system, scaling
+ * ------------------------------------------------------------------- */
+typedef struct {
+    int n;
+    orc_elec_sys *s;
+    const orc_mpole_sys *mp;
+    double *rp;      /* 10n global multipoles */
+    double f, a;     /* electric/dielec, Ewald coefficient (0 = none) */
+    double cut2;
+    int ewald;
+    /* connectivity offsets */
+    int *o12, *o13, *o14, *o15, *g11, *g12, *g13, *g14;
+    /* PME */
+    int K[3], order;
+    double A[3][3];  /* u = A r */
+    double *bsmod[3];
+    double *theta;   /* n * 3 * order * 4 */
+    int *igrid;      /* n * 3 first grid index */
+} ctx_t;
+
+static int *offsets(int n, const int *cnt)
+{
+    int *o = (int *)malloc((n + 1) * sizeof(int));
+    o[0] = 0;
+    for (int i = 0; i < n; i++) o[i + 1] = o[i] + (cnt ? cnt[i] : 0);
+    return o;
+}
+
+/* scale factors of atom i against every k: m (mpole), p, d, u (polpot.h:47-66) */
+static void set_scales(const ctx_t *c, int i, double *ms, double *ps, double *ds, double *us, double v)
+{
+    const orc_mpole_sys *m = c->mp;
+    const orc_elec_sys *s = c->s;
+    if (v == 1.0) {
+        /* reset */
+        for (int q = 0; q < 4; q++) {
+            const int *off = q == 0 ? c->o12 : q == 1 ? c->o13 : q == 2 ? c->o14 : c->o15;
+            const int *lst = q == 0 ? m->i12 : q == 1 ? m->i13 : q == 2 ? m->i14 : m->i15;
+            for (int j = off[i]; j < off[i + 1]; j++) { ms[lst[j]] = 1.0; ps[lst[j]] = 1.0; }
+        }
+        for (int q = 0; q < 4; q++) {
+            const int *off = q == 0 ? c->g11 : q == 1 ? c->g12 : q == 2 ? c->g13 : c->g14;
+            const int *lst = q == 0 ? s->ip11 : q == 1 ? s->ip12 : q == 2 ? s->ip13 : s->ip14;
+            if (!lst) continue;
+            for (int j = off[i]; j < off[i + 1]; j++) { ds[lst[j]] = 1.0; us[lst[j]] = 1.0; }
+        }
+        return;
+    }
+    const double mv[4] = {m->m2scale, m->m3scale, m->m4scale, m->m5scale};
+    const double pv[4] = {s->p2scale, s->p3scale, s->p4scale, s->p5scale};
+    const double piv[4] = {s->p2iscale, s->p3iscale, s->p4iscale, s->p5iscale};
+    const double dv[4] = {s->d1scale, s->d2scale, s->d3scale, s->d4scale};
+    const double uv[4] = {s->u1scale, s->u2scale, s->u3scale, s->u4scale};
+    for (int q = 0; q < 4; q++) {
+        const int *off = q == 0 ? c->o12 : q == 1 ? c->o13 : q == 2 ? c->o14 : c->o15;
+        const int *lst = q == 0 ? m->i12 : q == 1 ? m->i13 : q == 2 ? m->i14 : m->i15;
+        for (int j = off[i]; j < off[i + 1]; j++) {
+            const int k = lst[j];
+            ms[k] = mv[q];
+            ps[k] = pv[q];
+            /* same polarization group: the "intra" value */
+            if (s->ip11)
+                for (int g = c->g11[i]; g < c->g11[i + 1]; g++)
+                    if (s->ip11[g] == k) ps[k] = piv[q];
+        }
+    }
+    for (int q = 0; q < 4; q++) {
+        const int *off = q == 0 ? c->g11 : q == 1 ? c->g12 : q == 2 ? c->g13 : c->g14;
+        const int *lst = q == 0 ? s->ip11 : q == 1 ? s->ip12 : q == 2 ? s->ip13 : s->ip14;
+        if (!lst) continue;
+        for (int j = off[i]; j < off[i + 1]; j++) { ds[lst[j]] = dv[q]; us[lst[j]] = uv[q]; }
+    }
+}
+
+/* the pairs within the cutoff with their minimum-image separation */
+typedef struct {
+    int i, k;
+    double r[3], r2;
+    double m, p, d, u; /* scales */
+} pairrec;
+
+static pairrec *pair_list(ctx_t *c, long *np)
+{
+    const int n = c->n;
+    const orc_mpole_sys *m = c->mp;
+    double *ms = (double *)malloc(n * sizeof(double)), *ps = (double *)malloc(n * sizeof(double));
+    double *ds = (double *)malloc(n * sizeof(double)), *us = (double *)malloc(n * sizeof(double));
+    for (int i = 0; i < n; i++) ms[i] = ps[i] = ds[i] = us[i] = 1.0;
+    long cap = 1024, cnt = 0;
+    pairrec *L = (pairrec *)malloc(cap * sizeof(pairrec));
+    for (int i = 0; i < n - 1; i++) {
+        set_scales(c, i, ms, ps, ds, us, 0.0);
+        for (int k = i + 1; k < n; k++) {
+            double r[3] = {m->x[k] - m->x[i], m->y[k] - m->y[i], m->z[k] - m->z[i]};
+            if (m->box.kind != ORC_BOX_NONE) orc_image(&m->box, &r[0], &r[1], &r[2]);
+            const double r2 = dot3(r, r);
+            if (r2 > c->cut2) continue;
+            if (cnt == cap) { cap *= 2; L = (pairrec *)realloc(L, cap * sizeof(pairrec)); }
+            pairrec *q = &L[cnt++];
+            q->i = i; q->k = k;
+            memcpy(q->r, r, sizeof r);
+            q->r2 = r2;
+            q->m = ms[k]; q->p = ps[k]; q->d = ds[k]; q->u = us[k];
+        }
+        set_scales(c, i, ms, ps, ds, us, 1.0);
+    }
+    free(ms); free(ps); free(ds); free(us);
+    *np = cnt;
+    return L;
+}
+
+/* ---------------------------------------------------------------------
+ * PME: cardinal B-splines, spreading, FFT (direct separable DFT), gather
+ * ------------------------------------------------------------------- */
+static double bspline_m(int n, double x)
+{
+    /* cardinal B-spline M_n(x), support [0, n] */
+    if (x <= 0.0 || x >= n) return 0.0;
+    if (n == 2) return 1.0 - fabs(x - 1.0);
+    return (x * bspline_m(n - 1, x) + (n - x) * bspline_m(n - 1, x - 1.0)) / (n - 1);
+}
+/* d-th derivative of M_n at x: sum_j (-1)^j C(d,j) M_{n-d}(x - j) */
+static double bspline_deriv(int n, int d, double x)
+{
+    static const int C[4][4] = {{1, 0, 0, 0}, {1, 1, 0, 0}, {1, 2, 1, 0}, {1, 3, 3, 1}};
+    double v = 0.0;
+    for (int j = 0; j <= d; j++) v += ((j & 1) ? -1.0 : 1.0) * C[d][j] * bspline_m(n - d, x - j);
+    return v;
+}
+
+static void pme_setup(ctx_t *c)
+{
+    orc_elec_sys *s = c->s;
+    const orc_box *b = &c->mp->box;
+    c->K[0] = s->nfft1; c->K[1] = s->nfft2; c->K[2] = s->nfft3;
+    c->order = s->bsorder;
+    /* lattice vectors (lattice.cpp conventions) and reciprocal rows */
+    const double radian = 57.29577951308232088;
+    double bs = 1, bc = 0, gs = 1, gc = 0, bt = 0, gt = 1;
+    if (b->kind == ORC_BOX_TRI) {
+        const double ac = cos(b->alpha / radian);
+        bs = sin(b->beta / radian); bc = cos(b->beta / radian);
+        gs = sin(b->gamma / radian); gc = cos(b->gamma / radian);
+        bt = (ac - bc * gc) / gs;
+        gt = sqrt(bs * bs - bt * bt);
+    } else if (b->kind == ORC_BOX_MONO) {
+        bs = sin(b->beta / radian); bc = cos(b->beta / radian);
+        gt = bs;
+    }
+    const double av[3] = {b->xbox, 0, 0};
+    const double bv[3] = {b->ybox * gc, b->ybox * gs, 0};
+    const double cv[3] = {b->zbox * bc, b->zbox * bt, b->zbox * gt};
+    double bxc[3], cxa[3], axb[3];
+    cross3(bv, cv, bxc); cross3(cv, av, cxa); cross3(av, bv, axb);
+    const double vol = dot3(av, bxc);
+    for (int j = 0; j < 3; j++) {
+        c->A[0][j] = c->K[0] * bxc[j] / vol;
+        c->A[1][j] = c->K[1] * cxa[j] / vol;
+        c->A[2][j] = c->K[2] * axb[j] / vol;
+    }
+    /* B-spline moduli |sum_k M_n(k+1) e^{2 pi i m k / K}|^2, zeros replaced
+     * by the mean of their neighbours */
+    for (int a = 0; a < 3; a++) {
+        const int K = c->K[a];
+        c->bsmod[a] = (double *)malloc(K * sizeof(double));
+        for (int mm = 0; mm < K; mm++) {
+            double re = 0, im = 0;
+            for (int k = 0; k < c->order - 1; k++) {
+                const double w = bspline_m(c->order, k + 1.0);
+                const double arg = 2.0 * M_PI * mm * k / K;
+                re += w * cos(arg);
+                im += w * sin(arg);
+            }
+            c->bsmod[a][mm] = re * re + im * im;
+        }
+        for (int mm = 0; mm < K; mm++)
+            if (c->bsmod[a][mm] < 1e-7)
+                c->bsmod[a][mm] = 0.5 * (c->bsmod[a][(mm - 1 + K) % K] + c->bsmod[a][(mm + 1) % K]);
+    }
+    /* per-atom spline values and derivatives */
+    const int n = c->n, ord = c->order;
+    c->theta = (double *)malloc((size_t)n * 3 * ord * 4 * sizeof(double));
+    c->igrid = (int *)malloc((size_t)n * 3 * sizeof(int));
+    for (int i = 0; i < n; i++) {
+        const double p[3] = {c->mp->x[i], c->mp->y[i], c->mp->z[i]};
+        for (int a = 0; a < 3; a++) {
+            const double u = dot3(c->A[a], p);
+            const double fl = floor(u);
+            const double w = u - fl;
+            int i0 = (int)fl - ord + 1;
+            i0 %= c->K[a];
+            if (i0 < 0) i0 += c->K[a];
+            c->igrid[3 * i + a] = i0;
+            for (int k = 0; k < ord; k++)
+                for (int d = 0; d < 4; d++)
+                    c->theta[(((size_t)i * 3 + a) * ord + k) * 4 + d] = bspline_deriv(ord, d, w + ord - 1 - k);
+        }
+    }
+}
+
+static void pme_free(ctx_t *c)
+{
+    for (int a = 0; a < 3; a++) free(c->bsmod[a]);
+    free(c->theta);
+    free(c->igrid);
+}
+
+/* derivative operator indices of the 20 potential components, by
+ * (d1,d2,d3) derivative orders along u1,u2,u3 */
+static const int DER[20][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {0, 2, 0}, {0, 0, 2},
+                               {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {3, 0, 0}, {0, 3, 0}, {0, 0, 3}, {2, 1, 0},
+                               {2, 0, 1}, {1, 2, 0}, {1, 0, 2}, {0, 2, 1}, {0, 1, 2}, {1, 1, 1}};
+
+/* spread fractional multipoles fm (10 per atom: c, d1..3, q11,q22,q33,
+ * 2q12,2q13,2q23) onto the grid: rho(g) += sum_j fm_j D_j M(u - g) */
+static void pme_spread(const ctx_t *c, const double *fm, int ncomp, double *grid)
+{
+    const int K0 = c->K[0], K1 = c->K[1], K2 = c->K[2], ord = c->order;
+    memset(grid, 0, (size_t)K0 * K1 * K2 * sizeof(double));
+    for (int i = 0; i < c->n; i++) {
+        const double *f = fm + 10 * i;
+        const double *t0 = c->theta + (((size_t)i * 3 + 0) * ord) * 4;
+        const double *t1 = c->theta + (((size_t)i * 3 + 1) * ord) * 4;
+        const double *t2 = c->theta + (((size_t)i * 3 + 2) * ord) * 4;
+        for (int kz = 0; kz < ord; kz++) {
+            const int gz = (c->igrid[3 * i + 2] + kz) % K2;
+            for (int ky = 0; ky < ord; ky++) {
+                const int gy = (c->igrid[3 * i + 1] + ky) % K1;
+                for (int kx = 0; kx < ord; kx++) {
+                    const int gx = (c->igrid[3 * i] + kx) % K0;
+                    double v = 0.0;
+                    for (int j = 0; j < ncomp; j++)
+                        v += f[j] * t0[kx * 4 + DER[j][0]] * t1[ky * 4 + DER[j][1]] * t2[kz * 4 + DER[j][2]];
+                    grid[((size_t)gz * K1 + gy) * K0 + gx] += v;
+                }
+            }
+        }
+    }
+}
+
+/* fphi[j] = sum_g phi(g) D_j M(u - g), j < ncomp (<= 20) */
+static void pme_gather(const ctx_t *c, const double *grid, int ncomp, double *fphi)
+{
+    const int K0 = c->K[0], K1 = c->K[1], ord = c->order;
+    for (int i = 0; i < c->n; i++) {
+        double *o = fphi + 20 * i;
+        for (int j = 0; j < 20; j++) o[j] = 0.0;
+        const double *t0 = c->theta + (((size_t)i * 3 + 0) * ord) * 4;
+        const double *t1 = c->theta + (((size_t)i * 3 + 1) * ord) * 4;
+        const double *t2 = c->theta + (((size_t)i * 3 + 2) * ord) * 4;
+        for (int kz = 0; kz < ord; kz++) {
+            const int gz = (c->igrid[3 * i + 2] + kz) % c->K[2];
+            for (int ky = 0; ky < ord; ky++) {
+                const int gy = (c->igrid[3 * i + 1] + ky) % K1;
+                for (int kx = 0; kx < ord; kx++) {
+                    const int gx = (c->igrid[3 * i] + kx) % K0;
+                    const double g = grid[((size_t)gz * K1 + gy) * K0 + gx];
+                    for (int j = 0; j < ncomp; j++)
+                        o[j] += g * t0[kx * 4 + DER[j][0]] * t1[ky * 4 + DER[j][1]] * t2[kz * 4 + DER[j][2]];
+                }
+            }
+        }
+    }
+}
+
+/* one separable DFT pass along axis a of a complex grid (sign = +1/-1) */
+static void dft_axis(const ctx_t *c, double *re, double *im, int a, int sign)
+{
+    const int K[3] = {c->K[0], c->K[1], c->K[2]};
+    const int L = K[a];
+    double *cs = (double *)malloc(L * sizeof(double)), *sn = (double *)malloc(L * sizeof(double));
+    for (int j = 0; j < L; j++) { cs[j] = cos(2.0 * M_PI * j / L); sn[j] = sign * sin(2.0 * M_PI * j / L); }
+    double *tr = (double *)malloc(L * sizeof(double)), *ti = (double *)malloc(L * sizeof(double));
+    const size_t stride = a == 0 ? 1 : a == 1 ? (size_t)K[0] : (size_t)K[0] * K[1];
+    const int o1 = a == 0 ? 1 : 0, o2 = a == 2 ? 1 : 2;
+    for (int p = 0; p < K[o1]; p++)
+        for (int q = 0; q < K[o2]; q++) {
+            int idx[3] = {0, 0, 0};
+            idx[o1] = p; idx[o2] = q;
+            const size_t base = ((size_t)idx[2] * K[1] + idx[1]) * K[0] + idx[0];
+            for (int m = 0; m < L; m++) {
+                double sr = 0, si = 0;
+                for (int j = 0; j < L; j++) {
+                    const int w = (int)(((long)m * j) % L);
+                    const double xr = re[base + j * stride], xi = im[base + j * stride];
+                    sr += xr * cs[w] - xi * sn[w];
+                    si += xr * sn[w] + xi * cs[w];
+                }
+                tr[m] = sr; ti[m] = si;
+            }
+            for (int m = 0; m < L; m++) { re[base + m * stride] = tr[m]; im[base + m * stride] = ti[m]; }
+        }
+    free(cs); free(sn); free(tr); free(ti);
+}
+
+/* the reciprocal-space kernel G(m) = exp(-pi^2 h^2/a^2) / (pi V h^2 prod bsmod) */
+static double pme_volume(const ctx_t *c)
+{
+    /* V = K0 K1 K2 / det(A) */
+    const double (*A)[3] = c->A;
+    const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0])
+                     + A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+    return (double)c->K[0] * c->K[1] * c->K[2] / det;
+}
+
+/* phi = IDFT[G DFT[rho]] in place; returns sum_m G |rho^(m)|^2 */
+static double pme_convolve(const ctx_t *c, double *grid)
+{
+    const int K0 = c->K[0], K1 = c->K[1], K2 = c->K[2];
+    const size_t N = (size_t)K0 * K1 * K2;
+    double *im = (double *)calloc(N, sizeof(double));
+    for (int a = 0; a < 3; a++) dft_axis(c, grid, im, a, +1);
+    const double V = pme_volume(c);
+    double e = 0.0;
+    for (int m3 = 0; m3 < K2; m3++)
+        for (int m2 = 0; m2 < K1; m2++)
+            for (int m1 = 0; m1 < K0; m1++) {
+                const size_t id = ((size_t)m3 * K1 + m2) * K0 + m1;
+                if (m1 == 0 && m2 == 0 && m3 == 0) { grid[id] = 0; im[id] = 0; continue; }
+                const int f1 = m1 > K0 / 2 ? m1 - K0 : m1, f2 = m2 > K1 / 2 ? m2 - K1 : m2, f3 = m3 > K2 / 2 ? m3 - K2 : m3;
+                /* h = sum_a m_a (A row a)/K_a */
+                double h[3];
+                for (int j = 0; j < 3; j++)
+                    h[j] = f1 * c->A[0][j] / K0 + f2 * c->A[1][j] / K1 + f3 * c->A[2][j] / K2;
+                const double h2 = dot3(h, h);
+                const double G = exp(-M_PI * M_PI * h2 / (c->a * c->a)) /
+                                 (M_PI * V * h2 * c->bsmod[0][m1] * c->bsmod[1][m2] * c->bsmod[2][m3]);
+                e += G * (grid[id] * grid[id] + im[id] * im[id]);
+                grid[id] *= G;
+                im[id] *= G;
+            }
+    for (int a = 0; a < 3; a++) dft_axis(c, grid, im, a, -1);
+    free(im);
+    return e;
+}
+
+/* Cartesian multipole (10) -> fractional: d^f = A d, q^f = A q A^T, with
+ * off-diagonal q^f doubled */
+static void cart_to_frac(const ctx_t *c, const double *m, double *fm)
+{
+    const double (*A)[3] = c->A;
+    const double Q[3][3] = {{m[4], m[5], m[6]}, {m[5], m[7], m[8]}, {m[6], m[8], m[9]}};
+    fm[0] = m[0];
+    for (int a = 0; a < 3; a++) fm[1 + a] = A[a][0] * m[1] + A[a][1] * m[2] + A[a][2] * m[3];
+    double F[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            double v = 0;
+            for (int p = 0; p < 3; p++)
+                for (int q = 0; q < 3; q++) v += A[a][p] * Q[p][q] * A[b][q];
+            F[a][b] = v;
+        }
+    fm[4] = F[0][0]; fm[5] = F[1][1]; fm[6] = F[2][2];
+    fm[7] = 2 * F[0][1]; fm[8] = 2 * F[0][2]; fm[9] = 2 * F[1][2];
+}
+
+/* fractional derivatives (20) -> Cartesian potential phi, gradient g[3],
+ * Hessian H[3][3], third derivatives T[3][3][3] */
+static void frac_to_cart(const ctx_t *c, const double *fp, double *phi, double *g, double H[3][3], double T[3][3][3])
+{
+    const double (*A)[3] = c->A;
+    double F1[3], F2[3][3], F3[3][3][3];
+    F1[0] = fp[1]; F1[1] = fp[2]; F1[2] = fp[3];
+    F2[0][0] = fp[4]; F2[1][1] = fp[5]; F2[2][2] = fp[6];
+    F2[0][1] = F2[1][0] = fp[7]; F2[0][2] = F2[2][0] = fp[8]; F2[1][2] = F2[2][1] = fp[9];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            for (int d = 0; d < 3; d++) {
+                int cnt[3] = {0, 0, 0};
+                cnt[a]++; cnt[b]++; cnt[d]++;
+                int idx = -1;
+                for (int j = 10; j < 20; j++)
+                    if (DER[j][0] == cnt[0] && DER[j][1] == cnt[1] && DER[j][2] == cnt[2]) idx = j;
+                F3[a][b][d] = fp[idx];
+            }
+    *phi = fp[0];
+    for (int x = 0; x < 3; x++) {
+        g[x] = 0;
+        for (int a = 0; a < 3; a++) g[x] += A[a][x] * F1[a];
+    }
+    if (H)
+        for (int x = 0; x < 3; x++)
+            for (int y = 0; y < 3; y++) {
+                double v = 0;
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) v += A[a][x] * A[b][y] * F2[a][b];
+                H[x][y] = v;
+            }
+    if (T)
+        for (int x = 0; x < 3; x++)
+            for (int y = 0; y < 3; y++)
+                for (int z = 0; z < 3; z++) {
+                    double v = 0;
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++)
+                            for (int d = 0; d < 3; d++) v += A[a][x] * A[b][y] * A[d][z] * F3[a][b][d];
+                    T[x][y][z] = v;
+                }
+}
+
+/* potential derivatives at every site from a set of Cartesian multipoles
+ * (10 per atom); returns the reciprocal energy sum G|rho|^2 / 2 (no f) */
+typedef struct {
+    double phi, g[3], H[3][3], T[3][3][3];
+} sitepot;
+
+static double pme_potential(const ctx_t *c, const double *mpoles, sitepot *out)
+{
+    const int n = c->n;
+    const size_t N = (size_t)c->K[0] * c->K[1] * c->K[2];
+    double *fm = (double *)malloc(10 * (size_t)n * sizeof(double));
+    double *fp = (double *)malloc(20 * (size_t)n * sizeof(double));
+    double *grid = (double *)malloc(N * sizeof(double));
+    for (int i = 0; i < n; i++) cart_to_frac(c, mpoles + 10 * i, fm + 10 * i);
+    pme_spread(c, fm, 10, grid);
+    const double e = 0.5 * pme_convolve(c, grid);
+    pme_gather(c, grid, 20, fp);
+    for (int i = 0; i < n; i++) frac_to_cart(c, fp + 20 * i, &out[i].phi, out[i].g, out[i].H, out[i].T);
+    free(fm); free(fp); free(grid);
+    return e;
+}
+
+/* energy, gradient and torque of multipole m (10) in a potential p:
+ * U = c phi + d.g + q:H */
+static double site_energy(const double *m, const sitepot *p, double *grad, double *trq)
+{
+    const double Q[3][3] = {{m[4], m[5], m[6]}, {m[5], m[7], m[8]}, {m[6], m[8], m[9]}};
+    double U = m[0] * p->phi;
+    for (int a = 0; a < 3; a++) U += m[1 + a] * p->g[a];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) U += Q[a][b] * p->H[a][b];
+    if (grad)
+        for (int x = 0; x < 3; x++) {
+            double v = m[0] * p->g[x];
+            for (int a = 0; a < 3; a++) v += m[1 + a] * p->H[a][x];
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) v += Q[a][b] * p->T[a][b][x];
+            grad[x] = v;
+        }
+    if (trq) {
+        /* tau = d x E - 2 eps:(q H), E = -g */
+        double E[3] = {-p->g[0], -p->g[1], -p->g[2]};
+        cross3(m + 1, E, trq);
+        double QH[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) QH[a][b] = Q[a][0] * p->H[0][b] + Q[a][1] * p->H[1][b] + Q[a][2] * p->H[2][b];
+        trq[0] -= 2.0 * (QH[1][2] - QH[2][1]);
+        trq[1] -= 2.0 * (QH[2][0] - QH[0][2]);
+        trq[2] -= 2.0 * (QH[0][1] - QH[1][0]);
+    }
+    return U;
+}
+
+/* ---------------------------------------------------------------------
+ * fields
+ * ------------------------------------------------------------------- */
+/* field at i from multipole mk (and at k from mi): E_i = -dU/dmu_i */
+static void pair_fields(const double *R, const double *mi, const double *mk, const double *r, double *Ei, double *Ek)
+{
+    double qirv[3], qkrv[3];
+    qmul(mi + 4, r, qirv);
+    qmul(mk + 4, r, qkrv);
+    const double dir = dot3(mi + 1, r), dkr = dot3(mk + 1, r), qir = dot3(qirv, r), qkr = dot3(qkrv, r);
+    const double si = mk[0] * R[1] - dkr * R[2] + qkr * R[3];
+    const double sk = mi[0] * R[1] + dir * R[2] + qir * R[3];
+    for (int a = 0; a < 3; a++) {
+        Ei[a] = -si * r[a] - R[1] * mk[1 + a] + 2.0 * R[2] * qkrv[a];
+        Ek[a] = sk * r[a] - R[1] * mi[1 + a] - 2.0 * R[2] * qirv[a];
+    }
+}
+
+/* series for a polarization interaction: Ewald + scaled, Thole-damped */
+static void pol_series(const ctx_t *c, const pairrec *q, double scale, double *R)
+{
+    const orc_elec_sys *s = c->s;
+    double rr[6], lam[6];
+    series_rr(q->r2, rr);
+    const double pgamma = fmin(s->thole[q->i], s->thole[q->k]) != 0.0 ? fmin(s->thole[q->i], s->thole[q->k])
+                                                                    : fmax(s->thole[q->i], s->thole[q->k]);
+    thole_lambda(sqrt(q->r2), s->pdamp[q->i], s->pdamp[q->k], pgamma, lam);
+    if (c->ewald) {
+        double bn[6];
+        series_ewald(q->r2, c->a, bn);
+        for (int j = 0; j < 6; j++) R[j] = bn[j] - (1.0 - scale * lam[j]) * rr[j];
+    } else {
+        for (int j = 0; j < 6; j++) R[j] = scale * lam[j] * rr[j];
+    }
+}
+
+/* permanent fields E^d and E^p (3n each), field units (no f) */
+static void direct_field(ctx_t *c, const pairrec *L, long np, const sitepot *prec, double *Ed, double *Ep)
+{
+    const int n = c->n;
+    memset(Ed, 0, 3 * (size_t)n * sizeof(double));
+    memset(Ep, 0, 3 * (size_t)n * sizeof(double));
+    for (long t = 0; t < np; t++) {
+        const pairrec *q = &L[t];
+        double R[6], Ei[3], Ek[3];
+        pol_series(c, q, q->d, R);
+        pair_fields(R, c->rp + 10 * q->i, c->rp + 10 * q->k, q->r, Ei, Ek);
+        for (int a = 0; a < 3; a++) { Ed[3 * q->i + a] += Ei[a]; Ed[3 * q->k + a] += Ek[a]; }
+        pol_series(c, q, q->p, R);
+        pair_fields(R, c->rp + 10 * q->i, c->rp + 10 * q->k, q->r, Ei, Ek);
+        for (int a = 0; a < 3; a++) { Ep[3 * q->i + a] += Ei[a]; Ep[3 * q->k + a] += Ek[a]; }
+    }
+    if (c->ewald) {
+        const double self = 4.0 / 3.0 * c->a * c->a * c->a / SQRTPI;
+        for (int i = 0; i < n; i++)
+            for (int a = 0; a < 3; a++) {
+                const double v = -prec[i].g[a] + self * c->rp[10 * i + 1 + a];
+                Ed[3 * i + a] += v;
+                Ep[3 * i + a] += v;
+            }
+    }
+}
+
+/* mutual field T mu for two dipole sets (3n each) */
+static void mutual_field(ctx_t *c, const pairrec *L, long np, const double *ud, const double *up, double *Fd, double *Fp)
+{
+    const int n = c->n;
+    memset(Fd, 0, 3 * (size_t)n * sizeof(double));
+    memset(Fp, 0, 3 * (size_t)n * sizeof(double));
+    for (long t = 0; t < np; t++) {
+        const pairrec *q = &L[t];
+        double R[6];
+        pol_series(c, q, q->u, R);
+        for (int set = 0; set < 2; set++) {
+            const double *u = set ? up : ud;
+            double *F = set ? Fp : Fd;
+            const double *ui = u + 3 * q->i, *uk = u + 3 * q->k;
+            const double uir = dot3(ui, q->r), ukr = dot3(uk, q->r);
+            for (int a = 0; a < 3; a++) {
+                F[3 * q->i + a] += ukr * R[2] * q->r[a] - R[1] * uk[a];
+                F[3 * q->k + a] += uir * R[2] * q->r[a] - R[1] * ui[a];
+            }
+        }
+    }
+    if (c->ewald) {
+        double *m = (double *)calloc(10 * (size_t)n, sizeof(double));
+        sitepot *p = (sitepot *)malloc(n * sizeof(sitepot));
+        const double self = 4.0 / 3.0 * c->a * c->a * c->a / SQRTPI;
+        for (int set = 0; set < 2; set++) {
+            const double *u = set ? up : ud;
+            double *F = set ? Fp : Fd;
+            for (int i = 0; i < n; i++)
+                for (int a = 0; a < 3; a++) m[10 * i + 1 + a] = u[3 * i + a];
+            pme_potential(c, m, p);
+            for (int i = 0; i < n; i++)
+                for (int a = 0; a < 3; a++) F[3 * i + a] += -p[i].g[a] + self * u[3 * i + a];
+        }
+        free(m);
+        free(p);
+    }
+}
+
+/* ---------------------------------------------------------------------
+ * induced dipoles
+ * ------------------------------------------------------------------- */
+#define DEBYE 4.80321
+
+/* preconditioned conjugate gradient on (1/alpha - T) mu = E for both sets,
+ * Jacobi preconditioner, stop when the rms residual (Debye) < poleps */
+static int solve_pcg(ctx_t *c, const pairrec *L, long np, const double *Ed, const double *Ep, double *ud, double *up)
+{
+    const int n = c->n, n3 = 3 * n;
+    const orc_elec_sys *s = c->s;
+    double *poli = (double *)malloc(n * sizeof(double));
+    int npolar = 0;
+    for (int i = 0; i < n; i++) {
+        poli[i] = fmax(1e-8, s->polarity[i]);
+        if (s->polarity[i] != 0.0) npolar++;
+    }
+    double *rd = malloc(n3 * sizeof(double)), *rp_ = malloc(n3 * sizeof(double));
+    double *zd = malloc(n3 * sizeof(double)), *zp = malloc(n3 * sizeof(double));
+    double *pd = malloc(n3 * sizeof(double)), *pp = malloc(n3 * sizeof(double));
+    double *Fd = malloc(n3 * sizeof(double)), *Fp = malloc(n3 * sizeof(double));
+    /* guess: direct induced dipoles */
+    for (int i = 0; i < n; i++)
+        for (int a = 0; a < 3; a++) {
+            ud[3 * i + a] = s->polarity[i] * Ed[3 * i + a];
+            up[3 * i + a] = s->polarity[i] * Ep[3 * i + a];
+        }
+    mutual_field(c, L, np, ud, up, Fd, Fp);
+    for (int j = 0; j < n3; j++) {
+        const int i = j / 3;
+        const double pol = s->polarity[i];
+        rd[j] = pol != 0.0 ? Ed[j] + Fd[j] - ud[j] / poli[i] : 0.0;
+        rp_[j] = pol != 0.0 ? Ep[j] + Fp[j] - up[j] / poli[i] : 0.0;
+        zd[j] = poli[i] * rd[j];
+        zp[j] = poli[i] * rp_[j];
+        pd[j] = zd[j];
+        pp[j] = zp[j];
+    }
+    double sd = 0, sp = 0;
+    for (int j = 0; j < n3; j++) { sd += rd[j] * zd[j]; sp += rp_[j] * zp[j]; }
+    int iter = 0, done = 0;
+    while (!done && iter < s->politer) {
+        iter++;
+        mutual_field(c, L, np, pd, pp, Fd, Fp);
+        double ad = 0, ap = 0;
+        for (int j = 0; j < n3; j++) {
+            const int i = j / 3;
+            const double vd = s->polarity[i] != 0.0 ? pd[j] / poli[i] - Fd[j] : 0.0;
+            const double vp = s->polarity[i] != 0.0 ? pp[j] / poli[i] - Fp[j] : 0.0;
+            Fd[j] = vd; Fp[j] = vp;
+            ad += pd[j] * vd;
+            ap += pp[j] * vp;
+        }
+        ad = ad != 0.0 ? sd / ad : 0.0;
+        ap = ap != 0.0 ? sp / ap : 0.0;
+        double sd2 = 0, sp2 = 0, epsd = 0, epsp = 0;
+        for (int j = 0; j < n3; j++) {
+            const int i = j / 3;
+            ud[j] += ad * pd[j];
+            up[j] += ap * pp[j];
+            rd[j] -= ad * Fd[j];
+            rp_[j] -= ap * Fp[j];
+            zd[j] = poli[i] * rd[j];
+            zp[j] = poli[i] * rp_[j];
+            sd2 += rd[j] * zd[j];
+            sp2 += rp_[j] * zp[j];
+            epsd += rd[j] * rd[j];
+            epsp += rp_[j] * rp_[j];
+        }
+        const double bd = sd != 0.0 ? sd2 / sd : 0.0, bp = sp != 0.0 ? sp2 / sp : 0.0;
+        for (int j = 0; j < n3; j++) { pd[j] = zd[j] + bd * pd[j]; pp[j] = zp[j] + bp * pp[j]; }
+        sd = sd2;
+        sp = sp2;
+        const double eps = DEBYE * sqrt(fmax(epsd, epsp) / (npolar > 0 ? npolar : 1));
+        if (eps < s->poleps) done = 1;
+    }
+    free(poli); free(rd); free(rp_); free(zd); free(zp); free(pd); free(pp); free(Fd); free(Fp);
+    return done ? iter : -iter;
+}
+
+/* dense solve of (1/alpha - T) mu = E (small systems; tests) */
+static int solve_dense(ctx_t *c, const pairrec *L, long np, const double *Ed, const double *Ep, double *ud, double *up)
+{
+    const int n = c->n, n3 = 3 * n;
+    const orc_elec_sys *s = c->s;
+    double *M = (double *)malloc((size_t)n3 * n3 * sizeof(double));
+    double *e = calloc(n3, sizeof(double)), *Fd = malloc(n3 * sizeof(double)), *Fp = malloc(n3 * sizeof(double));
+    double *z = calloc(n3, sizeof(double));
+    /* columns of T by applying the field operator to unit vectors */
+    for (int col = 0; col < n3; col++) {
+        e[col] = 1.0;
+        mutual_field(c, L, np, e, z, Fd, Fp);
+        e[col] = 0.0;
+        for (int row = 0; row < n3; row++) M[(size_t)row * n3 + col] = -Fd[row];
+    }
+    for (int j = 0; j < n3; j++) {
+        const double pol = s->polarity[j / 3];
+        M[(size_t)j * n3 + j] += pol != 0.0 ? 1.0 / pol : 1.0;
+    }
+    /* Gaussian elimination with partial pivoting, two right-hand sides */
+    double *B = (double *)malloc((size_t)n3 * 2 * sizeof(double));
+    for (int j = 0; j < n3; j++) {
+        const double pol = s->polarity[j / 3];
+        B[2 * j] = pol != 0.0 ? Ed[j] : 0.0;
+        B[2 * j + 1] = pol != 0.0 ? Ep[j] : 0.0;
+    }
+    for (int k = 0; k < n3; k++) {
+        int piv = k;
+        for (int r = k + 1; r < n3; r++)
+            if (fabs(M[(size_t)r * n3 + k]) > fabs(M[(size_t)piv * n3 + k])) piv = r;
+        if (piv != k) {
+            for (int q = 0; q < n3; q++) { double t = M[(size_t)k * n3 + q]; M[(size_t)k * n3 + q] = M[(size_t)piv * n3 + q]; M[(size_t)piv * n3 + q] = t; }
+            for (int q = 0; q < 2; q++) { double t = B[2 * k + q]; B[2 * k + q] = B[2 * piv + q]; B[2 * piv + q] = t; }
+        }
+        for (int r = k + 1; r < n3; r++) {
+            const double fct = M[(size_t)r * n3 + k] / M[(size_t)k * n3 + k];
+            if (fct == 0.0) continue;
+            for (int q = k; q < n3; q++) M[(size_t)r * n3 + q] -= fct * M[(size_t)k * n3 + q];
+            B[2 * r] -= fct * B[2 * k];
+            B[2 * r + 1] -= fct * B[2 * k + 1];
+        }
+    }
+    for (int k = n3 - 1; k >= 0; k--) {
+        for (int q = 0; q < 2; q++) {
+            double v = B[2 * k + q];
+            for (int col = k + 1; col < n3; col++) v -= M[(size_t)k * n3 + col] * B[2 * col + q];
+            B[2 * k + q] = v / M[(size_t)k * n3 + k];
+        }
+    }
+    for (int j = 0; j < n3; j++) {
+        const double pol = s->polarity[j / 3];
+        ud[j] = pol != 0.0 ? B[2 * j] : 0.0;
+        up[j] = pol != 0.0 ? B[2 * j + 1] : 0.0;
+    }
+    free(M); free(e); free(Fd); free(Fp); free(z); free(B);
+    return 1;
+}
+
+/* ---------------------------------------------------------------------
+ * driver
+ * ------------------------------------------------------------------- */
+static void torque_to_grad(const orc_mpole_sys *mp, const double *te, double *de);
+
+int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
+{
+    orc_mpole_sys *mp = s->mp;
+    const int n = mp->n;
+    ctx_t c;
+    memset(&c, 0, sizeof c);
+    c.n = n;
+    c.s = s;
+    c.mp = mp;
+    c.f = mp->f;
+    c.ewald = s->use_ewald;
+    c.a = s->use_ewald ? s->aewald : 0.0;
+    c.cut2 = s->cut * s->cut;
+    if (c.ewald && !(mp->box.kind == ORC_BOX_ORTHO || mp->box.kind == ORC_BOX_MONO || mp->box.kind == ORC_BOX_TRI))
+        return -2;
+    c.o12 = offsets(n, mp->n12); c.o13 = offsets(n, mp->n13);
+    c.o14 = offsets(n, mp->n14); c.o15 = offsets(n, mp->n15);
+    c.g11 = offsets(n, s->np11); c.g12 = offsets(n, s->np12);
+    c.g13 = offsets(n, s->np13); c.g14 = offsets(n, s->np14);
+
+    /* global multipoles */
+    double *pole = (double *)malloc(13 * (size_t)n * sizeof(double));
+    double *rp13 = (double *)malloc(13 * (size_t)n * sizeof(double));
+    memcpy(pole, mp->pole, 13 * (size_t)n * sizeof(double));
+    orc_chkpole(mp, pole);
+    orc_rotpole(mp, pole, rp13);
+    c.rp = (double *)malloc(10 * (size_t)n * sizeof(double));
+    static const int idx[10] = {0, 1, 2, 3, 4, 5, 6, 8, 9, 12};
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < 10; j++) c.rp[10 * i + j] = rp13[13 * i + idx[j]];
+    if (c.ewald) pme_setup(&c);
+
+    long np = 0;
+    pairrec *L = pair_list(&c, &np);
+    double *te = (double *)calloc(3 * (size_t)n, sizeof(double));
+    double *dem = (double *)calloc(3 * (size_t)n, sizeof(double));
+    double *dep = (double *)calloc(3 * (size_t)n, sizeof(double));
+    const double f = c.f;
+
+    /* ---- permanent multipoles: real space */
+    double em_real = 0.0;
+    for (long t = 0; t < np; t++) {
+        const pairrec *q = &L[t];
+        double R[6], rr[6];
+        series_rr(q->r2, rr);
+        if (c.ewald) {
+            double bn[6];
+            series_ewald(q->r2, c.a, bn);
+            for (int j = 0; j < 6; j++) R[j] = f * (bn[j] - (1.0 - q->m) * rr[j]);
+        } else {
+            for (int j = 0; j < 6; j++) R[j] = f * q->m * rr[j];
+        }
+        gpair g;
+        gpair_eval(R, c.rp + 10 * q->i, c.rp + 10 * q->k, q->r, want_grad, &g);
+        em_real += g.e;
+        if (want_grad)
+            for (int a = 0; a < 3; a++) {
+                dem[3 * q->i + a] += g.frc[a];
+                dem[3 * q->k + a] -= g.frc[a];
+                te[3 * q->i + a] += g.trqi[a];
+                te[3 * q->k + a] += g.trqk[a];
+            }
+    }
+    /* ---- reciprocal space and self energy */
+    double em_recip = 0.0, em_self = 0.0;
+    sitepot *pm = NULL;
+    if (c.ewald) {
+        pm = (sitepot *)malloc(n * sizeof(sitepot));
+        em_recip = f * pme_potential(&c, c.rp, pm);
+        for (int i = 0; i < n; i++) {
+            const double *m = c.rp + 10 * i;
+            const double qq = m[4] * m[4] + m[7] * m[7] + m[9] * m[9] + 2.0 * (m[5] * m[5] + m[6] * m[6] + m[8] * m[8]);
+            const double dd = m[1] * m[1] + m[2] * m[2] + m[3] * m[3];
+            const double a2 = c.a * c.a;
+            em_self += -f * c.a / SQRTPI * (m[0] * m[0] + 2.0 * a2 / 3.0 * dd + 8.0 * a2 * a2 / 5.0 * qq);
+            if (want_grad) {
+                double g[3], tq[3];
+                site_energy(m, &pm[i], g, tq);
+                for (int a = 0; a < 3; a++) { dem[3 * i + a] += f * g[a]; te[3 * i + a] += f * tq[a]; }
+            }
+        }
+    }
+    o->em = em_real + em_recip + em_self;
+    o->em_real = em_real;
+    o->em_recip = em_recip;
+    o->em_self = em_self;
+    if (want_grad && o->dem) {
+        torque_to_grad(mp, te, dem);
+        memcpy(o->dem, dem, 3 * (size_t)n * sizeof(double));
+    }
+
+    /* ---- polarization */
+    o->ep = 0.0;
+    o->iters = 0;
+    if (s->use_polar) {
+        const int n3 = 3 * n;
+        double *Ed = malloc(n3 * sizeof(double)), *Ep = malloc(n3 * sizeof(double));
+        double *ud = o->uind ? o->uind : malloc(n3 * sizeof(double));
+        double *up = o->uinp ? o->uinp : malloc(n3 * sizeof(double));
+        direct_field(&c, L, np, pm, Ed, Ep);
+        o->iters = s->solver == 1 ? solve_dense(&c, L, np, Ed, Ep, ud, up) : solve_pcg(&c, L, np, Ed, Ep, ud, up);
+        double ep = 0.0;
+        for (int j = 0; j < n3; j++) ep += ud[j] * Ep[j];
+        o->ep = -0.5 * f * ep;
+        if (want_grad && o->dep) {
+            memset(te, 0, n3 * sizeof(double));
+            /* real space: (1/2)[U_p(mu_d, M) + U_d(mu_p, M) + U_u(mu_d, mu_p) + U_u(mu_p, mu_d)] */
+            for (long t = 0; t < np; t++) {
+                const pairrec *q = &L[t];
+                const int i = q->i, k = q->k;
+                const double *Mi = c.rp + 10 * i, *Mk = c.rp + 10 * k;
+                double mdi[10] = {0}, mdk[10] = {0}, mpi[10] = {0}, mpk[10] = {0};
+                for (int a = 0; a < 3; a++) {
+                    mdi[1 + a] = ud[3 * i + a]; mdk[1 + a] = ud[3 * k + a];
+                    mpi[1 + a] = up[3 * i + a]; mpk[1 + a] = up[3 * k + a];
+                }
+                double R[6];
+                gpair g1, g2;
+                double frc[3] = {0, 0, 0}, ti[3] = {0, 0, 0}, tk[3] = {0, 0, 0};
+                /* mu_d with p-scaled permanent field */
+                pol_series(&c, q, q->p, R);
+                gpair_eval(R, mdi, Mk, q->r, 1, &g1);
+                gpair_eval(R, Mi, mdk, q->r, 1, &g2);
+                for (int a = 0; a < 3; a++) { frc[a] += g1.frc[a] + g2.frc[a]; tk[a] += g1.trqk[a]; ti[a] += g2.trqi[a]; }
+                /* mu_p with d-scaled permanent field */
+                pol_series(&c, q, q->d, R);
+                gpair_eval(R, mpi, Mk, q->r, 1, &g1);
+                gpair_eval(R, Mi, mpk, q->r, 1, &g2);
+                for (int a = 0; a < 3; a++) { frc[a] += g1.frc[a] + g2.frc[a]; tk[a] += g1.trqk[a]; ti[a] += g2.trqi[a]; }
+                /* dipole-dipole, u-scaled */
+                pol_series(&c, q, q->u, R);
+                gpair_eval(R, mdi, mpk, q->r, 1, &g1);
+                gpair_eval(R, mpi, mdk, q->r, 1, &g2);
+                for (int a = 0; a < 3; a++) frc[a] += g1.frc[a] + g2.frc[a];
+                for (int a = 0; a < 3; a++) {
+                    dep[3 * i + a] += 0.5 * f * frc[a];
+                    dep[3 * k + a] -= 0.5 * f * frc[a];
+                    te[3 * i + a] += 0.5 * f * ti[a];
+                    te[3 * k + a] += 0.5 * f * tk[a];
+                }
+            }
+            if (c.ewald) {
+                /* reciprocal space: potentials of mu_d and mu_p */
+                double *md = calloc(10 * (size_t)n, sizeof(double)), *mq = calloc(10 * (size_t)n, sizeof(double));
+                for (int i = 0; i < n; i++)
+                    for (int a = 0; a < 3; a++) { md[10 * i + 1 + a] = ud[3 * i + a]; mq[10 * i + 1 + a] = up[3 * i + a]; }
+                sitepot *pd = (sitepot *)malloc(n * sizeof(sitepot)), *pq = (sitepot *)malloc(n * sizeof(sitepot));
+                pme_potential(&c, md, pd);
+                pme_potential(&c, mq, pq);
+                const double self = 4.0 / 3.0 * c.a * c.a * c.a / SQRTPI;
+                for (int i = 0; i < n; i++) {
+                    double g[3], tq[3];
+                    /* induced dipoles in the permanent potential and in each other's */
+                    for (int x = 0; x < 3; x++) {
+                        double v = 0.0;
+                        for (int a = 0; a < 3; a++)
+                            v += (ud[3 * i + a] + up[3 * i + a]) * pm[i].H[a][x] + ud[3 * i + a] * pq[i].H[a][x]
+                               + up[3 * i + a] * pd[i].H[a][x];
+                        dep[3 * i + x] += 0.5 * f * v;
+                    }
+                    /* permanent multipoles in the induced dipoles' potential */
+                    sitepot ps;
+                    ps.phi = pd[i].phi + pq[i].phi;
+                    for (int a = 0; a < 3; a++) {
+                        ps.g[a] = pd[i].g[a] + pq[i].g[a];
+                        for (int b = 0; b < 3; b++) {
+                            ps.H[a][b] = pd[i].H[a][b] + pq[i].H[a][b];
+                            for (int d = 0; d < 3; d++) ps.T[a][b][d] = pd[i].T[a][b][d] + pq[i].T[a][b][d];
+                        }
+                    }
+                    site_energy(c.rp + 10 * i, &ps, g, tq);
+                    for (int a = 0; a < 3; a++) { dep[3 * i + a] += 0.5 * f * g[a]; te[3 * i + a] += 0.5 * f * tq[a]; }
+                    /* self term: -(f/2) self (mu_d + mu_p) . d_i gives a torque on d_i */
+                    double Eeff[3], tt[3];
+                    for (int a = 0; a < 3; a++) Eeff[a] = 0.5 * f * self * (ud[3 * i + a] + up[3 * i + a]);
+                    cross3(c.rp + 10 * i + 1, Eeff, tt);
+                    for (int a = 0; a < 3; a++) te[3 * i + a] += tt[a];
+                }
+                free(md); free(mq); free(pd); free(pq);
+            }
+            torque_to_grad(mp, te, dep);
+            memcpy(o->dep, dep, n3 * sizeof(double));
+        }
+        free(Ed); free(Ep);
+        if (!o->uind) free(ud);
+        if (!o->uinp) free(up);
+    }
+
+    free(L); free(te); free(dem); free(dep); free(pole); free(rp13); free(c.rp); free(pm);
+    free(c.o12); free(c.o13); free(c.o14); free(c.o15); free(c.g11); free(c.g12); free(c.g13); free(c.g14);
+    if (c.ewald) pme_free(&c);
+    return 0;
+}
+
+static void torque_to_grad(const orc_mpole_sys *mp, const double *te, double *de) { orc_torque(mp, te, de, NULL); }

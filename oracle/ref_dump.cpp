@@ -206,12 +206,30 @@ int main(int argc, char** argv)
     num("poleps", poleps);
     str("poltyp", poltyp);
     iarr("np11", n, [](int i) { return np11[i]; });
+    iarr("np12", n, [](int i) { return np12[i]; });
+    iarr("np13", n, [](int i) { return np13[i]; });
+    iarr("np14", n, [](int i) { return np14[i]; });
     {
-        std::vector<int> v11;
-        for (int i = 0; i < n; i++)
+        std::vector<int> v11, v12, v13, v14;
+        for (int i = 0; i < n; i++) {
             for (int j = 0; j < np11[i]; j++) v11.push_back(ip11[i][j]);
+            for (int j = 0; j < np12[i]; j++) v12.push_back(ip12[i][j]);
+            for (int j = 0; j < np13[i]; j++) v13.push_back(ip13[i][j]);
+            for (int j = 0; j < np14[i]; j++) v14.push_back(ip14[i][j]);
+        }
         iarr("ip11", (int)v11.size(), [&](int i) { return v11[i]; });
+        iarr("ip12", (int)v12.size(), [&](int i) { return v12[i]; });
+        iarr("ip13", (int)v13.size(), [&](int i) { return v13[i]; });
+        iarr("ip14", (int)v14.size(), [&](int i) { return v14[i]; });
     }
+    num("w2scale", w2scale); num("w3scale", w3scale);
+    num("w4scale", w4scale); num("w5scale", w5scale);
+    // repulsion (krepel.cpp) and dispersion (kdsp.h) setup
+    arr("sizpr", n, [](int i) { return sizpr[i]; });
+    arr("dmppr", n, [](int i) { return dmppr[i]; });
+    arr("elepr", n, [](int i) { return elepr[i]; });
+    num("r2scale", r2scale); num("r3scale", r3scale);
+    num("r4scale", r4scale); num("r5scale", r5scale);
 
     // --- reference outputs ----------------------------------------------
     energy<CalcMode::Analysis>();

@@ -104,3 +104,123 @@ def write_tinker_xyz(path: str, xyz: np.ndarray, ff: str = "water03", title: str
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
+
+
+# ---------------------------------------------------------------------------
+# parameter assignment for water systems (what mechanic() leaves in the
+# reference's modules for these molecules)
+
+BOHR = 0.529177210903          # units.h:75
+COULOMB = 332.0637133          # units.h:79 (electric)
+FRAME_ZTHENX, FRAME_BISECTOR = 1, 3
+DATA = __import__("os").path.join(__import__("os").path.dirname(__import__("os").path.abspath(__file__)), "data")
+
+
+def _params(ff: str) -> dict:
+    from . import prm
+
+    return prm.load(__import__("os").path.join(DATA, f"{ff}.json"))
+
+
+def water_system(xyz: np.ndarray, ff: str = "water03", box: float | None = None, *, ewald: dict | None = None,
+                 polar: bool = True, cutoff: float | None = None) -> dict:
+    """System dict (the reference's global arrays) for rigid waters O,H,H.
+
+    Frames as kmpole assigns them for water (kmpole.cpp:290-441): O is a
+    bisector site with z = first H and x = second H, each H is Z-then-X with
+    z = O and x = the other H.  Multipoles in kmpole's units (dipole x bohr,
+    quadrupole x bohr^2/3, kmpole.cpp:543-552); one polarization group per
+    molecule (kpolar.cpp:592-853); Thole a and pdamp = alpha^(1/6)
+    (kpolar.cpp:461-476).  `ewald` = {aewald, nfft, order, cutoff} switches on
+    Ewald summation for a cubic box of side `box`.
+    """
+    P = _params(ff)
+    xyz = np.asarray(xyz, dtype=np.float64)
+    n = len(xyz)
+    nmol = n // 3
+    to, th = TYPES[ff]
+    mp = {m["ids"][0]: m for m in P["multipole"]}
+    pole = np.zeros((n, 13))
+    for t, rows in ((to, slice(0, None, 3)), (th, None)):
+        m = mp[t]
+        v = np.concatenate([[m["c"]], np.array(m["d"]) * BOHR, np.array(m["q"]) * BOHR * BOHR / 3.0])
+        if rows is not None:
+            pole[rows] = v
+        else:
+            pole[1::3] = v
+            pole[2::3] = v
+    idx = np.arange(n)
+    o = 3 * np.arange(nmol)
+    polaxe = np.full(n, FRAME_ZTHENX, dtype=np.int32)
+    polaxe[o] = FRAME_BISECTOR
+    zaxis = np.empty(n, dtype=np.int32)
+    xaxis = np.empty(n, dtype=np.int32)
+    zaxis[o], xaxis[o] = o + 2, o + 3            # 1-based
+    zaxis[o + 1], xaxis[o + 1] = o + 1, o + 3
+    zaxis[o + 2], xaxis[o + 2] = o + 1, o + 2
+    yaxis = np.zeros(n, dtype=np.int32)
+    n12 = np.where(idx % 3 == 0, 2, 1).astype(np.int32)
+    i12 = np.empty(2 * n - 2 * nmol + 2 * nmol, dtype=np.int32)
+    i12 = np.concatenate([np.array([[a + 1, a + 2], [a, -1], [a, -1]]) for a in o]).ravel()
+    i12 = i12[i12 >= 0].astype(np.int32)
+    n13 = np.where(idx % 3 == 0, 0, 1).astype(np.int32)
+    i13 = np.stack([o + 2, o + 1], 1).ravel().astype(np.int32)
+    zero = np.zeros(n, dtype=np.int32)
+    grp = np.repeat(o, 3)
+    ip11 = (grp[:, None] + np.arange(3)[None, :]).ravel().astype(np.int32)
+    types = np.where(idx % 3 == 0, to, th).astype(np.int32)
+    cls = np.array([P["atom"][int(t)]["class"] for t in types])
+    s = {
+        "n": n, "x": xyz[:, 0].copy(), "y": xyz[:, 1].copy(), "z": xyz[:, 2].copy(),
+        "type": types, "pole": pole.ravel(), "polaxe": polaxe, "zaxis": zaxis, "xaxis": xaxis, "yaxis": yaxis,
+        "use": np.concatenate([[0], np.ones(n, dtype=np.int32)]).astype(np.int32),
+        "molcule": (idx // 3).astype(np.int32),
+        "n12": n12, "i12": i12, "n13": n13, "i13": i13, "n14": zero, "i14": np.zeros(0, np.int32),
+        "n15": zero, "i15": np.zeros(0, np.int32),
+        "np11": np.full(n, 3, dtype=np.int32), "ip11": ip11,
+        "np12": zero, "ip12": np.zeros(0, np.int32), "np13": zero, "ip13": np.zeros(0, np.int32),
+        "np14": zero, "ip14": np.zeros(0, np.int32),
+        "electric": COULOMB, "dielec": float(P["keywords"].get("dielectric", 1.0)),
+        "pentype": 0, "use_chgflx": 0, "nbond": 0, "nangle": 0,
+        "ibnd": np.zeros(0, np.int32), "iang": np.zeros(0, np.int32), "balist": np.zeros(0, np.int32),
+        "bl": np.zeros(0), "bflx": np.zeros(0), "anat": np.zeros(0), "aflx": np.zeros(0), "abflx": np.zeros(0),
+        "mono0": pole[:, 0].copy(), "pcore": np.zeros(n), "pval0": pole[:, 0].copy(), "palpha": np.zeros(n),
+    }
+    sc = {"m2scale": 0.0, "m3scale": 0.0, "m4scale": 1.0, "m5scale": 1.0, "p2scale": 0.0, "p3scale": 0.0,
+          "p4scale": 1.0, "p5scale": 1.0, "p2iscale": 0.0, "p3iscale": 0.0, "p4iscale": 0.5, "p5iscale": 1.0,
+          "d1scale": 0.0, "d2scale": 1.0, "d3scale": 1.0, "d4scale": 1.0,
+          "u1scale": 1.0, "u2scale": 1.0, "u3scale": 1.0, "u4scale": 1.0}
+    sc.update({k: v for k, v in P["scales"].items() if k in sc})
+    s.update(sc)
+    # polarizability (by type, or by class for HIPPO files)
+    # POLARIZE: polarizability, then Thole a when the next field is not an
+    # integer group type (readprm.cpp:1250-1275)
+    def thole_of(v):
+        return v[1] if len(v) > 1 and float(v[1]) != int(v[1]) else 0.0
+    pol = np.array([P["polarize"][int(t)][0] for t in types])
+    thl = np.array([thole_of(P["polarize"][int(t)]) for t in types])
+    s["polarity"], s["thole"], s["pdamp"] = pol, thl, pol ** (1.0 / 6.0)
+    s["poleps"], s["politer"] = 1e-6, 100
+    s["use_polar"] = 1 if polar else 0
+    if P["chgpen"]:
+        cp = np.array([P["chgpen"][int(c)] for c in cls])
+        s["pentype"] = 1
+        s["pcore"], s["palpha"] = cp[:, 0].copy(), cp[:, 1].copy()
+        s["pval0"] = pole[:, 0] - cp[:, 0]
+    # box and method
+    if box is None:
+        s.update({"use_bounds": 0, "xbox": 0.0, "ybox": 0.0, "zbox": 0.0, "alpha": 90.0, "beta": 90.0,
+                  "gamma": 90.0, "orthogonal": 0, "monoclinic": 0, "triclinic": 0, "octahedron": 0,
+                  "dodecadron": 0})
+    else:
+        s.update({"use_bounds": 1, "xbox": float(box), "ybox": float(box), "zbox": float(box), "alpha": 90.0,
+                  "beta": 90.0, "gamma": 90.0, "orthogonal": 1, "monoclinic": 0, "triclinic": 0,
+                  "octahedron": 0, "dodecadron": 0})
+    s["mpolecut"] = float(cutoff) if cutoff is not None else (9.0 if box is not None else 1e12)
+    if ewald:
+        g = ewald["nfft"]
+        s.update({"use_ewald": 1, "aewald": float(ewald["aewald"]), "ewaldcut": float(ewald.get("cutoff", 7.0)),
+                  "nfft1": int(g), "nfft2": int(g), "nfft3": int(g), "bsorder": int(ewald.get("order", 5))})
+    else:
+        s.update({"use_ewald": 0, "aewald": 0.0, "ewaldcut": 7.0, "nfft1": 0, "nfft2": 0, "nfft3": 0, "bsorder": 5})
+    return s

@@ -60,10 +60,12 @@ SYSTEMS = {
     "bounds_octahedron": (T + "bounds/waterbox30.xyz", T + "bounds/octahedron.key", True),
     "bounds_dodecahedron": (T + "bounds/waterbox30.xyz", T + "bounds/dodecahedron.key", False),
 }
-# synthetic water boxes (non-Ewald multipoles, 7 A cutoff): name -> (k, L, seed, full gradient)
+# synthetic water boxes (non-Ewald multipoles, 7 A cutoff):
+# name -> (k, L, seed, full gradient, parameter file)
 SYNTH = {
-    "synth_water216": (6, 18.643, 216, True),
-    "synth_water4096": (16, 49.68, 4096, False),
+    "synth_water216": (6, 18.643, 216, True, "water03"),
+    "synth_water4096": (16, 49.68, 4096, False, "water03"),
+    "synth_hippo27": (3, 9.32, 27, True, "hippo19"),
 }
 SAMPLE = 300  # atoms whose gradient is kept when the full gradient is not
 
@@ -142,17 +144,31 @@ def save(name: str, d: dict, full_grad: bool) -> None:
     print(f"{name}: n={n} em={d['em']:.10f} nem={d['nem']}")
 
 
-def synth(name: str, k: int, L: float, seed: int, full_grad: bool) -> None:
+def synth(name: str, k: int, L: float, seed: int, full_grad: bool, ff: str) -> None:
     from polmdqc_amd import water
 
     tmp = tempfile.mkdtemp(prefix="pmq_golden_")
     xyzf = os.path.join(tmp, name + ".xyz")
     keyf = os.path.join(tmp, name + ".key")
-    water.write_tinker_xyz(xyzf, water.lattice_box(k, L, seed), "water03", name)
+    water.write_tinker_xyz(xyzf, water.lattice_box(k, L, seed), ff, name)
+    # multipoles only: with polarization on, kmpole.cpp:704-726 decides the
+    # penetration type before kpolar counts the charge-penetration sites, so
+    # a HIPPO file would run without charge penetration
+    terms = "multipoleterm only\n"
     with open(keyf, "w") as f:
-        f.write(f"parameters {REF}/params/water03.prm\ndigits 8\nmultipoleterm only\n"
+        f.write(f"parameters {REF}/params/{ff}.prm\ndigits 8\n{terms}"
                 f"a-axis {L}\nmpole-cutoff 7.0\n")
     save(name, run_dump(xyzf, keyf), full_grad)
+
+
+def write_water_params() -> None:
+    """Water parameter tables the package's water builder reads (data
+    extracted from the reference's params/*.prm)."""
+    from polmdqc_amd import prm
+
+    for ff, types in (("water03", {1, 2}), ("hippo19", {16, 17})):
+        d = prm.parse_prm(open(os.path.join(REF, "params", ff + ".prm")).read(), types)
+        prm.save(d, os.path.join(REPO, "polmdqc_amd", "data", ff + ".json"))
 
 
 def main() -> None:
@@ -166,10 +182,11 @@ def main() -> None:
         if args.only and name not in args.only:
             continue
         save(name, run_dump(xyz, key), full)
-    for name, (k, L, seed, full) in SYNTH.items():
+    for name, (k, L, seed, full, ff) in SYNTH.items():
         if args.only and name not in args.only:
             continue
-        synth(name, k, L, seed, full)
+        synth(name, k, L, seed, full, ff)
+    write_water_params()
 
 
 if __name__ == "__main__":

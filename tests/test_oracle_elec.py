@@ -1,0 +1,152 @@
+"""Pins the oracle's Ewald multipoles and AMOEBA polarization (CPU only).
+
+The reference computes none of these terms (empole.cpp:53-60, energy.cpp:267),
+so they are pinned by what the mathematics fixes:
+
+* the Ewald total (real + self + reciprocal) is independent of the Ewald
+  coefficient when the reciprocal sum is done exactly; the exact sum here is
+  an independent numpy structure-factor sum over k-vectors;
+* the smooth-PME reciprocal energy converges to that exact sum as the grid
+  is refined;
+* the induced dipoles from preconditioned CG equal a dense linear solve;
+* analytic gradients equal central finite differences of the energy, for
+  the Ewald-summed periodic box and for a free cluster;
+* the water parameter assignment equals the reference's own setup (kmpole,
+  kpolar) on the same water molecules.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import load_fixture
+from polmdqc_amd import water
+
+F = 332.0637133
+L = 9.32
+XYZ = water.lattice_box(3, L, 27)
+
+
+def ewald_box(aewald, nfft, polar, cutoff=4.6):
+    return water.water_system(XYZ, "water03", box=L, ewald={"aewald": aewald, "nfft": nfft, "cutoff": cutoff},
+                              polar=polar)
+
+
+def exact_recip(pos, m10, box, a, kmax):
+    """E_rec = f/(2 pi V) sum_{h!=0} exp(-pi^2 h^2/a^2)/h^2 |S(h)|^2 with the
+    multipole structure factor S(h) = sum_j [c + 2 pi i h.d - 4 pi^2 h.q.h] e^{2 pi i h.r}."""
+    V = box ** 3
+    r = np.arange(-kmax, kmax + 1)
+    m = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+    m = m[np.any(m != 0, axis=1)]
+    H = m / box
+    h2 = np.sum(H * H, 1)
+    c, d = m10[:, 0], m10[:, 1:4]
+    q = m10[:, [4, 5, 6, 5, 7, 8, 6, 8, 9]].reshape(-1, 3, 3)
+    S = np.zeros(len(H), complex)
+    for j in range(len(pos)):
+        hq = np.einsum("ka,ab,kb->k", H, q[j], H)
+        S += (c[j] + 2j * np.pi * (H @ d[j]) - 4 * np.pi ** 2 * hq) * np.exp(2j * np.pi * (H @ pos[j]))
+    return F / (2 * np.pi * V) * np.sum(np.exp(-np.pi ** 2 * h2 / a ** 2) / h2 * np.abs(S) ** 2)
+
+
+def global_multipoles(s):
+    rp = oracle.empole(s, oracle.ANALYSIS)["rpole"]
+    return rp[:, [0, 1, 2, 3, 4, 5, 6, 8, 9, 12]]
+
+
+def test_ewald_total_is_independent_of_alpha():
+    m10 = global_multipoles(ewald_box(1.0, 24, False))
+    totals = []
+    for a in (1.0, 1.1, 1.2):
+        r = oracle.elec(ewald_box(a, 24, False), want_grad=False)
+        totals.append(r["em_real"] + r["em_self"] + exact_recip(XYZ, m10, L, a, 30))
+    assert max(totals) - min(totals) < 1e-7 * abs(totals[0])
+
+
+def test_pme_reciprocal_converges_to_exact_sum():
+    a = 0.7
+    m10 = global_multipoles(ewald_box(a, 24, False))
+    exact = exact_recip(XYZ, m10, L, a, 25)
+    errs = [abs(oracle.elec(ewald_box(a, K, False), want_grad=False)["em_recip"] - exact) / exact for K in (16, 24, 48)]
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[2] < 5e-6
+
+
+def test_self_energy_closed_form():
+    """-f a/sqrt(pi) sum [c^2 + 2a^2/3 d.d + 8a^4/5 q:q]; a wrong factor breaks
+    the alpha-independence above, this pins the value itself."""
+    a = 0.9
+    s = ewald_box(a, 24, False)
+    m = global_multipoles(s)
+    qq = m[:, 4] ** 2 + m[:, 7] ** 2 + m[:, 9] ** 2 + 2 * (m[:, 5] ** 2 + m[:, 6] ** 2 + m[:, 8] ** 2)
+    dd = np.sum(m[:, 1:4] ** 2, 1)
+    expect = -F * a / np.sqrt(np.pi) * np.sum(m[:, 0] ** 2 + 2 * a * a / 3 * dd + 8 * a ** 4 / 5 * qq)
+    assert abs(oracle.elec(s, want_grad=False)["em_self"] - expect) < 1e-10 * abs(expect)
+
+
+def test_pcg_equals_dense_solve():
+    s = ewald_box(0.7, 24, True)
+    it = oracle.elec(s, want_grad=False, poleps=1e-10)
+    dn = oracle.elec(s, want_grad=False, solver=1)
+    assert it["iters"] > 0
+    np.testing.assert_allclose(it["uind"], dn["uind"], atol=1e-9)
+    np.testing.assert_allclose(it["uinp"], dn["uinp"], atol=1e-9)
+    assert abs(it["ep"] - dn["ep"]) < 1e-9 * abs(dn["ep"])
+
+
+def test_polarization_energy_nearly_independent_of_alpha():
+    e = [oracle.elec(ewald_box(a, 48, True), want_grad=False, poleps=1e-10)["ep"] for a in (0.8, 1.0)]
+    assert abs(e[0] - e[1]) < 5e-5 * abs(e[0])
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_gradients_match_finite_differences(periodic):
+    if periodic:
+        s, x0 = ewald_box(0.7, 24, True), XYZ
+    else:
+        x0 = water.cluster(6, np.random.default_rng(5))
+        s = water.water_system(x0, "water03")
+    r = oracle.elec(s, want_grad=True, poleps=1e-11)
+    h = 1e-5
+    rng = np.random.default_rng(1)
+    for i in rng.choice(len(x0), 3, replace=False):
+        for a in range(3):
+            xp, xm = x0.copy(), x0.copy()
+            xp[i, a] += h
+            xm[i, a] -= h
+            p = oracle.elec(s, want_grad=False, x=xp, poleps=1e-11)
+            m = oracle.elec(s, want_grad=False, x=xm, poleps=1e-11)
+            assert abs((p["em"] - m["em"]) / (2 * h) - r["dem"][i, a]) < 2e-6 * max(1, abs(r["dem"][i, a]))
+            assert abs((p["ep"] - m["ep"]) / (2 * h) - r["dep"][i, a]) < 2e-6 * max(1, abs(r["dep"][i, a]))
+
+
+def test_non_ewald_multipoles_equal_the_reference_path():
+    """Without Ewald the elec oracle's multipole energy is the reference's
+    empole_a (already pinned to the reference's goldens)."""
+    fx = load_fixture("synth_water216")
+    a = oracle.elec(fx, want_grad=True, polar=False)
+    b = oracle.empole(fx, oracle.GRADIENT)
+    assert abs(a["em"] - b["em"]) < 1e-10 * abs(b["em"])
+    np.testing.assert_allclose(a["dem"], b["dem"], atol=1e-9)
+
+
+@pytest.mark.parametrize("system,ff", [("bounds_orthogonal", "water03"), ("synth_hippo27", "hippo19")])
+def test_water_setup_matches_reference(system, ff):
+    fx = load_fixture(system)
+    n = int(fx["n"])
+    s = water.water_system(np.stack([fx["x"], fx["y"], fx["z"]], 1), ff, box=float(fx["xbox"]))
+    np.testing.assert_allclose(s["pole"], fx["pole"], rtol=0, atol=1e-15)
+    for k in ("polaxe", "zaxis", "xaxis", "yaxis", "n12", "i12", "n13", "i13", "n14", "n15", "np11", "ip11",
+              "molcule", "use"):
+        np.testing.assert_array_equal(np.asarray(s[k]), np.asarray(fx[k]), err_msg=k)
+    for k in ("polarity", "thole", "pdamp"):
+        np.testing.assert_allclose(s[k], fx[k], rtol=1e-15, err_msg=k)
+    for k in ("m2scale", "m3scale", "m4scale", "m5scale", "p2scale", "p3scale", "p4scale", "p5scale",
+              "p2iscale", "p3iscale", "p4iscale", "p5iscale", "d1scale", "d2scale", "d3scale", "d4scale",
+              "u1scale", "u2scale", "u3scale", "u4scale"):
+        assert s[k] == float(fx[k]), k
+    if ff == "hippo19":
+        np.testing.assert_allclose(s["pcore"], fx["pcore"], rtol=1e-15)
+        np.testing.assert_allclose(s["pval0"], fx["pval0"], rtol=1e-15)
+        np.testing.assert_allclose(s["palpha"], fx["palpha"], rtol=1e-15)
+    assert n == len(s["x"])
