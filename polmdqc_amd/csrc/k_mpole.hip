@@ -242,16 +242,15 @@ struct MpoleArgs {
     const double *sx, *sy, *sz, *smp, *scp;
     const int *sflag, *smol, *perm, *iperm;
     DevBox box;
-    double off2, f;
+    double off2, f, aewald;
     double *dem, *te, *pot, *aem;
     Totals *tot;
     // exclusion corrections
     const ExclPair *excl;
     int nexcl;
-    double mscale[6];
 };
 
-template <int PEN, bool CF, bool ANA, bool GRAD, bool VIR>
+template <int PEN, bool CF, bool ANA, bool GRAD, bool VIR, bool EW>
 __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
 {
     using R = double;
@@ -312,11 +311,18 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
             mk.qxx = jm[4][jj]; mk.qxy = jm[5][jj]; mk.qxz = jm[6][jj];
             mk.qyy = jm[7][jj]; mk.qyz = jm[8][jj]; mk.qzz = jm[9][jj];
             PairOut<R> o;
+            R bn[6];
+            if (EW) ewald_series<R>(r2, R(a.aewald), bn);
             if (CP) {
-                pair_cp<R, GRAD, (PEN == 0 ? 1 : PEN)>(r2, xr, yr, zr, a.f, corei, alphai, mi, jc[0][jj], jc[1][jj], mk, o);
+                pair_cp<R, GRAD, (PEN == 0 ? 1 : PEN), EW>(r2, xr, yr, zr, a.f, corei, alphai, mi, jc[0][jj],
+                                                           jc[1][jj], mk, o, bn);
             } else {
                 R Rs[6];
-                rr_series<R>(r2, a.f, Rs);
+                if (EW) {
+                    for (int q = 0; q < 6; q++) Rs[q] = R(a.f) * bn[q];
+                } else {
+                    rr_series<R>(r2, a.f, Rs);
+                }
                 pair_series<R, GRAD>(Rs, mi, mk, xr, yr, zr, o);
             }
             e += o.e;
@@ -392,13 +398,15 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
 // counted it at scale 1); a pair whose scale is 0 is not counted in nem
 template <int PEN, bool CF, bool ANA, bool GRAD, bool VIR>
 __global__ __launch_bounds__(64) void k_mpole_excl(MpoleArgs a)
+// with or without Ewald the correction is (scale - 1) times the undamped
+// (or Gordon-damped) pair: B_n - (1-m) rr_n = B_n + (m-1) rr_n
 {
     using R = double;
     constexpr bool CP = PEN != 0;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nexcl) return;
     const ExclPair q = a.excl[t];
-    const R sc = R(a.mscale[q.cls]);
+    const R sc = R(q.m);
     if (sc == R(1)) return;
     const int si = a.iperm[q.i], sk = a.iperm[q.k];
     const int fi = a.sflag[si], fk = a.sflag[sk];
@@ -456,7 +464,11 @@ template <int PEN, bool CF, bool ANA, bool GRAD, bool VIR>
 static int launch_mpole_t(pmq_ctx *c, const MpoleArgs &a)
 {
     const int grid = std::max(1, std::min(c->ntiles_cap, 16384));
-    hipLaunchKernelGGL((k_mpole_tiles<PEN, CF, ANA, GRAD, VIR>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
+    if (a.aewald > 0.0)
+        hipLaunchKernelGGL((k_mpole_tiles<PEN, CF, ANA, GRAD, VIR, true>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_mpole_tiles<PEN, CF, ANA, GRAD, VIR, false>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream,
+                           a);
     if (a.nexcl > 0)
         hipLaunchKernelGGL((k_mpole_excl<PEN, CF, ANA, GRAD, VIR>), dim3((a.nexcl + 63) / 64), dim3(64), 0,
                            c->stream, a);
@@ -492,9 +504,7 @@ int launch_mpole_direct(pmq_ctx *c, int mode, double off2)
     a.tot = c->totals.p;
     a.excl = c->excl.p;
     a.nexcl = (int)c->h_excl.size();
-    a.mscale[0] = a.mscale[1] = 1.0;
-    a.mscale[2] = c->prm.m2scale; a.mscale[3] = c->prm.m3scale;
-    a.mscale[4] = c->prm.m4scale; a.mscale[5] = c->prm.m5scale;
+    a.aewald = c->prm.use_ewald ? c->prm.aewald : 0.0;
     const int pen = c->prm.pentype;
     const bool cf = pen != PMQ_PEN_NONE && c->prm.use_chgflx;
     if (pen == PMQ_PEN_NONE) return launch_mpole_mode<0, false>(c, a, mode);
@@ -663,13 +673,13 @@ __global__ __launch_bounds__(128) void k_torque(TorqueArgs a)
     }
 }
 
-int launch_torque(pmq_ctx *c, int mode)
+int launch_torque(pmq_ctx *c, int mode, const double *te, double *de)
 {
     TorqueArgs a;
     a.n = c->n;
-    a.x = c->x.p; a.y = c->y.p; a.z = c->z.p; a.te = c->te.p;
+    a.x = c->x.p; a.y = c->y.p; a.z = c->z.p; a.te = te;
     a.polaxe = c->polaxe.p; a.zaxis = c->zaxis.p; a.xaxis = c->xaxis.p; a.yaxis = c->yaxis.p;
-    a.de = c->dem.p;
+    a.de = de;
     a.tot = c->totals.p;
     a.vir = mode == PMQ_VIRIAL;
     hipLaunchKernelGGL(k_torque, dim3((c->n + 127) / 128), dim3(128), 0, c->stream, a);

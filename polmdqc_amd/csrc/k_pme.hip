@@ -1,0 +1,700 @@
+// k_pme.hip -- smooth particle-mesh Ewald for multipoles and induced dipoles.
+//
+// Algorithm (Essmann et al. 1995; Sagui, Pedersen and Darden 2004): every
+// site's multipole is expressed in fractional grid coordinates u = A r
+// (A = K * reciprocal box), spread onto the K0 x K1 x K2 grid with order-p
+// cardinal B-splines and their derivatives, transformed with rocFFT,
+// multiplied by the reciprocal kernel
+//     G(m) = exp(-pi^2 h^2 / a^2) / (pi V h^2 |b1 b2 b3|^2)
+// and transformed back to the potential; the potential and its derivatives
+// (up to third order) are then interpolated back at every site.
+//
+// Grids: real grid index (g3*K1 + g2)*K0 + g1 (g1 fastest), complex half
+// grid ((K0/2+1) along g1).  One atom per thread; spreading uses FP64
+// atomics on the grid (it sits in L2: 48^3 doubles = 0.9 MB).
+#include <rocfft/rocfft.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "pmq_internal.h"
+#include "pme_common.h"
+
+namespace pmq {
+
+// ---------------------------------------------------------------------------
+// host setup: A matrix, B-spline moduli, reciprocal kernel, FFT plans
+static double bspline_m(int n, double x)
+{
+    if (x <= 0.0 || x >= n) return 0.0;
+    if (n == 2) return 1.0 - fabs(x - 1.0);
+    return (x * bspline_m(n - 1, x) + (n - x) * bspline_m(n - 1, x - 1.0)) / (n - 1);
+}
+
+static int fft_check(pmq_ctx *c, rocfft_status s, const char *what)
+{
+    if (s != rocfft_status_success) {
+        c->err = std::string("rocFFT: ") + what + " failed";
+        return PMQ_ERR_HIP;
+    }
+    return PMQ_OK;
+}
+
+static void destroy_plans(pmq_ctx *c)
+{
+    void **plans[4] = {&c->plan_f1, &c->plan_b1, &c->plan_f2, &c->plan_b2};
+    for (auto p : plans)
+        if (*p) {
+            rocfft_plan_destroy((rocfft_plan)*p);
+            *p = nullptr;
+        }
+    if (c->fft_info) {
+        rocfft_execution_info_destroy((rocfft_execution_info)c->fft_info);
+        c->fft_info = nullptr;
+    }
+}
+
+void pme_release(pmq_ctx *c) { destroy_plans(c); }
+
+int pme_setup(pmq_ctx *c)
+{
+    const pmq_params &p = c->prm;
+    const int K[3] = {p.nfft1, p.nfft2, p.nfft3};
+    if (K[0] < 8 || K[1] < 8 || K[2] < 8 || p.bsorder < 4 || p.bsorder > PMQ_MAX_ORDER) {
+        c->err = "PME: grid sizes must be >= 8 and the B-spline order 4..6";
+        return PMQ_ERR_ARG;
+    }
+    if (c->box.kind != PMQ_BOX_ORTHO && c->box.kind != PMQ_BOX_MONO && c->box.kind != PMQ_BOX_TRI) {
+        c->err = "PME: Ewald summation needs an orthogonal, monoclinic or triclinic box";
+        return PMQ_ERR_ARG;
+    }
+    const bool same_grid = K[0] == c->K[0] && K[1] == c->K[1] && K[2] == c->K[2] && c->bsorder == p.bsorder;
+    bool same_box = same_grid && c->pme_alpha == p.aewald && c->pme_dbox.kind == c->box.kind;
+    for (int a = 0; a < 3 && same_box; a++)
+        for (int b = 0; b < 3; b++) same_box = same_box && c->pme_dbox.recip[a][b] == c->box.recip[a][b];
+    if (same_box && c->plan_f1) return PMQ_OK;
+    c->K[0] = K[0]; c->K[1] = K[1]; c->K[2] = K[2];
+    c->bsorder = p.bsorder;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) c->pmeA[a][b] = K[a] * c->box.recip[a][b];
+    // B-spline moduli, zeros replaced by the mean of their neighbours
+    std::vector<double> bsmod[3];
+    for (int a = 0; a < 3; a++) {
+        bsmod[a].assign(K[a], 0.0);
+        for (int m = 0; m < K[a]; m++) {
+            double re = 0, im = 0;
+            for (int k = 0; k < p.bsorder - 1; k++) {
+                const double w = bspline_m(p.bsorder, k + 1.0);
+                const double arg = 2.0 * M_PI * m * k / K[a];
+                re += w * cos(arg);
+                im += w * sin(arg);
+            }
+            bsmod[a][m] = re * re + im * im;
+        }
+        for (int m = 0; m < K[a]; m++)
+            if (bsmod[a][m] < 1e-7) bsmod[a][m] = 0.5 * (bsmod[a][(m - 1 + K[a]) % K[a]] + bsmod[a][(m + 1) % K[a]]);
+    }
+    const double (*A)[3] = c->pmeA;
+    const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0])
+                     + A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+    const double V = (double)K[0] * K[1] * K[2] / det;
+    const int H0 = K[0] / 2 + 1;
+    std::vector<double> G((size_t)H0 * K[1] * K[2]);
+    const double a2 = p.aewald * p.aewald;
+    for (int m3 = 0; m3 < K[2]; m3++)
+        for (int m2 = 0; m2 < K[1]; m2++)
+            for (int m1 = 0; m1 < H0; m1++) {
+                const size_t id = ((size_t)m3 * K[1] + m2) * H0 + m1;
+                if (m1 == 0 && m2 == 0 && m3 == 0) { G[id] = 0.0; continue; }
+                const int f1 = m1 > K[0] / 2 ? m1 - K[0] : m1;
+                const int f2 = m2 > K[1] / 2 ? m2 - K[1] : m2;
+                const int f3 = m3 > K[2] / 2 ? m3 - K[2] : m3;
+                double h[3];
+                for (int j = 0; j < 3; j++) h[j] = f1 * A[0][j] / K[0] + f2 * A[1][j] / K[1] + f3 * A[2][j] / K[2];
+                const double h2 = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
+                G[id] = exp(-M_PI * M_PI * h2 / a2) / (M_PI * V * h2 * bsmod[0][m1] * bsmod[1][m2] * bsmod[2][m3]);
+            }
+    PMQ_HIP(c->pme_g.alloc(G.size()));
+    PMQ_HIP(hipMemcpyAsync(c->pme_g.p, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    PMQ_HIP(hipStreamSynchronize(c->stream)); // G is a host temporary
+    const size_t nreal = (size_t)K[0] * K[1] * K[2], ncplx = (size_t)H0 * K[1] * K[2];
+    PMQ_HIP(c->qgrid.alloc(2 * nreal));
+    PMQ_HIP(c->cgrid.alloc(2 * 2 * ncplx));
+    if (!same_grid || !c->plan_f1) {
+        destroy_plans(c);
+        int rc;
+        if ((rc = fft_check(c, rocfft_setup(), "setup"))) return rc;
+        const size_t len[3] = {(size_t)K[0], (size_t)K[1], (size_t)K[2]};
+        rocfft_plan pf1, pb1, pf2, pb2;
+        if ((rc = fft_check(c, rocfft_plan_create(&pf1, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                                                  rocfft_precision_double, 3, len, 1, nullptr), "plan"))) return rc;
+        c->plan_f1 = pf1;
+        if ((rc = fft_check(c, rocfft_plan_create(&pb1, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                                                  rocfft_precision_double, 3, len, 1, nullptr), "plan"))) return rc;
+        c->plan_b1 = pb1;
+        if ((rc = fft_check(c, rocfft_plan_create(&pf2, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                                                  rocfft_precision_double, 3, len, 2, nullptr), "plan"))) return rc;
+        c->plan_f2 = pf2;
+        if ((rc = fft_check(c, rocfft_plan_create(&pb2, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                                                  rocfft_precision_double, 3, len, 2, nullptr), "plan"))) return rc;
+        c->plan_b2 = pb2;
+        size_t wmax = 0;
+        for (rocfft_plan pl : {pf1, pb1, pf2, pb2}) {
+            size_t w = 0;
+            rocfft_plan_get_work_buffer_size(pl, &w);
+            wmax = std::max(wmax, w);
+        }
+        PMQ_HIP(c->fft_work.alloc(wmax));
+        rocfft_execution_info info;
+        if ((rc = fft_check(c, rocfft_execution_info_create(&info), "info"))) return rc;
+        c->fft_info = info;
+        if ((rc = fft_check(c, rocfft_execution_info_set_stream(info, c->stream), "stream"))) return rc;
+        if (wmax && (rc = fft_check(c, rocfft_execution_info_set_work_buffer(info, c->fft_work.p, wmax), "work"))) return rc;
+    }
+    c->pme_dbox = c->box;
+    c->pme_alpha = p.aewald;
+    return PMQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+struct PmeGeom {
+    int K0, K1, K2, order;
+    double A[3][3];
+};
+
+static PmeGeom geom(const pmq_ctx *c)
+{
+    PmeGeom g;
+    g.K0 = c->K[0]; g.K1 = c->K[1]; g.K2 = c->K[2];
+    g.order = c->bsorder;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) g.A[a][b] = c->pmeA[a][b];
+    return g;
+}
+
+// B-spline weights of one atom along the three axes: th[axis][k][deriv]
+template <int ND>
+__device__ __forceinline__ void atom_splines(const PmeGeom &g, double px, double py, double pz, double th[3][PMQ_MAX_ORDER][4],
+                                             int *i0)
+{
+    const double p[3] = {px, py, pz};
+    const int K[3] = {g.K0, g.K1, g.K2};
+    for (int a = 0; a < 3; a++) {
+        const double u = g.A[a][0] * p[0] + g.A[a][1] * p[1] + g.A[a][2] * p[2];
+        const double fl = floor(u);
+        const double w = u - fl;
+        int s = ((int)fl - g.order + 1) % K[a];
+        if (s < 0) s += K[a];
+        i0[a] = s;
+        bspline_weights<double, ND>(w, g.order, th[a]);
+    }
+}
+
+// fractional multipole: c, A d, A q A^T (off-diagonals doubled)
+__device__ __forceinline__ void frac_mpole(const PmeGeom &g, const double *m, double *fm)
+{
+    fm[0] = m[0];
+    for (int a = 0; a < 3; a++) fm[1 + a] = g.A[a][0] * m[1] + g.A[a][1] * m[2] + g.A[a][2] * m[3];
+    const double Q[3][3] = {{m[4], m[5], m[6]}, {m[5], m[7], m[8]}, {m[6], m[8], m[9]}};
+    double F[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = a; b < 3; b++) {
+            double v = 0;
+            for (int p = 0; p < 3; p++) {
+                const double t = Q[p][0] * g.A[b][0] + Q[p][1] * g.A[b][1] + Q[p][2] * g.A[b][2];
+                v += g.A[a][p] * t;
+            }
+            F[a][b] = v;
+        }
+    fm[4] = F[0][0]; fm[5] = F[1][1]; fm[6] = F[2][2];
+    fm[7] = 2 * F[0][1]; fm[8] = 2 * F[0][2]; fm[9] = 2 * F[1][2];
+}
+
+struct SpreadArgs {
+    int n, npad;
+    PmeGeom g;
+    const double *sx, *sy, *sz;
+    const double *src;   // sorted SoA sources: 10 multipole comps (mode 0) or 3 dipole comps x nset (mode 1)
+    const double *charge; // mode 0: total monopole per sorted atom (CP systems), or null
+    double *grid;        // nset grids
+    int nset;
+    size_t gstride;
+};
+
+// mode 0: permanent multipoles (one grid); mode 1: dipole sets (nset grids)
+template <int MODE>
+__global__ __launch_bounds__(128) void k_pme_spread(SpreadArgs a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.n) return;
+    double th[3][PMQ_MAX_ORDER][4];
+    int i0[3];
+    atom_splines<MODE == 0 ? 3 : 2>(a.g, a.sx[s], a.sy[s], a.sz[s], th, i0);
+    const int ord = a.g.order;
+    for (int set = 0; set < (MODE == 0 ? 1 : a.nset); set++) {
+        double fm[10];
+        if (MODE == 0) {
+            double m[10];
+            for (int c = 0; c < 10; c++) m[c] = a.src[c * a.npad + s];
+            if (a.charge) m[0] = a.charge[s];
+            frac_mpole(a.g, m, fm);
+        } else {
+            const double *d = a.src + (size_t)set * 3 * a.npad;
+            const double dx = d[s], dy = d[a.npad + s], dz = d[2 * a.npad + s];
+            fm[0] = 0;
+            for (int q = 0; q < 3; q++) fm[1 + q] = a.g.A[q][0] * dx + a.g.A[q][1] * dy + a.g.A[q][2] * dz;
+        }
+        double *grid = a.grid + set * a.gstride;
+        for (int kz = 0; kz < ord; kz++) {
+            int gz = i0[2] + kz;
+            if (gz >= a.g.K2) gz -= a.g.K2;
+            const double *tz = th[2][kz];
+            for (int ky = 0; ky < ord; ky++) {
+                int gy = i0[1] + ky;
+                if (gy >= a.g.K1) gy -= a.g.K1;
+                const double *ty = th[1][ky];
+                // coefficients of the x-spline derivatives 0, 1, 2 for this (y, z)
+                double c0, c1, c2 = 0.0;
+                if (MODE == 0) {
+                    c0 = fm[0] * ty[0] * tz[0] + fm[2] * ty[1] * tz[0] + fm[3] * ty[0] * tz[1] + fm[5] * ty[2] * tz[0]
+                       + fm[6] * ty[0] * tz[2] + fm[9] * ty[1] * tz[1];
+                    c1 = fm[1] * ty[0] * tz[0] + fm[7] * ty[1] * tz[0] + fm[8] * ty[0] * tz[1];
+                    c2 = fm[4] * ty[0] * tz[0];
+                } else {
+                    c0 = fm[2] * ty[1] * tz[0] + fm[3] * ty[0] * tz[1];
+                    c1 = fm[1] * ty[0] * tz[0];
+                }
+                double *row = grid + ((size_t)gz * a.g.K1 + gy) * a.g.K0;
+                for (int kx = 0; kx < ord; kx++) {
+                    int gx = i0[0] + kx;
+                    if (gx >= a.g.K0) gx -= a.g.K0;
+                    const double *tx = th[0][kx];
+                    atomicAdd(row + gx, c0 * tx[0] + c1 * tx[1] + c2 * tx[2]);
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_pme_convolve(int ncplx, int nset, const double *__restrict__ G, double *__restrict__ cg)
+{
+    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (t >= (size_t)ncplx * nset) return;
+    const double g = G[t % ncplx];
+    cg[2 * t] *= g;
+    cg[2 * t + 1] *= g;
+}
+
+// gather the potential derivatives at one site from one grid: out[20] in
+// the fractional order 0; 1,2,3; 11,22,33,12,13,23; 111,222,333,112,113,
+// 122,133,223,233,123
+template <int NOUT>
+__device__ __forceinline__ void gather_site(const PmeGeom &g, const double *grid, const double th[3][PMQ_MAX_ORDER][4],
+                                            const int *i0, double *o)
+{
+    for (int j = 0; j < NOUT; j++) o[j] = 0.0;
+    const int ord = g.order;
+    for (int kz = 0; kz < ord; kz++) {
+        int gz = i0[2] + kz;
+        if (gz >= g.K2) gz -= g.K2;
+        const double *tz = th[2][kz];
+        for (int ky = 0; ky < ord; ky++) {
+            int gy = i0[1] + ky;
+            if (gy >= g.K1) gy -= g.K1;
+            const double *ty = th[1][ky];
+            // x-direction partial sums for derivative orders 0..3
+            double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+            const double *row = grid + ((size_t)gz * g.K1 + gy) * g.K0;
+            for (int kx = 0; kx < ord; kx++) {
+                int gx = i0[0] + kx;
+                if (gx >= g.K0) gx -= g.K0;
+                const double q = row[gx];
+                const double *tx = th[0][kx];
+                s0 += q * tx[0];
+                s1 += q * tx[1];
+                if (NOUT > 4) s2 += q * tx[2];
+                if (NOUT > 10) s3 += q * tx[3];
+            }
+            o[0] += s0 * ty[0] * tz[0];
+            o[1] += s1 * ty[0] * tz[0];
+            o[2] += s0 * ty[1] * tz[0];
+            o[3] += s0 * ty[0] * tz[1];
+            if (NOUT > 4) {
+                o[4] += s2 * ty[0] * tz[0];
+                o[5] += s0 * ty[2] * tz[0];
+                o[6] += s0 * ty[0] * tz[2];
+                o[7] += s1 * ty[1] * tz[0];
+                o[8] += s1 * ty[0] * tz[1];
+                o[9] += s0 * ty[1] * tz[1];
+            }
+            if (NOUT > 10) {
+                o[10] += s3 * ty[0] * tz[0];
+                o[11] += s0 * ty[3] * tz[0];
+                o[12] += s0 * ty[0] * tz[3];
+                o[13] += s2 * ty[1] * tz[0];
+                o[14] += s2 * ty[0] * tz[1];
+                o[15] += s1 * ty[2] * tz[0];
+                o[16] += s1 * ty[0] * tz[2];
+                o[17] += s0 * ty[2] * tz[1];
+                o[18] += s0 * ty[1] * tz[2];
+                o[19] += s1 * ty[1] * tz[1];
+            }
+        }
+    }
+}
+
+// fractional derivatives -> Cartesian phi, g[3], H (xx,yy,zz,xy,xz,yz), T[10]
+// (xxx,yyy,zzz,xxy,xxz,xyy,xzz,yyz,yzz,xyz)
+template <int NOUT>
+__device__ __forceinline__ void frac_to_cart(const PmeGeom &g, const double *f, double *phi, double *grad, double *H,
+                                             double *T)
+{
+    *phi = f[0];
+    for (int x = 0; x < 3; x++) grad[x] = g.A[0][x] * f[1] + g.A[1][x] * f[2] + g.A[2][x] * f[3];
+    if (NOUT > 4) {
+        const double F2[3][3] = {{f[4], f[7], f[8]}, {f[7], f[5], f[9]}, {f[8], f[9], f[6]}};
+        const int hx[6] = {0, 1, 2, 0, 0, 1}, hy[6] = {0, 1, 2, 1, 2, 2};
+        for (int q = 0; q < 6; q++) {
+            double v = 0;
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) v += g.A[a][hx[q]] * g.A[b][hy[q]] * F2[a][b];
+            H[q] = v;
+        }
+    }
+    if (NOUT > 10) {
+        // F3[a][b][c] from the 10 stored combinations
+        double F3[3][3][3];
+        const int idx[10][3] = {{0, 0, 0}, {1, 1, 1}, {2, 2, 2}, {0, 0, 1}, {0, 0, 2},
+                                {0, 1, 1}, {0, 2, 2}, {1, 1, 2}, {1, 2, 2}, {0, 1, 2}};
+        for (int q = 0; q < 10; q++) {
+            const int a = idx[q][0], b = idx[q][1], d = idx[q][2];
+            const double v = f[10 + q];
+            F3[a][b][d] = F3[a][d][b] = F3[b][a][d] = F3[b][d][a] = F3[d][a][b] = F3[d][b][a] = v;
+        }
+        for (int q = 0; q < 10; q++) {
+            const int x = idx[q][0], y = idx[q][1], z = idx[q][2];
+            double v = 0;
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++)
+                    for (int d = 0; d < 3; d++) v += g.A[a][x] * g.A[b][y] * g.A[d][z] * F3[a][b][d];
+            T[q] = v;
+        }
+    }
+}
+
+__device__ __forceinline__ double wsum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+struct GatherMpoleArgs {
+    int n, npad;
+    PmeGeom g;
+    const double *sx, *sy, *sz, *smp, *charge;
+    const int *perm;
+    const double *grid;
+    double f, aewald;
+    int grad, polar;
+    double *dem, *te;        // caller order
+    double *cphi;            // 10 * npad sorted (phi, g, H)
+    double *fieldd, *fieldp; // 3 * npad sorted (+= reciprocal + self field)
+    Totals *tot;
+};
+
+// permanent multipoles in their own reciprocal potential: energy (f/2) U,
+// gradient, torque, the self energy, and the reciprocal + self field the
+// induced dipoles see
+__global__ __launch_bounds__(64) void k_pme_gather_mpole(GatherMpoleArgs a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (s < a.n) {
+        double th[3][PMQ_MAX_ORDER][4];
+        int i0[3];
+        atom_splines<3>(a.g, a.sx[s], a.sy[s], a.sz[s], th, i0);
+        double fp[20];
+        gather_site<20>(a.g, a.grid, th, i0, fp);
+        double phi, gr[3], H[6], T[10];
+        frac_to_cart<20>(a.g, fp, &phi, gr, H, T);
+        double m[10];
+        for (int c = 0; c < 10; c++) m[c] = a.smp[c * a.npad + s];
+        if (a.charge) m[0] = a.charge[s];
+        const double Hm[3][3] = {{H[0], H[3], H[4]}, {H[3], H[1], H[5]}, {H[4], H[5], H[2]}};
+        const double Q[3][3] = {{m[4], m[5], m[6]}, {m[5], m[7], m[8]}, {m[6], m[8], m[9]}};
+        double U = m[0] * phi + m[1] * gr[0] + m[2] * gr[1] + m[3] * gr[2];
+        for (int p = 0; p < 3; p++)
+            for (int q = 0; q < 3; q++) U += Q[p][q] * Hm[p][q];
+        // self energy -f a/sqrt(pi) [c^2 + 2a^2/3 d.d + 8a^4/5 q:q]
+        const double al = a.aewald, a2 = al * al;
+        const double dd = m[1] * m[1] + m[2] * m[2] + m[3] * m[3];
+        const double qq = m[4] * m[4] + m[7] * m[7] + m[9] * m[9] + 2.0 * (m[5] * m[5] + m[6] * m[6] + m[8] * m[8]);
+        e = 0.5 * a.f * U - a.f * al / 1.77245385090551602730 * (m[0] * m[0] + 2.0 * a2 / 3.0 * dd + 1.6 * a2 * a2 * qq);
+        if (a.polar) {
+            a.cphi[s] = phi;
+            for (int q = 0; q < 3; q++) a.cphi[(1 + q) * a.npad + s] = gr[q];
+            for (int q = 0; q < 6; q++) a.cphi[(4 + q) * a.npad + s] = H[q];
+            const double self = 4.0 / 3.0 * a2 * al / 1.77245385090551602730;
+            for (int q = 0; q < 3; q++) {
+                const double v = -gr[q] + self * m[1 + q];
+                a.fieldd[q * a.npad + s] += v;
+                a.fieldp[q * a.npad + s] += v;
+            }
+        }
+        if (a.grad) {
+            // T as a full tensor
+            const double T3[3][3][3] = {{{T[0], T[3], T[4]}, {T[3], T[5], T[9]}, {T[4], T[9], T[6]}},
+                                        {{T[3], T[5], T[9]}, {T[5], T[1], T[7]}, {T[9], T[7], T[8]}},
+                                        {{T[4], T[9], T[6]}, {T[9], T[7], T[8]}, {T[6], T[8], T[2]}}};
+            const int i = a.perm[s];
+            for (int x = 0; x < 3; x++) {
+                double v = m[0] * gr[x] + m[1] * Hm[0][x] + m[2] * Hm[1][x] + m[3] * Hm[2][x];
+                for (int p = 0; p < 3; p++)
+                    for (int q = 0; q < 3; q++) v += Q[p][q] * T3[p][q][x];
+                a.dem[3 * i + x] += a.f * v;
+            }
+            // torque d x E - 2 eps:(q H), E = -g
+            double QH[3][3];
+            for (int p = 0; p < 3; p++)
+                for (int q = 0; q < 3; q++) QH[p][q] = Q[p][0] * Hm[0][q] + Q[p][1] * Hm[1][q] + Q[p][2] * Hm[2][q];
+            const double tq[3] = {-(m[2] * gr[2] - m[3] * gr[1]) - 2.0 * (QH[1][2] - QH[2][1]),
+                                  -(m[3] * gr[0] - m[1] * gr[2]) - 2.0 * (QH[2][0] - QH[0][2]),
+                                  -(m[1] * gr[1] - m[2] * gr[0]) - 2.0 * (QH[0][1] - QH[1][0])};
+            for (int x = 0; x < 3; x++) a.te[3 * i + x] += a.f * tq[x];
+        }
+    }
+    if (a.tot) {
+        e = wsum(e);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&a.tot->em, e);
+    }
+}
+
+struct GatherFieldArgs {
+    int n, npad, nset;
+    PmeGeom g;
+    const double *sx, *sy, *sz;
+    const double *grid;
+    size_t gstride;
+    const double *dip;  // nset x 3 x npad dipoles (for the self term)
+    double self;        // 4/3 a^3 / sqrt(pi)
+    double *field;      // nset x 3 x npad (+=)
+};
+
+// reciprocal + self field of the induced-dipole sets at every site
+__global__ __launch_bounds__(128) void k_pme_gather_field(GatherFieldArgs a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.n) return;
+    double th[3][PMQ_MAX_ORDER][4];
+    int i0[3];
+    atom_splines<1>(a.g, a.sx[s], a.sy[s], a.sz[s], th, i0);
+    for (int set = 0; set < a.nset; set++) {
+        double fp[4], phi, gr[3];
+        gather_site<4>(a.g, a.grid + set * a.gstride, th, i0, fp);
+        frac_to_cart<4>(a.g, fp, &phi, gr, nullptr, nullptr);
+        const size_t o = (size_t)set * 3 * a.npad;
+        for (int q = 0; q < 3; q++) a.field[o + q * a.npad + s] += -gr[q] + a.self * a.dip[o + q * a.npad + s];
+    }
+}
+
+struct GatherPolGradArgs {
+    int n, npad;
+    PmeGeom g;
+    const double *sx, *sy, *sz, *smp, *charge;
+    const double *cphi;          // permanent potential (phi, g, H)
+    const double *ud, *up;       // sorted dipoles
+    const int *perm;
+    const double *grid;          // two grids: mu_d, mu_p
+    size_t gstride;
+    double f, self;
+    double *dep, *tep;           // caller order
+};
+
+// reciprocal-space polarization gradient:
+//   (f/2) [(mu_d+mu_p).H_M + mu_d.H_p + mu_p.H_d] on the induced dipoles,
+//   (f/2) gradient and torque of the permanent multipoles in phi_d + phi_p,
+//   and the self-term torque d x (f/2) self (mu_d + mu_p)
+__global__ __launch_bounds__(64) void k_pme_gather_polgrad(GatherPolGradArgs a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.n) return;
+    double th[3][PMQ_MAX_ORDER][4];
+    int i0[3];
+    atom_splines<3>(a.g, a.sx[s], a.sy[s], a.sz[s], th, i0);
+    double fd[20], fq[20];
+    gather_site<20>(a.g, a.grid, th, i0, fd);
+    gather_site<20>(a.g, a.grid + a.gstride, th, i0, fq);
+    double phd, gd[3], Hd[6], Td[10], php, gp[3], Hp[6], Tp[10];
+    frac_to_cart<20>(a.g, fd, &phd, gd, Hd, Td);
+    frac_to_cart<20>(a.g, fq, &php, gp, Hp, Tp);
+    const int np = a.npad;
+    const double ud[3] = {a.ud[s], a.ud[np + s], a.ud[2 * np + s]};
+    const double up[3] = {a.up[s], a.up[np + s], a.up[2 * np + s]};
+    double HM[6];
+    for (int q = 0; q < 6; q++) HM[q] = a.cphi[(4 + q) * np + s];
+    auto full = [](const double *h, double M[3][3]) {
+        M[0][0] = h[0]; M[1][1] = h[1]; M[2][2] = h[2];
+        M[0][1] = M[1][0] = h[3]; M[0][2] = M[2][0] = h[4]; M[1][2] = M[2][1] = h[5];
+    };
+    double Hm[3][3], Hdm[3][3], Hpm[3][3];
+    full(HM, Hm);
+    full(Hd, Hdm);
+    full(Hp, Hpm);
+    double m[10];
+    for (int c = 0; c < 10; c++) m[c] = a.smp[c * np + s];
+    if (a.charge) m[0] = a.charge[s];
+    const double Q[3][3] = {{m[4], m[5], m[6]}, {m[5], m[7], m[8]}, {m[6], m[8], m[9]}};
+    // potential of mu_d + mu_p
+    double g[3], H[3][3], T[10];
+    for (int q = 0; q < 3; q++) g[q] = gd[q] + gp[q];
+    for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++) H[p][q] = Hdm[p][q] + Hpm[p][q];
+    for (int q = 0; q < 10; q++) T[q] = Td[q] + Tp[q];
+    const double T3[3][3][3] = {{{T[0], T[3], T[4]}, {T[3], T[5], T[9]}, {T[4], T[9], T[6]}},
+                                {{T[3], T[5], T[9]}, {T[5], T[1], T[7]}, {T[9], T[7], T[8]}},
+                                {{T[4], T[9], T[6]}, {T[9], T[7], T[8]}, {T[6], T[8], T[2]}}};
+    const int i = a.perm[s];
+    const double hf = 0.5 * a.f;
+    for (int x = 0; x < 3; x++) {
+        double v = 0.0;
+        for (int p = 0; p < 3; p++) v += (ud[p] + up[p]) * Hm[p][x] + ud[p] * Hpm[p][x] + up[p] * Hdm[p][x];
+        v += m[0] * g[x] + m[1] * H[0][x] + m[2] * H[1][x] + m[3] * H[2][x];
+        for (int p = 0; p < 3; p++)
+            for (int q = 0; q < 3; q++) v += Q[p][q] * T3[p][q][x];
+        a.dep[3 * i + x] += hf * v;
+    }
+    double QH[3][3];
+    for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++) QH[p][q] = Q[p][0] * H[0][q] + Q[p][1] * H[1][q] + Q[p][2] * H[2][q];
+    // torque in phi_d + phi_p, plus the self term d x (self (mu_d + mu_p))
+    const double E[3] = {-g[0] + a.self * (ud[0] + up[0]), -g[1] + a.self * (ud[1] + up[1]),
+                         -g[2] + a.self * (ud[2] + up[2])};
+    const double tq[3] = {m[2] * E[2] - m[3] * E[1] - 2.0 * (QH[1][2] - QH[2][1]),
+                          m[3] * E[0] - m[1] * E[2] - 2.0 * (QH[2][0] - QH[0][2]),
+                          m[1] * E[1] - m[2] * E[0] - 2.0 * (QH[0][1] - QH[1][0])};
+    for (int x = 0; x < 3; x++) a.tep[3 * i + x] += hf * tq[x];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+static int fft(pmq_ctx *c, bool forward, int nset)
+{
+    rocfft_plan pl = (rocfft_plan)(forward ? (nset == 1 ? c->plan_f1 : c->plan_f2) : (nset == 1 ? c->plan_b1 : c->plan_b2));
+    void *in[1] = {forward ? (void *)c->qgrid.p : (void *)c->cgrid.p};
+    void *out[1] = {forward ? (void *)c->cgrid.p : (void *)c->qgrid.p};
+    return fft_check(c, rocfft_execute(pl, in, out, (rocfft_execution_info)c->fft_info), "execute");
+}
+
+static int pme_potential(pmq_ctx *c, int nset)
+{
+    int rc;
+    if ((rc = fft(c, true, nset))) return rc;
+    const int H0 = c->K[0] / 2 + 1;
+    const int ncplx = H0 * c->K[1] * c->K[2];
+    const size_t tot = (size_t)ncplx * nset;
+    hipLaunchKernelGGL(k_pme_convolve, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, ncplx, nset,
+                       c->pme_g.p, c->cgrid.p);
+    PMQ_HIP(hipGetLastError());
+    return fft(c, false, nset);
+}
+
+static size_t grid_size(const pmq_ctx *c) { return (size_t)c->K[0] * c->K[1] * c->K[2]; }
+
+int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar)
+{
+    int rc;
+    if ((rc = launch_zero(c, c->qgrid.p, grid_size(c)))) return rc;
+    SpreadArgs s;
+    s.n = c->n; s.npad = c->npad; s.g = geom(c);
+    s.sx = c->sx.p; s.sy = c->sy.p; s.sz = c->sz.p;
+    s.src = c->smp.p;
+    s.charge = c->prm.pentype != PMQ_PEN_NONE ? c->scp.p + 2 * (size_t)c->npad : nullptr;
+    s.grid = c->qgrid.p;
+    s.nset = 1;
+    s.gstride = grid_size(c);
+    hipLaunchKernelGGL(k_pme_spread<0>, dim3((c->n + 127) / 128), dim3(128), 0, c->stream, s);
+    PMQ_HIP(hipGetLastError());
+    if ((rc = pme_potential(c, 1))) return rc;
+    GatherMpoleArgs g;
+    g.n = c->n; g.npad = c->npad; g.g = geom(c);
+    g.sx = c->sx.p; g.sy = c->sy.p; g.sz = c->sz.p; g.smp = c->smp.p; g.charge = s.charge;
+    g.perm = c->perm.p;
+    g.grid = c->qgrid.p;
+    g.f = c->prm.electric / c->prm.dielec;
+    g.aewald = c->prm.aewald;
+    g.grad = mpole && (mode == PMQ_GRADIENT || mode == PMQ_VIRIAL);
+    g.polar = polar;
+    g.dem = c->dem.p; g.te = c->te.p;
+    g.cphi = c->cphi.p; g.fieldd = c->pfield.p; g.fieldp = c->pfield.p + 3 * (size_t)c->npad;
+    g.tot = mpole ? c->totals.p : nullptr;
+    hipLaunchKernelGGL(k_pme_gather_mpole, dim3((c->n + 63) / 64), dim3(64), 0, c->stream, g);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+int pme_dipole_field(pmq_ctx *c, const double *dip, double *field)
+{
+    // dip and field: two sets of 3 x npad, contiguous
+    int rc;
+    const size_t gs = grid_size(c);
+    if ((rc = launch_zero(c, c->qgrid.p, 2 * gs))) return rc;
+    SpreadArgs s;
+    s.n = c->n; s.npad = c->npad; s.g = geom(c);
+    s.sx = c->sx.p; s.sy = c->sy.p; s.sz = c->sz.p;
+    s.src = dip;
+    s.charge = nullptr;
+    s.grid = c->qgrid.p;
+    s.nset = 2;
+    s.gstride = gs;
+    hipLaunchKernelGGL(k_pme_spread<1>, dim3((c->n + 127) / 128), dim3(128), 0, c->stream, s);
+    PMQ_HIP(hipGetLastError());
+    if ((rc = pme_potential(c, 2))) return rc;
+    GatherFieldArgs g;
+    g.n = c->n; g.npad = c->npad; g.nset = 2; g.g = geom(c);
+    g.sx = c->sx.p; g.sy = c->sy.p; g.sz = c->sz.p;
+    g.grid = c->qgrid.p;
+    g.gstride = gs;
+    g.dip = dip;
+    g.self = 4.0 / 3.0 * c->prm.aewald * c->prm.aewald * c->prm.aewald / 1.77245385090551602730;
+    g.field = field;
+    hipLaunchKernelGGL(k_pme_gather_field, dim3((c->n + 127) / 128), dim3(128), 0, c->stream, g);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+int pme_polar_grad(pmq_ctx *c)
+{
+    int rc;
+    const size_t gs = grid_size(c);
+    if ((rc = launch_zero(c, c->qgrid.p, 2 * gs))) return rc;
+    SpreadArgs s;
+    s.n = c->n; s.npad = c->npad; s.g = geom(c);
+    s.sx = c->sx.p; s.sy = c->sy.p; s.sz = c->sz.p;
+    s.src = c->uind.p; // mu_d then mu_p
+    s.charge = nullptr;
+    s.grid = c->qgrid.p;
+    s.nset = 2;
+    s.gstride = gs;
+    hipLaunchKernelGGL(k_pme_spread<1>, dim3((c->n + 127) / 128), dim3(128), 0, c->stream, s);
+    PMQ_HIP(hipGetLastError());
+    if ((rc = pme_potential(c, 2))) return rc;
+    GatherPolGradArgs g;
+    g.n = c->n; g.npad = c->npad; g.g = geom(c);
+    g.sx = c->sx.p; g.sy = c->sy.p; g.sz = c->sz.p; g.smp = c->smp.p;
+    g.charge = c->prm.pentype != PMQ_PEN_NONE ? c->scp.p + 2 * (size_t)c->npad : nullptr;
+    g.cphi = c->cphi.p;
+    g.ud = c->uind.p; g.up = c->uind.p + 3 * (size_t)c->npad;
+    g.perm = c->perm.p;
+    g.grid = c->qgrid.p;
+    g.gstride = gs;
+    g.f = c->prm.electric / c->prm.dielec;
+    g.self = 4.0 / 3.0 * c->prm.aewald * c->prm.aewald * c->prm.aewald / 1.77245385090551602730;
+    g.dep = c->dep.p;
+    g.tep = c->tep.p;
+    hipLaunchKernelGGL(k_pme_gather_polgrad, dim3((c->n + 63) / 64), dim3(64), 0, c->stream, g);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+} // namespace pmq

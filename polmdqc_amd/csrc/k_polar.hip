@@ -1,0 +1,739 @@
+// k_polar.hip -- mutual induced-dipole polarization (AMOEBA, Thole damping).
+//
+// The reference declares the state of this term but computes nothing
+// (epolar is commented out at energy.cpp:267; induce/PCG defaults at
+// kpolar.cpp:80-103, polar.h:35-53, polpot.h:47-66, polpcg.h:21-25).  The
+// physics follows Ren and Ponder, J. Phys. Chem. B 107, 5933 (2003):
+//
+//   mu_d = alpha (E^d + T mu_d),   mu_p = alpha (E^p + T mu_p)
+//   E_pol = -(f/2) sum_i mu_d,i . E^p_i
+//
+// E^d / E^p are the permanent fields with the d (polarization group) and p
+// (bond connectivity) exclusion scales, T the dipole field operator with the
+// u scales; all Thole damped and, in a periodic box, Ewald summed.  The
+// gradient uses the variational form: at convergence it is the position
+// derivative at fixed dipoles of
+//   (1/2)[U_p(mu_d; M) + U_d(mu_p; M) + U_u(mu_d; mu_p) + U_u(mu_p; mu_d)]
+// so no second solve is needed.
+//
+// Kernels (sorted atom order, 2 dipole sets: 0 = d, 1 = p):
+//   k_pol_field_tiles / _excl   permanent field (scale 1 in the tiles,
+//                               (s-1) lambda rr corrections for scaled pairs)
+//   k_pol_mutual_tiles / _excl  T mu for both sets
+//   k_pcg_*                     preconditioned CG (diagonal preconditioner)
+//   k_pol_grad_tiles / _excl    real-space gradient and torques
+#include "pair_mpole.h"
+#include "pmq_internal.h"
+
+namespace pmq {
+
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+struct PolArgs {
+    int n, npad;
+    const int2 *tiles;
+    const int *ntiles;
+    const double *sx, *sy, *sz, *smp, *spol;
+    const int *sflag, *perm, *iperm;
+    DevBox box;
+    double off2, aewald, f;
+    const ExclPair *excl;
+    int nexcl;
+    const double *src;   // dipoles (2 x 3 x npad) for the mutual field / gradient
+    double *dst;         // fields (2 x 3 x npad)
+    double *dep, *tep;   // caller-order gradient and torque
+};
+
+// series of a polarization interaction at scale 1: Ewald + Thole, or Thole
+template <bool EW>
+__device__ __forceinline__ void pol_series(double r2, double aewald, const double *lam, double *R)
+{
+    double rr[6];
+    rr_series<double>(r2, 1.0, rr);
+    if (EW) {
+        double bn[6];
+        ewald_series<double>(r2, aewald, bn);
+        for (int q = 0; q < 6; q++) R[q] = bn[q] - (1.0 - lam[q]) * rr[q];
+    } else {
+        for (int q = 0; q < 6; q++) R[q] = lam[q] * rr[q];
+    }
+}
+
+// correction series (s - 1) lambda rr for a scaled pair
+__device__ __forceinline__ void corr_series(double r2, double s, const double *lam, double *R)
+{
+    double rr[6];
+    rr_series<double>(r2, 1.0, rr);
+    for (int q = 0; q < 6; q++) R[q] = (s - 1.0) * lam[q] * rr[q];
+}
+
+// field at i from multipole k and at k from multipole i
+__device__ __forceinline__ void mp_fields(const double *R, const Mp<double> &a, const Mp<double> &b, double xr, double yr,
+                                          double zr, double *Ei, double *Ek)
+{
+    const double qix = a.qxx * xr + a.qxy * yr + a.qxz * zr, qiy = a.qxy * xr + a.qyy * yr + a.qyz * zr,
+                 qiz = a.qxz * xr + a.qyz * yr + a.qzz * zr;
+    const double qkx = b.qxx * xr + b.qxy * yr + b.qxz * zr, qky = b.qxy * xr + b.qyy * yr + b.qyz * zr,
+                 qkz = b.qxz * xr + b.qyz * yr + b.qzz * zr;
+    const double dir = a.dx * xr + a.dy * yr + a.dz * zr, dkr = b.dx * xr + b.dy * yr + b.dz * zr;
+    const double qir = qix * xr + qiy * yr + qiz * zr, qkr = qkx * xr + qky * yr + qkz * zr;
+    const double si = b.c * R[1] - dkr * R[2] + qkr * R[3];
+    const double sk = a.c * R[1] + dir * R[2] + qir * R[3];
+    Ei[0] = -si * xr - R[1] * b.dx + 2.0 * R[2] * qkx;
+    Ei[1] = -si * yr - R[1] * b.dy + 2.0 * R[2] * qky;
+    Ei[2] = -si * zr - R[1] * b.dz + 2.0 * R[2] * qkz;
+    Ek[0] = sk * xr - R[1] * a.dx - 2.0 * R[2] * qix;
+    Ek[1] = sk * yr - R[1] * a.dy - 2.0 * R[2] * qiy;
+    Ek[2] = sk * zr - R[1] * a.dz - 2.0 * R[2] * qiz;
+}
+
+__device__ __forceinline__ Mp<double> load_mp(const double *smp, int np, int s)
+{
+    Mp<double> m;
+    m.c = smp[s]; m.dx = smp[np + s]; m.dy = smp[2 * np + s]; m.dz = smp[3 * np + s];
+    m.qxx = smp[4 * np + s]; m.qxy = smp[5 * np + s]; m.qxz = smp[6 * np + s];
+    m.qyy = smp[7 * np + s]; m.qyz = smp[8 * np + s]; m.qzz = smp[9 * np + s];
+    return m;
+}
+
+__device__ __forceinline__ Mp<double> dip(double x, double y, double z)
+{
+    Mp<double> m{};
+    m.dx = x; m.dy = y; m.dz = z;
+    return m;
+}
+
+// ---------------------------------------------------------------------------
+// tile kernels share this frame: lane = i atom of block I, j atoms of block J
+// in LDS, per-lane in-cutoff masks, j accumulators in LDS
+#define TILE_PROLOGUE                                                                                          \
+    const int lane = threadIdx.x;                                                                              \
+    const int np = a.npad;                                                                                     \
+    const int ntiles = *a.ntiles;
+
+template <bool EW>
+__global__ __launch_bounds__(64) void k_pol_field_tiles(PolArgs a)
+{
+    TILE_PROLOGUE
+    __shared__ double jx[64], jy[64], jz[64], jm[10][64], jpd[64], jth[64], acc[3][64];
+    __shared__ int jflag[64];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 IJ = a.tiles[t];
+        const int si = IJ.x * 64 + lane, sj = IJ.y * 64 + lane;
+        __syncthreads();
+        jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
+        for (int c = 0; c < 10; c++) jm[c][lane] = a.smp[c * np + sj];
+        jth[lane] = a.spol[np + sj];
+        jpd[lane] = a.spol[2 * np + sj];
+        jflag[lane] = a.sflag[sj];
+        acc[0][lane] = acc[1][lane] = acc[2][lane] = 0.0;
+        const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
+        const int fi = a.sflag[si];
+        const Mp<double> mi = load_mp(a.smp, np, si);
+        const double thi = a.spol[np + si], pdi = a.spol[2 * np + si];
+        __syncthreads();
+        unsigned long long mask = 0ull;
+        if (fi & 1) {
+            for (int jj = IJ.x == IJ.y ? lane + 1 : 0; jj < 64; jj++) {
+                if (!(jflag[jj] & 1)) continue;
+                double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
+                image<double>(a.box, xr, yr, zr);
+                if (xr * xr + yr * yr + zr * zr <= a.off2) mask |= 1ull << jj;
+            }
+        }
+        double ex = 0, ey = 0, ez = 0;
+        while (mask) {
+            const int jj = __ffsll((long long)mask) - 1;
+            mask &= mask - 1;
+            double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
+            image<double>(a.box, xr, yr, zr);
+            const double r2 = xr * xr + yr * yr + zr * zr;
+            double lam[6], R[6];
+            thole_lambda<double>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
+            pol_series<EW>(r2, a.aewald, lam, R);
+            Mp<double> mk;
+            mk.c = jm[0][jj]; mk.dx = jm[1][jj]; mk.dy = jm[2][jj]; mk.dz = jm[3][jj];
+            mk.qxx = jm[4][jj]; mk.qxy = jm[5][jj]; mk.qxz = jm[6][jj]; mk.qyy = jm[7][jj]; mk.qyz = jm[8][jj];
+            mk.qzz = jm[9][jj];
+            double Ei[3], Ek[3];
+            mp_fields(R, mi, mk, xr, yr, zr, Ei, Ek);
+            ex += Ei[0]; ey += Ei[1]; ez += Ei[2];
+            atomicAdd(&acc[0][jj], Ek[0]);
+            atomicAdd(&acc[1][jj], Ek[1]);
+            atomicAdd(&acc[2][jj], Ek[2]);
+        }
+        __syncthreads();
+        if (fi & 1)
+            for (int set = 0; set < 2; set++) {
+                double *d = a.dst + (size_t)set * 3 * np;
+                atomicAdd(&d[si], ex); atomicAdd(&d[np + si], ey); atomicAdd(&d[2 * np + si], ez);
+            }
+        if (jflag[lane] & 1)
+            for (int set = 0; set < 2; set++) {
+                double *d = a.dst + (size_t)set * 3 * np;
+                atomicAdd(&d[sj], acc[0][lane]); atomicAdd(&d[np + sj], acc[1][lane]);
+                atomicAdd(&d[2 * np + sj], acc[2][lane]);
+            }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_pol_field_excl(PolArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nexcl) return;
+    const ExclPair q = a.excl[t];
+    if (q.d == 1.0 && q.p == 1.0) return;
+    const int np = a.npad;
+    const int si = a.iperm[q.i], sk = a.iperm[q.k];
+    double xr = a.sx[sk] - a.sx[si], yr = a.sy[sk] - a.sy[si], zr = a.sz[sk] - a.sz[si];
+    image<double>(a.box, xr, yr, zr);
+    const double r2 = xr * xr + yr * yr + zr * zr;
+    if (r2 > a.off2) return;
+    double lam[6], R[6], Ei[3], Ek[3];
+    thole_lambda<double>(sqrt(r2), a.spol[2 * np + si], a.spol[2 * np + sk], a.spol[np + si], a.spol[np + sk], lam);
+    const Mp<double> mi = load_mp(a.smp, np, si), mk = load_mp(a.smp, np, sk);
+    const double sc[2] = {q.d, q.p};
+    for (int set = 0; set < 2; set++) {
+        if (sc[set] == 1.0) continue;
+        corr_series(r2, sc[set], lam, R);
+        mp_fields(R, mi, mk, xr, yr, zr, Ei, Ek);
+        double *d = a.dst + (size_t)set * 3 * np;
+        for (int c = 0; c < 3; c++) {
+            atomicAdd(&d[c * np + si], Ei[c]);
+            atomicAdd(&d[c * np + sk], Ek[c]);
+        }
+    }
+}
+
+// T mu for both dipole sets
+template <bool EW>
+__global__ __launch_bounds__(64) void k_pol_mutual_tiles(PolArgs a)
+{
+    TILE_PROLOGUE
+    __shared__ double jx[64], jy[64], jz[64], ju[6][64], jpd[64], jth[64], acc[6][64];
+    __shared__ int jflag[64];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 IJ = a.tiles[t];
+        const int si = IJ.x * 64 + lane, sj = IJ.y * 64 + lane;
+        __syncthreads();
+        jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
+        for (int c = 0; c < 6; c++) ju[c][lane] = a.src[c * np + sj];
+        jth[lane] = a.spol[np + sj];
+        jpd[lane] = a.spol[2 * np + sj];
+        jflag[lane] = a.sflag[sj];
+        for (int c = 0; c < 6; c++) acc[c][lane] = 0.0;
+        const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
+        const int fi = a.sflag[si];
+        double ui[6];
+        for (int c = 0; c < 6; c++) ui[c] = a.src[c * np + si];
+        const double thi = a.spol[np + si], pdi = a.spol[2 * np + si];
+        __syncthreads();
+        unsigned long long mask = 0ull;
+        if (fi & 1) {
+            for (int jj = IJ.x == IJ.y ? lane + 1 : 0; jj < 64; jj++) {
+                if (!(jflag[jj] & 1)) continue;
+                double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
+                image<double>(a.box, xr, yr, zr);
+                if (xr * xr + yr * yr + zr * zr <= a.off2) mask |= 1ull << jj;
+            }
+        }
+        double fi6[6] = {0, 0, 0, 0, 0, 0};
+        while (mask) {
+            const int jj = __ffsll((long long)mask) - 1;
+            mask &= mask - 1;
+            double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
+            image<double>(a.box, xr, yr, zr);
+            const double r2 = xr * xr + yr * yr + zr * zr;
+            double lam[6], R[6];
+            thole_lambda<double>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
+            pol_series<EW>(r2, a.aewald, lam, R);
+            for (int set = 0; set < 2; set++) {
+                const double *u = ui + 3 * set;
+                const double uk0 = ju[3 * set][jj], uk1 = ju[3 * set + 1][jj], uk2 = ju[3 * set + 2][jj];
+                const double ukr = uk0 * xr + uk1 * yr + uk2 * zr;
+                const double uir = u[0] * xr + u[1] * yr + u[2] * zr;
+                fi6[3 * set + 0] += ukr * R[2] * xr - R[1] * uk0;
+                fi6[3 * set + 1] += ukr * R[2] * yr - R[1] * uk1;
+                fi6[3 * set + 2] += ukr * R[2] * zr - R[1] * uk2;
+                atomicAdd(&acc[3 * set + 0][jj], uir * R[2] * xr - R[1] * u[0]);
+                atomicAdd(&acc[3 * set + 1][jj], uir * R[2] * yr - R[1] * u[1]);
+                atomicAdd(&acc[3 * set + 2][jj], uir * R[2] * zr - R[1] * u[2]);
+            }
+        }
+        __syncthreads();
+        if (fi & 1)
+            for (int c = 0; c < 6; c++) atomicAdd(&a.dst[c * np + si], fi6[c]);
+        if (jflag[lane] & 1)
+            for (int c = 0; c < 6; c++) atomicAdd(&a.dst[c * np + sj], acc[c][lane]);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_pol_mutual_excl(PolArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nexcl) return;
+    const ExclPair q = a.excl[t];
+    if (q.u == 1.0) return;
+    const int np = a.npad;
+    const int si = a.iperm[q.i], sk = a.iperm[q.k];
+    double xr = a.sx[sk] - a.sx[si], yr = a.sy[sk] - a.sy[si], zr = a.sz[sk] - a.sz[si];
+    image<double>(a.box, xr, yr, zr);
+    const double r2 = xr * xr + yr * yr + zr * zr;
+    if (r2 > a.off2) return;
+    double lam[6], R[6];
+    thole_lambda<double>(sqrt(r2), a.spol[2 * np + si], a.spol[2 * np + sk], a.spol[np + si], a.spol[np + sk], lam);
+    corr_series(r2, q.u, lam, R);
+    for (int set = 0; set < 2; set++) {
+        const double *u = a.src + (size_t)set * 3 * np;
+        double *d = a.dst + (size_t)set * 3 * np;
+        const double ui[3] = {u[si], u[np + si], u[2 * np + si]}, uk[3] = {u[sk], u[np + sk], u[2 * np + sk]};
+        const double r[3] = {xr, yr, zr};
+        const double uir = ui[0] * xr + ui[1] * yr + ui[2] * zr, ukr = uk[0] * xr + uk[1] * yr + uk[2] * zr;
+        for (int c = 0; c < 3; c++) {
+            atomicAdd(&d[c * np + si], ukr * R[2] * r[c] - R[1] * uk[c]);
+            atomicAdd(&d[c * np + sk], uir * R[2] * r[c] - R[1] * ui[c]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// real-space polarization gradient
+__device__ __forceinline__ void pol_pair_grad(const double *R, const Mp<double> &mi, const Mp<double> &mk,
+                                              const double *udi, const double *upi, const double *udk,
+                                              const double *upk, double xr, double yr, double zr, double sdp,
+                                              double sd, double sp, double su, double *F, double *Ti, double *Tk)
+{
+    // sdp weights the (mu_d + mu_p) permanent terms when d and p scales are equal
+    // (tiles); sd/sp separately otherwise; su the dipole-dipole term
+    PairOut<double> o;
+    if (sdp != 0.0) {
+        pair_series<double, true>(R, dip(udi[0] + upi[0], udi[1] + upi[1], udi[2] + upi[2]), mk, xr, yr, zr, o);
+        for (int c = 0; c < 3; c++) F[c] += sdp * (&o.fx)[c];
+        Tk[0] += sdp * o.tkx; Tk[1] += sdp * o.tky; Tk[2] += sdp * o.tkz;
+        pair_series<double, true>(R, mi, dip(udk[0] + upk[0], udk[1] + upk[1], udk[2] + upk[2]), xr, yr, zr, o);
+        for (int c = 0; c < 3; c++) F[c] += sdp * (&o.fx)[c];
+        Ti[0] += sdp * o.tix; Ti[1] += sdp * o.tiy; Ti[2] += sdp * o.tiz;
+    }
+    if (sp != 0.0) { // mu_d with the p-scaled permanent field
+        pair_series<double, true>(R, dip(udi[0], udi[1], udi[2]), mk, xr, yr, zr, o);
+        for (int c = 0; c < 3; c++) F[c] += sp * (&o.fx)[c];
+        Tk[0] += sp * o.tkx; Tk[1] += sp * o.tky; Tk[2] += sp * o.tkz;
+        pair_series<double, true>(R, mi, dip(udk[0], udk[1], udk[2]), xr, yr, zr, o);
+        for (int c = 0; c < 3; c++) F[c] += sp * (&o.fx)[c];
+        Ti[0] += sp * o.tix; Ti[1] += sp * o.tiy; Ti[2] += sp * o.tiz;
+    }
+    if (sd != 0.0) { // mu_p with the d-scaled permanent field
+        pair_series<double, true>(R, dip(upi[0], upi[1], upi[2]), mk, xr, yr, zr, o);
+        for (int c = 0; c < 3; c++) F[c] += sd * (&o.fx)[c];
+        Tk[0] += sd * o.tkx; Tk[1] += sd * o.tky; Tk[2] += sd * o.tkz;
+        pair_series<double, true>(R, mi, dip(upk[0], upk[1], upk[2]), xr, yr, zr, o);
+        for (int c = 0; c < 3; c++) F[c] += sd * (&o.fx)[c];
+        Ti[0] += sd * o.tix; Ti[1] += sd * o.tiy; Ti[2] += sd * o.tiz;
+    }
+    if (su != 0.0) {
+        pair_series<double, true>(R, dip(udi[0], udi[1], udi[2]), dip(upk[0], upk[1], upk[2]), xr, yr, zr, o);
+        for (int c = 0; c < 3; c++) F[c] += su * (&o.fx)[c];
+        pair_series<double, true>(R, dip(upi[0], upi[1], upi[2]), dip(udk[0], udk[1], udk[2]), xr, yr, zr, o);
+        for (int c = 0; c < 3; c++) F[c] += su * (&o.fx)[c];
+    }
+}
+
+template <bool EW>
+__global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
+{
+    TILE_PROLOGUE
+    __shared__ double jx[64], jy[64], jz[64], jm[10][64], ju[6][64], jpd[64], jth[64], acc[6][64];
+    __shared__ int jflag[64];
+    const double hf = 0.5 * a.f;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 IJ = a.tiles[t];
+        const int si = IJ.x * 64 + lane, sj = IJ.y * 64 + lane;
+        __syncthreads();
+        jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
+        for (int c = 0; c < 10; c++) jm[c][lane] = a.smp[c * np + sj];
+        for (int c = 0; c < 6; c++) ju[c][lane] = a.src[c * np + sj];
+        jth[lane] = a.spol[np + sj];
+        jpd[lane] = a.spol[2 * np + sj];
+        jflag[lane] = a.sflag[sj];
+        for (int c = 0; c < 6; c++) acc[c][lane] = 0.0;
+        const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
+        const int fi = a.sflag[si];
+        const Mp<double> mi = load_mp(a.smp, np, si);
+        double ui[6];
+        for (int c = 0; c < 6; c++) ui[c] = a.src[c * np + si];
+        const double thi = a.spol[np + si], pdi = a.spol[2 * np + si];
+        __syncthreads();
+        unsigned long long mask = 0ull;
+        if (fi & 1) {
+            for (int jj = IJ.x == IJ.y ? lane + 1 : 0; jj < 64; jj++) {
+                if (!(jflag[jj] & 1)) continue;
+                double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
+                image<double>(a.box, xr, yr, zr);
+                if (xr * xr + yr * yr + zr * zr <= a.off2) mask |= 1ull << jj;
+            }
+        }
+        double F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0};
+        while (mask) {
+            const int jj = __ffsll((long long)mask) - 1;
+            mask &= mask - 1;
+            double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
+            image<double>(a.box, xr, yr, zr);
+            const double r2 = xr * xr + yr * yr + zr * zr;
+            double lam[6], R[6];
+            thole_lambda<double>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
+            pol_series<EW>(r2, a.aewald, lam, R);
+            Mp<double> mk;
+            mk.c = jm[0][jj]; mk.dx = jm[1][jj]; mk.dy = jm[2][jj]; mk.dz = jm[3][jj];
+            mk.qxx = jm[4][jj]; mk.qxy = jm[5][jj]; mk.qxz = jm[6][jj]; mk.qyy = jm[7][jj]; mk.qyz = jm[8][jj];
+            mk.qzz = jm[9][jj];
+            const double udk[3] = {ju[0][jj], ju[1][jj], ju[2][jj]}, upk[3] = {ju[3][jj], ju[4][jj], ju[5][jj]};
+            double f3[3] = {0, 0, 0}, tk[3] = {0, 0, 0};
+            pol_pair_grad(R, mi, mk, ui, ui + 3, udk, upk, xr, yr, zr, 1.0, 0.0, 0.0, 1.0, f3, Ti, tk);
+            F[0] += f3[0]; F[1] += f3[1]; F[2] += f3[2];
+            atomicAdd(&acc[0][jj], -f3[0]);
+            atomicAdd(&acc[1][jj], -f3[1]);
+            atomicAdd(&acc[2][jj], -f3[2]);
+            atomicAdd(&acc[3][jj], tk[0]);
+            atomicAdd(&acc[4][jj], tk[1]);
+            atomicAdd(&acc[5][jj], tk[2]);
+        }
+        __syncthreads();
+        if (fi & 1) {
+            const int p = a.perm[si];
+            for (int c = 0; c < 3; c++) {
+                atomicAdd(&a.dep[3 * p + c], hf * F[c]);
+                atomicAdd(&a.tep[3 * p + c], hf * Ti[c]);
+            }
+        }
+        if (jflag[lane] & 1) {
+            const int p = a.perm[sj];
+            for (int c = 0; c < 3; c++) {
+                atomicAdd(&a.dep[3 * p + c], hf * acc[c][lane]);
+                atomicAdd(&a.tep[3 * p + c], hf * acc[3 + c][lane]);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_pol_grad_excl(PolArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nexcl) return;
+    const ExclPair q = a.excl[t];
+    if (q.d == 1.0 && q.p == 1.0 && q.u == 1.0) return;
+    const int np = a.npad;
+    const int si = a.iperm[q.i], sk = a.iperm[q.k];
+    double xr = a.sx[sk] - a.sx[si], yr = a.sy[sk] - a.sy[si], zr = a.sz[sk] - a.sz[si];
+    image<double>(a.box, xr, yr, zr);
+    const double r2 = xr * xr + yr * yr + zr * zr;
+    if (r2 > a.off2) return;
+    double lam[6], rr[6], R[6];
+    thole_lambda<double>(sqrt(r2), a.spol[2 * np + si], a.spol[2 * np + sk], a.spol[np + si], a.spol[np + sk], lam);
+    rr_series<double>(r2, 1.0, rr);
+    for (int c = 0; c < 6; c++) R[c] = lam[c] * rr[c];
+    const Mp<double> mi = load_mp(a.smp, np, si), mk = load_mp(a.smp, np, sk);
+    double udi[3], upi[3], udk[3], upk[3];
+    for (int c = 0; c < 3; c++) {
+        udi[c] = a.src[c * np + si]; upi[c] = a.src[(3 + c) * np + si];
+        udk[c] = a.src[c * np + sk]; upk[c] = a.src[(3 + c) * np + sk];
+    }
+    double F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0}, Tk[3] = {0, 0, 0};
+    pol_pair_grad(R, mi, mk, udi, upi, udk, upk, xr, yr, zr, 0.0, q.d - 1.0, q.p - 1.0, q.u - 1.0, F, Ti, Tk);
+    const double hf = 0.5 * a.f;
+    for (int c = 0; c < 3; c++) {
+        atomicAdd(&a.dep[3 * q.i + c], hf * F[c]);
+        atomicAdd(&a.dep[3 * q.k + c], -hf * F[c]);
+        atomicAdd(&a.tep[3 * q.i + c], hf * Ti[c]);
+        atomicAdd(&a.tep[3 * q.k + c], hf * Tk[c]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// sorted polarization parameters
+__global__ void k_pol_sort(int n, int npad, const int *perm, const double *pol, const double *thl, const double *pdm,
+                           double *spol)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= npad) return;
+    const int i = s < n ? perm[s] : -1;
+    spol[s] = i >= 0 ? pol[i] : 0.0;
+    spol[npad + s] = i >= 0 ? thl[i] : 0.0;
+    spol[2 * npad + s] = i >= 0 ? pdm[i] : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// preconditioned conjugate gradient on (1/alpha - T) mu = E, both sets at once.
+// Vectors are 2 x 3 x npad; index t covers both sets, set = t / (3 npad).
+// Scalars: pcg[0..1] r.z, [2..3] p.v, [4..5] new r.z, [6..7] r.r
+struct PcgArgs {
+    int npad;
+    const double *spol;
+    const double *E, *Tu; // fields
+    double *u, *r, *z, *p, *v;
+    double *sc;
+};
+
+__device__ __forceinline__ void block_add2(double a0, double a1, double *dst)
+{
+    __shared__ double red[2][256];
+    red[0][threadIdx.x] = a0;
+    red[1][threadIdx.x] = a1;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(dst, red[0][0]);
+        atomicAdd(dst + 1, red[1][0]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pcg_guess(PcgArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 6 * a.npad) return;
+    const int s = t % a.npad;
+    a.u[t] = a.spol[s] * a.E[t];
+}
+
+__global__ __launch_bounds__(256) void k_pcg_init(PcgArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double s0 = 0.0, s1 = 0.0;
+    if (t < 6 * a.npad) {
+        const int s = t % a.npad;
+        const double pol = a.spol[s];
+        const double r = pol != 0.0 ? a.E[t] + a.Tu[t] - a.u[t] / pol : 0.0;
+        const double z = pol * r;
+        a.r[t] = r;
+        a.z[t] = z;
+        a.p[t] = z;
+        (t < 3 * a.npad ? s0 : s1) = r * z;
+    }
+    block_add2(s0, s1, a.sc + 0);
+}
+
+// v = p/alpha - T p, and p.v
+__global__ __launch_bounds__(256) void k_pcg_a(PcgArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double s0 = 0.0, s1 = 0.0;
+    if (t < 6 * a.npad) {
+        const int s = t % a.npad;
+        const double pol = a.spol[s];
+        const double v = pol != 0.0 ? a.p[t] / pol - a.Tu[t] : 0.0;
+        a.v[t] = v;
+        (t < 3 * a.npad ? s0 : s1) = a.p[t] * v;
+    }
+    block_add2(s0, s1, a.sc + 2);
+}
+
+// u += alpha p, r -= alpha v, z = M r; new r.z and r.r
+__global__ __launch_bounds__(256) void k_pcg_b(PcgArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double s0 = 0.0, s1 = 0.0, q0 = 0.0, q1 = 0.0;
+    if (t < 6 * a.npad) {
+        const int set = t < 3 * a.npad ? 0 : 1;
+        const int s = t % a.npad;
+        const double pv = a.sc[2 + set];
+        const double al = pv != 0.0 ? a.sc[set] / pv : 0.0;
+        a.u[t] += al * a.p[t];
+        const double r = a.r[t] - al * a.v[t];
+        const double z = a.spol[s] * r;
+        a.r[t] = r;
+        a.z[t] = z;
+        (set == 0 ? s0 : s1) = r * z;
+        (set == 0 ? q0 : q1) = r * r;
+    }
+    block_add2(s0, s1, a.sc + 4);
+    __syncthreads();
+    block_add2(q0, q1, a.sc + 6);
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(256) void k_pcg_c(PcgArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 6 * a.npad) return;
+    const int set = t < 3 * a.npad ? 0 : 1;
+    const double beta = a.sc[set] != 0.0 ? a.sc[4 + set] / a.sc[set] : 0.0;
+    a.p[t] = a.z[t] + beta * a.p[t];
+}
+
+// roll the scalars for the next iteration
+__global__ void k_pcg_roll(double *sc)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        sc[0] = sc[4]; sc[1] = sc[5];
+        sc[2] = sc[3] = sc[4] = sc[5] = sc[6] = sc[7] = 0.0;
+    }
+}
+
+// E_pol = -(f/2) sum mu_d . E^p
+__global__ __launch_bounds__(256) void k_pol_energy(int npad, double f, const double *ud, const double *Ep, Totals *tot)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const double v = t < 3 * npad ? -0.5 * f * ud[t] * Ep[t] : 0.0;
+    block_add2(v, 0.0, &tot->ep);
+}
+
+// induced dipoles to the caller's order (AoS)
+__global__ void k_pol_scatter(int n, int npad, const int *perm, const double *u, double *ud_out, double *up_out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int i = perm[s];
+    for (int c = 0; c < 3; c++) {
+        ud_out[3 * i + c] = u[c * npad + s];
+        up_out[3 * i + c] = u[(3 + c) * npad + s];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+int pme_dipole_field(pmq_ctx *c, const double *dip, double *field);
+int pme_polar_grad(pmq_ctx *c);
+int launch_zero(pmq_ctx *c, double *p, size_t n);
+
+static PolArgs pol_args(pmq_ctx *c, double off2)
+{
+    PolArgs a;
+    a.n = c->n; a.npad = c->npad;
+    a.tiles = c->tiles.p; a.ntiles = c->ntiles_d.p;
+    a.sx = c->sx.p; a.sy = c->sy.p; a.sz = c->sz.p; a.smp = c->smp.p; a.spol = c->spol.p;
+    a.sflag = c->sflag.p; a.perm = c->perm.p; a.iperm = c->iperm.p;
+    a.box = c->box;
+    a.off2 = off2;
+    a.aewald = c->prm.use_ewald ? c->prm.aewald : 0.0;
+    a.f = c->prm.electric / c->prm.dielec;
+    a.excl = c->excl.p;
+    a.nexcl = (int)c->h_excl.size();
+    a.src = nullptr; a.dst = nullptr; a.dep = c->dep.p; a.tep = c->tep.p;
+    return a;
+}
+
+static int tile_grid(pmq_ctx *c) { return std::max(1, std::min(c->ntiles_cap, 16384)); }
+
+static int mutual_field(pmq_ctx *c, double off2, const double *src, double *dst)
+{
+    int rc;
+    const size_t nv = 6 * (size_t)c->npad;
+    if ((rc = launch_zero(c, dst, nv))) return rc;
+    PolArgs a = pol_args(c, off2);
+    a.src = src;
+    a.dst = dst;
+    if (a.aewald > 0)
+        hipLaunchKernelGGL(k_pol_mutual_tiles<true>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL(k_pol_mutual_tiles<false>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+    if (a.nexcl) hipLaunchKernelGGL(k_pol_mutual_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
+    PMQ_HIP(hipGetLastError());
+    if (a.aewald > 0 && (rc = pme_dipole_field(c, src, dst))) return rc;
+    return PMQ_OK;
+}
+
+int polar_prepare(pmq_ctx *c, double off2)
+{
+    // sorted parameters, zeroed permanent field, real-space permanent field
+    const int np = c->npad;
+    hipLaunchKernelGGL(k_pol_sort, dim3((np + 255) / 256), dim3(256), 0, c->stream, c->n, np, c->perm.p,
+                       c->polarity.p, c->thole.p, c->pdamp.p, c->spol.p);
+    int rc;
+    if ((rc = launch_zero(c, c->pfield.p, 6 * (size_t)np))) return rc;
+    PolArgs a = pol_args(c, off2);
+    a.dst = c->pfield.p;
+    if (a.aewald > 0)
+        hipLaunchKernelGGL(k_pol_field_tiles<true>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL(k_pol_field_tiles<false>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+    if (a.nexcl) hipLaunchKernelGGL(k_pol_field_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+int polar_solve(pmq_ctx *c, double off2, int *iters_out)
+{
+    const int np = c->npad;
+    const size_t nv = 6 * (size_t)np;
+    PcgArgs a;
+    a.npad = np; a.spol = c->spol.p;
+    a.E = c->pfield.p; a.Tu = c->pvec.p;
+    a.u = c->uind.p; a.r = c->pres.p; a.z = c->pz.p; a.p = c->pdir.p; a.v = c->pvec.p;
+    a.sc = c->pcg.p;
+    const int tb = 256, gb = (int)((nv + tb - 1) / tb);
+    int rc;
+    if ((rc = launch_zero(c, c->pcg.p, 8))) return rc;
+    hipLaunchKernelGGL(k_pcg_guess, dim3(gb), dim3(tb), 0, c->stream, a);
+    if ((rc = mutual_field(c, off2, c->uind.p, c->pvec.p))) return rc;
+    hipLaunchKernelGGL(k_pcg_init, dim3(gb), dim3(tb), 0, c->stream, a);
+    PMQ_HIP(hipGetLastError());
+    // count polarizable sites once (host copy of the parameters)
+    int npolar = 0;
+    for (double p : c->h_polarity) npolar += p != 0.0;
+    npolar = std::max(npolar, 1);
+    const double debye = 4.80321;
+    int it = 0;
+    bool done = false;
+    while (!done && it < c->prm.politer) {
+        it++;
+        if ((rc = mutual_field(c, off2, c->pdir.p, c->pvec.p))) return rc;
+        hipLaunchKernelGGL(k_pcg_a, dim3(gb), dim3(tb), 0, c->stream, a);
+        hipLaunchKernelGGL(k_pcg_b, dim3(gb), dim3(tb), 0, c->stream, a);
+        PMQ_HIP(hipGetLastError());
+        double sc[8];
+        PMQ_HIP(hipMemcpyAsync(sc, c->pcg.p, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+        PMQ_HIP(hipStreamSynchronize(c->stream));
+        const double eps = debye * sqrt(std::max(sc[6], sc[7]) / npolar);
+        if (eps < c->prm.poleps) {
+            done = true;
+            break;
+        }
+        hipLaunchKernelGGL(k_pcg_c, dim3(gb), dim3(tb), 0, c->stream, a);
+        hipLaunchKernelGGL(k_pcg_roll, dim3(1), dim3(1), 0, c->stream, c->pcg.p);
+        PMQ_HIP(hipGetLastError());
+    }
+    *iters_out = done ? it : -it;
+    return PMQ_OK;
+}
+
+int polar_energy(pmq_ctx *c)
+{
+    const int np = c->npad;
+    hipLaunchKernelGGL(k_pol_energy, dim3((3 * np + 255) / 256), dim3(256), 0, c->stream, np,
+                       c->prm.electric / c->prm.dielec, c->uind.p, c->pfield.p + 3 * (size_t)np, c->totals.p);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+int polar_gradient(pmq_ctx *c, double off2)
+{
+    PolArgs a = pol_args(c, off2);
+    a.src = c->uind.p;
+    if (a.aewald > 0)
+        hipLaunchKernelGGL(k_pol_grad_tiles<true>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL(k_pol_grad_tiles<false>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+    if (a.nexcl) hipLaunchKernelGGL(k_pol_grad_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
+    PMQ_HIP(hipGetLastError());
+    if (a.aewald > 0) return pme_polar_grad(c);
+    return PMQ_OK;
+}
+
+int polar_scatter(pmq_ctx *c, double *ud_out, double *up_out)
+{
+    hipLaunchKernelGGL(k_pol_scatter, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n, c->npad, c->perm.p,
+                       c->uind.p, ud_out, up_out);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+} // namespace pmq

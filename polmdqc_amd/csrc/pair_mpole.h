@@ -240,27 +240,36 @@ __device__ __forceinline__ void damp_gordon(R r, R ai, R ak, R *dmpi, R *dmpk, R
 // Charge-penetration pair (pairMpole.h:186-389): a.c/b.c hold the valence
 // charges; the interaction is core-core (undamped) + core_k with the
 // valence multipoles of i (dmpi) + core_i with those of k (dmpk) +
-// valence-valence (dmpik).
-template <class R, bool GRAD, int PEN>
+// valence-valence (dmpik).  Each class X uses the series sf*dmp_X*rr, or
+// with Ewald (EW) f*(B - (1 - dmp_X) rr): the real-space part of a damped
+// interaction whose reciprocal part is the undamped PME sum.
+template <class R, bool GRAD, int PEN, bool EW = false>
 __device__ __forceinline__ void pair_cp(R r2, R xr, R yr, R zr, R scale_f, R corei, R alphai, const Mp<R> &a,
-                                        R corek, R alphak, const Mp<R> &b, PairOut<R> &o)
+                                        R corek, R alphak, const Mp<R> &b, PairOut<R> &o, const R *bn = nullptr)
 {
     R rr[6], dmpi[5], dmpk[5], dmpik[6], Rs[6];
-    rr_series<R>(r2, scale_f, rr);
+    rr_series<R>(r2, R(1), rr);
     damp_gordon<R, PEN>(sqrt(r2), alphai, alphak, dmpi, dmpk, dmpik);
+    auto series = [&](const R *dmp, int nd) {
+        for (int j = 0; j < 6; j++) {
+            const R d = j < nd ? dmp[j] : R(0);
+            Rs[j] = EW ? scale_f * (bn[j] - (R(1) - d) * rr[j]) : scale_f * d * rr[j];
+        }
+    };
     Mp<R> ci{}, ck{};
     ci.c = corei;
     ck.c = corek;
-    pair_series<R, GRAD, false>(rr, ci, ck, xr, yr, zr, o);
+    const R one[6] = {R(1), R(1), R(1), R(1), R(1), R(1)};
+    series(one, 6);
+    pair_series<R, GRAD, false>(Rs, ci, ck, xr, yr, zr, o);
     // core-core has no potential on the valence charges
     o.phii = R(0);
     o.phik = R(0);
-    for (int j = 0; j < 6; j++) Rs[j] = dmpik[j] * rr[j];
+    series(dmpik, 6);
     pair_series<R, GRAD, true>(Rs, a, b, xr, yr, zr, o);
     {
         PairOut<R> t;
-        for (int j = 0; j < 5; j++) Rs[j] = dmpi[j] * rr[j];
-        Rs[5] = R(0);
+        series(dmpi, 5);
         pair_series<R, GRAD, false>(Rs, a, ck, xr, yr, zr, t);
         o.e += t.e;
         o.phii += t.phii;
@@ -268,7 +277,7 @@ __device__ __forceinline__ void pair_cp(R r2, R xr, R yr, R zr, R scale_f, R cor
             o.fx += t.fx; o.fy += t.fy; o.fz += t.fz;
             o.tix += t.tix; o.tiy += t.tiy; o.tiz += t.tiz;
         }
-        for (int j = 0; j < 5; j++) Rs[j] = dmpk[j] * rr[j];
+        series(dmpk, 5);
         pair_series<R, GRAD, false>(Rs, ci, b, xr, yr, zr, t);
         o.e += t.e;
         o.phik += t.phik;
@@ -277,6 +286,51 @@ __device__ __forceinline__ void pair_cp(R r2, R xr, R yr, R zr, R scale_f, R cor
             o.tkx += t.tkx; o.tky += t.tky; o.tkz += t.tkz;
         }
     }
+}
+
+} // namespace pmq
+
+namespace pmq {
+
+// Ewald-screened series B_0..B_5 (B_0 = erfc(a r)/r and
+// B_n = ((2n-1) B_{n-1} + (2a^2)^n e^{-a^2 r^2} / (a sqrt(pi))) / r^2):
+// the real-space part of each rr_n
+template <class R>
+__device__ __forceinline__ void ewald_series(R r2, R aewald, R *bn)
+{
+    const R r = sqrt(r2);
+    const R ra = aewald * r;
+    const R r2i = R(1) / r2;
+    bn[0] = erfc(ra) / r;
+    const R ex = exp(-ra * ra);
+    const R a2 = R(2) * aewald * aewald;
+    R an = R(1) / (R(1.77245385090551602730) * aewald);
+#pragma unroll
+    for (int j = 1; j < 6; j++) {
+        an *= a2;
+        bn[j] = (R(2 * j - 1) * bn[j - 1] + an * ex) * r2i;
+    }
+}
+
+// Thole damping factors of the polarization interactions (lambda_3..9 in
+// lam[1..4]); D = a (r / (pd_i pd_k))^3, the factor set that keeps
+// d(lambda_n rr_n)/dr = -r lambda_{n+2} rr_{n+2}
+template <class R>
+__device__ __forceinline__ void thole_lambda(R r, R pdi, R pdk, R ti, R tk, R *lam)
+{
+    lam[0] = lam[1] = lam[2] = lam[3] = lam[4] = lam[5] = R(1);
+    const R damp = pdi * pdk;
+    if (damp == R(0)) return;
+    const R mn = fmin(ti, tk);
+    const R pg = mn != R(0) ? mn : fmax(ti, tk);
+    const R u = r / damp;
+    const R D = pg * u * u * u;
+    if (D >= R(50)) return;
+    const R e = exp(-D);
+    lam[1] = R(1) - e;
+    lam[2] = R(1) - (R(1) + D) * e;
+    lam[3] = R(1) - (R(1) + D + R(0.6) * D * D) * e;
+    lam[4] = R(1) - (R(1) + D + R(18.0 / 35.0) * D * D + R(9.0 / 35.0) * D * D * D) * e;
 }
 
 } // namespace pmq

@@ -1,0 +1,51 @@
+// pme_common.h -- cardinal B-spline weights of one atom along one axis.
+//
+// For fractional offset w in [0,1) the order-n stencil has n nonzero
+// weights theta_k = M_n(w + n - 1 - k), k = 0..n-1.  They are built by the
+// recursion M_{m+1}(x) = [x M_m(x) + (m+1-x) M_m(x-1)] / m, and the d-th
+// derivative is the d-th backward difference of the order n-d weights
+// (M_n' (x) = M_{n-1}(x) - M_{n-1}(x-1)).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#define PMQ_MAX_ORDER 6
+
+namespace pmq {
+
+template <class R, int ND>
+__device__ __forceinline__ void bspline_weights(R w, int n, R th[PMQ_MAX_ORDER][4])
+{
+    // b_m[k] = M_m(w + m - 1 - k), kept for orders n-3 .. n
+    R b[PMQ_MAX_ORDER + 1];
+    R keep[4][PMQ_MAX_ORDER + 3];
+    for (int j = 0; j < PMQ_MAX_ORDER + 1; j++) b[j] = R(0);
+    for (int q = 0; q < 4; q++)
+        for (int j = 0; j < PMQ_MAX_ORDER + 3; j++) keep[q][j] = R(0);
+    b[0] = R(1);
+    for (int m = 1; m <= n; m++) {
+        if (m >= n - ND) {
+            // keep[n - m][k + (n - m)] = b_m[k] so differences index without bounds checks
+            const int q = n - m;
+            for (int k = 0; k < m; k++) keep[q][k + q] = b[k];
+        }
+        if (m == n) break;
+        R nb[PMQ_MAX_ORDER + 1];
+        const R inv = R(1) / R(m);
+        for (int k = 0; k <= m; k++) {
+            const R left = k > 0 ? b[k - 1] : R(0);
+            const R right = k < m ? b[k] : R(0);
+            nb[k] = ((w + R(m - k)) * left + (R(k + 1) - w) * right) * inv;
+        }
+        for (int k = 0; k <= m; k++) b[k] = nb[k];
+    }
+    for (int k = 0; k < n; k++) {
+        // keep[q][k + q - j] = b_{n-q}[k - j]; out of range entries are zero
+        th[k][0] = keep[0][k];
+        if (ND >= 1) th[k][1] = keep[1][k + 1 - 1] - keep[1][k + 1];
+        if (ND >= 2) th[k][2] = keep[2][k] - R(2) * keep[2][k + 1] + keep[2][k + 2];
+        if (ND >= 3) th[k][3] = keep[3][k] - R(3) * keep[3][k + 1] + R(3) * keep[3][k + 2] - keep[3][k + 3];
+    }
+}
+
+} // namespace pmq

@@ -11,8 +11,6 @@
 
 namespace pmq {
 size_t neighbor_scratch_bytes(int n, int nblk);
-int polar_setup(pmq_ctx *c);
-int run_polar(pmq_ctx *c, int mode, int precision);
 } // namespace pmq
 
 using namespace pmq;
@@ -62,6 +60,7 @@ int pmq_destroy(pmq_ctx *c)
     if (!c) return PMQ_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    pme_release(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PMQ_OK;
@@ -136,18 +135,25 @@ int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
             c->h_usef[i] = (t->use[i + 1] || t->use[iz] || t->use[ix] || t->use[iy]) ? 1 : 0;
         }
     }
-    // scaled pairs i < k; a later connectivity class overrides an earlier one
-    // (empole.cpp:224-235 assigns 1-2, 1-3, 1-4, 1-5 in that order)
+    // scaled pairs i < k.  A later connectivity class overrides an earlier
+    // one (empole.cpp:224-235 assigns 1-2, 1-3, 1-4, 1-5 in that order); the
+    // polarization-group classes 1-1..1-4 are kept beside them
     c->h_excl.clear();
     {
         const int *cnt[4] = {t->n12, t->n13, t->n14, t->n15};
         const int *lst[4] = {t->i12, t->i13, t->i14, t->i15};
-        std::vector<long long> off[4];
+        const int *gcnt[4] = {t->np11, t->np12, t->np13, t->np14};
+        const int *glst[4] = {t->ip11, t->ip12, t->ip13, t->ip14};
+        std::vector<long long> off[4], goff[4];
         for (int q = 0; q < 4; q++) {
             off[q].assign(n + 1, 0);
-            for (int i = 0; i < n; i++) off[q][i + 1] = off[q][i] + cnt[q][i];
+            goff[q].assign(n + 1, 0);
+            for (int i = 0; i < n; i++) {
+                off[q][i + 1] = off[q][i] + cnt[q][i];
+                goff[q][i + 1] = goff[q][i] + ((gcnt[q] && glst[q]) ? gcnt[q][i] : 0);
+            }
         }
-        std::map<int, int> cls;
+        std::map<int, std::pair<int, int>> cls; // k -> (bonded class, group class)
         for (int i = 0; i < n; i++) {
             cls.clear();
             for (int q = 0; q < 4; q++)
@@ -157,10 +163,31 @@ int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
                         c->err = "pmq_set_topology: connectivity atom out of range";
                         return PMQ_ERR_ARG;
                     }
-                    cls[k] = q + 2;
+                    cls[k].first = q + 2;
+                }
+            std::vector<int> same;
+            for (int q = 0; q < 4; q++)
+                for (long long j = goff[q][i]; j < goff[q][i + 1]; j++) {
+                    const int k = glst[q][j];
+                    if (k < 0 || k >= n) {
+                        c->err = "pmq_set_topology: polarization group atom out of range";
+                        return PMQ_ERR_ARG;
+                    }
+                    if (k == i) continue;
+                    cls[k].second = q + 1;
+                    if (q == 0) same.push_back(k);
                 }
             for (auto &kv : cls)
-                if (kv.first > i) c->h_excl.push_back({i, kv.first, kv.second, 0});
+                if (kv.first > i) {
+                    ExclPair e{};
+                    e.i = i;
+                    e.k = kv.first;
+                    e.mcls = kv.second.first;
+                    e.gcls = kv.second.second;
+                    e.intra = std::find(same.begin(), same.end(), kv.first) != same.end() ? 1 : 0;
+                    e.m = e.p = e.d = e.u = 1.0;
+                    c->h_excl.push_back(e);
+                }
         }
     }
     int rc;
@@ -170,7 +197,6 @@ int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
     if ((rc = upload(c, c->yaxis, t->yaxis, n))) return rc;
     if ((rc = upload(c, c->molcule, c->h_molcule.data(), n))) return rc;
     if ((rc = upload(c, c->usef, c->h_usef.data(), n))) return rc;
-    if ((rc = upload(c, c->excl, c->h_excl.data(), c->h_excl.size()))) return rc;
     c->nbond = t->nbond;
     c->nangle = t->nangle;
     if ((rc = upload(c, c->ibnd, t->ibnd, 2 * (size_t)t->nbond))) return rc;
@@ -250,6 +276,56 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
         if ((rc = upload(c, c->aflx, p->aflx, 2 * (size_t)c->nangle))) return rc;
         if ((rc = upload(c, c->abflx, p->abflx, 2 * (size_t)c->nangle))) return rc;
     }
+    // scale values of the scaled pairs (empole.cpp:224-235 for m; p, d and u
+    // as polpot.h:47-66 names them: p by connectivity with the "intra" value
+    // inside one polarization group, d and u by group connectivity)
+    for (auto &e : c->h_excl) {
+        const double mv[6] = {1, 1, p->m2scale, p->m3scale, p->m4scale, p->m5scale};
+        const double pv[6] = {1, 1, p->p2scale, p->p3scale, p->p4scale, p->p5scale};
+        const double piv[6] = {1, 1, p->p2iscale, p->p3iscale, p->p4iscale, p->p5iscale};
+        const double dv[5] = {1, p->d1scale, p->d2scale, p->d3scale, p->d4scale};
+        const double uv[5] = {1, p->u1scale, p->u2scale, p->u3scale, p->u4scale};
+        e.m = mv[e.mcls];
+        e.p = e.intra ? piv[e.mcls] : pv[e.mcls];
+        e.d = dv[e.gcls];
+        e.u = uv[e.gcls];
+    }
+    if ((rc = upload(c, c->excl, c->h_excl.data(), c->h_excl.size()))) return rc;
+    const bool polar = p->polarity != nullptr;
+    if (polar) {
+        if (!p->thole || !p->pdamp) {
+            c->err = "pmq_set_params: polarization needs polarity, thole and pdamp";
+            return PMQ_ERR_ARG;
+        }
+        if ((rc = upload(c, c->polarity, p->polarity, n))) return rc;
+        if ((rc = upload(c, c->thole, p->thole, n))) return rc;
+        if ((rc = upload(c, c->pdamp, p->pdamp, n))) return rc;
+        c->h_polarity.assign(p->polarity, p->polarity + n);
+        for (double v : c->h_polarity)
+            if (!(v >= 0.0)) {
+                c->err = "pmq_set_params: negative polarizability";
+                return PMQ_ERR_ARG;
+            }
+        const size_t nv = 6 * (size_t)c->npad;
+        PMQ_HIP(c->spol.alloc(3 * (size_t)c->npad));
+        PMQ_HIP(c->cphi.alloc(10 * (size_t)c->npad));
+        PMQ_HIP(c->pfield.alloc(nv)); PMQ_HIP(c->uind.alloc(nv));
+        PMQ_HIP(c->pres.alloc(nv)); PMQ_HIP(c->pz.alloc(nv));
+        PMQ_HIP(c->pdir.alloc(nv)); PMQ_HIP(c->pvec.alloc(nv));
+        PMQ_HIP(c->pcg.alloc(8));
+        PMQ_HIP(c->tep.alloc(3 * (size_t)n)); PMQ_HIP(c->dep.alloc(3 * (size_t)n));
+        PMQ_HIP(c->uind_out.alloc(6 * (size_t)n));
+    } else {
+        c->h_polarity.clear();
+    }
+    if (p->use_ewald && !(p->aewald > 0.0 && p->ewaldcut > 0.0)) {
+        c->err = "pmq_set_params: Ewald needs aewald > 0 and ewaldcut > 0";
+        return PMQ_ERR_ARG;
+    }
+    if (p->use_ewald && p->use_chgflx) {
+        c->err = "pmq_set_params: charge flux with Ewald summation is not built";
+        return PMQ_ERR_UNSUPPORTED;
+    }
     c->prm = *p;
     c->prm.pole = nullptr;
     c->prm.pcore = c->prm.pval0 = c->prm.palpha = nullptr;
@@ -305,12 +381,26 @@ int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
         c->err = "pmq_compute: only the plain FP64 form is built";
         return PMQ_ERR_UNSUPPORTED;
     }
-    if (terms & ~(unsigned)PMQ_TERM_MPOLE) {
-        c->err = "pmq_compute: only the multipole term is built";
+    if (terms & ~(unsigned)(PMQ_TERM_MPOLE | PMQ_TERM_POLAR)) {
+        c->err = "pmq_compute: repulsion and dispersion are not built";
         return PMQ_ERR_UNSUPPORTED;
     }
-    if (c->prm.use_ewald) {
-        c->err = "pmq_compute: Ewald summation is not built";
+    const bool do_mpole = terms & PMQ_TERM_MPOLE, do_polar = terms & PMQ_TERM_POLAR;
+    const bool ewald = c->prm.use_ewald;
+    if (do_polar && c->h_polarity.empty()) {
+        c->err = "pmq_compute: polarization requested but no polarizabilities were set";
+        return PMQ_ERR_STATE;
+    }
+    if (do_polar && c->prm.pentype != PMQ_PEN_NONE) {
+        c->err = "pmq_compute: charge-penetration (HIPPO) polarization is not built";
+        return PMQ_ERR_UNSUPPORTED;
+    }
+    if (ewald && (mode == PMQ_VIRIAL || mode == PMQ_ANALYSIS)) {
+        c->err = "pmq_compute: the Ewald path computes energy and gradient (no virial or analysis)";
+        return PMQ_ERR_UNSUPPORTED;
+    }
+    if (do_polar && (mode == PMQ_VIRIAL || mode == PMQ_ANALYSIS)) {
+        c->err = "pmq_compute: polarization computes energy and gradient (no virial or analysis)";
         return PMQ_ERR_UNSUPPORTED;
     }
     PMQ_HIP(hipSetDevice(c->device));
@@ -322,17 +412,38 @@ int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
         if ((rc = launch_zero(c, c->dem.p, 3 * (size_t)n))) return rc;
         if ((rc = launch_zero(c, c->te.p, 3 * (size_t)n))) return rc;
         if ((rc = launch_zero(c, c->pot.p, n))) return rc;
+        if (do_polar) {
+            if ((rc = launch_zero(c, c->dep.p, 3 * (size_t)n))) return rc;
+            if ((rc = launch_zero(c, c->tep.p, 3 * (size_t)n))) return rc;
+        }
     }
     if (mode == PMQ_ANALYSIS && (rc = launch_zero(c, c->aem.p, n))) return rc;
-    if (terms & PMQ_TERM_MPOLE) {
-        const double cut = c->box.kind == PMQ_BOX_NONE ? std::min(c->prm.mpolecut, 1e12) : c->prm.mpolecut;
+    c->pcg_iters = 0;
+    if (do_mpole || do_polar) {
+        double cut = ewald ? c->prm.ewaldcut : c->prm.mpolecut;
+        if (c->box.kind == PMQ_BOX_NONE) cut = std::min(cut, 1e12);
+        const double off2 = cut * cut;
         const bool cf = c->prm.pentype != PMQ_PEN_NONE && c->prm.use_chgflx;
         if (c->prm.use_chgflx && (rc = launch_chgflx_delta(c))) return rc;
         if ((rc = build_neighbors(c, cut))) return rc;
         if ((rc = launch_rotpole(c, true))) return rc;
-        if ((rc = launch_mpole_direct(c, mode, cut * cut))) return rc;
-        if (grad) {
-            if ((rc = launch_torque(c, mode))) return rc;
+        if (do_polar && (rc = polar_prepare(c, off2))) return rc;
+        if (do_mpole && (rc = launch_mpole_direct(c, mode, off2))) return rc;
+        if (ewald) {
+            if ((rc = pme_setup(c))) return rc;
+            if ((rc = pme_mpole(c, mode, do_mpole, do_polar))) return rc;
+        }
+        if (do_polar) {
+            if ((rc = polar_solve(c, off2, &c->pcg_iters))) return rc;
+            if ((rc = polar_energy(c))) return rc;
+            if (grad) {
+                if ((rc = polar_gradient(c, off2))) return rc;
+                if ((rc = launch_torque(c, PMQ_GRADIENT, c->tep.p, c->dep.p))) return rc;
+            }
+            if ((rc = polar_scatter(c, c->uind_out.p, c->uind_out.p + 3 * (size_t)n))) return rc;
+        }
+        if (grad && do_mpole) {
+            if ((rc = launch_torque(c, mode, c->te.p, c->dem.p))) return rc;
             if (cf && (rc = launch_chgflx_grad(c, mode))) return rc;
         }
     }
@@ -367,15 +478,39 @@ int pmq_fetch(pmq_ctx *c, pmq_result *r)
     r->einter = t.einter;
     r->nem = (long long)t.nem;
     for (int k = 0; k < 9; k++) r->vir[k] = c->last_mode == PMQ_VIRIAL ? t.vir[k] : 0.0;
-    r->pcg_iters = t.pcg_iters;
-    if (r->dem && grad) PMQ_HIP(hipMemcpy(r->dem, c->dem.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
-    if (r->desum && grad) PMQ_HIP(hipMemcpy(r->desum, c->dem.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    r->pcg_iters = c->pcg_iters;
+    const bool mp = c->last_terms & PMQ_TERM_MPOLE, pol = c->last_terms & PMQ_TERM_POLAR;
+    std::vector<double> gm, gp;
+    if (grad && mp) {
+        gm.resize(3 * (size_t)n);
+        PMQ_HIP(hipMemcpy(gm.data(), c->dem.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (grad && pol) {
+        gp.resize(3 * (size_t)n);
+        PMQ_HIP(hipMemcpy(gp.data(), c->dep.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (r->dem && grad) {
+        for (size_t j = 0; j < 3 * (size_t)n; j++) r->dem[j] = mp ? gm[j] : 0.0;
+    }
+    if (r->dep && grad) {
+        for (size_t j = 0; j < 3 * (size_t)n; j++) r->dep[j] = pol ? gp[j] : 0.0;
+    }
+    if (r->desum && grad) {
+        for (size_t j = 0; j < 3 * (size_t)n; j++) r->desum[j] = (mp ? gm[j] : 0.0) + (pol ? gp[j] : 0.0);
+    }
+    if (pol && r->uind) PMQ_HIP(hipMemcpy(r->uind, c->uind_out.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    if (pol && r->uinp)
+        PMQ_HIP(hipMemcpy(r->uinp, c->uind_out.p + 3 * (size_t)n, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
     if (r->aem && c->last_mode == PMQ_ANALYSIS)
         PMQ_HIP(hipMemcpy(r->aem, c->aem.p, n * sizeof(double), hipMemcpyDeviceToHost));
     if (r->rpole) PMQ_HIP(hipMemcpy(r->rpole, c->rpole13.p, 13 * n * sizeof(double), hipMemcpyDeviceToHost));
     if (!std::isfinite(r->esum)) {
         c->err = "pmq_fetch: non-finite total energy";
         return PMQ_ERR_NAN;
+    }
+    if (pol && c->pcg_iters < 0) {
+        c->err = "pmq_fetch: induced dipoles did not converge within politer iterations";
+        return PMQ_ERR_NOCONV;
     }
     return PMQ_OK;
 }

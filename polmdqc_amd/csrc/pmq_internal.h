@@ -129,11 +129,16 @@ struct Totals {
     int pad;
 };
 
-// scale classes of an excluded pair (connectivity 1-2 .. 1-5)
+// a pair whose interactions are scaled (connectivity 1-2..1-5 and
+// polarization groups 1-1..1-4): the class codes are fixed by the topology,
+// the scale values come with the parameters
 struct ExclPair {
-    int i, k;   // original atom indices, i < k
-    int cls;    // 2..5
+    int i, k;       // original atom indices, i < k
+    int mcls;       // 2..5 bonded class for m/p scaling, 0 = none
+    int gcls;       // 1..4 polarization-group class for d/u scaling, 0 = none
+    int intra;      // 1 if k is in i's own polarization group (pNiscale)
     int pad;
+    double m, p, d, u; // scale values (filled by set_params)
 };
 
 } // namespace pmq
@@ -191,6 +196,37 @@ struct pmq_ctx {
     pmq::DBuf<double> dem, te, pot, aem;
     pmq::DBuf<pmq::Totals> totals;
 
+    // ---- polarization and PME state (pmq_polar.cpp)
+    pmq::DBuf<double> polarity, thole, pdamp;    // caller order
+    std::vector<double> h_polarity;
+    pmq::DBuf<double> spol;                      // 3 * npad sorted: polarity, thole, pdamp
+    pmq::DBuf<double> cphi;                      // 10 * npad sorted: phi, grad(3), hessian(6) of the
+                                                 // permanent multipoles' reciprocal potential
+    // two-set vectors (set 0 = d, set 1 = p), each 2 x 3 x npad sorted SoA
+    pmq::DBuf<double> pfield;                    // permanent fields E^d, E^p
+    pmq::DBuf<double> uind;                      // induced dipoles mu_d, mu_p
+    pmq::DBuf<double> pres, pz, pdir, pvec;      // PCG residual, preconditioned residual,
+                                                 // search direction, (1/alpha - T) p
+    pmq::DBuf<double> pcg;                       // PCG scalars
+    pmq::DBuf<double> tep;                       // 3n torques of the polarization term
+    pmq::DBuf<double> dep;                       // 3n gradient of ep
+    bool have_uind = false;
+    // PME
+    int K[3] = {0, 0, 0};
+    int bsorder = 5;
+    double pmeA[3][3] = {{0}};                   // u = A r
+    pmq::DBuf<double> pme_g;                     // (K0/2+1) K1 K2 reciprocal kernel
+    pmq::DBuf<double> qgrid;                     // 2 real grids
+    pmq::DBuf<double> cgrid;                     // 2 complex half grids (interleaved re, im)
+    pmq::DBuf<unsigned char> fft_work;
+    void *plan_f1 = nullptr, *plan_b1 = nullptr, *plan_f2 = nullptr, *plan_b2 = nullptr;
+    void *fft_info = nullptr;
+    pmq::DevBox pme_dbox{};                       // box the kernel G was built for
+    double pme_alpha = 0.0;
+
+    int pcg_iters = 0;
+    pmq::DBuf<double> uind_out;                  // 6n caller-order induced dipoles (d then p)
+
     int last_mode = PMQ_ENERGY;
     unsigned last_terms = 0;
     int last_precision = PMQ_PLAIN_FP64;
@@ -203,7 +239,15 @@ int build_neighbors(pmq_ctx *c, double cutoff);
 int launch_rotpole(pmq_ctx *c, bool want_rpole13);
 int launch_chgflx_delta(pmq_ctx *c);
 int launch_mpole_direct(pmq_ctx *c, int mode, double off2);
-int launch_torque(pmq_ctx *c, int mode);
+int launch_torque(pmq_ctx *c, int mode, const double *te, double *de);
+int pme_setup(pmq_ctx *c);
+void pme_release(pmq_ctx *c);
+int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar);
+int polar_prepare(pmq_ctx *c, double off2);
+int polar_solve(pmq_ctx *c, double off2, int *iters_out);
+int polar_energy(pmq_ctx *c);
+int polar_gradient(pmq_ctx *c, double off2);
+int polar_scatter(pmq_ctx *c, double *ud_out, double *up_out);
 int launch_chgflx_grad(pmq_ctx *c, int mode);
 int launch_zero(pmq_ctx *c, double *p, size_t n);
 } // namespace pmq
