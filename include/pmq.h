@@ -176,6 +176,16 @@ PMQ_API int pmq_compute(pmq_ctx *ctx, int mode, unsigned terms, int precision);
 PMQ_API int pmq_fetch(pmq_ctx *ctx, pmq_result *res);
 PMQ_API int pmq_sync(pmq_ctx *ctx);
 
+/* Kernel timing for measurement.  No counterpart in the reference (it has no
+ * timers on this path); for benchmark and profiling harnesses.  While
+ * enabled, the context brackets every launch of each instrumented kernel
+ * family with HIP events on its own stream.  pmq_profile_read waits for the
+ * stream, returns the summed elapsed time and launch count of one family
+ * ("mpole_tiles", "pol_field_tiles", "pol_mutual_tiles", "pol_grad_tiles",
+ * "pme_spread", "pme_fft", "pme_gather", "neighbors", "pcg") and resets it. */
+PMQ_API int pmq_profile_enable(pmq_ctx *ctx, int on);
+PMQ_API int pmq_profile_read(pmq_ctx *ctx, const char *name, double *total_ms, long long *launches);
+
 #ifdef __cplusplus
 }
 #endif

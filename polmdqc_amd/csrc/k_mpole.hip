@@ -465,10 +465,16 @@ static int launch_mpole_t(pmq_ctx *c, const MpoleArgs &a)
 {
     const int grid = std::max(1, std::min(c->ntiles_cap, 16384));
     if (a.aewald > 0.0)
-        hipLaunchKernelGGL((k_mpole_tiles<PEN, CF, ANA, GRAD, VIR, true>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
+        {
+            ProfScope ps_(c, "mpole_tiles");
+            hipLaunchKernelGGL((k_mpole_tiles<PEN, CF, ANA, GRAD, VIR, true>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
+        }
     else
-        hipLaunchKernelGGL((k_mpole_tiles<PEN, CF, ANA, GRAD, VIR, false>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream,
-                           a);
+        {
+            ProfScope ps_(c, "mpole_tiles");
+            hipLaunchKernelGGL((k_mpole_tiles<PEN, CF, ANA, GRAD, VIR, false>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream,
+                               a);
+        }
     if (a.nexcl > 0)
         hipLaunchKernelGGL((k_mpole_excl<PEN, CF, ANA, GRAD, VIR>), dim3((a.nexcl + 63) / 64), dim3(64), 0,
                            c->stream, a);

@@ -72,6 +72,7 @@ int pme_setup(pmq_ctx *c)
     bool same_box = same_grid && c->pme_alpha == p.aewald && c->pme_dbox.kind == c->box.kind;
     for (int a = 0; a < 3 && same_box; a++)
         for (int b = 0; b < 3; b++) same_box = same_box && c->pme_dbox.recip[a][b] == c->box.recip[a][b];
+    PMQ_HIP(c->pme_frac.alloc(40 * (size_t)c->npad)); // follows the atom count
     if (same_box && c->plan_f1) return PMQ_OK;
     c->K[0] = K[0]; c->K[1] = K[1]; c->K[2] = K[2];
     c->bsorder = p.bsorder;
@@ -158,8 +159,15 @@ int pme_setup(pmq_ctx *c)
 
 // ---------------------------------------------------------------------------
 // kernels
+//
+// One group of G lanes serves one atom: lane (ky, kz) of the order x order
+// stencil plane handles the `order` grid points along x (G = 32 for order
+// <= 5, 64 for order 6).  Spreading issues its x-row of atomics; gathering
+// accumulates the x-row and the group reduces with butterfly shuffles.  The
+// atoms of one wave are taken far apart in the spatial order (a fixed
+// stride coprime with n) so that their stencils rarely share grid points.
 struct PmeGeom {
-    int K0, K1, K2, order;
+    int K0, K1, K2;
     double A[3][3];
 };
 
@@ -167,44 +175,63 @@ static PmeGeom geom(const pmq_ctx *c)
 {
     PmeGeom g;
     g.K0 = c->K[0]; g.K1 = c->K[1]; g.K2 = c->K[2];
-    g.order = c->bsorder;
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) g.A[a][b] = c->pmeA[a][b];
     return g;
 }
 
-// B-spline weights of one atom along the three axes: th[axis][k][deriv]
-template <int ND>
-__device__ __forceinline__ void atom_splines(const PmeGeom &g, double px, double py, double pz, double th[3][PMQ_MAX_ORDER][4],
-                                             int *i0)
+template <int ORD>
+struct PmeLanes {
+    static constexpr int G = ORD * ORD <= 32 ? 32 : 64;
+};
+
+// stencil start and fractional offset along each axis
+__device__ __forceinline__ void atom_frac(const PmeGeom &g, double px, double py, double pz, double *w, int *i0, int ord)
 {
     const double p[3] = {px, py, pz};
     const int K[3] = {g.K0, g.K1, g.K2};
+#pragma unroll
     for (int a = 0; a < 3; a++) {
         const double u = g.A[a][0] * p[0] + g.A[a][1] * p[1] + g.A[a][2] * p[2];
         const double fl = floor(u);
-        const double w = u - fl;
-        int s = ((int)fl - g.order + 1) % K[a];
+        w[a] = u - fl;
+        int s = ((int)fl - ord + 1) % K[a];
         if (s < 0) s += K[a];
         i0[a] = s;
-        bspline_weights<double, ND>(w, g.order, th[a]);
     }
+}
+
+// weight set of one axis at stencil index k (selected without dynamic
+// register indexing)
+template <int ORD, int ND>
+__device__ __forceinline__ void pick(const double th[PMQ_MAX_ORDER][4], int k, double *out)
+{
+#pragma unroll
+    for (int d = 0; d <= ND; d++) out[d] = 0.0;
+#pragma unroll
+    for (int q = 0; q < ORD; q++)
+        if (q == k) {
+#pragma unroll
+            for (int d = 0; d <= ND; d++) out[d] = th[q][d];
+        }
 }
 
 // fractional multipole: c, A d, A q A^T (off-diagonals doubled)
 __device__ __forceinline__ void frac_mpole(const PmeGeom &g, const double *m, double *fm)
 {
     fm[0] = m[0];
+#pragma unroll
     for (int a = 0; a < 3; a++) fm[1 + a] = g.A[a][0] * m[1] + g.A[a][1] * m[2] + g.A[a][2] * m[3];
     const double Q[3][3] = {{m[4], m[5], m[6]}, {m[5], m[7], m[8]}, {m[6], m[8], m[9]}};
     double F[3][3];
+#pragma unroll
     for (int a = 0; a < 3; a++)
-        for (int b = a; b < 3; b++) {
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
             double v = 0;
-            for (int p = 0; p < 3; p++) {
-                const double t = Q[p][0] * g.A[b][0] + Q[p][1] * g.A[b][1] + Q[p][2] * g.A[b][2];
-                v += g.A[a][p] * t;
-            }
+#pragma unroll
+            for (int p = 0; p < 3; p++)
+                v += g.A[a][p] * (Q[p][0] * g.A[b][0] + Q[p][1] * g.A[b][1] + Q[p][2] * g.A[b][2]);
             F[a][b] = v;
         }
     fm[4] = F[0][0]; fm[5] = F[1][1]; fm[6] = F[2][2];
@@ -212,67 +239,68 @@ __device__ __forceinline__ void frac_mpole(const PmeGeom &g, const double *m, do
 }
 
 struct SpreadArgs {
-    int n, npad;
+    int n, npad, stride;
     PmeGeom g;
     const double *sx, *sy, *sz;
-    const double *src;   // sorted SoA sources: 10 multipole comps (mode 0) or 3 dipole comps x nset (mode 1)
-    const double *charge; // mode 0: total monopole per sorted atom (CP systems), or null
-    double *grid;        // nset grids
+    const double *src;    // sorted SoA sources: 10 multipole comps (mode 0) or 3 dipole comps x nset (mode 1)
+    const double *charge; // mode 0: total monopole per sorted atom (charge penetration), or null
+    double *grid;         // nset grids
     int nset;
     size_t gstride;
 };
 
 // mode 0: permanent multipoles (one grid); mode 1: dipole sets (nset grids)
-template <int MODE>
-__global__ __launch_bounds__(128) void k_pme_spread(SpreadArgs a)
+template <int ORD, int MODE>
+__global__ __launch_bounds__(256) void k_pme_spread(SpreadArgs a)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.n) return;
-    double th[3][PMQ_MAX_ORDER][4];
+    constexpr int G = PmeLanes<ORD>::G;
+    constexpr int ND = MODE == 0 ? 2 : 1;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int slot = tid / G, l = tid % G;
+    if (slot >= a.n || l >= ORD * ORD) return; // no cross-lane work in this kernel
+    const int s = (int)(((long long)slot * a.stride) % a.n);
+    const int ky = l % ORD, kz = l / ORD;
+    double w[3];
     int i0[3];
-    atom_splines<MODE == 0 ? 3 : 2>(a.g, a.sx[s], a.sy[s], a.sz[s], th, i0);
-    const int ord = a.g.order;
+    atom_frac(a.g, a.sx[s], a.sy[s], a.sz[s], w, i0, ORD);
+    double tx[PMQ_MAX_ORDER][4], tyf[PMQ_MAX_ORDER][4], tzf[PMQ_MAX_ORDER][4];
+    bspline_weights<double, ND>(w[0], ORD, tx);
+    bspline_weights<double, ND>(w[1], ORD, tyf);
+    bspline_weights<double, ND>(w[2], ORD, tzf);
+    double ty[ND + 1], tz[ND + 1];
+    pick<ORD, ND>(tyf, ky, ty);
+    pick<ORD, ND>(tzf, kz, tz);
+    int gy = i0[1] + ky, gz = i0[2] + kz;
+    if (gy >= a.g.K1) gy -= a.g.K1;
+    if (gz >= a.g.K2) gz -= a.g.K2;
     for (int set = 0; set < (MODE == 0 ? 1 : a.nset); set++) {
-        double fm[10];
+        double c0, c1, c2 = 0.0;
         if (MODE == 0) {
-            double m[10];
+            double m[10], fm[10];
+#pragma unroll
             for (int c = 0; c < 10; c++) m[c] = a.src[c * a.npad + s];
             if (a.charge) m[0] = a.charge[s];
             frac_mpole(a.g, m, fm);
+            c0 = fm[0] * ty[0] * tz[0] + fm[2] * ty[1] * tz[0] + fm[3] * ty[0] * tz[1] + fm[5] * ty[2] * tz[0]
+               + fm[6] * ty[0] * tz[2] + fm[9] * ty[1] * tz[1];
+            c1 = fm[1] * ty[0] * tz[0] + fm[7] * ty[1] * tz[0] + fm[8] * ty[0] * tz[1];
+            c2 = fm[4] * ty[0] * tz[0];
         } else {
             const double *d = a.src + (size_t)set * 3 * a.npad;
             const double dx = d[s], dy = d[a.npad + s], dz = d[2 * a.npad + s];
-            fm[0] = 0;
-            for (int q = 0; q < 3; q++) fm[1 + q] = a.g.A[q][0] * dx + a.g.A[q][1] * dy + a.g.A[q][2] * dz;
+            const double f1 = a.g.A[0][0] * dx + a.g.A[0][1] * dy + a.g.A[0][2] * dz;
+            const double f2 = a.g.A[1][0] * dx + a.g.A[1][1] * dy + a.g.A[1][2] * dz;
+            const double f3 = a.g.A[2][0] * dx + a.g.A[2][1] * dy + a.g.A[2][2] * dz;
+            c0 = f2 * ty[1] * tz[0] + f3 * ty[0] * tz[1];
+            c1 = f1 * ty[0] * tz[0];
         }
-        double *grid = a.grid + set * a.gstride;
-        for (int kz = 0; kz < ord; kz++) {
-            int gz = i0[2] + kz;
-            if (gz >= a.g.K2) gz -= a.g.K2;
-            const double *tz = th[2][kz];
-            for (int ky = 0; ky < ord; ky++) {
-                int gy = i0[1] + ky;
-                if (gy >= a.g.K1) gy -= a.g.K1;
-                const double *ty = th[1][ky];
-                // coefficients of the x-spline derivatives 0, 1, 2 for this (y, z)
-                double c0, c1, c2 = 0.0;
-                if (MODE == 0) {
-                    c0 = fm[0] * ty[0] * tz[0] + fm[2] * ty[1] * tz[0] + fm[3] * ty[0] * tz[1] + fm[5] * ty[2] * tz[0]
-                       + fm[6] * ty[0] * tz[2] + fm[9] * ty[1] * tz[1];
-                    c1 = fm[1] * ty[0] * tz[0] + fm[7] * ty[1] * tz[0] + fm[8] * ty[0] * tz[1];
-                    c2 = fm[4] * ty[0] * tz[0];
-                } else {
-                    c0 = fm[2] * ty[1] * tz[0] + fm[3] * ty[0] * tz[1];
-                    c1 = fm[1] * ty[0] * tz[0];
-                }
-                double *row = grid + ((size_t)gz * a.g.K1 + gy) * a.g.K0;
-                for (int kx = 0; kx < ord; kx++) {
-                    int gx = i0[0] + kx;
-                    if (gx >= a.g.K0) gx -= a.g.K0;
-                    const double *tx = th[0][kx];
-                    atomicAdd(row + gx, c0 * tx[0] + c1 * tx[1] + c2 * tx[2]);
-                }
-            }
+        double *row = a.grid + set * a.gstride + ((size_t)gz * a.g.K1 + gy) * a.g.K0;
+#pragma unroll
+        for (int kx = 0; kx < ORD; kx++) {
+            int gx = i0[0] + kx;
+            if (gx >= a.g.K0) gx -= a.g.K0;
+            const double v = c0 * tx[kx][0] + c1 * tx[kx][1] + (MODE == 0 ? c2 * tx[kx][2] : 0.0);
+            atomicAdd(row + gx, v);
         }
     }
 }
@@ -286,61 +314,67 @@ __global__ void k_pme_convolve(int ncplx, int nset, const double *__restrict__ G
     cg[2 * t + 1] *= g;
 }
 
-// gather the potential derivatives at one site from one grid: out[20] in
-// the fractional order 0; 1,2,3; 11,22,33,12,13,23; 111,222,333,112,113,
-// 122,133,223,233,123
-template <int NOUT>
-__device__ __forceinline__ void gather_site(const PmeGeom &g, const double *grid, const double th[3][PMQ_MAX_ORDER][4],
-                                            const int *i0, double *o)
+// this lane's share of the potential derivatives at one site, in the
+// fractional order 0; 1,2,3; 11,22,33,12,13,23; 111,222,333,112,113,122,133,
+// 223,233,123 (NOUT = 4, 10 or 20); idle lanes (`on` false) give zeros
+template <int ORD, int NOUT>
+__device__ __forceinline__ void gather_part(const PmeGeom &g, const double *grid, const double tx[PMQ_MAX_ORDER][4],
+                                            const double *ty, const double *tz, const int *i0, int ky, int kz,
+                                            bool on, double *o)
 {
+#pragma unroll
     for (int j = 0; j < NOUT; j++) o[j] = 0.0;
-    const int ord = g.order;
-    for (int kz = 0; kz < ord; kz++) {
-        int gz = i0[2] + kz;
-        if (gz >= g.K2) gz -= g.K2;
-        const double *tz = th[2][kz];
-        for (int ky = 0; ky < ord; ky++) {
-            int gy = i0[1] + ky;
-            if (gy >= g.K1) gy -= g.K1;
-            const double *ty = th[1][ky];
-            // x-direction partial sums for derivative orders 0..3
-            double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-            const double *row = grid + ((size_t)gz * g.K1 + gy) * g.K0;
-            for (int kx = 0; kx < ord; kx++) {
-                int gx = i0[0] + kx;
-                if (gx >= g.K0) gx -= g.K0;
-                const double q = row[gx];
-                const double *tx = th[0][kx];
-                s0 += q * tx[0];
-                s1 += q * tx[1];
-                if (NOUT > 4) s2 += q * tx[2];
-                if (NOUT > 10) s3 += q * tx[3];
-            }
-            o[0] += s0 * ty[0] * tz[0];
-            o[1] += s1 * ty[0] * tz[0];
-            o[2] += s0 * ty[1] * tz[0];
-            o[3] += s0 * ty[0] * tz[1];
-            if (NOUT > 4) {
-                o[4] += s2 * ty[0] * tz[0];
-                o[5] += s0 * ty[2] * tz[0];
-                o[6] += s0 * ty[0] * tz[2];
-                o[7] += s1 * ty[1] * tz[0];
-                o[8] += s1 * ty[0] * tz[1];
-                o[9] += s0 * ty[1] * tz[1];
-            }
-            if (NOUT > 10) {
-                o[10] += s3 * ty[0] * tz[0];
-                o[11] += s0 * ty[3] * tz[0];
-                o[12] += s0 * ty[0] * tz[3];
-                o[13] += s2 * ty[1] * tz[0];
-                o[14] += s2 * ty[0] * tz[1];
-                o[15] += s1 * ty[2] * tz[0];
-                o[16] += s1 * ty[0] * tz[2];
-                o[17] += s0 * ty[2] * tz[1];
-                o[18] += s0 * ty[1] * tz[2];
-                o[19] += s1 * ty[1] * tz[1];
-            }
-        }
+    if (!on) return;
+    int gy = i0[1] + ky, gz = i0[2] + kz;
+    if (gy >= g.K1) gy -= g.K1;
+    if (gz >= g.K2) gz -= g.K2;
+    const double *row = grid + ((size_t)gz * g.K1 + gy) * g.K0;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+    for (int kx = 0; kx < ORD; kx++) {
+        int gx = i0[0] + kx;
+        if (gx >= g.K0) gx -= g.K0;
+        const double q = row[gx];
+        s0 += q * tx[kx][0];
+        s1 += q * tx[kx][1];
+        if (NOUT > 4) s2 += q * tx[kx][2];
+        if (NOUT > 10) s3 += q * tx[kx][3];
+    }
+    o[0] = s0 * ty[0] * tz[0];
+    o[1] = s1 * ty[0] * tz[0];
+    o[2] = s0 * ty[1] * tz[0];
+    o[3] = s0 * ty[0] * tz[1];
+    if (NOUT > 4) {
+        o[4] = s2 * ty[0] * tz[0];
+        o[5] = s0 * ty[2] * tz[0];
+        o[6] = s0 * ty[0] * tz[2];
+        o[7] = s1 * ty[1] * tz[0];
+        o[8] = s1 * ty[0] * tz[1];
+        o[9] = s0 * ty[1] * tz[1];
+    }
+    if (NOUT > 10) {
+        o[10] = s3 * ty[0] * tz[0];
+        o[11] = s0 * ty[3] * tz[0];
+        o[12] = s0 * ty[0] * tz[3];
+        o[13] = s2 * ty[1] * tz[0];
+        o[14] = s2 * ty[0] * tz[1];
+        o[15] = s1 * ty[2] * tz[0];
+        o[16] = s1 * ty[0] * tz[2];
+        o[17] = s0 * ty[2] * tz[1];
+        o[18] = s0 * ty[1] * tz[2];
+        o[19] = s1 * ty[1] * tz[1];
+    }
+}
+
+template <int G, int NOUT>
+__device__ __forceinline__ void group_reduce(double *o)
+{
+#pragma unroll
+    for (int j = 0; j < NOUT; j++) {
+        double v = o[j];
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        o[j] = v;
     }
 }
 
@@ -351,50 +385,98 @@ __device__ __forceinline__ void frac_to_cart(const PmeGeom &g, const double *f, 
                                              double *T)
 {
     *phi = f[0];
+#pragma unroll
     for (int x = 0; x < 3; x++) grad[x] = g.A[0][x] * f[1] + g.A[1][x] * f[2] + g.A[2][x] * f[3];
     if (NOUT > 4) {
         const double F2[3][3] = {{f[4], f[7], f[8]}, {f[7], f[5], f[9]}, {f[8], f[9], f[6]}};
         const int hx[6] = {0, 1, 2, 0, 0, 1}, hy[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
         for (int q = 0; q < 6; q++) {
             double v = 0;
+#pragma unroll
             for (int a = 0; a < 3; a++)
+#pragma unroll
                 for (int b = 0; b < 3; b++) v += g.A[a][hx[q]] * g.A[b][hy[q]] * F2[a][b];
             H[q] = v;
         }
     }
     if (NOUT > 10) {
-        // F3[a][b][c] from the 10 stored combinations
         double F3[3][3][3];
         const int idx[10][3] = {{0, 0, 0}, {1, 1, 1}, {2, 2, 2}, {0, 0, 1}, {0, 0, 2},
                                 {0, 1, 1}, {0, 2, 2}, {1, 1, 2}, {1, 2, 2}, {0, 1, 2}};
+#pragma unroll
         for (int q = 0; q < 10; q++) {
             const int a = idx[q][0], b = idx[q][1], d = idx[q][2];
             const double v = f[10 + q];
             F3[a][b][d] = F3[a][d][b] = F3[b][a][d] = F3[b][d][a] = F3[d][a][b] = F3[d][b][a] = v;
         }
+#pragma unroll
         for (int q = 0; q < 10; q++) {
             const int x = idx[q][0], y = idx[q][1], z = idx[q][2];
             double v = 0;
+#pragma unroll
             for (int a = 0; a < 3; a++)
+#pragma unroll
                 for (int b = 0; b < 3; b++)
+#pragma unroll
                     for (int d = 0; d < 3; d++) v += g.A[a][x] * g.A[b][y] * g.A[d][z] * F3[a][b][d];
             T[q] = v;
         }
     }
 }
 
-__device__ __forceinline__ double wsum(double v)
+// common lane setup of the gather kernels
+#define GATHER_SETUP(ND)                                                                                     \
+    constexpr int G = PmeLanes<ORD>::G;                                                                      \
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;                                                   \
+    const int slot = tid / G, l = tid % G;                                                                   \
+    const bool valid = slot < a.n;                                                                           \
+    const bool on = valid && l < ORD * ORD;                                                                  \
+    const int s = valid ? (int)(((long long)slot * a.stride) % a.n) : 0;                                     \
+    const int ky = l % ORD, kz = (l / ORD) % ORD;                                                            \
+    double w[3];                                                                                             \
+    int i0[3];                                                                                               \
+    atom_frac(a.g, a.sx[s], a.sy[s], a.sz[s], w, i0, ORD);                                                   \
+    double tx[PMQ_MAX_ORDER][4], tyf[PMQ_MAX_ORDER][4], tzf[PMQ_MAX_ORDER][4], ty[4], tz[4];                 \
+    bspline_weights<double, ND>(w[0], ORD, tx);                                                              \
+    bspline_weights<double, ND>(w[1], ORD, tyf);                                                             \
+    bspline_weights<double, ND>(w[2], ORD, tzf);                                                             \
+    pick<ORD, ND>(tyf, ky, ty);                                                                              \
+    pick<ORD, ND>(tzf, kz, tz);
+
+struct GatherFracArgs {
+    int n, stride, nset;
+    PmeGeom g;
+    const double *sx, *sy, *sz;
+    const double *grid;    // nset grids
+    size_t gstride;
+    double *frac;          // n x nset x 20 fractional derivatives, sorted atom order
+};
+
+// the 20 fractional potential derivatives of every atom in each of nset
+// grids; lane j < 20 of the group stores component j
+template <int ORD>
+__global__ __launch_bounds__(256) void k_pme_gather_frac(GatherFracArgs a)
 {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    GATHER_SETUP(3)
+    for (int set = 0; set < a.nset; set++) {
+        double fp[20];
+        gather_part<ORD, 20>(a.g, a.grid + set * a.gstride, tx, ty, tz, i0, ky, kz, on, fp);
+        group_reduce<G, 20>(fp);
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < 20; j++)
+            if (l == j) v = fp[j];
+        if (valid && l < 20) a.frac[((size_t)s * a.nset + set) * 20 + l] = v;
+    }
 }
 
-struct GatherMpoleArgs {
+struct MpolePostArgs {
     int n, npad;
     PmeGeom g;
-    const double *sx, *sy, *sz, *smp, *charge;
+    const double *smp, *charge;
     const int *perm;
-    const double *grid;
+    const double *frac;      // n x 20
     double f, aewald;
     int grad, polar;
     double *dem, *te;        // caller order
@@ -403,19 +485,16 @@ struct GatherMpoleArgs {
     Totals *tot;
 };
 
-// permanent multipoles in their own reciprocal potential: energy (f/2) U,
-// gradient, torque, the self energy, and the reciprocal + self field the
-// induced dipoles see
-__global__ __launch_bounds__(64) void k_pme_gather_mpole(GatherMpoleArgs a)
+// permanent multipoles in their own reciprocal potential (one thread per
+// sorted atom): energy (f/2) U, gradient, torque, the self energy, and the
+// reciprocal + self field the induced dipoles see
+__global__ __launch_bounds__(256) void k_pme_mpole_post(MpolePostArgs a)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     double e = 0.0;
     if (s < a.n) {
-        double th[3][PMQ_MAX_ORDER][4];
-        int i0[3];
-        atom_splines<3>(a.g, a.sx[s], a.sy[s], a.sz[s], th, i0);
         double fp[20];
-        gather_site<20>(a.g, a.grid, th, i0, fp);
+        for (int j = 0; j < 20; j++) fp[j] = a.frac[(size_t)s * 20 + j];
         double phi, gr[3], H[6], T[10];
         frac_to_cart<20>(a.g, fp, &phi, gr, H, T);
         double m[10];
@@ -443,7 +522,6 @@ __global__ __launch_bounds__(64) void k_pme_gather_mpole(GatherMpoleArgs a)
             }
         }
         if (a.grad) {
-            // T as a full tensor
             const double T3[3][3][3] = {{{T[0], T[3], T[4]}, {T[3], T[5], T[9]}, {T[4], T[9], T[6]}},
                                         {{T[3], T[5], T[9]}, {T[5], T[1], T[7]}, {T[9], T[7], T[8]}},
                                         {{T[4], T[9], T[6]}, {T[9], T[7], T[8]}, {T[6], T[8], T[2]}}};
@@ -465,13 +543,13 @@ __global__ __launch_bounds__(64) void k_pme_gather_mpole(GatherMpoleArgs a)
         }
     }
     if (a.tot) {
-        e = wsum(e);
+        for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
         if ((threadIdx.x & 63) == 0) atomicAdd(&a.tot->em, e);
     }
 }
 
 struct GatherFieldArgs {
-    int n, npad, nset;
+    int n, npad, stride, nset;
     PmeGeom g;
     const double *sx, *sy, *sz;
     const double *grid;
@@ -482,52 +560,50 @@ struct GatherFieldArgs {
 };
 
 // reciprocal + self field of the induced-dipole sets at every site
-__global__ __launch_bounds__(128) void k_pme_gather_field(GatherFieldArgs a)
+template <int ORD>
+__global__ __launch_bounds__(256) void k_pme_gather_field(GatherFieldArgs a)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.n) return;
-    double th[3][PMQ_MAX_ORDER][4];
-    int i0[3];
-    atom_splines<1>(a.g, a.sx[s], a.sy[s], a.sz[s], th, i0);
+    GATHER_SETUP(1)
     for (int set = 0; set < a.nset; set++) {
-        double fp[4], phi, gr[3];
-        gather_site<4>(a.g, a.grid + set * a.gstride, th, i0, fp);
-        frac_to_cart<4>(a.g, fp, &phi, gr, nullptr, nullptr);
-        const size_t o = (size_t)set * 3 * a.npad;
-        for (int q = 0; q < 3; q++) a.field[o + q * a.npad + s] += -gr[q] + a.self * a.dip[o + q * a.npad + s];
+        double fp[4];
+        gather_part<ORD, 4>(a.g, a.grid + set * a.gstride, tx, ty, tz, i0, ky, kz, on, fp);
+        group_reduce<G, 4>(fp);
+        if (valid && l == 0) {
+            double phi, gr[3];
+            frac_to_cart<4>(a.g, fp, &phi, gr, nullptr, nullptr);
+            const size_t o = (size_t)set * 3 * a.npad;
+            for (int q = 0; q < 3; q++) a.field[o + q * a.npad + s] += -gr[q] + a.self * a.dip[o + q * a.npad + s];
+        }
     }
 }
 
-struct GatherPolGradArgs {
+struct PolGradPostArgs {
     int n, npad;
     PmeGeom g;
-    const double *sx, *sy, *sz, *smp, *charge;
+    const double *smp, *charge;
     const double *cphi;          // permanent potential (phi, g, H)
     const double *ud, *up;       // sorted dipoles
     const int *perm;
-    const double *grid;          // two grids: mu_d, mu_p
-    size_t gstride;
+    const double *frac;          // n x 2 x 20: potentials of mu_d, mu_p
     double f, self;
     double *dep, *tep;           // caller order
 };
 
-// reciprocal-space polarization gradient:
+// reciprocal-space polarization gradient (one thread per sorted atom):
 //   (f/2) [(mu_d+mu_p).H_M + mu_d.H_p + mu_p.H_d] on the induced dipoles,
 //   (f/2) gradient and torque of the permanent multipoles in phi_d + phi_p,
 //   and the self-term torque d x (f/2) self (mu_d + mu_p)
-__global__ __launch_bounds__(64) void k_pme_gather_polgrad(GatherPolGradArgs a)
+__global__ __launch_bounds__(256) void k_pme_polgrad_post(PolGradPostArgs a)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.n) return;
-    double th[3][PMQ_MAX_ORDER][4];
-    int i0[3];
-    atom_splines<3>(a.g, a.sx[s], a.sy[s], a.sz[s], th, i0);
-    double fd[20], fq[20];
-    gather_site<20>(a.g, a.grid, th, i0, fd);
-    gather_site<20>(a.g, a.grid + a.gstride, th, i0, fq);
-    double phd, gd[3], Hd[6], Td[10], php, gp[3], Hp[6], Tp[10];
-    frac_to_cart<20>(a.g, fd, &phd, gd, Hd, Td);
-    frac_to_cart<20>(a.g, fq, &php, gp, Hp, Tp);
+    // the transform is linear: phi_d + phi_p in one pass, H_d on its own
+    double fs[20], fd[10];
+    for (int j = 0; j < 20; j++) fs[j] = a.frac[(size_t)s * 40 + j] + a.frac[(size_t)s * 40 + 20 + j];
+    for (int j = 0; j < 10; j++) fd[j] = a.frac[(size_t)s * 40 + j];
+    double phs, g[3], Hs[6], T[10], phd, gd[3], Hd[6];
+    frac_to_cart<20>(a.g, fs, &phs, g, Hs, T);
+    frac_to_cart<10>(a.g, fd, &phd, gd, Hd, nullptr);
     const int np = a.npad;
     const double ud[3] = {a.ud[s], a.ud[np + s], a.ud[2 * np + s]};
     const double up[3] = {a.up[s], a.up[np + s], a.up[2 * np + s]};
@@ -537,20 +613,16 @@ __global__ __launch_bounds__(64) void k_pme_gather_polgrad(GatherPolGradArgs a)
         M[0][0] = h[0]; M[1][1] = h[1]; M[2][2] = h[2];
         M[0][1] = M[1][0] = h[3]; M[0][2] = M[2][0] = h[4]; M[1][2] = M[2][1] = h[5];
     };
-    double Hm[3][3], Hdm[3][3], Hpm[3][3];
+    double Hm[3][3], Hdm[3][3], H[3][3], Hpm[3][3];
     full(HM, Hm);
     full(Hd, Hdm);
-    full(Hp, Hpm);
+    full(Hs, H);
+    for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++) Hpm[p][q] = H[p][q] - Hdm[p][q];
     double m[10];
     for (int c = 0; c < 10; c++) m[c] = a.smp[c * np + s];
     if (a.charge) m[0] = a.charge[s];
     const double Q[3][3] = {{m[4], m[5], m[6]}, {m[5], m[7], m[8]}, {m[6], m[8], m[9]}};
-    // potential of mu_d + mu_p
-    double g[3], H[3][3], T[10];
-    for (int q = 0; q < 3; q++) g[q] = gd[q] + gp[q];
-    for (int p = 0; p < 3; p++)
-        for (int q = 0; q < 3; q++) H[p][q] = Hdm[p][q] + Hpm[p][q];
-    for (int q = 0; q < 10; q++) T[q] = Td[q] + Tp[q];
     const double T3[3][3][3] = {{{T[0], T[3], T[4]}, {T[3], T[5], T[9]}, {T[4], T[9], T[6]}},
                                 {{T[3], T[5], T[9]}, {T[5], T[1], T[7]}, {T[9], T[7], T[8]}},
                                 {{T[4], T[9], T[6]}, {T[9], T[7], T[8]}, {T[6], T[8], T[2]}}};
@@ -583,6 +655,7 @@ static int fft(pmq_ctx *c, bool forward, int nset)
     rocfft_plan pl = (rocfft_plan)(forward ? (nset == 1 ? c->plan_f1 : c->plan_f2) : (nset == 1 ? c->plan_b1 : c->plan_b2));
     void *in[1] = {forward ? (void *)c->qgrid.p : (void *)c->cgrid.p};
     void *out[1] = {forward ? (void *)c->cgrid.p : (void *)c->qgrid.p};
+    ProfScope ps_(c, "pme_fft");
     return fft_check(c, rocfft_execute(pl, in, out, (rocfft_execution_info)c->fft_info), "execute");
 }
 
@@ -601,34 +674,83 @@ static int pme_potential(pmq_ctx *c, int nset)
 
 static size_t grid_size(const pmq_ctx *c) { return (size_t)c->K[0] * c->K[1] * c->K[2]; }
 
+// a stride near n / golden ratio that is coprime with n
+static int atom_stride(int n)
+{
+    auto gcd = [](long a, long b) { while (b) { long t = a % b; a = b; b = t; } return a; };
+    long s = std::max(1L, (long)(0.6180339887 * n));
+    while (gcd(s, n) != 1) s++;
+    return (int)s;
+}
+
+static dim3 lanes_grid(int n, int order) { return dim3((unsigned)(((size_t)n * (order * order <= 32 ? 32 : 64) + 255) / 256)); }
+
+#define PMQ_ORDER_DISPATCH(KERNEL, ...)                                                                    \
+    switch (c->bsorder) {                                                                                  \
+    case 4: hipLaunchKernelGGL((KERNEL<4 __VA_ARGS__>), lanes_grid(c->n, 4), dim3(256), 0, c->stream, args); break; \
+    case 5: hipLaunchKernelGGL((KERNEL<5 __VA_ARGS__>), lanes_grid(c->n, 5), dim3(256), 0, c->stream, args); break; \
+    default: hipLaunchKernelGGL((KERNEL<6 __VA_ARGS__>), lanes_grid(c->n, 6), dim3(256), 0, c->stream, args); break; \
+    }
+
+static int spread(pmq_ctx *c, int mode, const double *src, const double *charge, int nset)
+{
+    SpreadArgs args;
+    args.n = c->n; args.npad = c->npad; args.stride = atom_stride(c->n); args.g = geom(c);
+    args.sx = c->sx.p; args.sy = c->sy.p; args.sz = c->sz.p;
+    args.src = src;
+    args.charge = charge;
+    args.grid = c->qgrid.p;
+    args.nset = nset;
+    args.gstride = grid_size(c);
+    ProfScope ps_(c, "pme_spread");
+    if (mode == 0) {
+        PMQ_ORDER_DISPATCH(k_pme_spread, , 0)
+    } else {
+        PMQ_ORDER_DISPATCH(k_pme_spread, , 1)
+    }
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+// fractional potential derivatives of nset grids into pme_frac
+static int gather_frac(pmq_ctx *c, int nset)
+{
+    GatherFracArgs args;
+    args.n = c->n; args.stride = atom_stride(c->n); args.nset = nset; args.g = geom(c);
+    args.sx = c->sx.p; args.sy = c->sy.p; args.sz = c->sz.p;
+    args.grid = c->qgrid.p;
+    args.gstride = grid_size(c);
+    args.frac = c->pme_frac.p;
+    ProfScope ps_(c, "pme_gather");
+    PMQ_ORDER_DISPATCH(k_pme_gather_frac, )
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
 int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar)
 {
     int rc;
     if ((rc = launch_zero(c, c->qgrid.p, grid_size(c)))) return rc;
-    SpreadArgs s;
-    s.n = c->n; s.npad = c->npad; s.g = geom(c);
-    s.sx = c->sx.p; s.sy = c->sy.p; s.sz = c->sz.p;
-    s.src = c->smp.p;
-    s.charge = c->prm.pentype != PMQ_PEN_NONE ? c->scp.p + 2 * (size_t)c->npad : nullptr;
-    s.grid = c->qgrid.p;
-    s.nset = 1;
-    s.gstride = grid_size(c);
-    hipLaunchKernelGGL(k_pme_spread<0>, dim3((c->n + 127) / 128), dim3(128), 0, c->stream, s);
-    PMQ_HIP(hipGetLastError());
+    const double *charge = c->prm.pentype != PMQ_PEN_NONE ? c->scp.p + 2 * (size_t)c->npad : nullptr;
+    if ((rc = spread(c, 0, c->smp.p, charge, 1))) return rc;
     if ((rc = pme_potential(c, 1))) return rc;
-    GatherMpoleArgs g;
-    g.n = c->n; g.npad = c->npad; g.g = geom(c);
-    g.sx = c->sx.p; g.sy = c->sy.p; g.sz = c->sz.p; g.smp = c->smp.p; g.charge = s.charge;
-    g.perm = c->perm.p;
-    g.grid = c->qgrid.p;
-    g.f = c->prm.electric / c->prm.dielec;
-    g.aewald = c->prm.aewald;
-    g.grad = mpole && (mode == PMQ_GRADIENT || mode == PMQ_VIRIAL);
-    g.polar = polar;
-    g.dem = c->dem.p; g.te = c->te.p;
-    g.cphi = c->cphi.p; g.fieldd = c->pfield.p; g.fieldp = c->pfield.p + 3 * (size_t)c->npad;
-    g.tot = mpole ? c->totals.p : nullptr;
-    hipLaunchKernelGGL(k_pme_gather_mpole, dim3((c->n + 63) / 64), dim3(64), 0, c->stream, g);
+    if ((rc = gather_frac(c, 1))) return rc;
+    MpolePostArgs args;
+    args.n = c->n; args.npad = c->npad; args.g = geom(c);
+    args.smp = c->smp.p; args.charge = charge;
+    args.perm = c->perm.p;
+    args.frac = c->pme_frac.p;
+    args.f = c->prm.electric / c->prm.dielec;
+    args.aewald = c->prm.aewald;
+    args.grad = mpole && (mode == PMQ_GRADIENT || mode == PMQ_VIRIAL);
+    args.polar = polar;
+    args.dem = c->dem.p; args.te = c->te.p;
+    args.cphi = c->cphi.p; args.fieldd = c->pfield.p; args.fieldp = c->pfield.p + 3 * (size_t)c->npad;
+    args.tot = mpole ? c->totals.p : nullptr;
+    {
+        ProfScope ps_(c, "pme_gather");
+        hipLaunchKernelGGL(k_pme_mpole_post, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, args);
+    }
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }
@@ -639,26 +761,20 @@ int pme_dipole_field(pmq_ctx *c, const double *dip, double *field)
     int rc;
     const size_t gs = grid_size(c);
     if ((rc = launch_zero(c, c->qgrid.p, 2 * gs))) return rc;
-    SpreadArgs s;
-    s.n = c->n; s.npad = c->npad; s.g = geom(c);
-    s.sx = c->sx.p; s.sy = c->sy.p; s.sz = c->sz.p;
-    s.src = dip;
-    s.charge = nullptr;
-    s.grid = c->qgrid.p;
-    s.nset = 2;
-    s.gstride = gs;
-    hipLaunchKernelGGL(k_pme_spread<1>, dim3((c->n + 127) / 128), dim3(128), 0, c->stream, s);
-    PMQ_HIP(hipGetLastError());
+    if ((rc = spread(c, 1, dip, nullptr, 2))) return rc;
     if ((rc = pme_potential(c, 2))) return rc;
-    GatherFieldArgs g;
-    g.n = c->n; g.npad = c->npad; g.nset = 2; g.g = geom(c);
-    g.sx = c->sx.p; g.sy = c->sy.p; g.sz = c->sz.p;
-    g.grid = c->qgrid.p;
-    g.gstride = gs;
-    g.dip = dip;
-    g.self = 4.0 / 3.0 * c->prm.aewald * c->prm.aewald * c->prm.aewald / 1.77245385090551602730;
-    g.field = field;
-    hipLaunchKernelGGL(k_pme_gather_field, dim3((c->n + 127) / 128), dim3(128), 0, c->stream, g);
+    GatherFieldArgs args;
+    args.n = c->n; args.npad = c->npad; args.stride = atom_stride(c->n); args.nset = 2; args.g = geom(c);
+    args.sx = c->sx.p; args.sy = c->sy.p; args.sz = c->sz.p;
+    args.grid = c->qgrid.p;
+    args.gstride = gs;
+    args.dip = dip;
+    args.self = 4.0 / 3.0 * c->prm.aewald * c->prm.aewald * c->prm.aewald / 1.77245385090551602730;
+    args.field = field;
+    {
+        ProfScope ps_(c, "pme_gather");
+        PMQ_ORDER_DISPATCH(k_pme_gather_field, )
+    }
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }
@@ -668,31 +784,25 @@ int pme_polar_grad(pmq_ctx *c)
     int rc;
     const size_t gs = grid_size(c);
     if ((rc = launch_zero(c, c->qgrid.p, 2 * gs))) return rc;
-    SpreadArgs s;
-    s.n = c->n; s.npad = c->npad; s.g = geom(c);
-    s.sx = c->sx.p; s.sy = c->sy.p; s.sz = c->sz.p;
-    s.src = c->uind.p; // mu_d then mu_p
-    s.charge = nullptr;
-    s.grid = c->qgrid.p;
-    s.nset = 2;
-    s.gstride = gs;
-    hipLaunchKernelGGL(k_pme_spread<1>, dim3((c->n + 127) / 128), dim3(128), 0, c->stream, s);
-    PMQ_HIP(hipGetLastError());
+    if ((rc = spread(c, 1, c->uind.p, nullptr, 2))) return rc;
     if ((rc = pme_potential(c, 2))) return rc;
-    GatherPolGradArgs g;
-    g.n = c->n; g.npad = c->npad; g.g = geom(c);
-    g.sx = c->sx.p; g.sy = c->sy.p; g.sz = c->sz.p; g.smp = c->smp.p;
-    g.charge = c->prm.pentype != PMQ_PEN_NONE ? c->scp.p + 2 * (size_t)c->npad : nullptr;
-    g.cphi = c->cphi.p;
-    g.ud = c->uind.p; g.up = c->uind.p + 3 * (size_t)c->npad;
-    g.perm = c->perm.p;
-    g.grid = c->qgrid.p;
-    g.gstride = gs;
-    g.f = c->prm.electric / c->prm.dielec;
-    g.self = 4.0 / 3.0 * c->prm.aewald * c->prm.aewald * c->prm.aewald / 1.77245385090551602730;
-    g.dep = c->dep.p;
-    g.tep = c->tep.p;
-    hipLaunchKernelGGL(k_pme_gather_polgrad, dim3((c->n + 63) / 64), dim3(64), 0, c->stream, g);
+    if ((rc = gather_frac(c, 2))) return rc;
+    PolGradPostArgs args;
+    args.n = c->n; args.npad = c->npad; args.g = geom(c);
+    args.smp = c->smp.p;
+    args.charge = c->prm.pentype != PMQ_PEN_NONE ? c->scp.p + 2 * (size_t)c->npad : nullptr;
+    args.cphi = c->cphi.p;
+    args.ud = c->uind.p; args.up = c->uind.p + 3 * (size_t)c->npad;
+    args.perm = c->perm.p;
+    args.frac = c->pme_frac.p;
+    args.f = c->prm.electric / c->prm.dielec;
+    args.self = 4.0 / 3.0 * c->prm.aewald * c->prm.aewald * c->prm.aewald / 1.77245385090551602730;
+    args.dep = c->dep.p;
+    args.tep = c->tep.p;
+    {
+        ProfScope ps_(c, "pme_gather");
+        hipLaunchKernelGGL(k_pme_polgrad_post, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, args);
+    }
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }

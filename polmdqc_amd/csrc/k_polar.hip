@@ -632,9 +632,15 @@ static int mutual_field(pmq_ctx *c, double off2, const double *src, double *dst)
     a.src = src;
     a.dst = dst;
     if (a.aewald > 0)
-        hipLaunchKernelGGL(k_pol_mutual_tiles<true>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+        {
+            ProfScope ps_(c, "pol_mutual_tiles");
+            hipLaunchKernelGGL(k_pol_mutual_tiles<true>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+        }
     else
-        hipLaunchKernelGGL(k_pol_mutual_tiles<false>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+        {
+            ProfScope ps_(c, "pol_mutual_tiles");
+            hipLaunchKernelGGL(k_pol_mutual_tiles<false>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+        }
     if (a.nexcl) hipLaunchKernelGGL(k_pol_mutual_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     if (a.aewald > 0 && (rc = pme_dipole_field(c, src, dst))) return rc;
@@ -652,9 +658,15 @@ int polar_prepare(pmq_ctx *c, double off2)
     PolArgs a = pol_args(c, off2);
     a.dst = c->pfield.p;
     if (a.aewald > 0)
-        hipLaunchKernelGGL(k_pol_field_tiles<true>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+        {
+            ProfScope ps_(c, "pol_field_tiles");
+            hipLaunchKernelGGL(k_pol_field_tiles<true>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+        }
     else
-        hipLaunchKernelGGL(k_pol_field_tiles<false>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+        {
+            ProfScope ps_(c, "pol_field_tiles");
+            hipLaunchKernelGGL(k_pol_field_tiles<false>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+        }
     if (a.nexcl) hipLaunchKernelGGL(k_pol_field_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
@@ -719,9 +731,15 @@ int polar_gradient(pmq_ctx *c, double off2)
     PolArgs a = pol_args(c, off2);
     a.src = c->uind.p;
     if (a.aewald > 0)
-        hipLaunchKernelGGL(k_pol_grad_tiles<true>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+        {
+            ProfScope ps_(c, "pol_grad_tiles");
+            hipLaunchKernelGGL(k_pol_grad_tiles<true>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+        }
     else
-        hipLaunchKernelGGL(k_pol_grad_tiles<false>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+        {
+            ProfScope ps_(c, "pol_grad_tiles");
+            hipLaunchKernelGGL(k_pol_grad_tiles<false>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+        }
     if (a.nexcl) hipLaunchKernelGGL(k_pol_grad_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     if (a.aewald > 0) return pme_polar_grad(c);

@@ -15,6 +15,33 @@ size_t neighbor_scratch_bytes(int n, int nblk);
 
 using namespace pmq;
 
+namespace pmq {
+static hipEvent_t take_event(pmq_ctx *c)
+{
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+ProfScope::ProfScope(pmq_ctx *ctx, const char *nm) : c(ctx), name(nm)
+{
+    if (!c->profiling) return;
+    a = take_event(c);
+    b = take_event(c);
+    if (a) (void)hipEventRecord(a, c->stream);
+}
+
+ProfScope::~ProfScope()
+{
+    if (!c->profiling || !a || !b) return;
+    (void)hipEventRecord(b, c->stream);
+    c->prof[name].push_back({a, b});
+}
+} // namespace pmq
+
 #define CHECK_CTX(c)                                                                               \
     do {                                                                                           \
         if (!(c)) return PMQ_ERR_ARG;                                                              \
@@ -31,6 +58,41 @@ static int upload(pmq_ctx *c, DBuf<T> &d, const T *h, size_t count)
 extern "C" {
 
 int pmq_abi_version(void) { return PMQ_ABI_VERSION; }
+
+int pmq_profile_enable(pmq_ctx *c, int on)
+{
+    CHECK_CTX(c);
+    PMQ_HIP(hipStreamSynchronize(c->stream));
+    c->profiling = on != 0;
+    c->prof.clear();
+    c->ev_used = 0;
+    return PMQ_OK;
+}
+
+int pmq_profile_read(pmq_ctx *c, const char *name, double *total_ms, long long *launches)
+{
+    CHECK_CTX(c);
+    if (!name || !total_ms || !launches) return PMQ_ERR_ARG;
+    PMQ_HIP(hipStreamSynchronize(c->stream));
+    double sum = 0.0;
+    long long cnt = 0;
+    auto it = c->prof.find(name);
+    if (it != c->prof.end()) {
+        for (auto &pr : it->second) {
+            float ms = 0.0f;
+            PMQ_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+            sum += ms;
+            cnt++;
+        }
+        it->second.clear();
+    }
+    bool empty = true;
+    for (auto &kv : c->prof) empty = empty && kv.second.empty();
+    if (empty) c->ev_used = 0;
+    *total_ms = sum;
+    *launches = cnt;
+    return PMQ_OK;
+}
 
 int pmq_create(const pmq_config *cfg, pmq_ctx **out)
 {
@@ -61,6 +123,7 @@ int pmq_destroy(pmq_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     pme_release(c);
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PMQ_OK;
@@ -425,7 +488,10 @@ int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
         const double off2 = cut * cut;
         const bool cf = c->prm.pentype != PMQ_PEN_NONE && c->prm.use_chgflx;
         if (c->prm.use_chgflx && (rc = launch_chgflx_delta(c))) return rc;
-        if ((rc = build_neighbors(c, cut))) return rc;
+        {
+            ProfScope ps(c, "neighbors");
+            if ((rc = build_neighbors(c, cut))) return rc;
+        }
         if ((rc = launch_rotpole(c, true))) return rc;
         if (do_polar && (rc = polar_prepare(c, off2))) return rc;
         if (do_mpole && (rc = launch_mpole_direct(c, mode, off2))) return rc;

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -218,6 +219,7 @@ struct pmq_ctx {
     pmq::DBuf<double> pme_g;                     // (K0/2+1) K1 K2 reciprocal kernel
     pmq::DBuf<double> qgrid;                     // 2 real grids
     pmq::DBuf<double> cgrid;                     // 2 complex half grids (interleaved re, im)
+    pmq::DBuf<double> pme_frac;                  // 2 x 20 fractional potential derivatives per atom
     pmq::DBuf<unsigned char> fft_work;
     void *plan_f1 = nullptr, *plan_b1 = nullptr, *plan_f2 = nullptr, *plan_b2 = nullptr;
     void *fft_info = nullptr;
@@ -226,6 +228,12 @@ struct pmq_ctx {
 
     int pcg_iters = 0;
     pmq::DBuf<double> uind_out;                  // 6n caller-order induced dipoles (d then p)
+
+    // kernel timing (pmq_profile_*)
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> prof;
 
     int last_mode = PMQ_ENERGY;
     unsigned last_terms = 0;
@@ -250,6 +258,17 @@ int polar_gradient(pmq_ctx *c, double off2);
 int polar_scatter(pmq_ctx *c, double *ud_out, double *up_out);
 int launch_chgflx_grad(pmq_ctx *c, int mode);
 int launch_zero(pmq_ctx *c, double *p, size_t n);
+} // namespace pmq
+
+namespace pmq {
+// brackets the launches issued in its lifetime with events when profiling
+struct ProfScope {
+    pmq_ctx *c;
+    const char *name;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(pmq_ctx *ctx, const char *nm);
+    ~ProfScope();
+};
 } // namespace pmq
 
 #define PMQ_HIP(call)                                                                              \

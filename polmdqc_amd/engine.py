@@ -188,3 +188,33 @@ class Engine:
             self.set_coords(xyz)
         self.compute(mode, terms, precision)
         return self.fetch(want_rpole)
+
+
+def _profile_bind(L):
+    if getattr(L, "_prof_ready", False):
+        return
+    L.pmq_profile_enable.argtypes = [C.c_void_p, C.c_int]
+    L.pmq_profile_enable.restype = C.c_int
+    L.pmq_profile_read.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+    L.pmq_profile_read.restype = C.c_int
+    L._prof_ready = True
+
+
+KERNEL_FAMILIES = ("neighbors", "mpole_tiles", "pol_field_tiles", "pol_mutual_tiles", "pol_grad_tiles",
+                   "pme_spread", "pme_fft", "pme_gather")
+
+
+def profile_enable(eng: Engine, on: bool = True):
+    _profile_bind(eng._L)
+    eng._check(eng._L.pmq_profile_enable(eng._ctx, 1 if on else 0), "pmq_profile_enable")
+
+
+def profile_read(eng: Engine) -> dict:
+    """{family: (total_ms, launches)} of the instrumented kernel families."""
+    _profile_bind(eng._L)
+    out = {}
+    for name in KERNEL_FAMILIES:
+        ms, cnt = C.c_double(), C.c_longlong()
+        eng._check(eng._L.pmq_profile_read(eng._ctx, name.encode(), C.byref(ms), C.byref(cnt)), "pmq_profile_read")
+        out[name] = (ms.value, cnt.value)
+    return out
