@@ -39,7 +39,6 @@ WORKLOAD = {"workload": "amoeba-water-4096 (12288 atoms)", "box_A": 49.68, "pme_
 PAIR_FLOPS = {
     "mpole_tiles": 552,       # image 6, r2 5, Ewald series 37, 5 terms + energy 70, gradient 150, torques 284
     "pol_field_tiles": 206,   # image 6, r2 5, Thole 28, Ewald+Thole series 68, two fields 84, accumulate 15
-    "pol_mutual_tiles": 175,  # image 6, r2 5, Thole 28, series 68, two dipole sets 56, accumulate 12
     "pol_grad_tiles": 1480,   # image 6, r2 5, Thole 28, series 68, four pair gradients ~330 each after zero folding
 }
 PEAK_FP64_TFLOPS = 78.6

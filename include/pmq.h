@@ -181,8 +181,9 @@ PMQ_API int pmq_sync(pmq_ctx *ctx);
  * enabled, the context brackets every launch of each instrumented kernel
  * family with HIP events on its own stream.  pmq_profile_read waits for the
  * stream, returns the summed elapsed time and launch count of one family
- * ("mpole_tiles", "pol_field_tiles", "pol_mutual_tiles", "pol_grad_tiles",
- * "pme_spread", "pme_fft", "pme_gather", "neighbors", "pcg") and resets it. */
+ * ("neighbors", "mpole_tiles", "pol_field_tiles", "pol_pair_fill",
+ * "pol_mutual_spmv", "pol_grad_tiles", "pme_spread", "pme_fft", "pme_gather")
+ * and resets it. */
 PMQ_API int pmq_profile_enable(pmq_ctx *ctx, int on);
 PMQ_API int pmq_profile_read(pmq_ctx *ctx, const char *name, double *total_ms, long long *launches);
 

@@ -1,0 +1,389 @@
+// k_pairs.hip -- in-cutoff pair masks and the dipole-field operator T as a
+// row-owned sparse matrix.
+//
+// The induced-dipole solve applies T (the Thole-damped, Ewald real-space
+// dipole-dipole field tensor, u-scaled for excluded pairs) to a new pair of
+// dipole vectors in every PCG iteration while the geometry stays fixed.  So
+// T is built once per evaluation and every iteration is a sparse
+// matrix-vector product that reads it back: 6 symmetric tensor components
+// and a column index per stored entry, each pair stored in both rows.
+// Every row is owned by one group of lanes, so the product needs no atomics
+// and sums each row in a fixed order (bit-reproducible run to run).
+//
+// Build, all per evaluation on the library's stream:
+//   k_tile_masks   per (I,J) tile: 64 row masks (bit j of lane i: pair i,j
+//                  within the cutoff) and the transposed column masks
+//   k_block_adj    per block: the tiles it takes part in, ordered by the
+//                  other block's index
+//   k_row_counts   per block: each lane's entry offset within its row for
+//                  every tile of its list, and the row length
+//   (exclusive scan of row lengths -> row pointers)
+//   k_pair_fill    per tile: the tensor of every masked pair, written to the
+//                  two rows at the offsets above (no atomics)
+//   k_excl_fold    adds (u - 1) lambda rr corrections of excluded pairs to
+//                  their two entries
+// Apply:
+//   k_pol_spmv<G>  G lanes per row, both dipole sets at once.
+#include <cstring>
+
+#include <rocprim/device/device_scan.hpp>
+
+#include "pair_mpole.h"
+#include "pmq_internal.h"
+
+namespace pmq {
+
+struct MaskArgs {
+    const int2 *tiles;
+    const int *ntiles;
+    const double *sx, *sy, *sz;
+    const int *sflag;
+    DevBox box;
+    double off2;
+    unsigned long long *mask; // 128 per tile: [0,64) row masks, [64,128) column masks
+};
+
+__global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
+{
+    const int lane = threadIdx.x;
+    const int ntiles = *a.ntiles;
+    __shared__ double jx[64], jy[64], jz[64];
+    __shared__ int jflag[64];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 IJ = a.tiles[t];
+        const int si = IJ.x * 64 + lane, sj = IJ.y * 64 + lane;
+        __syncthreads();
+        jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
+        jflag[lane] = a.sflag[sj];
+        const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
+        const int fi = a.sflag[si];
+        __syncthreads();
+        unsigned long long m = 0ull;
+        if (fi & 1) {
+            for (int jj = IJ.x == IJ.y ? lane + 1 : 0; jj < 64; jj++) {
+                if (!(jflag[jj] & 1)) continue;
+                double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
+                image<double>(a.box, xr, yr, zr);
+                if (xr * xr + yr * yr + zr * zr <= a.off2) m |= 1ull << jj;
+            }
+        }
+        // column mask of lane c: bit l set where lane l's row mask has bit c
+        unsigned long long col = 0ull;
+        for (int c = 0; c < 64; c++) {
+            const unsigned long long b = __ballot((m >> c) & 1ull);
+            if (lane == c) col = b;
+        }
+        a.mask[(size_t)t * 128 + lane] = m;
+        a.mask[(size_t)t * 128 + 64 + lane] = col;
+    }
+}
+
+// tiles of block I in the order of the other block J (one wave per block)
+__global__ __launch_bounds__(64) void k_block_adj(int nblk, const int *__restrict__ flag, const int *__restrict__ scan,
+                                                  int *__restrict__ adj, int *__restrict__ adjn)
+{
+    const int I = blockIdx.x, lane = threadIdx.x;
+    int count = 0;
+    for (int base = 0; base < nblk; base += 64) {
+        const int J = base + lane;
+        int f = 0;
+        long long tri = 0;
+        if (J < nblk) {
+            const long long lo = min(I, J), hi = max(I, J);
+            tri = lo * nblk - lo * (lo - 1) / 2 + (hi - lo);
+            f = flag[tri];
+        }
+        const unsigned long long b = __ballot(f != 0);
+        const int pos = count + __popcll(b & ((1ull << lane) - 1ull));
+        if (f) adj[(size_t)I * nblk + pos] = scan[tri];
+        count += __popcll(b);
+    }
+    if (lane == 0) adjn[I] = count;
+}
+
+// the mask of lane `lane` of block I for tile t, in the orientation of I
+__device__ __forceinline__ unsigned long long side_mask(const unsigned long long *mask, int t, int2 IJ, int I,
+                                                         int lane)
+{
+    const unsigned long long r = mask[(size_t)t * 128 + lane], c = mask[(size_t)t * 128 + 64 + lane];
+    if (IJ.x == IJ.y) return r | c;
+    return IJ.x == I ? r : c;
+}
+
+__global__ __launch_bounds__(64) void k_row_counts(int nblk, const int2 *__restrict__ tiles,
+                                                   const unsigned long long *__restrict__ mask,
+                                                   const int *__restrict__ adj, const int *__restrict__ adjn,
+                                                   int *__restrict__ soff, int *__restrict__ rowcnt)
+{
+    const int I = blockIdx.x, lane = threadIdx.x;
+    const int na = adjn[I];
+    int run = 0;
+    for (int s = 0; s < na; s++) {
+        const int t = adj[(size_t)I * nblk + s];
+        const int2 IJ = tiles[t];
+        const unsigned long long m = side_mask(mask, t, IJ, I, lane);
+        if (IJ.x == IJ.y) {
+            soff[(size_t)t * 128 + lane] = run;
+            soff[(size_t)t * 128 + 64 + lane] = run;
+        } else {
+            soff[(size_t)t * 128 + (IJ.x == I ? 0 : 64) + lane] = run;
+        }
+        run += __popcll(m);
+    }
+    rowcnt[I * 64 + lane] = run;
+    if (I == 0 && lane == 0) rowcnt[nblk * 64] = 0; // the scan's last element becomes the entry total
+}
+
+struct FillArgs {
+    const int2 *tiles;
+    const int *ntiles;
+    const double *sx, *sy, *sz, *spol;
+    DevBox box;
+    double aewald;
+    const unsigned long long *mask;
+    const int *soff, *rowptr;
+    int npad;
+    size_t cap;
+    int *col;
+    double *val; // 6 x cap: xx yy zz xy xz yz
+};
+
+__device__ __forceinline__ void store_T(double *val, size_t cap, size_t e, const double *R, double xr, double yr,
+                                        double zr)
+{
+    val[e] = R[2] * xr * xr - R[1];
+    val[cap + e] = R[2] * yr * yr - R[1];
+    val[2 * cap + e] = R[2] * zr * zr - R[1];
+    val[3 * cap + e] = R[2] * xr * yr;
+    val[4 * cap + e] = R[2] * xr * zr;
+    val[5 * cap + e] = R[2] * yr * zr;
+}
+
+template <bool EW>
+__global__ __launch_bounds__(64) void k_pair_fill(FillArgs a)
+{
+    const int lane = threadIdx.x;
+    const int np = a.npad;
+    const int ntiles = *a.ntiles;
+    __shared__ double jx[64], jy[64], jz[64], jpd[64], jth[64];
+    __shared__ unsigned long long jmask[64];
+    __shared__ int jbase[64];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 IJ = a.tiles[t];
+        const bool diag = IJ.x == IJ.y;
+        const int si = IJ.x * 64 + lane, sj = IJ.y * 64 + lane;
+        const unsigned long long rm = a.mask[(size_t)t * 128 + lane], cm = a.mask[(size_t)t * 128 + 64 + lane];
+        __syncthreads();
+        jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
+        jth[lane] = a.spol[np + sj];
+        jpd[lane] = a.spol[2 * np + sj];
+        // lane as a j atom: its column-side mask and entry base
+        jmask[lane] = diag ? (rm | cm) : cm;
+        jbase[lane] = a.rowptr[sj] + a.soff[(size_t)t * 128 + 64 + lane];
+        const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
+        const double thi = a.spol[np + si], pdi = a.spol[2 * np + si];
+        const unsigned long long mi = diag ? (rm | cm) : rm;
+        const int ibase = a.rowptr[si] + a.soff[(size_t)t * 128 + lane];
+        __syncthreads();
+        unsigned long long m = rm;
+        while (m) {
+            const int jj = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
+            image<double>(a.box, xr, yr, zr);
+            const double r2 = xr * xr + yr * yr + zr * zr;
+            double lam[6], R[6];
+            thole_lambda<double>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
+            pol_series<EW>(r2, a.aewald, lam, R);
+            const size_t ei = (size_t)ibase + __popcll(mi & ((1ull << jj) - 1ull));
+            const size_t ej = (size_t)jbase[jj] + __popcll(jmask[jj] & ((1ull << lane) - 1ull));
+            a.col[ei] = IJ.y * 64 + jj;
+            a.col[ej] = si;
+            store_T(a.val, a.cap, ei, R, xr, yr, zr);
+            store_T(a.val, a.cap, ej, R, xr, yr, zr);
+        }
+    }
+}
+
+struct FoldArgs {
+    const ExclPair *excl;
+    int nexcl;
+    const int *iperm;
+    const double *sx, *sy, *sz, *spol;
+    DevBox box;
+    int npad;
+    const int *rowptr, *col;
+    size_t cap;
+    double *val;
+};
+
+__device__ __forceinline__ void add_T(double *val, size_t cap, size_t e, const double *R, double xr, double yr,
+                                      double zr)
+{
+    val[e] += R[2] * xr * xr - R[1];
+    val[cap + e] += R[2] * yr * yr - R[1];
+    val[2 * cap + e] += R[2] * zr * zr - R[1];
+    val[3 * cap + e] += R[2] * xr * yr;
+    val[4 * cap + e] += R[2] * xr * zr;
+    val[5 * cap + e] += R[2] * yr * zr;
+}
+
+// excluded pairs present in the matrix get (u - 1) lambda rr added to their
+// two entries; each entry belongs to one excluded pair, so no atomics
+__global__ __launch_bounds__(64) void k_excl_fold(FoldArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nexcl) return;
+    const ExclPair q = a.excl[t];
+    if (q.u == 1.0) return;
+    const int np = a.npad;
+    const int si = a.iperm[q.i], sk = a.iperm[q.k];
+    int ei = -1, ek = -1;
+    for (int e = a.rowptr[si]; e < a.rowptr[si + 1]; e++)
+        if (a.col[e] == sk) { ei = e; break; }
+    if (ei < 0) return; // beyond the cutoff: not in the matrix
+    for (int e = a.rowptr[sk]; e < a.rowptr[sk + 1]; e++)
+        if (a.col[e] == si) { ek = e; break; }
+    double xr = a.sx[sk] - a.sx[si], yr = a.sy[sk] - a.sy[si], zr = a.sz[sk] - a.sz[si];
+    image<double>(a.box, xr, yr, zr);
+    const double r2 = xr * xr + yr * yr + zr * zr;
+    double lam[6], R[6];
+    thole_lambda<double>(sqrt(r2), a.spol[2 * np + si], a.spol[2 * np + sk], a.spol[np + si], a.spol[np + sk], lam);
+    corr_series<double>(r2, q.u, lam, R);
+    add_T(a.val, a.cap, (size_t)ei, R, xr, yr, zr);
+    if (ek >= 0) add_T(a.val, a.cap, (size_t)ek, R, xr, yr, zr);
+}
+
+struct SpmvArgs {
+    int npad;
+    const int *rowptr, *col;
+    size_t cap;
+    const double *val;
+    const double *src; // 2 x 3 x npad
+    double *dst;       // 2 x 3 x npad (overwritten)
+};
+
+// dst = T src for both dipole sets; G lanes per row
+template <int G>
+__global__ __launch_bounds__(256) void k_pol_spmv(SpmvArgs a)
+{
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = tid / G, g = tid % G;
+    const int np = a.npad;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (row < np) {
+        const int e1 = a.rowptr[row + 1];
+        for (int e = a.rowptr[row] + g; e < e1; e += G) {
+            const int k = a.col[e];
+            const double txx = a.val[e], tyy = a.val[a.cap + e], tzz = a.val[2 * a.cap + e];
+            const double txy = a.val[3 * a.cap + e], txz = a.val[4 * a.cap + e], tyz = a.val[5 * a.cap + e];
+#pragma unroll
+            for (int set = 0; set < 2; set++) {
+                const double *u = a.src + (size_t)set * 3 * np;
+                const double ux = u[k], uy = u[np + k], uz = u[2 * np + k];
+                acc[3 * set + 0] += txx * ux + txy * uy + txz * uz;
+                acc[3 * set + 1] += txy * ux + tyy * uy + tyz * uz;
+                acc[3 * set + 2] += txz * ux + tyz * uy + tzz * uz;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++)
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+    if (row < np && g < 6) {
+        double v = acc[0];
+#pragma unroll
+        for (int c = 1; c < 6; c++)
+            if (g == c) v = acc[c];
+        a.dst[(size_t)g * np + row] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+int pair_matrix_build(pmq_ctx *c, double off2)
+{
+    const int nblk = c->nblk, np = c->npad;
+    hipStream_t st = c->stream;
+    // tile count of this evaluation
+    int ntiles = 0;
+    PMQ_HIP(hipMemcpyAsync(&ntiles, c->ntiles_d.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    PMQ_HIP(hipStreamSynchronize(st));
+    const size_t nt = (size_t)std::max(ntiles, 1);
+    PMQ_HIP(c->tmask.alloc(128 * nt));
+    PMQ_HIP(c->soff.alloc(128 * nt));
+    PMQ_HIP(c->adj.alloc((size_t)nblk * nblk));
+    PMQ_HIP(c->adjn.alloc(nblk));
+    PMQ_HIP(c->rowcnt.alloc(np + 1));
+    PMQ_HIP(c->rowptr.alloc(np + 1));
+    const int tgrid = (int)std::max<size_t>(1, std::min<size_t>(nt, 16384));
+
+    MaskArgs ma;
+    ma.tiles = c->tiles.p; ma.ntiles = c->ntiles_d.p;
+    ma.sx = c->sx.p; ma.sy = c->sy.p; ma.sz = c->sz.p; ma.sflag = c->sflag.p;
+    ma.box = c->box; ma.off2 = off2; ma.mask = c->tmask.p;
+    hipLaunchKernelGGL(k_tile_masks, dim3(tgrid), dim3(64), 0, st, ma);
+    hipLaunchKernelGGL(k_block_adj, dim3(nblk), dim3(64), 0, st, nblk, c->tileflag.p, c->tilescan.p, c->adj.p,
+                       c->adjn.p);
+    hipLaunchKernelGGL(k_row_counts, dim3(nblk), dim3(64), 0, st, nblk, c->tiles.p, c->tmask.p, c->adj.p, c->adjn.p,
+                       c->soff.p, c->rowcnt.p);
+    PMQ_HIP(hipGetLastError());
+    size_t bytes = 0;
+    PMQ_HIP(rocprim::exclusive_scan(nullptr, bytes, c->rowcnt.p, c->rowptr.p, 0, (size_t)np + 1,
+                                    rocprim::plus<int>(), st));
+    PMQ_HIP(c->pscan_tmp.alloc(bytes));
+    PMQ_HIP(rocprim::exclusive_scan(c->pscan_tmp.p, bytes, c->rowcnt.p, c->rowptr.p, 0, (size_t)np + 1,
+                                    rocprim::plus<int>(), st));
+    int nnz = 0;
+    PMQ_HIP(hipMemcpyAsync(&nnz, c->rowptr.p + np, sizeof(int), hipMemcpyDeviceToHost, st));
+    PMQ_HIP(hipStreamSynchronize(st));
+    if ((size_t)nnz > c->tcap) {
+        c->tcap = (size_t)(nnz * 1.25) + 64;
+        PMQ_HIP(c->tcol.alloc(c->tcap));
+        PMQ_HIP(c->tval.alloc(6 * c->tcap));
+    }
+    c->nnz = nnz;
+
+    FillArgs fa;
+    fa.tiles = c->tiles.p; fa.ntiles = c->ntiles_d.p;
+    fa.sx = c->sx.p; fa.sy = c->sy.p; fa.sz = c->sz.p; fa.spol = c->spol.p;
+    fa.box = c->box;
+    fa.aewald = c->prm.use_ewald ? c->prm.aewald : 0.0;
+    fa.mask = c->tmask.p; fa.soff = c->soff.p; fa.rowptr = c->rowptr.p;
+    fa.npad = np; fa.cap = c->tcap; fa.col = c->tcol.p; fa.val = c->tval.p;
+    {
+        ProfScope ps_(c, "pol_pair_fill");
+        if (fa.aewald > 0)
+            hipLaunchKernelGGL(k_pair_fill<true>, dim3(tgrid), dim3(64), 0, st, fa);
+        else
+            hipLaunchKernelGGL(k_pair_fill<false>, dim3(tgrid), dim3(64), 0, st, fa);
+    }
+    const int nexcl = (int)c->h_excl.size();
+    if (nexcl) {
+        FoldArgs fo;
+        fo.excl = c->excl.p; fo.nexcl = nexcl; fo.iperm = c->iperm.p;
+        fo.sx = c->sx.p; fo.sy = c->sy.p; fo.sz = c->sz.p; fo.spol = c->spol.p;
+        fo.box = c->box; fo.npad = np; fo.rowptr = c->rowptr.p; fo.col = c->tcol.p; fo.cap = c->tcap;
+        fo.val = c->tval.p;
+        hipLaunchKernelGGL(k_excl_fold, dim3((nexcl + 63) / 64), dim3(64), 0, st, fo);
+    }
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst)
+{
+    constexpr int G = 8;
+    SpmvArgs a;
+    a.npad = c->npad; a.rowptr = c->rowptr.p; a.col = c->tcol.p; a.cap = c->tcap; a.val = c->tval.p;
+    a.src = src; a.dst = dst;
+    const size_t lanes = (size_t)c->npad * G;
+    ProfScope ps_(c, "pol_mutual_spmv");
+    hipLaunchKernelGGL(k_pol_spmv<G>, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, c->stream, a);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+} // namespace pmq

@@ -10,9 +10,14 @@
 // (up to third order) are then interpolated back at every site.
 //
 // Grids: real grid index (g3*K1 + g2)*K0 + g1 (g1 fastest), complex half
-// grid ((K0/2+1) along g1).  One atom per thread; spreading uses FP64
-// atomics on the grid (it sits in L2: 48^3 doubles = 0.9 MB).
+// grid ((K0/2+1) along g1).  Spreading is owner-computes over grid bricks
+// (LDS accumulation, no global atomics); gathering uses one lane group per
+// atom.
 #include <rocfft/rocfft.h>
+
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <cmath>
 #include <cstring>
@@ -56,6 +61,10 @@ static void destroy_plans(pmq_ctx *c)
 
 void pme_release(pmq_ctx *c) { destroy_plans(c); }
 
+// brick partition of one grid axis for spreading: bricks at least
+// max(4, order - 1) points wide
+static int brick_count(int K, int order) { return std::max(1, K / std::max(4, order - 1)); }
+
 int pme_setup(pmq_ctx *c)
 {
     const pmq_params &p = c->prm;
@@ -73,9 +82,13 @@ int pme_setup(pmq_ctx *c)
     for (int a = 0; a < 3 && same_box; a++)
         for (int b = 0; b < 3; b++) same_box = same_box && c->pme_dbox.recip[a][b] == c->box.recip[a][b];
     PMQ_HIP(c->pme_frac.alloc(40 * (size_t)c->npad)); // follows the atom count
+    PMQ_HIP(c->pme_keys.alloc(c->n));
+    PMQ_HIP(c->pme_keys_sorted.alloc(c->n));
     if (same_box && c->plan_f1) return PMQ_OK;
     c->K[0] = K[0]; c->K[1] = K[1]; c->K[2] = K[2];
     c->bsorder = p.bsorder;
+    for (int a = 0; a < 3; a++) c->pme_nb[a] = brick_count(K[a], p.bsorder);
+    PMQ_HIP(c->pme_bstart.alloc((size_t)c->pme_nb[0] * c->pme_nb[1] * c->pme_nb[2] + 1));
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) c->pmeA[a][b] = K[a] * c->box.recip[a][b];
     // B-spline moduli, zeros replaced by the mean of their neighbours
@@ -238,70 +251,184 @@ __device__ __forceinline__ void frac_mpole(const PmeGeom &g, const double *m, do
     fm[7] = 2 * F[0][1]; fm[8] = 2 * F[0][2]; fm[9] = 2 * F[1][2];
 }
 
-struct SpreadArgs {
-    int n, npad, stride;
+// Spreading is owner-computes over bricks of the grid.  Atoms are binned by
+// the brick holding the last point of their stencil (floor(u) on each axis);
+// every brick is at least order-1 points wide, so the stencil of an atom
+// homed in brick h covers only bricks h-1 and h on each axis, and brick b
+// receives exactly the atoms homed in b and b+1.  Each grid point is summed
+// by one lane and stored once: no atomics and no zeroing of the grid.
+struct BrickArgs {
+    int n, sbits, nset;
+    int nb[3];
     PmeGeom g;
+    const unsigned long long *keys; // (home brick << sbits) | sorted atom, ascending
+    const int *bstart;              // nbricks + 1
+    int npad;
     const double *sx, *sy, *sz;
     const double *src;    // sorted SoA sources: 10 multipole comps (mode 0) or 3 dipole comps x nset (mode 1)
     const double *charge; // mode 0: total monopole per sorted atom (charge penetration), or null
     double *grid;         // nset grids
-    int nset;
     size_t gstride;
 };
 
-// mode 0: permanent multipoles (one grid); mode 1: dipole sets (nset grids)
-template <int ORD, int MODE>
-__global__ __launch_bounds__(256) void k_pme_spread(SpreadArgs a)
+__device__ __host__ __forceinline__ int brick_lo(int b, int K, int nb) { return (int)(((long long)b * K) / nb); }
+
+// home brick of every atom: the brick of floor(u) on each axis
+__global__ void k_pme_brick_keys(int n, int sbits, int nb0, int nb1, int nb2, PmeGeom g, const double *sx,
+                                 const double *sy, const double *sz, unsigned long long *keys)
 {
-    constexpr int G = PmeLanes<ORD>::G;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const double p[3] = {sx[s], sy[s], sz[s]};
+    const int K[3] = {g.K0, g.K1, g.K2}, nb[3] = {nb0, nb1, nb2};
+    int hb[3];
+    for (int a = 0; a < 3; a++) {
+        const double u = g.A[a][0] * p[0] + g.A[a][1] * p[1] + g.A[a][2] * p[2];
+        int h = (int)floor(u) % K[a];
+        if (h < 0) h += K[a];
+        int b = (int)(((long long)h * nb[a]) / K[a]);
+        while (b + 1 < nb[a] && brick_lo(b + 1, K[a], nb[a]) <= h) b++;
+        while (b > 0 && brick_lo(b, K[a], nb[a]) > h) b--;
+        hb[a] = b;
+    }
+    const unsigned long long brick = ((unsigned long long)hb[2] * nb1 + hb[1]) * nb0 + hb[0];
+    keys[s] = (brick << sbits) | (unsigned long long)s;
+}
+
+// first sorted position of every brick (lower bound), bstart[nbricks] = n
+__global__ void k_pme_brick_start(int n, int nbricks, int sbits, const unsigned long long *keys, int *bstart)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nbricks) return;
+    const unsigned long long key = (unsigned long long)b << sbits;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    bstart[b] = lo;
+}
+
+// One wave per brick; lane = grid point (bricks of more than 64 points are
+// done in chunks of 64).  The atoms homed in the 8 candidate bricks are
+// staged in LDS 64 at a time (one lane each: stencil start, B-spline
+// weights, fractional coefficients); every lane then sums the atoms whose
+// stencil holds its point, in key order, so the grid is bit-reproducible.
+template <int ORD, int MODE>
+__global__ __launch_bounds__(64) void k_pme_spread_brick(BrickArgs a)
+{
     constexpr int ND = MODE == 0 ? 2 : 1;
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int slot = tid / G, l = tid % G;
-    if (slot >= a.n || l >= ORD * ORD) return; // no cross-lane work in this kernel
-    const int s = (int)(((long long)slot * a.stride) % a.n);
-    const int ky = l % ORD, kz = l / ORD;
-    double w[3];
-    int i0[3];
-    atom_frac(a.g, a.sx[s], a.sy[s], a.sz[s], w, i0, ORD);
-    double tx[PMQ_MAX_ORDER][4], tyf[PMQ_MAX_ORDER][4], tzf[PMQ_MAX_ORDER][4];
-    bspline_weights<double, ND>(w[0], ORD, tx);
-    bspline_weights<double, ND>(w[1], ORD, tyf);
-    bspline_weights<double, ND>(w[2], ORD, tzf);
-    double ty[ND + 1], tz[ND + 1];
-    pick<ORD, ND>(tyf, ky, ty);
-    pick<ORD, ND>(tzf, kz, tz);
-    int gy = i0[1] + ky, gz = i0[2] + kz;
-    if (gy >= a.g.K1) gy -= a.g.K1;
-    if (gz >= a.g.K2) gz -= a.g.K2;
-    for (int set = 0; set < (MODE == 0 ? 1 : a.nset); set++) {
-        double c0, c1, c2 = 0.0;
-        if (MODE == 0) {
-            double m[10], fm[10];
+    constexpr int NC = MODE == 0 ? 10 : 6;
+    constexpr int NS = MODE == 0 ? 1 : 2;
+    __shared__ double th[3][ORD][ND + 1][64];
+    __shared__ double cf[NC][64];
+    __shared__ int si0[3][64];
+    const int lane = threadIdx.x;
+    const int K0 = a.g.K0, K1 = a.g.K1, K2 = a.g.K2;
+    const int b = blockIdx.x;
+    const int bx = b % a.nb[0], by = (b / a.nb[0]) % a.nb[1], bz = b / (a.nb[0] * a.nb[1]);
+    const int lo0 = brick_lo(bx, K0, a.nb[0]), lo1 = brick_lo(by, K1, a.nb[1]), lo2 = brick_lo(bz, K2, a.nb[2]);
+    const int s0 = brick_lo(bx + 1, K0, a.nb[0]) - lo0, s1 = brick_lo(by + 1, K1, a.nb[1]) - lo1,
+              s2 = brick_lo(bz + 1, K2, a.nb[2]) - lo2;
+    const int bsz = s0 * s1 * s2;
+    const int nset = MODE == 0 ? 1 : a.nset; // 1 or 2
+    const unsigned long long smask = (1ull << a.sbits) - 1ull;
+    // the atoms homed in bricks b + (0|1, 0|1, 0|1), each brick once
+    int hs[8], hc[8], ntot = 0;
 #pragma unroll
-            for (int c = 0; c < 10; c++) m[c] = a.src[c * a.npad + s];
-            if (a.charge) m[0] = a.charge[s];
-            frac_mpole(a.g, m, fm);
-            c0 = fm[0] * ty[0] * tz[0] + fm[2] * ty[1] * tz[0] + fm[3] * ty[0] * tz[1] + fm[5] * ty[2] * tz[0]
-               + fm[6] * ty[0] * tz[2] + fm[9] * ty[1] * tz[1];
-            c1 = fm[1] * ty[0] * tz[0] + fm[7] * ty[1] * tz[0] + fm[8] * ty[0] * tz[1];
-            c2 = fm[4] * ty[0] * tz[0];
-        } else {
-            const double *d = a.src + (size_t)set * 3 * a.npad;
-            const double dx = d[s], dy = d[a.npad + s], dz = d[2 * a.npad + s];
-            const double f1 = a.g.A[0][0] * dx + a.g.A[0][1] * dy + a.g.A[0][2] * dz;
-            const double f2 = a.g.A[1][0] * dx + a.g.A[1][1] * dy + a.g.A[1][2] * dz;
-            const double f3 = a.g.A[2][0] * dx + a.g.A[2][1] * dy + a.g.A[2][2] * dz;
-            c0 = f2 * ty[1] * tz[0] + f3 * ty[0] * tz[1];
-            c1 = f1 * ty[0] * tz[0];
-        }
-        double *row = a.grid + set * a.gstride + ((size_t)gz * a.g.K1 + gy) * a.g.K0;
+    for (int h = 0; h < 8; h++) {
+        const int dx = h & 1, dy = (h >> 1) & 1, dz = h >> 2;
+        hs[h] = 0;
+        hc[h] = 0;
+        if ((dx && a.nb[0] == 1) || (dy && a.nb[1] == 1) || (dz && a.nb[2] == 1)) continue;
+        const int hx = (bx + dx) % a.nb[0], hy = (by + dy) % a.nb[1], hz = (bz + dz) % a.nb[2];
+        const int hb = (hz * a.nb[1] + hy) * a.nb[0] + hx;
+        hs[h] = a.bstart[hb];
+        hc[h] = a.bstart[hb + 1] - hs[h];
+        ntot += hc[h];
+    }
+    for (int pbase = 0; pbase < bsz; pbase += 64) {
+        const int p = pbase + lane;
+        const bool pon = p < bsz;
+        const int lx = p % s0, ly = (p / s0) % s1, lz = p / (s0 * s1);
+        const int gx = lo0 + lx, gy = lo1 + ly, gz = lo2 + lz;
+        double acc[NS];
 #pragma unroll
-        for (int kx = 0; kx < ORD; kx++) {
-            int gx = i0[0] + kx;
-            if (gx >= a.g.K0) gx -= a.g.K0;
-            const double v = c0 * tx[kx][0] + c1 * tx[kx][1] + (MODE == 0 ? c2 * tx[kx][2] : 0.0);
-            atomicAdd(row + gx, v);
+        for (int q = 0; q < NS; q++) acc[q] = 0.0;
+        for (int base = 0; base < ntot; base += 64) {
+            const int cnt = min(64, ntot - base);
+            __syncthreads();
+            if (lane < cnt) {
+                // position of candidate base + lane in the concatenated home-brick ranges
+                int e = 0, off = 0;
+                const int idx = base + lane;
+#pragma unroll
+                for (int h = 0; h < 8; h++) {
+                    if (idx >= off && idx < off + hc[h]) e = hs[h] + idx - off;
+                    off += hc[h];
+                }
+                const int s = (int)(a.keys[e] & smask);
+                double w[3];
+                int i0[3];
+                atom_frac(a.g, a.sx[s], a.sy[s], a.sz[s], w, i0, ORD);
+#pragma unroll
+                for (int ax = 0; ax < 3; ax++) {
+                    double t[PMQ_MAX_ORDER][4];
+                    bspline_weights<double, ND>(w[ax], ORD, t);
+#pragma unroll
+                    for (int k = 0; k < ORD; k++)
+#pragma unroll
+                        for (int d = 0; d <= ND; d++) th[ax][k][d][lane] = t[k][d];
+                    si0[ax][lane] = i0[ax];
+                }
+                if (MODE == 0) {
+                    double m[10], fm[10];
+#pragma unroll
+                    for (int c = 0; c < 10; c++) m[c] = a.src[c * a.npad + s];
+                    if (a.charge) m[0] = a.charge[s];
+                    frac_mpole(a.g, m, fm);
+#pragma unroll
+                    for (int c = 0; c < 10; c++) cf[c][lane] = fm[c];
+                } else {
+                    for (int set = 0; set < nset; set++) {
+                        const double *d = a.src + (size_t)set * 3 * a.npad;
+                        const double ux = d[s], uy = d[a.npad + s], uz = d[2 * a.npad + s];
+#pragma unroll
+                        for (int r = 0; r < 3; r++)
+                            cf[3 * set + r][lane] = a.g.A[r][0] * ux + a.g.A[r][1] * uy + a.g.A[r][2] * uz;
+                    }
+                }
+            }
+            __syncthreads();
+            if (!pon) continue;
+            for (int j = 0; j < cnt; j++) {
+                int kx = gx - si0[0][j], ky = gy - si0[1][j], kz = gz - si0[2][j];
+                if (kx < 0) kx += K0;
+                if (ky < 0) ky += K1;
+                if (kz < 0) kz += K2;
+                if (kx >= ORD || ky >= ORD || kz >= ORD) continue;
+                if (MODE == 0) {
+                    const double x0 = th[0][kx][0][j], x1 = th[0][kx][1][j], x2 = th[0][kx][2][j];
+                    const double y0 = th[1][ky][0][j], y1 = th[1][ky][1][j], y2 = th[1][ky][2][j];
+                    const double z0 = th[2][kz][0][j], z1 = th[2][kz][1][j], z2 = th[2][kz][2][j];
+                    acc[0] += cf[0][j] * x0 * y0 * z0 + cf[1][j] * x1 * y0 * z0 + cf[2][j] * x0 * y1 * z0
+                            + cf[3][j] * x0 * y0 * z1 + cf[4][j] * x2 * y0 * z0 + cf[5][j] * x0 * y2 * z0
+                            + cf[6][j] * x0 * y0 * z2 + cf[7][j] * x1 * y1 * z0 + cf[8][j] * x1 * y0 * z1
+                            + cf[9][j] * x0 * y1 * z1;
+                } else {
+                    const double x0 = th[0][kx][0][j], x1 = th[0][kx][1][j];
+                    const double y0 = th[1][ky][0][j], y1 = th[1][ky][1][j];
+                    const double z0 = th[2][kz][0][j], z1 = th[2][kz][1][j];
+                    const double w100 = x1 * y0 * z0, w010 = x0 * y1 * z0, w001 = x0 * y0 * z1;
+#pragma unroll
+                    for (int q = 0; q < NS; q++)
+                        acc[q] += cf[3 * q][j] * w100 + cf[3 * q + 1][j] * w010 + cf[3 * q + 2][j] * w001;
+                }
+            }
         }
+        if (pon)
+            for (int q = 0; q < nset; q++)
+                a.grid[q * a.gstride + ((size_t)gz * K1 + gy) * K0 + gx] = acc[q];
     }
 }
 
@@ -692,22 +819,58 @@ static dim3 lanes_grid(int n, int order) { return dim3((unsigned)(((size_t)n * (
     default: hipLaunchKernelGGL((KERNEL<6 __VA_ARGS__>), lanes_grid(c->n, 6), dim3(256), 0, c->stream, args); break; \
     }
 
+// bin the atoms by home brick (once per evaluation; positions are fixed
+// for all spreads of the evaluation)
+static int pme_bin(pmq_ctx *c)
+{
+    const int n = c->n;
+    PmeGeom g = geom(c);
+    int sbits = 1;
+    while ((1 << sbits) < c->npad) sbits++;
+    c->pme_sbits = sbits;
+    const int nbricks = c->pme_nb[0] * c->pme_nb[1] * c->pme_nb[2];
+    int bbits = 1;
+    while ((1ll << bbits) <= nbricks) bbits++;
+    hipLaunchKernelGGL(k_pme_brick_keys, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, sbits, c->pme_nb[0],
+                       c->pme_nb[1], c->pme_nb[2], g, c->sx.p, c->sy.p, c->sz.p, c->pme_keys.p);
+    size_t bytes = c->scratch_sort_bytes;
+    PMQ_HIP(rocprim::radix_sort_keys(c->scratch_sort.p, bytes, c->pme_keys.p, c->pme_keys_sorted.p, (size_t)n, 0,
+                                     sbits + bbits, c->stream));
+    hipLaunchKernelGGL(k_pme_brick_start, dim3((nbricks + 1 + 255) / 256), dim3(256), 0, c->stream, n, nbricks, sbits,
+                       c->pme_keys_sorted.p, c->pme_bstart.p);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
 static int spread(pmq_ctx *c, int mode, const double *src, const double *charge, int nset)
 {
-    SpreadArgs args;
-    args.n = c->n; args.npad = c->npad; args.stride = atom_stride(c->n); args.g = geom(c);
+    BrickArgs args;
+    args.n = c->n; args.sbits = c->pme_sbits; args.nset = nset;
+    for (int a = 0; a < 3; a++) args.nb[a] = c->pme_nb[a];
+    args.g = geom(c);
+    args.keys = c->pme_keys_sorted.p; args.bstart = c->pme_bstart.p;
+    args.npad = c->npad;
     args.sx = c->sx.p; args.sy = c->sy.p; args.sz = c->sz.p;
     args.src = src;
     args.charge = charge;
     args.grid = c->qgrid.p;
-    args.nset = nset;
     args.gstride = grid_size(c);
-    ProfScope ps_(c, "pme_spread");
-    if (mode == 0) {
-        PMQ_ORDER_DISPATCH(k_pme_spread, , 0)
-    } else {
-        PMQ_ORDER_DISPATCH(k_pme_spread, , 1)
+    if (mode == 1 && nset != 2) {
+        c->err = "PME: dipole spreading takes two sets";
+        return PMQ_ERR_ARG;
     }
+    const dim3 grid((unsigned)(c->pme_nb[0] * c->pme_nb[1] * c->pme_nb[2]));
+    ProfScope ps_(c, "pme_spread");
+#define PMQ_SPREAD(ORD, MODE) hipLaunchKernelGGL((k_pme_spread_brick<ORD, MODE>), grid, dim3(64), 0, c->stream, args)
+    switch (c->bsorder * 2 + (mode ? 1 : 0)) {
+    case 8: PMQ_SPREAD(4, 0); break;
+    case 9: PMQ_SPREAD(4, 1); break;
+    case 10: PMQ_SPREAD(5, 0); break;
+    case 11: PMQ_SPREAD(5, 1); break;
+    case 12: PMQ_SPREAD(6, 0); break;
+    default: PMQ_SPREAD(6, 1); break;
+    }
+#undef PMQ_SPREAD
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }
@@ -730,7 +893,7 @@ static int gather_frac(pmq_ctx *c, int nset)
 int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar)
 {
     int rc;
-    if ((rc = launch_zero(c, c->qgrid.p, grid_size(c)))) return rc;
+    if ((rc = pme_bin(c))) return rc;
     const double *charge = c->prm.pentype != PMQ_PEN_NONE ? c->scp.p + 2 * (size_t)c->npad : nullptr;
     if ((rc = spread(c, 0, c->smp.p, charge, 1))) return rc;
     if ((rc = pme_potential(c, 1))) return rc;
@@ -760,7 +923,6 @@ int pme_dipole_field(pmq_ctx *c, const double *dip, double *field)
     // dip and field: two sets of 3 x npad, contiguous
     int rc;
     const size_t gs = grid_size(c);
-    if ((rc = launch_zero(c, c->qgrid.p, 2 * gs))) return rc;
     if ((rc = spread(c, 1, dip, nullptr, 2))) return rc;
     if ((rc = pme_potential(c, 2))) return rc;
     GatherFieldArgs args;
@@ -783,7 +945,6 @@ int pme_polar_grad(pmq_ctx *c)
 {
     int rc;
     const size_t gs = grid_size(c);
-    if ((rc = launch_zero(c, c->qgrid.p, 2 * gs))) return rc;
     if ((rc = spread(c, 1, c->uind.p, nullptr, 2))) return rc;
     if ((rc = pme_potential(c, 2))) return rc;
     if ((rc = gather_frac(c, 2))) return rc;

@@ -19,7 +19,7 @@
 // Kernels (sorted atom order, 2 dipole sets: 0 = d, 1 = p):
 //   k_pol_field_tiles / _excl   permanent field (scale 1 in the tiles,
 //                               (s-1) lambda rr corrections for scaled pairs)
-//   k_pol_mutual_tiles / _excl  T mu for both sets
+//   (T mu for both sets is the pair-matrix product of k_pairs.hip)
 //   k_pcg_*                     preconditioned CG (diagonal preconditioner)
 //   k_pol_grad_tiles / _excl    real-space gradient and torques
 #include "pair_mpole.h"
@@ -47,29 +47,6 @@ struct PolArgs {
     double *dst;         // fields (2 x 3 x npad)
     double *dep, *tep;   // caller-order gradient and torque
 };
-
-// series of a polarization interaction at scale 1: Ewald + Thole, or Thole
-template <bool EW>
-__device__ __forceinline__ void pol_series(double r2, double aewald, const double *lam, double *R)
-{
-    double rr[6];
-    rr_series<double>(r2, 1.0, rr);
-    if (EW) {
-        double bn[6];
-        ewald_series<double>(r2, aewald, bn);
-        for (int q = 0; q < 6; q++) R[q] = bn[q] - (1.0 - lam[q]) * rr[q];
-    } else {
-        for (int q = 0; q < 6; q++) R[q] = lam[q] * rr[q];
-    }
-}
-
-// correction series (s - 1) lambda rr for a scaled pair
-__device__ __forceinline__ void corr_series(double r2, double s, const double *lam, double *R)
-{
-    double rr[6];
-    rr_series<double>(r2, 1.0, rr);
-    for (int q = 0; q < 6; q++) R[q] = (s - 1.0) * lam[q] * rr[q];
-}
 
 // field at i from multipole k and at k from multipole i
 __device__ __forceinline__ void mp_fields(const double *R, const Mp<double> &a, const Mp<double> &b, double xr, double yr,
@@ -205,97 +182,6 @@ __global__ __launch_bounds__(64) void k_pol_field_excl(PolArgs a)
         for (int c = 0; c < 3; c++) {
             atomicAdd(&d[c * np + si], Ei[c]);
             atomicAdd(&d[c * np + sk], Ek[c]);
-        }
-    }
-}
-
-// T mu for both dipole sets
-template <bool EW>
-__global__ __launch_bounds__(64) void k_pol_mutual_tiles(PolArgs a)
-{
-    TILE_PROLOGUE
-    __shared__ double jx[64], jy[64], jz[64], ju[6][64], jpd[64], jth[64], acc[6][64];
-    __shared__ int jflag[64];
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int2 IJ = a.tiles[t];
-        const int si = IJ.x * 64 + lane, sj = IJ.y * 64 + lane;
-        __syncthreads();
-        jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
-        for (int c = 0; c < 6; c++) ju[c][lane] = a.src[c * np + sj];
-        jth[lane] = a.spol[np + sj];
-        jpd[lane] = a.spol[2 * np + sj];
-        jflag[lane] = a.sflag[sj];
-        for (int c = 0; c < 6; c++) acc[c][lane] = 0.0;
-        const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
-        const int fi = a.sflag[si];
-        double ui[6];
-        for (int c = 0; c < 6; c++) ui[c] = a.src[c * np + si];
-        const double thi = a.spol[np + si], pdi = a.spol[2 * np + si];
-        __syncthreads();
-        unsigned long long mask = 0ull;
-        if (fi & 1) {
-            for (int jj = IJ.x == IJ.y ? lane + 1 : 0; jj < 64; jj++) {
-                if (!(jflag[jj] & 1)) continue;
-                double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
-                image<double>(a.box, xr, yr, zr);
-                if (xr * xr + yr * yr + zr * zr <= a.off2) mask |= 1ull << jj;
-            }
-        }
-        double fi6[6] = {0, 0, 0, 0, 0, 0};
-        while (mask) {
-            const int jj = __ffsll((long long)mask) - 1;
-            mask &= mask - 1;
-            double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
-            image<double>(a.box, xr, yr, zr);
-            const double r2 = xr * xr + yr * yr + zr * zr;
-            double lam[6], R[6];
-            thole_lambda<double>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
-            pol_series<EW>(r2, a.aewald, lam, R);
-            for (int set = 0; set < 2; set++) {
-                const double *u = ui + 3 * set;
-                const double uk0 = ju[3 * set][jj], uk1 = ju[3 * set + 1][jj], uk2 = ju[3 * set + 2][jj];
-                const double ukr = uk0 * xr + uk1 * yr + uk2 * zr;
-                const double uir = u[0] * xr + u[1] * yr + u[2] * zr;
-                fi6[3 * set + 0] += ukr * R[2] * xr - R[1] * uk0;
-                fi6[3 * set + 1] += ukr * R[2] * yr - R[1] * uk1;
-                fi6[3 * set + 2] += ukr * R[2] * zr - R[1] * uk2;
-                atomicAdd(&acc[3 * set + 0][jj], uir * R[2] * xr - R[1] * u[0]);
-                atomicAdd(&acc[3 * set + 1][jj], uir * R[2] * yr - R[1] * u[1]);
-                atomicAdd(&acc[3 * set + 2][jj], uir * R[2] * zr - R[1] * u[2]);
-            }
-        }
-        __syncthreads();
-        if (fi & 1)
-            for (int c = 0; c < 6; c++) atomicAdd(&a.dst[c * np + si], fi6[c]);
-        if (jflag[lane] & 1)
-            for (int c = 0; c < 6; c++) atomicAdd(&a.dst[c * np + sj], acc[c][lane]);
-    }
-}
-
-__global__ __launch_bounds__(64) void k_pol_mutual_excl(PolArgs a)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nexcl) return;
-    const ExclPair q = a.excl[t];
-    if (q.u == 1.0) return;
-    const int np = a.npad;
-    const int si = a.iperm[q.i], sk = a.iperm[q.k];
-    double xr = a.sx[sk] - a.sx[si], yr = a.sy[sk] - a.sy[si], zr = a.sz[sk] - a.sz[si];
-    image<double>(a.box, xr, yr, zr);
-    const double r2 = xr * xr + yr * yr + zr * zr;
-    if (r2 > a.off2) return;
-    double lam[6], R[6];
-    thole_lambda<double>(sqrt(r2), a.spol[2 * np + si], a.spol[2 * np + sk], a.spol[np + si], a.spol[np + sk], lam);
-    corr_series(r2, q.u, lam, R);
-    for (int set = 0; set < 2; set++) {
-        const double *u = a.src + (size_t)set * 3 * np;
-        double *d = a.dst + (size_t)set * 3 * np;
-        const double ui[3] = {u[si], u[np + si], u[2 * np + si]}, uk[3] = {u[sk], u[np + sk], u[2 * np + sk]};
-        const double r[3] = {xr, yr, zr};
-        const double uir = ui[0] * xr + ui[1] * yr + ui[2] * zr, ukr = uk[0] * xr + uk[1] * yr + uk[2] * zr;
-        for (int c = 0; c < 3; c++) {
-            atomicAdd(&d[c * np + si], ukr * R[2] * r[c] - R[1] * uk[c]);
-            atomicAdd(&d[c * np + sk], uir * R[2] * r[c] - R[1] * ui[c]);
         }
     }
 }
@@ -623,27 +509,12 @@ static PolArgs pol_args(pmq_ctx *c, double off2)
 
 static int tile_grid(pmq_ctx *c) { return std::max(1, std::min(c->ntiles_cap, 16384)); }
 
-static int mutual_field(pmq_ctx *c, double off2, const double *src, double *dst)
+// dst = T src (real space, from the pair matrix) + reciprocal and self field
+static int mutual_field(pmq_ctx *c, const double *src, double *dst)
 {
     int rc;
-    const size_t nv = 6 * (size_t)c->npad;
-    if ((rc = launch_zero(c, dst, nv))) return rc;
-    PolArgs a = pol_args(c, off2);
-    a.src = src;
-    a.dst = dst;
-    if (a.aewald > 0)
-        {
-            ProfScope ps_(c, "pol_mutual_tiles");
-            hipLaunchKernelGGL(k_pol_mutual_tiles<true>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
-        }
-    else
-        {
-            ProfScope ps_(c, "pol_mutual_tiles");
-            hipLaunchKernelGGL(k_pol_mutual_tiles<false>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
-        }
-    if (a.nexcl) hipLaunchKernelGGL(k_pol_mutual_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
-    PMQ_HIP(hipGetLastError());
-    if (a.aewald > 0 && (rc = pme_dipole_field(c, src, dst))) return rc;
+    if ((rc = pair_matrix_apply(c, src, dst))) return rc;
+    if (c->prm.use_ewald && (rc = pme_dipole_field(c, src, dst))) return rc;
     return PMQ_OK;
 }
 
@@ -669,7 +540,7 @@ int polar_prepare(pmq_ctx *c, double off2)
         }
     if (a.nexcl) hipLaunchKernelGGL(k_pol_field_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
-    return PMQ_OK;
+    return pair_matrix_build(c, off2);
 }
 
 int polar_solve(pmq_ctx *c, double off2, int *iters_out)
@@ -685,7 +556,7 @@ int polar_solve(pmq_ctx *c, double off2, int *iters_out)
     int rc;
     if ((rc = launch_zero(c, c->pcg.p, 8))) return rc;
     hipLaunchKernelGGL(k_pcg_guess, dim3(gb), dim3(tb), 0, c->stream, a);
-    if ((rc = mutual_field(c, off2, c->uind.p, c->pvec.p))) return rc;
+    if ((rc = mutual_field(c, c->uind.p, c->pvec.p))) return rc;
     hipLaunchKernelGGL(k_pcg_init, dim3(gb), dim3(tb), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     // count polarizable sites once (host copy of the parameters)
@@ -697,7 +568,7 @@ int polar_solve(pmq_ctx *c, double off2, int *iters_out)
     bool done = false;
     while (!done && it < c->prm.politer) {
         it++;
-        if ((rc = mutual_field(c, off2, c->pdir.p, c->pvec.p))) return rc;
+        if ((rc = mutual_field(c, c->pdir.p, c->pvec.p))) return rc;
         hipLaunchKernelGGL(k_pcg_a, dim3(gb), dim3(tb), 0, c->stream, a);
         hipLaunchKernelGGL(k_pcg_b, dim3(gb), dim3(tb), 0, c->stream, a);
         PMQ_HIP(hipGetLastError());

@@ -333,4 +333,28 @@ __device__ __forceinline__ void thole_lambda(R r, R pdi, R pdk, R ti, R tk, R *l
     lam[4] = R(1) - (R(1) + D + R(18.0 / 35.0) * D * D + R(9.0 / 35.0) * D * D * D) * e;
 }
 
+// series of a polarization interaction at scale 1: Ewald + Thole, or Thole
+template <bool EW, class R_ = double>
+__device__ __forceinline__ void pol_series(R_ r2, R_ aewald, const R_ *lam, R_ *R)
+{
+    R_ rr[6];
+    rr_series<R_>(r2, R_(1), rr);
+    if (EW) {
+        R_ bn[6];
+        ewald_series<R_>(r2, aewald, bn);
+        for (int q = 0; q < 6; q++) R[q] = bn[q] - (R_(1) - lam[q]) * rr[q];
+    } else {
+        for (int q = 0; q < 6; q++) R[q] = lam[q] * rr[q];
+    }
+}
+
+// correction series (s - 1) lambda rr for a scaled pair
+template <class R_ = double>
+__device__ __forceinline__ void corr_series(R_ r2, R_ s, const R_ *lam, R_ *R)
+{
+    R_ rr[6];
+    rr_series<R_>(r2, R_(1), rr);
+    for (int q = 0; q < 6; q++) R[q] = (s - R_(1)) * lam[q] * rr[q];
+}
+
 } // namespace pmq

@@ -212,6 +212,16 @@ struct pmq_ctx {
     pmq::DBuf<double> tep;                       // 3n torques of the polarization term
     pmq::DBuf<double> dep;                       // 3n gradient of ep
     bool have_uind = false;
+    // dipole-field operator T as a row-owned sparse matrix (k_pairs.hip)
+    pmq::DBuf<unsigned long long> tmask;         // 128 per tile: 64 row masks, 64 column masks
+    pmq::DBuf<int> soff;                         // 128 per tile: each lane's entry offset within its row
+    pmq::DBuf<int> adj, adjn;                    // per block: tiles it takes part in (nblk x nblk), count
+    pmq::DBuf<int> rowcnt, rowptr;               // npad + 1
+    pmq::DBuf<unsigned char> pscan_tmp;
+    pmq::DBuf<int> tcol;                         // column (sorted atom) of each entry
+    pmq::DBuf<double> tval;                      // 6 x tcap: T xx yy zz xy xz yz
+    size_t tcap = 0;
+    int nnz = 0;
     // PME
     int K[3] = {0, 0, 0};
     int bsorder = 5;
@@ -220,6 +230,10 @@ struct pmq_ctx {
     pmq::DBuf<double> qgrid;                     // 2 real grids
     pmq::DBuf<double> cgrid;                     // 2 complex half grids (interleaved re, im)
     pmq::DBuf<double> pme_frac;                  // 2 x 20 fractional potential derivatives per atom
+    int pme_nb[3] = {1, 1, 1};                   // spreading bricks per axis
+    int pme_sbits = 1;                           // bits of the atom index in the brick keys
+    pmq::DBuf<unsigned long long> pme_keys, pme_keys_sorted; // (home brick << sbits) | sorted atom
+    pmq::DBuf<int> pme_bstart;                   // first key of each brick
     pmq::DBuf<unsigned char> fft_work;
     void *plan_f1 = nullptr, *plan_b1 = nullptr, *plan_f2 = nullptr, *plan_b2 = nullptr;
     void *fft_info = nullptr;
@@ -258,6 +272,8 @@ int polar_gradient(pmq_ctx *c, double off2);
 int polar_scatter(pmq_ctx *c, double *ud_out, double *up_out);
 int launch_chgflx_grad(pmq_ctx *c, int mode);
 int launch_zero(pmq_ctx *c, double *p, size_t n);
+int pair_matrix_build(pmq_ctx *c, double off2);
+int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst);
 } // namespace pmq
 
 namespace pmq {

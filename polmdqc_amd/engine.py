@@ -200,8 +200,8 @@ def _profile_bind(L):
     L._prof_ready = True
 
 
-KERNEL_FAMILIES = ("neighbors", "mpole_tiles", "pol_field_tiles", "pol_mutual_tiles", "pol_grad_tiles",
-                   "pme_spread", "pme_fft", "pme_gather")
+KERNEL_FAMILIES = ("neighbors", "mpole_tiles", "pol_field_tiles", "pol_pair_fill", "pol_mutual_spmv",
+                   "pol_grad_tiles", "pme_spread", "pme_fft", "pme_gather")
 
 
 def profile_enable(eng: Engine, on: bool = True):
