@@ -9,9 +9,13 @@ ABI (neighbor sort and tiles, frames, real-space multipoles, PME, direct
 field, the PCG solve, polarization energy and gradient, torques), with the
 coordinates already resident in HBM.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--precision fp64|mixed] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--precision mixed|fp64] [--no-cpu]
 
-Prints one JSON line.  `roofline` is the dominant kernel's achieved rate:
+Both forms are timed on the same system: the plain FP64 form first (the
+reference point), then the mixed form (FP32 pair and tensor arithmetic,
+64-bit fixed-point accumulation, FP64 reductions), whose rate is `value`;
+the line also carries the FP64 rate, the speedup and the RMS force error of
+the mixed result against the FP64 one.  Prints one JSON line.  `roofline` is the dominant kernel's achieved rate:
 algorithmic FLOPs per launch (per-pair operation count x pairs inside the
 cutoff) over its average launch time, from HIP events on the library's own
 stream around every launch in the timed region.  `cpu_baseline` is the
@@ -42,6 +46,7 @@ PAIR_FLOPS = {
     "pol_grad_tiles": 1480,   # image 6, r2 5, Thole 28, series 68, four pair gradients ~330 each after zero folding
 }
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
 
 
 def count_pairs(xyz: np.ndarray, box: float, cutoff: float) -> int:
@@ -71,12 +76,31 @@ def cpu_baseline(s) -> dict:
                       f"{o['iters']} PCG iterations, {dt:.1f} s)"}
 
 
+def timed(eng, L, terms, prec, steps, warmup):
+    """ms per evaluation over `steps` evaluations, and the per-family kernel times."""
+    from polmdqc_amd.engine import profile_enable, profile_read
+
+    for _ in range(warmup):
+        eng.compute(L.GRADIENT, terms, prec)
+    eng.sync()
+    profile_enable(eng, True)
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.compute(L.GRADIENT, terms, prec)
+    eng.sync()
+    t1 = time.perf_counter()
+    prof = profile_read(eng)
+    profile_enable(eng, False)
+    return (t1 - t0) * 1e3 / steps, prof, eng.fetch()
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -85,27 +109,18 @@ def main() -> int:
         return 2
 
     from polmdqc_amd import _lib as L
-    from polmdqc_amd.engine import Engine, profile_enable, profile_read
+    from polmdqc_amd.engine import Engine
 
     xyz, s = build_system()
-    prec = L.PLAIN_FP64 if args.precision == "fp64" else L.MIXED
     terms = L.TERM_MPOLE | L.TERM_POLAR
     eng = Engine(0)
     eng.set_system(s)
-    for _ in range(args.warmup):
-        eng.compute(L.GRADIENT, terms, prec)
-    eng.sync()
-    profile_enable(eng, True)
-    eng.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        eng.compute(L.GRADIENT, terms, prec)
-    eng.sync()
-    t1 = time.perf_counter()
-    prof = profile_read(eng)
-    profile_enable(eng, False)
-    res = eng.fetch()
-    ms = (t1 - t0) * 1e3 / args.steps
+    # the FP64 form is always run: it is the reference point of the mixed one
+    ms64, prof64, res64 = timed(eng, L, terms, L.PLAIN_FP64, args.steps, args.warmup)
+    if args.precision == "mixed":
+        ms, prof, res = timed(eng, L, terms, L.MIXED, args.steps, args.warmup)
+    else:
+        ms, prof, res = ms64, prof64, res64
     value = 1e3 / ms
 
     pairs = count_pairs(xyz, 49.68, 7.0)
@@ -114,18 +129,27 @@ def main() -> int:
     tot_ms, launches = tiles[dom]
     avg_s = tot_ms / launches * 1e-3
     achieved = PAIR_FLOPS[dom] * pairs / avg_s / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None, "kernel": dom,
+    mixed = args.precision == "mixed"
+    peak = PEAK_FP32_TFLOPS if mixed else PEAK_FP64_TFLOPS
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "traffic": None, "kernel": dom,
                 "avg_launch_us": round(avg_s * 1e6, 2), "flops_per_launch": PAIR_FLOPS[dom] * pairs,
-                "note": "FP64 VALU pair kernel; the MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s"}
+                "note": ("FP32 VALU pair kernel (vector peak, no MFMA)" if mixed
+                         else "FP64 VALU pair kernel; the MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s")}
+    frms = float(np.sqrt(np.mean(np.sum((res["desum"] - res64["desum"]) ** 2, axis=1))))
     line = {
         "metric": "AMOEBA 12k-atom water: force evals/s mixed vs FP64 GPU @ rms force err<1e-3",
         "value": round(value, 3), "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if prec == L.PLAIN_FP64 else "f32+i64", "data": "synthetic (seeded water lattice)",
+        "dtype": "f32+i64" if mixed else "f64", "data": "synthetic (seeded water lattice)",
         "config": dict(WORKLOAD, precision=args.precision),
         "roofline": roofline,
+        "fp64_evals_per_s": round(1e3 / ms64, 3),
+        "speedup_vs_fp64": round(ms64 / ms, 3),
+        "rms_force_err_vs_fp64": frms,
+        "energy_rel_err_vs_fp64": {k: abs(res[k] - res64[k]) / abs(res64[k]) for k in ("em", "ep")},
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items() if v[1]},
+        "kernel_ms_per_step_fp64": {k: round(v[0] / args.steps, 4) for k, v in prof64.items() if v[1]},
         "pairs_in_cutoff": pairs, "pcg_iters": res["pcg_iters"],
         "energy_kcal": {"em": res["em"], "ep": res["ep"]},
     }

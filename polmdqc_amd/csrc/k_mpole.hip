@@ -23,6 +23,25 @@ __global__ void k_zero(double *__restrict__ p, size_t n)
         p[i] = 0.0;
 }
 
+// d += q / scale, q = 0 (fixed-point accumulators back to FP64)
+__global__ void k_fx_flush(size_t n, long long *__restrict__ q, double *__restrict__ d, double inv_scale)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        d[i] += (double)q[i] * inv_scale;
+        q[i] = 0;
+    }
+}
+
+int launch_fx_flush(pmq_ctx *c, long long *q, double *d, size_t n, double scale)
+{
+    if (!n) return PMQ_OK;
+    const int tb = 256;
+    const int gb = (int)std::min<size_t>((n + tb - 1) / tb, 4096);
+    hipLaunchKernelGGL(k_fx_flush, dim3(gb), dim3(tb), 0, c->stream, n, q, d, 1.0 / scale);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
 int launch_zero(pmq_ctx *c, double *p, size_t n)
 {
     if (!n) return PMQ_OK;
@@ -244,18 +263,20 @@ struct MpoleArgs {
     DevBox box;
     double off2, f, aewald;
     double *dem, *te, *pot, *aem;
+    long long *demq, *teq;   // mixed precision: 64-bit fixed-point gradient and torque (else null)
     Totals *tot;
     // exclusion corrections
     const ExclPair *excl;
     int nexcl;
 };
 
-template <int PEN, bool CF, bool ANA, bool GRAD, bool VIR, bool EW>
+// R is the arithmetic of the pair terms: double (plain form) or float (mixed
+// form, whose displacements are still taken in FP64 before rounding)
+template <class R, int PEN, bool CF, bool ANA, bool GRAD, bool VIR, bool EW>
 __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
 {
-    using R = double;
     constexpr bool CP = PEN != 0;
-    __shared__ R jx[PMQ_BLOCK], jy[PMQ_BLOCK], jz[PMQ_BLOCK];
+    __shared__ double jx[PMQ_BLOCK], jy[PMQ_BLOCK], jz[PMQ_BLOCK];
     __shared__ R jm[10][PMQ_BLOCK];
     __shared__ R jc[2][PMQ_BLOCK];
     __shared__ int jflag[PMQ_BLOCK], jmol[PMQ_BLOCK];
@@ -273,8 +294,8 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
         if (CP) { jc[0][lane] = a.scp[sj]; jc[1][lane] = a.scp[np + sj]; }
         jflag[lane] = a.sflag[sj];
         if (ANA) jmol[lane] = a.smol[sj];
-        for (int c = 0; c < 8; c++) acc[c][lane] = 0.0;
-        const R xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
+        for (int c = 0; c < 8; c++) acc[c][lane] = R(0);
+        const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
         const int fi = a.sflag[si];
         const int moli = ANA ? a.smol[si] : 0;
         Mp<R> mi;
@@ -291,20 +312,22 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
             for (int jj = jstart; jj < PMQ_BLOCK; jj++) {
                 const int fj = jflag[jj];
                 if (!(fj & 1) || !((fi | fj) & 2)) continue;
-                R xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
-                image<R>(a.box, xr, yr, zr);
+                double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
+                image<double>(a.box, xr, yr, zr);
                 if (xr * xr + yr * yr + zr * zr <= a.off2) mask |= 1ull << jj;
             }
         }
 
-        R e = 0, fxi = 0, fyi = 0, fzi = 0, txi = 0, tyi = 0, tzi = 0, poti = 0, aemi = 0, ein = 0;
+        double e = 0; // energy sums stay FP64 whatever R is
+        R fxi = 0, fyi = 0, fzi = 0, txi = 0, tyi = 0, tzi = 0, poti = 0, aemi = 0, ein = 0;
         R vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
         unsigned nem = 0;
         while (mask) {
             const int jj = __ffsll((long long)mask) - 1;
             mask &= mask - 1;
-            R xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
-            image<R>(a.box, xr, yr, zr);
+            double xd = jx[jj] - xi, yd = jy[jj] - yi, zd = jz[jj] - zi;
+            image<double>(a.box, xd, yd, zd);
+            const R xr = R(xd), yr = R(yd), zr = R(zd);
             const R r2 = xr * xr + yr * yr + zr * zr;
             Mp<R> mk;
             mk.c = jm[0][jj]; mk.dx = jm[1][jj]; mk.dy = jm[2][jj]; mk.dz = jm[3][jj];
@@ -314,14 +337,14 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
             R bn[6];
             if (EW) ewald_series<R>(r2, R(a.aewald), bn);
             if (CP) {
-                pair_cp<R, GRAD, (PEN == 0 ? 1 : PEN), EW>(r2, xr, yr, zr, a.f, corei, alphai, mi, jc[0][jj],
+                pair_cp<R, GRAD, (PEN == 0 ? 1 : PEN), EW>(r2, xr, yr, zr, R(a.f), corei, alphai, mi, jc[0][jj],
                                                            jc[1][jj], mk, o, bn);
             } else {
                 R Rs[6];
                 if (EW) {
                     for (int q = 0; q < 6; q++) Rs[q] = R(a.f) * bn[q];
                 } else {
-                    rr_series<R>(r2, a.f, Rs);
+                    rr_series<R>(r2, R(a.f), Rs);
                 }
                 pair_series<R, GRAD>(Rs, mi, mk, xr, yr, zr, o);
             }
@@ -357,37 +380,41 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
             if (fi & 1) {
                 const int p = a.perm[si];
                 if (GRAD) {
-                    atomicAdd(&a.dem[3 * p + 0], fxi); atomicAdd(&a.dem[3 * p + 1], fyi); atomicAdd(&a.dem[3 * p + 2], fzi);
-                    atomicAdd(&a.te[3 * p + 0], txi); atomicAdd(&a.te[3 * p + 1], tyi); atomicAdd(&a.te[3 * p + 2], tzi);
-                    if (CF) atomicAdd(&a.pot[p], poti);
+                    grad_add(a.dem, a.demq, 3 * p + 0, fxi); grad_add(a.dem, a.demq, 3 * p + 1, fyi);
+                    grad_add(a.dem, a.demq, 3 * p + 2, fzi);
+                    grad_add(a.te, a.teq, 3 * p + 0, txi); grad_add(a.te, a.teq, 3 * p + 1, tyi);
+                    grad_add(a.te, a.teq, 3 * p + 2, tzi);
+                    if (CF) atomicAdd(&a.pot[p], (double)poti);
                 }
-                if (ANA) atomicAdd(&a.aem[p], aemi);
+                if (ANA) atomicAdd(&a.aem[p], (double)aemi);
             }
             if (jflag[lane] & 1) {
                 const int p = a.perm[sj];
                 if (GRAD) {
-                    for (int c = 0; c < 3; c++) atomicAdd(&a.dem[3 * p + c], acc[c][lane]);
-                    for (int c = 0; c < 3; c++) atomicAdd(&a.te[3 * p + c], acc[3 + c][lane]);
-                    if (CF) atomicAdd(&a.pot[p], acc[6][lane]);
+                    for (int c = 0; c < 3; c++) grad_add(a.dem, a.demq, 3 * p + c, acc[c][lane]);
+                    for (int c = 0; c < 3; c++) grad_add(a.te, a.teq, 3 * p + c, acc[3 + c][lane]);
+                    if (CF) atomicAdd(&a.pot[p], (double)acc[6][lane]);
                 }
-                if (ANA) atomicAdd(&a.aem[p], acc[7][lane]);
+                if (ANA) atomicAdd(&a.aem[p], (double)acc[7][lane]);
             }
         }
-        e = wave_sum(e);
+        // totals in FP64 whatever R is
+        const double ew = wave_sum(e);
+        double wxx = 0, wxy = 0, wxz = 0, wyy = 0, wyz = 0, wzz = 0, win = 0;
         if (VIR) {
-            vxx = wave_sum(vxx); vxy = wave_sum(vxy); vxz = wave_sum(vxz);
-            vyy = wave_sum(vyy); vyz = wave_sum(vyz); vzz = wave_sum(vzz);
+            wxx = wave_sum((double)vxx); wxy = wave_sum((double)vxy); wxz = wave_sum((double)vxz);
+            wyy = wave_sum((double)vyy); wyz = wave_sum((double)vyz); wzz = wave_sum((double)vzz);
         }
-        if (ANA) { ein = wave_sum(ein); nem = wave_sum(nem); }
+        if (ANA) { win = wave_sum((double)ein); nem = wave_sum(nem); }
         if (lane == 0) {
-            atomicAdd(&a.tot->em, e);
+            atomicAdd(&a.tot->em, ew);
             if (VIR) {
-                atomicAdd(&a.tot->vir[0], vxx); atomicAdd(&a.tot->vir[1], vxy); atomicAdd(&a.tot->vir[2], vxz);
-                atomicAdd(&a.tot->vir[3], vxy); atomicAdd(&a.tot->vir[4], vyy); atomicAdd(&a.tot->vir[5], vyz);
-                atomicAdd(&a.tot->vir[6], vxz); atomicAdd(&a.tot->vir[7], vyz); atomicAdd(&a.tot->vir[8], vzz);
+                atomicAdd(&a.tot->vir[0], wxx); atomicAdd(&a.tot->vir[1], wxy); atomicAdd(&a.tot->vir[2], wxz);
+                atomicAdd(&a.tot->vir[3], wxy); atomicAdd(&a.tot->vir[4], wyy); atomicAdd(&a.tot->vir[5], wyz);
+                atomicAdd(&a.tot->vir[6], wxz); atomicAdd(&a.tot->vir[7], wyz); atomicAdd(&a.tot->vir[8], wzz);
             }
             if (ANA) {
-                atomicAdd(&a.tot->einter, ein);
+                atomicAdd(&a.tot->einter, win);
                 atomicAdd(&a.tot->nem, (unsigned long long)nem);
             }
         }
@@ -442,10 +469,14 @@ __global__ __launch_bounds__(64) void k_mpole_excl(MpoleArgs a)
         if (a.smol[si] != a.smol[sk]) atomicAdd(&a.tot->einter, o.e);
     }
     if (GRAD) {
-        atomicAdd(&a.dem[3 * q.i + 0], o.fx); atomicAdd(&a.dem[3 * q.i + 1], o.fy); atomicAdd(&a.dem[3 * q.i + 2], o.fz);
-        atomicAdd(&a.dem[3 * q.k + 0], -o.fx); atomicAdd(&a.dem[3 * q.k + 1], -o.fy); atomicAdd(&a.dem[3 * q.k + 2], -o.fz);
-        atomicAdd(&a.te[3 * q.i + 0], o.tix); atomicAdd(&a.te[3 * q.i + 1], o.tiy); atomicAdd(&a.te[3 * q.i + 2], o.tiz);
-        atomicAdd(&a.te[3 * q.k + 0], o.tkx); atomicAdd(&a.te[3 * q.k + 1], o.tky); atomicAdd(&a.te[3 * q.k + 2], o.tkz);
+        grad_add(a.dem, a.demq, 3 * q.i + 0, o.fx); grad_add(a.dem, a.demq, 3 * q.i + 1, o.fy);
+        grad_add(a.dem, a.demq, 3 * q.i + 2, o.fz);
+        grad_add(a.dem, a.demq, 3 * q.k + 0, -o.fx); grad_add(a.dem, a.demq, 3 * q.k + 1, -o.fy);
+        grad_add(a.dem, a.demq, 3 * q.k + 2, -o.fz);
+        grad_add(a.te, a.teq, 3 * q.i + 0, o.tix); grad_add(a.te, a.teq, 3 * q.i + 1, o.tiy);
+        grad_add(a.te, a.teq, 3 * q.i + 2, o.tiz);
+        grad_add(a.te, a.teq, 3 * q.k + 0, o.tkx); grad_add(a.te, a.teq, 3 * q.k + 1, o.tky);
+        grad_add(a.te, a.teq, 3 * q.k + 2, o.tkz);
         if (CF) { atomicAdd(&a.pot[q.i], o.phii); atomicAdd(&a.pot[q.k], o.phik); }
         if (VIR) {
             const R vxy = -R(0.5) * (yr * o.fx + xr * o.fy);
@@ -464,17 +495,17 @@ template <int PEN, bool CF, bool ANA, bool GRAD, bool VIR>
 static int launch_mpole_t(pmq_ctx *c, const MpoleArgs &a)
 {
     const int grid = std::max(1, std::min(c->ntiles_cap, 16384));
-    if (a.aewald > 0.0)
-        {
-            ProfScope ps_(c, "mpole_tiles");
-            hipLaunchKernelGGL((k_mpole_tiles<PEN, CF, ANA, GRAD, VIR, true>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
+    {
+        ProfScope ps_(c, "mpole_tiles");
+        const bool ew = a.aewald > 0.0;
+        if (a.demq) { // mixed precision: ENERGY / GRADIENT only
+            if (ew) hipLaunchKernelGGL((k_mpole_tiles<float, PEN, false, false, GRAD, false, true>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
+            else hipLaunchKernelGGL((k_mpole_tiles<float, PEN, false, false, GRAD, false, false>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
+        } else {
+            if (ew) hipLaunchKernelGGL((k_mpole_tiles<double, PEN, CF, ANA, GRAD, VIR, true>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
+            else hipLaunchKernelGGL((k_mpole_tiles<double, PEN, CF, ANA, GRAD, VIR, false>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
         }
-    else
-        {
-            ProfScope ps_(c, "mpole_tiles");
-            hipLaunchKernelGGL((k_mpole_tiles<PEN, CF, ANA, GRAD, VIR, false>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream,
-                               a);
-        }
+    }
     if (a.nexcl > 0)
         hipLaunchKernelGGL((k_mpole_excl<PEN, CF, ANA, GRAD, VIR>), dim3((a.nexcl + 63) / 64), dim3(64), 0,
                            c->stream, a);
@@ -493,7 +524,7 @@ static int launch_mpole_mode(pmq_ctx *c, const MpoleArgs &a, int mode)
     }
 }
 
-int launch_mpole_direct(pmq_ctx *c, int mode, double off2)
+int launch_mpole_direct(pmq_ctx *c, int mode, double off2, bool mixed)
 {
     MpoleArgs a;
     a.n = c->n;
@@ -507,6 +538,8 @@ int launch_mpole_direct(pmq_ctx *c, int mode, double off2)
     a.off2 = off2;
     a.f = c->prm.electric / c->prm.dielec;
     a.dem = c->dem.p; a.te = c->te.p; a.pot = c->pot.p; a.aem = c->aem.p;
+    a.demq = mixed ? c->fx_grad.p : nullptr;
+    a.teq = mixed ? c->fx_grad.p + 3 * (size_t)c->n : nullptr;
     a.tot = c->totals.p;
     a.excl = c->excl.p;
     a.nexcl = (int)c->h_excl.size();
@@ -525,6 +558,7 @@ struct TorqueArgs {
     const double *x, *y, *z, *te;
     const int *polaxe, *zaxis, *xaxis, *yaxis;
     double *de;
+    long long *deq; // mixed precision: fixed-point accumulation of de (then flushed), else null
     Totals *tot;
     int vir;
 };
@@ -638,10 +672,10 @@ __global__ __launch_bounds__(128) void k_torque(TorqueArgs a)
             }
         }
         for (int j = 0; j < 3; j++) {
-            atomicAdd(&a.de[3 * ia + j], fz[j]);
-            if (frame != PMQ_FRAME_ZONLY) atomicAdd(&a.de[3 * ic + j], fx[j]);
-            if (frame == PMQ_FRAME_ZBISECT || frame == PMQ_FRAME_THREEFOLD) atomicAdd(&a.de[3 * id + j], fy[j]);
-            atomicAdd(&a.de[3 * i + j], -(fz[j] + fx[j] + fy[j]));
+            grad_add(a.de, a.deq, 3 * ia + j, fz[j]);
+            if (frame != PMQ_FRAME_ZONLY) grad_add(a.de, a.deq, 3 * ic + j, fx[j]);
+            if (frame == PMQ_FRAME_ZBISECT || frame == PMQ_FRAME_THREEFOLD) grad_add(a.de, a.deq, 3 * id + j, fy[j]);
+            grad_add(a.de, a.deq, 3 * i + j, -(fz[j] + fx[j] + fy[j]));
         }
         if (a.vir) {
             const int jz = ia >= 0 ? ia : i, jx = ic >= 0 ? ic : i, jy = id >= 0 ? id : i;
@@ -679,17 +713,19 @@ __global__ __launch_bounds__(128) void k_torque(TorqueArgs a)
     }
 }
 
-int launch_torque(pmq_ctx *c, int mode, const double *te, double *de)
+int launch_torque(pmq_ctx *c, int mode, const double *te, double *de, bool mixed)
 {
     TorqueArgs a;
     a.n = c->n;
     a.x = c->x.p; a.y = c->y.p; a.z = c->z.p; a.te = te;
     a.polaxe = c->polaxe.p; a.zaxis = c->zaxis.p; a.xaxis = c->xaxis.p; a.yaxis = c->yaxis.p;
     a.de = de;
+    a.deq = mixed ? c->fx_grad.p : nullptr;
     a.tot = c->totals.p;
     a.vir = mode == PMQ_VIRIAL;
     hipLaunchKernelGGL(k_torque, dim3((c->n + 127) / 128), dim3(128), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
+    if (mixed) return launch_fx_flush(c, c->fx_grad.p, de, 3 * (size_t)c->n, PMQ_FX_GRAD);
     return PMQ_OK;
 }
 

@@ -134,6 +134,9 @@ __global__ __launch_bounds__(64) void k_row_counts(int nblk, const int2 *__restr
     if (I == 0 && lane == 0) rowcnt[nblk * 64] = 0; // the scan's last element becomes the entry total
 }
 
+// V: stored type of the tensor (double, or float in the mixed form); T: the
+// arithmetic of the pair terms
+template <class V>
 struct FillArgs {
     const int2 *tiles;
     const int *ntiles;
@@ -145,11 +148,11 @@ struct FillArgs {
     int npad;
     size_t cap;
     int *col;
-    double *val; // 6 x cap: xx yy zz xy xz yz
+    V *val; // 6 x cap: xx yy zz xy xz yz
 };
 
-__device__ __forceinline__ void store_T(double *val, size_t cap, size_t e, const double *R, double xr, double yr,
-                                        double zr)
+template <class V, class T>
+__device__ __forceinline__ void store_T(V *val, size_t cap, size_t e, const T *R, T xr, T yr, T zr)
 {
     val[e] = R[2] * xr * xr - R[1];
     val[cap + e] = R[2] * yr * yr - R[1];
@@ -159,13 +162,14 @@ __device__ __forceinline__ void store_T(double *val, size_t cap, size_t e, const
     val[5 * cap + e] = R[2] * yr * zr;
 }
 
-template <bool EW>
-__global__ __launch_bounds__(64) void k_pair_fill(FillArgs a)
+template <bool EW, class T, class V>
+__global__ __launch_bounds__(64) void k_pair_fill(FillArgs<V> a)
 {
     const int lane = threadIdx.x;
     const int np = a.npad;
     const int ntiles = *a.ntiles;
-    __shared__ double jx[64], jy[64], jz[64], jpd[64], jth[64];
+    __shared__ double jx[64], jy[64], jz[64];
+    __shared__ T jpd[64], jth[64];
     __shared__ unsigned long long jmask[64];
     __shared__ int jbase[64];
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -181,7 +185,7 @@ __global__ __launch_bounds__(64) void k_pair_fill(FillArgs a)
         jmask[lane] = diag ? (rm | cm) : cm;
         jbase[lane] = a.rowptr[sj] + a.soff[(size_t)t * 128 + 64 + lane];
         const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
-        const double thi = a.spol[np + si], pdi = a.spol[2 * np + si];
+        const T thi = a.spol[np + si], pdi = a.spol[2 * np + si];
         const unsigned long long mi = diag ? (rm | cm) : rm;
         const int ibase = a.rowptr[si] + a.soff[(size_t)t * 128 + lane];
         __syncthreads();
@@ -189,12 +193,13 @@ __global__ __launch_bounds__(64) void k_pair_fill(FillArgs a)
         while (m) {
             const int jj = __ffsll((long long)m) - 1;
             m &= m - 1;
-            double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
-            image<double>(a.box, xr, yr, zr);
-            const double r2 = xr * xr + yr * yr + zr * zr;
-            double lam[6], R[6];
-            thole_lambda<double>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
-            pol_series<EW>(r2, a.aewald, lam, R);
+            double xd = jx[jj] - xi, yd = jy[jj] - yi, zd = jz[jj] - zi;
+            image<double>(a.box, xd, yd, zd);
+            const T xr = T(xd), yr = T(yd), zr = T(zd);
+            const T r2 = xr * xr + yr * yr + zr * zr;
+            T lam[6], R[6];
+            thole_lambda<T>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
+            pol_series<EW, T>(r2, T(a.aewald), lam, R);
             const size_t ei = (size_t)ibase + __popcll(mi & ((1ull << jj) - 1ull));
             const size_t ej = (size_t)jbase[jj] + __popcll(jmask[jj] & ((1ull << lane) - 1ull));
             a.col[ei] = IJ.y * 64 + jj;
@@ -205,6 +210,7 @@ __global__ __launch_bounds__(64) void k_pair_fill(FillArgs a)
     }
 }
 
+template <class V>
 struct FoldArgs {
     const ExclPair *excl;
     int nexcl;
@@ -214,11 +220,11 @@ struct FoldArgs {
     int npad;
     const int *rowptr, *col;
     size_t cap;
-    double *val;
+    V *val;
 };
 
-__device__ __forceinline__ void add_T(double *val, size_t cap, size_t e, const double *R, double xr, double yr,
-                                      double zr)
+template <class V>
+__device__ __forceinline__ void add_T(V *val, size_t cap, size_t e, const double *R, double xr, double yr, double zr)
 {
     val[e] += R[2] * xr * xr - R[1];
     val[cap + e] += R[2] * yr * yr - R[1];
@@ -230,7 +236,8 @@ __device__ __forceinline__ void add_T(double *val, size_t cap, size_t e, const d
 
 // excluded pairs present in the matrix get (u - 1) lambda rr added to their
 // two entries; each entry belongs to one excluded pair, so no atomics
-__global__ __launch_bounds__(64) void k_excl_fold(FoldArgs a)
+template <class V>
+__global__ __launch_bounds__(64) void k_excl_fold(FoldArgs<V> a)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nexcl) return;
@@ -254,18 +261,19 @@ __global__ __launch_bounds__(64) void k_excl_fold(FoldArgs a)
     if (ek >= 0) add_T(a.val, a.cap, (size_t)ek, R, xr, yr, zr);
 }
 
+template <class V>
 struct SpmvArgs {
     int npad;
     const int *rowptr, *col;
     size_t cap;
-    const double *val;
+    const V *val;
     const double *src; // 2 x 3 x npad
     double *dst;       // 2 x 3 x npad (overwritten)
 };
 
-// dst = T src for both dipole sets; G lanes per row
-template <int G>
-__global__ __launch_bounds__(256) void k_pol_spmv(SpmvArgs a)
+// dst = T src for both dipole sets; G lanes per row, FP64 arithmetic
+template <int G, class V>
+__global__ __launch_bounds__(256) void k_pol_spmv(SpmvArgs<V> a)
 {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int row = tid / G, g = tid % G;
@@ -303,7 +311,39 @@ __global__ __launch_bounds__(256) void k_pol_spmv(SpmvArgs a)
 // ---------------------------------------------------------------------------
 // host side
 
-int pair_matrix_build(pmq_ctx *c, double off2)
+template <class T, class V>
+static int pair_fill(pmq_ctx *c, V *val, int tgrid)
+{
+    const int np = c->npad;
+    hipStream_t st = c->stream;
+    FillArgs<V> fa;
+    fa.tiles = c->tiles.p; fa.ntiles = c->ntiles_d.p;
+    fa.sx = c->sx.p; fa.sy = c->sy.p; fa.sz = c->sz.p; fa.spol = c->spol.p;
+    fa.box = c->box;
+    fa.aewald = c->prm.use_ewald ? c->prm.aewald : 0.0;
+    fa.mask = c->tmask.p; fa.soff = c->soff.p; fa.rowptr = c->rowptr.p;
+    fa.npad = np; fa.cap = c->tcap; fa.col = c->tcol.p; fa.val = val;
+    {
+        ProfScope ps_(c, "pol_pair_fill");
+        if (fa.aewald > 0)
+            hipLaunchKernelGGL((k_pair_fill<true, T, V>), dim3(tgrid), dim3(64), 0, st, fa);
+        else
+            hipLaunchKernelGGL((k_pair_fill<false, T, V>), dim3(tgrid), dim3(64), 0, st, fa);
+    }
+    const int nexcl = (int)c->h_excl.size();
+    if (nexcl) {
+        FoldArgs<V> fo;
+        fo.excl = c->excl.p; fo.nexcl = nexcl; fo.iperm = c->iperm.p;
+        fo.sx = c->sx.p; fo.sy = c->sy.p; fo.sz = c->sz.p; fo.spol = c->spol.p;
+        fo.box = c->box; fo.npad = np; fo.rowptr = c->rowptr.p; fo.col = c->tcol.p; fo.cap = c->tcap;
+        fo.val = val;
+        hipLaunchKernelGGL((k_excl_fold<V>), dim3((nexcl + 63) / 64), dim3(64), 0, st, fo);
+    }
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+int pair_matrix_build(pmq_ctx *c, double off2, bool mixed)
 {
     const int nblk = c->nblk, np = c->npad;
     hipStream_t st = c->stream;
@@ -346,44 +386,29 @@ int pair_matrix_build(pmq_ctx *c, double off2)
     }
     c->nnz = nnz;
 
-    FillArgs fa;
-    fa.tiles = c->tiles.p; fa.ntiles = c->ntiles_d.p;
-    fa.sx = c->sx.p; fa.sy = c->sy.p; fa.sz = c->sz.p; fa.spol = c->spol.p;
-    fa.box = c->box;
-    fa.aewald = c->prm.use_ewald ? c->prm.aewald : 0.0;
-    fa.mask = c->tmask.p; fa.soff = c->soff.p; fa.rowptr = c->rowptr.p;
-    fa.npad = np; fa.cap = c->tcap; fa.col = c->tcol.p; fa.val = c->tval.p;
-    {
-        ProfScope ps_(c, "pol_pair_fill");
-        if (fa.aewald > 0)
-            hipLaunchKernelGGL(k_pair_fill<true>, dim3(tgrid), dim3(64), 0, st, fa);
-        else
-            hipLaunchKernelGGL(k_pair_fill<false>, dim3(tgrid), dim3(64), 0, st, fa);
-    }
-    const int nexcl = (int)c->h_excl.size();
-    if (nexcl) {
-        FoldArgs fo;
-        fo.excl = c->excl.p; fo.nexcl = nexcl; fo.iperm = c->iperm.p;
-        fo.sx = c->sx.p; fo.sy = c->sy.p; fo.sz = c->sz.p; fo.spol = c->spol.p;
-        fo.box = c->box; fo.npad = np; fo.rowptr = c->rowptr.p; fo.col = c->tcol.p; fo.cap = c->tcap;
-        fo.val = c->tval.p;
-        hipLaunchKernelGGL(k_excl_fold, dim3((nexcl + 63) / 64), dim3(64), 0, st, fo);
-    }
+    c->tmixed = mixed;
+    if (mixed) return pair_fill<float, float>(c, (float *)c->tval.p, tgrid);
+    return pair_fill<double, double>(c, c->tval.p, tgrid);
+}
+
+template <class V>
+static int spmv(pmq_ctx *c, const V *val, const double *src, double *dst)
+{
+    constexpr int G = 8;
+    SpmvArgs<V> a;
+    a.npad = c->npad; a.rowptr = c->rowptr.p; a.col = c->tcol.p; a.cap = c->tcap; a.val = val;
+    a.src = src; a.dst = dst;
+    const size_t lanes = (size_t)c->npad * G;
+    ProfScope ps_(c, "pol_mutual_spmv");
+    hipLaunchKernelGGL((k_pol_spmv<G, V>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }
 
 int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst)
 {
-    constexpr int G = 8;
-    SpmvArgs a;
-    a.npad = c->npad; a.rowptr = c->rowptr.p; a.col = c->tcol.p; a.cap = c->tcap; a.val = c->tval.p;
-    a.src = src; a.dst = dst;
-    const size_t lanes = (size_t)c->npad * G;
-    ProfScope ps_(c, "pol_mutual_spmv");
-    hipLaunchKernelGGL(k_pol_spmv<G>, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, c->stream, a);
-    PMQ_HIP(hipGetLastError());
-    return PMQ_OK;
+    if (c->tmixed) return spmv<float>(c, (const float *)c->tval.p, src, dst);
+    return spmv<double>(c, c->tval.p, src, dst);
 }
 
 } // namespace pmq

@@ -309,21 +309,25 @@ __global__ void k_pme_brick_start(int n, int nbricks, int sbits, const unsigned 
     bstart[b] = lo;
 }
 
-// One wave per brick; lane = grid point (bricks of more than 64 points are
-// done in chunks of 64).  The atoms homed in the 8 candidate bricks are
-// staged in LDS 64 at a time (one lane each: stencil start, B-spline
-// weights, fractional coefficients); every lane then sums the atoms whose
-// stencil holds its point, in key order, so the grid is bit-reproducible.
+// One workgroup of 4 waves per brick; lane = grid point (bricks of more than
+// 64 points are done in chunks of 64).  The atoms homed in the 8 candidate
+// bricks are staged in LDS, 64 at a time (one thread each: stencil start,
+// B-spline weights, fractional coefficients); wave w sums candidates w, w+4,
+// ... for its lane's point, and the four partial sums are added in wave
+// order, so the grid is bit-reproducible.
+constexpr int SPREAD_CH = 64;
+
 template <int ORD, int MODE>
-__global__ __launch_bounds__(64) void k_pme_spread_brick(BrickArgs a)
+__global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
 {
     constexpr int ND = MODE == 0 ? 2 : 1;
     constexpr int NC = MODE == 0 ? 10 : 6;
     constexpr int NS = MODE == 0 ? 1 : 2;
-    __shared__ double th[3][ORD][ND + 1][64];
-    __shared__ double cf[NC][64];
-    __shared__ int si0[3][64];
-    const int lane = threadIdx.x;
+    __shared__ double th[3][ORD][ND + 1][SPREAD_CH];
+    __shared__ double cf[NC][SPREAD_CH];
+    __shared__ int si0[3][SPREAD_CH];
+    __shared__ double red[4][NS][64];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int K0 = a.g.K0, K1 = a.g.K1, K2 = a.g.K2;
     const int b = blockIdx.x;
     const int bx = b % a.nb[0], by = (b / a.nb[0]) % a.nb[1], bz = b / (a.nb[0] * a.nb[1]);
@@ -355,13 +359,13 @@ __global__ __launch_bounds__(64) void k_pme_spread_brick(BrickArgs a)
         double acc[NS];
 #pragma unroll
         for (int q = 0; q < NS; q++) acc[q] = 0.0;
-        for (int base = 0; base < ntot; base += 64) {
-            const int cnt = min(64, ntot - base);
+        for (int base = 0; base < ntot; base += SPREAD_CH) {
+            const int cnt = min(SPREAD_CH, ntot - base);
             __syncthreads();
-            if (lane < cnt) {
-                // position of candidate base + lane in the concatenated home-brick ranges
+            if (tid < cnt) {
+                // position of candidate base + tid in the concatenated home-brick ranges
                 int e = 0, off = 0;
-                const int idx = base + lane;
+                const int idx = base + tid;
 #pragma unroll
                 for (int h = 0; h < 8; h++) {
                     if (idx >= off && idx < off + hc[h]) e = hs[h] + idx - off;
@@ -378,8 +382,8 @@ __global__ __launch_bounds__(64) void k_pme_spread_brick(BrickArgs a)
 #pragma unroll
                     for (int k = 0; k < ORD; k++)
 #pragma unroll
-                        for (int d = 0; d <= ND; d++) th[ax][k][d][lane] = t[k][d];
-                    si0[ax][lane] = i0[ax];
+                        for (int d = 0; d <= ND; d++) th[ax][k][d][tid] = t[k][d];
+                    si0[ax][tid] = i0[ax];
                 }
                 if (MODE == 0) {
                     double m[10], fm[10];
@@ -388,20 +392,20 @@ __global__ __launch_bounds__(64) void k_pme_spread_brick(BrickArgs a)
                     if (a.charge) m[0] = a.charge[s];
                     frac_mpole(a.g, m, fm);
 #pragma unroll
-                    for (int c = 0; c < 10; c++) cf[c][lane] = fm[c];
+                    for (int c = 0; c < 10; c++) cf[c][tid] = fm[c];
                 } else {
                     for (int set = 0; set < nset; set++) {
                         const double *d = a.src + (size_t)set * 3 * a.npad;
                         const double ux = d[s], uy = d[a.npad + s], uz = d[2 * a.npad + s];
 #pragma unroll
                         for (int r = 0; r < 3; r++)
-                            cf[3 * set + r][lane] = a.g.A[r][0] * ux + a.g.A[r][1] * uy + a.g.A[r][2] * uz;
+                            cf[3 * set + r][tid] = a.g.A[r][0] * ux + a.g.A[r][1] * uy + a.g.A[r][2] * uz;
                     }
                 }
             }
             __syncthreads();
             if (!pon) continue;
-            for (int j = 0; j < cnt; j++) {
+            for (int j = wv; j < cnt; j += 4) {
                 int kx = gx - si0[0][j], ky = gy - si0[1][j], kz = gz - si0[2][j];
                 if (kx < 0) kx += K0;
                 if (ky < 0) ky += K1;
@@ -426,9 +430,14 @@ __global__ __launch_bounds__(64) void k_pme_spread_brick(BrickArgs a)
                 }
             }
         }
-        if (pon)
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NS; q++) red[wv][q][lane] = acc[q];
+        __syncthreads();
+        if (wv == 0 && pon)
             for (int q = 0; q < nset; q++)
-                a.grid[q * a.gstride + ((size_t)gz * K1 + gy) * K0 + gx] = acc[q];
+                a.grid[q * a.gstride + ((size_t)gz * K1 + gy) * K0 + gx] =
+                    ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane];
     }
 }
 
@@ -861,7 +870,7 @@ static int spread(pmq_ctx *c, int mode, const double *src, const double *charge,
     }
     const dim3 grid((unsigned)(c->pme_nb[0] * c->pme_nb[1] * c->pme_nb[2]));
     ProfScope ps_(c, "pme_spread");
-#define PMQ_SPREAD(ORD, MODE) hipLaunchKernelGGL((k_pme_spread_brick<ORD, MODE>), grid, dim3(64), 0, c->stream, args)
+#define PMQ_SPREAD(ORD, MODE) hipLaunchKernelGGL((k_pme_spread_brick<ORD, MODE>), grid, dim3(256), 0, c->stream, args)
     switch (c->bsorder * 2 + (mode ? 1 : 0)) {
     case 8: PMQ_SPREAD(4, 0); break;
     case 9: PMQ_SPREAD(4, 1); break;

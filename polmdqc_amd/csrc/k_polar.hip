@@ -43,43 +43,54 @@ struct PolArgs {
     double off2, aewald, f;
     const ExclPair *excl;
     int nexcl;
+    const unsigned long long *mask; // per tile: 64 row masks of in-cutoff pairs, then 64 column masks
     const double *src;   // dipoles (2 x 3 x npad) for the mutual field / gradient
     double *dst;         // fields (2 x 3 x npad)
     double *dep, *tep;   // caller-order gradient and torque
+    long long *dstq;     // mixed precision: fixed-point fields (else null)
+    long long *depq, *tepq; // mixed precision: fixed-point gradient and torque (else null)
 };
 
-// field at i from multipole k and at k from multipole i
-__device__ __forceinline__ void mp_fields(const double *R, const Mp<double> &a, const Mp<double> &b, double xr, double yr,
-                                          double zr, double *Ei, double *Ek)
+__device__ __forceinline__ void field_add(double *d, long long *q, size_t i, double v)
 {
-    const double qix = a.qxx * xr + a.qxy * yr + a.qxz * zr, qiy = a.qxy * xr + a.qyy * yr + a.qyz * zr,
-                 qiz = a.qxz * xr + a.qyz * yr + a.qzz * zr;
-    const double qkx = b.qxx * xr + b.qxy * yr + b.qxz * zr, qky = b.qxy * xr + b.qyy * yr + b.qyz * zr,
-                 qkz = b.qxz * xr + b.qyz * yr + b.qzz * zr;
-    const double dir = a.dx * xr + a.dy * yr + a.dz * zr, dkr = b.dx * xr + b.dy * yr + b.dz * zr;
-    const double qir = qix * xr + qiy * yr + qiz * zr, qkr = qkx * xr + qky * yr + qkz * zr;
-    const double si = b.c * R[1] - dkr * R[2] + qkr * R[3];
-    const double sk = a.c * R[1] + dir * R[2] + qir * R[3];
-    Ei[0] = -si * xr - R[1] * b.dx + 2.0 * R[2] * qkx;
-    Ei[1] = -si * yr - R[1] * b.dy + 2.0 * R[2] * qky;
-    Ei[2] = -si * zr - R[1] * b.dz + 2.0 * R[2] * qkz;
-    Ek[0] = sk * xr - R[1] * a.dx - 2.0 * R[2] * qix;
-    Ek[1] = sk * yr - R[1] * a.dy - 2.0 * R[2] * qiy;
-    Ek[2] = sk * zr - R[1] * a.dz - 2.0 * R[2] * qiz;
+    if (q) fx_add(q, i, v, PMQ_FX_FIELD);
+    else atomicAdd(d + i, v);
 }
 
-__device__ __forceinline__ Mp<double> load_mp(const double *smp, int np, int s)
+// field at i from multipole k and at k from multipole i
+template <class T>
+__device__ __forceinline__ void mp_fields(const T *R, const Mp<T> &a, const Mp<T> &b, T xr, T yr, T zr, T *Ei, T *Ek)
 {
-    Mp<double> m;
+    const T qix = a.qxx * xr + a.qxy * yr + a.qxz * zr, qiy = a.qxy * xr + a.qyy * yr + a.qyz * zr,
+                 qiz = a.qxz * xr + a.qyz * yr + a.qzz * zr;
+    const T qkx = b.qxx * xr + b.qxy * yr + b.qxz * zr, qky = b.qxy * xr + b.qyy * yr + b.qyz * zr,
+            qkz = b.qxz * xr + b.qyz * yr + b.qzz * zr;
+    const T dir = a.dx * xr + a.dy * yr + a.dz * zr, dkr = b.dx * xr + b.dy * yr + b.dz * zr;
+    const T qir = qix * xr + qiy * yr + qiz * zr, qkr = qkx * xr + qky * yr + qkz * zr;
+    const T si = b.c * R[1] - dkr * R[2] + qkr * R[3];
+    const T sk = a.c * R[1] + dir * R[2] + qir * R[3];
+    Ei[0] = -si * xr - R[1] * b.dx + T(2) * R[2] * qkx;
+    Ei[1] = -si * yr - R[1] * b.dy + T(2) * R[2] * qky;
+    Ei[2] = -si * zr - R[1] * b.dz + T(2) * R[2] * qkz;
+    Ek[0] = sk * xr - R[1] * a.dx - T(2) * R[2] * qix;
+    Ek[1] = sk * yr - R[1] * a.dy - T(2) * R[2] * qiy;
+    Ek[2] = sk * zr - R[1] * a.dz - T(2) * R[2] * qiz;
+}
+
+template <class T = double>
+__device__ __forceinline__ Mp<T> load_mp(const double *smp, int np, int s)
+{
+    Mp<T> m;
     m.c = smp[s]; m.dx = smp[np + s]; m.dy = smp[2 * np + s]; m.dz = smp[3 * np + s];
     m.qxx = smp[4 * np + s]; m.qxy = smp[5 * np + s]; m.qxz = smp[6 * np + s];
     m.qyy = smp[7 * np + s]; m.qyz = smp[8 * np + s]; m.qzz = smp[9 * np + s];
     return m;
 }
 
-__device__ __forceinline__ Mp<double> dip(double x, double y, double z)
+template <class T = double>
+__device__ __forceinline__ Mp<T> dip(T x, T y, T z)
 {
-    Mp<double> m{};
+    Mp<T> m{};
     m.dx = x; m.dy = y; m.dz = z;
     return m;
 }
@@ -92,11 +103,14 @@ __device__ __forceinline__ Mp<double> dip(double x, double y, double z)
     const int np = a.npad;                                                                                     \
     const int ntiles = *a.ntiles;
 
-template <bool EW>
+// T: arithmetic of the pair terms (double, or float in the mixed form with
+// FP64 displacements)
+template <bool EW, class T>
 __global__ __launch_bounds__(64) void k_pol_field_tiles(PolArgs a)
 {
     TILE_PROLOGUE
-    __shared__ double jx[64], jy[64], jz[64], jm[10][64], jpd[64], jth[64], acc[3][64];
+    __shared__ double jx[64], jy[64], jz[64];
+    __shared__ T jm[10][64], jpd[64], jth[64], acc[3][64];
     __shared__ int jflag[64];
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 IJ = a.tiles[t];
@@ -107,37 +121,30 @@ __global__ __launch_bounds__(64) void k_pol_field_tiles(PolArgs a)
         jth[lane] = a.spol[np + sj];
         jpd[lane] = a.spol[2 * np + sj];
         jflag[lane] = a.sflag[sj];
-        acc[0][lane] = acc[1][lane] = acc[2][lane] = 0.0;
+        acc[0][lane] = acc[1][lane] = acc[2][lane] = T(0);
         const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
         const int fi = a.sflag[si];
-        const Mp<double> mi = load_mp(a.smp, np, si);
-        const double thi = a.spol[np + si], pdi = a.spol[2 * np + si];
+        const Mp<T> mi = load_mp<T>(a.smp, np, si);
+        const T thi = a.spol[np + si], pdi = a.spol[2 * np + si];
         __syncthreads();
-        unsigned long long mask = 0ull;
-        if (fi & 1) {
-            for (int jj = IJ.x == IJ.y ? lane + 1 : 0; jj < 64; jj++) {
-                if (!(jflag[jj] & 1)) continue;
-                double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
-                image<double>(a.box, xr, yr, zr);
-                if (xr * xr + yr * yr + zr * zr <= a.off2) mask |= 1ull << jj;
-            }
-        }
-        double ex = 0, ey = 0, ez = 0;
+        unsigned long long mask = a.mask[(size_t)t * 128 + lane]; // in-cutoff j of this lane
+        T ex = 0, ey = 0, ez = 0;
         while (mask) {
             const int jj = __ffsll((long long)mask) - 1;
             mask &= mask - 1;
-            double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
-            image<double>(a.box, xr, yr, zr);
-            const double r2 = xr * xr + yr * yr + zr * zr;
-            double lam[6], R[6];
-            thole_lambda<double>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
-            pol_series<EW>(r2, a.aewald, lam, R);
-            Mp<double> mk;
+            double xd = jx[jj] - xi, yd = jy[jj] - yi, zd = jz[jj] - zi;
+            image<double>(a.box, xd, yd, zd);
+            const T xr = T(xd), yr = T(yd), zr = T(zd);
+            const T r2 = xr * xr + yr * yr + zr * zr;
+            T lam[6], R[6];
+            thole_lambda<T>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
+            pol_series<EW, T>(r2, T(a.aewald), lam, R);
+            Mp<T> mk;
             mk.c = jm[0][jj]; mk.dx = jm[1][jj]; mk.dy = jm[2][jj]; mk.dz = jm[3][jj];
             mk.qxx = jm[4][jj]; mk.qxy = jm[5][jj]; mk.qxz = jm[6][jj]; mk.qyy = jm[7][jj]; mk.qyz = jm[8][jj];
             mk.qzz = jm[9][jj];
-            double Ei[3], Ek[3];
-            mp_fields(R, mi, mk, xr, yr, zr, Ei, Ek);
+            T Ei[3], Ek[3];
+            mp_fields<T>(R, mi, mk, xr, yr, zr, Ei, Ek);
             ex += Ei[0]; ey += Ei[1]; ez += Ei[2];
             atomicAdd(&acc[0][jj], Ek[0]);
             atomicAdd(&acc[1][jj], Ek[1]);
@@ -146,14 +153,17 @@ __global__ __launch_bounds__(64) void k_pol_field_tiles(PolArgs a)
         __syncthreads();
         if (fi & 1)
             for (int set = 0; set < 2; set++) {
-                double *d = a.dst + (size_t)set * 3 * np;
-                atomicAdd(&d[si], ex); atomicAdd(&d[np + si], ey); atomicAdd(&d[2 * np + si], ez);
+                const size_t o = (size_t)set * 3 * np;
+                field_add(a.dst, a.dstq, o + si, ex);
+                field_add(a.dst, a.dstq, o + np + si, ey);
+                field_add(a.dst, a.dstq, o + 2 * np + si, ez);
             }
         if (jflag[lane] & 1)
             for (int set = 0; set < 2; set++) {
-                double *d = a.dst + (size_t)set * 3 * np;
-                atomicAdd(&d[sj], acc[0][lane]); atomicAdd(&d[np + sj], acc[1][lane]);
-                atomicAdd(&d[2 * np + sj], acc[2][lane]);
+                const size_t o = (size_t)set * 3 * np;
+                field_add(a.dst, a.dstq, o + sj, acc[0][lane]);
+                field_add(a.dst, a.dstq, o + np + sj, acc[1][lane]);
+                field_add(a.dst, a.dstq, o + 2 * np + sj, acc[2][lane]);
             }
     }
 }
@@ -172,67 +182,68 @@ __global__ __launch_bounds__(64) void k_pol_field_excl(PolArgs a)
     if (r2 > a.off2) return;
     double lam[6], R[6], Ei[3], Ek[3];
     thole_lambda<double>(sqrt(r2), a.spol[2 * np + si], a.spol[2 * np + sk], a.spol[np + si], a.spol[np + sk], lam);
-    const Mp<double> mi = load_mp(a.smp, np, si), mk = load_mp(a.smp, np, sk);
+    const Mp<double> mi = load_mp<double>(a.smp, np, si), mk = load_mp<double>(a.smp, np, sk);
     const double sc[2] = {q.d, q.p};
     for (int set = 0; set < 2; set++) {
         if (sc[set] == 1.0) continue;
         corr_series(r2, sc[set], lam, R);
-        mp_fields(R, mi, mk, xr, yr, zr, Ei, Ek);
-        double *d = a.dst + (size_t)set * 3 * np;
+        mp_fields<double>(R, mi, mk, xr, yr, zr, Ei, Ek);
+        const size_t o = (size_t)set * 3 * np;
         for (int c = 0; c < 3; c++) {
-            atomicAdd(&d[c * np + si], Ei[c]);
-            atomicAdd(&d[c * np + sk], Ek[c]);
+            field_add(a.dst, a.dstq, o + c * np + si, Ei[c]);
+            field_add(a.dst, a.dstq, o + c * np + sk, Ek[c]);
         }
     }
 }
 
 // ---------------------------------------------------------------------------
 // real-space polarization gradient
-__device__ __forceinline__ void pol_pair_grad(const double *R, const Mp<double> &mi, const Mp<double> &mk,
-                                              const double *udi, const double *upi, const double *udk,
-                                              const double *upk, double xr, double yr, double zr, double sdp,
-                                              double sd, double sp, double su, double *F, double *Ti, double *Tk)
+template <class T>
+__device__ __forceinline__ void pol_pair_grad(const T *R, const Mp<T> &mi, const Mp<T> &mk, const T *udi, const T *upi,
+                                              const T *udk, const T *upk, T xr, T yr, T zr, T sdp, T sd, T sp, T su,
+                                              T *F, T *Ti, T *Tk)
 {
     // sdp weights the (mu_d + mu_p) permanent terms when d and p scales are equal
     // (tiles); sd/sp separately otherwise; su the dipole-dipole term
-    PairOut<double> o;
-    if (sdp != 0.0) {
-        pair_series<double, true>(R, dip(udi[0] + upi[0], udi[1] + upi[1], udi[2] + upi[2]), mk, xr, yr, zr, o);
+    PairOut<T> o;
+    if (sdp != T(0)) {
+        pair_series<T, true>(R, dip<T>(udi[0] + upi[0], udi[1] + upi[1], udi[2] + upi[2]), mk, xr, yr, zr, o);
         for (int c = 0; c < 3; c++) F[c] += sdp * (&o.fx)[c];
         Tk[0] += sdp * o.tkx; Tk[1] += sdp * o.tky; Tk[2] += sdp * o.tkz;
-        pair_series<double, true>(R, mi, dip(udk[0] + upk[0], udk[1] + upk[1], udk[2] + upk[2]), xr, yr, zr, o);
+        pair_series<T, true>(R, mi, dip<T>(udk[0] + upk[0], udk[1] + upk[1], udk[2] + upk[2]), xr, yr, zr, o);
         for (int c = 0; c < 3; c++) F[c] += sdp * (&o.fx)[c];
         Ti[0] += sdp * o.tix; Ti[1] += sdp * o.tiy; Ti[2] += sdp * o.tiz;
     }
-    if (sp != 0.0) { // mu_d with the p-scaled permanent field
-        pair_series<double, true>(R, dip(udi[0], udi[1], udi[2]), mk, xr, yr, zr, o);
+    if (sp != T(0)) { // mu_d with the p-scaled permanent field
+        pair_series<T, true>(R, dip<T>(udi[0], udi[1], udi[2]), mk, xr, yr, zr, o);
         for (int c = 0; c < 3; c++) F[c] += sp * (&o.fx)[c];
         Tk[0] += sp * o.tkx; Tk[1] += sp * o.tky; Tk[2] += sp * o.tkz;
-        pair_series<double, true>(R, mi, dip(udk[0], udk[1], udk[2]), xr, yr, zr, o);
+        pair_series<T, true>(R, mi, dip<T>(udk[0], udk[1], udk[2]), xr, yr, zr, o);
         for (int c = 0; c < 3; c++) F[c] += sp * (&o.fx)[c];
         Ti[0] += sp * o.tix; Ti[1] += sp * o.tiy; Ti[2] += sp * o.tiz;
     }
-    if (sd != 0.0) { // mu_p with the d-scaled permanent field
-        pair_series<double, true>(R, dip(upi[0], upi[1], upi[2]), mk, xr, yr, zr, o);
+    if (sd != T(0)) { // mu_p with the d-scaled permanent field
+        pair_series<T, true>(R, dip<T>(upi[0], upi[1], upi[2]), mk, xr, yr, zr, o);
         for (int c = 0; c < 3; c++) F[c] += sd * (&o.fx)[c];
         Tk[0] += sd * o.tkx; Tk[1] += sd * o.tky; Tk[2] += sd * o.tkz;
-        pair_series<double, true>(R, mi, dip(upk[0], upk[1], upk[2]), xr, yr, zr, o);
+        pair_series<T, true>(R, mi, dip<T>(upk[0], upk[1], upk[2]), xr, yr, zr, o);
         for (int c = 0; c < 3; c++) F[c] += sd * (&o.fx)[c];
         Ti[0] += sd * o.tix; Ti[1] += sd * o.tiy; Ti[2] += sd * o.tiz;
     }
-    if (su != 0.0) {
-        pair_series<double, true>(R, dip(udi[0], udi[1], udi[2]), dip(upk[0], upk[1], upk[2]), xr, yr, zr, o);
+    if (su != T(0)) {
+        pair_series<T, true>(R, dip<T>(udi[0], udi[1], udi[2]), dip<T>(upk[0], upk[1], upk[2]), xr, yr, zr, o);
         for (int c = 0; c < 3; c++) F[c] += su * (&o.fx)[c];
-        pair_series<double, true>(R, dip(upi[0], upi[1], upi[2]), dip(udk[0], udk[1], udk[2]), xr, yr, zr, o);
+        pair_series<T, true>(R, dip<T>(upi[0], upi[1], upi[2]), dip<T>(udk[0], udk[1], udk[2]), xr, yr, zr, o);
         for (int c = 0; c < 3; c++) F[c] += su * (&o.fx)[c];
     }
 }
 
-template <bool EW>
+template <bool EW, class T>
 __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
 {
     TILE_PROLOGUE
-    __shared__ double jx[64], jy[64], jz[64], jm[10][64], ju[6][64], jpd[64], jth[64], acc[6][64];
+    __shared__ double jx[64], jy[64], jz[64];
+    __shared__ T jm[10][64], ju[6][64], jpd[64], jth[64], acc[6][64];
     __shared__ int jflag[64];
     const double hf = 0.5 * a.f;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -245,40 +256,33 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
         jth[lane] = a.spol[np + sj];
         jpd[lane] = a.spol[2 * np + sj];
         jflag[lane] = a.sflag[sj];
-        for (int c = 0; c < 6; c++) acc[c][lane] = 0.0;
+        for (int c = 0; c < 6; c++) acc[c][lane] = T(0);
         const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
         const int fi = a.sflag[si];
-        const Mp<double> mi = load_mp(a.smp, np, si);
-        double ui[6];
+        const Mp<T> mi = load_mp<T>(a.smp, np, si);
+        T ui[6];
         for (int c = 0; c < 6; c++) ui[c] = a.src[c * np + si];
-        const double thi = a.spol[np + si], pdi = a.spol[2 * np + si];
+        const T thi = a.spol[np + si], pdi = a.spol[2 * np + si];
         __syncthreads();
-        unsigned long long mask = 0ull;
-        if (fi & 1) {
-            for (int jj = IJ.x == IJ.y ? lane + 1 : 0; jj < 64; jj++) {
-                if (!(jflag[jj] & 1)) continue;
-                double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
-                image<double>(a.box, xr, yr, zr);
-                if (xr * xr + yr * yr + zr * zr <= a.off2) mask |= 1ull << jj;
-            }
-        }
-        double F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0};
+        unsigned long long mask = a.mask[(size_t)t * 128 + lane]; // in-cutoff j of this lane
+        T F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0};
         while (mask) {
             const int jj = __ffsll((long long)mask) - 1;
             mask &= mask - 1;
-            double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
-            image<double>(a.box, xr, yr, zr);
-            const double r2 = xr * xr + yr * yr + zr * zr;
-            double lam[6], R[6];
-            thole_lambda<double>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
-            pol_series<EW>(r2, a.aewald, lam, R);
-            Mp<double> mk;
+            double xd = jx[jj] - xi, yd = jy[jj] - yi, zd = jz[jj] - zi;
+            image<double>(a.box, xd, yd, zd);
+            const T xr = T(xd), yr = T(yd), zr = T(zd);
+            const T r2 = xr * xr + yr * yr + zr * zr;
+            T lam[6], R[6];
+            thole_lambda<T>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
+            pol_series<EW, T>(r2, T(a.aewald), lam, R);
+            Mp<T> mk;
             mk.c = jm[0][jj]; mk.dx = jm[1][jj]; mk.dy = jm[2][jj]; mk.dz = jm[3][jj];
             mk.qxx = jm[4][jj]; mk.qxy = jm[5][jj]; mk.qxz = jm[6][jj]; mk.qyy = jm[7][jj]; mk.qyz = jm[8][jj];
             mk.qzz = jm[9][jj];
-            const double udk[3] = {ju[0][jj], ju[1][jj], ju[2][jj]}, upk[3] = {ju[3][jj], ju[4][jj], ju[5][jj]};
-            double f3[3] = {0, 0, 0}, tk[3] = {0, 0, 0};
-            pol_pair_grad(R, mi, mk, ui, ui + 3, udk, upk, xr, yr, zr, 1.0, 0.0, 0.0, 1.0, f3, Ti, tk);
+            const T udk[3] = {ju[0][jj], ju[1][jj], ju[2][jj]}, upk[3] = {ju[3][jj], ju[4][jj], ju[5][jj]};
+            T f3[3] = {0, 0, 0}, tk[3] = {0, 0, 0};
+            pol_pair_grad<T>(R, mi, mk, ui, ui + 3, udk, upk, xr, yr, zr, T(1), T(0), T(0), T(1), f3, Ti, tk);
             F[0] += f3[0]; F[1] += f3[1]; F[2] += f3[2];
             atomicAdd(&acc[0][jj], -f3[0]);
             atomicAdd(&acc[1][jj], -f3[1]);
@@ -291,15 +295,15 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
         if (fi & 1) {
             const int p = a.perm[si];
             for (int c = 0; c < 3; c++) {
-                atomicAdd(&a.dep[3 * p + c], hf * F[c]);
-                atomicAdd(&a.tep[3 * p + c], hf * Ti[c]);
+                grad_add(a.dep, a.depq, 3 * p + c, hf * F[c]);
+                grad_add(a.tep, a.tepq, 3 * p + c, hf * Ti[c]);
             }
         }
         if (jflag[lane] & 1) {
             const int p = a.perm[sj];
             for (int c = 0; c < 3; c++) {
-                atomicAdd(&a.dep[3 * p + c], hf * acc[c][lane]);
-                atomicAdd(&a.tep[3 * p + c], hf * acc[3 + c][lane]);
+                grad_add(a.dep, a.depq, 3 * p + c, hf * acc[c][lane]);
+                grad_add(a.tep, a.tepq, 3 * p + c, hf * acc[3 + c][lane]);
             }
         }
     }
@@ -321,20 +325,21 @@ __global__ __launch_bounds__(64) void k_pol_grad_excl(PolArgs a)
     thole_lambda<double>(sqrt(r2), a.spol[2 * np + si], a.spol[2 * np + sk], a.spol[np + si], a.spol[np + sk], lam);
     rr_series<double>(r2, 1.0, rr);
     for (int c = 0; c < 6; c++) R[c] = lam[c] * rr[c];
-    const Mp<double> mi = load_mp(a.smp, np, si), mk = load_mp(a.smp, np, sk);
+    const Mp<double> mi = load_mp<double>(a.smp, np, si), mk = load_mp<double>(a.smp, np, sk);
     double udi[3], upi[3], udk[3], upk[3];
     for (int c = 0; c < 3; c++) {
         udi[c] = a.src[c * np + si]; upi[c] = a.src[(3 + c) * np + si];
         udk[c] = a.src[c * np + sk]; upk[c] = a.src[(3 + c) * np + sk];
     }
     double F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0}, Tk[3] = {0, 0, 0};
-    pol_pair_grad(R, mi, mk, udi, upi, udk, upk, xr, yr, zr, 0.0, q.d - 1.0, q.p - 1.0, q.u - 1.0, F, Ti, Tk);
+    pol_pair_grad<double>(R, mi, mk, udi, upi, udk, upk, xr, yr, zr, 0.0, q.d - 1.0, q.p - 1.0, q.u - 1.0, F, Ti,
+                          Tk);
     const double hf = 0.5 * a.f;
     for (int c = 0; c < 3; c++) {
-        atomicAdd(&a.dep[3 * q.i + c], hf * F[c]);
-        atomicAdd(&a.dep[3 * q.k + c], -hf * F[c]);
-        atomicAdd(&a.tep[3 * q.i + c], hf * Ti[c]);
-        atomicAdd(&a.tep[3 * q.k + c], hf * Tk[c]);
+        grad_add(a.dep, a.depq, 3 * q.i + c, hf * F[c]);
+        grad_add(a.dep, a.depq, 3 * q.k + c, -hf * F[c]);
+        grad_add(a.tep, a.tepq, 3 * q.i + c, hf * Ti[c]);
+        grad_add(a.tep, a.tepq, 3 * q.k + c, hf * Tk[c]);
     }
 }
 
@@ -352,34 +357,63 @@ __global__ void k_pol_sort(int n, int npad, const int *perm, const double *pol, 
 }
 
 // ---------------------------------------------------------------------------
-// preconditioned conjugate gradient on (1/alpha - T) mu = E, both sets at once.
-// Vectors are 2 x 3 x npad; index t covers both sets, set = t / (3 npad).
-// Scalars: pcg[0..1] r.z, [2..3] p.v, [4..5] new r.z, [6..7] r.r
+// preconditioned conjugate gradient on (1/alpha - T) mu = E, both sets at once,
+// with the diagonal (polarizability) preconditioner.  Vectors are 2 x 3 x npad;
+// index t covers both sets, set = t / (3 npad).  Every reduction is two-stage
+// and deterministic: each block stores its partial sums, one block adds them
+// in a fixed order and updates the scalars.  The convergence test runs on the
+// device too, so iterations are enqueued in batches without host round trips;
+// once `done` is set the remaining kernels of a batch return at once.
+//
+// Scalars sc[]: 0..1 r.z, 2..3 alpha, 4..5 beta, 6..7 r.r, 8 done, 9 iterations
+// at convergence, 10 epsilon of the last iteration.
 struct PcgArgs {
-    int npad;
+    int npad, nblk;
     const double *spol;
     const double *E, *Tu; // fields
     double *u, *r, *z, *p, *v;
+    double *part;         // nblk x 4 partial sums
     double *sc;
+    double npolar, poleps;
+    int iter;
 };
 
-__device__ __forceinline__ void block_add2(double a0, double a1, double *dst)
+constexpr int PCG_TB = 256;
+
+// block sums of two pairs of values into part[blockIdx.x * 4 + 0..3]
+__device__ __forceinline__ void block_part4(double a0, double a1, double a2, double a3, double *part)
 {
-    __shared__ double red[2][256];
-    red[0][threadIdx.x] = a0;
-    red[1][threadIdx.x] = a1;
+    __shared__ double red[4][PCG_TB / 64];
+    double v[4] = {a0, a1, a2, a3};
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+    const int w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < 4; q++) red[q][w] = v[q];
     __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + w];
-            red[1][threadIdx.x] += red[1][threadIdx.x + w];
-        }
+    if (threadIdx.x < 4) {
+        double t = 0.0;
+        for (int k = 0; k < PCG_TB / 64; k++) t += red[threadIdx.x][k];
+        part[blockIdx.x * 4 + threadIdx.x] = t;
+    }
+}
+
+// sums of the block partials (one block of PCG_TB threads), fixed order
+__device__ __forceinline__ void sum_parts(const double *part, int nblk, double *out4)
+{
+    __shared__ double red[4][PCG_TB];
+    double v[4] = {0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nblk; b += PCG_TB)
+        for (int q = 0; q < 4; q++) v[q] += part[b * 4 + q];
+    for (int q = 0; q < 4; q++) red[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int w = PCG_TB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int q = 0; q < 4; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        atomicAdd(dst, red[0][0]);
-        atomicAdd(dst + 1, red[1][0]);
-    }
+    for (int q = 0; q < 4; q++) out4[q] = red[q][0];
 }
 
 __global__ __launch_bounds__(256) void k_pcg_guess(PcgArgs a)
@@ -390,7 +424,7 @@ __global__ __launch_bounds__(256) void k_pcg_guess(PcgArgs a)
     a.u[t] = a.spol[s] * a.E[t];
 }
 
-__global__ __launch_bounds__(256) void k_pcg_init(PcgArgs a)
+__global__ __launch_bounds__(PCG_TB) void k_pcg_init(PcgArgs a)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double s0 = 0.0, s1 = 0.0;
@@ -404,12 +438,23 @@ __global__ __launch_bounds__(256) void k_pcg_init(PcgArgs a)
         a.p[t] = z;
         (t < 3 * a.npad ? s0 : s1) = r * z;
     }
-    block_add2(s0, s1, a.sc + 0);
+    block_part4(s0, s1, 0.0, 0.0, a.part);
+}
+
+__global__ __launch_bounds__(PCG_TB) void k_pcg_init_fin(PcgArgs a)
+{
+    double tot[4];
+    sum_parts(a.part, a.nblk, tot);
+    if (threadIdx.x == 0) {
+        a.sc[0] = tot[0]; a.sc[1] = tot[1];
+        for (int q = 2; q < 11; q++) a.sc[q] = 0.0;
+    }
 }
 
 // v = p/alpha - T p, and p.v
-__global__ __launch_bounds__(256) void k_pcg_a(PcgArgs a)
+__global__ __launch_bounds__(PCG_TB) void k_pcg_a(PcgArgs a)
 {
+    if (a.sc[8] != 0.0) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double s0 = 0.0, s1 = 0.0;
     if (t < 6 * a.npad) {
@@ -419,19 +464,29 @@ __global__ __launch_bounds__(256) void k_pcg_a(PcgArgs a)
         a.v[t] = v;
         (t < 3 * a.npad ? s0 : s1) = a.p[t] * v;
     }
-    block_add2(s0, s1, a.sc + 2);
+    block_part4(s0, s1, 0.0, 0.0, a.part);
+}
+
+// step lengths alpha = r.z / p.v
+__global__ __launch_bounds__(PCG_TB) void k_pcg_a_fin(PcgArgs a)
+{
+    if (a.sc[8] != 0.0) return;
+    double tot[4];
+    sum_parts(a.part, a.nblk, tot);
+    if (threadIdx.x == 0)
+        for (int set = 0; set < 2; set++) a.sc[2 + set] = tot[set] != 0.0 ? a.sc[set] / tot[set] : 0.0;
 }
 
 // u += alpha p, r -= alpha v, z = M r; new r.z and r.r
-__global__ __launch_bounds__(256) void k_pcg_b(PcgArgs a)
+__global__ __launch_bounds__(PCG_TB) void k_pcg_b(PcgArgs a)
 {
+    if (a.sc[8] != 0.0) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double s0 = 0.0, s1 = 0.0, q0 = 0.0, q1 = 0.0;
     if (t < 6 * a.npad) {
         const int set = t < 3 * a.npad ? 0 : 1;
         const int s = t % a.npad;
-        const double pv = a.sc[2 + set];
-        const double al = pv != 0.0 ? a.sc[set] / pv : 0.0;
+        const double al = a.sc[2 + set];
         a.u[t] += al * a.p[t];
         const double r = a.r[t] - al * a.v[t];
         const double z = a.spol[s] * r;
@@ -440,27 +495,51 @@ __global__ __launch_bounds__(256) void k_pcg_b(PcgArgs a)
         (set == 0 ? s0 : s1) = r * z;
         (set == 0 ? q0 : q1) = r * r;
     }
-    block_add2(s0, s1, a.sc + 4);
-    __syncthreads();
-    block_add2(q0, q1, a.sc + 6);
+    block_part4(s0, s1, q0, q1, a.part);
+}
+
+// beta = new r.z / r.z, the convergence test debye sqrt(max r.r / npolar) < poleps
+__global__ __launch_bounds__(PCG_TB) void k_pcg_b_fin(PcgArgs a)
+{
+    if (a.sc[8] != 0.0) return;
+    double tot[4];
+    sum_parts(a.part, a.nblk, tot);
+    if (threadIdx.x == 0) {
+        const double debye = 4.80321;
+        const double eps = debye * sqrt(fmax(tot[2], tot[3]) / a.npolar);
+        a.sc[10] = eps;
+        a.sc[6] = tot[2]; a.sc[7] = tot[3];
+        for (int set = 0; set < 2; set++) {
+            a.sc[4 + set] = a.sc[set] != 0.0 ? tot[set] / a.sc[set] : 0.0;
+            a.sc[set] = tot[set];
+        }
+        if (eps < a.poleps) {
+            a.sc[8] = 1.0;
+            a.sc[9] = (double)a.iter;
+        }
+    }
 }
 
 // p = z + beta p
-__global__ __launch_bounds__(256) void k_pcg_c(PcgArgs a)
+__global__ __launch_bounds__(PCG_TB) void k_pcg_c(PcgArgs a)
 {
+    if (a.sc[8] != 0.0) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 6 * a.npad) return;
     const int set = t < 3 * a.npad ? 0 : 1;
-    const double beta = a.sc[set] != 0.0 ? a.sc[4 + set] / a.sc[set] : 0.0;
-    a.p[t] = a.z[t] + beta * a.p[t];
+    a.p[t] = a.z[t] + a.sc[4 + set] * a.p[t];
 }
 
-// roll the scalars for the next iteration
-__global__ void k_pcg_roll(double *sc)
+__device__ __forceinline__ void block_add2(double a0, double a1, double *dst)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        sc[0] = sc[4]; sc[1] = sc[5];
-        sc[2] = sc[3] = sc[4] = sc[5] = sc[6] = sc[7] = 0.0;
+    double v0 = a0, v1 = a1;
+    for (int o = 32; o > 0; o >>= 1) {
+        v0 += __shfl_xor(v0, o, 64);
+        v1 += __shfl_xor(v1, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(dst, v0);
+        atomicAdd(dst + 1, v1);
     }
 }
 
@@ -503,7 +582,9 @@ static PolArgs pol_args(pmq_ctx *c, double off2)
     a.f = c->prm.electric / c->prm.dielec;
     a.excl = c->excl.p;
     a.nexcl = (int)c->h_excl.size();
+    a.mask = c->tmask.p;
     a.src = nullptr; a.dst = nullptr; a.dep = c->dep.p; a.tep = c->tep.p;
+    a.dstq = nullptr; a.depq = nullptr; a.tepq = nullptr;
     return a;
 }
 
@@ -518,73 +599,85 @@ static int mutual_field(pmq_ctx *c, const double *src, double *dst)
     return PMQ_OK;
 }
 
-int polar_prepare(pmq_ctx *c, double off2)
+int polar_prepare(pmq_ctx *c, double off2, bool mixed)
 {
     // sorted parameters, zeroed permanent field, real-space permanent field
     const int np = c->npad;
     hipLaunchKernelGGL(k_pol_sort, dim3((np + 255) / 256), dim3(256), 0, c->stream, c->n, np, c->perm.p,
                        c->polarity.p, c->thole.p, c->pdamp.p, c->spol.p);
     int rc;
+    if ((rc = pair_matrix_build(c, off2, mixed))) return rc; // also the tile masks the kernels below read
     if ((rc = launch_zero(c, c->pfield.p, 6 * (size_t)np))) return rc;
     PolArgs a = pol_args(c, off2);
     a.dst = c->pfield.p;
-    if (a.aewald > 0)
-        {
-            ProfScope ps_(c, "pol_field_tiles");
-            hipLaunchKernelGGL(k_pol_field_tiles<true>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+    a.dstq = mixed ? c->fx_field.p : nullptr;
+    {
+        ProfScope ps_(c, "pol_field_tiles");
+        const dim3 g(tile_grid(c));
+        if (mixed) {
+            if (a.aewald > 0) hipLaunchKernelGGL((k_pol_field_tiles<true, float>), g, dim3(64), 0, c->stream, a);
+            else hipLaunchKernelGGL((k_pol_field_tiles<false, float>), g, dim3(64), 0, c->stream, a);
+        } else {
+            if (a.aewald > 0) hipLaunchKernelGGL((k_pol_field_tiles<true, double>), g, dim3(64), 0, c->stream, a);
+            else hipLaunchKernelGGL((k_pol_field_tiles<false, double>), g, dim3(64), 0, c->stream, a);
         }
-    else
-        {
-            ProfScope ps_(c, "pol_field_tiles");
-            hipLaunchKernelGGL(k_pol_field_tiles<false>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
-        }
+    }
     if (a.nexcl) hipLaunchKernelGGL(k_pol_field_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
-    return pair_matrix_build(c, off2);
+    if (mixed && (rc = launch_fx_flush(c, c->fx_field.p, c->pfield.p, 6 * (size_t)np, PMQ_FX_FIELD))) return rc;
+    return PMQ_OK;
 }
 
 int polar_solve(pmq_ctx *c, double off2, int *iters_out)
 {
+    (void)off2;
     const int np = c->npad;
     const size_t nv = 6 * (size_t)np;
+    const int gb = (int)((nv + PCG_TB - 1) / PCG_TB);
+    PMQ_HIP(c->pcg_part.alloc(4 * (size_t)gb));
     PcgArgs a;
-    a.npad = np; a.spol = c->spol.p;
+    a.npad = np; a.nblk = gb; a.spol = c->spol.p;
     a.E = c->pfield.p; a.Tu = c->pvec.p;
     a.u = c->uind.p; a.r = c->pres.p; a.z = c->pz.p; a.p = c->pdir.p; a.v = c->pvec.p;
+    a.part = c->pcg_part.p;
     a.sc = c->pcg.p;
-    const int tb = 256, gb = (int)((nv + tb - 1) / tb);
-    int rc;
-    if ((rc = launch_zero(c, c->pcg.p, 8))) return rc;
-    hipLaunchKernelGGL(k_pcg_guess, dim3(gb), dim3(tb), 0, c->stream, a);
-    if ((rc = mutual_field(c, c->uind.p, c->pvec.p))) return rc;
-    hipLaunchKernelGGL(k_pcg_init, dim3(gb), dim3(tb), 0, c->stream, a);
-    PMQ_HIP(hipGetLastError());
-    // count polarizable sites once (host copy of the parameters)
     int npolar = 0;
     for (double p : c->h_polarity) npolar += p != 0.0;
-    npolar = std::max(npolar, 1);
-    const double debye = 4.80321;
+    a.npolar = std::max(npolar, 1);
+    a.poleps = c->prm.poleps;
+    a.iter = 0;
+    int rc;
+    hipLaunchKernelGGL(k_pcg_guess, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
+    if ((rc = mutual_field(c, c->uind.p, c->pvec.p))) return rc;
+    hipLaunchKernelGGL(k_pcg_init, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
+    hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(PCG_TB), 0, c->stream, a);
+    PMQ_HIP(hipGetLastError());
+    // iterations go out in batches; the first batch is the previous solve's
+    // count (a good guess along a trajectory), later ones are single steps
+    int batch = std::max(1, c->pcg_guess_iters);
     int it = 0;
+    double sc[11];
     bool done = false;
     while (!done && it < c->prm.politer) {
-        it++;
-        if ((rc = mutual_field(c, c->pdir.p, c->pvec.p))) return rc;
-        hipLaunchKernelGGL(k_pcg_a, dim3(gb), dim3(tb), 0, c->stream, a);
-        hipLaunchKernelGGL(k_pcg_b, dim3(gb), dim3(tb), 0, c->stream, a);
-        PMQ_HIP(hipGetLastError());
-        double sc[8];
+        const int nb = std::min(batch, c->prm.politer - it);
+        for (int k = 0; k < nb; k++) {
+            a.iter = ++it;
+            if ((rc = mutual_field(c, c->pdir.p, c->pvec.p))) return rc;
+            hipLaunchKernelGGL(k_pcg_a, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
+            hipLaunchKernelGGL(k_pcg_a_fin, dim3(1), dim3(PCG_TB), 0, c->stream, a);
+            hipLaunchKernelGGL(k_pcg_b, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
+            hipLaunchKernelGGL(k_pcg_b_fin, dim3(1), dim3(PCG_TB), 0, c->stream, a);
+            hipLaunchKernelGGL(k_pcg_c, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
+            PMQ_HIP(hipGetLastError());
+        }
         PMQ_HIP(hipMemcpyAsync(sc, c->pcg.p, sizeof sc, hipMemcpyDeviceToHost, c->stream));
         PMQ_HIP(hipStreamSynchronize(c->stream));
-        const double eps = debye * sqrt(std::max(sc[6], sc[7]) / npolar);
-        if (eps < c->prm.poleps) {
-            done = true;
-            break;
-        }
-        hipLaunchKernelGGL(k_pcg_c, dim3(gb), dim3(tb), 0, c->stream, a);
-        hipLaunchKernelGGL(k_pcg_roll, dim3(1), dim3(1), 0, c->stream, c->pcg.p);
-        PMQ_HIP(hipGetLastError());
+        done = sc[8] != 0.0;
+        batch = 1;
     }
-    *iters_out = done ? it : -it;
+    const int iters = done ? (int)sc[9] : it;
+    if (done) c->pcg_guess_iters = iters;
+    *iters_out = done ? iters : -iters;
     return PMQ_OK;
 }
 
@@ -597,20 +690,25 @@ int polar_energy(pmq_ctx *c)
     return PMQ_OK;
 }
 
-int polar_gradient(pmq_ctx *c, double off2)
+int polar_gradient(pmq_ctx *c, double off2, bool mixed)
 {
     PolArgs a = pol_args(c, off2);
     a.src = c->uind.p;
-    if (a.aewald > 0)
-        {
-            ProfScope ps_(c, "pol_grad_tiles");
-            hipLaunchKernelGGL(k_pol_grad_tiles<true>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
+    if (mixed) {
+        a.depq = c->fx_grad.p;
+        a.tepq = c->fx_grad.p + 3 * (size_t)c->n;
+    }
+    {
+        ProfScope ps_(c, "pol_grad_tiles");
+        const dim3 g(tile_grid(c));
+        if (mixed) {
+            if (a.aewald > 0) hipLaunchKernelGGL((k_pol_grad_tiles<true, float>), g, dim3(64), 0, c->stream, a);
+            else hipLaunchKernelGGL((k_pol_grad_tiles<false, float>), g, dim3(64), 0, c->stream, a);
+        } else {
+            if (a.aewald > 0) hipLaunchKernelGGL((k_pol_grad_tiles<true, double>), g, dim3(64), 0, c->stream, a);
+            else hipLaunchKernelGGL((k_pol_grad_tiles<false, double>), g, dim3(64), 0, c->stream, a);
         }
-    else
-        {
-            ProfScope ps_(c, "pol_grad_tiles");
-            hipLaunchKernelGGL(k_pol_grad_tiles<false>, dim3(tile_grid(c)), dim3(64), 0, c->stream, a);
-        }
+    }
     if (a.nexcl) hipLaunchKernelGGL(k_pol_grad_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     if (a.aewald > 0) return pme_polar_grad(c);

@@ -285,6 +285,11 @@ int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
     PMQ_HIP(c->scratch_sort.alloc(c->scratch_sort_bytes));
     PMQ_HIP(c->dem.alloc(3 * (size_t)n)); PMQ_HIP(c->te.alloc(3 * (size_t)n));
     PMQ_HIP(c->pot.alloc(n)); PMQ_HIP(c->aem.alloc(n));
+    // fixed-point accumulators start (and are left) at zero
+    PMQ_HIP(c->fx_grad.alloc(6 * (size_t)n)); PMQ_HIP(c->fx_field.alloc(6 * (size_t)npad));
+    int rz;
+    if ((rz = launch_zero(c, (double *)c->fx_grad.p, 6 * (size_t)n))) return rz;
+    if ((rz = launch_zero(c, (double *)c->fx_field.p, 6 * (size_t)npad))) return rz;
     PMQ_HIP(hipStreamSynchronize(c->stream));
     c->have_top = true;
     c->have_prm = false;
@@ -375,7 +380,7 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
         PMQ_HIP(c->pfield.alloc(nv)); PMQ_HIP(c->uind.alloc(nv));
         PMQ_HIP(c->pres.alloc(nv)); PMQ_HIP(c->pz.alloc(nv));
         PMQ_HIP(c->pdir.alloc(nv)); PMQ_HIP(c->pvec.alloc(nv));
-        PMQ_HIP(c->pcg.alloc(8));
+        PMQ_HIP(c->pcg.alloc(16));
         PMQ_HIP(c->tep.alloc(3 * (size_t)n)); PMQ_HIP(c->dep.alloc(3 * (size_t)n));
         PMQ_HIP(c->uind_out.alloc(6 * (size_t)n));
     } else {
@@ -440,8 +445,13 @@ int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
         c->err = "pmq_compute: bad mode";
         return PMQ_ERR_ARG;
     }
-    if (precision != PMQ_PLAIN_FP64) {
-        c->err = "pmq_compute: only the plain FP64 form is built";
+    if (precision != PMQ_PLAIN_FP64 && precision != PMQ_MIXED) {
+        c->err = "pmq_compute: precision must be PMQ_PLAIN_FP64 or PMQ_MIXED";
+        return PMQ_ERR_ARG;
+    }
+    const bool mixed = precision == PMQ_MIXED;
+    if (mixed && (mode == PMQ_ANALYSIS || mode == PMQ_VIRIAL || c->prm.use_chgflx)) {
+        c->err = "pmq_compute: the mixed form computes energy and gradient without charge flux";
         return PMQ_ERR_UNSUPPORTED;
     }
     if (terms & ~(unsigned)(PMQ_TERM_MPOLE | PMQ_TERM_POLAR)) {
@@ -493,8 +503,12 @@ int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
             if ((rc = build_neighbors(c, cut))) return rc;
         }
         if ((rc = launch_rotpole(c, true))) return rc;
-        if (do_polar && (rc = polar_prepare(c, off2))) return rc;
-        if (do_mpole && (rc = launch_mpole_direct(c, mode, off2))) return rc;
+        if (do_polar && (rc = polar_prepare(c, off2, mixed))) return rc;
+        if (do_mpole && (rc = launch_mpole_direct(c, mode, off2, mixed))) return rc;
+        if (do_mpole && mixed && grad) {
+            if ((rc = launch_fx_flush(c, c->fx_grad.p, c->dem.p, 3 * (size_t)n, PMQ_FX_GRAD))) return rc;
+            if ((rc = launch_fx_flush(c, c->fx_grad.p + 3 * (size_t)n, c->te.p, 3 * (size_t)n, PMQ_FX_GRAD))) return rc;
+        }
         if (ewald) {
             if ((rc = pme_setup(c))) return rc;
             if ((rc = pme_mpole(c, mode, do_mpole, do_polar))) return rc;
@@ -503,13 +517,18 @@ int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
             if ((rc = polar_solve(c, off2, &c->pcg_iters))) return rc;
             if ((rc = polar_energy(c))) return rc;
             if (grad) {
-                if ((rc = polar_gradient(c, off2))) return rc;
-                if ((rc = launch_torque(c, PMQ_GRADIENT, c->tep.p, c->dep.p))) return rc;
+                if ((rc = polar_gradient(c, off2, mixed))) return rc;
+                if (mixed) {
+                    if ((rc = launch_fx_flush(c, c->fx_grad.p, c->dep.p, 3 * (size_t)n, PMQ_FX_GRAD))) return rc;
+                    if ((rc = launch_fx_flush(c, c->fx_grad.p + 3 * (size_t)n, c->tep.p, 3 * (size_t)n, PMQ_FX_GRAD)))
+                        return rc;
+                }
+                if ((rc = launch_torque(c, PMQ_GRADIENT, c->tep.p, c->dep.p, mixed))) return rc;
             }
             if ((rc = polar_scatter(c, c->uind_out.p, c->uind_out.p + 3 * (size_t)n))) return rc;
         }
         if (grad && do_mpole) {
-            if ((rc = launch_torque(c, mode, c->te.p, c->dem.p))) return rc;
+            if ((rc = launch_torque(c, mode, c->te.p, c->dem.p, mixed))) return rc;
             if (cf && (rc = launch_chgflx_grad(c, mode))) return rc;
         }
     }

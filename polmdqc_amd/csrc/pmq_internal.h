@@ -117,6 +117,25 @@ struct DBuf {
     }
 };
 
+// mixed precision accumulates gradients, torques and fields in 64-bit fixed
+// point: integer atomics are order independent, so the sums (and everything
+// downstream) are bit-reproducible run to run.  Scales: 2^32 per kcal/mol/A
+// (range 2^31), 2^40 per field unit (range 2^23).
+constexpr double PMQ_FX_GRAD = 4294967296.0;
+constexpr double PMQ_FX_FIELD = 1099511627776.0;
+
+__device__ __forceinline__ void fx_add(long long *q, size_t i, double v, double scale)
+{
+    atomicAdd((unsigned long long *)(q + i), (unsigned long long)__double2ll_rn(v * scale));
+}
+
+// gradient / torque accumulation: FP64 atomics (plain form) or fixed point
+__device__ __forceinline__ void grad_add(double *d, long long *q, size_t i, double v)
+{
+    if (q) fx_add(q, i, v, PMQ_FX_GRAD);
+    else atomicAdd(d + i, v);
+}
+
 // energy / virial / count accumulators (one per context, in device memory)
 struct Totals {
     double em, ep, er, edsp;
@@ -195,6 +214,8 @@ struct pmq_ctx {
     int ntiles_cap = 0;
     // outputs
     pmq::DBuf<double> dem, te, pot, aem;
+    pmq::DBuf<long long> fx_grad;    // 6n fixed-point gradient + torque (mixed form), zero between uses
+    pmq::DBuf<long long> fx_field;   // 6 npad fixed-point fields (mixed form), zero between uses
     pmq::DBuf<pmq::Totals> totals;
 
     // ---- polarization and PME state (pmq_polar.cpp)
@@ -209,6 +230,8 @@ struct pmq_ctx {
     pmq::DBuf<double> pres, pz, pdir, pvec;      // PCG residual, preconditioned residual,
                                                  // search direction, (1/alpha - T) p
     pmq::DBuf<double> pcg;                       // PCG scalars
+    pmq::DBuf<double> pcg_part;                  // PCG per-block partial sums
+    int pcg_guess_iters = 1;                     // iterations of the last converged solve
     pmq::DBuf<double> tep;                       // 3n torques of the polarization term
     pmq::DBuf<double> dep;                       // 3n gradient of ep
     bool have_uind = false;
@@ -222,6 +245,7 @@ struct pmq_ctx {
     pmq::DBuf<double> tval;                      // 6 x tcap: T xx yy zz xy xz yz
     size_t tcap = 0;
     int nnz = 0;
+    bool tmixed = false;                         // tval holds floats (mixed form)
     // PME
     int K[3] = {0, 0, 0};
     int bsorder = 5;
@@ -260,19 +284,20 @@ namespace pmq {
 int build_neighbors(pmq_ctx *c, double cutoff);
 int launch_rotpole(pmq_ctx *c, bool want_rpole13);
 int launch_chgflx_delta(pmq_ctx *c);
-int launch_mpole_direct(pmq_ctx *c, int mode, double off2);
-int launch_torque(pmq_ctx *c, int mode, const double *te, double *de);
+int launch_mpole_direct(pmq_ctx *c, int mode, double off2, bool mixed);
+int launch_fx_flush(pmq_ctx *c, long long *q, double *d, size_t n, double scale);
+int launch_torque(pmq_ctx *c, int mode, const double *te, double *de, bool mixed);
 int pme_setup(pmq_ctx *c);
 void pme_release(pmq_ctx *c);
 int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar);
-int polar_prepare(pmq_ctx *c, double off2);
+int polar_prepare(pmq_ctx *c, double off2, bool mixed);
 int polar_solve(pmq_ctx *c, double off2, int *iters_out);
 int polar_energy(pmq_ctx *c);
-int polar_gradient(pmq_ctx *c, double off2);
+int polar_gradient(pmq_ctx *c, double off2, bool mixed);
 int polar_scatter(pmq_ctx *c, double *ud_out, double *up_out);
 int launch_chgflx_grad(pmq_ctx *c, int mode);
 int launch_zero(pmq_ctx *c, double *p, size_t n);
-int pair_matrix_build(pmq_ctx *c, double off2);
+int pair_matrix_build(pmq_ctx *c, double off2, bool mixed);
 int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst);
 } // namespace pmq
 
