@@ -264,6 +264,9 @@ struct MpoleArgs {
     double off2, f, aewald;
     double *dem, *te, *pot, *aem;
     long long *demq, *teq;   // mixed precision: 64-bit fixed-point gradient and torque (else null)
+    const unsigned long long *mmask; // per tile: 64 row masks of the in-cutoff pairs
+    void *part;              // per tile partial sums (2 sides x PMQ_TILE_NV x 64 of R)
+    double *tscal;           // per tile scalars (PMQ_TILE_NS)
     Totals *tot;
     // exclusion corrections
     const ExclPair *excl;
@@ -305,18 +308,8 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
         const R corei = CP ? a.scp[si] : R(0), alphai = CP ? a.scp[np + si] : R(0);
         __syncthreads();
 
-        // which j of the tile does this lane's i interact with
-        unsigned long long mask = 0ull;
-        if (fi & 1) {
-            const int jstart = IJ.x == IJ.y ? lane + 1 : 0;
-            for (int jj = jstart; jj < PMQ_BLOCK; jj++) {
-                const int fj = jflag[jj];
-                if (!(fj & 1) || !((fi | fj) & 2)) continue;
-                double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
-                image<double>(a.box, xr, yr, zr);
-                if (xr * xr + yr * yr + zr * zr <= a.off2) mask |= 1ull << jj;
-            }
-        }
+        // the j of the tile this lane's i interacts with
+        unsigned long long mask = a.mmask[(size_t)t * 64 + lane];
 
         double e = 0; // energy sums stay FP64 whatever R is
         R fxi = 0, fyi = 0, fzi = 0, txi = 0, tyi = 0, tzi = 0, poti = 0, aemi = 0, ein = 0;
@@ -376,27 +369,19 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
             }
         }
         __syncthreads();
+        // partial sums of this tile: side 0 = the i atoms, side 1 = the j atoms
         if (GRAD || ANA) {
-            if (fi & 1) {
-                const int p = a.perm[si];
-                if (GRAD) {
-                    grad_add(a.dem, a.demq, 3 * p + 0, fxi); grad_add(a.dem, a.demq, 3 * p + 1, fyi);
-                    grad_add(a.dem, a.demq, 3 * p + 2, fzi);
-                    grad_add(a.te, a.teq, 3 * p + 0, txi); grad_add(a.te, a.teq, 3 * p + 1, tyi);
-                    grad_add(a.te, a.teq, 3 * p + 2, tzi);
-                    if (CF) atomicAdd(&a.pot[p], (double)poti);
-                }
-                if (ANA) atomicAdd(&a.aem[p], (double)aemi);
+            R *P = (R *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
+            if (GRAD) {
+                P[0 * 64 + lane] = fxi; P[1 * 64 + lane] = fyi; P[2 * 64 + lane] = fzi;
+                P[3 * 64 + lane] = txi; P[4 * 64 + lane] = tyi; P[5 * 64 + lane] = tzi;
+                if (CF) P[6 * 64 + lane] = poti;
             }
-            if (jflag[lane] & 1) {
-                const int p = a.perm[sj];
-                if (GRAD) {
-                    for (int c = 0; c < 3; c++) grad_add(a.dem, a.demq, 3 * p + c, acc[c][lane]);
-                    for (int c = 0; c < 3; c++) grad_add(a.te, a.teq, 3 * p + c, acc[3 + c][lane]);
-                    if (CF) atomicAdd(&a.pot[p], (double)acc[6][lane]);
-                }
-                if (ANA) atomicAdd(&a.aem[p], (double)acc[7][lane]);
-            }
+            if (ANA) P[7 * 64 + lane] = aemi;
+            R *Q = P + PMQ_TILE_NV * 64;
+            if (GRAD)
+                for (int c = 0; c < 6 + (CF ? 1 : 0); c++) Q[c * 64 + lane] = acc[c][lane];
+            if (ANA) Q[7 * 64 + lane] = acc[7][lane];
         }
         // totals in FP64 whatever R is
         const double ew = wave_sum(e);
@@ -407,15 +392,12 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
         }
         if (ANA) { win = wave_sum((double)ein); nem = wave_sum(nem); }
         if (lane == 0) {
-            atomicAdd(&a.tot->em, ew);
-            if (VIR) {
-                atomicAdd(&a.tot->vir[0], wxx); atomicAdd(&a.tot->vir[1], wxy); atomicAdd(&a.tot->vir[2], wxz);
-                atomicAdd(&a.tot->vir[3], wxy); atomicAdd(&a.tot->vir[4], wyy); atomicAdd(&a.tot->vir[5], wyz);
-                atomicAdd(&a.tot->vir[6], wxz); atomicAdd(&a.tot->vir[7], wyz); atomicAdd(&a.tot->vir[8], wzz);
-            }
-            if (ANA) {
-                atomicAdd(&a.tot->einter, win);
-                atomicAdd(&a.tot->nem, (unsigned long long)nem);
+            double *S = a.tscal + (size_t)t * PMQ_TILE_NS;
+            S[0] = ew;
+            if (VIR || ANA) { // the full set is summed when either is on
+                S[1] = wxx; S[2] = wxy; S[3] = wxz; S[4] = wxy; S[5] = wyy; S[6] = wyz; S[7] = wxz; S[8] = wyz;
+                S[9] = wzz;
+                S[10] = win; S[11] = (double)nem;
             }
         }
     }
@@ -494,7 +476,7 @@ __global__ __launch_bounds__(64) void k_mpole_excl(MpoleArgs a)
 template <int PEN, bool CF, bool ANA, bool GRAD, bool VIR>
 static int launch_mpole_t(pmq_ctx *c, const MpoleArgs &a)
 {
-    const int grid = std::max(1, std::min(c->ntiles_cap, 16384));
+    const int grid = c->tgrid;
     {
         ProfScope ps_(c, "mpole_tiles");
         const bool ew = a.aewald > 0.0;
@@ -506,6 +488,26 @@ static int launch_mpole_t(pmq_ctx *c, const MpoleArgs &a)
             else hipLaunchKernelGGL((k_mpole_tiles<double, PEN, CF, ANA, GRAD, VIR, false>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
         }
     }
+    PMQ_HIP(hipGetLastError());
+    const bool mixed = a.demq != nullptr;
+    int rc;
+    if (GRAD || ANA) {
+        TileReduceArgs r{};
+        for (int q = 0; q < 3; q++) {
+            if (GRAD) {
+                r.out[q] = a.dem; r.out[3 + q] = a.te;
+            }
+            r.caller_order[q] = r.caller_order[3 + q] = 1;
+            r.stride[q] = r.stride[3 + q] = 3;
+            r.off[q] = r.off[3 + q] = q;
+            r.fxscale[q] = r.fxscale[3 + q] = PMQ_FX_GRAD;
+        }
+        if (GRAD && CF && !mixed) r.out[6] = a.pot;
+        if (ANA && !mixed) r.out[7] = a.aem;
+        for (int q = 6; q < 8; q++) { r.caller_order[q] = 1; r.stride[q] = 1; r.off[q] = 0; r.fxscale[q] = PMQ_FX_GRAD; }
+        if ((rc = tile_reduce(c, r, 8, mixed, mixed))) return rc;
+    }
+    if ((rc = tile_scalars(c, 0, VIR || ANA))) return rc;
     if (a.nexcl > 0)
         hipLaunchKernelGGL((k_mpole_excl<PEN, CF, ANA, GRAD, VIR>), dim3((a.nexcl + 63) / 64), dim3(64), 0,
                            c->stream, a);
@@ -540,6 +542,9 @@ int launch_mpole_direct(pmq_ctx *c, int mode, double off2, bool mixed)
     a.dem = c->dem.p; a.te = c->te.p; a.pot = c->pot.p; a.aem = c->aem.p;
     a.demq = mixed ? c->fx_grad.p : nullptr;
     a.teq = mixed ? c->fx_grad.p + 3 * (size_t)c->n : nullptr;
+    a.mmask = c->mmask.p;
+    a.part = c->tpart.p;
+    a.tscal = c->tscal.p;
     a.tot = c->totals.p;
     a.excl = c->excl.p;
     a.nexcl = (int)c->h_excl.size();

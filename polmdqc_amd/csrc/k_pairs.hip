@@ -1,5 +1,13 @@
-// k_pairs.hip -- in-cutoff pair masks and the dipole-field operator T as a
-// row-owned sparse matrix.
+// k_pairs.hip -- the per-evaluation tile structure shared by the pair
+// kernels (in-cutoff masks, per-block tile lists, deterministic reduction of
+// per-tile partial sums) and the dipole-field operator T as a row-owned
+// sparse matrix.
+//
+// Pair kernels over (I,J) tiles do not accumulate into per-atom arrays with
+// atomics: every tile stores its i-side and j-side partial sums, and
+// k_tile_reduce adds, for every atom, the partials of the tiles of its block
+// in a fixed order (the block's tile list), so per-atom results are
+// bit-reproducible and no global atomics are issued.
 //
 // The induced-dipole solve applies T (the Thole-damped, Ewald real-space
 // dipole-dipole field tensor, u-scaled for excluded pairs) to a new pair of
@@ -40,7 +48,8 @@ struct MaskArgs {
     const int *sflag;
     DevBox box;
     double off2;
-    unsigned long long *mask; // 128 per tile: [0,64) row masks, [64,128) column masks
+    unsigned long long *mask;  // 128 per tile: [0,64) row masks, [64,128) column masks
+    unsigned long long *mmask; // 64 per tile: row masks of the pairs with an active atom (multipole energy)
 };
 
 __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
@@ -58,13 +67,17 @@ __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
         const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
         const int fi = a.sflag[si];
         __syncthreads();
-        unsigned long long m = 0ull;
+        unsigned long long m = 0ull, mm = 0ull;
         if (fi & 1) {
             for (int jj = IJ.x == IJ.y ? lane + 1 : 0; jj < 64; jj++) {
-                if (!(jflag[jj] & 1)) continue;
+                const int fj = jflag[jj];
+                if (!(fj & 1)) continue;
                 double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
                 image<double>(a.box, xr, yr, zr);
-                if (xr * xr + yr * yr + zr * zr <= a.off2) m |= 1ull << jj;
+                if (xr * xr + yr * yr + zr * zr <= a.off2) {
+                    m |= 1ull << jj;
+                    if ((fi | fj) & 2) mm |= 1ull << jj;
+                }
             }
         }
         // column mask of lane c: bit l set where lane l's row mask has bit c
@@ -75,6 +88,7 @@ __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
         }
         a.mask[(size_t)t * 128 + lane] = m;
         a.mask[(size_t)t * 128 + 64 + lane] = col;
+        a.mmask[(size_t)t * 64 + lane] = mm;
     }
 }
 
@@ -343,30 +357,42 @@ static int pair_fill(pmq_ctx *c, V *val, int tgrid)
     return PMQ_OK;
 }
 
-int pair_matrix_build(pmq_ctx *c, double off2, bool mixed)
+int tile_structure(pmq_ctx *c, double off2)
 {
-    const int nblk = c->nblk, np = c->npad;
+    const int nblk = c->nblk;
     hipStream_t st = c->stream;
     // tile count of this evaluation
     int ntiles = 0;
     PMQ_HIP(hipMemcpyAsync(&ntiles, c->ntiles_d.p, sizeof(int), hipMemcpyDeviceToHost, st));
     PMQ_HIP(hipStreamSynchronize(st));
+    c->ntiles = ntiles;
     const size_t nt = (size_t)std::max(ntiles, 1);
     PMQ_HIP(c->tmask.alloc(128 * nt));
+    PMQ_HIP(c->mmask.alloc(64 * nt));
     PMQ_HIP(c->soff.alloc(128 * nt));
     PMQ_HIP(c->adj.alloc((size_t)nblk * nblk));
     PMQ_HIP(c->adjn.alloc(nblk));
-    PMQ_HIP(c->rowcnt.alloc(np + 1));
-    PMQ_HIP(c->rowptr.alloc(np + 1));
-    const int tgrid = (int)std::max<size_t>(1, std::min<size_t>(nt, 16384));
+    PMQ_HIP(c->tpart.alloc(nt * 2 * PMQ_TILE_NV * 64));
+    PMQ_HIP(c->tscal.alloc(nt * PMQ_TILE_NS));
+    c->tgrid = (int)std::max<size_t>(1, std::min<size_t>(nt, 65536));
 
     MaskArgs ma;
     ma.tiles = c->tiles.p; ma.ntiles = c->ntiles_d.p;
     ma.sx = c->sx.p; ma.sy = c->sy.p; ma.sz = c->sz.p; ma.sflag = c->sflag.p;
-    ma.box = c->box; ma.off2 = off2; ma.mask = c->tmask.p;
-    hipLaunchKernelGGL(k_tile_masks, dim3(tgrid), dim3(64), 0, st, ma);
+    ma.box = c->box; ma.off2 = off2; ma.mask = c->tmask.p; ma.mmask = c->mmask.p;
+    hipLaunchKernelGGL(k_tile_masks, dim3(c->tgrid), dim3(64), 0, st, ma);
     hipLaunchKernelGGL(k_block_adj, dim3(nblk), dim3(64), 0, st, nblk, c->tileflag.p, c->tilescan.p, c->adj.p,
                        c->adjn.p);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+int pair_matrix_build(pmq_ctx *c, bool mixed)
+{
+    const int nblk = c->nblk, np = c->npad;
+    hipStream_t st = c->stream;
+    PMQ_HIP(c->rowcnt.alloc(np + 1));
+    PMQ_HIP(c->rowptr.alloc(np + 1));
     hipLaunchKernelGGL(k_row_counts, dim3(nblk), dim3(64), 0, st, nblk, c->tiles.p, c->tmask.p, c->adj.p, c->adjn.p,
                        c->soff.p, c->rowcnt.p);
     PMQ_HIP(hipGetLastError());
@@ -387,8 +413,106 @@ int pair_matrix_build(pmq_ctx *c, double off2, bool mixed)
     c->nnz = nnz;
 
     c->tmixed = mixed;
-    if (mixed) return pair_fill<float, float>(c, (float *)c->tval.p, tgrid);
-    return pair_fill<double, double>(c, c->tval.p, tgrid);
+    if (mixed) return pair_fill<float, float>(c, (float *)c->tval.p, c->tgrid);
+    return pair_fill<double, double>(c, c->tval.p, c->tgrid);
+}
+
+// ---------------------------------------------------------------------------
+// deterministic reduction of tile partials
+
+// per-atom sums of NV partials over the tiles of the atom's block; P is the
+// partial type (the pair arithmetic), FX sums in 64-bit fixed point
+template <class P, int NV, bool FX>
+__global__ __launch_bounds__(64) void k_tile_reduce(TileReduceArgs a)
+{
+    const int I = blockIdx.x, lane = threadIdx.x;
+    const int si = I * 64 + lane;
+    const P *part = (const P *)a.part;
+    double acc[NV];
+    long long q[NV];
+#pragma unroll
+    for (int c = 0; c < NV; c++) { acc[c] = 0.0; q[c] = 0; }
+    const int na = a.adjn[I];
+    for (int s = 0; s < na; s++) {
+        const int t = a.adj[(size_t)I * a.nblk + s];
+        const int2 IJ = a.tiles[t];
+        for (int side = 0; side < 2; side++) {
+            if (IJ.x != IJ.y && side != (IJ.x == I ? 0 : 1)) continue;
+            const P *pp = part + ((size_t)t * 2 + side) * PMQ_TILE_NV * 64 + lane;
+#pragma unroll
+            for (int c = 0; c < NV; c++) {
+                if (!a.out[c]) continue; // value not produced by this launch
+                if (FX) q[c] += __double2ll_rn((double)pp[c * 64] * a.fxscale[c]);
+                else acc[c] += (double)pp[c * 64];
+            }
+        }
+    }
+    if (si >= a.n) return;
+#pragma unroll
+    for (int c = 0; c < NV; c++) {
+        const double v = FX ? (double)q[c] / a.fxscale[c] : acc[c];
+        if (!a.out[c]) continue;
+        const size_t idx = a.caller_order[c] ? (size_t)a.stride[c] * a.perm[si] + a.off[c] : (size_t)a.stride[c] * si + a.off[c];
+        a.out[c][idx] += v;
+        if (a.out2[c]) a.out2[c][idx] += v;
+    }
+}
+
+// fixed-order sums of the per-tile scalars into the totals: scalar 0 into
+// energy component `which` (0 em, 1 ep, 2 er, 3 edsp); with `full` also the
+// virial (1..9), the inter-molecular energy (10) and the pair count (11)
+__global__ __launch_bounds__(256) void k_tile_scalars(int ntiles, int which, int full, const double *__restrict__ tscal,
+                                                      Totals *tot)
+{
+    __shared__ double red[256];
+    const int ns = full ? PMQ_TILE_NS : 1;
+    for (int k = 0; k < ns; k++) {
+        double v = 0.0;
+        for (int t = threadIdx.x; t < ntiles; t += 256) v += tscal[(size_t)t * PMQ_TILE_NS + k];
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            const double r = red[0];
+            if (k == 0) atomicAdd(&(&tot->em)[which], r);
+            else if (k <= 9) atomicAdd(&tot->vir[k - 1], r);
+            else if (k == 10) atomicAdd(&tot->einter, r);
+            else atomicAdd(&tot->nem, (unsigned long long)__double2ll_rn(r));
+        }
+        __syncthreads();
+    }
+}
+
+int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fixed)
+{
+    a.nblk = c->nblk; a.n = c->n;
+    a.tiles = c->tiles.p; a.adj = c->adj.p; a.adjn = c->adjn.p; a.perm = c->perm.p;
+    a.part = c->tpart.p;
+    const dim3 g(c->nblk), b(64);
+#define PMQ_RED(P, NV, FX) hipLaunchKernelGGL((k_tile_reduce<P, NV, FX>), g, b, 0, c->stream, a)
+    if (float_parts) {
+        if (nv == 3) { if (fixed) PMQ_RED(float, 3, true); else PMQ_RED(float, 3, false); }
+        else if (nv == 6) { if (fixed) PMQ_RED(float, 6, true); else PMQ_RED(float, 6, false); }
+        else { if (fixed) PMQ_RED(float, 8, true); else PMQ_RED(float, 8, false); }
+    } else {
+        if (nv == 3) PMQ_RED(double, 3, false);
+        else if (nv == 6) PMQ_RED(double, 6, false);
+        else PMQ_RED(double, 8, false);
+    }
+#undef PMQ_RED
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+int tile_scalars(pmq_ctx *c, int which, bool full)
+{
+    hipLaunchKernelGGL(k_tile_scalars, dim3(1), dim3(256), 0, c->stream, c->ntiles, which, full ? 1 : 0, c->tscal.p,
+                       c->totals.p);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
 }
 
 template <class V>

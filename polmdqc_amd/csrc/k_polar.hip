@@ -48,6 +48,7 @@ struct PolArgs {
     double *dst;         // fields (2 x 3 x npad)
     double *dep, *tep;   // caller-order gradient and torque
     long long *dstq;     // mixed precision: fixed-point fields (else null)
+    void *part;          // per tile partial sums (2 sides x PMQ_TILE_NV x 64 of the pair arithmetic type)
     long long *depq, *tepq; // mixed precision: fixed-point gradient and torque (else null)
 };
 
@@ -151,20 +152,11 @@ __global__ __launch_bounds__(64) void k_pol_field_tiles(PolArgs a)
             atomicAdd(&acc[2][jj], Ek[2]);
         }
         __syncthreads();
-        if (fi & 1)
-            for (int set = 0; set < 2; set++) {
-                const size_t o = (size_t)set * 3 * np;
-                field_add(a.dst, a.dstq, o + si, ex);
-                field_add(a.dst, a.dstq, o + np + si, ey);
-                field_add(a.dst, a.dstq, o + 2 * np + si, ez);
-            }
-        if (jflag[lane] & 1)
-            for (int set = 0; set < 2; set++) {
-                const size_t o = (size_t)set * 3 * np;
-                field_add(a.dst, a.dstq, o + sj, acc[0][lane]);
-                field_add(a.dst, a.dstq, o + np + sj, acc[1][lane]);
-                field_add(a.dst, a.dstq, o + 2 * np + sj, acc[2][lane]);
-            }
+        // partial sums: side 0 the i atoms, side 1 the j atoms
+        T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
+        P[lane] = ex; P[64 + lane] = ey; P[128 + lane] = ez;
+        T *Q = P + PMQ_TILE_NV * 64;
+        Q[lane] = acc[0][lane]; Q[64 + lane] = acc[1][lane]; Q[128 + lane] = acc[2][lane];
     }
 }
 
@@ -292,20 +284,14 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
             atomicAdd(&acc[5][jj], tk[2]);
         }
         __syncthreads();
-        if (fi & 1) {
-            const int p = a.perm[si];
-            for (int c = 0; c < 3; c++) {
-                grad_add(a.dep, a.depq, 3 * p + c, hf * F[c]);
-                grad_add(a.tep, a.tepq, 3 * p + c, hf * Ti[c]);
-            }
+        // partial sums: side 0 the i atoms, side 1 the j atoms
+        T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
+        T *Q = P + PMQ_TILE_NV * 64;
+        for (int c = 0; c < 3; c++) {
+            P[c * 64 + lane] = T(hf) * F[c];
+            P[(3 + c) * 64 + lane] = T(hf) * Ti[c];
         }
-        if (jflag[lane] & 1) {
-            const int p = a.perm[sj];
-            for (int c = 0; c < 3; c++) {
-                grad_add(a.dep, a.depq, 3 * p + c, hf * acc[c][lane]);
-                grad_add(a.tep, a.tepq, 3 * p + c, hf * acc[3 + c][lane]);
-            }
-        }
+        for (int c = 0; c < 6; c++) Q[c * 64 + lane] = T(hf) * acc[c][lane];
     }
 }
 
@@ -585,10 +571,11 @@ static PolArgs pol_args(pmq_ctx *c, double off2)
     a.mask = c->tmask.p;
     a.src = nullptr; a.dst = nullptr; a.dep = c->dep.p; a.tep = c->tep.p;
     a.dstq = nullptr; a.depq = nullptr; a.tepq = nullptr;
+    a.part = c->tpart.p;
     return a;
 }
 
-static int tile_grid(pmq_ctx *c) { return std::max(1, std::min(c->ntiles_cap, 16384)); }
+static int tile_grid(pmq_ctx *c) { return c->tgrid; }
 
 // dst = T src (real space, from the pair matrix) + reciprocal and self field
 static int mutual_field(pmq_ctx *c, const double *src, double *dst)
@@ -606,7 +593,7 @@ int polar_prepare(pmq_ctx *c, double off2, bool mixed)
     hipLaunchKernelGGL(k_pol_sort, dim3((np + 255) / 256), dim3(256), 0, c->stream, c->n, np, c->perm.p,
                        c->polarity.p, c->thole.p, c->pdamp.p, c->spol.p);
     int rc;
-    if ((rc = pair_matrix_build(c, off2, mixed))) return rc; // also the tile masks the kernels below read
+    if ((rc = pair_matrix_build(c, mixed))) return rc;
     if ((rc = launch_zero(c, c->pfield.p, 6 * (size_t)np))) return rc;
     PolArgs a = pol_args(c, off2);
     a.dst = c->pfield.p;
@@ -622,6 +609,15 @@ int polar_prepare(pmq_ctx *c, double off2, bool mixed)
             else hipLaunchKernelGGL((k_pol_field_tiles<false, double>), g, dim3(64), 0, c->stream, a);
         }
     }
+    PMQ_HIP(hipGetLastError());
+    TileReduceArgs r{};
+    for (int q = 0; q < 3; q++) {
+        r.out[q] = c->pfield.p + (size_t)q * np;
+        r.out2[q] = c->pfield.p + (size_t)(3 + q) * np;
+        r.stride[q] = 1;
+        r.fxscale[q] = PMQ_FX_FIELD;
+    }
+    if ((rc = tile_reduce(c, r, 3, mixed, mixed))) return rc;
     if (a.nexcl) hipLaunchKernelGGL(k_pol_field_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     if (mixed && (rc = launch_fx_flush(c, c->fx_field.p, c->pfield.p, 6 * (size_t)np, PMQ_FX_FIELD))) return rc;
@@ -709,6 +705,17 @@ int polar_gradient(pmq_ctx *c, double off2, bool mixed)
             else hipLaunchKernelGGL((k_pol_grad_tiles<false, double>), g, dim3(64), 0, c->stream, a);
         }
     }
+    PMQ_HIP(hipGetLastError());
+    TileReduceArgs r{};
+    for (int q = 0; q < 3; q++) {
+        r.out[q] = c->dep.p; r.out[3 + q] = c->tep.p;
+        r.caller_order[q] = r.caller_order[3 + q] = 1;
+        r.stride[q] = r.stride[3 + q] = 3;
+        r.off[q] = r.off[3 + q] = q;
+        r.fxscale[q] = r.fxscale[3 + q] = PMQ_FX_GRAD;
+    }
+    int rc;
+    if ((rc = tile_reduce(c, r, 6, mixed, mixed))) return rc;
     if (a.nexcl) hipLaunchKernelGGL(k_pol_grad_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     if (a.aewald > 0) return pme_polar_grad(c);

@@ -501,6 +501,7 @@ int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
         {
             ProfScope ps(c, "neighbors");
             if ((rc = build_neighbors(c, cut))) return rc;
+            if ((rc = tile_structure(c, off2))) return rc;
         }
         if ((rc = launch_rotpole(c, true))) return rc;
         if (do_polar && (rc = polar_prepare(c, off2, mixed))) return rc;

@@ -136,6 +136,24 @@ __device__ __forceinline__ void grad_add(double *d, long long *q, size_t i, doub
     else atomicAdd(d + i, v);
 }
 
+// per-tile partial sums of the pair kernels (k_pairs.hip): values per lane
+// and side, scalars per tile (0 energy, 1..9 virial, 10 inter-molecular
+// energy, 11 pair count)
+constexpr int PMQ_TILE_NV = 8;
+constexpr int PMQ_TILE_NS = 12;
+
+struct TileReduceArgs {
+    int nblk, n;
+    const int2 *tiles;
+    const int *adj, *adjn, *perm;
+    const void *part;
+    double *out[8];       // per value: destination (or null)
+    double *out2[8];      // second destination of the same value (or null)
+    int caller_order[8];  // index by caller order (perm) or by sorted order
+    int stride[8], off[8];
+    double fxscale[8];    // fixed-point scale of each value (mixed form)
+};
+
 // energy / virial / count accumulators (one per context, in device memory)
 struct Totals {
     double em, ep, er, edsp;
@@ -237,6 +255,10 @@ struct pmq_ctx {
     bool have_uind = false;
     // dipole-field operator T as a row-owned sparse matrix (k_pairs.hip)
     pmq::DBuf<unsigned long long> tmask;         // 128 per tile: 64 row masks, 64 column masks
+    pmq::DBuf<unsigned long long> mmask;         // 64 per tile: row masks of pairs with an active atom
+    pmq::DBuf<double> tpart;                     // per tile: 2 sides x PMQ_TILE_NV x 64 partial sums
+    pmq::DBuf<double> tscal;                     // per tile: PMQ_TILE_NS scalars (energy, virial, ...)
+    int tgrid = 1;                               // workgroups of the tile kernels
     pmq::DBuf<int> soff;                         // 128 per tile: each lane's entry offset within its row
     pmq::DBuf<int> adj, adjn;                    // per block: tiles it takes part in (nblk x nblk), count
     pmq::DBuf<int> rowcnt, rowptr;               // npad + 1
@@ -297,7 +319,10 @@ int polar_gradient(pmq_ctx *c, double off2, bool mixed);
 int polar_scatter(pmq_ctx *c, double *ud_out, double *up_out);
 int launch_chgflx_grad(pmq_ctx *c, int mode);
 int launch_zero(pmq_ctx *c, double *p, size_t n);
-int pair_matrix_build(pmq_ctx *c, double off2, bool mixed);
+int tile_structure(pmq_ctx *c, double off2);
+int pair_matrix_build(pmq_ctx *c, bool mixed);
+int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fixed);
+int tile_scalars(pmq_ctx *c, int which, bool full);
 int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst);
 } // namespace pmq
 
