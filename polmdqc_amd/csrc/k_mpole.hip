@@ -532,7 +532,7 @@ int launch_mpole_direct(pmq_ctx *c, int mode, double off2, bool mixed)
     a.n = c->n;
     a.npad = c->npad;
     a.tiles = c->tiles.p;
-    a.ntiles = c->ntiles_d.p;
+    a.ntiles = c->ntiles_eff.p;
     a.sx = c->sx.p; a.sy = c->sy.p; a.sz = c->sz.p;
     a.smp = c->smp.p; a.scp = c->scp.p;
     a.sflag = c->sflag.p; a.smol = c->smol.p; a.perm = c->perm.p; a.iperm = c->iperm.p;

@@ -92,9 +92,11 @@ __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
     }
 }
 
-// tiles of block I in the order of the other block J (one wave per block)
-__global__ __launch_bounds__(64) void k_block_adj(int nblk, const int *__restrict__ flag, const int *__restrict__ scan,
-                                                  int *__restrict__ adj, int *__restrict__ adjn)
+// tiles of block I in the order of the other block J (one wave per block);
+// tiles beyond the capacity are left out (the evaluation is then redone)
+__global__ __launch_bounds__(64) void k_block_adj(int nblk, int cap, const int *__restrict__ flag,
+                                                  const int *__restrict__ scan, int *__restrict__ adj,
+                                                  int *__restrict__ adjn)
 {
     const int I = blockIdx.x, lane = threadIdx.x;
     int count = 0;
@@ -105,7 +107,7 @@ __global__ __launch_bounds__(64) void k_block_adj(int nblk, const int *__restric
         if (J < nblk) {
             const long long lo = min(I, J), hi = max(I, J);
             tri = lo * nblk - lo * (lo - 1) / 2 + (hi - lo);
-            f = flag[tri];
+            f = flag[tri] && scan[tri] < cap;
         }
         const unsigned long long b = __ballot(f != 0);
         const int pos = count + __popcll(b & ((1ull << lane) - 1ull));
@@ -113,6 +115,30 @@ __global__ __launch_bounds__(64) void k_block_adj(int nblk, const int *__restric
         count += __popcll(b);
     }
     if (lane == 0) adjn[I] = count;
+}
+
+__global__ void k_status_reset(RunStatus *st)
+{
+    if (threadIdx.x == 0) *st = RunStatus{0, 0, 0, 0};
+}
+
+// the tile count the tile kernels loop over, and the overflow check
+__global__ void k_tile_clamp(const int *ntiles, int cap, int *eff, RunStatus *st)
+{
+    if (threadIdx.x == 0) {
+        const int n = *ntiles;
+        *eff = min(n, cap);
+        st->ntiles = n;
+        if (n > cap) st->overflow |= 1;
+    }
+}
+
+__global__ void k_nnz_check(const int *total, size_t cap, RunStatus *st)
+{
+    if (threadIdx.x == 0) {
+        st->nnz = *total;
+        if ((size_t)*total > cap) st->overflow |= 2;
+    }
 }
 
 // the mask of lane `lane` of block I for tile t, in the orientation of I
@@ -216,10 +242,14 @@ __global__ __launch_bounds__(64) void k_pair_fill(FillArgs<V> a)
             pol_series<EW, T>(r2, T(a.aewald), lam, R);
             const size_t ei = (size_t)ibase + __popcll(mi & ((1ull << jj) - 1ull));
             const size_t ej = (size_t)jbase[jj] + __popcll(jmask[jj] & ((1ull << lane) - 1ull));
-            a.col[ei] = IJ.y * 64 + jj;
-            a.col[ej] = si;
-            store_T(a.val, a.cap, ei, R, xr, yr, zr);
-            store_T(a.val, a.cap, ej, R, xr, yr, zr);
+            if (ei < a.cap) {
+                a.col[ei] = IJ.y * 64 + jj;
+                store_T(a.val, a.cap, ei, R, xr, yr, zr);
+            }
+            if (ej < a.cap) {
+                a.col[ej] = si;
+                store_T(a.val, a.cap, ej, R, xr, yr, zr);
+            }
         }
     }
 }
@@ -260,10 +290,11 @@ __global__ __launch_bounds__(64) void k_excl_fold(FoldArgs<V> a)
     const int np = a.npad;
     const int si = a.iperm[q.i], sk = a.iperm[q.k];
     int ei = -1, ek = -1;
-    for (int e = a.rowptr[si]; e < a.rowptr[si + 1]; e++)
+    const int ci = (int)min((size_t)a.rowptr[si + 1], a.cap), ck = (int)min((size_t)a.rowptr[sk + 1], a.cap);
+    for (int e = a.rowptr[si]; e < ci; e++)
         if (a.col[e] == sk) { ei = e; break; }
     if (ei < 0) return; // beyond the cutoff: not in the matrix
-    for (int e = a.rowptr[sk]; e < a.rowptr[sk + 1]; e++)
+    for (int e = a.rowptr[sk]; e < ck; e++)
         if (a.col[e] == si) { ek = e; break; }
     double xr = a.sx[sk] - a.sx[si], yr = a.sy[sk] - a.sy[si], zr = a.sz[sk] - a.sz[si];
     image<double>(a.box, xr, yr, zr);
@@ -294,7 +325,7 @@ __global__ __launch_bounds__(256) void k_pol_spmv(SpmvArgs<V> a)
     const int np = a.npad;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     if (row < np) {
-        const int e1 = a.rowptr[row + 1];
+        const int e1 = (int)min((size_t)a.rowptr[row + 1], a.cap);
         for (int e = a.rowptr[row] + g; e < e1; e += G) {
             const int k = a.col[e];
             const double txx = a.val[e], tyy = a.val[a.cap + e], tzz = a.val[2 * a.cap + e];
@@ -331,7 +362,7 @@ static int pair_fill(pmq_ctx *c, V *val, int tgrid)
     const int np = c->npad;
     hipStream_t st = c->stream;
     FillArgs<V> fa;
-    fa.tiles = c->tiles.p; fa.ntiles = c->ntiles_d.p;
+    fa.tiles = c->tiles.p; fa.ntiles = c->ntiles_eff.p;
     fa.sx = c->sx.p; fa.sy = c->sy.p; fa.sz = c->sz.p; fa.spol = c->spol.p;
     fa.box = c->box;
     fa.aewald = c->prm.use_ewald ? c->prm.aewald : 0.0;
@@ -357,16 +388,26 @@ static int pair_fill(pmq_ctx *c, V *val, int tgrid)
     return PMQ_OK;
 }
 
-int tile_structure(pmq_ctx *c, double off2)
+int status_reset(pmq_ctx *c)
+{
+    hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(1), 0, c->stream, c->status.p);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+int tile_structure(pmq_ctx *c, double off2, bool safe)
 {
     const int nblk = c->nblk;
     hipStream_t st = c->stream;
-    // tile count of this evaluation
-    int ntiles = 0;
-    PMQ_HIP(hipMemcpyAsync(&ntiles, c->ntiles_d.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    PMQ_HIP(hipStreamSynchronize(st));
-    c->ntiles = ntiles;
-    const size_t nt = (size_t)std::max(ntiles, 1);
+    if (safe || c->tile_cap == 0) {
+        // size the tile buffers from this evaluation's count (one round trip)
+        int ntiles = 0;
+        PMQ_HIP(hipMemcpyAsync(&ntiles, c->ntiles_d.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        PMQ_HIP(hipStreamSynchronize(st));
+        if (ntiles > c->tile_cap || c->tile_cap == 0)
+            c->tile_cap = std::min(std::max(1, ntiles + ntiles / 4 + 16), std::max(1, c->ntiles_cap));
+    }
+    const size_t nt = (size_t)c->tile_cap;
     PMQ_HIP(c->tmask.alloc(128 * nt));
     PMQ_HIP(c->mmask.alloc(64 * nt));
     PMQ_HIP(c->soff.alloc(128 * nt));
@@ -375,19 +416,21 @@ int tile_structure(pmq_ctx *c, double off2)
     PMQ_HIP(c->tpart.alloc(nt * 2 * PMQ_TILE_NV * 64));
     PMQ_HIP(c->tscal.alloc(nt * PMQ_TILE_NS));
     c->tgrid = (int)std::max<size_t>(1, std::min<size_t>(nt, 65536));
+    hipLaunchKernelGGL(k_tile_clamp, dim3(1), dim3(1), 0, st, c->ntiles_d.p, c->tile_cap, c->ntiles_eff.p,
+                       c->status.p);
 
     MaskArgs ma;
-    ma.tiles = c->tiles.p; ma.ntiles = c->ntiles_d.p;
+    ma.tiles = c->tiles.p; ma.ntiles = c->ntiles_eff.p;
     ma.sx = c->sx.p; ma.sy = c->sy.p; ma.sz = c->sz.p; ma.sflag = c->sflag.p;
     ma.box = c->box; ma.off2 = off2; ma.mask = c->tmask.p; ma.mmask = c->mmask.p;
     hipLaunchKernelGGL(k_tile_masks, dim3(c->tgrid), dim3(64), 0, st, ma);
-    hipLaunchKernelGGL(k_block_adj, dim3(nblk), dim3(64), 0, st, nblk, c->tileflag.p, c->tilescan.p, c->adj.p,
-                       c->adjn.p);
+    hipLaunchKernelGGL(k_block_adj, dim3(nblk), dim3(64), 0, st, nblk, c->tile_cap, c->tileflag.p, c->tilescan.p,
+                       c->adj.p, c->adjn.p);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }
 
-int pair_matrix_build(pmq_ctx *c, bool mixed)
+int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe)
 {
     const int nblk = c->nblk, np = c->npad;
     hipStream_t st = c->stream;
@@ -402,15 +445,17 @@ int pair_matrix_build(pmq_ctx *c, bool mixed)
     PMQ_HIP(c->pscan_tmp.alloc(bytes));
     PMQ_HIP(rocprim::exclusive_scan(c->pscan_tmp.p, bytes, c->rowcnt.p, c->rowptr.p, 0, (size_t)np + 1,
                                     rocprim::plus<int>(), st));
-    int nnz = 0;
-    PMQ_HIP(hipMemcpyAsync(&nnz, c->rowptr.p + np, sizeof(int), hipMemcpyDeviceToHost, st));
-    PMQ_HIP(hipStreamSynchronize(st));
-    if ((size_t)nnz > c->tcap) {
-        c->tcap = (size_t)(nnz * 1.25) + 64;
-        PMQ_HIP(c->tcol.alloc(c->tcap));
-        PMQ_HIP(c->tval.alloc(6 * c->tcap));
+    if (safe || c->tcap == 0) {
+        int nnz = 0;
+        PMQ_HIP(hipMemcpyAsync(&nnz, c->rowptr.p + np, sizeof(int), hipMemcpyDeviceToHost, st));
+        PMQ_HIP(hipStreamSynchronize(st));
+        if ((size_t)nnz > c->tcap || c->tcap == 0) {
+            c->tcap = (size_t)nnz + (size_t)nnz / 4 + 64;
+            PMQ_HIP(c->tcol.alloc(c->tcap));
+            PMQ_HIP(c->tval.alloc(6 * c->tcap));
+        }
     }
-    c->nnz = nnz;
+    hipLaunchKernelGGL(k_nnz_check, dim3(1), dim3(1), 0, st, c->rowptr.p + np, c->tcap, c->status.p);
 
     c->tmixed = mixed;
     if (mixed) return pair_fill<float, float>(c, (float *)c->tval.p, c->tgrid);
@@ -461,10 +506,11 @@ __global__ __launch_bounds__(64) void k_tile_reduce(TileReduceArgs a)
 // fixed-order sums of the per-tile scalars into the totals: scalar 0 into
 // energy component `which` (0 em, 1 ep, 2 er, 3 edsp); with `full` also the
 // virial (1..9), the inter-molecular energy (10) and the pair count (11)
-__global__ __launch_bounds__(256) void k_tile_scalars(int ntiles, int which, int full, const double *__restrict__ tscal,
-                                                      Totals *tot)
+__global__ __launch_bounds__(256) void k_tile_scalars(const int *ntiles_p, int which, int full,
+                                                      const double *__restrict__ tscal, Totals *tot)
 {
     __shared__ double red[256];
+    const int ntiles = *ntiles_p;
     const int ns = full ? PMQ_TILE_NS : 1;
     for (int k = 0; k < ns; k++) {
         double v = 0.0;
@@ -509,8 +555,8 @@ int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fi
 
 int tile_scalars(pmq_ctx *c, int which, bool full)
 {
-    hipLaunchKernelGGL(k_tile_scalars, dim3(1), dim3(256), 0, c->stream, c->ntiles, which, full ? 1 : 0, c->tscal.p,
-                       c->totals.p);
+    hipLaunchKernelGGL(k_tile_scalars, dim3(1), dim3(256), 0, c->stream, c->ntiles_eff.p, which, full ? 1 : 0,
+                       c->tscal.p, c->totals.p);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }

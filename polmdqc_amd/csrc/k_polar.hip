@@ -559,7 +559,7 @@ static PolArgs pol_args(pmq_ctx *c, double off2)
 {
     PolArgs a;
     a.n = c->n; a.npad = c->npad;
-    a.tiles = c->tiles.p; a.ntiles = c->ntiles_d.p;
+    a.tiles = c->tiles.p; a.ntiles = c->ntiles_eff.p;
     a.sx = c->sx.p; a.sy = c->sy.p; a.sz = c->sz.p; a.smp = c->smp.p; a.spol = c->spol.p;
     a.sflag = c->sflag.p; a.perm = c->perm.p; a.iperm = c->iperm.p;
     a.box = c->box;
@@ -586,14 +586,14 @@ static int mutual_field(pmq_ctx *c, const double *src, double *dst)
     return PMQ_OK;
 }
 
-int polar_prepare(pmq_ctx *c, double off2, bool mixed)
+int polar_prepare(pmq_ctx *c, double off2, bool mixed, bool safe)
 {
     // sorted parameters, zeroed permanent field, real-space permanent field
     const int np = c->npad;
     hipLaunchKernelGGL(k_pol_sort, dim3((np + 255) / 256), dim3(256), 0, c->stream, c->n, np, c->perm.p,
                        c->polarity.p, c->thole.p, c->pdamp.p, c->spol.p);
     int rc;
-    if ((rc = pair_matrix_build(c, mixed))) return rc;
+    if ((rc = pair_matrix_build(c, mixed, safe))) return rc;
     if ((rc = launch_zero(c, c->pfield.p, 6 * (size_t)np))) return rc;
     PolArgs a = pol_args(c, off2);
     a.dst = c->pfield.p;
@@ -624,9 +624,13 @@ int polar_prepare(pmq_ctx *c, double off2, bool mixed)
     return PMQ_OK;
 }
 
-int polar_solve(pmq_ctx *c, double off2, int *iters_out)
+// safe: iterations go out one at a time, each followed by a host check of
+// convergence; otherwise one batch of (last count + 1) iterations is
+// enqueued with no round trip, iterations past convergence return at once,
+// and the count is read back with the evaluation's status (it is redone
+// safely if the batch fell short)
+int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
 {
-    (void)off2;
     const int np = c->npad;
     const size_t nv = 6 * (size_t)np;
     const int gb = (int)((nv + PCG_TB - 1) / PCG_TB);
@@ -648,9 +652,8 @@ int polar_solve(pmq_ctx *c, double off2, int *iters_out)
     hipLaunchKernelGGL(k_pcg_init, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
     hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(PCG_TB), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
-    // iterations go out in batches; the first batch is the previous solve's
-    // count (a good guess along a trajectory), later ones are single steps
-    int batch = std::max(1, c->pcg_guess_iters);
+    const bool spec = !safe && c->pcg_guess_iters > 0;
+    int batch = spec ? std::min(c->pcg_guess_iters + 1, c->prm.politer) : 1;
     int it = 0;
     double sc[11];
     bool done = false;
@@ -666,10 +669,13 @@ int polar_solve(pmq_ctx *c, double off2, int *iters_out)
             hipLaunchKernelGGL(k_pcg_c, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
             PMQ_HIP(hipGetLastError());
         }
+        if (spec) {
+            *iters_out = 0; // known after the status check
+            return PMQ_OK;
+        }
         PMQ_HIP(hipMemcpyAsync(sc, c->pcg.p, sizeof sc, hipMemcpyDeviceToHost, c->stream));
         PMQ_HIP(hipStreamSynchronize(c->stream));
         done = sc[8] != 0.0;
-        batch = 1;
     }
     const int iters = done ? (int)sc[9] : it;
     if (done) c->pcg_guess_iters = iters;

@@ -109,7 +109,9 @@ int pmq_create(const pmq_config *cfg, pmq_ctx **out)
         delete c;
         return PMQ_ERR_HIP;
     }
-    if (c->totals.alloc(1) != hipSuccess || c->bbox.alloc(6) != hipSuccess || c->ntiles_d.alloc(1) != hipSuccess) {
+    if (c->totals.alloc(1) != hipSuccess || c->bbox.alloc(6) != hipSuccess || c->ntiles_d.alloc(1) != hipSuccess ||
+        c->ntiles_eff.alloc(1) != hipSuccess || c->status.alloc(1) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_status, sizeof(HostStatus), hipHostMallocDefault) != hipSuccess) {
         delete c;
         return PMQ_ERR_HIP;
     }
@@ -125,6 +127,7 @@ int pmq_destroy(pmq_ctx *c)
     pme_release(c);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->h_status) (void)hipHostFree(c->h_status);
     delete c;
     return PMQ_OK;
 }
@@ -294,6 +297,11 @@ int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
     c->have_top = true;
     c->have_prm = false;
     c->have_xyz = false;
+    c->tile_cap = 0;
+    c->tcap = 0;
+    c->safe = true;
+    c->pending = false;
+    c->pcg_guess_iters = 0;
     return PMQ_OK;
 }
 
@@ -401,6 +409,9 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
     c->prm.polarity = c->prm.thole = c->prm.pdamp = nullptr;
     PMQ_HIP(hipStreamSynchronize(c->stream));
     c->have_prm = true;
+    c->safe = true;
+    c->pending = false;
+    c->pcg_guess_iters = 0;
     return PMQ_OK;
 }
 
@@ -434,9 +445,8 @@ int pmq_set_coords(pmq_ctx *c, const double *x, const double *y, const double *z
     return PMQ_OK;
 }
 
-int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
+static int check_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
 {
-    CHECK_CTX(c);
     if (!c->have_top || !c->have_prm || !c->have_xyz) {
         c->err = "pmq_compute: topology, parameters and coordinates must be set first";
         return PMQ_ERR_STATE;
@@ -476,9 +486,21 @@ int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
         c->err = "pmq_compute: polarization computes energy and gradient (no virial or analysis)";
         return PMQ_ERR_UNSUPPORTED;
     }
+    return PMQ_OK;
+}
+
+// enqueue one evaluation; `safe` sizes the tile and pair-matrix buffers and
+// runs the PCG with host round trips, otherwise nothing waits for the device
+// and the run is checked afterwards (pmq_check)
+static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, bool safe)
+{
+    const bool mixed = precision == PMQ_MIXED;
+    const bool do_mpole = terms & PMQ_TERM_MPOLE, do_polar = terms & PMQ_TERM_POLAR;
+    const bool ewald = c->prm.use_ewald;
     PMQ_HIP(hipSetDevice(c->device));
     const int n = c->n;
     int rc;
+    if ((rc = status_reset(c))) return rc;
     if ((rc = launch_zero(c, (double *)c->totals.p, sizeof(Totals) / sizeof(double)))) return rc;
     const bool grad = mode == PMQ_GRADIENT || mode == PMQ_VIRIAL;
     if (grad) {
@@ -501,10 +523,10 @@ int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
         {
             ProfScope ps(c, "neighbors");
             if ((rc = build_neighbors(c, cut))) return rc;
-            if ((rc = tile_structure(c, off2))) return rc;
+            if ((rc = tile_structure(c, off2, safe))) return rc;
         }
         if ((rc = launch_rotpole(c, true))) return rc;
-        if (do_polar && (rc = polar_prepare(c, off2, mixed))) return rc;
+        if (do_polar && (rc = polar_prepare(c, off2, mixed, safe))) return rc;
         if (do_mpole && (rc = launch_mpole_direct(c, mode, off2, mixed))) return rc;
         if (do_mpole && mixed && grad) {
             if ((rc = launch_fx_flush(c, c->fx_grad.p, c->dem.p, 3 * (size_t)n, PMQ_FX_GRAD))) return rc;
@@ -515,7 +537,7 @@ int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
             if ((rc = pme_mpole(c, mode, do_mpole, do_polar))) return rc;
         }
         if (do_polar) {
-            if ((rc = polar_solve(c, off2, &c->pcg_iters))) return rc;
+            if ((rc = polar_solve(c, safe, &c->pcg_iters))) return rc;
             if ((rc = polar_energy(c))) return rc;
             if (grad) {
                 if ((rc = polar_gradient(c, off2, mixed))) return rc;
@@ -536,22 +558,67 @@ int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
     c->last_mode = mode;
     c->last_terms = terms;
     c->last_precision = precision;
+    // what the check needs, copied back as the evaluation's last step
+    PMQ_HIP(hipMemcpyAsync(&c->h_status->st, c->status.p, sizeof(RunStatus), hipMemcpyDeviceToHost, c->stream));
+    PMQ_HIP(hipMemcpyAsync(c->h_status->sc, c->pcg.p, sizeof(c->h_status->sc), hipMemcpyDeviceToHost, c->stream));
+    return PMQ_OK;
+}
+
+int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
+{
+    CHECK_CTX(c);
+    int rc = check_compute(c, mode, terms, precision);
+    if (rc) return rc;
+    c->pending = false;
+    rc = compute_enqueue(c, mode, terms, precision, c->safe);
+    if (rc) return rc;
+    c->pending = true;
+    c->pending_safe = c->safe;
+    return PMQ_OK;
+}
+
+// wait for the evaluation and check what was decided without the device:
+// buffer capacities and the speculative PCG batch.  A run that fell short
+// is redone with host round trips (and the capacities it measured).
+static int pmq_check(pmq_ctx *c)
+{
+    PMQ_HIP(hipSetDevice(c->device));
+    PMQ_HIP(hipStreamSynchronize(c->stream));
+    if (!c->pending) return PMQ_OK;
+    c->pending = false;
+    const HostStatus h = *c->h_status;
+    const bool pol = c->last_terms & PMQ_TERM_POLAR;
+    const bool over = h.st.overflow != 0;
+    const bool short_pcg = pol && !c->pending_safe && h.sc[8] == 0.0;
+    if (!over && !short_pcg) {
+        if (pol && !c->pending_safe) {
+            c->pcg_iters = (int)h.sc[9];
+            c->pcg_guess_iters = c->pcg_iters;
+        }
+        c->safe = false;
+        return PMQ_OK;
+    }
+    if (h.st.ntiles > c->tile_cap) c->tile_cap = 0;      // resized from the count on the safe run
+    if ((size_t)h.st.nnz > c->tcap) c->tcap = 0;
+    int rc = compute_enqueue(c, c->last_mode, c->last_terms, c->last_precision, true);
+    if (rc) return rc;
+    PMQ_HIP(hipStreamSynchronize(c->stream));
+    c->safe = false;
     return PMQ_OK;
 }
 
 int pmq_sync(pmq_ctx *c)
 {
     CHECK_CTX(c);
-    PMQ_HIP(hipStreamSynchronize(c->stream));
-    return PMQ_OK;
+    return pmq_check(c);
 }
 
 int pmq_fetch(pmq_ctx *c, pmq_result *r)
 {
     CHECK_CTX(c);
     if (!r) return PMQ_ERR_ARG;
-    PMQ_HIP(hipSetDevice(c->device));
-    PMQ_HIP(hipStreamSynchronize(c->stream));
+    int rc0 = pmq_check(c);
+    if (rc0) return rc0;
     Totals t;
     PMQ_HIP(hipMemcpy(&t, c->totals.p, sizeof t, hipMemcpyDeviceToHost));
     const int n = c->n;

@@ -136,6 +136,17 @@ __device__ __forceinline__ void grad_add(double *d, long long *q, size_t i, doub
     else atomicAdd(d + i, v);
 }
 
+// what the host must check after an evaluation enqueued without round trips:
+// the true tile and pair-matrix sizes and whether either exceeded the
+// capacity the evaluation ran with
+struct RunStatus {
+    int ntiles, nnz, overflow, pad;
+};
+struct HostStatus {       // pinned host mirror, written by the evaluation's last copies
+    RunStatus st;
+    double sc[11];        // PCG scalars (8 = converged, 9 = iterations)
+};
+
 // per-tile partial sums of the pair kernels (k_pairs.hip): values per lane
 // and side, scalars per tile (0 energy, 1..9 virial, 10 inter-molecular
 // energy, 11 pair count)
@@ -249,7 +260,7 @@ struct pmq_ctx {
                                                  // search direction, (1/alpha - T) p
     pmq::DBuf<double> pcg;                       // PCG scalars
     pmq::DBuf<double> pcg_part;                  // PCG per-block partial sums
-    int pcg_guess_iters = 1;                     // iterations of the last converged solve
+    int pcg_guess_iters = 0;                     // iterations of the last converged solve (0 = none yet)
     pmq::DBuf<double> tep;                       // 3n torques of the polarization term
     pmq::DBuf<double> dep;                       // 3n gradient of ep
     bool have_uind = false;
@@ -259,6 +270,13 @@ struct pmq_ctx {
     pmq::DBuf<double> tpart;                     // per tile: 2 sides x PMQ_TILE_NV x 64 partial sums
     pmq::DBuf<double> tscal;                     // per tile: PMQ_TILE_NS scalars (energy, virial, ...)
     int tgrid = 1;                               // workgroups of the tile kernels
+    int tile_cap = 0;                            // tile capacity of the tile buffers (0 = not sized yet)
+    pmq::DBuf<int> ntiles_eff;                   // min(tile count, tile_cap): what the tile kernels loop over
+    pmq::DBuf<pmq::RunStatus> status;
+    pmq::HostStatus *h_status = nullptr;         // pinned
+    bool safe = true;                            // next evaluation sizes buffers and the PCG with host round trips
+    bool pending = false;                        // an evaluation awaits its check (pmq_sync / pmq_fetch)
+    bool pending_safe = false;                   // ... and it ran in safe mode
     pmq::DBuf<int> soff;                         // 128 per tile: each lane's entry offset within its row
     pmq::DBuf<int> adj, adjn;                    // per block: tiles it takes part in (nblk x nblk), count
     pmq::DBuf<int> rowcnt, rowptr;               // npad + 1
@@ -312,15 +330,16 @@ int launch_torque(pmq_ctx *c, int mode, const double *te, double *de, bool mixed
 int pme_setup(pmq_ctx *c);
 void pme_release(pmq_ctx *c);
 int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar);
-int polar_prepare(pmq_ctx *c, double off2, bool mixed);
-int polar_solve(pmq_ctx *c, double off2, int *iters_out);
+int polar_prepare(pmq_ctx *c, double off2, bool mixed, bool safe);
+int polar_solve(pmq_ctx *c, bool safe, int *iters_out);
 int polar_energy(pmq_ctx *c);
 int polar_gradient(pmq_ctx *c, double off2, bool mixed);
 int polar_scatter(pmq_ctx *c, double *ud_out, double *up_out);
 int launch_chgflx_grad(pmq_ctx *c, int mode);
 int launch_zero(pmq_ctx *c, double *p, size_t n);
-int tile_structure(pmq_ctx *c, double off2);
-int pair_matrix_build(pmq_ctx *c, bool mixed);
+int tile_structure(pmq_ctx *c, double off2, bool safe);
+int status_reset(pmq_ctx *c);
+int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe);
 int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fixed);
 int tile_scalars(pmq_ctx *c, int which, bool full);
 int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst);

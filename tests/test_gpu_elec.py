@@ -97,3 +97,29 @@ def test_polar_only_energy_equals_combined(engine):
     assert pol["em"] == 0.0 and mp["ep"] == 0.0
     assert abs(both["ep"] - pol["ep"]) <= 1e-12 * abs(pol["ep"])
     assert abs(both["em"] - mp["em"]) <= 1e-12 * abs(mp["em"])
+
+
+def test_run_past_capacity_is_redone(engine):
+    """Evaluations after the first size their buffers and the PCG batch from
+    the previous one without waiting for the device; when a new geometry
+    needs more (denser: more pairs in the cutoff, more iterations) the run is
+    redone and must equal a fresh context's result."""
+    from polmdqc_amd import _lib as L
+    from polmdqc_amd.engine import Engine
+
+    s = box216()
+    engine.set_system(s)
+    engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR)
+    engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR)  # speculative from here on
+    xyz = np.stack([s["x"], s["y"], s["z"]], 1)
+    dense = xyz * np.array([0.6, 1.0, 1.0])
+    g = engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR, xyz=dense)
+    fresh = Engine(0)
+    try:
+        fresh.set_system(s)
+        f = fresh.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR, xyz=dense)
+    finally:
+        fresh.close()
+    assert g["pcg_iters"] == f["pcg_iters"]
+    assert abs(g["ep"] - f["ep"]) <= 1e-12 * abs(f["ep"])
+    assert rel(g["desum"], f["desum"]) <= 1e-12
