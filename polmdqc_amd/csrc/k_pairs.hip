@@ -13,8 +13,8 @@
 // dipole-dipole field tensor, u-scaled for excluded pairs) to a new pair of
 // dipole vectors in every PCG iteration while the geometry stays fixed.  So
 // T is built once per evaluation and every iteration is a sparse
-// matrix-vector product that reads it back: 6 symmetric tensor components
-// and a column index per stored entry, each pair stored in both rows.
+// matrix-vector product that reads it back: one record per stored entry (6
+// symmetric tensor components and the column), each pair stored in both rows.
 // Every row is owned by one group of lanes, so the product needs no atomics
 // and sums each row in a fixed order (bit-reproducible run to run).
 //
@@ -174,6 +174,16 @@ __global__ __launch_bounds__(64) void k_row_counts(int nblk, const int2 *__restr
     if (I == 0 && lane == 0) rowcnt[nblk * 64] = 0; // the scan's last element becomes the entry total
 }
 
+// One stored entry: the 6 tensor components and the column, one 64-byte
+// (double) or 32-byte (float) record, so the fill writes and the product
+// reads whole records.
+template <class V>
+struct alignas(8 * sizeof(V)) TEnt {
+    V t[6]; // xx yy zz xy xz yz
+    int col;
+    int pad;
+};
+
 // V: stored type of the tensor (double, or float in the mixed form); T: the
 // arithmetic of the pair terms
 template <class V>
@@ -187,19 +197,22 @@ struct FillArgs {
     const int *soff, *rowptr;
     int npad;
     size_t cap;
-    int *col;
-    V *val; // 6 x cap: xx yy zz xy xz yz
+    TEnt<V> *ent;
 };
 
 template <class V, class T>
-__device__ __forceinline__ void store_T(V *val, size_t cap, size_t e, const T *R, T xr, T yr, T zr)
+__device__ __forceinline__ void store_T(TEnt<V> *ent, size_t e, int col, const T *R, T xr, T yr, T zr)
 {
-    val[e] = R[2] * xr * xr - R[1];
-    val[cap + e] = R[2] * yr * yr - R[1];
-    val[2 * cap + e] = R[2] * zr * zr - R[1];
-    val[3 * cap + e] = R[2] * xr * yr;
-    val[4 * cap + e] = R[2] * xr * zr;
-    val[5 * cap + e] = R[2] * yr * zr;
+    TEnt<V> r;
+    r.t[0] = R[2] * xr * xr - R[1];
+    r.t[1] = R[2] * yr * yr - R[1];
+    r.t[2] = R[2] * zr * zr - R[1];
+    r.t[3] = R[2] * xr * yr;
+    r.t[4] = R[2] * xr * zr;
+    r.t[5] = R[2] * yr * zr;
+    r.col = col;
+    r.pad = 0;
+    ent[e] = r;
 }
 
 template <bool EW, class T, class V>
@@ -242,14 +255,8 @@ __global__ __launch_bounds__(64) void k_pair_fill(FillArgs<V> a)
             pol_series<EW, T>(r2, T(a.aewald), lam, R);
             const size_t ei = (size_t)ibase + __popcll(mi & ((1ull << jj) - 1ull));
             const size_t ej = (size_t)jbase[jj] + __popcll(jmask[jj] & ((1ull << lane) - 1ull));
-            if (ei < a.cap) {
-                a.col[ei] = IJ.y * 64 + jj;
-                store_T(a.val, a.cap, ei, R, xr, yr, zr);
-            }
-            if (ej < a.cap) {
-                a.col[ej] = si;
-                store_T(a.val, a.cap, ej, R, xr, yr, zr);
-            }
+            if (ei < a.cap) store_T(a.ent, ei, IJ.y * 64 + jj, R, xr, yr, zr);
+            if (ej < a.cap) store_T(a.ent, ej, si, R, xr, yr, zr);
         }
     }
 }
@@ -262,20 +269,21 @@ struct FoldArgs {
     const double *sx, *sy, *sz, *spol;
     DevBox box;
     int npad;
-    const int *rowptr, *col;
+    const int *rowptr;
     size_t cap;
-    V *val;
+    TEnt<V> *ent;
 };
 
 template <class V>
-__device__ __forceinline__ void add_T(V *val, size_t cap, size_t e, const double *R, double xr, double yr, double zr)
+__device__ __forceinline__ void add_T(TEnt<V> *ent, size_t e, const double *R, double xr, double yr, double zr)
 {
-    val[e] += R[2] * xr * xr - R[1];
-    val[cap + e] += R[2] * yr * yr - R[1];
-    val[2 * cap + e] += R[2] * zr * zr - R[1];
-    val[3 * cap + e] += R[2] * xr * yr;
-    val[4 * cap + e] += R[2] * xr * zr;
-    val[5 * cap + e] += R[2] * yr * zr;
+    V *t = ent[e].t;
+    t[0] += R[2] * xr * xr - R[1];
+    t[1] += R[2] * yr * yr - R[1];
+    t[2] += R[2] * zr * zr - R[1];
+    t[3] += R[2] * xr * yr;
+    t[4] += R[2] * xr * zr;
+    t[5] += R[2] * yr * zr;
 }
 
 // excluded pairs present in the matrix get (u - 1) lambda rr added to their
@@ -292,26 +300,26 @@ __global__ __launch_bounds__(64) void k_excl_fold(FoldArgs<V> a)
     int ei = -1, ek = -1;
     const int ci = (int)min((size_t)a.rowptr[si + 1], a.cap), ck = (int)min((size_t)a.rowptr[sk + 1], a.cap);
     for (int e = a.rowptr[si]; e < ci; e++)
-        if (a.col[e] == sk) { ei = e; break; }
+        if (a.ent[e].col == sk) { ei = e; break; }
     if (ei < 0) return; // beyond the cutoff: not in the matrix
     for (int e = a.rowptr[sk]; e < ck; e++)
-        if (a.col[e] == si) { ek = e; break; }
+        if (a.ent[e].col == si) { ek = e; break; }
     double xr = a.sx[sk] - a.sx[si], yr = a.sy[sk] - a.sy[si], zr = a.sz[sk] - a.sz[si];
     image<double>(a.box, xr, yr, zr);
     const double r2 = xr * xr + yr * yr + zr * zr;
     double lam[6], R[6];
     thole_lambda<double>(sqrt(r2), a.spol[2 * np + si], a.spol[2 * np + sk], a.spol[np + si], a.spol[np + sk], lam);
     corr_series<double>(r2, q.u, lam, R);
-    add_T(a.val, a.cap, (size_t)ei, R, xr, yr, zr);
-    if (ek >= 0) add_T(a.val, a.cap, (size_t)ek, R, xr, yr, zr);
+    add_T(a.ent, (size_t)ei, R, xr, yr, zr);
+    if (ek >= 0) add_T(a.ent, (size_t)ek, R, xr, yr, zr);
 }
 
 template <class V>
 struct SpmvArgs {
     int npad;
-    const int *rowptr, *col;
+    const int *rowptr;
     size_t cap;
-    const V *val;
+    const TEnt<V> *ent;
     const double *src; // 2 x 3 x npad
     double *dst;       // 2 x 3 x npad (overwritten)
 };
@@ -327,9 +335,10 @@ __global__ __launch_bounds__(256) void k_pol_spmv(SpmvArgs<V> a)
     if (row < np) {
         const int e1 = (int)min((size_t)a.rowptr[row + 1], a.cap);
         for (int e = a.rowptr[row] + g; e < e1; e += G) {
-            const int k = a.col[e];
-            const double txx = a.val[e], tyy = a.val[a.cap + e], tzz = a.val[2 * a.cap + e];
-            const double txy = a.val[3 * a.cap + e], txz = a.val[4 * a.cap + e], tyz = a.val[5 * a.cap + e];
+            const TEnt<V> r = a.ent[e];
+            const int k = r.col;
+            const double txx = r.t[0], tyy = r.t[1], tzz = r.t[2];
+            const double txy = r.t[3], txz = r.t[4], tyz = r.t[5];
 #pragma unroll
             for (int set = 0; set < 2; set++) {
                 const double *u = a.src + (size_t)set * 3 * np;
@@ -357,7 +366,7 @@ __global__ __launch_bounds__(256) void k_pol_spmv(SpmvArgs<V> a)
 // host side
 
 template <class T, class V>
-static int pair_fill(pmq_ctx *c, V *val, int tgrid)
+static int pair_fill(pmq_ctx *c, int tgrid)
 {
     const int np = c->npad;
     hipStream_t st = c->stream;
@@ -367,7 +376,7 @@ static int pair_fill(pmq_ctx *c, V *val, int tgrid)
     fa.box = c->box;
     fa.aewald = c->prm.use_ewald ? c->prm.aewald : 0.0;
     fa.mask = c->tmask.p; fa.soff = c->soff.p; fa.rowptr = c->rowptr.p;
-    fa.npad = np; fa.cap = c->tcap; fa.col = c->tcol.p; fa.val = val;
+    fa.npad = np; fa.cap = c->tcap; fa.ent = (TEnt<V> *)c->tval.p;
     {
         ProfScope ps_(c, "pol_pair_fill");
         if (fa.aewald > 0)
@@ -380,8 +389,8 @@ static int pair_fill(pmq_ctx *c, V *val, int tgrid)
         FoldArgs<V> fo;
         fo.excl = c->excl.p; fo.nexcl = nexcl; fo.iperm = c->iperm.p;
         fo.sx = c->sx.p; fo.sy = c->sy.p; fo.sz = c->sz.p; fo.spol = c->spol.p;
-        fo.box = c->box; fo.npad = np; fo.rowptr = c->rowptr.p; fo.col = c->tcol.p; fo.cap = c->tcap;
-        fo.val = val;
+        fo.box = c->box; fo.npad = np; fo.rowptr = c->rowptr.p; fo.cap = c->tcap;
+        fo.ent = (TEnt<V> *)c->tval.p;
         hipLaunchKernelGGL((k_excl_fold<V>), dim3((nexcl + 63) / 64), dim3(64), 0, st, fo);
     }
     PMQ_HIP(hipGetLastError());
@@ -451,52 +460,84 @@ int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe)
         PMQ_HIP(hipStreamSynchronize(st));
         if ((size_t)nnz > c->tcap || c->tcap == 0) {
             c->tcap = (size_t)nnz + (size_t)nnz / 4 + 64;
-            PMQ_HIP(c->tcol.alloc(c->tcap));
-            PMQ_HIP(c->tval.alloc(6 * c->tcap));
+            PMQ_HIP(c->tval.alloc(8 * c->tcap)); // one 64-byte record per entry (32 bytes used by the mixed form)
         }
     }
     hipLaunchKernelGGL(k_nnz_check, dim3(1), dim3(1), 0, st, c->rowptr.p + np, c->tcap, c->status.p);
 
     c->tmixed = mixed;
-    if (mixed) return pair_fill<float, float>(c, (float *)c->tval.p, c->tgrid);
-    return pair_fill<double, double>(c, c->tval.p, c->tgrid);
+    if (mixed) return pair_fill<float, float>(c, c->tgrid);
+    return pair_fill<double, double>(c, c->tgrid);
 }
 
 // ---------------------------------------------------------------------------
 // deterministic reduction of tile partials
 
 // per-atom sums of NV partials over the tiles of the atom's block; P is the
-// partial type (the pair arithmetic), FX sums in 64-bit fixed point
+// partial type (the pair arithmetic), FX sums in 64-bit fixed point.  One
+// workgroup of 4 waves per block: the block's tile list is staged in LDS,
+// wave w sums tiles w, w+4, ... (independent loads, several in flight), and
+// the four partial sums are added in wave order.
+constexpr int RED_MAXT = 1024; // tiles per block staged at a time
+
 template <class P, int NV, bool FX>
-__global__ __launch_bounds__(64) void k_tile_reduce(TileReduceArgs a)
+__global__ __launch_bounds__(256) void k_tile_reduce(TileReduceArgs a)
 {
-    const int I = blockIdx.x, lane = threadIdx.x;
+    __shared__ int lt[RED_MAXT];
+    __shared__ double red_d[4][NV][64];
+    __shared__ long long red_q[4][NV][64];
+    const int I = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int si = I * 64 + lane;
     const P *part = (const P *)a.part;
     double acc[NV];
     long long q[NV];
 #pragma unroll
     for (int c = 0; c < NV; c++) { acc[c] = 0.0; q[c] = 0; }
-    const int na = a.adjn[I];
-    for (int s = 0; s < na; s++) {
-        const int t = a.adj[(size_t)I * a.nblk + s];
-        const int2 IJ = a.tiles[t];
-        for (int side = 0; side < 2; side++) {
-            if (IJ.x != IJ.y && side != (IJ.x == I ? 0 : 1)) continue;
-            const P *pp = part + ((size_t)t * 2 + side) * PMQ_TILE_NV * 64 + lane;
+    bool want[NV];
 #pragma unroll
-            for (int c = 0; c < NV; c++) {
-                if (!a.out[c]) continue; // value not produced by this launch
-                if (FX) q[c] += __double2ll_rn((double)pp[c * 64] * a.fxscale[c]);
-                else acc[c] += (double)pp[c * 64];
+    for (int c = 0; c < NV; c++) want[c] = a.out[c] != nullptr;
+    const int na = a.adjn[I];
+    for (int base = 0; base < na; base += RED_MAXT) {
+        const int cnt = min(RED_MAXT, na - base);
+        __syncthreads();
+        // entry = partial-slot index (tile * 2 + side), bit 30 set: a diagonal tile (both sides)
+        for (int s = tid; s < cnt; s += 256) {
+            const int t = a.adj[(size_t)I * a.nblk + base + s];
+            const int2 IJ = a.tiles[t];
+            lt[s] = IJ.x == IJ.y ? ((t * 2) | (1 << 30)) : (t * 2 + (IJ.x == I ? 0 : 1));
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int s = wv; s < cnt; s += 4) {
+            const int e = lt[s];
+            const int slot = e & ~(1 << 30);
+            const int nside = (e >> 30) & 1 ? 2 : 1;
+            for (int k = 0; k < nside; k++) {
+                const P *pp = part + ((size_t)slot + k) * PMQ_TILE_NV * 64 + lane;
+                P v[NV];
+#pragma unroll
+                for (int c = 0; c < NV; c++) v[c] = want[c] ? pp[c * 64] : P(0);
+#pragma unroll
+                for (int c = 0; c < NV; c++) {
+                    if (FX) q[c] += __double2ll_rn((double)v[c] * a.fxscale[c]);
+                    else acc[c] += (double)v[c];
+                }
             }
         }
     }
-    if (si >= a.n) return;
 #pragma unroll
     for (int c = 0; c < NV; c++) {
-        const double v = FX ? (double)q[c] / a.fxscale[c] : acc[c];
-        if (!a.out[c]) continue;
+        if (FX) red_q[wv][c][lane] = q[c];
+        else red_d[wv][c][lane] = acc[c];
+    }
+    __syncthreads();
+    if (wv != 0 || si >= a.n) return;
+#pragma unroll
+    for (int c = 0; c < NV; c++) {
+        if (!want[c]) continue;
+        const double v = FX ? (double)(red_q[0][c][lane] + red_q[1][c][lane] + red_q[2][c][lane] + red_q[3][c][lane]) /
+                                  a.fxscale[c]
+                            : ((red_d[0][c][lane] + red_d[1][c][lane]) + red_d[2][c][lane]) + red_d[3][c][lane];
         const size_t idx = a.caller_order[c] ? (size_t)a.stride[c] * a.perm[si] + a.off[c] : (size_t)a.stride[c] * si + a.off[c];
         a.out[c][idx] += v;
         if (a.out2[c]) a.out2[c][idx] += v;
@@ -537,7 +578,7 @@ int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fi
     a.nblk = c->nblk; a.n = c->n;
     a.tiles = c->tiles.p; a.adj = c->adj.p; a.adjn = c->adjn.p; a.perm = c->perm.p;
     a.part = c->tpart.p;
-    const dim3 g(c->nblk), b(64);
+    const dim3 g(c->nblk), b(256);
 #define PMQ_RED(P, NV, FX) hipLaunchKernelGGL((k_tile_reduce<P, NV, FX>), g, b, 0, c->stream, a)
     if (float_parts) {
         if (nv == 3) { if (fixed) PMQ_RED(float, 3, true); else PMQ_RED(float, 3, false); }
@@ -562,11 +603,11 @@ int tile_scalars(pmq_ctx *c, int which, bool full)
 }
 
 template <class V>
-static int spmv(pmq_ctx *c, const V *val, const double *src, double *dst)
+static int spmv(pmq_ctx *c, const double *src, double *dst)
 {
     constexpr int G = 8;
     SpmvArgs<V> a;
-    a.npad = c->npad; a.rowptr = c->rowptr.p; a.col = c->tcol.p; a.cap = c->tcap; a.val = val;
+    a.npad = c->npad; a.rowptr = c->rowptr.p; a.cap = c->tcap; a.ent = (const TEnt<V> *)c->tval.p;
     a.src = src; a.dst = dst;
     const size_t lanes = (size_t)c->npad * G;
     ProfScope ps_(c, "pol_mutual_spmv");
@@ -577,8 +618,8 @@ static int spmv(pmq_ctx *c, const V *val, const double *src, double *dst)
 
 int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst)
 {
-    if (c->tmixed) return spmv<float>(c, (const float *)c->tval.p, src, dst);
-    return spmv<double>(c, c->tval.p, src, dst);
+    if (c->tmixed) return spmv<float>(c, src, dst);
+    return spmv<double>(c, src, dst);
 }
 
 } // namespace pmq

@@ -281,8 +281,7 @@ struct pmq_ctx {
     pmq::DBuf<int> adj, adjn;                    // per block: tiles it takes part in (nblk x nblk), count
     pmq::DBuf<int> rowcnt, rowptr;               // npad + 1
     pmq::DBuf<unsigned char> pscan_tmp;
-    pmq::DBuf<int> tcol;                         // column (sorted atom) of each entry
-    pmq::DBuf<double> tval;                      // 6 x tcap: T xx yy zz xy xz yz
+    pmq::DBuf<double> tval;                      // tcap records of 64 bytes: T xx yy zz xy xz yz, column
     size_t tcap = 0;
     int nnz = 0;
     bool tmixed = false;                         // tval holds floats (mixed form)
