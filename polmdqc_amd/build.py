@@ -4,7 +4,10 @@
 
 Every translation unit under polmdqc_amd/csrc is compiled as HIP for gfx950
 and linked into one shared library whose exported symbols are exactly the
-C ABI of include/pmq.h.
+C ABI of include/pmq.h.  The C++ host layer (polmdqc_amd/host: the
+reference's energy/empole/epolar<CalcMode>() over global arrays, calling the
+C ABI) is built with g++ into libpmq_host.so, and its test driver
+tests/host/host_check against both.
 """
 from __future__ import annotations
 
@@ -70,7 +73,32 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources")
+    build_host(verbose)
     return LIB
+
+
+HOST_SRC = os.path.join(HERE, "host", "pmq_host.cpp")
+HOST_LIB = os.path.join(HERE, "libpmq_host.so")
+HOST_CHECK_SRC = os.path.join(ROOT, "tests", "host", "host_check.cpp")
+HOST_CHECK = os.path.join(ROOT, "tests", "host", "host_check")
+
+
+def build_host(verbose: bool = True) -> str:
+    """g++: libpmq_host.so (links libpmq.so) and the host_check driver."""
+    gxx = shutil.which("g++") or "g++"
+    rpath = "-Wl,-rpath,$ORIGIN"
+    cmds = [
+        [gxx, "-std=c++17", "-O2", "-fPIC", "-shared", HOST_SRC, "-o", HOST_LIB, f"-L{HERE}", "-lpmq", rpath],
+        [gxx, "-std=c++17", "-O2", HOST_CHECK_SRC, "-o", HOST_CHECK, f"-L{HERE}", "-lpmq_host", "-lpmq",
+         f"-Wl,-rpath,{os.path.relpath(HERE, os.path.dirname(HOST_CHECK)).join(['$ORIGIN/', ''])}"],
+    ]
+    for cmd in cmds:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed:\n{' '.join(cmd)}\n{r.stderr[-4000:]}")
+    if verbose:
+        print(f"built {HOST_LIB} and {HOST_CHECK}")
+    return HOST_LIB
 
 
 def clean() -> None:
