@@ -176,6 +176,21 @@ PMQ_API int pmq_compute(pmq_ctx *ctx, int mode, unsigned terms, int precision);
 PMQ_API int pmq_fetch(pmq_ctx *ctx, pmq_result *res);
 PMQ_API int pmq_sync(pmq_ctx *ctx);
 
+/* Batched clusters.  No counterpart in the reference: the batched workload of
+ * this path (many small non-periodic systems).  After pmq_set_batch the
+ * context's system is a set of independent clusters, cluster c being atoms
+ * [offsets[c], offsets[c+1]) (at most 256 atoms each); no pair across clusters
+ * interacts, no cutoff applies, and excluded pairs must lie inside a cluster.
+ * pmq_compute (ENERGY or GRADIENT, terms MPOLE|POLAR, either precision, no
+ * charge penetration or flux) then solves every cluster, each to its own PCG
+ * convergence, in one persistent launch; the evaluation is captured in a HIP
+ * graph after the first call and replayed.  Totals and gradients come from
+ * pmq_fetch as for one system; pmq_fetch_batch gives the per-cluster energies
+ * and PCG iteration counts (negative: not converged).  nsys = 0 returns the
+ * context to one system. */
+PMQ_API int pmq_set_batch(pmq_ctx *ctx, int nsys, const int *offsets);
+PMQ_API int pmq_fetch_batch(pmq_ctx *ctx, double *em, double *ep, int *pcg_iters);
+
 /* Kernel timing for measurement.  No counterpart in the reference (it has no
  * timers on this path); for benchmark and profiling harnesses.  While
  * enabled, the context brackets every launch of each instrumented kernel

@@ -94,6 +94,14 @@ int pmq_profile_read(pmq_ctx *c, const char *name, double *total_ms, long long *
     return PMQ_OK;
 }
 
+static void batch_graph_release(pmq_ctx *c)
+{
+    if (c->b_exec) (void)hipGraphExecDestroy((hipGraphExec_t)c->b_exec);
+    if (c->b_graph) (void)hipGraphDestroy((hipGraph_t)c->b_graph);
+    c->b_exec = c->b_graph = nullptr;
+    c->b_graph_key = -1;
+}
+
 int pmq_create(const pmq_config *cfg, pmq_ctx **out)
 {
     if (!out) return PMQ_ERR_ARG;
@@ -115,6 +123,9 @@ int pmq_create(const pmq_config *cfg, pmq_ctx **out)
         delete c;
         return PMQ_ERR_HIP;
     }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->num_cus = prop.multiProcessorCount;
     *out = c;
     return PMQ_OK;
 }
@@ -128,6 +139,7 @@ int pmq_destroy(pmq_ctx *c)
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->h_status) (void)hipHostFree(c->h_status);
+    batch_graph_release(c);
     delete c;
     return PMQ_OK;
 }
@@ -299,6 +311,8 @@ int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
     c->have_xyz = false;
     c->tile_cap = 0;
     c->tcap = 0;
+    c->nsys = 0;
+    batch_graph_release(c);
     c->safe = true;
     c->pending = false;
     c->pcg_guess_iters = 0;
@@ -409,6 +423,8 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
     c->prm.polarity = c->prm.thole = c->prm.pdamp = nullptr;
     PMQ_HIP(hipStreamSynchronize(c->stream));
     c->have_prm = true;
+    batch_graph_release(c);
+    c->batch_variant = -1;
     c->safe = true;
     c->pending = false;
     c->pcg_guess_iters = 0;
@@ -564,11 +580,168 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
     return PMQ_OK;
 }
 
+// one evaluation of the batched clusters: frames, the persistent cluster
+// kernel, torques
+static int batch_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, int variant)
+{
+    const bool mixed = precision == PMQ_MIXED;
+    const bool mp = terms & PMQ_TERM_MPOLE, pol = terms & PMQ_TERM_POLAR;
+    int rc;
+    if ((rc = launch_zero(c, (double *)c->totals.p, sizeof(Totals) / sizeof(double)))) return rc;
+    if ((rc = launch_rotpole(c, false))) return rc;
+    if ((rc = launch_batch(c, mode, mp, pol, mixed, variant))) return rc;
+    if (mode == PMQ_GRADIENT) {
+        if (mp && (rc = launch_torque(c, PMQ_GRADIENT, c->te.p, c->dem.p, mixed))) return rc;
+        if (pol && (rc = launch_torque(c, PMQ_GRADIENT, c->tep.p, c->dep.p, mixed))) return rc;
+    }
+    return PMQ_OK;
+}
+
+static int batch_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
+{
+    if (mode != PMQ_ENERGY && mode != PMQ_GRADIENT) {
+        c->err = "pmq_compute: batched clusters compute energy and gradient";
+        return PMQ_ERR_UNSUPPORTED;
+    }
+    if (c->prm.pentype != PMQ_PEN_NONE || c->prm.use_chgflx || c->prm.use_ewald || c->box.kind != PMQ_BOX_NONE) {
+        c->err = "pmq_compute: batched clusters take non-periodic systems without Ewald, charge penetration or flux";
+        return PMQ_ERR_UNSUPPORTED;
+    }
+    if ((terms & PMQ_TERM_POLAR) && c->h_polarity.empty()) {
+        c->err = "pmq_compute: polarization requested but no polarizabilities were set";
+        return PMQ_ERR_STATE;
+    }
+    PMQ_HIP(hipSetDevice(c->device));
+    int rc;
+    c->last_mode = mode;
+    c->last_terms = terms;
+    c->last_precision = precision;
+    c->pending = false;
+    c->pcg_iters = 0;
+    if (c->batch_variant < 0) {
+        // autotune the lanes-per-row variant on this batch: the faster of two
+        // timed runs each (the first run of a variant also warms it up)
+        hipEvent_t e0, e1;
+        PMQ_HIP(hipEventCreate(&e0));
+        PMQ_HIP(hipEventCreate(&e1));
+        float best = 1e30f;
+        int bestv = 0;
+        for (int v = 0; v < batch_variants(); v++) {
+            if ((rc = batch_enqueue(c, mode, terms, precision, v))) return rc;
+            PMQ_HIP(hipEventRecord(e0, c->stream));
+            if ((rc = batch_enqueue(c, mode, terms, precision, v))) return rc;
+            PMQ_HIP(hipEventRecord(e1, c->stream));
+            PMQ_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            PMQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) { best = ms; bestv = v; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c->batch_variant = bestv;
+    }
+    const int key = ((mode * 4 + (int)terms) * 2 + precision) * 8 + c->batch_variant;
+    if (c->profiling) return batch_enqueue(c, mode, terms, precision, c->batch_variant); // events are not captured
+    if (c->b_graph_key != key) {
+        batch_graph_release(c);
+        PMQ_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = batch_enqueue(c, mode, terms, precision, c->batch_variant);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+        if (rc) return rc;
+        PMQ_HIP(ce);
+        c->b_graph = g;
+        hipGraphExec_t ex = nullptr;
+        PMQ_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+        c->b_exec = ex;
+        c->b_graph_key = key;
+    }
+    PMQ_HIP(hipGraphLaunch((hipGraphExec_t)c->b_exec, c->stream));
+    return PMQ_OK;
+}
+
+int pmq_set_batch(pmq_ctx *c, int nsys, const int *off)
+{
+    CHECK_CTX(c);
+    if (!c->have_top) {
+        c->err = "pmq_set_batch: call pmq_set_topology first";
+        return PMQ_ERR_STATE;
+    }
+    batch_graph_release(c);
+    c->batch_variant = -1;
+    if (nsys <= 0) {
+        c->nsys = 0;
+        return PMQ_OK;
+    }
+    if (!off || off[0] != 0 || off[nsys] != c->n) {
+        c->err = "pmq_set_batch: offsets must start at 0 and end at n";
+        return PMQ_ERR_ARG;
+    }
+    std::vector<int> sizes(nsys), owner(c->n);
+    for (int k = 0; k < nsys; k++) {
+        sizes[k] = off[k + 1] - off[k];
+        if (sizes[k] <= 0 || sizes[k] > 256) {
+            c->err = "pmq_set_batch: every cluster needs 1..256 atoms";
+            return PMQ_ERR_ARG;
+        }
+        for (int i = off[k]; i < off[k + 1]; i++) owner[i] = k;
+    }
+    // excluded pairs (sorted by first atom) must stay inside their cluster
+    std::vector<int> eoff(nsys + 1, 0);
+    for (const ExclPair &e : c->h_excl) {
+        if (owner[e.i] != owner[e.k]) {
+            c->err = "pmq_set_batch: a scaled pair joins two clusters";
+            return PMQ_ERR_ARG;
+        }
+        eoff[owner[e.i] + 1]++;
+    }
+    for (int k = 0; k < nsys; k++) eoff[k + 1] += eoff[k];
+    std::vector<int> order(nsys);
+    for (int k = 0; k < nsys; k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sizes[a] > sizes[b]; });
+    int rc;
+    if ((rc = upload(c, c->b_order, order.data(), nsys))) return rc;
+    if ((rc = upload(c, c->b_off, off, nsys + 1))) return rc;
+    if ((rc = upload(c, c->b_eoff, eoff.data(), nsys + 1))) return rc;
+    PMQ_HIP(c->b_em.alloc(nsys)); PMQ_HIP(c->b_ep.alloc(nsys)); PMQ_HIP(c->b_it.alloc(nsys));
+    PMQ_HIP(c->b_counter.alloc(1));
+    // frames run in caller order
+    std::vector<int> iota(c->npad);
+    for (int s = 0; s < c->npad; s++) iota[s] = s < c->n ? s : -1;
+    if ((rc = upload(c, c->perm, iota.data(), c->npad))) return rc;
+    PMQ_HIP(c->dep.alloc(3 * (size_t)c->n));
+    PMQ_HIP(c->tep.alloc(3 * (size_t)c->n));
+    PMQ_HIP(hipStreamSynchronize(c->stream));
+    c->h_boff.assign(off, off + nsys + 1);
+    c->nsys = nsys;
+    return PMQ_OK;
+}
+
+int pmq_fetch_batch(pmq_ctx *c, double *em, double *ep, int *iters)
+{
+    CHECK_CTX(c);
+    if (c->nsys <= 0) {
+        c->err = "pmq_fetch_batch: no batch set";
+        return PMQ_ERR_STATE;
+    }
+    PMQ_HIP(hipStreamSynchronize(c->stream));
+    const int ns = c->nsys;
+    if (em) PMQ_HIP(hipMemcpy(em, c->b_em.p, ns * sizeof(double), hipMemcpyDeviceToHost));
+    if (ep) PMQ_HIP(hipMemcpy(ep, c->b_ep.p, ns * sizeof(double), hipMemcpyDeviceToHost));
+    if (iters) {
+        std::vector<double> it(ns);
+        PMQ_HIP(hipMemcpy(it.data(), c->b_it.p, ns * sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 0; k < ns; k++) iters[k] = (int)it[k];
+    }
+    return PMQ_OK;
+}
+
 int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
 {
     CHECK_CTX(c);
     int rc = check_compute(c, mode, terms, precision);
     if (rc) return rc;
+    if (c->nsys > 0) return batch_compute(c, mode, terms, precision);
     c->pending = false;
     rc = compute_enqueue(c, mode, terms, precision, c->safe);
     if (rc) return rc;

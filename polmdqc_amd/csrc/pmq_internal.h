@@ -285,6 +285,15 @@ struct pmq_ctx {
     size_t tcap = 0;
     int nnz = 0;
     bool tmixed = false;                         // tval holds floats (mixed form)
+    // batched clusters (k_batch.hip)
+    int nsys = 0;                                // 0 = one system
+    std::vector<int> h_boff;                     // nsys + 1 atom offsets
+    pmq::DBuf<int> b_order, b_off, b_eoff, b_counter;
+    pmq::DBuf<double> b_em, b_ep, b_it;
+    int num_cus = 256;
+    int batch_variant = -1;                      // autotuned lanes-per-row variant (-1 = not yet)
+    void *b_graph = nullptr, *b_exec = nullptr;  // captured evaluation
+    int b_graph_key = -1;
     // PME
     int K[3] = {0, 0, 0};
     int bsorder = 5;
@@ -342,6 +351,8 @@ int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe);
 int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fixed);
 int tile_scalars(pmq_ctx *c, int which, bool full);
 int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst);
+int launch_batch(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int variant);
+int batch_variants();
 } // namespace pmq
 
 namespace pmq {

@@ -50,6 +50,7 @@ class Engine:
             raise PmqError(f"pmq_create failed: {L.ERRORS.get(rc, rc)}")
         self._keep = []
         self.n = 0
+        self.nsys = 0
 
     # ------------------------------------------------------------------
     def close(self):
@@ -108,6 +109,7 @@ class Engine:
         bal = s.get("balist")
         t.balist = self._i(bal) if bal is not None and len(bal) else None
         self._check(self._L.pmq_set_topology(self._ctx, C.byref(t)), "pmq_set_topology")
+        self.nsys = 0
 
         p = L.Params()
         p.pole = self._d(s["pole"])
@@ -182,6 +184,25 @@ class Engine:
             out["rpole"] = rpole.reshape(n, 13)
         return out
 
+    def set_batch(self, offsets):
+        """Treat the system as independent non-periodic clusters: cluster c is
+        atoms offsets[c]:offsets[c+1] (offsets[0] = 0, offsets[-1] = n);
+        None returns to one system."""
+        if offsets is None:
+            self._check(self._L.pmq_set_batch(self._ctx, 0, None), "pmq_set_batch")
+            self.nsys = 0
+            return
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        self._check(self._L.pmq_set_batch(self._ctx, len(off) - 1, off.ctypes.data_as(_ip)), "pmq_set_batch")
+        self.nsys = len(off) - 1
+
+    def fetch_batch(self) -> dict:
+        ns = self.nsys
+        em, ep, it = np.zeros(ns), np.zeros(ns), np.zeros(ns, dtype=np.int32)
+        self._check(self._L.pmq_fetch_batch(self._ctx, em.ctypes.data_as(_dp), ep.ctypes.data_as(_dp),
+                                            it.ctypes.data_as(_ip)), "pmq_fetch_batch")
+        return {"em": em, "ep": ep, "pcg_iters": it}
+
     def eval(self, mode: int = L.VIRIAL, terms: int = L.TERM_MPOLE, precision: int = L.PLAIN_FP64,
              xyz: np.ndarray | None = None, want_rpole: bool = False) -> dict:
         if xyz is not None:
@@ -201,7 +222,7 @@ def _profile_bind(L):
 
 
 KERNEL_FAMILIES = ("neighbors", "mpole_tiles", "pol_field_tiles", "pol_pair_fill", "pol_mutual_spmv",
-                   "pol_grad_tiles", "pme_spread", "pme_fft", "pme_gather")
+                   "pol_grad_tiles", "pme_spread", "pme_fft", "pme_gather", "batch_clusters")
 
 
 def profile_enable(eng: Engine, on: bool = True):

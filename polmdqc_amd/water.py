@@ -80,6 +80,17 @@ def cluster(nmol: int, rng: np.random.Generator, min_oo: float = 2.5) -> np.ndar
     return (mol + np.array(oxy)[:, None, :]).reshape(3 * nmol, 3)
 
 
+def cluster_batch(nclusters: int, seed: int, min_mol: int = 2, max_mol: int = 64):
+    """Independent clusters of U{min_mol..max_mol} waters (seeded): the
+    concatenated coordinates and the atom offsets of each cluster.  Each
+    cluster sits at its own origin; clusters never interact."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(min_mol, max_mol + 1, size=nclusters)
+    xyz = [cluster(int(m), rng) for m in sizes]
+    off = np.concatenate([[0], np.cumsum(3 * sizes)]).astype(np.int32)
+    return np.concatenate(xyz), off
+
+
 def topology(nmol: int, ff: str = "water03"):
     """Atom types and 0-based bond list (O-H1, O-H2) for nmol waters."""
     to, th = TYPES[ff]

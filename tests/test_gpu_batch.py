@@ -1,0 +1,127 @@
+"""Batched non-periodic clusters in one persistent launch (graph-captured).
+
+Every cluster of the batch must equal that cluster solved alone: by the
+oracle (FP64, 1e-9 on energies, 1e-8 on gradients, PCG tightened to 1e-10 D
+so both solvers sit on the same solution; equal iteration counts at the
+production 1e-6 D), and by the one-system GPU path.  The mixed form keeps the
+RMS force error below 1e-3 kcal/mol/A.  Graph replays, with eager work and
+synchronizations between them, must repeat the first (captured) result.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from polmdqc_amd import water
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def engine():
+    from polmdqc_amd.engine import Engine
+
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+def batch(nclusters, seed, max_mol=12):
+    xyz, off = water.cluster_batch(nclusters, seed, 2, max_mol)
+    return water.water_system(xyz, "water03"), off
+
+
+def sub(s, a, b):
+    """The system of atoms a:b of a water system, as its own system."""
+    return water.water_system(np.stack([s["x"][a:b], s["y"][a:b], s["z"][a:b]], 1), "water03")
+
+
+def test_batch_matches_oracle_per_cluster(engine):
+    from polmdqc_amd import _lib as L
+
+    print("[gpu] batch 12 clusters vs oracle", flush=True)
+    s, off = batch(12, 5)
+    s["poleps"] = 1e-10
+    engine.set_system(s)
+    engine.set_batch(off)
+    g = engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR)
+    b = engine.fetch_batch()
+    for c in range(len(off) - 1):
+        lo, hi = off[c], off[c + 1]
+        sc = sub(s, lo, hi)
+        sc["poleps"] = 1e-10
+        o = oracle.elec(sc, want_grad=True)
+        assert abs(b["em"][c] - o["em"]) <= 1e-9 * max(1.0, abs(o["em"]))
+        assert abs(b["ep"][c] - o["ep"]) <= 1e-9 * max(1.0, abs(o["ep"]))
+        scale = max(1.0, np.abs(o["dem"] + o["dep"]).max())
+        assert np.max(np.abs(g["desum"][lo:hi] - (o["dem"] + o["dep"]))) <= 1e-8 * scale
+    assert abs(g["em"] - b["em"].sum()) <= 1e-12 * abs(g["em"])
+
+
+def test_batch_iterations_match_oracle(engine):
+    from polmdqc_amd import _lib as L
+
+    s, off = batch(8, 11)
+    engine.set_system(s)
+    engine.set_batch(off)
+    engine.eval(L.ENERGY, L.TERM_MPOLE | L.TERM_POLAR)
+    b = engine.fetch_batch()
+    for c in range(len(off) - 1):
+        o = oracle.elec(sub(s, off[c], off[c + 1]), want_grad=False)
+        assert b["pcg_iters"][c] == o["iters"]
+
+
+def test_batch_equals_single_system_path(engine):
+    from polmdqc_amd import _lib as L
+
+    s, off = batch(6, 3)
+    engine.set_system(s)
+    engine.set_batch(off)
+    g = engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR)
+    b = engine.fetch_batch()
+    for c in range(len(off) - 1):
+        engine.set_system(sub(s, off[c], off[c + 1]))
+        one = engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR)
+        assert abs(one["em"] - b["em"][c]) <= 1e-10 * max(1.0, abs(one["em"]))
+        assert abs(one["ep"] - b["ep"][c]) <= 1e-10 * max(1.0, abs(one["ep"]))
+        assert one["pcg_iters"] == b["pcg_iters"][c]
+        np.testing.assert_allclose(g["desum"][off[c]:off[c + 1]], one["desum"], rtol=0,
+                                   atol=1e-9 * np.abs(one["desum"]).max())
+
+
+def test_batch_mixed_precision(engine):
+    from polmdqc_amd import _lib as L
+
+    s, off = batch(16, 7, 64)
+    engine.set_system(s)
+    engine.set_batch(off)
+    d = engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR, L.PLAIN_FP64)
+    m = engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR, L.MIXED)
+    err = np.sqrt(np.mean(np.sum((m["desum"] - d["desum"]) ** 2, axis=1)))
+    assert err < 1e-3
+    assert abs(m["ep"] - d["ep"]) <= 1e-5 * s["n"]
+    m2 = engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR, L.MIXED)
+    assert np.array_equal(m["desum"], m2["desum"])
+
+
+def test_graph_replays_repeat_the_captured_result(engine):
+    from polmdqc_amd import _lib as L
+    from polmdqc_amd.engine import profile_enable
+
+    s, off = batch(10, 21, 40)
+    engine.set_system(s)
+    engine.set_batch(off)
+    # eager reference: instrumented runs are not captured
+    profile_enable(engine, True)
+    ref = engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR, L.MIXED)
+    profile_enable(engine, False)
+    xyz = np.stack([s["x"], s["y"], s["z"]], 1)
+    for rep in range(4):
+        g = engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR, L.MIXED, xyz=xyz)
+        assert np.array_equal(g["desum"], ref["desum"]), rep
+        assert g["em"] == pytest.approx(ref["em"], rel=1e-14)
+        assert g["ep"] == pytest.approx(ref["ep"], rel=1e-14)
+        # eager work on the same stream (an instrumented evaluation) between replays
+        profile_enable(engine, True)
+        e = engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR, L.MIXED)
+        profile_enable(engine, False)
+        assert np.array_equal(e["desum"], ref["desum"]), rep
