@@ -176,6 +176,18 @@ PMQ_API int pmq_compute(pmq_ctx *ctx, int mode, unsigned terms, int precision);
 PMQ_API int pmq_fetch(pmq_ctx *ctx, pmq_result *res);
 PMQ_API int pmq_sync(pmq_ctx *ctx);
 
+/* Parameter gradients of the last evaluation (plain FP64, GRADIENT, one
+ * system; no charge penetration or flux).  No reference counterpart: the
+ * reference has no derivatives with respect to parameters.  Per atom, in
+ * caller order: dE/dpole[13n] with respect to the local-frame components as
+ * passed in pmq_params.pole (each stored component on its own), and, with
+ * polarization, dE/dpolarity[n] at fixed pdamp, dE/dpdamp[n] and dE/dthole[n]
+ * (the Thole parameter of a pair is the smaller of the two, so equal values
+ * share the derivative half and half).  E is em + ep of that evaluation.  The
+ * polarization part uses the variational form (mu_p is the adjoint of mu_d):
+ * no further solve.  NULL outputs are skipped. */
+PMQ_API int pmq_param_grad(pmq_ctx *ctx, double *dpole, double *dpolarity, double *dpdamp, double *dthole);
+
 /* Batched clusters.  No counterpart in the reference: the batched workload of
  * this path (many small non-periodic systems).  After pmq_set_batch the
  * context's system is a set of independent clusters, cluster c being atoms

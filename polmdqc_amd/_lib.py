@@ -71,7 +71,7 @@ class Result(C.Structure):
 
 EXPORTS = ["pmq_create", "pmq_destroy", "pmq_last_error", "pmq_abi_version", "pmq_set_topology",
            "pmq_set_params", "pmq_eval", "pmq_set_coords", "pmq_compute", "pmq_fetch", "pmq_sync",
-           "pmq_profile_enable", "pmq_profile_read", "pmq_set_batch", "pmq_fetch_batch"]
+           "pmq_profile_enable", "pmq_profile_read", "pmq_set_batch", "pmq_fetch_batch", "pmq_param_grad"]
 
 _lib = None
 
@@ -99,6 +99,7 @@ def lib() -> C.CDLL:
     L.pmq_sync.argtypes = [vp]
     L.pmq_set_batch.argtypes = [vp, C.c_int, _ip]
     L.pmq_fetch_batch.argtypes = [vp, _dp, _dp, _ip]
+    L.pmq_param_grad.argtypes = [vp, _dp, _dp, _dp, _dp]
     for name in EXPORTS:
         if name != "pmq_last_error":
             getattr(L, name).restype = C.c_int

@@ -92,33 +92,33 @@ __device__ __forceinline__ void polgrad_pair(const T *R, const Mp<T> &mi, const 
     PairOut<T> o;
     if (sdp != T(0)) {
         pair_series<T, true>(R, dipole<T>(udi[0] + upi[0], udi[1] + upi[1], udi[2] + upi[2]), mk, xr, yr, zr, o);
-        for (int c = 0; c < 3; c++) F[c] += sdp * (&o.fx)[c];
+        F[0] += sdp * o.fx; F[1] += sdp * o.fy; F[2] += sdp * o.fz;
         Tk[0] += sdp * o.tkx; Tk[1] += sdp * o.tky; Tk[2] += sdp * o.tkz;
         pair_series<T, true>(R, mi, dipole<T>(udk[0] + upk[0], udk[1] + upk[1], udk[2] + upk[2]), xr, yr, zr, o);
-        for (int c = 0; c < 3; c++) F[c] += sdp * (&o.fx)[c];
+        F[0] += sdp * o.fx; F[1] += sdp * o.fy; F[2] += sdp * o.fz;
         Ti[0] += sdp * o.tix; Ti[1] += sdp * o.tiy; Ti[2] += sdp * o.tiz;
     }
     if (sp != T(0)) {
         pair_series<T, true>(R, dipole<T>(udi[0], udi[1], udi[2]), mk, xr, yr, zr, o);
-        for (int c = 0; c < 3; c++) F[c] += sp * (&o.fx)[c];
+        F[0] += sp * o.fx; F[1] += sp * o.fy; F[2] += sp * o.fz;
         Tk[0] += sp * o.tkx; Tk[1] += sp * o.tky; Tk[2] += sp * o.tkz;
         pair_series<T, true>(R, mi, dipole<T>(udk[0], udk[1], udk[2]), xr, yr, zr, o);
-        for (int c = 0; c < 3; c++) F[c] += sp * (&o.fx)[c];
+        F[0] += sp * o.fx; F[1] += sp * o.fy; F[2] += sp * o.fz;
         Ti[0] += sp * o.tix; Ti[1] += sp * o.tiy; Ti[2] += sp * o.tiz;
     }
     if (sd != T(0)) {
         pair_series<T, true>(R, dipole<T>(upi[0], upi[1], upi[2]), mk, xr, yr, zr, o);
-        for (int c = 0; c < 3; c++) F[c] += sd * (&o.fx)[c];
+        F[0] += sd * o.fx; F[1] += sd * o.fy; F[2] += sd * o.fz;
         Tk[0] += sd * o.tkx; Tk[1] += sd * o.tky; Tk[2] += sd * o.tkz;
         pair_series<T, true>(R, mi, dipole<T>(upk[0], upk[1], upk[2]), xr, yr, zr, o);
-        for (int c = 0; c < 3; c++) F[c] += sd * (&o.fx)[c];
+        F[0] += sd * o.fx; F[1] += sd * o.fy; F[2] += sd * o.fz;
         Ti[0] += sd * o.tix; Ti[1] += sd * o.tiy; Ti[2] += sd * o.tiz;
     }
     if (su != T(0)) {
         pair_series<T, true>(R, dipole<T>(udi[0], udi[1], udi[2]), dipole<T>(upk[0], upk[1], upk[2]), xr, yr, zr, o);
-        for (int c = 0; c < 3; c++) F[c] += su * (&o.fx)[c];
+        F[0] += su * o.fx; F[1] += su * o.fy; F[2] += su * o.fz;
         pair_series<T, true>(R, dipole<T>(upi[0], upi[1], upi[2]), dipole<T>(udk[0], udk[1], udk[2]), xr, yr, zr, o);
-        for (int c = 0; c < 3; c++) F[c] += su * (&o.fx)[c];
+        F[0] += su * o.fx; F[1] += su * o.fy; F[2] += su * o.fz;
     }
 }
 
@@ -243,9 +243,8 @@ __global__ __launch_bounds__(BATCH_MAX * TPR) void k_batch_clusters(BatchArgs a)
                 double lam[6], R[6], Ei[3], Ek[3];
                 thole_lambda<double>(sqrt(r2), a.pdm[ex.i], a.pdm[ex.k], a.thl[ex.i], a.thl[ex.k], lam);
                 Mp<double> m1, m2;
-                for (int c = 0; c < 10; c++) {
-                    (&m1.c)[c] = 0; (&m2.c)[c] = 0;
-                }
+                m1 = Mp<double>{};
+                m2 = Mp<double>{};
                 m1.c = a.smp[ex.i]; m1.dx = a.smp[a.npad + ex.i]; m1.dy = a.smp[2 * (size_t)a.npad + ex.i];
                 m1.dz = a.smp[3 * (size_t)a.npad + ex.i]; m1.qxx = a.smp[4 * (size_t)a.npad + ex.i];
                 m1.qxy = a.smp[5 * (size_t)a.npad + ex.i]; m1.qxz = a.smp[6 * (size_t)a.npad + ex.i];

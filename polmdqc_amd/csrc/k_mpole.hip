@@ -65,6 +65,8 @@ struct RotArgs {
     double *smp;                      // 10 * npad sorted
     double *scp;                      // 3 * npad sorted: core, alpha, charge (monopole)
     double *rpole13;                  // 13n caller order or null
+    double *rotm;                     // 9n caller order: rows x, y, z of the local frame (or null)
+    int *flip;                        // n: 1 where chkpole inverted the y-dependent components (or null)
 };
 
 __device__ __forceinline__ void dvec(const RotArgs &a, int from, int to, double *d)
@@ -109,7 +111,12 @@ __global__ void k_rotpole(RotArgs a)
         const double vol = ad[0] * c1 + bd[0] * c2 + cd[0] * c3;
         if ((ky < 0 && vol > 0.0) || (ky > 0 && vol < 0.0)) {
             p[2] = -p[2]; p[5] = -p[5]; p[7] = -p[7]; p[9] = -p[9]; p[11] = -p[11];
+            if (a.flip) a.flip[i] = 1;
+        } else if (a.flip) {
+            a.flip[i] = 0;
         }
+    } else if (a.flip) {
+        a.flip[i] = 0;
     }
     // rotation matrix rows: ax (x axis), ay, az
     double ax[3] = {1, 0, 0}, ay[3], az[3] = {0, 0, 1}, d[3], r, dot;
@@ -222,6 +229,12 @@ __global__ void k_rotpole(RotArgs a)
     a.scp[2 * np + s] = g[0];
     if (a.rpole13)
         for (int c = 0; c < 13; c++) a.rpole13[13 * i + c] = g[c];
+    if (a.rotm)
+        for (int j = 0; j < 3; j++) {
+            a.rotm[9 * (size_t)i + j] = ax[j];
+            a.rotm[9 * (size_t)i + 3 + j] = ay[j];
+            a.rotm[9 * (size_t)i + 6 + j] = az[j];
+        }
 }
 
 int launch_rotpole(pmq_ctx *c, bool want_rpole13)
@@ -239,6 +252,8 @@ int launch_rotpole(pmq_ctx *c, bool want_rpole13)
     a.smp = c->smp.p;
     a.scp = c->scp.p;
     a.rpole13 = want_rpole13 ? c->rpole13.p : nullptr;
+    a.rotm = c->rotm.p;
+    a.flip = c->flip.p;
     const int tb = 128;
     hipLaunchKernelGGL(k_rotpole, dim3((c->npad + tb - 1) / tb), dim3(tb), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());

@@ -646,10 +646,10 @@ __global__ __launch_bounds__(256) void k_pme_mpole_post(MpolePostArgs a)
         const double dd = m[1] * m[1] + m[2] * m[2] + m[3] * m[3];
         const double qq = m[4] * m[4] + m[7] * m[7] + m[9] * m[9] + 2.0 * (m[5] * m[5] + m[6] * m[6] + m[8] * m[8]);
         e = 0.5 * a.f * U - a.f * al / 1.77245385090551602730 * (m[0] * m[0] + 2.0 * a2 / 3.0 * dd + 1.6 * a2 * a2 * qq);
+        a.cphi[s] = phi;
+        for (int q = 0; q < 3; q++) a.cphi[(1 + q) * a.npad + s] = gr[q];
+        for (int q = 0; q < 6; q++) a.cphi[(4 + q) * a.npad + s] = H[q];
         if (a.polar) {
-            a.cphi[s] = phi;
-            for (int q = 0; q < 3; q++) a.cphi[(1 + q) * a.npad + s] = gr[q];
-            for (int q = 0; q < 6; q++) a.cphi[(4 + q) * a.npad + s] = H[q];
             const double self = 4.0 / 3.0 * a2 * al / 1.77245385090551602730;
             for (int q = 0; q < 3; q++) {
                 const double v = -gr[q] + self * m[1 + q];

@@ -200,33 +200,33 @@ __device__ __forceinline__ void pol_pair_grad(const T *R, const Mp<T> &mi, const
     PairOut<T> o;
     if (sdp != T(0)) {
         pair_series<T, true>(R, dip<T>(udi[0] + upi[0], udi[1] + upi[1], udi[2] + upi[2]), mk, xr, yr, zr, o);
-        for (int c = 0; c < 3; c++) F[c] += sdp * (&o.fx)[c];
+        F[0] += sdp * o.fx; F[1] += sdp * o.fy; F[2] += sdp * o.fz;
         Tk[0] += sdp * o.tkx; Tk[1] += sdp * o.tky; Tk[2] += sdp * o.tkz;
         pair_series<T, true>(R, mi, dip<T>(udk[0] + upk[0], udk[1] + upk[1], udk[2] + upk[2]), xr, yr, zr, o);
-        for (int c = 0; c < 3; c++) F[c] += sdp * (&o.fx)[c];
+        F[0] += sdp * o.fx; F[1] += sdp * o.fy; F[2] += sdp * o.fz;
         Ti[0] += sdp * o.tix; Ti[1] += sdp * o.tiy; Ti[2] += sdp * o.tiz;
     }
     if (sp != T(0)) { // mu_d with the p-scaled permanent field
         pair_series<T, true>(R, dip<T>(udi[0], udi[1], udi[2]), mk, xr, yr, zr, o);
-        for (int c = 0; c < 3; c++) F[c] += sp * (&o.fx)[c];
+        F[0] += sp * o.fx; F[1] += sp * o.fy; F[2] += sp * o.fz;
         Tk[0] += sp * o.tkx; Tk[1] += sp * o.tky; Tk[2] += sp * o.tkz;
         pair_series<T, true>(R, mi, dip<T>(udk[0], udk[1], udk[2]), xr, yr, zr, o);
-        for (int c = 0; c < 3; c++) F[c] += sp * (&o.fx)[c];
+        F[0] += sp * o.fx; F[1] += sp * o.fy; F[2] += sp * o.fz;
         Ti[0] += sp * o.tix; Ti[1] += sp * o.tiy; Ti[2] += sp * o.tiz;
     }
     if (sd != T(0)) { // mu_p with the d-scaled permanent field
         pair_series<T, true>(R, dip<T>(upi[0], upi[1], upi[2]), mk, xr, yr, zr, o);
-        for (int c = 0; c < 3; c++) F[c] += sd * (&o.fx)[c];
+        F[0] += sd * o.fx; F[1] += sd * o.fy; F[2] += sd * o.fz;
         Tk[0] += sd * o.tkx; Tk[1] += sd * o.tky; Tk[2] += sd * o.tkz;
         pair_series<T, true>(R, mi, dip<T>(upk[0], upk[1], upk[2]), xr, yr, zr, o);
-        for (int c = 0; c < 3; c++) F[c] += sd * (&o.fx)[c];
+        F[0] += sd * o.fx; F[1] += sd * o.fy; F[2] += sd * o.fz;
         Ti[0] += sd * o.tix; Ti[1] += sd * o.tiy; Ti[2] += sd * o.tiz;
     }
     if (su != T(0)) {
         pair_series<T, true>(R, dip<T>(udi[0], udi[1], udi[2]), dip<T>(upk[0], upk[1], upk[2]), xr, yr, zr, o);
-        for (int c = 0; c < 3; c++) F[c] += su * (&o.fx)[c];
+        F[0] += su * o.fx; F[1] += su * o.fy; F[2] += su * o.fz;
         pair_series<T, true>(R, dip<T>(upi[0], upi[1], upi[2]), dip<T>(udk[0], udk[1], udk[2]), xr, yr, zr, o);
-        for (int c = 0; c < 3; c++) F[c] += su * (&o.fx)[c];
+        F[0] += su * o.fx; F[1] += su * o.fy; F[2] += su * o.fz;
     }
 }
 

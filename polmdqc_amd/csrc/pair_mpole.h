@@ -20,6 +20,41 @@ struct Mp {  // c, d[3], q{xx,xy,xz,yy,yz,zz}
     R c, dx, dy, dz, qxx, qxy, qxz, qyy, qyz, qzz;
 };
 
+// component c of a multipole (c, dx, dy, dz, qxx, qxy, qxz, qyy, qyz, qzz)
+template <class R>
+__device__ __forceinline__ R mp_get(const Mp<R> &m, int c)
+{
+    switch (c) {
+    case 0: return m.c;
+    case 1: return m.dx;
+    case 2: return m.dy;
+    case 3: return m.dz;
+    case 4: return m.qxx;
+    case 5: return m.qxy;
+    case 6: return m.qxz;
+    case 7: return m.qyy;
+    case 8: return m.qyz;
+    default: return m.qzz;
+    }
+}
+
+template <class R>
+__device__ __forceinline__ void mp_set(Mp<R> &m, int c, R v)
+{
+    switch (c) {
+    case 0: m.c = v; break;
+    case 1: m.dx = v; break;
+    case 2: m.dy = v; break;
+    case 3: m.dz = v; break;
+    case 4: m.qxx = v; break;
+    case 5: m.qxy = v; break;
+    case 6: m.qxz = v; break;
+    case 7: m.qyy = v; break;
+    case 8: m.qyz = v; break;
+    default: m.qzz = v; break;
+    }
+}
+
 template <class R>
 struct PairOut {
     R e;

@@ -300,6 +300,8 @@ int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
     PMQ_HIP(c->scratch_sort.alloc(c->scratch_sort_bytes));
     PMQ_HIP(c->dem.alloc(3 * (size_t)n)); PMQ_HIP(c->te.alloc(3 * (size_t)n));
     PMQ_HIP(c->pot.alloc(n)); PMQ_HIP(c->aem.alloc(n));
+    PMQ_HIP(c->rotm.alloc(9 * (size_t)n)); PMQ_HIP(c->flip.alloc(n));
+    PMQ_HIP(c->cphi.alloc(10 * (size_t)npad));
     // fixed-point accumulators start (and are left) at zero
     PMQ_HIP(c->fx_grad.alloc(6 * (size_t)n)); PMQ_HIP(c->fx_field.alloc(6 * (size_t)npad));
     int rz;
@@ -778,6 +780,23 @@ static int pmq_check(pmq_ctx *c)
     PMQ_HIP(hipStreamSynchronize(c->stream));
     c->safe = false;
     return PMQ_OK;
+}
+
+int pmq_param_grad(pmq_ctx *c, double *dpole, double *dpolarity, double *dpdamp, double *dthole)
+{
+    CHECK_CTX(c);
+    int rc = pmq_check(c);
+    if (rc) return rc;
+    if (c->nsys > 0 || c->last_mode != PMQ_GRADIENT || c->last_precision != PMQ_PLAIN_FP64) {
+        c->err = "pmq_param_grad: needs a plain-FP64 GRADIENT evaluation of one system first";
+        return PMQ_ERR_STATE;
+    }
+    if (c->prm.pentype != PMQ_PEN_NONE || c->prm.use_chgflx) {
+        c->err = "pmq_param_grad: charge penetration and charge flux are not differentiated";
+        return PMQ_ERR_UNSUPPORTED;
+    }
+    return param_gradients(c, c->last_terms & PMQ_TERM_MPOLE, c->last_terms & PMQ_TERM_POLAR, dpole, dpolarity,
+                           dpdamp, dthole);
 }
 
 int pmq_sync(pmq_ctx *c)

@@ -244,6 +244,10 @@ struct pmq_ctx {
     // outputs
     pmq::DBuf<double> dem, te, pot, aem;
     pmq::DBuf<long long> fx_grad;    // 6n fixed-point gradient + torque (mixed form), zero between uses
+    pmq::DBuf<double> rotm;          // 9n local-frame rotation of each site (rows x, y, z)
+    pmq::DBuf<int> flip;             // n chirality flips of the last frame pass
+    // parameter gradients (k_params.hip): sorted dE/d(global multipole), dE/d(pdamp), dE/d(thole)
+    pmq::DBuf<double> pg_dm, pg_dpd, pg_dth;
     pmq::DBuf<long long> fx_field;   // 6 npad fixed-point fields (mixed form), zero between uses
     pmq::DBuf<pmq::Totals> totals;
 
@@ -352,6 +356,8 @@ int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fi
 int tile_scalars(pmq_ctx *c, int which, bool full);
 int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst);
 int launch_batch(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int variant);
+int param_gradients(pmq_ctx *c, bool mpole, bool polar, double *dpole, double *dpolarity, double *dpdamp,
+                    double *dthole);
 int batch_variants();
 } // namespace pmq
 

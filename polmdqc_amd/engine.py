@@ -184,6 +184,15 @@ class Engine:
             out["rpole"] = rpole.reshape(n, 13)
         return out
 
+    def param_grad(self) -> dict:
+        """dE/d(parameters) of the last FP64 GRADIENT evaluation, per atom:
+        dpole (n, 13) in the local frame, dpolarity (fixed pdamp), dpdamp, dthole."""
+        n = self.n
+        dpole, dpol, dpd, dth = np.zeros(13 * n), np.zeros(n), np.zeros(n), np.zeros(n)
+        self._check(self._L.pmq_param_grad(self._ctx, dpole.ctypes.data_as(_dp), dpol.ctypes.data_as(_dp),
+                                           dpd.ctypes.data_as(_dp), dth.ctypes.data_as(_dp)), "pmq_param_grad")
+        return {"dpole": dpole.reshape(n, 13), "dpolarity": dpol, "dpdamp": dpd, "dthole": dth}
+
     def set_batch(self, offsets):
         """Treat the system as independent non-periodic clusters: cluster c is
         atoms offsets[c]:offsets[c+1] (offsets[0] = 0, offsets[-1] = n);
