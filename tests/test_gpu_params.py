@@ -123,3 +123,62 @@ def test_parameter_gradients_match_finite_differences(engine, name):
         fd = (es[0] - es[1]) / (2 * h)
         e0 = abs(es[0]) + abs(es[1])
         assert abs(fd - analytic) <= 1e-5 * abs(fd) + 1e-11 * e0 / h, (kind, t, c, fd, analytic)
+
+
+def test_positions_match_finite_differences(engine):
+    """dE/dx of 32 random coordinates per box, central differences 1e-5 A."""
+    from polmdqc_amd import _lib as L
+
+    for name, s in systems().items():
+        s = dict(s)
+        s["poleps"] = 1e-12
+        engine.set_system(s)
+        g = engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR)
+        xyz = np.stack([s["x"], s["y"], s["z"]], 1)
+        rng = np.random.default_rng(4)
+        for k in range(32):
+            i, a = int(rng.integers(len(xyz))), int(rng.integers(3))
+            es = []
+            for sgn in (1, -1):
+                x = xyz.copy()
+                x[i, a] += sgn * 1e-5
+                e = engine.eval(L.ENERGY, L.TERM_MPOLE | L.TERM_POLAR, xyz=x)
+                es.append(e["em"] + e["ep"])
+            fd = (es[0] - es[1]) / 2e-5
+            an = g["desum"][i, a]
+            assert abs(fd - an) <= 1e-5 * max(1.0, abs(an)) + 1e-11 * abs(es[0]) / 1e-5, (name, i, a, fd, an)
+
+
+def test_autograd_function(engine):
+    import torch
+
+    from polmdqc_amd.autograd import ElecEnergy, ElecModel
+
+    s = dict(systems()["cluster40"])
+    s["poleps"] = 1e-12
+    types = np.asarray(s["type"])
+    ut, tidx = np.unique(types, return_inverse=True)
+    pole = np.asarray(s["pole"], dtype=float).reshape(-1, 13)
+    pole_t = torch.tensor(np.stack([pole[tidx == k][0] for k in range(len(ut))]), requires_grad=True)
+    pol_t = torch.tensor([float(np.asarray(s["polarity"])[tidx == k][0]) for k in range(len(ut))], requires_grad=True)
+    th = np.asarray(s["thole"], dtype=float)
+    th_t = torch.tensor([float(th[tidx == k][0]) * (1.0 if k == 0 else 0.85) for k in range(len(ut))],
+                        requires_grad=True)
+    xyz = torch.tensor(np.stack([s["x"], s["y"], s["z"]], 1), requires_grad=True)
+    model = ElecModel(s, tidx, engine)
+    e = ElecEnergy.apply(xyz, pole_t, pol_t, th_t, model)
+    e.backward()
+    # one parameter of each kind against a central difference
+    for tensor, idx, h in ((pole_t, (0, 0), 1e-6), (pol_t, (1,), 1e-3 * float(pol_t[1])), (th_t, (0,), 1e-3 * float(th_t[0]))):
+        vals = []
+        for sgn in (1, -1):
+            with torch.no_grad():
+                t2 = tensor.detach().clone()
+                t2[idx] += sgn * h
+                args = {id(pole_t): pole_t.detach(), id(pol_t): pol_t.detach(), id(th_t): th_t.detach()}
+                args[id(tensor)] = t2
+                vals.append(float(ElecEnergy.apply(xyz.detach(), args[id(pole_t)], args[id(pol_t)], args[id(th_t)],
+                                                   model)))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert abs(fd - float(tensor.grad[idx])) <= 1e-5 * abs(fd) + 1e-11 * abs(vals[0]) / h
+    assert xyz.grad.shape == xyz.shape
