@@ -173,7 +173,8 @@ class ElecSys(C.Structure):
                [(k, C.c_double) for k in ("p2scale", "p3scale", "p4scale", "p5scale", "p2iscale", "p3iscale",
                                           "p4iscale", "p5iscale", "d1scale", "d2scale", "d3scale", "d4scale",
                                           "u1scale", "u2scale", "u3scale", "u4scale", "poleps")] + \
-               [("politer", C.c_int), ("solver", C.c_int)]
+               [("politer", C.c_int), ("solver", C.c_int)] + \
+               [(k, C.c_double) for k in ("w2scale", "w3scale", "w4scale", "w5scale")] + [("dpequal", C.c_int)]
 
 
 class ElecOut(C.Structure):
@@ -213,6 +214,9 @@ def elec(fx, want_grad: bool = True, solver: int = 0, x=None, poleps: float | No
     s.poleps = float(fx.get("poleps", 1e-6)) if poleps is None else poleps
     s.politer = int(fx.get("politer", 100))
     s.solver = solver
+    for k in ("w2scale", "w3scale", "w4scale", "w5scale"):
+        setattr(s, k, float(fx.get(k, 1.0)))
+    s.dpequal = int(fx.get("dpequal", 0))
     o = ElecOut()
     dem, dep, ud, up = (np.zeros(3 * n) for _ in range(4))
     o.dem, o.dep = dem.ctypes.data_as(_dp), dep.ctypes.data_as(_dp)
@@ -223,3 +227,66 @@ def elec(fx, want_grad: bool = True, solver: int = 0, x=None, poleps: float | No
     return {"em": o.em, "ep": o.ep, "em_real": o.em_real, "em_recip": o.em_recip, "em_self": o.em_self,
             "iters": o.iters, "dem": dem.reshape(n, 3), "dep": dep.reshape(n, 3), "uind": ud.reshape(n, 3),
             "uinp": up.reshape(n, 3)}
+
+
+class HippoSys(C.Structure):
+    _fields_ = [("mp", C.POINTER(MpoleSys)), ("csix", _dp), ("adisp", _dp)] + \
+               [(k, C.c_double) for k in ("dsp2scale", "dsp3scale", "dsp4scale", "dsp5scale")] + \
+               [("use_dewald", C.c_int), ("dewald", C.c_double), ("dispcut", C.c_double), ("disptaper", C.c_double),
+                ("nfft1", C.c_int), ("nfft2", C.c_int), ("nfft3", C.c_int), ("bsorder", C.c_int),
+                ("sizpr", _dp), ("dmppr", _dp), ("elepr", _dp)] + \
+               [(k, C.c_double) for k in ("r2scale", "r3scale", "r4scale", "r5scale", "repcut", "reptaper")]
+
+
+class HippoOut(C.Structure):
+    _fields_ = [("edsp", C.c_double), ("er", C.c_double), ("edsp_real", C.c_double), ("edsp_recip", C.c_double),
+                ("edsp_self", C.c_double), ("dedsp", _dp), ("der", _dp)]
+
+
+def taper_of(fx, key_cut: str, key_taper: str, default: float = 0.9) -> float:
+    """Absolute taper distance: a value below 1 is a fraction of the cutoff
+    (cutoffs.cpp:259-261)."""
+    t = float(fx.get(key_taper, default))
+    return t * float(fx[key_cut]) if t < 1.0 else t
+
+
+def hippo_terms(fx, want_grad: bool = True, x=None) -> dict:
+    """HIPPO dispersion (real + PME + self) and Pauli repulsion on a system dict."""
+    L = lib()
+    if not hasattr(L, "_hippo_ready"):
+        L.orc_hippo.argtypes = [C.POINTER(HippoSys), C.c_int, C.POINTER(HippoOut)]
+        L.orc_hippo.restype = C.c_int
+        L.orc_rep_series.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
+        L._hippo_ready = True
+    keep = _Keep()
+    mp = mpole_sys(fx, keep, x=x)
+    s = HippoSys()
+    s.mp = C.pointer(mp)
+    s.csix, s.adisp = keep.d(fx["csix"]), keep.d(fx["adisp"])
+    for k in ("dsp2scale", "dsp3scale", "dsp4scale", "dsp5scale", "r2scale", "r3scale", "r4scale", "r5scale"):
+        setattr(s, k, float(fx[k]))
+    s.use_dewald, s.dewald = int(fx.get("use_dewald", 0)), float(fx.get("dewald", 0.0))
+    s.dispcut = float(fx["dispcut"])
+    s.disptaper = taper_of(fx, "dispcut", "disptaper")
+    s.nfft1, s.nfft2, s.nfft3 = int(fx.get("nfft1", 0)), int(fx.get("nfft2", 0)), int(fx.get("nfft3", 0))
+    s.bsorder = int(fx.get("bsorder", 5))
+    s.sizpr, s.dmppr, s.elepr = keep.d(fx["sizpr"]), keep.d(fx["dmppr"]), keep.d(fx["elepr"])
+    s.repcut = float(fx["repcut"])
+    s.reptaper = taper_of(fx, "repcut", "reptaper")
+    n = int(fx["n"])
+    o = HippoOut()
+    dd, dr = np.zeros(3 * n), np.zeros(3 * n)
+    o.dedsp, o.der = dd.ctypes.data_as(_dp), dr.ctypes.data_as(_dp)
+    rc = L.orc_hippo(C.byref(s), 1 if want_grad else 0, C.byref(o))
+    if rc != 0:
+        raise RuntimeError(f"orc_hippo failed ({rc})")
+    return {"edsp": o.edsp, "er": o.er, "edsp_real": o.edsp_real, "edsp_recip": o.edsp_recip,
+            "edsp_self": o.edsp_self, "dedsp": dd.reshape(n, 3), "der": dr.reshape(n, 3)}
+
+
+def rep_series(r: float, ai: float, ak: float) -> np.ndarray:
+    hippo_lib = lib()
+    hippo_lib.orc_rep_series.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
+    B = np.zeros(6)
+    hippo_lib.orc_rep_series(float(r), float(ai), float(ak), B.ctypes.data_as(_dp))
+    return B

@@ -915,3 +915,8 @@ int orc_empole(orc_mpole_sys *s, int mode, orc_mpole_out *out)
     free(o12); free(o13); free(o14); free(o15);
     return 0;
 }
+
+void orc_damp_gordon1(double r, double ai, double ak, double *dmpi, double *dmpk, double *dmpik)
+{
+    damp_gordon1(r, ai, ak, dmpi, dmpk, dmpik);
+}

@@ -92,6 +92,10 @@ void orc_chkpole(orc_mpole_sys *s, double *pole);
 /* torque -> gradient on frame atoms, torque.cpp:42-397 (vir may be NULL) */
 void orc_torque(const orc_mpole_sys *s, const double *te, double *de, double *vir);
 
+/* Gordon1 damping factors (damping.h:32-196) of orders 1..9 (dmpi, dmpk: 5)
+ * and 1..11 (dmpik: 6) */
+void orc_damp_gordon1(double r, double ai, double ak, double *dmpi, double *dmpk, double *dmpik);
+
 /* ---------------------------------------------------------------------
  * Part 2: Ewald multipoles and AMOEBA polarization (pmq_oracle_elec.c)
  * ------------------------------------------------------------------- */
@@ -111,6 +115,11 @@ typedef struct {
     double poleps;
     int politer;
     int solver;             /* 0 preconditioned CG, 1 dense direct solve */
+    /* with charge penetration (mp->pentype != 0) polarization is HIPPO's:
+     * direct field of core + damped valence, mutual field with overlap
+     * damping and the w scales (1-2..1-5), d = p when dpequal */
+    double w2scale, w3scale, w4scale, w5scale;
+    int dpequal;
 } orc_elec_sys;
 
 typedef struct {
@@ -122,6 +131,36 @@ typedef struct {
 } orc_elec_out;
 
 int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o);
+
+/* ---------------------------------------------------------------------
+ * Part 3: HIPPO dispersion and Pauli repulsion (pmq_oracle_elec.c).  The
+ * reference has parameters but no energy for either (energy.cpp:279-280,
+ * erepel.cpp:149-151).
+ * ------------------------------------------------------------------- */
+typedef struct {
+    orc_mpole_sys *mp;                /* frames, connectivity, box */
+    /* dispersion: E = -sum s C_i C_k f^2 / r^6, f = 1.5 d5 - 0.5 d3 */
+    const double *csix, *adisp;
+    double dsp2scale, dsp3scale, dsp4scale, dsp5scale;
+    int use_dewald;
+    double dewald, dispcut, disptaper; /* taper (absolute) used without Ewald only */
+    int nfft1, nfft2, nfft3, bsorder;
+    /* repulsion: E = K_i K_k s / r * M(B), B_n = D^n S^2 */
+    const double *sizpr, *dmppr, *elepr;
+    double r2scale, r3scale, r4scale, r5scale;
+    double repcut, reptaper;
+} orc_hippo_sys;
+
+typedef struct {
+    double edsp, er;
+    double edsp_real, edsp_recip, edsp_self;
+    double *dedsp, *der;   /* 3n (may be NULL) */
+} orc_hippo_out;
+
+int orc_hippo(orc_hippo_sys *s, int want_grad, orc_hippo_out *o);
+
+/* the overlap series B_0..B_5 of one pair (tests) */
+void orc_rep_series(double r, double ai, double ak, double *B);
 
 #ifdef __cplusplus
 }

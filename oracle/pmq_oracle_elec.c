@@ -168,6 +168,9 @@ typedef struct {
     int ewald;
     /* connectivity offsets */
     int *o12, *o13, *o14, *o15, *g11, *g12, *g13, *g14;
+    /* charge penetration (HIPPO): core charges, valence exponents */
+    int cp;
+    const double *core, *alpha;
     /* PME */
     int K[3], order;
     double A[3][3];  /* u = A r */
@@ -185,7 +188,7 @@ static int *offsets(int n, const int *cnt)
 }
 
 /* scale factors of atom i against every k: m (mpole), p, d, u (polpot.h:47-66) */
-static void set_scales(const ctx_t *c, int i, double *ms, double *ps, double *ds, double *us, double v)
+static void set_scales(const ctx_t *c, int i, double *ms, double *ps, double *ds, double *us, double *ws, double v)
 {
     const orc_mpole_sys *m = c->mp;
     const orc_elec_sys *s = c->s;
@@ -194,7 +197,7 @@ static void set_scales(const ctx_t *c, int i, double *ms, double *ps, double *ds
         for (int q = 0; q < 4; q++) {
             const int *off = q == 0 ? c->o12 : q == 1 ? c->o13 : q == 2 ? c->o14 : c->o15;
             const int *lst = q == 0 ? m->i12 : q == 1 ? m->i13 : q == 2 ? m->i14 : m->i15;
-            for (int j = off[i]; j < off[i + 1]; j++) { ms[lst[j]] = 1.0; ps[lst[j]] = 1.0; }
+            for (int j = off[i]; j < off[i + 1]; j++) { ms[lst[j]] = 1.0; ps[lst[j]] = 1.0; ws[lst[j]] = 1.0; }
         }
         for (int q = 0; q < 4; q++) {
             const int *off = q == 0 ? c->g11 : q == 1 ? c->g12 : q == 2 ? c->g13 : c->g14;
@@ -209,6 +212,7 @@ static void set_scales(const ctx_t *c, int i, double *ms, double *ps, double *ds
     const double piv[4] = {s->p2iscale, s->p3iscale, s->p4iscale, s->p5iscale};
     const double dv[4] = {s->d1scale, s->d2scale, s->d3scale, s->d4scale};
     const double uv[4] = {s->u1scale, s->u2scale, s->u3scale, s->u4scale};
+    const double wv[4] = {s->w2scale, s->w3scale, s->w4scale, s->w5scale};
     for (int q = 0; q < 4; q++) {
         const int *off = q == 0 ? c->o12 : q == 1 ? c->o13 : q == 2 ? c->o14 : c->o15;
         const int *lst = q == 0 ? m->i12 : q == 1 ? m->i13 : q == 2 ? m->i14 : m->i15;
@@ -216,6 +220,7 @@ static void set_scales(const ctx_t *c, int i, double *ms, double *ps, double *ds
             const int k = lst[j];
             ms[k] = mv[q];
             ps[k] = pv[q];
+            ws[k] = wv[q];
             /* same polarization group: the "intra" value */
             if (s->ip11)
                 for (int g = c->g11[i]; g < c->g11[i + 1]; g++)
@@ -234,7 +239,7 @@ static void set_scales(const ctx_t *c, int i, double *ms, double *ps, double *ds
 typedef struct {
     int i, k;
     double r[3], r2;
-    double m, p, d, u; /* scales */
+    double m, p, d, u, w; /* scales */
 } pairrec;
 
 static pairrec *pair_list(ctx_t *c, long *np)
@@ -243,11 +248,12 @@ static pairrec *pair_list(ctx_t *c, long *np)
     const orc_mpole_sys *m = c->mp;
     double *ms = (double *)malloc(n * sizeof(double)), *ps = (double *)malloc(n * sizeof(double));
     double *ds = (double *)malloc(n * sizeof(double)), *us = (double *)malloc(n * sizeof(double));
-    for (int i = 0; i < n; i++) ms[i] = ps[i] = ds[i] = us[i] = 1.0;
+    double *ws = (double *)malloc(n * sizeof(double));
+    for (int i = 0; i < n; i++) ms[i] = ps[i] = ds[i] = us[i] = ws[i] = 1.0;
     long cap = 1024, cnt = 0;
     pairrec *L = (pairrec *)malloc(cap * sizeof(pairrec));
     for (int i = 0; i < n - 1; i++) {
-        set_scales(c, i, ms, ps, ds, us, 0.0);
+        set_scales(c, i, ms, ps, ds, us, ws, 0.0);
         for (int k = i + 1; k < n; k++) {
             double r[3] = {m->x[k] - m->x[i], m->y[k] - m->y[i], m->z[k] - m->z[i]};
             if (m->box.kind != ORC_BOX_NONE) orc_image(&m->box, &r[0], &r[1], &r[2]);
@@ -258,11 +264,11 @@ static pairrec *pair_list(ctx_t *c, long *np)
             q->i = i; q->k = k;
             memcpy(q->r, r, sizeof r);
             q->r2 = r2;
-            q->m = ms[k]; q->p = ps[k]; q->d = ds[k]; q->u = us[k];
+            q->m = ms[k]; q->p = ps[k]; q->d = c->s->dpequal ? ps[k] : ds[k]; q->u = us[k]; q->w = ws[k];
         }
-        set_scales(c, i, ms, ps, ds, us, 1.0);
+        set_scales(c, i, ms, ps, ds, us, ws, 1.0);
     }
-    free(ms); free(ps); free(ds); free(us);
+    free(ms); free(ps); free(ds); free(us); free(ws);
     *np = cnt;
     return L;
 }
@@ -627,11 +633,31 @@ static void pair_fields(const double *R, const double *mi, const double *mk, con
     }
 }
 
-/* series for a polarization interaction: Ewald + scaled, Thole-damped */
+/* damped series: Ewald real space bn - (1 - s dmp) rr, or s dmp rr */
+static void damped_series(const ctx_t *c, const pairrec *q, double scale, const double *dmp, int nd, double *R)
+{
+    double rr[6], bn[6];
+    series_rr(q->r2, rr);
+    if (c->ewald) series_ewald(q->r2, c->a, bn);
+    for (int j = 0; j < 6; j++) {
+        const double d = j < nd ? dmp[j] : 1.0;
+        R[j] = c->ewald ? bn[j] - (1.0 - scale * d) * rr[j] : scale * d * rr[j];
+    }
+}
+
+/* series for a polarization interaction: Ewald + scaled, Thole-damped; with
+ * charge penetration the overlap damping of the two valence densities
+ * (the dipole-dipole damping of HIPPO) */
 static void pol_series(const ctx_t *c, const pairrec *q, double scale, double *R)
 {
     const orc_elec_sys *s = c->s;
     double rr[6], lam[6];
+    if (c->cp) {
+        double dmpi[5], dmpk[5], dmpik[6];
+        orc_damp_gordon1(sqrt(q->r2), c->alpha[q->i], c->alpha[q->k], dmpi, dmpk, dmpik);
+        damped_series(c, q, scale, dmpik, 6, R);
+        return;
+    }
     series_rr(q->r2, rr);
     const double pgamma = fmin(s->thole[q->i], s->thole[q->k]) != 0.0 ? fmin(s->thole[q->i], s->thole[q->k])
                                                                     : fmax(s->thole[q->i], s->thole[q->k]);
@@ -654,6 +680,32 @@ static void direct_field(ctx_t *c, const pairrec *L, long np, const sitepot *pre
     for (long t = 0; t < np; t++) {
         const pairrec *q = &L[t];
         double R[6], Ei[3], Ek[3];
+        if (c->cp) {
+            /* field at i: core of k (point) + valence multipole of k damped by
+             * its one-site function (and the reverse at k) */
+            double dmpi[5], dmpk[5], dmpik[6], one[1] = {1.0};
+            orc_damp_gordon1(sqrt(q->r2), c->alpha[q->i], c->alpha[q->k], dmpi, dmpk, dmpik);
+            double ci[10] = {0}, ck[10] = {0}, vi[10], vk[10], z[10] = {0};
+            memcpy(vi, c->rp + 10 * q->i, sizeof vi);
+            memcpy(vk, c->rp + 10 * q->k, sizeof vk);
+            ci[0] = c->core[q->i]; ck[0] = c->core[q->k];
+            vi[0] -= ci[0]; vk[0] -= ck[0];
+            for (int set = 0; set < 2; set++) {
+                double *E = set ? Ep : Ed;
+                const double sc = set ? q->p : q->d;
+                double Ei2[3], Ek2[3];
+                damped_series(c, q, sc, one, 1, R);
+                pair_fields(R, ci, ck, q->r, Ei, Ek);
+                damped_series(c, q, sc, dmpk, 5, R);
+                pair_fields(R, z, vk, q->r, Ei2, Ek2);
+                for (int a = 0; a < 3; a++) Ei[a] += Ei2[a];
+                damped_series(c, q, sc, dmpi, 5, R);
+                pair_fields(R, vi, z, q->r, Ei2, Ek2);
+                for (int a = 0; a < 3; a++) Ek[a] += Ek2[a];
+                for (int a = 0; a < 3; a++) { E[3 * q->i + a] += Ei[a]; E[3 * q->k + a] += Ek[a]; }
+            }
+            continue;
+        }
         pol_series(c, q, q->d, R);
         pair_fields(R, c->rp + 10 * q->i, c->rp + 10 * q->k, q->r, Ei, Ek);
         for (int a = 0; a < 3; a++) { Ed[3 * q->i + a] += Ei[a]; Ed[3 * q->k + a] += Ek[a]; }
@@ -681,7 +733,7 @@ static void mutual_field(ctx_t *c, const pairrec *L, long np, const double *ud, 
     for (long t = 0; t < np; t++) {
         const pairrec *q = &L[t];
         double R[6];
-        pol_series(c, q, q->u, R);
+        pol_series(c, q, c->cp ? q->w : q->u, R);
         for (int set = 0; set < 2; set++) {
             const double *u = set ? up : ud;
             double *F = set ? Fp : Fd;
@@ -867,6 +919,10 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
     c.ewald = s->use_ewald;
     c.a = s->use_ewald ? s->aewald : 0.0;
     c.cut2 = s->cut * s->cut;
+    c.cp = mp->pentype != 0;
+    if (c.cp && mp->pentype != 1) return -3; /* Gordon1 (HIPPO) only */
+    c.core = mp->pcore;
+    c.alpha = mp->palpha;
     if (c.ewald && !(mp->box.kind == ORC_BOX_ORTHO || mp->box.kind == ORC_BOX_MONO || mp->box.kind == ORC_BOX_TRI))
         return -2;
     c.o12 = offsets(n, mp->n12); c.o13 = offsets(n, mp->n13);
@@ -898,6 +954,35 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
     for (long t = 0; t < np; t++) {
         const pairrec *q = &L[t];
         double R[6], rr[6];
+        if (c.cp) {
+            /* core-core (undamped), core k with valence i (dmpi), core i with
+             * valence k (dmpk), valence-valence (dmpik) */
+            double dmpi[5], dmpk[5], dmpik[6], one[1] = {1.0};
+            orc_damp_gordon1(sqrt(q->r2), c.alpha[q->i], c.alpha[q->k], dmpi, dmpk, dmpik);
+            double ci[10] = {0}, ck[10] = {0}, vi[10], vk[10];
+            memcpy(vi, c.rp + 10 * q->i, sizeof vi);
+            memcpy(vk, c.rp + 10 * q->k, sizeof vk);
+            ci[0] = c.core[q->i]; ck[0] = c.core[q->k];
+            vi[0] -= ci[0]; vk[0] -= ck[0];
+            const double *dm[4] = {one, dmpi, dmpk, dmpik};
+            const int nd[4] = {1, 5, 5, 6};
+            const double *A[4] = {ci, vi, ci, vi}, *B[4] = {ck, ck, vk, vk};
+            for (int part = 0; part < 4; part++) {
+                damped_series(&c, q, q->m, dm[part], nd[part], R);
+                for (int j = 0; j < 6; j++) R[j] *= f;
+                gpair g;
+                gpair_eval(R, A[part], B[part], q->r, want_grad, &g);
+                em_real += g.e;
+                if (want_grad)
+                    for (int a = 0; a < 3; a++) {
+                        dem[3 * q->i + a] += g.frc[a];
+                        dem[3 * q->k + a] -= g.frc[a];
+                        te[3 * q->i + a] += g.trqi[a];
+                        te[3 * q->k + a] += g.trqk[a];
+                    }
+            }
+            continue;
+        }
         series_rr(q->r2, rr);
         if (c.ewald) {
             double bn[6];
@@ -973,6 +1058,43 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
                 double R[6];
                 gpair g1, g2;
                 double frc[3] = {0, 0, 0}, ti[3] = {0, 0, 0}, tk[3] = {0, 0, 0};
+                if (c.cp) {
+                    /* the induced dipoles are points: each sees the core of the
+                     * other site undamped and its valence multipole damped by
+                     * that site's one-site function */
+                    double dmpi[5], dmpk[5], dmpik[6], one[1] = {1.0};
+                    orc_damp_gordon1(sqrt(q->r2), c.alpha[i], c.alpha[k], dmpi, dmpk, dmpik);
+                    double ci[10] = {0}, ck[10] = {0}, vi[10], vk[10];
+                    memcpy(vi, Mi, sizeof vi);
+                    memcpy(vk, Mk, sizeof vk);
+                    ci[0] = c.core[i]; ck[0] = c.core[k];
+                    vi[0] -= ci[0]; vk[0] -= ck[0];
+                    for (int set = 0; set < 2; set++) {
+                        const double sc = set ? q->d : q->p;  /* mu_d sees E^p, mu_p sees E^d */
+                        const double *di_ = set ? mpi : mdi, *dk_ = set ? mpk : mdk;
+                        damped_series(&c, q, sc, one, 1, R);
+                        gpair_eval(R, di_, ck, q->r, 1, &g1);
+                        gpair_eval(R, ci, dk_, q->r, 1, &g2);
+                        for (int a = 0; a < 3; a++) frc[a] += g1.frc[a] + g2.frc[a];
+                        damped_series(&c, q, sc, dmpk, 5, R);
+                        gpair_eval(R, di_, vk, q->r, 1, &g1);
+                        for (int a = 0; a < 3; a++) { frc[a] += g1.frc[a]; tk[a] += g1.trqk[a]; }
+                        damped_series(&c, q, sc, dmpi, 5, R);
+                        gpair_eval(R, vi, dk_, q->r, 1, &g2);
+                        for (int a = 0; a < 3; a++) { frc[a] += g2.frc[a]; ti[a] += g2.trqi[a]; }
+                    }
+                    pol_series(&c, q, q->w, R);
+                    gpair_eval(R, mdi, mpk, q->r, 1, &g1);
+                    gpair_eval(R, mpi, mdk, q->r, 1, &g2);
+                    for (int a = 0; a < 3; a++) frc[a] += g1.frc[a] + g2.frc[a];
+                    for (int a = 0; a < 3; a++) {
+                        dep[3 * i + a] += 0.5 * f * frc[a];
+                        dep[3 * k + a] -= 0.5 * f * frc[a];
+                        te[3 * i + a] += 0.5 * f * ti[a];
+                        te[3 * k + a] += 0.5 * f * tk[a];
+                    }
+                    continue;
+                }
                 /* mu_d with p-scaled permanent field */
                 pol_series(&c, q, q->p, R);
                 gpair_eval(R, mdi, Mk, q->r, 1, &g1);
@@ -1049,3 +1171,313 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
 }
 
 static void torque_to_grad(const orc_mpole_sys *mp, const double *te, double *de) { orc_torque(mp, te, de, NULL); }
+
+/* =====================================================================
+ * Part 3: HIPPO dispersion and Pauli repulsion
+ *
+ * Dispersion (Rackers et al., JCP 149, 084115 (2018)): the C6 pair energy
+ * -C_i C_k f^2 / r^6 with f = 1.5 d5 - 0.5 d3, d3/d5 the Gordon overlap
+ * damping factors of the adisp exponents (dmpik orders 3 and 5 of
+ * damping.h); scales dsp2..5 by connectivity.  With dispersion Ewald
+ * (coefficient a, g(x) = (1 + x^2 + x^4/2) e^{-x^2}):
+ *   real   -C_i C_k [s f^2 - 1 + g(a r)] / r^6,
+ *   recip  (1/2) sum_m G(m) |rho(m)|^2, G = -psi(2 pi |h|) / (V bsmod),
+ *          psi(k) = (2 pi^1.5 a^3 / 3)[sqrt(pi) b^3 erfc(b) + (1/2 - b^2) e^{-b^2}],
+ *          b = k / (2a), m = 0 included,
+ *   self   + a^6 / 12 sum C^2.
+ * Without Ewald the pair energy is switched off between disptaper and
+ * dispcut by the quintic of cutoffSwitch.cpp:125-133.
+ *
+ * Repulsion (Rackers and Ponder, JCP 150, 084104 (2019), cited at
+ * erepel.cpp:36-40): E = K_i K_k s_ik / r * M(B), M the multipole pair
+ * contraction (valence count elepr as monopole, the rotated dipoles and
+ * quadrupoles) over the radial series B_n = D^n (S^2), D = -(1/r) d/dr, S
+ * the overlap of two normalized 1s Slater functions of exponents
+ * dmppr/2 (equal exponents within 1e-3 use their mean).  The pair energy is
+ * switched off between reptaper and repcut as above.
+ * ===================================================================== */
+
+/* S = sum_c P_c(r) e^{-c r} with Laurent polynomials P_c (powers -12..2) */
+#define RS_LO (-12)
+#define RS_N 15
+typedef struct {
+    double c[2];
+    double p[2][RS_N]; /* p[t][m - RS_LO] */
+    int nt;
+} rs_fn;
+
+static void rs_overlap(double ai, double ak, rs_fn *S)
+{
+    const double a = 0.5 * ai, b = 0.5 * ak;
+    memset(S, 0, sizeof *S);
+    if (fabs(ai - ak) < 1e-3) {
+        const double z = 0.25 * (ai + ak);
+        S->nt = 1;
+        S->c[0] = z;
+        S->p[0][0 - RS_LO] = 1.0;
+        S->p[0][1 - RS_LO] = z;
+        S->p[0][2 - RS_LO] = z * z / 3.0;
+        return;
+    }
+    const double d = a * a - b * b, t = 4.0 * a * b / d, A = 8.0 * pow(a * b, 1.5) / (d * d);
+    S->nt = 2;
+    S->c[0] = b;
+    S->p[0][0 - RS_LO] = A * a;
+    S->p[0][-1 - RS_LO] = -A * t;
+    S->c[1] = a;
+    S->p[1][0 - RS_LO] = A * b;
+    S->p[1][-1 - RS_LO] = A * t;
+}
+
+/* D (P e^{-c r}) = sum_m (-m p_m r^{m-2} + c p_m r^{m-1}) e^{-c r} */
+static void rs_apply_D(rs_fn *S)
+{
+    for (int t = 0; t < S->nt; t++) {
+        double q[RS_N] = {0};
+        for (int j = 0; j < RS_N; j++) {
+            const double v = S->p[t][j];
+            if (v == 0.0) continue;
+            const int m = j + RS_LO;
+            if (j >= 2) q[j - 2] += -m * v;
+            if (j >= 1) q[j - 1] += S->c[t] * v;
+        }
+        memcpy(S->p[t], q, sizeof q);
+    }
+}
+
+static double rs_eval(const rs_fn *S, double r)
+{
+    double v = 0.0;
+    for (int t = 0; t < S->nt; t++) {
+        double s = 0.0, rp = pow(r, RS_LO);
+        for (int j = 0; j < RS_N; j++) { s += S->p[t][j] * rp; rp *= r; }
+        v += s * exp(-S->c[t] * r);
+    }
+    return v;
+}
+
+void orc_rep_series(double r, double ai, double ak, double *B)
+{
+    rs_fn S;
+    rs_overlap(ai, ak, &S);
+    double Sn[6];
+    for (int n = 0; n < 6; n++) {
+        Sn[n] = rs_eval(&S, r);
+        rs_apply_D(&S);
+    }
+    static const double C[6][6] = {{1}, {1, 1}, {1, 2, 1}, {1, 3, 3, 1}, {1, 4, 6, 4, 1}, {1, 5, 10, 10, 5, 1}};
+    for (int n = 0; n < 6; n++) {
+        double v = 0.0;
+        for (int k = 0; k <= n; k++) v += C[n][k] * Sn[k] * Sn[n - k];
+        B[n] = v;
+    }
+}
+
+/* multiplicative quintic switch (cutoffSwitch.cpp:125-133) and d/dr */
+static void taper5(double r, double cut, double off, double *T, double *dT)
+{
+    *T = 1.0;
+    *dT = 0.0;
+    if (!(cut < off) || r <= cut) return;
+    const double den = pow(off - cut, 5), off2 = off * off, cut2 = cut * cut;
+    const double c0 = off * off2 * (off2 - 5.0 * off * cut + 10.0 * cut2) / den;
+    const double c1 = -30.0 * off2 * cut2 / den;
+    const double c2 = 30.0 * (off2 * cut + off * cut2) / den;
+    const double c3 = -10.0 * (off2 + 4.0 * off * cut + cut2) / den;
+    const double c4 = 15.0 * (off + cut) / den;
+    const double c5 = -6.0 / den;
+    *T = c0 + r * (c1 + r * (c2 + r * (c3 + r * (c4 + r * c5))));
+    *dT = c1 + r * (2.0 * c2 + r * (3.0 * c3 + r * (4.0 * c4 + r * 5.0 * c5)));
+}
+
+/* connectivity scale of atom i against every k: vals for 1-2..1-5 (a later
+ * class overrides an earlier one); v = 1 resets */
+static void conn_scales(const ctx_t *c, int i, const double *vals, double *sc, int reset)
+{
+    const orc_mpole_sys *m = c->mp;
+    for (int q = 0; q < 4; q++) {
+        const int *off = q == 0 ? c->o12 : q == 1 ? c->o13 : q == 2 ? c->o14 : c->o15;
+        const int *lst = q == 0 ? m->i12 : q == 1 ? m->i13 : q == 2 ? m->i14 : m->i15;
+        for (int j = off[i]; j < off[i + 1]; j++) sc[lst[j]] = reset ? 1.0 : vals[q];
+    }
+}
+
+/* dispersion reciprocal: phi = IDFT[G DFT[rho]], returns sum_m G |rho|^2 */
+static double disp_convolve(const ctx_t *c, double *grid, double a)
+{
+    const int K0 = c->K[0], K1 = c->K[1], K2 = c->K[2];
+    const size_t N = (size_t)K0 * K1 * K2;
+    double *im = (double *)calloc(N, sizeof(double));
+    for (int ax = 0; ax < 3; ax++) dft_axis(c, grid, im, ax, +1);
+    const double V = pme_volume(c);
+    const double pre = 2.0 * pow(M_PI, 1.5) * a * a * a / 3.0;
+    double e = 0.0;
+    for (int m3 = 0; m3 < K2; m3++)
+        for (int m2 = 0; m2 < K1; m2++)
+            for (int m1 = 0; m1 < K0; m1++) {
+                const size_t id = ((size_t)m3 * K1 + m2) * K0 + m1;
+                const int f1 = m1 > K0 / 2 ? m1 - K0 : m1, f2 = m2 > K1 / 2 ? m2 - K1 : m2, f3 = m3 > K2 / 2 ? m3 - K2 : m3;
+                double h[3];
+                for (int j = 0; j < 3; j++)
+                    h[j] = f1 * c->A[0][j] / K0 + f2 * c->A[1][j] / K1 + f3 * c->A[2][j] / K2;
+                const double b = M_PI * sqrt(dot3(h, h)) / a;
+                const double psi = pre * (SQRTPI * b * b * b * erfc(b) + (0.5 - b * b) * exp(-b * b));
+                const double G = -psi / (V * c->bsmod[0][m1] * c->bsmod[1][m2] * c->bsmod[2][m3]);
+                e += G * (grid[id] * grid[id] + im[id] * im[id]);
+                grid[id] *= G;
+                im[id] *= G;
+            }
+    for (int ax = 0; ax < 3; ax++) dft_axis(c, grid, im, ax, -1);
+    free(im);
+    return e;
+}
+
+int orc_hippo(orc_hippo_sys *s, int want_grad, orc_hippo_out *o)
+{
+    orc_mpole_sys *mp = s->mp;
+    const int n = mp->n;
+    ctx_t c;
+    memset(&c, 0, sizeof c);
+    c.n = n;
+    c.mp = mp;
+    c.o12 = offsets(n, mp->n12); c.o13 = offsets(n, mp->n13);
+    c.o14 = offsets(n, mp->n14); c.o15 = offsets(n, mp->n15);
+    const int periodic = mp->box.kind != ORC_BOX_NONE;
+    const int ew = s->use_dewald && periodic;
+    if (ew && !(mp->box.kind == ORC_BOX_ORTHO || mp->box.kind == ORC_BOX_MONO || mp->box.kind == ORC_BOX_TRI))
+        return -2;
+    double *sc = (double *)malloc(n * sizeof(double));
+    double *dd = (double *)calloc(3 * (size_t)n, sizeof(double));
+    double *dr = (double *)calloc(3 * (size_t)n, sizeof(double));
+    double *te = (double *)calloc(3 * (size_t)n, sizeof(double));
+    /* ---- dispersion, real space */
+    const double a = s->dewald, a2 = a * a;
+    const double dvals[4] = {s->dsp2scale, s->dsp3scale, s->dsp4scale, s->dsp5scale};
+    const double dcut2 = s->dispcut * s->dispcut;
+    double ed_real = 0.0;
+    for (int i = 0; i < n; i++) sc[i] = 1.0;
+    for (int i = 0; i < n - 1; i++) {
+        conn_scales(&c, i, dvals, sc, 0);
+        for (int k = i + 1; k < n; k++) {
+            double r3[3] = {mp->x[k] - mp->x[i], mp->y[k] - mp->y[i], mp->z[k] - mp->z[i]};
+            if (periodic) orc_image(&mp->box, &r3[0], &r3[1], &r3[2]);
+            const double r2 = dot3(r3, r3);
+            if (r2 > dcut2) continue;
+            const double r = sqrt(r2), cc = s->csix[i] * s->csix[k];
+            double dmpi[5], dmpk[5], dmpik[6];
+            orc_damp_gordon1(r, s->adisp[i], s->adisp[k], dmpi, dmpk, dmpik);
+            const double fd = 1.5 * dmpik[2] - 0.5 * dmpik[1];
+            const double fp = 7.5 * (dmpik[2] - dmpik[3]) / r - 1.5 * (dmpik[1] - dmpik[2]) / r;
+            const double r6 = r2 * r2 * r2;
+            double e, de;
+            if (ew) {
+                const double x2 = a2 * r2, ex = exp(-x2);
+                const double g = (1.0 + x2 + 0.5 * x2 * x2) * ex, gp = -a2 * r * x2 * x2 * ex;
+                e = -cc * (sc[k] * fd * fd - 1.0 + g) / r6;
+                de = -cc * ((2.0 * sc[k] * fd * fp + gp) / r6 - 6.0 * (sc[k] * fd * fd - 1.0 + g) / (r6 * r));
+            } else {
+                double T, dT;
+                taper5(r, s->disptaper, s->dispcut, &T, &dT);
+                const double e0 = -cc * sc[k] * fd * fd / r6;
+                const double de0 = -cc * sc[k] * (2.0 * fd * fp / r6 - 6.0 * fd * fd / (r6 * r));
+                e = e0 * T;
+                de = de0 * T + e0 * dT;
+            }
+            ed_real += e;
+            if (want_grad)
+                for (int q = 0; q < 3; q++) {
+                    dd[3 * k + q] += de * r3[q] / r;
+                    dd[3 * i + q] -= de * r3[q] / r;
+                }
+        }
+        conn_scales(&c, i, dvals, sc, 1);
+    }
+    /* ---- dispersion, reciprocal and self */
+    double ed_recip = 0.0, ed_self = 0.0;
+    if (ew) {
+        orc_elec_sys es;
+        memset(&es, 0, sizeof es);
+        es.nfft1 = s->nfft1; es.nfft2 = s->nfft2; es.nfft3 = s->nfft3; es.bsorder = s->bsorder;
+        c.s = &es;
+        pme_setup(&c);
+        const size_t N = (size_t)c.K[0] * c.K[1] * c.K[2];
+        double *fm = (double *)calloc(10 * (size_t)n, sizeof(double));
+        double *fp = (double *)malloc(20 * (size_t)n * sizeof(double));
+        double *grid = (double *)malloc(N * sizeof(double));
+        for (int i = 0; i < n; i++) fm[10 * i] = s->csix[i];
+        pme_spread(&c, fm, 1, grid);
+        ed_recip = 0.5 * disp_convolve(&c, grid, a);
+        pme_gather(&c, grid, 4, fp);
+        for (int i = 0; i < n; i++) {
+            ed_self += a2 * a2 * a2 / 12.0 * s->csix[i] * s->csix[i];
+            if (want_grad) {
+                double phi, g[3], H[3][3], T[3][3][3];
+                double f20[20] = {0};
+                for (int j = 0; j < 4; j++) f20[j] = fp[20 * i + j];
+                frac_to_cart(&c, f20, &phi, g, H, T);
+                for (int q = 0; q < 3; q++) dd[3 * i + q] += s->csix[i] * g[q];
+            }
+        }
+        free(fm); free(fp); free(grid);
+        pme_free(&c);
+    }
+    o->edsp_real = ed_real;
+    o->edsp_recip = ed_recip;
+    o->edsp_self = ed_self;
+    o->edsp = ed_real + ed_recip + ed_self;
+
+    /* ---- repulsion */
+    double *pole = (double *)malloc(13 * (size_t)n * sizeof(double));
+    double *rp13 = (double *)malloc(13 * (size_t)n * sizeof(double));
+    memcpy(pole, mp->pole, 13 * (size_t)n * sizeof(double));
+    orc_chkpole(mp, pole);
+    orc_rotpole(mp, pole, rp13);
+    static const int idx[10] = {0, 1, 2, 3, 4, 5, 6, 8, 9, 12};
+    const double rvals[4] = {s->r2scale, s->r3scale, s->r4scale, s->r5scale};
+    const double rcut2 = s->repcut * s->repcut;
+    double er = 0.0;
+    for (int i = 0; i < n; i++) sc[i] = 1.0;
+    for (int i = 0; i < n - 1; i++) {
+        if (s->sizpr[i] == 0.0) continue;
+        conn_scales(&c, i, rvals, sc, 0);
+        for (int k = i + 1; k < n; k++) {
+            if (s->sizpr[k] == 0.0 || sc[k] == 0.0) continue;
+            double r3[3] = {mp->x[k] - mp->x[i], mp->y[k] - mp->y[i], mp->z[k] - mp->z[i]};
+            if (periodic) orc_image(&mp->box, &r3[0], &r3[1], &r3[2]);
+            const double r2 = dot3(r3, r3);
+            if (r2 > rcut2) continue;
+            const double r = sqrt(r2);
+            double mi[10], mk[10], B[6];
+            for (int j = 0; j < 10; j++) { mi[j] = rp13[13 * i + idx[j]]; mk[j] = rp13[13 * k + idx[j]]; }
+            mi[0] = s->elepr[i];
+            mk[0] = s->elepr[k];
+            orc_rep_series(r, s->dmppr[i], s->dmppr[k], B);
+            gpair g;
+            gpair_eval(B, mi, mk, r3, want_grad, &g);
+            const double K = s->sizpr[i] * s->sizpr[k] * sc[k];
+            double T, dT;
+            taper5(r, s->reptaper, s->repcut, &T, &dT);
+            const double e0 = K * g.e / r;
+            er += e0 * T;
+            if (want_grad)
+                for (int q = 0; q < 3; q++) {
+                    /* d/dx_i of K M / r: K (dM/dx_i / r + M r_vec / r^3); taper d/dx_i = -T' r_vec / r */
+                    const double f = T * K * (g.frc[q] / r + g.e * r3[q] / (r2 * r)) - e0 * dT * r3[q] / r;
+                    dr[3 * i + q] += f;
+                    dr[3 * k + q] -= f;
+                    te[3 * i + q] += T * K * g.trqi[q] / r;
+                    te[3 * k + q] += T * K * g.trqk[q] / r;
+                }
+        }
+        conn_scales(&c, i, rvals, sc, 1);
+    }
+    o->er = er;
+    if (want_grad) {
+        orc_torque(mp, te, dr, NULL);
+        if (o->dedsp) memcpy(o->dedsp, dd, 3 * (size_t)n * sizeof(double));
+        if (o->der) memcpy(o->der, dr, 3 * (size_t)n * sizeof(double));
+    }
+    free(sc); free(dd); free(dr); free(te); free(pole); free(rp13);
+    free(c.o12); free(c.o13); free(c.o14); free(c.o15);
+    return 0;
+}

@@ -218,6 +218,24 @@ def water_system(xyz: np.ndarray, ff: str = "water03", box: float | None = None,
         s["pentype"] = 1
         s["pcore"], s["palpha"] = cp[:, 0].copy(), cp[:, 1].copy()
         s["pval0"] = pole[:, 0] - cp[:, 0]
+    # HIPPO dispersion (csix, adisp by class: kdisp) and Pauli repulsion (size,
+    # damping exponent, valence electrons by class: krepel)
+    n0 = np.zeros(n)
+    s.update({"csix": n0.copy(), "adisp": n0.copy(), "sizpr": n0.copy(), "dmppr": n0.copy(), "elepr": n0.copy()})
+    # defaults of initprm.cpp:332-341,390-393
+    for k in ("dsp2scale", "dsp3scale", "dsp4scale", "dsp5scale", "r2scale", "r3scale", "r4scale", "r5scale"):
+        s[k] = float(P["scales"].get(k, 1.0 if k[-6] in "45" else 0.0))
+    for k in ("w2scale", "w3scale", "w4scale", "w5scale"):
+        s[k] = float(P["scales"].get(k, 1.0))
+    # d-equals-p: the direct-field scales follow the p scales (prmkey.cpp:531)
+    s["dpequal"] = 1 if P["keywords"].get("d-equals-p") else 0
+    if P.get("dispersion"):
+        dp = np.array([P["dispersion"][int(c)] for c in cls])
+        s["csix"], s["adisp"] = dp[:, 0].copy(), dp[:, 1].copy()
+    if P.get("repulsion"):
+        rp = np.array([P["repulsion"][int(c)] for c in cls])
+        # valence electron counts are stored negative (readprm.cpp:1042)
+        s["sizpr"], s["dmppr"], s["elepr"] = rp[:, 0].copy(), rp[:, 1].copy(), -np.abs(rp[:, 2])
     # box and method
     if box is None:
         s.update({"use_bounds": 0, "xbox": 0.0, "ybox": 0.0, "zbox": 0.0, "alpha": 90.0, "beta": 90.0,
@@ -232,6 +250,11 @@ def water_system(xyz: np.ndarray, ff: str = "water03", box: float | None = None,
         g = ewald["nfft"]
         s.update({"use_ewald": 1, "aewald": float(ewald["aewald"]), "ewaldcut": float(ewald.get("cutoff", 7.0)),
                   "nfft1": int(g), "nfft2": int(g), "nfft3": int(g), "bsorder": int(ewald.get("order", 5))})
+        # dispersion PME on the same grid (dewald: its Gaussian coefficient)
+        s.update({"use_dewald": 1 if "dewald" in ewald else 0, "dewald": float(ewald.get("dewald", 0.0))})
     else:
-        s.update({"use_ewald": 0, "aewald": 0.0, "ewaldcut": 7.0, "nfft1": 0, "nfft2": 0, "nfft3": 0, "bsorder": 5})
+        s.update({"use_ewald": 0, "aewald": 0.0, "ewaldcut": 7.0, "nfft1": 0, "nfft2": 0, "nfft3": 0, "bsorder": 5,
+                  "use_dewald": 0, "dewald": 0.0})
+    s["dispcut"] = float(ewald.get("cutoff", 7.0)) if ewald else s["mpolecut"]
+    s["repcut"] = float(ewald.get("cutoff", 7.0)) if ewald else s["mpolecut"]
     return s

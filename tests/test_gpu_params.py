@@ -169,7 +169,8 @@ def test_autograd_function(engine):
     e = ElecEnergy.apply(xyz, pole_t, pol_t, th_t, model)
     e.backward()
     # one parameter of each kind against a central difference
-    for tensor, idx, h in ((pole_t, (0, 0), 1e-6), (pol_t, (1,), 1e-3 * float(pol_t[1])), (th_t, (0,), 1e-3 * float(th_t[0]))):
+    for tensor, idx, h in ((pole_t, (0, 0), 1e-6), (pol_t, (1,), 1e-3 * pol_t.detach()[1].item()),
+                           (th_t, (0,), 1e-3 * th_t.detach()[0].item())):
         vals = []
         for sgn in (1, -1):
             with torch.no_grad():

@@ -13,6 +13,11 @@ so they are pinned by what the mathematics fixes:
   the Ewald-summed periodic box and for a free cluster;
 * the water parameter assignment equals the reference's own setup (kmpole,
   kpolar) on the same water molecules.
+
+The same checks run on HIPPO water (hippo19: charge penetration, the HIPPO
+direct/mutual damping, w scales, d-equals-p); without Ewald its
+charge-penetration multipole energy equals the reference's pairMpoleCP path
+(pairMpole.h:186-389) as restated in the oracle's first part.
 """
 import numpy as np
 import pytest
@@ -26,8 +31,8 @@ L = 9.32
 XYZ = water.lattice_box(3, L, 27)
 
 
-def ewald_box(aewald, nfft, polar, cutoff=4.6):
-    return water.water_system(XYZ, "water03", box=L, ewald={"aewald": aewald, "nfft": nfft, "cutoff": cutoff},
+def ewald_box(aewald, nfft, polar, cutoff=4.6, ff="water03"):
+    return water.water_system(XYZ, ff, box=L, ewald={"aewald": aewald, "nfft": nfft, "cutoff": cutoff},
                               polar=polar)
 
 
@@ -54,11 +59,12 @@ def global_multipoles(s):
     return rp[:, [0, 1, 2, 3, 4, 5, 6, 8, 9, 12]]
 
 
-def test_ewald_total_is_independent_of_alpha():
-    m10 = global_multipoles(ewald_box(1.0, 24, False))
+@pytest.mark.parametrize("ff", ["water03", "hippo19"])
+def test_ewald_total_is_independent_of_alpha(ff):
+    m10 = global_multipoles(ewald_box(1.0, 24, False, ff=ff))
     totals = []
     for a in (1.0, 1.1, 1.2):
-        r = oracle.elec(ewald_box(a, 24, False), want_grad=False)
+        r = oracle.elec(ewald_box(a, 24, False, ff=ff), want_grad=False)
         totals.append(r["em_real"] + r["em_self"] + exact_recip(XYZ, m10, L, a, 30))
     assert max(totals) - min(totals) < 1e-7 * abs(totals[0])
 
@@ -84,8 +90,9 @@ def test_self_energy_closed_form():
     assert abs(oracle.elec(s, want_grad=False)["em_self"] - expect) < 1e-10 * abs(expect)
 
 
-def test_pcg_equals_dense_solve():
-    s = ewald_box(0.7, 24, True)
+@pytest.mark.parametrize("ff", ["water03", "hippo19"])
+def test_pcg_equals_dense_solve(ff):
+    s = ewald_box(0.7, 24, True, ff=ff)
     it = oracle.elec(s, want_grad=False, poleps=1e-10)
     dn = oracle.elec(s, want_grad=False, solver=1)
     assert it["iters"] > 0
@@ -94,18 +101,20 @@ def test_pcg_equals_dense_solve():
     assert abs(it["ep"] - dn["ep"]) < 1e-9 * abs(dn["ep"])
 
 
-def test_polarization_energy_nearly_independent_of_alpha():
-    e = [oracle.elec(ewald_box(a, 48, True), want_grad=False, poleps=1e-10)["ep"] for a in (0.8, 1.0)]
+@pytest.mark.parametrize("ff", ["water03", "hippo19"])
+def test_polarization_energy_nearly_independent_of_alpha(ff):
+    e = [oracle.elec(ewald_box(a, 48, True, ff=ff), want_grad=False, poleps=1e-10)["ep"] for a in (0.8, 1.0)]
     assert abs(e[0] - e[1]) < 5e-5 * abs(e[0])
 
 
+@pytest.mark.parametrize("ff", ["water03", "hippo19"])
 @pytest.mark.parametrize("periodic", [True, False])
-def test_gradients_match_finite_differences(periodic):
+def test_gradients_match_finite_differences(periodic, ff):
     if periodic:
-        s, x0 = ewald_box(0.7, 24, True), XYZ
+        s, x0 = ewald_box(0.7, 24, True, ff=ff), XYZ
     else:
         x0 = water.cluster(6, np.random.default_rng(5))
-        s = water.water_system(x0, "water03")
+        s = water.water_system(x0, ff)
     r = oracle.elec(s, want_grad=True, poleps=1e-11)
     h = 1e-5
     rng = np.random.default_rng(1)
@@ -128,6 +137,30 @@ def test_non_ewald_multipoles_equal_the_reference_path():
     b = oracle.empole(fx, oracle.GRADIENT)
     assert abs(a["em"] - b["em"]) < 1e-10 * abs(b["em"])
     np.testing.assert_allclose(a["dem"], b["dem"], atol=1e-9)
+
+
+def test_non_ewald_charge_penetration_equals_the_reference_path():
+    """HIPPO multipoles without Ewald: the elec oracle's core/valence split
+    (undamped core-core, one-site damped core-valence, overlap-damped
+    valence-valence) equals the restated pairMpoleCP (Gordon1)."""
+    s = water.water_system(XYZ, "hippo19", box=L, cutoff=4.6)
+    a = oracle.elec(s, want_grad=True, polar=False)
+    b = oracle.empole(s, oracle.GRADIENT)
+    assert abs(a["em"] - b["em"]) < 1e-12 * abs(b["em"])
+    np.testing.assert_allclose(a["dem"], b["dem"], atol=1e-10)
+
+
+def test_hippo_dipole_damping_is_the_overlap_of_the_valence_densities():
+    """The mutual (dipole-dipole) damping of HIPPO is the valence-valence
+    overlap damping of the charge-penetration model at orders 3 and 5, and
+    the induce-1x scales reach the solve: with w2 = 1 the result changes.
+    (For water d-equals-p changes nothing: the direct-11 and the intra-group
+    polar-12/13 scales are all zero.)"""
+    s = ewald_box(0.8, 24, True, ff="hippo19")
+    assert s["dpequal"] == 1 and s["w2scale"] == 0.2
+    base = oracle.elec(s, want_grad=False, poleps=1e-10)["ep"]
+    s2 = dict(s, w2scale=1.0)
+    assert abs(oracle.elec(s2, want_grad=False, poleps=1e-10)["ep"] - base) > 1e-4 * abs(base)
 
 
 @pytest.mark.parametrize("system,ff", [("bounds_orthogonal", "water03"), ("synth_hippo27", "hippo19")])
