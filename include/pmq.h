@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PMQ_ABI_VERSION 1
+#define PMQ_ABI_VERSION 2
 
 #if defined(__GNUC__) || defined(__clang__)
 #define PMQ_API __attribute__((visibility("default")))
@@ -136,6 +136,32 @@ typedef struct {
     double ewaldcut; /* Ewald real-space cutoff, A */
     double aewald;   /* Ewald coefficient, 1/A */
     int nfft1, nfft2, nfft3, bsorder;
+    /* HIPPO polarization: with charge penetration (pentype != NONE) the
+     * polarization uses HIPPO damping, as kpolar.cpp:577 turns Thole off
+     * when chgpen sites exist: the direct field sees the core of each site
+     * undamped and its valence multipole damped by the one-site function of
+     * its palpha; the dipole-dipole field is damped by the overlap of the two
+     * valence densities and scaled by w (1-2..1-5, polpot.h:47-66); dpequal
+     * (d-equals-p, prmkey.cpp:531) makes the direct-field scales the p ones. */
+    double w2scale, w3scale, w4scale, w5scale;
+    int dpequal;
+    /* HIPPO dispersion (kdsp.h:18-19, dsppot.h:21-25, ewald.h, pme.h):
+     * E = -sum s C_i C_k f^2 / r^6, f = 1.5 d5 - 0.5 d3 of the adisp
+     * exponents; with use_dewald real space + smooth PME (grid dfft, order
+     * dbsorder, which must equal the multipole grid when both are on) +
+     * self.  disptaper (absolute, A) starts the quintic switch to dispcut
+     * when dispersion Ewald is off. */
+    const double *csix, *adisp;  /* n each (NULL = no dispersion) */
+    double dsp2scale, dsp3scale, dsp4scale, dsp5scale;
+    int use_dewald;
+    double dewald, dispcut, disptaper;
+    int dfft1, dfft2, dfft3, dbsorder;
+    /* HIPPO Pauli repulsion (repel.h:25-32, reppot.h:20-23, krepel.cpp:92-153):
+     * size, damping exponent and valence count (stored negative) per atom;
+     * switched off between reptaper (absolute) and repcut (cutoffSwitch.cpp). */
+    const double *sizpr, *dmppr, *elepr; /* n each (NULL = no repulsion) */
+    double r2scale, r3scale, r4scale, r5scale;
+    double repcut, reptaper;
 } pmq_params;
 
 typedef struct {
@@ -152,6 +178,8 @@ typedef struct {
     double *aem;   /* n per-atom em partition (analysis) */
     double *rpole; /* 13n global-frame multipoles */
     double *uind, *uinp; /* 3n induced dipoles (d and p fields) */
+    double *der;   /* 3n gradient of er (HIPPO repulsion) */
+    double *dedsp; /* 3n gradient of edsp (HIPPO dispersion) */
 } pmq_result;
 
 PMQ_API int pmq_create(const pmq_config *cfg, pmq_ctx **ctx);

@@ -169,40 +169,47 @@ static void damp_gordon1(double r, double ai, double ak, double *dmpi, double *d
     if (same) memcpy(dmpk, dmpi, 5 * sizeof(double));
     else { CV1(q, ek, dmpk); }
 #undef CV1
+    /* valence-valence factors: symmetric in the pair; evaluated with the
+     * smaller exponent first so that both orders of a pair round alike (the
+     * unequal-exponent form cancels terms up to ~1e5 times larger than the
+     * result when the exponents are close) */
+    const int sw = ai > ak;
+    const double *P = sw ? q : p, *Q = sw ? p : q;
+    const double eI = sw ? ek : ei, eK = sw ? ei : ek, aI = sw ? ak : ai, aK = sw ? ai : ak;
     if (same) {
-        dmpik[0] = 1.0 - (1.0 + 11.0 * p[1] / 16.0 + 3.0 * p[2] / 16.0 + p[3] / 48.0) * ei;
-        dmpik[1] = 1.0 - (1.0 + p[1] + 0.5 * p[2] + 7.0 * p[3] / 48.0 + p[4] / 48.0) * ei;
-        dmpik[2] = 1.0 - (1.0 + p[1] + 0.5 * p[2] + p[3] / 6.0 + p[4] / 24.0 + p[5] / 144.0) * ei;
-        dmpik[3] = 1.0 - (1.0 + p[1] + 0.5 * p[2] + p[3] / 6.0 + p[4] / 24.0 + p[5] / 120.0 + p[6] / 720.0) * ei;
-        dmpik[4] = 1.0 - (1.0 + p[1] + 0.5 * p[2] + p[3] / 6.0 + p[4] / 24.0 + p[5] / 120.0 + p[6] / 720.0
-                          + p[7] / 5040.0) * ei;
-        dmpik[5] = 1.0 - (1.0 + p[1] + 0.5 * p[2] + p[3] / 6.0 + p[4] / 24.0 + p[5] / 120.0 + p[6] / 720.0
-                          + p[7] / 5040.0 + p[8] / 45360.0) * ei;
+        dmpik[0] = 1.0 - (1.0 + 11.0 * P[1] / 16.0 + 3.0 * P[2] / 16.0 + P[3] / 48.0) * eI;
+        dmpik[1] = 1.0 - (1.0 + P[1] + 0.5 * P[2] + 7.0 * P[3] / 48.0 + P[4] / 48.0) * eI;
+        dmpik[2] = 1.0 - (1.0 + P[1] + 0.5 * P[2] + P[3] / 6.0 + P[4] / 24.0 + P[5] / 144.0) * eI;
+        dmpik[3] = 1.0 - (1.0 + P[1] + 0.5 * P[2] + P[3] / 6.0 + P[4] / 24.0 + P[5] / 120.0 + P[6] / 720.0) * eI;
+        dmpik[4] = 1.0 - (1.0 + P[1] + 0.5 * P[2] + P[3] / 6.0 + P[4] / 24.0 + P[5] / 120.0 + P[6] / 720.0
+                          + P[7] / 5040.0) * eI;
+        dmpik[5] = 1.0 - (1.0 + P[1] + 0.5 * P[2] + P[3] / 6.0 + P[4] / 24.0 + P[5] / 120.0 + P[6] / 720.0
+                          + P[7] / 5040.0 + P[8] / 45360.0) * eI;
     } else {
-        const double ai2 = ai * ai, ak2 = ak * ak;
+        const double ai2 = aI * aI, ak2 = aK * aK;
         const double ti = ak2 / (ak2 - ai2), tk = ai2 / (ai2 - ak2);
         const double ti2 = ti * ti, tk2 = tk * tk;
-        dmpik[0] = 1.0 - ti2 * (1.0 + 2.0 * tk + 0.5 * p[1]) * ei - tk2 * (1.0 + 2.0 * ti + 0.5 * q[1]) * ek;
-        dmpik[1] = 1.0 - ti2 * (1.0 + p[1] + 0.5 * p[2]) * ei - tk2 * (1.0 + q[1] + 0.5 * q[2]) * ek
-                 - 2.0 * ti2 * tk * (1.0 + p[1]) * ei - 2.0 * tk2 * ti * (1.0 + q[1]) * ek;
-        dmpik[2] = 1.0 - ti2 * (1.0 + p[1] + 0.5 * p[2] + p[3] / 6.0) * ei
-                 - tk2 * (1.0 + q[1] + 0.5 * q[2] + q[3] / 6.0) * ek
-                 - 2.0 * ti2 * tk * (1.0 + p[1] + p[2] / 3.0) * ei
-                 - 2.0 * tk2 * ti * (1.0 + q[1] + q[2] / 3.0) * ek;
-        dmpik[3] = 1.0 - ti2 * (1.0 + p[1] + 0.5 * p[2] + p[3] / 6.0 + p[4] / 30.0) * ei
-                 - tk2 * (1.0 + q[1] + 0.5 * q[2] + q[3] / 6.0 + q[4] / 30.0) * ek
-                 - 2.0 * ti2 * tk * (1.0 + p[1] + 2.0 * p[2] / 5.0 + p[3] / 15.0) * ei
-                 - 2.0 * tk2 * ti * (1.0 + q[1] + 2.0 * q[2] / 5.0 + q[3] / 15.0) * ek;
-        dmpik[4] = 1.0 - ti2 * (1.0 + p[1] + 0.5 * p[2] + p[3] / 6.0 + 4.0 * p[4] / 105.0 + p[5] / 210.0) * ei
-                 - tk2 * (1.0 + q[1] + 0.5 * q[2] + q[3] / 6.0 + 4.0 * q[4] / 105.0 + q[5] / 210.0) * ek
-                 - 2.0 * ti2 * tk * (1.0 + p[1] + 3.0 * p[2] / 7.0 + 2.0 * p[3] / 21.0 + p[4] / 105.0) * ei
-                 - 2.0 * tk2 * ti * (1.0 + q[1] + 3.0 * q[2] / 7.0 + 2.0 * q[3] / 21.0 + q[4] / 105.0) * ek;
-        dmpik[5] = 1.0 - ti2 * (1.0 + p[1] + 0.5 * p[2] + p[3] / 6.0 + 5.0 * p[4] / 126.0 + 2.0 * p[5] / 315.0
-                                + p[6] / 1890.0) * ei
-                 - tk2 * (1.0 + q[1] + 0.5 * q[2] + q[3] / 6.0 + 5.0 * q[4] / 126.0 + 2.0 * q[5] / 315.0
-                          + q[6] / 1890.0) * ek
-                 - 2.0 * ti2 * tk * (1.0 + p[1] + 4.0 * p[2] / 9.0 + p[3] / 9.0 + p[4] / 63.0 + p[5] / 945.0) * ei
-                 - 2.0 * tk2 * ti * (1.0 + q[1] + 4.0 * q[2] / 9.0 + q[3] / 9.0 + q[4] / 63.0 + q[5] / 945.0) * ek;
+        dmpik[0] = 1.0 - ti2 * (1.0 + 2.0 * tk + 0.5 * P[1]) * eI - tk2 * (1.0 + 2.0 * ti + 0.5 * Q[1]) * eK;
+        dmpik[1] = 1.0 - ti2 * (1.0 + P[1] + 0.5 * P[2]) * eI - tk2 * (1.0 + Q[1] + 0.5 * Q[2]) * eK
+                 - 2.0 * ti2 * tk * (1.0 + P[1]) * eI - 2.0 * tk2 * ti * (1.0 + Q[1]) * eK;
+        dmpik[2] = 1.0 - ti2 * (1.0 + P[1] + 0.5 * P[2] + P[3] / 6.0) * eI
+                 - tk2 * (1.0 + Q[1] + 0.5 * Q[2] + Q[3] / 6.0) * eK
+                 - 2.0 * ti2 * tk * (1.0 + P[1] + P[2] / 3.0) * eI
+                 - 2.0 * tk2 * ti * (1.0 + Q[1] + Q[2] / 3.0) * eK;
+        dmpik[3] = 1.0 - ti2 * (1.0 + P[1] + 0.5 * P[2] + P[3] / 6.0 + P[4] / 30.0) * eI
+                 - tk2 * (1.0 + Q[1] + 0.5 * Q[2] + Q[3] / 6.0 + Q[4] / 30.0) * eK
+                 - 2.0 * ti2 * tk * (1.0 + P[1] + 2.0 * P[2] / 5.0 + P[3] / 15.0) * eI
+                 - 2.0 * tk2 * ti * (1.0 + Q[1] + 2.0 * Q[2] / 5.0 + Q[3] / 15.0) * eK;
+        dmpik[4] = 1.0 - ti2 * (1.0 + P[1] + 0.5 * P[2] + P[3] / 6.0 + 4.0 * P[4] / 105.0 + P[5] / 210.0) * eI
+                 - tk2 * (1.0 + Q[1] + 0.5 * Q[2] + Q[3] / 6.0 + 4.0 * Q[4] / 105.0 + Q[5] / 210.0) * eK
+                 - 2.0 * ti2 * tk * (1.0 + P[1] + 3.0 * P[2] / 7.0 + 2.0 * P[3] / 21.0 + P[4] / 105.0) * eI
+                 - 2.0 * tk2 * ti * (1.0 + Q[1] + 3.0 * Q[2] / 7.0 + 2.0 * Q[3] / 21.0 + Q[4] / 105.0) * eK;
+        dmpik[5] = 1.0 - ti2 * (1.0 + P[1] + 0.5 * P[2] + P[3] / 6.0 + 5.0 * P[4] / 126.0 + 2.0 * P[5] / 315.0
+                                + P[6] / 1890.0) * eI
+                 - tk2 * (1.0 + Q[1] + 0.5 * Q[2] + Q[3] / 6.0 + 5.0 * Q[4] / 126.0 + 2.0 * Q[5] / 315.0
+                          + Q[6] / 1890.0) * eK
+                 - 2.0 * ti2 * tk * (1.0 + P[1] + 4.0 * P[2] / 9.0 + P[3] / 9.0 + P[4] / 63.0 + P[5] / 945.0) * eI
+                 - 2.0 * tk2 * ti * (1.0 + Q[1] + 4.0 * Q[2] / 9.0 + Q[3] / 9.0 + Q[4] / 63.0 + Q[5] / 945.0) * eK;
     }
 }
 

@@ -58,7 +58,18 @@ class Params(C.Structure):
                 ("u1scale", C.c_double), ("u2scale", C.c_double), ("u3scale", C.c_double), ("u4scale", C.c_double),
                 ("poleps", C.c_double), ("politer", C.c_int),
                 ("use_ewald", C.c_int), ("mpolecut", C.c_double), ("ewaldcut", C.c_double), ("aewald", C.c_double),
-                ("nfft1", C.c_int), ("nfft2", C.c_int), ("nfft3", C.c_int), ("bsorder", C.c_int)]
+                ("nfft1", C.c_int), ("nfft2", C.c_int), ("nfft3", C.c_int), ("bsorder", C.c_int),
+                # HIPPO polarization, dispersion, repulsion
+                ("w2scale", C.c_double), ("w3scale", C.c_double), ("w4scale", C.c_double), ("w5scale", C.c_double),
+                ("dpequal", C.c_int),
+                ("csix", _dp), ("adisp", _dp),
+                ("dsp2scale", C.c_double), ("dsp3scale", C.c_double), ("dsp4scale", C.c_double),
+                ("dsp5scale", C.c_double),
+                ("use_dewald", C.c_int), ("dewald", C.c_double), ("dispcut", C.c_double), ("disptaper", C.c_double),
+                ("dfft1", C.c_int), ("dfft2", C.c_int), ("dfft3", C.c_int), ("dbsorder", C.c_int),
+                ("sizpr", _dp), ("dmppr", _dp), ("elepr", _dp),
+                ("r2scale", C.c_double), ("r3scale", C.c_double), ("r4scale", C.c_double), ("r5scale", C.c_double),
+                ("repcut", C.c_double), ("reptaper", C.c_double)]
 
 
 class Result(C.Structure):
@@ -66,7 +77,7 @@ class Result(C.Structure):
                 ("esum", C.c_double), ("einter", C.c_double), ("nem", C.c_longlong),
                 ("vir", C.c_double * 9), ("pcg_iters", C.c_int),
                 ("dem", _dp), ("dep", _dp), ("desum", _dp), ("aem", _dp), ("rpole", _dp),
-                ("uind", _dp), ("uinp", _dp)]
+                ("uind", _dp), ("uinp", _dp), ("der", _dp), ("dedsp", _dp)]
 
 
 EXPORTS = ["pmq_create", "pmq_destroy", "pmq_last_error", "pmq_abi_version", "pmq_set_topology",

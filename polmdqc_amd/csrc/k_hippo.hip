@@ -1,0 +1,434 @@
+// k_hippo.hip -- HIPPO damped dispersion and Pauli repulsion.
+//
+// The reference carries the parameters of both terms and no energy:
+// dispersion is absent (energy.cpp:280; dspsix/dspdmp per class at
+// kdsp.h:18-19, dsp2..5scale at dsppot.h:21-25, DISPERSION at
+// readprm.cpp:1047); erepel_a is an empty pair loop after chkrepole,
+// rotpole(Repel) and cutoffSwitch(Repuls) (erepel.cpp:42-153; sizpr, dmppr,
+// elepr, r2..5scale at repel.h:25-32, reppot.h:20-23, krepel.cpp:92-153).
+//
+// Dispersion (Rackers et al., JCP 149, 084115 (2018)):
+//   E_ik = -s C_i C_k f^2 / r^6,  f = 1.5 d5 - 0.5 d3,
+// d3, d5 the overlap damping factors of the two adisp exponents (the
+// valence-valence factors of the charge-penetration model).  With
+// dispersion Ewald (coefficient a) the real-space pair is
+// -C_i C_k [s f^2 - 1 + g(a r)] / r^6, g(x) = (1 + x^2 + x^4/2) e^{-x^2}; the
+// reciprocal part and the self term a^6/12 C^2 are in k_pme.hip.  Without
+// Ewald the pair is switched off between disptaper and dispcut.
+//
+// Repulsion (Rackers and Ponder, JCP 150, 084104 (2019)):
+//   E_ik = K_i K_k s / r * M(B),
+// M the multipole pair contraction (monopole = elepr, the valence electron
+// count; the rotated dipoles and quadrupoles) over the radial series
+// B_n = D^n S^2 (rep_series in pair_mpole.h), switched off between
+// reptaper and repcut.  The torques go to the frames through k_torque.
+//
+// Kernels follow the multipole tiles: every unscaled pair in k_*_tiles
+// (per-tile partial sums reduced per atom in a fixed order), and the scaled
+// pairs whole, in FP64, in k_*_excl.
+#include "pair_mpole.h"
+#include "pmq_internal.h"
+
+namespace pmq {
+
+__global__ void k_hippo_sort(int n, int npad, const int *perm, const double *csix, const double *adisp,
+                             const double *sizpr, const double *dmppr, const double *elepr, double *shp)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= npad) return;
+    const int i = s < n ? perm[s] : -1;
+    shp[s] = i >= 0 && csix ? csix[i] : 0.0;
+    shp[npad + s] = i >= 0 && adisp ? adisp[i] : 1.0;
+    shp[2 * npad + s] = i >= 0 && sizpr ? sizpr[i] : 0.0;
+    shp[3 * npad + s] = i >= 0 && dmppr ? dmppr[i] : 1.0;
+    shp[4 * npad + s] = i >= 0 && elepr ? elepr[i] : 0.0;
+}
+
+int hippo_sort(pmq_ctx *c)
+{
+    const int np = c->npad;
+    hipLaunchKernelGGL(k_hippo_sort, dim3((np + 255) / 256), dim3(256), 0, c->stream, c->n, np, c->perm.p,
+                       c->have_disp ? c->csix.p : nullptr, c->have_disp ? c->adisp.p : nullptr,
+                       c->have_rep ? c->sizpr.p : nullptr, c->have_rep ? c->dmppr.p : nullptr,
+                       c->have_rep ? c->elepr.p : nullptr, c->shp.p);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+struct HippoArgs {
+    int n, npad;
+    const int2 *tiles;
+    const int *ntiles;
+    const double *sx, *sy, *sz, *smp, *shp;
+    const int *perm, *iperm;
+    DevBox box;
+    double off2, dewald;   // term cutoff^2; dispersion Ewald coefficient (0 = none)
+    Taper taper;
+    const unsigned long long *mask; // 128 per tile: [0,64) row masks of the in-cutoff pairs, scaled pairs left out
+    void *part;            // per tile partial sums
+    double *tscal;
+    const ExclPair *excl;
+    int nexcl;
+    double *de, *te;       // caller order gradient (and torque, repulsion)
+    long long *deq, *teq;  // mixed form: fixed-point accumulators
+    Totals *tot;
+};
+
+template <class T>
+__device__ __forceinline__ T wsum(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// dispersion pair: energy and dE/dr at scale s (EW: the Ewald real-space form)
+template <class T, bool EW>
+__device__ __forceinline__ void disp_pair(T r2, T ci, T ck, T ai, T ak, T s, T dewald, const Taper &tp, T &e, T &de)
+{
+    const T r = sqrt(r2);
+    T dmp[6];
+    overlap_damp<T>(r, ai, ak, dmp);
+    const T f = T(1.5) * dmp[2] - T(0.5) * dmp[1];
+    // the factors obey d(d_n r^-n)/dr = -n d_{n+2} r^-(n+1), so
+    // d3' = 3 (d3 - d5) / r and d5' = 5 (d5 - d7) / r
+    const T fp = (T(7.5) * (dmp[2] - dmp[3]) - T(1.5) * (dmp[1] - dmp[2])) / r;
+    const T r6 = r2 * r2 * r2, cc = ci * ck;
+    if (EW) {
+        const T a2 = dewald * dewald, x2 = a2 * r2, ex = exp(-x2);
+        const T g = (T(1) + x2 + T(0.5) * x2 * x2) * ex, gp = -a2 * r * x2 * x2 * ex;
+        const T u = s * f * f - T(1) + g;
+        e = -cc * u / r6;
+        de = -cc * ((T(2) * s * f * fp + gp) / r6 - T(6) * u / (r6 * r));
+    } else {
+        T tv, tdv;
+        taper_eval<T>(tp, r, tv, tdv);
+        const T e0 = -cc * s * f * f / r6;
+        const T de0 = -cc * s * (T(2) * f * fp / r6 - T(6) * f * f / (r6 * r));
+        e = e0 * tv;
+        de = de0 * tv + e0 * tdv;
+    }
+}
+
+template <bool EW, class T>
+__global__ __launch_bounds__(64) void k_disp_tiles(HippoArgs a)
+{
+    __shared__ double jx[64], jy[64], jz[64];
+    __shared__ T jc[64], ja[64], acc[3][64];
+    const int lane = threadIdx.x, np = a.npad;
+    const int ntiles = *a.ntiles;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 IJ = a.tiles[t];
+        const int si = IJ.x * 64 + lane, sj = IJ.y * 64 + lane;
+        __syncthreads();
+        jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
+        jc[lane] = a.shp[sj];
+        ja[lane] = a.shp[np + sj];
+        acc[0][lane] = acc[1][lane] = acc[2][lane] = T(0);
+        const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
+        const T ci = a.shp[si], ai = a.shp[np + si];
+        __syncthreads();
+        unsigned long long mask = a.mask[(size_t)t * 128 + lane];
+        double e = 0.0;
+        T fx = 0, fy = 0, fz = 0;
+        while (mask) {
+            const int jj = __ffsll((long long)mask) - 1;
+            mask &= mask - 1;
+            double xd = jx[jj] - xi, yd = jy[jj] - yi, zd = jz[jj] - zi;
+            image<double>(a.box, xd, yd, zd);
+            const double r2d = xd * xd + yd * yd + zd * zd;
+            if (r2d > a.off2 || ci == T(0) || jc[jj] == T(0)) continue;
+            const T xr = T(xd), yr = T(yd), zr = T(zd), r2 = T(r2d);
+            T ep, de;
+            disp_pair<T, EW>(r2, ci, jc[jj], ai, ja[jj], T(1), T(a.dewald), a.taper, ep, de);
+            e += (double)ep;
+            const T g = de / sqrt(r2); // dE/dx_k = g r_vec
+            fx -= g * xr; fy -= g * yr; fz -= g * zr;
+            atomicAdd(&acc[0][jj], g * xr);
+            atomicAdd(&acc[1][jj], g * yr);
+            atomicAdd(&acc[2][jj], g * zr);
+        }
+        __syncthreads();
+        T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
+        T *Q = P + PMQ_TILE_NV * 64;
+        P[lane] = fx; P[64 + lane] = fy; P[128 + lane] = fz;
+        Q[lane] = acc[0][lane]; Q[64 + lane] = acc[1][lane]; Q[128 + lane] = acc[2][lane];
+        const double ew = wsum(e);
+        if (lane == 0) a.tscal[(size_t)t * PMQ_TILE_NS] = ew;
+    }
+}
+
+// scaled pairs, whole, at their scale (Ewald form, or switched by the taper)
+__global__ __launch_bounds__(64) void k_disp_excl(HippoArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nexcl) return;
+    const ExclPair q = a.excl[t];
+    const int np = a.npad;
+    if (q.dsp == 0.0 && !(a.dewald > 0.0)) return;
+    const int si = a.iperm[q.i], sk = a.iperm[q.k];
+    double xr = a.sx[sk] - a.sx[si], yr = a.sy[sk] - a.sy[si], zr = a.sz[sk] - a.sz[si];
+    image<double>(a.box, xr, yr, zr);
+    const double r2 = xr * xr + yr * yr + zr * zr;
+    if (r2 > a.off2) return;
+    double e, de;
+    if (a.dewald > 0.0)
+        disp_pair<double, true>(r2, a.shp[si], a.shp[sk], a.shp[np + si], a.shp[np + sk], q.dsp, a.dewald, a.taper, e,
+                                de);
+    else
+        disp_pair<double, false>(r2, a.shp[si], a.shp[sk], a.shp[np + si], a.shp[np + sk], q.dsp, 0.0, a.taper, e, de);
+    atomicAdd(&a.tot->edsp, e);
+    const double g = de / sqrt(r2);
+    grad_add(a.de, a.deq, 3 * q.k + 0, g * xr); grad_add(a.de, a.deq, 3 * q.k + 1, g * yr);
+    grad_add(a.de, a.deq, 3 * q.k + 2, g * zr);
+    grad_add(a.de, a.deq, 3 * q.i + 0, -g * xr); grad_add(a.de, a.deq, 3 * q.i + 1, -g * yr);
+    grad_add(a.de, a.deq, 3 * q.i + 2, -g * zr);
+}
+
+// repulsion pair at scale s: energy, dE/dx_i, torques
+template <class T>
+__device__ __forceinline__ T rep_pair(T r2, T xr, T yr, T zr, T si_, T sk_, T di, T dk, const Mp<T> &mi,
+                                      const Mp<T> &mk, T s, const Taper &tp, T *F, T *Ti, T *Tk)
+{
+    const T r = sqrt(r2), rinv = T(1) / r;
+    T B[6];
+    rep_series<T>(r, di, dk, B);
+    PairOut<T> o;
+    pair_series<T, true>(B, mi, mk, xr, yr, zr, o);
+    const T K = si_ * sk_ * s;
+    T tv, tdv;
+    taper_eval<T>(tp, r, tv, tdv);
+    const T e0 = K * o.e * rinv;
+    const T w = tv * K * rinv;
+    const T c1 = tv * K * o.e * rinv * rinv * rinv - e0 * tdv * rinv;
+    F[0] = w * o.fx + c1 * xr;
+    F[1] = w * o.fy + c1 * yr;
+    F[2] = w * o.fz + c1 * zr;
+    Ti[0] = w * o.tix; Ti[1] = w * o.tiy; Ti[2] = w * o.tiz;
+    Tk[0] = w * o.tkx; Tk[1] = w * o.tky; Tk[2] = w * o.tkz;
+    return e0 * tv;
+}
+
+template <class T>
+__device__ __forceinline__ Mp<T> rep_mp(const double *smp, const double *shp, int np, int s)
+{
+    Mp<T> m;
+    m.c = shp[4 * np + s];
+    m.dx = smp[np + s]; m.dy = smp[2 * np + s]; m.dz = smp[3 * np + s];
+    m.qxx = smp[4 * np + s]; m.qxy = smp[5 * np + s]; m.qxz = smp[6 * np + s];
+    m.qyy = smp[7 * np + s]; m.qyz = smp[8 * np + s]; m.qzz = smp[9 * np + s];
+    return m;
+}
+
+template <class T>
+__global__ __launch_bounds__(64) void k_rep_tiles(HippoArgs a)
+{
+    __shared__ double jx[64], jy[64], jz[64];
+    __shared__ T jm[10][64], jsz[64], jdm[64], acc[6][64];
+    const int lane = threadIdx.x, np = a.npad;
+    const int ntiles = *a.ntiles;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 IJ = a.tiles[t];
+        const int si = IJ.x * 64 + lane, sj = IJ.y * 64 + lane;
+        __syncthreads();
+        jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
+        {
+            const Mp<T> m = rep_mp<T>(a.smp, a.shp, np, sj);
+            jm[0][lane] = m.c; jm[1][lane] = m.dx; jm[2][lane] = m.dy; jm[3][lane] = m.dz;
+            jm[4][lane] = m.qxx; jm[5][lane] = m.qxy; jm[6][lane] = m.qxz; jm[7][lane] = m.qyy;
+            jm[8][lane] = m.qyz; jm[9][lane] = m.qzz;
+        }
+        jsz[lane] = a.shp[2 * np + sj];
+        jdm[lane] = a.shp[3 * np + sj];
+        for (int c = 0; c < 6; c++) acc[c][lane] = T(0);
+        const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
+        const Mp<T> mi = rep_mp<T>(a.smp, a.shp, np, si);
+        const T szi = a.shp[2 * np + si], dmi = a.shp[3 * np + si];
+        __syncthreads();
+        unsigned long long mask = a.mask[(size_t)t * 128 + lane];
+        double e = 0.0;
+        T F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0};
+        while (mask) {
+            const int jj = __ffsll((long long)mask) - 1;
+            mask &= mask - 1;
+            double xd = jx[jj] - xi, yd = jy[jj] - yi, zd = jz[jj] - zi;
+            image<double>(a.box, xd, yd, zd);
+            const double r2d = xd * xd + yd * yd + zd * zd;
+            if (r2d > a.off2 || szi == T(0) || jsz[jj] == T(0)) continue;
+            const T xr = T(xd), yr = T(yd), zr = T(zd), r2 = T(r2d);
+            Mp<T> mk;
+            mk.c = jm[0][jj]; mk.dx = jm[1][jj]; mk.dy = jm[2][jj]; mk.dz = jm[3][jj];
+            mk.qxx = jm[4][jj]; mk.qxy = jm[5][jj]; mk.qxz = jm[6][jj]; mk.qyy = jm[7][jj];
+            mk.qyz = jm[8][jj]; mk.qzz = jm[9][jj];
+            T f[3], ti[3], tk[3];
+            e += (double)rep_pair<T>(r2, xr, yr, zr, szi, jsz[jj], dmi, jdm[jj], mi, mk, T(1), a.taper, f, ti, tk);
+            F[0] += f[0]; F[1] += f[1]; F[2] += f[2];
+            Ti[0] += ti[0]; Ti[1] += ti[1]; Ti[2] += ti[2];
+            atomicAdd(&acc[0][jj], -f[0]);
+            atomicAdd(&acc[1][jj], -f[1]);
+            atomicAdd(&acc[2][jj], -f[2]);
+            atomicAdd(&acc[3][jj], tk[0]);
+            atomicAdd(&acc[4][jj], tk[1]);
+            atomicAdd(&acc[5][jj], tk[2]);
+        }
+        __syncthreads();
+        T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
+        T *Q = P + PMQ_TILE_NV * 64;
+        for (int c = 0; c < 3; c++) {
+            P[c * 64 + lane] = F[c];
+            P[(3 + c) * 64 + lane] = Ti[c];
+        }
+        for (int c = 0; c < 6; c++) Q[c * 64 + lane] = acc[c][lane];
+        const double ew = wsum(e);
+        if (lane == 0) a.tscal[(size_t)t * PMQ_TILE_NS] = ew;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_rep_excl(HippoArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nexcl) return;
+    const ExclPair q = a.excl[t];
+    if (q.rep == 0.0) return;
+    const int np = a.npad;
+    const int si = a.iperm[q.i], sk = a.iperm[q.k];
+    const double szi = a.shp[2 * np + si], szk = a.shp[2 * np + sk];
+    if (szi == 0.0 || szk == 0.0) return;
+    double xr = a.sx[sk] - a.sx[si], yr = a.sy[sk] - a.sy[si], zr = a.sz[sk] - a.sz[si];
+    image<double>(a.box, xr, yr, zr);
+    const double r2 = xr * xr + yr * yr + zr * zr;
+    if (r2 > a.off2) return;
+    const Mp<double> mi = rep_mp<double>(a.smp, a.shp, np, si), mk = rep_mp<double>(a.smp, a.shp, np, sk);
+    double F[3], Ti[3], Tk[3];
+    const double e = rep_pair<double>(r2, xr, yr, zr, szi, szk, a.shp[3 * np + si], a.shp[3 * np + sk], mi, mk,
+                                      q.rep, a.taper, F, Ti, Tk);
+    atomicAdd(&a.tot->er, e);
+    for (int c = 0; c < 3; c++) {
+        grad_add(a.de, a.deq, 3 * q.i + c, F[c]);
+        grad_add(a.de, a.deq, 3 * q.k + c, -F[c]);
+        grad_add(a.te, a.teq, 3 * q.i + c, Ti[c]);
+        grad_add(a.te, a.teq, 3 * q.k + c, Tk[c]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+// quintic switch coefficients between cut and off (cutoffSwitch.cpp:125-133);
+// no switch when cut >= off
+static Taper make_taper(double cut, double off)
+{
+    Taper tp;
+    tp.cut = 1e300;
+    for (int q = 0; q < 6; q++) tp.c[q] = 0.0;
+    if (!(cut < off)) return tp;
+    const double den = pow(off - cut, 5), off2 = off * off, cut2 = cut * cut;
+    tp.cut = cut;
+    tp.c[0] = off * off2 * (off2 - 5.0 * off * cut + 10.0 * cut2) / den;
+    tp.c[1] = -30.0 * off2 * cut2 / den;
+    tp.c[2] = 30.0 * (off2 * cut + off * cut2) / den;
+    tp.c[3] = -10.0 * (off2 + 4.0 * off * cut + cut2) / den;
+    tp.c[4] = 15.0 * (off + cut) / den;
+    tp.c[5] = -6.0 / den;
+    return tp;
+}
+
+static HippoArgs hippo_args(pmq_ctx *c, double off2)
+{
+    HippoArgs a;
+    a.n = c->n; a.npad = c->npad;
+    a.tiles = c->tiles.p; a.ntiles = c->ntiles_eff.p;
+    a.sx = c->sx.p; a.sy = c->sy.p; a.sz = c->sz.p; a.smp = c->smp.p; a.shp = c->shp.p;
+    a.perm = c->perm.p; a.iperm = c->iperm.p;
+    a.box = c->box;
+    a.off2 = off2;
+    a.dewald = 0.0;
+    a.taper = make_taper(1e300, 0.0);
+    a.mask = c->xmask.p;
+    a.part = c->tpart.p;
+    a.tscal = c->tscal.p;
+    a.excl = c->excl.p;
+    a.nexcl = (int)c->h_excl.size();
+    a.de = nullptr; a.te = nullptr; a.deq = nullptr; a.teq = nullptr;
+    a.tot = c->totals.p;
+    return a;
+}
+
+int launch_disp(pmq_ctx *c, double off2, bool grad, bool mixed)
+{
+    HippoArgs a = hippo_args(c, off2);
+    const bool ew = c->prm.use_dewald && c->box.kind != PMQ_BOX_NONE;
+    a.dewald = ew ? c->prm.dewald : 0.0;
+    if (!ew) a.taper = make_taper(c->prm.disptaper, sqrt(off2));
+    a.de = c->dedsp.p;
+    a.deq = mixed ? c->fx_grad.p : nullptr;
+    {
+        ProfScope ps_(c, "disp_tiles");
+        const dim3 g(c->tgrid);
+        if (mixed) {
+            if (ew) hipLaunchKernelGGL((k_disp_tiles<true, float>), g, dim3(64), 0, c->stream, a);
+            else hipLaunchKernelGGL((k_disp_tiles<false, float>), g, dim3(64), 0, c->stream, a);
+        } else {
+            if (ew) hipLaunchKernelGGL((k_disp_tiles<true, double>), g, dim3(64), 0, c->stream, a);
+            else hipLaunchKernelGGL((k_disp_tiles<false, double>), g, dim3(64), 0, c->stream, a);
+        }
+    }
+    PMQ_HIP(hipGetLastError());
+    int rc;
+    if (grad) {
+        TileReduceArgs r{};
+        for (int q = 0; q < 3; q++) {
+            r.out[q] = c->dedsp.p;
+            r.caller_order[q] = 1;
+            r.stride[q] = 3;
+            r.off[q] = q;
+            r.fxscale[q] = PMQ_FX_GRAD;
+        }
+        if ((rc = tile_reduce(c, r, 3, mixed, mixed))) return rc;
+    }
+    if ((rc = tile_scalars(c, 3, false))) return rc;
+    if (a.nexcl) hipLaunchKernelGGL(k_disp_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
+    PMQ_HIP(hipGetLastError());
+    if (mixed && grad && (rc = launch_fx_flush(c, c->fx_grad.p, c->dedsp.p, 3 * (size_t)c->n, PMQ_FX_GRAD)))
+        return rc;
+    return PMQ_OK;
+}
+
+int launch_rep(pmq_ctx *c, double off2, bool grad, bool mixed)
+{
+    HippoArgs a = hippo_args(c, off2);
+    a.taper = make_taper(c->prm.reptaper, sqrt(off2));
+    a.de = c->der.p;
+    a.te = c->ter.p;
+    a.deq = mixed ? c->fx_grad.p : nullptr;
+    a.teq = mixed ? c->fx_grad.p + 3 * (size_t)c->n : nullptr;
+    {
+        ProfScope ps_(c, "rep_tiles");
+        const dim3 g(c->tgrid);
+        if (mixed) hipLaunchKernelGGL((k_rep_tiles<float>), g, dim3(64), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_rep_tiles<double>), g, dim3(64), 0, c->stream, a);
+    }
+    PMQ_HIP(hipGetLastError());
+    int rc;
+    if (grad) {
+        TileReduceArgs r{};
+        for (int q = 0; q < 3; q++) {
+            r.out[q] = c->der.p; r.out[3 + q] = c->ter.p;
+            r.caller_order[q] = r.caller_order[3 + q] = 1;
+            r.stride[q] = r.stride[3 + q] = 3;
+            r.off[q] = r.off[3 + q] = q;
+            r.fxscale[q] = r.fxscale[3 + q] = PMQ_FX_GRAD;
+        }
+        if ((rc = tile_reduce(c, r, 6, mixed, mixed))) return rc;
+    }
+    if ((rc = tile_scalars(c, 2, false))) return rc;
+    if (a.nexcl) hipLaunchKernelGGL(k_rep_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
+    PMQ_HIP(hipGetLastError());
+    if (mixed && grad) {
+        if ((rc = launch_fx_flush(c, c->fx_grad.p, c->der.p, 3 * (size_t)c->n, PMQ_FX_GRAD))) return rc;
+        if ((rc = launch_fx_flush(c, c->fx_grad.p + 3 * (size_t)c->n, c->ter.p, 3 * (size_t)c->n, PMQ_FX_GRAD)))
+            return rc;
+    }
+    return PMQ_OK;
+}
+
+} // namespace pmq

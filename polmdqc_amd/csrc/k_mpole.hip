@@ -7,9 +7,10 @@
 //                  charge-flux monopole shift (alterchg.cpp:50-53), one
 //                  thread per atom, writes the sorted SoA multipoles
 //   k_mpole_tiles  the i<k loop of empole_a (empole.cpp:198-345) over 64x64
-//                  block tiles: every pair gets scale 1 here
-//   k_mpole_excl   adds (scale-1) x pair for the 1-2..1-5 pairs whose scale
-//                  is not 1 (empole.cpp:224-235: later classes override)
+//                  block tiles, every pair but the scaled ones at scale 1
+//   k_mpole_excl   the 1-2..1-5 (and polarization-group) pairs whole, at
+//                  their scale (empole.cpp:224-235: later classes override),
+//                  in FP64
 //   k_torque       torque.cpp:42-397
 //   k_cf_delta / k_cf_grad   alterchg.cpp:36-169 / dcflux.cpp:31-225
 #include "pair_mpole.h"
@@ -279,7 +280,8 @@ struct MpoleArgs {
     double off2, f, aewald;
     double *dem, *te, *pot, *aem;
     long long *demq, *teq;   // mixed precision: 64-bit fixed-point gradient and torque (else null)
-    const unsigned long long *mmask; // per tile: 64 row masks of the in-cutoff pairs
+    const unsigned long long *xmask; // per tile: [64,128) row masks of the in-cutoff pairs with an active atom,
+                                     // scaled pairs left out
     void *part;              // per tile partial sums (2 sides x PMQ_TILE_NV x 64 of R)
     double *tscal;           // per tile scalars (PMQ_TILE_NS)
     Totals *tot;
@@ -296,7 +298,7 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
     constexpr bool CP = PEN != 0;
     __shared__ double jx[PMQ_BLOCK], jy[PMQ_BLOCK], jz[PMQ_BLOCK];
     __shared__ R jm[10][PMQ_BLOCK];
-    __shared__ R jc[2][PMQ_BLOCK];
+    __shared__ R jc[3][PMQ_BLOCK]; // core, valence exponent, total charge
     __shared__ int jflag[PMQ_BLOCK], jmol[PMQ_BLOCK];
     __shared__ R acc[8][PMQ_BLOCK]; // f(3), t(3), pot, aem
     const int lane = threadIdx.x;
@@ -309,7 +311,7 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
         __syncthreads();
         jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
         for (int c = 0; c < 10; c++) jm[c][lane] = a.smp[c * np + sj];
-        if (CP) { jc[0][lane] = a.scp[sj]; jc[1][lane] = a.scp[np + sj]; }
+        if (CP) { jc[0][lane] = a.scp[sj]; jc[1][lane] = a.scp[np + sj]; jc[2][lane] = a.scp[2 * np + sj]; }
         jflag[lane] = a.sflag[sj];
         if (ANA) jmol[lane] = a.smol[sj];
         for (int c = 0; c < 8; c++) acc[c][lane] = R(0);
@@ -321,10 +323,11 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
         mi.qxx = a.smp[4 * np + si]; mi.qxy = a.smp[5 * np + si]; mi.qxz = a.smp[6 * np + si];
         mi.qyy = a.smp[7 * np + si]; mi.qyz = a.smp[8 * np + si]; mi.qzz = a.smp[9 * np + si];
         const R corei = CP ? a.scp[si] : R(0), alphai = CP ? a.scp[np + si] : R(0);
+        const R qi = CP ? a.scp[2 * np + si] : R(0);
         __syncthreads();
 
         // the j of the tile this lane's i interacts with
-        unsigned long long mask = a.mmask[(size_t)t * 64 + lane];
+        unsigned long long mask = a.xmask[(size_t)t * 128 + 64 + lane];
 
         double e = 0; // energy sums stay FP64 whatever R is
         R fxi = 0, fyi = 0, fzi = 0, txi = 0, tyi = 0, tzi = 0, poti = 0, aemi = 0, ein = 0;
@@ -345,8 +348,8 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
             R bn[6];
             if (EW) ewald_series<R>(r2, R(a.aewald), bn);
             if (CP) {
-                pair_cp<R, GRAD, (PEN == 0 ? 1 : PEN), EW>(r2, xr, yr, zr, R(a.f), corei, alphai, mi, jc[0][jj],
-                                                           jc[1][jj], mk, o, bn);
+                pair_cp<R, GRAD, (PEN == 0 ? 1 : PEN), EW>(r2, xr, yr, zr, R(a.f), corei, qi, alphai, mi, jc[0][jj],
+                                                           jc[2][jj], jc[1][jj], mk, o, bn);
             } else {
                 R Rs[6];
                 if (EW) {
@@ -418,12 +421,12 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
     }
 }
 
-// excluded / scaled pairs: add (scale - 1) times the pair (the tile kernel
-// counted it at scale 1); a pair whose scale is 0 is not counted in nem
+// scaled pairs, whole (the tiles leave them out), counted in nem when their
+// energy is not zero.  The Ewald real-space pair at scale m is
+// f (B_n - (1 - m dmp) rr_n) = f (B_n - rr_n) [total multipoles] + m x the
+// plain (or charge-penetration damped) pair
 template <int PEN, bool CF, bool ANA, bool GRAD, bool VIR>
 __global__ __launch_bounds__(64) void k_mpole_excl(MpoleArgs a)
-// with or without Ewald the correction is (scale - 1) times the undamped
-// (or Gordon-damped) pair: B_n - (1-m) rr_n = B_n + (m-1) rr_n
 {
     using R = double;
     constexpr bool CP = PEN != 0;
@@ -431,10 +434,10 @@ __global__ __launch_bounds__(64) void k_mpole_excl(MpoleArgs a)
     if (t >= a.nexcl) return;
     const ExclPair q = a.excl[t];
     const R sc = R(q.m);
-    if (sc == R(1)) return;
     const int si = a.iperm[q.i], sk = a.iperm[q.k];
     const int fi = a.sflag[si], fk = a.sflag[sk];
     if (!((fi | fk) & 2)) return;
+    if (sc == R(0) && !(a.aewald > 0.0)) return;
     R xr = a.sx[sk] - a.sx[si], yr = a.sy[sk] - a.sy[si], zr = a.sz[sk] - a.sz[si];
     image<R>(a.box, xr, yr, zr);
     const R r2 = xr * xr + yr * yr + zr * zr;
@@ -448,19 +451,29 @@ __global__ __launch_bounds__(64) void k_mpole_excl(MpoleArgs a)
     mk.c = m[sk]; mk.dx = m[np + sk]; mk.dy = m[2 * np + sk]; mk.dz = m[3 * np + sk];
     mk.qxx = m[4 * np + sk]; mk.qxy = m[5 * np + sk]; mk.qxz = m[6 * np + sk];
     mk.qyy = m[7 * np + sk]; mk.qyz = m[8 * np + sk]; mk.qzz = m[9 * np + sk];
-    const R sf = (sc - R(1)) * R(a.f);
+    const R sf = sc * R(a.f);
     PairOut<R> o;
     if (CP) {
-        pair_cp<R, GRAD, (PEN == 0 ? 1 : PEN)>(r2, xr, yr, zr, sf, a.scp[si], a.scp[np + si], mi, a.scp[sk],
-                                               a.scp[np + sk], mk, o);
+        pair_cp<R, GRAD, (PEN == 0 ? 1 : PEN)>(r2, xr, yr, zr, sf, a.scp[si], a.scp[2 * np + si], a.scp[np + si], mi,
+                                               a.scp[sk], a.scp[2 * np + sk], a.scp[np + sk], mk, o);
     } else {
         R Rs[6];
         rr_series<R>(r2, sf, Rs);
         pair_series<R, GRAD>(Rs, mi, mk, xr, yr, zr, o);
     }
+    if (a.aewald > 0.0) {
+        // f (B_n - rr_n) between the total multipoles
+        R bn[6], rr[6], Rs[6];
+        ewald_series<R>(r2, R(a.aewald), bn);
+        rr_series<R>(r2, R(1), rr);
+        for (int j = 0; j < 6; j++) Rs[j] = R(a.f) * (bn[j] - rr[j]);
+        Mp<R> ti = mi, tk = mk;
+        if (CP) { ti.c = a.scp[2 * np + si]; tk.c = a.scp[2 * np + sk]; }
+        pair_series<R, GRAD, true>(Rs, ti, tk, xr, yr, zr, o);
+    }
     atomicAdd(&a.tot->em, o.e);
     if (ANA) {
-        if (sc == R(0) && o.e != R(0)) atomicAdd(&a.tot->nem, (unsigned long long)-1ll);
+        if (o.e != R(0)) atomicAdd(&a.tot->nem, 1ull);
         atomicAdd(&a.aem[q.i], R(0.5) * o.e);
         atomicAdd(&a.aem[q.k], R(0.5) * o.e);
         if (a.smol[si] != a.smol[sk]) atomicAdd(&a.tot->einter, o.e);
@@ -557,7 +570,7 @@ int launch_mpole_direct(pmq_ctx *c, int mode, double off2, bool mixed)
     a.dem = c->dem.p; a.te = c->te.p; a.pot = c->pot.p; a.aem = c->aem.p;
     a.demq = mixed ? c->fx_grad.p : nullptr;
     a.teq = mixed ? c->fx_grad.p + 3 * (size_t)c->n : nullptr;
-    a.mmask = c->mmask.p;
+    a.xmask = c->xmask.p;
     a.part = c->tpart.p;
     a.tscal = c->tscal.p;
     a.tot = c->totals.p;

@@ -191,6 +191,7 @@ struct FillArgs {
     const int2 *tiles;
     const int *ntiles;
     const double *sx, *sy, *sz, *spol;
+    const double *scp; // sorted core charge, valence exponent (HIPPO damping)
     DevBox box;
     double aewald;
     const unsigned long long *mask;
@@ -215,7 +216,9 @@ __device__ __forceinline__ void store_T(TEnt<V> *ent, size_t e, int col, const T
     ent[e] = r;
 }
 
-template <bool EW, class T, class V>
+// CPD: HIPPO damping (overlap factors of the two valence exponents) instead
+// of Thole
+template <bool EW, class T, class V, bool CPD = false>
 __global__ __launch_bounds__(64) void k_pair_fill(FillArgs<V> a)
 {
     const int lane = threadIdx.x;
@@ -232,13 +235,13 @@ __global__ __launch_bounds__(64) void k_pair_fill(FillArgs<V> a)
         const unsigned long long rm = a.mask[(size_t)t * 128 + lane], cm = a.mask[(size_t)t * 128 + 64 + lane];
         __syncthreads();
         jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
-        jth[lane] = a.spol[np + sj];
-        jpd[lane] = a.spol[2 * np + sj];
+        jth[lane] = CPD ? 0.0 : a.spol[np + sj];
+        jpd[lane] = CPD ? a.scp[np + sj] : a.spol[2 * np + sj];
         // lane as a j atom: its column-side mask and entry base
         jmask[lane] = diag ? (rm | cm) : cm;
         jbase[lane] = a.rowptr[sj] + a.soff[(size_t)t * 128 + 64 + lane];
         const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
-        const T thi = a.spol[np + si], pdi = a.spol[2 * np + si];
+        const T thi = CPD ? 0.0 : a.spol[np + si], pdi = CPD ? a.scp[np + si] : a.spol[2 * np + si];
         const unsigned long long mi = diag ? (rm | cm) : rm;
         const int ibase = a.rowptr[si] + a.soff[(size_t)t * 128 + lane];
         __syncthreads();
@@ -251,7 +254,8 @@ __global__ __launch_bounds__(64) void k_pair_fill(FillArgs<V> a)
             const T xr = T(xd), yr = T(yd), zr = T(zd);
             const T r2 = xr * xr + yr * yr + zr * zr;
             T lam[6], R[6];
-            thole_lambda<T>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
+            if (CPD) overlap_damp<T>(sqrt(r2), pdi, jpd[jj], lam);
+            else thole_lambda<T>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
             pol_series<EW, T>(r2, T(a.aewald), lam, R);
             const size_t ei = (size_t)ibase + __popcll(mi & ((1ull << jj) - 1ull));
             const size_t ej = (size_t)jbase[jj] + __popcll(jmask[jj] & ((1ull << lane) - 1ull));
@@ -267,6 +271,8 @@ struct FoldArgs {
     int nexcl;
     const int *iperm;
     const double *sx, *sy, *sz, *spol;
+    const double *scp;
+    int cpd;
     DevBox box;
     int npad;
     const int *rowptr;
@@ -308,7 +314,8 @@ __global__ __launch_bounds__(64) void k_excl_fold(FoldArgs<V> a)
     image<double>(a.box, xr, yr, zr);
     const double r2 = xr * xr + yr * yr + zr * zr;
     double lam[6], R[6];
-    thole_lambda<double>(sqrt(r2), a.spol[2 * np + si], a.spol[2 * np + sk], a.spol[np + si], a.spol[np + sk], lam);
+    if (a.cpd) overlap_damp<double>(sqrt(r2), a.scp[np + si], a.scp[np + sk], lam);
+    else thole_lambda<double>(sqrt(r2), a.spol[2 * np + si], a.spol[2 * np + sk], a.spol[np + si], a.spol[np + sk], lam);
     corr_series<double>(r2, q.u, lam, R);
     add_T(a.ent, (size_t)ei, R, xr, yr, zr);
     if (ek >= 0) add_T(a.ent, (size_t)ek, R, xr, yr, zr);
@@ -373,22 +380,29 @@ static int pair_fill(pmq_ctx *c, int tgrid)
     FillArgs<V> fa;
     fa.tiles = c->tiles.p; fa.ntiles = c->ntiles_eff.p;
     fa.sx = c->sx.p; fa.sy = c->sy.p; fa.sz = c->sz.p; fa.spol = c->spol.p;
+    fa.scp = c->scp.p;
     fa.box = c->box;
     fa.aewald = c->prm.use_ewald ? c->prm.aewald : 0.0;
     fa.mask = c->tmask.p; fa.soff = c->soff.p; fa.rowptr = c->rowptr.p;
     fa.npad = np; fa.cap = c->tcap; fa.ent = (TEnt<V> *)c->tval.p;
     {
         ProfScope ps_(c, "pol_pair_fill");
-        if (fa.aewald > 0)
-            hipLaunchKernelGGL((k_pair_fill<true, T, V>), dim3(tgrid), dim3(64), 0, st, fa);
-        else
-            hipLaunchKernelGGL((k_pair_fill<false, T, V>), dim3(tgrid), dim3(64), 0, st, fa);
+        const bool cpd = c->prm.pentype != PMQ_PEN_NONE;
+        if (fa.aewald > 0) {
+            if (cpd) hipLaunchKernelGGL((k_pair_fill<true, T, V, true>), dim3(tgrid), dim3(64), 0, st, fa);
+            else hipLaunchKernelGGL((k_pair_fill<true, T, V, false>), dim3(tgrid), dim3(64), 0, st, fa);
+        } else {
+            if (cpd) hipLaunchKernelGGL((k_pair_fill<false, T, V, true>), dim3(tgrid), dim3(64), 0, st, fa);
+            else hipLaunchKernelGGL((k_pair_fill<false, T, V, false>), dim3(tgrid), dim3(64), 0, st, fa);
+        }
     }
     const int nexcl = (int)c->h_excl.size();
     if (nexcl) {
         FoldArgs<V> fo;
         fo.excl = c->excl.p; fo.nexcl = nexcl; fo.iperm = c->iperm.p;
         fo.sx = c->sx.p; fo.sy = c->sy.p; fo.sz = c->sz.p; fo.spol = c->spol.p;
+        fo.scp = c->scp.p;
+        fo.cpd = c->prm.pentype != PMQ_PEN_NONE ? 1 : 0;
         fo.box = c->box; fo.npad = np; fo.rowptr = c->rowptr.p; fo.cap = c->tcap;
         fo.ent = (TEnt<V> *)c->tval.p;
         hipLaunchKernelGGL((k_excl_fold<V>), dim3((nexcl + 63) / 64), dim3(64), 0, st, fo);
@@ -402,6 +416,52 @@ int status_reset(pmq_ctx *c)
     hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(1), 0, c->stream, c->status.p);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
+}
+
+// Rows of the tile masks without the scaled pairs (the 1-2..1-5 and
+// polarization-group pairs): [0,64) the in-cutoff rows, [64,128) the rows
+// with an active atom.  The pair kernels skip those pairs and the exclusion
+// kernels evaluate them whole, at their scales, in FP64: large intramolecular
+// pair terms are then never formed in one precision and cancelled in another.
+__global__ void k_xmask_copy(const int *ntiles, const unsigned long long *__restrict__ mask,
+                             const unsigned long long *__restrict__ mmask, unsigned long long *__restrict__ xmask)
+{
+    const size_t tot = (size_t)*ntiles * 64;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t t = e >> 6, l = e & 63;
+        xmask[t * 128 + l] = mask[t * 128 + l];
+        xmask[t * 128 + 64 + l] = mmask[t * 64 + l];
+    }
+}
+
+__global__ void k_xmask_clear(int nexcl, const ExclPair *__restrict__ excl, const int *__restrict__ iperm,
+                              const int2 *__restrict__ tiles, const int *__restrict__ adj,
+                              const int *__restrict__ adjn, int nblk, const int *ntiles,
+                              unsigned long long *__restrict__ xmask)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nexcl) return;
+    const int s1 = iperm[excl[q].i], s2 = iperm[excl[q].k];
+    const int a = min(s1, s2), b = max(s1, s2);
+    const int I = a >> 6, J = b >> 6;
+    // the tile (I, J) in I's list, which is ordered by the other block
+    const int *lst = adj + (size_t)I * nblk;
+    int lo = 0, hi = adjn[I];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int2 IJ = tiles[lst[mid]];
+        const int other = IJ.x == I ? IJ.y : IJ.x;
+        if (other < J) lo = mid + 1; else hi = mid;
+    }
+    if (lo >= adjn[I]) return;
+    const int t = lst[lo];
+    const int2 IJ = tiles[t];
+    if ((IJ.x == I ? IJ.y : IJ.x) != J || t >= *ntiles) return; // beyond the cutoff: no tile holds it
+    // row: the atom of block IJ.x (the smaller one on a diagonal tile)
+    const int row = IJ.x == I ? (a & 63) : (b & 63), bit = IJ.x == I ? (b & 63) : (a & 63);
+    const unsigned long long clr = ~(1ull << bit);
+    atomicAnd(&xmask[(size_t)t * 128 + row], clr);
+    atomicAnd(&xmask[(size_t)t * 128 + 64 + row], clr);
 }
 
 int tile_structure(pmq_ctx *c, double off2, bool safe)
@@ -419,6 +479,7 @@ int tile_structure(pmq_ctx *c, double off2, bool safe)
     const size_t nt = (size_t)c->tile_cap;
     PMQ_HIP(c->tmask.alloc(128 * nt));
     PMQ_HIP(c->mmask.alloc(64 * nt));
+    PMQ_HIP(c->xmask.alloc(128 * nt));
     PMQ_HIP(c->soff.alloc(128 * nt));
     PMQ_HIP(c->adj.alloc((size_t)nblk * nblk));
     PMQ_HIP(c->adjn.alloc(nblk));
@@ -435,6 +496,12 @@ int tile_structure(pmq_ctx *c, double off2, bool safe)
     hipLaunchKernelGGL(k_tile_masks, dim3(c->tgrid), dim3(64), 0, st, ma);
     hipLaunchKernelGGL(k_block_adj, dim3(nblk), dim3(64), 0, st, nblk, c->tile_cap, c->tileflag.p, c->tilescan.p,
                        c->adj.p, c->adjn.p);
+    hipLaunchKernelGGL(k_xmask_copy, dim3((unsigned)std::min<size_t>((nt * 64 + 255) / 256, 8192)), dim3(256), 0, st,
+                       c->ntiles_eff.p, c->tmask.p, c->mmask.p, c->xmask.p);
+    const int nexcl = (int)c->h_excl.size();
+    if (nexcl)
+        hipLaunchKernelGGL(k_xmask_clear, dim3((nexcl + 255) / 256), dim3(256), 0, st, nexcl, c->excl.p, c->iperm.p,
+                           c->tiles.p, c->adj.p, c->adjn.p, nblk, c->ntiles_eff.p, c->xmask.p);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }

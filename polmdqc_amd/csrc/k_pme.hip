@@ -65,11 +65,25 @@ void pme_release(pmq_ctx *c) { destroy_plans(c); }
 // max(4, order - 1) points wide
 static int brick_count(int K, int order) { return std::max(1, K / std::max(4, order - 1)); }
 
+// psi(k) (2 pi^1.5 a^3 / 3)^-1: the Fourier transform of the long-range
+// part (1 - g(a r)) r^-6 of dispersion, g(x) = (1 + x^2 + x^4/2) e^{-x^2},
+// b = k / (2a)
+static double disp_psi_unit(double b)
+{
+    return 1.77245385090551602730 * b * b * b * erfc(b) + (0.5 - b * b) * exp(-b * b);
+}
+
 int pme_setup(pmq_ctx *c)
 {
     const pmq_params &p = c->prm;
-    const int K[3] = {p.nfft1, p.nfft2, p.nfft3};
-    if (K[0] < 8 || K[1] < 8 || K[2] < 8 || p.bsorder < 4 || p.bsorder > PMQ_MAX_ORDER) {
+    const bool ew = p.use_ewald, dew = c->have_disp && p.use_dewald;
+    if (ew && dew && (p.dfft1 != p.nfft1 || p.dfft2 != p.nfft2 || p.dfft3 != p.nfft3 || p.dbsorder != p.bsorder)) {
+        c->err = "PME: dispersion and multipole PME must share the grid and B-spline order";
+        return PMQ_ERR_UNSUPPORTED;
+    }
+    const int K[3] = {ew ? p.nfft1 : p.dfft1, ew ? p.nfft2 : p.dfft2, ew ? p.nfft3 : p.dfft3};
+    const int order = ew ? p.bsorder : p.dbsorder;
+    if (K[0] < 8 || K[1] < 8 || K[2] < 8 || order < 4 || order > PMQ_MAX_ORDER) {
         c->err = "PME: grid sizes must be >= 8 and the B-spline order 4..6";
         return PMQ_ERR_ARG;
     }
@@ -77,8 +91,9 @@ int pme_setup(pmq_ctx *c)
         c->err = "PME: Ewald summation needs an orthogonal, monoclinic or triclinic box";
         return PMQ_ERR_ARG;
     }
-    const bool same_grid = K[0] == c->K[0] && K[1] == c->K[1] && K[2] == c->K[2] && c->bsorder == p.bsorder;
-    bool same_box = same_grid && c->pme_alpha == p.aewald && c->pme_dbox.kind == c->box.kind;
+    const double alpha = ew ? p.aewald : 0.0, dalpha = dew ? p.dewald : 0.0;
+    const bool same_grid = K[0] == c->K[0] && K[1] == c->K[1] && K[2] == c->K[2] && c->bsorder == order;
+    bool same_box = same_grid && c->pme_alpha == alpha && c->pme_dalpha == dalpha && c->pme_dbox.kind == c->box.kind;
     for (int a = 0; a < 3 && same_box; a++)
         for (int b = 0; b < 3; b++) same_box = same_box && c->pme_dbox.recip[a][b] == c->box.recip[a][b];
     PMQ_HIP(c->pme_frac.alloc(40 * (size_t)c->npad)); // follows the atom count
@@ -86,8 +101,8 @@ int pme_setup(pmq_ctx *c)
     PMQ_HIP(c->pme_keys_sorted.alloc(c->n));
     if (same_box && c->plan_f1) return PMQ_OK;
     c->K[0] = K[0]; c->K[1] = K[1]; c->K[2] = K[2];
-    c->bsorder = p.bsorder;
-    for (int a = 0; a < 3; a++) c->pme_nb[a] = brick_count(K[a], p.bsorder);
+    c->bsorder = order;
+    for (int a = 0; a < 3; a++) c->pme_nb[a] = brick_count(K[a], order);
     PMQ_HIP(c->pme_bstart.alloc((size_t)c->pme_nb[0] * c->pme_nb[1] * c->pme_nb[2] + 1));
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) c->pmeA[a][b] = K[a] * c->box.recip[a][b];
@@ -97,8 +112,8 @@ int pme_setup(pmq_ctx *c)
         bsmod[a].assign(K[a], 0.0);
         for (int m = 0; m < K[a]; m++) {
             double re = 0, im = 0;
-            for (int k = 0; k < p.bsorder - 1; k++) {
-                const double w = bspline_m(p.bsorder, k + 1.0);
+            for (int k = 0; k < order - 1; k++) {
+                const double w = bspline_m(order, k + 1.0);
                 const double arg = 2.0 * M_PI * m * k / K[a];
                 re += w * cos(arg);
                 im += w * sin(arg);
@@ -113,24 +128,33 @@ int pme_setup(pmq_ctx *c)
                      + A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
     const double V = (double)K[0] * K[1] * K[2] / det;
     const int H0 = K[0] / 2 + 1;
-    std::vector<double> G((size_t)H0 * K[1] * K[2]);
-    const double a2 = p.aewald * p.aewald;
+    // electrostatics: G = exp(-pi^2 h^2/a^2) / (pi V h^2 bsmod), m = 0 dropped;
+    // dispersion: Gd = -psi(2 pi |h|) / (V bsmod), m = 0 kept
+    std::vector<double> G((size_t)H0 * K[1] * K[2]), Gd(dew ? G.size() : 0);
+    const double a2 = alpha * alpha;
+    const double dpre = 2.0 * pow(M_PI, 1.5) * dalpha * dalpha * dalpha / 3.0;
     for (int m3 = 0; m3 < K[2]; m3++)
         for (int m2 = 0; m2 < K[1]; m2++)
             for (int m1 = 0; m1 < H0; m1++) {
                 const size_t id = ((size_t)m3 * K[1] + m2) * H0 + m1;
-                if (m1 == 0 && m2 == 0 && m3 == 0) { G[id] = 0.0; continue; }
                 const int f1 = m1 > K[0] / 2 ? m1 - K[0] : m1;
                 const int f2 = m2 > K[1] / 2 ? m2 - K[1] : m2;
                 const int f3 = m3 > K[2] / 2 ? m3 - K[2] : m3;
                 double h[3];
                 for (int j = 0; j < 3; j++) h[j] = f1 * A[0][j] / K[0] + f2 * A[1][j] / K[1] + f3 * A[2][j] / K[2];
                 const double h2 = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
-                G[id] = exp(-M_PI * M_PI * h2 / a2) / (M_PI * V * h2 * bsmod[0][m1] * bsmod[1][m2] * bsmod[2][m3]);
+                const double bs = bsmod[0][m1] * bsmod[1][m2] * bsmod[2][m3];
+                if (dew) Gd[id] = -dpre * disp_psi_unit(M_PI * sqrt(h2) / dalpha) / (V * bs);
+                if (m1 == 0 && m2 == 0 && m3 == 0) { G[id] = 0.0; continue; }
+                G[id] = ew ? exp(-M_PI * M_PI * h2 / a2) / (M_PI * V * h2 * bs) : 0.0;
             }
     PMQ_HIP(c->pme_g.alloc(G.size()));
     PMQ_HIP(hipMemcpyAsync(c->pme_g.p, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    PMQ_HIP(hipStreamSynchronize(c->stream)); // G is a host temporary
+    if (dew) {
+        PMQ_HIP(c->pme_gd.alloc(Gd.size()));
+        PMQ_HIP(hipMemcpyAsync(c->pme_gd.p, Gd.data(), Gd.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    PMQ_HIP(hipStreamSynchronize(c->stream)); // G, Gd are host temporaries
     const size_t nreal = (size_t)K[0] * K[1] * K[2], ncplx = (size_t)H0 * K[1] * K[2];
     PMQ_HIP(c->qgrid.alloc(2 * nreal));
     PMQ_HIP(c->cgrid.alloc(2 * 2 * ncplx));
@@ -166,7 +190,8 @@ int pme_setup(pmq_ctx *c)
         if (wmax && (rc = fft_check(c, rocfft_execution_info_set_work_buffer(info, c->fft_work.p, wmax), "work"))) return rc;
     }
     c->pme_dbox = c->box;
-    c->pme_alpha = p.aewald;
+    c->pme_alpha = alpha;
+    c->pme_dalpha = dalpha;
     return PMQ_OK;
 }
 
@@ -265,7 +290,8 @@ struct BrickArgs {
     const int *bstart;              // nbricks + 1
     int npad;
     const double *sx, *sy, *sz;
-    const double *src;    // sorted SoA sources: 10 multipole comps (mode 0) or 3 dipole comps x nset (mode 1)
+    const double *src;    // sorted SoA sources: 10 multipole comps (mode 0), 3 dipole comps x nset (mode 1)
+                          // or one scalar per atom (mode 2: dispersion C6 coefficients)
     const double *charge; // mode 0: total monopole per sorted atom (charge penetration), or null
     double *grid;         // nset grids
     size_t gstride;
@@ -320,9 +346,9 @@ constexpr int SPREAD_CH = 64;
 template <int ORD, int MODE>
 __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
 {
-    constexpr int ND = MODE == 0 ? 2 : 1;
-    constexpr int NC = MODE == 0 ? 10 : 6;
-    constexpr int NS = MODE == 0 ? 1 : 2;
+    constexpr int ND = MODE == 0 ? 2 : MODE == 1 ? 1 : 0;
+    constexpr int NC = MODE == 0 ? 10 : MODE == 1 ? 6 : 1;
+    constexpr int NS = MODE == 1 ? 2 : 1;
     __shared__ double th[3][ORD][ND + 1][SPREAD_CH];
     __shared__ double cf[NC][SPREAD_CH];
     __shared__ int si0[3][SPREAD_CH];
@@ -335,7 +361,7 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
     const int s0 = brick_lo(bx + 1, K0, a.nb[0]) - lo0, s1 = brick_lo(by + 1, K1, a.nb[1]) - lo1,
               s2 = brick_lo(bz + 1, K2, a.nb[2]) - lo2;
     const int bsz = s0 * s1 * s2;
-    const int nset = MODE == 0 ? 1 : a.nset; // 1 or 2
+    const int nset = MODE == 1 ? a.nset : 1; // 1 or 2
     const unsigned long long smask = (1ull << a.sbits) - 1ull;
     // the atoms homed in bricks b + (0|1, 0|1, 0|1), each brick once
     int hs[8], hc[8], ntot = 0;
@@ -393,6 +419,8 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
                     frac_mpole(a.g, m, fm);
 #pragma unroll
                     for (int c = 0; c < 10; c++) cf[c][tid] = fm[c];
+                } else if (MODE == 2) {
+                    cf[0][tid] = a.src[s];
                 } else {
                     for (int set = 0; set < nset; set++) {
                         const double *d = a.src + (size_t)set * 3 * a.npad;
@@ -419,6 +447,8 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
                             + cf[3][j] * x0 * y0 * z1 + cf[4][j] * x2 * y0 * z0 + cf[5][j] * x0 * y2 * z0
                             + cf[6][j] * x0 * y0 * z2 + cf[7][j] * x1 * y1 * z0 + cf[8][j] * x1 * y0 * z1
                             + cf[9][j] * x0 * y1 * z1;
+                } else if (MODE == 2) {
+                    acc[0] += cf[0][j] * th[0][kx][0][j] * th[1][ky][0][j] * th[2][kz][0][j];
                 } else {
                     const double x0 = th[0][kx][0][j], x1 = th[0][kx][1][j];
                     const double y0 = th[1][ky][0][j], y1 = th[1][ky][1][j];
@@ -795,7 +825,7 @@ static int fft(pmq_ctx *c, bool forward, int nset)
     return fft_check(c, rocfft_execute(pl, in, out, (rocfft_execution_info)c->fft_info), "execute");
 }
 
-static int pme_potential(pmq_ctx *c, int nset)
+static int pme_potential(pmq_ctx *c, int nset, const double *G = nullptr)
 {
     int rc;
     if ((rc = fft(c, true, nset))) return rc;
@@ -803,7 +833,7 @@ static int pme_potential(pmq_ctx *c, int nset)
     const int ncplx = H0 * c->K[1] * c->K[2];
     const size_t tot = (size_t)ncplx * nset;
     hipLaunchKernelGGL(k_pme_convolve, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, ncplx, nset,
-                       c->pme_g.p, c->cgrid.p);
+                       G ? G : c->pme_g.p, c->cgrid.p);
     PMQ_HIP(hipGetLastError());
     return fft(c, false, nset);
 }
@@ -871,13 +901,16 @@ static int spread(pmq_ctx *c, int mode, const double *src, const double *charge,
     const dim3 grid((unsigned)(c->pme_nb[0] * c->pme_nb[1] * c->pme_nb[2]));
     ProfScope ps_(c, "pme_spread");
 #define PMQ_SPREAD(ORD, MODE) hipLaunchKernelGGL((k_pme_spread_brick<ORD, MODE>), grid, dim3(256), 0, c->stream, args)
-    switch (c->bsorder * 2 + (mode ? 1 : 0)) {
-    case 8: PMQ_SPREAD(4, 0); break;
-    case 9: PMQ_SPREAD(4, 1); break;
-    case 10: PMQ_SPREAD(5, 0); break;
-    case 11: PMQ_SPREAD(5, 1); break;
-    case 12: PMQ_SPREAD(6, 0); break;
-    default: PMQ_SPREAD(6, 1); break;
+    switch (c->bsorder * 3 + mode) {
+    case 12: PMQ_SPREAD(4, 0); break;
+    case 13: PMQ_SPREAD(4, 1); break;
+    case 14: PMQ_SPREAD(4, 2); break;
+    case 15: PMQ_SPREAD(5, 0); break;
+    case 16: PMQ_SPREAD(5, 1); break;
+    case 17: PMQ_SPREAD(5, 2); break;
+    case 18: PMQ_SPREAD(6, 0); break;
+    case 19: PMQ_SPREAD(6, 1); break;
+    default: PMQ_SPREAD(6, 2); break;
     }
 #undef PMQ_SPREAD
     PMQ_HIP(hipGetLastError());
@@ -945,6 +978,69 @@ int pme_dipole_field(pmq_ctx *c, const double *dip, double *field)
     {
         ProfScope ps_(c, "pme_gather");
         PMQ_ORDER_DISPATCH(k_pme_gather_field, )
+    }
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+// dispersion: energy (1/2) C phi + a^6/12 C^2 and gradient C grad(phi) of
+// every atom (one lane group per atom); the gradient is added in caller
+// order (each atom written once), the energy summed per wave into edsp
+struct DispPostArgs {
+    int n, npad, stride;
+    PmeGeom g;
+    const double *sx, *sy, *sz, *cs; // sorted positions, C6 coefficients
+    const int *perm;
+    const double *grid;
+    double self;                     // a^6 / 12
+    int grad;
+    double *de;                      // caller order (+=)
+    Totals *tot;
+};
+
+template <int ORD>
+__global__ __launch_bounds__(256) void k_pme_disp_gather(DispPostArgs a)
+{
+    GATHER_SETUP(1)
+    double fp[4];
+    gather_part<ORD, 4>(a.g, a.grid, tx, ty, tz, i0, ky, kz, on, fp);
+    group_reduce<G, 4>(fp);
+    double e = 0.0;
+    if (valid && l == 0) {
+        double phi, gr[3];
+        frac_to_cart<4>(a.g, fp, &phi, gr, nullptr, nullptr);
+        const double C = a.cs[s];
+        e = 0.5 * C * phi + a.self * C * C;
+        if (a.grad) {
+            const int i = a.perm[s];
+            for (int q = 0; q < 3; q++) a.de[3 * i + q] += C * gr[q];
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&a.tot->edsp, e);
+}
+
+// need_bin: the atoms are not yet binned by brick in this evaluation (no
+// multipole PME ran before)
+int pme_disp(pmq_ctx *c, bool grad, bool need_bin)
+{
+    int rc;
+    if (need_bin && (rc = pme_bin(c))) return rc;
+    if ((rc = spread(c, 2, c->shp.p, nullptr, 1))) return rc;
+    if ((rc = pme_potential(c, 1, c->pme_gd.p))) return rc;
+    DispPostArgs args;
+    args.n = c->n; args.npad = c->npad; args.stride = atom_stride(c->n); args.g = geom(c);
+    args.sx = c->sx.p; args.sy = c->sy.p; args.sz = c->sz.p; args.cs = c->shp.p;
+    args.perm = c->perm.p;
+    args.grid = c->qgrid.p;
+    const double a2 = c->prm.dewald * c->prm.dewald;
+    args.self = a2 * a2 * a2 / 12.0;
+    args.grad = grad ? 1 : 0;
+    args.de = c->dedsp.p;
+    args.tot = c->totals.p;
+    {
+        ProfScope ps_(c, "pme_gather");
+        PMQ_ORDER_DISPATCH(k_pme_disp_gather, )
     }
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;

@@ -1,4 +1,5 @@
-// k_polar.hip -- mutual induced-dipole polarization (AMOEBA, Thole damping).
+// k_polar.hip -- mutual induced-dipole polarization (AMOEBA with Thole
+// damping; HIPPO with charge-penetration damping).
 //
 // The reference declares the state of this term but computes nothing
 // (epolar is commented out at energy.cpp:267; induce/PCG defaults at
@@ -15,6 +16,12 @@
 // derivative at fixed dipoles of
 //   (1/2)[U_p(mu_d; M) + U_d(mu_p; M) + U_u(mu_d; mu_p) + U_u(mu_p; mu_d)]
 // so no second solve is needed.
+//
+// HIPPO (charge penetration on; kpolar.cpp:577 turns Thole off then): the
+// induced dipoles are points; the permanent field at i is that of the core
+// charge of k (undamped) plus its valence multipole damped by the one-site
+// factors of k's exponent; T is damped by the overlap factors of the two
+// valence densities and scaled by w (induce-1x) instead of u.
 //
 // Kernels (sorted atom order, 2 dipole sets: 0 = d, 1 = p):
 //   k_pol_field_tiles / _excl   permanent field (scale 1 in the tiles,
@@ -38,6 +45,8 @@ struct PolArgs {
     const int2 *tiles;
     const int *ntiles;
     const double *sx, *sy, *sz, *smp, *spol;
+    const double *scp;   // sorted core charge, valence exponent (charge penetration)
+    int cpd;             // HIPPO damping (charge penetration) instead of Thole
     const int *sflag, *perm, *iperm;
     DevBox box;
     double off2, aewald, f;
@@ -78,6 +87,54 @@ __device__ __forceinline__ void mp_fields(const T *R, const Mp<T> &a, const Mp<T
     Ek[2] = sk * zr - R[1] * a.dz - T(2) * R[2] * qiz;
 }
 
+// one side of mp_fields: the field at i of multipole b at k, and at k of a at i
+template <class T>
+__device__ __forceinline__ void field_at_i(const T *R, const Mp<T> &b, T xr, T yr, T zr, T *Ei)
+{
+    const T qkx = b.qxx * xr + b.qxy * yr + b.qxz * zr, qky = b.qxy * xr + b.qyy * yr + b.qyz * zr,
+            qkz = b.qxz * xr + b.qyz * yr + b.qzz * zr;
+    const T dkr = b.dx * xr + b.dy * yr + b.dz * zr, qkr = qkx * xr + qky * yr + qkz * zr;
+    const T si = b.c * R[1] - dkr * R[2] + qkr * R[3];
+    Ei[0] += -si * xr - R[1] * b.dx + T(2) * R[2] * qkx;
+    Ei[1] += -si * yr - R[1] * b.dy + T(2) * R[2] * qky;
+    Ei[2] += -si * zr - R[1] * b.dz + T(2) * R[2] * qkz;
+}
+
+template <class T>
+__device__ __forceinline__ void field_at_k(const T *R, const Mp<T> &a, T xr, T yr, T zr, T *Ek)
+{
+    const T qix = a.qxx * xr + a.qxy * yr + a.qxz * zr, qiy = a.qxy * xr + a.qyy * yr + a.qyz * zr,
+            qiz = a.qxz * xr + a.qyz * yr + a.qzz * zr;
+    const T dir = a.dx * xr + a.dy * yr + a.dz * zr, qir = qix * xr + qiy * yr + qiz * zr;
+    const T sk = a.c * R[1] + dir * R[2] + qir * R[3];
+    Ek[0] += sk * xr - R[1] * a.dx - T(2) * R[2] * qix;
+    Ek[1] += sk * yr - R[1] * a.dy - T(2) * R[2] * qiy;
+    Ek[2] += sk * zr - R[1] * a.dz - T(2) * R[2] * qiz;
+}
+
+// the HIPPO series of one pair at scale s: the permanent field of site k at
+// i is that of its core undamped plus its valence multipole damped by k's
+// one-site factors, summed as [total multipole, Rt] + [valence multipole,
+// Xk = -s (1 - dmpk) rr] so the core/valence cancellation never forms (Xi
+// the same for i); Rm the dipole-dipole series (overlap factors).  With
+// Ewald (ew) Rt = B - (1 - s) rr and Rm = B - (1 - s dmpik) rr, else s rr
+// and s dmpik rr.
+template <class T>
+__device__ __forceinline__ void hippo_pol_series(T r2, T ai, T ak, bool ew, T aewald, T s, T *Rt, T *Xi, T *Xk, T *Rm)
+{
+    T rr[6], bn[6], xi[5], xk[5], xik[6];
+    rr_series<T>(r2, T(1), rr);
+    if (ew) ewald_series<T>(r2, aewald, bn);
+    damp_gordon_comp<T, 1>(sqrt(r2), ai, ak, xi, xk, xik);
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        Rt[j] = ew ? bn[j] - (T(1) - s) * rr[j] : s * rr[j];
+        Xi[j] = j < 5 ? -s * xi[j] * rr[j] : T(0);
+        Xk[j] = j < 5 ? -s * xk[j] * rr[j] : T(0);
+        Rm[j] = ew ? bn[j] - (T(1) - s + s * xik[j]) * rr[j] : s * (T(1) - xik[j]) * rr[j];
+    }
+}
+
 template <class T = double>
 __device__ __forceinline__ Mp<T> load_mp(const double *smp, int np, int s)
 {
@@ -106,7 +163,7 @@ __device__ __forceinline__ Mp<T> dip(T x, T y, T z)
 
 // T: arithmetic of the pair terms (double, or float in the mixed form with
 // FP64 displacements)
-template <bool EW, class T>
+template <bool EW, class T, bool CPD = false>
 __global__ __launch_bounds__(64) void k_pol_field_tiles(PolArgs a)
 {
     TILE_PROLOGUE
@@ -119,14 +176,15 @@ __global__ __launch_bounds__(64) void k_pol_field_tiles(PolArgs a)
         __syncthreads();
         jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
         for (int c = 0; c < 10; c++) jm[c][lane] = a.smp[c * np + sj];
-        jth[lane] = a.spol[np + sj];
-        jpd[lane] = a.spol[2 * np + sj];
+        // Thole: thole and pdamp; HIPPO: total charge and valence exponent
+        jth[lane] = CPD ? a.scp[2 * np + sj] : a.spol[np + sj];
+        jpd[lane] = CPD ? a.scp[np + sj] : a.spol[2 * np + sj];
         jflag[lane] = a.sflag[sj];
         acc[0][lane] = acc[1][lane] = acc[2][lane] = T(0);
         const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
         const int fi = a.sflag[si];
         const Mp<T> mi = load_mp<T>(a.smp, np, si);
-        const T thi = a.spol[np + si], pdi = a.spol[2 * np + si];
+        const T thi = CPD ? a.scp[2 * np + si] : a.spol[np + si], pdi = CPD ? a.scp[np + si] : a.spol[2 * np + si];
         __syncthreads();
         unsigned long long mask = a.mask[(size_t)t * 128 + lane]; // in-cutoff j of this lane
         T ex = 0, ey = 0, ez = 0;
@@ -137,15 +195,27 @@ __global__ __launch_bounds__(64) void k_pol_field_tiles(PolArgs a)
             image<double>(a.box, xd, yd, zd);
             const T xr = T(xd), yr = T(yd), zr = T(zd);
             const T r2 = xr * xr + yr * yr + zr * zr;
-            T lam[6], R[6];
-            thole_lambda<T>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
-            pol_series<EW, T>(r2, T(a.aewald), lam, R);
             Mp<T> mk;
             mk.c = jm[0][jj]; mk.dx = jm[1][jj]; mk.dy = jm[2][jj]; mk.dz = jm[3][jj];
             mk.qxx = jm[4][jj]; mk.qxy = jm[5][jj]; mk.qxz = jm[6][jj]; mk.qyy = jm[7][jj]; mk.qyz = jm[8][jj];
             mk.qzz = jm[9][jj];
-            T Ei[3], Ek[3];
-            mp_fields<T>(R, mi, mk, xr, yr, zr, Ei, Ek);
+            T Ei[3] = {0, 0, 0}, Ek[3] = {0, 0, 0};
+            if (CPD) {
+                T Rt[6], Xi[6], Xk[6], Rm[6];
+                hippo_pol_series<T>(r2, pdi, jpd[jj], EW, T(a.aewald), T(1), Rt, Xi, Xk, Rm);
+                Mp<T> tk = mk, ti = mi;
+                tk.c = jth[jj];
+                ti.c = thi;
+                field_at_i<T>(Rt, tk, xr, yr, zr, Ei);
+                field_at_i<T>(Xk, mk, xr, yr, zr, Ei);
+                field_at_k<T>(Rt, ti, xr, yr, zr, Ek);
+                field_at_k<T>(Xi, mi, xr, yr, zr, Ek);
+            } else {
+                T lam[6], R[6];
+                thole_lambda<T>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
+                pol_series<EW, T>(r2, T(a.aewald), lam, R);
+                mp_fields<T>(R, mi, mk, xr, yr, zr, Ei, Ek);
+            }
             ex += Ei[0]; ey += Ei[1]; ez += Ei[2];
             atomicAdd(&acc[0][jj], Ek[0]);
             atomicAdd(&acc[1][jj], Ek[1]);
@@ -165,7 +235,6 @@ __global__ __launch_bounds__(64) void k_pol_field_excl(PolArgs a)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nexcl) return;
     const ExclPair q = a.excl[t];
-    if (q.d == 1.0 && q.p == 1.0) return;
     const int np = a.npad;
     const int si = a.iperm[q.i], sk = a.iperm[q.k];
     double xr = a.sx[sk] - a.sx[si], yr = a.sy[sk] - a.sy[si], zr = a.sz[sk] - a.sz[si];
@@ -173,13 +242,33 @@ __global__ __launch_bounds__(64) void k_pol_field_excl(PolArgs a)
     const double r2 = xr * xr + yr * yr + zr * zr;
     if (r2 > a.off2) return;
     double lam[6], R[6], Ei[3], Ek[3];
-    thole_lambda<double>(sqrt(r2), a.spol[2 * np + si], a.spol[2 * np + sk], a.spol[np + si], a.spol[np + sk], lam);
+    if (!a.cpd)
+        thole_lambda<double>(sqrt(r2), a.spol[2 * np + si], a.spol[2 * np + sk], a.spol[np + si], a.spol[np + sk],
+                             lam);
     const Mp<double> mi = load_mp<double>(a.smp, np, si), mk = load_mp<double>(a.smp, np, sk);
     const double sc[2] = {q.d, q.p};
+    const bool ew = a.aewald > 0.0;
+    double rr[6], bn[6];
+    rr_series<double>(r2, 1.0, rr);
+    if (ew) ewald_series<double>(r2, a.aewald, bn);
     for (int set = 0; set < 2; set++) {
-        if (sc[set] == 1.0) continue;
-        corr_series(r2, sc[set], lam, R);
-        mp_fields<double>(R, mi, mk, xr, yr, zr, Ei, Ek);
+        // the whole pair at its scale (the tiles leave the scaled pairs out)
+        if (sc[set] == 0.0 && !ew) continue;
+        if (a.cpd) {
+            double Rt[6], Xi[6], Xk[6], Rm[6];
+            hippo_pol_series<double>(r2, a.scp[np + si], a.scp[np + sk], ew, a.aewald, sc[set], Rt, Xi, Xk, Rm);
+            Mp<double> tk = mk, ti = mi;
+            tk.c = a.scp[2 * np + sk];
+            ti.c = a.scp[2 * np + si];
+            Ei[0] = Ei[1] = Ei[2] = Ek[0] = Ek[1] = Ek[2] = 0.0;
+            field_at_i<double>(Rt, tk, xr, yr, zr, Ei);
+            field_at_i<double>(Xk, mk, xr, yr, zr, Ei);
+            field_at_k<double>(Rt, ti, xr, yr, zr, Ek);
+            field_at_k<double>(Xi, mi, xr, yr, zr, Ek);
+        } else {
+            for (int j = 0; j < 6; j++) R[j] = ew ? bn[j] - (1.0 - sc[set] * lam[j]) * rr[j] : sc[set] * lam[j] * rr[j];
+            mp_fields<double>(R, mi, mk, xr, yr, zr, Ei, Ek);
+        }
         const size_t o = (size_t)set * 3 * np;
         for (int c = 0; c < 3; c++) {
             field_add(a.dst, a.dstq, o + c * np + si, Ei[c]);
@@ -230,7 +319,48 @@ __device__ __forceinline__ void pol_pair_grad(const T *R, const Mp<T> &mi, const
     }
 }
 
-template <bool EW, class T>
+// HIPPO form of pol_pair_grad: the induced dipole at i sees the total
+// multipole of k through Rt plus its valence multipole through Xk (and the
+// reverse through Xi); the dipole-dipole term uses Rm.  qi/qk: total charges
+// (mi/mk carry the valence charges)
+template <class T>
+__device__ __forceinline__ void pol_pair_grad_cp(const T *Rt, const T *Xi, const T *Xk, const T *Rm, T qi, T qk,
+                                                 const Mp<T> &mi, const Mp<T> &mk, const T *udi, const T *upi,
+                                                 const T *udk, const T *upk, T xr, T yr, T zr, T sdp, T sd, T sp,
+                                                 T su, T *F, T *Ti, T *Tk)
+{
+    PairOut<T> o;
+    Mp<T> ti = mi, tk = mk;
+    ti.c = qi;
+    tk.c = qk;
+    auto perm = [&](T w, T dxi, T dyi, T dzi, T dxk, T dyk, T dzk) {
+        if (w == T(0)) return;
+        const Mp<T> ui = dip<T>(dxi, dyi, dzi), uk = dip<T>(dxk, dyk, dzk);
+        pair_series<T, true>(Rt, ui, tk, xr, yr, zr, o);
+        F[0] += w * o.fx; F[1] += w * o.fy; F[2] += w * o.fz;
+        Tk[0] += w * o.tkx; Tk[1] += w * o.tky; Tk[2] += w * o.tkz;
+        pair_series<T, true>(Xk, ui, mk, xr, yr, zr, o);
+        F[0] += w * o.fx; F[1] += w * o.fy; F[2] += w * o.fz;
+        Tk[0] += w * o.tkx; Tk[1] += w * o.tky; Tk[2] += w * o.tkz;
+        pair_series<T, true>(Rt, ti, uk, xr, yr, zr, o);
+        F[0] += w * o.fx; F[1] += w * o.fy; F[2] += w * o.fz;
+        Ti[0] += w * o.tix; Ti[1] += w * o.tiy; Ti[2] += w * o.tiz;
+        pair_series<T, true>(Xi, mi, uk, xr, yr, zr, o);
+        F[0] += w * o.fx; F[1] += w * o.fy; F[2] += w * o.fz;
+        Ti[0] += w * o.tix; Ti[1] += w * o.tiy; Ti[2] += w * o.tiz;
+    };
+    perm(sdp, udi[0] + upi[0], udi[1] + upi[1], udi[2] + upi[2], udk[0] + upk[0], udk[1] + upk[1], udk[2] + upk[2]);
+    perm(sp, udi[0], udi[1], udi[2], udk[0], udk[1], udk[2]); // mu_d with the p-scaled permanent field
+    perm(sd, upi[0], upi[1], upi[2], upk[0], upk[1], upk[2]); // mu_p with the d-scaled permanent field
+    if (su != T(0)) {
+        pair_series<T, true>(Rm, dip<T>(udi[0], udi[1], udi[2]), dip<T>(upk[0], upk[1], upk[2]), xr, yr, zr, o);
+        F[0] += su * o.fx; F[1] += su * o.fy; F[2] += su * o.fz;
+        pair_series<T, true>(Rm, dip<T>(upi[0], upi[1], upi[2]), dip<T>(udk[0], udk[1], udk[2]), xr, yr, zr, o);
+        F[0] += su * o.fx; F[1] += su * o.fy; F[2] += su * o.fz;
+    }
+}
+
+template <bool EW, class T, bool CPD = false>
 __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
 {
     TILE_PROLOGUE
@@ -245,8 +375,9 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
         jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
         for (int c = 0; c < 10; c++) jm[c][lane] = a.smp[c * np + sj];
         for (int c = 0; c < 6; c++) ju[c][lane] = a.src[c * np + sj];
-        jth[lane] = a.spol[np + sj];
-        jpd[lane] = a.spol[2 * np + sj];
+        // Thole: thole and pdamp; HIPPO: total charge and valence exponent
+        jth[lane] = CPD ? a.scp[2 * np + sj] : a.spol[np + sj];
+        jpd[lane] = CPD ? a.scp[np + sj] : a.spol[2 * np + sj];
         jflag[lane] = a.sflag[sj];
         for (int c = 0; c < 6; c++) acc[c][lane] = T(0);
         const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
@@ -254,7 +385,7 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
         const Mp<T> mi = load_mp<T>(a.smp, np, si);
         T ui[6];
         for (int c = 0; c < 6; c++) ui[c] = a.src[c * np + si];
-        const T thi = a.spol[np + si], pdi = a.spol[2 * np + si];
+        const T thi = CPD ? a.scp[2 * np + si] : a.spol[np + si], pdi = CPD ? a.scp[np + si] : a.spol[2 * np + si];
         __syncthreads();
         unsigned long long mask = a.mask[(size_t)t * 128 + lane]; // in-cutoff j of this lane
         T F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0};
@@ -265,16 +396,23 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
             image<double>(a.box, xd, yd, zd);
             const T xr = T(xd), yr = T(yd), zr = T(zd);
             const T r2 = xr * xr + yr * yr + zr * zr;
-            T lam[6], R[6];
-            thole_lambda<T>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
-            pol_series<EW, T>(r2, T(a.aewald), lam, R);
             Mp<T> mk;
             mk.c = jm[0][jj]; mk.dx = jm[1][jj]; mk.dy = jm[2][jj]; mk.dz = jm[3][jj];
             mk.qxx = jm[4][jj]; mk.qxy = jm[5][jj]; mk.qxz = jm[6][jj]; mk.qyy = jm[7][jj]; mk.qyz = jm[8][jj];
             mk.qzz = jm[9][jj];
             const T udk[3] = {ju[0][jj], ju[1][jj], ju[2][jj]}, upk[3] = {ju[3][jj], ju[4][jj], ju[5][jj]};
             T f3[3] = {0, 0, 0}, tk[3] = {0, 0, 0};
-            pol_pair_grad<T>(R, mi, mk, ui, ui + 3, udk, upk, xr, yr, zr, T(1), T(0), T(0), T(1), f3, Ti, tk);
+            if (CPD) {
+                T Rt[6], Xi[6], Xk[6], Rm[6];
+                hippo_pol_series<T>(r2, pdi, jpd[jj], EW, T(a.aewald), T(1), Rt, Xi, Xk, Rm);
+                pol_pair_grad_cp<T>(Rt, Xi, Xk, Rm, thi, jth[jj], mi, mk, ui, ui + 3, udk, upk, xr, yr, zr, T(1), T(0),
+                                    T(0), T(1), f3, Ti, tk);
+            } else {
+                T lam[6], R[6];
+                thole_lambda<T>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
+                pol_series<EW, T>(r2, T(a.aewald), lam, R);
+                pol_pair_grad<T>(R, mi, mk, ui, ui + 3, udk, upk, xr, yr, zr, T(1), T(0), T(0), T(1), f3, Ti, tk);
+            }
             F[0] += f3[0]; F[1] += f3[1]; F[2] += f3[2];
             atomicAdd(&acc[0][jj], -f3[0]);
             atomicAdd(&acc[1][jj], -f3[1]);
@@ -300,17 +438,12 @@ __global__ __launch_bounds__(64) void k_pol_grad_excl(PolArgs a)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nexcl) return;
     const ExclPair q = a.excl[t];
-    if (q.d == 1.0 && q.p == 1.0 && q.u == 1.0) return;
     const int np = a.npad;
     const int si = a.iperm[q.i], sk = a.iperm[q.k];
     double xr = a.sx[sk] - a.sx[si], yr = a.sy[sk] - a.sy[si], zr = a.sz[sk] - a.sz[si];
     image<double>(a.box, xr, yr, zr);
     const double r2 = xr * xr + yr * yr + zr * zr;
     if (r2 > a.off2) return;
-    double lam[6], rr[6], R[6];
-    thole_lambda<double>(sqrt(r2), a.spol[2 * np + si], a.spol[2 * np + sk], a.spol[np + si], a.spol[np + sk], lam);
-    rr_series<double>(r2, 1.0, rr);
-    for (int c = 0; c < 6; c++) R[c] = lam[c] * rr[c];
     const Mp<double> mi = load_mp<double>(a.smp, np, si), mk = load_mp<double>(a.smp, np, sk);
     double udi[3], upi[3], udk[3], upk[3];
     for (int c = 0; c < 3; c++) {
@@ -318,8 +451,34 @@ __global__ __launch_bounds__(64) void k_pol_grad_excl(PolArgs a)
         udk[c] = a.src[c * np + sk]; upk[c] = a.src[(3 + c) * np + sk];
     }
     double F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0}, Tk[3] = {0, 0, 0};
-    pol_pair_grad<double>(R, mi, mk, udi, upi, udk, upk, xr, yr, zr, 0.0, q.d - 1.0, q.p - 1.0, q.u - 1.0, F, Ti,
-                          Tk);
+    // the whole pair (the tiles leave the scaled pairs out): the series at
+    // scale s is linear in s, B - rr + s dmp rr, so the damped part takes the
+    // scales as weights and the Ewald part B - rr weight 1
+    double rr[6];
+    rr_series<double>(r2, 1.0, rr);
+    if (a.cpd) {
+        double Rt[6], Xi[6], Xk[6], Rm[6];
+        hippo_pol_series<double>(r2, a.scp[np + si], a.scp[np + sk], false, 0.0, 1.0, Rt, Xi, Xk, Rm);
+        pol_pair_grad_cp<double>(Rt, Xi, Xk, Rm, a.scp[2 * np + si], a.scp[2 * np + sk], mi, mk, udi, upi, udk, upk,
+                                 xr, yr, zr, 0.0, q.d, q.p, q.u, F, Ti, Tk);
+    } else {
+        double lam[6], R[6];
+        thole_lambda<double>(sqrt(r2), a.spol[2 * np + si], a.spol[2 * np + sk], a.spol[np + si], a.spol[np + sk],
+                             lam);
+        for (int c = 0; c < 6; c++) R[c] = lam[c] * rr[c];
+        pol_pair_grad<double>(R, mi, mk, udi, upi, udk, upk, xr, yr, zr, 0.0, q.d, q.p, q.u, F, Ti, Tk);
+    }
+    if (a.aewald > 0.0) {
+        double bn[6], R0[6];
+        ewald_series<double>(r2, a.aewald, bn);
+        for (int c = 0; c < 6; c++) R0[c] = bn[c] - rr[c];
+        const double Z[6] = {0, 0, 0, 0, 0, 0};
+        if (a.cpd)
+            pol_pair_grad_cp<double>(R0, Z, Z, R0, a.scp[2 * np + si], a.scp[2 * np + sk], mi, mk, udi, upi, udk, upk,
+                                     xr, yr, zr, 0.0, 1.0, 1.0, 1.0, F, Ti, Tk);
+        else
+            pol_pair_grad<double>(R0, mi, mk, udi, upi, udk, upk, xr, yr, zr, 0.0, 1.0, 1.0, 1.0, F, Ti, Tk);
+    }
     const double hf = 0.5 * a.f;
     for (int c = 0; c < 3; c++) {
         grad_add(a.dep, a.depq, 3 * q.i + c, hf * F[c]);
@@ -561,6 +720,8 @@ static PolArgs pol_args(pmq_ctx *c, double off2)
     a.n = c->n; a.npad = c->npad;
     a.tiles = c->tiles.p; a.ntiles = c->ntiles_eff.p;
     a.sx = c->sx.p; a.sy = c->sy.p; a.sz = c->sz.p; a.smp = c->smp.p; a.spol = c->spol.p;
+    a.scp = c->scp.p;
+    a.cpd = c->prm.pentype != PMQ_PEN_NONE ? 1 : 0;
     a.sflag = c->sflag.p; a.perm = c->perm.p; a.iperm = c->iperm.p;
     a.box = c->box;
     a.off2 = off2;
@@ -568,7 +729,7 @@ static PolArgs pol_args(pmq_ctx *c, double off2)
     a.f = c->prm.electric / c->prm.dielec;
     a.excl = c->excl.p;
     a.nexcl = (int)c->h_excl.size();
-    a.mask = c->tmask.p;
+    a.mask = c->xmask.p; // the scaled pairs are done whole by the exclusion kernels
     a.src = nullptr; a.dst = nullptr; a.dep = c->dep.p; a.tep = c->tep.p;
     a.dstq = nullptr; a.depq = nullptr; a.tepq = nullptr;
     a.part = c->tpart.p;
@@ -601,13 +762,19 @@ int polar_prepare(pmq_ctx *c, double off2, bool mixed, bool safe)
     {
         ProfScope ps_(c, "pol_field_tiles");
         const dim3 g(tile_grid(c));
-        if (mixed) {
-            if (a.aewald > 0) hipLaunchKernelGGL((k_pol_field_tiles<true, float>), g, dim3(64), 0, c->stream, a);
-            else hipLaunchKernelGGL((k_pol_field_tiles<false, float>), g, dim3(64), 0, c->stream, a);
-        } else {
-            if (a.aewald > 0) hipLaunchKernelGGL((k_pol_field_tiles<true, double>), g, dim3(64), 0, c->stream, a);
-            else hipLaunchKernelGGL((k_pol_field_tiles<false, double>), g, dim3(64), 0, c->stream, a);
+#define PMQ_FIELD(EW, T, CPD) hipLaunchKernelGGL((k_pol_field_tiles<EW, T, CPD>), g, dim3(64), 0, c->stream, a)
+        const int sel = (mixed ? 4 : 0) + (a.aewald > 0 ? 2 : 0) + (a.cpd ? 1 : 0);
+        switch (sel) {
+        case 0: PMQ_FIELD(false, double, false); break;
+        case 1: PMQ_FIELD(false, double, true); break;
+        case 2: PMQ_FIELD(true, double, false); break;
+        case 3: PMQ_FIELD(true, double, true); break;
+        case 4: PMQ_FIELD(false, float, false); break;
+        case 5: PMQ_FIELD(false, float, true); break;
+        case 6: PMQ_FIELD(true, float, false); break;
+        default: PMQ_FIELD(true, float, true); break;
         }
+#undef PMQ_FIELD
     }
     PMQ_HIP(hipGetLastError());
     TileReduceArgs r{};
@@ -703,13 +870,19 @@ int polar_gradient(pmq_ctx *c, double off2, bool mixed)
     {
         ProfScope ps_(c, "pol_grad_tiles");
         const dim3 g(tile_grid(c));
-        if (mixed) {
-            if (a.aewald > 0) hipLaunchKernelGGL((k_pol_grad_tiles<true, float>), g, dim3(64), 0, c->stream, a);
-            else hipLaunchKernelGGL((k_pol_grad_tiles<false, float>), g, dim3(64), 0, c->stream, a);
-        } else {
-            if (a.aewald > 0) hipLaunchKernelGGL((k_pol_grad_tiles<true, double>), g, dim3(64), 0, c->stream, a);
-            else hipLaunchKernelGGL((k_pol_grad_tiles<false, double>), g, dim3(64), 0, c->stream, a);
+#define PMQ_GRAD(EW, T, CPD) hipLaunchKernelGGL((k_pol_grad_tiles<EW, T, CPD>), g, dim3(64), 0, c->stream, a)
+        const int sel = (mixed ? 4 : 0) + (a.aewald > 0 ? 2 : 0) + (a.cpd ? 1 : 0);
+        switch (sel) {
+        case 0: PMQ_GRAD(false, double, false); break;
+        case 1: PMQ_GRAD(false, double, true); break;
+        case 2: PMQ_GRAD(true, double, false); break;
+        case 3: PMQ_GRAD(true, double, true); break;
+        case 4: PMQ_GRAD(false, float, false); break;
+        case 5: PMQ_GRAD(false, float, true); break;
+        case 6: PMQ_GRAD(true, float, false); break;
+        default: PMQ_GRAD(true, float, true); break;
         }
+#undef PMQ_GRAD
     }
     PMQ_HIP(hipGetLastError());
     TileReduceArgs r{};

@@ -172,6 +172,56 @@ __device__ __forceinline__ void rr_series(R r2, R scale_f, R *Rs)
     Rs[5] = R(9) * Rs[4] * rinv2;
 }
 
+// valence-valence (overlap) factors of the Gordon1 model, orders 1..11
+// (damping.h:32-196): shared by charge penetration, the HIPPO dipole-dipole
+// damping and the dispersion damping
+template <class R>
+__device__ __forceinline__ void overlap_gordon1(R ai_, R ak_, R di_, R dk_, R ei_, R ek_, bool same, R *dmpik)
+{
+    // symmetric in the pair: evaluated with the smaller exponent first so
+    // that both orders of a pair round alike (the unequal-exponent form
+    // cancels terms up to ~1e5 times the result when the exponents are close)
+    const bool sw = ai_ > ak_;
+    const R ai = sw ? ak_ : ai_, ak = sw ? ai_ : ak_, di = sw ? dk_ : di_, dk = sw ? di_ : dk_;
+    const R ei = sw ? ek_ : ei_, ek = sw ? ei_ : ek_;
+    const R di2 = di * di, di3 = di2 * di, di4 = di2 * di2, di5 = di4 * di;
+    const R dk2 = dk * dk, dk3 = dk2 * dk, dk4 = dk2 * dk2, dk5 = dk4 * dk;
+    if (same) {
+        const R di6 = di3 * di3, di7 = di3 * di4, di8 = di4 * di4;
+        const R b5 = R(1) + di + R(0.5) * di2 + di3 / R(6) + di4 / R(24) + di5 / R(120) + di6 / R(720);
+        dmpik[0] = R(1) - (R(1) + R(11) * di / R(16) + R(3) * di2 / R(16) + di3 / R(48)) * ei;
+        dmpik[1] = R(1) - (R(1) + di + R(0.5) * di2 + R(7) * di3 / R(48) + di4 / R(48)) * ei;
+        dmpik[2] = R(1) - (R(1) + di + R(0.5) * di2 + di3 / R(6) + di4 / R(24) + di5 / R(144)) * ei;
+        dmpik[3] = R(1) - b5 * ei;
+        dmpik[4] = R(1) - (b5 + di7 / R(5040)) * ei;
+        dmpik[5] = R(1) - (b5 + di7 / R(5040) + di8 / R(45360)) * ei;
+    } else {
+        const R ai2 = ai * ai, ak2 = ak * ak;
+        const R ti = ak2 / (ak2 - ai2), tk = ai2 / (ai2 - ak2);
+        const R ti2 = ti * ti, tk2 = tk * tk;
+        const R di6 = di3 * di3, dk6 = dk3 * dk3;
+        dmpik[0] = R(1) - ti2 * (R(1) + R(2) * tk + R(0.5) * di) * ei - tk2 * (R(1) + R(2) * ti + R(0.5) * dk) * ek;
+        dmpik[1] = R(1) - ti2 * (R(1) + di + R(0.5) * di2) * ei - tk2 * (R(1) + dk + R(0.5) * dk2) * ek
+                 - R(2) * ti2 * tk * (R(1) + di) * ei - R(2) * tk2 * ti * (R(1) + dk) * ek;
+        dmpik[2] = R(1) - ti2 * (R(1) + di + R(0.5) * di2 + di3 / R(6)) * ei
+                 - tk2 * (R(1) + dk + R(0.5) * dk2 + dk3 / R(6)) * ek
+                 - R(2) * ti2 * tk * (R(1) + di + di2 / R(3)) * ei - R(2) * tk2 * ti * (R(1) + dk + dk2 / R(3)) * ek;
+        dmpik[3] = R(1) - ti2 * (R(1) + di + R(0.5) * di2 + di3 / R(6) + di4 / R(30)) * ei
+                 - tk2 * (R(1) + dk + R(0.5) * dk2 + dk3 / R(6) + dk4 / R(30)) * ek
+                 - R(2) * ti2 * tk * (R(1) + di + R(2) * di2 / R(5) + di3 / R(15)) * ei
+                 - R(2) * tk2 * ti * (R(1) + dk + R(2) * dk2 / R(5) + dk3 / R(15)) * ek;
+        dmpik[4] = R(1) - ti2 * (R(1) + di + R(0.5) * di2 + di3 / R(6) + R(4) * di4 / R(105) + di5 / R(210)) * ei
+                 - tk2 * (R(1) + dk + R(0.5) * dk2 + dk3 / R(6) + R(4) * dk4 / R(105) + dk5 / R(210)) * ek
+                 - R(2) * ti2 * tk * (R(1) + di + R(3) * di2 / R(7) + R(2) * di3 / R(21) + di4 / R(105)) * ei
+                 - R(2) * tk2 * ti * (R(1) + dk + R(3) * dk2 / R(7) + R(2) * dk3 / R(21) + dk4 / R(105)) * ek;
+        dmpik[5] = R(1)
+                 - ti2 * (R(1) + di + R(0.5) * di2 + di3 / R(6) + R(5) * di4 / R(126) + R(2) * di5 / R(315) + di6 / R(1890)) * ei
+                 - tk2 * (R(1) + dk + R(0.5) * dk2 + dk3 / R(6) + R(5) * dk4 / R(126) + R(2) * dk5 / R(315) + dk6 / R(1890)) * ek
+                 - R(2) * ti2 * tk * (R(1) + di + R(4) * di2 / R(9) + di3 / R(9) + di4 / R(63) + di5 / R(945)) * ei
+                 - R(2) * tk2 * ti * (R(1) + dk + R(4) * dk2 / R(9) + dk3 / R(9) + dk4 / R(63) + dk5 / R(945)) * ek;
+    }
+}
+
 // Gordon charge-penetration damping factors (damping.h:32-196): dmpi damps
 // the valence multipoles of i seen by core k, dmpk the reverse, dmpik the
 // valence-valence interaction.  Orders 1..11 of the radial series.
@@ -198,40 +248,13 @@ __device__ __forceinline__ void damp_gordon(R r, R ai, R ak, R *dmpi, R *dmpk, R
             dmpk[3] = R(1) - (R(1) + dk + R(0.5) * dk2 + dk3 / R(6) + dk4 / R(30)) * ek;
             dmpk[4] = R(1) - (R(1) + dk + R(0.5) * dk2 + dk3 / R(6) + R(4) * dk4 / R(105) + dk5 / R(210)) * ek;
         }
-        if (same) {
-            const R di6 = di3 * di3, di7 = di3 * di4, di8 = di4 * di4;
-            const R b5 = R(1) + di + R(0.5) * di2 + di3 / R(6) + di4 / R(24) + di5 / R(120) + di6 / R(720);
-            dmpik[0] = R(1) - (R(1) + R(11) * di / R(16) + R(3) * di2 / R(16) + di3 / R(48)) * ei;
-            dmpik[1] = R(1) - (R(1) + di + R(0.5) * di2 + R(7) * di3 / R(48) + di4 / R(48)) * ei;
-            dmpik[2] = R(1) - (R(1) + di + R(0.5) * di2 + di3 / R(6) + di4 / R(24) + di5 / R(144)) * ei;
-            dmpik[3] = R(1) - b5 * ei;
-            dmpik[4] = R(1) - (b5 + di7 / R(5040)) * ei;
-            dmpik[5] = R(1) - (b5 + di7 / R(5040) + di8 / R(45360)) * ei;
-        } else {
-            const R ai2 = ai * ai, ak2 = ak * ak;
-            const R ti = ak2 / (ak2 - ai2), tk = ai2 / (ai2 - ak2);
-            const R ti2 = ti * ti, tk2 = tk * tk;
-            const R di6 = di3 * di3, dk6 = dk3 * dk3;
-            dmpik[0] = R(1) - ti2 * (R(1) + R(2) * tk + R(0.5) * di) * ei - tk2 * (R(1) + R(2) * ti + R(0.5) * dk) * ek;
-            dmpik[1] = R(1) - ti2 * (R(1) + di + R(0.5) * di2) * ei - tk2 * (R(1) + dk + R(0.5) * dk2) * ek
-                     - R(2) * ti2 * tk * (R(1) + di) * ei - R(2) * tk2 * ti * (R(1) + dk) * ek;
-            dmpik[2] = R(1) - ti2 * (R(1) + di + R(0.5) * di2 + di3 / R(6)) * ei
-                     - tk2 * (R(1) + dk + R(0.5) * dk2 + dk3 / R(6)) * ek
-                     - R(2) * ti2 * tk * (R(1) + di + di2 / R(3)) * ei - R(2) * tk2 * ti * (R(1) + dk + dk2 / R(3)) * ek;
-            dmpik[3] = R(1) - ti2 * (R(1) + di + R(0.5) * di2 + di3 / R(6) + di4 / R(30)) * ei
-                     - tk2 * (R(1) + dk + R(0.5) * dk2 + dk3 / R(6) + dk4 / R(30)) * ek
-                     - R(2) * ti2 * tk * (R(1) + di + R(2) * di2 / R(5) + di3 / R(15)) * ei
-                     - R(2) * tk2 * ti * (R(1) + dk + R(2) * dk2 / R(5) + dk3 / R(15)) * ek;
-            dmpik[4] = R(1) - ti2 * (R(1) + di + R(0.5) * di2 + di3 / R(6) + R(4) * di4 / R(105) + di5 / R(210)) * ei
-                     - tk2 * (R(1) + dk + R(0.5) * dk2 + dk3 / R(6) + R(4) * dk4 / R(105) + dk5 / R(210)) * ek
-                     - R(2) * ti2 * tk * (R(1) + di + R(3) * di2 / R(7) + R(2) * di3 / R(21) + di4 / R(105)) * ei
-                     - R(2) * tk2 * ti * (R(1) + dk + R(3) * dk2 / R(7) + R(2) * dk3 / R(21) + dk4 / R(105)) * ek;
-            dmpik[5] = R(1)
-                     - ti2 * (R(1) + di + R(0.5) * di2 + di3 / R(6) + R(5) * di4 / R(126) + R(2) * di5 / R(315) + di6 / R(1890)) * ei
-                     - tk2 * (R(1) + dk + R(0.5) * dk2 + dk3 / R(6) + R(5) * dk4 / R(126) + R(2) * dk5 / R(315) + dk6 / R(1890)) * ek
-                     - R(2) * ti2 * tk * (R(1) + di + R(4) * di2 / R(9) + di3 / R(9) + di4 / R(63) + di5 / R(945)) * ei
-                     - R(2) * tk2 * ti * (R(1) + dk + R(4) * dk2 / R(9) + dk3 / R(9) + dk4 / R(63) + dk5 / R(945)) * ek;
-        }
+        // in FP64 whatever R is: the unequal-exponent form cancels terms up to
+        // ~1e5 times its result when the exponents are close (O and H of
+        // HIPPO water differ by 1%), which FP32 cannot carry
+        double d6[6];
+        const double rd = (double)r, aid = (double)ai, akd = (double)ak;
+        overlap_gordon1<double>(aid, akd, aid * rd, akd * rd, exp(-aid * rd), exp(-akd * rd), same, d6);
+        for (int j = 0; j < 6; j++) dmpik[j] = R(d6[j]);
     } else {
         dmpi[0] = R(1) - ei;
         dmpi[1] = R(1) - (R(1) + di) * ei;
@@ -272,54 +295,116 @@ __device__ __forceinline__ void damp_gordon(R r, R ai, R ak, R *dmpi, R *dmpk, R
     }
 }
 
-// Charge-penetration pair (pairMpole.h:186-389): a.c/b.c hold the valence
-// charges; the interaction is core-core (undamped) + core_k with the
-// valence multipoles of i (dmpi) + core_i with those of k (dmpk) +
-// valence-valence (dmpik).  Each class X uses the series sf*dmp_X*rr, or
-// with Ewald (EW) f*(B - (1 - dmp_X) rr): the real-space part of a damped
-// interaction whose reciprocal part is the undamped PME sum.
-template <class R, bool GRAD, int PEN, bool EW = false>
-__device__ __forceinline__ void pair_cp(R r2, R xr, R yr, R zr, R scale_f, R corei, R alphai, const Mp<R> &a,
-                                        R corek, R alphak, const Mp<R> &b, PairOut<R> &o, const R *bn = nullptr)
+// Complements 1 - dmp of the Gordon damping factors (damping.h:32-196),
+// formed directly (no 1 - (1 - x) cancellation): xi/xk the one-site factors
+// of i and k (orders 1..9), xik the overlap factors (orders 1..11, in FP64
+// whatever R is: the unequal-exponent form cancels terms up to ~1e5 times its
+// result when the exponents are close).
+template <class R, int PEN>
+__device__ __forceinline__ void damp_gordon_comp(R r, R ai, R ak, R *xi, R *xk, R *xik)
 {
-    R rr[6], dmpi[5], dmpk[5], dmpik[6], Rs[6];
-    rr_series<R>(r2, R(1), rr);
-    damp_gordon<R, PEN>(sqrt(r2), alphai, alphak, dmpi, dmpk, dmpik);
-    auto series = [&](const R *dmp, int nd) {
-        for (int j = 0; j < 6; j++) {
-            const R d = j < nd ? dmp[j] : R(0);
-            Rs[j] = EW ? scale_f * (bn[j] - (R(1) - d) * rr[j]) : scale_f * d * rr[j];
+    const bool same = fabs(ai - ak) < R(1e-3);
+    auto onesite = [](R d, R *x) {
+        const R e = exp(-d);
+        const R d2 = d * d, d3 = d2 * d, d4 = d2 * d2, d5 = d4 * d;
+        if (PEN == 1) {
+            x[0] = (R(1) + R(0.5) * d) * e;
+            x[1] = (R(1) + d + R(0.5) * d2) * e;
+            x[2] = (R(1) + d + R(0.5) * d2 + d3 / R(6)) * e;
+            x[3] = (R(1) + d + R(0.5) * d2 + d3 / R(6) + d4 / R(30)) * e;
+            x[4] = (R(1) + d + R(0.5) * d2 + d3 / R(6) + R(4) * d4 / R(105) + d5 / R(210)) * e;
+        } else {
+            x[0] = e;
+            x[1] = (R(1) + d) * e;
+            x[2] = (R(1) + d + d2 / R(3)) * e;
+            x[3] = (R(1) + d + R(0.4) * d2 + d3 / R(15)) * e;
+            x[4] = R(1); // no order-9 factor in this model (dmp = 0)
         }
     };
-    Mp<R> ci{}, ck{};
+    onesite(ai * r, xi);
+    if (same) {
+        for (int j = 0; j < 5; j++) xk[j] = xi[j];
+    } else {
+        onesite(ak * r, xk);
+    }
+    const double rd = (double)r, aid = (double)ai, akd = (double)ak;
+    const double di = aid * rd, dk = akd * rd;
+    double d6[6];
+    if (PEN == 1) {
+        overlap_gordon1<double>(aid, akd, di, dk, exp(-di), exp(-dk), same, d6);
+        for (int j = 0; j < 6; j++) xik[j] = R(1.0 - d6[j]);
+    } else {
+        // symmetric: the smaller exponent first so both orders round alike
+        const bool sw = aid > akd;
+        const double a1 = sw ? akd : aid, a2 = sw ? aid : akd, p = a1 * rd, q = a2 * rd;
+        const double ep = exp(-p), eq = exp(-q);
+        const double p2 = p * p, p3 = p2 * p, p4 = p2 * p2, p5 = p4 * p, q2 = q * q, q3 = q2 * q, q4 = q2 * q2,
+                     q5 = q4 * q;
+        if (same) {
+            xik[0] = R((1.0 + 0.5 * p) * ep);
+            xik[1] = R((1.0 + p + 0.5 * p2) * ep);
+            xik[2] = R((1.0 + p + 0.5 * p2 + p3 / 6.0) * ep);
+            xik[3] = R((1.0 + p + 0.5 * p2 + p3 / 6.0 + p4 / 30.0) * ep);
+            xik[4] = R((1.0 + p + 0.5 * p2 + p3 / 6.0 + 4.0 * p4 / 105.0 + p5 / 210.0) * ep);
+            xik[5] = R((1.0 + p + 0.5 * p2 + p3 / 6.0 + 5.0 * p4 / 126.0 + 2.0 * p5 / 315.0 + p3 * p3 / 1890.0) * ep);
+        } else {
+            const double t1 = a2 * a2 / (a2 * a2 - a1 * a1), t2 = a1 * a1 / (a1 * a1 - a2 * a2);
+            xik[0] = R(t1 * ep + t2 * eq);
+            xik[1] = R(t1 * (1.0 + p) * ep + t2 * (1.0 + q) * eq);
+            xik[2] = R(t1 * (1.0 + p + p2 / 3.0) * ep + t2 * (1.0 + q + q2 / 3.0) * eq);
+            xik[3] = R(t1 * (1.0 + p + 0.4 * p2 + p3 / 15.0) * ep + t2 * (1.0 + q + 0.4 * q2 + q3 / 15.0) * eq);
+            xik[4] = R(t1 * (1.0 + p + 3.0 * p2 / 7.0 + 2.0 * p3 / 21.0 + p4 / 105.0) * ep
+                       + t2 * (1.0 + q + 3.0 * q2 / 7.0 + 2.0 * q3 / 21.0 + q4 / 105.0) * eq);
+            xik[5] = R(t1 * (1.0 + p + 4.0 * p2 / 9.0 + p3 / 9.0 + p4 / 63.0 + p5 / 945.0) * ep
+                       + t2 * (1.0 + q + 4.0 * q2 / 9.0 + q3 / 9.0 + q4 / 63.0 + q5 / 945.0) * eq);
+        }
+    }
+}
+
+// Charge-penetration pair (pairMpole.h:186-389): a.c/b.c hold the valence
+// charges, qi/qk the total charges.  The model's four parts (core-core
+// undamped, core_k with the valence multipoles of i damped by dmpi, core_i
+// with those of k by dmpk, valence-valence by dmpik) are summed as
+//   [total multipoles, undamped] - core_k x val_i (1-dmpi) - core_i x val_k
+//   (1-dmpk) - val_i x val_k (1-dmpik)
+// which is the same sum without the large core/valence cancellation (cores
+// of 6 and valence charges of -6.4 for water O), so it stays accurate in
+// FP32.  Each series is scale_f x (factor) x rr; with Ewald (EW) the
+// undamped part uses B instead of rr: the real-space part of a damped
+// interaction whose reciprocal part is the undamped PME sum.
+template <class R, bool GRAD, int PEN, bool EW = false>
+__device__ __forceinline__ void pair_cp(R r2, R xr, R yr, R zr, R scale_f, R corei, R qi, R alphai, const Mp<R> &a,
+                                        R corek, R qk, R alphak, const Mp<R> &b, PairOut<R> &o,
+                                        const R *bn = nullptr)
+{
+    R rr[6], xi[5], xk[5], xik[6], Rs[6];
+    rr_series<R>(r2, R(1), rr);
+    damp_gordon_comp<R, PEN>(sqrt(r2), alphai, alphak, xi, xk, xik);
+    Mp<R> ta = a, tb = b, ci{}, ck{};
+    ta.c = qi;
+    tb.c = qk;
     ci.c = corei;
     ck.c = corek;
-    const R one[6] = {R(1), R(1), R(1), R(1), R(1), R(1)};
-    series(one, 6);
-    pair_series<R, GRAD, false>(Rs, ci, ck, xr, yr, zr, o);
-    // core-core has no potential on the valence charges
-    o.phii = R(0);
-    o.phik = R(0);
-    series(dmpik, 6);
+    for (int j = 0; j < 6; j++) Rs[j] = scale_f * (EW ? bn[j] : rr[j]);
+    pair_series<R, GRAD, false>(Rs, ta, tb, xr, yr, zr, o);  // phii/phik: derivatives in the valence charges
+    for (int j = 0; j < 6; j++) Rs[j] = -scale_f * xik[j] * rr[j];
     pair_series<R, GRAD, true>(Rs, a, b, xr, yr, zr, o);
-    {
-        PairOut<R> t;
-        series(dmpi, 5);
-        pair_series<R, GRAD, false>(Rs, a, ck, xr, yr, zr, t);
-        o.e += t.e;
-        o.phii += t.phii;
-        if (GRAD) {
-            o.fx += t.fx; o.fy += t.fy; o.fz += t.fz;
-            o.tix += t.tix; o.tiy += t.tiy; o.tiz += t.tiz;
-        }
-        series(dmpk, 5);
-        pair_series<R, GRAD, false>(Rs, ci, b, xr, yr, zr, t);
-        o.e += t.e;
-        o.phik += t.phik;
-        if (GRAD) {
-            o.fx += t.fx; o.fy += t.fy; o.fz += t.fz;
-            o.tkx += t.tkx; o.tky += t.tky; o.tkz += t.tkz;
-        }
+    PairOut<R> t;
+    for (int j = 0; j < 6; j++) Rs[j] = j < 5 ? -scale_f * xi[j] * rr[j] : R(0);
+    pair_series<R, GRAD, false>(Rs, a, ck, xr, yr, zr, t);
+    o.e += t.e;
+    o.phii += t.phii;
+    if (GRAD) {
+        o.fx += t.fx; o.fy += t.fy; o.fz += t.fz;
+        o.tix += t.tix; o.tiy += t.tiy; o.tiz += t.tiz;
+    }
+    for (int j = 0; j < 6; j++) Rs[j] = j < 5 ? -scale_f * xk[j] * rr[j] : R(0);
+    pair_series<R, GRAD, false>(Rs, ci, b, xr, yr, zr, t);
+    o.e += t.e;
+    o.phik += t.phik;
+    if (GRAD) {
+        o.fx += t.fx; o.fy += t.fy; o.fz += t.fz;
+        o.tkx += t.tkx; o.tky += t.tky; o.tkz += t.tkz;
     }
 }
 
@@ -390,6 +475,125 @@ __device__ __forceinline__ void corr_series(R_ r2, R_ s, const R_ *lam, R_ *R)
     R_ rr[6];
     rr_series<R_>(r2, R_(1), rr);
     for (int q = 0; q < 6; q++) R[q] = (s - R_(1)) * lam[q] * rr[q];
+}
+
+} // namespace pmq
+
+namespace pmq {
+
+// the overlap (valence-valence) factors of two exponential densities at r,
+// evaluated in FP64 whatever R is (see damp_gordon)
+template <class R>
+__device__ __forceinline__ void overlap_damp(R r, R ai, R ak, R *dmpik)
+{
+    const double rd = (double)r, aid = (double)ai, akd = (double)ak;
+    double d6[6];
+    overlap_gordon1<double>(aid, akd, aid * rd, akd * rd, exp(-aid * rd), exp(-akd * rd), fabs(aid - akd) < 1e-3, d6);
+    for (int j = 0; j < 6; j++) dmpik[j] = R(d6[j]);
+}
+
+// one-site factors of one exponential density (Gordon1), orders 1..9: the
+// field of a valence multipole of exponent a seen by a point at distance r
+template <class R>
+__device__ __forceinline__ void onesite_damp(R r, R a, R *dmp)
+{
+    const R d = a * r, e = exp(-d);
+    const R d2 = d * d, d3 = d2 * d, d4 = d2 * d2, d5 = d4 * d;
+    dmp[0] = R(1) - (R(1) + R(0.5) * d) * e;
+    dmp[1] = R(1) - (R(1) + d + R(0.5) * d2) * e;
+    dmp[2] = R(1) - (R(1) + d + R(0.5) * d2 + d3 / R(6)) * e;
+    dmp[3] = R(1) - (R(1) + d + R(0.5) * d2 + d3 / R(6) + d4 / R(30)) * e;
+    dmp[4] = R(1) - (R(1) + d + R(0.5) * d2 + d3 / R(6) + R(4) * d4 / R(105) + d5 / R(210)) * e;
+}
+
+// HIPPO repulsion radial series B_n = D^n (S^2), n = 0..5, D = -(1/r) d/dr,
+// S the overlap of two normalized 1s Slater functions of exponents ai/2 and
+// ak/2 (Rackers and Ponder, JCP 150, 084104 (2019)).  D obeys the product
+// rule, so B_n = sum_k C(n,k) S_k S_{n-k} with S_k = D^k S.
+//   equal exponents z (within 1e-3 in a; their mean is used):
+//     S = (1 + z r + z^2 r^2/3) e^{-zr}, S_1 = z^2/3 (1 + z r) e^{-zr},
+//     S_2 = z^4/3 e^{-zr}, S_3 = z^5/3 x e^{-zr}, S_4 = z^5/3 (x^3 + z x^2) e^{-zr},
+//     S_5 = z^5/3 (3 x^5 + 3 z x^4 + z^2 x^3) e^{-zr}   (x = 1/r)
+//   unequal a, b: S = A [(a - t x) e^{-br} + (b + t x) e^{-ar}],
+//     t = 4ab/(a^2-b^2), A = 8 (ab)^{3/2}/(a^2-b^2)^2, and D maps the
+//     coefficient q_m of x^m e^{-cr} to m q_m x^{m+2} + c q_m x^{m+1}.
+template <class R>
+__device__ __forceinline__ void rep_series(R r, R ai, R ak, R *B)
+{
+    R S[6];
+    const R x = R(1) / r;
+    if (fabs(ai - ak) < R(1e-3)) {
+        const R z = R(0.25) * (ai + ak);
+        const R e = exp(-z * r), z2 = z * z, z5 = z2 * z2 * z;
+        const R x2 = x * x, x3 = x2 * x;
+        S[0] = (R(1) + z * r + z2 * r * r / R(3)) * e;
+        S[1] = z2 / R(3) * (R(1) + z * r) * e;
+        S[2] = z2 * z2 / R(3) * e;
+        S[3] = z5 / R(3) * x * e;
+        S[4] = z5 / R(3) * (x3 + z * x2) * e;
+        S[5] = z5 / R(3) * (R(3) * x3 * x2 + R(3) * z * x2 * x2 + z2 * x3) * e;
+    } else {
+        const R a = R(0.5) * ai, b = R(0.5) * ak;
+        const R d = a * a - b * b, t = R(4) * a * b / d;
+        const R ab = a * b;
+        const R A = R(8) * ab * sqrt(ab) / (d * d);
+#pragma unroll
+        for (int n = 0; n < 6; n++) S[n] = R(0);
+#pragma unroll
+        for (int term = 0; term < 2; term++) {
+            const R c = term == 0 ? b : a;
+            R q[12];
+#pragma unroll
+            for (int m = 0; m < 12; m++) q[m] = R(0);
+            q[0] = A * (term == 0 ? a : b);
+            q[1] = A * (term == 0 ? -t : t);
+            const R e = exp(-c * r);
+#pragma unroll
+            for (int n = 0; n < 6; n++) {
+                // value of sum_m q_m x^m (powers up to 1 + 2n)
+                R v = R(0);
+#pragma unroll
+                for (int m = 11; m >= 0; m--)
+                    if (m <= 1 + 2 * n) v = v * x + q[m];
+                S[n] += v * e;
+                if (n == 5) break;
+                R nq[12];
+#pragma unroll
+                for (int m = 0; m < 12; m++) nq[m] = R(0);
+#pragma unroll
+                for (int m = 0; m < 12; m++) {
+                    if (m > 1 + 2 * n) continue;
+                    if (m + 2 < 12) nq[m + 2] += R(m) * q[m];
+                    if (m + 1 < 12) nq[m + 1] += c * q[m];
+                }
+#pragma unroll
+                for (int m = 0; m < 12; m++) q[m] = nq[m];
+            }
+        }
+    }
+    B[0] = S[0] * S[0];
+    B[1] = R(2) * S[0] * S[1];
+    B[2] = R(2) * (S[0] * S[2] + S[1] * S[1]);
+    B[3] = R(2) * (S[0] * S[3] + R(3) * S[1] * S[2]);
+    B[4] = R(2) * (S[0] * S[4] + R(4) * S[1] * S[3]) + R(6) * S[2] * S[2];
+    B[5] = R(2) * (S[0] * S[5] + R(5) * S[1] * S[4] + R(10) * S[2] * S[3]);
+}
+
+// quintic multiplicative switch T(r) = c0 + c1 r + ... + c5 r^5 between the
+// taper distance and the cutoff (cutoffSwitch.cpp:125-133), and dT/dr
+struct Taper {
+    double cut, c[6];
+};
+
+template <class R>
+__device__ __forceinline__ void taper_eval(const Taper &tp, R r, R &T, R &dT)
+{
+    T = R(1);
+    dT = R(0);
+    if (r <= R(tp.cut)) return;
+    const R c0 = R(tp.c[0]), c1 = R(tp.c[1]), c2 = R(tp.c[2]), c3 = R(tp.c[3]), c4 = R(tp.c[4]), c5 = R(tp.c[5]);
+    T = c0 + r * (c1 + r * (c2 + r * (c3 + r * (c4 + r * c5))));
+    dT = c1 + r * (R(2) * c2 + r * (R(3) * c3 + r * (R(4) * c4 + r * R(5) * c5)));
 }
 
 } // namespace pmq

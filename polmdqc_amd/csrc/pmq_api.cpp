@@ -377,10 +377,17 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
         const double piv[6] = {1, 1, p->p2iscale, p->p3iscale, p->p4iscale, p->p5iscale};
         const double dv[5] = {1, p->d1scale, p->d2scale, p->d3scale, p->d4scale};
         const double uv[5] = {1, p->u1scale, p->u2scale, p->u3scale, p->u4scale};
+        const double wv[6] = {1, 1, p->w2scale, p->w3scale, p->w4scale, p->w5scale};
+        const double dsv[6] = {1, 1, p->dsp2scale, p->dsp3scale, p->dsp4scale, p->dsp5scale};
+        const double rv[6] = {1, 1, p->r2scale, p->r3scale, p->r4scale, p->r5scale};
         e.m = mv[e.mcls];
         e.p = e.intra ? piv[e.mcls] : pv[e.mcls];
-        e.d = dv[e.gcls];
-        e.u = uv[e.gcls];
+        e.d = p->dpequal ? e.p : dv[e.gcls];
+        // the dipole-dipole scale: u by group (AMOEBA), w by connectivity
+        // (HIPPO, i.e. with charge penetration)
+        e.u = p->pentype != PMQ_PEN_NONE ? wv[e.mcls] : uv[e.gcls];
+        e.dsp = dsv[e.mcls];
+        e.rep = rv[e.mcls];
     }
     if ((rc = upload(c, c->excl, c->h_excl.data(), c->h_excl.size()))) return rc;
     const bool polar = p->polarity != nullptr;
@@ -414,6 +421,46 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
         c->err = "pmq_set_params: Ewald needs aewald > 0 and ewaldcut > 0";
         return PMQ_ERR_ARG;
     }
+    // HIPPO dispersion and repulsion
+    c->have_disp = p->csix != nullptr;
+    c->have_rep = p->sizpr != nullptr;
+    if (c->have_disp) {
+        if (!p->adisp || !(p->dispcut > 0.0)) {
+            c->err = "pmq_set_params: dispersion needs csix, adisp and dispcut > 0";
+            return PMQ_ERR_ARG;
+        }
+        if (p->use_dewald && !(p->dewald > 0.0)) {
+            c->err = "pmq_set_params: dispersion Ewald needs dewald > 0";
+            return PMQ_ERR_ARG;
+        }
+        if ((rc = upload(c, c->csix, p->csix, n))) return rc;
+        if ((rc = upload(c, c->adisp, p->adisp, n))) return rc;
+        for (int i = 0; i < n; i++)
+            if (p->csix[i] != 0.0 && !(p->adisp[i] > 0.0)) {
+                c->err = "pmq_set_params: dispersion damping exponents must be positive";
+                return PMQ_ERR_ARG;
+            }
+    }
+    if (c->have_rep) {
+        if (!p->dmppr || !p->elepr || !(p->repcut > 0.0)) {
+            c->err = "pmq_set_params: repulsion needs sizpr, dmppr, elepr and repcut > 0";
+            return PMQ_ERR_ARG;
+        }
+        for (int i = 0; i < n; i++)
+            if (p->sizpr[i] != 0.0 && !(p->dmppr[i] > 0.0)) {
+                c->err = "pmq_set_params: repulsion damping exponents must be positive";
+                return PMQ_ERR_ARG;
+            }
+        if ((rc = upload(c, c->sizpr, p->sizpr, n))) return rc;
+        if ((rc = upload(c, c->dmppr, p->dmppr, n))) return rc;
+        if ((rc = upload(c, c->elepr, p->elepr, n))) return rc;
+    }
+    if (c->have_disp || c->have_rep) {
+        PMQ_HIP(c->shp.alloc(5 * (size_t)c->npad));
+        PMQ_HIP(c->der.alloc(3 * (size_t)n));
+        PMQ_HIP(c->dedsp.alloc(3 * (size_t)n));
+        PMQ_HIP(c->ter.alloc(3 * (size_t)n));
+    }
     if (p->use_ewald && p->use_chgflx) {
         c->err = "pmq_set_params: charge flux with Ewald summation is not built";
         return PMQ_ERR_UNSUPPORTED;
@@ -423,6 +470,7 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
     c->prm.pcore = c->prm.pval0 = c->prm.palpha = nullptr;
     c->prm.bl = c->prm.bflx = c->prm.anat = c->prm.aflx = c->prm.abflx = nullptr;
     c->prm.polarity = c->prm.thole = c->prm.pdamp = nullptr;
+    c->prm.csix = c->prm.adisp = c->prm.sizpr = c->prm.dmppr = c->prm.elepr = nullptr;
     PMQ_HIP(hipStreamSynchronize(c->stream));
     c->have_prm = true;
     batch_graph_release(c);
@@ -482,18 +530,37 @@ static int check_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
         c->err = "pmq_compute: the mixed form computes energy and gradient without charge flux";
         return PMQ_ERR_UNSUPPORTED;
     }
-    if (terms & ~(unsigned)(PMQ_TERM_MPOLE | PMQ_TERM_POLAR)) {
-        c->err = "pmq_compute: repulsion and dispersion are not built";
-        return PMQ_ERR_UNSUPPORTED;
+    if (terms & ~(unsigned)(PMQ_TERM_MPOLE | PMQ_TERM_POLAR | PMQ_TERM_REPEL | PMQ_TERM_DISP)) {
+        c->err = "pmq_compute: unknown term";
+        return PMQ_ERR_ARG;
     }
     const bool do_mpole = terms & PMQ_TERM_MPOLE, do_polar = terms & PMQ_TERM_POLAR;
+    const bool do_rep = terms & PMQ_TERM_REPEL, do_disp = terms & PMQ_TERM_DISP;
+    if ((do_rep && !c->have_rep) || (do_disp && !c->have_disp)) {
+        c->err = "pmq_compute: repulsion or dispersion requested but its parameters were not set";
+        return PMQ_ERR_STATE;
+    }
+    if ((do_rep || do_disp) && (mode == PMQ_VIRIAL || mode == PMQ_ANALYSIS)) {
+        c->err = "pmq_compute: repulsion and dispersion compute energy and gradient (no virial or analysis)";
+        return PMQ_ERR_UNSUPPORTED;
+    }
+    if (do_rep || do_disp) {
+        // the pair structure is built at the electrostatic cutoff when those
+        // terms run; repcut and dispcut must fit inside it
+        const double ecut = c->prm.use_ewald ? c->prm.ewaldcut : c->prm.mpolecut;
+        if ((do_mpole || do_polar) && c->box.kind != PMQ_BOX_NONE &&
+            ((do_rep && c->prm.repcut > ecut) || (do_disp && c->prm.dispcut > ecut))) {
+            c->err = "pmq_compute: repcut and dispcut may not exceed the electrostatic cutoff";
+            return PMQ_ERR_UNSUPPORTED;
+        }
+    }
     const bool ewald = c->prm.use_ewald;
     if (do_polar && c->h_polarity.empty()) {
         c->err = "pmq_compute: polarization requested but no polarizabilities were set";
         return PMQ_ERR_STATE;
     }
-    if (do_polar && c->prm.pentype != PMQ_PEN_NONE) {
-        c->err = "pmq_compute: charge-penetration (HIPPO) polarization is not built";
+    if (do_polar && c->prm.pentype == PMQ_PEN_GORDON2) {
+        c->err = "pmq_compute: HIPPO polarization damping is built for the Gordon1 charge penetration";
         return PMQ_ERR_UNSUPPORTED;
     }
     if (ewald && (mode == PMQ_VIRIAL || mode == PMQ_ANALYSIS)) {
@@ -514,7 +581,9 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
 {
     const bool mixed = precision == PMQ_MIXED;
     const bool do_mpole = terms & PMQ_TERM_MPOLE, do_polar = terms & PMQ_TERM_POLAR;
+    const bool do_rep = terms & PMQ_TERM_REPEL, do_disp = terms & PMQ_TERM_DISP;
     const bool ewald = c->prm.use_ewald;
+    const bool dewald = do_disp && c->prm.use_dewald && c->box.kind != PMQ_BOX_NONE;
     PMQ_HIP(hipSetDevice(c->device));
     const int n = c->n;
     int rc;
@@ -529,12 +598,19 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
             if ((rc = launch_zero(c, c->dep.p, 3 * (size_t)n))) return rc;
             if ((rc = launch_zero(c, c->tep.p, 3 * (size_t)n))) return rc;
         }
+        if (do_rep) {
+            if ((rc = launch_zero(c, c->der.p, 3 * (size_t)n))) return rc;
+            if ((rc = launch_zero(c, c->ter.p, 3 * (size_t)n))) return rc;
+        }
+        if (do_disp && (rc = launch_zero(c, c->dedsp.p, 3 * (size_t)n))) return rc;
     }
     if (mode == PMQ_ANALYSIS && (rc = launch_zero(c, c->aem.p, n))) return rc;
     c->pcg_iters = 0;
-    if (do_mpole || do_polar) {
+    if (do_mpole || do_polar || do_rep || do_disp) {
         double cut = ewald ? c->prm.ewaldcut : c->prm.mpolecut;
-        if (c->box.kind == PMQ_BOX_NONE) cut = std::min(cut, 1e12);
+        if (!(do_mpole || do_polar)) cut = std::max(do_rep ? c->prm.repcut : 0.0, do_disp ? c->prm.dispcut : 0.0);
+        if (c->box.kind == PMQ_BOX_NONE)
+            cut = std::min(std::max({cut, do_rep ? c->prm.repcut : 0.0, do_disp ? c->prm.dispcut : 0.0}), 1e12);
         const double off2 = cut * cut;
         const bool cf = c->prm.pentype != PMQ_PEN_NONE && c->prm.use_chgflx;
         if (c->prm.use_chgflx && (rc = launch_chgflx_delta(c))) return rc;
@@ -544,15 +620,26 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
             if ((rc = tile_structure(c, off2, safe))) return rc;
         }
         if ((rc = launch_rotpole(c, true))) return rc;
+        if ((do_rep || do_disp) && (rc = hippo_sort(c))) return rc;
+        if (do_rep) {
+            const double rc2 = std::min(c->prm.repcut * c->prm.repcut, off2);
+            if ((rc = launch_rep(c, rc2, grad, mixed))) return rc;
+            if (grad && (rc = launch_torque(c, PMQ_GRADIENT, c->ter.p, c->der.p, mixed))) return rc;
+        }
+        if (do_disp) {
+            const double dc2 = std::min(c->prm.dispcut * c->prm.dispcut, off2);
+            if ((rc = launch_disp(c, dc2, grad, mixed))) return rc;
+        }
         if (do_polar && (rc = polar_prepare(c, off2, mixed, safe))) return rc;
         if (do_mpole && (rc = launch_mpole_direct(c, mode, off2, mixed))) return rc;
         if (do_mpole && mixed && grad) {
             if ((rc = launch_fx_flush(c, c->fx_grad.p, c->dem.p, 3 * (size_t)n, PMQ_FX_GRAD))) return rc;
             if ((rc = launch_fx_flush(c, c->fx_grad.p + 3 * (size_t)n, c->te.p, 3 * (size_t)n, PMQ_FX_GRAD))) return rc;
         }
-        if (ewald) {
+        if ((ewald && (do_mpole || do_polar)) || dewald) {
             if ((rc = pme_setup(c))) return rc;
-            if ((rc = pme_mpole(c, mode, do_mpole, do_polar))) return rc;
+            if (ewald && (do_mpole || do_polar) && (rc = pme_mpole(c, mode, do_mpole, do_polar))) return rc;
+            if (dewald && (rc = pme_disp(c, grad, !(ewald && (do_mpole || do_polar))))) return rc;
         }
         if (do_polar) {
             if ((rc = polar_solve(c, safe, &c->pcg_iters))) return rc;
@@ -603,6 +690,10 @@ static int batch_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
 {
     if (mode != PMQ_ENERGY && mode != PMQ_GRADIENT) {
         c->err = "pmq_compute: batched clusters compute energy and gradient";
+        return PMQ_ERR_UNSUPPORTED;
+    }
+    if (terms & (PMQ_TERM_REPEL | PMQ_TERM_DISP)) {
+        c->err = "pmq_compute: batched clusters compute the multipole and polarization terms";
         return PMQ_ERR_UNSUPPORTED;
     }
     if (c->prm.pentype != PMQ_PEN_NONE || c->prm.use_chgflx || c->prm.use_ewald || c->box.kind != PMQ_BOX_NONE) {
@@ -843,6 +934,22 @@ int pmq_fetch(pmq_ctx *c, pmq_result *r)
     if (r->desum && grad) {
         for (size_t j = 0; j < 3 * (size_t)n; j++) r->desum[j] = (mp ? gm[j] : 0.0) + (pol ? gp[j] : 0.0);
     }
+    const bool rp = c->last_terms & PMQ_TERM_REPEL, dsp = c->last_terms & PMQ_TERM_DISP;
+    std::vector<double> gr, gd;
+    if (grad && rp) {
+        gr.resize(3 * (size_t)n);
+        PMQ_HIP(hipMemcpy(gr.data(), c->der.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (grad && dsp) {
+        gd.resize(3 * (size_t)n);
+        PMQ_HIP(hipMemcpy(gd.data(), c->dedsp.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (r->der && grad)
+        for (size_t j = 0; j < 3 * (size_t)n; j++) r->der[j] = rp ? gr[j] : 0.0;
+    if (r->dedsp && grad)
+        for (size_t j = 0; j < 3 * (size_t)n; j++) r->dedsp[j] = dsp ? gd[j] : 0.0;
+    if (r->desum && grad)
+        for (size_t j = 0; j < 3 * (size_t)n; j++) r->desum[j] += (rp ? gr[j] : 0.0) + (dsp ? gd[j] : 0.0);
     if (pol && r->uind) PMQ_HIP(hipMemcpy(r->uind, c->uind_out.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
     if (pol && r->uinp)
         PMQ_HIP(hipMemcpy(r->uinp, c->uind_out.p + 3 * (size_t)n, 3 * n * sizeof(double), hipMemcpyDeviceToHost));

@@ -187,7 +187,8 @@ struct ExclPair {
     int gcls;       // 1..4 polarization-group class for d/u scaling, 0 = none
     int intra;      // 1 if k is in i's own polarization group (pNiscale)
     int pad;
-    double m, p, d, u; // scale values (filled by set_params)
+    double m, p, d, u; // scale values (filled by set_params); u is the mutual-field scale (w for HIPPO)
+    double dsp, rep;   // HIPPO dispersion and repulsion scales (by connectivity)
 };
 
 } // namespace pmq
@@ -270,6 +271,7 @@ struct pmq_ctx {
     bool have_uind = false;
     // dipole-field operator T as a row-owned sparse matrix (k_pairs.hip)
     pmq::DBuf<unsigned long long> tmask;         // 128 per tile: 64 row masks, 64 column masks
+    pmq::DBuf<unsigned long long> xmask;         // 128 per tile: tmask rows and mmask rows without the scaled pairs
     pmq::DBuf<unsigned long long> mmask;         // 64 per tile: row masks of pairs with an active atom
     pmq::DBuf<double> tpart;                     // per tile: 2 sides x PMQ_TILE_NV x 64 partial sums
     pmq::DBuf<double> tscal;                     // per tile: PMQ_TILE_NS scalars (energy, virial, ...)
@@ -303,6 +305,8 @@ struct pmq_ctx {
     int bsorder = 5;
     double pmeA[3][3] = {{0}};                   // u = A r
     pmq::DBuf<double> pme_g;                     // (K0/2+1) K1 K2 reciprocal kernel
+    pmq::DBuf<double> pme_gd;                    // the same for dispersion (m = 0 included)
+    double pme_dalpha = 0.0;                     // dispersion Ewald coefficient pme_gd was built for
     pmq::DBuf<double> qgrid;                     // 2 real grids
     pmq::DBuf<double> cgrid;                     // 2 complex half grids (interleaved re, im)
     pmq::DBuf<double> pme_frac;                  // 2 x 20 fractional potential derivatives per atom
@@ -317,6 +321,11 @@ struct pmq_ctx {
     double pme_alpha = 0.0;
 
     int pcg_iters = 0;
+    // HIPPO dispersion and repulsion
+    pmq::DBuf<double> csix, adisp, sizpr, dmppr, elepr; // caller order
+    pmq::DBuf<double> shp;                       // 5 * npad sorted: csix, adisp, sizpr, dmppr, elepr
+    pmq::DBuf<double> der, dedsp, ter;           // 3n gradients of er, edsp; torques of er
+    bool have_disp = false, have_rep = false;
     pmq::DBuf<double> uind_out;                  // 6n caller-order induced dipoles (d then p)
 
     // kernel timing (pmq_profile_*)
@@ -340,6 +349,10 @@ int launch_mpole_direct(pmq_ctx *c, int mode, double off2, bool mixed);
 int launch_fx_flush(pmq_ctx *c, long long *q, double *d, size_t n, double scale);
 int launch_torque(pmq_ctx *c, int mode, const double *te, double *de, bool mixed);
 int pme_setup(pmq_ctx *c);
+int pme_disp(pmq_ctx *c, bool grad, bool need_bin);
+int hippo_sort(pmq_ctx *c);
+int launch_disp(pmq_ctx *c, double off2, bool grad, bool mixed);
+int launch_rep(pmq_ctx *c, double off2, bool grad, bool mixed);
 void pme_release(pmq_ctx *c);
 int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar);
 int polar_prepare(pmq_ctx *c, double off2, bool mixed, bool safe);

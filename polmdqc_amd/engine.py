@@ -141,6 +141,34 @@ class Engine:
         p.aewald = float(s.get("aewald", 0.0))
         p.nfft1, p.nfft2, p.nfft3 = int(s.get("nfft1", 0)), int(s.get("nfft2", 0)), int(s.get("nfft3", 0))
         p.bsorder = int(s.get("bsorder", 5))
+        # HIPPO: w scales and d-equals-p (polarization with charge
+        # penetration), dispersion and repulsion; tapers below 1 are
+        # fractions of their cutoff (cutoffs.cpp:259-261)
+        for k in ("w2scale", "w3scale", "w4scale", "w5scale"):
+            setattr(p, k, float(s.get(k, 1.0)))
+        p.dpequal = int(s.get("dpequal", 0))
+        for k, d in (("dsp2scale", 0.0), ("dsp3scale", 0.0), ("dsp4scale", 1.0), ("dsp5scale", 1.0),
+                     ("r2scale", 0.0), ("r3scale", 0.0), ("r4scale", 1.0), ("r5scale", 1.0)):
+            setattr(p, k, float(s.get(k, d)))
+
+        def taper(cut, key):
+            t = float(s.get(key, 0.9))
+            return t * cut if t < 1.0 else t
+
+        csix = s.get("csix")
+        if csix is not None and np.any(np.asarray(csix) != 0.0):
+            p.csix, p.adisp = self._d(csix), self._d(s["adisp"])
+            p.use_dewald, p.dewald = int(s.get("use_dewald", 0)), float(s.get("dewald", 0.0))
+            p.dispcut = float(s.get("dispcut", 9.0))
+            p.disptaper = taper(p.dispcut, "disptaper")
+            p.dfft1, p.dfft2, p.dfft3 = (int(s.get(k, s.get(k.replace("d", "n", 1), 0))) for k in
+                                         ("dfft1", "dfft2", "dfft3"))
+            p.dbsorder = int(s.get("dbsorder", s.get("bsorder", 5)))
+        sizpr = s.get("sizpr")
+        if sizpr is not None and np.any(np.asarray(sizpr) != 0.0):
+            p.sizpr, p.dmppr, p.elepr = self._d(sizpr), self._d(s["dmppr"]), self._d(s["elepr"])
+            p.repcut = float(s.get("repcut", 6.0))
+            p.reptaper = taper(p.repcut, "reptaper")
         self._check(self._L.pmq_set_params(self._ctx, C.byref(p)), "pmq_set_params")
         self.n = n
         self.box = box_from(s)
@@ -170,7 +198,10 @@ class Engine:
         uinp = np.zeros(3 * n)
         dep = np.zeros(3 * n)
         desum = np.zeros(3 * n)
+        der = np.zeros(3 * n)
+        dedsp = np.zeros(3 * n)
         r = L.Result()
+        r.der, r.dedsp = der.ctypes.data_as(_dp), dedsp.ctypes.data_as(_dp)
         r.dem, r.aem = dem.ctypes.data_as(_dp), aem.ctypes.data_as(_dp)
         r.dep, r.desum = dep.ctypes.data_as(_dp), desum.ctypes.data_as(_dp)
         r.rpole = rpole.ctypes.data_as(_dp) if want_rpole else None
@@ -179,7 +210,8 @@ class Engine:
         out = {"em": r.em, "ep": r.ep, "er": r.er, "edsp": r.edsp, "esum": r.esum, "einter": r.einter,
                "nem": int(r.nem), "vir": np.array(r.vir[:]).reshape(3, 3), "pcg_iters": r.pcg_iters,
                "dem": dem.reshape(n, 3), "dep": dep.reshape(n, 3), "desum": desum.reshape(n, 3), "aem": aem,
-               "uind": uind.reshape(n, 3), "uinp": uinp.reshape(n, 3)}
+               "uind": uind.reshape(n, 3), "uinp": uinp.reshape(n, 3), "der": der.reshape(n, 3),
+               "dedsp": dedsp.reshape(n, 3)}
         if want_rpole:
             out["rpole"] = rpole.reshape(n, 13)
         return out
@@ -230,7 +262,8 @@ def _profile_bind(L):
     L._prof_ready = True
 
 
-KERNEL_FAMILIES = ("neighbors", "mpole_tiles", "pol_field_tiles", "pol_pair_fill", "pol_mutual_spmv",
+KERNEL_FAMILIES = ("neighbors", "mpole_tiles", "rep_tiles", "disp_tiles", "pol_field_tiles", "pol_pair_fill",
+                   "pol_mutual_spmv",
                    "pol_grad_tiles", "pme_spread", "pme_fft", "pme_gather", "batch_clusters")
 
 

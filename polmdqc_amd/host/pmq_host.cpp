@@ -107,7 +107,24 @@ int upload(std::string *err)
     p.p2iscale = g.p2iscale; p.p3iscale = g.p3iscale; p.p4iscale = g.p4iscale; p.p5iscale = g.p5iscale;
     p.d1scale = g.d1scale; p.d2scale = g.d2scale; p.d3scale = g.d3scale; p.d4scale = g.d4scale;
     p.u1scale = g.u1scale; p.u2scale = g.u2scale; p.u3scale = g.u3scale; p.u4scale = g.u4scale;
+    p.w2scale = g.w2scale; p.w3scale = g.w3scale; p.w4scale = g.w4scale; p.w5scale = g.w5scale;
+    p.dpequal = g.dpequal;
     p.poleps = g.poleps; p.politer = g.politer;
+    auto taper = [](double t, double cut) { return t < 1.0 ? t * cut : t; }; // cutoffs.cpp:259-261
+    if (!g.csix.empty()) {
+        p.csix = g.csix.data(); p.adisp = g.adisp.data();
+        p.dsp2scale = g.dsp2scale; p.dsp3scale = g.dsp3scale; p.dsp4scale = g.dsp4scale; p.dsp5scale = g.dsp5scale;
+        p.use_dewald = g.use_dewald; p.dewald = g.dewald;
+        p.dispcut = g.use_dewald ? g.dewaldcut : g.dispcut; // cutoffs.cpp:254-256
+        p.disptaper = taper(g.disptaper, p.dispcut);
+        p.dfft1 = g.dfft1 ? g.dfft1 : g.nfft1; p.dfft2 = g.dfft2 ? g.dfft2 : g.nfft2;
+        p.dfft3 = g.dfft3 ? g.dfft3 : g.nfft3; p.dbsorder = g.dbsorder;
+    }
+    if (!g.sizpr.empty()) {
+        p.sizpr = g.sizpr.data(); p.dmppr = g.dmppr.data(); p.elepr = g.elepr.data();
+        p.r2scale = g.r2scale; p.r3scale = g.r3scale; p.r4scale = g.r4scale; p.r5scale = g.r5scale;
+        p.repcut = g.repcut; p.reptaper = taper(g.reptaper, g.repcut);
+    }
     p.use_ewald = g.use_ewald; p.mpolecut = g.mpolecut; p.ewaldcut = g.ewaldcut; p.aewald = g.aewald;
     p.nfft1 = g.nfft1; p.nfft2 = g.nfft2; p.nfft3 = g.nfft3; p.bsorder = g.bsorder;
     if ((rc = pmq_set_params(ctx, &p))) return rc;
@@ -137,8 +154,10 @@ int run(CalcMode m, unsigned terms, std::string *err)
     const int n = g.n;
     const bool grad = m == CalcMode::Gradient || m == CalcMode::Virial;
     std::vector<double> dm(3 * (size_t)n), dp(3 * (size_t)n), am(n), ui(3 * (size_t)n), up(3 * (size_t)n);
+    std::vector<double> dr(3 * (size_t)n), dd(3 * (size_t)n);
     pmq_result r{};
     r.dem = dm.data(); r.dep = dp.data(); r.aem = am.data(); r.uind = ui.data(); r.uinp = up.data();
+    r.der = dr.data(); r.dedsp = dd.data();
     const pmq_box b = box_of(g);
     rc = pmq_eval(ctx, mode_of(m), terms, g.precision, g.x.data(), g.y.data(), g.z.data(), &b, &r);
     if (rc) {
@@ -168,6 +187,20 @@ int run(CalcMode m, unsigned terms, std::string *err)
             for (size_t j = 0; j < 3 * (size_t)n; j++) g.dep[j] += dp[j];
         }
     }
+    if (terms & PMQ_TERM_REPEL) {
+        g.er += r.er;
+        if (grad) {
+            g.der.resize(3 * (size_t)n, 0.0);
+            for (size_t j = 0; j < 3 * (size_t)n; j++) g.der[j] += dr[j];
+        }
+    }
+    if (terms & PMQ_TERM_DISP) {
+        g.edsp += r.edsp;
+        if (grad) {
+            g.dedsp.resize(3 * (size_t)n, 0.0);
+            for (size_t j = 0; j < 3 * (size_t)n; j++) g.dedsp[j] += dd[j];
+        }
+    }
     if (m == CalcMode::Virial)
         for (int a = 0; a < 3; a++)
             for (int c = 0; c < 3; c++) g.vir[a][c] += r.vir[3 * a + c];
@@ -177,10 +210,12 @@ int run(CalcMode m, unsigned terms, std::string *err)
 void zero_outputs()
 {
     const size_t n3 = 3 * (size_t)g.n;
-    g.em = g.ep = g.esum = g.einter = 0.0;
+    g.em = g.ep = g.er = g.edsp = g.esum = g.einter = 0.0;
     g.nem = 0;
     g.dem.assign(n3, 0.0);
     g.dep.assign(n3, 0.0);
+    g.der.assign(n3, 0.0);
+    g.dedsp.assign(n3, 0.0);
     g.desum.assign(n3, 0.0);
     g.aem.assign(g.n, 0.0);
     for (auto &row : g.vir)
@@ -216,17 +251,33 @@ template <CalcMode M> void epolar()
     if (run(M, PMQ_TERM_POLAR, &err)) fatal("epolar: " + err);
 }
 
+// erepel<CalcMode> replaces the reference's empty pair loop (erepel.cpp:42-153);
+// edisp<CalcMode> the absent dispersion term (energy.cpp:280)
+template <CalcMode M> void erepel()
+{
+    std::string err;
+    if (run(M, PMQ_TERM_REPEL, &err)) fatal("erepel: " + err);
+}
+
+template <CalcMode M> void edisp()
+{
+    std::string err;
+    if (run(M, PMQ_TERM_DISP, &err)) fatal("edisp: " + err);
+}
+
 int try_energy(CalcMode m, std::string *err)
 {
     zero_outputs();
     unsigned terms = 0;
     if (g.use_mpole) terms |= PMQ_TERM_MPOLE;
     if (g.use_polar) terms |= PMQ_TERM_POLAR;
+    if (g.use_repel) terms |= PMQ_TERM_REPEL;
+    if (g.use_disp) terms |= PMQ_TERM_DISP;
     if (!terms) return PMQ_OK;
     int rc = run(m, terms, err);
     if (rc) return rc;
-    g.esum = g.em + g.ep;
-    for (size_t j = 0; j < g.desum.size(); j++) g.desum[j] = g.dem[j] + g.dep[j];
+    g.esum = g.em + g.ep + g.er + g.edsp;
+    for (size_t j = 0; j < g.desum.size(); j++) g.desum[j] = g.dem[j] + g.dep[j] + g.der[j] + g.dedsp[j];
     return PMQ_OK;
 }
 
@@ -242,6 +293,10 @@ template void empole<CalcMode::Gradient>();
 template void empole<CalcMode::Virial>();
 template void epolar<CalcMode::Energy>();
 template void epolar<CalcMode::Gradient>();
+template void erepel<CalcMode::Energy>();
+template void erepel<CalcMode::Gradient>();
+template void edisp<CalcMode::Energy>();
+template void edisp<CalcMode::Gradient>();
 template void energy<CalcMode::Energy>();
 template void energy<CalcMode::Analysis>();
 template void energy<CalcMode::Gradient>();

@@ -54,10 +54,22 @@ struct Globals {
     double p2iscale = 0, p3iscale = 0, p4iscale = 0.5, p5iscale = 1;
     double d1scale = 0, d2scale = 1, d3scale = 1, d4scale = 1;
     double u1scale = 1, u2scale = 1, u3scale = 1, u4scale = 1;
+    double w2scale = 1, w3scale = 1, w4scale = 1, w5scale = 1;   // initprm.cpp:390-393
+    bool dpequal = false;
     double poleps = 1e-6;
     int politer = 100;
+    // HIPPO dispersion: kdsp.h (csix, adisp per atom), dsppot.h, ewald.h, pme.h, limits.h
+    std::vector<double> csix, adisp;
+    double dsp2scale = 0, dsp3scale = 0, dsp4scale = 1, dsp5scale = 1; // initprm.cpp:338-341
+    bool use_dewald = false;
+    double dewald = 0, dewaldcut = 7.0, dispcut = 9.0, disptaper = 0.9; // taper < 1: fraction of the cutoff
+    int dfft1 = 0, dfft2 = 0, dfft3 = 0, dbsorder = 5;
+    // HIPPO repulsion: repel.h, reppot.h
+    std::vector<double> sizpr, dmppr, elepr;
+    double r2scale = 0, r3scale = 0, r4scale = 1, r5scale = 1;       // initprm.cpp:332-335
+    double repcut = 6.0, reptaper = 0.9;                 // cutoffs.cpp:57,65
     // potent.h
-    bool use_mpole = true, use_polar = false;
+    bool use_mpole = true, use_polar = false, use_repel = false, use_disp = false;
     // boxes.h
     bool use_bounds = false, monoclinic = false, triclinic = false, octahedron = false, dodecadron = false;
     double xbox = 0, ybox = 0, zbox = 0, alpha = 90, beta = 90, gamma = 90;
@@ -66,9 +78,9 @@ struct Globals {
     double aewald = 0, mpolecut = 9.0, ewaldcut = 7.0;
     int nfft1 = 0, nfft2 = 0, nfft3 = 0, bsorder = 5;
     // outputs: energi.h, deriv.h (n x 3), virial.h, analyz.h, action.h, inter.h, polar.h
-    double em = 0, ep = 0, esum = 0, einter = 0;
+    double em = 0, ep = 0, er = 0, edsp = 0, esum = 0, einter = 0;
     int nem = 0;
-    std::vector<double> dem, dep, desum, aem, uind, uinp;
+    std::vector<double> dem, dep, der, dedsp, desum, aem, uind, uinp;
     double vir[3][3] = {{0}};
     int pcg_iters = 0;
     // precision of the GPU evaluation (PMQ_PLAIN_FP64 = 0, PMQ_MIXED = 1)
@@ -82,6 +94,8 @@ void mechanic_changed();
 
 template <CalcMode M> void empole();
 template <CalcMode M> void epolar();
+template <CalcMode M> void erepel();
+template <CalcMode M> void edisp();
 template <CalcMode M> void energy();
 
 // energy<M>() with the library's status instead of an exit

@@ -256,5 +256,7 @@ def water_system(xyz: np.ndarray, ff: str = "water03", box: float | None = None,
         s.update({"use_ewald": 0, "aewald": 0.0, "ewaldcut": 7.0, "nfft1": 0, "nfft2": 0, "nfft3": 0, "bsorder": 5,
                   "use_dewald": 0, "dewald": 0.0})
     s["dispcut"] = float(ewald.get("cutoff", 7.0)) if ewald else s["mpolecut"]
-    s["repcut"] = float(ewald.get("cutoff", 7.0)) if ewald else s["mpolecut"]
+    # repulsion cutoff: 6 A by default (cutoffs.cpp:57), kept inside the
+    # electrostatic cutoff
+    s["repcut"] = min(6.0, float(ewald.get("cutoff", 7.0)) if ewald else s["mpolecut"])
     return s
