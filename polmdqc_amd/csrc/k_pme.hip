@@ -47,7 +47,8 @@ static int fft_check(pmq_ctx *c, rocfft_status s, const char *what)
 
 static void destroy_plans(pmq_ctx *c)
 {
-    void **plans[4] = {&c->plan_f1, &c->plan_b1, &c->plan_f2, &c->plan_b2};
+    void **plans[8] = {&c->plan_f1, &c->plan_b1, &c->plan_f2, &c->plan_b2,
+                       &c->plan_f1s, &c->plan_b1s, &c->plan_f2s, &c->plan_b2s};
     for (auto p : plans)
         if (*p) {
             rocfft_plan_destroy((rocfft_plan)*p);
@@ -150,6 +151,9 @@ int pme_setup(pmq_ctx *c)
             }
     PMQ_HIP(c->pme_g.alloc(G.size()));
     PMQ_HIP(hipMemcpyAsync(c->pme_g.p, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    std::vector<float> Gs(G.begin(), G.end());
+    PMQ_HIP(c->pme_g_s.alloc(Gs.size()));
+    PMQ_HIP(hipMemcpyAsync(c->pme_g_s.p, Gs.data(), Gs.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
     if (dew) {
         PMQ_HIP(c->pme_gd.alloc(Gd.size()));
         PMQ_HIP(hipMemcpyAsync(c->pme_gd.p, Gd.data(), Gd.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -176,8 +180,16 @@ int pme_setup(pmq_ctx *c)
         if ((rc = fft_check(c, rocfft_plan_create(&pb2, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
                                                   rocfft_precision_double, 3, len, 2, nullptr), "plan"))) return rc;
         c->plan_b2 = pb2;
+        rocfft_plan sp[4];
+        const rocfft_transform_type tt[4] = {rocfft_transform_type_real_forward, rocfft_transform_type_real_inverse,
+                                             rocfft_transform_type_real_forward, rocfft_transform_type_real_inverse};
+        for (int q = 0; q < 4; q++)
+            if ((rc = fft_check(c, rocfft_plan_create(&sp[q], rocfft_placement_notinplace, tt[q], rocfft_precision_single,
+                                                      3, len, q < 2 ? 1 : 2, nullptr), "plan"))) return rc;
+        c->plan_f1s = sp[0]; c->plan_b1s = sp[1]; c->plan_f2s = sp[2]; c->plan_b2s = sp[3];
+        c->pme_fft32 = -1; // to be timed
         size_t wmax = 0;
-        for (rocfft_plan pl : {pf1, pb1, pf2, pb2}) {
+        for (rocfft_plan pl : {pf1, pb1, pf2, pb2, sp[0], sp[1], sp[2], sp[3]}) {
             size_t w = 0;
             rocfft_plan_get_work_buffer_size(pl, &w);
             wmax = std::max(wmax, w);
@@ -241,11 +253,11 @@ __device__ __forceinline__ void atom_frac(const PmeGeom &g, double px, double py
 
 // weight set of one axis at stencil index k (selected without dynamic
 // register indexing)
-template <int ORD, int ND>
-__device__ __forceinline__ void pick(const double th[PMQ_MAX_ORDER][4], int k, double *out)
+template <int ORD, int ND, class T>
+__device__ __forceinline__ void pick(const T th[PMQ_MAX_ORDER][4], int k, T *out)
 {
 #pragma unroll
-    for (int d = 0; d <= ND; d++) out[d] = 0.0;
+    for (int d = 0; d <= ND; d++) out[d] = T(0);
 #pragma unroll
     for (int q = 0; q < ORD; q++)
         if (q == k) {
@@ -293,7 +305,7 @@ struct BrickArgs {
     const double *src;    // sorted SoA sources: 10 multipole comps (mode 0), 3 dipole comps x nset (mode 1)
                           // or one scalar per atom (mode 2: dispersion C6 coefficients)
     const double *charge; // mode 0: total monopole per sorted atom (charge penetration), or null
-    double *grid;         // nset grids
+    void *grid;           // nset grids of the kernel's arithmetic type
     size_t gstride;
 };
 
@@ -343,16 +355,19 @@ __global__ void k_pme_brick_start(int n, int nbricks, int sbits, const unsigned 
 // order, so the grid is bit-reproducible.
 constexpr int SPREAD_CH = 64;
 
-template <int ORD, int MODE>
+// T: the arithmetic of the weights and sums (double in the plain form, float
+// for the electrostatic grids of the mixed form); GT: the grid's type
+template <int ORD, int MODE, class T, class GT = T>
 __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
 {
     constexpr int ND = MODE == 0 ? 2 : MODE == 1 ? 1 : 0;
     constexpr int NC = MODE == 0 ? 10 : MODE == 1 ? 6 : 1;
     constexpr int NS = MODE == 1 ? 2 : 1;
-    __shared__ double th[3][ORD][ND + 1][SPREAD_CH];
-    __shared__ double cf[NC][SPREAD_CH];
+    __shared__ T th[3][ORD][ND + 1][SPREAD_CH];
+    __shared__ T cf[NC][SPREAD_CH];
     __shared__ int si0[3][SPREAD_CH];
-    __shared__ double red[4][NS][64];
+    __shared__ T red[4][NS][64];
+    GT *grid = (GT *)a.grid;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int K0 = a.g.K0, K1 = a.g.K1, K2 = a.g.K2;
     const int b = blockIdx.x;
@@ -382,9 +397,9 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
         const bool pon = p < bsz;
         const int lx = p % s0, ly = (p / s0) % s1, lz = p / (s0 * s1);
         const int gx = lo0 + lx, gy = lo1 + ly, gz = lo2 + lz;
-        double acc[NS];
+        T acc[NS];
 #pragma unroll
-        for (int q = 0; q < NS; q++) acc[q] = 0.0;
+        for (int q = 0; q < NS; q++) acc[q] = T(0);
         for (int base = 0; base < ntot; base += SPREAD_CH) {
             const int cnt = min(SPREAD_CH, ntot - base);
             __syncthreads();
@@ -403,8 +418,8 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
                 atom_frac(a.g, a.sx[s], a.sy[s], a.sz[s], w, i0, ORD);
 #pragma unroll
                 for (int ax = 0; ax < 3; ax++) {
-                    double t[PMQ_MAX_ORDER][4];
-                    bspline_weights<double, ND>(w[ax], ORD, t);
+                    T t[PMQ_MAX_ORDER][4];
+                    bspline_weights<T, ND>(T(w[ax]), ORD, t);
 #pragma unroll
                     for (int k = 0; k < ORD; k++)
 #pragma unroll
@@ -418,16 +433,16 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
                     if (a.charge) m[0] = a.charge[s];
                     frac_mpole(a.g, m, fm);
 #pragma unroll
-                    for (int c = 0; c < 10; c++) cf[c][tid] = fm[c];
+                    for (int c = 0; c < 10; c++) cf[c][tid] = T(fm[c]);
                 } else if (MODE == 2) {
-                    cf[0][tid] = a.src[s];
+                    cf[0][tid] = T(a.src[s]);
                 } else {
                     for (int set = 0; set < nset; set++) {
                         const double *d = a.src + (size_t)set * 3 * a.npad;
                         const double ux = d[s], uy = d[a.npad + s], uz = d[2 * a.npad + s];
 #pragma unroll
                         for (int r = 0; r < 3; r++)
-                            cf[3 * set + r][tid] = a.g.A[r][0] * ux + a.g.A[r][1] * uy + a.g.A[r][2] * uz;
+                            cf[3 * set + r][tid] = T(a.g.A[r][0] * ux + a.g.A[r][1] * uy + a.g.A[r][2] * uz);
                     }
                 }
             }
@@ -440,9 +455,9 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
                 if (kz < 0) kz += K2;
                 if (kx >= ORD || ky >= ORD || kz >= ORD) continue;
                 if (MODE == 0) {
-                    const double x0 = th[0][kx][0][j], x1 = th[0][kx][1][j], x2 = th[0][kx][2][j];
-                    const double y0 = th[1][ky][0][j], y1 = th[1][ky][1][j], y2 = th[1][ky][2][j];
-                    const double z0 = th[2][kz][0][j], z1 = th[2][kz][1][j], z2 = th[2][kz][2][j];
+                    const T x0 = th[0][kx][0][j], x1 = th[0][kx][1][j], x2 = th[0][kx][2][j];
+                    const T y0 = th[1][ky][0][j], y1 = th[1][ky][1][j], y2 = th[1][ky][2][j];
+                    const T z0 = th[2][kz][0][j], z1 = th[2][kz][1][j], z2 = th[2][kz][2][j];
                     acc[0] += cf[0][j] * x0 * y0 * z0 + cf[1][j] * x1 * y0 * z0 + cf[2][j] * x0 * y1 * z0
                             + cf[3][j] * x0 * y0 * z1 + cf[4][j] * x2 * y0 * z0 + cf[5][j] * x0 * y2 * z0
                             + cf[6][j] * x0 * y0 * z2 + cf[7][j] * x1 * y1 * z0 + cf[8][j] * x1 * y0 * z1
@@ -450,10 +465,10 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
                 } else if (MODE == 2) {
                     acc[0] += cf[0][j] * th[0][kx][0][j] * th[1][ky][0][j] * th[2][kz][0][j];
                 } else {
-                    const double x0 = th[0][kx][0][j], x1 = th[0][kx][1][j];
-                    const double y0 = th[1][ky][0][j], y1 = th[1][ky][1][j];
-                    const double z0 = th[2][kz][0][j], z1 = th[2][kz][1][j];
-                    const double w100 = x1 * y0 * z0, w010 = x0 * y1 * z0, w001 = x0 * y0 * z1;
+                    const T x0 = th[0][kx][0][j], x1 = th[0][kx][1][j];
+                    const T y0 = th[1][ky][0][j], y1 = th[1][ky][1][j];
+                    const T z0 = th[2][kz][0][j], z1 = th[2][kz][1][j];
+                    const T w100 = x1 * y0 * z0, w010 = x0 * y1 * z0, w001 = x0 * y0 * z1;
 #pragma unroll
                     for (int q = 0; q < NS; q++)
                         acc[q] += cf[3 * q][j] * w100 + cf[3 * q + 1][j] * w010 + cf[3 * q + 2][j] * w001;
@@ -466,16 +481,17 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
         __syncthreads();
         if (wv == 0 && pon)
             for (int q = 0; q < nset; q++)
-                a.grid[q * a.gstride + ((size_t)gz * K1 + gy) * K0 + gx] =
-                    ((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane];
+                grid[q * a.gstride + ((size_t)gz * K1 + gy) * K0 + gx] =
+                    GT(((red[0][q][lane] + red[1][q][lane]) + red[2][q][lane]) + red[3][q][lane]);
     }
 }
 
-__global__ void k_pme_convolve(int ncplx, int nset, const double *__restrict__ G, double *__restrict__ cg)
+template <class T>
+__global__ void k_pme_convolve(int ncplx, int nset, const T *__restrict__ G, T *__restrict__ cg)
 {
     const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (t >= (size_t)ncplx * nset) return;
-    const double g = G[t % ncplx];
+    const T g = G[t % ncplx];
     cg[2 * t] *= g;
     cg[2 * t + 1] *= g;
 }
@@ -483,9 +499,9 @@ __global__ void k_pme_convolve(int ncplx, int nset, const double *__restrict__ G
 // this lane's share of the potential derivatives at one site, in the
 // fractional order 0; 1,2,3; 11,22,33,12,13,23; 111,222,333,112,113,122,133,
 // 223,233,123 (NOUT = 4, 10 or 20); idle lanes (`on` false) give zeros
-template <int ORD, int NOUT>
-__device__ __forceinline__ void gather_part(const PmeGeom &g, const double *grid, const double tx[PMQ_MAX_ORDER][4],
-                                            const double *ty, const double *tz, const int *i0, int ky, int kz,
+template <int ORD, int NOUT, class T, class GT>
+__device__ __forceinline__ void gather_part(const PmeGeom &g, const GT *grid, const T tx[PMQ_MAX_ORDER][4],
+                                            const T *ty, const T *tz, const int *i0, int ky, int kz,
                                             bool on, double *o)
 {
 #pragma unroll
@@ -494,13 +510,13 @@ __device__ __forceinline__ void gather_part(const PmeGeom &g, const double *grid
     int gy = i0[1] + ky, gz = i0[2] + kz;
     if (gy >= g.K1) gy -= g.K1;
     if (gz >= g.K2) gz -= g.K2;
-    const double *row = grid + ((size_t)gz * g.K1 + gy) * g.K0;
-    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    const GT *row = grid + ((size_t)gz * g.K1 + gy) * g.K0;
+    T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll
     for (int kx = 0; kx < ORD; kx++) {
         int gx = i0[0] + kx;
         if (gx >= g.K0) gx -= g.K0;
-        const double q = row[gx];
+        const T q = T(row[gx]);
         s0 += q * tx[kx][0];
         s1 += q * tx[kx][1];
         if (NOUT > 4) s2 += q * tx[kx][2];
@@ -592,7 +608,7 @@ __device__ __forceinline__ void frac_to_cart(const PmeGeom &g, const double *f, 
 }
 
 // common lane setup of the gather kernels
-#define GATHER_SETUP(ND)                                                                                     \
+#define GATHER_SETUP(ND, T)                                                                                  \
     constexpr int G = PmeLanes<ORD>::G;                                                                      \
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;                                                   \
     const int slot = tid / G, l = tid % G;                                                                   \
@@ -603,31 +619,31 @@ __device__ __forceinline__ void frac_to_cart(const PmeGeom &g, const double *f, 
     double w[3];                                                                                             \
     int i0[3];                                                                                               \
     atom_frac(a.g, a.sx[s], a.sy[s], a.sz[s], w, i0, ORD);                                                   \
-    double tx[PMQ_MAX_ORDER][4], tyf[PMQ_MAX_ORDER][4], tzf[PMQ_MAX_ORDER][4], ty[4], tz[4];                 \
-    bspline_weights<double, ND>(w[0], ORD, tx);                                                              \
-    bspline_weights<double, ND>(w[1], ORD, tyf);                                                             \
-    bspline_weights<double, ND>(w[2], ORD, tzf);                                                             \
-    pick<ORD, ND>(tyf, ky, ty);                                                                              \
-    pick<ORD, ND>(tzf, kz, tz);
+    T tx[PMQ_MAX_ORDER][4], tyf[PMQ_MAX_ORDER][4], tzf[PMQ_MAX_ORDER][4], ty[4], tz[4];                      \
+    bspline_weights<T, ND>(T(w[0]), ORD, tx);                                                                \
+    bspline_weights<T, ND>(T(w[1]), ORD, tyf);                                                               \
+    bspline_weights<T, ND>(T(w[2]), ORD, tzf);                                                               \
+    pick<ORD, ND, T>(tyf, ky, ty);                                                                           \
+    pick<ORD, ND, T>(tzf, kz, tz);
 
 struct GatherFracArgs {
     int n, stride, nset;
     PmeGeom g;
     const double *sx, *sy, *sz;
-    const double *grid;    // nset grids
+    const void *grid;      // nset grids (double, or float in the mixed form)
     size_t gstride;
     double *frac;          // n x nset x 20 fractional derivatives, sorted atom order
 };
 
 // the 20 fractional potential derivatives of every atom in each of nset
 // grids; lane j < 20 of the group stores component j
-template <int ORD>
+template <int ORD, class T, class GT>
 __global__ __launch_bounds__(256) void k_pme_gather_frac(GatherFracArgs a)
 {
-    GATHER_SETUP(3)
+    GATHER_SETUP(3, T)
     for (int set = 0; set < a.nset; set++) {
         double fp[20];
-        gather_part<ORD, 20>(a.g, a.grid + set * a.gstride, tx, ty, tz, i0, ky, kz, on, fp);
+        gather_part<ORD, 20, T, GT>(a.g, (const GT *)a.grid + set * a.gstride, tx, ty, tz, i0, ky, kz, on, fp);
         group_reduce<G, 20>(fp);
         double v = 0.0;
 #pragma unroll
@@ -718,7 +734,7 @@ struct GatherFieldArgs {
     int n, npad, stride, nset;
     PmeGeom g;
     const double *sx, *sy, *sz;
-    const double *grid;
+    const void *grid;   // double, or float in the mixed form
     size_t gstride;
     const double *dip;  // nset x 3 x npad dipoles (for the self term)
     double self;        // 4/3 a^3 / sqrt(pi)
@@ -726,13 +742,13 @@ struct GatherFieldArgs {
 };
 
 // reciprocal + self field of the induced-dipole sets at every site
-template <int ORD>
+template <int ORD, class T, class GT>
 __global__ __launch_bounds__(256) void k_pme_gather_field(GatherFieldArgs a)
 {
-    GATHER_SETUP(1)
+    GATHER_SETUP(1, T)
     for (int set = 0; set < a.nset; set++) {
         double fp[4];
-        gather_part<ORD, 4>(a.g, a.grid + set * a.gstride, tx, ty, tz, i0, ky, kz, on, fp);
+        gather_part<ORD, 4, T, GT>(a.g, (const GT *)a.grid + set * a.gstride, tx, ty, tz, i0, ky, kz, on, fp);
         group_reduce<G, 4>(fp);
         if (valid && l == 0) {
             double phi, gr[3];
@@ -816,26 +832,39 @@ __global__ __launch_bounds__(256) void k_pme_polgrad_post(PolGradPostArgs a)
 
 // ---------------------------------------------------------------------------
 // launchers
-static int fft(pmq_ctx *c, bool forward, int nset)
+static int fft(pmq_ctx *c, bool forward, int nset, bool fp32)
 {
-    rocfft_plan pl = (rocfft_plan)(forward ? (nset == 1 ? c->plan_f1 : c->plan_f2) : (nset == 1 ? c->plan_b1 : c->plan_b2));
-    void *in[1] = {forward ? (void *)c->qgrid.p : (void *)c->cgrid.p};
+    rocfft_plan pl = fp32 ? (rocfft_plan)(forward ? (nset == 1 ? c->plan_f1s : c->plan_f2s)
+                                                  : (nset == 1 ? c->plan_b1s : c->plan_b2s))
+                          : (rocfft_plan)(forward ? (nset == 1 ? c->plan_f1 : c->plan_f2)
+                                                  : (nset == 1 ? c->plan_b1 : c->plan_b2));
+    void *in[1] = {forward ? (void *)c->qgrid.p : (void *)c->cgrid.p}; // read as float by the FP32 plans
     void *out[1] = {forward ? (void *)c->cgrid.p : (void *)c->qgrid.p};
     ProfScope ps_(c, "pme_fft");
     return fft_check(c, rocfft_execute(pl, in, out, (rocfft_execution_info)c->fft_info), "execute");
 }
 
+// FP32 grids and FFTs: the mixed form when its timing chose them
+static bool grid32(const pmq_ctx *c) { return c->pme_fp32 && c->pme_fft32 == 1; }
+
+// G: a kernel table of the plain form (null = the electrostatic one)
 static int pme_potential(pmq_ctx *c, int nset, const double *G = nullptr)
 {
     int rc;
-    if ((rc = fft(c, true, nset))) return rc;
+    const bool fp32 = grid32(c) && !G;
+    if ((rc = fft(c, true, nset, fp32))) return rc;
     const int H0 = c->K[0] / 2 + 1;
     const int ncplx = H0 * c->K[1] * c->K[2];
     const size_t tot = (size_t)ncplx * nset;
-    hipLaunchKernelGGL(k_pme_convolve, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, ncplx, nset,
-                       G ? G : c->pme_g.p, c->cgrid.p);
+    const dim3 grid((unsigned)((tot + 255) / 256));
+    if (fp32)
+        hipLaunchKernelGGL(k_pme_convolve<float>, grid, dim3(256), 0, c->stream, ncplx, nset, c->pme_g_s.p,
+                           (float *)c->cgrid.p);
+    else
+        hipLaunchKernelGGL(k_pme_convolve<double>, grid, dim3(256), 0, c->stream, ncplx, nset, G ? G : c->pme_g.p,
+                           c->cgrid.p);
     PMQ_HIP(hipGetLastError());
-    return fft(c, false, nset);
+    return fft(c, false, nset, fp32);
 }
 
 static size_t grid_size(const pmq_ctx *c) { return (size_t)c->K[0] * c->K[1] * c->K[2]; }
@@ -881,8 +910,11 @@ static int pme_bin(pmq_ctx *c)
     return PMQ_OK;
 }
 
+// FP32 grids for the electrostatic modes of the mixed form; dispersion
+// (mode 2) stays FP64
 static int spread(pmq_ctx *c, int mode, const double *src, const double *charge, int nset)
 {
+    const bool fp32 = c->pme_fp32 && mode != 2, g32 = grid32(c) && mode != 2;
     BrickArgs args;
     args.n = c->n; args.sbits = c->pme_sbits; args.nset = nset;
     for (int a = 0; a < 3; a++) args.nb[a] = c->pme_nb[a];
@@ -900,7 +932,12 @@ static int spread(pmq_ctx *c, int mode, const double *src, const double *charge,
     }
     const dim3 grid((unsigned)(c->pme_nb[0] * c->pme_nb[1] * c->pme_nb[2]));
     ProfScope ps_(c, "pme_spread");
-#define PMQ_SPREAD(ORD, MODE) hipLaunchKernelGGL((k_pme_spread_brick<ORD, MODE>), grid, dim3(256), 0, c->stream, args)
+#define PMQ_SPREAD(ORD, MODE)                                                                                  \
+    do {                                                                                                       \
+        if (g32) hipLaunchKernelGGL((k_pme_spread_brick<ORD, MODE, float, float>), grid, dim3(256), 0, c->stream, args); \
+        else if (fp32) hipLaunchKernelGGL((k_pme_spread_brick<ORD, MODE, float, double>), grid, dim3(256), 0, c->stream, args); \
+        else hipLaunchKernelGGL((k_pme_spread_brick<ORD, MODE, double, double>), grid, dim3(256), 0, c->stream, args); \
+    } while (0)
     switch (c->bsorder * 3 + mode) {
     case 12: PMQ_SPREAD(4, 0); break;
     case 13: PMQ_SPREAD(4, 1); break;
@@ -927,14 +964,44 @@ static int gather_frac(pmq_ctx *c, int nset)
     args.gstride = grid_size(c);
     args.frac = c->pme_frac.p;
     ProfScope ps_(c, "pme_gather");
-    PMQ_ORDER_DISPATCH(k_pme_gather_frac, )
+    if (grid32(c)) { PMQ_ORDER_DISPATCH(k_pme_gather_frac, , float, float) }
+    else if (c->pme_fp32) { PMQ_ORDER_DISPATCH(k_pme_gather_frac, , float, double) }
+    else { PMQ_ORDER_DISPATCH(k_pme_gather_frac, , double, double) }
     PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+// the mixed form's FFT precision: rocFFT's FP32 transforms are not faster
+// than its FP64 ones at every grid, so both are timed once per grid (three
+// forward + inverse pairs of the two-set transform each) and the faster
+// kept (FP32 wins ties)
+static int choose_fft_precision(pmq_ctx *c)
+{
+    if (!c->pme_fp32 || c->pme_fft32 >= 0) return PMQ_OK;
+    hipEvent_t e[3];
+    for (auto &x : e) PMQ_HIP(hipEventCreate(&x));
+    float ms[2] = {0, 0};
+    int rc = PMQ_OK;
+    for (int prec = 0; prec < 2 && !rc; prec++) {
+        for (int it = 0; it < 4 && !rc; it++) {
+            if (it == 1) PMQ_HIP(hipEventRecord(e[prec], c->stream));
+            if ((rc = fft(c, true, 2, prec == 1))) break;
+            rc = fft(c, false, 2, prec == 1);
+        }
+        PMQ_HIP(hipEventRecord(e[2], c->stream));
+        PMQ_HIP(hipEventSynchronize(e[2]));
+        PMQ_HIP(hipEventElapsedTime(&ms[prec], e[prec], e[2]));
+    }
+    for (auto &x : e) (void)hipEventDestroy(x);
+    if (rc) return rc;
+    c->pme_fft32 = ms[1] <= ms[0] ? 1 : 0;
     return PMQ_OK;
 }
 
 int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar)
 {
     int rc;
+    if ((rc = choose_fft_precision(c))) return rc;
     if ((rc = pme_bin(c))) return rc;
     const double *charge = c->prm.pentype != PMQ_PEN_NONE ? c->scp.p + 2 * (size_t)c->npad : nullptr;
     if ((rc = spread(c, 0, c->smp.p, charge, 1))) return rc;
@@ -977,7 +1044,9 @@ int pme_dipole_field(pmq_ctx *c, const double *dip, double *field)
     args.field = field;
     {
         ProfScope ps_(c, "pme_gather");
-        PMQ_ORDER_DISPATCH(k_pme_gather_field, )
+        if (grid32(c)) { PMQ_ORDER_DISPATCH(k_pme_gather_field, , float, float) }
+        else if (c->pme_fp32) { PMQ_ORDER_DISPATCH(k_pme_gather_field, , float, double) }
+        else { PMQ_ORDER_DISPATCH(k_pme_gather_field, , double, double) }
     }
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
@@ -1001,9 +1070,9 @@ struct DispPostArgs {
 template <int ORD>
 __global__ __launch_bounds__(256) void k_pme_disp_gather(DispPostArgs a)
 {
-    GATHER_SETUP(1)
+    GATHER_SETUP(1, double)
     double fp[4];
-    gather_part<ORD, 4>(a.g, a.grid, tx, ty, tz, i0, ky, kz, on, fp);
+    gather_part<ORD, 4, double, double>(a.g, a.grid, tx, ty, tz, i0, ky, kz, on, fp);
     group_reduce<G, 4>(fp);
     double e = 0.0;
     if (valid && l == 0) {

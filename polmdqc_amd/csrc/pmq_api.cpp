@@ -636,6 +636,7 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
             if ((rc = launch_fx_flush(c, c->fx_grad.p, c->dem.p, 3 * (size_t)n, PMQ_FX_GRAD))) return rc;
             if ((rc = launch_fx_flush(c, c->fx_grad.p + 3 * (size_t)n, c->te.p, 3 * (size_t)n, PMQ_FX_GRAD))) return rc;
         }
+        c->pme_fp32 = mixed; // electrostatic grids in FP32 for the mixed form
         if ((ewald && (do_mpole || do_polar)) || dewald) {
             if ((rc = pme_setup(c))) return rc;
             if (ewald && (do_mpole || do_polar) && (rc = pme_mpole(c, mode, do_mpole, do_polar))) return rc;

@@ -316,6 +316,12 @@ struct pmq_ctx {
     pmq::DBuf<int> pme_bstart;                   // first key of each brick
     pmq::DBuf<unsigned char> fft_work;
     void *plan_f1 = nullptr, *plan_b1 = nullptr, *plan_f2 = nullptr, *plan_b2 = nullptr;
+    // FP32 plans, kernel table and flag of the mixed form's electrostatic PME
+    // (grids in the same buffers, read as float)
+    void *plan_f1s = nullptr, *plan_b1s = nullptr, *plan_f2s = nullptr, *plan_b2s = nullptr;
+    pmq::DBuf<float> pme_g_s;
+    bool pme_fp32 = false;   // FP32 weights and sums in spreading and gathering (mixed form)
+    int pme_fft32 = -1;      // mixed form: FP32 grids and FFTs (1) or FP64 (0), timed once per grid (-1)
     void *fft_info = nullptr;
     pmq::DevBox pme_dbox{};                       // box the kernel G was built for
     double pme_alpha = 0.0;

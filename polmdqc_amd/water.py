@@ -51,12 +51,18 @@ def random_rotations(rng: np.random.Generator, count: int) -> np.ndarray:
     return r
 
 
-def lattice_box(k: int, L: float, seed: int) -> np.ndarray:
-    """Coordinates (3*k^3, 3) of the seeded lattice water box, O,H,H per molecule."""
+def lattice_box(k: int, L: float, seed: int, nmol: int | None = None) -> np.ndarray:
+    """Coordinates (3*nmol, 3) of the seeded lattice water box, O,H,H per
+    molecule.  nmol defaults to k^3 (every site); a smaller count leaves
+    k^3 - nmol sites empty, chosen by the seed (the 32,000-molecule HIPPO box
+    on a 32^3 lattice at 1 g/cm^3)."""
     rng = np.random.default_rng(seed)
-    nmol = k ** 3
     g = (np.arange(k) + 0.5) * (L / k)
-    ox = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(nmol, 3)
+    ox = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(k ** 3, 3)
+    if nmol is not None and nmol < k ** 3:
+        keep = np.sort(rng.choice(k ** 3, size=nmol, replace=False))
+        ox = ox[keep]
+    nmol = len(ox)
     rot = random_rotations(rng, nmol)
     jitter = rng.uniform(-0.1, 0.1, size=(nmol, 3))
     mol = np.einsum("mij,aj->mai", rot, _molecule_frame())
