@@ -510,18 +510,31 @@ __global__ void k_pol_sort(int n, int npad, const int *perm, const double *pol, 
 // device too, so iterations are enqueued in batches without host round trips;
 // once `done` is set the remaining kernels of a batch return at once.
 //
-// Scalars sc[]: 0..1 r.z, 2..3 alpha, 4..5 beta, 6..7 r.r, 8 done, 9 iterations
-// at convergence, 10 epsilon of the last iteration.
+// One iteration is three kernels: k_pcg_a (v = M p, p.v partials), k_pcg_b
+// (every block first sums the p.v partials itself for the step lengths, then
+// updates u, r, z and writes r.z / r.r partials to a second buffer) and
+// k_pcg_c (every block sums those for beta and the convergence test, then
+// updates p).  Each block adds the same partials in the same order, so all
+// agree bit for bit; r.z alternates between two slots by iteration parity so
+// block 0 can store the new value while the others still read the old one.
+//
+// Scalars sc[]: 0..1 r.z (even iterations), 11..12 r.z (odd), 4..5 beta,
+// 6..7 r.r, 8 done, 9 iterations at convergence, 10 epsilon of the last
+// iteration.
 struct PcgArgs {
     int npad, nblk;
     const double *spol;
     const double *E, *Tu; // fields
     double *u, *r, *z, *p, *v;
-    double *part;         // nblk x 4 partial sums
+    double *part;         // nblk x 4 partial sums of k_pcg_a (and of the initial residual)
+    double *part2;        // nblk x 4 partial sums of k_pcg_b
     double *sc;
     double npolar, poleps;
     int iter;
 };
+
+// slot of r.z after iteration k (k = 0: the initial residual)
+__device__ __forceinline__ int rz_slot(int k) { return (k & 1) ? 11 : 0; }
 
 constexpr int PCG_TB = 256;
 
@@ -592,7 +605,7 @@ __global__ __launch_bounds__(PCG_TB) void k_pcg_init_fin(PcgArgs a)
     sum_parts(a.part, a.nblk, tot);
     if (threadIdx.x == 0) {
         a.sc[0] = tot[0]; a.sc[1] = tot[1];
-        for (int q = 2; q < 11; q++) a.sc[q] = 0.0;
+        for (int q = 2; q < 16; q++) a.sc[q] = 0.0;
     }
 }
 
@@ -612,26 +625,21 @@ __global__ __launch_bounds__(PCG_TB) void k_pcg_a(PcgArgs a)
     block_part4(s0, s1, 0.0, 0.0, a.part);
 }
 
-// step lengths alpha = r.z / p.v
-__global__ __launch_bounds__(PCG_TB) void k_pcg_a_fin(PcgArgs a)
-{
-    if (a.sc[8] != 0.0) return;
-    double tot[4];
-    sum_parts(a.part, a.nblk, tot);
-    if (threadIdx.x == 0)
-        for (int set = 0; set < 2; set++) a.sc[2 + set] = tot[set] != 0.0 ? a.sc[set] / tot[set] : 0.0;
-}
-
-// u += alpha p, r -= alpha v, z = M r; new r.z and r.r
+// alpha = r.z / p.v (each block from the partials), u += alpha p,
+// r -= alpha v, z = M r; new r.z and r.r partials
 __global__ __launch_bounds__(PCG_TB) void k_pcg_b(PcgArgs a)
 {
     if (a.sc[8] != 0.0) return;
+    double pv[4];
+    sum_parts(a.part, a.nblk, pv);
+    const int so = rz_slot(a.iter - 1);
+    const double alpha[2] = {pv[0] != 0.0 ? a.sc[so] / pv[0] : 0.0, pv[1] != 0.0 ? a.sc[so + 1] / pv[1] : 0.0};
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double s0 = 0.0, s1 = 0.0, q0 = 0.0, q1 = 0.0;
     if (t < 6 * a.npad) {
         const int set = t < 3 * a.npad ? 0 : 1;
         const int s = t % a.npad;
-        const double al = a.sc[2 + set];
+        const double al = alpha[set];
         a.u[t] += al * a.p[t];
         const double r = a.r[t] - al * a.v[t];
         const double z = a.spol[s] * r;
@@ -640,39 +648,36 @@ __global__ __launch_bounds__(PCG_TB) void k_pcg_b(PcgArgs a)
         (set == 0 ? s0 : s1) = r * z;
         (set == 0 ? q0 : q1) = r * r;
     }
-    block_part4(s0, s1, q0, q1, a.part);
+    block_part4(s0, s1, q0, q1, a.part2);
 }
 
-// beta = new r.z / r.z, the convergence test debye sqrt(max r.r / npolar) < poleps
-__global__ __launch_bounds__(PCG_TB) void k_pcg_b_fin(PcgArgs a)
+// beta = new r.z / r.z and the convergence test debye sqrt(max r.r / npolar)
+// < poleps (each block from the partials; block 0 stores the scalars), then
+// p = z + beta p unless converged
+__global__ __launch_bounds__(PCG_TB) void k_pcg_c(PcgArgs a)
 {
     if (a.sc[8] != 0.0) return;
     double tot[4];
-    sum_parts(a.part, a.nblk, tot);
-    if (threadIdx.x == 0) {
-        const double debye = 4.80321;
-        const double eps = debye * sqrt(fmax(tot[2], tot[3]) / a.npolar);
+    sum_parts(a.part2, a.nblk, tot);
+    const int so = rz_slot(a.iter - 1), sn = rz_slot(a.iter);
+    const double beta[2] = {a.sc[so] != 0.0 ? tot[0] / a.sc[so] : 0.0, a.sc[so + 1] != 0.0 ? tot[1] / a.sc[so + 1] : 0.0};
+    const double eps = 4.80321 * sqrt(fmax(tot[2], tot[3]) / a.npolar); // debye x rms residual
+    const bool done = eps < a.poleps;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.sc[10] = eps;
         a.sc[6] = tot[2]; a.sc[7] = tot[3];
-        for (int set = 0; set < 2; set++) {
-            a.sc[4 + set] = a.sc[set] != 0.0 ? tot[set] / a.sc[set] : 0.0;
-            a.sc[set] = tot[set];
-        }
-        if (eps < a.poleps) {
+        a.sc[4] = beta[0]; a.sc[5] = beta[1];
+        a.sc[sn] = tot[0]; a.sc[sn + 1] = tot[1];
+        if (done) {
             a.sc[8] = 1.0;
             a.sc[9] = (double)a.iter;
         }
     }
-}
-
-// p = z + beta p
-__global__ __launch_bounds__(PCG_TB) void k_pcg_c(PcgArgs a)
-{
-    if (a.sc[8] != 0.0) return;
+    if (done) return; // p is not used again
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 6 * a.npad) return;
     const int set = t < 3 * a.npad ? 0 : 1;
-    a.p[t] = a.z[t] + a.sc[4 + set] * a.p[t];
+    a.p[t] = a.z[t] + beta[set] * a.p[t];
 }
 
 __device__ __forceinline__ void block_add2(double a0, double a1, double *dst)
@@ -801,12 +806,13 @@ int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
     const int np = c->npad;
     const size_t nv = 6 * (size_t)np;
     const int gb = (int)((nv + PCG_TB - 1) / PCG_TB);
-    PMQ_HIP(c->pcg_part.alloc(4 * (size_t)gb));
+    PMQ_HIP(c->pcg_part.alloc(8 * (size_t)gb));
     PcgArgs a;
     a.npad = np; a.nblk = gb; a.spol = c->spol.p;
     a.E = c->pfield.p; a.Tu = c->pvec.p;
     a.u = c->uind.p; a.r = c->pres.p; a.z = c->pz.p; a.p = c->pdir.p; a.v = c->pvec.p;
     a.part = c->pcg_part.p;
+    a.part2 = c->pcg_part.p + 4 * (size_t)gb;
     a.sc = c->pcg.p;
     int npolar = 0;
     for (double p : c->h_polarity) npolar += p != 0.0;
@@ -830,9 +836,7 @@ int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
             a.iter = ++it;
             if ((rc = mutual_field(c, c->pdir.p, c->pvec.p))) return rc;
             hipLaunchKernelGGL(k_pcg_a, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
-            hipLaunchKernelGGL(k_pcg_a_fin, dim3(1), dim3(PCG_TB), 0, c->stream, a);
             hipLaunchKernelGGL(k_pcg_b, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
-            hipLaunchKernelGGL(k_pcg_b_fin, dim3(1), dim3(PCG_TB), 0, c->stream, a);
             hipLaunchKernelGGL(k_pcg_c, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
             PMQ_HIP(hipGetLastError());
         }
