@@ -174,7 +174,8 @@ class ElecSys(C.Structure):
                                           "p4iscale", "p5iscale", "d1scale", "d2scale", "d3scale", "d4scale",
                                           "u1scale", "u2scale", "u3scale", "u4scale", "poleps")] + \
                [("politer", C.c_int), ("solver", C.c_int)] + \
-               [(k, C.c_double) for k in ("w2scale", "w3scale", "w4scale", "w5scale")] + [("dpequal", C.c_int)]
+               [(k, C.c_double) for k in ("w2scale", "w3scale", "w4scale", "w5scale")] + [("dpequal", C.c_int)] + \
+               [("pcgprec", C.c_int), ("usolvcut", C.c_double), ("udiag", C.c_double)]
 
 
 class ElecOut(C.Structure):
@@ -217,6 +218,11 @@ def elec(fx, want_grad: bool = True, solver: int = 0, x=None, poleps: float | No
     for k in ("w2scale", "w3scale", "w4scale", "w5scale"):
         setattr(s, k, float(fx.get(k, 1.0)))
     s.dpequal = int(fx.get("dpequal", 0))
+    # PCG preconditioner: off by default, as in the GPU solver (diagonal
+    # preconditioner); on, usolvcut 4.5 A as cutoffs.cpp:61
+    s.pcgprec = int(fx.get("pcgprec", 0))
+    s.usolvcut = float(fx.get("usolvcut", 4.5))
+    s.udiag = float(fx.get("udiag", 2.0))
     o = ElecOut()
     dem, dep, ud, up = (np.zeros(3 * n) for _ in range(4))
     o.dem, o.dep = dem.ctypes.data_as(_dp), dep.ctypes.data_as(_dp)

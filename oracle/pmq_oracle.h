@@ -120,6 +120,13 @@ typedef struct {
      * damping and the w scales (1-2..1-5), d = p when dpequal */
     double w2scale, w3scale, w4scale, w5scale;
     int dpequal;
+    /* PCG preconditioner (pcgprec of polpcg.h; usolvcut 4.5 A at
+     * cutoffs.cpp:61): z = udiag a r + a T_s (a r), T_s the dipole field of
+     * the pairs within usolvcut, damped and scaled as the mutual field,
+     * without Ewald; pcgprec = 0 gives the diagonal z = a r (the GPU
+     * solver's choice) */
+    int pcgprec;
+    double usolvcut, udiag;
 } orc_elec_sys;
 
 typedef struct {

@@ -768,6 +768,48 @@ static void mutual_field(ctx_t *c, const pairrec *L, long np, const double *ud, 
  * ------------------------------------------------------------------- */
 #define DEBYE 4.80321
 
+/* the preconditioner z = M r for both sets: udiag a r + a T_s (a r) with T_s
+ * the real-space dipole field (no Ewald) of the pairs within usolvcut,
+ * damped and scaled like the mutual field; without pcgprec z = a r */
+static void precondition(ctx_t *c, const pairrec *L, long np, const double *poli, const double *rd,
+                         const double *rp, double *zd, double *zp)
+{
+    const orc_elec_sys *s = c->s;
+    const int n = c->n, n3 = 3 * n;
+    if (!s->pcgprec) {
+        for (int j = 0; j < n3; j++) { zd[j] = poli[j / 3] * rd[j]; zp[j] = poli[j / 3] * rp[j]; }
+        return;
+    }
+    double *wd = malloc(n3 * sizeof(double)), *wp = malloc(n3 * sizeof(double));
+    double *fd = calloc(n3, sizeof(double)), *fp = calloc(n3, sizeof(double));
+    for (int j = 0; j < n3; j++) { wd[j] = poli[j / 3] * rd[j]; wp[j] = poli[j / 3] * rp[j]; }
+    const double us2 = s->usolvcut * s->usolvcut;
+    const int ew = c->ewald;
+    c->ewald = 0; /* the series without Ewald */
+    for (long t = 0; t < np; t++) {
+        const pairrec *q = &L[t];
+        if (q->r2 > us2) continue;
+        double R[6];
+        pol_series(c, q, c->cp ? q->w : q->u, R);
+        for (int set = 0; set < 2; set++) {
+            const double *u = set ? wp : wd;
+            double *F = set ? fp : fd;
+            const double *ui = u + 3 * q->i, *uk = u + 3 * q->k;
+            const double uir = dot3(ui, q->r), ukr = dot3(uk, q->r);
+            for (int a = 0; a < 3; a++) {
+                F[3 * q->i + a] += ukr * R[2] * q->r[a] - R[1] * uk[a];
+                F[3 * q->k + a] += uir * R[2] * q->r[a] - R[1] * ui[a];
+            }
+        }
+    }
+    c->ewald = ew;
+    for (int j = 0; j < n3; j++) {
+        zd[j] = s->udiag * wd[j] + poli[j / 3] * fd[j];
+        zp[j] = s->udiag * wp[j] + poli[j / 3] * fp[j];
+    }
+    free(wd); free(wp); free(fd); free(fp);
+}
+
 /* preconditioned conjugate gradient on (1/alpha - T) mu = E for both sets,
  * Jacobi preconditioner, stop when the rms residual (Debye) < poleps */
 static int solve_pcg(ctx_t *c, const pairrec *L, long np, const double *Ed, const double *Ep, double *ud, double *up)
@@ -796,11 +838,9 @@ static int solve_pcg(ctx_t *c, const pairrec *L, long np, const double *Ed, cons
         const double pol = s->polarity[i];
         rd[j] = pol != 0.0 ? Ed[j] + Fd[j] - ud[j] / poli[i] : 0.0;
         rp_[j] = pol != 0.0 ? Ep[j] + Fp[j] - up[j] / poli[i] : 0.0;
-        zd[j] = poli[i] * rd[j];
-        zp[j] = poli[i] * rp_[j];
-        pd[j] = zd[j];
-        pp[j] = zp[j];
     }
+    precondition(c, L, np, poli, rd, rp_, zd, zp);
+    for (int j = 0; j < n3; j++) { pd[j] = zd[j]; pp[j] = zp[j]; }
     double sd = 0, sp = 0;
     for (int j = 0; j < n3; j++) { sd += rd[j] * zd[j]; sp += rp_[j] * zp[j]; }
     int iter = 0, done = 0;
@@ -820,13 +860,13 @@ static int solve_pcg(ctx_t *c, const pairrec *L, long np, const double *Ed, cons
         ap = ap != 0.0 ? sp / ap : 0.0;
         double sd2 = 0, sp2 = 0, epsd = 0, epsp = 0;
         for (int j = 0; j < n3; j++) {
-            const int i = j / 3;
             ud[j] += ad * pd[j];
             up[j] += ap * pp[j];
             rd[j] -= ad * Fd[j];
             rp_[j] -= ap * Fp[j];
-            zd[j] = poli[i] * rd[j];
-            zp[j] = poli[i] * rp_[j];
+        }
+        precondition(c, L, np, poli, rd, rp_, zd, zp);
+        for (int j = 0; j < n3; j++) {
             sd2 += rd[j] * zd[j];
             sp2 += rp_[j] * zp[j];
             epsd += rd[j] * rd[j];

@@ -28,6 +28,7 @@
 // pairs whole, in FP64, in k_*_excl.
 #include "pair_mpole.h"
 #include "pmq_internal.h"
+#include "tile_pairs.h"
 
 namespace pmq {
 
@@ -112,46 +113,55 @@ __device__ __forceinline__ void disp_pair(T r2, T ci, T ck, T ai, T ak, T s, T d
 template <bool EW, class T>
 __global__ __launch_bounds__(64) void k_disp_tiles(HippoArgs a)
 {
-    __shared__ double jx[64], jy[64], jz[64];
-    __shared__ T jc[64], ja[64], acc[3][64];
+    // side 0: block I, side 1: block J; compacted pair schedule (tile_pairs.h)
+    __shared__ double bx[2][64], by[2][64], bz[2][64];
+    __shared__ T bc[2][64], ba[2][64];
+    __shared__ double acc[2][3][64];
+    __shared__ unsigned short plist[64 * 64];
     const int lane = threadIdx.x, np = a.npad;
     const int ntiles = *a.ntiles;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 IJ = a.tiles[t];
-        const int si = IJ.x * 64 + lane, sj = IJ.y * 64 + lane;
         __syncthreads();
-        jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
-        jc[lane] = a.shp[sj];
-        ja[lane] = a.shp[np + sj];
-        acc[0][lane] = acc[1][lane] = acc[2][lane] = T(0);
-        const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
-        const T ci = a.shp[si], ai = a.shp[np + si];
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const int s = (side ? IJ.y : IJ.x) * 64 + lane;
+            bx[side][lane] = a.sx[s]; by[side][lane] = a.sy[s]; bz[side][lane] = a.sz[s];
+            bc[side][lane] = a.shp[s];
+            ba[side][lane] = a.shp[np + s];
+            acc[side][0][lane] = acc[side][1][lane] = acc[side][2][lane] = 0.0;
+        }
+        const int npair = tile_pair_list(a.mask[(size_t)t * 128 + lane], plist, lane);
         __syncthreads();
-        unsigned long long mask = a.mask[(size_t)t * 128 + lane];
         double e = 0.0;
-        T fx = 0, fy = 0, fz = 0;
-        while (mask) {
-            const int jj = __ffsll((long long)mask) - 1;
-            mask &= mask - 1;
-            double xd = jx[jj] - xi, yd = jy[jj] - yi, zd = jz[jj] - zi;
+        for (int base = 0; base < npair; base += 64) {
+            if (base + lane >= npair) break;
+            const int ent = plist[base + lane];
+            const int il = ent >> 8, jl = ent & 63;
+            double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
             image<double>(a.box, xd, yd, zd);
             const double r2d = xd * xd + yd * yd + zd * zd;
-            if (r2d > a.off2 || ci == T(0) || jc[jj] == T(0)) continue;
+            const T ci = bc[0][il], ck = bc[1][jl];
+            if (r2d > a.off2 || ci == T(0) || ck == T(0)) continue;
             const T xr = T(xd), yr = T(yd), zr = T(zd), r2 = T(r2d);
             T ep, de;
-            disp_pair<T, EW>(r2, ci, jc[jj], ai, ja[jj], T(1), T(a.dewald), a.taper, ep, de);
+            disp_pair<T, EW>(r2, ci, ck, ba[0][il], ba[1][jl], T(1), T(a.dewald), a.taper, ep, de);
             e += (double)ep;
-            const T g = de / sqrt(r2); // dE/dx_k = g r_vec
-            fx -= g * xr; fy -= g * yr; fz -= g * zr;
-            atomicAdd(&acc[0][jj], g * xr);
-            atomicAdd(&acc[1][jj], g * yr);
-            atomicAdd(&acc[2][jj], g * zr);
+            const double g = (double)(de / sqrt(r2)); // dE/dx_k = g r_vec
+            atomicAdd(&acc[0][0][il], -g * (double)xr);
+            atomicAdd(&acc[0][1][il], -g * (double)yr);
+            atomicAdd(&acc[0][2][il], -g * (double)zr);
+            atomicAdd(&acc[1][0][jl], g * (double)xr);
+            atomicAdd(&acc[1][1][jl], g * (double)yr);
+            atomicAdd(&acc[1][2][jl], g * (double)zr);
         }
         __syncthreads();
         T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
         T *Q = P + PMQ_TILE_NV * 64;
-        P[lane] = fx; P[64 + lane] = fy; P[128 + lane] = fz;
-        Q[lane] = acc[0][lane]; Q[64 + lane] = acc[1][lane]; Q[128 + lane] = acc[2][lane];
+        for (int c = 0; c < 3; c++) {
+            P[c * 64 + lane] = T(acc[0][c][lane]);
+            Q[c * 64 + lane] = T(acc[1][c][lane]);
+        }
         const double ew = wsum(e);
         if (lane == 0) a.tscal[(size_t)t * PMQ_TILE_NS] = ew;
     }
@@ -222,62 +232,64 @@ __device__ __forceinline__ Mp<T> rep_mp(const double *smp, const double *shp, in
 template <class T>
 __global__ __launch_bounds__(64) void k_rep_tiles(HippoArgs a)
 {
-    __shared__ double jx[64], jy[64], jz[64];
-    __shared__ T jm[10][64], jsz[64], jdm[64], acc[6][64];
+    // side 0: block I, side 1: block J; compacted pair schedule (tile_pairs.h)
+    __shared__ double bx[2][64], by[2][64], bz[2][64];
+    __shared__ T bm[2][10][64], bsz[2][64], bdm[2][64];
+    __shared__ double acc[2][6][64];
+    __shared__ unsigned short plist[64 * 64];
     const int lane = threadIdx.x, np = a.npad;
     const int ntiles = *a.ntiles;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 IJ = a.tiles[t];
-        const int si = IJ.x * 64 + lane, sj = IJ.y * 64 + lane;
         __syncthreads();
-        jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
-        {
-            const Mp<T> m = rep_mp<T>(a.smp, a.shp, np, sj);
-            jm[0][lane] = m.c; jm[1][lane] = m.dx; jm[2][lane] = m.dy; jm[3][lane] = m.dz;
-            jm[4][lane] = m.qxx; jm[5][lane] = m.qxy; jm[6][lane] = m.qxz; jm[7][lane] = m.qyy;
-            jm[8][lane] = m.qyz; jm[9][lane] = m.qzz;
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const int s = (side ? IJ.y : IJ.x) * 64 + lane;
+            bx[side][lane] = a.sx[s]; by[side][lane] = a.sy[s]; bz[side][lane] = a.sz[s];
+            const Mp<T> m = rep_mp<T>(a.smp, a.shp, np, s);
+            bm[side][0][lane] = m.c; bm[side][1][lane] = m.dx; bm[side][2][lane] = m.dy; bm[side][3][lane] = m.dz;
+            bm[side][4][lane] = m.qxx; bm[side][5][lane] = m.qxy; bm[side][6][lane] = m.qxz;
+            bm[side][7][lane] = m.qyy; bm[side][8][lane] = m.qyz; bm[side][9][lane] = m.qzz;
+            bsz[side][lane] = a.shp[2 * np + s];
+            bdm[side][lane] = a.shp[3 * np + s];
+            for (int c = 0; c < 6; c++) acc[side][c][lane] = 0.0;
         }
-        jsz[lane] = a.shp[2 * np + sj];
-        jdm[lane] = a.shp[3 * np + sj];
-        for (int c = 0; c < 6; c++) acc[c][lane] = T(0);
-        const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
-        const Mp<T> mi = rep_mp<T>(a.smp, a.shp, np, si);
-        const T szi = a.shp[2 * np + si], dmi = a.shp[3 * np + si];
+        const int npair = tile_pair_list(a.mask[(size_t)t * 128 + lane], plist, lane);
         __syncthreads();
-        unsigned long long mask = a.mask[(size_t)t * 128 + lane];
         double e = 0.0;
-        T F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0};
-        while (mask) {
-            const int jj = __ffsll((long long)mask) - 1;
-            mask &= mask - 1;
-            double xd = jx[jj] - xi, yd = jy[jj] - yi, zd = jz[jj] - zi;
+        for (int base = 0; base < npair; base += 64) {
+            if (base + lane >= npair) break;
+            const int ent = plist[base + lane];
+            const int il = ent >> 8, jl = ent & 63;
+            double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
             image<double>(a.box, xd, yd, zd);
             const double r2d = xd * xd + yd * yd + zd * zd;
-            if (r2d > a.off2 || szi == T(0) || jsz[jj] == T(0)) continue;
+            if (r2d > a.off2 || bsz[0][il] == T(0) || bsz[1][jl] == T(0)) continue;
             const T xr = T(xd), yr = T(yd), zr = T(zd), r2 = T(r2d);
-            Mp<T> mk;
-            mk.c = jm[0][jj]; mk.dx = jm[1][jj]; mk.dy = jm[2][jj]; mk.dz = jm[3][jj];
-            mk.qxx = jm[4][jj]; mk.qxy = jm[5][jj]; mk.qxz = jm[6][jj]; mk.qyy = jm[7][jj];
-            mk.qyz = jm[8][jj]; mk.qzz = jm[9][jj];
+            Mp<T> mi, mk;
+            mi.c = bm[0][0][il]; mi.dx = bm[0][1][il]; mi.dy = bm[0][2][il]; mi.dz = bm[0][3][il];
+            mi.qxx = bm[0][4][il]; mi.qxy = bm[0][5][il]; mi.qxz = bm[0][6][il]; mi.qyy = bm[0][7][il];
+            mi.qyz = bm[0][8][il]; mi.qzz = bm[0][9][il];
+            mk.c = bm[1][0][jl]; mk.dx = bm[1][1][jl]; mk.dy = bm[1][2][jl]; mk.dz = bm[1][3][jl];
+            mk.qxx = bm[1][4][jl]; mk.qxy = bm[1][5][jl]; mk.qxz = bm[1][6][jl]; mk.qyy = bm[1][7][jl];
+            mk.qyz = bm[1][8][jl]; mk.qzz = bm[1][9][jl];
             T f[3], ti[3], tk[3];
-            e += (double)rep_pair<T>(r2, xr, yr, zr, szi, jsz[jj], dmi, jdm[jj], mi, mk, T(1), a.taper, f, ti, tk);
-            F[0] += f[0]; F[1] += f[1]; F[2] += f[2];
-            Ti[0] += ti[0]; Ti[1] += ti[1]; Ti[2] += ti[2];
-            atomicAdd(&acc[0][jj], -f[0]);
-            atomicAdd(&acc[1][jj], -f[1]);
-            atomicAdd(&acc[2][jj], -f[2]);
-            atomicAdd(&acc[3][jj], tk[0]);
-            atomicAdd(&acc[4][jj], tk[1]);
-            atomicAdd(&acc[5][jj], tk[2]);
+            e += (double)rep_pair<T>(r2, xr, yr, zr, bsz[0][il], bsz[1][jl], bdm[0][il], bdm[1][jl], mi, mk, T(1),
+                                     a.taper, f, ti, tk);
+            for (int c = 0; c < 3; c++) {
+                atomicAdd(&acc[0][c][il], (double)f[c]);
+                atomicAdd(&acc[0][3 + c][il], (double)ti[c]);
+                atomicAdd(&acc[1][c][jl], -(double)f[c]);
+                atomicAdd(&acc[1][3 + c][jl], (double)tk[c]);
+            }
         }
         __syncthreads();
         T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
         T *Q = P + PMQ_TILE_NV * 64;
-        for (int c = 0; c < 3; c++) {
-            P[c * 64 + lane] = F[c];
-            P[(3 + c) * 64 + lane] = Ti[c];
+        for (int c = 0; c < 6; c++) {
+            P[c * 64 + lane] = T(acc[0][c][lane]);
+            Q[c * 64 + lane] = T(acc[1][c][lane]);
         }
-        for (int c = 0; c < 6; c++) Q[c * 64 + lane] = acc[c][lane];
         const double ew = wsum(e);
         if (lane == 0) a.tscal[(size_t)t * PMQ_TILE_NS] = ew;
     }

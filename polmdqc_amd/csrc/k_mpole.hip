@@ -15,6 +15,7 @@
 //   k_cf_delta / k_cf_grad   alterchg.cpp:36-169 / dcflux.cpp:31-225
 #include "pair_mpole.h"
 #include "pmq_internal.h"
+#include "tile_pairs.h"
 
 namespace pmq {
 
@@ -296,60 +297,62 @@ template <class R, int PEN, bool CF, bool ANA, bool GRAD, bool VIR, bool EW>
 __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
 {
     constexpr bool CP = PEN != 0;
-    __shared__ double jx[PMQ_BLOCK], jy[PMQ_BLOCK], jz[PMQ_BLOCK];
-    __shared__ R jm[10][PMQ_BLOCK];
-    __shared__ R jc[3][PMQ_BLOCK]; // core, valence exponent, total charge
-    __shared__ int jflag[PMQ_BLOCK], jmol[PMQ_BLOCK];
-    __shared__ R acc[8][PMQ_BLOCK]; // f(3), t(3), pot, aem
+    // side 0: the atoms of block I, side 1: those of block J
+    __shared__ double bx[2][PMQ_BLOCK], by[2][PMQ_BLOCK], bz[2][PMQ_BLOCK];
+    __shared__ R bm[2][10][PMQ_BLOCK];
+    __shared__ R bc[2][3][PMQ_BLOCK]; // core, valence exponent, total charge
+    __shared__ int bmol[2][PMQ_BLOCK];
+    // FP64 accumulators whatever R is (ds_add_f64 measured faster than
+    // ds_add_f32 here)
+    __shared__ double acc[2][8][PMQ_BLOCK]; // f(3), t(3), pot, aem
+    __shared__ unsigned short plist[PMQ_BLOCK * PMQ_BLOCK];
     const int lane = threadIdx.x;
     const int ntiles = *a.ntiles;
     const int np = a.npad;
 
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 IJ = a.tiles[t];
-        const int si = IJ.x * PMQ_BLOCK + lane, sj = IJ.y * PMQ_BLOCK + lane;
         __syncthreads();
-        jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
-        for (int c = 0; c < 10; c++) jm[c][lane] = a.smp[c * np + sj];
-        if (CP) { jc[0][lane] = a.scp[sj]; jc[1][lane] = a.scp[np + sj]; jc[2][lane] = a.scp[2 * np + sj]; }
-        jflag[lane] = a.sflag[sj];
-        if (ANA) jmol[lane] = a.smol[sj];
-        for (int c = 0; c < 8; c++) acc[c][lane] = R(0);
-        const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
-        const int fi = a.sflag[si];
-        const int moli = ANA ? a.smol[si] : 0;
-        Mp<R> mi;
-        mi.c = a.smp[0 * np + si]; mi.dx = a.smp[1 * np + si]; mi.dy = a.smp[2 * np + si]; mi.dz = a.smp[3 * np + si];
-        mi.qxx = a.smp[4 * np + si]; mi.qxy = a.smp[5 * np + si]; mi.qxz = a.smp[6 * np + si];
-        mi.qyy = a.smp[7 * np + si]; mi.qyz = a.smp[8 * np + si]; mi.qzz = a.smp[9 * np + si];
-        const R corei = CP ? a.scp[si] : R(0), alphai = CP ? a.scp[np + si] : R(0);
-        const R qi = CP ? a.scp[2 * np + si] : R(0);
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const int s = (side ? IJ.y : IJ.x) * PMQ_BLOCK + lane;
+            bx[side][lane] = a.sx[s]; by[side][lane] = a.sy[s]; bz[side][lane] = a.sz[s];
+            for (int c = 0; c < 10; c++) bm[side][c][lane] = a.smp[c * np + s];
+            if (CP) {
+                bc[side][0][lane] = a.scp[s]; bc[side][1][lane] = a.scp[np + s]; bc[side][2][lane] = a.scp[2 * np + s];
+            }
+            if (ANA) bmol[side][lane] = a.smol[s];
+            for (int c = 0; c < 8; c++) acc[side][c][lane] = 0.0;
+        }
+        const int npair = tile_pair_list(a.xmask[(size_t)t * 128 + 64 + lane], plist, lane);
         __syncthreads();
-
-        // the j of the tile this lane's i interacts with
-        unsigned long long mask = a.xmask[(size_t)t * 128 + 64 + lane];
 
         double e = 0; // energy sums stay FP64 whatever R is
-        R fxi = 0, fyi = 0, fzi = 0, txi = 0, tyi = 0, tzi = 0, poti = 0, aemi = 0, ein = 0;
+        R ein = 0;
         R vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
         unsigned nem = 0;
-        while (mask) {
-            const int jj = __ffsll((long long)mask) - 1;
-            mask &= mask - 1;
-            double xd = jx[jj] - xi, yd = jy[jj] - yi, zd = jz[jj] - zi;
+        for (int base = 0; base < npair; base += PMQ_BLOCK) {
+            if (base + lane >= npair) break;
+            const int ent = plist[base + lane];
+            const int il = ent >> 8, jl = ent & 63;
+            double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
             image<double>(a.box, xd, yd, zd);
             const R xr = R(xd), yr = R(yd), zr = R(zd);
             const R r2 = xr * xr + yr * yr + zr * zr;
-            Mp<R> mk;
-            mk.c = jm[0][jj]; mk.dx = jm[1][jj]; mk.dy = jm[2][jj]; mk.dz = jm[3][jj];
-            mk.qxx = jm[4][jj]; mk.qxy = jm[5][jj]; mk.qxz = jm[6][jj];
-            mk.qyy = jm[7][jj]; mk.qyz = jm[8][jj]; mk.qzz = jm[9][jj];
+            Mp<R> mi, mk;
+            mi.c = bm[0][0][il]; mi.dx = bm[0][1][il]; mi.dy = bm[0][2][il]; mi.dz = bm[0][3][il];
+            mi.qxx = bm[0][4][il]; mi.qxy = bm[0][5][il]; mi.qxz = bm[0][6][il];
+            mi.qyy = bm[0][7][il]; mi.qyz = bm[0][8][il]; mi.qzz = bm[0][9][il];
+            mk.c = bm[1][0][jl]; mk.dx = bm[1][1][jl]; mk.dy = bm[1][2][jl]; mk.dz = bm[1][3][jl];
+            mk.qxx = bm[1][4][jl]; mk.qxy = bm[1][5][jl]; mk.qxz = bm[1][6][jl];
+            mk.qyy = bm[1][7][jl]; mk.qyz = bm[1][8][jl]; mk.qzz = bm[1][9][jl];
             PairOut<R> o;
             R bn[6];
             if (EW) ewald_series<R>(r2, R(a.aewald), bn);
             if (CP) {
-                pair_cp<R, GRAD, (PEN == 0 ? 1 : PEN), EW>(r2, xr, yr, zr, R(a.f), corei, qi, alphai, mi, jc[0][jj],
-                                                           jc[2][jj], jc[1][jj], mk, o, bn);
+                pair_cp<R, GRAD, (PEN == 0 ? 1 : PEN), EW>(r2, xr, yr, zr, R(a.f), bc[0][0][il], bc[0][2][il],
+                                                           bc[0][1][il], mi, bc[1][0][jl], bc[1][2][jl],
+                                                           bc[1][1][jl], mk, o, bn);
             } else {
                 R Rs[6];
                 if (EW) {
@@ -362,20 +365,24 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
             e += o.e;
             if (ANA) {
                 if (o.e != R(0)) nem++;
-                aemi += R(0.5) * o.e;
-                atomicAdd(&acc[7][jj], R(0.5) * o.e);
-                if (moli != jmol[jj]) ein += o.e;
+                atomicAdd(&acc[0][7][il], (double)(R(0.5) * o.e));
+                atomicAdd(&acc[1][7][jl], (double)(R(0.5) * o.e));
+                if (bmol[0][il] != bmol[1][jl]) ein += o.e;
             }
             if (GRAD) {
-                fxi += o.fx; fyi += o.fy; fzi += o.fz;
-                txi += o.tix; tyi += o.tiy; tzi += o.tiz;
-                atomicAdd(&acc[0][jj], -o.fx);
-                atomicAdd(&acc[1][jj], -o.fy);
-                atomicAdd(&acc[2][jj], -o.fz);
-                atomicAdd(&acc[3][jj], o.tkx);
-                atomicAdd(&acc[4][jj], o.tky);
-                atomicAdd(&acc[5][jj], o.tkz);
-                if (CF) { poti += o.phii; atomicAdd(&acc[6][jj], o.phik); }
+                atomicAdd(&acc[0][0][il], (double)o.fx);
+                atomicAdd(&acc[0][1][il], (double)o.fy);
+                atomicAdd(&acc[0][2][il], (double)o.fz);
+                atomicAdd(&acc[0][3][il], (double)o.tix);
+                atomicAdd(&acc[0][4][il], (double)o.tiy);
+                atomicAdd(&acc[0][5][il], (double)o.tiz);
+                atomicAdd(&acc[1][0][jl], -(double)o.fx);
+                atomicAdd(&acc[1][1][jl], -(double)o.fy);
+                atomicAdd(&acc[1][2][jl], -(double)o.fz);
+                atomicAdd(&acc[1][3][jl], (double)o.tkx);
+                atomicAdd(&acc[1][4][jl], (double)o.tky);
+                atomicAdd(&acc[1][5][jl], (double)o.tkz);
+                if (CF) { atomicAdd(&acc[0][6][il], (double)o.phii); atomicAdd(&acc[1][6][jl], (double)o.phik); }
                 if (VIR) {
                     vxx -= xr * o.fx;
                     vxy -= R(0.5) * (yr * o.fx + xr * o.fy);
@@ -390,16 +397,16 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
         // partial sums of this tile: side 0 = the i atoms, side 1 = the j atoms
         if (GRAD || ANA) {
             R *P = (R *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
-            if (GRAD) {
-                P[0 * 64 + lane] = fxi; P[1 * 64 + lane] = fyi; P[2 * 64 + lane] = fzi;
-                P[3 * 64 + lane] = txi; P[4 * 64 + lane] = tyi; P[5 * 64 + lane] = tzi;
-                if (CF) P[6 * 64 + lane] = poti;
-            }
-            if (ANA) P[7 * 64 + lane] = aemi;
             R *Q = P + PMQ_TILE_NV * 64;
             if (GRAD)
-                for (int c = 0; c < 6 + (CF ? 1 : 0); c++) Q[c * 64 + lane] = acc[c][lane];
-            if (ANA) Q[7 * 64 + lane] = acc[7][lane];
+                for (int c = 0; c < 6 + (CF ? 1 : 0); c++) {
+                    P[c * 64 + lane] = R(acc[0][c][lane]);
+                    Q[c * 64 + lane] = R(acc[1][c][lane]);
+                }
+            if (ANA) {
+                P[7 * 64 + lane] = R(acc[0][7][lane]);
+                Q[7 * 64 + lane] = R(acc[1][7][lane]);
+            }
         }
         // totals in FP64 whatever R is
         const double ew = wave_sum(e);
@@ -487,7 +494,7 @@ __global__ __launch_bounds__(64) void k_mpole_excl(MpoleArgs a)
         grad_add(a.te, a.teq, 3 * q.i + 2, o.tiz);
         grad_add(a.te, a.teq, 3 * q.k + 0, o.tkx); grad_add(a.te, a.teq, 3 * q.k + 1, o.tky);
         grad_add(a.te, a.teq, 3 * q.k + 2, o.tkz);
-        if (CF) { atomicAdd(&a.pot[q.i], o.phii); atomicAdd(&a.pot[q.k], o.phik); }
+        if (CF) { atomicAdd(&a.pot[q.i], (double)o.phii); atomicAdd(&a.pot[q.k], (double)o.phik); }
         if (VIR) {
             const R vxy = -R(0.5) * (yr * o.fx + xr * o.fy);
             const R vxz = -R(0.5) * (zr * o.fx + xr * o.fz);

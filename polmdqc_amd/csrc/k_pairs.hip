@@ -38,6 +38,7 @@
 
 #include "pair_mpole.h"
 #include "pmq_internal.h"
+#include "tile_pairs.h"
 
 namespace pmq {
 
@@ -217,50 +218,53 @@ __device__ __forceinline__ void store_T(TEnt<V> *ent, size_t e, int col, const T
 }
 
 // CPD: HIPPO damping (overlap factors of the two valence exponents) instead
-// of Thole
+// of Thole.  Pairs are taken 64 at a time from the tile's compacted list
+// (tile_pairs.h); each pair's two entries sit at its row's base plus the
+// number of the row's partners before it.
 template <bool EW, class T, class V, bool CPD = false>
 __global__ __launch_bounds__(64) void k_pair_fill(FillArgs<V> a)
 {
     const int lane = threadIdx.x;
     const int np = a.npad;
     const int ntiles = *a.ntiles;
-    __shared__ double jx[64], jy[64], jz[64];
-    __shared__ T jpd[64], jth[64];
-    __shared__ unsigned long long jmask[64];
-    __shared__ int jbase[64];
+    __shared__ double bx[2][64], by[2][64], bz[2][64];
+    __shared__ T bpd[2][64], bth[2][64];
+    __shared__ unsigned long long bmask[2][64];
+    __shared__ int bbase[2][64];
+    __shared__ unsigned short plist[64 * 64];
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 IJ = a.tiles[t];
         const bool diag = IJ.x == IJ.y;
-        const int si = IJ.x * 64 + lane, sj = IJ.y * 64 + lane;
         const unsigned long long rm = a.mask[(size_t)t * 128 + lane], cm = a.mask[(size_t)t * 128 + 64 + lane];
         __syncthreads();
-        jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
-        jth[lane] = CPD ? 0.0 : a.spol[np + sj];
-        jpd[lane] = CPD ? a.scp[np + sj] : a.spol[2 * np + sj];
-        // lane as a j atom: its column-side mask and entry base
-        jmask[lane] = diag ? (rm | cm) : cm;
-        jbase[lane] = a.rowptr[sj] + a.soff[(size_t)t * 128 + 64 + lane];
-        const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
-        const T thi = CPD ? 0.0 : a.spol[np + si], pdi = CPD ? a.scp[np + si] : a.spol[2 * np + si];
-        const unsigned long long mi = diag ? (rm | cm) : rm;
-        const int ibase = a.rowptr[si] + a.soff[(size_t)t * 128 + lane];
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const int s = (side ? IJ.y : IJ.x) * 64 + lane;
+            bx[side][lane] = a.sx[s]; by[side][lane] = a.sy[s]; bz[side][lane] = a.sz[s];
+            bth[side][lane] = CPD ? 0.0 : a.spol[np + s];
+            bpd[side][lane] = CPD ? a.scp[np + s] : a.spol[2 * np + s];
+            // this lane's atom on each side: its mask of partners in the tile and its entry base
+            bmask[side][lane] = diag ? (rm | cm) : (side ? cm : rm);
+            bbase[side][lane] = a.rowptr[s] + a.soff[(size_t)t * 128 + side * 64 + lane];
+        }
+        const int npair = tile_pair_list(rm, plist, lane);
         __syncthreads();
-        unsigned long long m = rm;
-        while (m) {
-            const int jj = __ffsll((long long)m) - 1;
-            m &= m - 1;
-            double xd = jx[jj] - xi, yd = jy[jj] - yi, zd = jz[jj] - zi;
+        for (int base = 0; base < npair; base += 64) {
+            if (base + lane >= npair) break;
+            const int ent = plist[base + lane];
+            const int il = ent >> 8, jl = ent & 63;
+            double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
             image<double>(a.box, xd, yd, zd);
             const T xr = T(xd), yr = T(yd), zr = T(zd);
             const T r2 = xr * xr + yr * yr + zr * zr;
             T lam[6], R[6];
-            if (CPD) overlap_damp<T>(sqrt(r2), pdi, jpd[jj], lam);
-            else thole_lambda<T>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
+            if (CPD) overlap_damp<T>(sqrt(r2), bpd[0][il], bpd[1][jl], lam);
+            else thole_lambda<T>(sqrt(r2), bpd[0][il], bpd[1][jl], bth[0][il], bth[1][jl], lam);
             pol_series<EW, T>(r2, T(a.aewald), lam, R);
-            const size_t ei = (size_t)ibase + __popcll(mi & ((1ull << jj) - 1ull));
-            const size_t ej = (size_t)jbase[jj] + __popcll(jmask[jj] & ((1ull << lane) - 1ull));
-            if (ei < a.cap) store_T(a.ent, ei, IJ.y * 64 + jj, R, xr, yr, zr);
-            if (ej < a.cap) store_T(a.ent, ej, si, R, xr, yr, zr);
+            const size_t ei = (size_t)bbase[0][il] + __popcll(bmask[0][il] & ((1ull << jl) - 1ull));
+            const size_t ej = (size_t)bbase[1][jl] + __popcll(bmask[1][jl] & ((1ull << il) - 1ull));
+            if (ei < a.cap) store_T(a.ent, ei, IJ.y * 64 + jl, R, xr, yr, zr);
+            if (ej < a.cap) store_T(a.ent, ej, IJ.x * 64 + il, R, xr, yr, zr);
         }
     }
 }

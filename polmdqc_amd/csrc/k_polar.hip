@@ -31,6 +31,7 @@
 //   k_pol_grad_tiles / _excl    real-space gradient and torques
 #include "pair_mpole.h"
 #include "pmq_internal.h"
+#include "tile_pairs.h"
 
 namespace pmq {
 
@@ -167,66 +168,71 @@ template <bool EW, class T, bool CPD = false>
 __global__ __launch_bounds__(64) void k_pol_field_tiles(PolArgs a)
 {
     TILE_PROLOGUE
-    __shared__ double jx[64], jy[64], jz[64];
-    __shared__ T jm[10][64], jpd[64], jth[64], acc[3][64];
-    __shared__ int jflag[64];
+    // side 0: block I, side 1: block J; compacted pair schedule (tile_pairs.h)
+    __shared__ double bx[2][64], by[2][64], bz[2][64];
+    __shared__ T bm[2][10][64], bpd[2][64], bth[2][64];
+    __shared__ double acc[2][3][64];
+    __shared__ unsigned short plist[64 * 64];
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 IJ = a.tiles[t];
-        const int si = IJ.x * 64 + lane, sj = IJ.y * 64 + lane;
         __syncthreads();
-        jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
-        for (int c = 0; c < 10; c++) jm[c][lane] = a.smp[c * np + sj];
-        // Thole: thole and pdamp; HIPPO: total charge and valence exponent
-        jth[lane] = CPD ? a.scp[2 * np + sj] : a.spol[np + sj];
-        jpd[lane] = CPD ? a.scp[np + sj] : a.spol[2 * np + sj];
-        jflag[lane] = a.sflag[sj];
-        acc[0][lane] = acc[1][lane] = acc[2][lane] = T(0);
-        const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
-        const int fi = a.sflag[si];
-        const Mp<T> mi = load_mp<T>(a.smp, np, si);
-        const T thi = CPD ? a.scp[2 * np + si] : a.spol[np + si], pdi = CPD ? a.scp[np + si] : a.spol[2 * np + si];
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const int s = (side ? IJ.y : IJ.x) * 64 + lane;
+            bx[side][lane] = a.sx[s]; by[side][lane] = a.sy[s]; bz[side][lane] = a.sz[s];
+            for (int c = 0; c < 10; c++) bm[side][c][lane] = a.smp[c * np + s];
+            // Thole: thole and pdamp; HIPPO: total charge and valence exponent
+            bth[side][lane] = CPD ? a.scp[2 * np + s] : a.spol[np + s];
+            bpd[side][lane] = CPD ? a.scp[np + s] : a.spol[2 * np + s];
+            acc[side][0][lane] = acc[side][1][lane] = acc[side][2][lane] = 0.0;
+        }
+        const int npair = tile_pair_list(a.mask[(size_t)t * 128 + lane], plist, lane);
         __syncthreads();
-        unsigned long long mask = a.mask[(size_t)t * 128 + lane]; // in-cutoff j of this lane
-        T ex = 0, ey = 0, ez = 0;
-        while (mask) {
-            const int jj = __ffsll((long long)mask) - 1;
-            mask &= mask - 1;
-            double xd = jx[jj] - xi, yd = jy[jj] - yi, zd = jz[jj] - zi;
+        for (int base = 0; base < npair; base += 64) {
+            if (base + lane >= npair) break;
+            const int ent = plist[base + lane];
+            const int il = ent >> 8, jl = ent & 63;
+            double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
             image<double>(a.box, xd, yd, zd);
             const T xr = T(xd), yr = T(yd), zr = T(zd);
             const T r2 = xr * xr + yr * yr + zr * zr;
-            Mp<T> mk;
-            mk.c = jm[0][jj]; mk.dx = jm[1][jj]; mk.dy = jm[2][jj]; mk.dz = jm[3][jj];
-            mk.qxx = jm[4][jj]; mk.qxy = jm[5][jj]; mk.qxz = jm[6][jj]; mk.qyy = jm[7][jj]; mk.qyz = jm[8][jj];
-            mk.qzz = jm[9][jj];
+            Mp<T> mi, mk;
+            mi.c = bm[0][0][il]; mi.dx = bm[0][1][il]; mi.dy = bm[0][2][il]; mi.dz = bm[0][3][il];
+            mi.qxx = bm[0][4][il]; mi.qxy = bm[0][5][il]; mi.qxz = bm[0][6][il]; mi.qyy = bm[0][7][il];
+            mi.qyz = bm[0][8][il]; mi.qzz = bm[0][9][il];
+            mk.c = bm[1][0][jl]; mk.dx = bm[1][1][jl]; mk.dy = bm[1][2][jl]; mk.dz = bm[1][3][jl];
+            mk.qxx = bm[1][4][jl]; mk.qxy = bm[1][5][jl]; mk.qxz = bm[1][6][jl]; mk.qyy = bm[1][7][jl];
+            mk.qyz = bm[1][8][jl]; mk.qzz = bm[1][9][jl];
             T Ei[3] = {0, 0, 0}, Ek[3] = {0, 0, 0};
             if (CPD) {
                 T Rt[6], Xi[6], Xk[6], Rm[6];
-                hippo_pol_series<T>(r2, pdi, jpd[jj], EW, T(a.aewald), T(1), Rt, Xi, Xk, Rm);
+                hippo_pol_series<T>(r2, bpd[0][il], bpd[1][jl], EW, T(a.aewald), T(1), Rt, Xi, Xk, Rm);
                 Mp<T> tk = mk, ti = mi;
-                tk.c = jth[jj];
-                ti.c = thi;
+                tk.c = bth[1][jl];
+                ti.c = bth[0][il];
                 field_at_i<T>(Rt, tk, xr, yr, zr, Ei);
                 field_at_i<T>(Xk, mk, xr, yr, zr, Ei);
                 field_at_k<T>(Rt, ti, xr, yr, zr, Ek);
                 field_at_k<T>(Xi, mi, xr, yr, zr, Ek);
             } else {
                 T lam[6], R[6];
-                thole_lambda<T>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
+                thole_lambda<T>(sqrt(r2), bpd[0][il], bpd[1][jl], bth[0][il], bth[1][jl], lam);
                 pol_series<EW, T>(r2, T(a.aewald), lam, R);
                 mp_fields<T>(R, mi, mk, xr, yr, zr, Ei, Ek);
             }
-            ex += Ei[0]; ey += Ei[1]; ez += Ei[2];
-            atomicAdd(&acc[0][jj], Ek[0]);
-            atomicAdd(&acc[1][jj], Ek[1]);
-            atomicAdd(&acc[2][jj], Ek[2]);
+            for (int c = 0; c < 3; c++) {
+                atomicAdd(&acc[0][c][il], (double)Ei[c]);
+                atomicAdd(&acc[1][c][jl], (double)Ek[c]);
+            }
         }
         __syncthreads();
         // partial sums: side 0 the i atoms, side 1 the j atoms
         T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
-        P[lane] = ex; P[64 + lane] = ey; P[128 + lane] = ez;
         T *Q = P + PMQ_TILE_NV * 64;
-        Q[lane] = acc[0][lane]; Q[64 + lane] = acc[1][lane]; Q[128 + lane] = acc[2][lane];
+        for (int c = 0; c < 3; c++) {
+            P[c * 64 + lane] = T(acc[0][c][lane]);
+            Q[c * 64 + lane] = T(acc[1][c][lane]);
+        }
     }
 }
 
@@ -364,72 +370,72 @@ template <bool EW, class T, bool CPD = false>
 __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
 {
     TILE_PROLOGUE
-    __shared__ double jx[64], jy[64], jz[64];
-    __shared__ T jm[10][64], ju[6][64], jpd[64], jth[64], acc[6][64];
-    __shared__ int jflag[64];
+    // side 0: block I, side 1: block J; compacted pair schedule (tile_pairs.h)
+    __shared__ double bx[2][64], by[2][64], bz[2][64];
+    __shared__ T bm[2][10][64], bu[2][6][64], bpd[2][64], bth[2][64];
+    __shared__ double acc[2][6][64];
+    __shared__ unsigned short plist[64 * 64];
     const double hf = 0.5 * a.f;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 IJ = a.tiles[t];
-        const int si = IJ.x * 64 + lane, sj = IJ.y * 64 + lane;
         __syncthreads();
-        jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
-        for (int c = 0; c < 10; c++) jm[c][lane] = a.smp[c * np + sj];
-        for (int c = 0; c < 6; c++) ju[c][lane] = a.src[c * np + sj];
-        // Thole: thole and pdamp; HIPPO: total charge and valence exponent
-        jth[lane] = CPD ? a.scp[2 * np + sj] : a.spol[np + sj];
-        jpd[lane] = CPD ? a.scp[np + sj] : a.spol[2 * np + sj];
-        jflag[lane] = a.sflag[sj];
-        for (int c = 0; c < 6; c++) acc[c][lane] = T(0);
-        const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
-        const int fi = a.sflag[si];
-        const Mp<T> mi = load_mp<T>(a.smp, np, si);
-        T ui[6];
-        for (int c = 0; c < 6; c++) ui[c] = a.src[c * np + si];
-        const T thi = CPD ? a.scp[2 * np + si] : a.spol[np + si], pdi = CPD ? a.scp[np + si] : a.spol[2 * np + si];
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const int s = (side ? IJ.y : IJ.x) * 64 + lane;
+            bx[side][lane] = a.sx[s]; by[side][lane] = a.sy[s]; bz[side][lane] = a.sz[s];
+            for (int c = 0; c < 10; c++) bm[side][c][lane] = a.smp[c * np + s];
+            for (int c = 0; c < 6; c++) bu[side][c][lane] = a.src[c * np + s];
+            // Thole: thole and pdamp; HIPPO: total charge and valence exponent
+            bth[side][lane] = CPD ? a.scp[2 * np + s] : a.spol[np + s];
+            bpd[side][lane] = CPD ? a.scp[np + s] : a.spol[2 * np + s];
+            for (int c = 0; c < 6; c++) acc[side][c][lane] = 0.0;
+        }
+        const int npair = tile_pair_list(a.mask[(size_t)t * 128 + lane], plist, lane);
         __syncthreads();
-        unsigned long long mask = a.mask[(size_t)t * 128 + lane]; // in-cutoff j of this lane
-        T F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0};
-        while (mask) {
-            const int jj = __ffsll((long long)mask) - 1;
-            mask &= mask - 1;
-            double xd = jx[jj] - xi, yd = jy[jj] - yi, zd = jz[jj] - zi;
+        for (int base = 0; base < npair; base += 64) {
+            if (base + lane >= npair) break;
+            const int ent = plist[base + lane];
+            const int il = ent >> 8, jl = ent & 63;
+            double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
             image<double>(a.box, xd, yd, zd);
             const T xr = T(xd), yr = T(yd), zr = T(zd);
             const T r2 = xr * xr + yr * yr + zr * zr;
-            Mp<T> mk;
-            mk.c = jm[0][jj]; mk.dx = jm[1][jj]; mk.dy = jm[2][jj]; mk.dz = jm[3][jj];
-            mk.qxx = jm[4][jj]; mk.qxy = jm[5][jj]; mk.qxz = jm[6][jj]; mk.qyy = jm[7][jj]; mk.qyz = jm[8][jj];
-            mk.qzz = jm[9][jj];
-            const T udk[3] = {ju[0][jj], ju[1][jj], ju[2][jj]}, upk[3] = {ju[3][jj], ju[4][jj], ju[5][jj]};
-            T f3[3] = {0, 0, 0}, tk[3] = {0, 0, 0};
+            Mp<T> mi, mk;
+            mi.c = bm[0][0][il]; mi.dx = bm[0][1][il]; mi.dy = bm[0][2][il]; mi.dz = bm[0][3][il];
+            mi.qxx = bm[0][4][il]; mi.qxy = bm[0][5][il]; mi.qxz = bm[0][6][il]; mi.qyy = bm[0][7][il];
+            mi.qyz = bm[0][8][il]; mi.qzz = bm[0][9][il];
+            mk.c = bm[1][0][jl]; mk.dx = bm[1][1][jl]; mk.dy = bm[1][2][jl]; mk.dz = bm[1][3][jl];
+            mk.qxx = bm[1][4][jl]; mk.qxy = bm[1][5][jl]; mk.qxz = bm[1][6][jl]; mk.qyy = bm[1][7][jl];
+            mk.qyz = bm[1][8][jl]; mk.qzz = bm[1][9][jl];
+            const T ui[6] = {bu[0][0][il], bu[0][1][il], bu[0][2][il], bu[0][3][il], bu[0][4][il], bu[0][5][il]};
+            const T udk[3] = {bu[1][0][jl], bu[1][1][jl], bu[1][2][jl]}, upk[3] = {bu[1][3][jl], bu[1][4][jl], bu[1][5][jl]};
+            T f3[3] = {0, 0, 0}, ti[3] = {0, 0, 0}, tk[3] = {0, 0, 0};
             if (CPD) {
                 T Rt[6], Xi[6], Xk[6], Rm[6];
-                hippo_pol_series<T>(r2, pdi, jpd[jj], EW, T(a.aewald), T(1), Rt, Xi, Xk, Rm);
-                pol_pair_grad_cp<T>(Rt, Xi, Xk, Rm, thi, jth[jj], mi, mk, ui, ui + 3, udk, upk, xr, yr, zr, T(1), T(0),
-                                    T(0), T(1), f3, Ti, tk);
+                hippo_pol_series<T>(r2, bpd[0][il], bpd[1][jl], EW, T(a.aewald), T(1), Rt, Xi, Xk, Rm);
+                pol_pair_grad_cp<T>(Rt, Xi, Xk, Rm, bth[0][il], bth[1][jl], mi, mk, ui, ui + 3, udk, upk, xr, yr, zr,
+                                    T(1), T(0), T(0), T(1), f3, ti, tk);
             } else {
                 T lam[6], R[6];
-                thole_lambda<T>(sqrt(r2), pdi, jpd[jj], thi, jth[jj], lam);
+                thole_lambda<T>(sqrt(r2), bpd[0][il], bpd[1][jl], bth[0][il], bth[1][jl], lam);
                 pol_series<EW, T>(r2, T(a.aewald), lam, R);
-                pol_pair_grad<T>(R, mi, mk, ui, ui + 3, udk, upk, xr, yr, zr, T(1), T(0), T(0), T(1), f3, Ti, tk);
+                pol_pair_grad<T>(R, mi, mk, ui, ui + 3, udk, upk, xr, yr, zr, T(1), T(0), T(0), T(1), f3, ti, tk);
             }
-            F[0] += f3[0]; F[1] += f3[1]; F[2] += f3[2];
-            atomicAdd(&acc[0][jj], -f3[0]);
-            atomicAdd(&acc[1][jj], -f3[1]);
-            atomicAdd(&acc[2][jj], -f3[2]);
-            atomicAdd(&acc[3][jj], tk[0]);
-            atomicAdd(&acc[4][jj], tk[1]);
-            atomicAdd(&acc[5][jj], tk[2]);
+            for (int c = 0; c < 3; c++) {
+                atomicAdd(&acc[0][c][il], (double)f3[c]);
+                atomicAdd(&acc[0][3 + c][il], (double)ti[c]);
+                atomicAdd(&acc[1][c][jl], -(double)f3[c]);
+                atomicAdd(&acc[1][3 + c][jl], (double)tk[c]);
+            }
         }
         __syncthreads();
         // partial sums: side 0 the i atoms, side 1 the j atoms
         T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
         T *Q = P + PMQ_TILE_NV * 64;
-        for (int c = 0; c < 3; c++) {
-            P[c * 64 + lane] = T(hf) * F[c];
-            P[(3 + c) * 64 + lane] = T(hf) * Ti[c];
+        for (int c = 0; c < 6; c++) {
+            P[c * 64 + lane] = T(hf * acc[0][c][lane]);
+            Q[c * 64 + lane] = T(hf * acc[1][c][lane]);
         }
-        for (int c = 0; c < 6; c++) Q[c * 64 + lane] = T(hf) * acc[c][lane];
     }
 }
 
@@ -680,17 +686,14 @@ __global__ __launch_bounds__(PCG_TB) void k_pcg_c(PcgArgs a)
     a.p[t] = a.z[t] + beta[set] * a.p[t];
 }
 
-__device__ __forceinline__ void block_add2(double a0, double a1, double *dst)
+// block sum of one value, one atomic per block (blockDim 256)
+__device__ __forceinline__ void block_add1(double v, double *dst)
 {
-    double v0 = a0, v1 = a1;
-    for (int o = 32; o > 0; o >>= 1) {
-        v0 += __shfl_xor(v0, o, 64);
-        v1 += __shfl_xor(v1, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(dst, v0);
-        atomicAdd(dst + 1, v1);
-    }
+    __shared__ double red[4];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(dst, ((red[0] + red[1]) + red[2]) + red[3]);
 }
 
 // E_pol = -(f/2) sum mu_d . E^p
@@ -698,7 +701,7 @@ __global__ __launch_bounds__(256) void k_pol_energy(int npad, double f, const do
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const double v = t < 3 * npad ? -0.5 * f * ud[t] * Ep[t] : 0.0;
-    block_add2(v, 0.0, &tot->ep);
+    block_add1(v, &tot->ep);
 }
 
 // induced dipoles to the caller's order (AoS)

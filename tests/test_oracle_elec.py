@@ -102,6 +102,20 @@ def test_pcg_equals_dense_solve(ff):
 
 
 @pytest.mark.parametrize("ff", ["water03", "hippo19"])
+def test_sparse_preconditioner_reaches_the_dense_solution(ff):
+    """The short-range preconditioner (udiag a r + a T_s a r within 4.5 A)
+    converges to the same dipoles as the dense solve, in no more iterations
+    than the diagonal one."""
+    s = ewald_box(0.7, 24, True, ff=ff)
+    dn = oracle.elec(s, want_grad=False, solver=1)
+    jac = oracle.elec(s, want_grad=False, poleps=1e-10)
+    pre = oracle.elec(dict(s, pcgprec=1), want_grad=False, poleps=1e-10)
+    assert 0 < pre["iters"] <= jac["iters"]
+    np.testing.assert_allclose(pre["uind"], dn["uind"], atol=1e-9)
+    assert abs(pre["ep"] - dn["ep"]) < 1e-9 * abs(dn["ep"])
+
+
+@pytest.mark.parametrize("ff", ["water03", "hippo19"])
 def test_polarization_energy_nearly_independent_of_alpha(ff):
     e = [oracle.elec(ewald_box(a, 48, True, ff=ff), want_grad=False, poleps=1e-10)["ep"] for a in (0.8, 1.0)]
     assert abs(e[0] - e[1]) < 5e-5 * abs(e[0])
