@@ -214,6 +214,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--parts", default="amoeba,clusters,hippo",
+                    help="workloads to run (profiling runs take one at a time); the JSON line needs amoeba")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy kernel of MB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -224,6 +226,21 @@ def main() -> int:
 
     from polmdqc_amd import _lib as L
     from polmdqc_amd.engine import Engine
+
+    parts = set(args.parts.split(","))
+    if "amoeba" not in parts:
+        # a profiling run of the other workloads alone
+        eng = Engine(0)
+        out = {}
+        if "clusters" in parts:
+            out["batched_clusters"] = bench_clusters(eng, L, args.steps, args.warmup)
+        if "hippo" in parts:
+            out["hippo_96k"] = bench_hippo(eng, L, args.steps, args.warmup)
+        eng.close()
+        if args.calibrate_copy:
+            out["calibration_copy_bytes"] = copy_calibration(args.calibrate_copy)
+        print(json.dumps(out))
+        return 0
 
     xyz, s = build_system()
     terms = L.TERM_MPOLE | L.TERM_POLAR
@@ -283,8 +300,10 @@ def main() -> int:
         "pairs_in_cutoff": pairs, "pcg_iters": res["pcg_iters"],
         "energy_kcal": {"em": res["em"], "ep": res["ep"]},
     }
-    line["batched_clusters"] = bench_clusters(eng, L, args.steps, args.warmup)
-    line["hippo_96k"] = bench_hippo(eng, L, args.steps, args.warmup)
+    if "clusters" in parts:
+        line["batched_clusters"] = bench_clusters(eng, L, args.steps, args.warmup)
+    if "hippo" in parts:
+        line["hippo_96k"] = bench_hippo(eng, L, args.steps, args.warmup)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(s)
     else:
