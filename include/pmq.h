@@ -242,6 +242,16 @@ PMQ_API int pmq_fetch_batch(pmq_ctx *ctx, double *em, double *ep, int *pcg_iters
 PMQ_API int pmq_profile_enable(pmq_ctx *ctx, int on);
 PMQ_API int pmq_profile_read(pmq_ctx *ctx, const char *name, double *total_ms, long long *launches);
 
+/* PME transform engine.  No counterpart in the reference (it has no FFT
+ * code); for tuning and tests.  PMQ_FFT_AUTO (the default) times, once per
+ * grid, the direct LDS transforms (grids whose planes fit in LDS) against
+ * rocFFT and keeps the faster; PMQ_FFT_LIBRARY always takes rocFFT;
+ * PMQ_FFT_DIRECT takes the direct transforms wherever they fit. */
+#define PMQ_FFT_AUTO 0
+#define PMQ_FFT_LIBRARY 1
+#define PMQ_FFT_DIRECT 2
+PMQ_API int pmq_set_fft(pmq_ctx *ctx, int engine);
+
 #ifdef __cplusplus
 }
 #endif

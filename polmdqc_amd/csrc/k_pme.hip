@@ -188,6 +188,8 @@ int pme_setup(pmq_ctx *c)
                                                       3, len, q < 2 ? 1 : 2, nullptr), "plan"))) return rc;
         c->plan_f1s = sp[0]; c->plan_b1s = sp[1]; c->plan_f2s = sp[2]; c->plan_b2s = sp[3];
         c->pme_fft32 = -1; // to be timed
+        c->pme_own64 = c->pme_own32 = -1;
+        if ((rc = fft_own_setup(c))) return rc;
         size_t wmax = 0;
         for (rocfft_plan pl : {pf1, pb1, pf2, pb2, sp[0], sp[1], sp[2], sp[3]}) {
             size_t w = 0;
@@ -852,6 +854,10 @@ static int pme_potential(pmq_ctx *c, int nset, const double *G = nullptr)
 {
     int rc;
     const bool fp32 = grid32(c) && !G;
+    if ((fp32 ? c->pme_own32 : c->pme_own64) == 1) {
+        ProfScope ps_(c, "pme_fft");
+        return fft_own_convolve(c, nset, fp32 ? (const void *)c->pme_g_s.p : (const void *)(G ? G : c->pme_g.p), fp32);
+    }
     if ((rc = fft(c, true, nset, fp32))) return rc;
     const int H0 = c->K[0] / 2 + 1;
     const int ncplx = H0 * c->K[1] * c->K[2];
@@ -975,33 +981,67 @@ static int gather_frac(pmq_ctx *c, int nset)
 // than its FP64 ones at every grid, so both are timed once per grid (three
 // forward + inverse pairs of the two-set transform each) and the faster
 // kept (FP32 wins ties)
-static int choose_fft_precision(pmq_ctx *c)
+// The transforms of the grid, timed once per grid on the evaluation's own
+// stream (two-set convolutions, the shape of every PCG iteration): for FP64
+// grids the direct LDS transforms against rocFFT; in the mixed form also
+// FP32 grids (the faster of the two engines), kept when they beat FP64.
+static int time_convolution(pmq_ctx *c, float *ms)
 {
-    if (!c->pme_fp32 || c->pme_fft32 >= 0) return PMQ_OK;
-    hipEvent_t e[3];
+    hipEvent_t e[2];
     for (auto &x : e) PMQ_HIP(hipEventCreate(&x));
-    float ms[2] = {0, 0};
     int rc = PMQ_OK;
-    for (int prec = 0; prec < 2 && !rc; prec++) {
-        for (int it = 0; it < 4 && !rc; it++) {
-            if (it == 1) PMQ_HIP(hipEventRecord(e[prec], c->stream));
-            if ((rc = fft(c, true, 2, prec == 1))) break;
-            rc = fft(c, false, 2, prec == 1);
-        }
-        PMQ_HIP(hipEventRecord(e[2], c->stream));
-        PMQ_HIP(hipEventSynchronize(e[2]));
-        PMQ_HIP(hipEventElapsedTime(&ms[prec], e[prec], e[2]));
+    for (int it = 0; it < 4 && !rc; it++) {
+        if (it == 1) PMQ_HIP(hipEventRecord(e[0], c->stream));
+        rc = pme_potential(c, 2);
+    }
+    if (!rc) {
+        PMQ_HIP(hipEventRecord(e[1], c->stream));
+        PMQ_HIP(hipEventSynchronize(e[1]));
+        PMQ_HIP(hipEventElapsedTime(ms, e[0], e[1]));
     }
     for (auto &x : e) (void)hipEventDestroy(x);
-    if (rc) return rc;
-    c->pme_fft32 = ms[1] <= ms[0] ? 1 : 0;
-    return PMQ_OK;
+    return rc;
+}
+
+static int choose_fft(pmq_ctx *c)
+{
+    if (c->pme_own64 >= 0 && (!c->pme_fp32 || c->pme_fft32 >= 0)) return PMQ_OK;
+    const bool fp32_form = c->pme_fp32;
+    int rc = PMQ_OK;
+    const bool lib = c->fft_engine != PMQ_FFT_DIRECT, own = c->fft_engine != PMQ_FFT_LIBRARY;
+    float t64 = 1e30f, t32 = 1e30f, t;
+    // FP64 grids
+    c->pme_fp32 = false;
+    c->pme_own64 = own && fft_own_fits(c, false) ? 1 : 0;
+    if (c->pme_own64 && lib) {
+        if ((rc = time_convolution(c, &t64))) goto done;
+        c->pme_own64 = 0;
+        if ((rc = time_convolution(c, &t))) goto done;
+        if (t < t64) t64 = t;
+        else c->pme_own64 = 1;
+    } else if (fp32_form && (rc = time_convolution(c, &t64))) goto done;
+    if (fp32_form && c->pme_fft32 < 0) {
+        c->pme_fp32 = true;
+        c->pme_fft32 = 1;
+        c->pme_own32 = own && fft_own_fits(c, true) ? 1 : 0;
+        if ((rc = time_convolution(c, &t32))) goto done;
+        if (c->pme_own32 && lib) {
+            c->pme_own32 = 0;
+            if ((rc = time_convolution(c, &t))) goto done;
+            if (t < t32) t32 = t;
+            else c->pme_own32 = 1;
+        }
+        c->pme_fft32 = t32 <= t64 ? 1 : 0;
+    }
+done:
+    c->pme_fp32 = fp32_form;
+    return rc;
 }
 
 int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar)
 {
     int rc;
-    if ((rc = choose_fft_precision(c))) return rc;
+    if ((rc = choose_fft(c))) return rc;
     if ((rc = pme_bin(c))) return rc;
     const double *charge = c->prm.pentype != PMQ_PEN_NONE ? c->scp.p + 2 * (size_t)c->npad : nullptr;
     if ((rc = spread(c, 0, c->smp.p, charge, 1))) return rc;
@@ -1094,6 +1134,7 @@ __global__ __launch_bounds__(256) void k_pme_disp_gather(DispPostArgs a)
 int pme_disp(pmq_ctx *c, bool grad, bool need_bin)
 {
     int rc;
+    if ((rc = choose_fft(c))) return rc;
     if (need_bin && (rc = pme_bin(c))) return rc;
     if ((rc = spread(c, 2, c->shp.p, nullptr, 1))) return rc;
     if ((rc = pme_potential(c, 1, c->pme_gd.p))) return rc;

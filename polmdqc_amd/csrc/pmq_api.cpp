@@ -94,6 +94,19 @@ int pmq_profile_read(pmq_ctx *c, const char *name, double *total_ms, long long *
     return PMQ_OK;
 }
 
+int pmq_set_fft(pmq_ctx *c, int engine)
+{
+    CHECK_CTX(c);
+    if (engine < PMQ_FFT_AUTO || engine > PMQ_FFT_DIRECT) {
+        c->err = "pmq_set_fft: engine must be PMQ_FFT_AUTO, PMQ_FFT_LIBRARY or PMQ_FFT_DIRECT";
+        return PMQ_ERR_ARG;
+    }
+    c->fft_engine = engine;
+    c->pme_own64 = c->pme_own32 = -1;
+    c->pme_fft32 = -1;
+    return PMQ_OK;
+}
+
 static void batch_graph_release(pmq_ctx *c)
 {
     if (c->b_exec) (void)hipGraphExecDestroy((hipGraphExec_t)c->b_exec);

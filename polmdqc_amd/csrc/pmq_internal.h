@@ -322,6 +322,10 @@ struct pmq_ctx {
     pmq::DBuf<float> pme_g_s;
     bool pme_fp32 = false;   // FP32 weights and sums in spreading and gathering (mixed form)
     int pme_fft32 = -1;      // mixed form: FP32 grids and FFTs (1) or FP64 (0), timed once per grid (-1)
+    int fft_engine = PMQ_FFT_AUTO;
+    int pme_own64 = -1, pme_own32 = -1; // FP64 / FP32 grids: the direct LDS transforms (1) or rocFFT (0), timed once per grid (-1)
+    pmq::DBuf<double> fft_tw;    // twiddles of the direct transforms, axes 0..2, FP64 and FP32
+    pmq::DBuf<float> fft_tw_s;
     void *fft_info = nullptr;
     pmq::DevBox pme_dbox{};                       // box the kernel G was built for
     double pme_alpha = 0.0;
@@ -355,6 +359,9 @@ int launch_mpole_direct(pmq_ctx *c, int mode, double off2, bool mixed);
 int launch_fx_flush(pmq_ctx *c, long long *q, double *d, size_t n, double scale);
 int launch_torque(pmq_ctx *c, int mode, const double *te, double *de, bool mixed);
 int pme_setup(pmq_ctx *c);
+int fft_own_setup(pmq_ctx *c);
+bool fft_own_fits(const pmq_ctx *c, bool fp32);
+int fft_own_convolve(pmq_ctx *c, int nset, const void *G, bool fp32);
 int pme_disp(pmq_ctx *c, bool grad, bool need_bin);
 int hippo_sort(pmq_ctx *c);
 int launch_disp(pmq_ctx *c, double off2, bool grad, bool mixed);

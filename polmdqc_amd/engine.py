@@ -267,6 +267,18 @@ KERNEL_FAMILIES = ("neighbors", "mpole_tiles", "rep_tiles", "disp_tiles", "pol_f
                    "pol_grad_tiles", "pme_spread", "pme_fft", "pme_gather", "batch_clusters")
 
 
+FFT_AUTO, FFT_LIBRARY, FFT_DIRECT = 0, 1, 2
+
+
+def set_fft(eng: Engine, engine: int = FFT_AUTO):
+    """PME transform engine: timed choice (FFT_AUTO), rocFFT (FFT_LIBRARY)
+    or the direct LDS transforms where the grid fits them (FFT_DIRECT)."""
+    L = eng._L
+    L.pmq_set_fft.argtypes = [C.c_void_p, C.c_int]
+    L.pmq_set_fft.restype = C.c_int
+    eng._check(L.pmq_set_fft(eng._ctx, int(engine)), "pmq_set_fft")
+
+
 def profile_enable(eng: Engine, on: bool = True):
     _profile_bind(eng._L)
     eng._check(eng._L.pmq_profile_enable(eng._ctx, 1 if on else 0), "pmq_profile_enable")
