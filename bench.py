@@ -150,6 +150,7 @@ def bench_clusters(eng, L, steps, warmup) -> dict:
 
     xyz, off = water.cluster_batch(512, 512, 2, 64)
     s = water.water_system(xyz, "water03")
+    s["pcgprec"] = 0  # the batched solver's diagonal preconditioner
     eng.set_system(s)
     eng.set_batch(off)
     terms = L.TERM_MPOLE | L.TERM_POLAR

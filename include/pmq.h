@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PMQ_ABI_VERSION 2
+#define PMQ_ABI_VERSION 3
 
 #if defined(__GNUC__) || defined(__clang__)
 #define PMQ_API __attribute__((visibility("default")))
@@ -162,6 +162,16 @@ typedef struct {
     const double *sizpr, *dmppr, *elepr; /* n each (NULL = no repulsion) */
     double r2scale, r3scale, r4scale, r5scale;
     double repcut, reptaper;
+    /* PCG preconditioner (polpcg.h:21-25, kpolar.cpp:84, cutoffs.cpp:61; the
+     * reference declares the flag and the cutoff but has no solver, so udiag
+     * follows the published Tinker method, default 2):
+     * with pcgprec the residual r is preconditioned as
+     * z = udiag a r + a T_s (a r), a = max(polarity, 1e-8) and T_s the
+     * damped, u-scaled (w-scaled with charge penetration) real-space
+     * dipole field tensor without Ewald of the pairs within usolvcut (taken
+     * no longer than the real-space cutoff); without it z = a r. */
+    int pcgprec;
+    double usolvcut, udiag;
 } pmq_params;
 
 typedef struct {
@@ -243,10 +253,12 @@ PMQ_API int pmq_profile_enable(pmq_ctx *ctx, int on);
 PMQ_API int pmq_profile_read(pmq_ctx *ctx, const char *name, double *total_ms, long long *launches);
 
 /* PME transform engine.  No counterpart in the reference (it has no FFT
- * code); for tuning and tests.  PMQ_FFT_AUTO (the default) times, once per
- * grid, the direct LDS transforms (grids whose planes fit in LDS) against
- * rocFFT and keeps the faster; PMQ_FFT_LIBRARY always takes rocFFT;
- * PMQ_FFT_DIRECT takes the direct transforms wherever they fit. */
+ * code); for tuning and tests.  PMQ_FFT_LIBRARY (the default) takes rocFFT;
+ * PMQ_FFT_DIRECT takes the direct LDS transforms (three launches per
+ * convolution, the reciprocal kernel applied between the z transforms)
+ * wherever the grid's planes fit in LDS; PMQ_FFT_AUTO times the two once per
+ * grid and keeps the faster (a profiler that adds a cost per launch biases
+ * that choice). */
 #define PMQ_FFT_AUTO 0
 #define PMQ_FFT_LIBRARY 1
 #define PMQ_FFT_DIRECT 2

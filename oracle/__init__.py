@@ -218,9 +218,9 @@ def elec(fx, want_grad: bool = True, solver: int = 0, x=None, poleps: float | No
     for k in ("w2scale", "w3scale", "w4scale", "w5scale"):
         setattr(s, k, float(fx.get(k, 1.0)))
     s.dpequal = int(fx.get("dpequal", 0))
-    # PCG preconditioner: off by default, as in the GPU solver (diagonal
-    # preconditioner); on, usolvcut 4.5 A as cutoffs.cpp:61
-    s.pcgprec = int(fx.get("pcgprec", 0))
+    # PCG preconditioner: on by default with usolvcut 4.5 A and udiag 2, the
+    # reference's defaults (kpolar.cpp:84, cutoffs.cpp:61)
+    s.pcgprec = int(fx.get("pcgprec", 1))
     s.usolvcut = float(fx.get("usolvcut", 4.5))
     s.udiag = float(fx.get("udiag", 2.0))
     o = ElecOut()

@@ -69,7 +69,8 @@ class Params(C.Structure):
                 ("dfft1", C.c_int), ("dfft2", C.c_int), ("dfft3", C.c_int), ("dbsorder", C.c_int),
                 ("sizpr", _dp), ("dmppr", _dp), ("elepr", _dp),
                 ("r2scale", C.c_double), ("r3scale", C.c_double), ("r4scale", C.c_double), ("r5scale", C.c_double),
-                ("repcut", C.c_double), ("reptaper", C.c_double)]
+                ("repcut", C.c_double), ("reptaper", C.c_double),
+                ("pcgprec", C.c_int), ("usolvcut", C.c_double), ("udiag", C.c_double)]
 
 
 class Result(C.Structure):

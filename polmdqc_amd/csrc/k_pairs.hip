@@ -51,6 +51,8 @@ struct MaskArgs {
     double off2;
     unsigned long long *mask;  // 128 per tile: [0,64) row masks, [64,128) column masks
     unsigned long long *mmask; // 64 per tile: row masks of the pairs with an active atom (multipole energy)
+    double us2;                // preconditioner cutoff squared (<= off2)
+    unsigned long long *nmask; // 128 per tile: the pairs within it (row, column masks), or null
 };
 
 __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
@@ -68,24 +70,38 @@ __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
         const double xi = a.sx[si], yi = a.sy[si], zi = a.sz[si];
         const int fi = a.sflag[si];
         __syncthreads();
-        unsigned long long m = 0ull, mm = 0ull;
+        unsigned long long m = 0ull, mm = 0ull, nm = 0ull;
         if (fi & 1) {
             for (int jj = IJ.x == IJ.y ? lane + 1 : 0; jj < 64; jj++) {
                 const int fj = jflag[jj];
                 if (!(fj & 1)) continue;
                 double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
                 image<double>(a.box, xr, yr, zr);
-                if (xr * xr + yr * yr + zr * zr <= a.off2) {
+                const double r2 = xr * xr + yr * yr + zr * zr;
+                if (r2 <= a.off2) {
                     m |= 1ull << jj;
                     if ((fi | fj) & 2) mm |= 1ull << jj;
+                    if (r2 <= a.us2) nm |= 1ull << jj;
                 }
             }
         }
         // column mask of lane c: bit l set where lane l's row mask has bit c
-        unsigned long long col = 0ull;
-        for (int c = 0; c < 64; c++) {
-            const unsigned long long b = __ballot((m >> c) & 1ull);
-            if (lane == c) col = b;
+        unsigned long long col = 0ull, ncol = 0ull;
+        if (a.nmask) {
+            for (int c = 0; c < 64; c++) {
+                const unsigned long long b = __ballot((m >> c) & 1ull), nb = __ballot((nm >> c) & 1ull);
+                if (lane == c) {
+                    col = b;
+                    ncol = nb;
+                }
+            }
+            a.nmask[(size_t)t * 128 + lane] = nm;
+            a.nmask[(size_t)t * 128 + 64 + lane] = ncol;
+        } else {
+            for (int c = 0; c < 64; c++) {
+                const unsigned long long b = __ballot((m >> c) & 1ull);
+                if (lane == c) col = b;
+            }
         }
         a.mask[(size_t)t * 128 + lane] = m;
         a.mask[(size_t)t * 128 + 64 + lane] = col;
@@ -151,27 +167,59 @@ __device__ __forceinline__ unsigned long long side_mask(const unsigned long long
     return IJ.x == I ? r : c;
 }
 
+// Row layout: with NM a row's entries within the preconditioner cutoff come
+// first (nmask; their count in rownear), then the others.  soff and nsoff
+// are each tile's offsets within the two parts.  Tiles are taken U at a time
+// with every load issued unconditionally (indices clamped to the last tile
+// of the list), so the loads of a group are in flight together.
+template <bool NM>
 __global__ __launch_bounds__(64) void k_row_counts(int nblk, const int2 *__restrict__ tiles,
                                                    const unsigned long long *__restrict__ mask,
+                                                   const unsigned long long *__restrict__ nmask,
                                                    const int *__restrict__ adj, const int *__restrict__ adjn,
-                                                   int *__restrict__ soff, int *__restrict__ rowcnt)
+                                                   int *__restrict__ soff, int *__restrict__ nsoff,
+                                                   int *__restrict__ rowcnt, int *__restrict__ rownear)
 {
+    constexpr int U = 8;
     const int I = blockIdx.x, lane = threadIdx.x;
     const int na = adjn[I];
-    int run = 0;
-    for (int s = 0; s < na; s++) {
-        const int t = adj[(size_t)I * nblk + s];
-        const int2 IJ = tiles[t];
-        const unsigned long long m = side_mask(mask, t, IJ, I, lane);
-        if (IJ.x == IJ.y) {
-            soff[(size_t)t * 128 + lane] = run;
-            soff[(size_t)t * 128 + 64 + lane] = run;
-        } else {
-            soff[(size_t)t * 128 + (IJ.x == I ? 0 : 64) + lane] = run;
+    const int *lst = adj + (size_t)I * nblk;
+    int run = 0, nrun = 0;
+    for (int s0 = 0; s0 < na; s0 += U) {
+        int tt[U];
+        int2 ij[U];
+        unsigned long long m[U], nm[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) tt[k] = lst[min(s0 + k, na - 1)];
+#pragma unroll
+        for (int k = 0; k < U; k++) ij[k] = tiles[tt[k]];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            m[k] = side_mask(mask, tt[k], ij[k], I, lane);
+            nm[k] = NM ? side_mask(nmask, tt[k], ij[k], I, lane) : 0ull;
         }
-        run += __popcll(m);
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            if (s0 + k >= na) break;
+            const size_t t = (size_t)tt[k];
+            const int2 IJ = ij[k];
+            if (IJ.x == IJ.y) {
+                soff[t * 128 + lane] = run;
+                soff[t * 128 + 64 + lane] = run;
+                if (NM) {
+                    nsoff[t * 128 + lane] = nrun;
+                    nsoff[t * 128 + 64 + lane] = nrun;
+                }
+            } else {
+                soff[t * 128 + (IJ.x == I ? 0 : 64) + lane] = run;
+                if (NM) nsoff[t * 128 + (IJ.x == I ? 0 : 64) + lane] = nrun;
+            }
+            run += __popcll(m[k] & ~nm[k]);
+            nrun += __popcll(nm[k]);
+        }
     }
-    rowcnt[I * 64 + lane] = run;
+    rowcnt[I * 64 + lane] = run + nrun;
+    if (NM) rownear[I * 64 + lane] = nrun;
     if (I == 0 && lane == 0) rowcnt[nblk * 64] = 0; // the scan's last element becomes the entry total
 }
 
@@ -200,6 +248,11 @@ struct FillArgs {
     int npad;
     size_t cap;
     TEnt<V> *ent;
+    // preconditioner: the pairs within its cutoff (null = none), their
+    // offsets, each row's count of them, and their plain tensors
+    const unsigned long long *nmask;
+    const int *nsoff, *rownear;
+    TEnt<V> *ent_s;
 };
 
 template <class V, class T>
@@ -220,7 +273,9 @@ __device__ __forceinline__ void store_T(TEnt<V> *ent, size_t e, int col, const T
 // CPD: HIPPO damping (overlap factors of the two valence exponents) instead
 // of Thole.  Pairs are taken 64 at a time from the tile's compacted list
 // (tile_pairs.h); each pair's two entries sit at its row's base plus the
-// number of the row's partners before it.
+// number of the row's partners before it in the same part of the row (near
+// pairs first).  Near pairs also store the preconditioner's tensor (no
+// Ewald) at the same entry of ent_s.
 template <bool EW, class T, class V, bool CPD = false>
 __global__ __launch_bounds__(64) void k_pair_fill(FillArgs<V> a)
 {
@@ -229,13 +284,15 @@ __global__ __launch_bounds__(64) void k_pair_fill(FillArgs<V> a)
     const int ntiles = *a.ntiles;
     __shared__ double bx[2][64], by[2][64], bz[2][64];
     __shared__ T bpd[2][64], bth[2][64];
-    __shared__ unsigned long long bmask[2][64];
-    __shared__ int bbase[2][64];
+    __shared__ unsigned long long bmask[2][64], bnmask[2][64];
+    __shared__ int bbase[2][64], bnbase[2][64];
     __shared__ unsigned short plist[64 * 64];
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 IJ = a.tiles[t];
         const bool diag = IJ.x == IJ.y;
         const unsigned long long rm = a.mask[(size_t)t * 128 + lane], cm = a.mask[(size_t)t * 128 + 64 + lane];
+        const unsigned long long nr = a.nmask ? a.nmask[(size_t)t * 128 + lane] : 0ull;
+        const unsigned long long nc = a.nmask ? a.nmask[(size_t)t * 128 + 64 + lane] : 0ull;
         __syncthreads();
 #pragma unroll
         for (int side = 0; side < 2; side++) {
@@ -243,9 +300,13 @@ __global__ __launch_bounds__(64) void k_pair_fill(FillArgs<V> a)
             bx[side][lane] = a.sx[s]; by[side][lane] = a.sy[s]; bz[side][lane] = a.sz[s];
             bth[side][lane] = CPD ? 0.0 : a.spol[np + s];
             bpd[side][lane] = CPD ? a.scp[np + s] : a.spol[2 * np + s];
-            // this lane's atom on each side: its mask of partners in the tile and its entry base
+            // this lane's atom on each side: its masks of partners in the tile
+            // (all, near) and the entry bases of the two parts of its row
             bmask[side][lane] = diag ? (rm | cm) : (side ? cm : rm);
-            bbase[side][lane] = a.rowptr[s] + a.soff[(size_t)t * 128 + side * 64 + lane];
+            bnmask[side][lane] = diag ? (nr | nc) : (side ? nc : nr);
+            const int so = (int)((size_t)t * 128 + side * 64 + lane);
+            bbase[side][lane] = a.rowptr[s] + (a.nmask ? a.rownear[s] : 0) + a.soff[so];
+            bnbase[side][lane] = a.nmask ? a.rowptr[s] + a.nsoff[so] : 0;
         }
         const int npair = tile_pair_list(rm, plist, lane);
         __syncthreads();
@@ -261,10 +322,21 @@ __global__ __launch_bounds__(64) void k_pair_fill(FillArgs<V> a)
             if (CPD) overlap_damp<T>(sqrt(r2), bpd[0][il], bpd[1][jl], lam);
             else thole_lambda<T>(sqrt(r2), bpd[0][il], bpd[1][jl], bth[0][il], bth[1][jl], lam);
             pol_series<EW, T>(r2, T(a.aewald), lam, R);
-            const size_t ei = (size_t)bbase[0][il] + __popcll(bmask[0][il] & ((1ull << jl) - 1ull));
-            const size_t ej = (size_t)bbase[1][jl] + __popcll(bmask[1][jl] & ((1ull << il) - 1ull));
+            const unsigned long long lo_j = (1ull << jl) - 1ull, lo_i = (1ull << il) - 1ull;
+            const unsigned long long ni = bnmask[0][il], nj = bnmask[1][jl];
+            const bool nearp = (ni >> jl) & 1ull;
+            const size_t ei = nearp ? (size_t)bnbase[0][il] + __popcll(ni & lo_j)
+                                    : (size_t)bbase[0][il] + __popcll(bmask[0][il] & ~ni & lo_j);
+            const size_t ej = nearp ? (size_t)bnbase[1][jl] + __popcll(nj & lo_i)
+                                    : (size_t)bbase[1][jl] + __popcll(bmask[1][jl] & ~nj & lo_i);
             if (ei < a.cap) store_T(a.ent, ei, IJ.y * 64 + jl, R, xr, yr, zr);
             if (ej < a.cap) store_T(a.ent, ej, IJ.x * 64 + il, R, xr, yr, zr);
+            if (nearp) {
+                T Rs[6];
+                pol_series<false, T>(r2, T(0), lam, Rs);
+                if (ei < a.cap) store_T(a.ent_s, ei, IJ.y * 64 + jl, Rs, xr, yr, zr);
+                if (ej < a.cap) store_T(a.ent_s, ej, IJ.x * 64 + il, Rs, xr, yr, zr);
+            }
         }
     }
 }
@@ -282,6 +354,8 @@ struct FoldArgs {
     const int *rowptr;
     size_t cap;
     TEnt<V> *ent;
+    const int *rownear; // preconditioner: near entries per row (null = none)
+    TEnt<V> *ent_s;
 };
 
 template <class V>
@@ -323,6 +397,11 @@ __global__ __launch_bounds__(64) void k_excl_fold(FoldArgs<V> a)
     corr_series<double>(r2, q.u, lam, R);
     add_T(a.ent, (size_t)ei, R, xr, yr, zr);
     if (ek >= 0) add_T(a.ent, (size_t)ek, R, xr, yr, zr);
+    // the preconditioner's tensor of a near pair gets the same correction
+    if (a.rownear) {
+        if (ei < a.rowptr[si] + a.rownear[si]) add_T(a.ent_s, (size_t)ei, R, xr, yr, zr);
+        if (ek >= 0 && ek < a.rowptr[sk] + a.rownear[sk]) add_T(a.ent_s, (size_t)ek, R, xr, yr, zr);
+    }
 }
 
 template <class V>
@@ -373,6 +452,83 @@ __global__ __launch_bounds__(256) void k_pol_spmv(SpmvArgs<V> a)
     }
 }
 
+// The PCG preconditioner z = udiag w + a T_s w (w = a r already formed,
+// a = max(polarity, 1e-8)) over the near part of every row, both dipole
+// sets, G lanes per row; with init also p = z.  Block partials of r.z (two
+// sets) and r.r (two sets) go to part[block * 4 + 0..3].
+template <class V>
+struct PrecArgs {
+    int npad;
+    const int *rowptr, *rownear;
+    size_t cap;
+    const TEnt<V> *ent_s;
+    const double *spol;
+    double udiag;
+    const double *w, *r; // 2 x 3 x npad
+    double *z, *p;
+    int init;
+    const double *sc;    // sc[8] != 0: converged, nothing to do (not read with init)
+    double *part;
+};
+
+template <int G, class V>
+__global__ __launch_bounds__(256) void k_pcg_prec(PrecArgs<V> a)
+{
+    if (!a.init && a.sc[8] != 0.0) return;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = tid / G, g = tid % G;
+    const int np = a.npad;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (row < np) {
+        const int e1 = (int)min((size_t)(a.rowptr[row] + a.rownear[row]), a.cap);
+        for (int e = a.rowptr[row] + g; e < e1; e += G) {
+            const TEnt<V> r = a.ent_s[e];
+            const int k = r.col;
+            const double txx = r.t[0], tyy = r.t[1], tzz = r.t[2];
+            const double txy = r.t[3], txz = r.t[4], tyz = r.t[5];
+#pragma unroll
+            for (int set = 0; set < 2; set++) {
+                const double *u = a.w + (size_t)set * 3 * np;
+                const double ux = u[k], uy = u[np + k], uz = u[2 * np + k];
+                acc[3 * set + 0] += txx * ux + txy * uy + txz * uz;
+                acc[3 * set + 1] += txy * ux + tyy * uy + tyz * uz;
+                acc[3 * set + 2] += txz * ux + tyz * uy + tzz * uz;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++)
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+    double rz[2] = {0, 0}, rr[2] = {0, 0};
+    if (row < np && g < 6) {
+        double v = acc[0];
+#pragma unroll
+        for (int c = 1; c < 6; c++)
+            if (g == c) v = acc[c];
+        const size_t i = (size_t)g * np + row;
+        const double z = a.udiag * a.w[i] + fmax(1e-8, a.spol[row]) * v;
+        a.z[i] = z;
+        if (a.init) a.p[i] = z;
+        const double r = a.r[i];
+        rz[g / 3] = r * z;
+        rr[g / 3] = r * r;
+    }
+    // block partials, fixed order: wave shuffles, then the 4 waves in order
+    __shared__ double red[4][4];
+    double v4[4] = {rz[0], rz[1], rr[0], rr[1]};
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        for (int o = 32; o > 0; o >>= 1) v4[q] += __shfl_xor(v4[q], o, 64);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < 4; q++) red[q][w] = v4[q];
+    __syncthreads();
+    if (threadIdx.x < 4)
+        a.part[blockIdx.x * 4 + threadIdx.x] =
+            ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
+}
+
 // ---------------------------------------------------------------------------
 // host side
 
@@ -389,6 +545,9 @@ static int pair_fill(pmq_ctx *c, int tgrid)
     fa.aewald = c->prm.use_ewald ? c->prm.aewald : 0.0;
     fa.mask = c->tmask.p; fa.soff = c->soff.p; fa.rowptr = c->rowptr.p;
     fa.npad = np; fa.cap = c->tcap; fa.ent = (TEnt<V> *)c->tval.p;
+    fa.nmask = c->prec_on ? c->nmask.p : nullptr;
+    fa.nsoff = c->nsoff.p; fa.rownear = c->rownear.p;
+    fa.ent_s = c->prec_on ? (TEnt<V> *)c->tval_s.p : nullptr;
     {
         ProfScope ps_(c, "pol_pair_fill");
         const bool cpd = c->prm.pentype != PMQ_PEN_NONE;
@@ -409,6 +568,8 @@ static int pair_fill(pmq_ctx *c, int tgrid)
         fo.cpd = c->prm.pentype != PMQ_PEN_NONE ? 1 : 0;
         fo.box = c->box; fo.npad = np; fo.rowptr = c->rowptr.p; fo.cap = c->tcap;
         fo.ent = (TEnt<V> *)c->tval.p;
+        fo.rownear = c->prec_on ? c->rownear.p : nullptr;
+        fo.ent_s = c->prec_on ? (TEnt<V> *)c->tval_s.p : nullptr;
         hipLaunchKernelGGL((k_excl_fold<V>), dim3((nexcl + 63) / 64), dim3(64), 0, st, fo);
     }
     PMQ_HIP(hipGetLastError());
@@ -485,6 +646,10 @@ int tile_structure(pmq_ctx *c, double off2, bool safe)
     PMQ_HIP(c->mmask.alloc(64 * nt));
     PMQ_HIP(c->xmask.alloc(128 * nt));
     PMQ_HIP(c->soff.alloc(128 * nt));
+    if (c->prec_on) {
+        PMQ_HIP(c->nmask.alloc(128 * nt));
+        PMQ_HIP(c->nsoff.alloc(128 * nt));
+    }
     PMQ_HIP(c->adj.alloc((size_t)nblk * nblk));
     PMQ_HIP(c->adjn.alloc(nblk));
     PMQ_HIP(c->tpart.alloc(nt * 2 * PMQ_TILE_NV * 64));
@@ -497,6 +662,8 @@ int tile_structure(pmq_ctx *c, double off2, bool safe)
     ma.tiles = c->tiles.p; ma.ntiles = c->ntiles_eff.p;
     ma.sx = c->sx.p; ma.sy = c->sy.p; ma.sz = c->sz.p; ma.sflag = c->sflag.p;
     ma.box = c->box; ma.off2 = off2; ma.mask = c->tmask.p; ma.mmask = c->mmask.p;
+    ma.us2 = c->prec_on ? std::min(c->prm.usolvcut * c->prm.usolvcut, off2) : -1.0;
+    ma.nmask = c->prec_on ? c->nmask.p : nullptr;
     hipLaunchKernelGGL(k_tile_masks, dim3(c->tgrid), dim3(64), 0, st, ma);
     hipLaunchKernelGGL(k_block_adj, dim3(nblk), dim3(64), 0, st, nblk, c->tile_cap, c->tileflag.p, c->tilescan.p,
                        c->adj.p, c->adjn.p);
@@ -516,8 +683,13 @@ int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe)
     hipStream_t st = c->stream;
     PMQ_HIP(c->rowcnt.alloc(np + 1));
     PMQ_HIP(c->rowptr.alloc(np + 1));
-    hipLaunchKernelGGL(k_row_counts, dim3(nblk), dim3(64), 0, st, nblk, c->tiles.p, c->tmask.p, c->adj.p, c->adjn.p,
-                       c->soff.p, c->rowcnt.p);
+    PMQ_HIP(c->rownear.alloc(np));
+    if (c->prec_on)
+        hipLaunchKernelGGL(k_row_counts<true>, dim3(nblk), dim3(64), 0, st, nblk, c->tiles.p, c->tmask.p, c->nmask.p,
+                           c->adj.p, c->adjn.p, c->soff.p, c->nsoff.p, c->rowcnt.p, c->rownear.p);
+    else
+        hipLaunchKernelGGL(k_row_counts<false>, dim3(nblk), dim3(64), 0, st, nblk, c->tiles.p, c->tmask.p, nullptr,
+                           c->adj.p, c->adjn.p, c->soff.p, nullptr, c->rowcnt.p, nullptr);
     PMQ_HIP(hipGetLastError());
     size_t bytes = 0;
     PMQ_HIP(rocprim::exclusive_scan(nullptr, bytes, c->rowcnt.p, c->rowptr.p, 0, (size_t)np + 1,
@@ -534,6 +706,8 @@ int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe)
             PMQ_HIP(c->tval.alloc(8 * c->tcap)); // one 64-byte record per entry (32 bytes used by the mixed form)
         }
     }
+    // the preconditioner's tensors sit at the entries of the near pairs
+    if (c->prec_on) PMQ_HIP(c->tval_s.alloc(8 * c->tcap));
     hipLaunchKernelGGL(k_nnz_check, dim3(1), dim3(1), 0, st, c->rowptr.p + np, c->tcap, c->status.p);
 
     c->tmixed = mixed;
@@ -685,6 +859,34 @@ static int spmv(pmq_ctx *c, const double *src, double *dst)
     hipLaunchKernelGGL((k_pol_spmv<G, V>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
+}
+
+template <class V>
+static int precondition(pmq_ctx *c, const double *w, const double *r, double *z, double *p, bool init,
+                        const double *sc, double *part)
+{
+    constexpr int G = 8;
+    PrecArgs<V> a;
+    a.npad = c->npad; a.rowptr = c->rowptr.p; a.rownear = c->rownear.p; a.cap = c->tcap;
+    a.ent_s = (const TEnt<V> *)c->tval_s.p;
+    a.spol = c->spol.p; a.udiag = c->prm.udiag;
+    a.w = w; a.r = r; a.z = z; a.p = p; a.init = init ? 1 : 0; a.sc = sc; a.part = part;
+    hipLaunchKernelGGL((k_pcg_prec<G, V>), dim3((unsigned)pair_matrix_prec_blocks(c)), dim3(256), 0, c->stream, a);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+int pair_matrix_prec_blocks(const pmq_ctx *c) { return (int)(((size_t)c->npad * 8 + 255) / 256); }
+
+int pair_matrix_precondition(pmq_ctx *c, const double *w, const double *r, double *z, double *p, bool init,
+                             const double *sc, double *part)
+{
+    if (!c->prec_on) {
+        c->err = "preconditioner applied without its matrix";
+        return PMQ_ERR_STATE;
+    }
+    if (c->tmixed) return precondition<float>(c, w, r, z, p, init, sc, part);
+    return precondition<double>(c, w, r, z, p, init, sc, part);
 }
 
 int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst)

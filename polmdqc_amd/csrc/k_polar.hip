@@ -533,10 +533,13 @@ struct PcgArgs {
     const double *E, *Tu; // fields
     double *u, *r, *z, *p, *v;
     double *part;         // nblk x 4 partial sums of k_pcg_a (and of the initial residual)
-    double *part2;        // nblk x 4 partial sums of k_pcg_b
+    double *part2;        // nblk2 x 4 partial sums of r.z and r.r (k_pcg_b, or the preconditioner)
     double *sc;
     double npolar, poleps;
     int iter;
+    int prec;             // sparse preconditioner: r is handed to it as w = a r
+    double *w;
+    int nblk2;
 };
 
 // slot of r.z after iteration k (k = 0: the initial residual)
@@ -588,6 +591,8 @@ __global__ __launch_bounds__(256) void k_pcg_guess(PcgArgs a)
     a.u[t] = a.spol[s] * a.E[t];
 }
 
+// initial residual; z = p = a r and the r.z partials, or (prec) w = a r for
+// the preconditioner, which forms z, p and the partials
 __global__ __launch_bounds__(PCG_TB) void k_pcg_init(PcgArgs a)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -598,17 +603,22 @@ __global__ __launch_bounds__(PCG_TB) void k_pcg_init(PcgArgs a)
         const double r = pol != 0.0 ? a.E[t] + a.Tu[t] - a.u[t] / pol : 0.0;
         const double z = pol * r;
         a.r[t] = r;
-        a.z[t] = z;
-        a.p[t] = z;
-        (t < 3 * a.npad ? s0 : s1) = r * z;
+        if (a.prec) {
+            a.w[t] = z;
+        } else {
+            a.z[t] = z;
+            a.p[t] = z;
+            (t < 3 * a.npad ? s0 : s1) = r * z;
+        }
     }
-    block_part4(s0, s1, 0.0, 0.0, a.part);
+    if (!a.prec) block_part4(s0, s1, 0.0, 0.0, a.part);
 }
 
 __global__ __launch_bounds__(PCG_TB) void k_pcg_init_fin(PcgArgs a)
 {
     double tot[4];
-    sum_parts(a.part, a.nblk, tot);
+    if (a.prec) sum_parts(a.part2, a.nblk2, tot);
+    else sum_parts(a.part, a.nblk, tot);
     if (threadIdx.x == 0) {
         a.sc[0] = tot[0]; a.sc[1] = tot[1];
         for (int q = 2; q < 16; q++) a.sc[q] = 0.0;
@@ -632,7 +642,8 @@ __global__ __launch_bounds__(PCG_TB) void k_pcg_a(PcgArgs a)
 }
 
 // alpha = r.z / p.v (each block from the partials), u += alpha p,
-// r -= alpha v, z = M r; new r.z and r.r partials
+// r -= alpha v, z = a r with new r.z and r.r partials (or, with the sparse
+// preconditioner, w = a r for it to finish z and the partials)
 __global__ __launch_bounds__(PCG_TB) void k_pcg_b(PcgArgs a)
 {
     if (a.sc[8] != 0.0) return;
@@ -650,11 +661,15 @@ __global__ __launch_bounds__(PCG_TB) void k_pcg_b(PcgArgs a)
         const double r = a.r[t] - al * a.v[t];
         const double z = a.spol[s] * r;
         a.r[t] = r;
-        a.z[t] = z;
-        (set == 0 ? s0 : s1) = r * z;
-        (set == 0 ? q0 : q1) = r * r;
+        if (a.prec) {
+            a.w[t] = z;
+        } else {
+            a.z[t] = z;
+            (set == 0 ? s0 : s1) = r * z;
+            (set == 0 ? q0 : q1) = r * r;
+        }
     }
-    block_part4(s0, s1, q0, q1, a.part2);
+    if (!a.prec) block_part4(s0, s1, q0, q1, a.part2);
 }
 
 // beta = new r.z / r.z and the convergence test debye sqrt(max r.r / npolar)
@@ -664,7 +679,7 @@ __global__ __launch_bounds__(PCG_TB) void k_pcg_c(PcgArgs a)
 {
     if (a.sc[8] != 0.0) return;
     double tot[4];
-    sum_parts(a.part2, a.nblk, tot);
+    sum_parts(a.part2, a.nblk2, tot);
     const int so = rz_slot(a.iter - 1), sn = rz_slot(a.iter);
     const double beta[2] = {a.sc[so] != 0.0 ? tot[0] / a.sc[so] : 0.0, a.sc[so + 1] != 0.0 ? tot[1] / a.sc[so + 1] : 0.0};
     const double eps = 4.80321 * sqrt(fmax(tot[2], tot[3]) / a.npolar); // debye x rms residual
@@ -809,13 +824,18 @@ int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
     const int np = c->npad;
     const size_t nv = 6 * (size_t)np;
     const int gb = (int)((nv + PCG_TB - 1) / PCG_TB);
-    PMQ_HIP(c->pcg_part.alloc(8 * (size_t)gb));
+    const bool prec = c->prec_on;
+    const int gb2 = prec ? pair_matrix_prec_blocks(c) : gb;
+    PMQ_HIP(c->pcg_part.alloc(4 * (size_t)gb + 4 * (size_t)gb2));
     PcgArgs a;
     a.npad = np; a.nblk = gb; a.spol = c->spol.p;
     a.E = c->pfield.p; a.Tu = c->pvec.p;
     a.u = c->uind.p; a.r = c->pres.p; a.z = c->pz.p; a.p = c->pdir.p; a.v = c->pvec.p;
     a.part = c->pcg_part.p;
     a.part2 = c->pcg_part.p + 4 * (size_t)gb;
+    a.nblk2 = gb2;
+    a.prec = prec ? 1 : 0;
+    a.w = c->pw.p;
     a.sc = c->pcg.p;
     int npolar = 0;
     for (double p : c->h_polarity) npolar += p != 0.0;
@@ -826,6 +846,7 @@ int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
     hipLaunchKernelGGL(k_pcg_guess, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
     if ((rc = mutual_field(c, c->uind.p, c->pvec.p))) return rc;
     hipLaunchKernelGGL(k_pcg_init, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
+    if (prec && (rc = pair_matrix_precondition(c, a.w, a.r, a.z, a.p, true, a.sc, a.part2))) return rc;
     hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(PCG_TB), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     const bool spec = !safe && c->pcg_guess_iters > 0;
@@ -840,6 +861,7 @@ int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
             if ((rc = mutual_field(c, c->pdir.p, c->pvec.p))) return rc;
             hipLaunchKernelGGL(k_pcg_a, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
             hipLaunchKernelGGL(k_pcg_b, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
+            if (prec && (rc = pair_matrix_precondition(c, a.w, a.r, a.z, a.p, false, a.sc, a.part2))) return rc;
             hipLaunchKernelGGL(k_pcg_c, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
             PMQ_HIP(hipGetLastError());
         }

@@ -366,6 +366,10 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
         c->err = "pmq_set_params: electric and dielec must be nonzero";
         return PMQ_ERR_ARG;
     }
+    if (p->pcgprec && !(p->usolvcut >= 0.0 && std::isfinite(p->udiag))) {
+        c->err = "pmq_set_params: the PCG preconditioner needs usolvcut >= 0 and a finite udiag";
+        return PMQ_ERR_ARG;
+    }
     PMQ_HIP(hipSetDevice(c->device));
     const int n = c->n;
     int rc;
@@ -424,6 +428,7 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
         PMQ_HIP(c->pfield.alloc(nv)); PMQ_HIP(c->uind.alloc(nv));
         PMQ_HIP(c->pres.alloc(nv)); PMQ_HIP(c->pz.alloc(nv));
         PMQ_HIP(c->pdir.alloc(nv)); PMQ_HIP(c->pvec.alloc(nv));
+        PMQ_HIP(c->pw.alloc(nv));
         PMQ_HIP(c->pcg.alloc(16));
         PMQ_HIP(c->tep.alloc(3 * (size_t)n)); PMQ_HIP(c->dep.alloc(3 * (size_t)n));
         PMQ_HIP(c->uind_out.alloc(6 * (size_t)n));
@@ -627,6 +632,7 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
         const double off2 = cut * cut;
         const bool cf = c->prm.pentype != PMQ_PEN_NONE && c->prm.use_chgflx;
         if (c->prm.use_chgflx && (rc = launch_chgflx_delta(c))) return rc;
+        c->prec_on = do_polar && c->prm.pcgprec && c->prm.usolvcut > 0.0;
         {
             ProfScope ps(c, "neighbors");
             if ((rc = build_neighbors(c, cut))) return rc;
@@ -717,6 +723,10 @@ static int batch_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
     if ((terms & PMQ_TERM_POLAR) && c->h_polarity.empty()) {
         c->err = "pmq_compute: polarization requested but no polarizabilities were set";
         return PMQ_ERR_STATE;
+    }
+    if ((terms & PMQ_TERM_POLAR) && c->prm.pcgprec) {
+        c->err = "pmq_compute: batched clusters solve with the diagonal preconditioner (set pcgprec = 0)";
+        return PMQ_ERR_UNSUPPORTED;
     }
     PMQ_HIP(hipSetDevice(c->device));
     int rc;

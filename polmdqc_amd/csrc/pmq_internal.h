@@ -263,6 +263,7 @@ struct pmq_ctx {
     pmq::DBuf<double> uind;                      // induced dipoles mu_d, mu_p
     pmq::DBuf<double> pres, pz, pdir, pvec;      // PCG residual, preconditioned residual,
                                                  // search direction, (1/alpha - T) p
+    pmq::DBuf<double> pw;                        // a r, the preconditioner's input
     pmq::DBuf<double> pcg;                       // PCG scalars
     pmq::DBuf<double> pcg_part;                  // PCG per-block partial sums
     int pcg_guess_iters = 0;                     // iterations of the last converged solve (0 = none yet)
@@ -288,6 +289,12 @@ struct pmq_ctx {
     pmq::DBuf<int> rowcnt, rowptr;               // npad + 1
     pmq::DBuf<unsigned char> pscan_tmp;
     pmq::DBuf<double> tval;                      // tcap records of 64 bytes: T xx yy zz xy xz yz, column
+    // PCG preconditioner: rows keep their pairs within usolvcut first
+    bool prec_on = false;                        // this evaluation solves with the sparse preconditioner
+    pmq::DBuf<unsigned long long> nmask;         // 128 per tile: the pairs within usolvcut (row, column masks)
+    pmq::DBuf<int> nsoff;                        // 128 per tile: offsets within the near part of the row
+    pmq::DBuf<int> rownear;                      // npad: near entries per row
+    pmq::DBuf<double> tval_s;                    // tcap records: the near pairs' tensors without Ewald
     size_t tcap = 0;
     int nnz = 0;
     bool tmixed = false;                         // tval holds floats (mixed form)
@@ -322,7 +329,7 @@ struct pmq_ctx {
     pmq::DBuf<float> pme_g_s;
     bool pme_fp32 = false;   // FP32 weights and sums in spreading and gathering (mixed form)
     int pme_fft32 = -1;      // mixed form: FP32 grids and FFTs (1) or FP64 (0), timed once per grid (-1)
-    int fft_engine = PMQ_FFT_AUTO;
+    int fft_engine = PMQ_FFT_LIBRARY;
     int pme_own64 = -1, pme_own32 = -1; // FP64 / FP32 grids: the direct LDS transforms (1) or rocFFT (0), timed once per grid (-1)
     pmq::DBuf<double> fft_tw;    // twiddles of the direct transforms, axes 0..2, FP64 and FP32
     pmq::DBuf<float> fft_tw_s;
@@ -381,6 +388,9 @@ int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe);
 int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fixed);
 int tile_scalars(pmq_ctx *c, int which, bool full);
 int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst);
+int pair_matrix_prec_blocks(const pmq_ctx *c);
+int pair_matrix_precondition(pmq_ctx *c, const double *w, const double *r, double *z, double *p, bool init,
+                             const double *sc, double *part);
 int launch_batch(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int variant);
 int param_gradients(pmq_ctx *c, bool mpole, bool polar, double *dpole, double *dpolarity, double *dpdamp,
                     double *dthole);

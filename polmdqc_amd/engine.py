@@ -169,6 +169,9 @@ class Engine:
             p.sizpr, p.dmppr, p.elepr = self._d(sizpr), self._d(s["dmppr"]), self._d(s["elepr"])
             p.repcut = float(s.get("repcut", 6.0))
             p.reptaper = taper(p.repcut, "reptaper")
+        # the reference's PCG defaults: preconditioner on, 4.5 A, udiag 2
+        p.pcgprec = int(s.get("pcgprec", 1))
+        p.usolvcut, p.udiag = float(s.get("usolvcut", 4.5)), float(s.get("udiag", 2.0))
         self._check(self._L.pmq_set_params(self._ctx, C.byref(p)), "pmq_set_params")
         self.n = n
         self.box = box_from(s)
@@ -270,9 +273,10 @@ KERNEL_FAMILIES = ("neighbors", "mpole_tiles", "rep_tiles", "disp_tiles", "pol_f
 FFT_AUTO, FFT_LIBRARY, FFT_DIRECT = 0, 1, 2
 
 
-def set_fft(eng: Engine, engine: int = FFT_AUTO):
-    """PME transform engine: timed choice (FFT_AUTO), rocFFT (FFT_LIBRARY)
-    or the direct LDS transforms where the grid fits them (FFT_DIRECT)."""
+def set_fft(eng: Engine, engine: int = FFT_LIBRARY):
+    """PME transform engine: rocFFT (FFT_LIBRARY, the default), the direct
+    LDS transforms where the grid fits them (FFT_DIRECT), or a timed choice
+    of the two (FFT_AUTO)."""
     L = eng._L
     L.pmq_set_fft.argtypes = [C.c_void_p, C.c_int]
     L.pmq_set_fft.restype = C.c_int

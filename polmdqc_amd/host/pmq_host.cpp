@@ -110,6 +110,7 @@ int upload(std::string *err)
     p.w2scale = g.w2scale; p.w3scale = g.w3scale; p.w4scale = g.w4scale; p.w5scale = g.w5scale;
     p.dpequal = g.dpequal;
     p.poleps = g.poleps; p.politer = g.politer;
+    p.pcgprec = g.pcgprec ? 1 : 0; p.usolvcut = g.usolvcut; p.udiag = g.udiag;
     auto taper = [](double t, double cut) { return t < 1.0 ? t * cut : t; }; // cutoffs.cpp:259-261
     if (!g.csix.empty()) {
         p.csix = g.csix.data(); p.adisp = g.adisp.data();

@@ -58,6 +58,9 @@ struct Globals {
     bool dpequal = false;
     double poleps = 1e-6;
     int politer = 100;
+    bool pcgprec = true;    // polpcg.h:24, default kpolar.cpp:84
+    double usolvcut = 4.5;  // cutoffs.cpp:61
+    double udiag = 2.0;     // diagonal factor of the sparse preconditioner (the reference declares pcgprec, not the solver)
     // HIPPO dispersion: kdsp.h (csix, adisp per atom), dsppot.h, ewald.h, pme.h, limits.h
     std::vector<double> csix, adisp;
     double dsp2scale = 0, dsp3scale = 0, dsp4scale = 1, dsp5scale = 1; // initprm.cpp:338-341
