@@ -44,6 +44,41 @@ int launch_fx_flush(pmq_ctx *c, long long *q, double *d, size_t n, double scale)
     return PMQ_OK;
 }
 
+// several buffers cleared by one launch (segment k: blockIdx.y == k)
+struct ZeroList {
+    double *p[PMQ_ZERO_MAX];
+    size_t n[PMQ_ZERO_MAX];
+};
+
+__global__ void k_zero_many(ZeroList z)
+{
+    double *d = z.p[blockIdx.y];
+    const size_t n = z.n[blockIdx.y];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        d[i] = 0.0;
+}
+
+int launch_zero_many(pmq_ctx *c, int k, double *const *p, const size_t *n)
+{
+    if (k <= 0) return PMQ_OK;
+    if (k > PMQ_ZERO_MAX) {
+        c->err = "launch_zero_many: too many buffers";
+        return PMQ_ERR_ARG;
+    }
+    ZeroList z{};
+    size_t mx = 0;
+    for (int i = 0; i < k; i++) {
+        z.p[i] = p[i];
+        z.n[i] = n[i];
+        mx = std::max(mx, n[i]);
+    }
+    const int tb = 256;
+    const int gb = (int)std::max<size_t>(1, std::min<size_t>((mx + tb - 1) / tb, 1024));
+    hipLaunchKernelGGL(k_zero_many, dim3(gb, k), dim3(tb), 0, c->stream, z);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
 int launch_zero(pmq_ctx *c, double *p, size_t n)
 {
     if (!n) return PMQ_OK;

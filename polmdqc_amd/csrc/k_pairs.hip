@@ -720,17 +720,17 @@ int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe)
 
 // per-atom sums of NV partials over the tiles of the atom's block; P is the
 // partial type (the pair arithmetic), FX sums in 64-bit fixed point.  One
-// workgroup of 4 waves per block: the block's tile list is staged in LDS,
-// wave w sums tiles w, w+4, ... (independent loads, several in flight), and
-// the four partial sums are added in wave order.
+// workgroup of RED_W waves per block: the block's tile list is staged in LDS,
+// wave w sums tiles w, w+RED_W, ... (independent loads, several in flight),
+// and the RED_W partial sums are added in wave order.
 constexpr int RED_MAXT = 1024; // tiles per block staged at a time
+constexpr int RED_W = 8;       // waves per block
 
 template <class P, int NV, bool FX>
-__global__ __launch_bounds__(256) void k_tile_reduce(TileReduceArgs a)
+__global__ __launch_bounds__(64 * RED_W) void k_tile_reduce(TileReduceArgs a)
 {
     __shared__ int lt[RED_MAXT];
-    __shared__ double red_d[4][NV][64];
-    __shared__ long long red_q[4][NV][64];
+    __shared__ unsigned long long red[RED_W][NV][64]; // fixed-point sums, or the bits of double sums
     const int I = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int si = I * 64 + lane;
     const P *part = (const P *)a.part;
@@ -746,14 +746,14 @@ __global__ __launch_bounds__(256) void k_tile_reduce(TileReduceArgs a)
         const int cnt = min(RED_MAXT, na - base);
         __syncthreads();
         // entry = partial-slot index (tile * 2 + side), bit 30 set: a diagonal tile (both sides)
-        for (int s = tid; s < cnt; s += 256) {
+        for (int s = tid; s < cnt; s += 64 * RED_W) {
             const int t = a.adj[(size_t)I * a.nblk + base + s];
             const int2 IJ = a.tiles[t];
             lt[s] = IJ.x == IJ.y ? ((t * 2) | (1 << 30)) : (t * 2 + (IJ.x == I ? 0 : 1));
         }
         __syncthreads();
 #pragma unroll 2
-        for (int s = wv; s < cnt; s += 4) {
+        for (int s = wv; s < cnt; s += RED_W) {
             const int e = lt[s];
             const int slot = e & ~(1 << 30);
             const int nside = (e >> 30) & 1 ? 2 : 1;
@@ -771,18 +771,22 @@ __global__ __launch_bounds__(256) void k_tile_reduce(TileReduceArgs a)
         }
     }
 #pragma unroll
-    for (int c = 0; c < NV; c++) {
-        if (FX) red_q[wv][c][lane] = q[c];
-        else red_d[wv][c][lane] = acc[c];
-    }
+    for (int c = 0; c < NV; c++)
+        red[wv][c][lane] = FX ? (unsigned long long)q[c] : (unsigned long long)__double_as_longlong(acc[c]);
     __syncthreads();
     if (wv != 0 || si >= a.n) return;
 #pragma unroll
     for (int c = 0; c < NV; c++) {
         if (!want[c]) continue;
-        const double v = FX ? (double)(red_q[0][c][lane] + red_q[1][c][lane] + red_q[2][c][lane] + red_q[3][c][lane]) /
-                                  a.fxscale[c]
-                            : ((red_d[0][c][lane] + red_d[1][c][lane]) + red_d[2][c][lane]) + red_d[3][c][lane];
+        double v;
+        if (FX) {
+            long long sq = 0;
+            for (int w = 0; w < RED_W; w++) sq += (long long)red[w][c][lane];
+            v = (double)sq / a.fxscale[c];
+        } else {
+            v = 0.0;
+            for (int w = 0; w < RED_W; w++) v += __longlong_as_double((long long)red[w][c][lane]);
+        }
         const size_t idx = a.caller_order[c] ? (size_t)a.stride[c] * a.perm[si] + a.off[c] : (size_t)a.stride[c] * si + a.off[c];
         a.out[c][idx] += v;
         if (a.out2[c]) a.out2[c][idx] += v;
@@ -823,7 +827,7 @@ int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fi
     a.nblk = c->nblk; a.n = c->n;
     a.tiles = c->tiles.p; a.adj = c->adj.p; a.adjn = c->adjn.p; a.perm = c->perm.p;
     a.part = c->tpart.p;
-    const dim3 g(c->nblk), b(256);
+    const dim3 g(c->nblk), b(64 * RED_W);
 #define PMQ_RED(P, NV, FX) hipLaunchKernelGGL((k_tile_reduce<P, NV, FX>), g, b, 0, c->stream, a)
     if (float_parts) {
         if (nv == 3) { if (fixed) PMQ_RED(float, 3, true); else PMQ_RED(float, 3, false); }

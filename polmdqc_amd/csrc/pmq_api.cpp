@@ -606,23 +606,31 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
     const int n = c->n;
     int rc;
     if ((rc = status_reset(c))) return rc;
-    if ((rc = launch_zero(c, (double *)c->totals.p, sizeof(Totals) / sizeof(double)))) return rc;
     const bool grad = mode == PMQ_GRADIENT || mode == PMQ_VIRIAL;
-    if (grad) {
-        if ((rc = launch_zero(c, c->dem.p, 3 * (size_t)n))) return rc;
-        if ((rc = launch_zero(c, c->te.p, 3 * (size_t)n))) return rc;
-        if ((rc = launch_zero(c, c->pot.p, n))) return rc;
-        if (do_polar) {
-            if ((rc = launch_zero(c, c->dep.p, 3 * (size_t)n))) return rc;
-            if ((rc = launch_zero(c, c->tep.p, 3 * (size_t)n))) return rc;
+    {
+        // the evaluation's accumulators, cleared by one launch
+        double *zp[PMQ_ZERO_MAX];
+        size_t zn[PMQ_ZERO_MAX];
+        int k = 0;
+        auto add = [&](double *p, size_t cnt) { zp[k] = p; zn[k] = cnt; k++; };
+        add((double *)c->totals.p, sizeof(Totals) / sizeof(double));
+        if (grad) {
+            add(c->dem.p, 3 * (size_t)n);
+            add(c->te.p, 3 * (size_t)n);
+            add(c->pot.p, n);
+            if (do_polar) {
+                add(c->dep.p, 3 * (size_t)n);
+                add(c->tep.p, 3 * (size_t)n);
+            }
+            if (do_rep) {
+                add(c->der.p, 3 * (size_t)n);
+                add(c->ter.p, 3 * (size_t)n);
+            }
+            if (do_disp) add(c->dedsp.p, 3 * (size_t)n);
         }
-        if (do_rep) {
-            if ((rc = launch_zero(c, c->der.p, 3 * (size_t)n))) return rc;
-            if ((rc = launch_zero(c, c->ter.p, 3 * (size_t)n))) return rc;
-        }
-        if (do_disp && (rc = launch_zero(c, c->dedsp.p, 3 * (size_t)n))) return rc;
+        if (mode == PMQ_ANALYSIS) add(c->aem.p, n);
+        if ((rc = launch_zero_many(c, k, zp, zn))) return rc;
     }
-    if (mode == PMQ_ANALYSIS && (rc = launch_zero(c, c->aem.p, n))) return rc;
     c->pcg_iters = 0;
     if (do_mpole || do_polar || do_rep || do_disp) {
         double cut = ewald ? c->prm.ewaldcut : c->prm.mpolecut;

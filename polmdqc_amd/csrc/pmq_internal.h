@@ -365,6 +365,8 @@ int launch_chgflx_delta(pmq_ctx *c);
 int launch_mpole_direct(pmq_ctx *c, int mode, double off2, bool mixed);
 int launch_fx_flush(pmq_ctx *c, long long *q, double *d, size_t n, double scale);
 int launch_torque(pmq_ctx *c, int mode, const double *te, double *de, bool mixed);
+constexpr int PMQ_ZERO_MAX = 12;
+int launch_zero_many(pmq_ctx *c, int k, double *const *p, const size_t *n);
 int pme_setup(pmq_ctx *c);
 int fft_own_setup(pmq_ctx *c);
 bool fft_own_fits(const pmq_ctx *c, bool fp32);
