@@ -2,8 +2,8 @@
 //
 // Every evaluation: (1) key each atom by the Morton index of its cell
 // (cells about 4 A wide in the box's fractional frame, or in the bounding box
-// of a non-periodic system) with the atom index in the low 32 bits, so the
-// sort is total and deterministic; (2) radix-sort the keys; (3) gather the
+// of a non-periodic system; with PME the grid's spreading bricks) with the
+// atom index in the low 32 bits, so the sort is total and deterministic; (2) radix-sort the keys; (3) gather the
 // permutation; (4) bound each 64-atom block; (5) keep every block pair
 // (I <= J) whose bounds come within the cutoff, in (I, J) order.
 //
@@ -17,6 +17,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
+#include "pme_common.h"
 #include "pmq_internal.h"
 
 namespace pmq {
@@ -133,6 +134,19 @@ __global__ void k_cell_keys(int n, DevBox b, const double *__restrict__ bbox, in
     const unsigned cz = min((int)(f[2] * ncz), ncz - 1);
     const unsigned m = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
     keys[i] = ((unsigned long long)m << 32) | (unsigned)i;
+}
+
+// with PME on: the cells are the spreading bricks (home_brick, the same
+// arithmetic the spreading kernels use), so every brick's atoms end up
+// contiguous in the sorted order
+__global__ void k_brick_keys(int n, BrickGeom g, const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ z, unsigned long long *__restrict__ keys)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int hb[3];
+    home_brick(g.A, g.K, g.nb, x[i], y[i], z[i], hb);
+    keys[i] = ((unsigned long long)morton3(hb[0], hb[1], hb[2]) << 32) | (unsigned)i;
 }
 
 // permutation, sorted positions and per-atom flags (bit0 valid, bit1 active)
@@ -298,8 +312,19 @@ int build_neighbors(pmq_ctx *c, double cutoff)
         for (int a = 0; a < 3; a++) nc[a] = std::max(1, std::min(1024, (int)(len[a] / 4.0)));
     }
     const int tb = 256;
-    hipLaunchKernelGGL(k_cell_keys, dim3((n + tb - 1) / tb), dim3(tb), 0, st, n, b, c->bbox.p, nc[0], nc[1], nc[2],
-                       c->x.p, c->y.p, c->z.p, c->keys.p);
+    if (c->pme_cells) {
+        BrickGeom g;
+        for (int a = 0; a < 3; a++) {
+            for (int q = 0; q < 3; q++) g.A[a][q] = c->pmeA[a][q];
+            g.K[a] = c->K[a];
+            g.nb[a] = c->pme_nb[a];
+        }
+        hipLaunchKernelGGL(k_brick_keys, dim3((n + tb - 1) / tb), dim3(tb), 0, st, n, g, c->x.p, c->y.p, c->z.p,
+                           c->keys.p);
+    } else {
+        hipLaunchKernelGGL(k_cell_keys, dim3((n + tb - 1) / tb), dim3(tb), 0, st, n, b, c->bbox.p, nc[0], nc[1],
+                           nc[2], c->x.p, c->y.p, c->z.p, c->keys.p);
+    }
     size_t bytes = c->scratch_sort_bytes;
     PMQ_HIP(rocprim::radix_sort_keys(c->scratch_sort.p, bytes, c->keys.p, c->keys_sorted.p, (size_t)n, 0, 64, st));
     hipLaunchKernelGGL(k_gather, dim3((npad + tb - 1) / tb), dim3(tb), 0, st, n, npad, c->keys_sorted.p, c->x.p,

@@ -98,13 +98,12 @@ int pme_setup(pmq_ctx *c)
     for (int a = 0; a < 3 && same_box; a++)
         for (int b = 0; b < 3; b++) same_box = same_box && c->pme_dbox.recip[a][b] == c->box.recip[a][b];
     PMQ_HIP(c->pme_frac.alloc(40 * (size_t)c->npad)); // follows the atom count
-    PMQ_HIP(c->pme_keys.alloc(c->n));
-    PMQ_HIP(c->pme_keys_sorted.alloc(c->n));
     if (same_box && c->plan_f1) return PMQ_OK;
     c->K[0] = K[0]; c->K[1] = K[1]; c->K[2] = K[2];
     c->bsorder = order;
     for (int a = 0; a < 3; a++) c->pme_nb[a] = brick_count(K[a], order);
-    PMQ_HIP(c->pme_bstart.alloc((size_t)c->pme_nb[0] * c->pme_nb[1] * c->pme_nb[2] + 1));
+    PMQ_HIP(c->pme_bstart.alloc((size_t)c->pme_nb[0] * c->pme_nb[1] * c->pme_nb[2]));
+    PMQ_HIP(c->pme_bend.alloc((size_t)c->pme_nb[0] * c->pme_nb[1] * c->pme_nb[2]));
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++) c->pmeA[a][b] = K[a] * c->box.recip[a][b];
     // B-spline moduli, zeros replaced by the mean of their neighbours
@@ -244,7 +243,7 @@ __device__ __forceinline__ void atom_frac(const PmeGeom &g, double px, double py
     const int K[3] = {g.K0, g.K1, g.K2};
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        const double u = g.A[a][0] * p[0] + g.A[a][1] * p[1] + g.A[a][2] * p[2];
+        const double u = grid_coord(g.A[a], p[0], p[1], p[2]);
         const double fl = floor(u);
         w[a] = u - fl;
         int s = ((int)fl - ord + 1) % K[a];
@@ -290,18 +289,20 @@ __device__ __forceinline__ void frac_mpole(const PmeGeom &g, const double *m, do
     fm[7] = 2 * F[0][1]; fm[8] = 2 * F[0][2]; fm[9] = 2 * F[1][2];
 }
 
-// Spreading is owner-computes over bricks of the grid.  Atoms are binned by
+// Spreading is owner-computes over bricks of the grid.  Atoms are homed in
 // the brick holding the last point of their stencil (floor(u) on each axis);
 // every brick is at least order-1 points wide, so the stencil of an atom
 // homed in brick h covers only bricks h-1 and h on each axis, and brick b
 // receives exactly the atoms homed in b and b+1.  Each grid point is summed
-// by one lane and stored once: no atomics and no zeroing of the grid.
+// by one lane and stored once: no atomics and no zeroing of the grid.  With
+// PME on, the atom sort orders atoms by the Morton index of their home brick
+// (home_brick on the same coordinates), so each brick's atoms are one
+// contiguous range of the sorted arrays, [bstart, bend).
 struct BrickArgs {
-    int n, sbits, nset;
+    int n, nset;
     int nb[3];
     PmeGeom g;
-    const unsigned long long *keys; // (home brick << sbits) | sorted atom, ascending
-    const int *bstart;              // nbricks + 1
+    const int *bstart, *bend; // sorted-atom range of each brick (lexicographic brick index)
     int npad;
     const double *sx, *sy, *sz;
     const double *src;    // sorted SoA sources: 10 multipole comps (mode 0), 3 dipole comps x nset (mode 1)
@@ -311,42 +312,27 @@ struct BrickArgs {
     size_t gstride;
 };
 
-__device__ __host__ __forceinline__ int brick_lo(int b, int K, int nb) { return (int)(((long long)b * K) / nb); }
-
-// home brick of every atom: the brick of floor(u) on each axis
-__global__ void k_pme_brick_keys(int n, int sbits, int nb0, int nb1, int nb2, PmeGeom g, const double *sx,
-                                 const double *sy, const double *sz, unsigned long long *keys)
-{
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const double p[3] = {sx[s], sy[s], sz[s]};
-    const int K[3] = {g.K0, g.K1, g.K2}, nb[3] = {nb0, nb1, nb2};
-    int hb[3];
-    for (int a = 0; a < 3; a++) {
-        const double u = g.A[a][0] * p[0] + g.A[a][1] * p[1] + g.A[a][2] * p[2];
-        int h = (int)floor(u) % K[a];
-        if (h < 0) h += K[a];
-        int b = (int)(((long long)h * nb[a]) / K[a]);
-        while (b + 1 < nb[a] && brick_lo(b + 1, K[a], nb[a]) <= h) b++;
-        while (b > 0 && brick_lo(b, K[a], nb[a]) > h) b--;
-        hb[a] = b;
-    }
-    const unsigned long long brick = ((unsigned long long)hb[2] * nb1 + hb[1]) * nb0 + hb[0];
-    keys[s] = (brick << sbits) | (unsigned long long)s;
-}
-
-// first sorted position of every brick (lower bound), bstart[nbricks] = n
-__global__ void k_pme_brick_start(int n, int nbricks, int sbits, const unsigned long long *keys, int *bstart)
+// sorted-atom range of every brick: the keys of the atom sort are
+// (Morton index of the home brick << 32) | atom, ascending
+__global__ void k_pme_brick_ranges(int n, int nb0, int nb1, int nb2, const unsigned long long *keys, int *bstart,
+                                   int *bend)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b > nbricks) return;
-    const unsigned long long key = (unsigned long long)b << sbits;
+    if (b >= nb0 * nb1 * nb2) return;
+    const unsigned m = morton3(b % nb0, (b / nb0) % nb1, b / (nb0 * nb1));
+    const unsigned long long k0 = (unsigned long long)m << 32, k1 = (unsigned long long)(m + 1) << 32;
     int lo = 0, hi = n;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (keys[mid] < key) lo = mid + 1; else hi = mid;
+        if (keys[mid] < k0) lo = mid + 1; else hi = mid;
     }
     bstart[b] = lo;
+    hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] < k1) lo = mid + 1; else hi = mid;
+    }
+    bend[b] = lo;
 }
 
 // One workgroup of 4 waves per brick; lane = grid point (bricks of more than
@@ -379,7 +365,6 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
               s2 = brick_lo(bz + 1, K2, a.nb[2]) - lo2;
     const int bsz = s0 * s1 * s2;
     const int nset = MODE == 1 ? a.nset : 1; // 1 or 2
-    const unsigned long long smask = (1ull << a.sbits) - 1ull;
     // the atoms homed in bricks b + (0|1, 0|1, 0|1), each brick once
     int hs[8], hc[8], ntot = 0;
 #pragma unroll
@@ -391,7 +376,7 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
         const int hx = (bx + dx) % a.nb[0], hy = (by + dy) % a.nb[1], hz = (bz + dz) % a.nb[2];
         const int hb = (hz * a.nb[1] + hy) * a.nb[0] + hx;
         hs[h] = a.bstart[hb];
-        hc[h] = a.bstart[hb + 1] - hs[h];
+        hc[h] = a.bend[hb] - hs[h];
         ntot += hc[h];
     }
     for (int pbase = 0; pbase < bsz; pbase += 64) {
@@ -414,7 +399,7 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
                     if (idx >= off && idx < off + hc[h]) e = hs[h] + idx - off;
                     off += hc[h];
                 }
-                const int s = (int)(a.keys[e] & smask);
+                const int s = e;
                 double w[3];
                 int i0[3];
                 atom_frac(a.g, a.sx[s], a.sy[s], a.sz[s], w, i0, ORD);
@@ -893,25 +878,17 @@ static dim3 lanes_grid(int n, int order) { return dim3((unsigned)(((size_t)n * (
     default: hipLaunchKernelGGL((KERNEL<6 __VA_ARGS__>), lanes_grid(c->n, 6), dim3(256), 0, c->stream, args); break; \
     }
 
-// bin the atoms by home brick (once per evaluation; positions are fixed
-// for all spreads of the evaluation)
-static int pme_bin(pmq_ctx *c)
+// the sorted-atom range of every brick (once per evaluation; the atom sort
+// of this evaluation ordered the atoms by home brick)
+static int pme_brick_ranges(pmq_ctx *c)
 {
-    const int n = c->n;
-    PmeGeom g = geom(c);
-    int sbits = 1;
-    while ((1 << sbits) < c->npad) sbits++;
-    c->pme_sbits = sbits;
+    if (!c->pme_cells) {
+        c->err = "PME: the atoms were not sorted by spreading brick";
+        return PMQ_ERR_STATE;
+    }
     const int nbricks = c->pme_nb[0] * c->pme_nb[1] * c->pme_nb[2];
-    int bbits = 1;
-    while ((1ll << bbits) <= nbricks) bbits++;
-    hipLaunchKernelGGL(k_pme_brick_keys, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, sbits, c->pme_nb[0],
-                       c->pme_nb[1], c->pme_nb[2], g, c->sx.p, c->sy.p, c->sz.p, c->pme_keys.p);
-    size_t bytes = c->scratch_sort_bytes;
-    PMQ_HIP(rocprim::radix_sort_keys(c->scratch_sort.p, bytes, c->pme_keys.p, c->pme_keys_sorted.p, (size_t)n, 0,
-                                     sbits + bbits, c->stream));
-    hipLaunchKernelGGL(k_pme_brick_start, dim3((nbricks + 1 + 255) / 256), dim3(256), 0, c->stream, n, nbricks, sbits,
-                       c->pme_keys_sorted.p, c->pme_bstart.p);
+    hipLaunchKernelGGL(k_pme_brick_ranges, dim3((nbricks + 255) / 256), dim3(256), 0, c->stream, c->n, c->pme_nb[0],
+                       c->pme_nb[1], c->pme_nb[2], c->keys_sorted.p, c->pme_bstart.p, c->pme_bend.p);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }
@@ -922,10 +899,10 @@ static int spread(pmq_ctx *c, int mode, const double *src, const double *charge,
 {
     const bool fp32 = c->pme_fp32 && mode != 2, g32 = grid32(c) && mode != 2;
     BrickArgs args;
-    args.n = c->n; args.sbits = c->pme_sbits; args.nset = nset;
+    args.n = c->n; args.nset = nset;
     for (int a = 0; a < 3; a++) args.nb[a] = c->pme_nb[a];
     args.g = geom(c);
-    args.keys = c->pme_keys_sorted.p; args.bstart = c->pme_bstart.p;
+    args.bstart = c->pme_bstart.p; args.bend = c->pme_bend.p;
     args.npad = c->npad;
     args.sx = c->sx.p; args.sy = c->sy.p; args.sz = c->sz.p;
     args.src = src;
@@ -1042,7 +1019,7 @@ int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar)
 {
     int rc;
     if ((rc = choose_fft(c))) return rc;
-    if ((rc = pme_bin(c))) return rc;
+    if ((rc = pme_brick_ranges(c))) return rc;
     const double *charge = c->prm.pentype != PMQ_PEN_NONE ? c->scp.p + 2 * (size_t)c->npad : nullptr;
     if ((rc = spread(c, 0, c->smp.p, charge, 1))) return rc;
     if ((rc = pme_potential(c, 1))) return rc;
@@ -1135,7 +1112,7 @@ int pme_disp(pmq_ctx *c, bool grad, bool need_bin)
 {
     int rc;
     if ((rc = choose_fft(c))) return rc;
-    if (need_bin && (rc = pme_bin(c))) return rc;
+    if (need_bin && (rc = pme_brick_ranges(c))) return rc;
     if ((rc = spread(c, 2, c->shp.p, nullptr, 1))) return rc;
     if ((rc = pme_potential(c, 1, c->pme_gd.p))) return rc;
     DispPostArgs args;

@@ -13,6 +13,51 @@
 
 namespace pmq {
 
+// grid coordinate u = A p of one axis (A row = K times a reciprocal box
+// row), with explicit fused multiply-adds so every kernel that bins or
+// stencils an atom gets the same bits
+__device__ __forceinline__ double grid_coord(const double *A, double px, double py, double pz)
+{
+    return fma(A[0], px, fma(A[1], py, A[2] * pz));
+}
+
+// spreading bricks: axis a is cut at brick_lo(b) = floor(b K / nb)
+__device__ __host__ __forceinline__ int brick_lo(int b, int K, int nb) { return (int)(((long long)b * K) / nb); }
+
+// Morton interleave of three 10-bit cell or brick indices
+__device__ __forceinline__ unsigned morton3(unsigned x, unsigned y, unsigned z)
+{
+    auto spread = [](unsigned v) {
+        v &= 0x3ff;
+        v = (v | (v << 16)) & 0x030000ff;
+        v = (v | (v << 8)) & 0x0300f00f;
+        v = (v | (v << 4)) & 0x030c30c3;
+        v = (v | (v << 2)) & 0x09249249;
+        return v;
+    };
+    return spread(x) | (spread(y) << 1) | (spread(z) << 2);
+}
+
+// the grid's brick layout, handed to the atom sort
+struct BrickGeom {
+    double A[3][3];
+    int K[3], nb[3];
+};
+
+// home brick of a point: per axis the brick holding floor(u) mod K
+__device__ __forceinline__ void home_brick(const double (*A)[3], const int *K, const int *nb, double px, double py,
+                                           double pz, int *hb)
+{
+    for (int a = 0; a < 3; a++) {
+        int h = (int)floor(grid_coord(A[a], px, py, pz)) % K[a];
+        if (h < 0) h += K[a];
+        int b = (int)(((long long)h * nb[a]) / K[a]);
+        while (b + 1 < nb[a] && brick_lo(b + 1, K[a], nb[a]) <= h) b++;
+        while (b > 0 && brick_lo(b, K[a], nb[a]) > h) b--;
+        hb[a] = b;
+    }
+}
+
 template <class R, int ND>
 __device__ __forceinline__ void bspline_weights(R w, int n, R th[PMQ_MAX_ORDER][4])
 {

@@ -641,6 +641,11 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
         const bool cf = c->prm.pentype != PMQ_PEN_NONE && c->prm.use_chgflx;
         if (c->prm.use_chgflx && (rc = launch_chgflx_delta(c))) return rc;
         c->prec_on = do_polar && c->prm.pcgprec && c->prm.usolvcut > 0.0;
+        // with PME the atom sort follows the grid's spreading bricks
+        c->pme_fp32 = mixed; // electrostatic grids in FP32 for the mixed form
+        const bool pme = (ewald && (do_mpole || do_polar)) || dewald;
+        if (pme && (rc = pme_setup(c))) return rc;
+        c->pme_cells = pme;
         {
             ProfScope ps(c, "neighbors");
             if ((rc = build_neighbors(c, cut))) return rc;
@@ -663,9 +668,7 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
             if ((rc = launch_fx_flush(c, c->fx_grad.p, c->dem.p, 3 * (size_t)n, PMQ_FX_GRAD))) return rc;
             if ((rc = launch_fx_flush(c, c->fx_grad.p + 3 * (size_t)n, c->te.p, 3 * (size_t)n, PMQ_FX_GRAD))) return rc;
         }
-        c->pme_fp32 = mixed; // electrostatic grids in FP32 for the mixed form
-        if ((ewald && (do_mpole || do_polar)) || dewald) {
-            if ((rc = pme_setup(c))) return rc;
+        if (pme) {
             if (ewald && (do_mpole || do_polar) && (rc = pme_mpole(c, mode, do_mpole, do_polar))) return rc;
             if (dewald && (rc = pme_disp(c, grad, !(ewald && (do_mpole || do_polar))))) return rc;
         }

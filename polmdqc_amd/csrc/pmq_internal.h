@@ -318,9 +318,8 @@ struct pmq_ctx {
     pmq::DBuf<double> cgrid;                     // 2 complex half grids (interleaved re, im)
     pmq::DBuf<double> pme_frac;                  // 2 x 20 fractional potential derivatives per atom
     int pme_nb[3] = {1, 1, 1};                   // spreading bricks per axis
-    int pme_sbits = 1;                           // bits of the atom index in the brick keys
-    pmq::DBuf<unsigned long long> pme_keys, pme_keys_sorted; // (home brick << sbits) | sorted atom
-    pmq::DBuf<int> pme_bstart;                   // first key of each brick
+    bool pme_cells = false;                      // this evaluation's atom sort ordered atoms by spreading brick
+    pmq::DBuf<int> pme_bstart, pme_bend;         // sorted-atom range of each brick
     pmq::DBuf<unsigned char> fft_work;
     void *plan_f1 = nullptr, *plan_b1 = nullptr, *plan_f2 = nullptr, *plan_b2 = nullptr;
     // FP32 plans, kernel table and flag of the mixed form's electrostatic PME
