@@ -405,8 +405,10 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
                 atom_frac(a.g, a.sx[s], a.sy[s], a.sz[s], w, i0, ORD);
 #pragma unroll
                 for (int ax = 0; ax < 3; ax++) {
-                    T t[PMQ_MAX_ORDER][4];
-                    bspline_weights<T, ND>(T(w[ax]), ORD, t);
+                    // weights in FP64 whatever T is (FP64 costs what FP32
+                    // does here and keeps the mixed form's energy error down)
+                    double t[PMQ_MAX_ORDER][4];
+                    bspline_weights<double, ND>(w[ax], ORD, t);
 #pragma unroll
                     for (int k = 0; k < ORD; k++)
 #pragma unroll
@@ -606,12 +608,19 @@ __device__ __forceinline__ void frac_to_cart(const PmeGeom &g, const double *f, 
     double w[3];                                                                                             \
     int i0[3];                                                                                               \
     atom_frac(a.g, a.sx[s], a.sy[s], a.sz[s], w, i0, ORD);                                                   \
-    T tx[PMQ_MAX_ORDER][4], tyf[PMQ_MAX_ORDER][4], tzf[PMQ_MAX_ORDER][4], ty[4], tz[4];                      \
-    bspline_weights<T, ND>(T(w[0]), ORD, tx);                                                                \
-    bspline_weights<T, ND>(T(w[1]), ORD, tyf);                                                               \
-    bspline_weights<T, ND>(T(w[2]), ORD, tzf);                                                               \
-    pick<ORD, ND, T>(tyf, ky, ty);                                                                           \
-    pick<ORD, ND, T>(tzf, kz, tz);
+    T tx[PMQ_MAX_ORDER][4], ty[4], tz[4];                                                                    \
+    {                                                                                                        \
+        /* weights in FP64 whatever T is, then rounded to T */                                               \
+        double bx[PMQ_MAX_ORDER][4], byf[PMQ_MAX_ORDER][4], bzf[PMQ_MAX_ORDER][4], by[4], bz[4];             \
+        bspline_weights<double, ND>(w[0], ORD, bx);                                                          \
+        bspline_weights<double, ND>(w[1], ORD, byf);                                                         \
+        bspline_weights<double, ND>(w[2], ORD, bzf);                                                         \
+        pick<ORD, ND, double>(byf, ky, by);                                                                  \
+        pick<ORD, ND, double>(bzf, kz, bz);                                                                  \
+        _Pragma("unroll") for (int k = 0; k < ORD; k++)                                                      \
+            _Pragma("unroll") for (int d = 0; d <= ND; d++) tx[k][d] = T(bx[k][d]);                          \
+        _Pragma("unroll") for (int d = 0; d <= ND; d++) { ty[d] = T(by[d]); tz[d] = T(bz[d]); }            \
+    }
 
 struct GatherFracArgs {
     int n, stride, nset;
