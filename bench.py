@@ -205,6 +205,8 @@ def bench_hippo(eng, L, steps, warmup) -> dict:
     out["pcg_iters"] = {"fp64": r64["pcg_iters"], "mixed": r32["pcg_iters"]}
     out["kernel_ms_per_step_fp64"] = {k: round(v[0] / steps, 4) for k, v in prof64.items() if v[1]}
     out["kernel_ms_per_step_mixed"] = {k: round(v[0] / steps, 4) for k, v in prof32.items() if v[1]}
+    from polmdqc_amd.engine import fft_choice
+    out["pme_transforms"] = fft_choice(eng)
     return out
 
 
@@ -226,7 +228,7 @@ def main() -> int:
         return 2
 
     from polmdqc_amd import _lib as L
-    from polmdqc_amd.engine import Engine
+    from polmdqc_amd.engine import Engine, fft_choice
 
     parts = set(args.parts.split(","))
     if "amoeba" not in parts:
@@ -299,6 +301,7 @@ def main() -> int:
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items() if v[1]},
         "kernel_ms_per_step_fp64": {k: round(v[0] / args.steps, 4) for k, v in prof64.items() if v[1]},
         "pairs_in_cutoff": pairs, "pcg_iters": res["pcg_iters"],
+        "pme_transforms": fft_choice(eng),
         "energy_kcal": {"em": res["em"], "ep": res["ep"]},
     }
     if "clusters" in parts:

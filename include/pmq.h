@@ -253,16 +253,21 @@ PMQ_API int pmq_profile_enable(pmq_ctx *ctx, int on);
 PMQ_API int pmq_profile_read(pmq_ctx *ctx, const char *name, double *total_ms, long long *launches);
 
 /* PME transform engine.  No counterpart in the reference (it has no FFT
- * code); for tuning and tests.  PMQ_FFT_LIBRARY (the default) takes rocFFT;
- * PMQ_FFT_DIRECT takes the direct LDS transforms (three launches per
- * convolution, the reciprocal kernel applied between the z transforms)
- * wherever the grid's planes fit in LDS; PMQ_FFT_AUTO times the two once per
- * grid and keeps the faster (a profiler that adds a cost per launch biases
- * that choice). */
+ * code); for tuning and tests.  PMQ_FFT_AUTO (the default) times, once per
+ * grid, rocFFT against the direct LDS transforms (three launches per
+ * convolution, the reciprocal kernel applied between the z transforms;
+ * grids whose planes fit in LDS) and keeps the faster, separately for FP64
+ * and FP32 grids (a profiler that adds a cost per launch biases that
+ * choice); PMQ_FFT_LIBRARY always takes rocFFT; PMQ_FFT_DIRECT takes the
+ * direct transforms wherever they fit.  pmq_fft_choice reports the choice
+ * made for the current grid: direct transforms for FP64 grids, FP32 grids
+ * in the mixed form, direct transforms for FP32 grids (each 0/1, or -1 while
+ * not yet chosen). */
 #define PMQ_FFT_AUTO 0
 #define PMQ_FFT_LIBRARY 1
 #define PMQ_FFT_DIRECT 2
 PMQ_API int pmq_set_fft(pmq_ctx *ctx, int engine);
+PMQ_API int pmq_fft_choice(pmq_ctx *ctx, int *fp64_direct, int *fp32_grids, int *fp32_direct);
 
 #ifdef __cplusplus
 }

@@ -968,9 +968,10 @@ static int gather_frac(pmq_ctx *c, int nset)
 // forward + inverse pairs of the two-set transform each) and the faster
 // kept (FP32 wins ties)
 // The transforms of the grid, timed once per grid on the evaluation's own
-// stream (two-set convolutions, the shape of every PCG iteration): for FP64
-// grids the direct LDS transforms against rocFFT; in the mixed form also
-// FP32 grids (the faster of the two engines), kept when they beat FP64.
+// stream (two-set convolutions, the shape of every PCG iteration; two
+// rounds, the best of each): for FP64 grids the direct LDS transforms
+// against rocFFT; in the mixed form also FP32 grids (the faster of the two
+// engines), kept when they beat FP64.
 static int time_convolution(pmq_ctx *c, float *ms)
 {
     hipEvent_t e[2];
@@ -993,35 +994,35 @@ static int choose_fft(pmq_ctx *c)
 {
     if (c->pme_own64 >= 0 && (!c->pme_fp32 || c->pme_fft32 >= 0)) return PMQ_OK;
     const bool fp32_form = c->pme_fp32;
-    int rc = PMQ_OK;
     const bool lib = c->fft_engine != PMQ_FFT_DIRECT, own = c->fft_engine != PMQ_FFT_LIBRARY;
-    float t64 = 1e30f, t32 = 1e30f, t;
-    // FP64 grids
-    c->pme_fp32 = false;
-    c->pme_own64 = own && fft_own_fits(c, false) ? 1 : 0;
-    if (c->pme_own64 && lib) {
-        if ((rc = time_convolution(c, &t64))) goto done;
-        c->pme_own64 = 0;
-        if ((rc = time_convolution(c, &t))) goto done;
-        if (t < t64) t64 = t;
-        else c->pme_own64 = 1;
-    } else if (fp32_form && (rc = time_convolution(c, &t64))) goto done;
-    if (fp32_form && c->pme_fft32 < 0) {
-        c->pme_fp32 = true;
-        c->pme_fft32 = 1;
-        c->pme_own32 = own && fft_own_fits(c, true) ? 1 : 0;
-        if ((rc = time_convolution(c, &t32))) goto done;
-        if (c->pme_own32 && lib) {
-            c->pme_own32 = 0;
-            if ((rc = time_convolution(c, &t))) goto done;
-            if (t < t32) t32 = t;
-            else c->pme_own32 = 1;
-        }
-        c->pme_fft32 = t32 <= t64 ? 1 : 0;
+    // candidates: (FP32 grids?, direct transforms?); two rounds, best of each
+    struct Cand { bool fp32; int own; float best; };
+    Cand cand[4];
+    int nc = 0;
+    for (int p = 0; p < (fp32_form ? 2 : 1); p++) {
+        const bool fits = fft_own_fits(c, p == 1);
+        if (lib || !fits) cand[nc++] = Cand{p == 1, 0, 1e30f};
+        if (own && fits) cand[nc++] = Cand{p == 1, 1, 1e30f};
     }
-done:
+    int rc = PMQ_OK;
+    if (nc > 1 || fp32_form)
+        for (int round = 0; round < 2 && !rc; round++)
+            for (int q = 0; q < nc && !rc; q++) {
+                c->pme_fp32 = cand[q].fp32;
+                if (cand[q].fp32) { c->pme_fft32 = 1; c->pme_own32 = cand[q].own; }
+                else c->pme_own64 = cand[q].own;
+                float ms = 0.0f;
+                if (!(rc = time_convolution(c, &ms))) cand[q].best = std::min(cand[q].best, ms);
+            }
     c->pme_fp32 = fp32_form;
-    return rc;
+    if (rc) return rc;
+    float b64 = 1e30f, b32 = 1e30f;
+    for (int q = 0; q < nc; q++) {
+        if (!cand[q].fp32 && (cand[q].best < b64 || b64 == 1e30f)) { b64 = cand[q].best; c->pme_own64 = cand[q].own; }
+        if (cand[q].fp32 && (cand[q].best < b32 || b32 == 1e30f)) { b32 = cand[q].best; c->pme_own32 = cand[q].own; }
+    }
+    if (fp32_form) c->pme_fft32 = b32 <= b64 ? 1 : 0;
+    return PMQ_OK;
 }
 
 int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar)

@@ -107,6 +107,16 @@ int pmq_set_fft(pmq_ctx *c, int engine)
     return PMQ_OK;
 }
 
+int pmq_fft_choice(pmq_ctx *c, int *fp64_direct, int *fp32_grids, int *fp32_direct)
+{
+    CHECK_CTX(c);
+    if (!fp64_direct || !fp32_grids || !fp32_direct) return PMQ_ERR_ARG;
+    *fp64_direct = c->pme_own64;
+    *fp32_grids = c->pme_fft32;
+    *fp32_direct = c->pme_own32;
+    return PMQ_OK;
+}
+
 static void batch_graph_release(pmq_ctx *c)
 {
     if (c->b_exec) (void)hipGraphExecDestroy((hipGraphExec_t)c->b_exec);

@@ -273,14 +273,26 @@ KERNEL_FAMILIES = ("neighbors", "mpole_tiles", "rep_tiles", "disp_tiles", "pol_f
 FFT_AUTO, FFT_LIBRARY, FFT_DIRECT = 0, 1, 2
 
 
-def set_fft(eng: Engine, engine: int = FFT_LIBRARY):
-    """PME transform engine: rocFFT (FFT_LIBRARY, the default), the direct
-    LDS transforms where the grid fits them (FFT_DIRECT), or a timed choice
-    of the two (FFT_AUTO)."""
+def set_fft(eng: Engine, engine: int = FFT_AUTO):
+    """PME transform engine: a timed choice (FFT_AUTO, the default), rocFFT
+    (FFT_LIBRARY), or the direct LDS transforms where the grid fits them
+    (FFT_DIRECT)."""
     L = eng._L
     L.pmq_set_fft.argtypes = [C.c_void_p, C.c_int]
     L.pmq_set_fft.restype = C.c_int
     eng._check(L.pmq_set_fft(eng._ctx, int(engine)), "pmq_set_fft")
+
+
+def fft_choice(eng: Engine) -> dict:
+    """The transforms chosen for the current grid (after a PME evaluation)."""
+    L = eng._L
+    L.pmq_fft_choice.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 3
+    L.pmq_fft_choice.restype = C.c_int
+    v = [C.c_int() for _ in range(3)]
+    eng._check(L.pmq_fft_choice(eng._ctx, *[C.byref(x) for x in v]), "pmq_fft_choice")
+    name = {-1: None, 0: "rocFFT", 1: "direct"}
+    return {"fp64_grids": name[v[0].value], "mixed_grids": {-1: None, 0: "fp64", 1: "fp32"}[v[1].value],
+            "fp32_grids": name[v[2].value]}
 
 
 def profile_enable(eng: Engine, on: bool = True):
