@@ -51,18 +51,18 @@ PEAK_HBM_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
-def hbm_bytes_per_launch(family: str, n: int, npad: int, nnz: int, grid: int, w: int):
+def hbm_bytes_per_launch(family: str, n: int, npad: int, nnz: int, grid: int, w: int, gw: int = 8):
     """Algorithmic HBM bytes of one launch of the memory-bound kernel families:
     every array the launch needs read once and every output written once.
     pol_mutual_spmv: column index (4 B) and 6 tensor components (w B each)
     per stored entry, the row pointers, both dipole sets read and both field
     sets written (6 x 8 B per atom each).  pme_spread (induced-dipole form):
     coordinates and both dipole sets read (9 x 8 B per atom), two grids
-    written."""
+    written (gw B per point: 4 for FP32 grids, else 8)."""
     if family == "pol_mutual_spmv":
         return nnz * (4 + 6 * w) + (npad + 1) * 4 + 2 * 6 * 8 * npad
     if family == "pme_spread":
-        return n * 9 * 8 + 2 * grid * 8
+        return n * 9 * 8 + 2 * grid * gw
     return None
 
 
@@ -281,7 +281,8 @@ def main() -> int:
                     "avg_launch_us": round(avg_s * 1e6, 2), "flops_per_launch": PAIR_FLOPS[dom] * pairs,
                     "note": "VALU pair kernel (no MFMA: pair terms are not a dense contraction); vector peak"}
     else:
-        nbytes = hbm_bytes_per_launch(dom, n, npad, 2 * pairs, 48 ** 3, w)
+        gw = 4 if mixed and fft_choice(eng)["mixed_grids"] == "fp32" else 8
+        nbytes = hbm_bytes_per_launch(dom, n, npad, 2 * pairs, 48 ** 3, w, gw)
         achieved = nbytes / avg_s / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": dom,
