@@ -211,12 +211,12 @@ int pme_setup(pmq_ctx *c)
 // ---------------------------------------------------------------------------
 // kernels
 //
-// One group of G lanes serves one atom: lane (ky, kz) of the order x order
-// stencil plane handles the `order` grid points along x (G = 32 for order
-// <= 5, 64 for order 6).  Spreading issues its x-row of atomics; gathering
-// accumulates the x-row and the group reduces with butterfly shuffles.  The
-// atoms of one wave are taken far apart in the spatial order (a fixed
-// stride coprime with n) so that their stencils rarely share grid points.
+// Gathering: one group of G lanes serves one atom: lane (ky, kz) of the
+// order x order stencil plane accumulates the `order` grid points along x
+// (G = 32 for order <= 5, 64 for order 6) and the group reduces with
+// butterfly shuffles.  Atoms go to workgroups in sorted (spatial) order,
+// each XCD taking a contiguous eighth, so the grid points a workgroup reads
+// are shared with its neighbours in the same L2.
 struct PmeGeom {
     int K0, K1, K2;
     double A[3][3];
@@ -358,7 +358,8 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
     GT *grid = (GT *)a.grid;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int K0 = a.g.K0, K1 = a.g.K1, K2 = a.g.K2;
-    const int b = blockIdx.x;
+    const int b = xcd_item(blockIdx.x, gridDim.x / PMQ_NXCD);
+    if (b >= a.nb[0] * a.nb[1] * a.nb[2]) return;
     const int bx = b % a.nb[0], by = (b / a.nb[0]) % a.nb[1], bz = b / (a.nb[0] * a.nb[1]);
     const int lo0 = brick_lo(bx, K0, a.nb[0]), lo1 = brick_lo(by, K1, a.nb[1]), lo2 = brick_lo(bz, K2, a.nb[2]);
     const int s0 = brick_lo(bx + 1, K0, a.nb[0]) - lo0, s1 = brick_lo(by + 1, K1, a.nb[1]) - lo1,
@@ -599,11 +600,11 @@ __device__ __forceinline__ void frac_to_cart(const PmeGeom &g, const double *f, 
 // common lane setup of the gather kernels
 #define GATHER_SETUP(ND, T)                                                                                  \
     constexpr int G = PmeLanes<ORD>::G;                                                                      \
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;                                                   \
-    const int slot = tid / G, l = tid % G;                                                                   \
+    const int slot = xcd_item(blockIdx.x, gridDim.x / PMQ_NXCD) * (blockDim.x / G) + threadIdx.x / G;        \
+    const int l = threadIdx.x % G;                                                                           \
     const bool valid = slot < a.n;                                                                           \
     const bool on = valid && l < ORD * ORD;                                                                  \
-    const int s = valid ? (int)(((long long)slot * a.stride) % a.n) : 0;                                     \
+    const int s = valid ? slot : 0;                                                                          \
     const int ky = l % ORD, kz = (l / ORD) % ORD;                                                            \
     double w[3];                                                                                             \
     int i0[3];                                                                                               \
@@ -623,7 +624,7 @@ __device__ __forceinline__ void frac_to_cart(const PmeGeom &g, const double *f, 
     }
 
 struct GatherFracArgs {
-    int n, stride, nset;
+    int n, nset;
     PmeGeom g;
     const double *sx, *sy, *sz;
     const void *grid;      // nset grids (double, or float in the mixed form)
@@ -727,7 +728,7 @@ __global__ __launch_bounds__(256) void k_pme_mpole_post(MpolePostArgs a)
 }
 
 struct GatherFieldArgs {
-    int n, npad, stride, nset;
+    int n, npad, nset;
     PmeGeom g;
     const double *sx, *sy, *sz;
     const void *grid;   // double, or float in the mixed form
@@ -869,16 +870,10 @@ static int pme_potential(pmq_ctx *c, int nset, const double *G = nullptr)
 
 static size_t grid_size(const pmq_ctx *c) { return (size_t)c->K[0] * c->K[1] * c->K[2]; }
 
-// a stride near n / golden ratio that is coprime with n
-static int atom_stride(int n)
+static dim3 lanes_grid(int n, int order)
 {
-    auto gcd = [](long a, long b) { while (b) { long t = a % b; a = b; b = t; } return a; };
-    long s = std::max(1L, (long)(0.6180339887 * n));
-    while (gcd(s, n) != 1) s++;
-    return (int)s;
+    return dim3(xcd_grid(((size_t)n * (order * order <= 32 ? 32 : 64) + 255) / 256));
 }
-
-static dim3 lanes_grid(int n, int order) { return dim3((unsigned)(((size_t)n * (order * order <= 32 ? 32 : 64) + 255) / 256)); }
 
 #define PMQ_ORDER_DISPATCH(KERNEL, ...)                                                                    \
     switch (c->bsorder) {                                                                                  \
@@ -922,7 +917,7 @@ static int spread(pmq_ctx *c, int mode, const double *src, const double *charge,
         c->err = "PME: dipole spreading takes two sets";
         return PMQ_ERR_ARG;
     }
-    const dim3 grid((unsigned)(c->pme_nb[0] * c->pme_nb[1] * c->pme_nb[2]));
+    const dim3 grid(xcd_grid((size_t)c->pme_nb[0] * c->pme_nb[1] * c->pme_nb[2]));
     ProfScope ps_(c, "pme_spread");
 #define PMQ_SPREAD(ORD, MODE)                                                                                  \
     do {                                                                                                       \
@@ -950,7 +945,7 @@ static int spread(pmq_ctx *c, int mode, const double *src, const double *charge,
 static int gather_frac(pmq_ctx *c, int nset)
 {
     GatherFracArgs args;
-    args.n = c->n; args.stride = atom_stride(c->n); args.nset = nset; args.g = geom(c);
+    args.n = c->n; args.nset = nset; args.g = geom(c);
     args.sx = c->sx.p; args.sy = c->sy.p; args.sz = c->sz.p;
     args.grid = c->qgrid.p;
     args.gstride = grid_size(c);
@@ -1062,7 +1057,7 @@ int pme_dipole_field(pmq_ctx *c, const double *dip, double *field)
     if ((rc = spread(c, 1, dip, nullptr, 2))) return rc;
     if ((rc = pme_potential(c, 2))) return rc;
     GatherFieldArgs args;
-    args.n = c->n; args.npad = c->npad; args.stride = atom_stride(c->n); args.nset = 2; args.g = geom(c);
+    args.n = c->n; args.npad = c->npad; args.nset = 2; args.g = geom(c);
     args.sx = c->sx.p; args.sy = c->sy.p; args.sz = c->sz.p;
     args.grid = c->qgrid.p;
     args.gstride = gs;
@@ -1083,7 +1078,7 @@ int pme_dipole_field(pmq_ctx *c, const double *dip, double *field)
 // every atom (one lane group per atom); the gradient is added in caller
 // order (each atom written once), the energy summed per wave into edsp
 struct DispPostArgs {
-    int n, npad, stride;
+    int n, npad;
     PmeGeom g;
     const double *sx, *sy, *sz, *cs; // sorted positions, C6 coefficients
     const int *perm;
@@ -1126,7 +1121,7 @@ int pme_disp(pmq_ctx *c, bool grad, bool need_bin)
     if ((rc = spread(c, 2, c->shp.p, nullptr, 1))) return rc;
     if ((rc = pme_potential(c, 1, c->pme_gd.p))) return rc;
     DispPostArgs args;
-    args.n = c->n; args.npad = c->npad; args.stride = atom_stride(c->n); args.g = geom(c);
+    args.n = c->n; args.npad = c->npad; args.g = geom(c);
     args.sx = c->sx.p; args.sy = c->sy.p; args.sz = c->sz.p; args.cs = c->shp.p;
     args.perm = c->perm.p;
     args.grid = c->qgrid.p;

@@ -13,6 +13,17 @@
 
 namespace pmq {
 
+// XCD-aware order of a launch of gridDim.x = 8 * per workgroups: the chip
+// deals workgroups to its 8 XCDs round robin (w % 8), so workgroup w takes
+// item (w % 8) * per + w / 8 of the launch's items: each XCD, with its own
+// L2, works on one contiguous eighth (spatially compact in the sorted order)
+constexpr int PMQ_NXCD = 8;
+__device__ __forceinline__ int xcd_item(int w, int per) { return (w % PMQ_NXCD) * per + w / PMQ_NXCD; }
+__host__ __device__ __forceinline__ unsigned xcd_grid(size_t items)
+{
+    return (unsigned)(((items + PMQ_NXCD - 1) / PMQ_NXCD) * PMQ_NXCD);
+}
+
 // grid coordinate u = A p of one axis (A row = K times a reciprocal box
 // row), with explicit fused multiply-adds so every kernel that bins or
 // stencils an atom gets the same bits
