@@ -269,6 +269,15 @@ PMQ_API int pmq_profile_read(pmq_ctx *ctx, const char *name, double *total_ms, l
 PMQ_API int pmq_set_fft(pmq_ctx *ctx, int engine);
 PMQ_API int pmq_fft_choice(pmq_ctx *ctx, int *fp64_direct, int *fp32_grids, int *fp32_direct);
 
+/* PME gather lanes.  No counterpart in the reference; for tuning and tests.
+ * PMQ_GATHER_AUTO (the default) gives each atom 8 lanes, one stencil plane
+ * each, from 32,768 atoms on and one lane per stencil row below; the other
+ * two values force one layout. */
+#define PMQ_GATHER_AUTO 0
+#define PMQ_GATHER_ROWS 1
+#define PMQ_GATHER_PLANES 2
+PMQ_API int pmq_set_gather_lanes(pmq_ctx *ctx, int mode);
+
 #ifdef __cplusplus
 }
 #endif

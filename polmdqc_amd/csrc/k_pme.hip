@@ -211,10 +211,12 @@ int pme_setup(pmq_ctx *c)
 // ---------------------------------------------------------------------------
 // kernels
 //
-// Gathering: one group of G lanes serves one atom: lane (ky, kz) of the
-// order x order stencil plane accumulates the `order` grid points along x
-// (G = 32 for order <= 5, 64 for order 6) and the group reduces with
-// butterfly shuffles.  Atoms go to workgroups in sorted (spatial) order,
+// Gathering: one group of G lanes serves one atom and reduces with butterfly
+// shuffles.  Planes (G = 8): lane kz < order sums the order x order points of
+// stencil plane kz, the others idle; rows (G = 32, or 64 for order 6): lane
+// (ky, kz) sums one row of `order` points.  Planes form the B-spline weights
+// once per plane instead of once per row, rows give four times the lanes
+// for small systems.  Atoms go to workgroups in sorted (spatial) order,
 // each XCD taking a contiguous eighth, so the grid points a workgroup reads
 // are shared with its neighbours in the same L2.
 struct PmeGeom {
@@ -231,10 +233,6 @@ static PmeGeom geom(const pmq_ctx *c)
     return g;
 }
 
-template <int ORD>
-struct PmeLanes {
-    static constexpr int G = ORD * ORD <= 32 ? 32 : 64;
-};
 
 // stencil start and fractional offset along each axis
 __device__ __forceinline__ void atom_frac(const PmeGeom &g, double px, double py, double pz, double *w, int *i0, int ord)
@@ -488,11 +486,72 @@ __global__ void k_pme_convolve(int ncplx, int nset, const T *__restrict__ G, T *
 
 // this lane's share of the potential derivatives at one site, in the
 // fractional order 0; 1,2,3; 11,22,33,12,13,23; 111,222,333,112,113,122,133,
-// 223,233,123 (NOUT = 4, 10 or 20); idle lanes (`on` false) give zeros
+// 223,233,123 (NOUT = 4, 10 or 20): the plane kz of the stencil, all ky and
+// kx; idle lanes (`on` false) give zeros
 template <int ORD, int NOUT, class T, class GT>
 __device__ __forceinline__ void gather_part(const PmeGeom &g, const GT *grid, const T tx[PMQ_MAX_ORDER][4],
-                                            const T *ty, const T *tz, const int *i0, int ky, int kz,
+                                            const T ty[PMQ_MAX_ORDER][4], const T *tz, const int *i0, int kz,
                                             bool on, double *o)
+{
+#pragma unroll
+    for (int j = 0; j < NOUT; j++) o[j] = 0.0;
+    if (!on) return;
+    int gz = i0[2] + kz;
+    if (gz >= g.K2) gz -= g.K2;
+    T acc[NOUT];
+#pragma unroll
+    for (int j = 0; j < NOUT; j++) acc[j] = T(0);
+#pragma unroll
+    for (int ky = 0; ky < ORD; ky++) {
+        int gy = i0[1] + ky;
+        if (gy >= g.K1) gy -= g.K1;
+        const GT *row = grid + ((size_t)gz * g.K1 + gy) * g.K0;
+        T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+        for (int kx = 0; kx < ORD; kx++) {
+            int gx = i0[0] + kx;
+            if (gx >= g.K0) gx -= g.K0;
+            const T q = T(row[gx]);
+            s0 += q * tx[kx][0];
+            s1 += q * tx[kx][1];
+            if (NOUT > 4) s2 += q * tx[kx][2];
+            if (NOUT > 10) s3 += q * tx[kx][3];
+        }
+        const T *y = ty[ky];
+        acc[0] += s0 * y[0] * tz[0];
+        acc[1] += s1 * y[0] * tz[0];
+        acc[2] += s0 * y[1] * tz[0];
+        acc[3] += s0 * y[0] * tz[1];
+        if (NOUT > 4) {
+            acc[4] += s2 * y[0] * tz[0];
+            acc[5] += s0 * y[2] * tz[0];
+            acc[6] += s0 * y[0] * tz[2];
+            acc[7] += s1 * y[1] * tz[0];
+            acc[8] += s1 * y[0] * tz[1];
+            acc[9] += s0 * y[1] * tz[1];
+        }
+        if (NOUT > 10) {
+            acc[10] += s3 * y[0] * tz[0];
+            acc[11] += s0 * y[3] * tz[0];
+            acc[12] += s0 * y[0] * tz[3];
+            acc[13] += s2 * y[1] * tz[0];
+            acc[14] += s2 * y[0] * tz[1];
+            acc[15] += s1 * y[2] * tz[0];
+            acc[16] += s1 * y[0] * tz[2];
+            acc[17] += s0 * y[2] * tz[1];
+            acc[18] += s0 * y[1] * tz[2];
+            acc[19] += s1 * y[1] * tz[1];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NOUT; j++) o[j] = acc[j];
+}
+
+// the row variant: lane (ky, kz) sums the `order` points of one stencil row
+template <int ORD, int NOUT, class T, class GT>
+__device__ __forceinline__ void gather_row(const PmeGeom &g, const GT *grid, const T tx[PMQ_MAX_ORDER][4],
+                                           const T *ty, const T *tz, const int *i0, int ky, int kz, bool on,
+                                           double *o)
 {
 #pragma unroll
     for (int j = 0; j < NOUT; j++) o[j] = 0.0;
@@ -536,6 +595,17 @@ __device__ __forceinline__ void gather_part(const PmeGeom &g, const GT *grid, co
         o[18] = s0 * ty[1] * tz[2];
         o[19] = s1 * ty[1] * tz[1];
     }
+}
+
+// one site's share for lane l of a G-lane group: G = 8, the plane kz = l;
+// otherwise the row (ky, kz) = (l % ORD, l / ORD)
+template <int ORD, int G, int NOUT, class T, class GT>
+__device__ __forceinline__ void gather_share(const PmeGeom &g, const GT *grid, const T tx[PMQ_MAX_ORDER][4],
+                                             const T ty[PMQ_MAX_ORDER][4], const T *ty1, const T *tz,
+                                             const int *i0, int ky, int kz, bool on, double *o)
+{
+    if constexpr (G == 8) gather_part<ORD, NOUT, T, GT>(g, grid, tx, ty, tz, i0, kz, on, o);
+    else gather_row<ORD, NOUT, T, GT>(g, grid, tx, ty1, tz, i0, ky, kz, on, o);
 }
 
 template <int G, int NOUT>
@@ -599,28 +669,28 @@ __device__ __forceinline__ void frac_to_cart(const PmeGeom &g, const double *f, 
 
 // common lane setup of the gather kernels
 #define GATHER_SETUP(ND, T)                                                                                  \
-    constexpr int G = PmeLanes<ORD>::G;                                                                      \
+    constexpr int G = GL;                                                                                    \
     const int slot = xcd_item(blockIdx.x, gridDim.x / PMQ_NXCD) * (blockDim.x / G) + threadIdx.x / G;        \
     const int l = threadIdx.x % G;                                                                           \
     const bool valid = slot < a.n;                                                                           \
-    const bool on = valid && l < ORD * ORD;                                                                  \
+    const bool on = valid && l < (G == 8 ? ORD : ORD * ORD);                                                 \
     const int s = valid ? slot : 0;                                                                          \
-    const int ky = l % ORD, kz = (l / ORD) % ORD;                                                            \
+    const int ky = G == 8 ? 0 : l % ORD, kz = G == 8 ? (l < ORD ? l : 0) : (l / ORD) % ORD;                  \
     double w[3];                                                                                             \
     int i0[3];                                                                                               \
     atom_frac(a.g, a.sx[s], a.sy[s], a.sz[s], w, i0, ORD);                                                   \
-    T tx[PMQ_MAX_ORDER][4], ty[4], tz[4];                                                                    \
+    T tx[PMQ_MAX_ORDER][4], ty[PMQ_MAX_ORDER][4], ty1[4], tz[4];                                             \
     {                                                                                                        \
         /* weights in FP64 whatever T is, then rounded to T */                                               \
-        double bx[PMQ_MAX_ORDER][4], byf[PMQ_MAX_ORDER][4], bzf[PMQ_MAX_ORDER][4], by[4], bz[4];             \
+        double bx[PMQ_MAX_ORDER][4], by[PMQ_MAX_ORDER][4], bzf[PMQ_MAX_ORDER][4], bz[4], by1[4];             \
         bspline_weights<double, ND>(w[0], ORD, bx);                                                          \
-        bspline_weights<double, ND>(w[1], ORD, byf);                                                         \
+        bspline_weights<double, ND>(w[1], ORD, by);                                                          \
         bspline_weights<double, ND>(w[2], ORD, bzf);                                                         \
-        pick<ORD, ND, double>(byf, ky, by);                                                                  \
         pick<ORD, ND, double>(bzf, kz, bz);                                                                  \
+        pick<ORD, ND, double>(by, ky, by1);                                                                  \
         _Pragma("unroll") for (int k = 0; k < ORD; k++)                                                      \
-            _Pragma("unroll") for (int d = 0; d <= ND; d++) tx[k][d] = T(bx[k][d]);                          \
-        _Pragma("unroll") for (int d = 0; d <= ND; d++) { ty[d] = T(by[d]); tz[d] = T(bz[d]); }            \
+            _Pragma("unroll") for (int d = 0; d <= ND; d++) { tx[k][d] = T(bx[k][d]); ty[k][d] = T(by[k][d]); } \
+        _Pragma("unroll") for (int d = 0; d <= ND; d++) { tz[d] = T(bz[d]); ty1[d] = T(by1[d]); }          \
     }
 
 struct GatherFracArgs {
@@ -634,19 +704,23 @@ struct GatherFracArgs {
 
 // the 20 fractional potential derivatives of every atom in each of nset
 // grids; lane j < 20 of the group stores component j
-template <int ORD, class T, class GT>
+template <int ORD, int GL, class T, class GT>
 __global__ __launch_bounds__(256) void k_pme_gather_frac(GatherFracArgs a)
 {
     GATHER_SETUP(3, T)
     for (int set = 0; set < a.nset; set++) {
         double fp[20];
-        gather_part<ORD, 20, T, GT>(a.g, (const GT *)a.grid + set * a.gstride, tx, ty, tz, i0, ky, kz, on, fp);
+        gather_share<ORD, G, 20, T, GT>(a.g, (const GT *)a.grid + set * a.gstride, tx, ty, ty1, tz, i0, ky, kz, on, fp);
         group_reduce<G, 20>(fp);
-        double v = 0.0;
+        // lane l of the group stores components l, l + G, l + 2G < 20
 #pragma unroll
-        for (int j = 0; j < 20; j++)
-            if (l == j) v = fp[j];
-        if (valid && l < 20) a.frac[((size_t)s * a.nset + set) * 20 + l] = v;
+        for (int base = 0; base < 20; base += G) {
+            double v = 0.0;
+#pragma unroll
+            for (int j = base; j < base + G && j < 20; j++)
+                if (l == j - base) v = fp[j];
+            if (valid && base + l < 20) a.frac[((size_t)s * a.nset + set) * 20 + base + l] = v;
+        }
     }
 }
 
@@ -739,13 +813,13 @@ struct GatherFieldArgs {
 };
 
 // reciprocal + self field of the induced-dipole sets at every site
-template <int ORD, class T, class GT>
+template <int ORD, int GL, class T, class GT>
 __global__ __launch_bounds__(256) void k_pme_gather_field(GatherFieldArgs a)
 {
     GATHER_SETUP(1, T)
     for (int set = 0; set < a.nset; set++) {
         double fp[4];
-        gather_part<ORD, 4, T, GT>(a.g, (const GT *)a.grid + set * a.gstride, tx, ty, tz, i0, ky, kz, on, fp);
+        gather_share<ORD, G, 4, T, GT>(a.g, (const GT *)a.grid + set * a.gstride, tx, ty, ty1, tz, i0, ky, kz, on, fp);
         group_reduce<G, 4>(fp);
         if (valid && l == 0) {
             double phi, gr[3];
@@ -870,16 +944,33 @@ static int pme_potential(pmq_ctx *c, int nset, const double *G = nullptr)
 
 static size_t grid_size(const pmq_ctx *c) { return (size_t)c->K[0] * c->K[1] * c->K[2]; }
 
-static dim3 lanes_grid(int n, int order)
+// lanes per atom in the gathers: 8 (one stencil plane per lane, the weights
+// formed once per plane) when the system is large enough to fill the chip
+// that way, else one stencil row per lane (ORD^2 rows in 32 or 64 lanes)
+constexpr int PMQ_GATHER_PLANE_MIN = 32768;
+constexpr int gather_row_lanes(int ord) { return ord * ord <= 32 ? 32 : 64; }
+static dim3 lanes_grid(int n, int lanes) { return dim3(xcd_grid(((size_t)n * lanes + 255) / 256)); }
+
+#define PMQ_GATHER_LAUNCH(KERNEL, ORD, GL, ...) \
+    hipLaunchKernelGGL((KERNEL<ORD, GL __VA_ARGS__>), lanes_grid(c->n, GL), dim3(256), 0, c->stream, args)
+static bool gather_planes(const pmq_ctx *c)
 {
-    return dim3(xcd_grid(((size_t)n * (order * order <= 32 ? 32 : 64) + 255) / 256));
+    return c->gather_lanes == PMQ_GATHER_PLANES || (c->gather_lanes == PMQ_GATHER_AUTO && c->n >= PMQ_GATHER_PLANE_MIN);
 }
 
 #define PMQ_ORDER_DISPATCH(KERNEL, ...)                                                                    \
-    switch (c->bsorder) {                                                                                  \
-    case 4: hipLaunchKernelGGL((KERNEL<4 __VA_ARGS__>), lanes_grid(c->n, 4), dim3(256), 0, c->stream, args); break; \
-    case 5: hipLaunchKernelGGL((KERNEL<5 __VA_ARGS__>), lanes_grid(c->n, 5), dim3(256), 0, c->stream, args); break; \
-    default: hipLaunchKernelGGL((KERNEL<6 __VA_ARGS__>), lanes_grid(c->n, 6), dim3(256), 0, c->stream, args); break; \
+    if (gather_planes(c)) {                                                                                \
+        switch (c->bsorder) {                                                                              \
+        case 4: PMQ_GATHER_LAUNCH(KERNEL, 4, 8, __VA_ARGS__); break;                                       \
+        case 5: PMQ_GATHER_LAUNCH(KERNEL, 5, 8, __VA_ARGS__); break;                                       \
+        default: PMQ_GATHER_LAUNCH(KERNEL, 6, 8, __VA_ARGS__); break;                                      \
+        }                                                                                                  \
+    } else {                                                                                               \
+        switch (c->bsorder) {                                                                              \
+        case 4: PMQ_GATHER_LAUNCH(KERNEL, 4, gather_row_lanes(4), __VA_ARGS__); break;                     \
+        case 5: PMQ_GATHER_LAUNCH(KERNEL, 5, gather_row_lanes(5), __VA_ARGS__); break;                     \
+        default: PMQ_GATHER_LAUNCH(KERNEL, 6, gather_row_lanes(6), __VA_ARGS__); break;                    \
+        }                                                                                                  \
     }
 
 // the sorted-atom range of every brick (once per evaluation; the atom sort
@@ -1089,12 +1180,12 @@ struct DispPostArgs {
     Totals *tot;
 };
 
-template <int ORD>
+template <int ORD, int GL>
 __global__ __launch_bounds__(256) void k_pme_disp_gather(DispPostArgs a)
 {
     GATHER_SETUP(1, double)
     double fp[4];
-    gather_part<ORD, 4, double, double>(a.g, a.grid, tx, ty, tz, i0, ky, kz, on, fp);
+    gather_share<ORD, G, 4, double, double>(a.g, a.grid, tx, ty, ty1, tz, i0, ky, kz, on, fp);
     group_reduce<G, 4>(fp);
     double e = 0.0;
     if (valid && l == 0) {

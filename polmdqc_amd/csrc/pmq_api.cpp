@@ -107,6 +107,17 @@ int pmq_set_fft(pmq_ctx *c, int engine)
     return PMQ_OK;
 }
 
+int pmq_set_gather_lanes(pmq_ctx *c, int mode)
+{
+    CHECK_CTX(c);
+    if (mode < PMQ_GATHER_AUTO || mode > PMQ_GATHER_PLANES) {
+        c->err = "pmq_set_gather_lanes: mode must be PMQ_GATHER_AUTO, PMQ_GATHER_ROWS or PMQ_GATHER_PLANES";
+        return PMQ_ERR_ARG;
+    }
+    c->gather_lanes = mode;
+    return PMQ_OK;
+}
+
 int pmq_fft_choice(pmq_ctx *c, int *fp64_direct, int *fp32_grids, int *fp32_direct)
 {
     CHECK_CTX(c);

@@ -283,6 +283,18 @@ def set_fft(eng: Engine, engine: int = FFT_AUTO):
     eng._check(L.pmq_set_fft(eng._ctx, int(engine)), "pmq_set_fft")
 
 
+GATHER_AUTO, GATHER_ROWS, GATHER_PLANES = 0, 1, 2
+
+
+def set_gather_lanes(eng: Engine, mode: int = GATHER_AUTO):
+    """PME gather layout: by system size (GATHER_AUTO), one lane per stencil
+    row (GATHER_ROWS) or one per stencil plane (GATHER_PLANES)."""
+    L = eng._L
+    L.pmq_set_gather_lanes.argtypes = [C.c_void_p, C.c_int]
+    L.pmq_set_gather_lanes.restype = C.c_int
+    eng._check(L.pmq_set_gather_lanes(eng._ctx, int(mode)), "pmq_set_gather_lanes")
+
+
 def fft_choice(eng: Engine) -> dict:
     """The transforms chosen for the current grid (after a PME evaluation)."""
     L = eng._L

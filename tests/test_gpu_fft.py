@@ -1,6 +1,7 @@
-"""GPU parity of the PME transform engines: the direct LDS transforms
-(k_fft.hip: forward planes, z columns with the reciprocal kernel, inverse
-planes) against rocFFT and against the oracle.
+"""GPU parity of the PME transform engines and gather layouts: the direct
+LDS transforms (k_fft.hip: forward planes, z columns with the reciprocal
+kernel, inverse planes) against rocFFT and against the oracle, and the two
+gather layouts (a lane per stencil row, a lane per stencil plane).
 
 Both engines compute the same unnormalized DFTs in different summation
 orders, so the plain FP64 results agree to round-off (1e-12 relative) and
@@ -109,3 +110,42 @@ def test_direct_transforms_dispersion(engine):
     o = oracle.hippo_terms(s, want_grad=True)
     assert abs(g["edsp"] - o["edsp"]) <= REL * abs(o["edsp"])
     assert rel(g["dedsp"], o["dedsp"]) <= REL
+
+
+@pytest.mark.parametrize("mode", ["rows", "planes"])
+def test_gather_layouts_match_oracle(engine, mode):
+    """Both gather layouts (one lane per stencil row, one per stencil plane)
+    on multipoles, polarization and dispersion: each within the oracle bar,
+    and the two within round-off of each other."""
+    from polmdqc_amd import _lib as L
+    from polmdqc_amd.engine import GATHER_AUTO, GATHER_PLANES, GATHER_ROWS, set_gather_lanes
+
+    print(f"[gpu] gather layout {mode}", flush=True)
+    s = box216()
+    engine.set_system(s)
+    set_gather_lanes(engine, GATHER_PLANES if mode == "planes" else GATHER_ROWS)
+    try:
+        g = engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR)
+        set_gather_lanes(engine, GATHER_ROWS if mode == "planes" else GATHER_PLANES)
+        h = engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR)
+    finally:
+        set_gather_lanes(engine, GATHER_AUTO)
+    o = oracle.elec(s, want_grad=True, poleps=1e-10)
+    for k in ("em", "ep"):
+        assert abs(g[k] - o[k]) <= REL * abs(o[k])
+        assert abs(g[k] - h[k]) <= 1e-12 * abs(h[k])
+    assert rel(g["dem"], o["dem"]) <= REL
+    assert rel(g["dep"], o["dep"]) <= REL
+
+    xyz = water.lattice_box(6, 18.643, 216)
+    sd = water.water_system(xyz, "hippo19", box=18.643,
+                            ewald={"aewald": 0.5446, "nfft": 24, "order": 5, "cutoff": 7.0, "dewald": 0.45})
+    engine.set_system(sd)
+    set_gather_lanes(engine, GATHER_PLANES if mode == "planes" else GATHER_ROWS)
+    try:
+        d = engine.eval(L.GRADIENT, L.TERM_DISP)
+    finally:
+        set_gather_lanes(engine, GATHER_AUTO)
+    od = oracle.hippo_terms(sd, want_grad=True)
+    assert abs(d["edsp"] - od["edsp"]) <= REL * abs(od["edsp"])
+    assert rel(d["dedsp"], od["dedsp"]) <= REL
