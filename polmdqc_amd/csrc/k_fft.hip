@@ -52,6 +52,7 @@ struct FftArgs {
     cpx<T> *cplx;     // nset complex half grids, ncplx apart
     const T *G;       // H0 K1 K2 reciprocal kernel
     size_t nreal, ncplx;
+    const double *skip; // nonzero: return at once (or null)
 };
 
 template <class T>
@@ -140,6 +141,7 @@ __device__ __forceinline__ void copy_in(V *dst, const V *src, int n, int tid, in
 template <class T>
 __global__ __launch_bounds__(FFT_MAX_TB) void k_fft_planes_fwd(FftArgs<T> a)
 {
+    if (a.skip && *a.skip != 0.0) return;
     extern __shared__ double fft_lds[];
     const int K0 = a.K0, K1 = a.K1, H0 = a.H0;
     const int z = blockIdx.x, set = blockIdx.y, tid = threadIdx.x, bd = blockDim.x;
@@ -199,6 +201,7 @@ __global__ __launch_bounds__(FFT_MAX_TB) void k_fft_planes_fwd(FftArgs<T> a)
 template <class T>
 __global__ __launch_bounds__(FFT_MAX_TB) void k_fft_columns(FftArgs<T> a)
 {
+    if (a.skip && *a.skip != 0.0) return;
     extern __shared__ double fft_lds[];
     const int K1 = a.K1, K2 = a.K2, H0 = a.H0;
     const int y = blockIdx.x, set = blockIdx.y, tid = threadIdx.x, bd = blockDim.x;
@@ -258,6 +261,7 @@ __global__ __launch_bounds__(FFT_MAX_TB) void k_fft_columns(FftArgs<T> a)
 template <class T>
 __global__ __launch_bounds__(FFT_MAX_TB) void k_fft_planes_inv(FftArgs<T> a)
 {
+    if (a.skip && *a.skip != 0.0) return;
     extern __shared__ double fft_lds[];
     const int K0 = a.K0, K1 = a.K1, H0 = a.H0;
     const int z = blockIdx.x, set = blockIdx.y, tid = threadIdx.x, bd = blockDim.x;
@@ -380,7 +384,7 @@ int fft_own_setup(pmq_ctx *c)
 }
 
 template <class T>
-static int own_convolve(pmq_ctx *c, int nset, const T *G, const T *tw)
+static int own_convolve(pmq_ctx *c, int nset, const T *G, const T *tw, const double *skip)
 {
     FftArgs<T> a;
     a.K0 = c->K[0]; a.K1 = c->K[1]; a.K2 = c->K[2]; a.H0 = c->K[0] / 2 + 1;
@@ -388,6 +392,7 @@ static int own_convolve(pmq_ctx *c, int nset, const T *G, const T *tw)
     a.real = reinterpret_cast<T *>(c->qgrid.p);
     a.cplx = reinterpret_cast<cpx<T> *>(c->cgrid.p);
     a.G = G;
+    a.skip = skip;
     a.nreal = (size_t)a.K0 * a.K1 * a.K2;
     a.ncplx = (size_t)a.H0 * a.K1 * a.K2;
     const size_t w = sizeof(T);
@@ -402,14 +407,14 @@ static int own_convolve(pmq_ctx *c, int nset, const T *G, const T *tw)
 }
 
 // qgrid (nset real grids) <- IFFT(G FFT(qgrid)), cgrid as the complex scratch
-int fft_own_convolve(pmq_ctx *c, int nset, const void *G, bool fp32)
+int fft_own_convolve(pmq_ctx *c, int nset, const void *G, bool fp32, const double *skip)
 {
     if (!fft_own_fits(c, fp32)) {
         c->err = "PME: the grid planes do not fit the LDS of the direct transforms";
         return PMQ_ERR_UNSUPPORTED;
     }
-    if (fp32) return own_convolve<float>(c, nset, (const float *)G, c->fft_tw_s.p);
-    return own_convolve<double>(c, nset, (const double *)G, c->fft_tw.p);
+    if (fp32) return own_convolve<float>(c, nset, (const float *)G, c->fft_tw_s.p, skip);
+    return own_convolve<double>(c, nset, (const double *)G, c->fft_tw.p, skip);
 }
 
 } // namespace pmq

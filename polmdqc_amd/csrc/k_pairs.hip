@@ -412,12 +412,14 @@ struct SpmvArgs {
     const TEnt<V> *ent;
     const double *src; // 2 x 3 x npad
     double *dst;       // 2 x 3 x npad (overwritten)
+    const double *skip; // nonzero: return at once (or null)
 };
 
 // dst = T src for both dipole sets; G lanes per row, FP64 arithmetic
 template <int G, class V>
 __global__ __launch_bounds__(256) void k_pol_spmv(SpmvArgs<V> a)
 {
+    if (a.skip && *a.skip != 0.0) return;
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int row = tid / G, g = tid % G;
     const int np = a.npad;
@@ -852,12 +854,12 @@ int tile_scalars(pmq_ctx *c, int which, bool full)
 }
 
 template <class V>
-static int spmv(pmq_ctx *c, const double *src, double *dst)
+static int spmv(pmq_ctx *c, const double *src, double *dst, const double *skip)
 {
     constexpr int G = 8;
     SpmvArgs<V> a;
     a.npad = c->npad; a.rowptr = c->rowptr.p; a.cap = c->tcap; a.ent = (const TEnt<V> *)c->tval.p;
-    a.src = src; a.dst = dst;
+    a.src = src; a.dst = dst; a.skip = skip;
     const size_t lanes = (size_t)c->npad * G;
     ProfScope ps_(c, "pol_mutual_spmv");
     hipLaunchKernelGGL((k_pol_spmv<G, V>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, c->stream, a);
@@ -893,10 +895,10 @@ int pair_matrix_precondition(pmq_ctx *c, const double *w, const double *r, doubl
     return precondition<double>(c, w, r, z, p, init, sc, part);
 }
 
-int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst)
+int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst, const double *skip)
 {
-    if (c->tmixed) return spmv<float>(c, src, dst);
-    return spmv<double>(c, src, dst);
+    if (c->tmixed) return spmv<float>(c, src, dst, skip);
+    return spmv<double>(c, src, dst, skip);
 }
 
 } // namespace pmq

@@ -308,6 +308,7 @@ struct BrickArgs {
     const double *charge; // mode 0: total monopole per sorted atom (charge penetration), or null
     void *grid;           // nset grids of the kernel's arithmetic type
     size_t gstride;
+    const double *skip;   // nonzero: return at once (or null)
 };
 
 // sorted-atom range of every brick: the keys of the atom sort are
@@ -346,6 +347,7 @@ constexpr int SPREAD_CH = 64;
 template <int ORD, int MODE, class T, class GT = T>
 __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
 {
+    if (a.skip && *a.skip != 0.0) return;
     constexpr int ND = MODE == 0 ? 2 : MODE == 1 ? 1 : 0;
     constexpr int NC = MODE == 0 ? 10 : MODE == 1 ? 6 : 1;
     constexpr int NS = MODE == 1 ? 2 : 1;
@@ -475,8 +477,9 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
 }
 
 template <class T>
-__global__ void k_pme_convolve(int ncplx, int nset, const T *__restrict__ G, T *__restrict__ cg)
+__global__ void k_pme_convolve(int ncplx, int nset, const T *__restrict__ G, T *__restrict__ cg, const double *skip)
 {
+    if (skip && *skip != 0.0) return;
     const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (t >= (size_t)ncplx * nset) return;
     const T g = G[t % ncplx];
@@ -810,12 +813,14 @@ struct GatherFieldArgs {
     const double *dip;  // nset x 3 x npad dipoles (for the self term)
     double self;        // 4/3 a^3 / sqrt(pi)
     double *field;      // nset x 3 x npad (+=)
+    const double *skip; // nonzero: return at once (or null)
 };
 
 // reciprocal + self field of the induced-dipole sets at every site
 template <int ORD, int GL, class T, class GT>
 __global__ __launch_bounds__(256) void k_pme_gather_field(GatherFieldArgs a)
 {
+    if (a.skip && *a.skip != 0.0) return;
     GATHER_SETUP(1, T)
     for (int set = 0; set < a.nset; set++) {
         double fp[4];
@@ -919,13 +924,14 @@ static int fft(pmq_ctx *c, bool forward, int nset, bool fp32)
 static bool grid32(const pmq_ctx *c) { return c->pme_fp32 && c->pme_fft32 == 1; }
 
 // G: a kernel table of the plain form (null = the electrostatic one)
-static int pme_potential(pmq_ctx *c, int nset, const double *G = nullptr)
+static int pme_potential(pmq_ctx *c, int nset, const double *G = nullptr, const double *skip = nullptr)
 {
     int rc;
     const bool fp32 = grid32(c) && !G;
     if ((fp32 ? c->pme_own32 : c->pme_own64) == 1) {
         ProfScope ps_(c, "pme_fft");
-        return fft_own_convolve(c, nset, fp32 ? (const void *)c->pme_g_s.p : (const void *)(G ? G : c->pme_g.p), fp32);
+        return fft_own_convolve(c, nset, fp32 ? (const void *)c->pme_g_s.p : (const void *)(G ? G : c->pme_g.p), fp32,
+                                skip);
     }
     if ((rc = fft(c, true, nset, fp32))) return rc;
     const int H0 = c->K[0] / 2 + 1;
@@ -934,10 +940,10 @@ static int pme_potential(pmq_ctx *c, int nset, const double *G = nullptr)
     const dim3 grid((unsigned)((tot + 255) / 256));
     if (fp32)
         hipLaunchKernelGGL(k_pme_convolve<float>, grid, dim3(256), 0, c->stream, ncplx, nset, c->pme_g_s.p,
-                           (float *)c->cgrid.p);
+                           (float *)c->cgrid.p, skip);
     else
         hipLaunchKernelGGL(k_pme_convolve<double>, grid, dim3(256), 0, c->stream, ncplx, nset, G ? G : c->pme_g.p,
-                           c->cgrid.p);
+                           c->cgrid.p, skip);
     PMQ_HIP(hipGetLastError());
     return fft(c, false, nset, fp32);
 }
@@ -990,7 +996,8 @@ static int pme_brick_ranges(pmq_ctx *c)
 
 // FP32 grids for the electrostatic modes of the mixed form; dispersion
 // (mode 2) stays FP64
-static int spread(pmq_ctx *c, int mode, const double *src, const double *charge, int nset)
+static int spread(pmq_ctx *c, int mode, const double *src, const double *charge, int nset,
+                  const double *skip = nullptr)
 {
     const bool fp32 = c->pme_fp32 && mode != 2, g32 = grid32(c) && mode != 2;
     BrickArgs args;
@@ -1002,6 +1009,7 @@ static int spread(pmq_ctx *c, int mode, const double *src, const double *charge,
     args.sx = c->sx.p; args.sy = c->sy.p; args.sz = c->sz.p;
     args.src = src;
     args.charge = charge;
+    args.skip = skip;
     args.grid = c->qgrid.p;
     args.gstride = grid_size(c);
     if (mode == 1 && nset != 2) {
@@ -1140,13 +1148,13 @@ int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar)
     return PMQ_OK;
 }
 
-int pme_dipole_field(pmq_ctx *c, const double *dip, double *field)
+int pme_dipole_field(pmq_ctx *c, const double *dip, double *field, const double *skip)
 {
     // dip and field: two sets of 3 x npad, contiguous
     int rc;
     const size_t gs = grid_size(c);
-    if ((rc = spread(c, 1, dip, nullptr, 2))) return rc;
-    if ((rc = pme_potential(c, 2))) return rc;
+    if ((rc = spread(c, 1, dip, nullptr, 2, skip))) return rc;
+    if ((rc = pme_potential(c, 2, nullptr, skip))) return rc;
     GatherFieldArgs args;
     args.n = c->n; args.npad = c->npad; args.nset = 2; args.g = geom(c);
     args.sx = c->sx.p; args.sy = c->sy.p; args.sz = c->sz.p;
@@ -1155,6 +1163,7 @@ int pme_dipole_field(pmq_ctx *c, const double *dip, double *field)
     args.dip = dip;
     args.self = 4.0 / 3.0 * c->prm.aewald * c->prm.aewald * c->prm.aewald / 1.77245385090551602730;
     args.field = field;
+    args.skip = skip;
     {
         ProfScope ps_(c, "pme_gather");
         if (grid32(c)) { PMQ_ORDER_DISPATCH(k_pme_gather_field, , float, float) }

@@ -733,7 +733,7 @@ __global__ void k_pol_scatter(int n, int npad, const int *perm, const double *u,
 
 // ---------------------------------------------------------------------------
 // launchers
-int pme_dipole_field(pmq_ctx *c, const double *dip, double *field);
+int pme_dipole_field(pmq_ctx *c, const double *dip, double *field, const double *skip);
 int pme_polar_grad(pmq_ctx *c);
 int launch_zero(pmq_ctx *c, double *p, size_t n);
 
@@ -761,12 +761,14 @@ static PolArgs pol_args(pmq_ctx *c, double off2)
 
 static int tile_grid(pmq_ctx *c) { return c->tgrid; }
 
-// dst = T src (real space, from the pair matrix) + reciprocal and self field
-static int mutual_field(pmq_ctx *c, const double *src, double *dst)
+// dst = T src (real space, from the pair matrix) + reciprocal and self field;
+// skip: the solve's converged flag for the iterations of a speculative batch
+// (their kernels return at once once it is set), or null
+static int mutual_field(pmq_ctx *c, const double *src, double *dst, const double *skip = nullptr)
 {
     int rc;
-    if ((rc = pair_matrix_apply(c, src, dst))) return rc;
-    if (c->prm.use_ewald && (rc = pme_dipole_field(c, src, dst))) return rc;
+    if ((rc = pair_matrix_apply(c, src, dst, skip))) return rc;
+    if (c->prm.use_ewald && (rc = pme_dipole_field(c, src, dst, skip))) return rc;
     return PMQ_OK;
 }
 
@@ -858,7 +860,8 @@ int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
         const int nb = std::min(batch, c->prm.politer - it);
         for (int k = 0; k < nb; k++) {
             a.iter = ++it;
-            if ((rc = mutual_field(c, c->pdir.p, c->pvec.p))) return rc;
+            // sc[8] is the converged flag k_pcg_c sets (cleared by k_pcg_init_fin)
+            if ((rc = mutual_field(c, c->pdir.p, c->pvec.p, a.sc + 8))) return rc;
             hipLaunchKernelGGL(k_pcg_a, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
             hipLaunchKernelGGL(k_pcg_b, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
             if (prec && (rc = pair_matrix_precondition(c, a.w, a.r, a.z, a.p, false, a.sc, a.part2))) return rc;

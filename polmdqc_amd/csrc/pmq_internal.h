@@ -370,7 +370,7 @@ int launch_zero_many(pmq_ctx *c, int k, double *const *p, const size_t *n);
 int pme_setup(pmq_ctx *c);
 int fft_own_setup(pmq_ctx *c);
 bool fft_own_fits(const pmq_ctx *c, bool fp32);
-int fft_own_convolve(pmq_ctx *c, int nset, const void *G, bool fp32);
+int fft_own_convolve(pmq_ctx *c, int nset, const void *G, bool fp32, const double *skip = nullptr);
 int pme_disp(pmq_ctx *c, bool grad, bool need_bin);
 int hippo_sort(pmq_ctx *c);
 int launch_disp(pmq_ctx *c, double off2, bool grad, bool mixed);
@@ -389,7 +389,9 @@ int status_reset(pmq_ctx *c);
 int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe);
 int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fixed);
 int tile_scalars(pmq_ctx *c, int which, bool full);
-int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst);
+// skip: a device flag; when it is nonzero the launched kernels return at once
+// (the speculative PCG iterations after convergence)
+int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst, const double *skip = nullptr);
 int pair_matrix_prec_blocks(const pmq_ctx *c);
 int pair_matrix_precondition(pmq_ctx *c, const double *w, const double *r, double *z, double *p, bool init,
                              const double *sc, double *part);
