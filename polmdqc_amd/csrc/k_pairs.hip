@@ -754,20 +754,32 @@ __global__ __launch_bounds__(64 * RED_W) void k_tile_reduce(TileReduceArgs a)
             lt[s] = IJ.x == IJ.y ? ((t * 2) | (1 << 30)) : (t * 2 + (IJ.x == I ? 0 : 1));
         }
         __syncthreads();
-#pragma unroll 2
+        // loads are unconditional (a diagonal tile's second side is read from
+        // the next slot, any other tile's "second side" is its own slot again
+        // and dropped), so the loads of several tiles are in flight together;
+        // components nobody wants are read and never stored
+        constexpr int RU = sizeof(P) == 4 ? 4 : 2; // tiles in flight per wave
+#pragma unroll RU
         for (int s = wv; s < cnt; s += RED_W) {
             const int e = lt[s];
             const int slot = e & ~(1 << 30);
-            const int nside = (e >> 30) & 1 ? 2 : 1;
-            for (int k = 0; k < nside; k++) {
-                const P *pp = part + ((size_t)slot + k) * PMQ_TILE_NV * 64 + lane;
-                P v[NV];
+            const bool two = (e >> 30) & 1;
+            const P *pp = part + (size_t)slot * PMQ_TILE_NV * 64 + lane;
+            const P *pq = two ? pp + PMQ_TILE_NV * 64 : pp;
+            P v[NV], w[NV];
 #pragma unroll
-                for (int c = 0; c < NV; c++) v[c] = want[c] ? pp[c * 64] : P(0);
+            for (int c = 0; c < NV; c++) {
+                v[c] = pp[c * 64];
+                w[c] = pq[c * 64];
+            }
 #pragma unroll
-                for (int c = 0; c < NV; c++) {
-                    if (FX) q[c] += __double2ll_rn((double)v[c] * a.fxscale[c]);
-                    else acc[c] += (double)v[c];
+            for (int c = 0; c < NV; c++) {
+                if (FX) {
+                    q[c] += __double2ll_rn((double)v[c] * a.fxscale[c]);
+                    if (two) q[c] += __double2ll_rn((double)w[c] * a.fxscale[c]);
+                } else {
+                    acc[c] += (double)v[c];
+                    if (two) acc[c] += (double)w[c];
                 }
             }
         }
