@@ -404,6 +404,63 @@ __global__ __launch_bounds__(64) void k_excl_fold(FoldArgs<V> a)
     }
 }
 
+// this lane's share (entries e0 + g, e0 + g + G, ... < e1) of T u for both
+// dipole sets, U entries at a time: their records are loaded together, then
+// the dipoles of their columns together, so a lane waits on two loads per U
+// entries rather than two per entry.  Entries past e1 are clamped to e1 - 1
+// and weighted by zero.
+template <int G, int U, class V>
+__device__ __forceinline__ void row_apply(const TEnt<V> *ent, int e0, int e1, int g, const double *u, int np,
+                                          double acc[6])
+{
+    if constexpr (U == 1) {
+        for (int e = e0 + g; e < e1; e += G) {
+            const TEnt<V> r = ent[e];
+            const int k = r.col;
+            const double txx = r.t[0], tyy = r.t[1], tzz = r.t[2];
+            const double txy = r.t[3], txz = r.t[4], tyz = r.t[5];
+#pragma unroll
+            for (int set = 0; set < 2; set++) {
+                const double *us = u + (size_t)set * 3 * np;
+                const double ux = us[k], uy = us[np + k], uz = us[2 * np + k];
+                acc[3 * set + 0] += txx * ux + txy * uy + txz * uz;
+                acc[3 * set + 1] += txy * ux + tyy * uy + tyz * uz;
+                acc[3 * set + 2] += txz * ux + tyz * uy + tzz * uz;
+            }
+        }
+        return;
+    }
+    for (int base = e0 + g; base < e1; base += G * U) {
+        TEnt<V> r[U];
+        double wgt[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const int e = base + k * G;
+            wgt[k] = e < e1 ? 1.0 : 0.0;
+            r[k] = ent[min(e, e1 - 1)];
+        }
+        double d[U][6];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const int col = r[k].col;
+#pragma unroll
+            for (int q = 0; q < 6; q++) d[k][q] = u[(size_t)q * np + col];
+        }
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+            const double txx = wgt[k] * r[k].t[0], tyy = wgt[k] * r[k].t[1], tzz = wgt[k] * r[k].t[2];
+            const double txy = wgt[k] * r[k].t[3], txz = wgt[k] * r[k].t[4], tyz = wgt[k] * r[k].t[5];
+#pragma unroll
+            for (int set = 0; set < 2; set++) {
+                const double ux = d[k][3 * set], uy = d[k][3 * set + 1], uz = d[k][3 * set + 2];
+                acc[3 * set + 0] += txx * ux + txy * uy + txz * uz;
+                acc[3 * set + 1] += txy * ux + tyy * uy + tyz * uz;
+                acc[3 * set + 2] += txz * ux + tyz * uy + tzz * uz;
+            }
+        }
+    }
+}
+
 template <class V>
 struct SpmvArgs {
     int npad;
@@ -426,20 +483,9 @@ __global__ __launch_bounds__(256) void k_pol_spmv(SpmvArgs<V> a)
     double acc[6] = {0, 0, 0, 0, 0, 0};
     if (row < np) {
         const int e1 = (int)min((size_t)a.rowptr[row + 1], a.cap);
-        for (int e = a.rowptr[row] + g; e < e1; e += G) {
-            const TEnt<V> r = a.ent[e];
-            const int k = r.col;
-            const double txx = r.t[0], tyy = r.t[1], tzz = r.t[2];
-            const double txy = r.t[3], txz = r.t[4], tyz = r.t[5];
-#pragma unroll
-            for (int set = 0; set < 2; set++) {
-                const double *u = a.src + (size_t)set * 3 * np;
-                const double ux = u[k], uy = u[np + k], uz = u[2 * np + k];
-                acc[3 * set + 0] += txx * ux + txy * uy + txz * uz;
-                acc[3 * set + 1] += txy * ux + tyy * uy + tyz * uz;
-                acc[3 * set + 2] += txz * ux + tyz * uy + tzz * uz;
-            }
-        }
+        // 4 records of 32 bytes in flight (FP32 tensor); the 64-byte FP64
+        // records run one at a time (more in flight cost occupancy, measured)
+        row_apply<G, sizeof(V) == 4 ? 4 : 1, V>(a.ent, a.rowptr[row], e1, g, a.src, np, acc);
     }
 #pragma unroll
     for (int c = 0; c < 6; c++)
@@ -483,20 +529,7 @@ __global__ __launch_bounds__(256) void k_pcg_prec(PrecArgs<V> a)
     double acc[6] = {0, 0, 0, 0, 0, 0};
     if (row < np) {
         const int e1 = (int)min((size_t)(a.rowptr[row] + a.rownear[row]), a.cap);
-        for (int e = a.rowptr[row] + g; e < e1; e += G) {
-            const TEnt<V> r = a.ent_s[e];
-            const int k = r.col;
-            const double txx = r.t[0], tyy = r.t[1], tzz = r.t[2];
-            const double txy = r.t[3], txz = r.t[4], tyz = r.t[5];
-#pragma unroll
-            for (int set = 0; set < 2; set++) {
-                const double *u = a.w + (size_t)set * 3 * np;
-                const double ux = u[k], uy = u[np + k], uz = u[2 * np + k];
-                acc[3 * set + 0] += txx * ux + txy * uy + txz * uz;
-                acc[3 * set + 1] += txy * ux + tyy * uy + tyz * uz;
-                acc[3 * set + 2] += txz * ux + tyz * uy + tzz * uz;
-            }
-        }
+        row_apply<G, 2, V>(a.ent_s, a.rowptr[row], e1, g, a.w, np, acc);
     }
 #pragma unroll
     for (int c = 0; c < 6; c++)
