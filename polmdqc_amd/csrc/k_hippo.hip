@@ -436,8 +436,7 @@ int launch_rep(pmq_ctx *c, double off2, bool grad, bool mixed)
     if (a.nexcl) hipLaunchKernelGGL(k_rep_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     if (mixed && grad) {
-        if ((rc = launch_fx_flush(c, c->fx_grad.p, c->der.p, 3 * (size_t)c->n, PMQ_FX_GRAD))) return rc;
-        if ((rc = launch_fx_flush(c, c->fx_grad.p + 3 * (size_t)c->n, c->ter.p, 3 * (size_t)c->n, PMQ_FX_GRAD)))
+        if ((rc = launch_fx_flush_pair(c, c->fx_grad.p, c->der.p, c->ter.p, 3 * (size_t)c->n, PMQ_FX_GRAD)))
             return rc;
     }
     return PMQ_OK;

@@ -44,6 +44,28 @@ int launch_fx_flush(pmq_ctx *c, long long *q, double *d, size_t n, double scale)
     return PMQ_OK;
 }
 
+// two accumulators held back to back (q[0, n) and q[n, 2n), e.g. forces
+// then torques) flushed into two separate FP64 arrays by one launch
+__global__ void k_fx_flush_pair(size_t n, long long *__restrict__ q, double *__restrict__ d0,
+                                double *__restrict__ d1, double inv_scale)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < 2 * n; i += (size_t)gridDim.x * blockDim.x) {
+        double *d = i < n ? d0 + i : d1 + (i - n);
+        *d += (double)q[i] * inv_scale;
+        q[i] = 0;
+    }
+}
+
+int launch_fx_flush_pair(pmq_ctx *c, long long *q, double *d0, double *d1, size_t n, double scale)
+{
+    if (!n) return PMQ_OK;
+    const int tb = 256;
+    const int gb = (int)std::min<size_t>((2 * n + tb - 1) / tb, 4096);
+    hipLaunchKernelGGL(k_fx_flush_pair, dim3(gb), dim3(tb), 0, c->stream, n, q, d0, d1, 1.0 / scale);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
 // several buffers cleared by one launch (segment k: blockIdx.y == k)
 struct ZeroList {
     double *p[PMQ_ZERO_MAX];

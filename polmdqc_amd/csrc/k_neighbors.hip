@@ -325,8 +325,24 @@ int build_neighbors(pmq_ctx *c, double cutoff)
         hipLaunchKernelGGL(k_cell_keys, dim3((n + tb - 1) / tb), dim3(tb), 0, st, n, b, c->bbox.p, nc[0], nc[1],
                            nc[2], c->x.p, c->y.p, c->z.p, c->keys.p);
     }
-    size_t bytes = c->scratch_sort_bytes;
-    PMQ_HIP(rocprim::radix_sort_keys(c->scratch_sort.p, bytes, c->keys.p, c->keys_sorted.p, (size_t)n, 0, 64, st));
+    // The low 32 bits of a key hold the atom index and keys are written in
+    // index order, so a stable sort on the Morton bits alone gives the same
+    // order as a sort on all 64: only bits [32, 32 + 3 b) are passed, b the
+    // bits of the largest cell (or brick) index on any axis.
+    int mx = 1;
+    for (int a = 0; a < 3; a++) mx = std::max(mx, c->pme_cells ? c->pme_nb[a] : nc[a]);
+    int bits = 1;
+    while ((1 << bits) < mx) bits++;
+    const unsigned end_bit = 32u + 3u * (unsigned)bits;
+    size_t bytes = 0;
+    PMQ_HIP(rocprim::radix_sort_keys(nullptr, bytes, c->keys.p, c->keys_sorted.p, (size_t)n, 32u, end_bit, st));
+    if (bytes > c->scratch_sort_bytes) {
+        c->err = "build_neighbors: sort scratch too small";
+        return PMQ_ERR_STATE;
+    }
+    bytes = c->scratch_sort_bytes;
+    PMQ_HIP(rocprim::radix_sort_keys(c->scratch_sort.p, bytes, c->keys.p, c->keys_sorted.p, (size_t)n, 32u, end_bit,
+                                     st));
     hipLaunchKernelGGL(k_gather, dim3((npad + tb - 1) / tb), dim3(tb), 0, st, n, npad, c->keys_sorted.p, c->x.p,
                        c->y.p, c->z.p, c->usef.p, c->molcule.p, c->perm.p, c->iperm.p, c->sx.p, c->sy.p, c->sz.p,
                        c->sflag.p, c->smol.p);

@@ -686,8 +686,7 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
         if (do_polar && (rc = polar_prepare(c, off2, mixed, safe))) return rc;
         if (do_mpole && (rc = launch_mpole_direct(c, mode, off2, mixed))) return rc;
         if (do_mpole && mixed && grad) {
-            if ((rc = launch_fx_flush(c, c->fx_grad.p, c->dem.p, 3 * (size_t)n, PMQ_FX_GRAD))) return rc;
-            if ((rc = launch_fx_flush(c, c->fx_grad.p + 3 * (size_t)n, c->te.p, 3 * (size_t)n, PMQ_FX_GRAD))) return rc;
+            if ((rc = launch_fx_flush_pair(c, c->fx_grad.p, c->dem.p, c->te.p, 3 * (size_t)n, PMQ_FX_GRAD))) return rc;
         }
         if (pme) {
             if (ewald && (do_mpole || do_polar) && (rc = pme_mpole(c, mode, do_mpole, do_polar))) return rc;
@@ -699,8 +698,7 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
             if (grad) {
                 if ((rc = polar_gradient(c, off2, mixed))) return rc;
                 if (mixed) {
-                    if ((rc = launch_fx_flush(c, c->fx_grad.p, c->dep.p, 3 * (size_t)n, PMQ_FX_GRAD))) return rc;
-                    if ((rc = launch_fx_flush(c, c->fx_grad.p + 3 * (size_t)n, c->tep.p, 3 * (size_t)n, PMQ_FX_GRAD)))
+                    if ((rc = launch_fx_flush_pair(c, c->fx_grad.p, c->dep.p, c->tep.p, 3 * (size_t)n, PMQ_FX_GRAD)))
                         return rc;
                 }
                 if ((rc = launch_torque(c, PMQ_GRADIENT, c->tep.p, c->dep.p, mixed))) return rc;

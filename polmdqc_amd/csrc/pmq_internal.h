@@ -364,6 +364,7 @@ int launch_rotpole(pmq_ctx *c, bool want_rpole13);
 int launch_chgflx_delta(pmq_ctx *c);
 int launch_mpole_direct(pmq_ctx *c, int mode, double off2, bool mixed);
 int launch_fx_flush(pmq_ctx *c, long long *q, double *d, size_t n, double scale);
+int launch_fx_flush_pair(pmq_ctx *c, long long *q, double *d0, double *d1, size_t n, double scale);
 int launch_torque(pmq_ctx *c, int mode, const double *te, double *de, bool mixed);
 constexpr int PMQ_ZERO_MAX = 12;
 int launch_zero_many(pmq_ctx *c, int k, double *const *p, const size_t *n);
