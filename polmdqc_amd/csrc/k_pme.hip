@@ -340,7 +340,7 @@ __global__ void k_pme_brick_ranges(int n, int nb0, int nb1, int nb2, const unsig
 // B-spline weights, fractional coefficients); wave w sums candidates w, w+4,
 // ... for its lane's point, and the four partial sums are added in wave
 // order, so the grid is bit-reproducible.
-constexpr int SPREAD_CH = 64;
+constexpr int SPREAD_CH = 64; // one wave: candidate j is prepared by lane j of each wave
 
 // T: the arithmetic of the weights and sums (double in the plain form, float
 // for the electrostatic grids of the mixed form); GT: the grid's type
@@ -391,48 +391,54 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
         for (int base = 0; base < ntot; base += SPREAD_CH) {
             const int cnt = min(SPREAD_CH, ntot - base);
             __syncthreads();
-            if (tid < cnt) {
-                // position of candidate base + tid in the concatenated home-brick ranges
+            // candidate preparation split over the four waves: wave ax < 3
+            // forms the axis-ax weights of candidate base + lane, wave 3 its
+            // source coefficients
+            if (lane < cnt) {
+                // position of candidate base + lane in the concatenated home-brick ranges
                 int e = 0, off = 0;
-                const int idx = base + tid;
+                const int idx = base + lane;
 #pragma unroll
                 for (int h = 0; h < 8; h++) {
                     if (idx >= off && idx < off + hc[h]) e = hs[h] + idx - off;
                     off += hc[h];
                 }
                 const int s = e;
-                double w[3];
-                int i0[3];
-                atom_frac(a.g, a.sx[s], a.sy[s], a.sz[s], w, i0, ORD);
-#pragma unroll
-                for (int ax = 0; ax < 3; ax++) {
+                if (wv < 3) {
+                    // the per-axis steps of atom_frac (same grid_coord, so the
+                    // same stencil origin as the gathers and the brick sort)
+                    const double *Ar = wv == 0 ? a.g.A[0] : wv == 1 ? a.g.A[1] : a.g.A[2];
+                    const int Ka = wv == 0 ? K0 : wv == 1 ? K1 : K2;
+                    const double u = grid_coord(Ar, a.sx[s], a.sy[s], a.sz[s]);
+                    const double fl = floor(u);
+                    int i0 = ((int)fl - ORD + 1) % Ka;
+                    if (i0 < 0) i0 += Ka;
                     // weights in FP64 whatever T is (FP64 costs what FP32
                     // does here and keeps the mixed form's energy error down)
                     double t[PMQ_MAX_ORDER][4];
-                    bspline_weights<double, ND>(w[ax], ORD, t);
+                    bspline_weights<double, ND>(u - fl, ORD, t);
 #pragma unroll
                     for (int k = 0; k < ORD; k++)
 #pragma unroll
-                        for (int d = 0; d <= ND; d++) th[ax][k][d][tid] = t[k][d];
-                    si0[ax][tid] = i0[ax];
-                }
-                if (MODE == 0) {
+                        for (int d = 0; d <= ND; d++) th[wv][k][d][lane] = t[k][d];
+                    si0[wv][lane] = i0;
+                } else if (MODE == 0) {
                     double m[10], fm[10];
 #pragma unroll
                     for (int c = 0; c < 10; c++) m[c] = a.src[c * a.npad + s];
                     if (a.charge) m[0] = a.charge[s];
                     frac_mpole(a.g, m, fm);
 #pragma unroll
-                    for (int c = 0; c < 10; c++) cf[c][tid] = T(fm[c]);
+                    for (int c = 0; c < 10; c++) cf[c][lane] = T(fm[c]);
                 } else if (MODE == 2) {
-                    cf[0][tid] = T(a.src[s]);
+                    cf[0][lane] = T(a.src[s]);
                 } else {
                     for (int set = 0; set < nset; set++) {
                         const double *d = a.src + (size_t)set * 3 * a.npad;
                         const double ux = d[s], uy = d[a.npad + s], uz = d[2 * a.npad + s];
 #pragma unroll
                         for (int r = 0; r < 3; r++)
-                            cf[3 * set + r][tid] = T(a.g.A[r][0] * ux + a.g.A[r][1] * uy + a.g.A[r][2] * uz);
+                            cf[3 * set + r][lane] = T(a.g.A[r][0] * ux + a.g.A[r][1] * uy + a.g.A[r][2] * uz);
                     }
                 }
             }
