@@ -129,24 +129,37 @@ int pme_setup(pmq_ctx *c)
     const double V = (double)K[0] * K[1] * K[2] / det;
     const int H0 = K[0] / 2 + 1;
     // electrostatics: G = exp(-pi^2 h^2/a^2) / (pi V h^2 bsmod), m = 0 dropped;
-    // dispersion: Gd = -psi(2 pi |h|) / (V bsmod), m = 0 kept
+    // dispersion: Gd = -psi(2 pi |h|) / (V bsmod), m = 0 kept.  Index m maps
+    // to the frequency m or m - K (m > K/2).  In a skewed cell G(m) and G(-m)
+    // differ where a component sits at K/2 (that index stands for +K/2 on both
+    // sides, while the cross terms of h^2 change sign), and the real part of
+    // a full complex convolution equals a convolution with their mean, which
+    // is what the tables hold: the half-complex transforms then give the real
+    // potential of the full complex sum (and the same result whichever sign
+    // K/2 is given).
     std::vector<double> G((size_t)H0 * K[1] * K[2]), Gd(dew ? G.size() : 0);
     const double a2 = alpha * alpha;
     const double dpre = 2.0 * pow(M_PI, 1.5) * dalpha * dalpha * dalpha / 3.0;
+    auto kernels = [&](int m1, int m2, int m3, double *ge, double *gd) {
+        const int f1 = m1 > K[0] / 2 ? m1 - K[0] : m1;
+        const int f2 = m2 > K[1] / 2 ? m2 - K[1] : m2;
+        const int f3 = m3 > K[2] / 2 ? m3 - K[2] : m3;
+        double h[3];
+        for (int j = 0; j < 3; j++) h[j] = f1 * A[0][j] / K[0] + f2 * A[1][j] / K[1] + f3 * A[2][j] / K[2];
+        const double h2 = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
+        const double bs = bsmod[0][m1] * bsmod[1][m2] * bsmod[2][m3];
+        *gd = dew ? -dpre * disp_psi_unit(M_PI * sqrt(h2) / dalpha) / (V * bs) : 0.0;
+        *ge = (ew && (m1 || m2 || m3)) ? exp(-M_PI * M_PI * h2 / a2) / (M_PI * V * h2 * bs) : 0.0;
+    };
     for (int m3 = 0; m3 < K[2]; m3++)
         for (int m2 = 0; m2 < K[1]; m2++)
             for (int m1 = 0; m1 < H0; m1++) {
                 const size_t id = ((size_t)m3 * K[1] + m2) * H0 + m1;
-                const int f1 = m1 > K[0] / 2 ? m1 - K[0] : m1;
-                const int f2 = m2 > K[1] / 2 ? m2 - K[1] : m2;
-                const int f3 = m3 > K[2] / 2 ? m3 - K[2] : m3;
-                double h[3];
-                for (int j = 0; j < 3; j++) h[j] = f1 * A[0][j] / K[0] + f2 * A[1][j] / K[1] + f3 * A[2][j] / K[2];
-                const double h2 = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
-                const double bs = bsmod[0][m1] * bsmod[1][m2] * bsmod[2][m3];
-                if (dew) Gd[id] = -dpre * disp_psi_unit(M_PI * sqrt(h2) / dalpha) / (V * bs);
-                if (m1 == 0 && m2 == 0 && m3 == 0) { G[id] = 0.0; continue; }
-                G[id] = ew ? exp(-M_PI * M_PI * h2 / a2) / (M_PI * V * h2 * bs) : 0.0;
+                double ge, gd, ge2, gd2;
+                kernels(m1, m2, m3, &ge, &gd);
+                kernels((K[0] - m1) % K[0], (K[1] - m2) % K[1], (K[2] - m3) % K[2], &ge2, &gd2);
+                G[id] = ge == ge2 ? ge : 0.5 * (ge + ge2);
+                if (dew) Gd[id] = gd == gd2 ? gd : 0.5 * (gd + gd2);
             }
     PMQ_HIP(c->pme_g.alloc(G.size()));
     PMQ_HIP(hipMemcpyAsync(c->pme_g.p, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
