@@ -223,14 +223,14 @@ __global__ __launch_bounds__(64) void k_row_counts(int nblk, const int2 *__restr
     if (I == 0 && lane == 0) rowcnt[nblk * 64] = 0; // the scan's last element becomes the entry total
 }
 
-// One stored entry: the 6 tensor components and the column, one 64-byte
-// (double) or 32-byte (float) record, so the fill writes and the product
-// reads whole records.
+// One stored entry: the pair tensor is T = R2 r r^T - R1 I (r the minimum
+// image vector from the row's atom to the column's), so an entry keeps the
+// two scalars (R1, R2), 8 (float) or 16 (double) bytes, beside its column in
+// a separate int array shared by the full and the preconditioner matrices;
+// the products rebuild r from the sorted positions.
 template <class V>
-struct alignas(8 * sizeof(V)) TEnt {
-    V t[6]; // xx yy zz xy xz yz
-    int col;
-    int pad;
+struct alignas(2 * sizeof(V)) TEnt {
+    V r1, r2;
 };
 
 // V: stored type of the tensor (double, or float in the mixed form); T: the
@@ -248,6 +248,7 @@ struct FillArgs {
     int npad;
     size_t cap;
     TEnt<V> *ent;
+    int *col;
     // preconditioner: the pairs within its cutoff (null = none), their
     // offsets, each row's count of them, and their plain tensors
     const unsigned long long *nmask;
@@ -256,17 +257,11 @@ struct FillArgs {
 };
 
 template <class V, class T>
-__device__ __forceinline__ void store_T(TEnt<V> *ent, size_t e, int col, const T *R, T xr, T yr, T zr)
+__device__ __forceinline__ void store_T(TEnt<V> *ent, size_t e, const T *R)
 {
     TEnt<V> r;
-    r.t[0] = R[2] * xr * xr - R[1];
-    r.t[1] = R[2] * yr * yr - R[1];
-    r.t[2] = R[2] * zr * zr - R[1];
-    r.t[3] = R[2] * xr * yr;
-    r.t[4] = R[2] * xr * zr;
-    r.t[5] = R[2] * yr * zr;
-    r.col = col;
-    r.pad = 0;
+    r.r1 = V(R[1]);
+    r.r2 = V(R[2]);
     ent[e] = r;
 }
 
@@ -329,13 +324,19 @@ __global__ __launch_bounds__(64) void k_pair_fill(FillArgs<V> a)
                                     : (size_t)bbase[0][il] + __popcll(bmask[0][il] & ~ni & lo_j);
             const size_t ej = nearp ? (size_t)bnbase[1][jl] + __popcll(nj & lo_i)
                                     : (size_t)bbase[1][jl] + __popcll(bmask[1][jl] & ~nj & lo_i);
-            if (ei < a.cap) store_T(a.ent, ei, IJ.y * 64 + jl, R, xr, yr, zr);
-            if (ej < a.cap) store_T(a.ent, ej, IJ.x * 64 + il, R, xr, yr, zr);
+            if (ei < a.cap) {
+                store_T(a.ent, ei, R);
+                a.col[ei] = IJ.y * 64 + jl;
+            }
+            if (ej < a.cap) {
+                store_T(a.ent, ej, R);
+                a.col[ej] = IJ.x * 64 + il;
+            }
             if (nearp) {
                 T Rs[6];
                 pol_series<false, T>(r2, T(0), lam, Rs);
-                if (ei < a.cap) store_T(a.ent_s, ei, IJ.y * 64 + jl, Rs, xr, yr, zr);
-                if (ej < a.cap) store_T(a.ent_s, ej, IJ.x * 64 + il, Rs, xr, yr, zr);
+                if (ei < a.cap) store_T(a.ent_s, ei, Rs);
+                if (ej < a.cap) store_T(a.ent_s, ej, Rs);
             }
         }
     }
@@ -354,20 +355,16 @@ struct FoldArgs {
     const int *rowptr;
     size_t cap;
     TEnt<V> *ent;
+    const int *col;
     const int *rownear; // preconditioner: near entries per row (null = none)
     TEnt<V> *ent_s;
 };
 
 template <class V>
-__device__ __forceinline__ void add_T(TEnt<V> *ent, size_t e, const double *R, double xr, double yr, double zr)
+__device__ __forceinline__ void add_T(TEnt<V> *ent, size_t e, const double *R)
 {
-    V *t = ent[e].t;
-    t[0] += R[2] * xr * xr - R[1];
-    t[1] += R[2] * yr * yr - R[1];
-    t[2] += R[2] * zr * zr - R[1];
-    t[3] += R[2] * xr * yr;
-    t[4] += R[2] * xr * zr;
-    t[5] += R[2] * yr * zr;
+    ent[e].r1 = V((double)ent[e].r1 + R[1]);
+    ent[e].r2 = V((double)ent[e].r2 + R[2]);
 }
 
 // excluded pairs present in the matrix get (u - 1) lambda rr added to their
@@ -384,10 +381,10 @@ __global__ __launch_bounds__(64) void k_excl_fold(FoldArgs<V> a)
     int ei = -1, ek = -1;
     const int ci = (int)min((size_t)a.rowptr[si + 1], a.cap), ck = (int)min((size_t)a.rowptr[sk + 1], a.cap);
     for (int e = a.rowptr[si]; e < ci; e++)
-        if (a.ent[e].col == sk) { ei = e; break; }
+        if (a.col[e] == sk) { ei = e; break; }
     if (ei < 0) return; // beyond the cutoff: not in the matrix
     for (int e = a.rowptr[sk]; e < ck; e++)
-        if (a.ent[e].col == si) { ek = e; break; }
+        if (a.col[e] == si) { ek = e; break; }
     double xr = a.sx[sk] - a.sx[si], yr = a.sy[sk] - a.sy[si], zr = a.sz[sk] - a.sz[si];
     image<double>(a.box, xr, yr, zr);
     const double r2 = xr * xr + yr * yr + zr * zr;
@@ -395,71 +392,100 @@ __global__ __launch_bounds__(64) void k_excl_fold(FoldArgs<V> a)
     if (a.cpd) overlap_damp<double>(sqrt(r2), a.scp[np + si], a.scp[np + sk], lam);
     else thole_lambda<double>(sqrt(r2), a.spol[2 * np + si], a.spol[2 * np + sk], a.spol[np + si], a.spol[np + sk], lam);
     corr_series<double>(r2, q.u, lam, R);
-    add_T(a.ent, (size_t)ei, R, xr, yr, zr);
-    if (ek >= 0) add_T(a.ent, (size_t)ek, R, xr, yr, zr);
+    add_T(a.ent, (size_t)ei, R);
+    if (ek >= 0) add_T(a.ent, (size_t)ek, R);
     // the preconditioner's tensor of a near pair gets the same correction
     if (a.rownear) {
-        if (ei < a.rowptr[si] + a.rownear[si]) add_T(a.ent_s, (size_t)ei, R, xr, yr, zr);
-        if (ek >= 0 && ek < a.rowptr[sk] + a.rownear[sk]) add_T(a.ent_s, (size_t)ek, R, xr, yr, zr);
+        if (ei < a.rowptr[si] + a.rownear[si]) add_T(a.ent_s, (size_t)ei, R);
+        if (ek >= 0 && ek < a.rowptr[sk] + a.rownear[sk]) add_T(a.ent_s, (size_t)ek, R);
+    }
+}
+
+// The geometry a product needs to rebuild each entry's pair vector: the
+// sorted positions and the periodic box (the minimum image convention of the
+// fill).
+struct RowGeom {
+    const double *sx, *sy, *sz;
+    DevBox box;
+    double len[3], inv[3]; // rectangular box: edge lengths and their inverses
+};
+
+// box forms the products are built for: none, rectangular (nearest image by
+// rounding, the same image the fill's wrapping finds), any other (the
+// fill's own image function)
+enum { ROW_BOX_NONE = 0, ROW_BOX_ORTHO = 1, ROW_BOX_ANY = 2 };
+
+template <int BK>
+__device__ __forceinline__ void row_image(const RowGeom &geo, double &x, double &y, double &z)
+{
+    if constexpr (BK == ROW_BOX_ORTHO) {
+        x -= geo.len[0] * rint(x * geo.inv[0]);
+        y -= geo.len[1] * rint(y * geo.inv[1]);
+        z -= geo.len[2] * rint(z * geo.inv[2]);
+    } else if constexpr (BK == ROW_BOX_ANY) {
+        image<double>(geo.box, x, y, z);
+    }
+}
+
+// T u for one entry (column k, scalars R1 R2) added to acc, both dipole sets:
+// T u = R2 r (r . u) - R1 u with r rebuilt from the positions
+__device__ __forceinline__ void entry_apply(double r1, double r2, double xr, double yr, double zr, const double d[6],
+                                            double acc[6])
+{
+#pragma unroll
+    for (int set = 0; set < 2; set++) {
+        const double ux = d[3 * set], uy = d[3 * set + 1], uz = d[3 * set + 2];
+        const double ru = r2 * (xr * ux + yr * uy + zr * uz);
+        acc[3 * set + 0] += ru * xr - r1 * ux;
+        acc[3 * set + 1] += ru * yr - r1 * uy;
+        acc[3 * set + 2] += ru * zr - r1 * uz;
     }
 }
 
 // this lane's share (entries e0 + g, e0 + g + G, ... < e1) of T u for both
-// dipole sets, U entries at a time: their records are loaded together, then
-// the dipoles of their columns together, so a lane waits on two loads per U
+// dipole sets of the row whose atom sits at (xi, yi, zi), U entries at a
+// time: their columns and scalars are loaded together, then the positions
+// and dipoles of those columns together, so a lane waits on two loads per U
 // entries rather than two per entry.  Entries past e1 are clamped to e1 - 1
 // and weighted by zero.
-template <int G, int U, class V>
-__device__ __forceinline__ void row_apply(const TEnt<V> *ent, int e0, int e1, int g, const double *u, int np,
-                                          double acc[6])
+template <int G, int U, int BK, class V>
+__device__ __forceinline__ void row_apply(const int *cols, const TEnt<V> *ent, int e0, int e1, int g,
+                                          const RowGeom &geo, double xi, double yi, double zi, const double *u,
+                                          int np, double acc[6])
 {
-    if constexpr (U == 1) {
-        for (int e = e0 + g; e < e1; e += G) {
-            const TEnt<V> r = ent[e];
-            const int k = r.col;
-            const double txx = r.t[0], tyy = r.t[1], tzz = r.t[2];
-            const double txy = r.t[3], txz = r.t[4], tyz = r.t[5];
-#pragma unroll
-            for (int set = 0; set < 2; set++) {
-                const double *us = u + (size_t)set * 3 * np;
-                const double ux = us[k], uy = us[np + k], uz = us[2 * np + k];
-                acc[3 * set + 0] += txx * ux + txy * uy + txz * uz;
-                acc[3 * set + 1] += txy * ux + tyy * uy + tyz * uz;
-                acc[3 * set + 2] += txz * ux + tyz * uy + tzz * uz;
-            }
-        }
-        return;
-    }
     for (int base = e0 + g; base < e1; base += G * U) {
+        int k[U];
         TEnt<V> r[U];
         double wgt[U];
 #pragma unroll
-        for (int k = 0; k < U; k++) {
-            const int e = base + k * G;
-            wgt[k] = e < e1 ? 1.0 : 0.0;
-            r[k] = ent[min(e, e1 - 1)];
+        for (int q = 0; q < U; q++) {
+            const int e = min(base + q * G, e1 - 1);
+            wgt[q] = base + q * G < e1 ? 1.0 : 0.0;
+            k[q] = cols[e];
+            r[q] = ent[e];
         }
-        double d[U][6];
+        double p[U][3], d[U][6];
 #pragma unroll
-        for (int k = 0; k < U; k++) {
-            const int col = r[k].col;
+        for (int q = 0; q < U; q++) {
+            p[q][0] = geo.sx[k[q]]; p[q][1] = geo.sy[k[q]]; p[q][2] = geo.sz[k[q]];
 #pragma unroll
-            for (int q = 0; q < 6; q++) d[k][q] = u[(size_t)q * np + col];
+            for (int c = 0; c < 6; c++) d[q][c] = u[(size_t)c * np + k[q]];
         }
 #pragma unroll
-        for (int k = 0; k < U; k++) {
-            const double txx = wgt[k] * r[k].t[0], tyy = wgt[k] * r[k].t[1], tzz = wgt[k] * r[k].t[2];
-            const double txy = wgt[k] * r[k].t[3], txz = wgt[k] * r[k].t[4], tyz = wgt[k] * r[k].t[5];
-#pragma unroll
-            for (int set = 0; set < 2; set++) {
-                const double ux = d[k][3 * set], uy = d[k][3 * set + 1], uz = d[k][3 * set + 2];
-                acc[3 * set + 0] += txx * ux + txy * uy + txz * uz;
-                acc[3 * set + 1] += txy * ux + tyy * uy + tyz * uz;
-                acc[3 * set + 2] += txz * ux + tyz * uy + tzz * uz;
-            }
+        for (int q = 0; q < U; q++) {
+            double xr = p[q][0] - xi, yr = p[q][1] - yi, zr = p[q][2] - zi;
+            row_image<BK>(geo, xr, yr, zr);
+            entry_apply(wgt[q] * (double)r[q].r1, wgt[q] * (double)r[q].r2, xr, yr, zr, d[q], acc);
         }
     }
 }
+
+// entries per lane taken together by the product and the preconditioner,
+// and the lanes per row of each
+constexpr int SPMV_U = 2;
+constexpr int PREC_U = 2;
+constexpr int SPMV_G = 16;
+constexpr int PREC_G = 16;
 
 template <class V>
 struct SpmvArgs {
@@ -467,13 +493,15 @@ struct SpmvArgs {
     const int *rowptr;
     size_t cap;
     const TEnt<V> *ent;
+    const int *col;
+    RowGeom geo;
     const double *src; // 2 x 3 x npad
     double *dst;       // 2 x 3 x npad (overwritten)
     const double *skip; // nonzero: return at once (or null)
 };
 
 // dst = T src for both dipole sets; G lanes per row, FP64 arithmetic
-template <int G, class V>
+template <int G, int BK, class V>
 __global__ __launch_bounds__(256) void k_pol_spmv(SpmvArgs<V> a)
 {
     if (a.skip && *a.skip != 0.0) return;
@@ -483,9 +511,8 @@ __global__ __launch_bounds__(256) void k_pol_spmv(SpmvArgs<V> a)
     double acc[6] = {0, 0, 0, 0, 0, 0};
     if (row < np) {
         const int e1 = (int)min((size_t)a.rowptr[row + 1], a.cap);
-        // 4 records of 32 bytes in flight (FP32 tensor); the 64-byte FP64
-        // records run one at a time (more in flight cost occupancy, measured)
-        row_apply<G, sizeof(V) == 4 ? 4 : 1, V>(a.ent, a.rowptr[row], e1, g, a.src, np, acc);
+        row_apply<G, SPMV_U, BK, V>(a.col, a.ent, a.rowptr[row], e1, g, a.geo, a.geo.sx[row], a.geo.sy[row],
+                                a.geo.sz[row], a.src, np, acc);
     }
 #pragma unroll
     for (int c = 0; c < 6; c++)
@@ -510,6 +537,8 @@ struct PrecArgs {
     const int *rowptr, *rownear;
     size_t cap;
     const TEnt<V> *ent_s;
+    const int *col;
+    RowGeom geo;
     const double *spol;
     double udiag;
     const double *w, *r; // 2 x 3 x npad
@@ -519,7 +548,7 @@ struct PrecArgs {
     double *part;
 };
 
-template <int G, class V>
+template <int G, int BK, class V>
 __global__ __launch_bounds__(256) void k_pcg_prec(PrecArgs<V> a)
 {
     if (!a.init && a.sc[8] != 0.0) return;
@@ -529,7 +558,8 @@ __global__ __launch_bounds__(256) void k_pcg_prec(PrecArgs<V> a)
     double acc[6] = {0, 0, 0, 0, 0, 0};
     if (row < np) {
         const int e1 = (int)min((size_t)(a.rowptr[row] + a.rownear[row]), a.cap);
-        row_apply<G, 2, V>(a.ent_s, a.rowptr[row], e1, g, a.w, np, acc);
+        row_apply<G, PREC_U, BK, V>(a.col, a.ent_s, a.rowptr[row], e1, g, a.geo, a.geo.sx[row], a.geo.sy[row],
+                                a.geo.sz[row], a.w, np, acc);
     }
 #pragma unroll
     for (int c = 0; c < 6; c++)
@@ -579,7 +609,7 @@ static int pair_fill(pmq_ctx *c, int tgrid)
     fa.box = c->box;
     fa.aewald = c->prm.use_ewald ? c->prm.aewald : 0.0;
     fa.mask = c->tmask.p; fa.soff = c->soff.p; fa.rowptr = c->rowptr.p;
-    fa.npad = np; fa.cap = c->tcap; fa.ent = (TEnt<V> *)c->tval.p;
+    fa.npad = np; fa.cap = c->tcap; fa.ent = (TEnt<V> *)c->tval.p; fa.col = c->tcol.p;
     fa.nmask = c->prec_on ? c->nmask.p : nullptr;
     fa.nsoff = c->nsoff.p; fa.rownear = c->rownear.p;
     fa.ent_s = c->prec_on ? (TEnt<V> *)c->tval_s.p : nullptr;
@@ -603,6 +633,7 @@ static int pair_fill(pmq_ctx *c, int tgrid)
         fo.cpd = c->prm.pentype != PMQ_PEN_NONE ? 1 : 0;
         fo.box = c->box; fo.npad = np; fo.rowptr = c->rowptr.p; fo.cap = c->tcap;
         fo.ent = (TEnt<V> *)c->tval.p;
+        fo.col = c->tcol.p;
         fo.rownear = c->prec_on ? c->rownear.p : nullptr;
         fo.ent_s = c->prec_on ? (TEnt<V> *)c->tval_s.p : nullptr;
         hipLaunchKernelGGL((k_excl_fold<V>), dim3((nexcl + 63) / 64), dim3(64), 0, st, fo);
@@ -738,11 +769,12 @@ int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe)
         PMQ_HIP(hipStreamSynchronize(st));
         if ((size_t)nnz > c->tcap || c->tcap == 0) {
             c->tcap = (size_t)nnz + (size_t)nnz / 4 + 64;
-            PMQ_HIP(c->tval.alloc(8 * c->tcap)); // one 64-byte record per entry (32 bytes used by the mixed form)
+            PMQ_HIP(c->tval.alloc(2 * c->tcap)); // 16 bytes per entry (8 used by the mixed form)
+            PMQ_HIP(c->tcol.alloc(c->tcap));
         }
     }
     // the preconditioner's tensors sit at the entries of the near pairs
-    if (c->prec_on) PMQ_HIP(c->tval_s.alloc(8 * c->tcap));
+    if (c->prec_on) PMQ_HIP(c->tval_s.alloc(2 * c->tcap));
     hipLaunchKernelGGL(k_nnz_check, dim3(1), dim3(1), 0, st, c->rowptr.p + np, c->tcap, c->status.p);
 
     c->tmixed = mixed;
@@ -898,16 +930,42 @@ int tile_scalars(pmq_ctx *c, int which, bool full)
     return PMQ_OK;
 }
 
+static RowGeom row_geom(const pmq_ctx *c)
+{
+    RowGeom g;
+    g.sx = c->sx.p; g.sy = c->sy.p; g.sz = c->sz.p;
+    g.box = c->box;
+    const double len[3] = {c->box.xbox, c->box.ybox, c->box.zbox};
+    for (int q = 0; q < 3; q++) {
+        g.len[q] = len[q];
+        g.inv[q] = len[q] > 0.0 ? 1.0 / len[q] : 0.0;
+    }
+    return g;
+}
+
+static int row_box(const pmq_ctx *c)
+{
+    if (c->box.kind == PMQ_BOX_NONE) return ROW_BOX_NONE;
+    if (c->box.kind == PMQ_BOX_ORTHO) return ROW_BOX_ORTHO;
+    return ROW_BOX_ANY;
+}
+
 template <class V>
 static int spmv(pmq_ctx *c, const double *src, double *dst, const double *skip)
 {
-    constexpr int G = 8;
+    constexpr int G = SPMV_G;
     SpmvArgs<V> a;
     a.npad = c->npad; a.rowptr = c->rowptr.p; a.cap = c->tcap; a.ent = (const TEnt<V> *)c->tval.p;
+    a.col = c->tcol.p; a.geo = row_geom(c);
     a.src = src; a.dst = dst; a.skip = skip;
     const size_t lanes = (size_t)c->npad * G;
     ProfScope ps_(c, "pol_mutual_spmv");
-    hipLaunchKernelGGL((k_pol_spmv<G, V>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, c->stream, a);
+    const dim3 grid((unsigned)((lanes + 255) / 256));
+    switch (row_box(c)) {
+    case ROW_BOX_NONE: hipLaunchKernelGGL((k_pol_spmv<G, ROW_BOX_NONE, V>), grid, dim3(256), 0, c->stream, a); break;
+    case ROW_BOX_ORTHO: hipLaunchKernelGGL((k_pol_spmv<G, ROW_BOX_ORTHO, V>), grid, dim3(256), 0, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_pol_spmv<G, ROW_BOX_ANY, V>), grid, dim3(256), 0, c->stream, a); break;
+    }
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }
@@ -916,18 +974,24 @@ template <class V>
 static int precondition(pmq_ctx *c, const double *w, const double *r, double *z, double *p, bool init,
                         const double *sc, double *part)
 {
-    constexpr int G = 8;
+    constexpr int G = PREC_G;
     PrecArgs<V> a;
     a.npad = c->npad; a.rowptr = c->rowptr.p; a.rownear = c->rownear.p; a.cap = c->tcap;
     a.ent_s = (const TEnt<V> *)c->tval_s.p;
+    a.col = c->tcol.p; a.geo = row_geom(c);
     a.spol = c->spol.p; a.udiag = c->prm.udiag;
     a.w = w; a.r = r; a.z = z; a.p = p; a.init = init ? 1 : 0; a.sc = sc; a.part = part;
-    hipLaunchKernelGGL((k_pcg_prec<G, V>), dim3((unsigned)pair_matrix_prec_blocks(c)), dim3(256), 0, c->stream, a);
+    const dim3 grid((unsigned)pair_matrix_prec_blocks(c));
+    switch (row_box(c)) {
+    case ROW_BOX_NONE: hipLaunchKernelGGL((k_pcg_prec<G, ROW_BOX_NONE, V>), grid, dim3(256), 0, c->stream, a); break;
+    case ROW_BOX_ORTHO: hipLaunchKernelGGL((k_pcg_prec<G, ROW_BOX_ORTHO, V>), grid, dim3(256), 0, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_pcg_prec<G, ROW_BOX_ANY, V>), grid, dim3(256), 0, c->stream, a); break;
+    }
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }
 
-int pair_matrix_prec_blocks(const pmq_ctx *c) { return (int)(((size_t)c->npad * 8 + 255) / 256); }
+int pair_matrix_prec_blocks(const pmq_ctx *c) { return (int)(((size_t)c->npad * PREC_G + 255) / 256); }
 
 int pair_matrix_precondition(pmq_ctx *c, const double *w, const double *r, double *z, double *p, bool init,
                              const double *sc, double *part)

@@ -288,7 +288,8 @@ struct pmq_ctx {
     pmq::DBuf<int> adj, adjn;                    // per block: tiles it takes part in (nblk x nblk), count
     pmq::DBuf<int> rowcnt, rowptr;               // npad + 1
     pmq::DBuf<unsigned char> pscan_tmp;
-    pmq::DBuf<double> tval;                      // tcap records of 64 bytes: T xx yy zz xy xz yz, column
+    pmq::DBuf<double> tval;                      // tcap entries (R1, R2) of T = R2 r r^T - R1 I (double2, or float2 mixed)
+    pmq::DBuf<int> tcol;                         // tcap columns of those entries (shared with tval_s)
     // PCG preconditioner: rows keep their pairs within usolvcut first
     bool prec_on = false;                        // this evaluation solves with the sparse preconditioner
     pmq::DBuf<unsigned long long> nmask;         // 128 per tile: the pairs within usolvcut (row, column masks)
