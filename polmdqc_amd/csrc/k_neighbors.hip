@@ -184,9 +184,15 @@ __global__ void k_block_bounds(int n, int nblk, DevBox b, const double *__restri
     const int I = blockIdx.x;
     const int lane = threadIdx.x;
     const int s = I * PMQ_BLOCK + lane;
-    __shared__ double lo[3][PMQ_BLOCK], hi[3][PMQ_BLOCK];
+    __shared__ double lo[3][PMQ_BLOCK], hi[3][PMQ_BLOCK], rlo[3][PMQ_BLOCK], rhi[3][PMQ_BLOCK];
     double p[3] = {0, 0, 0};
     bool valid = s < n;
+    // span of the stored (unwrapped) coordinates of every slot of the block,
+    // padding included: the tile mask kernel's one-image test reads it
+    {
+        const double q[3] = {sx[s], sy[s], sz[s]};
+        for (int a = 0; a < 3; a++) rlo[a][lane] = rhi[a][lane] = q[a];
+    }
     if (valid) {
         p[0] = sx[s]; p[1] = sy[s]; p[2] = sz[s];
         if (b.kind != PMQ_BOX_NONE) {
@@ -209,12 +215,16 @@ __global__ void k_block_bounds(int n, int nblk, DevBox b, const double *__restri
             for (int a = 0; a < 3; a++) {
                 lo[a][lane] = fmin(lo[a][lane], lo[a][lane + w]);
                 hi[a][lane] = fmax(hi[a][lane], hi[a][lane + w]);
+                rlo[a][lane] = fmin(rlo[a][lane], rlo[a][lane + w]);
+                rhi[a][lane] = fmax(rhi[a][lane], rhi[a][lane + w]);
             }
         __syncthreads();
     }
     if (lane < 3) {
-        blk[6 * I + lane] = 0.5 * (lo[lane][0] + hi[lane][0]);
-        blk[6 * I + 3 + lane] = 0.5 * (hi[lane][0] - lo[lane][0]);
+        blk[PMQ_BLK_STRIDE * I + lane] = 0.5 * (lo[lane][0] + hi[lane][0]);
+        blk[PMQ_BLK_STRIDE * I + 3 + lane] = 0.5 * (hi[lane][0] - lo[lane][0]);
+        blk[PMQ_BLK_STRIDE * I + 6 + lane] = rlo[lane][0];
+        blk[PMQ_BLK_STRIDE * I + 9 + lane] = rhi[lane][0];
     }
 }
 
@@ -239,7 +249,7 @@ __global__ void k_tile_flags(int nblk, long long npairs, DevBox b, double cut, c
     int I, J;
     tri_index(t, nblk, I, J);
     bool keep = true;
-    const double *bi = blk + 6 * I, *bj = blk + 6 * J;
+    const double *bi = blk + PMQ_BLK_STRIDE * I, *bj = blk + PMQ_BLK_STRIDE * J;
     double d[3] = {bj[0] - bi[0], bj[1] - bi[1], bj[2] - bi[2]};
     if (b.kind == PMQ_BOX_NONE || b.kind == PMQ_BOX_ORTHO) {
         if (b.kind == PMQ_BOX_ORTHO) {

@@ -53,6 +53,7 @@ struct MaskArgs {
     unsigned long long *mmask; // 64 per tile: row masks of the pairs with an active atom (multipole energy)
     double us2;                // preconditioner cutoff squared (<= off2)
     unsigned long long *nmask; // 128 per tile: the pairs within it (row, column masks), or null
+    const double *blk;         // block bounds (PMQ_BLK_STRIDE per block)
 };
 
 __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
@@ -61,9 +62,30 @@ __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
     const int ntiles = *a.ntiles;
     __shared__ double jx[64], jy[64], jz[64];
     __shared__ int jflag[64];
+    const double L[3] = {a.box.xbox, a.box.ybox, a.box.zbox};
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 IJ = a.tiles[t];
         const int si = IJ.x * 64 + lane, sj = IJ.y * 64 + lane;
+        // One image for the whole tile where every pair's offset v = xj - xi
+        // (bounded by the two blocks' coordinate spans) lies inside one window
+        // of image()'s wrapping: |v| < L/2 (left as is), or L/2 < |v| < 3L/2
+        // (one edge subtracted toward zero).  The subtraction is then the
+        // same operation on the same operand as image()'s, so the vector is
+        // the same to the bit.  Other boxes, and tiles that straddle a window
+        // edge, take image() per pair.
+        double sh[3] = {0.0, 0.0, 0.0};
+        bool one = a.box.kind == PMQ_BOX_NONE || a.box.kind == PMQ_BOX_ORTHO;
+        if (a.box.kind == PMQ_BOX_ORTHO) {
+            const double *bi = a.blk + PMQ_BLK_STRIDE * IJ.x, *bj = a.blk + PMQ_BLK_STRIDE * IJ.y;
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const double dlo = bj[6 + q] - bi[9 + q], dhi = bj[9 + q] - bi[6 + q];
+                const double h = 0.5 * L[q], eps = 1e-9 * L[q];
+                if (dlo > h + eps && dhi < 3.0 * h - eps) sh[q] = L[q];
+                else if (dlo > -3.0 * h + eps && dhi < -h - eps) sh[q] = -L[q];
+                else if (!(dlo > -h + eps && dhi < h - eps)) one = false;
+            }
+        }
         __syncthreads();
         jx[lane] = a.sx[sj]; jy[lane] = a.sy[sj]; jz[lane] = a.sz[sj];
         jflag[lane] = a.sflag[sj];
@@ -72,16 +94,30 @@ __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
         __syncthreads();
         unsigned long long m = 0ull, mm = 0ull, nm = 0ull;
         if (fi & 1) {
-            for (int jj = IJ.x == IJ.y ? lane + 1 : 0; jj < 64; jj++) {
-                const int fj = jflag[jj];
-                if (!(fj & 1)) continue;
-                double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
-                image<double>(a.box, xr, yr, zr);
-                const double r2 = xr * xr + yr * yr + zr * zr;
-                if (r2 <= a.off2) {
-                    m |= 1ull << jj;
-                    if ((fi | fj) & 2) mm |= 1ull << jj;
-                    if (r2 <= a.us2) nm |= 1ull << jj;
+            const bool diag = IJ.x == IJ.y;
+            if (one) {
+#pragma unroll 8
+                for (int jj = 0; jj < 64; jj++) {
+                    const int fj = jflag[jj];
+                    const double xr = (jx[jj] - xi) - sh[0], yr = (jy[jj] - yi) - sh[1], zr = (jz[jj] - zi) - sh[2];
+                    const double r2 = xr * xr + yr * yr + zr * zr;
+                    const bool in = (fj & 1) && (!diag || jj > lane) && r2 <= a.off2;
+                    m |= (unsigned long long)in << jj;
+                    mm |= (unsigned long long)(in && ((fi | fj) & 2)) << jj;
+                    nm |= (unsigned long long)(in && r2 <= a.us2) << jj;
+                }
+            } else {
+                for (int jj = diag ? lane + 1 : 0; jj < 64; jj++) {
+                    const int fj = jflag[jj];
+                    if (!(fj & 1)) continue;
+                    double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
+                    image<double>(a.box, xr, yr, zr);
+                    const double r2 = xr * xr + yr * yr + zr * zr;
+                    if (r2 <= a.off2) {
+                        m |= 1ull << jj;
+                        if ((fi | fj) & 2) mm |= 1ull << jj;
+                        if (r2 <= a.us2) nm |= 1ull << jj;
+                    }
                 }
             }
         }
@@ -730,6 +766,7 @@ int tile_structure(pmq_ctx *c, double off2, bool safe)
     ma.box = c->box; ma.off2 = off2; ma.mask = c->tmask.p; ma.mmask = c->mmask.p;
     ma.us2 = c->prec_on ? std::min(c->prm.usolvcut * c->prm.usolvcut, off2) : -1.0;
     ma.nmask = c->prec_on ? c->nmask.p : nullptr;
+    ma.blk = c->blk.p;
     hipLaunchKernelGGL(k_tile_masks, dim3(c->tgrid), dim3(64), 0, st, ma);
     hipLaunchKernelGGL(k_block_adj, dim3(nblk), dim3(64), 0, st, nblk, c->tile_cap, c->tileflag.p, c->tilescan.p,
                        c->adj.p, c->adjn.p);

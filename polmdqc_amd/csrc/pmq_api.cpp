@@ -326,7 +326,7 @@ int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
     PMQ_HIP(c->smp.alloc(10 * (size_t)npad)); PMQ_HIP(c->scp.alloc(3 * (size_t)npad));
     PMQ_HIP(c->sflag.alloc(npad)); PMQ_HIP(c->smol.alloc(npad));
     PMQ_HIP(c->keys.alloc(n)); PMQ_HIP(c->keys_sorted.alloc(n));
-    PMQ_HIP(c->blk.alloc(6 * (size_t)nblk));
+    PMQ_HIP(c->blk.alloc(PMQ_BLK_STRIDE * (size_t)nblk));
     PMQ_HIP(c->tileflag.alloc(npairs)); PMQ_HIP(c->tilescan.alloc(npairs));
     PMQ_HIP(c->tiles.alloc(npairs));
     c->ntiles_cap = (int)std::min<long long>(npairs, 1 << 30);

@@ -23,6 +23,7 @@
 #include "../../include/pmq.h"
 
 #define PMQ_BLOCK 64
+#define PMQ_BLK_STRIDE 12 // doubles per block in the block-bounds array
 
 namespace pmq {
 
@@ -233,7 +234,8 @@ struct pmq_ctx {
     pmq::DBuf<int> smol;             // npad molecule ids
     // neighbor structure
     pmq::DBuf<unsigned long long> keys, keys_sorted;
-    pmq::DBuf<double> blk;           // 6 * nblk: center xyz, half extent xyz
+    pmq::DBuf<double> blk;           // PMQ_BLK_STRIDE * nblk: center xyz, half extent xyz (wrapped into the
+                                     // cell), then lowest and highest stored coordinate xyz
     pmq::DBuf<int> tileflag, tilescan;
     pmq::DBuf<int2> tiles;
     pmq::DBuf<int> ntiles_d;
