@@ -364,7 +364,10 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
     constexpr int ND = MODE == 0 ? 2 : MODE == 1 ? 1 : 0;
     constexpr int NC = MODE == 0 ? 10 : MODE == 1 ? 6 : 1;
     constexpr int NS = MODE == 1 ? 2 : 1;
-    __shared__ T th[3][ORD][ND + 1][SPREAD_CH];
+    // weights of candidate j: th[axis][derivative][j][k]; k fastest, so the
+    // lanes of a wave (grid points at different stencil offsets k, the same
+    // candidate j) read different banks
+    __shared__ T th[3][ND + 1][SPREAD_CH][ORD];
     __shared__ T cf[NC][SPREAD_CH];
     __shared__ int si0[3][SPREAD_CH];
     __shared__ T red[4][NS][64];
@@ -433,7 +436,7 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
 #pragma unroll
                     for (int k = 0; k < ORD; k++)
 #pragma unroll
-                        for (int d = 0; d <= ND; d++) th[wv][k][d][lane] = t[k][d];
+                        for (int d = 0; d <= ND; d++) th[wv][d][lane][k] = t[k][d];
                     si0[wv][lane] = i0;
                 } else if (MODE == 0) {
                     double m[10], fm[10];
@@ -464,19 +467,19 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
                 if (kz < 0) kz += K2;
                 if (kx >= ORD || ky >= ORD || kz >= ORD) continue;
                 if (MODE == 0) {
-                    const T x0 = th[0][kx][0][j], x1 = th[0][kx][1][j], x2 = th[0][kx][2][j];
-                    const T y0 = th[1][ky][0][j], y1 = th[1][ky][1][j], y2 = th[1][ky][2][j];
-                    const T z0 = th[2][kz][0][j], z1 = th[2][kz][1][j], z2 = th[2][kz][2][j];
+                    const T x0 = th[0][0][j][kx], x1 = th[0][1][j][kx], x2 = th[0][2][j][kx];
+                    const T y0 = th[1][0][j][ky], y1 = th[1][1][j][ky], y2 = th[1][2][j][ky];
+                    const T z0 = th[2][0][j][kz], z1 = th[2][1][j][kz], z2 = th[2][2][j][kz];
                     acc[0] += cf[0][j] * x0 * y0 * z0 + cf[1][j] * x1 * y0 * z0 + cf[2][j] * x0 * y1 * z0
                             + cf[3][j] * x0 * y0 * z1 + cf[4][j] * x2 * y0 * z0 + cf[5][j] * x0 * y2 * z0
                             + cf[6][j] * x0 * y0 * z2 + cf[7][j] * x1 * y1 * z0 + cf[8][j] * x1 * y0 * z1
                             + cf[9][j] * x0 * y1 * z1;
                 } else if (MODE == 2) {
-                    acc[0] += cf[0][j] * th[0][kx][0][j] * th[1][ky][0][j] * th[2][kz][0][j];
+                    acc[0] += cf[0][j] * th[0][0][j][kx] * th[1][0][j][ky] * th[2][0][j][kz];
                 } else {
-                    const T x0 = th[0][kx][0][j], x1 = th[0][kx][1][j];
-                    const T y0 = th[1][ky][0][j], y1 = th[1][ky][1][j];
-                    const T z0 = th[2][kz][0][j], z1 = th[2][kz][1][j];
+                    const T x0 = th[0][0][j][kx], x1 = th[0][1][j][kx];
+                    const T y0 = th[1][0][j][ky], y1 = th[1][1][j][ky];
+                    const T z0 = th[2][0][j][kz], z1 = th[2][1][j][kz];
                     const T w100 = x1 * y0 * z0, w010 = x0 * y1 * z0, w001 = x0 * y0 * z1;
 #pragma unroll
                     for (int q = 0; q < NS; q++)
