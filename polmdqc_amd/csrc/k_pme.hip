@@ -355,6 +355,13 @@ __global__ void k_pme_brick_ranges(int n, int nb0, int nb1, int nb2, const unsig
 // order, so the grid is bit-reproducible.
 constexpr int SPREAD_CH = 64; // one wave: candidate j is prepared by lane j of each wave
 
+// N values read or written by one LDS instruction (two or three for 32 and
+// 48 bytes)
+template <class T, int N>
+struct alignas(N * sizeof(T) >= 16 ? 16 : N * sizeof(T)) LdsVec {
+    T v[N];
+};
+
 // T: the arithmetic of the weights and sums (double in the plain form, float
 // for the electrostatic grids of the mixed form); GT: the grid's type
 template <int ORD, int MODE, class T, class GT = T>
@@ -364,12 +371,15 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
     constexpr int ND = MODE == 0 ? 2 : MODE == 1 ? 1 : 0;
     constexpr int NC = MODE == 0 ? 10 : MODE == 1 ? 6 : 1;
     constexpr int NS = MODE == 1 ? 2 : 1;
-    // weights of candidate j: th[axis][derivative][j][k]; k fastest, so the
-    // lanes of a wave (grid points at different stencil offsets k, the same
-    // candidate j) read different banks
-    __shared__ T th[3][ND + 1][SPREAD_CH][ORD];
-    __shared__ T cf[NC][SPREAD_CH];
-    __shared__ int si0[3][SPREAD_CH];
+    // per candidate j: th[axis][j][k] the weight and its derivatives at
+    // stencil offset k (one vector read; the lanes of a wave, at different k
+    // for the same j, read different banks), cf[j] the source coefficients,
+    // si0[j] the stencil origin
+    constexpr int DP = ND == 2 ? 4 : ND + 1;
+    constexpr int NCP = MODE == 0 ? 12 : MODE == 1 ? 8 : 1;
+    __shared__ LdsVec<T, DP> th[3][SPREAD_CH][ORD];
+    __shared__ LdsVec<T, NCP> cf[SPREAD_CH];
+    __shared__ int4 si0[SPREAD_CH];
     __shared__ T red[4][NS][64];
     GT *grid = (GT *)a.grid;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -436,8 +446,8 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
 #pragma unroll
                     for (int k = 0; k < ORD; k++)
 #pragma unroll
-                        for (int d = 0; d <= ND; d++) th[wv][d][lane][k] = t[k][d];
-                    si0[wv][lane] = i0;
+                        for (int d = 0; d <= ND; d++) th[wv][lane][k].v[d] = t[k][d];
+                    (wv == 0 ? si0[lane].x : wv == 1 ? si0[lane].y : si0[lane].z) = i0;
                 } else if (MODE == 0) {
                     double m[10], fm[10];
 #pragma unroll
@@ -445,45 +455,49 @@ __global__ __launch_bounds__(256) void k_pme_spread_brick(BrickArgs a)
                     if (a.charge) m[0] = a.charge[s];
                     frac_mpole(a.g, m, fm);
 #pragma unroll
-                    for (int c = 0; c < 10; c++) cf[c][lane] = T(fm[c]);
+                    for (int c = 0; c < 10; c++) cf[lane].v[c] = T(fm[c]);
                 } else if (MODE == 2) {
-                    cf[0][lane] = T(a.src[s]);
+                    cf[lane].v[0] = T(a.src[s]);
                 } else {
                     for (int set = 0; set < nset; set++) {
                         const double *d = a.src + (size_t)set * 3 * a.npad;
                         const double ux = d[s], uy = d[a.npad + s], uz = d[2 * a.npad + s];
 #pragma unroll
                         for (int r = 0; r < 3; r++)
-                            cf[3 * set + r][lane] = T(a.g.A[r][0] * ux + a.g.A[r][1] * uy + a.g.A[r][2] * uz);
+                            cf[lane].v[3 * set + r] = T(a.g.A[r][0] * ux + a.g.A[r][1] * uy + a.g.A[r][2] * uz);
                     }
                 }
             }
             __syncthreads();
             if (!pon) continue;
             for (int j = wv; j < cnt; j += 4) {
-                int kx = gx - si0[0][j], ky = gy - si0[1][j], kz = gz - si0[2][j];
+                const int4 o = si0[j];
+                int kx = gx - o.x, ky = gy - o.y, kz = gz - o.z;
                 if (kx < 0) kx += K0;
                 if (ky < 0) ky += K1;
                 if (kz < 0) kz += K2;
                 if (kx >= ORD || ky >= ORD || kz >= ORD) continue;
+                const LdsVec<T, DP> X = th[0][j][kx], Y = th[1][j][ky], Z = th[2][j][kz];
+                const LdsVec<T, NCP> C = cf[j];
                 if (MODE == 0) {
-                    const T x0 = th[0][0][j][kx], x1 = th[0][1][j][kx], x2 = th[0][2][j][kx];
-                    const T y0 = th[1][0][j][ky], y1 = th[1][1][j][ky], y2 = th[1][2][j][ky];
-                    const T z0 = th[2][0][j][kz], z1 = th[2][1][j][kz], z2 = th[2][2][j][kz];
-                    acc[0] += cf[0][j] * x0 * y0 * z0 + cf[1][j] * x1 * y0 * z0 + cf[2][j] * x0 * y1 * z0
-                            + cf[3][j] * x0 * y0 * z1 + cf[4][j] * x2 * y0 * z0 + cf[5][j] * x0 * y2 * z0
-                            + cf[6][j] * x0 * y0 * z2 + cf[7][j] * x1 * y1 * z0 + cf[8][j] * x1 * y0 * z1
-                            + cf[9][j] * x0 * y1 * z1;
+                    const T x0 = X.v[0], x1 = X.v[1], x2 = X.v[2 % DP];
+                    const T y0 = Y.v[0], y1 = Y.v[1], y2 = Y.v[2 % DP];
+                    const T z0 = Z.v[0], z1 = Z.v[1], z2 = Z.v[2 % DP];
+                    acc[0] += C.v[0] * x0 * y0 * z0 + C.v[1] * x1 * y0 * z0 + C.v[2] * x0 * y1 * z0
+                            + C.v[3] * x0 * y0 * z1 + C.v[4] * x2 * y0 * z0 + C.v[5] * x0 * y2 * z0
+                            + C.v[6] * x0 * y0 * z2 + C.v[7] * x1 * y1 * z0 + C.v[8] * x1 * y0 * z1
+                            + C.v[9] * x0 * y1 * z1;
                 } else if (MODE == 2) {
-                    acc[0] += cf[0][j] * th[0][0][j][kx] * th[1][0][j][ky] * th[2][0][j][kz];
+                    acc[0] += C.v[0] * X.v[0] * Y.v[0] * Z.v[0];
                 } else {
-                    const T x0 = th[0][0][j][kx], x1 = th[0][1][j][kx];
-                    const T y0 = th[1][0][j][ky], y1 = th[1][1][j][ky];
-                    const T z0 = th[2][0][j][kz], z1 = th[2][1][j][kz];
+                    const T x0 = X.v[0], x1 = X.v[1 % DP];
+                    const T y0 = Y.v[0], y1 = Y.v[1 % DP];
+                    const T z0 = Z.v[0], z1 = Z.v[1 % DP];
                     const T w100 = x1 * y0 * z0, w010 = x0 * y1 * z0, w001 = x0 * y0 * z1;
 #pragma unroll
                     for (int q = 0; q < NS; q++)
-                        acc[q] += cf[3 * q][j] * w100 + cf[3 * q + 1][j] * w010 + cf[3 * q + 2][j] * w001;
+                        acc[q] += C.v[(3 * q) % NCP] * w100 + C.v[(3 * q + 1) % NCP] * w010
+                                + C.v[(3 * q + 2) % NCP] * w001;
                 }
             }
         }
