@@ -1,7 +1,7 @@
 """GPU parity of the permanent-multipole path (plain FP64) through the C ABI.
 
-Each reference test system (and the two synthetic BASELINE-sized water boxes
-in non-Ewald form) is run on the GPU and compared with the reference's own
+Each reference test system (and the two synthetic water boxes, 648 and
+12,288 atoms, in non-Ewald form) is run on the GPU and compared with the reference's own
 dumped outputs and with the oracle.  Bar: relative error <= 1e-9 on energy,
 gradient and virial; nem exact; einter and the per-atom partition aem too.
 """
