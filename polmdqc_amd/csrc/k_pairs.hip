@@ -54,6 +54,7 @@ struct MaskArgs {
     double us2;                // preconditioner cutoff squared (<= off2)
     unsigned long long *nmask; // 128 per tile: the pairs within it (row, column masks), or null
     const double *blk;         // block bounds (PMQ_BLK_STRIDE per block)
+    unsigned long long *xmask; // 128 per tile: copies of the in-cutoff and active rows (scaled pairs cleared later)
 };
 
 __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
@@ -142,6 +143,8 @@ __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
         a.mask[(size_t)t * 128 + lane] = m;
         a.mask[(size_t)t * 128 + 64 + lane] = col;
         a.mmask[(size_t)t * 64 + lane] = mm;
+        a.xmask[(size_t)t * 128 + lane] = m;
+        a.xmask[(size_t)t * 128 + 64 + lane] = mm;
     }
 }
 
@@ -687,20 +690,10 @@ int status_reset(pmq_ctx *c)
 
 // Rows of the tile masks without the scaled pairs (the 1-2..1-5 and
 // polarization-group pairs): [0,64) the in-cutoff rows, [64,128) the rows
-// with an active atom.  The pair kernels skip those pairs and the exclusion
+// with an active atom, written by k_tile_masks and cleared of the scaled
+// pairs here.  The pair kernels skip those pairs and the exclusion
 // kernels evaluate them whole, at their scales, in FP64: large intramolecular
 // pair terms are then never formed in one precision and cancelled in another.
-__global__ void k_xmask_copy(const int *ntiles, const unsigned long long *__restrict__ mask,
-                             const unsigned long long *__restrict__ mmask, unsigned long long *__restrict__ xmask)
-{
-    const size_t tot = (size_t)*ntiles * 64;
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot; e += (size_t)gridDim.x * blockDim.x) {
-        const size_t t = e >> 6, l = e & 63;
-        xmask[t * 128 + l] = mask[t * 128 + l];
-        xmask[t * 128 + 64 + l] = mmask[t * 64 + l];
-    }
-}
-
 __global__ void k_xmask_clear(int nexcl, const ExclPair *__restrict__ excl, const int *__restrict__ iperm,
                               const int2 *__restrict__ tiles, const int *__restrict__ adj,
                               const int *__restrict__ adjn, int nblk, const int *ntiles,
@@ -767,11 +760,10 @@ int tile_structure(pmq_ctx *c, double off2, bool safe)
     ma.us2 = c->prec_on ? std::min(c->prm.usolvcut * c->prm.usolvcut, off2) : -1.0;
     ma.nmask = c->prec_on ? c->nmask.p : nullptr;
     ma.blk = c->blk.p;
+    ma.xmask = c->xmask.p;
     hipLaunchKernelGGL(k_tile_masks, dim3(c->tgrid), dim3(64), 0, st, ma);
     hipLaunchKernelGGL(k_block_adj, dim3(nblk), dim3(64), 0, st, nblk, c->tile_cap, c->tileflag.p, c->tilescan.p,
                        c->adj.p, c->adjn.p);
-    hipLaunchKernelGGL(k_xmask_copy, dim3((unsigned)std::min<size_t>((nt * 64 + 255) / 256, 8192)), dim3(256), 0, st,
-                       c->ntiles_eff.p, c->tmask.p, c->mmask.p, c->xmask.p);
     const int nexcl = (int)c->h_excl.size();
     if (nexcl)
         hipLaunchKernelGGL(k_xmask_clear, dim3((nexcl + 255) / 256), dim3(256), 0, st, nexcl, c->excl.p, c->iperm.p,
