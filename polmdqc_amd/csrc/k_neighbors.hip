@@ -288,20 +288,17 @@ __global__ void k_tile_flags(int nblk, long long npairs, DevBox b, double cut, c
     flag[t] = keep ? 1 : 0;
 }
 
+// the listed tiles in pair order; the last pair's thread also stores the
+// count
 __global__ void k_tile_write(int nblk, long long npairs, const int *__restrict__ flag, const int *__restrict__ scan,
-                             int2 *__restrict__ tiles)
+                             int2 *__restrict__ tiles, int *__restrict__ count)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == npairs - 1) count[0] = scan[t] + flag[t];
     if (t >= npairs || !flag[t]) return;
     int I, J;
     tri_index(t, nblk, I, J);
     tiles[scan[t]] = make_int2(I, J);
-}
-
-__global__ void k_tile_count(long long npairs, const int *__restrict__ flag, const int *__restrict__ scan,
-                             int *__restrict__ count)
-{
-    if (blockIdx.x == 0 && threadIdx.x == 0) count[0] = npairs ? scan[npairs - 1] + flag[npairs - 1] : 0;
 }
 
 int build_neighbors(pmq_ctx *c, double cutoff)
@@ -365,8 +362,7 @@ int build_neighbors(pmq_ctx *c, double cutoff)
     PMQ_HIP(rocprim::exclusive_scan(c->scratch_sort.p, bytes, c->tileflag.p, c->tilescan.p, 0, (size_t)npairs,
                                     rocprim::plus<int>(), st));
     hipLaunchKernelGGL(k_tile_write, dim3(gb), dim3(tb), 0, st, nblk, npairs, c->tileflag.p, c->tilescan.p,
-                       c->tiles.p);
-    hipLaunchKernelGGL(k_tile_count, dim3(1), dim3(1), 0, st, npairs, c->tileflag.p, c->tilescan.p, c->ntiles_d.p);
+                       c->tiles.p, c->ntiles_d.p);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }

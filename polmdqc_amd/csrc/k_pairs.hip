@@ -173,11 +173,6 @@ __global__ __launch_bounds__(64) void k_block_adj(int nblk, int cap, const int *
     if (lane == 0) adjn[I] = count;
 }
 
-__global__ void k_status_reset(RunStatus *st)
-{
-    if (threadIdx.x == 0) *st = RunStatus{0, 0, 0, 0};
-}
-
 // the tile count the tile kernels loop over, and the overflow check
 __global__ void k_tile_clamp(const int *ntiles, int cap, int *eff, RunStatus *st)
 {
@@ -186,14 +181,6 @@ __global__ void k_tile_clamp(const int *ntiles, int cap, int *eff, RunStatus *st
         *eff = min(n, cap);
         st->ntiles = n;
         if (n > cap) st->overflow |= 1;
-    }
-}
-
-__global__ void k_nnz_check(const int *total, size_t cap, RunStatus *st)
-{
-    if (threadIdx.x == 0) {
-        st->nnz = *total;
-        if ((size_t)*total > cap) st->overflow |= 2;
     }
 }
 
@@ -293,6 +280,7 @@ struct FillArgs {
     const unsigned long long *nmask;
     const int *nsoff, *rownear;
     TEnt<V> *ent_s;
+    RunStatus *st;  // the entry total (rowptr[npad]) against cap: overflow bit 2
 };
 
 template <class V, class T>
@@ -321,6 +309,11 @@ __global__ __launch_bounds__(64) void k_pair_fill(FillArgs<V> a)
     __shared__ unsigned long long bmask[2][64], bnmask[2][64];
     __shared__ int bbase[2][64], bnbase[2][64];
     __shared__ unsigned short plist[64 * 64];
+    if (blockIdx.x == 0 && lane == 0) {
+        const int total = a.rowptr[np];
+        a.st->nnz = total;
+        if ((size_t)total > a.cap) a.st->overflow |= 2;
+    }
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 IJ = a.tiles[t];
         const bool diag = IJ.x == IJ.y;
@@ -652,6 +645,7 @@ static int pair_fill(pmq_ctx *c, int tgrid)
     fa.nmask = c->prec_on ? c->nmask.p : nullptr;
     fa.nsoff = c->nsoff.p; fa.rownear = c->rownear.p;
     fa.ent_s = c->prec_on ? (TEnt<V> *)c->tval_s.p : nullptr;
+    fa.st = c->status.p;
     {
         ProfScope ps_(c, "pol_pair_fill");
         const bool cpd = c->prm.pentype != PMQ_PEN_NONE;
@@ -677,13 +671,6 @@ static int pair_fill(pmq_ctx *c, int tgrid)
         fo.ent_s = c->prec_on ? (TEnt<V> *)c->tval_s.p : nullptr;
         hipLaunchKernelGGL((k_excl_fold<V>), dim3((nexcl + 63) / 64), dim3(64), 0, st, fo);
     }
-    PMQ_HIP(hipGetLastError());
-    return PMQ_OK;
-}
-
-int status_reset(pmq_ctx *c)
-{
-    hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(1), 0, c->stream, c->status.p);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }
@@ -804,7 +791,6 @@ int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe)
     }
     // the preconditioner's tensors sit at the entries of the near pairs
     if (c->prec_on) PMQ_HIP(c->tval_s.alloc(2 * c->tcap));
-    hipLaunchKernelGGL(k_nnz_check, dim3(1), dim3(1), 0, st, c->rowptr.p + np, c->tcap, c->status.p);
 
     c->tmixed = mixed;
     if (mixed) return pair_fill<float, float>(c, c->tgrid);

@@ -626,7 +626,6 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
     PMQ_HIP(hipSetDevice(c->device));
     const int n = c->n;
     int rc;
-    if ((rc = status_reset(c))) return rc;
     const bool grad = mode == PMQ_GRADIENT || mode == PMQ_VIRIAL;
     {
         // the evaluation's accumulators, cleared by one launch
@@ -635,6 +634,8 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
         int k = 0;
         auto add = [&](double *p, size_t cnt) { zp[k] = p; zn[k] = cnt; k++; };
         add((double *)c->totals.p, sizeof(Totals) / sizeof(double));
+        static_assert(sizeof(RunStatus) % sizeof(double) == 0, "RunStatus is cleared as doubles");
+        add((double *)c->status.p, sizeof(RunStatus) / sizeof(double)); // all-zero bits: every field 0
         if (grad) {
             add(c->dem.p, 3 * (size_t)n);
             add(c->te.p, 3 * (size_t)n);

@@ -389,7 +389,6 @@ int polar_scatter(pmq_ctx *c, double *ud_out, double *up_out);
 int launch_chgflx_grad(pmq_ctx *c, int mode);
 int launch_zero(pmq_ctx *c, double *p, size_t n);
 int tile_structure(pmq_ctx *c, double off2, bool safe);
-int status_reset(pmq_ctx *c);
 int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe);
 int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fixed);
 int tile_scalars(pmq_ctx *c, int which, bool full);
