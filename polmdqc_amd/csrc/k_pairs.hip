@@ -44,7 +44,10 @@ namespace pmq {
 
 struct MaskArgs {
     const int2 *tiles;
-    const int *ntiles;
+    const int *ntiles;         // tiles listed by the neighbor search
+    int cap;                   // tiles the buffers hold: the count used is min(ntiles, cap)
+    int *ntiles_eff;           // that count, for the later tile kernels
+    RunStatus *st;             // the listed count, and overflow bit 1 when it exceeds cap
     const double *sx, *sy, *sz;
     const int *sflag;
     DevBox box;
@@ -60,7 +63,13 @@ struct MaskArgs {
 __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
 {
     const int lane = threadIdx.x;
-    const int ntiles = *a.ntiles;
+    const int nlisted = *a.ntiles;
+    const int ntiles = min(nlisted, a.cap);
+    if (blockIdx.x == 0 && lane == 0) {
+        *a.ntiles_eff = ntiles;
+        a.st->ntiles = nlisted;
+        if (nlisted > a.cap) a.st->overflow |= 1;
+    }
     __shared__ double jx[64], jy[64], jz[64];
     __shared__ int jflag[64];
     const double L[3] = {a.box.xbox, a.box.ybox, a.box.zbox};
@@ -171,17 +180,6 @@ __global__ __launch_bounds__(64) void k_block_adj(int nblk, int cap, const int *
         count += __popcll(b);
     }
     if (lane == 0) adjn[I] = count;
-}
-
-// the tile count the tile kernels loop over, and the overflow check
-__global__ void k_tile_clamp(const int *ntiles, int cap, int *eff, RunStatus *st)
-{
-    if (threadIdx.x == 0) {
-        const int n = *ntiles;
-        *eff = min(n, cap);
-        st->ntiles = n;
-        if (n > cap) st->overflow |= 1;
-    }
 }
 
 // the mask of lane `lane` of block I for tile t, in the orientation of I
@@ -737,11 +735,10 @@ int tile_structure(pmq_ctx *c, double off2, bool safe)
     PMQ_HIP(c->tpart.alloc(nt * 2 * PMQ_TILE_NV * 64));
     PMQ_HIP(c->tscal.alloc(nt * PMQ_TILE_NS));
     c->tgrid = (int)std::max<size_t>(1, std::min<size_t>(nt, 65536));
-    hipLaunchKernelGGL(k_tile_clamp, dim3(1), dim3(1), 0, st, c->ntiles_d.p, c->tile_cap, c->ntiles_eff.p,
-                       c->status.p);
 
     MaskArgs ma;
-    ma.tiles = c->tiles.p; ma.ntiles = c->ntiles_eff.p;
+    ma.tiles = c->tiles.p; ma.ntiles = c->ntiles_d.p; ma.cap = c->tile_cap; ma.ntiles_eff = c->ntiles_eff.p;
+    ma.st = c->status.p;
     ma.sx = c->sx.p; ma.sy = c->sy.p; ma.sz = c->sz.p; ma.sflag = c->sflag.p;
     ma.box = c->box; ma.off2 = off2; ma.mask = c->tmask.p; ma.mmask = c->mmask.p;
     ma.us2 = c->prec_on ? std::min(c->prm.usolvcut * c->prm.usolvcut, off2) : -1.0;
@@ -890,18 +887,22 @@ __global__ __launch_bounds__(64 * RED_W) void k_tile_reduce(TileReduceArgs a)
 // fixed-order sums of the per-tile scalars into the totals: scalar 0 into
 // energy component `which` (0 em, 1 ep, 2 er, 3 edsp); with `full` also the
 // virial (1..9), the inter-molecular energy (10) and the pair count (11)
-__global__ __launch_bounds__(256) void k_tile_scalars(const int *ntiles_p, int which, int full,
-                                                      const double *__restrict__ tscal, Totals *tot)
+constexpr int SCAL_TB = 1024;
+
+__global__ __launch_bounds__(SCAL_TB) void k_tile_scalars(const int *ntiles_p, int which, int full,
+                                                          const double *__restrict__ tscal, Totals *tot)
 {
-    __shared__ double red[256];
+    __shared__ double red[SCAL_TB];
     const int ntiles = *ntiles_p;
     const int ns = full ? PMQ_TILE_NS : 1;
     for (int k = 0; k < ns; k++) {
+        // each thread's tiles in order (the unrolled loads go out together)
         double v = 0.0;
-        for (int t = threadIdx.x; t < ntiles; t += 256) v += tscal[(size_t)t * PMQ_TILE_NS + k];
+#pragma unroll 8
+        for (int t = threadIdx.x; t < ntiles; t += SCAL_TB) v += tscal[(size_t)t * PMQ_TILE_NS + k];
         red[threadIdx.x] = v;
         __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
+        for (int w = SCAL_TB / 2; w > 0; w >>= 1) {
             if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
             __syncthreads();
         }
@@ -939,7 +940,7 @@ int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fi
 
 int tile_scalars(pmq_ctx *c, int which, bool full)
 {
-    hipLaunchKernelGGL(k_tile_scalars, dim3(1), dim3(256), 0, c->stream, c->ntiles_eff.p, which, full ? 1 : 0,
+    hipLaunchKernelGGL(k_tile_scalars, dim3(1), dim3(SCAL_TB), 0, c->stream, c->ntiles_eff.p, which, full ? 1 : 0,
                        c->tscal.p, c->totals.p);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
