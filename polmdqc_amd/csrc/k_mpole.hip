@@ -672,8 +672,10 @@ __device__ __forceinline__ void normalize(double *v)
     v[0] /= l; v[1] /= l; v[2] /= l;
 }
 
-__global__ __launch_bounds__(128) void k_torque(TorqueArgs a)
+// blockIdx.y selects one of two torque sets (the same kernel for both)
+__global__ __launch_bounds__(128) void k_torque(TorqueArgs a0, TorqueArgs a1)
 {
+    const TorqueArgs a = blockIdx.y == 0 ? a0 : a1;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double W[6] = {0, 0, 0, 0, 0, 0};
     if (i < a.n && a.polaxe[i] != PMQ_FRAME_NONE) {
@@ -810,19 +812,39 @@ __global__ __launch_bounds__(128) void k_torque(TorqueArgs a)
     }
 }
 
-int launch_torque(pmq_ctx *c, int mode, const double *te, double *de, bool mixed)
+static TorqueArgs torque_args(pmq_ctx *c, int mode, const double *te, double *de, long long *deq)
 {
     TorqueArgs a;
     a.n = c->n;
     a.x = c->x.p; a.y = c->y.p; a.z = c->z.p; a.te = te;
     a.polaxe = c->polaxe.p; a.zaxis = c->zaxis.p; a.xaxis = c->xaxis.p; a.yaxis = c->yaxis.p;
     a.de = de;
-    a.deq = mixed ? c->fx_grad.p : nullptr;
+    a.deq = deq;
     a.tot = c->totals.p;
     a.vir = mode == PMQ_VIRIAL;
-    hipLaunchKernelGGL(k_torque, dim3((c->n + 127) / 128), dim3(128), 0, c->stream, a);
+    return a;
+}
+
+int launch_torque(pmq_ctx *c, int mode, const double *te, double *de, bool mixed)
+{
+    const TorqueArgs a = torque_args(c, mode, te, de, mixed ? c->fx_grad.p : nullptr);
+    hipLaunchKernelGGL(k_torque, dim3((c->n + 127) / 128), dim3(128), 0, c->stream, a, a);
     PMQ_HIP(hipGetLastError());
     if (mixed) return launch_fx_flush(c, c->fx_grad.p, de, 3 * (size_t)c->n, PMQ_FX_GRAD);
+    return PMQ_OK;
+}
+
+// two torque sets in one launch (mixed: two fixed-point halves of fx_grad,
+// flushed together)
+int launch_torque_pair(pmq_ctx *c, int mode0, const double *te0, double *de0, int mode1, const double *te1,
+                       double *de1, bool mixed)
+{
+    const size_t n3 = 3 * (size_t)c->n;
+    const TorqueArgs a0 = torque_args(c, mode0, te0, de0, mixed ? c->fx_grad.p : nullptr);
+    const TorqueArgs a1 = torque_args(c, mode1, te1, de1, mixed ? c->fx_grad.p + n3 : nullptr);
+    hipLaunchKernelGGL(k_torque, dim3((c->n + 127) / 128, 2), dim3(128), 0, c->stream, a0, a1);
+    PMQ_HIP(hipGetLastError());
+    if (mixed) return launch_fx_flush_pair(c, c->fx_grad.p, de0, de1, n3, PMQ_FX_GRAD);
     return PMQ_OK;
 }
 

@@ -702,14 +702,20 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
                     if ((rc = launch_fx_flush_pair(c, c->fx_grad.p, c->dep.p, c->tep.p, 3 * (size_t)n, PMQ_FX_GRAD)))
                         return rc;
                 }
-                if ((rc = launch_torque(c, PMQ_GRADIENT, c->tep.p, c->dep.p, mixed))) return rc;
             }
             if ((rc = polar_scatter(c, c->uind_out.p, c->uind_out.p + 3 * (size_t)n))) return rc;
         }
-        if (grad && do_mpole) {
+        // the torques of both terms in one launch (the polarization one adds
+        // no virial here)
+        if (grad && do_mpole && do_polar) {
+            if ((rc = launch_torque_pair(c, mode, c->te.p, c->dem.p, PMQ_GRADIENT, c->tep.p, c->dep.p, mixed)))
+                return rc;
+        } else if (grad && do_polar) {
+            if ((rc = launch_torque(c, PMQ_GRADIENT, c->tep.p, c->dep.p, mixed))) return rc;
+        } else if (grad && do_mpole) {
             if ((rc = launch_torque(c, mode, c->te.p, c->dem.p, mixed))) return rc;
-            if (cf && (rc = launch_chgflx_grad(c, mode))) return rc;
         }
+        if (grad && do_mpole && cf && (rc = launch_chgflx_grad(c, mode))) return rc;
     }
     c->last_mode = mode;
     c->last_terms = terms;

@@ -369,6 +369,8 @@ int launch_mpole_direct(pmq_ctx *c, int mode, double off2, bool mixed);
 int launch_fx_flush(pmq_ctx *c, long long *q, double *d, size_t n, double scale);
 int launch_fx_flush_pair(pmq_ctx *c, long long *q, double *d0, double *d1, size_t n, double scale);
 int launch_torque(pmq_ctx *c, int mode, const double *te, double *de, bool mixed);
+int launch_torque_pair(pmq_ctx *c, int mode0, const double *te0, double *de0, int mode1, const double *te1,
+                       double *de1, bool mixed);
 constexpr int PMQ_ZERO_MAX = 12;
 int launch_zero_many(pmq_ctx *c, int k, double *const *p, const size_t *n);
 int pme_setup(pmq_ctx *c);
