@@ -712,23 +712,24 @@ __device__ __forceinline__ void block_add1(double v, double *dst)
 }
 
 // E_pol = -(f/2) sum mu_d . E^p
-__global__ __launch_bounds__(256) void k_pol_energy(int npad, double f, const double *ud, const double *Ep, Totals *tot)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const double v = t < 3 * npad ? -0.5 * f * ud[t] * Ep[t] : 0.0;
-    block_add1(v, &tot->ep);
-}
-
-// induced dipoles to the caller's order (AoS)
-__global__ void k_pol_scatter(int n, int npad, const int *perm, const double *u, double *ud_out, double *up_out)
+// the induced dipoles to the caller's order (AoS), and the polarization
+// energy -(f/2) mu_d . E^p summed per atom, then per block
+__global__ __launch_bounds__(256) void k_pol_finish(int n, int npad, const int *perm, const double *u, double *ud_out,
+                                                   double *up_out, double f, const double *Ep, Totals *tot)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const int i = perm[s];
-    for (int c = 0; c < 3; c++) {
-        ud_out[3 * i + c] = u[c * npad + s];
-        up_out[3 * i + c] = u[(3 + c) * npad + s];
+    double v = 0.0;
+    if (s < n) {
+        const int i = perm[s];
+        for (int c = 0; c < 3; c++) {
+            const double ud = u[c * npad + s];
+            ud_out[3 * i + c] = ud;
+            up_out[3 * i + c] = u[(3 + c) * npad + s];
+            v += ud * Ep[c * npad + s];
+        }
+        v *= -0.5 * f;
     }
+    block_add1(v, &tot->ep);
 }
 
 // ---------------------------------------------------------------------------
@@ -882,15 +883,6 @@ int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
     return PMQ_OK;
 }
 
-int polar_energy(pmq_ctx *c)
-{
-    const int np = c->npad;
-    hipLaunchKernelGGL(k_pol_energy, dim3((3 * np + 255) / 256), dim3(256), 0, c->stream, np,
-                       c->prm.electric / c->prm.dielec, c->uind.p, c->pfield.p + 3 * (size_t)np, c->totals.p);
-    PMQ_HIP(hipGetLastError());
-    return PMQ_OK;
-}
-
 int polar_gradient(pmq_ctx *c, double off2, bool mixed)
 {
     PolArgs a = pol_args(c, off2);
@@ -933,10 +925,12 @@ int polar_gradient(pmq_ctx *c, double off2, bool mixed)
     return PMQ_OK;
 }
 
-int polar_scatter(pmq_ctx *c, double *ud_out, double *up_out)
+int polar_finish(pmq_ctx *c, double *ud_out, double *up_out)
 {
-    hipLaunchKernelGGL(k_pol_scatter, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n, c->npad, c->perm.p,
-                       c->uind.p, ud_out, up_out);
+    const int np = c->npad;
+    hipLaunchKernelGGL(k_pol_finish, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n, np, c->perm.p,
+                       c->uind.p, ud_out, up_out, c->prm.electric / c->prm.dielec, c->pfield.p + 3 * (size_t)np,
+                       c->totals.p);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }

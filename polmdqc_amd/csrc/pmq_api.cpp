@@ -695,7 +695,6 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
         }
         if (do_polar) {
             if ((rc = polar_solve(c, safe, &c->pcg_iters))) return rc;
-            if ((rc = polar_energy(c))) return rc;
             if (grad) {
                 if ((rc = polar_gradient(c, off2, mixed))) return rc;
                 if (mixed) {
@@ -703,7 +702,7 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
                         return rc;
                 }
             }
-            if ((rc = polar_scatter(c, c->uind_out.p, c->uind_out.p + 3 * (size_t)n))) return rc;
+            if ((rc = polar_finish(c, c->uind_out.p, c->uind_out.p + 3 * (size_t)n))) return rc;
         }
         // the torques of both terms in one launch (the polarization one adds
         // no virial here)

@@ -385,9 +385,8 @@ void pme_release(pmq_ctx *c);
 int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar);
 int polar_prepare(pmq_ctx *c, double off2, bool mixed, bool safe);
 int polar_solve(pmq_ctx *c, bool safe, int *iters_out);
-int polar_energy(pmq_ctx *c);
 int polar_gradient(pmq_ctx *c, double off2, bool mixed);
-int polar_scatter(pmq_ctx *c, double *ud_out, double *up_out);
+int polar_finish(pmq_ctx *c, double *ud_out, double *up_out);
 int launch_chgflx_grad(pmq_ctx *c, int mode);
 int launch_zero(pmq_ctx *c, double *p, size_t n);
 int tile_structure(pmq_ctx *c, double off2, bool safe);
