@@ -115,11 +115,31 @@ def cpu_baseline(s) -> dict:
                       f"{o['iters']} PCG iterations, {dt:.1f} s)"}
 
 
+PIN_FFT = None  # (FP64-form engine, mixed-form engine) from --pin-fft, or None (timed choice)
+
+
+def pin_fft_values(spec: str):
+    """`--pin-fft FP64/GRIDS-ENGINE`, e.g. `library/fp32-direct`: the PME
+    transform engine of the FP64 form, then the mixed form's grid precision
+    and engine (pmq_set_fft values).  For profiled runs, whose per-launch
+    profiler cost would otherwise steer the timed choice away from the one an
+    unprofiled run makes."""
+    from polmdqc_amd.engine import FFT_AUTO, FFT_DIRECT, FFT_GRIDS_FP32, FFT_GRIDS_FP64, FFT_LIBRARY
+
+    eng = {"auto": FFT_AUTO, "library": FFT_LIBRARY, "direct": FFT_DIRECT}
+    f64, mixed = spec.split("/")
+    grids, engm = mixed.split("-")
+    return eng[f64], eng[engm] | {"fp32": FFT_GRIDS_FP32, "fp64": FFT_GRIDS_FP64}[grids]
+
+
 def timed(eng, L, terms, prec, steps, warmup):
     """ms per evaluation over `steps` evaluations (no instrumentation), then
     the per-family kernel times from HIP events over a second pass of the
     same evaluations."""
-    from polmdqc_amd.engine import profile_enable, profile_read
+    from polmdqc_amd.engine import profile_enable, profile_read, set_fft
+
+    if PIN_FFT is not None:
+        set_fft(eng, PIN_FFT[0] if prec == L.PLAIN_FP64 else PIN_FFT[1])
 
     for _ in range(warmup):
         eng.compute(L.GRADIENT, terms, prec)
@@ -217,6 +237,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pin-fft", default=None, metavar="FP64/GRIDS-ENGINE",
+                    help="fix the PME transform choice (e.g. library/fp32-direct) instead of timing it")
     ap.add_argument("--parts", default="amoeba,clusters,hippo",
                     help="workloads to run (profiling runs take one at a time); the JSON line needs amoeba")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
@@ -229,6 +251,10 @@ def main() -> int:
 
     from polmdqc_amd import _lib as L
     from polmdqc_amd.engine import Engine, fft_choice
+
+    global PIN_FFT
+    if args.pin_fft:
+        PIN_FFT = pin_fft_values(args.pin_fft)
 
     parts = set(args.parts.split(","))
     if "amoeba" not in parts:

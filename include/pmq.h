@@ -259,13 +259,17 @@ PMQ_API int pmq_profile_read(pmq_ctx *ctx, const char *name, double *total_ms, l
  * grids whose planes fit in LDS) and keeps the faster, separately for FP64
  * and FP32 grids (a profiler that adds a cost per launch biases that
  * choice); PMQ_FFT_LIBRARY always takes rocFFT; PMQ_FFT_DIRECT takes the
- * direct transforms wherever they fit.  pmq_fft_choice reports the choice
- * made for the current grid: direct transforms for FP64 grids, FP32 grids
- * in the mixed form, direct transforms for FP32 grids (each 0/1, or -1 while
- * not yet chosen). */
+ * direct transforms wherever they fit.  One of PMQ_FFT_GRIDS_FP32 and
+ * PMQ_FFT_GRIDS_FP64 may be or'ed into the engine to fix the grid precision
+ * of the mixed form (timed against each other otherwise).  pmq_fft_choice
+ * reports the choice made for the current grid: direct transforms for FP64
+ * grids, FP32 grids in the mixed form, direct transforms for FP32 grids
+ * (each 0/1, or -1 while not yet chosen). */
 #define PMQ_FFT_AUTO 0
 #define PMQ_FFT_LIBRARY 1
 #define PMQ_FFT_DIRECT 2
+#define PMQ_FFT_GRIDS_FP32 16
+#define PMQ_FFT_GRIDS_FP64 32
 PMQ_API int pmq_set_fft(pmq_ctx *ctx, int engine);
 PMQ_API int pmq_fft_choice(pmq_ctx *ctx, int *fp64_direct, int *fp32_grids, int *fp32_direct);
 

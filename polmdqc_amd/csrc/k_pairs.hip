@@ -60,6 +60,9 @@ struct MaskArgs {
     unsigned long long *xmask; // 128 per tile: copies of the in-cutoff and active rows (scaled pairs cleared later)
 };
 
+// BK 0: no box or a rectangular one (one image per tile where it fits, else
+// image()'s per-axis wrapping); BK 1: other cells (image() per pair)
+template <int BK>
 __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
 {
     const int lane = threadIdx.x;
@@ -84,8 +87,8 @@ __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
         // the same to the bit.  Other boxes, and tiles that straddle a window
         // edge, take image() per pair.
         double sh[3] = {0.0, 0.0, 0.0};
-        bool one = a.box.kind == PMQ_BOX_NONE || a.box.kind == PMQ_BOX_ORTHO;
-        if (a.box.kind == PMQ_BOX_ORTHO) {
+        bool one = BK == 0;
+        if (BK == 0 && a.box.kind == PMQ_BOX_ORTHO) {
             const double *bi = a.blk + PMQ_BLK_STRIDE * IJ.x, *bj = a.blk + PMQ_BLK_STRIDE * IJ.y;
 #pragma unroll
             for (int q = 0; q < 3; q++) {
@@ -121,7 +124,14 @@ __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
                     const int fj = jflag[jj];
                     if (!(fj & 1)) continue;
                     double xr = jx[jj] - xi, yr = jy[jj] - yi, zr = jz[jj] - zi;
-                    image<double>(a.box, xr, yr, zr);
+                    if constexpr (BK == 0) {
+                        // image() of a rectangular box
+                        image_wrap<double>(xr, L[0]);
+                        image_wrap<double>(yr, L[1]);
+                        image_wrap<double>(zr, L[2]);
+                    } else {
+                        image<double>(a.box, xr, yr, zr);
+                    }
                     const double r2 = xr * xr + yr * yr + zr * zr;
                     if (r2 <= a.off2) {
                         m |= 1ull << jj;
@@ -745,7 +755,10 @@ int tile_structure(pmq_ctx *c, double off2, bool safe)
     ma.nmask = c->prec_on ? c->nmask.p : nullptr;
     ma.blk = c->blk.p;
     ma.xmask = c->xmask.p;
-    hipLaunchKernelGGL(k_tile_masks, dim3(c->tgrid), dim3(64), 0, st, ma);
+    if (c->box.kind == PMQ_BOX_NONE || c->box.kind == PMQ_BOX_ORTHO)
+        hipLaunchKernelGGL(k_tile_masks<0>, dim3(c->tgrid), dim3(64), 0, st, ma);
+    else
+        hipLaunchKernelGGL(k_tile_masks<1>, dim3(c->tgrid), dim3(64), 0, st, ma);
     hipLaunchKernelGGL(k_block_adj, dim3(nblk), dim3(64), 0, st, nblk, c->tile_cap, c->tileflag.p, c->tilescan.p,
                        c->adj.p, c->adjn.p);
     const int nexcl = (int)c->h_excl.size();

@@ -1151,7 +1151,7 @@ static int choose_fft(pmq_ctx *c)
         if (!cand[q].fp32 && (cand[q].best < b64 || b64 == 1e30f)) { b64 = cand[q].best; c->pme_own64 = cand[q].own; }
         if (cand[q].fp32 && (cand[q].best < b32 || b32 == 1e30f)) { b32 = cand[q].best; c->pme_own32 = cand[q].own; }
     }
-    if (fp32_form) c->pme_fft32 = b32 <= b64 ? 1 : 0;
+    if (fp32_form) c->pme_fft32 = c->fft_grids == 1 ? 1 : c->fft_grids == 2 ? 0 : (b32 <= b64 ? 1 : 0);
     return PMQ_OK;
 }
 

@@ -97,11 +97,15 @@ int pmq_profile_read(pmq_ctx *c, const char *name, double *total_ms, long long *
 int pmq_set_fft(pmq_ctx *c, int engine)
 {
     CHECK_CTX(c);
-    if (engine < PMQ_FFT_AUTO || engine > PMQ_FFT_DIRECT) {
-        c->err = "pmq_set_fft: engine must be PMQ_FFT_AUTO, PMQ_FFT_LIBRARY or PMQ_FFT_DIRECT";
+    const int grids = engine & (PMQ_FFT_GRIDS_FP32 | PMQ_FFT_GRIDS_FP64);
+    engine &= ~(PMQ_FFT_GRIDS_FP32 | PMQ_FFT_GRIDS_FP64);
+    if (engine < PMQ_FFT_AUTO || engine > PMQ_FFT_DIRECT || grids == (PMQ_FFT_GRIDS_FP32 | PMQ_FFT_GRIDS_FP64)) {
+        c->err = "pmq_set_fft: engine must be PMQ_FFT_AUTO, PMQ_FFT_LIBRARY or PMQ_FFT_DIRECT, with at most one of "
+                 "PMQ_FFT_GRIDS_FP32 and PMQ_FFT_GRIDS_FP64";
         return PMQ_ERR_ARG;
     }
     c->fft_engine = engine;
+    c->fft_grids = grids == PMQ_FFT_GRIDS_FP32 ? 1 : grids == PMQ_FFT_GRIDS_FP64 ? 2 : 0;
     c->pme_own64 = c->pme_own32 = -1;
     c->pme_fft32 = -1;
     return PMQ_OK;

@@ -332,6 +332,7 @@ struct pmq_ctx {
     bool pme_fp32 = false;   // FP32 weights and sums in spreading and gathering (mixed form)
     int pme_fft32 = -1;      // mixed form: FP32 grids and FFTs (1) or FP64 (0), timed once per grid (-1)
     int fft_engine = PMQ_FFT_AUTO;
+    int fft_grids = 0;                           // mixed-form grid precision: 0 timed, 1 FP32, 2 FP64
     int gather_lanes = PMQ_GATHER_AUTO;
     int pme_own64 = -1, pme_own32 = -1; // FP64 / FP32 grids: the direct LDS transforms (1) or rocFFT (0), timed once per grid (-1)
     pmq::DBuf<double> fft_tw;    // twiddles of the direct transforms, axes 0..2, FP64 and FP32

@@ -271,12 +271,14 @@ KERNEL_FAMILIES = ("neighbors", "mpole_tiles", "rep_tiles", "disp_tiles", "pol_f
 
 
 FFT_AUTO, FFT_LIBRARY, FFT_DIRECT = 0, 1, 2
+FFT_GRIDS_FP32, FFT_GRIDS_FP64 = 16, 32
 
 
 def set_fft(eng: Engine, engine: int = FFT_AUTO):
     """PME transform engine: a timed choice (FFT_AUTO, the default), rocFFT
     (FFT_LIBRARY), or the direct LDS transforms where the grid fits them
-    (FFT_DIRECT)."""
+    (FFT_DIRECT); FFT_GRIDS_FP32 or FFT_GRIDS_FP64 or'ed in fixes the mixed
+    form's grid precision."""
     L = eng._L
     L.pmq_set_fft.argtypes = [C.c_void_p, C.c_int]
     L.pmq_set_fft.restype = C.c_int

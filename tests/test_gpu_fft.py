@@ -149,3 +149,26 @@ def test_gather_layouts_match_oracle(engine, mode):
     od = oracle.hippo_terms(sd, want_grad=True)
     assert abs(d["edsp"] - od["edsp"]) <= REL * abs(od["edsp"])
     assert rel(d["dedsp"], od["dedsp"]) <= REL
+
+
+@pytest.mark.parametrize("grids", ["fp32", "fp64"])
+def test_pinned_grid_precision(engine, grids):
+    """FFT_GRIDS_FP32 / FFT_GRIDS_FP64 or'ed into the engine fix the mixed
+    form's grid precision; either stays within the mixed form's bar."""
+    from polmdqc_amd import _lib as L
+    from polmdqc_amd.engine import FFT_DIRECT, FFT_GRIDS_FP32, FFT_GRIDS_FP64, PmqError, fft_choice, set_fft
+
+    print(f"[gpu] pinned grids {grids}", flush=True)
+    s = box216()
+    ref = run(engine, s, FFT_DIRECT, L.PLAIN_FP64)
+    engine.set_system(s)
+    set_fft(engine, FFT_DIRECT | (FFT_GRIDS_FP32 if grids == "fp32" else FFT_GRIDS_FP64))
+    try:
+        m = engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR, L.MIXED)
+        assert fft_choice(engine)["mixed_grids"] == grids
+    finally:
+        set_fft(engine)
+    for k in ("em", "ep"):
+        assert abs(m[k] - ref[k]) <= 1e-5 * abs(ref[k]), (k, m[k], ref[k])
+    with pytest.raises(PmqError):
+        set_fft(engine, FFT_DIRECT | FFT_GRIDS_FP32 | FFT_GRIDS_FP64)
