@@ -13,25 +13,31 @@
 // dipole-dipole field tensor, u-scaled for excluded pairs) to a new pair of
 // dipole vectors in every PCG iteration while the geometry stays fixed.  So
 // T is built once per evaluation and every iteration is a sparse
-// matrix-vector product that reads it back: one record per stored entry (6
-// symmetric tensor components and the column), each pair stored in both rows.
+// matrix-vector product that reads it back.  Each pair tensor is
+// R2 r r^T - R1 I, so a stored entry is its two scalars and its column (the
+// product rebuilds r from the sorted positions); each pair is stored in both
+// rows.
 // Every row is owned by one group of lanes, so the product needs no atomics
 // and sums each row in a fixed order (bit-reproducible run to run).
 //
 // Build, all per evaluation on the library's stream:
 //   k_tile_masks   per (I,J) tile: 64 row masks (bit j of lane i: pair i,j
-//                  within the cutoff) and the transposed column masks
+//                  within the cutoff) and the transposed column masks, the
+//                  scaled-pair mask rows, and the tile count clamped to the
+//                  buffers' capacity
 //   k_block_adj    per block: the tiles it takes part in, ordered by the
 //                  other block's index
 //   k_row_counts   per block: each lane's entry offset within its row for
 //                  every tile of its list, and the row length
 //   (exclusive scan of row lengths -> row pointers)
-//   k_pair_fill    per tile: the tensor of every masked pair, written to the
-//                  two rows at the offsets above (no atomics)
+//   k_pair_fill    per tile: the tensor scalars of every masked pair, written
+//                  to the two rows at the offsets above (no atomics); block 0
+//                  also checks the entry total against the capacity
 //   k_excl_fold    adds (u - 1) lambda rr corrections of excluded pairs to
 //                  their two entries
 // Apply:
-//   k_pol_spmv<G>  G lanes per row, both dipole sets at once.
+//   k_pol_spmv<G,BK>  G lanes per row, both dipole sets at once (BK: the box
+//                     form, for rebuilding r).
 #include <cstring>
 
 #include <rocprim/device/device_scan.hpp>
