@@ -54,13 +54,15 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 def hbm_bytes_per_launch(family: str, n: int, npad: int, nnz: int, grid: int, w: int, gw: int = 8):
     """Algorithmic HBM bytes of one launch of the memory-bound kernel families:
     every array the launch needs read once and every output written once.
-    pol_mutual_spmv: column index (4 B) and 6 tensor components (w B each)
-    per stored entry, the row pointers, both dipole sets read and both field
-    sets written (6 x 8 B per atom each).  pme_spread (induced-dipole form):
+    pol_mutual_spmv: column index (4 B) and the two tensor scalars (w B each)
+    per stored entry, the row pointers, the sorted positions (3 x 8 B per
+    atom, from which each entry's pair vector is rebuilt), both dipole sets
+    read and both field sets written (6 x 8 B per atom each).  pme_spread
+    (induced-dipole form):
     coordinates and both dipole sets read (9 x 8 B per atom), two grids
     written (gw B per point: 4 for FP32 grids, else 8)."""
     if family == "pol_mutual_spmv":
-        return nnz * (4 + 6 * w) + (npad + 1) * 4 + 2 * 6 * 8 * npad
+        return nnz * (4 + 2 * w) + (npad + 1) * 4 + 3 * 8 * npad + 2 * 6 * 8 * npad
     if family == "pme_spread":
         return n * 9 * 8 + 2 * grid * gw
     return None
