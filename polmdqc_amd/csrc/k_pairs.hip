@@ -490,14 +490,15 @@ __device__ __forceinline__ void entry_apply(double r1, double r2, double xr, dou
 
 // this lane's share (entries e0 + g, e0 + g + G, ... < e1) of T u for both
 // dipole sets of the row whose atom sits at (xi, yi, zi), U entries at a
-// time: their columns and scalars are loaded together, then the positions
-// and dipoles of those columns together, so a lane waits on two loads per U
-// entries rather than two per entry.  Entries past e1 are clamped to e1 - 1
-// and weighted by zero.
+// time: their columns and scalars are loaded together, then the packed
+// records of those columns (position and both dipoles, 80 bytes as five
+// 16-byte loads) together, so a lane waits on two loads per U entries and
+// each entry touches one record rather than nine arrays.  Entries past e1
+// are clamped to e1 - 1 and weighted by zero.
 template <int G, int U, int BK, class V>
 __device__ __forceinline__ void row_apply(const int *cols, const TEnt<V> *ent, int e0, int e1, int g,
-                                          const RowGeom &geo, double xi, double yi, double zi, const double *u,
-                                          int np, double acc[6])
+                                          const RowGeom &geo, double xi, double yi, double zi, const double *pk,
+                                          double acc[6])
 {
     for (int base = e0 + g; base < e1; base += G * U) {
         int k[U];
@@ -510,18 +511,19 @@ __device__ __forceinline__ void row_apply(const int *cols, const TEnt<V> *ent, i
             k[q] = cols[e];
             r[q] = ent[e];
         }
-        double p[U][3], d[U][6];
+        double2 rec[U][5];
 #pragma unroll
         for (int q = 0; q < U; q++) {
-            p[q][0] = geo.sx[k[q]]; p[q][1] = geo.sy[k[q]]; p[q][2] = geo.sz[k[q]];
+            const double2 *cr = reinterpret_cast<const double2 *>(pk + (size_t)PMQ_PK * k[q]);
 #pragma unroll
-            for (int c = 0; c < 6; c++) d[q][c] = u[(size_t)c * np + k[q]];
+            for (int h = 0; h < 5; h++) rec[q][h] = cr[h];
         }
 #pragma unroll
         for (int q = 0; q < U; q++) {
-            double xr = p[q][0] - xi, yr = p[q][1] - yi, zr = p[q][2] - zi;
+            double xr = rec[q][0].x - xi, yr = rec[q][0].y - yi, zr = rec[q][1].x - zi;
             row_image<BK>(geo, xr, yr, zr);
-            entry_apply(wgt[q] * (double)r[q].r1, wgt[q] * (double)r[q].r2, xr, yr, zr, d[q], acc);
+            const double d[6] = {rec[q][1].y, rec[q][2].x, rec[q][2].y, rec[q][3].x, rec[q][3].y, rec[q][4].x};
+            entry_apply(wgt[q] * (double)r[q].r1, wgt[q] * (double)r[q].r2, xr, yr, zr, d, acc);
         }
     }
 }
@@ -541,7 +543,7 @@ struct SpmvArgs {
     const TEnt<V> *ent;
     const int *col;
     RowGeom geo;
-    const double *src; // 2 x 3 x npad
+    const double *pk;  // the source vector packed with the positions (PMQ_PK per atom)
     double *dst;       // 2 x 3 x npad (overwritten)
     const double *skip; // nonzero: return at once (or null)
 };
@@ -558,7 +560,7 @@ __global__ __launch_bounds__(256) void k_pol_spmv(SpmvArgs<V> a)
     if (row < np) {
         const int e1 = (int)min((size_t)a.rowptr[row + 1], a.cap);
         row_apply<G, SPMV_U, BK, V>(a.col, a.ent, a.rowptr[row], e1, g, a.geo, a.geo.sx[row], a.geo.sy[row],
-                                a.geo.sz[row], a.src, np, acc);
+                                    a.geo.sz[row], a.pk, acc);
     }
 #pragma unroll
     for (int c = 0; c < 6; c++)
@@ -588,7 +590,9 @@ struct PrecArgs {
     const double *spol;
     double udiag;
     const double *w, *r; // 2 x 3 x npad
+    const double *pkw;   // w packed with the positions (PMQ_PK per atom), for the columns
     double *z, *p;
+    double *pks;         // with init: p also into the packed input of T
     int init;
     const double *sc;    // sc[8] != 0: converged, nothing to do (not read with init)
     double *part;
@@ -605,7 +609,7 @@ __global__ __launch_bounds__(256) void k_pcg_prec(PrecArgs<V> a)
     if (row < np) {
         const int e1 = (int)min((size_t)(a.rowptr[row] + a.rownear[row]), a.cap);
         row_apply<G, PREC_U, BK, V>(a.col, a.ent_s, a.rowptr[row], e1, g, a.geo, a.geo.sx[row], a.geo.sy[row],
-                                a.geo.sz[row], a.w, np, acc);
+                                    a.geo.sz[row], a.pkw, acc);
     }
 #pragma unroll
     for (int c = 0; c < 6; c++)
@@ -620,7 +624,10 @@ __global__ __launch_bounds__(256) void k_pcg_prec(PrecArgs<V> a)
         const size_t i = (size_t)g * np + row;
         const double z = a.udiag * a.w[i] + fmax(1e-8, a.spol[row]) * v;
         a.z[i] = z;
-        if (a.init) a.p[i] = z;
+        if (a.init) {
+            a.p[i] = z;
+            a.pks[(size_t)PMQ_PK * row + 3 + g] = z;
+        }
         const double r = a.r[i];
         rz[g / 3] = r * z;
         rr[g / 3] = r * r;
@@ -986,13 +993,13 @@ static int row_box(const pmq_ctx *c)
 }
 
 template <class V>
-static int spmv(pmq_ctx *c, const double *src, double *dst, const double *skip)
+static int spmv(pmq_ctx *c, double *dst, const double *skip)
 {
     constexpr int G = SPMV_G;
     SpmvArgs<V> a;
     a.npad = c->npad; a.rowptr = c->rowptr.p; a.cap = c->tcap; a.ent = (const TEnt<V> *)c->tval.p;
     a.col = c->tcol.p; a.geo = row_geom(c);
-    a.src = src; a.dst = dst; a.skip = skip;
+    a.pk = c->pk_src.p; a.dst = dst; a.skip = skip;
     const size_t lanes = (size_t)c->npad * G;
     ProfScope ps_(c, "pol_mutual_spmv");
     const dim3 grid((unsigned)((lanes + 255) / 256));
@@ -1016,6 +1023,7 @@ static int precondition(pmq_ctx *c, const double *w, const double *r, double *z,
     a.col = c->tcol.p; a.geo = row_geom(c);
     a.spol = c->spol.p; a.udiag = c->prm.udiag;
     a.w = w; a.r = r; a.z = z; a.p = p; a.init = init ? 1 : 0; a.sc = sc; a.part = part;
+    a.pkw = c->pk_w.p; a.pks = c->pk_src.p;
     const dim3 grid((unsigned)pair_matrix_prec_blocks(c));
     switch (row_box(c)) {
     case ROW_BOX_NONE: hipLaunchKernelGGL((k_pcg_prec<G, ROW_BOX_NONE, V>), grid, dim3(256), 0, c->stream, a); break;
@@ -1039,10 +1047,10 @@ int pair_matrix_precondition(pmq_ctx *c, const double *w, const double *r, doubl
     return precondition<double>(c, w, r, z, p, init, sc, part);
 }
 
-int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst, const double *skip)
+int pair_matrix_apply(pmq_ctx *c, double *dst, const double *skip)
 {
-    if (c->tmixed) return spmv<float>(c, src, dst, skip);
-    return spmv<double>(c, src, dst, skip);
+    if (c->tmixed) return spmv<float>(c, dst, skip);
+    return spmv<double>(c, dst, skip);
 }
 
 } // namespace pmq

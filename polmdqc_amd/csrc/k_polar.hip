@@ -540,7 +540,21 @@ struct PcgArgs {
     int prec;             // sparse preconditioner: r is handed to it as w = a r
     double *w;
     int nblk2;
+    // the inputs of T (u for the first product, then p) and of the
+    // preconditioner (w), each also written into its packed copy (PMQ_PK per
+    // atom: position, then the six dipole components), which the row
+    // products read; positions from the sorted arrays
+    double *pks, *pkw;
+    const double *sx, *sy, *sz;
 };
+
+// component t (set-major, t = c npad + s) of a PCG vector into slot 3 + c of
+// atom s's packed record
+__device__ __forceinline__ void pk_store(double *pk, int npad, int t, double v)
+{
+    const int s = t % npad, c = t / npad;
+    pk[(size_t)PMQ_PK * s + 3 + c] = v;
+}
 
 // slot of r.z after iteration k (k = 0: the initial residual)
 __device__ __forceinline__ int rz_slot(int k) { return (k & 1) ? 11 : 0; }
@@ -587,8 +601,16 @@ __global__ __launch_bounds__(256) void k_pcg_guess(PcgArgs a)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 6 * a.npad) return;
-    const int s = t % a.npad;
-    a.u[t] = a.spol[s] * a.E[t];
+    const int s = t % a.npad, c = t / a.npad;
+    const double u = a.spol[s] * a.E[t];
+    a.u[t] = u;
+    pk_store(a.pks, a.npad, t, u);
+    if (c < 3) {
+        // the positions of both packed records (fixed during the solve)
+        const double x = c == 0 ? a.sx[s] : c == 1 ? a.sy[s] : a.sz[s];
+        a.pks[(size_t)PMQ_PK * s + c] = x;
+        if (a.prec) a.pkw[(size_t)PMQ_PK * s + c] = x;
+    }
 }
 
 // initial residual; z = p = a r and the r.z partials, or (prec) w = a r for
@@ -605,9 +627,11 @@ __global__ __launch_bounds__(PCG_TB) void k_pcg_init(PcgArgs a)
         a.r[t] = r;
         if (a.prec) {
             a.w[t] = z;
+            pk_store(a.pkw, a.npad, t, z);
         } else {
             a.z[t] = z;
             a.p[t] = z;
+            pk_store(a.pks, a.npad, t, z);
             (t < 3 * a.npad ? s0 : s1) = r * z;
         }
     }
@@ -663,6 +687,7 @@ __global__ __launch_bounds__(PCG_TB) void k_pcg_b(PcgArgs a)
         a.r[t] = r;
         if (a.prec) {
             a.w[t] = z;
+            pk_store(a.pkw, a.npad, t, z);
         } else {
             a.z[t] = z;
             (set == 0 ? s0 : s1) = r * z;
@@ -698,7 +723,9 @@ __global__ __launch_bounds__(PCG_TB) void k_pcg_c(PcgArgs a)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 6 * a.npad) return;
     const int set = t < 3 * a.npad ? 0 : 1;
-    a.p[t] = a.z[t] + beta[set] * a.p[t];
+    const double p = a.z[t] + beta[set] * a.p[t];
+    a.p[t] = p;
+    pk_store(a.pks, a.npad, t, p);
 }
 
 // block sum of one value, one atomic per block (blockDim 256)
@@ -767,8 +794,10 @@ static int tile_grid(pmq_ctx *c) { return c->tgrid; }
 // (their kernels return at once once it is set), or null
 static int mutual_field(pmq_ctx *c, const double *src, double *dst, const double *skip = nullptr)
 {
+    // the product reads src from its packed copy (written with src by the
+    // PCG kernels); the reciprocal part reads src itself
     int rc;
-    if ((rc = pair_matrix_apply(c, src, dst, skip))) return rc;
+    if ((rc = pair_matrix_apply(c, dst, skip))) return rc;
     if (c->prm.use_ewald && (rc = pme_dipole_field(c, src, dst, skip))) return rc;
     return PMQ_OK;
 }
@@ -839,6 +868,11 @@ int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
     a.nblk2 = gb2;
     a.prec = prec ? 1 : 0;
     a.w = c->pw.p;
+    PMQ_HIP(c->pk_src.alloc((size_t)PMQ_PK * np));
+    if (prec) PMQ_HIP(c->pk_w.alloc((size_t)PMQ_PK * np));
+    a.pks = c->pk_src.p;
+    a.pkw = prec ? c->pk_w.p : nullptr;
+    a.sx = c->sx.p; a.sy = c->sy.p; a.sz = c->sz.p;
     a.sc = c->pcg.p;
     int npolar = 0;
     for (double p : c->h_polarity) npolar += p != 0.0;

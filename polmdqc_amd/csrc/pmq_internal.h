@@ -24,6 +24,7 @@
 
 #define PMQ_BLOCK 64
 #define PMQ_BLK_STRIDE 12 // doubles per block in the block-bounds array
+#define PMQ_PK 10         // doubles per atom in a packed product input: x y z, two dipole sets, pad
 
 namespace pmq {
 
@@ -292,6 +293,9 @@ struct pmq_ctx {
     pmq::DBuf<unsigned char> pscan_tmp;
     pmq::DBuf<double> tval;                      // tcap entries (R1, R2) of T = R2 r r^T - R1 I (double2, or float2 mixed)
     pmq::DBuf<int> tcol;                         // tcap columns of those entries (shared with tval_s)
+    pmq::DBuf<double> pk_src, pk_w;              // PMQ_PK per sorted atom: the inputs of T (SpMV) and of the
+                                                 // preconditioner, packed with the positions (written by the
+                                                 // PCG kernels that produce them)
     // PCG preconditioner: rows keep their pairs within usolvcut first
     bool prec_on = false;                        // this evaluation solves with the sparse preconditioner
     pmq::DBuf<unsigned long long> nmask;         // 128 per tile: the pairs within usolvcut (row, column masks)
@@ -396,7 +400,7 @@ int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fi
 int tile_scalars(pmq_ctx *c, int which, bool full);
 // skip: a device flag; when it is nonzero the launched kernels return at once
 // (the speculative PCG iterations after convergence)
-int pair_matrix_apply(pmq_ctx *c, const double *src, double *dst, const double *skip = nullptr);
+int pair_matrix_apply(pmq_ctx *c, double *dst, const double *skip = nullptr);
 int pair_matrix_prec_blocks(const pmq_ctx *c);
 int pair_matrix_precondition(pmq_ctx *c, const double *w, const double *r, double *z, double *p, bool init,
                              const double *sc, double *part);
