@@ -39,3 +39,23 @@ def test_library_exports_nothing_else():
     syms = {line.split()[-1] for line in out.splitlines() if " T " in line}
     pmq = {s for s in syms if s.startswith("pmq_")}
     assert pmq == set(declared())
+
+
+def test_header_and_bindings_agree_on_fft_flags():
+    """The PME transform constants of the header match the Python bindings,
+    and bench.py's --pin-fft spellings map onto them (no GPU needed)."""
+    import sys
+
+    sys.path.insert(0, ROOT)
+    import bench
+    from polmdqc_amd import engine as E
+
+    hdr = open(os.path.join(ROOT, "include", "pmq.h")).read()
+    consts = dict(re.findall(r"#define (PMQ_FFT_\w+) (\d+)", hdr))
+    assert int(consts["PMQ_FFT_AUTO"]) == E.FFT_AUTO
+    assert int(consts["PMQ_FFT_LIBRARY"]) == E.FFT_LIBRARY
+    assert int(consts["PMQ_FFT_DIRECT"]) == E.FFT_DIRECT
+    assert int(consts["PMQ_FFT_GRIDS_FP32"]) == E.FFT_GRIDS_FP32
+    assert int(consts["PMQ_FFT_GRIDS_FP64"]) == E.FFT_GRIDS_FP64
+    assert bench.pin_fft_values("library/fp32-direct") == (E.FFT_LIBRARY, E.FFT_DIRECT | E.FFT_GRIDS_FP32)
+    assert bench.pin_fft_values("direct/fp64-library") == (E.FFT_DIRECT, E.FFT_LIBRARY | E.FFT_GRIDS_FP64)
