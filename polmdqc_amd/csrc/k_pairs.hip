@@ -533,7 +533,7 @@ __device__ __forceinline__ void row_apply(const int *cols, const TEnt<V> *ent, i
 constexpr int SPMV_U = 2;
 constexpr int PREC_U = 2;
 constexpr int SPMV_G = 16;
-constexpr int PREC_G = 16;
+constexpr int PREC_G = 8;
 
 template <class V>
 struct SpmvArgs {
