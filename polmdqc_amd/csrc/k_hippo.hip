@@ -163,7 +163,7 @@ __global__ __launch_bounds__(64) void k_disp_tiles(HippoArgs a)
             Q[c * 64 + lane] = T(acc[1][c][lane]);
         }
         const double ew = wsum(e);
-        if (lane == 0) a.tscal[(size_t)t * PMQ_TILE_NS] = ew;
+        if (lane == 0) a.tscal[t] = ew; // scalar 0 of tile t
     }
 }
 
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(64) void k_rep_tiles(HippoArgs a)
             Q[c * 64 + lane] = T(acc[1][c][lane]);
         }
         const double ew = wsum(e);
-        if (lane == 0) a.tscal[(size_t)t * PMQ_TILE_NS] = ew;
+        if (lane == 0) a.tscal[t] = ew; // scalar 0 of tile t
     }
 }
 

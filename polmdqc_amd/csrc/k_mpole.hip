@@ -341,7 +341,8 @@ struct MpoleArgs {
     const unsigned long long *xmask; // per tile: [64,128) row masks of the in-cutoff pairs with an active atom,
                                      // scaled pairs left out
     void *part;              // per tile partial sums (2 sides x PMQ_TILE_NV x 64 of R)
-    double *tscal;           // per tile scalars (PMQ_TILE_NS)
+    double *tscal;           // per tile scalars: scalar q of tile t at tscal[q * tstride + t]
+    size_t tstride;
     Totals *tot;
     // exclusion corrections
     const ExclPair *excl;
@@ -474,12 +475,13 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
         }
         if (ANA) { win = wave_sum((double)ein); nem = wave_sum(nem); }
         if (lane == 0) {
-            double *S = a.tscal + (size_t)t * PMQ_TILE_NS;
+            double *S = a.tscal + t;
+            const size_t ts = a.tstride;
             S[0] = ew;
             if (VIR || ANA) { // the full set is summed when either is on
-                S[1] = wxx; S[2] = wxy; S[3] = wxz; S[4] = wxy; S[5] = wyy; S[6] = wyz; S[7] = wxz; S[8] = wyz;
-                S[9] = wzz;
-                S[10] = win; S[11] = (double)nem;
+                S[ts] = wxx; S[2 * ts] = wxy; S[3 * ts] = wxz; S[4 * ts] = wxy; S[5 * ts] = wyy; S[6 * ts] = wyz;
+                S[7 * ts] = wxz; S[8 * ts] = wyz; S[9 * ts] = wzz;
+                S[10 * ts] = win; S[11 * ts] = (double)nem;
             }
         }
     }
@@ -637,6 +639,7 @@ int launch_mpole_direct(pmq_ctx *c, int mode, double off2, bool mixed)
     a.xmask = c->xmask.p;
     a.part = c->tpart.p;
     a.tscal = c->tscal.p;
+    a.tstride = (size_t)c->tile_cap;
     a.tot = c->totals.p;
     a.excl = c->excl.p;
     a.nexcl = (int)c->h_excl.size();

@@ -916,7 +916,8 @@ __global__ __launch_bounds__(64 * RED_W) void k_tile_reduce(TileReduceArgs a)
 constexpr int SCAL_TB = 1024;
 
 __global__ __launch_bounds__(SCAL_TB) void k_tile_scalars(const int *ntiles_p, int which, int full,
-                                                          const double *__restrict__ tscal, Totals *tot)
+                                                          const double *__restrict__ tscal, size_t stride,
+                                                          Totals *tot)
 {
     __shared__ double red[SCAL_TB];
     const int ntiles = *ntiles_p;
@@ -925,7 +926,7 @@ __global__ __launch_bounds__(SCAL_TB) void k_tile_scalars(const int *ntiles_p, i
         // each thread's tiles in order (the unrolled loads go out together)
         double v = 0.0;
 #pragma unroll 8
-        for (int t = threadIdx.x; t < ntiles; t += SCAL_TB) v += tscal[(size_t)t * PMQ_TILE_NS + k];
+        for (int t = threadIdx.x; t < ntiles; t += SCAL_TB) v += tscal[(size_t)k * stride + t];
         red[threadIdx.x] = v;
         __syncthreads();
         for (int w = SCAL_TB / 2; w > 0; w >>= 1) {
@@ -967,7 +968,7 @@ int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fi
 int tile_scalars(pmq_ctx *c, int which, bool full)
 {
     hipLaunchKernelGGL(k_tile_scalars, dim3(1), dim3(SCAL_TB), 0, c->stream, c->ntiles_eff.p, which, full ? 1 : 0,
-                       c->tscal.p, c->totals.p);
+                       c->tscal.p, (size_t)c->tile_cap, c->totals.p);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }

@@ -151,7 +151,8 @@ struct HostStatus {       // pinned host mirror, written by the evaluation's las
 
 // per-tile partial sums of the pair kernels (k_pairs.hip): values per lane
 // and side, scalars per tile (0 energy, 1..9 virial, 10 inter-molecular
-// energy, 11 pair count)
+// energy, 11 pair count), stored scalar-major (scalar q of tile t at
+// q * tile capacity + t) so each scalar's sum reads one contiguous array
 constexpr int PMQ_TILE_NV = 8;
 constexpr int PMQ_TILE_NS = 12;
 
