@@ -188,6 +188,8 @@ def bench_clusters(eng, L, steps, warmup) -> dict:
             eng.sync()
         ms = (time.perf_counter() - t0) * 1e3 / steps
         res[name] = eng.fetch()
+        if not np.all(eng.fetch_batch()["pcg_iters"] > 0):
+            raise RuntimeError("bench: a cluster's induced dipoles did not converge")
         out[f"{name}_evals_per_s"] = round(1e3 / ms, 3)
         out[f"{name}_ms_per_eval"] = round(ms, 4)
     d = res["mixed"]["desum"] - res["fp64"]["desum"]

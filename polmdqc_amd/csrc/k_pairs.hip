@@ -534,6 +534,10 @@ constexpr int SPMV_U = 2;
 constexpr int PREC_U = 2;
 constexpr int SPMV_G = 16;
 constexpr int PREC_G = 8;
+// each row's six outputs (two dipole sets x xyz) are written by its lanes
+// g < 6 after the shuffle reduction, and the reduction needs a power of two
+static_assert(SPMV_G >= 6 && (SPMV_G & (SPMV_G - 1)) == 0 && SPMV_G <= 64, "SpMV lanes per row");
+static_assert(PREC_G >= 6 && (PREC_G & (PREC_G - 1)) == 0 && PREC_G <= 64, "preconditioner lanes per row");
 
 template <class V>
 struct SpmvArgs {

@@ -55,6 +55,19 @@ static int upload(pmq_ctx *c, DBuf<T> &d, const T *h, size_t count)
     return PMQ_OK;
 }
 
+static int pmq_check(pmq_ctx *c);
+
+// Setters first complete an evaluation that is still pending (pmq_check:
+// wait, and redo it on its own inputs if it fell short), so a later fetch
+// returns the results of the inputs that evaluation was enqueued with.
+#define SETTLE_PENDING(c)                                                                          \
+    do {                                                                                           \
+        if ((c)->pending) {                                                                        \
+            const int rc_ = pmq_check(c);                                                          \
+            if (rc_) return rc_;                                                                   \
+        }                                                                                          \
+    } while (0)
+
 extern "C" {
 
 int pmq_abi_version(void) { return PMQ_ABI_VERSION; }
@@ -187,6 +200,11 @@ const char *pmq_last_error(const pmq_ctx *c) { return c ? c->err.c_str() : "null
 int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
 {
     CHECK_CTX(c);
+    SETTLE_PENDING(c);
+    // a call that fails part way leaves no usable system behind: nothing
+    // computes until a later set_topology (and set_params, set_coords) succeeds
+    c->have_top = c->have_prm = c->have_xyz = false;
+    c->nsys = 0;
     if (!t || t->n <= 0 || !t->polaxe || !t->zaxis || !t->xaxis || !t->yaxis || !t->n12 || !t->n13 || !t->n14 ||
         !t->n15) {
         c->err = "pmq_set_topology: missing arrays or n <= 0";
@@ -362,6 +380,11 @@ int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
 int pmq_set_params(pmq_ctx *c, const pmq_params *p)
 {
     CHECK_CTX(c);
+    SETTLE_PENDING(c);
+    // a rejected call leaves no parameters set (the device copies may be
+    // half updated): compute fails with PMQ_ERR_STATE until a call succeeds
+    c->have_prm = false;
+    batch_graph_release(c);
     if (!c->have_top) {
         c->err = "pmq_set_params: call pmq_set_topology first";
         return PMQ_ERR_STATE;
@@ -527,6 +550,7 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
 int pmq_set_coords(pmq_ctx *c, const double *x, const double *y, const double *z, const pmq_box *box)
 {
     CHECK_CTX(c);
+    SETTLE_PENDING(c);
     if (!c->have_top) {
         c->err = "pmq_set_coords: call pmq_set_topology first";
         return PMQ_ERR_STATE;
@@ -820,6 +844,7 @@ static int batch_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
 int pmq_set_batch(pmq_ctx *c, int nsys, const int *off)
 {
     CHECK_CTX(c);
+    SETTLE_PENDING(c);
     if (!c->have_top) {
         c->err = "pmq_set_batch: call pmq_set_topology first";
         return PMQ_ERR_STATE;
@@ -853,6 +878,7 @@ int pmq_set_batch(pmq_ctx *c, int nsys, const int *off)
         eoff[owner[e.i] + 1]++;
     }
     for (int k = 0; k < nsys; k++) eoff[k + 1] += eoff[k];
+    c->nsys = 0; // one system until the batch is fully uploaded
     std::vector<int> order(nsys);
     for (int k = 0; k < nsys; k++) order[k] = k;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sizes[a] > sizes[b]; });
@@ -942,6 +968,10 @@ int pmq_param_grad(pmq_ctx *c, double *dpole, double *dpolarity, double *dpdamp,
     CHECK_CTX(c);
     int rc = pmq_check(c);
     if (rc) return rc;
+    if (!c->have_top || !c->have_prm || !c->have_xyz) {
+        c->err = "pmq_param_grad: topology, parameters and coordinates must be set and evaluated first";
+        return PMQ_ERR_STATE;
+    }
     if (c->nsys > 0 || c->last_mode != PMQ_GRADIENT || c->last_precision != PMQ_PLAIN_FP64) {
         c->err = "pmq_param_grad: needs a plain-FP64 GRADIENT evaluation of one system first";
         return PMQ_ERR_STATE;
@@ -1027,6 +1057,18 @@ int pmq_fetch(pmq_ctx *c, pmq_result *r)
     if (pol && c->pcg_iters < 0) {
         c->err = "pmq_fetch: induced dipoles did not converge within politer iterations";
         return PMQ_ERR_NOCONV;
+    }
+    if (pol && c->nsys > 0) {
+        // batched clusters: each cluster's iteration count, negative when
+        // its solve ran out of politer iterations
+        std::vector<double> it(c->nsys);
+        PMQ_HIP(hipMemcpy(it.data(), c->b_it.p, c->nsys * sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 0; k < c->nsys; k++)
+            if (it[k] < 0.0) {
+                c->err = "pmq_fetch: the induced dipoles of cluster " + std::to_string(k) +
+                         " did not converge within politer iterations";
+                return PMQ_ERR_NOCONV;
+            }
     }
     return PMQ_OK;
 }
