@@ -103,7 +103,8 @@ def test_amoeba12k_iterations_at_production_tolerance(engine):
     o = oracle.elec(s, want_grad=False)
     assert g["pcg_iters"] == o["iters"] == g2["pcg_iters"]
     assert abs(g["ep"] - o["ep"]) <= 1e-6 * abs(o["ep"])
-    assert g2["ep"] == g["ep"]
+    # FP64 atomics add in a run-dependent order: equal to rounding
+    assert abs(g2["ep"] - g["ep"]) <= 1e-12 * abs(g["ep"])
 
 
 def test_amoeba12k_mixed_against_fp64(engine):

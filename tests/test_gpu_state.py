@@ -69,7 +69,7 @@ def test_rejected_set_params_blocks_compute(engine):
         engine.compute(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR)
     engine.set_system(s)
     g = engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR)
-    assert g["ep"] == ref["ep"]
+    assert abs(g["ep"] - ref["ep"]) <= 1e-12 * abs(ref["ep"])  # FP64 atomics: equal to rounding
 
 
 def test_set_coords_completes_the_pending_evaluation(engine):
