@@ -241,6 +241,27 @@ PMQ_API int pmq_param_grad(pmq_ctx *ctx, double *dpole, double *dpolarity, doubl
 PMQ_API int pmq_set_batch(pmq_ctx *ctx, int nsys, const int *offsets);
 PMQ_API int pmq_fetch_batch(pmq_ctx *ctx, double *em, double *ep, int *pcg_iters);
 
+/* The batched evaluation without allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory while they
+ * run (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  After pmq_set_topology, pmq_set_params, pmq_set_coords
+ * and pmq_set_batch (which allocate), this enqueues the same evaluation as
+ * pmq_compute on a batch -- same kernels, the lanes-per-row variant a previous
+ * pmq_compute tuned, else the first -- on the context's stream, calling no
+ * hipMalloc, hipFree or allocating routine.  Its scratch (work-queue counter,
+ * per-cluster energies and iteration counts, energy totals) lives in one
+ * fixed device array per GPU, held until the process ends, for at most
+ * PMQ_NOALLOC_MAX_CLUSTERS (65536) clusters; larger batches return
+ * PMQ_ERR_ARG.  Every call on one GPU shares that array, so calls that
+ * overlap on one GPU (two contexts, two threads or streams) race.  Outputs
+ * are caller DEVICE pointers, each may be NULL: em, ep and pcg_iters per
+ * cluster (nsys), etot = {em, ep} totals (2), grad = 3n total gradient
+ * (dem + dep, GRADIENT only).  The call is asynchronous: pmq_sync waits for
+ * it.  pmq_fetch and pmq_fetch_batch do not see its results. */
+#define PMQ_NOALLOC_MAX_CLUSTERS 65536
+PMQ_API int pmq_compute_batch_noalloc(pmq_ctx *ctx, int mode, unsigned terms, int precision, double *em, double *ep,
+                                      int *pcg_iters, double *etot, double *grad);
+
 /* Kernel timing for measurement.  No counterpart in the reference (it has no
  * timers on this path); for benchmark and profiling harnesses.  While
  * enabled, the context brackets every launch of each instrumented kernel

@@ -84,7 +84,7 @@ class Result(C.Structure):
 EXPORTS = ["pmq_create", "pmq_destroy", "pmq_last_error", "pmq_abi_version", "pmq_set_topology",
            "pmq_set_params", "pmq_eval", "pmq_set_coords", "pmq_compute", "pmq_fetch", "pmq_sync",
            "pmq_profile_enable", "pmq_profile_read", "pmq_set_batch", "pmq_fetch_batch", "pmq_param_grad",
-           "pmq_set_fft", "pmq_fft_choice", "pmq_set_gather_lanes"]
+           "pmq_set_fft", "pmq_fft_choice", "pmq_set_gather_lanes", "pmq_compute_batch_noalloc"]
 
 _lib = None
 
@@ -113,6 +113,8 @@ def lib() -> C.CDLL:
     L.pmq_set_batch.argtypes = [vp, C.c_int, _ip]
     L.pmq_fetch_batch.argtypes = [vp, _dp, _dp, _ip]
     L.pmq_param_grad.argtypes = [vp, _dp, _dp, _dp, _dp]
+    # device pointers (integers) for the allocation-free batched evaluation
+    L.pmq_compute_batch_noalloc.argtypes = [vp, C.c_int, C.c_uint, C.c_int, vp, vp, vp, vp, vp]
     for name in EXPORTS:
         if name != "pmq_last_error":
             getattr(L, name).restype = C.c_int

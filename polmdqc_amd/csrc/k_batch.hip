@@ -18,6 +18,10 @@
 // corrections, PCG with the polarizability preconditioner and the test
 // debye * sqrt(max_set r.r / npolar) < poleps, E_pol = -(f/2) mu_d . E^p, and
 // the variational gradient.
+#include <algorithm>
+#include <cstddef>
+#include <mutex>
+
 #include "pair_mpole.h"
 #include "pmq_internal.h"
 
@@ -570,7 +574,7 @@ __global__ void k_batch_reset(int *counter)
 
 int batch_variants() { return 2; }
 
-int launch_batch(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int variant)
+int launch_batch(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int variant, const BatchScratch *bs)
 {
     BatchArgs a;
     a.nsys = c->nsys;
@@ -586,10 +590,12 @@ int launch_batch(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int v
     a.grad = mode == PMQ_GRADIENT;
     a.do_mpole = mpole; a.do_polar = polar;
     a.dem = c->dem.p; a.te = c->te.p; a.dep = c->dep.p; a.tep = c->tep.p;
-    a.em_c = c->b_em.p; a.ep_c = c->b_ep.p; a.it_c = c->b_it.p;
-    a.counter = c->b_counter.p;
-    a.tot = c->totals.p;
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, c->stream, c->b_counter.p);
+    a.em_c = bs ? bs->em_c : c->b_em.p;
+    a.ep_c = bs ? bs->ep_c : c->b_ep.p;
+    a.it_c = bs ? bs->it_c : c->b_it.p;
+    a.counter = bs ? bs->counter : c->b_counter.p;
+    a.tot = bs ? bs->tot : c->totals.p;
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, c->stream, a.counter);
     const int wg = std::max(1, std::min(c->nsys, c->num_cus * 2));
     {
         ProfScope ps_(c, "batch_clusters");
@@ -601,7 +607,103 @@ int launch_batch(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int v
             else hipLaunchKernelGGL((k_batch_clusters<double, 2, false>), dim3(wg), dim3(2 * BATCH_MAX), 0, c->stream, a);
         }
     }
-    hipLaunchKernelGGL(k_batch_totals, dim3(1), dim3(256), 0, c->stream, c->nsys, c->b_em.p, c->b_ep.p, c->totals.p);
+    hipLaunchKernelGGL(k_batch_totals, dim3(1), dim3(256), 0, c->stream, c->nsys, a.em_c, a.ep_c, a.tot);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free batched evaluation (pmq_compute_batch_noalloc) keeps
+// its scratch -- the work-queue counter, the per-cluster energies and
+// iteration counts, the energy totals -- in this fixed device array instead
+// of context buffers.  Each GPU holds its own copy for the life of the
+// process; its device address is taken once per device when a context is
+// created (static_scratch_attach), and kernels are only ever given that
+// address.  Every call on one GPU shares the array: calls that overlap on
+// one GPU race.
+struct StaticScratchLayout {
+    Totals tot;
+    int counter;
+    int pad[63];
+    double em_c[PMQ_NOALLOC_MAX_CLUSTERS];
+    double ep_c[PMQ_NOALLOC_MAX_CLUSTERS];
+    double it_c[PMQ_NOALLOC_MAX_CLUSTERS];
+};
+static __device__ __align__(256) unsigned char scratch[sizeof(StaticScratchLayout)];
+
+static std::mutex scratch_mu;
+static void *scratch_dev[PMQ_NOALLOC_MAX_DEVICES] = {};
+
+size_t static_scratch_bytes() { return sizeof(StaticScratchLayout); }
+
+// the calling thread's current device must be `device`
+int static_scratch_attach(pmq_ctx *c, int device)
+{
+    if (device < 0 || device >= PMQ_NOALLOC_MAX_DEVICES) {
+        c->err = "device ordinal beyond the static scratch table";
+        return PMQ_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_dev[device]) {
+        void *p = nullptr;
+        PMQ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            c->err = "hipGetSymbolAddress returned no address for the static scratch";
+            return PMQ_ERR_HIP;
+        }
+        scratch_dev[device] = p;
+    }
+    c->static_scratch = scratch_dev[device];
+    return PMQ_OK;
+}
+
+// results of the allocation-free evaluation into the caller's device arrays
+__global__ void k_batch_static_out(int nsys, int n3, const double *em_c, const double *ep_c, const double *it_c,
+                                   const Totals *tot, double *em, double *ep, int *iters, double *etot,
+                                   const double *dem, const double *dep, int mp, int pol, double *grad)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    for (int k = t; k < nsys; k += stride) {
+        if (em) em[k] = em_c[k];
+        if (ep) ep[k] = ep_c[k];
+        if (iters) iters[k] = (int)it_c[k];
+    }
+    if (grad)
+        for (int j = t; j < n3; j += stride) grad[j] = (mp ? dem[j] : 0.0) + (pol ? dep[j] : 0.0);
+    if (etot && t == 0) {
+        etot[0] = tot->em;
+        etot[1] = tot->ep;
+    }
+}
+
+int batch_noalloc_enqueue(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int variant, double *em,
+                          double *ep, int *iters, double *etot, double *grad)
+{
+    // device addresses inside this device's array (from static_scratch_attach)
+    unsigned char *base = (unsigned char *)c->static_scratch;
+    BatchScratch bs;
+    bs.tot = (Totals *)(base + offsetof(StaticScratchLayout, tot));
+    bs.counter = (int *)(base + offsetof(StaticScratchLayout, counter));
+    bs.em_c = (double *)(base + offsetof(StaticScratchLayout, em_c));
+    bs.ep_c = (double *)(base + offsetof(StaticScratchLayout, ep_c));
+    bs.it_c = (double *)(base + offsetof(StaticScratchLayout, it_c));
+    const bool grad_mode = mode == PMQ_GRADIENT;
+    int rc;
+    // the totals start at zero (a fill kernel: nothing here records a memset)
+    if ((rc = launch_zero(c, (double *)bs.tot, sizeof(Totals) / sizeof(double)))) return rc;
+    if ((rc = launch_rotpole(c, false))) return rc;
+    if ((rc = launch_batch(c, mode, mpole, polar, mixed, variant, &bs))) return rc;
+    if (grad_mode) {
+        if (mpole && (rc = launch_torque(c, PMQ_GRADIENT, c->te.p, c->dem.p, mixed))) return rc;
+        if (polar && (rc = launch_torque(c, PMQ_GRADIENT, c->tep.p, c->dep.p, mixed))) return rc;
+    }
+    const int n3 = 3 * c->n;
+    const int work = std::max(c->nsys, grad_mode && grad ? n3 : 0);
+    const int blocks = std::max(1, std::min((work + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_batch_static_out, dim3(blocks), dim3(256), 0, c->stream, c->nsys, n3, bs.em_c, bs.ep_c,
+                       bs.it_c, bs.tot, em, ep, iters, etot, c->dem.p, c->dep.p, mpole ? 1 : 0, polar ? 1 : 0,
+                       grad_mode ? grad : nullptr);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }

@@ -177,6 +177,11 @@ int pmq_create(const pmq_config *cfg, pmq_ctx **out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
         c->num_cus = prop.multiProcessorCount;
+    // this device's address of the fixed scratch of pmq_compute_batch_noalloc
+    if (static_scratch_attach(c, c->device) != PMQ_OK) {
+        pmq_destroy(c);
+        return PMQ_ERR_HIP;
+    }
     *out = c;
     return PMQ_OK;
 }
@@ -770,7 +775,7 @@ static int batch_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, in
     return PMQ_OK;
 }
 
-static int batch_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
+static int batch_check(pmq_ctx *c, int mode, unsigned terms)
 {
     if (mode != PMQ_ENERGY && mode != PMQ_GRADIENT) {
         c->err = "pmq_compute: batched clusters compute energy and gradient";
@@ -792,8 +797,14 @@ static int batch_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
         c->err = "pmq_compute: batched clusters solve with the diagonal preconditioner (set pcgprec = 0)";
         return PMQ_ERR_UNSUPPORTED;
     }
+    return PMQ_OK;
+}
+
+static int batch_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
+{
+    int rc = batch_check(c, mode, terms);
+    if (rc) return rc;
     PMQ_HIP(hipSetDevice(c->device));
-    int rc;
     c->last_mode = mode;
     c->last_terms = terms;
     c->last_precision = precision;
@@ -907,6 +918,11 @@ int pmq_fetch_batch(pmq_ctx *c, double *em, double *ep, int *iters)
         c->err = "pmq_fetch_batch: no batch set";
         return PMQ_ERR_STATE;
     }
+    if (c->noalloc_last) {
+        c->err = "pmq_fetch_batch: the last evaluation (pmq_compute_batch_noalloc) wrote its results to the caller's "
+                 "arrays";
+        return PMQ_ERR_STATE;
+    }
     PMQ_HIP(hipStreamSynchronize(c->stream));
     const int ns = c->nsys;
     if (em) PMQ_HIP(hipMemcpy(em, c->b_em.p, ns * sizeof(double), hipMemcpyDeviceToHost));
@@ -919,11 +935,49 @@ int pmq_fetch_batch(pmq_ctx *c, double *em, double *ep, int *iters)
     return PMQ_OK;
 }
 
+int pmq_compute_batch_noalloc(pmq_ctx *c, int mode, unsigned terms, int precision, double *em, double *ep,
+                              int *pcg_iters, double *etot, double *grad)
+{
+    CHECK_CTX(c);
+    SETTLE_PENDING(c);
+    int rc = check_compute(c, mode, terms, precision);
+    if (rc) return rc;
+    if (c->nsys <= 0) {
+        c->err = "pmq_compute_batch_noalloc: no batch set (pmq_set_batch)";
+        return PMQ_ERR_STATE;
+    }
+    if (c->nsys > PMQ_NOALLOC_MAX_CLUSTERS) {
+        c->err = "pmq_compute_batch_noalloc: " + std::to_string(c->nsys) + " clusters exceed the fixed scratch (" +
+                 std::to_string(PMQ_NOALLOC_MAX_CLUSTERS) + " clusters)";
+        return PMQ_ERR_ARG;
+    }
+    if (c->profiling) {
+        c->err = "pmq_compute_batch_noalloc: kernel timing creates events; switch profiling off first";
+        return PMQ_ERR_STATE;
+    }
+    if (!c->static_scratch) {
+        c->err = "pmq_compute_batch_noalloc: no static scratch address for this device";
+        return PMQ_ERR_STATE;
+    }
+    if ((rc = batch_check(c, mode, terms))) return rc;
+    PMQ_HIP(hipSetDevice(c->device));
+    c->last_mode = mode;
+    c->last_terms = terms;
+    c->last_precision = precision;
+    c->pcg_iters = 0;
+    c->noalloc_last = true;
+    // the lanes-per-row variant a previous pmq_compute tuned, else the first
+    const int variant = c->batch_variant >= 0 ? c->batch_variant : 0;
+    return batch_noalloc_enqueue(c, mode, terms & PMQ_TERM_MPOLE, terms & PMQ_TERM_POLAR, precision == PMQ_MIXED,
+                                 variant, em, ep, pcg_iters, etot, grad);
+}
+
 int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
 {
     CHECK_CTX(c);
     int rc = check_compute(c, mode, terms, precision);
     if (rc) return rc;
+    c->noalloc_last = false;
     if (c->nsys > 0) return batch_compute(c, mode, terms, precision);
     c->pending = false;
     rc = compute_enqueue(c, mode, terms, precision, c->safe);
@@ -996,6 +1050,10 @@ int pmq_fetch(pmq_ctx *c, pmq_result *r)
     if (!r) return PMQ_ERR_ARG;
     int rc0 = pmq_check(c);
     if (rc0) return rc0;
+    if (c->noalloc_last) {
+        c->err = "pmq_fetch: the last evaluation (pmq_compute_batch_noalloc) wrote its results to the caller's arrays";
+        return PMQ_ERR_STATE;
+    }
     Totals t;
     PMQ_HIP(hipMemcpy(&t, c->totals.p, sizeof t, hipMemcpyDeviceToHost));
     const int n = c->n;

@@ -25,6 +25,7 @@
 #define PMQ_BLOCK 64
 #define PMQ_BLK_STRIDE 12 // doubles per block in the block-bounds array
 #define PMQ_PK 10         // doubles per atom in a packed product input: x y z, two dipole sets, pad
+#define PMQ_NOALLOC_MAX_DEVICES 64
 
 namespace pmq {
 
@@ -314,6 +315,8 @@ struct pmq_ctx {
     int num_cus = 256;
     int batch_variant = -1;                      // autotuned lanes-per-row variant (-1 = not yet)
     void *b_graph = nullptr, *b_exec = nullptr;  // captured evaluation
+    void *static_scratch = nullptr;              // this device's fixed scratch array (pmq_compute_batch_noalloc)
+    bool noalloc_last = false;                   // the last evaluation was pmq_compute_batch_noalloc
     int b_graph_key = -1;
     // PME
     int K[3] = {0, 0, 0};
@@ -405,7 +408,18 @@ int pair_matrix_apply(pmq_ctx *c, double *dst, const double *skip = nullptr);
 int pair_matrix_prec_blocks(const pmq_ctx *c);
 int pair_matrix_precondition(pmq_ctx *c, const double *w, const double *r, double *z, double *p, bool init,
                              const double *sc, double *part);
-int launch_batch(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int variant);
+// device addresses of the batch scratch when it lives outside the context
+struct BatchScratch {
+    Totals *tot;
+    int *counter;
+    double *em_c, *ep_c, *it_c;
+};
+int launch_batch(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int variant,
+                 const BatchScratch *bs = nullptr);
+int static_scratch_attach(pmq_ctx *c, int device);
+size_t static_scratch_bytes();
+int batch_noalloc_enqueue(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int variant, double *em,
+                          double *ep, int *iters, double *etot, double *grad);
 int param_gradients(pmq_ctx *c, bool mpole, bool polar, double *dpole, double *dpolarity, double *dpdamp,
                     double *dthole);
 int batch_variants();

@@ -247,6 +247,16 @@ class Engine:
                                             it.ctypes.data_as(_ip)), "pmq_fetch_batch")
         return {"em": em, "ep": ep, "pcg_iters": it}
 
+    def compute_batch_noalloc(self, mode: int, terms: int, precision: int, em=0, ep=0, iters=0, etot=0,
+                              grad=0):
+        """The batched evaluation without allocation (pmq_compute_batch_noalloc):
+        outputs are device addresses (integers, 0 = not wanted) of em, ep
+        (double, one per cluster), iters (int32, one per cluster), etot
+        (double, 2) and grad (double, 3n).  Asynchronous: sync() waits."""
+        self._check(self._L.pmq_compute_batch_noalloc(self._ctx, mode, terms, precision, em or None, ep or None,
+                                                      iters or None, etot or None, grad or None),
+                    "pmq_compute_batch_noalloc")
+
     def eval(self, mode: int = L.VIRIAL, terms: int = L.TERM_MPOLE, precision: int = L.PLAIN_FP64,
              xyz: np.ndarray | None = None, want_rpole: bool = False) -> dict:
         if xyz is not None:
