@@ -181,7 +181,7 @@ class ElecSys(C.Structure):
 class ElecOut(C.Structure):
     _fields_ = [("em", C.c_double), ("ep", C.c_double), ("em_real", C.c_double), ("em_recip", C.c_double),
                 ("em_self", C.c_double), ("iters", C.c_int), ("dem", _dp), ("dep", _dp), ("uind", _dp),
-                ("uinp", _dp)]
+                ("uinp", _dp), ("vir_m", _dp), ("vir_p", _dp)]
 
 
 def elec(fx, want_grad: bool = True, solver: int = 0, x=None, poleps: float | None = None,
@@ -225,14 +225,16 @@ def elec(fx, want_grad: bool = True, solver: int = 0, x=None, poleps: float | No
     s.udiag = float(fx.get("udiag", 2.0))
     o = ElecOut()
     dem, dep, ud, up = (np.zeros(3 * n) for _ in range(4))
+    vm, vp = np.zeros(9), np.zeros(9)
     o.dem, o.dep = dem.ctypes.data_as(_dp), dep.ctypes.data_as(_dp)
     o.uind, o.uinp = ud.ctypes.data_as(_dp), up.ctypes.data_as(_dp)
+    o.vir_m, o.vir_p = vm.ctypes.data_as(_dp), vp.ctypes.data_as(_dp)
     rc = L.orc_elec(C.byref(s), 1 if want_grad else 0, C.byref(o))
     if rc != 0:
         raise RuntimeError(f"orc_elec failed ({rc})")
     return {"em": o.em, "ep": o.ep, "em_real": o.em_real, "em_recip": o.em_recip, "em_self": o.em_self,
             "iters": o.iters, "dem": dem.reshape(n, 3), "dep": dep.reshape(n, 3), "uind": ud.reshape(n, 3),
-            "uinp": up.reshape(n, 3)}
+            "uinp": up.reshape(n, 3), "vir_m": vm.reshape(3, 3), "vir_p": vp.reshape(3, 3)}
 
 
 class HippoSys(C.Structure):
@@ -246,7 +248,7 @@ class HippoSys(C.Structure):
 
 class HippoOut(C.Structure):
     _fields_ = [("edsp", C.c_double), ("er", C.c_double), ("edsp_real", C.c_double), ("edsp_recip", C.c_double),
-                ("edsp_self", C.c_double), ("dedsp", _dp), ("der", _dp)]
+                ("edsp_self", C.c_double), ("dedsp", _dp), ("der", _dp), ("vir_d", _dp), ("vir_r", _dp)]
 
 
 def taper_of(fx, key_cut: str, key_taper: str, default: float = 0.9) -> float:
@@ -282,12 +284,15 @@ def hippo_terms(fx, want_grad: bool = True, x=None) -> dict:
     n = int(fx["n"])
     o = HippoOut()
     dd, dr = np.zeros(3 * n), np.zeros(3 * n)
+    vd, vr = np.zeros(9), np.zeros(9)
     o.dedsp, o.der = dd.ctypes.data_as(_dp), dr.ctypes.data_as(_dp)
+    o.vir_d, o.vir_r = vd.ctypes.data_as(_dp), vr.ctypes.data_as(_dp)
     rc = L.orc_hippo(C.byref(s), 1 if want_grad else 0, C.byref(o))
     if rc != 0:
         raise RuntimeError(f"orc_hippo failed ({rc})")
     return {"edsp": o.edsp, "er": o.er, "edsp_real": o.edsp_real, "edsp_recip": o.edsp_recip,
-            "edsp_self": o.edsp_self, "dedsp": dd.reshape(n, 3), "der": dr.reshape(n, 3)}
+            "edsp_self": o.edsp_self, "dedsp": dd.reshape(n, 3), "der": dr.reshape(n, 3),
+            "vir_d": vd.reshape(3, 3), "vir_r": vr.reshape(3, 3)}
 
 
 def rep_series(r: float, ai: float, ak: float) -> np.ndarray:

@@ -135,6 +135,11 @@ typedef struct {
     int iters;              /* CG iterations (negative if not converged) */
     double *dem, *dep;      /* 3n gradients (may be NULL) */
     double *uind, *uinp;    /* 3n induced dipoles (may be NULL) */
+    /* virial of em and of ep, 9 each, row-major, the reference's convention
+     * (empole.cpp:310-320, torque.cpp:361-395: W_ab = sum_i x_ia dE/dx_ib,
+     * i.e. dE/d(strain) with positions and box strained); may be NULL;
+     * computed only with want_grad */
+    double *vir_m, *vir_p;
 } orc_elec_out;
 
 int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o);
@@ -162,6 +167,7 @@ typedef struct {
     double edsp, er;
     double edsp_real, edsp_recip, edsp_self;
     double *dedsp, *der;   /* 3n (may be NULL) */
+    double *vir_d, *vir_r; /* virial of edsp and of er, 9 each (may be NULL; with want_grad) */
 } orc_hippo_out;
 
 int orc_hippo(orc_hippo_sys *s, int want_grad, orc_hippo_out *o);

@@ -22,6 +22,20 @@
  * independence of the Ewald coefficient, a dense linear solve, central
  * finite differences of the energy, and the variational property.
  *
+ * Virial (W_ab = dE/d(strain_ab) with every position and the box strained
+ * and the local-frame multipoles carried by their frame atoms, symmetrized;
+ * the reference's convention of empole.cpp:310-320): real-space pairs give
+ * -(r_a g_b + r_b g_a)/2 with g the pair gradient on the first atom; the
+ * frame rotation gives torque.cpp:361-395's frame virial; the reciprocal
+ * energy of a distribution X, (f/2) sum_m G(m) |rho_X(m)|^2, gives
+ *   (f/2) sum_m G |rho_X|^2 [2 h_a h_b (pi^2/a^2 + 1/h^2) - delta_ab]
+ *   - f sum_j [d_jb phi_a + 2 sum_c q_jbc phi_ac]  (symmetrized),
+ * the second line because dipoles and quadrupoles keep their Cartesian
+ * components while the reciprocal vectors h strain (phi: X's reciprocal
+ * potential at site j); cross energies between two distributions A, B
+ * follow from E(A+B) - E(A) - E(B).  The self energies do not depend on
+ * the box.  Pinned by central differences of the energy under strain.
+ *
  * Energy definitions used (f = electric/dielec):
  *   E_mpole = real space [f (B_n - (1-m) rr_n) series] + reciprocal + self,
  *     self = -f a/sqrt(pi) sum_i [c^2 + (2a^2/3) d.d + (8a^4/5) q:q]
@@ -467,8 +481,40 @@ static double pme_volume(const ctx_t *c)
     return (double)c->K[0] * c->K[1] * c->K[2] / det;
 }
 
-/* phi = IDFT[G DFT[rho]] in place; returns sum_m G |rho^(m)|^2 */
-static double pme_convolve(const ctx_t *c, double *grid)
+/* a reciprocal potential and its derivatives at one site */
+typedef struct {
+    double phi, g[3], H[3][3], T[3][3][3];
+} sitepot;
+
+/* symmetrized pair virial of a pair gradient g (on the first atom) at
+ * separation r = x_k - x_i, times s (empole.cpp:310-320, pairMpole.h:166-173) */
+static void pair_vir(double *v, const double *r, const double *g, double s)
+{
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) v[3 * a + b] -= 0.5 * s * (r[a] * g[b] + r[b] * g[a]);
+}
+
+/* the strain term of the dipole and quadrupole of multipole mA (10, the
+ * reference's traceless x1/3 quadrupole) in a reciprocal potential pB,
+ * times s: -s sum [dA_b phi_a + 2 q_bc phi_ac], symmetrized */
+static void cterm_vir(double *v, const double *mA, const sitepot *pB, double s)
+{
+    const double Q[3][3] = {{mA[4], mA[5], mA[6]}, {mA[5], mA[7], mA[8]}, {mA[6], mA[8], mA[9]}};
+    double W[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            double w = mA[1 + b] * pB->g[a];
+            for (int c3 = 0; c3 < 3; c3++) w += 2.0 * Q[b][c3] * pB->H[a][c3];
+            W[a][b] = w;
+        }
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) v[3 * a + b] -= 0.5 * s * (W[a][b] + W[b][a]);
+}
+
+/* phi = IDFT[G DFT[rho]] in place; returns sum_m G |rho^(m)|^2 and, when
+ * vk is given, adds sum_m G |rho|^2 [2 h_a h_b (pi^2/a^2 + 1/h^2) - delta_ab]
+ * to vk (9) */
+static double pme_convolve(const ctx_t *c, double *grid, double *vk)
 {
     const int K0 = c->K[0], K1 = c->K[1], K2 = c->K[2];
     const size_t N = (size_t)K0 * K1 * K2;
@@ -489,7 +535,13 @@ static double pme_convolve(const ctx_t *c, double *grid)
                 const double h2 = dot3(h, h);
                 const double G = exp(-M_PI * M_PI * h2 / (c->a * c->a)) /
                                  (M_PI * V * h2 * c->bsmod[0][m1] * c->bsmod[1][m2] * c->bsmod[2][m3]);
-                e += G * (grid[id] * grid[id] + im[id] * im[id]);
+                const double t = G * (grid[id] * grid[id] + im[id] * im[id]);
+                e += t;
+                if (vk) {
+                    const double w = 2.0 * (M_PI * M_PI / (c->a * c->a) + 1.0 / h2);
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++) vk[3 * a + b] += t * (w * h[a] * h[b] - (a == b ? 1.0 : 0.0));
+                }
                 grid[id] *= G;
                 im[id] *= G;
             }
@@ -564,11 +616,8 @@ static void frac_to_cart(const ctx_t *c, const double *fp, double *phi, double *
 
 /* potential derivatives at every site from a set of Cartesian multipoles
  * (10 per atom); returns the reciprocal energy sum G|rho|^2 / 2 (no f) */
-typedef struct {
-    double phi, g[3], H[3][3], T[3][3][3];
-} sitepot;
 
-static double pme_potential(const ctx_t *c, const double *mpoles, sitepot *out)
+static double pme_potential(const ctx_t *c, const double *mpoles, sitepot *out, double *vk)
 {
     const int n = c->n;
     const size_t N = (size_t)c->K[0] * c->K[1] * c->K[2];
@@ -577,9 +626,11 @@ static double pme_potential(const ctx_t *c, const double *mpoles, sitepot *out)
     double *grid = (double *)malloc(N * sizeof(double));
     for (int i = 0; i < n; i++) cart_to_frac(c, mpoles + 10 * i, fm + 10 * i);
     pme_spread(c, fm, 10, grid);
-    const double e = 0.5 * pme_convolve(c, grid);
-    pme_gather(c, grid, 20, fp);
-    for (int i = 0; i < n; i++) frac_to_cart(c, fp + 20 * i, &out[i].phi, out[i].g, out[i].H, out[i].T);
+    const double e = 0.5 * pme_convolve(c, grid, vk);
+    if (out) {
+        pme_gather(c, grid, 20, fp);
+        for (int i = 0; i < n; i++) frac_to_cart(c, fp + 20 * i, &out[i].phi, out[i].g, out[i].H, out[i].T);
+    }
     free(fm); free(fp); free(grid);
     return e;
 }
@@ -754,7 +805,7 @@ static void mutual_field(ctx_t *c, const pairrec *L, long np, const double *ud, 
             double *F = set ? Fp : Fd;
             for (int i = 0; i < n; i++)
                 for (int a = 0; a < 3; a++) m[10 * i + 1 + a] = u[3 * i + a];
-            pme_potential(c, m, p);
+            pme_potential(c, m, p, NULL);
             for (int i = 0; i < n; i++)
                 for (int a = 0; a < 3; a++) F[3 * i + a] += -p[i].g[a] + self * u[3 * i + a];
         }
@@ -944,7 +995,6 @@ static int solve_dense(ctx_t *c, const pairrec *L, long np, const double *Ed, co
 /* ---------------------------------------------------------------------
  * driver
  * ------------------------------------------------------------------- */
-static void torque_to_grad(const orc_mpole_sys *mp, const double *te, double *de);
 
 int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
 {
@@ -988,6 +1038,8 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
     double *dem = (double *)calloc(3 * (size_t)n, sizeof(double));
     double *dep = (double *)calloc(3 * (size_t)n, sizeof(double));
     const double f = c.f;
+    double vm[9] = {0}, vp[9] = {0}, vkm[9] = {0};
+    const int want_vir = want_grad && (o->vir_m || o->vir_p);
 
     /* ---- permanent multipoles: real space */
     double em_real = 0.0;
@@ -1013,6 +1065,7 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
                 gpair g;
                 gpair_eval(R, A[part], B[part], q->r, want_grad, &g);
                 em_real += g.e;
+                if (want_vir) pair_vir(vm, q->r, g.frc, 1.0);
                 if (want_grad)
                     for (int a = 0; a < 3; a++) {
                         dem[3 * q->i + a] += g.frc[a];
@@ -1034,6 +1087,7 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
         gpair g;
         gpair_eval(R, c.rp + 10 * q->i, c.rp + 10 * q->k, q->r, want_grad, &g);
         em_real += g.e;
+        if (want_vir) pair_vir(vm, q->r, g.frc, 1.0);
         if (want_grad)
             for (int a = 0; a < 3; a++) {
                 dem[3 * q->i + a] += g.frc[a];
@@ -1047,7 +1101,11 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
     sitepot *pm = NULL;
     if (c.ewald) {
         pm = (sitepot *)malloc(n * sizeof(sitepot));
-        em_recip = f * pme_potential(&c, c.rp, pm);
+        em_recip = f * pme_potential(&c, c.rp, pm, want_vir ? vkm : NULL);
+        if (want_vir) {
+            for (int j = 0; j < 9; j++) vm[j] += 0.5 * f * vkm[j];
+            for (int i = 0; i < n; i++) cterm_vir(vm, c.rp + 10 * i, &pm[i], f);
+        }
         for (int i = 0; i < n; i++) {
             const double *m = c.rp + 10 * i;
             const double qq = m[4] * m[4] + m[7] * m[7] + m[9] * m[9] + 2.0 * (m[5] * m[5] + m[6] * m[6] + m[8] * m[8]);
@@ -1065,9 +1123,9 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
     o->em_real = em_real;
     o->em_recip = em_recip;
     o->em_self = em_self;
-    if (want_grad && o->dem) {
-        torque_to_grad(mp, te, dem);
-        memcpy(o->dem, dem, 3 * (size_t)n * sizeof(double));
+    if (want_grad && (o->dem || want_vir)) {
+        orc_torque(mp, te, dem, want_vir ? vm : NULL);
+        if (o->dem) memcpy(o->dem, dem, 3 * (size_t)n * sizeof(double));
     }
 
     /* ---- polarization */
@@ -1083,7 +1141,7 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
         double ep = 0.0;
         for (int j = 0; j < n3; j++) ep += ud[j] * Ep[j];
         o->ep = -0.5 * f * ep;
-        if (want_grad && o->dep) {
+        if (want_grad && (o->dep || want_vir)) {
             memset(te, 0, n3 * sizeof(double));
             /* real space: (1/2)[U_p(mu_d, M) + U_d(mu_p, M) + U_u(mu_d, mu_p) + U_u(mu_p, mu_d)] */
             for (long t = 0; t < np; t++) {
@@ -1127,6 +1185,7 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
                     gpair_eval(R, mdi, mpk, q->r, 1, &g1);
                     gpair_eval(R, mpi, mdk, q->r, 1, &g2);
                     for (int a = 0; a < 3; a++) frc[a] += g1.frc[a] + g2.frc[a];
+                    if (want_vir) pair_vir(vp, q->r, frc, 0.5 * f);
                     for (int a = 0; a < 3; a++) {
                         dep[3 * i + a] += 0.5 * f * frc[a];
                         dep[3 * k + a] -= 0.5 * f * frc[a];
@@ -1150,6 +1209,7 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
                 gpair_eval(R, mdi, mpk, q->r, 1, &g1);
                 gpair_eval(R, mpi, mdk, q->r, 1, &g2);
                 for (int a = 0; a < 3; a++) frc[a] += g1.frc[a] + g2.frc[a];
+                if (want_vir) pair_vir(vp, q->r, frc, 0.5 * f);
                 for (int a = 0; a < 3; a++) {
                     dep[3 * i + a] += 0.5 * f * frc[a];
                     dep[3 * k + a] -= 0.5 * f * frc[a];
@@ -1163,8 +1223,19 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
                 for (int i = 0; i < n; i++)
                     for (int a = 0; a < 3; a++) { md[10 * i + 1 + a] = ud[3 * i + a]; mq[10 * i + 1 + a] = up[3 * i + a]; }
                 sitepot *pd = (sitepot *)malloc(n * sizeof(sitepot)), *pq = (sitepot *)malloc(n * sizeof(sitepot));
-                pme_potential(&c, md, pd);
-                pme_potential(&c, mq, pq);
+                double vkd[9] = {0}, vkp[9] = {0}, vka[9] = {0};
+                pme_potential(&c, md, pd, want_vir ? vkd : NULL);
+                pme_potential(&c, mq, pq, want_vir ? vkp : NULL);
+                if (want_vir) {
+                    /* the reciprocal part of the polarization energy, (1/2) E(mu_d + mu_p, M)
+                     * + (1/2) E(mu_d, mu_p) as cross energies: its k-space sum is
+                     * (1/2)[V(M + mu_d + mu_p) - V(M) - V(mu_d) - V(mu_p)] */
+                    double *ma = (double *)malloc(10 * (size_t)n * sizeof(double));
+                    for (size_t j = 0; j < 10 * (size_t)n; j++) ma[j] = c.rp[j] + md[j] + mq[j];
+                    pme_potential(&c, ma, NULL, vka);
+                    free(ma);
+                    for (int j = 0; j < 9; j++) vp[j] += 0.25 * f * (vka[j] - vkm[j] - vkd[j] - vkp[j]);
+                }
                 const double self = 4.0 / 3.0 * c.a * c.a * c.a / SQRTPI;
                 for (int i = 0; i < n; i++) {
                     double g[3], tq[3];
@@ -1188,6 +1259,16 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
                     }
                     site_energy(c.rp + 10 * i, &ps, g, tq);
                     for (int a = 0; a < 3; a++) { dep[3 * i + a] += 0.5 * f * g[a]; te[3 * i + a] += 0.5 * f * tq[a]; }
+                    if (want_vir) {
+                        /* strain terms of the cross energies: (1/2)[C(mu_d + mu_p in phi_M)
+                         * + C(M in phi_(mu_d + mu_p)) + C(mu_d in phi_p) + C(mu_p in phi_d)] */
+                        double mu[10] = {0};
+                        for (int a = 0; a < 3; a++) mu[1 + a] = ud[3 * i + a] + up[3 * i + a];
+                        cterm_vir(vp, mu, &pm[i], 0.5 * f);
+                        cterm_vir(vp, c.rp + 10 * i, &ps, 0.5 * f);
+                        cterm_vir(vp, md + 10 * i, &pq[i], 0.5 * f);
+                        cterm_vir(vp, mq + 10 * i, &pd[i], 0.5 * f);
+                    }
                     /* self term: -(f/2) self (mu_d + mu_p) . d_i gives a torque on d_i */
                     double Eeff[3], tt[3];
                     for (int a = 0; a < 3; a++) Eeff[a] = 0.5 * f * self * (ud[3 * i + a] + up[3 * i + a]);
@@ -1196,21 +1277,22 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
                 }
                 free(md); free(mq); free(pd); free(pq);
             }
-            torque_to_grad(mp, te, dep);
-            memcpy(o->dep, dep, n3 * sizeof(double));
+            orc_torque(mp, te, dep, want_vir ? vp : NULL);
+            if (o->dep) memcpy(o->dep, dep, n3 * sizeof(double));
         }
         free(Ed); free(Ep);
         if (!o->uind) free(ud);
         if (!o->uinp) free(up);
     }
 
+    if (want_vir && o->vir_m) memcpy(o->vir_m, vm, sizeof vm);
+    if (want_vir && o->vir_p) memcpy(o->vir_p, vp, sizeof vp);
     free(L); free(te); free(dem); free(dep); free(pole); free(rp13); free(c.rp); free(pm);
     free(c.o12); free(c.o13); free(c.o14); free(c.o15); free(c.g11); free(c.g12); free(c.g13); free(c.g14);
     if (c.ewald) pme_free(&c);
     return 0;
 }
 
-static void torque_to_grad(const orc_mpole_sys *mp, const double *te, double *de) { orc_torque(mp, te, de, NULL); }
 
 /* =====================================================================
  * Part 3: HIPPO dispersion and Pauli repulsion
@@ -1342,8 +1424,11 @@ static void conn_scales(const ctx_t *c, int i, const double *vals, double *sc, i
     }
 }
 
-/* dispersion reciprocal: phi = IDFT[G DFT[rho]], returns sum_m G |rho|^2 */
-static double disp_convolve(const ctx_t *c, double *grid, double a)
+/* dispersion reciprocal: phi = IDFT[G DFT[rho]], returns sum_m G |rho|^2;
+ * with vk adds its strain derivative sum_m |rho|^2 dG/d(strain_ab): G =
+ * -psi(b)/(V bsmod), b = pi |h|/a, dV = V delta_ab, d|h| = -h_a h_b/|h|,
+ * psi'(b) = 3 pre b (sqrt(pi) b erfc(b) - exp(-b^2)) */
+static double disp_convolve(const ctx_t *c, double *grid, double a, double *vk)
 {
     const int K0 = c->K[0], K1 = c->K[1], K2 = c->K[2];
     const size_t N = (size_t)K0 * K1 * K2;
@@ -1362,8 +1447,16 @@ static double disp_convolve(const ctx_t *c, double *grid, double a)
                     h[j] = f1 * c->A[0][j] / K0 + f2 * c->A[1][j] / K1 + f3 * c->A[2][j] / K2;
                 const double b = M_PI * sqrt(dot3(h, h)) / a;
                 const double psi = pre * (SQRTPI * b * b * b * erfc(b) + (0.5 - b * b) * exp(-b * b));
-                const double G = -psi / (V * c->bsmod[0][m1] * c->bsmod[1][m2] * c->bsmod[2][m3]);
-                e += G * (grid[id] * grid[id] + im[id] * im[id]);
+                const double bs = c->bsmod[0][m1] * c->bsmod[1][m2] * c->bsmod[2][m3];
+                const double G = -psi / (V * bs);
+                const double rho2 = grid[id] * grid[id] + im[id] * im[id];
+                e += G * rho2;
+                if (vk) {
+                    const double w = 3.0 * pre * M_PI * M_PI / (a * a) * (SQRTPI * b * erfc(b) - exp(-b * b)) / (V * bs);
+                    for (int p = 0; p < 3; p++)
+                        for (int q = 0; q < 3; q++)
+                            vk[3 * p + q] += rho2 * (w * h[p] * h[q] - (p == q ? G : 0.0));
+                }
                 grid[id] *= G;
                 im[id] *= G;
             }
@@ -1390,6 +1483,8 @@ int orc_hippo(orc_hippo_sys *s, int want_grad, orc_hippo_out *o)
     double *dd = (double *)calloc(3 * (size_t)n, sizeof(double));
     double *dr = (double *)calloc(3 * (size_t)n, sizeof(double));
     double *te = (double *)calloc(3 * (size_t)n, sizeof(double));
+    double vd[9] = {0}, vr[9] = {0};
+    const int want_vir = want_grad && (o->vir_d || o->vir_r);
     /* ---- dispersion, real space */
     const double a = s->dewald, a2 = a * a;
     const double dvals[4] = {s->dsp2scale, s->dsp3scale, s->dsp4scale, s->dsp5scale};
@@ -1424,6 +1519,10 @@ int orc_hippo(orc_hippo_sys *s, int want_grad, orc_hippo_out *o)
                 de = de0 * T + e0 * dT;
             }
             ed_real += e;
+            if (want_vir) {
+                const double g[3] = {-de * r3[0] / r, -de * r3[1] / r, -de * r3[2] / r};
+                pair_vir(vd, r3, g, 1.0);
+            }
             if (want_grad)
                 for (int q = 0; q < 3; q++) {
                     dd[3 * k + q] += de * r3[q] / r;
@@ -1446,7 +1545,9 @@ int orc_hippo(orc_hippo_sys *s, int want_grad, orc_hippo_out *o)
         double *grid = (double *)malloc(N * sizeof(double));
         for (int i = 0; i < n; i++) fm[10 * i] = s->csix[i];
         pme_spread(&c, fm, 1, grid);
-        ed_recip = 0.5 * disp_convolve(&c, grid, a);
+        double vkd[9] = {0};
+        ed_recip = 0.5 * disp_convolve(&c, grid, a, want_vir ? vkd : NULL);
+        for (int j = 0; j < 9; j++) vd[j] += 0.5 * vkd[j];
         pme_gather(&c, grid, 4, fp);
         for (int i = 0; i < n; i++) {
             ed_self += a2 * a2 * a2 / 12.0 * s->csix[i] * s->csix[i];
@@ -1499,22 +1600,28 @@ int orc_hippo(orc_hippo_sys *s, int want_grad, orc_hippo_out *o)
             taper5(r, s->reptaper, s->repcut, &T, &dT);
             const double e0 = K * g.e / r;
             er += e0 * T;
-            if (want_grad)
+            if (want_grad) {
+                double fv[3];
                 for (int q = 0; q < 3; q++) {
                     /* d/dx_i of K M / r: K (dM/dx_i / r + M r_vec / r^3); taper d/dx_i = -T' r_vec / r */
                     const double f = T * K * (g.frc[q] / r + g.e * r3[q] / (r2 * r)) - e0 * dT * r3[q] / r;
+                    fv[q] = f;
                     dr[3 * i + q] += f;
                     dr[3 * k + q] -= f;
                     te[3 * i + q] += T * K * g.trqi[q] / r;
                     te[3 * k + q] += T * K * g.trqk[q] / r;
                 }
+                if (want_vir) pair_vir(vr, r3, fv, 1.0);
+            }
         }
         conn_scales(&c, i, rvals, sc, 1);
     }
     o->er = er;
     if (want_grad) {
-        orc_torque(mp, te, dr, NULL);
+        orc_torque(mp, te, dr, want_vir ? vr : NULL);
         if (o->dedsp) memcpy(o->dedsp, dd, 3 * (size_t)n * sizeof(double));
+        if (want_vir && o->vir_d) memcpy(o->vir_d, vd, sizeof vd);
+        if (want_vir && o->vir_r) memcpy(o->vir_r, vr, sizeof vr);
         if (o->der) memcpy(o->der, dr, 3 * (size_t)n * sizeof(double));
     }
     free(sc); free(dd); free(dr); free(te); free(pole); free(rp13);
