@@ -73,6 +73,8 @@ struct HippoArgs {
     double *de, *te;       // caller order gradient (and torque, repulsion)
     long long *deq, *teq;  // mixed form: fixed-point accumulators
     Totals *tot;
+    int vir;               // also the pair virial (tile scalars 1..9; totals for scaled pairs)
+    size_t tstride;        // scalar stride of tscal
 };
 
 template <class T>
@@ -134,6 +136,7 @@ __global__ __launch_bounds__(64) void k_disp_tiles(HippoArgs a)
         const int npair = tile_pair_list(a.mask[(size_t)t * 128 + lane], plist, lane);
         __syncthreads();
         double e = 0.0;
+        Vir6 vir;
         for (int base = 0; base < npair; base += 64) {
             if (base + lane >= npair) break;
             const int ent = plist[base + lane];
@@ -154,6 +157,7 @@ __global__ __launch_bounds__(64) void k_disp_tiles(HippoArgs a)
             atomicAdd(&acc[1][0][jl], g * (double)xr);
             atomicAdd(&acc[1][1][jl], g * (double)yr);
             atomicAdd(&acc[1][2][jl], g * (double)zr);
+            if (a.vir) vir.pair(xd, yd, zd, -g * (double)xr, -g * (double)yr, -g * (double)zr);
         }
         __syncthreads();
         T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
@@ -163,7 +167,11 @@ __global__ __launch_bounds__(64) void k_disp_tiles(HippoArgs a)
             Q[c * 64 + lane] = T(acc[1][c][lane]);
         }
         const double ew = wsum(e);
-        if (lane == 0) a.tscal[t] = ew; // scalar 0 of tile t
+        if (a.vir) vir.wave_sum();
+        if (lane == 0) {
+            a.tscal[t] = ew; // scalar 0 of tile t
+            if (a.vir) vir.store_tile(a.tscal, a.tstride, t);
+        }
     }
 }
 
@@ -192,6 +200,11 @@ __global__ __launch_bounds__(64) void k_disp_excl(HippoArgs a)
     grad_add(a.de, a.deq, 3 * q.k + 2, g * zr);
     grad_add(a.de, a.deq, 3 * q.i + 0, -g * xr); grad_add(a.de, a.deq, 3 * q.i + 1, -g * yr);
     grad_add(a.de, a.deq, 3 * q.i + 2, -g * zr);
+    if (a.vir) {
+        Vir6 v;
+        v.pair(xr, yr, zr, -g * xr, -g * yr, -g * zr);
+        a.tot->add_vir(v);
+    }
 }
 
 // repulsion pair at scale s: energy, dE/dx_i, torques
@@ -257,6 +270,7 @@ __global__ __launch_bounds__(64) void k_rep_tiles(HippoArgs a)
         const int npair = tile_pair_list(a.mask[(size_t)t * 128 + lane], plist, lane);
         __syncthreads();
         double e = 0.0;
+        Vir6 vir;
         for (int base = 0; base < npair; base += 64) {
             if (base + lane >= npair) break;
             const int ent = plist[base + lane];
@@ -282,6 +296,7 @@ __global__ __launch_bounds__(64) void k_rep_tiles(HippoArgs a)
                 atomicAdd(&acc[1][c][jl], -(double)f[c]);
                 atomicAdd(&acc[1][3 + c][jl], (double)tk[c]);
             }
+            if (a.vir) vir.pair(xd, yd, zd, (double)f[0], (double)f[1], (double)f[2]);
         }
         __syncthreads();
         T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
@@ -291,7 +306,11 @@ __global__ __launch_bounds__(64) void k_rep_tiles(HippoArgs a)
             Q[c * 64 + lane] = T(acc[1][c][lane]);
         }
         const double ew = wsum(e);
-        if (lane == 0) a.tscal[t] = ew; // scalar 0 of tile t
+        if (a.vir) vir.wave_sum();
+        if (lane == 0) {
+            a.tscal[t] = ew; // scalar 0 of tile t
+            if (a.vir) vir.store_tile(a.tscal, a.tstride, t);
+        }
     }
 }
 
@@ -319,6 +338,11 @@ __global__ __launch_bounds__(64) void k_rep_excl(HippoArgs a)
         grad_add(a.de, a.deq, 3 * q.k + c, -F[c]);
         grad_add(a.te, a.teq, 3 * q.i + c, Ti[c]);
         grad_add(a.te, a.teq, 3 * q.k + c, Tk[c]);
+    }
+    if (a.vir) {
+        Vir6 v;
+        v.pair(xr, yr, zr, F[0], F[1], F[2]);
+        a.tot->add_vir(v);
     }
 }
 
@@ -362,12 +386,15 @@ static HippoArgs hippo_args(pmq_ctx *c, double off2)
     a.nexcl = (int)c->h_excl.size();
     a.de = nullptr; a.te = nullptr; a.deq = nullptr; a.teq = nullptr;
     a.tot = c->totals.p;
+    a.vir = 0;
+    a.tstride = (size_t)c->tile_cap;
     return a;
 }
 
-int launch_disp(pmq_ctx *c, double off2, bool grad, bool mixed)
+int launch_disp(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir)
 {
     HippoArgs a = hippo_args(c, off2);
+    a.vir = vir ? 1 : 0;
     const bool ew = c->prm.use_dewald && c->box.kind != PMQ_BOX_NONE;
     a.dewald = ew ? c->prm.dewald : 0.0;
     if (!ew) a.taper = make_taper(c->prm.disptaper, sqrt(off2));
@@ -398,6 +425,7 @@ int launch_disp(pmq_ctx *c, double off2, bool grad, bool mixed)
         if ((rc = tile_reduce(c, r, 3, mixed, mixed))) return rc;
     }
     if ((rc = tile_scalars(c, 3, false))) return rc;
+    if (vir && (rc = tile_virial(c))) return rc;
     if (a.nexcl) hipLaunchKernelGGL(k_disp_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     if (mixed && grad && (rc = launch_fx_flush(c, c->fx_grad.p, c->dedsp.p, 3 * (size_t)c->n, PMQ_FX_GRAD)))
@@ -405,9 +433,10 @@ int launch_disp(pmq_ctx *c, double off2, bool grad, bool mixed)
     return PMQ_OK;
 }
 
-int launch_rep(pmq_ctx *c, double off2, bool grad, bool mixed)
+int launch_rep(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir)
 {
     HippoArgs a = hippo_args(c, off2);
+    a.vir = vir ? 1 : 0;
     a.taper = make_taper(c->prm.reptaper, sqrt(off2));
     a.de = c->der.p;
     a.te = c->ter.p;
@@ -433,6 +462,7 @@ int launch_rep(pmq_ctx *c, double off2, bool grad, bool mixed)
         if ((rc = tile_reduce(c, r, 6, mixed, mixed))) return rc;
     }
     if ((rc = tile_scalars(c, 2, false))) return rc;
+    if (vir && (rc = tile_virial(c))) return rc;
     if (a.nexcl) hipLaunchKernelGGL(k_rep_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     if (mixed && grad) {

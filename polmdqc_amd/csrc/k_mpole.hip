@@ -387,7 +387,7 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
 
         double e = 0; // energy sums stay FP64 whatever R is
         R ein = 0;
-        R vxx = 0, vxy = 0, vxz = 0, vyy = 0, vyz = 0, vzz = 0;
+        Vir6 vir; // FP64 whatever R is
         unsigned nem = 0;
         for (int base = 0; base < npair; base += PMQ_BLOCK) {
             if (base + lane >= npair) break;
@@ -441,14 +441,7 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
                 atomicAdd(&acc[1][4][jl], (double)o.tky);
                 atomicAdd(&acc[1][5][jl], (double)o.tkz);
                 if (CF) { atomicAdd(&acc[0][6][il], (double)o.phii); atomicAdd(&acc[1][6][jl], (double)o.phik); }
-                if (VIR) {
-                    vxx -= xr * o.fx;
-                    vxy -= R(0.5) * (yr * o.fx + xr * o.fy);
-                    vxz -= R(0.5) * (zr * o.fx + xr * o.fz);
-                    vyy -= yr * o.fy;
-                    vyz -= R(0.5) * (zr * o.fy + yr * o.fz);
-                    vzz -= zr * o.fz;
-                }
+                if (VIR) vir.pair(xd, yd, zd, (double)o.fx, (double)o.fy, (double)o.fz);
             }
         }
         __syncthreads();
@@ -468,19 +461,15 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
         }
         // totals in FP64 whatever R is
         const double ew = wave_sum(e);
-        double wxx = 0, wxy = 0, wxz = 0, wyy = 0, wyz = 0, wzz = 0, win = 0;
-        if (VIR) {
-            wxx = wave_sum((double)vxx); wxy = wave_sum((double)vxy); wxz = wave_sum((double)vxz);
-            wyy = wave_sum((double)vyy); wyz = wave_sum((double)vyz); wzz = wave_sum((double)vzz);
-        }
+        double win = 0;
+        if (VIR) vir.wave_sum();
         if (ANA) { win = wave_sum((double)ein); nem = wave_sum(nem); }
         if (lane == 0) {
             double *S = a.tscal + t;
             const size_t ts = a.tstride;
             S[0] = ew;
             if (VIR || ANA) { // the full set is summed when either is on
-                S[ts] = wxx; S[2 * ts] = wxy; S[3 * ts] = wxz; S[4 * ts] = wxy; S[5 * ts] = wyy; S[6 * ts] = wyz;
-                S[7 * ts] = wxz; S[8 * ts] = wyz; S[9 * ts] = wzz;
+                vir.store_tile(a.tscal, ts, t);
                 S[10 * ts] = win; S[11 * ts] = (double)nem;
             }
         }
@@ -555,14 +544,9 @@ __global__ __launch_bounds__(64) void k_mpole_excl(MpoleArgs a)
         grad_add(a.te, a.teq, 3 * q.k + 2, o.tkz);
         if (CF) { atomicAdd(&a.pot[q.i], (double)o.phii); atomicAdd(&a.pot[q.k], (double)o.phik); }
         if (VIR) {
-            const R vxy = -R(0.5) * (yr * o.fx + xr * o.fy);
-            const R vxz = -R(0.5) * (zr * o.fx + xr * o.fz);
-            const R vyz = -R(0.5) * (zr * o.fy + yr * o.fz);
-            atomicAdd(&a.tot->vir[0], -xr * o.fx); atomicAdd(&a.tot->vir[4], -yr * o.fy);
-            atomicAdd(&a.tot->vir[8], -zr * o.fz);
-            atomicAdd(&a.tot->vir[1], vxy); atomicAdd(&a.tot->vir[3], vxy);
-            atomicAdd(&a.tot->vir[2], vxz); atomicAdd(&a.tot->vir[6], vxz);
-            atomicAdd(&a.tot->vir[5], vyz); atomicAdd(&a.tot->vir[7], vyz);
+            Vir6 v;
+            v.pair(xr, yr, zr, o.fx, o.fy, o.fz);
+            a.tot->add_vir(v);
         }
     }
 }
@@ -574,9 +558,9 @@ static int launch_mpole_t(pmq_ctx *c, const MpoleArgs &a)
     {
         ProfScope ps_(c, "mpole_tiles");
         const bool ew = a.aewald > 0.0;
-        if (a.demq) { // mixed precision: ENERGY / GRADIENT only
-            if (ew) hipLaunchKernelGGL((k_mpole_tiles<float, PEN, false, false, GRAD, false, true>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
-            else hipLaunchKernelGGL((k_mpole_tiles<float, PEN, false, false, GRAD, false, false>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
+        if (a.demq) { // mixed precision: ENERGY, GRADIENT and VIRIAL (no analysis or charge flux)
+            if (ew) hipLaunchKernelGGL((k_mpole_tiles<float, PEN, false, false, GRAD, VIR, true>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
+            else hipLaunchKernelGGL((k_mpole_tiles<float, PEN, false, false, GRAD, VIR, false>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
         } else {
             if (ew) hipLaunchKernelGGL((k_mpole_tiles<double, PEN, CF, ANA, GRAD, VIR, true>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
             else hipLaunchKernelGGL((k_mpole_tiles<double, PEN, CF, ANA, GRAD, VIR, false>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);

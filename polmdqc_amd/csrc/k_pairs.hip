@@ -919,14 +919,15 @@ __global__ __launch_bounds__(64 * RED_W) void k_tile_reduce(TileReduceArgs a)
 // virial (1..9), the inter-molecular energy (10) and the pair count (11)
 constexpr int SCAL_TB = 1024;
 
-__global__ __launch_bounds__(SCAL_TB) void k_tile_scalars(const int *ntiles_p, int which, int full,
+// tile scalars k0..k1-1 of every tile into the totals (0 energy, 1..9 virial,
+// 10 inter-molecular energy, 11 pair count)
+__global__ __launch_bounds__(SCAL_TB) void k_tile_scalars(const int *ntiles_p, int which, int k0, int k1,
                                                           const double *__restrict__ tscal, size_t stride,
                                                           Totals *tot)
 {
     __shared__ double red[SCAL_TB];
     const int ntiles = *ntiles_p;
-    const int ns = full ? PMQ_TILE_NS : 1;
-    for (int k = 0; k < ns; k++) {
+    for (int k = k0; k < k1; k++) {
         // each thread's tiles in order (the unrolled loads go out together)
         double v = 0.0;
 #pragma unroll 8
@@ -971,8 +972,16 @@ int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fi
 
 int tile_scalars(pmq_ctx *c, int which, bool full)
 {
-    hipLaunchKernelGGL(k_tile_scalars, dim3(1), dim3(SCAL_TB), 0, c->stream, c->ntiles_eff.p, which, full ? 1 : 0,
-                       c->tscal.p, (size_t)c->tile_cap, c->totals.p);
+    hipLaunchKernelGGL(k_tile_scalars, dim3(1), dim3(SCAL_TB), 0, c->stream, c->ntiles_eff.p, which, 0,
+                       full ? PMQ_TILE_NS : 1, c->tscal.p, (size_t)c->tile_cap, c->totals.p);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+int tile_virial(pmq_ctx *c)
+{
+    hipLaunchKernelGGL(k_tile_scalars, dim3(1), dim3(SCAL_TB), 0, c->stream, c->ntiles_eff.p, 0, 1, 10, c->tscal.p,
+                       (size_t)c->tile_cap, c->totals.p);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }

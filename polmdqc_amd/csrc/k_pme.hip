@@ -161,6 +161,14 @@ int pme_setup(pmq_ctx *c)
                 G[id] = ge == ge2 ? ge : 0.5 * (ge + ge2);
                 if (dew) Gd[id] = gd == gd2 ? gd : 0.5 * (gd + gd2);
             }
+    {
+        std::vector<double> bsm;
+        for (int a = 0; a < 3; a++) bsm.insert(bsm.end(), bsmod[a].begin(), bsmod[a].end());
+        PMQ_HIP(c->pme_bsmod.alloc(bsm.size()));
+        PMQ_HIP(hipMemcpyAsync(c->pme_bsmod.p, bsm.data(), bsm.size() * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
+        PMQ_HIP(hipStreamSynchronize(c->stream)); // bsm is a host temporary
+    }
     PMQ_HIP(c->pme_g.alloc(G.size()));
     PMQ_HIP(hipMemcpyAsync(c->pme_g.p, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     std::vector<float> Gs(G.begin(), G.end());
@@ -775,7 +783,29 @@ struct MpolePostArgs {
     double *cphi;            // 10 * npad sorted (phi, g, H)
     double *fieldd, *fieldp; // 3 * npad sorted (+= reciprocal + self field)
     Totals *tot;
+    int vir;                 // the strain term of the dipoles and quadrupoles (with tot)
 };
+
+// the strain term of a dipole d and quadrupole Q (the reference's traceless
+// x1/3 form) in a reciprocal potential of gradient g and Hessian H: the
+// Cartesian components stay fixed while the reciprocal vectors strain,
+// which adds -s [d_b g_a + 2 Q_bc H_ac] (symmetrized) to the virial
+__device__ __forceinline__ void vir_cterm(Vir6 &v, const double *d, const double Q[3][3], const double *g,
+                                          const double H[3][3], double s)
+{
+    double W[3][3];
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+            W[p][q] = d[q] * g[p] + 2.0 * (Q[q][0] * H[p][0] + Q[q][1] * H[p][1] + Q[q][2] * H[p][2]);
+    v.xx -= s * W[0][0];
+    v.yy -= s * W[1][1];
+    v.zz -= s * W[2][2];
+    v.xy -= 0.5 * s * (W[0][1] + W[1][0]);
+    v.xz -= 0.5 * s * (W[0][2] + W[2][0]);
+    v.yz -= 0.5 * s * (W[1][2] + W[2][1]);
+}
 
 // permanent multipoles in their own reciprocal potential (one thread per
 // sorted atom): energy (f/2) U, gradient, torque, the self energy, and the
@@ -784,6 +814,7 @@ __global__ __launch_bounds__(256) void k_pme_mpole_post(MpolePostArgs a)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     double e = 0.0;
+    Vir6 vir;
     if (s < a.n) {
         double fp[20];
         for (int j = 0; j < 20; j++) fp[j] = a.frac[(size_t)s * 20 + j];
@@ -802,6 +833,7 @@ __global__ __launch_bounds__(256) void k_pme_mpole_post(MpolePostArgs a)
         const double dd = m[1] * m[1] + m[2] * m[2] + m[3] * m[3];
         const double qq = m[4] * m[4] + m[7] * m[7] + m[9] * m[9] + 2.0 * (m[5] * m[5] + m[6] * m[6] + m[8] * m[8]);
         e = 0.5 * a.f * U - a.f * al / 1.77245385090551602730 * (m[0] * m[0] + 2.0 * a2 / 3.0 * dd + 1.6 * a2 * a2 * qq);
+        if (a.vir && a.tot) vir_cterm(vir, m + 1, Q, gr, Hm, a.f);
         a.cphi[s] = phi;
         for (int q = 0; q < 3; q++) a.cphi[(1 + q) * a.npad + s] = gr[q];
         for (int q = 0; q < 6; q++) a.cphi[(4 + q) * a.npad + s] = H[q];
@@ -837,6 +869,10 @@ __global__ __launch_bounds__(256) void k_pme_mpole_post(MpolePostArgs a)
     if (a.tot) {
         for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
         if ((threadIdx.x & 63) == 0) atomicAdd(&a.tot->em, e);
+        if (a.vir) {
+            vir.wave_sum();
+            if ((threadIdx.x & 63) == 0) a.tot->add_vir(vir);
+        }
     }
 }
 
@@ -881,16 +917,32 @@ struct PolGradPostArgs {
     const double *frac;          // n x 2 x 20: potentials of mu_d, mu_p
     double f, self;
     double *dep, *tep;           // caller order
+    int vir;                     // the strain terms of the cross energies into tot
+    Totals *tot;
 };
 
 // reciprocal-space polarization gradient (one thread per sorted atom):
 //   (f/2) [(mu_d+mu_p).H_M + mu_d.H_p + mu_p.H_d] on the induced dipoles,
 //   (f/2) gradient and torque of the permanent multipoles in phi_d + phi_p,
 //   and the self-term torque d x (f/2) self (mu_d + mu_p)
+__device__ __forceinline__ void polgrad_site(const PolGradPostArgs &a, int s, Vir6 &vir);
+
 __global__ __launch_bounds__(256) void k_pme_polgrad_post(PolGradPostArgs a)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.n) return;
+    Vir6 vir;
+    if (s < a.n) polgrad_site(a, s, vir);
+    if (a.vir) {
+        vir.wave_sum();
+        if ((threadIdx.x & 63) == 0) a.tot->add_vir(vir);
+    }
+}
+
+// one site of k_pme_polgrad_post; with a.vir also the strain terms of the
+// reciprocal cross energies (1/2)[E(mu_d + mu_p, M) + E(mu_d, mu_p)]:
+// (f/2) [C(mu in phi_M) + C(M in phi_mu) + C(mu_d in phi_p) + C(mu_p in phi_d)]
+__device__ __forceinline__ void polgrad_site(const PolGradPostArgs &a, int s, Vir6 &vir)
+{
     // the transform is linear: phi_d + phi_p in one pass, H_d on its own
     double fs[20], fd[10];
     for (int j = 0; j < 20; j++) fs[j] = a.frac[(size_t)s * 40 + j] + a.frac[(size_t)s * 40 + 20 + j];
@@ -940,10 +992,132 @@ __global__ __launch_bounds__(256) void k_pme_polgrad_post(PolGradPostArgs a)
                           m[3] * E[0] - m[1] * E[2] - 2.0 * (QH[2][0] - QH[0][2]),
                           m[1] * E[1] - m[2] * E[0] - 2.0 * (QH[0][1] - QH[1][0])};
     for (int x = 0; x < 3; x++) a.tep[3 * i + x] += hf * tq[x];
+    if (a.vir) {
+        const double gM[3] = {a.cphi[np + s], a.cphi[2 * np + s], a.cphi[3 * np + s]};
+        const double gp[3] = {g[0] - gd[0], g[1] - gd[1], g[2] - gd[2]};
+        const double mu[3] = {ud[0] + up[0], ud[1] + up[1], ud[2] + up[2]};
+        const double Z[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        vir_cterm(vir, mu, Z, gM, Z, hf);
+        vir_cterm(vir, m + 1, Q, g, H, hf);
+        vir_cterm(vir, ud, Z, gp, Z, hf);
+        vir_cterm(vir, up, Z, gd, Z, hf);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// VIRIAL: k-space sum of a reciprocal energy, (coef) sum_m G |rho(m)|^2
+// [2 h_a h_b (pi^2/a^2 + 1/h^2) - delta_ab] for the electrostatic kernel
+// G = exp(-pi^2 h^2/a^2)/(pi V h^2 bsmod); for dispersion (G = -psi(b)/(V
+// bsmod), b = pi |h|/a) sum_m |rho|^2 [-G delta_ab + 3 pre (pi/a)^2
+// (sqrt(pi) b erfc(b) - exp(-b^2)) h_a h_b / (V bsmod)].  One thread per
+// half-complex entry: interior columns stand for m and -m (weight 2), and
+// every entry takes the mean of the weight at m and at -m, as the energy's
+// kernel table does (they differ where a component sits at K/2 in a skewed
+// cell).
+struct VirKArgs {
+    PmeGeom g;
+    int H0;
+    double a;              // Ewald coefficient of this kernel
+    double V;
+    const double *bsmod;   // K0, K1, K2 moduli
+    const double *cg;      // nset complex half grids (interleaved re, im)
+    size_t cstride;        // complex entries per set
+    int nset;
+    double coef[2];        // weight of each set (energy prefactor included)
+    int disp;
+    Totals *tot;
+};
+
+__device__ __forceinline__ void virk_weight(const VirKArgs &a, int m1, int m2, int m3, double W[6])
+{
+    const int K[3] = {a.g.K0, a.g.K1, a.g.K2};
+    const int m[3] = {m1, m2, m3};
+    double h[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int f = m[d] > K[d] / 2 ? m[d] - K[d] : m[d];
+#pragma unroll
+        for (int j = 0; j < 3; j++) h[j] += f * a.g.A[d][j] / K[d];
+    }
+    const double h2 = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
+    const double bs = a.bsmod[m1] * a.bsmod[K[0] + m2] * a.bsmod[K[0] + K[1] + m3];
+    double diag, off;
+    const double pi = 3.14159265358979323846, sqrtpi = 1.77245385090551602730;
+    if (!a.disp) {
+        if (h2 == 0.0) {
+            for (int q = 0; q < 6; q++) W[q] = 0.0;
+            return;
+        }
+        const double G = exp(-pi * pi * h2 / (a.a * a.a)) / (pi * a.V * h2 * bs);
+        diag = -G;
+        off = G * 2.0 * (pi * pi / (a.a * a.a) + 1.0 / h2);
+    } else {
+        const double pre = 2.0 * pi * sqrtpi * a.a * a.a * a.a / 3.0;
+        const double b = pi * sqrt(h2) / a.a;
+        const double G = -pre * (sqrtpi * b * b * b * erfc(b) + (0.5 - b * b) * exp(-b * b)) / (a.V * bs);
+        diag = -G;
+        off = 3.0 * pre * pi * pi / (a.a * a.a) * (sqrtpi * b * erfc(b) - exp(-b * b)) / (a.V * bs);
+    }
+    W[0] = off * h[0] * h[0] + diag; W[1] = off * h[0] * h[1]; W[2] = off * h[0] * h[2];
+    W[3] = off * h[1] * h[1] + diag; W[4] = off * h[1] * h[2]; W[5] = off * h[2] * h[2] + diag;
+}
+
+__global__ __launch_bounds__(256) void k_pme_vir_ksum(VirKArgs a)
+{
+    const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    Vir6 v; // accumulated with the sign of a strain derivative (added to the virial as is)
+    if (id < a.cstride) {
+        const int m1 = (int)(id % a.H0), m2 = (int)((id / a.H0) % a.g.K1), m3 = (int)(id / ((size_t)a.H0 * a.g.K1));
+        double rho2 = 0.0;
+        for (int set = 0; set < a.nset; set++) {
+            const double re = a.cg[2 * (set * a.cstride + id)], im = a.cg[2 * (set * a.cstride + id) + 1];
+            rho2 += a.coef[set] * (re * re + im * im);
+        }
+        double Wp[6], Wm[6];
+        virk_weight(a, m1, m2, m3, Wp);
+        virk_weight(a, (a.g.K0 - m1) % a.g.K0, (a.g.K1 - m2) % a.g.K1, (a.g.K2 - m3) % a.g.K2, Wm);
+        const double wt = (m1 == 0 || 2 * m1 == a.g.K0) ? 1.0 : 2.0;
+        const double s = 0.5 * wt * rho2;
+        v.xx = s * (Wp[0] + Wm[0]); v.xy = s * (Wp[1] + Wm[1]); v.xz = s * (Wp[2] + Wm[2]);
+        v.yy = s * (Wp[3] + Wm[3]); v.yz = s * (Wp[4] + Wm[4]); v.zz = s * (Wp[5] + Wm[5]);
+    }
+    v.wave_sum();
+    __shared__ double red[4][6];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[w][0] = v.xx; red[w][1] = v.xy; red[w][2] = v.xz; red[w][3] = v.yy; red[w][4] = v.yz; red[w][5] = v.zz;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Vir6 t;
+        t.xx = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+        t.xy = red[0][1] + red[1][1] + red[2][1] + red[3][1];
+        t.xz = red[0][2] + red[1][2] + red[2][2] + red[3][2];
+        t.yy = red[0][3] + red[1][3] + red[2][3] + red[3][3];
+        t.yz = red[0][4] + red[1][4] + red[2][4] + red[3][4];
+        t.zz = red[0][5] + red[1][5] + red[2][5] + red[3][5];
+        a.tot->add_vir(t);
+    }
+}
+
+// combined source of the polarization virial: the permanent multipoles
+// (total charges with charge penetration) plus both induced-dipole sets
+__global__ void k_pme_vir_src(int npad, const double *smp, const double *charge, const double *uind, double *out)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= npad) return;
+    for (int c = 0; c < 10; c++) {
+        double v = smp[(size_t)c * npad + s];
+        if (c == 0 && charge) v = charge[s];
+        if (c >= 1 && c <= 3) v += uind[(size_t)(c - 1) * npad + s] + uind[(size_t)(c + 2) * npad + s];
+        out[(size_t)c * npad + s] = v;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // launchers
+static int vir_ksum(pmq_ctx *c, int nset, double coef0, double coef1, bool disp, double alpha);
+
 static int fft(pmq_ctx *c, bool forward, int nset, bool fp32)
 {
     rocfft_plan pl = fp32 ? (rocfft_plan)(forward ? (nset == 1 ? c->plan_f1s : c->plan_f2s)
@@ -1176,6 +1350,7 @@ int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar)
     args.dem = c->dem.p; args.te = c->te.p;
     args.cphi = c->cphi.p; args.fieldd = c->pfield.p; args.fieldp = c->pfield.p + 3 * (size_t)c->npad;
     args.tot = mpole ? c->totals.p : nullptr;
+    args.vir = mode == PMQ_VIRIAL ? 1 : 0;
     {
         ProfScope ps_(c, "pme_gather");
         hipLaunchKernelGGL(k_pme_mpole_post, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, args);
@@ -1249,7 +1424,7 @@ __global__ __launch_bounds__(256) void k_pme_disp_gather(DispPostArgs a)
 
 // need_bin: the atoms are not yet binned by brick in this evaluation (no
 // multipole PME ran before)
-int pme_disp(pmq_ctx *c, bool grad, bool need_bin)
+int pme_disp(pmq_ctx *c, bool grad, bool need_bin, bool vir)
 {
     int rc;
     if ((rc = choose_fft(c))) return rc;
@@ -1271,10 +1446,17 @@ int pme_disp(pmq_ctx *c, bool grad, bool need_bin)
         PMQ_ORDER_DISPATCH(k_pme_disp_gather, )
     }
     PMQ_HIP(hipGetLastError());
+    if (vir) {
+        // the k-space sum of (1/2) sum_m G_d |rho|^2, from a second spreading
+        // of the C6 coefficients (dispersion grids are FP64 in both forms)
+        if ((rc = spread(c, 2, c->shp.p, nullptr, 1))) return rc;
+        if ((rc = fft(c, true, 1, false))) return rc;
+        if ((rc = vir_ksum(c, 1, 0.5, 0.0, true, c->prm.dewald))) return rc;
+    }
     return PMQ_OK;
 }
 
-int pme_polar_grad(pmq_ctx *c)
+int pme_polar_grad(pmq_ctx *c, bool vir)
 {
     int rc;
     const size_t gs = grid_size(c);
@@ -1293,12 +1475,80 @@ int pme_polar_grad(pmq_ctx *c)
     args.self = 4.0 / 3.0 * c->prm.aewald * c->prm.aewald * c->prm.aewald / 1.77245385090551602730;
     args.dep = c->dep.p;
     args.tep = c->tep.p;
+    args.vir = vir ? 1 : 0;
+    args.tot = c->totals.p;
     {
         ProfScope ps_(c, "pme_gather");
         hipLaunchKernelGGL(k_pme_polgrad_post, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, args);
     }
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
+}
+
+static double cell_volume(const pmq_ctx *c)
+{
+    const double (*A)[3] = c->pmeA;
+    const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0])
+                     + A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+    return (double)c->K[0] * c->K[1] * c->K[2] / det;
+}
+
+// the k-space virial sum of the nset transformed grids in cgrid (FP64)
+static int vir_ksum(pmq_ctx *c, int nset, double coef0, double coef1, bool disp, double alpha)
+{
+    VirKArgs k;
+    k.g = geom(c);
+    k.H0 = c->K[0] / 2 + 1;
+    k.a = alpha;
+    k.V = cell_volume(c);
+    k.bsmod = c->pme_bsmod.p;
+    k.cg = c->cgrid.p;
+    k.cstride = (size_t)k.H0 * c->K[1] * c->K[2];
+    k.nset = nset;
+    k.coef[0] = coef0;
+    k.coef[1] = coef1;
+    k.disp = disp ? 1 : 0;
+    k.tot = c->totals.p;
+    hipLaunchKernelGGL(k_pme_vir_ksum, dim3((unsigned)((k.cstride + 255) / 256)), dim3(256), 0, c->stream, k);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+// VIRIAL, after the solve: the k-space sums of the reciprocal multipole
+// energy (f/2) S(M) and of the polarization's cross energies
+// (1/2)[E(mu_d + mu_p, M) + E(mu_d, mu_p)] = (f/4)[S(M + mu_d + mu_p) - S(M)
+// - S(mu_d) - S(mu_p)], S(X) the k-space sum of X's grid.  Grids and
+// transforms in FP64 whatever the precision form (the virial is an FP64
+// quantity of both forms); the atoms are binned by brick already.
+int pme_virial(pmq_ctx *c, bool mpole, bool polar)
+{
+    int rc;
+    const bool keep32 = c->pme_fp32;
+    c->pme_fp32 = false;
+    const double f = c->prm.electric / c->prm.dielec, a = c->prm.aewald;
+    const double *charge = c->prm.pentype != PMQ_PEN_NONE ? c->scp.p + 2 * (size_t)c->npad : nullptr;
+    const double cm = (mpole ? 1.0 : 0.0) - (polar ? 0.5 : 0.0);
+    rc = PMQ_OK;
+    if (cm != 0.0 && !rc) {
+        if (!(rc = spread(c, 0, c->smp.p, charge, 1)) && !(rc = fft(c, true, 1, false)))
+            rc = vir_ksum(c, 1, 0.5 * f * cm, 0.0, false, a);
+    }
+    if (polar && !rc) {
+        const int np = c->npad;
+        if (c->pme_vsrc.alloc(10 * (size_t)np) != hipSuccess) {
+            c->err = "PME virial: out of device memory";
+            rc = PMQ_ERR_HIP;
+        } else {
+            hipLaunchKernelGGL(k_pme_vir_src, dim3((np + 255) / 256), dim3(256), 0, c->stream, np, c->smp.p, charge,
+                               c->uind.p, c->pme_vsrc.p);
+            if (!(rc = spread(c, 0, c->pme_vsrc.p, nullptr, 1)) && !(rc = fft(c, true, 1, false)))
+                rc = vir_ksum(c, 1, 0.25 * f, 0.0, false, a);
+            if (!rc && !(rc = spread(c, 1, c->uind.p, nullptr, 2)) && !(rc = fft(c, true, 2, false)))
+                rc = vir_ksum(c, 2, -0.25 * f, -0.25 * f, false, a);
+        }
+    }
+    c->pme_fp32 = keep32;
+    return rc;
 }
 
 } // namespace pmq

@@ -60,6 +60,10 @@ struct PolArgs {
     long long *dstq;     // mixed precision: fixed-point fields (else null)
     void *part;          // per tile partial sums (2 sides x PMQ_TILE_NV x 64 of the pair arithmetic type)
     long long *depq, *tepq; // mixed precision: fixed-point gradient and torque (else null)
+    int vir;             // gradient kernels: also the pair virial (tile scalars 1..9; totals for scaled pairs)
+    double *tscal;
+    size_t tstride;
+    Totals *tot;
 };
 
 __device__ __forceinline__ void field_add(double *d, long long *q, size_t i, double v)
@@ -378,6 +382,7 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
     const double hf = 0.5 * a.f;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 IJ = a.tiles[t];
+        Vir6 vir;
         __syncthreads();
 #pragma unroll
         for (int side = 0; side < 2; side++) {
@@ -427,6 +432,7 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
                 atomicAdd(&acc[1][c][jl], -(double)f3[c]);
                 atomicAdd(&acc[1][3 + c][jl], (double)tk[c]);
             }
+            if (a.vir) vir.pair(xd, yd, zd, hf * (double)f3[0], hf * (double)f3[1], hf * (double)f3[2]);
         }
         __syncthreads();
         // partial sums: side 0 the i atoms, side 1 the j atoms
@@ -435,6 +441,10 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
         for (int c = 0; c < 6; c++) {
             P[c * 64 + lane] = T(hf * acc[0][c][lane]);
             Q[c * 64 + lane] = T(hf * acc[1][c][lane]);
+        }
+        if (a.vir) {
+            vir.wave_sum();
+            if (lane == 0) vir.store_tile(a.tscal, a.tstride, t);
         }
     }
 }
@@ -491,6 +501,11 @@ __global__ __launch_bounds__(64) void k_pol_grad_excl(PolArgs a)
         grad_add(a.dep, a.depq, 3 * q.k + c, -hf * F[c]);
         grad_add(a.tep, a.tepq, 3 * q.i + c, hf * Ti[c]);
         grad_add(a.tep, a.tepq, 3 * q.k + c, hf * Tk[c]);
+    }
+    if (a.vir) {
+        Vir6 v;
+        v.pair(xr, yr, zr, hf * F[0], hf * F[1], hf * F[2]);
+        a.tot->add_vir(v);
     }
 }
 
@@ -762,7 +777,7 @@ __global__ __launch_bounds__(256) void k_pol_finish(int n, int npad, const int *
 // ---------------------------------------------------------------------------
 // launchers
 int pme_dipole_field(pmq_ctx *c, const double *dip, double *field, const double *skip);
-int pme_polar_grad(pmq_ctx *c);
+int pme_polar_grad(pmq_ctx *c, bool vir);
 int launch_zero(pmq_ctx *c, double *p, size_t n);
 
 static PolArgs pol_args(pmq_ctx *c, double off2)
@@ -784,6 +799,10 @@ static PolArgs pol_args(pmq_ctx *c, double off2)
     a.src = nullptr; a.dst = nullptr; a.dep = c->dep.p; a.tep = c->tep.p;
     a.dstq = nullptr; a.depq = nullptr; a.tepq = nullptr;
     a.part = c->tpart.p;
+    a.vir = 0;
+    a.tscal = c->tscal.p;
+    a.tstride = (size_t)c->tile_cap;
+    a.tot = c->totals.p;
     return a;
 }
 
@@ -917,10 +936,11 @@ int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
     return PMQ_OK;
 }
 
-int polar_gradient(pmq_ctx *c, double off2, bool mixed)
+int polar_gradient(pmq_ctx *c, double off2, bool mixed, bool vir)
 {
     PolArgs a = pol_args(c, off2);
     a.src = c->uind.p;
+    a.vir = vir ? 1 : 0;
     if (mixed) {
         a.depq = c->fx_grad.p;
         a.tepq = c->fx_grad.p + 3 * (size_t)c->n;
@@ -953,9 +973,10 @@ int polar_gradient(pmq_ctx *c, double off2, bool mixed)
     }
     int rc;
     if ((rc = tile_reduce(c, r, 6, mixed, mixed))) return rc;
+    if (vir && (rc = tile_virial(c))) return rc;
     if (a.nexcl) hipLaunchKernelGGL(k_pol_grad_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
-    if (a.aewald > 0) return pme_polar_grad(c);
+    if (a.aewald > 0) return pme_polar_grad(c, vir);
     return PMQ_OK;
 }
 

@@ -598,8 +598,8 @@ static int check_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
         return PMQ_ERR_ARG;
     }
     const bool mixed = precision == PMQ_MIXED;
-    if (mixed && (mode == PMQ_ANALYSIS || mode == PMQ_VIRIAL || c->prm.use_chgflx)) {
-        c->err = "pmq_compute: the mixed form computes energy and gradient without charge flux";
+    if (mixed && (mode == PMQ_ANALYSIS || c->prm.use_chgflx)) {
+        c->err = "pmq_compute: the mixed form computes energy, gradient and virial without charge flux";
         return PMQ_ERR_UNSUPPORTED;
     }
     if (terms & ~(unsigned)(PMQ_TERM_MPOLE | PMQ_TERM_POLAR | PMQ_TERM_REPEL | PMQ_TERM_DISP)) {
@@ -612,8 +612,8 @@ static int check_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
         c->err = "pmq_compute: repulsion or dispersion requested but its parameters were not set";
         return PMQ_ERR_STATE;
     }
-    if ((do_rep || do_disp) && (mode == PMQ_VIRIAL || mode == PMQ_ANALYSIS)) {
-        c->err = "pmq_compute: repulsion and dispersion compute energy and gradient (no virial or analysis)";
+    if ((do_rep || do_disp) && mode == PMQ_ANALYSIS) {
+        c->err = "pmq_compute: repulsion and dispersion compute energy, gradient and virial (no analysis)";
         return PMQ_ERR_UNSUPPORTED;
     }
     if (do_rep || do_disp) {
@@ -635,12 +635,12 @@ static int check_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
         c->err = "pmq_compute: HIPPO polarization damping is built for the Gordon1 charge penetration";
         return PMQ_ERR_UNSUPPORTED;
     }
-    if (ewald && (mode == PMQ_VIRIAL || mode == PMQ_ANALYSIS)) {
-        c->err = "pmq_compute: the Ewald path computes energy and gradient (no virial or analysis)";
+    if (ewald && mode == PMQ_ANALYSIS) {
+        c->err = "pmq_compute: the Ewald path computes energy, gradient and virial (no analysis)";
         return PMQ_ERR_UNSUPPORTED;
     }
-    if (do_polar && (mode == PMQ_VIRIAL || mode == PMQ_ANALYSIS)) {
-        c->err = "pmq_compute: polarization computes energy and gradient (no virial or analysis)";
+    if (do_polar && mode == PMQ_ANALYSIS) {
+        c->err = "pmq_compute: polarization computes energy, gradient and virial (no analysis)";
         return PMQ_ERR_UNSUPPORTED;
     }
     return PMQ_OK;
@@ -660,6 +660,8 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
     const int n = c->n;
     int rc;
     const bool grad = mode == PMQ_GRADIENT || mode == PMQ_VIRIAL;
+    const bool vir = mode == PMQ_VIRIAL;
+    const int fmode = vir ? PMQ_VIRIAL : PMQ_GRADIENT; // the mode of the torque (frame) conversions
     {
         // the evaluation's accumulators, cleared by one launch
         double *zp[PMQ_ZERO_MAX];
@@ -710,12 +712,12 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
         if ((do_rep || do_disp) && (rc = hippo_sort(c))) return rc;
         if (do_rep) {
             const double rc2 = std::min(c->prm.repcut * c->prm.repcut, off2);
-            if ((rc = launch_rep(c, rc2, grad, mixed))) return rc;
-            if (grad && (rc = launch_torque(c, PMQ_GRADIENT, c->ter.p, c->der.p, mixed))) return rc;
+            if ((rc = launch_rep(c, rc2, grad, mixed, vir))) return rc;
+            if (grad && (rc = launch_torque(c, fmode, c->ter.p, c->der.p, mixed))) return rc;
         }
         if (do_disp) {
             const double dc2 = std::min(c->prm.dispcut * c->prm.dispcut, off2);
-            if ((rc = launch_disp(c, dc2, grad, mixed))) return rc;
+            if ((rc = launch_disp(c, dc2, grad, mixed, vir))) return rc;
         }
         if (do_polar && (rc = polar_prepare(c, off2, mixed, safe))) return rc;
         if (do_mpole && (rc = launch_mpole_direct(c, mode, off2, mixed))) return rc;
@@ -724,12 +726,12 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
         }
         if (pme) {
             if (ewald && (do_mpole || do_polar) && (rc = pme_mpole(c, mode, do_mpole, do_polar))) return rc;
-            if (dewald && (rc = pme_disp(c, grad, !(ewald && (do_mpole || do_polar))))) return rc;
+            if (dewald && (rc = pme_disp(c, grad, !(ewald && (do_mpole || do_polar)), vir))) return rc;
         }
         if (do_polar) {
             if ((rc = polar_solve(c, safe, &c->pcg_iters))) return rc;
             if (grad) {
-                if ((rc = polar_gradient(c, off2, mixed))) return rc;
+                if ((rc = polar_gradient(c, off2, mixed, vir))) return rc;
                 if (mixed) {
                     if ((rc = launch_fx_flush_pair(c, c->fx_grad.p, c->dep.p, c->tep.p, 3 * (size_t)n, PMQ_FX_GRAD)))
                         return rc;
@@ -737,13 +739,15 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
             }
             if ((rc = polar_finish(c, c->uind_out.p, c->uind_out.p + 3 * (size_t)n))) return rc;
         }
-        // the torques of both terms in one launch (the polarization one adds
-        // no virial here)
+        // the k-space virial sums of the reciprocal energies (after the solve)
+        if (vir && ewald && (do_mpole || do_polar) && (rc = pme_virial(c, do_mpole, do_polar))) return rc;
+        // the torques of both terms in one launch (each adds its frame virial
+        // in VIRIAL mode)
         if (grad && do_mpole && do_polar) {
-            if ((rc = launch_torque_pair(c, mode, c->te.p, c->dem.p, PMQ_GRADIENT, c->tep.p, c->dep.p, mixed)))
+            if ((rc = launch_torque_pair(c, mode, c->te.p, c->dem.p, fmode, c->tep.p, c->dep.p, mixed)))
                 return rc;
         } else if (grad && do_polar) {
-            if ((rc = launch_torque(c, PMQ_GRADIENT, c->tep.p, c->dep.p, mixed))) return rc;
+            if ((rc = launch_torque(c, fmode, c->tep.p, c->dep.p, mixed))) return rc;
         } else if (grad && do_mpole) {
             if ((rc = launch_torque(c, mode, c->te.p, c->dem.p, mixed))) return rc;
         }

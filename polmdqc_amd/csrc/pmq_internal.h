@@ -150,6 +150,39 @@ struct HostStatus {       // pinned host mirror, written by the evaluation's las
     double sc[11];        // PCG scalars (8 = converged, 9 = iterations)
 };
 
+// Symmetrized pair virial, the reference's convention (empole.cpp:310-320,
+// pairMpole.h:166-173): for a pair gradient g on the first atom at
+// separation r = x_k - x_i, W_ab -= (r_a g_b + r_b g_a) / 2.  Six FP64
+// accumulators whatever the pair arithmetic is.
+struct Vir6 {
+    double xx = 0, xy = 0, xz = 0, yy = 0, yz = 0, zz = 0;
+    __device__ __forceinline__ void pair(double x, double y, double z, double gx, double gy, double gz)
+    {
+        xx -= x * gx;
+        xy -= 0.5 * (y * gx + x * gy);
+        xz -= 0.5 * (z * gx + x * gz);
+        yy -= y * gy;
+        yz -= 0.5 * (z * gy + y * gz);
+        zz -= z * gz;
+    }
+    // sums of the 64 lanes of a wavefront, in every lane
+    __device__ __forceinline__ void wave_sum()
+    {
+        double *v[6] = {&xx, &xy, &xz, &yy, &yz, &zz};
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) *v[q] += __shfl_xor(*v[q], o, 64);
+    }
+    // tile scalars 1..9 (row-major 3x3) of tile t, scalar q at tscal[q * ts + t]
+    __device__ __forceinline__ void store_tile(double *tscal, size_t ts, int t) const
+    {
+        double *S = tscal + t;
+        S[ts] = xx; S[2 * ts] = xy; S[3 * ts] = xz; S[4 * ts] = xy; S[5 * ts] = yy; S[6 * ts] = yz;
+        S[7 * ts] = xz; S[8 * ts] = yz; S[9 * ts] = zz;
+    }
+};
+
 // per-tile partial sums of the pair kernels (k_pairs.hip): values per lane
 // and side, scalars per tile (0 energy, 1..9 virial, 10 inter-molecular
 // energy, 11 pair count), stored scalar-major (scalar q of tile t at
@@ -175,6 +208,13 @@ struct Totals {
     double einter;
     unsigned long long nem;
     double vir[9];
+    __device__ __forceinline__ void add_vir(const Vir6 &v)
+    {
+        atomicAdd(&vir[0], v.xx); atomicAdd(&vir[4], v.yy); atomicAdd(&vir[8], v.zz);
+        atomicAdd(&vir[1], v.xy); atomicAdd(&vir[3], v.xy);
+        atomicAdd(&vir[2], v.xz); atomicAdd(&vir[6], v.xz);
+        atomicAdd(&vir[5], v.yz); atomicAdd(&vir[7], v.yz);
+    }
     // fixed-point accumulators of the mixed path
     long long em_fx, ep_fx;
     long long vir_fx[9];
@@ -324,6 +364,8 @@ struct pmq_ctx {
     double pmeA[3][3] = {{0}};                   // u = A r
     pmq::DBuf<double> pme_g;                     // (K0/2+1) K1 K2 reciprocal kernel
     pmq::DBuf<double> pme_gd;                    // the same for dispersion (m = 0 included)
+    pmq::DBuf<double> pme_bsmod;                 // B-spline moduli of the three axes (K0, then K1, then K2)
+    pmq::DBuf<double> pme_vsrc;                  // VIRIAL: 10 x npad multipoles plus both induced-dipole sets
     double pme_dalpha = 0.0;                     // dispersion Ewald coefficient pme_gd was built for
     pmq::DBuf<double> qgrid;                     // 2 real grids
     pmq::DBuf<double> cgrid;                     // 2 complex half grids (interleaved re, im)
@@ -386,15 +428,18 @@ int pme_setup(pmq_ctx *c);
 int fft_own_setup(pmq_ctx *c);
 bool fft_own_fits(const pmq_ctx *c, bool fp32);
 int fft_own_convolve(pmq_ctx *c, int nset, const void *G, bool fp32, const double *skip = nullptr);
-int pme_disp(pmq_ctx *c, bool grad, bool need_bin);
+int pme_disp(pmq_ctx *c, bool grad, bool need_bin, bool vir);
 int hippo_sort(pmq_ctx *c);
-int launch_disp(pmq_ctx *c, double off2, bool grad, bool mixed);
-int launch_rep(pmq_ctx *c, double off2, bool grad, bool mixed);
+int launch_disp(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir);
+int launch_rep(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir);
 void pme_release(pmq_ctx *c);
 int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar);
+// VIRIAL: the k-space sums of the multipole and polarization reciprocal
+// energies (after the solve; FP64 grids whatever the precision form)
+int pme_virial(pmq_ctx *c, bool mpole, bool polar);
 int polar_prepare(pmq_ctx *c, double off2, bool mixed, bool safe);
 int polar_solve(pmq_ctx *c, bool safe, int *iters_out);
-int polar_gradient(pmq_ctx *c, double off2, bool mixed);
+int polar_gradient(pmq_ctx *c, double off2, bool mixed, bool vir);
 int polar_finish(pmq_ctx *c, double *ud_out, double *up_out);
 int launch_chgflx_grad(pmq_ctx *c, int mode);
 int launch_zero(pmq_ctx *c, double *p, size_t n);
@@ -402,6 +447,7 @@ int tile_structure(pmq_ctx *c, double off2, bool safe);
 int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe);
 int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fixed);
 int tile_scalars(pmq_ctx *c, int which, bool full);
+int tile_virial(pmq_ctx *c); // tile scalars 1..9 (the pair virial) into the totals
 // skip: a device flag; when it is nonzero the launched kernels return at once
 // (the speculative PCG iterations after convergence)
 int pair_matrix_apply(pmq_ctx *c, double *dst, const double *skip = nullptr);

@@ -294,10 +294,13 @@ template void empole<CalcMode::Gradient>();
 template void empole<CalcMode::Virial>();
 template void epolar<CalcMode::Energy>();
 template void epolar<CalcMode::Gradient>();
+template void epolar<CalcMode::Virial>();
 template void erepel<CalcMode::Energy>();
 template void erepel<CalcMode::Gradient>();
+template void erepel<CalcMode::Virial>();
 template void edisp<CalcMode::Energy>();
 template void edisp<CalcMode::Gradient>();
+template void edisp<CalcMode::Virial>();
 template void energy<CalcMode::Energy>();
 template void energy<CalcMode::Analysis>();
 template void energy<CalcMode::Gradient>();

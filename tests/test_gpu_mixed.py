@@ -75,4 +75,4 @@ def test_mixed_rejects_unsupported_modes(engine):
 
     engine.set_system(box216())
     with pytest.raises(PmqError):
-        engine.compute(L.VIRIAL, L.TERM_MPOLE, L.MIXED)
+        engine.compute(L.ANALYSIS, L.TERM_MPOLE, L.MIXED)
