@@ -144,6 +144,10 @@ typedef struct {
 
 int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o);
 
+/* Thole damping factors of one pair (tests): lam[1..4] = lambda_3,5,7,9 for
+ * u = r/(pdi pdk) and parameter pgamma (lam[0] = lam[5] = 1) */
+void orc_thole_lambda(double r, double pdi, double pdk, double pgamma, double *lam);
+
 /* ---------------------------------------------------------------------
  * Part 3: HIPPO dispersion and Pauli repulsion (pmq_oracle_elec.c).  The
  * reference has parameters but no energy for either (energy.cpp:279-280,

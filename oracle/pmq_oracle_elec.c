@@ -169,6 +169,12 @@ static void thole_lambda(double r, double pdi, double pdk, double pgamma, double
     lam[5] = 1.0; /* unused by dipole interactions */
 }
 
+/* the Thole factors of one pair (tests): lam[0..5] as thole_lambda */
+void orc_thole_lambda(double r, double pdi, double pdk, double pgamma, double *lam)
+{
+    thole_lambda(r, pdi, pdk, pgamma, lam);
+}
+
 /* ---------------------------------------------------------------------
  * system, scaling
  * ------------------------------------------------------------------- */
