@@ -267,8 +267,13 @@ PMQ_API int pmq_compute_batch_noalloc(pmq_ctx *ctx, int mode, unsigned terms, in
  * enabled, the context brackets every launch of each instrumented kernel
  * family with HIP events on its own stream.  pmq_profile_read waits for the
  * stream, returns the summed elapsed time and launch count of one family
- * ("neighbors", "mpole_tiles", "pol_field_tiles", "pol_pair_fill",
- * "pol_mutual_spmv", "pol_grad_tiles", "pme_spread", "pme_fft", "pme_gather")
+ * ("neighbors", "mpole_tiles", "rep_tiles", "disp_tiles", "pol_field_tiles",
+ * "pol_pair_fill", "pol_mutual_spmv", "pol_grad_tiles", "pme_spread" (the
+ * two induced-dipole sets), "pme_spread_mpole" (multipoles or dispersion
+ * coefficients), "pme_fft" (one two-set convolution: forward transforms,
+ * kernel, inverse transforms), "pme_fft1" (a one-set convolution),
+ * "pme_gather_field" (the induced-dipole fields of a PCG step), "pme_gather"
+ * (the other gathers and their per-atom post-processing), "batch_clusters")
  * and resets it. */
 PMQ_API int pmq_profile_enable(pmq_ctx *ctx, int on);
 PMQ_API int pmq_profile_read(pmq_ctx *ctx, const char *name, double *total_ms, long long *launches);

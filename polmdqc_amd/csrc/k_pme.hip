@@ -1126,7 +1126,6 @@ static int fft(pmq_ctx *c, bool forward, int nset, bool fp32)
                                                   : (nset == 1 ? c->plan_b1 : c->plan_b2));
     void *in[1] = {forward ? (void *)c->qgrid.p : (void *)c->cgrid.p}; // read as float by the FP32 plans
     void *out[1] = {forward ? (void *)c->cgrid.p : (void *)c->qgrid.p};
-    ProfScope ps_(c, "pme_fft");
     return fft_check(c, rocfft_execute(pl, in, out, (rocfft_execution_info)c->fft_info), "execute");
 }
 
@@ -1138,8 +1137,9 @@ static int pme_potential(pmq_ctx *c, int nset, const double *G = nullptr, const 
 {
     int rc;
     const bool fp32 = grid32(c) && !G;
+    // one timed launch of the family = one whole convolution, either engine
+    ProfScope ps_(c, nset == 2 ? "pme_fft" : "pme_fft1");
     if ((fp32 ? c->pme_own32 : c->pme_own64) == 1) {
-        ProfScope ps_(c, "pme_fft");
         return fft_own_convolve(c, nset, fp32 ? (const void *)c->pme_g_s.p : (const void *)(G ? G : c->pme_g.p), fp32,
                                 skip);
     }
@@ -1227,7 +1227,7 @@ static int spread(pmq_ctx *c, int mode, const double *src, const double *charge,
         return PMQ_ERR_ARG;
     }
     const dim3 grid(xcd_grid((size_t)c->pme_nb[0] * c->pme_nb[1] * c->pme_nb[2]));
-    ProfScope ps_(c, "pme_spread");
+    ProfScope ps_(c, mode == 1 ? "pme_spread" : "pme_spread_mpole");
 #define PMQ_SPREAD(ORD, MODE)                                                                                  \
     do {                                                                                                       \
         if (g32) hipLaunchKernelGGL((k_pme_spread_brick<ORD, MODE, float, float>), grid, dim3(256), 0, c->stream, args); \
@@ -1376,7 +1376,7 @@ int pme_dipole_field(pmq_ctx *c, const double *dip, double *field, const double 
     args.field = field;
     args.skip = skip;
     {
-        ProfScope ps_(c, "pme_gather");
+        ProfScope ps_(c, "pme_gather_field");
         if (grid32(c)) { PMQ_ORDER_DISPATCH(k_pme_gather_field, , float, float) }
         else if (c->pme_fp32) { PMQ_ORDER_DISPATCH(k_pme_gather_field, , float, double) }
         else { PMQ_ORDER_DISPATCH(k_pme_gather_field, , double, double) }

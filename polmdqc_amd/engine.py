@@ -276,8 +276,8 @@ def _profile_bind(L):
 
 
 KERNEL_FAMILIES = ("neighbors", "mpole_tiles", "rep_tiles", "disp_tiles", "pol_field_tiles", "pol_pair_fill",
-                   "pol_mutual_spmv",
-                   "pol_grad_tiles", "pme_spread", "pme_fft", "pme_gather", "batch_clusters")
+                   "pol_mutual_spmv", "pol_grad_tiles", "pme_spread", "pme_spread_mpole", "pme_fft", "pme_fft1",
+                   "pme_gather_field", "pme_gather", "batch_clusters")
 
 
 FFT_AUTO, FFT_LIBRARY, FFT_DIRECT = 0, 1, 2
