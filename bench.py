@@ -79,10 +79,13 @@ def hbm_bytes_per_launch(family: str, n: int, npad: int, nnz: int, grid: tuple, 
         return 2 * (2 * nreal * gw + 4 * 2 * ncplx * gw) + ncplx * gw
     if family == "pme_gather_field":
         return 2 * nreal * gw + n * 3 * 8 + 2 * n * 3 * 8 + 2 * 2 * n * 3 * 8
+    if family == "pol_spmv_spread":  # the two in one launch (both read the dipoles)
+        return (hbm_bytes_per_launch("pol_mutual_spmv", n, npad, nnz, grid, w, gw)
+                + hbm_bytes_per_launch("pme_spread", n, npad, nnz, grid, w, gw))
     return None
 
 
-HBM_FAMILIES = ("pol_mutual_spmv", "pme_spread", "pme_fft", "pme_gather_field")
+HBM_FAMILIES = ("pol_mutual_spmv", "pol_spmv_spread", "pme_spread", "pme_fft", "pme_gather_field")
 
 
 def copy_calibration(mb: int) -> int:

@@ -268,7 +268,9 @@ PMQ_API int pmq_compute_batch_noalloc(pmq_ctx *ctx, int mode, unsigned terms, in
  * family with HIP events on its own stream.  pmq_profile_read waits for the
  * stream, returns the summed elapsed time and launch count of one family
  * ("neighbors", "mpole_tiles", "rep_tiles", "disp_tiles", "pol_field_tiles",
- * "pol_pair_fill", "pol_mutual_spmv", "pol_grad_tiles", "pme_spread" (the
+ * "pol_pair_fill", "pol_mutual_spmv" (the pair-matrix product T p alone),
+ * "pol_spmv_spread" (T p and the spreading of p in one launch, with Ewald),
+ * "pol_grad_tiles", "pme_spread" (the
  * two induced-dipole sets), "pme_spread_mpole" (multipoles or dispersion
  * coefficients), "pme_fft" (one two-set convolution: forward transforms,
  * kernel, inverse transforms), "pme_fft1" (a one-set convolution),

@@ -777,6 +777,7 @@ __global__ __launch_bounds__(256) void k_pol_finish(int n, int npad, const int *
 // ---------------------------------------------------------------------------
 // launchers
 int pme_dipole_field(pmq_ctx *c, const double *dip, double *field, const double *skip);
+int pme_mutual_field(pmq_ctx *c, const double *dip, double *field, const double *skip);
 int pme_polar_grad(pmq_ctx *c, bool vir);
 int launch_zero(pmq_ctx *c, double *p, size_t n);
 
@@ -815,10 +816,8 @@ static int mutual_field(pmq_ctx *c, const double *src, double *dst, const double
 {
     // the product reads src from its packed copy (written with src by the
     // PCG kernels); the reciprocal part reads src itself
-    int rc;
-    if ((rc = pair_matrix_apply(c, dst, skip))) return rc;
-    if (c->prm.use_ewald && (rc = pme_dipole_field(c, src, dst, skip))) return rc;
-    return PMQ_OK;
+    if (c->prm.use_ewald) return pme_mutual_field(c, src, dst, skip); // product and spreading in one launch
+    return pair_matrix_apply(c, dst, skip);
 }
 
 int polar_prepare(pmq_ctx *c, double off2, bool mixed, bool safe)

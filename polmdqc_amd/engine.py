@@ -276,7 +276,7 @@ def _profile_bind(L):
 
 
 KERNEL_FAMILIES = ("neighbors", "mpole_tiles", "rep_tiles", "disp_tiles", "pol_field_tiles", "pol_pair_fill",
-                   "pol_mutual_spmv", "pol_grad_tiles", "pme_spread", "pme_spread_mpole", "pme_fft", "pme_fft1",
+                   "pol_mutual_spmv", "pol_spmv_spread", "pol_grad_tiles", "pme_spread", "pme_spread_mpole", "pme_fft", "pme_fft1",
                    "pme_gather_field", "pme_gather", "batch_clusters")
 
 
