@@ -283,6 +283,8 @@ def main() -> int:
                     help="fix the PME transform choice (e.g. library/fp32-direct) instead of timing it")
     ap.add_argument("--parts", default="amoeba,clusters,hippo",
                     help="workloads to run (profiling runs take one at a time); the JSON line needs amoeba")
+    ap.add_argument("--pair-form", choices=["cartesian", "qi"], default="cartesian",
+                    help="multipole pair algebra: Cartesian or in each pair's quasi-internal frame")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy kernel of MB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -292,7 +294,13 @@ def main() -> int:
         return 2
 
     from polmdqc_amd import _lib as L
-    from polmdqc_amd.engine import Engine, fft_choice
+    from polmdqc_amd.engine import PAIR_CARTESIAN, PAIR_QI, fft_choice, set_pair_form
+    from polmdqc_amd.engine import Engine as _Engine
+
+    def Engine(dev):
+        e = _Engine(dev)
+        set_pair_form(e, PAIR_QI if args.pair_form == "qi" else PAIR_CARTESIAN)
+        return e
 
     global PIN_FFT
     if args.pin_fft:
@@ -364,7 +372,7 @@ def main() -> int:
         "value": round(value, 3), "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+i64" if mixed else "f64", "data": "synthetic (seeded water lattice)",
-        "config": dict(WORKLOAD, precision=args.precision),
+        "config": dict(WORKLOAD, precision=args.precision, pair_form=args.pair_form),
         "roofline": roofline,
         "fp64_evals_per_s": round(1e3 / ms64, 3),
         "speedup_vs_fp64": round(ms64 / ms, 3),

@@ -310,6 +310,16 @@ PMQ_API int pmq_fft_choice(pmq_ctx *ctx, int *fp64_direct, int *fp32_grids, int 
 #define PMQ_GATHER_PLANES 2
 PMQ_API int pmq_set_gather_lanes(pmq_ctx *ctx, int mode);
 
+/* Pair algebra of the multipole tiles.  PMQ_PAIR_CARTESIAN (the default)
+ * evaluates each pair as pairMpole.h:21-175 does; PMQ_PAIR_QI rotates each
+ * pair into its quasi-internal frame (z along the pair vector) first, the
+ * formulation of pairMpoleQI.h:22-278 (qi.h:11-64), which the reference
+ * includes but never calls.  The two give the same result up to rounding;
+ * the scaled (excluded) pairs always take the Cartesian form. */
+#define PMQ_PAIR_CARTESIAN 0
+#define PMQ_PAIR_QI 1
+PMQ_API int pmq_set_pair_form(pmq_ctx *ctx, int form);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,11 +42,14 @@ def sources():
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
 
 
-# Kernels whose pair algebra is evaluated with some multipole components
-# known to be zero (induced dipoles carry no charge or quadrupole): these
+# Kernels whose pair algebra is evaluated with some multipole components or
+# pair-vector components known to be zero (induced dipoles carry no charge
+# or quadrupole; the quasi-internal frame's pair vector lies on z): these
 # flags let the compiler drop the x*0 terms (no signed zeros, no NaN/Inf
 # inputs).  They do not reassociate arithmetic.
-EXTRA = {"k_polar.hip": ["-fno-signed-zeros", "-ffinite-math-only"]}
+EXTRA = {"k_polar.hip": ["-fno-signed-zeros", "-ffinite-math-only"],
+         # the quasi-internal-frame pairs evaluate the algebra at r = (0, 0, |r|)
+         "k_mpole.hip": ["-fno-signed-zeros", "-ffinite-math-only"]}
 
 
 def _compile(src: str) -> str:

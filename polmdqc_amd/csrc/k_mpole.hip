@@ -351,7 +351,8 @@ struct MpoleArgs {
 
 // R is the arithmetic of the pair terms: double (plain form) or float (mixed
 // form, whose displacements are still taken in FP64 before rounding)
-template <class R, int PEN, bool CF, bool ANA, bool GRAD, bool VIR, bool EW>
+// QI: each pair in its quasi-internal frame (pair_mpole.h qi_frame)
+template <class R, int PEN, bool CF, bool ANA, bool GRAD, bool VIR, bool EW, bool QI = false>
 __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
 {
     constexpr bool CP = PEN != 0;
@@ -407,7 +408,25 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
             PairOut<R> o;
             R bn[6];
             if (EW) ewald_series<R>(r2, R(a.aewald), bn);
-            if (CP) {
+            if (QI) {
+                // the pair algebra at r = (0, 0, |r|) on the rotated multipoles
+                const QiFrame<R> fr = qi_frame<R>(xr, yr, zr, r2);
+                const Mp<R> qa = qi_rotate<R>(fr, mi), qb = qi_rotate<R>(fr, mk);
+                if (CP) {
+                    pair_cp<R, GRAD, (PEN == 0 ? 1 : PEN), EW>(r2, R(0), R(0), fr.len, R(a.f), bc[0][0][il],
+                                                               bc[0][2][il], bc[0][1][il], qa, bc[1][0][jl],
+                                                               bc[1][2][jl], bc[1][1][jl], qb, o, bn);
+                } else {
+                    R Rs[6];
+                    if (EW) {
+                        for (int q = 0; q < 6; q++) Rs[q] = R(a.f) * bn[q];
+                    } else {
+                        rr_series<R>(r2, R(a.f), Rs);
+                    }
+                    pair_series<R, GRAD>(Rs, qa, qb, R(0), R(0), fr.len, o);
+                }
+                if (GRAD) qi_back<R>(fr, o);
+            } else if (CP) {
                 pair_cp<R, GRAD, (PEN == 0 ? 1 : PEN), EW>(r2, xr, yr, zr, R(a.f), bc[0][0][il], bc[0][2][il],
                                                            bc[0][1][il], mi, bc[1][0][jl], bc[1][2][jl],
                                                            bc[1][1][jl], mk, o, bn);
@@ -558,13 +577,20 @@ static int launch_mpole_t(pmq_ctx *c, const MpoleArgs &a)
     {
         ProfScope ps_(c, "mpole_tiles");
         const bool ew = a.aewald > 0.0;
+        const bool qi = c->pair_form == PMQ_PAIR_QI;
+#define PMQ_MT(R, CF_, ANA_, EW_)                                                                              \
+        do {                                                                                                   \
+            if (qi) hipLaunchKernelGGL((k_mpole_tiles<R, PEN, CF_, ANA_, GRAD, VIR, EW_, true>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a); \
+            else hipLaunchKernelGGL((k_mpole_tiles<R, PEN, CF_, ANA_, GRAD, VIR, EW_, false>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a); \
+        } while (0)
         if (a.demq) { // mixed precision: ENERGY, GRADIENT and VIRIAL (no analysis or charge flux)
-            if (ew) hipLaunchKernelGGL((k_mpole_tiles<float, PEN, false, false, GRAD, VIR, true>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
-            else hipLaunchKernelGGL((k_mpole_tiles<float, PEN, false, false, GRAD, VIR, false>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
+            if (ew) PMQ_MT(float, false, false, true);
+            else PMQ_MT(float, false, false, false);
         } else {
-            if (ew) hipLaunchKernelGGL((k_mpole_tiles<double, PEN, CF, ANA, GRAD, VIR, true>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
-            else hipLaunchKernelGGL((k_mpole_tiles<double, PEN, CF, ANA, GRAD, VIR, false>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a);
+            if (ew) PMQ_MT(double, CF, ANA, true);
+            else PMQ_MT(double, CF, ANA, false);
         }
+#undef PMQ_MT
     }
     PMQ_HIP(hipGetLastError());
     const bool mixed = a.demq != nullptr;

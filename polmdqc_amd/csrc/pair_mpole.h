@@ -408,6 +408,88 @@ __device__ __forceinline__ void pair_cp(R r2, R xr, R yr, R zr, R scale_f, R cor
     }
 }
 
+// ---------------------------------------------------------------------------
+// Quasi-internal frame of a pair (the formulation of the reference's
+// pairMpoleQI.h:22-278 and qi.h:11-64, which it includes but never calls):
+// rows ex, ey, ez of an orthonormal frame with ez along r; any ex normal to
+// r serves.  In that frame r = (0, 0, |r|), so every pair-algebra term that
+// carries the two normal components of r vanishes; the caller rotates both
+// multipoles in (qi_rotate), evaluates the ordinary pair algebra at
+// (0, 0, |r|) with literal zeros (the kernels are built without signed
+// zeros or non-finite values, so the compiler drops those terms), and
+// rotates the force and torques back (qi_back): they are vectors, so the
+// result equals the Cartesian form's up to rounding.
+template <class R>
+struct QiFrame {
+    R m[3][3]; // rows ex, ey, ez
+    R len;     // |r|
+};
+
+template <class R>
+__device__ __forceinline__ QiFrame<R> qi_frame(R xr, R yr, R zr, R r2)
+{
+    QiFrame<R> f;
+    const R rinv = R(1) / sqrt(r2);
+    f.len = r2 * rinv;
+    const R z[3] = {xr * rinv, yr * rinv, zr * rinv};
+    // ex, ey completing ez to a right-handed orthonormal basis with one
+    // division and no square root (Duff, Burgess, Christensen, Hery, Kensler,
+    // Liani and Villemin, "Building an orthonormal basis, revisited",
+    // J. Computer Graphics Techniques 6(1), 2017)
+    const R sg = z[2] >= R(0) ? R(1) : R(-1);
+    const R a = R(-1) / (sg + z[2]);
+    const R b = z[0] * z[1] * a;
+    const R x[3] = {R(1) + sg * z[0] * z[0] * a, sg * b, -sg * z[0]};
+    const R y[3] = {b, sg + z[1] * z[1] * a, -z[1]};
+#pragma unroll
+    for (int c = 0; c < 3; c++) { f.m[0][c] = x[c]; f.m[1][c] = y[c]; f.m[2][c] = z[c]; }
+    return f;
+}
+
+// a multipole in the frame: d' = M d, Q' = M Q M^T (Q' traceless by
+// construction: its zz is minus the sum of the other two diagonal entries)
+template <class R>
+__device__ __forceinline__ Mp<R> qi_rotate(const QiFrame<R> &f, const Mp<R> &a)
+{
+    Mp<R> o;
+    o.c = a.c;
+    const R (*M)[3] = f.m;
+    o.dx = M[0][0] * a.dx + M[0][1] * a.dy + M[0][2] * a.dz;
+    o.dy = M[1][0] * a.dx + M[1][1] * a.dy + M[1][2] * a.dz;
+    o.dz = M[2][0] * a.dx + M[2][1] * a.dy + M[2][2] * a.dz;
+    // rows of M Q: (Q m_p) for p = x, y, z
+    R q[3][3];
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        q[p][0] = a.qxx * M[p][0] + a.qxy * M[p][1] + a.qxz * M[p][2];
+        q[p][1] = a.qxy * M[p][0] + a.qyy * M[p][1] + a.qyz * M[p][2];
+        q[p][2] = a.qxz * M[p][0] + a.qyz * M[p][1] + a.qzz * M[p][2];
+    }
+    o.qxx = q[0][0] * M[0][0] + q[0][1] * M[0][1] + q[0][2] * M[0][2];
+    o.qxy = q[0][0] * M[1][0] + q[0][1] * M[1][1] + q[0][2] * M[1][2];
+    o.qxz = q[0][0] * M[2][0] + q[0][1] * M[2][1] + q[0][2] * M[2][2];
+    o.qyy = q[1][0] * M[1][0] + q[1][1] * M[1][1] + q[1][2] * M[1][2];
+    o.qyz = q[1][0] * M[2][0] + q[1][1] * M[2][1] + q[1][2] * M[2][2];
+    o.qzz = -(o.qxx + o.qyy);
+    return o;
+}
+
+// force and torques back to the global frame (v = M^T v')
+template <class R>
+__device__ __forceinline__ void qi_back(const QiFrame<R> &f, PairOut<R> &o)
+{
+    const R (*M)[3] = f.m;
+    auto back = [&](R &x, R &y, R &z) {
+        const R a = x, b = y, c = z;
+        x = M[0][0] * a + M[1][0] * b + M[2][0] * c;
+        y = M[0][1] * a + M[1][1] * b + M[2][1] * c;
+        z = M[0][2] * a + M[1][2] * b + M[2][2] * c;
+    };
+    back(o.fx, o.fy, o.fz);
+    back(o.tix, o.tiy, o.tiz);
+    back(o.tkx, o.tky, o.tkz);
+}
+
 } // namespace pmq
 
 namespace pmq {

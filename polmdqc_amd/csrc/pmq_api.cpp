@@ -135,6 +135,18 @@ int pmq_set_gather_lanes(pmq_ctx *c, int mode)
     return PMQ_OK;
 }
 
+int pmq_set_pair_form(pmq_ctx *c, int form)
+{
+    CHECK_CTX(c);
+    if (form != PMQ_PAIR_CARTESIAN && form != PMQ_PAIR_QI) {
+        c->err = "pmq_set_pair_form: form must be PMQ_PAIR_CARTESIAN or PMQ_PAIR_QI";
+        return PMQ_ERR_ARG;
+    }
+    SETTLE_PENDING(c);
+    c->pair_form = form;
+    return PMQ_OK;
+}
+
 int pmq_fft_choice(pmq_ctx *c, int *fp64_direct, int *fp32_grids, int *fp32_direct)
 {
     CHECK_CTX(c);

@@ -384,6 +384,7 @@ struct pmq_ctx {
     int fft_engine = PMQ_FFT_AUTO;
     int fft_grids = 0;                           // mixed-form grid precision: 0 timed, 1 FP32, 2 FP64
     int gather_lanes = PMQ_GATHER_AUTO;
+    int pair_form = PMQ_PAIR_CARTESIAN;          // multipole tiles: Cartesian or quasi-internal-frame pair algebra
     int pme_own64 = -1, pme_own32 = -1; // FP64 / FP32 grids: the direct LDS transforms (1) or rocFFT (0), timed once per grid (-1)
     pmq::DBuf<double> fft_tw;    // twiddles of the direct transforms, axes 0..2, FP64 and FP32
     pmq::DBuf<float> fft_tw_s;

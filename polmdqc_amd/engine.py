@@ -295,6 +295,18 @@ def set_fft(eng: Engine, engine: int = FFT_AUTO):
     eng._check(L.pmq_set_fft(eng._ctx, int(engine)), "pmq_set_fft")
 
 
+PAIR_CARTESIAN, PAIR_QI = 0, 1
+
+
+def set_pair_form(eng: Engine, form: int = PAIR_CARTESIAN):
+    """Pair algebra of the multipole tiles: Cartesian (PAIR_CARTESIAN) or in
+    each pair's quasi-internal frame (PAIR_QI)."""
+    L = eng._L
+    L.pmq_set_pair_form.argtypes = [C.c_void_p, C.c_int]
+    L.pmq_set_pair_form.restype = C.c_int
+    eng._check(L.pmq_set_pair_form(eng._ctx, int(form)), "pmq_set_pair_form")
+
+
 GATHER_AUTO, GATHER_ROWS, GATHER_PLANES = 0, 1, 2
 
 
