@@ -438,48 +438,45 @@ __global__ __launch_bounds__(64) void k_excl_fold(FoldArgs<V> a)
     }
 }
 
-// dst = T src for both dipole sets (pair_matrix.h)
-template <int G, int BK, class V>
+// dst = T src for both dipole sets, or with EPI the PCG step E1 on T z
+// (pair_matrix.h)
+template <int G, int BK, class V, bool EPI>
 __global__ __launch_bounds__(256) void k_pol_spmv(SpmvArgs a)
 {
-    spmv_rows<G, BK, V>(a, blockIdx.x);
+    if (a.check && a.pcg.iter >= 2 && blockIdx.x == 0) pcg_converged(a.pcg, a.pcg.iter);
+    spmv_rows<G, BK, V, EPI>(a, blockIdx.x);
 }
 
-// The PCG preconditioner z = udiag w + a T_s w (w = a r already formed,
-// a = max(polarity, 1e-8)) over the near part of every row, both dipole
-// sets, G lanes per row; with init also p = z.  Block partials of r.z (two
-// sets) and r.r (two sets) go to part[block * 4 + 0..3].
+// The PCG preconditioner P x = udiag a x + max(a, 1e-8) T_s (a x) over the
+// near part of every row, both dipole sets, G lanes per row, applied to
+// x = r (the initial residual: z = P r) or x = v (an iteration: the step E2,
+// z -= step P v; pcg_steps.h).  The columns' a x come packed with their
+// positions (pcg.pkw); the row's own a x is formed here.
 template <class V>
 struct PrecArgs {
-    int npad;
     const int *rowptr, *rownear;
     size_t cap;
     const TEnt<V> *ent_s;
     const int *col;
     RowGeom geo;
-    const double *spol;
-    double udiag;
-    const double *w, *r; // 2 x 3 x npad
-    const double *pkw;   // w packed with the positions (PMQ_PK per atom), for the columns
-    double *z, *p;
-    double *pks;         // with init: p also into the packed input of T
-    int init;
-    const double *sc;    // sc[8] != 0: converged, nothing to do (not read with init)
-    double *part;
+    PcgVec pcg;
 };
 
 template <int G, int BK, class V>
-__global__ __launch_bounds__(256) void k_pcg_prec(PrecArgs<V> a)
+__global__ __launch_bounds__(PCG_TB) void k_pcg_prec(PrecArgs<V> a)
 {
-    if (!a.init && a.sc[8] != 0.0) return;
+    const PcgVec &pc = a.pcg;
+    if (pc.iter > 0 && pc.sc[PCG_DONE] != 0.0) return;
+    double step[2] = {0.0, 0.0};
+    if (pc.iter > 0) pcg_e2_begin(pc, step);
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int row = tid / G, g = tid % G;
-    const int np = a.npad;
+    const int np = pc.npad;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     if (row < np) {
         const int e1 = (int)min((size_t)(a.rowptr[row] + a.rownear[row]), a.cap);
         row_apply<G, PREC_U, BK, V>(a.col, a.ent_s, a.rowptr[row], e1, g, a.geo, a.geo.sx[row], a.geo.sy[row],
-                                    a.geo.sz[row], a.pkw, acc);
+                                    a.geo.sz[row], pc.pkw, acc);
     }
 #pragma unroll
     for (int c = 0; c < 6; c++)
@@ -492,29 +489,12 @@ __global__ __launch_bounds__(256) void k_pcg_prec(PrecArgs<V> a)
         for (int c = 1; c < 6; c++)
             if (g == c) v = acc[c];
         const size_t i = (size_t)g * np + row;
-        const double z = a.udiag * a.w[i] + fmax(1e-8, a.spol[row]) * v;
-        a.z[i] = z;
-        if (a.init) {
-            a.p[i] = z;
-            a.pks[(size_t)PMQ_PK * row + 3 + g] = z;
-        }
-        const double r = a.r[i];
-        rz[g / 3] = r * z;
-        rr[g / 3] = r * r;
+        const double pol = pc.spol[row];
+        const double x = pc.iter == 0 ? pc.r[i] : pc.v[i];
+        const double q = pc.udiag * (pol * x) + fmax(1e-8, pol) * v;
+        pcg_e2(pc, row, g, q, step, rz[g / 3], rr[g / 3]);
     }
-    // block partials, fixed order: wave shuffles, then the 4 waves in order
-    __shared__ double red[4][4];
-    double v4[4] = {rz[0], rz[1], rr[0], rr[1]};
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-        for (int o = 32; o > 0; o >>= 1) v4[q] += __shfl_xor(v4[q], o, 64);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0)
-        for (int q = 0; q < 4; q++) red[q][w] = v4[q];
-    __syncthreads();
-    if (threadIdx.x < 4)
-        a.part[blockIdx.x * 4 + threadIdx.x] =
-            ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
+    pcg_e2_end(pc, rz[0], rz[1], rr[0], rr[1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -872,73 +852,92 @@ static int row_box(const pmq_ctx *c)
     return ROW_BOX_ANY;
 }
 
-SpmvArgs spmv_args(pmq_ctx *c, double *dst, const double *skip)
+SpmvArgs spmv_args(pmq_ctx *c, double *dst, const double *skip, const PcgVec *pcg, int check)
 {
     SpmvArgs a;
     a.npad = c->npad; a.rowptr = c->rowptr.p; a.cap = c->tcap; a.ent = c->tval.p;
     a.col = c->tcol.p; a.geo = row_geom(c);
     a.pk = c->pk_src.p; a.dst = dst; a.skip = skip;
+    a.pcg = pcg ? *pcg : PcgVec{};
+    a.check = check;
     return a;
 }
 
 int spmv_box(const pmq_ctx *c) { return row_box(c); }
 
 template <class V>
-static int spmv(pmq_ctx *c, double *dst, const double *skip)
+static int spmv(pmq_ctx *c, double *dst, const double *skip, PcgVec *pcg, int check, bool epi)
 {
     constexpr int G = SPMV_G;
-    const SpmvArgs a = spmv_args(c, dst, skip);
     const size_t lanes = (size_t)c->npad * G;
-    ProfScope ps_(c, "pol_mutual_spmv");
     const dim3 grid((unsigned)((lanes + 255) / 256));
-    switch (row_box(c)) {
-    case ROW_BOX_NONE: hipLaunchKernelGGL((k_pol_spmv<G, ROW_BOX_NONE, V>), grid, dim3(256), 0, c->stream, a); break;
-    case ROW_BOX_ORTHO: hipLaunchKernelGGL((k_pol_spmv<G, ROW_BOX_ORTHO, V>), grid, dim3(256), 0, c->stream, a); break;
-    default: hipLaunchKernelGGL((k_pol_spmv<G, ROW_BOX_ANY, V>), grid, dim3(256), 0, c->stream, a); break;
+    if (epi) {
+        if ((size_t)grid.x * 2 > c->pcg_part1_cap) {
+            c->err = "PCG: partial-sum buffer too small for the row product";
+            return PMQ_ERR_STATE;
+        }
+        pcg->n1 = (int)grid.x;
     }
+    const SpmvArgs a = spmv_args(c, dst, skip, pcg, check);
+    ProfScope ps_(c, "pol_mutual_spmv");
+#define PMQ_SPMV(BK, EPI) hipLaunchKernelGGL((k_pol_spmv<G, BK, V, EPI>), grid, dim3(256), 0, c->stream, a)
+#define PMQ_SPMV_BOX(EPI)                                                                                      \
+    do {                                                                                                       \
+        switch (row_box(c)) {                                                                                  \
+        case ROW_BOX_NONE: PMQ_SPMV(ROW_BOX_NONE, EPI); break;                                               \
+        case ROW_BOX_ORTHO: PMQ_SPMV(ROW_BOX_ORTHO, EPI); break;                                             \
+        default: PMQ_SPMV(ROW_BOX_ANY, EPI); break;                                                          \
+        }                                                                                                      \
+    } while (0)
+    if (epi) PMQ_SPMV_BOX(true);
+    else PMQ_SPMV_BOX(false);
+#undef PMQ_SPMV_BOX
+#undef PMQ_SPMV
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }
 
 template <class V>
-static int precondition(pmq_ctx *c, const double *w, const double *r, double *z, double *p, bool init,
-                        const double *sc, double *part)
+static int precondition(pmq_ctx *c, const PcgVec &pcg)
 {
     constexpr int G = PREC_G;
     PrecArgs<V> a;
-    a.npad = c->npad; a.rowptr = c->rowptr.p; a.rownear = c->rownear.p; a.cap = c->tcap;
+    a.rowptr = c->rowptr.p; a.rownear = c->rownear.p; a.cap = c->tcap;
     a.ent_s = (const TEnt<V> *)c->tval_s.p;
     a.col = c->tcol.p; a.geo = row_geom(c);
-    a.spol = c->spol.p; a.udiag = c->prm.udiag;
-    a.w = w; a.r = r; a.z = z; a.p = p; a.init = init ? 1 : 0; a.sc = sc; a.part = part;
-    a.pkw = c->pk_w.p; a.pks = c->pk_src.p;
+    a.pcg = pcg;
     const dim3 grid((unsigned)pair_matrix_prec_blocks(c));
+    if (grid.x != (unsigned)pcg.n2) {
+        c->err = "PCG: the preconditioner's partial count";
+        return PMQ_ERR_STATE;
+    }
     switch (row_box(c)) {
-    case ROW_BOX_NONE: hipLaunchKernelGGL((k_pcg_prec<G, ROW_BOX_NONE, V>), grid, dim3(256), 0, c->stream, a); break;
-    case ROW_BOX_ORTHO: hipLaunchKernelGGL((k_pcg_prec<G, ROW_BOX_ORTHO, V>), grid, dim3(256), 0, c->stream, a); break;
-    default: hipLaunchKernelGGL((k_pcg_prec<G, ROW_BOX_ANY, V>), grid, dim3(256), 0, c->stream, a); break;
+    case ROW_BOX_NONE: hipLaunchKernelGGL((k_pcg_prec<G, ROW_BOX_NONE, V>), grid, dim3(PCG_TB), 0, c->stream, a); break;
+    case ROW_BOX_ORTHO: hipLaunchKernelGGL((k_pcg_prec<G, ROW_BOX_ORTHO, V>), grid, dim3(PCG_TB), 0, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_pcg_prec<G, ROW_BOX_ANY, V>), grid, dim3(PCG_TB), 0, c->stream, a); break;
     }
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }
 
-int pair_matrix_prec_blocks(const pmq_ctx *c) { return (int)(((size_t)c->npad * PREC_G + 255) / 256); }
+int pair_matrix_prec_blocks(const pmq_ctx *c) { return (int)(((size_t)c->npad * PREC_G + PCG_TB - 1) / PCG_TB); }
 
-int pair_matrix_precondition(pmq_ctx *c, const double *w, const double *r, double *z, double *p, bool init,
-                             const double *sc, double *part)
+int pair_matrix_precondition(pmq_ctx *c, const PcgVec &pcg)
 {
     if (!c->prec_on) {
         c->err = "preconditioner applied without its matrix";
         return PMQ_ERR_STATE;
     }
-    if (c->tmixed) return precondition<float>(c, w, r, z, p, init, sc, part);
-    return precondition<double>(c, w, r, z, p, init, sc, part);
+    if (c->tmixed) return precondition<float>(c, pcg);
+    return precondition<double>(c, pcg);
 }
 
-int pair_matrix_apply(pmq_ctx *c, double *dst, const double *skip)
+int pair_matrix_apply(pmq_ctx *c, double *dst, const double *skip, PcgVec *pcg, int mode)
 {
-    if (c->tmixed) return spmv<float>(c, dst, skip);
-    return spmv<double>(c, dst, skip);
+    const int check = pcg && mode != PCG_APPLY_NONE;
+    const bool epi = pcg && mode == PCG_APPLY_E1;
+    if (c->tmixed) return spmv<float>(c, dst, skip, pcg, check, epi);
+    return spmv<double>(c, dst, skip, pcg, check, epi);
 }
 
 } // namespace pmq

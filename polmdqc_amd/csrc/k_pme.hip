@@ -677,16 +677,51 @@ struct GatherFieldArgs {
     size_t gstride;
     const double *dip;  // nset x 3 x npad dipoles (for the self term)
     double self;        // 4/3 a^3 / sqrt(pi)
-    double *field;      // nset x 3 x npad (+=)
+    double *field;      // nset x 3 x npad (+=; read only with the PCG step)
     const double *skip; // nonzero: return at once (or null)
+    PcgVec pcg;         // EPI: the PCG step E1 on each finished site (pcg_steps.h)
 };
 
-// reciprocal + self field of the induced-dipole sets at every site
-template <int ORD, int GL, class T, class GT>
+// reciprocal + self field of the induced-dipole sets at every site, added
+// to field; with EPI (two sets, dip = z) the sum, T z at the site, goes to
+// the PCG step E1 instead: lane c < 6 of the atom's group takes component c,
+// its loads issued before the stencil sums
+template <int ORD, int GL, class T, class GT, bool EPI>
 __global__ __launch_bounds__(256) void k_pme_gather_field(GatherFieldArgs a)
 {
     if (a.skip && *a.skip != 0.0) return;
     GATHER_SETUP(1, T)
+    if (EPI) {
+        static_assert(!EPI || GL >= 6, "one lane per component");
+        const bool mine = valid && l < 6;
+        PcgE1In in{0.0, 0.0, 0.0, 0.0};
+        double fr = 0.0;
+        if (mine) {
+            in = pcg_e1_load(a.pcg, s, l);
+            fr = a.field[(size_t)l * a.npad + s];
+        }
+        double beta[2];
+        pcg_e1_begin(a.pcg, beta);
+        double fp0[4], fp1[4];
+        gather_share<ORD, G, 4, T, GT>(a.g, (const GT *)a.grid, tx, ty, ty1, tz, i0, ky, kz, on, fp0);
+        gather_share<ORD, G, 4, T, GT>(a.g, (const GT *)a.grid + a.gstride, tx, ty, ty1, tz, i0, ky, kz, on, fp1);
+        group_reduce<G, 4>(fp0);
+        group_reduce<G, 4>(fp1);
+        double pv0 = 0.0, pv1 = 0.0;
+        if (mine) {
+            double fp[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) fp[j] = l < 3 ? fp0[j] : fp1[j];
+            double phi, gr[3];
+            frac_to_cart<4>(a.g, fp, &phi, gr, nullptr, nullptr);
+            const int q = l % 3;
+            const double g = q == 0 ? gr[0] : q == 1 ? gr[1] : gr[2];
+            const double f = fr + (-g + a.self * in.z);
+            (l < 3 ? pv0 : pv1) = pcg_e1_apply(a.pcg, s, l, f, in, beta);
+        }
+        pcg_e1_end(a.pcg, pv0, pv1);
+        return;
+    }
     for (int set = 0; set < a.nset; set++) {
         double fp[4];
         gather_share<ORD, G, 4, T, GT>(a.g, (const GT *)a.grid + set * a.gstride, tx, ty, ty1, tz, i0, ky, kz, on, fp);
@@ -1159,7 +1194,8 @@ int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar)
 }
 
 // reciprocal + self field of the two spread dipole sets, added to field
-static int dipole_gather(pmq_ctx *c, const double *dip, double *field, const double *skip)
+// (or with pcg, the PCG step E1 on the total)
+static int dipole_gather(pmq_ctx *c, const double *dip, double *field, const double *skip, PcgVec *pcg)
 {
     int rc;
     const size_t gs = grid_size(c);
@@ -1173,22 +1209,37 @@ static int dipole_gather(pmq_ctx *c, const double *dip, double *field, const dou
     args.self = 4.0 / 3.0 * c->prm.aewald * c->prm.aewald * c->prm.aewald / 1.77245385090551602730;
     args.field = field;
     args.skip = skip;
+    if (pcg) {
+        const unsigned nb = lanes_grid(c->n, gather_planes(c) ? 8 : gather_row_lanes(c->bsorder)).x;
+        if ((size_t)nb * 2 > c->pcg_part1_cap) {
+            c->err = "PCG: partial-sum buffer too small for the field gather";
+            return PMQ_ERR_STATE;
+        }
+        pcg->n1 = (int)nb;
+    }
+    args.pcg = pcg ? *pcg : PcgVec{};
     {
         ProfScope ps_(c, "pme_gather_field");
-        if (grid32(c)) { PMQ_ORDER_DISPATCH(k_pme_gather_field, , float, float) }
-        else if (c->pme_fp32) { PMQ_ORDER_DISPATCH(k_pme_gather_field, , float, double) }
-        else { PMQ_ORDER_DISPATCH(k_pme_gather_field, , double, double) }
+        if (pcg) {
+            if (grid32(c)) { PMQ_ORDER_DISPATCH(k_pme_gather_field, , float, float, true) }
+            else if (c->pme_fp32) { PMQ_ORDER_DISPATCH(k_pme_gather_field, , float, double, true) }
+            else { PMQ_ORDER_DISPATCH(k_pme_gather_field, , double, double, true) }
+        } else {
+            if (grid32(c)) { PMQ_ORDER_DISPATCH(k_pme_gather_field, , float, float, false) }
+            else if (c->pme_fp32) { PMQ_ORDER_DISPATCH(k_pme_gather_field, , float, double, false) }
+            else { PMQ_ORDER_DISPATCH(k_pme_gather_field, , double, double, false) }
+        }
     }
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }
 
-int pme_dipole_field(pmq_ctx *c, const double *dip, double *field, const double *skip)
+int pme_dipole_field(pmq_ctx *c, const double *dip, double *field, const double *skip, PcgVec *pcg)
 {
     // dip and field: two sets of 3 x npad, contiguous
     int rc;
     if ((rc = spread(c, 1, dip, nullptr, 2, skip))) return rc;
-    return dipole_gather(c, dip, field, skip);
+    return dipole_gather(c, dip, field, skip, pcg);
 }
 
 // The mutual field's two independent first steps in one launch: workgroups
@@ -1200,24 +1251,28 @@ int pme_dipole_field(pmq_ctx *c, const double *dip, double *field, const double 
 template <int ORD, class T, class GT, int BK, class V>
 __global__ __launch_bounds__(256) void k_spread_spmv(BrickArgs sa, SpmvArgs ma, int nspread)
 {
+    // the first launch of a PCG iteration: block 0 tests the last one
+    if (ma.check && ma.pcg.iter >= 2 && blockIdx.x == 0) pcg_converged(ma.pcg, ma.pcg.iter);
     if ((int)blockIdx.x < nspread) spread_brick<ORD, 1, T, GT>(sa, blockIdx.x, nspread);
     else spmv_rows<SPMV_G, BK, V>(ma, (int)blockIdx.x - nspread);
 }
 
 // field = T dip (real space, the pair matrix) + the reciprocal and self
-// field of dip; skip: the solve's converged flag (or null)
-int pme_mutual_field(pmq_ctx *c, const double *dip, double *field, const double *skip)
+// field of dip; skip: the solve's converged flag (or null); pcg: a PCG
+// iteration (dip = z): the first launch runs the convergence test first,
+// the gather the step E1 on the total instead of storing it; or null
+int pme_mutual_field(pmq_ctx *c, const double *dip, double *field, const double *skip, PcgVec *pcg)
 {
     const bool fp32 = c->pme_fp32, g32 = grid32(c);
     const int bk = spmv_box(c);
     if (fp32 != c->tmixed || bk == ROW_BOX_NONE) {
         // forms the fused launch is not built for: the two launches
         int rc;
-        if ((rc = pair_matrix_apply(c, field, skip))) return rc;
-        return pme_dipole_field(c, dip, field, skip);
+        if ((rc = pair_matrix_apply(c, field, skip, pcg, pcg ? PCG_APPLY_CHECK : PCG_APPLY_NONE))) return rc;
+        return pme_dipole_field(c, dip, field, skip, pcg);
     }
     const BrickArgs sa = brick_args(c, dip, nullptr, 2, skip);
-    const SpmvArgs ma = spmv_args(c, field, skip);
+    const SpmvArgs ma = spmv_args(c, field, skip, pcg, pcg ? 1 : 0);
     const int nspread = (int)xcd_grid((size_t)c->pme_nb[0] * c->pme_nb[1] * c->pme_nb[2]);
     const int nspmv = (int)(((size_t)c->npad * SPMV_G + 255) / 256);
     const dim3 grid(nspread + nspmv);
@@ -1245,7 +1300,7 @@ int pme_mutual_field(pmq_ctx *c, const double *dip, double *field, const double 
 #undef PMQ_SS
     }
     PMQ_HIP(hipGetLastError());
-    return dipole_gather(c, dip, field, skip);
+    return dipole_gather(c, dip, field, skip, pcg);
 }
 
 // dispersion: energy (1/2) C phi + a^6/12 C^2 and gradient C grad(phi) of

@@ -27,9 +27,13 @@
 //   k_pol_field_tiles / _excl   permanent field (scale 1 in the tiles,
 //                               (s-1) lambda rr corrections for scaled pairs)
 //   (T mu for both sets is the pair-matrix product of k_pairs.hip)
-//   k_pcg_*                     preconditioned CG (diagonal preconditioner)
+//   k_pcg_*                     preconditioned CG: the start, and the step E2
+//                               with the diagonal preconditioner (the sparse
+//                               one and E1 run in k_pairs.hip / k_pme.hip;
+//                               pcg_steps.h)
 //   k_pol_grad_tiles / _excl    real-space gradient and torques
 #include "pair_mpole.h"
+#include "pcg_steps.h"
 #include "pmq_internal.h"
 #include "tile_pairs.h"
 
@@ -523,224 +527,74 @@ __global__ void k_pol_sort(int n, int npad, const int *perm, const double *pol, 
 }
 
 // ---------------------------------------------------------------------------
-// preconditioned conjugate gradient on (1/alpha - T) mu = E, both sets at once,
-// with the diagonal (polarizability) preconditioner.  Vectors are 2 x 3 x npad;
-// index t covers both sets, set = t / (3 npad).  Every reduction is two-stage
-// and deterministic: each block stores its partial sums, one block adds them
-// in a fixed order and updates the scalars.  The convergence test runs on the
-// device too, so iterations are enqueued in batches without host round trips;
-// once `done` is set the remaining kernels of a batch return at once.
-//
-// One iteration is three kernels: k_pcg_a (v = M p, p.v partials), k_pcg_b
-// (every block first sums the p.v partials itself for the step lengths, then
-// updates u, r, z and writes r.z / r.r partials to a second buffer) and
-// k_pcg_c (every block sums those for beta and the convergence test, then
-// updates p).  Each block adds the same partials in the same order, so all
-// agree bit for bit; r.z alternates between two slots by iteration parity so
-// block 0 can store the new value while the others still read the old one.
-//
-// Scalars sc[]: 0..1 r.z (even iterations), 11..12 r.z (odd), 4..5 beta,
-// 6..7 r.r, 8 done, 9 iterations at convergence, 10 epsilon of the last
-// iteration.
-struct PcgArgs {
-    int npad, nblk;
-    const double *spol;
-    const double *E, *Tu; // fields
-    double *u, *r, *z, *p, *v;
-    double *part;         // nblk x 4 partial sums of k_pcg_a (and of the initial residual)
-    double *part2;        // nblk2 x 4 partial sums of r.z and r.r (k_pcg_b, or the preconditioner)
-    double *sc;
-    double npolar, poleps;
-    int iter;
-    int prec;             // sparse preconditioner: r is handed to it as w = a r
-    double *w;
-    int nblk2;
-    // the inputs of T (u for the first product, then p) and of the
-    // preconditioner (w), each also written into its packed copy (PMQ_PK per
-    // atom: position, then the six dipole components), which the row
-    // products read; positions from the sorted arrays
-    double *pks, *pkw;
-    const double *sx, *sy, *sz;
-};
+// preconditioned conjugate gradient on (1/alpha - T) mu = E, both sets at
+// once (pcg_steps.h: the iteration, its scalars and reductions).  Vectors
+// are 2 x 3 x npad, component t = c npad + s.  The convergence test runs on
+// the device, so iterations are enqueued in batches without host round
+// trips; once PCG_DONE is set the remaining launches of a batch return at
+// once.
 
-// component t (set-major, t = c npad + s) of a PCG vector into slot 3 + c of
-// atom s's packed record
-__device__ __forceinline__ void pk_store(double *pk, int npad, int t, double v)
-{
-    const int s = t % npad, c = t / npad;
-    pk[(size_t)PMQ_PK * s + 3 + c] = v;
-}
-
-// slot of r.z after iteration k (k = 0: the initial residual)
-__device__ __forceinline__ int rz_slot(int k) { return (k & 1) ? 11 : 0; }
-
-constexpr int PCG_TB = 256;
-
-// block sums of two pairs of values into part[blockIdx.x * 4 + 0..3]
-__device__ __forceinline__ void block_part4(double a0, double a1, double a2, double a3, double *part)
-{
-    __shared__ double red[4][PCG_TB / 64];
-    double v[4] = {a0, a1, a2, a3};
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
-    const int w = threadIdx.x / 64;
-    if ((threadIdx.x & 63) == 0)
-        for (int q = 0; q < 4; q++) red[q][w] = v[q];
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        double t = 0.0;
-        for (int k = 0; k < PCG_TB / 64; k++) t += red[threadIdx.x][k];
-        part[blockIdx.x * 4 + threadIdx.x] = t;
-    }
-}
-
-// sums of the block partials (one block of PCG_TB threads), fixed order
-__device__ __forceinline__ void sum_parts(const double *part, int nblk, double *out4)
-{
-    __shared__ double red[4][PCG_TB];
-    double v[4] = {0, 0, 0, 0};
-    for (int b = threadIdx.x; b < nblk; b += PCG_TB)
-        for (int q = 0; q < 4; q++) v[q] += part[b * 4 + q];
-    for (int q = 0; q < 4; q++) red[q][threadIdx.x] = v[q];
-    __syncthreads();
-    for (int w = PCG_TB / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-            for (int q = 0; q < 4; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
-        __syncthreads();
-    }
-    for (int q = 0; q < 4; q++) out4[q] = red[q][0];
-}
-
-__global__ __launch_bounds__(256) void k_pcg_guess(PcgArgs a)
+// the first guess u = a E, and the positions of both packed records (fixed
+// during the solve); u also into the row product's packed input
+__global__ __launch_bounds__(256) void k_pcg_guess(PcgVec a, const double *E, const double *sx, const double *sy,
+                                                   const double *sz)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 6 * a.npad) return;
     const int s = t % a.npad, c = t / a.npad;
-    const double u = a.spol[s] * a.E[t];
+    const double u = a.spol[s] * E[t];
     a.u[t] = u;
-    pk_store(a.pks, a.npad, t, u);
+    a.pks[(size_t)PMQ_PK * s + 3 + c] = u;
     if (c < 3) {
-        // the positions of both packed records (fixed during the solve)
-        const double x = c == 0 ? a.sx[s] : c == 1 ? a.sy[s] : a.sz[s];
+        const double x = c == 0 ? sx[s] : c == 1 ? sy[s] : sz[s];
         a.pks[(size_t)PMQ_PK * s + c] = x;
         if (a.prec) a.pkw[(size_t)PMQ_PK * s + c] = x;
     }
 }
 
-// initial residual; z = p = a r and the r.z partials, or (prec) w = a r for
-// the preconditioner, which forms z, p and the partials
-__global__ __launch_bounds__(PCG_TB) void k_pcg_init(PcgArgs a)
+// the initial residual r = E + T u - u/a (Tu = T u, the whole operator's
+// field), p = v = 0, the scalars cleared; with the sparse preconditioner
+// a r into its packed input (it forms z and the partials), else z = a r and
+// the partials here
+__global__ __launch_bounds__(PCG_TB) void k_pcg_init(PcgVec a, const double *E, const double *Tu)
 {
+    if (blockIdx.x == 0 && threadIdx.x < 16) a.sc[threadIdx.x] = 0.0;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    double s0 = 0.0, s1 = 0.0;
+    double rz[2] = {0, 0}, rr[2] = {0, 0};
+    const double none[2] = {0.0, 0.0};
     if (t < 6 * a.npad) {
-        const int s = t % a.npad;
+        const int s = t % a.npad, c = t / a.npad;
         const double pol = a.spol[s];
-        const double r = pol != 0.0 ? a.E[t] + a.Tu[t] - a.u[t] / pol : 0.0;
-        const double z = pol * r;
+        const double r = pol != 0.0 ? E[t] + Tu[t] - a.u[t] / pol : 0.0;
         a.r[t] = r;
-        if (a.prec) {
-            a.w[t] = z;
-            pk_store(a.pkw, a.npad, t, z);
-        } else {
-            a.z[t] = z;
-            a.p[t] = z;
-            pk_store(a.pks, a.npad, t, z);
-            (t < 3 * a.npad ? s0 : s1) = r * z;
-        }
+        a.p[t] = 0.0;
+        a.v[t] = 0.0;
+        if (a.prec) a.pkw[(size_t)PMQ_PK * s + 3 + c] = pol * r;
+        else pcg_e2(a, s, c, pol * r, none, rz[c / 3], rr[c / 3]);
     }
-    if (!a.prec) block_part4(s0, s1, 0.0, 0.0, a.part);
+    if (!a.prec) pcg_e2_end(a, rz[0], rz[1], rr[0], rr[1]);
 }
 
-__global__ __launch_bounds__(PCG_TB) void k_pcg_init_fin(PcgArgs a)
+// the step E2 with the diagonal preconditioner (P v = a v)
+__global__ __launch_bounds__(PCG_TB) void k_pcg_step(PcgVec a)
 {
-    double tot[4];
-    if (a.prec) sum_parts(a.part2, a.nblk2, tot);
-    else sum_parts(a.part, a.nblk, tot);
-    if (threadIdx.x == 0) {
-        a.sc[0] = tot[0]; a.sc[1] = tot[1];
-        for (int q = 2; q < 16; q++) a.sc[q] = 0.0;
-    }
-}
-
-// v = p/alpha - T p, and p.v
-__global__ __launch_bounds__(PCG_TB) void k_pcg_a(PcgArgs a)
-{
-    if (a.sc[8] != 0.0) return;
+    if (a.sc[PCG_DONE] != 0.0) return;
+    double step[2];
+    pcg_e2_begin(a, step);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    double s0 = 0.0, s1 = 0.0;
+    double rz[2] = {0, 0}, rr[2] = {0, 0};
     if (t < 6 * a.npad) {
-        const int s = t % a.npad;
-        const double pol = a.spol[s];
-        const double v = pol != 0.0 ? a.p[t] / pol - a.Tu[t] : 0.0;
-        a.v[t] = v;
-        (t < 3 * a.npad ? s0 : s1) = a.p[t] * v;
+        const int s = t % a.npad, c = t / a.npad;
+        pcg_e2(a, s, c, a.spol[s] * a.v[t], step, rz[c / 3], rr[c / 3]);
     }
-    block_part4(s0, s1, 0.0, 0.0, a.part);
+    pcg_e2_end(a, rz[0], rz[1], rr[0], rr[1]);
 }
 
-// alpha = r.z / p.v (each block from the partials), u += alpha p,
-// r -= alpha v, z = a r with new r.z and r.r partials (or, with the sparse
-// preconditioner, w = a r for it to finish z and the partials)
-__global__ __launch_bounds__(PCG_TB) void k_pcg_b(PcgArgs a)
+// the convergence test of iteration a.iter, the last of a batch (one block)
+__global__ __launch_bounds__(PCG_TB) void k_pcg_check(PcgVec a)
 {
-    if (a.sc[8] != 0.0) return;
-    double pv[4];
-    sum_parts(a.part, a.nblk, pv);
-    const int so = rz_slot(a.iter - 1);
-    const double alpha[2] = {pv[0] != 0.0 ? a.sc[so] / pv[0] : 0.0, pv[1] != 0.0 ? a.sc[so + 1] / pv[1] : 0.0};
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    double s0 = 0.0, s1 = 0.0, q0 = 0.0, q1 = 0.0;
-    if (t < 6 * a.npad) {
-        const int set = t < 3 * a.npad ? 0 : 1;
-        const int s = t % a.npad;
-        const double al = alpha[set];
-        a.u[t] += al * a.p[t];
-        const double r = a.r[t] - al * a.v[t];
-        const double z = a.spol[s] * r;
-        a.r[t] = r;
-        if (a.prec) {
-            a.w[t] = z;
-            pk_store(a.pkw, a.npad, t, z);
-        } else {
-            a.z[t] = z;
-            (set == 0 ? s0 : s1) = r * z;
-            (set == 0 ? q0 : q1) = r * r;
-        }
-    }
-    if (!a.prec) block_part4(s0, s1, q0, q1, a.part2);
-}
-
-// beta = new r.z / r.z and the convergence test debye sqrt(max r.r / npolar)
-// < poleps (each block from the partials; block 0 stores the scalars), then
-// p = z + beta p unless converged
-__global__ __launch_bounds__(PCG_TB) void k_pcg_c(PcgArgs a)
-{
-    if (a.sc[8] != 0.0) return;
-    double tot[4];
-    sum_parts(a.part2, a.nblk2, tot);
-    const int so = rz_slot(a.iter - 1), sn = rz_slot(a.iter);
-    const double beta[2] = {a.sc[so] != 0.0 ? tot[0] / a.sc[so] : 0.0, a.sc[so + 1] != 0.0 ? tot[1] / a.sc[so + 1] : 0.0};
-    const double eps = 4.80321 * sqrt(fmax(tot[2], tot[3]) / a.npolar); // debye x rms residual
-    const bool done = eps < a.poleps;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.sc[10] = eps;
-        a.sc[6] = tot[2]; a.sc[7] = tot[3];
-        a.sc[4] = beta[0]; a.sc[5] = beta[1];
-        a.sc[sn] = tot[0]; a.sc[sn + 1] = tot[1];
-        if (done) {
-            a.sc[8] = 1.0;
-            a.sc[9] = (double)a.iter;
-        }
-    }
-    if (done) return; // p is not used again
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 6 * a.npad) return;
-    const int set = t < 3 * a.npad ? 0 : 1;
-    const double p = a.z[t] + beta[set] * a.p[t];
-    a.p[t] = p;
-    pk_store(a.pks, a.npad, t, p);
+    if (a.sc[PCG_DONE] != 0.0) return;
+    pcg_converged(a, a.iter + 1);
 }
 
 // block sum of one value, one atomic per block (blockDim 256)
@@ -776,8 +630,7 @@ __global__ __launch_bounds__(256) void k_pol_finish(int n, int npad, const int *
 
 // ---------------------------------------------------------------------------
 // launchers
-int pme_dipole_field(pmq_ctx *c, const double *dip, double *field, const double *skip);
-int pme_mutual_field(pmq_ctx *c, const double *dip, double *field, const double *skip);
+int pme_mutual_field(pmq_ctx *c, const double *dip, double *field, const double *skip, PcgVec *pcg);
 int pme_polar_grad(pmq_ctx *c, bool vir);
 int launch_zero(pmq_ctx *c, double *p, size_t n);
 
@@ -809,15 +662,17 @@ static PolArgs pol_args(pmq_ctx *c, double off2)
 
 static int tile_grid(pmq_ctx *c) { return c->tgrid; }
 
-// dst = T src (real space, from the pair matrix) + reciprocal and self field;
-// skip: the solve's converged flag for the iterations of a speculative batch
-// (their kernels return at once once it is set), or null
-static int mutual_field(pmq_ctx *c, const double *src, double *dst, const double *skip = nullptr)
+// dst = T src (real space, from the pair matrix) + reciprocal and self
+// field; skip: the solve's converged flag for the iterations of a
+// speculative batch (their launches return at once once it is set), or null;
+// pcg: the PCG step E1 on T src (src = z) instead of storing it, or null
+static int mutual_field(pmq_ctx *c, const double *src, double *dst, const double *skip = nullptr,
+                        PcgVec *pcg = nullptr)
 {
     // the product reads src from its packed copy (written with src by the
     // PCG kernels); the reciprocal part reads src itself
-    if (c->prm.use_ewald) return pme_mutual_field(c, src, dst, skip); // product and spreading in one launch
-    return pair_matrix_apply(c, dst, skip);
+    if (c->prm.use_ewald) return pme_mutual_field(c, src, dst, skip, pcg); // product and spreading in one launch
+    return pair_matrix_apply(c, dst, skip, pcg, pcg ? PCG_APPLY_E1 : PCG_APPLY_NONE);
 }
 
 int polar_prepare(pmq_ctx *c, double off2, bool mixed, bool safe)
@@ -875,34 +730,33 @@ int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
     const size_t nv = 6 * (size_t)np;
     const int gb = (int)((nv + PCG_TB - 1) / PCG_TB);
     const bool prec = c->prec_on;
-    const int gb2 = prec ? pair_matrix_prec_blocks(c) : gb;
-    PMQ_HIP(c->pcg_part.alloc(4 * (size_t)gb + 4 * (size_t)gb2));
-    PcgArgs a;
-    a.npad = np; a.nblk = gb; a.spol = c->spol.p;
-    a.E = c->pfield.p; a.Tu = c->pvec.p;
-    a.u = c->uind.p; a.r = c->pres.p; a.z = c->pz.p; a.p = c->pdir.p; a.v = c->pvec.p;
-    a.part = c->pcg_part.p;
-    a.part2 = c->pcg_part.p + 4 * (size_t)gb;
-    a.nblk2 = gb2;
-    a.prec = prec ? 1 : 0;
-    a.w = c->pw.p;
+    // E1 partials (the field gather, at most 64 lanes per atom, or the row
+    // product), then the E2 ones (the preconditioner or the vector kernels)
+    const int n2 = prec ? pair_matrix_prec_blocks(c) : gb;
+    c->pcg_part1_cap = 2 * ((size_t)np / 4 + 64);
+    PMQ_HIP(c->pcg_part.alloc(c->pcg_part1_cap + 4 * (size_t)n2));
     PMQ_HIP(c->pk_src.alloc((size_t)PMQ_PK * np));
     if (prec) PMQ_HIP(c->pk_w.alloc((size_t)PMQ_PK * np));
+    PcgVec a;
+    a.npad = np; a.iter = 0; a.prec = prec ? 1 : 0;
+    a.spol = c->spol.p; a.udiag = c->prm.udiag;
+    a.u = c->uind.p; a.r = c->pres.p; a.z = c->pz.p; a.p = c->pdir.p; a.v = c->pav.p;
     a.pks = c->pk_src.p;
     a.pkw = prec ? c->pk_w.p : nullptr;
-    a.sx = c->sx.p; a.sy = c->sy.p; a.sz = c->sz.p;
+    a.part1 = c->pcg_part.p;
+    a.n1 = 0;
+    a.part2 = c->pcg_part.p + c->pcg_part1_cap;
+    a.n2 = n2;
     a.sc = c->pcg.p;
     int npolar = 0;
     for (double p : c->h_polarity) npolar += p != 0.0;
     a.npolar = std::max(npolar, 1);
     a.poleps = c->prm.poleps;
-    a.iter = 0;
     int rc;
-    hipLaunchKernelGGL(k_pcg_guess, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
+    hipLaunchKernelGGL(k_pcg_guess, dim3(gb), dim3(256), 0, c->stream, a, c->pfield.p, c->sx.p, c->sy.p, c->sz.p);
     if ((rc = mutual_field(c, c->uind.p, c->pvec.p))) return rc;
-    hipLaunchKernelGGL(k_pcg_init, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
-    if (prec && (rc = pair_matrix_precondition(c, a.w, a.r, a.z, a.p, true, a.sc, a.part2))) return rc;
-    hipLaunchKernelGGL(k_pcg_init_fin, dim3(1), dim3(PCG_TB), 0, c->stream, a);
+    hipLaunchKernelGGL(k_pcg_init, dim3(gb), dim3(PCG_TB), 0, c->stream, a, c->pfield.p, c->pvec.p);
+    if (prec && (rc = pair_matrix_precondition(c, a))) return rc;
     PMQ_HIP(hipGetLastError());
     const bool spec = !safe && c->pcg_guess_iters > 0;
     int batch = spec ? std::min(c->pcg_guess_iters + 1, c->prm.politer) : 1;
@@ -913,23 +767,27 @@ int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
         const int nb = std::min(batch, c->prm.politer - it);
         for (int k = 0; k < nb; k++) {
             a.iter = ++it;
-            // sc[8] is the converged flag k_pcg_c sets (cleared by k_pcg_init_fin)
-            if ((rc = mutual_field(c, c->pdir.p, c->pvec.p, a.sc + 8))) return rc;
-            hipLaunchKernelGGL(k_pcg_a, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
-            hipLaunchKernelGGL(k_pcg_b, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
-            if (prec && (rc = pair_matrix_precondition(c, a.w, a.r, a.z, a.p, false, a.sc, a.part2))) return rc;
-            hipLaunchKernelGGL(k_pcg_c, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
+            // the convergence test of the last iteration, the operator on z
+            // ending in the step E1 (which records its block count), then E2
+            if ((rc = mutual_field(c, a.z, c->pvec.p, a.sc + PCG_DONE, &a))) return rc;
+            if (prec) {
+                if ((rc = pair_matrix_precondition(c, a))) return rc;
+            } else {
+                hipLaunchKernelGGL(k_pcg_step, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
+            }
             PMQ_HIP(hipGetLastError());
         }
+        hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(PCG_TB), 0, c->stream, a);
+        PMQ_HIP(hipGetLastError());
         if (spec) {
             *iters_out = 0; // known after the status check
             return PMQ_OK;
         }
         PMQ_HIP(hipMemcpyAsync(sc, c->pcg.p, sizeof sc, hipMemcpyDeviceToHost, c->stream));
         PMQ_HIP(hipStreamSynchronize(c->stream));
-        done = sc[8] != 0.0;
+        done = sc[PCG_DONE] != 0.0;
     }
-    const int iters = done ? (int)sc[9] : it;
+    const int iters = done ? (int)sc[PCG_ITERS] : it;
     if (done) c->pcg_guess_iters = iters;
     *iters_out = done ? iters : -iters;
     return PMQ_OK;

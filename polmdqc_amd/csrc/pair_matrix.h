@@ -4,6 +4,7 @@
 // product kernel and the one fused with the PME spreading both run it).
 #pragma once
 
+#include "pcg_steps.h"
 #include "pmq_internal.h"
 
 namespace pmq {
@@ -119,13 +120,16 @@ struct SpmvArgs {
     const int *col;
     RowGeom geo;
     const double *pk;  // the source vector packed with the positions (PMQ_PK per atom)
-    double *dst;       // 2 x 3 x npad (overwritten)
+    double *dst;       // 2 x 3 x npad (overwritten; not written with the PCG step)
     const double *skip; // nonzero: return at once (or null)
+    PcgVec pcg;        // EPI: the PCG step E1 on each finished row (pcg_steps.h)
+    int check;         // the first launch of a PCG iteration: its convergence test first
 };
 
 // dst = T src for both dipole sets, the rows of 256-lane workgroup `wg`
-// (G lanes per row), FP64 arithmetic
-template <int G, int BK, class V>
+// (G lanes per row), FP64 arithmetic; with EPI the rows' values go to the
+// PCG step E1 instead (T src is then the whole operator: no PME part)
+template <int G, int BK, class V, bool EPI = false>
 __device__ __forceinline__ void spmv_rows(const SpmvArgs &a, int wg)
 {
     if (a.skip && *a.skip != 0.0) return;
@@ -142,17 +146,21 @@ __device__ __forceinline__ void spmv_rows(const SpmvArgs &a, int wg)
     for (int c = 0; c < 6; c++)
 #pragma unroll
         for (int off = G / 2; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+    double beta[2] = {0.0, 0.0}, pv[2] = {0.0, 0.0};
+    if (EPI) pcg_e1_begin(a.pcg, beta);
     if (row < np && g < 6) {
         double v = acc[0];
 #pragma unroll
         for (int c = 1; c < 6; c++)
             if (g == c) v = acc[c];
-        a.dst[(size_t)g * np + row] = v;
+        if (EPI) pv[g / 3] = pcg_e1(a.pcg, row, g, v, beta);
+        else a.dst[(size_t)g * np + row] = v;
     }
+    if (EPI) pcg_e1_end(a.pcg, pv[0], pv[1]);
 }
 
 // the product's arguments and box form of a context (k_pairs.hip)
-SpmvArgs spmv_args(pmq_ctx *c, double *dst, const double *skip);
+SpmvArgs spmv_args(pmq_ctx *c, double *dst, const double *skip, const PcgVec *pcg = nullptr, int check = 0);
 int spmv_box(const pmq_ctx *c);
 
 } // namespace pmq

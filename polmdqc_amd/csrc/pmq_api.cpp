@@ -493,7 +493,7 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
         PMQ_HIP(c->pfield.alloc(nv)); PMQ_HIP(c->uind.alloc(nv));
         PMQ_HIP(c->pres.alloc(nv)); PMQ_HIP(c->pz.alloc(nv));
         PMQ_HIP(c->pdir.alloc(nv)); PMQ_HIP(c->pvec.alloc(nv));
-        PMQ_HIP(c->pw.alloc(nv));
+        PMQ_HIP(c->pav.alloc(nv));
         PMQ_HIP(c->pcg.alloc(16));
         PMQ_HIP(c->tep.alloc(3 * (size_t)n)); PMQ_HIP(c->dep.alloc(3 * (size_t)n));
         PMQ_HIP(c->uind_out.alloc(6 * (size_t)n));

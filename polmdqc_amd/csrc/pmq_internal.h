@@ -307,10 +307,11 @@ struct pmq_ctx {
     pmq::DBuf<double> pfield;                    // permanent fields E^d, E^p
     pmq::DBuf<double> uind;                      // induced dipoles mu_d, mu_p
     pmq::DBuf<double> pres, pz, pdir, pvec;      // PCG residual, preconditioned residual,
-                                                 // search direction, (1/alpha - T) p
-    pmq::DBuf<double> pw;                        // a r, the preconditioner's input
+                                                 // search direction, T applied to the operator's input
+    pmq::DBuf<double> pav;                       // A p = (1/alpha - T) p
     pmq::DBuf<double> pcg;                       // PCG scalars
     pmq::DBuf<double> pcg_part;                  // PCG per-block partial sums
+    size_t pcg_part1_cap = 0;                    // E1 partials that fit before the E2 ones in pcg_part
     int pcg_guess_iters = 0;                     // iterations of the last converged solve (0 = none yet)
     pmq::DBuf<double> tep;                       // 3n torques of the polarization term
     pmq::DBuf<double> dep;                       // 3n gradient of ep
@@ -451,10 +452,16 @@ int tile_scalars(pmq_ctx *c, int which, bool full);
 int tile_virial(pmq_ctx *c); // tile scalars 1..9 (the pair virial) into the totals
 // skip: a device flag; when it is nonzero the launched kernels return at once
 // (the speculative PCG iterations after convergence)
-int pair_matrix_apply(pmq_ctx *c, double *dst, const double *skip = nullptr);
+// pcg (or null) with mode PCG_APPLY_CHECK: the first launch of a PCG
+// iteration (its convergence test first); PCG_APPLY_E1: also the step E1 on
+// the rows instead of storing dst (the whole operator when there is no PME
+// part), recording the launch's block count in pcg->n1 (pcg_steps.h)
+struct PcgVec;
+enum { PCG_APPLY_NONE = 0, PCG_APPLY_CHECK = 1, PCG_APPLY_E1 = 2 };
+int pair_matrix_apply(pmq_ctx *c, double *dst, const double *skip = nullptr, PcgVec *pcg = nullptr,
+                      int mode = PCG_APPLY_NONE);
 int pair_matrix_prec_blocks(const pmq_ctx *c);
-int pair_matrix_precondition(pmq_ctx *c, const double *w, const double *r, double *z, double *p, bool init,
-                             const double *sc, double *part);
+int pair_matrix_precondition(pmq_ctx *c, const PcgVec &pcg);
 // device addresses of the batch scratch when it lives outside the context
 struct BatchScratch {
     Totals *tot;
