@@ -84,7 +84,7 @@ class Result(C.Structure):
 EXPORTS = ["pmq_create", "pmq_destroy", "pmq_last_error", "pmq_abi_version", "pmq_set_topology",
            "pmq_set_params", "pmq_eval", "pmq_set_coords", "pmq_compute", "pmq_fetch", "pmq_sync",
            "pmq_profile_enable", "pmq_profile_read", "pmq_set_batch", "pmq_fetch_batch", "pmq_param_grad",
-           "pmq_set_fft", "pmq_fft_choice", "pmq_set_gather_lanes", "pmq_compute_batch_noalloc", "pmq_set_pair_form"]
+           "pmq_set_fft", "pmq_fft_choice", "pmq_set_gather_lanes", "pmq_compute_batch_noalloc", "pmq_set_pair_form", "pmq_set_graphs"]
 
 _lib = None
 

@@ -60,6 +60,8 @@ static int pmq_check(pmq_ctx *c);
 // Setters first complete an evaluation that is still pending (pmq_check:
 // wait, and redo it on its own inputs if it fell short), so a later fetch
 // returns the results of the inputs that evaluation was enqueued with.
+static void new_epoch(pmq_ctx *c);
+
 #define SETTLE_PENDING(c)                                                                          \
     do {                                                                                           \
         if ((c)->pending) {                                                                        \
@@ -77,6 +79,7 @@ int pmq_profile_enable(pmq_ctx *c, int on)
     CHECK_CTX(c);
     PMQ_HIP(hipStreamSynchronize(c->stream));
     c->profiling = on != 0;
+    new_epoch(c); // events are recorded around launches, which a graph does not hold
     c->prof.clear();
     c->ev_used = 0;
     return PMQ_OK;
@@ -117,6 +120,8 @@ int pmq_set_fft(pmq_ctx *c, int engine)
                  "PMQ_FFT_GRIDS_FP32 and PMQ_FFT_GRIDS_FP64";
         return PMQ_ERR_ARG;
     }
+    SETTLE_PENDING(c);
+    new_epoch(c);
     c->fft_engine = engine;
     c->fft_grids = grids == PMQ_FFT_GRIDS_FP32 ? 1 : grids == PMQ_FFT_GRIDS_FP64 ? 2 : 0;
     c->pme_own64 = c->pme_own32 = -1;
@@ -131,6 +136,8 @@ int pmq_set_gather_lanes(pmq_ctx *c, int mode)
         c->err = "pmq_set_gather_lanes: mode must be PMQ_GATHER_AUTO, PMQ_GATHER_ROWS or PMQ_GATHER_PLANES";
         return PMQ_ERR_ARG;
     }
+    SETTLE_PENDING(c);
+    new_epoch(c);
     c->gather_lanes = mode;
     return PMQ_OK;
 }
@@ -143,6 +150,7 @@ int pmq_set_pair_form(pmq_ctx *c, int form)
         return PMQ_ERR_ARG;
     }
     SETTLE_PENDING(c);
+    new_epoch(c);
     c->pair_form = form;
     return PMQ_OK;
 }
@@ -155,6 +163,41 @@ int pmq_fft_choice(pmq_ctx *c, int *fp64_direct, int *fp32_grids, int *fp32_dire
     *fp32_grids = c->pme_fft32;
     *fp32_direct = c->pme_own32;
     return PMQ_OK;
+}
+
+static void eval_graph_release(pmq_ctx *c)
+{
+    if (c->e_exec) (void)hipGraphExecDestroy((hipGraphExec_t)c->e_exec);
+    if (c->e_graph) (void)hipGraphDestroy((hipGraph_t)c->e_graph);
+    c->e_exec = c->e_graph = nullptr;
+}
+
+// whatever changes the launches of an evaluation or their arguments (sizes,
+// buffers, parameters, transform choices, profiling) moves the epoch on, so
+// the next evaluation is enqueued afresh
+static void new_epoch(pmq_ctx *c)
+{
+    c->epoch++;
+    c->e_last_ok = false;
+    c->e_nocapture = false;
+}
+
+static pmq_ctx::EvalKey eval_key(const pmq_ctx *c, int mode, unsigned terms, int precision)
+{
+    pmq_ctx::EvalKey k;
+    std::memset(&k, 0, sizeof k); // padding too: keys are compared bytewise
+    k.mode = mode;
+    k.terms = terms;
+    k.precision = precision;
+    k.pcg_batch = c->pcg_guess_iters;
+    k.epoch = c->epoch;
+    k.box = c->box;
+    return k;
+}
+
+static bool same_key(const pmq_ctx::EvalKey &a, const pmq_ctx::EvalKey &b)
+{
+    return std::memcmp(&a, &b, sizeof a) == 0;
 }
 
 static void batch_graph_release(pmq_ctx *c)
@@ -207,6 +250,7 @@ int pmq_destroy(pmq_ctx *c)
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->h_status) (void)hipHostFree(c->h_status);
+    eval_graph_release(c);
     batch_graph_release(c);
     delete c;
     return PMQ_OK;
@@ -218,6 +262,7 @@ int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
 {
     CHECK_CTX(c);
     SETTLE_PENDING(c);
+    new_epoch(c);
     // a call that fails part way leaves no usable system behind: nothing
     // computes until a later set_topology (and set_params, set_coords) succeeds
     c->have_top = c->have_prm = c->have_xyz = false;
@@ -398,6 +443,7 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
 {
     CHECK_CTX(c);
     SETTLE_PENDING(c);
+    new_epoch(c);
     // a rejected call leaves no parameters set (the device copies may be
     // half updated): compute fails with PMQ_ERR_STATE until a call succeeds
     c->have_prm = false;
@@ -872,6 +918,7 @@ int pmq_set_batch(pmq_ctx *c, int nsys, const int *off)
 {
     CHECK_CTX(c);
     SETTLE_PENDING(c);
+    new_epoch(c);
     if (!c->have_top) {
         c->err = "pmq_set_batch: call pmq_set_topology first";
         return PMQ_ERR_STATE;
@@ -956,6 +1003,7 @@ int pmq_compute_batch_noalloc(pmq_ctx *c, int mode, unsigned terms, int precisio
 {
     CHECK_CTX(c);
     SETTLE_PENDING(c);
+    new_epoch(c);
     int rc = check_compute(c, mode, terms, precision);
     if (rc) return rc;
     if (c->nsys <= 0) {
@@ -988,6 +1036,50 @@ int pmq_compute_batch_noalloc(pmq_ctx *c, int mode, unsigned terms, int precisio
                                  variant, em, ep, pcg_iters, etot, grad);
 }
 
+// A speculative evaluation (no host round trips) of one system.  The first
+// one with a given key is enqueued launch by launch; the second is captured
+// into a HIP graph (every buffer it touches was sized and every host-side
+// choice made by the first), and later ones with the same key replay it.
+// Replays reproduce the host-side effects of compute_enqueue that the check
+// reads.  A capture that fails leaves the key to plain enqueueing.
+static int compute_speculative(pmq_ctx *c, int mode, unsigned terms, int precision)
+{
+    const pmq_ctx::EvalKey k = eval_key(c, mode, terms, precision);
+    if (c->e_exec && same_key(k, c->e_key)) {
+        PMQ_HIP(hipSetDevice(c->device));
+        PMQ_HIP(hipGraphLaunch((hipGraphExec_t)c->e_exec, c->stream));
+        c->pcg_iters = 0;
+        c->last_mode = mode;
+        c->last_terms = terms;
+        c->last_precision = precision;
+        return PMQ_OK;
+    }
+    if (c->e_last_ok && same_key(k, c->e_last) && !c->e_nocapture) {
+        eval_graph_release(c);
+        PMQ_HIP(hipSetDevice(c->device));
+        PMQ_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = compute_enqueue(c, mode, terms, precision, false);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+        hipGraphExec_t ex = nullptr;
+        if (rc == PMQ_OK && ce == hipSuccess && g && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess) {
+            c->e_graph = g;
+            c->e_exec = ex;
+            c->e_key = k;
+            PMQ_HIP(hipGraphLaunch(ex, c->stream));
+            return PMQ_OK;
+        }
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        c->e_nocapture = true;
+    }
+    const int rc = compute_enqueue(c, mode, terms, precision, false);
+    if (rc) return rc;
+    c->e_last = k;
+    c->e_last_ok = true;
+    return PMQ_OK;
+}
+
 int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
 {
     CHECK_CTX(c);
@@ -996,10 +1088,25 @@ int pmq_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
     c->noalloc_last = false;
     if (c->nsys > 0) return batch_compute(c, mode, terms, precision);
     c->pending = false;
-    rc = compute_enqueue(c, mode, terms, precision, c->safe);
+    if (c->safe || !c->graphs_on || c->profiling) {
+        if (c->safe) new_epoch(c); // a safe run sizes buffers
+        rc = compute_enqueue(c, mode, terms, precision, c->safe);
+    } else {
+        rc = compute_speculative(c, mode, terms, precision);
+    }
     if (rc) return rc;
     c->pending = true;
     c->pending_safe = c->safe;
+    return PMQ_OK;
+}
+
+int pmq_set_graphs(pmq_ctx *c, int enable)
+{
+    CHECK_CTX(c);
+    SETTLE_PENDING(c);
+    new_epoch(c);
+    eval_graph_release(c);
+    c->graphs_on = enable != 0;
     return PMQ_OK;
 }
 
@@ -1026,6 +1133,7 @@ static int pmq_check(pmq_ctx *c)
     }
     if (h.st.ntiles > c->tile_cap) c->tile_cap = 0;      // resized from the count on the safe run
     if ((size_t)h.st.nnz > c->tcap) c->tcap = 0;
+    new_epoch(c);
     int rc = compute_enqueue(c, c->last_mode, c->last_terms, c->last_precision, true);
     if (rc) return rc;
     PMQ_HIP(hipStreamSynchronize(c->stream));
@@ -1050,6 +1158,7 @@ int pmq_param_grad(pmq_ctx *c, double *dpole, double *dpolarity, double *dpdamp,
         c->err = "pmq_param_grad: charge penetration and charge flux are not differentiated";
         return PMQ_ERR_UNSUPPORTED;
     }
+    new_epoch(c); // its buffers may grow
     return param_gradients(c, c->last_terms & PMQ_TERM_MPOLE, c->last_terms & PMQ_TERM_POLAR, dpole, dpolarity,
                            dpdamp, dthole);
 }

@@ -109,8 +109,10 @@ struct DBuf {
     {
         if (count <= n && p) return hipSuccess;
         release();
-        n = count;
-        return hipMalloc((void **)&p, (count ? count : 1) * sizeof(T));
+        const hipError_t e = hipMalloc((void **)&p, (count ? count : 1) * sizeof(T));
+        if (e == hipSuccess) n = count;
+        else p = nullptr;
+        return e;
     }
     void release()
     {
@@ -355,6 +357,20 @@ struct pmq_ctx {
     pmq::DBuf<double> b_em, b_ep, b_it;
     int num_cus = 256;
     int batch_variant = -1;                      // autotuned lanes-per-row variant (-1 = not yet)
+    // one evaluation of a single system as a HIP graph: captured on the
+    // second consecutive speculative evaluation with the same key, replayed
+    // while the key holds (pmq_api.cpp eval_key)
+    struct EvalKey {
+        int mode, precision, pcg_batch;
+        unsigned terms;
+        unsigned long long epoch;
+        pmq::DevBox box;
+    };
+    unsigned long long epoch = 0;                // bumped by whatever changes launches or their arguments
+    bool graphs_on = true;                       // pmq_set_graphs
+    EvalKey e_last{}, e_key{};                   // last eager speculative evaluation; the captured one
+    bool e_last_ok = false, e_nocapture = false; // e_last is set; capturing e_last failed
+    void *e_graph = nullptr, *e_exec = nullptr;
     void *b_graph = nullptr, *b_exec = nullptr;  // captured evaluation
     void *static_scratch = nullptr;              // this device's fixed scratch array (pmq_compute_batch_noalloc)
     bool noalloc_last = false;                   // the last evaluation was pmq_compute_batch_noalloc

@@ -307,6 +307,14 @@ def set_pair_form(eng: Engine, form: int = PAIR_CARTESIAN):
     eng._check(L.pmq_set_pair_form(eng._ctx, int(form)), "pmq_set_pair_form")
 
 
+def set_graphs(eng: Engine, enable: bool = True):
+    """HIP graphs for repeated evaluations of one system (on by default)."""
+    L = eng._L
+    L.pmq_set_graphs.argtypes = [C.c_void_p, C.c_int]
+    L.pmq_set_graphs.restype = C.c_int
+    eng._check(L.pmq_set_graphs(eng._ctx, int(bool(enable))), "pmq_set_graphs")
+
+
 GATHER_AUTO, GATHER_ROWS, GATHER_PLANES = 0, 1, 2
 
 
