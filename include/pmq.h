@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PMQ_ABI_VERSION 3
+#define PMQ_ABI_VERSION 4
 
 #if defined(__GNUC__) || defined(__clang__)
 #define PMQ_API __attribute__((visibility("default")))
@@ -177,8 +177,16 @@ typedef struct {
 typedef struct {
     /* energies, kcal/mol */
     double em, ep, er, edsp, esum;
+    /* analysis of em (empole.cpp:285-292 without Ewald; with Ewald, as
+     * Tinker's Ewald analysis is organized): nem counts the real-space
+     * pairs with a nonzero energy, plus one self term per multipole site
+     * with Ewald; aem gives each atom half of each of its pair energies and,
+     * with Ewald, its own reciprocal (f/2 M.phi) and self energy, so aem sums
+     * to em; einter is em minus the unscreened, scaled energies of the pairs
+     * within the cutoff inside one molecule (without Ewald: the sum of the
+     * intermolecular pair energies) */
     double einter;     /* intermolecular part of em (analysis) */
-    long long nem;     /* number of nonzero multipole pair energies (analysis) */
+    long long nem;     /* multipole interactions counted (analysis) */
     double vir[9];     /* row-major */
     int pcg_iters;     /* iterations the induced-dipole solve took */
     /* optional outputs (NULL = not wanted) */
@@ -190,6 +198,11 @@ typedef struct {
     double *uind, *uinp; /* 3n induced dipoles (d and p fields) */
     double *der;   /* 3n gradient of er (HIPPO repulsion) */
     double *dedsp; /* 3n gradient of edsp (HIPPO dispersion) */
+    /* analysis of the polarization energy (no counterpart in the reference,
+     * which has no polarization energy): ep = sum of aep, aep[i] =
+     * -(f/2) mu_d(i) . E^p(i), nep = the sites with a nonzero aep */
+    long long nep;
+    double *aep;   /* n per-atom ep partition (analysis) */
 } pmq_result;
 
 PMQ_API int pmq_create(const pmq_config *cfg, pmq_ctx **ctx);

@@ -181,7 +181,8 @@ class ElecSys(C.Structure):
 class ElecOut(C.Structure):
     _fields_ = [("em", C.c_double), ("ep", C.c_double), ("em_real", C.c_double), ("em_recip", C.c_double),
                 ("em_self", C.c_double), ("iters", C.c_int), ("dem", _dp), ("dep", _dp), ("uind", _dp),
-                ("uinp", _dp), ("vir_m", _dp), ("vir_p", _dp)]
+                ("uinp", _dp), ("vir_m", _dp), ("vir_p", _dp), ("aem", _dp), ("aep", _dp), ("nem", C.c_long),
+                ("nep", C.c_long), ("einter", C.c_double)]
 
 
 def elec(fx, want_grad: bool = True, solver: int = 0, x=None, poleps: float | None = None,
@@ -229,12 +230,15 @@ def elec(fx, want_grad: bool = True, solver: int = 0, x=None, poleps: float | No
     o.dem, o.dep = dem.ctypes.data_as(_dp), dep.ctypes.data_as(_dp)
     o.uind, o.uinp = ud.ctypes.data_as(_dp), up.ctypes.data_as(_dp)
     o.vir_m, o.vir_p = vm.ctypes.data_as(_dp), vp.ctypes.data_as(_dp)
+    aem, aep = np.zeros(n), np.zeros(n)
+    o.aem, o.aep = aem.ctypes.data_as(_dp), aep.ctypes.data_as(_dp)
     rc = L.orc_elec(C.byref(s), 1 if want_grad else 0, C.byref(o))
     if rc != 0:
         raise RuntimeError(f"orc_elec failed ({rc})")
     return {"em": o.em, "ep": o.ep, "em_real": o.em_real, "em_recip": o.em_recip, "em_self": o.em_self,
             "iters": o.iters, "dem": dem.reshape(n, 3), "dep": dep.reshape(n, 3), "uind": ud.reshape(n, 3),
-            "uinp": up.reshape(n, 3), "vir_m": vm.reshape(3, 3), "vir_p": vp.reshape(3, 3)}
+            "uinp": up.reshape(n, 3), "vir_m": vm.reshape(3, 3), "vir_p": vp.reshape(3, 3),
+            "aem": aem, "aep": aep, "nem": int(o.nem), "nep": int(o.nep), "einter": o.einter}
 
 
 class HippoSys(C.Structure):

@@ -140,6 +140,16 @@ typedef struct {
      * i.e. dE/d(strain) with positions and box strained); may be NULL;
      * computed only with want_grad */
     double *vir_m, *vir_p;
+    /* analysis (aem, aep may be NULL; the counts and einter are always
+     * formed): nem = real-space pairs with a nonzero energy + one self term
+     * per multipole site with Ewald; aem[i] = half of each pair energy of i
+     * + (Ewald) its reciprocal energy (f/2) M_i.phi_i and self energy;
+     * einter = em - the unscreened scaled energies of the same-molecule
+     * pairs within the cutoff (without Ewald: the intermolecular pair
+     * energies); aep[i] = -(f/2) mu_d(i).E^p(i), nep = sites with aep != 0 */
+    double *aem, *aep;
+    long nem, nep;
+    double einter;
 } orc_elec_out;
 
 int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o);

@@ -1049,9 +1049,19 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
 
     /* ---- permanent multipoles: real space */
     double em_real = 0.0;
+    /* analysis: pair energies halved onto the atoms, counted when nonzero;
+     * the same-molecule pairs' unscreened energies (Ewald) or the
+     * intermolecular pair energies (without) */
+    double *aem = o->aem;
+    if (aem) memset(aem, 0, n * sizeof(double));
+    long nem = 0;
+    double eintra = 0.0, einter = 0.0;
+    ctx_t c0 = c; /* the same pairs without Ewald */
+    c0.ewald = 0;
     for (long t = 0; t < np; t++) {
         const pairrec *q = &L[t];
         double R[6], rr[6];
+        const int same = mp->molcule[q->i] == mp->molcule[q->k];
         if (c.cp) {
             /* core-core (undamped), core k with valence i (dmpi), core i with
              * valence k (dmpk), valence-valence (dmpik) */
@@ -1065,12 +1075,22 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
             const double *dm[4] = {one, dmpi, dmpk, dmpik};
             const int nd[4] = {1, 5, 5, 6};
             const double *A[4] = {ci, vi, ci, vi}, *B[4] = {ck, ck, vk, vk};
+            double epair = 0.0, efull = 0.0;
             for (int part = 0; part < 4; part++) {
                 damped_series(&c, q, q->m, dm[part], nd[part], R);
                 for (int j = 0; j < 6; j++) R[j] *= f;
                 gpair g;
                 gpair_eval(R, A[part], B[part], q->r, want_grad, &g);
                 em_real += g.e;
+                epair += g.e;
+                if (c.ewald && same) {
+                    double R0[6];
+                    gpair g0;
+                    damped_series(&c0, q, q->m, dm[part], nd[part], R0);
+                    for (int j = 0; j < 6; j++) R0[j] *= f;
+                    gpair_eval(R0, A[part], B[part], q->r, 0, &g0);
+                    efull += g0.e;
+                }
                 if (want_vir) pair_vir(vm, q->r, g.frc, 1.0);
                 if (want_grad)
                     for (int a = 0; a < 3; a++) {
@@ -1080,6 +1100,10 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
                         te[3 * q->k + a] += g.trqk[a];
                     }
             }
+            if (epair != 0.0) nem++;
+            if (aem) { aem[q->i] += 0.5 * epair; aem[q->k] += 0.5 * epair; }
+            if (c.ewald) { if (same) eintra += efull; }
+            else if (!same) einter += epair;
             continue;
         }
         series_rr(q->r2, rr);
@@ -1093,6 +1117,19 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
         gpair g;
         gpair_eval(R, c.rp + 10 * q->i, c.rp + 10 * q->k, q->r, want_grad, &g);
         em_real += g.e;
+        if (g.e != 0.0) nem++;
+        if (aem) { aem[q->i] += 0.5 * g.e; aem[q->k] += 0.5 * g.e; }
+        if (c.ewald) {
+            if (same) {
+                double R0[6];
+                gpair g0;
+                for (int j = 0; j < 6; j++) R0[j] = f * q->m * rr[j];
+                gpair_eval(R0, c.rp + 10 * q->i, c.rp + 10 * q->k, q->r, 0, &g0);
+                eintra += g0.e;
+            }
+        } else if (!same) {
+            einter += g.e;
+        }
         if (want_vir) pair_vir(vm, q->r, g.frc, 1.0);
         if (want_grad)
             for (int a = 0; a < 3; a++) {
@@ -1117,7 +1154,12 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
             const double qq = m[4] * m[4] + m[7] * m[7] + m[9] * m[9] + 2.0 * (m[5] * m[5] + m[6] * m[6] + m[8] * m[8]);
             const double dd = m[1] * m[1] + m[2] * m[2] + m[3] * m[3];
             const double a2 = c.a * c.a;
-            em_self += -f * c.a / SQRTPI * (m[0] * m[0] + 2.0 * a2 / 3.0 * dd + 8.0 * a2 * a2 / 5.0 * qq);
+            const double es = -f * c.a / SQRTPI * (m[0] * m[0] + 2.0 * a2 / 3.0 * dd + 8.0 * a2 * a2 / 5.0 * qq);
+            em_self += es;
+            int site = 0;
+            for (int j = 0; j < 10; j++) site |= m[j] != 0.0;
+            nem += site;
+            if (aem) aem[i] += es + 0.5 * f * site_energy(m, &pm[i], NULL, NULL);
             if (want_grad) {
                 double g[3], tq[3];
                 site_energy(m, &pm[i], g, tq);
@@ -1126,6 +1168,8 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
         }
     }
     o->em = em_real + em_recip + em_self;
+    o->nem = nem;
+    o->einter = c.ewald ? o->em - eintra : einter;
     o->em_real = em_real;
     o->em_recip = em_recip;
     o->em_self = em_self;
@@ -1147,6 +1191,12 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
         double ep = 0.0;
         for (int j = 0; j < n3; j++) ep += ud[j] * Ep[j];
         o->ep = -0.5 * f * ep;
+        o->nep = 0;
+        for (int i = 0; i < n; i++) {
+            const double ei = -0.5 * f * (ud[3 * i] * Ep[3 * i] + ud[3 * i + 1] * Ep[3 * i + 1] + ud[3 * i + 2] * Ep[3 * i + 2]);
+            if (o->aep) o->aep[i] = ei;
+            o->nep += ei != 0.0;
+        }
         if (want_grad && (o->dep || want_vir)) {
             memset(te, 0, n3 * sizeof(double));
             /* real space: (1/2)[U_p(mu_d, M) + U_d(mu_p, M) + U_u(mu_d, mu_p) + U_u(mu_p, mu_d)] */

@@ -78,7 +78,8 @@ class Result(C.Structure):
                 ("esum", C.c_double), ("einter", C.c_double), ("nem", C.c_longlong),
                 ("vir", C.c_double * 9), ("pcg_iters", C.c_int),
                 ("dem", _dp), ("dep", _dp), ("desum", _dp), ("aem", _dp), ("rpole", _dp),
-                ("uind", _dp), ("uinp", _dp), ("der", _dp), ("dedsp", _dp)]
+                ("uind", _dp), ("uinp", _dp), ("der", _dp), ("dedsp", _dp), ("nep", C.c_longlong),
+                ("aep", _dp)]
 
 
 EXPORTS = ["pmq_create", "pmq_destroy", "pmq_last_error", "pmq_abi_version", "pmq_set_topology",

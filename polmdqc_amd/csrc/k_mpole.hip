@@ -444,7 +444,23 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
                 if (o.e != R(0)) nem++;
                 atomicAdd(&acc[0][7][il], (double)(R(0.5) * o.e));
                 atomicAdd(&acc[1][7][jl], (double)(R(0.5) * o.e));
-                if (bmol[0][il] != bmol[1][jl]) ein += o.e;
+                if (!EW) {
+                    if (bmol[0][il] != bmol[1][jl]) ein += o.e;
+                } else if (bmol[0][il] == bmol[1][jl]) {
+                    // with Ewald the intermolecular energy is em minus the
+                    // molecules' own unscreened pair energies: those go to ein
+                    PairOut<R> u;
+                    if (CP) {
+                        pair_cp<R, false, (PEN == 0 ? 1 : PEN), false>(r2, xr, yr, zr, R(a.f), bc[0][0][il],
+                                                                       bc[0][2][il], bc[0][1][il], mi, bc[1][0][jl],
+                                                                       bc[1][2][jl], bc[1][1][jl], mk, u);
+                    } else {
+                        R Rs[6];
+                        rr_series<R>(r2, R(a.f), Rs);
+                        pair_series<R, false>(Rs, mi, mk, xr, yr, zr, u);
+                    }
+                    ein += u.e;
+                }
             }
             if (GRAD) {
                 atomicAdd(&acc[0][0][il], (double)o.fx);
@@ -535,6 +551,7 @@ __global__ __launch_bounds__(64) void k_mpole_excl(MpoleArgs a)
         rr_series<R>(r2, sf, Rs);
         pair_series<R, GRAD>(Rs, mi, mk, xr, yr, zr, o);
     }
+    const R e_scaled = o.e; // the pair's own scaled energy (without Ewald)
     if (a.aewald > 0.0) {
         // f (B_n - rr_n) between the total multipoles
         R bn[6], rr[6], Rs[6];
@@ -550,7 +567,13 @@ __global__ __launch_bounds__(64) void k_mpole_excl(MpoleArgs a)
         if (o.e != R(0)) atomicAdd(&a.tot->nem, 1ull);
         atomicAdd(&a.aem[q.i], R(0.5) * o.e);
         atomicAdd(&a.aem[q.k], R(0.5) * o.e);
-        if (a.smol[si] != a.smol[sk]) atomicAdd(&a.tot->einter, o.e);
+        // without Ewald the intermolecular pair energies, with it the
+        // intramolecular pairs' own (unscreened) energies: einter = em - those
+        if (!(a.aewald > 0.0)) {
+            if (a.smol[si] != a.smol[sk]) atomicAdd(&a.tot->einter, o.e);
+        } else if (a.smol[si] == a.smol[sk]) {
+            atomicAdd(&a.tot->einter, e_scaled);
+        }
     }
     if (GRAD) {
         grad_add(a.dem, a.demq, 3 * q.i + 0, o.fx); grad_add(a.dem, a.demq, 3 * q.i + 1, o.fy);

@@ -577,6 +577,7 @@ struct MpolePostArgs {
     double *fieldd, *fieldp; // 3 * npad sorted (+= reciprocal + self field)
     Totals *tot;
     int vir;                 // the strain term of the dipoles and quadrupoles (with tot)
+    double *aem;             // analysis: per-atom partition (caller order, +=), or null
 };
 
 // the strain term of a dipole d and quadrupole Q (the reference's traceless
@@ -658,6 +659,15 @@ __global__ __launch_bounds__(256) void k_pme_mpole_post(MpolePostArgs a)
                                   -(m[1] * gr[1] - m[2] * gr[0]) - 2.0 * (QH[0][1] - QH[1][0])};
             for (int x = 0; x < 3; x++) a.te[3 * i + x] += a.f * tq[x];
         }
+    }
+    if (a.aem && s < a.n) {
+        // analysis: the site's reciprocal and self energy, and one self term
+        // per multipole site in nem
+        a.aem[a.perm[s]] += e;
+        bool site = false;
+        for (int c = 0; c < 10; c++) site = site || a.smp[c * a.npad + s] != 0.0;
+        if (a.charge) site = site || a.charge[s] != 0.0;
+        if (site) atomicAdd(&a.tot->nem, 1ull);
     }
     if (a.tot) {
         for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
@@ -1185,6 +1195,7 @@ int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar)
     args.cphi = c->cphi.p; args.fieldd = c->pfield.p; args.fieldp = c->pfield.p + 3 * (size_t)c->npad;
     args.tot = mpole ? c->totals.p : nullptr;
     args.vir = mode == PMQ_VIRIAL ? 1 : 0;
+    args.aem = mpole && mode == PMQ_ANALYSIS ? c->aem.p : nullptr;
     {
         ProfScope ps_(c, "pme_gather");
         hipLaunchKernelGGL(k_pme_mpole_post, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, args);

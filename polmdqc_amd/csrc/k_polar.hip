@@ -610,8 +610,11 @@ __device__ __forceinline__ void block_add1(double v, double *dst)
 // E_pol = -(f/2) sum mu_d . E^p
 // the induced dipoles to the caller's order (AoS), and the polarization
 // energy -(f/2) mu_d . E^p summed per atom, then per block
+// (analysis: the per-atom energies into aep and the nonzero ones counted
+// in nep)
 __global__ __launch_bounds__(256) void k_pol_finish(int n, int npad, const int *perm, const double *u, double *ud_out,
-                                                   double *up_out, double f, const double *Ep, Totals *tot)
+                                                   double *up_out, double f, const double *Ep, Totals *tot,
+                                                   double *aep)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0.0;
@@ -624,6 +627,10 @@ __global__ __launch_bounds__(256) void k_pol_finish(int n, int npad, const int *
             v += ud * Ep[c * npad + s];
         }
         v *= -0.5 * f;
+        if (aep) {
+            aep[i] = v;
+            if (v != 0.0) atomicAdd(&tot->nep, 1ull);
+        }
     }
     block_add1(v, &tot->ep);
 }
@@ -837,12 +844,12 @@ int polar_gradient(pmq_ctx *c, double off2, bool mixed, bool vir)
     return PMQ_OK;
 }
 
-int polar_finish(pmq_ctx *c, double *ud_out, double *up_out)
+int polar_finish(pmq_ctx *c, double *ud_out, double *up_out, double *aep)
 {
     const int np = c->npad;
     hipLaunchKernelGGL(k_pol_finish, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n, np, c->perm.p,
                        c->uind.p, ud_out, up_out, c->prm.electric / c->prm.dielec, c->pfield.p + 3 * (size_t)np,
-                       c->totals.p);
+                       c->totals.p, aep);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }

@@ -417,7 +417,7 @@ int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
     c->scratch_sort_bytes = neighbor_scratch_bytes(n, nblk);
     PMQ_HIP(c->scratch_sort.alloc(c->scratch_sort_bytes));
     PMQ_HIP(c->dem.alloc(3 * (size_t)n)); PMQ_HIP(c->te.alloc(3 * (size_t)n));
-    PMQ_HIP(c->pot.alloc(n)); PMQ_HIP(c->aem.alloc(n));
+    PMQ_HIP(c->pot.alloc(n)); PMQ_HIP(c->aem.alloc(n)); PMQ_HIP(c->aep.alloc(n));
     PMQ_HIP(c->rotm.alloc(9 * (size_t)n)); PMQ_HIP(c->flip.alloc(n));
     PMQ_HIP(c->cphi.alloc(10 * (size_t)npad));
     // fixed-point accumulators start (and are left) at zero
@@ -693,14 +693,6 @@ static int check_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
         c->err = "pmq_compute: HIPPO polarization damping is built for the Gordon1 charge penetration";
         return PMQ_ERR_UNSUPPORTED;
     }
-    if (ewald && mode == PMQ_ANALYSIS) {
-        c->err = "pmq_compute: the Ewald path computes energy, gradient and virial (no analysis)";
-        return PMQ_ERR_UNSUPPORTED;
-    }
-    if (do_polar && mode == PMQ_ANALYSIS) {
-        c->err = "pmq_compute: polarization computes energy, gradient and virial (no analysis)";
-        return PMQ_ERR_UNSUPPORTED;
-    }
     return PMQ_OK;
 }
 
@@ -795,7 +787,9 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
                         return rc;
                 }
             }
-            if ((rc = polar_finish(c, c->uind_out.p, c->uind_out.p + 3 * (size_t)n))) return rc;
+            if ((rc = polar_finish(c, c->uind_out.p, c->uind_out.p + 3 * (size_t)n,
+                                   mode == PMQ_ANALYSIS ? c->aep.p : nullptr)))
+                return rc;
         }
         // the k-space virial sums of the reciprocal energies (after the solve)
         if (vir && ewald && (do_mpole || do_polar) && (rc = pme_virial(c, do_mpole, do_polar))) return rc;
@@ -1188,8 +1182,12 @@ int pmq_fetch(pmq_ctx *c, pmq_result *r)
     r->er = t.er;
     r->edsp = t.edsp;
     r->esum = t.em + t.ep + t.er + t.edsp;
-    r->einter = t.einter;
+    // with Ewald the kernels sum the intramolecular pairs' own energies:
+    // the intermolecular part is the rest of em
+    const bool ewald_ana = c->last_mode == PMQ_ANALYSIS && c->prm.use_ewald && (c->last_terms & PMQ_TERM_MPOLE);
+    r->einter = ewald_ana ? t.em - t.einter : t.einter;
     r->nem = (long long)t.nem;
+    r->nep = c->last_mode == PMQ_ANALYSIS ? (long long)t.nep : 0;
     for (int k = 0; k < 9; k++) r->vir[k] = c->last_mode == PMQ_VIRIAL ? t.vir[k] : 0.0;
     r->pcg_iters = c->pcg_iters;
     const bool mp = c->last_terms & PMQ_TERM_MPOLE, pol = c->last_terms & PMQ_TERM_POLAR;
@@ -1232,6 +1230,8 @@ int pmq_fetch(pmq_ctx *c, pmq_result *r)
         PMQ_HIP(hipMemcpy(r->uinp, c->uind_out.p + 3 * (size_t)n, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
     if (r->aem && c->last_mode == PMQ_ANALYSIS)
         PMQ_HIP(hipMemcpy(r->aem, c->aem.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (r->aep && c->last_mode == PMQ_ANALYSIS && pol)
+        PMQ_HIP(hipMemcpy(r->aep, c->aep.p, n * sizeof(double), hipMemcpyDeviceToHost));
     if (r->rpole) PMQ_HIP(hipMemcpy(r->rpole, c->rpole13.p, 13 * n * sizeof(double), hipMemcpyDeviceToHost));
     if (!std::isfinite(r->esum)) {
         c->err = "pmq_fetch: non-finite total energy";

@@ -222,6 +222,7 @@ struct Totals {
     long long vir_fx[9];
     int pcg_iters;
     int pad;
+    unsigned long long nep; // analysis: polarizable sites with a nonzero energy
 };
 
 // a pair whose interactions are scaled (connectivity 1-2..1-5 and
@@ -290,7 +291,7 @@ struct pmq_ctx {
     int ntiles = 0;
     int ntiles_cap = 0;
     // outputs
-    pmq::DBuf<double> dem, te, pot, aem;
+    pmq::DBuf<double> dem, te, pot, aem, aep;
     pmq::DBuf<long long> fx_grad;    // 6n fixed-point gradient + torque (mixed form), zero between uses
     pmq::DBuf<double> rotm;          // 9n local-frame rotation of each site (rows x, y, z)
     pmq::DBuf<int> flip;             // n chirality flips of the last frame pass
@@ -458,7 +459,7 @@ int pme_virial(pmq_ctx *c, bool mpole, bool polar);
 int polar_prepare(pmq_ctx *c, double off2, bool mixed, bool safe);
 int polar_solve(pmq_ctx *c, bool safe, int *iters_out);
 int polar_gradient(pmq_ctx *c, double off2, bool mixed, bool vir);
-int polar_finish(pmq_ctx *c, double *ud_out, double *up_out);
+int polar_finish(pmq_ctx *c, double *ud_out, double *up_out, double *aep = nullptr);
 int launch_chgflx_grad(pmq_ctx *c, int mode);
 int launch_zero(pmq_ctx *c, double *p, size_t n);
 int tile_structure(pmq_ctx *c, double off2, bool safe);

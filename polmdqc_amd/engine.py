@@ -203,7 +203,9 @@ class Engine:
         desum = np.zeros(3 * n)
         der = np.zeros(3 * n)
         dedsp = np.zeros(3 * n)
+        aep = np.zeros(n)
         r = L.Result()
+        r.aep = aep.ctypes.data_as(_dp)
         r.der, r.dedsp = der.ctypes.data_as(_dp), dedsp.ctypes.data_as(_dp)
         r.dem, r.aem = dem.ctypes.data_as(_dp), aem.ctypes.data_as(_dp)
         r.dep, r.desum = dep.ctypes.data_as(_dp), desum.ctypes.data_as(_dp)
@@ -214,7 +216,7 @@ class Engine:
                "nem": int(r.nem), "vir": np.array(r.vir[:]).reshape(3, 3), "pcg_iters": r.pcg_iters,
                "dem": dem.reshape(n, 3), "dep": dep.reshape(n, 3), "desum": desum.reshape(n, 3), "aem": aem,
                "uind": uind.reshape(n, 3), "uinp": uinp.reshape(n, 3), "der": der.reshape(n, 3),
-               "dedsp": dedsp.reshape(n, 3)}
+               "dedsp": dedsp.reshape(n, 3), "nep": int(r.nep), "aep": aep}
         if want_rpole:
             out["rpole"] = rpole.reshape(n, 13)
         return out
