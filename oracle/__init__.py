@@ -305,3 +305,55 @@ def rep_series(r: float, ai: float, ak: float) -> np.ndarray:
     B = np.zeros(6)
     hippo_lib.orc_rep_series(float(r), float(ai), float(ak), B.ctypes.data_as(_dp))
     return B
+
+
+class VdwCtSys(C.Structure):
+    _fields_ = [("mp", C.POINTER(MpoleSys)), ("use_vdw", C.c_int), ("nv", C.c_int), ("jv", _ip),
+                ("radmin", _dp), ("epsilon", _dp), ("radmin4", _dp), ("epsilon4", _dp), ("kred", _dp),
+                ("ired", _ip)] + [(k, C.c_double) for k in ("v2scale", "v3scale", "v4scale", "v5scale", "vdwcut",
+                                                            "vdwtaper")] + \
+               [("use_ctrn", C.c_int), ("chgct", _dp), ("dmpct", _dp), ("ctrncut", C.c_double),
+                ("ctrntaper", C.c_double)]
+
+
+class VdwCtOut(C.Structure):
+    _fields_ = [("ev", C.c_double), ("ect", C.c_double), ("dev", _dp), ("dect", _dp), ("vir_v", _dp),
+                ("vir_c", _dp)]
+
+
+def vdwct(fx, want_grad: bool = True, x=None) -> dict:
+    """Buffered 14-7 van der Waals and charge transfer on a system dict (the
+    keys tinker.build_system writes)."""
+    L = lib()
+    if not hasattr(L, "_vdwct_ready"):
+        L.orc_vdwct.argtypes = [C.POINTER(VdwCtSys), C.c_int, C.POINTER(VdwCtOut)]
+        L.orc_vdwct.restype = C.c_int
+        L._vdwct_ready = True
+    keep = _Keep()
+    mp = mpole_sys(fx, keep, x=x)
+    n = int(fx["n"])
+    s = VdwCtSys()
+    s.mp = C.pointer(mp)
+    s.use_vdw = int(fx.get("use_vdw", 0))
+    s.nv = int(fx.get("nvdw", 1))
+    z = np.zeros(max(1, n))
+    s.jv = keep.i(fx.get("vdwclass", np.zeros(n, np.int32)))
+    for k in ("radmin", "epsilon", "radmin4", "epsilon4"):
+        setattr(s, k, keep.d(fx.get(k, np.zeros(1))))
+    s.kred = keep.d(fx.get("kred", z))
+    s.ired = keep.i(fx.get("ired", np.arange(n, dtype=np.int32)))
+    for k in ("v2scale", "v3scale", "v4scale", "v5scale"):
+        setattr(s, k, float(fx.get(k, 1.0)))
+    s.vdwcut, s.vdwtaper = float(fx.get("vdwcut", 9.0)), float(fx.get("vdwtaper", 8.1))
+    s.use_ctrn = int(fx.get("use_ctrn", 0))
+    s.chgct, s.dmpct = keep.d(fx.get("chgct", z)), keep.d(fx.get("dmpct", z))
+    s.ctrncut, s.ctrntaper = float(fx.get("ctrncut", 6.0)), float(fx.get("ctrntaper", 5.4))
+    o = VdwCtOut()
+    dev, dect, vv, vc = np.zeros(3 * n), np.zeros(3 * n), np.zeros(9), np.zeros(9)
+    o.dev, o.dect = dev.ctypes.data_as(_dp), dect.ctypes.data_as(_dp)
+    o.vir_v, o.vir_c = vv.ctypes.data_as(_dp), vc.ctypes.data_as(_dp)
+    rc = L.orc_vdwct(C.byref(s), 1 if want_grad else 0, C.byref(o))
+    if rc != 0:
+        raise RuntimeError(f"orc_vdwct failed ({rc})")
+    return {"ev": o.ev, "ect": o.ect, "dev": dev.reshape(n, 3), "dect": dect.reshape(n, 3),
+            "vir_v": vv.reshape(3, 3), "vir_c": vc.reshape(3, 3)}

@@ -186,6 +186,43 @@ typedef struct {
 
 int orc_hippo(orc_hippo_sys *s, int want_grad, orc_hippo_out *o);
 
+/* ---------------------------------------------------------------------
+ * Part 4: AMOEBA buffered 14-7 van der Waals (ehal) and HIPPO charge
+ * transfer (ectrn), after Tinker's published ehal/echgtrn; the reference
+ * has their parameters (kvdw.cpp, kchgtrn.cpp) and no energy
+ * (energy.cpp:268,276).
+ *   vdW: rho = r/R (r between the reduced sites, R = radmin of the pair's
+ *   classes), E = s eps (1.07/(rho+0.07))^7 (1.12/(rho^7+0.12) - 2), s the
+ *   1-n vdW scale (the 1-4 tables for 1-4 pairs), switched off between
+ *   vdwtaper and vdwcut; hydrogen sites moved to kred x_i + (1-kred) x_ired
+ *   and their gradient split the same way.
+ *   charge transfer: E = -s f (c_i e^{-a_k r} + c_k e^{-a_i r}), s the 1-n
+ *   multipole scale, f = electric/dielec, a = 0 read as 1000, switched off
+ *   between ctrntaper and ctrncut.
+ * ------------------------------------------------------------------- */
+typedef struct {
+    orc_mpole_sys *mp;           /* coordinates, connectivity, box, m-scales, f */
+    int use_vdw;
+    int nv;                      /* vdW classes */
+    const int *jv;               /* n: class index of each atom */
+    const double *radmin, *epsilon, *radmin4, *epsilon4; /* nv x nv */
+    const double *kred;          /* n reduction factors */
+    const int *ired;             /* n: 0-based reduction partner (itself when none) */
+    double v2scale, v3scale, v4scale, v5scale;
+    double vdwcut, vdwtaper;     /* absolute */
+    int use_ctrn;
+    const double *chgct, *dmpct; /* n */
+    double ctrncut, ctrntaper;   /* absolute */
+} orc_vdwct_sys;
+
+typedef struct {
+    double ev, ect;
+    double *dev, *dect;          /* 3n (may be NULL) */
+    double *vir_v, *vir_c;       /* 9 each (may be NULL) */
+} orc_vdwct_out;
+
+int orc_vdwct(orc_vdwct_sys *s, int want_grad, orc_vdwct_out *o);
+
 /* the overlap series B_0..B_5 of one pair (tests) */
 void orc_rep_series(double r, double ai, double ak, double *B);
 

@@ -1684,3 +1684,130 @@ int orc_hippo(orc_hippo_sys *s, int want_grad, orc_hippo_out *o)
     free(c.o12); free(c.o13); free(c.o14); free(c.o15);
     return 0;
 }
+
+/* ---------------------------------------------------------------------
+ * Part 4: buffered 14-7 van der Waals and charge transfer (Tinker's
+ * published ehal1 and echgtrn1 forms; see pmq_oracle.h)
+ * ------------------------------------------------------------------- */
+static void hal_pair(double r, double rv, double eps, double *e, double *de)
+{
+    const double dhal = 0.07, ghal = 0.12;
+    const double rho = r / rv, rho6 = pow(rho, 6), rho7 = rho6 * rho;
+    const double s1 = 1.0 / (rho + dhal), s2 = 1.0 / (rho7 + ghal);
+    const double t1 = pow((1.0 + dhal) * s1, 7), t2 = (1.0 + ghal) * s2;
+    *e = eps * t1 * (t2 - 2.0);
+    /* dt1/drho = -7 t1 s1, dt2/drho = -7 rho^6 t2 s2 */
+    *de = eps * (-7.0 * t1 * s1 * (t2 - 2.0) - 7.0 * t1 * rho6 * t2 * s2) / rv;
+}
+
+int orc_vdwct(orc_vdwct_sys *s, int want_grad, orc_vdwct_out *o)
+{
+    orc_mpole_sys *m = s->mp;
+    const int n = m->n;
+    int *o12 = offsets(n, m->n12), *o13 = offsets(n, m->n13), *o14 = offsets(n, m->n14), *o15 = offsets(n, m->n15);
+    double *vs = malloc(n * sizeof(double)), *ms = malloc(n * sizeof(double));
+    int *is14 = calloc(n, sizeof(int));
+    double *xr = malloc(3 * (size_t)n * sizeof(double));
+    double *gv = calloc(3 * (size_t)n, sizeof(double)), *gc = calloc(3 * (size_t)n, sizeof(double));
+    double vv[9] = {0}, vc[9] = {0};
+    for (int i = 0; i < n; i++) vs[i] = ms[i] = 1.0;
+    /* reduced vdW sites */
+    for (int i = 0; i < n; i++) {
+        const double p[3] = {m->x[i], m->y[i], m->z[i]};
+        if (s->use_vdw && s->ired[i] != i) {
+            const int j = s->ired[i];
+            const double q[3] = {m->x[j], m->y[j], m->z[j]};
+            for (int a = 0; a < 3; a++) xr[3 * i + a] = s->kred[i] * (p[a] - q[a]) + q[a];
+        } else {
+            for (int a = 0; a < 3; a++) xr[3 * i + a] = p[a];
+        }
+    }
+    const double f = m->f;
+    const double vsc[4] = {s->v2scale, s->v3scale, s->v4scale, s->v5scale};
+    const double msc[4] = {m->m2scale, m->m3scale, m->m4scale, m->m5scale};
+    double ev = 0.0, ect = 0.0;
+    for (int i = 0; i < n - 1; i++) {
+        for (int q = 0; q < 4; q++) {
+            const int *off = q == 0 ? o12 : q == 1 ? o13 : q == 2 ? o14 : o15;
+            const int *lst = q == 0 ? m->i12 : q == 1 ? m->i13 : q == 2 ? m->i14 : m->i15;
+            for (int j = off[i]; j < off[i + 1]; j++) { vs[lst[j]] = vsc[q]; ms[lst[j]] = msc[q]; }
+        }
+        for (int j = o14[i]; j < o14[i + 1]; j++) is14[m->i14[j]] = 1;
+        for (int k = i + 1; k < n; k++) {
+            if (s->use_vdw && vs[k] != 0.0) {
+                double d[3] = {xr[3 * k] - xr[3 * i], xr[3 * k + 1] - xr[3 * i + 1], xr[3 * k + 2] - xr[3 * i + 2]};
+                if (m->box.kind != ORC_BOX_NONE) orc_image(&m->box, &d[0], &d[1], &d[2]);
+                const double r2 = dot3(d, d);
+                if (r2 <= s->vdwcut * s->vdwcut) {
+                    const double r = sqrt(r2);
+                    const int t = s->jv[i] * s->nv + s->jv[k];
+                    const double rv = is14[k] ? s->radmin4[t] : s->radmin[t];
+                    const double ep = (is14[k] ? s->epsilon4[t] : s->epsilon[t]) * vs[k];
+                    double e, de, T, dT;
+                    hal_pair(r, rv, ep, &e, &de);
+                    taper5(r, s->vdwtaper, s->vdwcut, &T, &dT);
+                    de = de * T + e * dT;
+                    e *= T;
+                    ev += e;
+                    if (want_grad) {
+                        const double g[3] = {de * d[0] / r, de * d[1] / r, de * d[2] / r}; /* dE/dx_k */
+                        const int pi[2] = {i, k};
+                        for (int side = 0; side < 2; side++) {
+                            const int a = pi[side], b = s->ired[a];
+                            const double sg = side ? 1.0 : -1.0, kr = s->kred[a];
+                            for (int c = 0; c < 3; c++) {
+                                if (b != a) {
+                                    gv[3 * a + c] += sg * kr * g[c];
+                                    gv[3 * b + c] += sg * (1.0 - kr) * g[c];
+                                } else {
+                                    gv[3 * a + c] += sg * g[c];
+                                }
+                            }
+                        }
+                        pair_vir(vv, d, g, -1.0);
+                    }
+                }
+            }
+            if (s->use_ctrn && ms[k] != 0.0) {
+                double d[3] = {m->x[k] - m->x[i], m->y[k] - m->y[i], m->z[k] - m->z[i]};
+                if (m->box.kind != ORC_BOX_NONE) orc_image(&m->box, &d[0], &d[1], &d[2]);
+                const double r2 = dot3(d, d);
+                if (r2 <= s->ctrncut * s->ctrncut) {
+                    const double r = sqrt(r2);
+                    const double ai = s->dmpct[i] != 0.0 ? s->dmpct[i] : 1000.0;
+                    const double ak = s->dmpct[k] != 0.0 ? s->dmpct[k] : 1000.0;
+                    const double ei = exp(-ai * r), ek = exp(-ak * r);
+                    const double sc = f * ms[k];
+                    double e = -sc * (s->chgct[i] * ek + s->chgct[k] * ei);
+                    double de = sc * (s->chgct[i] * ak * ek + s->chgct[k] * ai * ei);
+                    double T, dT;
+                    taper5(r, s->ctrntaper, s->ctrncut, &T, &dT);
+                    de = de * T + e * dT;
+                    e *= T;
+                    ect += e;
+                    if (want_grad) {
+                        const double g[3] = {de * d[0] / r, de * d[1] / r, de * d[2] / r};
+                        for (int c = 0; c < 3; c++) { gc[3 * k + c] += g[c]; gc[3 * i + c] -= g[c]; }
+                        pair_vir(vc, d, g, -1.0);
+                    }
+                }
+            }
+        }
+        for (int q = 0; q < 4; q++) {
+            const int *off = q == 0 ? o12 : q == 1 ? o13 : q == 2 ? o14 : o15;
+            const int *lst = q == 0 ? m->i12 : q == 1 ? m->i13 : q == 2 ? m->i14 : m->i15;
+            for (int j = off[i]; j < off[i + 1]; j++) { vs[lst[j]] = 1.0; ms[lst[j]] = 1.0; }
+        }
+        for (int j = o14[i]; j < o14[i + 1]; j++) is14[m->i14[j]] = 0;
+    }
+    o->ev = ev;
+    o->ect = ect;
+    if (want_grad) {
+        if (o->dev) memcpy(o->dev, gv, 3 * (size_t)n * sizeof(double));
+        if (o->dect) memcpy(o->dect, gc, 3 * (size_t)n * sizeof(double));
+        if (o->vir_v) memcpy(o->vir_v, vv, sizeof vv);
+        if (o->vir_c) memcpy(o->vir_c, vc, sizeof vc);
+    }
+    free(o12); free(o13); free(o14); free(o15); free(vs); free(ms); free(is14); free(xr); free(gv); free(gc);
+    return 0;
+}

@@ -157,7 +157,7 @@ def parse_prm(path: str) -> dict:
         lines = f.read().splitlines()
     P = {"keywords": [], "atom": {}, "multipole": [], "polarize": {}, "chgpen": {}, "repulsion": {},
          "dispersion": {}, "bond": [], "angle": [], "anglep": [], "bndcflux": [], "angcflux": [], "vdw": {},
-         "chgtrn": {}}
+         "vdw14": {}, "vdwpair": {}, "chgtrn": {}}
     i = 0
     while i < len(lines):
         line = lines[i].split("!!")[0].strip()
@@ -191,7 +191,12 @@ def parse_prm(path: str) -> dict:
         elif key == "dispersion":
             P["dispersion"][int(w[1])] = (float(w[2]), float(w[3]))
         elif key == "vdw":
-            P["vdw"][int(w[1])] = tuple(float(v) for v in w[2:5])
+            v = [float(x) for x in w[2:5]]
+            P["vdw"][int(w[1])] = tuple(v + [0.0] * (3 - len(v)))
+        elif key == "vdw14":
+            P["vdw14"][int(w[1])] = (float(w[2]), float(w[3]))
+        elif key in ("vdwpair", "vdwpr"):
+            P["vdwpair"][tuple(sorted((int(w[1]), int(w[2]))))] = (float(w[3]), float(w[4]))
         elif key == "chgtrn":
             P["chgtrn"][int(w[1])] = (float(w[2]), float(w[3]))
         elif key == "bond":
@@ -303,7 +308,11 @@ def settings(lines: list[str]) -> dict:
               "pcg-precond": True, "dielectric": 1.0, "d-equals-p": False, "penetration": "GORDON1",
               "box": {}, "octahedron": False, "dodecahedron": False, "active": [], "inactive": [],
               "active-sphere": [], "repuls-cutoff": None, "disp-cutoff": None, "dewald": False,
-              "dewald-alpha": None})
+              "dewald-alpha": None, "vdw-cutoff": None, "vdw-taper": 0.9, "chgtrn-cutoff": None,
+              "chgtrn-taper": 0.9, "vdwtype": "LENNARD-JONES", "radiusrule": "ARITHMETIC", "radiustype": "R-MIN",
+              "radiussize": "RADIUS", "epsilonrule": "GEOMETRIC", "vdwindex": "CLASS",
+              "v2scale": 0.0, "v3scale": 0.0, "v4scale": 1.0, "v5scale": 1.0})
+    s["terms"].update({"vdwterm": True, "chgtrnterm": True})
     for line in lines:
         try:
             _keyword(s, line)
@@ -325,7 +334,7 @@ def _keyword(s: dict, line: str) -> None:
             if v == "ONLY":
                 # potoff (prmkey.cpp:635-665) clears every energy term but
                 # leaves charge flux as it is
-                for t in TERMS:
+                for t in list(s["terms"]):
                     if t != "chgflxterm":
                         s["terms"][t] = False
                 s["terms"][k] = True
@@ -333,8 +342,19 @@ def _keyword(s: dict, line: str) -> None:
                 s["terms"][k] = False
             else:
                 s["terms"][k] = True
-        elif k in ("cutoff", "mpole-cutoff", "ewald-cutoff", "repuls-cutoff", "disp-cutoff"):
+        elif k in ("cutoff", "mpole-cutoff", "ewald-cutoff", "repuls-cutoff", "disp-cutoff", "vdw-cutoff",
+                   "chgtrn-cutoff"):
             s[k] = float(arg[0])
+        elif k in ("vdw-taper", "chgtrn-taper"):
+            s[k] = float(arg[0])
+        elif k in ("vdwtype", "radiusrule", "radiustype", "radiussize", "epsilonrule", "vdwindex"):
+            s[k] = arg[0].upper()
+        elif k in ("vdw-12-scale", "vdw-13-scale", "vdw-14-scale", "vdw-15-scale"):
+            v = float(arg[0])
+            s["v" + k[4] + "scale"] = 1.0 / v if v > 1.0 else v  # prmkey.cpp:391-404
+        elif k in ("vdwterm", "chgtrnterm"):
+            v = arg[0].upper() if arg else ""
+            s["terms"][k] = v != "NONE"
         elif k == "ewald":
             s["ewald"] = True
         elif k == "dewald":
@@ -832,6 +852,11 @@ def build_system(X: dict, P: dict, key: list[str]) -> dict:
     # kchgflx.cpp:362-363
     use_chgflx = 1 if (nflx > 0 and terms["chgflxterm"] and (use_mpole or use_polar)) else 0
 
+    s_vdw = vdw_setup(S, P, types, cls, i12)
+    ct = [P["chgtrn"].get(int(c)) for c in cls]
+    chgct = np.array([v[0] if v else 0.0 for v in ct])
+    dmpct = np.array([v[1] if v else 0.0 for v in ct])
+
     nb, ib12 = flat(i12)
     n13, ib13 = flat(i13)
     n14, ib14 = flat(i14)
@@ -901,7 +926,85 @@ def build_system(X: dict, P: dict, key: list[str]) -> dict:
         s.update({"aewald": 0.0, "nfft1": 0, "nfft2": 0, "nfft3": 0})
     s["dewald"] = (S["dewald-alpha"] if S["dewald-alpha"] is not None else ewald_coefficient(s["dispcut"])) \
         if s["use_dewald"] else 0.0
+    # van der Waals and charge transfer (cutoffs.cpp:44-70: vdW 9 A with a
+    # box, else none; charge transfer 6 A; tapers given as a fraction of the
+    # cutoff when below 1)
+    s.update(s_vdw)
+    s["vdwcut"] = S["vdw-cutoff"] or cut or (9.0 if periodic else big)
+    s["vdwtaper"] = S["vdw-taper"] * s["vdwcut"] if S["vdw-taper"] < 1.0 else S["vdw-taper"]
+    s["use_vdw"] = 1 if (terms["vdwterm"] and s_vdw["nvdw"] and np.any(s_vdw["vdw_eps"] != 0.0)) else 0
+    s["chgct"], s["dmpct"] = chgct, dmpct
+    s["ctrncut"] = S["chgtrn-cutoff"] or 6.0
+    s["ctrntaper"] = S["chgtrn-taper"] * s["ctrncut"] if S["chgtrn-taper"] < 1.0 else S["chgtrn-taper"]
+    s["use_ctrn"] = 1 if (terms["chgtrnterm"] and np.any(chgct != 0.0)) else 0
     return s
+
+
+def vdw_setup(S: dict, P: dict, types, cls, i12) -> dict:
+    """Buffered 14-7 parameters (kvdw.cpp): per class (or type) radius and
+    well depth, halved when the file gives diameters, 2^(1/6) x when it gives
+    sigma; pair tables by the radius and epsilon rules (VDWPAIR records
+    override), the 1-4 tables from VDW14; hydrogen reduction factors and
+    their bonded partner."""
+    key = types if S["vdwindex"] == "TYPE" else cls
+    present = sorted({int(k) for k in key})
+    nv = len(present)
+    idx = {k: q for q, k in enumerate(present)}
+    rad, eps, red, rad4, eps4 = (np.zeros(nv) for _ in range(5))
+    for q, k in enumerate(present):
+        r, e, rd = P["vdw"].get(k, (0.0, 0.0, 0.0))
+        r4, e4 = P["vdw14"].get(k, (0.0, 0.0))
+        r4 = r4 or r
+        e4 = e4 or e
+        if S["radiustype"] == "SIGMA":
+            r, r4 = r * 2.0 ** (1.0 / 6.0), r4 * 2.0 ** (1.0 / 6.0)
+        if S["radiussize"] == "DIAMETER":
+            r, r4 = 0.5 * r, 0.5 * r4
+        rad[q], eps[q], red[q], rad4[q], eps4[q] = r, abs(e), rd, r4, abs(e4)
+
+    def rule(ri, rk, ei, ek):
+        if ri == 0.0 and rk == 0.0:
+            rd = 0.0
+        elif S["radiusrule"] == "ARITHMETIC":
+            rd = ri + rk
+        elif S["radiusrule"] == "GEOMETRIC":
+            rd = 2.0 * math.sqrt(ri) * math.sqrt(rk)
+        elif S["radiusrule"] == "CUBIC-MEAN":
+            rd = 2.0 * (ri ** 3 + rk ** 3) / (ri * ri + rk * rk)
+        else:
+            rd = ri + rk
+        if ei == 0.0 and ek == 0.0:
+            ep = 0.0
+        elif S["epsilonrule"] == "ARITHMETIC":
+            ep = 0.5 * (ei + ek)
+        elif S["epsilonrule"] == "HARMONIC":
+            ep = 2.0 * ei * ek / (ei + ek)
+        elif S["epsilonrule"] == "HHG":
+            ep = 4.0 * ei * ek / (math.sqrt(ei) + math.sqrt(ek)) ** 2
+        else:
+            ep = math.sqrt(ei) * math.sqrt(ek)
+        return rd, ep
+
+    radmin, epsilon, radmin4, epsilon4 = (np.zeros((nv, nv)) for _ in range(4))
+    for a in range(nv):
+        for b in range(nv):
+            radmin[a, b], epsilon[a, b] = rule(rad[a], rad[b], eps[a], eps[b])
+            radmin4[a, b], epsilon4[a, b] = rule(rad4[a], rad4[b], eps4[a], eps4[b])
+    for (ka, kb), (rp, ep) in P["vdwpair"].items():
+        if ka in idx and kb in idx:
+            a, b = idx[ka], idx[kb]
+            rp = rp * 2.0 ** (1.0 / 6.0) if S["radiustype"] == "SIGMA" else rp
+            for A, B in ((a, b), (b, a)):
+                radmin[A, B] = radmin4[A, B] = rp
+                epsilon[A, B] = epsilon4[A, B] = abs(ep)
+    jv = np.array([idx[int(k)] for k in key], dtype=np.int32)
+    kred = red[jv].copy()
+    ired = np.array([i12[i][0] if (len(i12[i]) == 1 and kred[i] != 0.0) else i for i in range(len(types))],
+                    dtype=np.int32)
+    return {"nvdw": nv, "vdwclass": jv, "radmin": radmin.ravel(), "epsilon": epsilon.ravel(),
+            "radmin4": radmin4.ravel(), "epsilon4": epsilon4.ravel(), "kred": kred, "ired": ired,
+            "vdw_eps": eps, "v2scale": S["v2scale"], "v3scale": S["v3scale"], "v4scale": S["v4scale"],
+            "v5scale": S["v5scale"]}
 
 
 def ewald_coefficient(cut: float, eps: float = 1e-8) -> float:
