@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PMQ_ABI_VERSION 4
+#define PMQ_ABI_VERSION 5
 
 #if defined(__GNUC__) || defined(__clang__)
 #define PMQ_API __attribute__((visibility("default")))
@@ -59,7 +59,9 @@ enum {
     PMQ_TERM_MPOLE = 1u << 0, /* empole: permanent multipoles */
     PMQ_TERM_POLAR = 1u << 1, /* epolar: induced dipoles (mutual, PCG) */
     PMQ_TERM_REPEL = 1u << 2, /* erepel: HIPPO Pauli repulsion */
-    PMQ_TERM_DISP = 1u << 3   /* edisp:  HIPPO damped dispersion */
+    PMQ_TERM_DISP = 1u << 3,  /* edisp:  HIPPO damped dispersion */
+    PMQ_TERM_VDW = 1u << 4,   /* ehal:   AMOEBA buffered 14-7 van der Waals */
+    PMQ_TERM_CTRN = 1u << 5   /* echgtrn: HIPPO charge transfer */
 };
 
 /* arithmetic of the pair/PME kernels */
@@ -172,6 +174,36 @@ typedef struct {
      * no longer than the real-space cutoff); without it z = a r. */
     int pcgprec;
     double usolvcut, udiag;
+    /* AMOEBA buffered 14-7 van der Waals (vdw.h, vdwpot.h, kvdw.cpp; the
+     * reference assigns these and has no energy, energy.cpp:268; the form is
+     * Tinker's published ehal):
+     *   rho = r/R, E = s eps (1.07/(rho + 0.07))^7 (1.12/(rho^7 + 0.12) - 2),
+     * r between the reduced sites (atom i at kred_i (x_i - x_ired) + x_ired),
+     * R and eps from the nvdw x nvdw tables of the two atoms' classes (the
+     * 1-4 tables for pairs listed 1-4), s the v2..v5 scale of the pair's
+     * connectivity (later classes override earlier, as the m scales),
+     * switched off between vdwtaper and vdwcut (absolute, A).  A pair counts
+     * when either reduced site is active (the atom or its partner in use[]).
+     * The gradient of a reduced site goes kred to the atom and 1 - kred to
+     * its partner.  vdwcut may exceed the electrostatic cutoff: the term
+     * builds its own pair structure over the reduced sites. */
+    int nvdw;                  /* classes (0 = no vdW) */
+    const int *vdwclass;       /* n: class index 0..nvdw-1 */
+    const double *radmin, *epsilon, *radmin4, *epsilon4; /* nvdw*nvdw, row-major */
+    const double *kred;        /* n reduction factors */
+    const int *ired;           /* n: 0-based reduction partner (the atom itself: none) */
+    double v2scale, v3scale, v4scale, v5scale;
+    double vdwcut, vdwtaper;
+    /* HIPPO charge transfer (ctrpot.h, kchgtrn.cpp; the reference assigns
+     * these and has no energy, energy.cpp:276; Tinker's published echgtrn):
+     *   E = -s f (chgct_i e^{-a_k r} + chgct_k e^{-a_i r}),
+     * a = dmpct (0 read as 1000), f = electric/dielec, s the m2..m5 scale of
+     * the pair, switched off between ctrntaper and ctrncut (absolute, A;
+     * ctrncut may not exceed the electrostatic cutoff when multipoles or
+     * polarization run in a periodic box).  A pair counts when either atom
+     * is active. */
+    const double *chgct, *dmpct; /* n each (NULL = no charge transfer) */
+    double ctrncut, ctrntaper;
 } pmq_params;
 
 typedef struct {
@@ -203,6 +235,10 @@ typedef struct {
      * -(f/2) mu_d(i) . E^p(i), nep = the sites with a nonzero aep */
     long long nep;
     double *aep;   /* n per-atom ep partition (analysis) */
+    /* van der Waals and charge transfer (esum and desum include them) */
+    double ev, ect;
+    double *dev;   /* 3n gradient of ev */
+    double *dect;  /* 3n gradient of ect */
 } pmq_result;
 
 PMQ_API int pmq_create(const pmq_config *cfg, pmq_ctx **ctx);

@@ -199,6 +199,8 @@ int orc_hippo(orc_hippo_sys *s, int want_grad, orc_hippo_out *o);
  *   charge transfer: E = -s f (c_i e^{-a_k r} + c_k e^{-a_i r}), s the 1-n
  *   multipole scale, f = electric/dielec, a = 0 read as 1000, switched off
  *   between ctrntaper and ctrncut.
+ *   A pair counts when either atom is active (mp->use); for vdW an atom
+ *   also counts as active when its reduction partner is.
  * ------------------------------------------------------------------- */
 typedef struct {
     orc_mpole_sys *mp;           /* coordinates, connectivity, box, m-scales, f */

@@ -1733,16 +1733,22 @@ int orc_vdwct(orc_vdwct_sys *s, int want_grad, orc_vdwct_out *o)
             for (int j = off[i]; j < off[i + 1]; j++) { vs[lst[j]] = vsc[q]; ms[lst[j]] = msc[q]; }
         }
         for (int j = o14[i]; j < o14[i + 1]; j++) is14[m->i14[j]] = 1;
+        const int *u = m->use;
+        const int ui = !u || u[i + 1], uvi = ui || (s->use_vdw && u[s->ired[i] + 1]);
         for (int k = i + 1; k < n; k++) {
-            if (s->use_vdw && vs[k] != 0.0) {
+            /* a pair counts when either atom is active; for vdW an atom's
+             * reduced site is also active when its reduction partner is */
+            const int uk = !u || u[k + 1];
+            if (s->use_vdw && vs[k] != 0.0 && (uvi || uk || u[s->ired[k] + 1])) {
                 double d[3] = {xr[3 * k] - xr[3 * i], xr[3 * k + 1] - xr[3 * i + 1], xr[3 * k + 2] - xr[3 * i + 2]};
                 if (m->box.kind != ORC_BOX_NONE) orc_image(&m->box, &d[0], &d[1], &d[2]);
                 const double r2 = dot3(d, d);
-                if (r2 <= s->vdwcut * s->vdwcut) {
+                const int t = s->jv[i] * s->nv + s->jv[k];
+                const double rv = is14[k] ? s->radmin4[t] : s->radmin[t];
+                const double ep = (is14[k] ? s->epsilon4[t] : s->epsilon[t]) * vs[k];
+                /* a pair with no well depth or no radius has no energy */
+                if (r2 <= s->vdwcut * s->vdwcut && ep != 0.0 && rv > 0.0) {
                     const double r = sqrt(r2);
-                    const int t = s->jv[i] * s->nv + s->jv[k];
-                    const double rv = is14[k] ? s->radmin4[t] : s->radmin[t];
-                    const double ep = (is14[k] ? s->epsilon4[t] : s->epsilon[t]) * vs[k];
                     double e, de, T, dT;
                     hal_pair(r, rv, ep, &e, &de);
                     taper5(r, s->vdwtaper, s->vdwcut, &T, &dT);
@@ -1768,7 +1774,7 @@ int orc_vdwct(orc_vdwct_sys *s, int want_grad, orc_vdwct_out *o)
                     }
                 }
             }
-            if (s->use_ctrn && ms[k] != 0.0) {
+            if (s->use_ctrn && ms[k] != 0.0 && (ui || uk)) {
                 double d[3] = {m->x[k] - m->x[i], m->y[k] - m->y[i], m->z[k] - m->z[i]};
                 if (m->box.kind != ORC_BOX_NONE) orc_image(&m->box, &d[0], &d[1], &d[2]);
                 const double r2 = dot3(d, d);

@@ -21,7 +21,7 @@ PMQ_OK = 0
 ERRORS = {-1: "PMQ_ERR_ARG", -2: "PMQ_ERR_HIP", -3: "PMQ_ERR_STATE", -4: "PMQ_ERR_UNSUPPORTED",
           -5: "PMQ_ERR_NAN", -6: "PMQ_ERR_NOCONV"}
 ENERGY, ANALYSIS, GRADIENT, VIRIAL = 0, 1, 2, 3
-TERM_MPOLE, TERM_POLAR, TERM_REPEL, TERM_DISP = 1, 2, 4, 8
+TERM_MPOLE, TERM_POLAR, TERM_REPEL, TERM_DISP, TERM_VDW, TERM_CTRN = 1, 2, 4, 8, 16, 32
 PLAIN_FP64, MIXED = 0, 1
 BOX_NONE, BOX_ORTHO, BOX_MONO, BOX_TRI, BOX_OCTA, BOX_DODECA = range(6)
 
@@ -70,7 +70,14 @@ class Params(C.Structure):
                 ("sizpr", _dp), ("dmppr", _dp), ("elepr", _dp),
                 ("r2scale", C.c_double), ("r3scale", C.c_double), ("r4scale", C.c_double), ("r5scale", C.c_double),
                 ("repcut", C.c_double), ("reptaper", C.c_double),
-                ("pcgprec", C.c_int), ("usolvcut", C.c_double), ("udiag", C.c_double)]
+                ("pcgprec", C.c_int), ("usolvcut", C.c_double), ("udiag", C.c_double),
+                # van der Waals, charge transfer
+                ("nvdw", C.c_int), ("vdwclass", _ip),
+                ("radmin", _dp), ("epsilon", _dp), ("radmin4", _dp), ("epsilon4", _dp),
+                ("kred", _dp), ("ired", _ip),
+                ("v2scale", C.c_double), ("v3scale", C.c_double), ("v4scale", C.c_double), ("v5scale", C.c_double),
+                ("vdwcut", C.c_double), ("vdwtaper", C.c_double),
+                ("chgct", _dp), ("dmpct", _dp), ("ctrncut", C.c_double), ("ctrntaper", C.c_double)]
 
 
 class Result(C.Structure):
@@ -79,7 +86,7 @@ class Result(C.Structure):
                 ("vir", C.c_double * 9), ("pcg_iters", C.c_int),
                 ("dem", _dp), ("dep", _dp), ("desum", _dp), ("aem", _dp), ("rpole", _dp),
                 ("uind", _dp), ("uinp", _dp), ("der", _dp), ("dedsp", _dp), ("nep", C.c_longlong),
-                ("aep", _dp)]
+                ("aep", _dp), ("ev", C.c_double), ("ect", C.c_double), ("dev", _dp), ("dect", _dp)]
 
 
 EXPORTS = ["pmq_create", "pmq_destroy", "pmq_last_error", "pmq_abi_version", "pmq_set_topology",

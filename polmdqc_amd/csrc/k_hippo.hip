@@ -23,6 +23,13 @@
 // B_n = D^n S^2 (rep_series in pair_mpole.h), switched off between
 // reptaper and repcut.  The torques go to the frames through k_torque.
 //
+// Charge transfer (Rackers and Ponder, JCP 150, 084104 (2019); the reference
+// assigns chgct and dmpct, kchgtrn.cpp, and has no energy, energy.cpp:276):
+//   E_ik = -s f (c_i e^{-a_k r} + c_k e^{-a_i r}),
+// f = electric/dielec, a = dmpct (0 read as 1000), s the pair's m scale,
+// switched off between ctrntaper and ctrncut; a pair counts when either
+// atom is active.
+//
 // Kernels follow the multipole tiles: every unscaled pair in k_*_tiles
 // (per-tile partial sums reduced per atom in a fixed order), and the scaled
 // pairs whole, in FP64, in k_*_excl.
@@ -33,7 +40,8 @@
 namespace pmq {
 
 __global__ void k_hippo_sort(int n, int npad, const int *perm, const double *csix, const double *adisp,
-                             const double *sizpr, const double *dmppr, const double *elepr, double *shp)
+                             const double *sizpr, const double *dmppr, const double *elepr, const double *chgct,
+                             const double *dmpct, const int *use, double *shp)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= npad) return;
@@ -43,6 +51,10 @@ __global__ void k_hippo_sort(int n, int npad, const int *perm, const double *csi
     shp[2 * npad + s] = i >= 0 && sizpr ? sizpr[i] : 0.0;
     shp[3 * npad + s] = i >= 0 && dmppr ? dmppr[i] : 1.0;
     shp[4 * npad + s] = i >= 0 && elepr ? elepr[i] : 0.0;
+    shp[5 * npad + s] = i >= 0 && chgct ? chgct[i] : 0.0;
+    const double a = i >= 0 && dmpct ? dmpct[i] : 0.0;
+    shp[6 * npad + s] = a != 0.0 ? a : 1000.0;
+    shp[7 * npad + s] = i >= 0 && use[i] ? 1.0 : 0.0;
 }
 
 int hippo_sort(pmq_ctx *c)
@@ -51,7 +63,8 @@ int hippo_sort(pmq_ctx *c)
     hipLaunchKernelGGL(k_hippo_sort, dim3((np + 255) / 256), dim3(256), 0, c->stream, c->n, np, c->perm.p,
                        c->have_disp ? c->csix.p : nullptr, c->have_disp ? c->adisp.p : nullptr,
                        c->have_rep ? c->sizpr.p : nullptr, c->have_rep ? c->dmppr.p : nullptr,
-                       c->have_rep ? c->elepr.p : nullptr, c->shp.p);
+                       c->have_rep ? c->elepr.p : nullptr, c->have_ct ? c->chgct.p : nullptr,
+                       c->have_ct ? c->dmpct.p : nullptr, c->use_raw.p, c->shp.p);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }
@@ -64,6 +77,7 @@ struct HippoArgs {
     const int *perm, *iperm;
     DevBox box;
     double off2, dewald;   // term cutoff^2; dispersion Ewald coefficient (0 = none)
+    double felec;          // electric / dielec (charge transfer)
     Taper taper;
     const unsigned long long *mask; // 128 per tile: [0,64) row masks of the in-cutoff pairs, scaled pairs left out
     void *part;            // per tile partial sums
@@ -346,19 +360,129 @@ __global__ __launch_bounds__(64) void k_rep_excl(HippoArgs a)
     }
 }
 
+// charge-transfer pair at scale-times-f sf: energy and dE/dr
+template <class T>
+__device__ __forceinline__ void ct_pair(T r, T ci, T ck, T ai, T ak, T sf, const Taper &tp, T &e, T &de)
+{
+    const T ei = exp(-ai * r), ek = exp(-ak * r);
+    const T e0 = -sf * (ci * ek + ck * ei);
+    const T de0 = sf * (ci * ak * ek + ck * ai * ei);
+    T tv, tdv;
+    taper_eval<T>(tp, r, tv, tdv);
+    e = e0 * tv;
+    de = de0 * tv + e0 * tdv;
+}
+
+template <class T>
+__global__ __launch_bounds__(64) void k_ct_tiles(HippoArgs a)
+{
+    // side 0: block I, side 1: block J; compacted pair schedule (tile_pairs.h)
+    __shared__ double bx[2][64], by[2][64], bz[2][64];
+    __shared__ T bc[2][64], ba[2][64];
+    __shared__ int bu[2][64];
+    __shared__ double acc[2][3][64];
+    __shared__ unsigned short plist[64 * 64];
+    const int lane = threadIdx.x, np = a.npad;
+    const int ntiles = *a.ntiles;
+    const T f = T(a.felec);
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 IJ = a.tiles[t];
+        __syncthreads();
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const int s = (side ? IJ.y : IJ.x) * 64 + lane;
+            bx[side][lane] = a.sx[s]; by[side][lane] = a.sy[s]; bz[side][lane] = a.sz[s];
+            bc[side][lane] = a.shp[5 * np + s];
+            ba[side][lane] = a.shp[6 * np + s];
+            bu[side][lane] = a.shp[7 * np + s] != 0.0;
+            acc[side][0][lane] = acc[side][1][lane] = acc[side][2][lane] = 0.0;
+        }
+        const int npair = tile_pair_list(a.mask[(size_t)t * 128 + lane], plist, lane);
+        __syncthreads();
+        double e = 0.0;
+        Vir6 vir;
+        for (int base = 0; base < npair; base += 64) {
+            if (base + lane >= npair) break;
+            const int ent = plist[base + lane];
+            const int il = ent >> 8, jl = ent & 63;
+            double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
+            image<double>(a.box, xd, yd, zd);
+            const double r2d = xd * xd + yd * yd + zd * zd;
+            const T ci = bc[0][il], ck = bc[1][jl];
+            if (r2d > a.off2 || !(bu[0][il] | bu[1][jl]) || (ci == T(0) && ck == T(0))) continue;
+            const T r = T(sqrt(r2d));
+            T ep, de;
+            ct_pair<T>(r, ci, ck, ba[0][il], ba[1][jl], f, a.taper, ep, de);
+            e += (double)ep;
+            const double g = (double)(de / r); // dE/dx_k = g r_vec
+            atomicAdd(&acc[0][0][il], -g * xd);
+            atomicAdd(&acc[0][1][il], -g * yd);
+            atomicAdd(&acc[0][2][il], -g * zd);
+            atomicAdd(&acc[1][0][jl], g * xd);
+            atomicAdd(&acc[1][1][jl], g * yd);
+            atomicAdd(&acc[1][2][jl], g * zd);
+            if (a.vir) vir.pair(xd, yd, zd, -g * xd, -g * yd, -g * zd);
+        }
+        __syncthreads();
+        T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
+        T *Q = P + PMQ_TILE_NV * 64;
+        for (int c = 0; c < 3; c++) {
+            P[c * 64 + lane] = T(acc[0][c][lane]);
+            Q[c * 64 + lane] = T(acc[1][c][lane]);
+        }
+        const double ew = wsum(e);
+        if (a.vir) vir.wave_sum();
+        if (lane == 0) {
+            a.tscal[t] = ew; // scalar 0 of tile t
+            if (a.vir) vir.store_tile(a.tscal, a.tstride, t);
+        }
+    }
+}
+
+// scaled pairs, whole, at their m scale
+__global__ __launch_bounds__(64) void k_ct_excl(HippoArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nexcl) return;
+    const ExclPair q = a.excl[t];
+    if (q.m == 0.0) return;
+    const int np = a.npad;
+    const int si = a.iperm[q.i], sk = a.iperm[q.k];
+    const double ci = a.shp[5 * np + si], ck = a.shp[5 * np + sk];
+    if ((ci == 0.0 && ck == 0.0) || !(a.shp[7 * np + si] != 0.0 || a.shp[7 * np + sk] != 0.0)) return;
+    double xr = a.sx[sk] - a.sx[si], yr = a.sy[sk] - a.sy[si], zr = a.sz[sk] - a.sz[si];
+    image<double>(a.box, xr, yr, zr);
+    const double r2 = xr * xr + yr * yr + zr * zr;
+    if (r2 > a.off2) return;
+    const double r = sqrt(r2);
+    double e, de;
+    ct_pair<double>(r, ci, ck, a.shp[6 * np + si], a.shp[6 * np + sk], q.m * a.felec, a.taper, e, de);
+    atomicAdd(&a.tot->ect, e);
+    const double g = de / r;
+    grad_add(a.de, a.deq, 3 * q.k + 0, g * xr); grad_add(a.de, a.deq, 3 * q.k + 1, g * yr);
+    grad_add(a.de, a.deq, 3 * q.k + 2, g * zr);
+    grad_add(a.de, a.deq, 3 * q.i + 0, -g * xr); grad_add(a.de, a.deq, 3 * q.i + 1, -g * yr);
+    grad_add(a.de, a.deq, 3 * q.i + 2, -g * zr);
+    if (a.vir) {
+        Vir6 v;
+        v.pair(xr, yr, zr, -g * xr, -g * yr, -g * zr);
+        a.tot->add_vir(v);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 
-// quintic switch coefficients between cut and off (cutoffSwitch.cpp:125-133);
-// no switch when cut >= off
-static Taper make_taper(double cut, double off)
+Taper make_taper(double cut, double off)
 {
     Taper tp;
     tp.cut = 1e300;
+    tp.w = 0.0;
     for (int q = 0; q < 6; q++) tp.c[q] = 0.0;
     if (!(cut < off)) return tp;
     const double den = pow(off - cut, 5), off2 = off * off, cut2 = cut * cut;
     tp.cut = cut;
+    tp.w = 1.0 / (off - cut);
     tp.c[0] = off * off2 * (off2 - 5.0 * off * cut + 10.0 * cut2) / den;
     tp.c[1] = -30.0 * off2 * cut2 / den;
     tp.c[2] = 30.0 * (off2 * cut + off * cut2) / den;
@@ -378,6 +502,7 @@ static HippoArgs hippo_args(pmq_ctx *c, double off2)
     a.box = c->box;
     a.off2 = off2;
     a.dewald = 0.0;
+    a.felec = c->prm.electric / c->prm.dielec;
     a.taper = make_taper(1e300, 0.0);
     a.mask = c->xmask.p;
     a.part = c->tpart.p;
@@ -469,6 +594,44 @@ int launch_rep(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir)
         if ((rc = launch_fx_flush_pair(c, c->fx_grad.p, c->der.p, c->ter.p, 3 * (size_t)c->n, PMQ_FX_GRAD)))
             return rc;
     }
+    return PMQ_OK;
+}
+
+} // namespace pmq
+
+namespace pmq {
+
+int launch_ct(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir)
+{
+    HippoArgs a = hippo_args(c, off2);
+    a.vir = vir ? 1 : 0;
+    a.taper = make_taper(c->prm.ctrntaper, sqrt(off2));
+    a.de = c->dect.p;
+    a.deq = mixed ? c->fx_grad.p : nullptr;
+    {
+        ProfScope ps_(c, "ct_tiles");
+        if (mixed) hipLaunchKernelGGL((k_ct_tiles<float>), dim3(c->tgrid), dim3(64), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_ct_tiles<double>), dim3(c->tgrid), dim3(64), 0, c->stream, a);
+    }
+    PMQ_HIP(hipGetLastError());
+    int rc;
+    if (grad) {
+        TileReduceArgs r{};
+        for (int q = 0; q < 3; q++) {
+            r.out[q] = c->dect.p;
+            r.caller_order[q] = 1;
+            r.stride[q] = 3;
+            r.off[q] = q;
+            r.fxscale[q] = PMQ_FX_GRAD;
+        }
+        if ((rc = tile_reduce(c, r, 3, mixed, mixed))) return rc;
+    }
+    if ((rc = tile_scalars(c, 5, false))) return rc; // energy component 5: ect
+    if (vir && (rc = tile_virial(c))) return rc;
+    if (a.nexcl) hipLaunchKernelGGL(k_ct_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
+    PMQ_HIP(hipGetLastError());
+    if (mixed && grad && (rc = launch_fx_flush(c, c->fx_grad.p, c->dect.p, 3 * (size_t)c->n, PMQ_FX_GRAD)))
+        return rc;
     return PMQ_OK;
 }
 

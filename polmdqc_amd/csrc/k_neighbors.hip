@@ -301,13 +301,20 @@ __global__ void k_tile_write(int nblk, long long npairs, const int *__restrict__
     tiles[scan[t]] = make_int2(I, J);
 }
 
-int build_neighbors(pmq_ctx *c, double cutoff)
+int build_neighbors(pmq_ctx *c, double cutoff, const double *px, const double *py, const double *pz,
+                    const int *use)
 {
     const int n = c->n, npad = c->npad, nblk = c->nblk;
     hipStream_t st = c->stream;
     const DevBox &b = c->box;
+    if (!px) {
+        px = c->x.p;
+        py = c->y.p;
+        pz = c->z.p;
+    }
+    if (!use) use = c->usef.p;
     if (b.kind == PMQ_BOX_NONE) {
-        hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, st, n, c->x.p, c->y.p, c->z.p, c->bbox.p);
+        hipLaunchKernelGGL(k_bbox, dim3(1), dim3(1024), 0, st, n, px, py, pz, c->bbox.p);
     }
     // cells about 4 A wide (at most 1024 per axis)
     int nc[3];
@@ -326,11 +333,10 @@ int build_neighbors(pmq_ctx *c, double cutoff)
             g.K[a] = c->K[a];
             g.nb[a] = c->pme_nb[a];
         }
-        hipLaunchKernelGGL(k_brick_keys, dim3((n + tb - 1) / tb), dim3(tb), 0, st, n, g, c->x.p, c->y.p, c->z.p,
-                           c->keys.p);
+        hipLaunchKernelGGL(k_brick_keys, dim3((n + tb - 1) / tb), dim3(tb), 0, st, n, g, px, py, pz, c->keys.p);
     } else {
         hipLaunchKernelGGL(k_cell_keys, dim3((n + tb - 1) / tb), dim3(tb), 0, st, n, b, c->bbox.p, nc[0], nc[1],
-                           nc[2], c->x.p, c->y.p, c->z.p, c->keys.p);
+                           nc[2], px, py, pz, c->keys.p);
     }
     // The low 32 bits of a key hold the atom index and keys are written in
     // index order, so a stable sort on the Morton bits alone gives the same
@@ -350,9 +356,8 @@ int build_neighbors(pmq_ctx *c, double cutoff)
     bytes = c->scratch_sort_bytes;
     PMQ_HIP(rocprim::radix_sort_keys(c->scratch_sort.p, bytes, c->keys.p, c->keys_sorted.p, (size_t)n, 32u, end_bit,
                                      st));
-    hipLaunchKernelGGL(k_gather, dim3((npad + tb - 1) / tb), dim3(tb), 0, st, n, npad, c->keys_sorted.p, c->x.p,
-                       c->y.p, c->z.p, c->usef.p, c->molcule.p, c->perm.p, c->iperm.p, c->sx.p, c->sy.p, c->sz.p,
-                       c->sflag.p, c->smol.p);
+    hipLaunchKernelGGL(k_gather, dim3((npad + tb - 1) / tb), dim3(tb), 0, st, n, npad, c->keys_sorted.p, px, py, pz,
+                       use, c->molcule.p, c->perm.p, c->iperm.p, c->sx.p, c->sy.p, c->sz.p, c->sflag.p, c->smol.p);
     hipLaunchKernelGGL(k_block_bounds, dim3(nblk), dim3(PMQ_BLOCK), 0, st, n, nblk, b, c->bbox.p, c->sx.p, c->sy.p,
                        c->sz.p, c->blk.p);
     const long long npairs = (long long)nblk * (nblk + 1) / 2;

@@ -761,7 +761,7 @@ __global__ __launch_bounds__(64 * RED_W) void k_tile_reduce(TileReduceArgs a)
 }
 
 // fixed-order sums of the per-tile scalars into the totals: scalar 0 into
-// energy component `which` (0 em, 1 ep, 2 er, 3 edsp); with `full` also the
+// energy component `which` (0 em, 1 ep, 2 er, 3 edsp, 4 ev, 5 ect); with `full` also the
 // virial (1..9), the inter-molecular energy (10) and the pair count (11)
 constexpr int SCAL_TB = 1024;
 

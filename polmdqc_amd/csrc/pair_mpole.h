@@ -665,17 +665,33 @@ __device__ __forceinline__ void rep_series(R r, R ai, R ak, R *B)
 // taper distance and the cutoff (cutoffSwitch.cpp:125-133), and dT/dr
 struct Taper {
     double cut, c[6];
+    double w; // 1 / (off - cut)
 };
 
+// quintic switch coefficients between cut and off (cutoffSwitch.cpp:125-133);
+// no switch when cut >= off (k_hippo.hip)
+Taper make_taper(double cut, double off);
+
+// the switch T(r) and dT/dr beyond cut.  FP64 sums the power series of
+// cutoffSwitch.cpp; FP32 the same polynomial in u = (r - cut) / (off - cut),
+// T = 1 - 10 u^3 + 15 u^4 - 6 u^5 (the one quintic with T = 1 at cut, 0 at
+// off and zero first and second derivatives at both), whose terms stay near
+// 1 where the power series' terms reach off^5 / (off - cut)^5 and cancel
 template <class R>
 __device__ __forceinline__ void taper_eval(const Taper &tp, R r, R &T, R &dT)
 {
     T = R(1);
     dT = R(0);
     if (r <= R(tp.cut)) return;
-    const R c0 = R(tp.c[0]), c1 = R(tp.c[1]), c2 = R(tp.c[2]), c3 = R(tp.c[3]), c4 = R(tp.c[4]), c5 = R(tp.c[5]);
-    T = c0 + r * (c1 + r * (c2 + r * (c3 + r * (c4 + r * c5))));
-    dT = c1 + r * (R(2) * c2 + r * (R(3) * c3 + r * (R(4) * c4 + r * R(5) * c5)));
+    if constexpr (sizeof(R) == 4) {
+        const R w = R(tp.w), u = (r - R(tp.cut)) * w, u2 = u * u;
+        T = R(1) + u2 * u * (R(-10) + u * (R(15) - R(6) * u));
+        dT = u2 * (R(-30) + u * (R(60) - R(30) * u)) * w;
+    } else {
+        const R c0 = R(tp.c[0]), c1 = R(tp.c[1]), c2 = R(tp.c[2]), c3 = R(tp.c[3]), c4 = R(tp.c[4]), c5 = R(tp.c[5]);
+        T = c0 + r * (c1 + r * (c2 + r * (c3 + r * (c4 + r * c5))));
+        dT = c1 + r * (R(2) * c2 + r * (R(3) * c3 + r * (R(4) * c4 + r * R(5) * c5)));
+    }
 }
 
 } // namespace pmq

@@ -325,6 +325,9 @@ int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
             }
     // an atom takes part when it or one of its frame atoms is active (empole.cpp:221,242-243)
     c->h_usef.assign(n, 1);
+    c->h_use.assign(n, 1);
+    if (t->use)
+        for (int i = 0; i < n; i++) c->h_use[i] = t->use[i + 1] ? 1 : 0;
     if (t->use) {
         for (int i = 0; i < n; i++) {
             const int iz = t->zaxis[i], ix = t->xaxis[i], iy = std::abs(t->yaxis[i]);
@@ -350,8 +353,11 @@ int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
             }
         }
         std::map<int, std::pair<int, int>> cls; // k -> (bonded class, group class)
+        std::vector<int> l14;                   // listed 1-4 of i (the vdW 1-4 tables)
         for (int i = 0; i < n; i++) {
             cls.clear();
+            l14.clear();
+            for (long long j = off[2][i]; j < off[2][i + 1]; j++) l14.push_back(lst[2][j]);
             for (int q = 0; q < 4; q++)
                 for (long long j = off[q][i]; j < off[q][i + 1]; j++) {
                     const int k = lst[q][j];
@@ -381,6 +387,7 @@ int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
                     e.mcls = kv.second.first;
                     e.gcls = kv.second.second;
                     e.intra = std::find(same.begin(), same.end(), kv.first) != same.end() ? 1 : 0;
+                    e.in14 = std::find(l14.begin(), l14.end(), kv.first) != l14.end() ? 1 : 0;
                     e.m = e.p = e.d = e.u = 1.0;
                     c->h_excl.push_back(e);
                 }
@@ -393,6 +400,7 @@ int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
     if ((rc = upload(c, c->yaxis, t->yaxis, n))) return rc;
     if ((rc = upload(c, c->molcule, c->h_molcule.data(), n))) return rc;
     if ((rc = upload(c, c->usef, c->h_usef.data(), n))) return rc;
+    if ((rc = upload(c, c->use_raw, c->h_use.data(), n))) return rc;
     c->nbond = t->nbond;
     c->nangle = t->nangle;
     if ((rc = upload(c, c->ibnd, t->ibnd, 2 * (size_t)t->nbond))) return rc;
@@ -516,6 +524,8 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
         e.u = p->pentype != PMQ_PEN_NONE ? wv[e.mcls] : uv[e.gcls];
         e.dsp = dsv[e.mcls];
         e.rep = rv[e.mcls];
+        const double vv[6] = {1, 1, p->v2scale, p->v3scale, p->v4scale, p->v5scale};
+        e.v = vv[e.mcls];
     }
     if ((rc = upload(c, c->excl, c->h_excl.data(), c->h_excl.size()))) return rc;
     const bool polar = p->polarity != nullptr;
@@ -584,8 +594,70 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
         if ((rc = upload(c, c->dmppr, p->dmppr, n))) return rc;
         if ((rc = upload(c, c->elepr, p->elepr, n))) return rc;
     }
-    if (c->have_disp || c->have_rep) {
-        PMQ_HIP(c->shp.alloc(5 * (size_t)c->npad));
+    // van der Waals
+    c->have_vdw = p->nvdw > 0;
+    if (c->have_vdw) {
+        const int nv = p->nvdw;
+        if (!p->vdwclass || !p->radmin || !p->epsilon || !p->radmin4 || !p->epsilon4 || !p->kred || !p->ired ||
+            !(p->vdwcut > 0.0)) {
+            c->err = "pmq_set_params: van der Waals needs vdwclass, the four tables, kred, ired and vdwcut > 0";
+            return PMQ_ERR_ARG;
+        }
+        if (nv > 4096) {
+            c->err = "pmq_set_params: at most 4096 van der Waals classes";
+            return PMQ_ERR_ARG;
+        }
+        std::vector<int> vuse(n), koff(n + 1, 0), kid;
+        for (int i = 0; i < n; i++) {
+            if (p->vdwclass[i] < 0 || p->vdwclass[i] >= nv || p->ired[i] < 0 || p->ired[i] >= n ||
+                !std::isfinite(p->kred[i])) {
+                c->err = "pmq_set_params: van der Waals class, reduction partner or factor out of range";
+                return PMQ_ERR_ARG;
+            }
+            // a reduced site is active when its atom or the atom's partner is
+            vuse[i] = c->h_use[i] || c->h_use[p->ired[i]];
+            if (p->ired[i] != i) koff[p->ired[i] + 1]++;
+        }
+        for (int i = 0; i < n; i++) koff[i + 1] += koff[i];
+        kid.assign(std::max(1, koff[n]), 0);
+        {
+            std::vector<int> fill(koff.begin(), koff.end() - 1);
+            for (int j = 0; j < n; j++)
+                if (p->ired[j] != j) kid[fill[p->ired[j]]++] = j; // ascending j within each list
+        }
+        std::vector<double> tab(4 * (size_t)nv * nv);
+        const double *src[4] = {p->radmin, p->epsilon, p->radmin4, p->epsilon4};
+        for (int q = 0; q < 4; q++) std::copy(src[q], src[q] + (size_t)nv * nv, tab.begin() + (size_t)q * nv * nv);
+        if ((rc = upload(c, c->vcls, p->vdwclass, n))) return rc;
+        if ((rc = upload(c, c->ired, p->ired, n))) return rc;
+        if ((rc = upload(c, c->kred, p->kred, n))) return rc;
+        if ((rc = upload(c, c->vuse, vuse.data(), n))) return rc;
+        if ((rc = upload(c, c->vkid_off, koff.data(), n + 1))) return rc;
+        if ((rc = upload(c, c->vkid, kid.data(), kid.size()))) return rc;
+        if ((rc = upload(c, c->vtab, tab.data(), tab.size()))) return rc;
+        PMQ_HIP(c->vx.alloc(n)); PMQ_HIP(c->vy.alloc(n)); PMQ_HIP(c->vz.alloc(n));
+        PMQ_HIP(c->gvr.alloc(3 * (size_t)n)); PMQ_HIP(c->dev.alloc(3 * (size_t)n));
+        c->nvdw = nv;
+        PMQ_HIP(hipStreamSynchronize(c->stream)); // the copies read these host vectors
+    }
+    // charge transfer
+    c->have_ct = p->chgct != nullptr;
+    if (c->have_ct) {
+        if (!p->dmpct || !(p->ctrncut > 0.0)) {
+            c->err = "pmq_set_params: charge transfer needs chgct, dmpct and ctrncut > 0";
+            return PMQ_ERR_ARG;
+        }
+        for (int i = 0; i < n; i++)
+            if (!(p->dmpct[i] >= 0.0) || !std::isfinite(p->chgct[i])) {
+                c->err = "pmq_set_params: charge-transfer damping exponents must be >= 0";
+                return PMQ_ERR_ARG;
+            }
+        if ((rc = upload(c, c->chgct, p->chgct, n))) return rc;
+        if ((rc = upload(c, c->dmpct, p->dmpct, n))) return rc;
+        PMQ_HIP(c->dect.alloc(3 * (size_t)n));
+    }
+    if (c->have_disp || c->have_rep || c->have_ct) {
+        PMQ_HIP(c->shp.alloc(PMQ_SHP_ROWS * (size_t)c->npad));
         PMQ_HIP(c->der.alloc(3 * (size_t)n));
         PMQ_HIP(c->dedsp.alloc(3 * (size_t)n));
         PMQ_HIP(c->ter.alloc(3 * (size_t)n));
@@ -600,6 +672,9 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
     c->prm.bl = c->prm.bflx = c->prm.anat = c->prm.aflx = c->prm.abflx = nullptr;
     c->prm.polarity = c->prm.thole = c->prm.pdamp = nullptr;
     c->prm.csix = c->prm.adisp = c->prm.sizpr = c->prm.dmppr = c->prm.elepr = nullptr;
+    c->prm.vdwclass = c->prm.ired = nullptr;
+    c->prm.radmin = c->prm.epsilon = c->prm.radmin4 = c->prm.epsilon4 = c->prm.kred = nullptr;
+    c->prm.chgct = c->prm.dmpct = nullptr;
     PMQ_HIP(hipStreamSynchronize(c->stream));
     c->have_prm = true;
     batch_graph_release(c);
@@ -656,13 +731,25 @@ static int check_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
         return PMQ_ERR_ARG;
     }
     const bool mixed = precision == PMQ_MIXED;
-    if (mixed && (mode == PMQ_ANALYSIS || c->prm.use_chgflx)) {
+    // charge flux moves only the monopoles, which the multipole and
+    // polarization terms see
+    if (mixed && (mode == PMQ_ANALYSIS || (c->prm.use_chgflx && (terms & (PMQ_TERM_MPOLE | PMQ_TERM_POLAR))))) {
         c->err = "pmq_compute: the mixed form computes energy, gradient and virial without charge flux";
         return PMQ_ERR_UNSUPPORTED;
     }
-    if (terms & ~(unsigned)(PMQ_TERM_MPOLE | PMQ_TERM_POLAR | PMQ_TERM_REPEL | PMQ_TERM_DISP)) {
+    if (terms & ~(unsigned)(PMQ_TERM_MPOLE | PMQ_TERM_POLAR | PMQ_TERM_REPEL | PMQ_TERM_DISP | PMQ_TERM_VDW |
+                            PMQ_TERM_CTRN)) {
         c->err = "pmq_compute: unknown term";
         return PMQ_ERR_ARG;
+    }
+    const bool do_vdw = terms & PMQ_TERM_VDW, do_ct = terms & PMQ_TERM_CTRN;
+    if ((do_vdw && !c->have_vdw) || (do_ct && !c->have_ct)) {
+        c->err = "pmq_compute: van der Waals or charge transfer requested but its parameters were not set";
+        return PMQ_ERR_STATE;
+    }
+    if ((do_vdw || do_ct) && mode == PMQ_ANALYSIS) {
+        c->err = "pmq_compute: van der Waals and charge transfer compute energy, gradient and virial (no analysis)";
+        return PMQ_ERR_UNSUPPORTED;
     }
     const bool do_mpole = terms & PMQ_TERM_MPOLE, do_polar = terms & PMQ_TERM_POLAR;
     const bool do_rep = terms & PMQ_TERM_REPEL, do_disp = terms & PMQ_TERM_DISP;
@@ -674,13 +761,14 @@ static int check_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
         c->err = "pmq_compute: repulsion and dispersion compute energy, gradient and virial (no analysis)";
         return PMQ_ERR_UNSUPPORTED;
     }
-    if (do_rep || do_disp) {
+    if (do_rep || do_disp || do_ct) {
         // the pair structure is built at the electrostatic cutoff when those
-        // terms run; repcut and dispcut must fit inside it
+        // terms run; repcut, dispcut and ctrncut must fit inside it
         const double ecut = c->prm.use_ewald ? c->prm.ewaldcut : c->prm.mpolecut;
         if ((do_mpole || do_polar) && c->box.kind != PMQ_BOX_NONE &&
-            ((do_rep && c->prm.repcut > ecut) || (do_disp && c->prm.dispcut > ecut))) {
-            c->err = "pmq_compute: repcut and dispcut may not exceed the electrostatic cutoff";
+            ((do_rep && c->prm.repcut > ecut) || (do_disp && c->prm.dispcut > ecut) ||
+             (do_ct && c->prm.ctrncut > ecut))) {
+            c->err = "pmq_compute: repcut, dispcut and ctrncut may not exceed the electrostatic cutoff";
             return PMQ_ERR_UNSUPPORTED;
         }
     }
@@ -704,6 +792,7 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
     const bool mixed = precision == PMQ_MIXED;
     const bool do_mpole = terms & PMQ_TERM_MPOLE, do_polar = terms & PMQ_TERM_POLAR;
     const bool do_rep = terms & PMQ_TERM_REPEL, do_disp = terms & PMQ_TERM_DISP;
+    const bool do_vdw = terms & PMQ_TERM_VDW, do_ct = terms & PMQ_TERM_CTRN;
     const bool ewald = c->prm.use_ewald;
     const bool dewald = do_disp && c->prm.use_dewald && c->box.kind != PMQ_BOX_NONE;
     PMQ_HIP(hipSetDevice(c->device));
@@ -734,16 +823,22 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
                 add(c->ter.p, 3 * (size_t)n);
             }
             if (do_disp) add(c->dedsp.p, 3 * (size_t)n);
+            if (do_vdw) add(c->gvr.p, 3 * (size_t)n);
+            if (do_ct) add(c->dect.p, 3 * (size_t)n);
         }
         if (mode == PMQ_ANALYSIS) add(c->aem.p, n);
         if ((rc = launch_zero_many(c, k, zp, zn))) return rc;
     }
     c->pcg_iters = 0;
-    if (do_mpole || do_polar || do_rep || do_disp) {
+    // van der Waals first: its structure over the reduced sites is replaced
+    // by the one the other terms build (which pmq_param_grad reuses)
+    if (do_vdw && (rc = vdw_pass(c, grad, mixed, vir, safe))) return rc;
+    if (do_mpole || do_polar || do_rep || do_disp || do_ct) {
         double cut = ewald ? c->prm.ewaldcut : c->prm.mpolecut;
-        if (!(do_mpole || do_polar)) cut = std::max(do_rep ? c->prm.repcut : 0.0, do_disp ? c->prm.dispcut : 0.0);
-        if (c->box.kind == PMQ_BOX_NONE)
-            cut = std::min(std::max({cut, do_rep ? c->prm.repcut : 0.0, do_disp ? c->prm.dispcut : 0.0}), 1e12);
+        const double hcut = std::max({do_rep ? c->prm.repcut : 0.0, do_disp ? c->prm.dispcut : 0.0,
+                                      do_ct ? c->prm.ctrncut : 0.0});
+        if (!(do_mpole || do_polar)) cut = hcut;
+        if (c->box.kind == PMQ_BOX_NONE) cut = std::min(std::max(cut, hcut), 1e12);
         const double off2 = cut * cut;
         const bool cf = c->prm.pentype != PMQ_PEN_NONE && c->prm.use_chgflx;
         if (c->prm.use_chgflx && (rc = launch_chgflx_delta(c))) return rc;
@@ -759,7 +854,7 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
             if ((rc = tile_structure(c, off2, safe))) return rc;
         }
         if ((rc = launch_rotpole(c, true))) return rc;
-        if ((do_rep || do_disp) && (rc = hippo_sort(c))) return rc;
+        if ((do_rep || do_disp || do_ct) && (rc = hippo_sort(c))) return rc;
         if (do_rep) {
             const double rc2 = std::min(c->prm.repcut * c->prm.repcut, off2);
             if ((rc = launch_rep(c, rc2, grad, mixed, vir))) return rc;
@@ -768,6 +863,10 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
         if (do_disp) {
             const double dc2 = std::min(c->prm.dispcut * c->prm.dispcut, off2);
             if ((rc = launch_disp(c, dc2, grad, mixed, vir))) return rc;
+        }
+        if (do_ct) {
+            const double cc2 = std::min(c->prm.ctrncut * c->prm.ctrncut, off2);
+            if ((rc = launch_ct(c, cc2, grad, mixed, vir))) return rc;
         }
         if (do_polar && (rc = polar_prepare(c, off2, mixed, safe))) return rc;
         if (do_mpole && (rc = launch_mpole_direct(c, mode, off2, mixed))) return rc;
@@ -837,7 +936,7 @@ static int batch_check(pmq_ctx *c, int mode, unsigned terms)
         c->err = "pmq_compute: batched clusters compute energy and gradient";
         return PMQ_ERR_UNSUPPORTED;
     }
-    if (terms & (PMQ_TERM_REPEL | PMQ_TERM_DISP)) {
+    if (terms & (PMQ_TERM_REPEL | PMQ_TERM_DISP | PMQ_TERM_VDW | PMQ_TERM_CTRN)) {
         c->err = "pmq_compute: batched clusters compute the multipole and polarization terms";
         return PMQ_ERR_UNSUPPORTED;
     }
@@ -1181,7 +1280,9 @@ int pmq_fetch(pmq_ctx *c, pmq_result *r)
     r->ep = t.ep;
     r->er = t.er;
     r->edsp = t.edsp;
-    r->esum = t.em + t.ep + t.er + t.edsp;
+    r->ev = t.ev;
+    r->ect = t.ect;
+    r->esum = t.em + t.ep + t.er + t.edsp + t.ev + t.ect;
     // with Ewald the kernels sum the intramolecular pairs' own energies:
     // the intermolecular part is the rest of em
     const bool ewald_ana = c->last_mode == PMQ_ANALYSIS && c->prm.use_ewald && (c->last_terms & PMQ_TERM_MPOLE);
@@ -1225,6 +1326,22 @@ int pmq_fetch(pmq_ctx *c, pmq_result *r)
         for (size_t j = 0; j < 3 * (size_t)n; j++) r->dedsp[j] = dsp ? gd[j] : 0.0;
     if (r->desum && grad)
         for (size_t j = 0; j < 3 * (size_t)n; j++) r->desum[j] += (rp ? gr[j] : 0.0) + (dsp ? gd[j] : 0.0);
+    const bool vd = c->last_terms & PMQ_TERM_VDW, ct = c->last_terms & PMQ_TERM_CTRN;
+    std::vector<double> gv, gc;
+    if (grad && vd) {
+        gv.resize(3 * (size_t)n);
+        PMQ_HIP(hipMemcpy(gv.data(), c->dev.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (grad && ct) {
+        gc.resize(3 * (size_t)n);
+        PMQ_HIP(hipMemcpy(gc.data(), c->dect.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (r->dev && grad)
+        for (size_t j = 0; j < 3 * (size_t)n; j++) r->dev[j] = vd ? gv[j] : 0.0;
+    if (r->dect && grad)
+        for (size_t j = 0; j < 3 * (size_t)n; j++) r->dect[j] = ct ? gc[j] : 0.0;
+    if (r->desum && grad)
+        for (size_t j = 0; j < 3 * (size_t)n; j++) r->desum[j] += (vd ? gv[j] : 0.0) + (ct ? gc[j] : 0.0);
     if (pol && r->uind) PMQ_HIP(hipMemcpy(r->uind, c->uind_out.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
     if (pol && r->uinp)
         PMQ_HIP(hipMemcpy(r->uinp, c->uind_out.p + 3 * (size_t)n, 3 * n * sizeof(double), hipMemcpyDeviceToHost));

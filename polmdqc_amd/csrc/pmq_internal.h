@@ -191,6 +191,7 @@ struct Vir6 {
 // q * tile capacity + t) so each scalar's sum reads one contiguous array
 constexpr int PMQ_TILE_NV = 8;
 constexpr int PMQ_TILE_NS = 12;
+constexpr int PMQ_SHP_ROWS = 8; // rows of the sorted per-atom HIPPO, charge-transfer and activity table
 
 struct TileReduceArgs {
     int nblk, n;
@@ -206,7 +207,7 @@ struct TileReduceArgs {
 
 // energy / virial / count accumulators (one per context, in device memory)
 struct Totals {
-    double em, ep, er, edsp;
+    double em, ep, er, edsp, ev, ect; // indexed 0..5 from em by tile_scalars
     double einter;
     unsigned long long nem;
     double vir[9];
@@ -233,9 +234,10 @@ struct ExclPair {
     int mcls;       // 2..5 bonded class for m/p scaling, 0 = none
     int gcls;       // 1..4 polarization-group class for d/u scaling, 0 = none
     int intra;      // 1 if k is in i's own polarization group (pNiscale)
-    int pad;
+    int in14;       // 1 if k is listed 1-4 of i (the vdW 1-4 tables), whatever mcls says
     double m, p, d, u; // scale values (filled by set_params); u is the mutual-field scale (w for HIPPO)
     double dsp, rep;   // HIPPO dispersion and repulsion scales (by connectivity)
+    double v;          // van der Waals scale (by connectivity)
 };
 
 } // namespace pmq
@@ -254,11 +256,12 @@ struct pmq_ctx {
 
     // ---- topology (host copies kept for the exclusion list)
     std::vector<int> h_polaxe, h_zaxis, h_xaxis, h_yaxis, h_molcule, h_usef;
+    std::vector<int> h_use;          // n: use[] itself (without the frame atoms usef adds)
     std::vector<pmq::ExclPair> h_excl;
     int nbond = 0, nangle = 0;
 
     // ---- device: static per topology / params
-    pmq::DBuf<int> polaxe, zaxis, xaxis, yaxis, molcule, usef;
+    pmq::DBuf<int> polaxe, zaxis, xaxis, yaxis, molcule, usef, use_raw;
     pmq::DBuf<pmq::ExclPair> excl;
     pmq::DBuf<double> pole;          // 13n local
     pmq::DBuf<double> pcore, pval0, palpha;
@@ -413,9 +416,20 @@ struct pmq_ctx {
     int pcg_iters = 0;
     // HIPPO dispersion and repulsion
     pmq::DBuf<double> csix, adisp, sizpr, dmppr, elepr; // caller order
-    pmq::DBuf<double> shp;                       // 5 * npad sorted: csix, adisp, sizpr, dmppr, elepr
+    pmq::DBuf<double> shp;                       // PMQ_SHP_ROWS * npad sorted: csix, adisp, sizpr, dmppr, elepr,
+                                                 // chgct, dmpct (0 read as 1000), use[] (1 or 0)
     pmq::DBuf<double> der, dedsp, ter;           // 3n gradients of er, edsp; torques of er
     bool have_disp = false, have_rep = false;
+    // van der Waals (k_vdw.hip) and charge transfer
+    int nvdw = 0;
+    bool have_vdw = false, have_ct = false;
+    pmq::DBuf<int> vcls, ired, vuse;             // n: class, reduction partner, reduced site active
+    pmq::DBuf<int> vkid_off, vkid;               // n + 1 offsets, list: the atoms reduced toward each atom
+    pmq::DBuf<double> kred;                      // n reduction factors
+    pmq::DBuf<double> vtab;                      // 4 nvdw^2: radmin, epsilon, radmin4, epsilon4
+    pmq::DBuf<double> vx, vy, vz;                // n reduced sites
+    pmq::DBuf<double> gvr, dev;                  // 3n: gradient at the reduced sites; of ev at the atoms
+    pmq::DBuf<double> chgct, dmpct, dect;        // caller order; 3n gradient of ect
     pmq::DBuf<double> uind_out;                  // 6n caller-order induced dipoles (d then p)
 
     // kernel timing (pmq_profile_*)
@@ -432,7 +446,9 @@ struct pmq_ctx {
 // ---------------------------------------------------------------------------
 // launchers (defined in the .hip files)
 namespace pmq {
-int build_neighbors(pmq_ctx *c, double cutoff);
+// positions and active flags default to the atoms' (x, y, z, usef)
+int build_neighbors(pmq_ctx *c, double cutoff, const double *x = nullptr, const double *y = nullptr,
+                    const double *z = nullptr, const int *use = nullptr);
 int launch_rotpole(pmq_ctx *c, bool want_rpole13);
 int launch_chgflx_delta(pmq_ctx *c);
 int launch_mpole_direct(pmq_ctx *c, int mode, double off2, bool mixed);
@@ -441,7 +457,7 @@ int launch_fx_flush_pair(pmq_ctx *c, long long *q, double *d0, double *d1, size_
 int launch_torque(pmq_ctx *c, int mode, const double *te, double *de, bool mixed);
 int launch_torque_pair(pmq_ctx *c, int mode0, const double *te0, double *de0, int mode1, const double *te1,
                        double *de1, bool mixed);
-constexpr int PMQ_ZERO_MAX = 12;
+constexpr int PMQ_ZERO_MAX = 16;
 int launch_zero_many(pmq_ctx *c, int k, double *const *p, const size_t *n);
 int pme_setup(pmq_ctx *c);
 int fft_own_setup(pmq_ctx *c);
@@ -451,6 +467,10 @@ int pme_disp(pmq_ctx *c, bool grad, bool need_bin, bool vir);
 int hippo_sort(pmq_ctx *c);
 int launch_disp(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir);
 int launch_rep(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir);
+int launch_ct(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir);
+// the van der Waals term with its own pair structure over the reduced
+// sites; run before the other terms build theirs
+int vdw_pass(pmq_ctx *c, bool grad, bool mixed, bool vir, bool safe);
 void pme_release(pmq_ctx *c);
 int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar);
 // VIRIAL: the k-space sums of the multipole and polarization reciprocal

@@ -169,6 +169,21 @@ class Engine:
             p.sizpr, p.dmppr, p.elepr = self._d(sizpr), self._d(s["dmppr"]), self._d(s["elepr"])
             p.repcut = float(s.get("repcut", 6.0))
             p.reptaper = taper(p.repcut, "reptaper")
+        if int(s.get("nvdw", 0)) > 0:
+            p.nvdw = int(s["nvdw"])
+            p.vdwclass, p.ired = self._i(s["vdwclass"]), self._i(s["ired"])
+            p.radmin, p.epsilon = self._d(s["radmin"]), self._d(s["epsilon"])
+            p.radmin4, p.epsilon4 = self._d(s["radmin4"]), self._d(s["epsilon4"])
+            p.kred = self._d(s["kred"])
+            for k, d in (("v2scale", 0.0), ("v3scale", 0.0), ("v4scale", 1.0), ("v5scale", 1.0)):
+                setattr(p, k, float(s.get(k, d)))
+            p.vdwcut = float(s.get("vdwcut", 9.0))
+            p.vdwtaper = taper(p.vdwcut, "vdwtaper")
+        chgct = s.get("chgct")
+        if chgct is not None and np.any(np.asarray(chgct) != 0.0):
+            p.chgct, p.dmpct = self._d(chgct), self._d(s["dmpct"])
+            p.ctrncut = float(s.get("ctrncut", 6.0))
+            p.ctrntaper = taper(p.ctrncut, "ctrntaper")
         # the reference's PCG defaults: preconditioner on, 4.5 A, udiag 2
         p.pcgprec = int(s.get("pcgprec", 1))
         p.usolvcut, p.udiag = float(s.get("usolvcut", 4.5)), float(s.get("udiag", 2.0))
@@ -204,7 +219,9 @@ class Engine:
         der = np.zeros(3 * n)
         dedsp = np.zeros(3 * n)
         aep = np.zeros(n)
+        dev, dect = np.zeros(3 * n), np.zeros(3 * n)
         r = L.Result()
+        r.dev, r.dect = dev.ctypes.data_as(_dp), dect.ctypes.data_as(_dp)
         r.aep = aep.ctypes.data_as(_dp)
         r.der, r.dedsp = der.ctypes.data_as(_dp), dedsp.ctypes.data_as(_dp)
         r.dem, r.aem = dem.ctypes.data_as(_dp), aem.ctypes.data_as(_dp)
@@ -216,7 +233,8 @@ class Engine:
                "nem": int(r.nem), "vir": np.array(r.vir[:]).reshape(3, 3), "pcg_iters": r.pcg_iters,
                "dem": dem.reshape(n, 3), "dep": dep.reshape(n, 3), "desum": desum.reshape(n, 3), "aem": aem,
                "uind": uind.reshape(n, 3), "uinp": uinp.reshape(n, 3), "der": der.reshape(n, 3),
-               "dedsp": dedsp.reshape(n, 3), "nep": int(r.nep), "aep": aep}
+               "dedsp": dedsp.reshape(n, 3), "nep": int(r.nep), "aep": aep, "ev": r.ev, "ect": r.ect,
+               "dev": dev.reshape(n, 3), "dect": dect.reshape(n, 3)}
         if want_rpole:
             out["rpole"] = rpole.reshape(n, 13)
         return out

@@ -6,16 +6,22 @@ is captured and later ones replay the graph.  Every replay is checked, with
 other GPU work (a second context's evaluation) and host synchronizations
 between replays, at new coordinates (the graph reads them from the device)
 and after a change of box (a new key: enqueued plainly, then captured
-again).  The mixed form's gradients are fixed-point sums, so they must match
+again).  Two cases add the van der Waals term (its own pair structure
+built and replaced inside the graph) and charge transfer.  The mixed form's gradients are fixed-point sums, so they must match
 exactly; energy totals and FP64 gradients are summed with FP64 atomics and
 match to 1e-12.  Covered: AMOEBA with PME and the
 sparse preconditioner (fused SpMV + spreading), HIPPO with all four terms,
-and a cluster without PME (the row product ends the operator).
+a cluster without PME (the row product ends the operator), the 30 A AMOEBA
+water box read from its input files with van der Waals, and a 64-water
+HIPPO box with charge transfer beside its other terms.
 """
+import os
+
 import numpy as np
 import pytest
 
-from polmdqc_amd import water
+from conftest import GOLDEN
+from polmdqc_amd import tinker, water
 
 pytestmark = pytest.mark.gpu
 
@@ -44,6 +50,24 @@ def cluster():
     return water.water_system(water.cluster(40, np.random.default_rng(4)), "water03")
 
 
+def tinker_vdw_box(tmp):
+    td = os.path.join(GOLDEN, "tinker")
+    key = os.path.join(tmp, "vdw.key")
+    open(key, "w").write("parameters water03\na-axis 30\newald\newald-cutoff 7.0\nvdw-cutoff 9.0\n")
+    s = tinker.system_from_files(os.path.join(td, "bounds/waterbox30.xyz"), key, [os.path.join(td, "params")])
+    s["use_vdw"] = 1
+    return s
+
+
+def hippo_ct_box(tmp):
+    from test_gpu_vdwct import hippo_box
+    import pathlib
+
+    s = hippo_box(pathlib.Path(tmp))
+    s["use_chgflx"] = 0
+    return s
+
+
 def keys(terms):
     from polmdqc_amd import _lib as L
 
@@ -54,6 +78,10 @@ def keys(terms):
         k += ["er"]
     if terms & L.TERM_DISP:
         k += ["edsp"]
+    if terms & L.TERM_VDW:
+        k += ["ev", "dev"]
+    if terms & L.TERM_CTRN:
+        k += ["ect", "dect"]
     return k + ["desum"]
 
 
@@ -82,17 +110,26 @@ def eager(b, s, mode, terms, prec, xyz, box=None):
     return r
 
 
-@pytest.mark.parametrize("name", ["amoeba", "hippo", "cluster"])
-def test_graph_replays_match_plain_enqueue(engines, name):
+@pytest.mark.parametrize("name", ["amoeba", "hippo", "cluster", "amoeba_vdw", "hippo_ct"])
+def test_graph_replays_match_plain_enqueue(engines, name, tmp_path):
     from polmdqc_amd import _lib as L
     from polmdqc_amd.engine import box_from, set_graphs
 
     print(f"[gpu] graphs {name}", flush=True)
     a, b = engines
-    s = {"amoeba": amoeba_box, "hippo": hippo_box, "cluster": cluster}[name]()
+    if name == "amoeba_vdw":
+        s = tinker_vdw_box(str(tmp_path))
+    elif name == "hippo_ct":
+        s = hippo_ct_box(str(tmp_path))
+    else:
+        s = {"amoeba": amoeba_box, "hippo": hippo_box, "cluster": cluster}[name]()
     terms = L.TERM_MPOLE | L.TERM_POLAR
-    if name == "hippo":
+    if name in ("hippo", "hippo_ct"):
         terms |= L.TERM_REPEL | L.TERM_DISP
+    if name == "amoeba_vdw":
+        terms |= L.TERM_VDW
+    if name == "hippo_ct":
+        terms |= L.TERM_CTRN
     xyz = np.stack([s["x"], s["y"], s["z"]], 1)
     rng = np.random.default_rng(7)
     moved = xyz + rng.normal(scale=0.02, size=xyz.shape)

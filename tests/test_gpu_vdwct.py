@@ -1,0 +1,187 @@
+"""Buffered 14-7 van der Waals (PMQ_TERM_VDW) and HIPPO charge transfer
+(PMQ_TERM_CTRN) on the GPU against the oracle (oracle/pmq_oracle_elec.c
+orc_vdwct), on systems read from the reference's test inputs
+(tests/golden/tinker) by polmdqc_amd/tinker.py.
+
+FP64: energies to 1e-11 relative, gradients and virials to 1e-10 of their
+largest entry.  The mixed form (FP32 pairs, fixed-point sums) to the bars of
+the other pair terms' mixed forms: 1e-6 relative on the energy, 1e-5 of the
+largest gradient entry.  Covered: a water cluster and the alanine
+tetrapeptide (hydrogen reduction, many classes, 1-4 tables and a 1-4 scale
+made to differ from the 1-2/1-3/1-5 ones), the 30 A water box in four cell
+shapes with the vdW cutoff (9 A) longer than the Ewald one (7 A) and the
+multipoles computed in the same evaluation, inactive atoms, and charge
+transfer in the HIPPO water dimer and in a 64-water HIPPO box with all of
+its terms.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN
+from polmdqc_amd import tinker
+
+pytestmark = pytest.mark.gpu
+
+TD = os.path.join(GOLDEN, "tinker")
+PRM = [os.path.join(TD, "params")]
+
+
+@pytest.fixture(scope="module")
+def engine():
+    from polmdqc_amd.engine import Engine
+
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+def system(xyz, key=None, tmp=None, keytext=None):
+    if keytext is not None:
+        kp = os.path.join(tmp, "case.key")
+        open(kp, "w").write(keytext)
+        key = kp
+    elif key is not None:
+        key = os.path.join(TD, key)
+    return tinker.system_from_files(os.path.join(TD, xyz), key, PRM)
+
+
+def worst(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def check(g, o, term, prec):
+    from polmdqc_amd import _lib as L
+
+    e, de, vk = ("ev", "dev", "vir_v") if term == "vdw" else ("ect", "dect", "vir_c")
+    if prec == L.PLAIN_FP64:
+        assert abs(g[e] - o[e]) <= 1e-11 * max(1.0, abs(o[e])), (g[e], o[e])
+        assert worst(g[de], o[de]) <= 1e-10
+        assert worst(g["vir"], o[vk]) <= 1e-10
+    else:
+        assert abs(g[e] - o[e]) <= 1e-6 * max(1.0, abs(o[e])), (g[e], o[e])
+        assert worst(g[de], o[de]) <= 1e-5
+        assert worst(g["vir"], o[vk]) <= 1e-5
+
+
+CLUSTERS = {
+    "water09": ("amoeba/water09.xyz", None),
+    "alatet": ("chkpole/alatet.xyz", None),
+    "active": ("active/water09_Na_Cls.xyz", "active/active.key"),
+    "inactive": ("active/water09_Na_Cls.xyz", "active/inactive.key"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CLUSTERS))
+def test_vdw_cluster(engine, name):
+    from polmdqc_amd import _lib as L
+
+    print(f"[gpu] vdw cluster {name}", flush=True)
+    s = system(*CLUSTERS[name])
+    s["use_vdw"] = 1
+    if name == "alatet":
+        # 1-4 tables and scale apart from the rest, so a pair put in the
+        # wrong class moves the energy
+        s["radmin4"] = 0.9 * s["radmin"]
+        s["epsilon4"] = 0.5 * s["epsilon"]
+        s["v4scale"] = 0.75
+    o = oracle.vdwct(s)
+    engine.set_system(s)
+    for prec in (L.PLAIN_FP64, L.MIXED):
+        g = engine.eval(L.VIRIAL, L.TERM_VDW, prec)
+        check(g, o, "vdw", prec)
+        assert np.array_equal(g["desum"], g["dev"])
+
+
+@pytest.mark.parametrize("key", ["orthogonal", "triclinic", "octahedron", "dodecahedron"])
+def test_vdw_box_beside_ewald_multipoles(engine, key, tmp_path):
+    """vdW over its own 9 A structure, then the multipoles at 7 A (Ewald
+    where the cell allows it) in the same evaluation: each term as when
+    computed alone."""
+    from polmdqc_amd import _lib as L
+
+    print(f"[gpu] vdw box {key}", flush=True)
+    text = open(os.path.join(TD, "bounds", key + ".key")).read().replace("../../params/", "")
+    # Ewald takes the lattice cells; the other two run the multipoles with a
+    # plain 7 A cutoff
+    ew = "ewald\newald-cutoff 7.0" if key in ("orthogonal", "triclinic") else "mpole-cutoff 7.0"
+    text = text.replace("multipoleterm only", "") + f"\n{ew}\nvdw-cutoff 9.0\nvdw-taper 8.0\n"
+    s = system("bounds/waterbox30.xyz", tmp=str(tmp_path), keytext=text)
+    s["use_vdw"] = 1
+    o = oracle.vdwct(s)
+    engine.set_system(s)
+    for prec in (L.PLAIN_FP64, L.MIXED):
+        both = engine.eval(L.VIRIAL, L.TERM_MPOLE | L.TERM_VDW, prec)
+        v = engine.eval(L.VIRIAL, L.TERM_VDW, prec)
+        m = engine.eval(L.VIRIAL, L.TERM_MPOLE, prec)
+        check(v, o, "vdw", prec)
+        bar = 1e-12 if prec == L.PLAIN_FP64 else 1e-9
+        assert abs(both["em"] - m["em"]) <= bar * abs(m["em"])
+        assert abs(both["ev"] - v["ev"]) <= bar * abs(v["ev"])
+        assert worst(both["dem"], m["dem"]) <= bar and worst(both["dev"], v["dev"]) <= bar
+        assert worst(both["vir"], m["vir"] + v["vir"]) <= (1e-11 if prec == L.PLAIN_FP64 else 1e-8)
+
+
+def hippo_box(tmp_path):
+    """64 waters of the 30 A box around its first one in a 24 A cell, HIPPO
+    water21 with Ewald (7 A) and charge transfer at 6 A."""
+    src = open(os.path.join(TD, "bounds/waterbox30.xyz")).read().splitlines()[1:]
+    xyz = np.array([[float(v) for v in l.split()[2:5]] for l in src])
+    d = xyz - xyz[0]
+    mol = np.arange(len(xyz)) // 3
+    keep = sorted({int(m) for m in mol[np.max(np.abs(d), axis=1) < 12.0]})[:64]
+    rows = []
+    for q, m in enumerate(keep):
+        for a in range(3):
+            w = src[3 * m + a].split()
+            bonds = [3 * q + 2, 3 * q + 3] if a == 0 else [3 * q + 1]
+            p = xyz[3 * m + a] - xyz[0]
+            rows.append(f"{3 * q + a + 1:6d}  {w[1]:<3}{p[0]:12.6f}{p[1]:12.6f}{p[2]:12.6f}{int(w[5]):6d}"
+                        + "".join(f"{b:6d}" for b in bonds))
+    (tmp_path / "hbox.xyz").write_text(f"{len(rows):6d}  box\n" + "\n".join(rows) + "\n")
+    (tmp_path / "hbox.key").write_text("parameters water21\na-axis 24.0\newald\newald-cutoff 7.0\n"
+                                       "dewald\ndisp-cutoff 7.0\nrepuls-cutoff 6.0\n"
+                                       "chgtrn-cutoff 6.0\nchgtrn-taper 5.4\n")
+    return tinker.system_from_files(str(tmp_path / "hbox.xyz"), str(tmp_path / "hbox.key"), PRM)
+
+
+@pytest.mark.parametrize("case", ["dimer", "box"])
+def test_charge_transfer(engine, case, tmp_path):
+    from polmdqc_amd import _lib as L
+
+    print(f"[gpu] charge transfer {case}", flush=True)
+    s = system("hippo/water21.xyz") if case == "dimer" else hippo_box(tmp_path)
+    s["use_ctrn"] = 1
+    if case == "box":
+        s["use_chgflx"] = 0  # charge flux with Ewald is not built (and charge transfer does not see it)
+    o = oracle.vdwct(s)
+    assert abs(o["ect"]) > 1e-3
+    engine.set_system(s)
+    for prec in (L.PLAIN_FP64, L.MIXED):
+        g = engine.eval(L.VIRIAL, L.TERM_CTRN, prec)
+        check(g, o, "ctrn", prec)
+    if case == "box":
+        # beside every other HIPPO term: each as when computed alone
+        rest = L.TERM_MPOLE | L.TERM_POLAR | L.TERM_REPEL | L.TERM_DISP
+        a = engine.eval(L.GRADIENT, rest | L.TERM_CTRN, L.PLAIN_FP64)
+        b = engine.eval(L.GRADIENT, rest, L.PLAIN_FP64)
+        assert abs(a["ect"] - o["ect"]) <= 1e-11 * abs(o["ect"])
+        assert worst(a["dect"], o["dect"]) <= 1e-10
+        for k in ("em", "ep", "er", "edsp"):
+            assert abs(a[k] - b[k]) <= 1e-10 * max(1.0, abs(b[k])), k
+        assert worst(a["desum"], b["desum"] + a["dect"]) <= 1e-10
+
+
+def test_refusals(engine):
+    from polmdqc_amd import _lib as L
+
+    s = system("amoeba/water09.xyz")
+    s["use_vdw"] = 1
+    engine.set_system(s)
+    with pytest.raises(RuntimeError, match="PMQ_ERR_UNSUPPORTED"):
+        engine.eval(L.ANALYSIS, L.TERM_VDW, L.PLAIN_FP64)
+    with pytest.raises(RuntimeError, match="PMQ_ERR_STATE"):
+        engine.eval(L.ENERGY, L.TERM_CTRN, L.PLAIN_FP64)  # no charge-transfer parameters

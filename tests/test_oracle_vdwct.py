@@ -186,3 +186,63 @@ def test_lone_molecule_has_no_energy():
     one["dmpct"] = np.full(3, 3.0)
     o = oracle.vdwct(one, want_grad=False)
     assert o["ev"] == 0.0 and o["ect"] == 0.0
+
+
+def subset_xyz(src, atoms, dst):
+    """Atoms (1-based, whole molecules) of a Tinker .xyz file, renumbered,
+    written as their own file."""
+    lines = open(src).read().splitlines()
+    n = int(lines[0].split()[0])
+    body = lines[1:1 + n] if len(lines[1].split()) > 1 and not _is_cell(lines[1]) else lines[2:2 + n]
+    new = {a: q + 1 for q, a in enumerate(sorted(atoms))}
+    rows = []
+    for a in sorted(atoms):
+        w = body[a - 1].split()
+        bonds = [new[int(b)] for b in w[6:]]
+        rows.append(f"{new[a]:6d}  {w[1]:<3}{float(w[2]):12.6f}{float(w[3]):12.6f}{float(w[4]):12.6f}{int(w[5]):6d}"
+                    + "".join(f"{b:6d}" for b in bonds))
+    open(dst, "w").write(f"{len(rows):6d}  subset\n" + "\n".join(rows) + "\n")
+
+
+def _is_cell(line):
+    try:
+        return len([float(v) for v in line.split()]) == 6
+    except ValueError:
+        return False
+
+
+@pytest.mark.parametrize("term", ["vdw", "ctrn"])
+def test_active_pairs_are_total_minus_the_inactive_subsystem(term, tmp_path):
+    """With a set A of active atoms, the pairs that count are those with an
+    atom of A: the energy is E(all) - E(the rest alone), the rest's energy
+    computed from a file holding only those atoms."""
+    if term == "vdw":
+        xyz, prm, active = os.path.join(TD, "active/water09_Na_Cls.xyz"), "amoebabio09", [5, 6, 7, 14, 15, 16]
+        key, flag = "ev", "use_vdw"
+    else:
+        # the 8 waters of the 30 A box nearest its first one, as a cluster
+        box = os.path.join(TD, "bounds/waterbox30.xyz")
+        lines = open(box).read().splitlines()[1:]
+        ox = np.array([[float(v) for v in lines[3 * m].split()[2:5]] for m in range(len(lines) // 3)])
+        near = np.argsort(np.linalg.norm(ox - ox[0], axis=1))[:8]
+        xyz = str(tmp_path / "cluster.xyz")
+        subset_xyz(box, [3 * int(m) + a for m in near for a in (1, 2, 3)], xyz)
+        prm, active, key, flag = "water21", [1, 2, 3], "ect", "use_ctrn"
+    n = int(open(xyz).readline().split()[0])
+    rest = [a for a in range(1, n + 1) if a not in active]
+    (tmp_path / "a.key").write_text(f"parameters {prm}\nactive " + " ".join(map(str, active)) + "\n")
+    (tmp_path / "all.key").write_text(f"parameters {prm}\n")
+    subset_xyz(xyz, rest, str(tmp_path / "rest.xyz"))
+    (tmp_path / "rest.key").write_text(f"parameters {prm}\n")
+
+    def energy(x, k):
+        s = tinker.system_from_files(x, str(k), PRM)
+        s[flag] = 1
+        return oracle.vdwct(s, want_grad=False)[key], s
+
+    e_act, s_act = energy(xyz, tmp_path / "a.key")
+    assert int(np.sum(s_act["use"][1:])) == len(active)
+    e_all, _ = energy(xyz, tmp_path / "all.key")
+    e_rest, _ = energy(str(tmp_path / "rest.xyz"), tmp_path / "rest.key")
+    assert abs(e_rest) > 1e-3 and abs(e_act - e_all) > 1e-3
+    assert abs(e_act - (e_all - e_rest)) < 1e-12 * max(1.0, abs(e_all))
