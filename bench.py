@@ -272,6 +272,167 @@ def bench_hippo(eng, L, steps, warmup) -> dict:
     return out
 
 
+# Kernels of each timed family (demangled-name prefixes, after "void pmq::"),
+# for the HBM counter summary; every one runs once per launch of its family.
+FAMILY_KERNELS = {
+    "pol_spmv_spread": ("k_spread_spmv<",),
+    "pme_fft": ("k_fft_planes_fwd<", "k_fft_columns<", "k_fft_planes_inv<"),
+    "pme_gather_field": ("k_pme_gather_field<",),
+    "pme_spread": ("k_pme_spread_brick<5, 1,",),
+    "mpole_tiles": ("k_mpole_tiles<",),
+    "pol_field_tiles": ("k_pol_field_tiles<",),
+    "pol_grad_tiles": ("k_pol_grad_tiles<",),
+    "pol_pair_fill": ("k_pair_fill<",),
+}
+
+PROFILE_COMMANDS = {
+    "trace": "rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof -o bench -- python3 bench.py "
+             "--steps 5 --warmup 2 --no-cpu --parts amoeba --pin-fft library/fp32-direct > gpurun_out/prof_amoeba.log 2>&1",
+    "fetch": "rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmcF -o p -- python3 bench.py --steps 2 "
+             "--warmup 1 --no-cpu --parts amoeba --calibrate-copy 256 --pin-fft library/fp32-direct > gpurun_out/pmcF.log 2>&1",
+    "write": "rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmcW -o p -- python3 bench.py --steps 2 "
+             "--warmup 1 --no-cpu --parts amoeba --calibrate-copy 256 --pin-fft library/fp32-direct > gpurun_out/pmcW.log 2>&1",
+    "hippo": "rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/profh -o hippo -- python3 bench.py "
+             "--steps 3 --warmup 1 --no-cpu --parts hippo --pin-fft library/fp32-library > gpurun_out/prof_hippo.log 2>&1",
+}
+
+
+def _short(name: str) -> str:
+    return name.split("(")[0].replace("void ", "").replace("pmq::", "")
+
+
+def _last_json(path: str):
+    with open(path) as fh:
+        lines = [ln for ln in fh if ln.startswith("{")]
+    return json.loads(lines[-1]) if lines else None
+
+
+def summarize_profiles(src: str, dst: str) -> None:
+    """Turn one profiling pass (the four commands of PROFILE_COMMANDS, run
+    from the repository root with their output under `src`) into the
+    summaries under `dst`: kernel time of the AMOEBA and HIPPO workloads,
+    HBM bytes per launch of every kernel, and traffic.json (bytes per launch
+    of each timed family, which the bench line reports as roofline.traffic)."""
+    import csv
+    from collections import defaultdict
+
+    cmds = "\n".join(["export TMPDIR=/tmp"] + [PROFILE_COMMANDS[k] for k in ("trace", "fetch", "write", "hippo")])
+    cmds += f"\npython3 bench.py --summarize-profiles {src}"
+
+    def stats_table(path, top):
+        rows = list(csv.DictReader(open(path)))
+        rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+        out = ["| kernel | calls | total ms | avg us | % |", "|---|---:|---:|---:|---:|"]
+        for r in rows[:top]:
+            out.append(f"| `{_short(r['Name'])[:90]}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+                       f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+        return out, {_short(r["Name"]): (int(r["Calls"]), float(r["AverageNs"]) / 1e3) for r in rows}
+
+    # kernel time, AMOEBA 12,288 atoms
+    amo = _last_json(os.path.join(src, "prof_amoeba.log"))
+    tab, avg = stats_table(os.path.join(src, "prof", "bench_kernel_stats.csv"), 40)
+    dom = amo["roofline"]["kernel"] if amo else None
+    fam_us = ""
+    if dom in FAMILY_KERNELS:
+        parts = [(k, v[1]) for k, v in avg.items() if k.startswith(FAMILY_KERNELS[dom]) and "float" in k]
+        fam_us = (f"\n\nThe roofline family `{dom}` (mixed form), average kernel time per launch from this trace: "
+                  + ", ".join(f"`{k}` {us:.2f} us" for k, us in parts)
+                  + f"; sum {sum(us for _, us in parts):.2f} us per family launch (the library's HIP events, which "
+                  f"also count each launch's dispatch: {amo['roofline']['avg_launch_us']} us).")
+    md = ["# Kernel time, AMOEBA water 12,288 atoms (bench workload), FP64 then mixed", "",
+          "Commands (one MI355X, from the repository root; this file is the last command's output):", "",
+          "```", cmds, "```", "",
+          "The trace covers 2+5 FP64 evaluations, then 2+5 mixed evaluations, each set twice (timed, then with the "
+          "library's own HIP-event timing on).  PME transforms pinned (`--pin-fft`) to an unprofiled run's choice: "
+          "FP64 form on rocFFT, mixed form on FP32 grids with the direct LDS transforms."]
+    if amo:
+        md += ["", f"Profiled run's bench line: {amo['value']} mixed evals/s, {amo['fp64_evals_per_s']} FP64 evals/s "
+               f"(kernel tracing slows the launch-bound parts); PCG iterations {amo['pcg_iters']}; roofline family "
+               f"`{dom}`."]
+    md += ["", "Top kernels by total time (`bench_kernel_stats.csv`):", ""] + tab
+    with open(os.path.join(dst, "amoeba_water_12288_kernel_stats.md"), "w") as fh:
+        fh.write("\n".join(md) + fam_us + "\n")
+
+    # kernel time, HIPPO 96,000 atoms
+    hp = _last_json(os.path.join(src, "prof_hippo.log"))
+    tab, _ = stats_table(os.path.join(src, "profh", "hippo_kernel_stats.csv"), 30)
+    md = ["# Kernel time, HIPPO water 96,000 atoms (the HIPPO bench workload), FP64 then mixed", "",
+          "Commands: as for `amoeba_water_12288_kernel_stats.md`; this file comes from", "", "```",
+          "export TMPDIR=/tmp", PROFILE_COMMANDS["hippo"], f"python3 bench.py --summarize-profiles {src}", "```", "",
+          "Multipoles with charge penetration, HIPPO polarization, repulsion and dispersion (PME 100^3, order 5); "
+          "transforms pinned to rocFFT for both forms, FP32 grids in the mixed form."]
+    if hp and "hippo_96k" in hp:
+        h = hp["hippo_96k"]
+        md += ["", f"Profiled run: {h['fp64_evals_per_s']} FP64 / {h['mixed_evals_per_s']} mixed evals/s; "
+               f"PCG iterations {h['pcg_iters']}."]
+    md += ["", "Top kernels by total time (`hippo_kernel_stats.csv`):", ""] + tab
+    with open(os.path.join(dst, "hippo_water_96000_kernel_stats.md"), "w") as fh:
+        fh.write("\n".join(md) + "\n")
+
+    # HBM bytes per launch from the two counter runs
+    def counters(path, counter):
+        per = defaultdict(list)  # kernel -> [(grid, KiB)]
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == counter:
+                per[_short(r["Kernel_Name"])].append((int(r["Grid_Size"]), float(r["Counter_Value"])))
+        return per
+
+    fetch = counters(os.path.join(src, "pmcF", "p_counter_collection.csv"), "FETCH_SIZE")
+    write = counters(os.path.join(src, "pmcW", "p_counter_collection.csv"), "WRITE_SIZE")
+    copy_bytes = 256 << 20
+    cf = max(fetch["__amd_rocclr_copyBuffer"])  # the 256 MiB calibration copy has the largest grid
+    cw = max(write["__amd_rocclr_copyBuffer"])
+    f_fac = copy_bytes / (cf[1] * 1024.0)
+    w_fac = copy_bytes / (cw[1] * 1024.0)
+
+    def modal(vals):  # dispatches of the most common grid size (one workload shape)
+        grids = defaultdict(list)
+        for g, v in vals:
+            grids[g].append(v)
+        g = max(grids, key=lambda k: len(grids[k]))
+        return sum(grids[g]) / len(grids[g]), len(grids[g])
+
+    per_kernel = {}
+    for k in fetch:
+        if k in write and k != "__amd_rocclr_copyBuffer":
+            fk, nf = modal(fetch[k])
+            wk, _ = modal(write[k])
+            per_kernel[k] = (nf, fk, wk, (fk * f_fac + wk * w_fac) * 1024.0)
+    traffic = {"fp64": {}, "mixed": {}, "kernels": {}}
+    for fam, pref in FAMILY_KERNELS.items():
+        for form in ("fp64", "mixed"):
+            # the mixed form's kernels are the ones instantiated on float
+            ks = sorted(k for k in per_kernel if k.startswith(pref) and ("float" in k) == (form == "mixed"))
+            if not ks:
+                continue
+            # variants of one kernel (e.g. with and without fused vector steps): their mean per dispatch
+            by_base = defaultdict(list)
+            for k in ks:
+                by_base[pref[[k.startswith(p) for p in pref].index(True)]].append(per_kernel[k][3])
+            traffic[form][fam] = int(round(sum(sum(v) / len(v) for v in by_base.values())))
+            traffic["kernels"][f"{fam}/{form}"] = ks
+    traffic["note"] = (f"HBM bytes per family launch: FETCH_SIZE x {f_fac:.3f} + WRITE_SIZE x {w_fac:.3f} "
+                       f"(KiB x 1024), the factors that make a 256 MiB device-to-device copy read and write 256 MiB; "
+                       f"dispatches of each kernel's most common grid size")
+    with open(os.path.join(dst, "traffic.json"), "w") as fh:
+        json.dump(traffic, fh, indent=1)
+    md = ["# HBM traffic per launch, AMOEBA water 12,288 atoms (bench workload)", "",
+          "Commands: the `fetch` and `write` counter runs and the summary step of", "", "```", "export TMPDIR=/tmp",
+          PROFILE_COMMANDS["fetch"], PROFILE_COMMANDS["write"], f"python3 bench.py --summarize-profiles {src}", "```",
+          "", f"Calibration: the 256 MiB device-to-device copy (`__amd_rocclr_copyBuffer`) reads FETCH_SIZE = "
+          f"{cf[1]:.0f} KiB and writes WRITE_SIZE = {cw[1]:.0f} KiB, so FETCH_SIZE is scaled by {f_fac:.3f} and "
+          f"WRITE_SIZE by {w_fac:.3f}.  Each kernel's figures are the mean over its dispatches of the most common "
+          f"grid size.", "", "Per family launch (`traffic.json`, the bench line's `roofline.traffic`): "
+          + "; ".join(f"{fam} {form} {v / 1e6:.2f} MB" for form in ("fp64", "mixed")
+                      for fam, v in traffic[form].items()) + ".", "",
+          "| kernel | dispatches | FETCH KiB (raw) | WRITE KiB (raw) | HBM MB/launch (scaled) |",
+          "|---|---:|---:|---:|---:|"]
+    for k, (nd, fk, wk, b) in sorted(per_kernel.items(), key=lambda kv: -kv[1][3] * kv[1][0])[:40]:
+        md.append(f"| `{k[:90]}` | {nd} | {fk:.1f} | {wk:.1f} | {b / 1e6:.2f} |")
+    with open(os.path.join(dst, "amoeba_water_12288_hbm_traffic.md"), "w") as fh:
+        fh.write("\n".join(md) + "\n")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,7 +448,12 @@ def main() -> int:
                     help="multipole pair algebra: Cartesian or in each pair's quasi-internal frame")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy kernel of MB MiB (HBM counter calibration)")
+    ap.add_argument("--summarize-profiles", default=None, metavar="DIR",
+                    help="write profiles/*.md and profiles/traffic.json from the rocprofv3 output under DIR and exit")
     args = ap.parse_args()
+    if args.summarize_profiles:
+        summarize_profiles(args.summarize_profiles, os.path.join(ROOT, "profiles"))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU runs are not part of this build",
               file=sys.stderr)
