@@ -204,8 +204,10 @@ def timed(eng, L, terms, prec, steps, warmup):
 
 def bench_clusters(eng, L, steps, warmup) -> dict:
     """The batched workload: 512 non-periodic clusters of U{2..64} waters
-    (seed 512), no cutoff, mutual PCG to 1e-6 D per cluster, energy + forces of
-    every cluster per evaluation (one graph launch), both forms."""
+    (seed 512), no cutoff, mutual PCG to 1e-6 D per cluster; energy, forces and
+    parameter gradients of every cluster per evaluation (one graph launch),
+    both forms, with the autotuner's shape per size bucket; the same without
+    parameter gradients beside it."""
     from polmdqc_amd import water
 
     xyz, off = water.cluster_batch(512, 512, 2, 64)
@@ -215,24 +217,33 @@ def bench_clusters(eng, L, steps, warmup) -> dict:
     eng.set_batch(off)
     terms = L.TERM_MPOLE | L.TERM_POLAR
     out = {"workload": "512 AMOEBA water clusters, 2-64 molecules, seed 512", "atoms": int(s["n"])}
-    res = {}
-    for name, prec in (("fp64", L.PLAIN_FP64), ("mixed", L.MIXED)):
-        for _ in range(warmup):
-            eng.compute(L.GRADIENT, terms, prec)
-            eng.sync()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            eng.compute(L.GRADIENT, terms, prec)
-            eng.sync()
-        ms = (time.perf_counter() - t0) * 1e3 / steps
-        res[name] = eng.fetch()
-        if not np.all(eng.fetch_batch()["pcg_iters"] > 0):
-            raise RuntimeError("bench: a cluster's induced dipoles did not converge")
-        out[f"{name}_evals_per_s"] = round(1e3 / ms, 3)
-        out[f"{name}_ms_per_eval"] = round(ms, 4)
+    res, pgs = {}, {}
+    for pgrad in (True, False):
+        eng.set_batch_param_grad(pgrad)
+        tag = "" if pgrad else "_no_param_grad"
+        for name, prec in (("fp64", L.PLAIN_FP64), ("mixed", L.MIXED)):
+            for _ in range(warmup):
+                eng.compute(L.GRADIENT, terms, prec)
+                eng.sync()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                eng.compute(L.GRADIENT, terms, prec)
+                eng.sync()
+            ms = (time.perf_counter() - t0) * 1e3 / steps
+            if pgrad:
+                res[name] = eng.fetch()
+                pgs[name] = eng.param_grad()
+                out[f"tuned_shapes_{name}"] = eng.batch_tuning(prec)
+            if not np.all(eng.fetch_batch()["pcg_iters"] > 0):
+                raise RuntimeError("bench: a cluster's induced dipoles did not converge")
+            out[f"{name}_evals_per_s{tag}"] = round(1e3 / ms, 3)
+            out[f"{name}_ms_per_eval{tag}"] = round(ms, 4)
     d = res["mixed"]["desum"] - res["fp64"]["desum"]
     out["rms_force_err_mixed_vs_fp64"] = float(np.sqrt(np.mean(np.sum(d * d, axis=1))))
+    dp = pgs["mixed"]["dpole"] - pgs["fp64"]["dpole"]
+    out["max_dpole_err_mixed_vs_fp64_rel"] = float(np.abs(dp).max() / np.abs(pgs["fp64"]["dpole"]).max())
     out["speedup_mixed_vs_fp64"] = round(out["fp64_ms_per_eval"] / out["mixed_ms_per_eval"], 3)
+    eng.set_batch_param_grad(False)
     eng.set_batch(None)
     return out
 

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define PMQ_ABI_VERSION 5
+#define PMQ_ABI_VERSION 6
 
 #if defined(__GNUC__) || defined(__clang__)
 #define PMQ_API __attribute__((visibility("default")))
@@ -264,7 +264,8 @@ PMQ_API int pmq_fetch(pmq_ctx *ctx, pmq_result *res);
 PMQ_API int pmq_sync(pmq_ctx *ctx);
 
 /* Parameter gradients of the last evaluation (plain FP64, GRADIENT, one
- * system; no charge penetration or flux).  No reference counterpart: the
+ * system; no charge penetration or flux; or a batch, see
+ * pmq_set_batch_param_grad).  No reference counterpart: the
  * reference has no derivatives with respect to parameters.  Per atom, in
  * caller order: dE/dpole[13n] with respect to the local-frame components as
  * passed in pmq_params.pole (each stored component on its own), and, with
@@ -289,6 +290,43 @@ PMQ_API int pmq_param_grad(pmq_ctx *ctx, double *dpole, double *dpolarity, doubl
  * context to one system. */
 PMQ_API int pmq_set_batch(pmq_ctx *ctx, int nsys, const int *offsets);
 PMQ_API int pmq_fetch_batch(pmq_ctx *ctx, double *em, double *ep, int *pcg_iters);
+
+/* Parameter gradients of batched clusters.  No counterpart in the reference
+ * (it has no parameter derivatives).  With enable = 1 every GRADIENT
+ * evaluation of a batch, in either precision, also computes dE/d(parameters)
+ * of every atom inside the same persistent launch (row-owned sums over each
+ * cluster's pairs, the scaled pairs corrected afterwards; the mixed form
+ * evaluates the pair terms in FP32 and sums in FP64) and the per-site step
+ * in the same graph; pmq_param_grad then returns them, with the meaning and
+ * layout it has for one system (E = em + ep of all clusters; each atom's
+ * derivatives come from its own cluster).  pmq_compute_batch_noalloc does
+ * not compute them. */
+PMQ_API int pmq_set_batch_param_grad(pmq_ctx *ctx, int enable);
+
+/* The batched-cluster autotuner's choice.  No counterpart in the reference.
+ * Clusters are bucketed by size (more than 128 atoms, more than 64, the rest;
+ * workgroups of 256, 128 and 64 rows), each bucket one persistent launch, the
+ * three side by side on forked streams.  On the first evaluation of a batch
+ * in each precision the tuner walks the buckets (largest clusters first) and
+ * times every admissible shape of one bucket -- lanes per row (1-2, 1-4, 1-8
+ * by bucket) times 1, 2 or 4 persistent workgroups per compute unit -- on the
+ * whole batch with the other buckets at their current shapes, keeping the
+ * fastest; then one merged launch of all clusters in 256-row workgroups is
+ * timed in each of its shapes and kept if faster (reported as bucket 0
+ * holding every cluster).  Per bucket: rows per workgroup, lanes per row, workgroups per CU,
+ * the bucket's cluster count and the whole batch's timed launch in ms when
+ * that bucket's shape was chosen (-1 where no bucket or not yet tuned).
+ * Arrays of PMQ_BATCH_BUCKETS; NULL outputs are skipped. */
+#define PMQ_BATCH_BUCKETS 3
+PMQ_API int pmq_batch_tuning(pmq_ctx *ctx, int precision, int *rows, int *lanes, int *wgs_per_cu, int *nclusters,
+                             float *ms);
+
+/* Fix one bucket's shape instead of timing it (for tests and reproducible
+ * runs): lanes per row and workgroups per CU among those the bucket admits
+ * (PMQ_ERR_ARG otherwise); lanes = 0 returns the bucket to the autotuner.
+ * A fixed shape keeps the buckets apart (no merged launch).  Kept across
+ * pmq_set_batch calls.  No counterpart in the reference. */
+PMQ_API int pmq_set_batch_shape(pmq_ctx *ctx, int precision, int bucket, int lanes, int wgs_per_cu);
 
 /* The batched evaluation without allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory while they

@@ -49,7 +49,10 @@ def sources():
 # inputs).  They do not reassociate arithmetic.
 EXTRA = {"k_polar.hip": ["-fno-signed-zeros", "-ffinite-math-only"],
          # the quasi-internal-frame pairs evaluate the algebra at r = (0, 0, |r|)
-         "k_mpole.hip": ["-fno-signed-zeros", "-ffinite-math-only"]}
+         "k_mpole.hip": ["-fno-signed-zeros", "-ffinite-math-only"],
+         # parameter derivatives evaluate the pair terms with unit multipoles
+         "k_params.hip": ["-fno-signed-zeros", "-ffinite-math-only"],
+         "k_batch.hip": ["-fno-signed-zeros", "-ffinite-math-only"]}
 
 
 def _compile(src: str) -> str:

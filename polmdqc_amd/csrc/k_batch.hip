@@ -23,6 +23,7 @@
 #include <mutex>
 
 #include "pair_mpole.h"
+#include "param_pair.h"
 #include "pmq_internal.h"
 
 namespace pmq {
@@ -47,10 +48,15 @@ struct BatchArgs {
     double *em_c, *ep_c, *it_c;        // per cluster
     int *counter;
     Totals *tot;
+    // parameter gradients (pgrad): per atom, caller order, global frame
+    int pgrad;
+    double *pg_dm;                     // 10 x npad: dE/d(rotated multipole components)
+    double *pg_dpd, *pg_dth;           // dE/dpdamp, dE/dthole
+    double *uout;                      // 6 x npad: the dipoles (d, p), for dE/dpolarity
 };
 
-template <class T>
-__device__ __forceinline__ Mp<T> mp_at(const T (*m)[BATCH_MAX], int k)
+template <class T, int ROWS>
+__device__ __forceinline__ Mp<T> mp_at(const T (*m)[ROWS], int k)
 {
     Mp<T> r;
     r.c = m[0][k]; r.dx = m[1][k]; r.dy = m[2][k]; r.dz = m[3][k];
@@ -128,13 +134,13 @@ __device__ __forceinline__ void polgrad_pair(const T *R, const Mp<T> &mi, const 
 
 // LDS accumulator: FP64 atomics, or 64-bit fixed point (FX) so that the sum
 // does not depend on the order of the corrections
-template <bool FX>
+template <bool FX, int ROWS>
 struct LdsAcc {
-    double d[6][BATCH_MAX];
-    long long q[6][BATCH_MAX];
+    double d[6][ROWS];
+    long long q[6][ROWS];
     __device__ void clear(int tid, int nthr)
     {
-        for (int k = tid; k < 6 * BATCH_MAX; k += nthr) {
+        for (int k = tid; k < 6 * ROWS; k += nthr) {
             (&d[0][0])[k] = 0.0;
             (&q[0][0])[k] = 0;
         }
@@ -176,15 +182,15 @@ __device__ __forceinline__ void block_sum2(double a, double b, double *red, doub
     __syncthreads();
 }
 
-template <class T, int TPR, bool FX>
-__global__ __launch_bounds__(BATCH_MAX * TPR) void k_batch_clusters(BatchArgs a)
+template <class T, int ROWS, int TPR, bool FX>
+__global__ __launch_bounds__(ROWS * TPR) void k_batch_clusters(BatchArgs a)
 {
-    constexpr int NT = BATCH_MAX * TPR;
-    __shared__ double px[BATCH_MAX], py[BATCH_MAX], pz[BATCH_MAX];
-    __shared__ T mp[10][BATCH_MAX];
-    __shared__ T spd[BATCH_MAX], sth[BATCH_MAX];
-    __shared__ double vec[6][BATCH_MAX]; // shared vector: PCG direction, then the dipoles
-    __shared__ LdsAcc<FX> acc;
+    constexpr int NT = ROWS * TPR;
+    __shared__ double px[ROWS], py[ROWS], pz[ROWS];
+    __shared__ T mp[10][ROWS];
+    __shared__ T spd[ROWS], sth[ROWS];
+    __shared__ double vec[6][ROWS]; // shared vector: PCG direction, then the dipoles
+    __shared__ LdsAcc<FX, ROWS> acc;
     __shared__ double red[2 * NT];
     __shared__ int job;
     const int tid = threadIdx.x;
@@ -201,7 +207,7 @@ __global__ __launch_bounds__(BATCH_MAX * TPR) void k_batch_clusters(BatchArgs a)
         const int o0 = a.off[cl], nc = a.off[cl + 1] - o0;
         const int e0 = a.eoff[cl], e1 = a.eoff[cl + 1];
         // ---- load the cluster
-        for (int k = tid; k < BATCH_MAX; k += NT) {
+        for (int k = tid; k < ROWS; k += NT) {
             const bool on = k < nc;
             const int g = o0 + k;
             px[k] = on ? a.x[g] : 0.0; py[k] = on ? a.y[g] : 0.0; pz[k] = on ? a.z[g] : 0.0;
@@ -216,7 +222,7 @@ __global__ __launch_bounds__(BATCH_MAX * TPR) void k_batch_clusters(BatchArgs a)
         const int gi = o0 + (rowon ? row : 0);
         const double alpha = rowon ? a.pol[gi] : 0.0;
         const double xi = px[rowon ? row : 0], yi = py[rowon ? row : 0], zi = pz[rowon ? row : 0];
-        const Mp<T> mi = mp_at<T>(mp, rowon ? row : 0);
+        const Mp<T> mi = mp_at<T, ROWS>(mp, rowon ? row : 0);
         const T pdi = spd[rowon ? row : 0], thi = sth[rowon ? row : 0];
         double emc = 0.0, epc = 0.0;
         int iters = 0;
@@ -234,7 +240,7 @@ __global__ __launch_bounds__(BATCH_MAX * TPR) void k_batch_clusters(BatchArgs a)
                     T lam[6], R[6], Ei[3], Ek[3];
                     thole_lambda<T>(sqrt(r2), pdi, spd[k], thi, sth[k], lam);
                     pol_series<false, T>(r2, T(0), lam, R);
-                    pair_fields<T>(R, mi, mp_at<T>(mp, k), xr, yr, zr, Ei, Ek);
+                    pair_fields<T>(R, mi, mp_at<T, ROWS>(mp, k), xr, yr, zr, Ei, Ek);
                     for (int c = 0; c < 3; c++) e3[c] += Ei[c];
                 }
             for (int c = 0; c < 3; c++) e3[c] = row_sum<TPR>(e3[c]);
@@ -411,7 +417,7 @@ __global__ __launch_bounds__(BATCH_MAX * TPR) void k_batch_clusters(BatchArgs a)
                 const double xd = px[k] - xi, yd = py[k] - yi, zd = pz[k] - zi;
                 const T xr = T(xd), yr = T(yd), zr = T(zd);
                 const T r2 = xr * xr + yr * yr + zr * zr;
-                const Mp<T> mk = mp_at<T>(mp, k);
+                const Mp<T> mk = mp_at<T, ROWS>(mp, k);
                 if (a.do_mpole) {
                     T Rs[6];
                     rr_series<T>(r2, T(a.f), Rs);
@@ -534,6 +540,126 @@ __global__ __launch_bounds__(BATCH_MAX * TPR) void k_batch_clusters(BatchArgs a)
                 }
             }
         }
+        if (a.pgrad) {
+            // dE/d(parameters) of every atom of the cluster (param_pair.h):
+            // row-owned sums over all partners at scale 1, then the scaled
+            // pairs' corrections through the LDS accumulator in two passes
+            // (components 0-5; components 6-9, pdamp, Thole)
+            double g[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, gpd = 0.0, gth = 0.0;
+            if (rowon) {
+                T udi[3], upi[3];
+                for (int c = 0; c < 3; c++) {
+                    udi[c] = a.do_polar ? T(vec[c][row]) : T(0);
+                    upi[c] = a.do_polar ? T(vec[3 + c][row]) : T(0);
+                }
+                for (int k = part; k < nc; k += TPR) {
+                    if (k == row) continue;
+                    const double xd = px[k] - xi, yd = py[k] - yi, zd = pz[k] - zi;
+                    const T xr = T(xd), yr = T(yd), zr = T(zd);
+                    const T r2 = xr * xr + yr * yr + zr * zr;
+                    T rr[6], Rm[6], R[6], dl[6], D = T(0), pg = T(0);
+                    rr_series<T>(r2, T(1), rr);
+                    for (int q = 0; q < 6; q++) Rm[q] = T(a.f) * rr[q];
+                    if (a.do_polar) {
+                        T lam[6];
+                        thole_lambda<T>(sqrt(r2), pdi, spd[k], thi, sth[k], lam);
+                        for (int q = 0; q < 6; q++) R[q] = lam[q] * rr[q];
+                        thole_dlambda<T>(sqrt(r2), pdi, spd[k], thi, sth[k], dl, &D, &pg);
+                    } else {
+                        for (int q = 0; q < 6; q++) R[q] = dl[q] = T(0);
+                    }
+                    const Mp<T> mk = mp_at<T, ROWS>(mp, k);
+                    T udk[3], upk[3];
+                    for (int c = 0; c < 3; c++) {
+                        udk[c] = a.do_polar ? T(vec[c][k]) : T(0);
+                        upk[c] = a.do_polar ? T(vec[3 + c][k]) : T(0);
+                    }
+                    double dEdD = 0.0;
+                    param_pair<T, false>(a.do_mpole, a.do_polar, a.f, xr, yr, zr, mi, mk, Rm, R, rr, dl, T(1), T(1),
+                                         T(1), T(1), udi, upi, udk, upk, g, nullptr, &dEdD);
+                    if (D > T(0)) {
+                        gpd += dEdD * (-3.0 * (double)D / (double)pdi);
+                        gth += dEdD * ((double)D / (double)pg) * (double)thole_share<T>(thi, sth[k]);
+                    }
+                }
+            }
+            for (int c = 0; c < 10; c++) g[c] = row_sum<TPR>(g[c]);
+            gpd = row_sum<TPR>(gpd);
+            gth = row_sum<TPR>(gth);
+            const size_t np = a.npad;
+            for (int pass = 0; pass < 2; pass++) {
+                __syncthreads();
+                acc.clear(tid, NT);
+                __syncthreads();
+                for (int q = e0 + tid; q < e1; q += NT) {
+                    const ExclPair ex = a.excl[q];
+                    if (ex.m == 1.0 && ex.d == 1.0 && ex.p == 1.0 && ex.u == 1.0) continue;
+                    const int i = ex.i - o0, k = ex.k - o0;
+                    const double xr = px[k] - px[i], yr = py[k] - py[i], zr = pz[k] - pz[i];
+                    const double r2 = xr * xr + yr * yr + zr * zr;
+                    double rr[6], Rm[6], R[6], dl[6], D = 0.0, pg = 0.0;
+                    rr_series<double>(r2, 1.0, rr);
+                    for (int c = 0; c < 6; c++) Rm[c] = a.f * rr[c];
+                    const double pdi2 = a.pdm[ex.i], pdk2 = a.pdm[ex.k], thi2 = a.thl[ex.i], thk2 = a.thl[ex.k];
+                    if (a.do_polar) {
+                        double lam[6];
+                        thole_lambda<double>(sqrt(r2), pdi2, pdk2, thi2, thk2, lam);
+                        for (int c = 0; c < 6; c++) R[c] = lam[c] * rr[c];
+                        thole_dlambda<double>(sqrt(r2), pdi2, pdk2, thi2, thk2, dl, &D, &pg);
+                    } else {
+                        for (int c = 0; c < 6; c++) R[c] = dl[c] = 0.0;
+                    }
+                    Mp<double> m1, m2;
+                    for (int c = 0; c < 10; c++) {
+                        mp_set(m1, c, a.smp[c * np + ex.i]);
+                        mp_set(m2, c, a.smp[c * np + ex.k]);
+                    }
+                    double udi[3], upi[3], udk[3], upk[3];
+                    for (int c = 0; c < 3; c++) {
+                        udi[c] = a.do_polar ? vec[c][i] : 0.0; upi[c] = a.do_polar ? vec[3 + c][i] : 0.0;
+                        udk[c] = a.do_polar ? vec[c][k] : 0.0; upk[c] = a.do_polar ? vec[3 + c][k] : 0.0;
+                    }
+                    double gi[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, gk[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                    double dEdD = 0.0;
+                    param_pair<double, true>(a.do_mpole, a.do_polar, a.f, xr, yr, zr, m1, m2, Rm, R, rr, dl, ex.m - 1.0,
+                                             ex.d - 1.0, ex.p - 1.0, ex.u - 1.0, udi, upi, udk, upk, gi, gk, &dEdD);
+                    if (pass == 0) {
+                        for (int c = 0; c < 6; c++) {
+                            acc.add(c, i, gi[c], fx_grad);
+                            acc.add(c, k, gk[c], fx_grad);
+                        }
+                    } else {
+                        for (int c = 0; c < 4; c++) {
+                            acc.add(c, i, gi[6 + c], fx_grad);
+                            acc.add(c, k, gk[6 + c], fx_grad);
+                        }
+                        if (D > 0.0) {
+                            acc.add(4, i, dEdD * (-3.0 * D / pdi2), fx_grad);
+                            acc.add(4, k, dEdD * (-3.0 * D / pdk2), fx_grad);
+                            acc.add(5, i, dEdD * (D / pg) * thole_share<double>(thi2, thk2), fx_grad);
+                            acc.add(5, k, dEdD * (D / pg) * thole_share<double>(thk2, thi2), fx_grad);
+                        }
+                    }
+                }
+                __syncthreads();
+                if (own) {
+                    if (pass == 0) {
+                        for (int c = 0; c < 6; c++) g[c] += acc.get(c, row, fx_grad);
+                    } else {
+                        for (int c = 0; c < 4; c++) g[6 + c] += acc.get(c, row, fx_grad);
+                        gpd += acc.get(4, row, fx_grad);
+                        gth += acc.get(5, row, fx_grad);
+                    }
+                }
+            }
+            if (own) {
+                const size_t gidx = (size_t)(o0 + row);
+                for (int c = 0; c < 10; c++) a.pg_dm[c * np + gidx] = g[c];
+                a.pg_dpd[gidx] = gpd;
+                a.pg_dth[gidx] = gth;
+                for (int c = 0; c < 6; c++) a.uout[c * np + gidx] = a.do_polar ? vec[c][row] : 0.0;
+            }
+        }
         if (tid == 0) {
             a.em_c[cl] = a.do_mpole ? emc : 0.0;
             a.ep_c[cl] = epc;
@@ -566,15 +692,59 @@ __global__ void k_batch_totals(int nsys, const double *em_c, const double *ep_c,
 
 __global__ void k_batch_reset(int *counter)
 {
-    if (threadIdx.x == 0) *counter = 0;
+    if (threadIdx.x < PMQ_BATCH_BUCKETS) counter[threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------------------
 // host
+//
+// Clusters are grouped by size into buckets of up to 256, 128 and 64 atoms
+// (the workgroup's row count), each bucket one persistent launch over its
+// own slice of the largest-first order.  A shape is (rows, lanes per row,
+// workgroups per CU); the autotuner times every shape a bucket admits on
+// that bucket and keeps the fastest, per precision.
 
-int batch_variants() { return 2; }
+static const int kBucketRows[PMQ_BATCH_BUCKETS] = {256, 128, 64};
+static const int kLanes[PMQ_BATCH_BUCKETS][4] = {{1, 2, 0, 0}, {1, 2, 4, 0}, {1, 2, 4, 8}};
+static const int kWgPerCu[3] = {1, 2, 4};
 
-int launch_batch(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int variant, const BatchScratch *bs)
+int batch_bucket_rows(int b) { return kBucketRows[b]; }
+
+static int lane_options(int b)
+{
+    int n = 0;
+    while (n < 4 && kLanes[b][n]) n++;
+    return n;
+}
+
+int batch_variants(int b) { return lane_options(b) * 3; }
+
+void batch_shape(int b, int v, int *rows, int *lanes, int *wpc)
+{
+    const int nl = lane_options(b);
+    *rows = kBucketRows[b];
+    *lanes = kLanes[b][v % nl];
+    *wpc = kWgPerCu[v / nl];
+}
+
+template <class T, bool FX>
+static void launch_shape(int rows, int tpr, int wg, hipStream_t st, const BatchArgs &a)
+{
+    const dim3 g(wg), t(rows * tpr);
+    switch (rows * 16 + tpr) {
+    case 256 * 16 + 1: hipLaunchKernelGGL((k_batch_clusters<T, 256, 1, FX>), g, t, 0, st, a); break;
+    case 256 * 16 + 2: hipLaunchKernelGGL((k_batch_clusters<T, 256, 2, FX>), g, t, 0, st, a); break;
+    case 128 * 16 + 1: hipLaunchKernelGGL((k_batch_clusters<T, 128, 1, FX>), g, t, 0, st, a); break;
+    case 128 * 16 + 2: hipLaunchKernelGGL((k_batch_clusters<T, 128, 2, FX>), g, t, 0, st, a); break;
+    case 128 * 16 + 4: hipLaunchKernelGGL((k_batch_clusters<T, 128, 4, FX>), g, t, 0, st, a); break;
+    case 64 * 16 + 1: hipLaunchKernelGGL((k_batch_clusters<T, 64, 1, FX>), g, t, 0, st, a); break;
+    case 64 * 16 + 2: hipLaunchKernelGGL((k_batch_clusters<T, 64, 2, FX>), g, t, 0, st, a); break;
+    case 64 * 16 + 4: hipLaunchKernelGGL((k_batch_clusters<T, 64, 4, FX>), g, t, 0, st, a); break;
+    default: hipLaunchKernelGGL((k_batch_clusters<T, 64, 8, FX>), g, t, 0, st, a); break;
+    }
+}
+
+static BatchArgs batch_args(pmq_ctx *c, int mode, bool mpole, bool polar, const BatchScratch *bs, bool pgrad)
 {
     BatchArgs a;
     a.nsys = c->nsys;
@@ -595,20 +765,141 @@ int launch_batch(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int v
     a.it_c = bs ? bs->it_c : c->b_it.p;
     a.counter = bs ? bs->counter : c->b_counter.p;
     a.tot = bs ? bs->tot : c->totals.p;
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, c->stream, a.counter);
-    const int wg = std::max(1, std::min(c->nsys, c->num_cus * 2));
+    a.pgrad = pgrad && mode == PMQ_GRADIENT;
+    a.pg_dm = c->pg_dm.p; a.pg_dpd = c->pg_dpd.p; a.pg_dth = c->pg_dth.p;
+    a.uout = c->uind.p;
+    return a;
+}
+
+// one bucket's persistent launch with shape v on stream st (its counter already zero)
+static void launch_bucket(pmq_ctx *c, BatchArgs a, int b, int v, bool mixed, hipStream_t st, bool merged)
+{
+    const int n = merged ? c->nsys : c->b_beg[b + 1] - c->b_beg[b];
+    int rows, tpr, wpc;
+    batch_shape(b, v, &rows, &tpr, &wpc);
+    a.order = c->b_order.p + (merged ? 0 : c->b_beg[b]);
+    a.nsys = n;
+    a.counter += b;
+    const int wg = std::max(1, std::min(n, c->num_cus * wpc));
+    if (mixed) launch_shape<float, true>(rows, tpr, wg, st, a);
+    else launch_shape<double, false>(rows, tpr, wg, st, a);
+}
+
+// every bucket's launch, the buckets side by side: the first on the
+// context's stream, the others on their own streams forked from it and
+// joined back (the same in a captured graph).  merged: all clusters in one
+// launch with the first bucket's workgroups (256 rows).
+static int launch_buckets(pmq_ctx *c, const BatchArgs &a, const int *variants, bool mixed, bool merged)
+{
+    int act[PMQ_BATCH_BUCKETS], na = 0;
+    for (int b = 0; b < PMQ_BATCH_BUCKETS; b++)
+        if (merged ? b == 0 : c->b_beg[b + 1] > c->b_beg[b]) act[na++] = b;
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, c->stream, a.counter);
+    if (na > 1) PMQ_HIP(hipEventRecord(c->b_fork, c->stream));
+    for (int k = 1; k < na; k++) {
+        PMQ_HIP(hipStreamWaitEvent(c->b_side[k - 1], c->b_fork, 0));
+        launch_bucket(c, a, act[k], variants[act[k]], mixed, c->b_side[k - 1], false);
+        PMQ_HIP(hipEventRecord(c->b_join[k - 1], c->b_side[k - 1]));
+    }
+    if (na > 0) launch_bucket(c, a, act[0], variants[act[0]], mixed, c->stream, merged);
+    for (int k = 1; k < na; k++) PMQ_HIP(hipStreamWaitEvent(c->stream, c->b_join[k - 1], 0));
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+int launch_batch(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, const int *variants, const BatchScratch *bs,
+                 bool pgrad, bool merged)
+{
+    const BatchArgs a = batch_args(c, mode, mpole, polar, bs, pgrad);
+    int rc;
     {
         ProfScope ps_(c, "batch_clusters");
-        if (variant == 0) {
-            if (mixed) hipLaunchKernelGGL((k_batch_clusters<float, 1, true>), dim3(wg), dim3(BATCH_MAX), 0, c->stream, a);
-            else hipLaunchKernelGGL((k_batch_clusters<double, 1, false>), dim3(wg), dim3(BATCH_MAX), 0, c->stream, a);
-        } else {
-            if (mixed) hipLaunchKernelGGL((k_batch_clusters<float, 2, true>), dim3(wg), dim3(2 * BATCH_MAX), 0, c->stream, a);
-            else hipLaunchKernelGGL((k_batch_clusters<double, 2, false>), dim3(wg), dim3(2 * BATCH_MAX), 0, c->stream, a);
-        }
+        if ((rc = launch_buckets(c, a, variants, mixed, merged))) return rc;
     }
     hipLaunchKernelGGL(k_batch_totals, dim3(1), dim3(256), 0, c->stream, c->nsys, a.em_c, a.ep_c, a.tot);
     PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
+// The autotuner: coordinate descent over the buckets (largest clusters
+// first), each bucket's shapes timed on the whole batch with the other
+// buckets at their current shapes (the buckets run side by side, so one
+// bucket's best shape depends on the others'); then every shape of one
+// merged launch of all clusters in 256-row workgroups, kept if faster.  The
+// second of two launches is timed.  Records the choice in c->b_var, c->b_ms
+// and c->b_merged for this precision; shapes fixed by pmq_set_batch_shape are
+// kept (and rule out the merged launch).
+int batch_tune(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, bool pgrad)
+{
+    const int p = mixed ? 1 : 0;
+    int rc;
+    if ((rc = launch_rotpole(c, false))) return rc;
+    const BatchArgs a = batch_args(c, mode, mpole, polar, nullptr, pgrad);
+    hipEvent_t e0, e1;
+    PMQ_HIP(hipEventCreate(&e0));
+    PMQ_HIP(hipEventCreate(&e1));
+    int var[PMQ_BATCH_BUCKETS];
+    bool pinned = false;
+    for (int b = 0; b < PMQ_BATCH_BUCKETS; b++) {
+        var[b] = c->b_pin[p][b] ? c->b_var[p][b] : 0;
+        pinned = pinned || c->b_pin[p][b];
+    }
+    auto timed = [&](bool merged, float *ms) -> int {
+        for (int rep = 0; rep < 2; rep++) {
+            if (rep) PMQ_HIP(hipEventRecord(e0, c->stream));
+            int r = launch_buckets(c, a, var, mixed, merged);
+            if (r) return r;
+            if (rep) PMQ_HIP(hipEventRecord(e1, c->stream));
+        }
+        PMQ_HIP(hipEventSynchronize(e1));
+        PMQ_HIP(hipEventElapsedTime(ms, e0, e1));
+        return PMQ_OK;
+    };
+    float split = 1e30f;
+    for (int b = 0; b < PMQ_BATCH_BUCKETS; b++) {
+        if (c->b_pin[p][b]) continue;
+        c->b_ms[p][b] = -1.0f;
+        if (c->b_beg[b + 1] == c->b_beg[b]) continue;
+        float best = 1e30f;
+        int bestv = 0;
+        for (int v = 0; v < batch_variants(b); v++) {
+            var[b] = v;
+            float ms = 0.0f;
+            if ((rc = timed(false, &ms))) return rc;
+            if (ms < best) {
+                best = ms;
+                bestv = v;
+            }
+        }
+        var[b] = bestv;
+        c->b_ms[p][b] = best;
+        split = best;
+    }
+    bool merged = false;
+    if (!pinned && c->nsys > 0) {
+        const int keep0 = var[0];
+        float best = split;
+        int bestv = -1;
+        for (int v = 0; v < batch_variants(0); v++) {
+            var[0] = v;
+            float ms = 0.0f;
+            if ((rc = timed(true, &ms))) return rc;
+            if (ms < best) {
+                best = ms;
+                bestv = v;
+            }
+        }
+        var[0] = keep0;
+        if (bestv >= 0) {
+            merged = true;
+            var[0] = bestv;
+            for (int b = 0; b < PMQ_BATCH_BUCKETS; b++) c->b_ms[p][b] = b == 0 ? best : -1.0f;
+        }
+    }
+    for (int b = 0; b < PMQ_BATCH_BUCKETS; b++) c->b_var[p][b] = var[b];
+    c->b_merged[p] = merged;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return PMQ_OK;
 }
 
@@ -623,8 +914,8 @@ int launch_batch(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int v
 // one GPU race.
 struct StaticScratchLayout {
     Totals tot;
-    int counter;
-    int pad[63];
+    int counter[PMQ_BATCH_BUCKETS];
+    int pad[64 - PMQ_BATCH_BUCKETS];
     double em_c[PMQ_NOALLOC_MAX_CLUSTERS];
     double ep_c[PMQ_NOALLOC_MAX_CLUSTERS];
     double it_c[PMQ_NOALLOC_MAX_CLUSTERS];
@@ -677,8 +968,8 @@ __global__ void k_batch_static_out(int nsys, int n3, const double *em_c, const d
     }
 }
 
-int batch_noalloc_enqueue(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int variant, double *em,
-                          double *ep, int *iters, double *etot, double *grad)
+int batch_noalloc_enqueue(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, const int *variants, bool merged,
+                          double *em, double *ep, int *iters, double *etot, double *grad)
 {
     // device addresses inside this device's array (from static_scratch_attach)
     unsigned char *base = (unsigned char *)c->static_scratch;
@@ -693,7 +984,7 @@ int batch_noalloc_enqueue(pmq_ctx *c, int mode, bool mpole, bool polar, bool mix
     // the totals start at zero (a fill kernel: nothing here records a memset)
     if ((rc = launch_zero(c, (double *)bs.tot, sizeof(Totals) / sizeof(double)))) return rc;
     if ((rc = launch_rotpole(c, false))) return rc;
-    if ((rc = launch_batch(c, mode, mpole, polar, mixed, variant, &bs))) return rc;
+    if ((rc = launch_batch(c, mode, mpole, polar, mixed, variants, &bs, false, merged))) return rc;
     if (grad_mode) {
         if (mpole && (rc = launch_torque(c, PMQ_GRADIENT, c->te.p, c->dem.p, mixed))) return rc;
         if (polar && (rc = launch_torque(c, PMQ_GRADIENT, c->tep.p, c->dep.p, mixed))) return rc;

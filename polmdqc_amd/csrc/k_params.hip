@@ -23,6 +23,7 @@
 // is distributed to pd_i, pd_k (-3 D / pd) and to the Thole parameter of the
 // smaller (half to each when equal, as a central difference sees it).
 #include "pair_mpole.h"
+#include "param_pair.h"
 #include "pmq_internal.h"
 #include "pme_common.h"
 
@@ -45,143 +46,11 @@ struct ParamArgs {
     double *dm, *dpd, *dth;   // sorted outputs (10 npad, npad, npad)
 };
 
-__device__ __forceinline__ Mp<double> unit_mp(int c)
-{
-    Mp<double> m{};
-    mp_set(m, c, 1.0);
-    return m;
-}
-
 __device__ __forceinline__ Mp<double> mp_sorted(const double *smp, int np, int s)
 {
     Mp<double> m;
     for (int c = 0; c < 10; c++) mp_set(m, c, smp[(size_t)c * np + s]);
     return m;
-}
-
-// dlambda_n / dD in dl[1..4] (dl[0] = dl[5] = 0); zero where the factors are 1
-__device__ __forceinline__ double thole_dlambda(double r, double pdi, double pdk, double ti, double tk, double *dl,
-                                                double *Dout, double *pgout)
-{
-    for (int q = 0; q < 6; q++) dl[q] = 0.0;
-    *Dout = 0.0;
-    *pgout = 0.0;
-    const double damp = pdi * pdk;
-    if (damp == 0.0) return 0.0;
-    const double mn = fmin(ti, tk);
-    const double pg = mn != 0.0 ? mn : fmax(ti, tk);
-    const double uu = r / damp;
-    const double D = pg * uu * uu * uu;
-    if (D >= 50.0) return 0.0;
-    const double e = exp(-D);
-    dl[1] = e;
-    dl[2] = D * e;
-    dl[3] = D * (3.0 * D - 1.0) / 5.0 * e;
-    dl[4] = D * (9.0 * D * D - 9.0 * D - 1.0) / 35.0 * e;
-    *Dout = D;
-    *pgout = pg;
-    return D;
-}
-
-// share of d a_ik / d a_i for a_ik = min(a_i, a_k) (max when the min is 0)
-__device__ __forceinline__ double thole_share(double ti, double tk)
-{
-    const double mn = fmin(ti, tk);
-    if (mn != 0.0) return ti < tk ? 1.0 : (ti > tk ? 0.0 : 0.5);
-    return ti > tk ? 1.0 : (ti < tk ? 0.0 : 0.5);
-}
-
-// T v at i from dipole v at k with series R (the dipole-field operator)
-__device__ __forceinline__ void tdip(const double *R, const double *v, double xr, double yr, double zr, double *o)
-{
-    const double vr = v[0] * xr + v[1] * yr + v[2] * zr;
-    o[0] = vr * R[2] * xr - R[1] * v[0];
-    o[1] = vr * R[2] * yr - R[1] * v[1];
-    o[2] = vr * R[2] * zr - R[1] * v[2];
-}
-
-// one pair's contributions, scaled: m (multipole energy), sd / sp (direct
-// fields E^d / E^p), su (T).  Rm: multipole series (f included); R: the
-// polarization series at scale 1 (Thole-damped, Ewald-screened where it
-// applies); rr: the undamped series; dl: dlambda/dD.
-__device__ void param_pair(const ParamArgs &a, double xr, double yr, double zr, const Mp<double> &mi,
-                           const Mp<double> &mk, const double *Rm, const double *R, const double *rr, const double *dl,
-                           double sm, double sd, double sp, double su, const double *udi, const double *upi,
-                           const double *udk, const double *upk, double *gi, double *gk, double *dEdD)
-{
-    if (a.do_mpole && sm != 0.0) {
-        double Rs[6];
-        for (int q = 0; q < 6; q++) Rs[q] = sm * Rm[q];
-        for (int c = 0; c < 10; c++) {
-            const Mp<double> e = unit_mp(c);
-            PairOut<double> o;
-            pair_series<double, false>(Rs, e, mk, xr, yr, zr, o);
-            gi[c] += o.e;
-            pair_series<double, false>(Rs, mi, e, xr, yr, zr, o);
-            gk[c] += o.e;
-        }
-    }
-    if (!a.do_polar) return;
-    const double hf = 0.5 * a.f;
-    // direct fields: E_i(e_c at k) weighted by (sd mu_p,i + sp mu_d,i), and the k side
-    if (sd != 0.0 || sp != 0.0) {
-        const double wi[3] = {sd * upi[0] + sp * udi[0], sd * upi[1] + sp * udi[1], sd * upi[2] + sp * udi[2]};
-        const double wk[3] = {sd * upk[0] + sp * udk[0], sd * upk[1] + sp * udk[1], sd * upk[2] + sp * udk[2]};
-        for (int c = 0; c < 10; c++) {
-            const Mp<double> e = unit_mp(c);
-            double Ei[3], Ek[3];
-            // field at i from e_c at k, and at k from e_c at i
-            {
-                const double qkx = e.qxx * xr + e.qxy * yr + e.qxz * zr, qky = e.qxy * xr + e.qyy * yr + e.qyz * zr,
-                             qkz = e.qxz * xr + e.qyz * yr + e.qzz * zr;
-                const double dkr = e.dx * xr + e.dy * yr + e.dz * zr;
-                const double qkr = qkx * xr + qky * yr + qkz * zr;
-                const double si = e.c * R[1] - dkr * R[2] + qkr * R[3];
-                Ei[0] = -si * xr - R[1] * e.dx + 2.0 * R[2] * qkx;
-                Ei[1] = -si * yr - R[1] * e.dy + 2.0 * R[2] * qky;
-                Ei[2] = -si * zr - R[1] * e.dz + 2.0 * R[2] * qkz;
-                const double dir = dkr, qir = qkr;
-                const double sk = e.c * R[1] + dir * R[2] + qir * R[3];
-                Ek[0] = sk * xr - R[1] * e.dx - 2.0 * R[2] * qkx;
-                Ek[1] = sk * yr - R[1] * e.dy - 2.0 * R[2] * qky;
-                Ek[2] = sk * zr - R[1] * e.dz - 2.0 * R[2] * qkz;
-            }
-            gk[c] -= hf * (wi[0] * Ei[0] + wi[1] * Ei[1] + wi[2] * Ei[2]);
-            gi[c] -= hf * (wk[0] * Ek[0] + wk[1] * Ek[1] + wk[2] * Ek[2]);
-        }
-    }
-    // damping: dE/dD of this pair
-    double dR[6];
-    for (int q = 0; q < 6; q++) dR[q] = dl[q] * rr[q];
-    if (dR[1] == 0.0 && dR[2] == 0.0 && dR[3] == 0.0) return;
-    double acc = 0.0;
-    if (sd != 0.0 || sp != 0.0) {
-        double Ei[3], Ek[3];
-        // fields of M_k at i and of M_i at k with the derivative series
-        const double qix = mi.qxx * xr + mi.qxy * yr + mi.qxz * zr, qiy = mi.qxy * xr + mi.qyy * yr + mi.qyz * zr,
-                     qiz = mi.qxz * xr + mi.qyz * yr + mi.qzz * zr;
-        const double qkx = mk.qxx * xr + mk.qxy * yr + mk.qxz * zr, qky = mk.qxy * xr + mk.qyy * yr + mk.qyz * zr,
-                     qkz = mk.qxz * xr + mk.qyz * yr + mk.qzz * zr;
-        const double dir = mi.dx * xr + mi.dy * yr + mi.dz * zr, dkr = mk.dx * xr + mk.dy * yr + mk.dz * zr;
-        const double qir = qix * xr + qiy * yr + qiz * zr, qkr = qkx * xr + qky * yr + qkz * zr;
-        const double si = mk.c * dR[1] - dkr * dR[2] + qkr * dR[3];
-        const double sk = mi.c * dR[1] + dir * dR[2] + qir * dR[3];
-        Ei[0] = -si * xr - dR[1] * mk.dx + 2.0 * dR[2] * qkx;
-        Ei[1] = -si * yr - dR[1] * mk.dy + 2.0 * dR[2] * qky;
-        Ei[2] = -si * zr - dR[1] * mk.dz + 2.0 * dR[2] * qkz;
-        Ek[0] = sk * xr - dR[1] * mi.dx - 2.0 * dR[2] * qix;
-        Ek[1] = sk * yr - dR[1] * mi.dy - 2.0 * dR[2] * qiy;
-        Ek[2] = sk * zr - dR[1] * mi.dz - 2.0 * dR[2] * qiz;
-        for (int c = 0; c < 3; c++)
-            acc += (sd * upi[c] + sp * udi[c]) * Ei[c] + (sd * upk[c] + sp * udk[c]) * Ek[c];
-    }
-    if (su != 0.0) {
-        double t1[3], t2[3];
-        tdip(dR, udk, xr, yr, zr, t1); // at i from mu_d,k
-        tdip(dR, udi, xr, yr, zr, t2); // at k from mu_d,i
-        for (int c = 0; c < 3; c++) acc += su * (upi[c] * t1[c] + upk[c] * t2[c]);
-    }
-    *dEdD += -hf * acc;
 }
 
 __global__ __launch_bounds__(64) void k_param_tiles(ParamArgs a)
@@ -240,8 +109,9 @@ __global__ __launch_bounds__(64) void k_param_tiles(ParamArgs a)
             double gk[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, dEdD = 0.0;
             const double sm = ((mp_mask >> jj) & 1ull) ? 1.0 : 0.0;
             const bool pol_pair = (a.mask[(size_t)t * 128 + lane] >> jj) & 1ull;
-            param_pair(a, xr, yr, zr, mi, mk, Rm, R, rr, dl, sm, pol_pair ? 1.0 : 0.0, pol_pair ? 1.0 : 0.0,
-                       pol_pair ? 1.0 : 0.0, udi, upi, udk, upk, gi, gk, &dEdD);
+            param_pair<double, true>(a.do_mpole, a.do_polar, a.f, xr, yr, zr, mi, mk, Rm, R, rr, dl, sm,
+                                     pol_pair ? 1.0 : 0.0, pol_pair ? 1.0 : 0.0, pol_pair ? 1.0 : 0.0, udi, upi, udk,
+                                     upk, gi, gk, &dEdD);
             for (int c = 0; c < 10; c++) atomicAdd(&acc[c][jj], gk[c]);
             if (D > 0.0 && pol_pair) {
                 dpdi += dEdD * (-3.0 * D / pdi);
@@ -295,8 +165,8 @@ __global__ __launch_bounds__(64) void k_param_excl(ParamArgs a)
         udk[c] = a.do_polar ? a.u[(size_t)c * np + sk] : 0.0; upk[c] = a.do_polar ? a.u[(size_t)(3 + c) * np + sk] : 0.0;
     }
     double gi[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, gk[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, dEdD = 0.0;
-    param_pair(a, xr, yr, zr, mi, mk, Rm, R, rr, dl, q.m - 1.0, q.d - 1.0, q.p - 1.0, q.u - 1.0, udi, upi, udk, upk,
-               gi, gk, &dEdD);
+    param_pair<double, true>(a.do_mpole, a.do_polar, a.f, xr, yr, zr, mi, mk, Rm, R, rr, dl, q.m - 1.0, q.d - 1.0,
+                             q.p - 1.0, q.u - 1.0, udi, upi, udk, upk, gi, gk, &dEdD);
     for (int c = 0; c < 10; c++) {
         atomicAdd(&a.dm[(size_t)c * np + si], gi[c]);
         atomicAdd(&a.dm[(size_t)c * np + sk], gk[c]);
@@ -415,6 +285,33 @@ __global__ __launch_bounds__(128) void k_param_site(SiteArgs a)
     }
 }
 
+// the per-site step: reciprocal and self parts (Ewald), rotation of the
+// global derivatives to the local frames, dE/dpolarity; outputs in caller
+// order (13n, n, n, n)
+int launch_param_site(pmq_ctx *c, bool mpole, bool polar, double *dpole, double *dpolarity, double *dpdamp,
+                      double *dthole)
+{
+    const int np = c->npad, n = c->n;
+    SiteArgs s;
+    s.n = n; s.npad = np; s.perm = c->perm.p; s.smp = c->smp.p; s.spol = c->spol.p;
+    s.cphi = c->cphi.p;
+    s.frac = (polar && c->prm.use_ewald) ? c->pme_frac.p : nullptr;
+    s.u = c->uind.p;
+    for (int p = 0; p < 3; p++)
+        for (int q = 0; q < 3; q++) s.A[p][q] = c->pmeA[p][q];
+    s.f = c->prm.electric / c->prm.dielec; s.aewald = c->prm.aewald; s.ewald = c->prm.use_ewald;
+    s.do_mpole = mpole; s.do_polar = polar;
+    s.dm = c->pg_dm.p; s.dpd = c->pg_dpd.p; s.dth = c->pg_dth.p;
+    s.rotm = c->rotm.p; s.flip = c->flip.p;
+    s.polarity = c->polarity.p;
+    s.dpole = dpole;
+    const bool pol_out = polar && !c->h_polarity.empty();
+    s.dpolarity = pol_out ? dpolarity : nullptr; s.dpdamp = dpdamp; s.dthole = dthole;
+    hipLaunchKernelGGL(k_param_site, dim3((n + 127) / 128), dim3(128), 0, c->stream, s);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
 int param_gradients(pmq_ctx *c, bool mpole, bool polar, double *dpole, double *dpolarity, double *dpdamp,
                     double *dthole)
 {
@@ -449,23 +346,8 @@ int param_gradients(pmq_ctx *c, bool mpole, bool polar, double *dpole, double *d
     DBuf<double> d_pole, d_pol, d_pd, d_th;
     PMQ_HIP(d_pole.alloc(13 * (size_t)n));
     PMQ_HIP(d_pol.alloc(n)); PMQ_HIP(d_pd.alloc(n)); PMQ_HIP(d_th.alloc(n));
-    SiteArgs s;
-    s.n = n; s.npad = np; s.perm = c->perm.p; s.smp = c->smp.p; s.spol = c->spol.p;
-    s.cphi = c->cphi.p;
-    s.frac = (polar && c->prm.use_ewald) ? c->pme_frac.p : nullptr;
-    s.u = c->uind.p;
-    for (int p = 0; p < 3; p++)
-        for (int q = 0; q < 3; q++) s.A[p][q] = c->pmeA[p][q];
-    s.f = a.f; s.aewald = c->prm.aewald; s.ewald = c->prm.use_ewald;
-    s.do_mpole = mpole; s.do_polar = polar;
-    s.dm = c->pg_dm.p; s.dpd = c->pg_dpd.p; s.dth = c->pg_dth.p;
-    s.rotm = c->rotm.p; s.flip = c->flip.p;
-    s.polarity = c->polarity.p;
-    s.dpole = d_pole.p;
+    if ((rc = launch_param_site(c, mpole, polar, d_pole.p, d_pol.p, d_pd.p, d_th.p))) return rc;
     const bool pol_out = polar && !c->h_polarity.empty();
-    s.dpolarity = pol_out ? d_pol.p : nullptr; s.dpdamp = d_pd.p; s.dthole = d_th.p;
-    hipLaunchKernelGGL(k_param_site, dim3((n + 127) / 128), dim3(128), 0, c->stream, s);
-    PMQ_HIP(hipGetLastError());
     PMQ_HIP(hipStreamSynchronize(c->stream));
     if (dpole) PMQ_HIP(hipMemcpy(dpole, d_pole.p, 13 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     if (pol_out) {

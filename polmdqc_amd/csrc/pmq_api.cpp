@@ -248,10 +248,16 @@ int pmq_destroy(pmq_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     pme_release(c);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    if (c->h_status) (void)hipHostFree(c->h_status);
     eval_graph_release(c);
     batch_graph_release(c);
+    for (int k = 0; k < PMQ_BATCH_BUCKETS - 1; k++) {
+        if (c->b_side[k]) (void)hipStreamSynchronize(c->b_side[k]);
+        if (c->b_side[k]) (void)hipStreamDestroy(c->b_side[k]);
+        if (c->b_join[k]) (void)hipEventDestroy(c->b_join[k]);
+    }
+    if (c->b_fork) (void)hipEventDestroy(c->b_fork);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->h_status) (void)hipHostFree(c->h_status);
     delete c;
     return PMQ_OK;
 }
@@ -678,7 +684,8 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
     PMQ_HIP(hipStreamSynchronize(c->stream));
     c->have_prm = true;
     batch_graph_release(c);
-    c->batch_variant = -1;
+    c->b_tuned[0] = c->b_tuned[1] = false;
+    c->b_pg_valid = false;
     c->safe = true;
     c->pending = false;
     c->pcg_guess_iters = 0;
@@ -914,18 +921,26 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
 }
 
 // one evaluation of the batched clusters: frames, the persistent cluster
-// kernel, torques
-static int batch_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, int variant)
+// kernels (one per size bucket), torques, and with parameter gradients the
+// per-site step that turns them into local-frame derivatives
+static int batch_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, const int *variants)
 {
     const bool mixed = precision == PMQ_MIXED;
     const bool mp = terms & PMQ_TERM_MPOLE, pol = terms & PMQ_TERM_POLAR;
+    const bool pg = c->b_pgrad && mode == PMQ_GRADIENT;
     int rc;
     if ((rc = launch_zero(c, (double *)c->totals.p, sizeof(Totals) / sizeof(double)))) return rc;
     if ((rc = launch_rotpole(c, false))) return rc;
-    if ((rc = launch_batch(c, mode, mp, pol, mixed, variant))) return rc;
+    const bool merged = c->b_merged[mixed ? 1 : 0];
+    if ((rc = launch_batch(c, mode, mp, pol, mixed, variants, nullptr, pg, merged))) return rc;
     if (mode == PMQ_GRADIENT) {
         if (mp && (rc = launch_torque(c, PMQ_GRADIENT, c->te.p, c->dem.p, mixed))) return rc;
         if (pol && (rc = launch_torque(c, PMQ_GRADIENT, c->tep.p, c->dep.p, mixed))) return rc;
+    }
+    if (pg) {
+        const size_t n = c->n;
+        double *o = c->pgo.p;
+        if ((rc = launch_param_site(c, mp, pol, o, o + 13 * n, o + 14 * n, o + 15 * n))) return rc;
     }
     return PMQ_OK;
 }
@@ -955,6 +970,22 @@ static int batch_check(pmq_ctx *c, int mode, unsigned terms)
     return PMQ_OK;
 }
 
+// buffers of the batched parameter gradients; true if any moved (a captured
+// graph holds their addresses)
+static int batch_pgrad_buffers(pmq_ctx *c, bool *moved)
+{
+    const void *before[5] = {c->pg_dm.p, c->pg_dpd.p, c->pg_dth.p, c->pgo.p, c->uind.p};
+    PMQ_HIP(c->pg_dm.alloc(10 * (size_t)c->npad));
+    PMQ_HIP(c->pg_dpd.alloc(c->npad));
+    PMQ_HIP(c->pg_dth.alloc(c->npad));
+    PMQ_HIP(c->pgo.alloc(16 * (size_t)c->n));
+    PMQ_HIP(c->uind.alloc(6 * (size_t)c->npad));
+    const void *after[5] = {c->pg_dm.p, c->pg_dpd.p, c->pg_dth.p, c->pgo.p, c->uind.p};
+    *moved = false;
+    for (int k = 0; k < 5; k++) *moved = *moved || before[k] != after[k];
+    return PMQ_OK;
+}
+
 static int batch_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
 {
     int rc = batch_check(c, mode, terms);
@@ -965,34 +996,31 @@ static int batch_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
     c->last_precision = precision;
     c->pending = false;
     c->pcg_iters = 0;
-    if (c->batch_variant < 0) {
-        // autotune the lanes-per-row variant on this batch: the faster of two
-        // timed runs each (the first run of a variant also warms it up)
-        hipEvent_t e0, e1;
-        PMQ_HIP(hipEventCreate(&e0));
-        PMQ_HIP(hipEventCreate(&e1));
-        float best = 1e30f;
-        int bestv = 0;
-        for (int v = 0; v < batch_variants(); v++) {
-            if ((rc = batch_enqueue(c, mode, terms, precision, v))) return rc;
-            PMQ_HIP(hipEventRecord(e0, c->stream));
-            if ((rc = batch_enqueue(c, mode, terms, precision, v))) return rc;
-            PMQ_HIP(hipEventRecord(e1, c->stream));
-            PMQ_HIP(hipEventSynchronize(e1));
-            float ms = 0.0f;
-            PMQ_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best) { best = ms; bestv = v; }
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        c->batch_variant = bestv;
+    const bool pg = c->b_pgrad && mode == PMQ_GRADIENT;
+    if (pg) {
+        bool moved = false;
+        if ((rc = batch_pgrad_buffers(c, &moved))) return rc;
+        if (moved) batch_graph_release(c);
     }
-    const int key = ((mode * 4 + (int)terms) * 2 + precision) * 8 + c->batch_variant;
-    if (c->profiling) return batch_enqueue(c, mode, terms, precision, c->batch_variant); // events are not captured
+    const int p = precision == PMQ_MIXED ? 1 : 0;
+    if (!c->b_tuned[p] || c->b_tuned_pg[p] != pg) {
+        // the autotuner: the shapes of the size buckets timed on this batch
+        if ((rc = batch_tune(c, mode, terms & PMQ_TERM_MPOLE, terms & PMQ_TERM_POLAR, p == 1, pg))) return rc;
+        c->b_tuned[p] = true;
+        c->b_tuned_pg[p] = pg;
+    }
+    const int *var = c->b_var[p];
+    const int key = (((((mode * 8 + (int)terms) * 2 + precision) * 2 + (pg ? 1 : 0)) * 2 + (c->b_merged[p] ? 1 : 0)) *
+                         16 + var[0]) * 256 + var[1] * 16 + var[2];
+    if (c->profiling) { // events are not captured
+        rc = batch_enqueue(c, mode, terms, precision, var);
+        c->b_pg_valid = pg && !rc;
+        return rc;
+    }
     if (c->b_graph_key != key) {
         batch_graph_release(c);
         PMQ_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = batch_enqueue(c, mode, terms, precision, c->batch_variant);
+        rc = batch_enqueue(c, mode, terms, precision, var);
         hipGraph_t g = nullptr;
         const hipError_t ce = hipStreamEndCapture(c->stream, &g);
         if (rc) return rc;
@@ -1004,6 +1032,72 @@ static int batch_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
         c->b_graph_key = key;
     }
     PMQ_HIP(hipGraphLaunch((hipGraphExec_t)c->b_exec, c->stream));
+    c->b_pg_valid = pg;
+    return PMQ_OK;
+}
+
+int pmq_set_batch_param_grad(pmq_ctx *c, int enable)
+{
+    CHECK_CTX(c);
+    SETTLE_PENDING(c);
+    new_epoch(c);
+    batch_graph_release(c);
+    c->b_pgrad = enable != 0;
+    c->b_pg_valid = false;
+    return PMQ_OK;
+}
+
+int pmq_set_batch_shape(pmq_ctx *c, int precision, int bucket, int lanes, int wgs_per_cu)
+{
+    CHECK_CTX(c);
+    SETTLE_PENDING(c);
+    if ((precision != PMQ_PLAIN_FP64 && precision != PMQ_MIXED) || bucket < 0 || bucket >= PMQ_BATCH_BUCKETS) {
+        c->err = "pmq_set_batch_shape: precision PMQ_PLAIN_FP64 or PMQ_MIXED, bucket 0..2";
+        return PMQ_ERR_ARG;
+    }
+    const int p = precision == PMQ_MIXED ? 1 : 0;
+    if (lanes == 0) {
+        c->b_pin[p][bucket] = false;
+        c->b_tuned[p] = false; // the tuner decides again
+        batch_graph_release(c);
+        return PMQ_OK;
+    }
+    for (int v = 0; v < batch_variants(bucket); v++) {
+        int r, l, w;
+        batch_shape(bucket, v, &r, &l, &w);
+        if (l == lanes && w == wgs_per_cu) {
+            new_epoch(c);
+            batch_graph_release(c);
+            c->b_var[p][bucket] = v;
+            c->b_ms[p][bucket] = -1.0f;
+            c->b_pin[p][bucket] = true;
+            c->b_merged[p] = false;
+            return PMQ_OK;
+        }
+    }
+    c->err = "pmq_set_batch_shape: no such shape for this bucket";
+    return PMQ_ERR_ARG;
+}
+
+int pmq_batch_tuning(pmq_ctx *c, int precision, int *rows, int *lanes, int *wgs_per_cu, int *nclusters, float *ms)
+{
+    CHECK_CTX(c);
+    if (precision != PMQ_PLAIN_FP64 && precision != PMQ_MIXED) {
+        c->err = "pmq_batch_tuning: precision is PMQ_PLAIN_FP64 or PMQ_MIXED";
+        return PMQ_ERR_ARG;
+    }
+    const int p = precision == PMQ_MIXED ? 1 : 0;
+    const bool merged = c->b_tuned[p] && c->b_merged[p];
+    for (int b = 0; b < PMQ_BATCH_BUCKETS; b++) {
+        const int nb = c->nsys > 0 ? (merged ? (b == 0 ? c->nsys : 0) : c->b_beg[b + 1] - c->b_beg[b]) : 0;
+        int r = -1, l = -1, w = -1;
+        if (c->nsys > 0 && (c->b_tuned[p] || c->b_pin[p][b]) && nb > 0) batch_shape(b, c->b_var[p][b], &r, &l, &w);
+        if (rows) rows[b] = c->nsys > 0 ? batch_bucket_rows(b) : -1;
+        if (lanes) lanes[b] = l;
+        if (wgs_per_cu) wgs_per_cu[b] = w;
+        if (nclusters) nclusters[b] = nb;
+        if (ms) ms[b] = (c->nsys > 0 && c->b_tuned[p] && nb > 0) ? c->b_ms[p][b] : -1.0f;
+    }
     return PMQ_OK;
 }
 
@@ -1017,7 +1111,8 @@ int pmq_set_batch(pmq_ctx *c, int nsys, const int *off)
         return PMQ_ERR_STATE;
     }
     batch_graph_release(c);
-    c->batch_variant = -1;
+    c->b_tuned[0] = c->b_tuned[1] = false;
+    c->b_pg_valid = false;
     if (nsys <= 0) {
         c->nsys = 0;
         return PMQ_OK;
@@ -1049,12 +1144,25 @@ int pmq_set_batch(pmq_ctx *c, int nsys, const int *off)
     std::vector<int> order(nsys);
     for (int k = 0; k < nsys; k++) order[k] = k;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sizes[a] > sizes[b]; });
+    // size buckets over the largest-first order: more than 128 atoms, more
+    // than 64, the rest (each bucket's workgroups hold 256, 128, 64 rows)
+    int beg[PMQ_BATCH_BUCKETS + 1] = {0, 0, 0, nsys};
+    for (int k = 0; k < nsys; k++) {
+        if (sizes[order[k]] > 128) beg[1] = k + 1;
+        if (sizes[order[k]] > 64) beg[2] = k + 1;
+    }
     int rc;
     if ((rc = upload(c, c->b_order, order.data(), nsys))) return rc;
     if ((rc = upload(c, c->b_off, off, nsys + 1))) return rc;
     if ((rc = upload(c, c->b_eoff, eoff.data(), nsys + 1))) return rc;
     PMQ_HIP(c->b_em.alloc(nsys)); PMQ_HIP(c->b_ep.alloc(nsys)); PMQ_HIP(c->b_it.alloc(nsys));
-    PMQ_HIP(c->b_counter.alloc(1));
+    PMQ_HIP(c->b_counter.alloc(PMQ_BATCH_BUCKETS));
+    // streams and events of the side-by-side bucket launches (default priority)
+    for (int k = 0; k < PMQ_BATCH_BUCKETS - 1; k++) {
+        if (!c->b_side[k]) PMQ_HIP(hipStreamCreateWithFlags(&c->b_side[k], hipStreamNonBlocking));
+        if (!c->b_join[k]) PMQ_HIP(hipEventCreateWithFlags(&c->b_join[k], hipEventDisableTiming));
+    }
+    if (!c->b_fork) PMQ_HIP(hipEventCreateWithFlags(&c->b_fork, hipEventDisableTiming));
     // frames run in caller order
     std::vector<int> iota(c->npad);
     for (int s = 0; s < c->npad; s++) iota[s] = s < c->n ? s : -1;
@@ -1063,6 +1171,7 @@ int pmq_set_batch(pmq_ctx *c, int nsys, const int *off)
     PMQ_HIP(c->tep.alloc(3 * (size_t)c->n));
     PMQ_HIP(hipStreamSynchronize(c->stream));
     c->h_boff.assign(off, off + nsys + 1);
+    for (int b = 0; b <= PMQ_BATCH_BUCKETS; b++) c->b_beg[b] = beg[b];
     c->nsys = nsys;
     return PMQ_OK;
 }
@@ -1123,10 +1232,13 @@ int pmq_compute_batch_noalloc(pmq_ctx *c, int mode, unsigned terms, int precisio
     c->last_precision = precision;
     c->pcg_iters = 0;
     c->noalloc_last = true;
-    // the lanes-per-row variant a previous pmq_compute tuned, else the first
-    const int variant = c->batch_variant >= 0 ? c->batch_variant : 0;
+    c->b_pg_valid = false;
+    // the shapes a previous pmq_compute tuned for this precision, else the first of each bucket
+    const int p = precision == PMQ_MIXED ? 1 : 0;
+    static const int first[PMQ_BATCH_BUCKETS] = {0, 0, 0};
     return batch_noalloc_enqueue(c, mode, terms & PMQ_TERM_MPOLE, terms & PMQ_TERM_POLAR, precision == PMQ_MIXED,
-                                 variant, em, ep, pcg_iters, etot, grad);
+                                 c->b_tuned[p] ? c->b_var[p] : first, c->b_tuned[p] && c->b_merged[p], em, ep,
+                                 pcg_iters, etot, grad);
 }
 
 // A speculative evaluation (no host round trips) of one system.  The first
@@ -1243,7 +1355,27 @@ int pmq_param_grad(pmq_ctx *c, double *dpole, double *dpolarity, double *dpdamp,
         c->err = "pmq_param_grad: topology, parameters and coordinates must be set and evaluated first";
         return PMQ_ERR_STATE;
     }
-    if (c->nsys > 0 || c->last_mode != PMQ_GRADIENT || c->last_precision != PMQ_PLAIN_FP64) {
+    if (c->nsys > 0) {
+        // batched clusters: computed inside the evaluation's own launch
+        if (!c->b_pg_valid || c->noalloc_last) {
+            c->err = "pmq_param_grad: a batch needs pmq_set_batch_param_grad(ctx, 1) and then a GRADIENT evaluation";
+            return PMQ_ERR_STATE;
+        }
+        const size_t n = c->n;
+        const bool pol = (c->last_terms & PMQ_TERM_POLAR) && !c->h_polarity.empty();
+        double *outs[4] = {dpole, dpolarity, dpdamp, dthole};
+        const size_t at[4] = {0, 13 * n, 14 * n, 15 * n}, len[4] = {13 * n, n, n, n};
+        for (int k = 0; k < 4; k++) {
+            if (!outs[k]) continue;
+            if (k > 0 && !pol) {
+                for (size_t i = 0; i < n; i++) outs[k][i] = 0.0;
+                continue;
+            }
+            PMQ_HIP(hipMemcpy(outs[k], c->pgo.p + at[k], len[k] * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        return PMQ_OK;
+    }
+    if (c->last_mode != PMQ_GRADIENT || c->last_precision != PMQ_PLAIN_FP64) {
         c->err = "pmq_param_grad: needs a plain-FP64 GRADIENT evaluation of one system first";
         return PMQ_ERR_STATE;
     }

@@ -360,7 +360,18 @@ struct pmq_ctx {
     pmq::DBuf<int> b_order, b_off, b_eoff, b_counter;
     pmq::DBuf<double> b_em, b_ep, b_it;
     int num_cus = 256;
-    int batch_variant = -1;                      // autotuned lanes-per-row variant (-1 = not yet)
+    int b_beg[PMQ_BATCH_BUCKETS + 1] = {0, 0, 0, 0}; // size buckets: slices of b_order (>128, >64, <=64 atoms)
+    int b_var[2][PMQ_BATCH_BUCKETS] = {{0, 0, 0}, {0, 0, 0}}; // tuned shape per [FP64, mixed][bucket]
+    float b_ms[2][PMQ_BATCH_BUCKETS] = {{-1, -1, -1}, {-1, -1, -1}}; // its timed launch
+    bool b_tuned[2] = {false, false};
+    bool b_tuned_pg[2] = {false, false};         // the parameter-gradient setting the tuning ran with
+    bool b_merged[2] = {false, false};           // tuned: all clusters in one 256-row launch
+    hipStream_t b_side[PMQ_BATCH_BUCKETS - 1] = {}; // the other buckets' streams (created by pmq_set_batch)
+    hipEvent_t b_fork = nullptr, b_join[PMQ_BATCH_BUCKETS - 1] = {};
+    bool b_pin[2][PMQ_BATCH_BUCKETS] = {{false, false, false}, {false, false, false}}; // pmq_set_batch_shape
+    bool b_pgrad = false;                        // batched GRADIENT evaluations also give parameter gradients
+    bool b_pg_valid = false;                     // pgo holds the last batched evaluation's
+    pmq::DBuf<double> pgo;                       // 16n: dpole (13n), dpolarity, dpdamp, dthole (caller order)
     // one evaluation of a single system as a HIP graph: captured on the
     // second consecutive speculative evaluation with the same key, replayed
     // while the key holds (pmq_api.cpp eval_key)
@@ -505,15 +516,21 @@ struct BatchScratch {
     int *counter;
     double *em_c, *ep_c, *it_c;
 };
-int launch_batch(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int variant,
-                 const BatchScratch *bs = nullptr);
+int launch_batch(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, const int *variants,
+                 const BatchScratch *bs, bool pgrad, bool merged);
+int batch_tune(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, bool pgrad);
+int batch_variants(int bucket);
+int batch_bucket_rows(int bucket);
+void batch_shape(int bucket, int variant, int *rows, int *lanes, int *wgs_per_cu);
+int launch_param_site(pmq_ctx *c, bool mpole, bool polar, double *dpole, double *dpolarity, double *dpdamp,
+                      double *dthole);
 int static_scratch_attach(pmq_ctx *c, int device);
 size_t static_scratch_bytes();
-int batch_noalloc_enqueue(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, int variant, double *em,
+int batch_noalloc_enqueue(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed, const int *variants, bool merged,
+                          double *em,
                           double *ep, int *iters, double *etot, double *grad);
 int param_gradients(pmq_ctx *c, bool mpole, bool polar, double *dpole, double *dpolarity, double *dpdamp,
                     double *dthole);
-int batch_variants();
 } // namespace pmq
 
 namespace pmq {

@@ -260,6 +260,31 @@ class Engine:
         self._check(self._L.pmq_set_batch(self._ctx, len(off) - 1, off.ctypes.data_as(_ip)), "pmq_set_batch")
         self.nsys = len(off) - 1
 
+    def set_batch_param_grad(self, on: bool = True):
+        """Batched GRADIENT evaluations also compute parameter gradients
+        (read with param_grad())."""
+        self._L.pmq_set_batch_param_grad.argtypes = [C.c_void_p, C.c_int]
+        self._L.pmq_set_batch_param_grad.restype = C.c_int
+        self._check(self._L.pmq_set_batch_param_grad(self._ctx, 1 if on else 0), "pmq_set_batch_param_grad")
+
+    def set_batch_shape(self, precision: int, bucket: int, lanes: int, wgs_per_cu: int = 1):
+        """Fix one size bucket's shape (lanes = 0: back to the autotuner)."""
+        self._L.pmq_set_batch_shape.argtypes = [C.c_void_p] + [C.c_int] * 4
+        self._L.pmq_set_batch_shape.restype = C.c_int
+        self._check(self._L.pmq_set_batch_shape(self._ctx, precision, bucket, lanes, wgs_per_cu), "pmq_set_batch_shape")
+
+    def batch_tuning(self, precision: int) -> list:
+        """The autotuner's shape per cluster-size bucket: dicts of rows, lanes
+        per row, workgroups per CU, cluster count and timed ms (-1: none)."""
+        nb = 3
+        rows, lanes, wpc, ncl = [(C.c_int * nb)() for _ in range(4)]
+        ms = (C.c_float * nb)()
+        self._L.pmq_batch_tuning.argtypes = [C.c_void_p, C.c_int] + [C.POINTER(C.c_int)] * 4 + [C.POINTER(C.c_float)]
+        self._L.pmq_batch_tuning.restype = C.c_int
+        self._check(self._L.pmq_batch_tuning(self._ctx, precision, rows, lanes, wpc, ncl, ms), "pmq_batch_tuning")
+        return [{"rows": rows[b], "lanes_per_row": lanes[b], "wgs_per_cu": wpc[b], "clusters": ncl[b],
+                 "ms": round(float(ms[b]), 4)} for b in range(nb)]
+
     def fetch_batch(self) -> dict:
         ns = self.nsys
         em, ep, it = np.zeros(ns), np.zeros(ns), np.zeros(ns, dtype=np.int32)
