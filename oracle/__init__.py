@@ -185,6 +185,18 @@ class ElecOut(C.Structure):
                 ("nep", C.c_long), ("einter", C.c_double)]
 
 
+def charge_flux_delta(fx, x=None) -> np.ndarray:
+    """The monopole change of every atom from charge flux (alterchg.cpp:36-169)."""
+    L = lib()
+    L.orc_charge_flux_delta.argtypes = [C.POINTER(MpoleSys), _dp]
+    L.orc_charge_flux_delta.restype = None
+    keep = _Keep()
+    mp = mpole_sys(fx, keep, x=x)
+    out = np.zeros(int(fx["n"]))
+    L.orc_charge_flux_delta(C.byref(mp), out.ctypes.data_as(_dp))
+    return out
+
+
 def elec(fx, want_grad: bool = True, solver: int = 0, x=None, poleps: float | None = None,
          polar: bool | None = None) -> dict:
     """Ewald/PME multipoles and AMOEBA polarization on a system dict."""

@@ -714,7 +714,7 @@ void orc_torque(const orc_mpole_sys *s, const double *te, double *de, double *vi
 /* ------------------------------------------------------------------------
  * Charge flux: alterchg.cpp:36-169 and dcflux.cpp:31-225
  * ---------------------------------------------------------------------- */
-static void charge_flux_delta(const orc_mpole_sys *s, double *pdelta)
+void orc_charge_flux_delta(const orc_mpole_sys *s, double *pdelta)
 {
     memset(pdelta, 0, s->n * sizeof(double));
     for (int b = 0; b < s->nbond; b++) {
@@ -744,7 +744,7 @@ static void charge_flux_delta(const orc_mpole_sys *s, double *pdelta)
     }
 }
 
-static void charge_flux_grad(const orc_mpole_sys *s, const double *pot, double *de, double *vir)
+void orc_charge_flux_grad(const orc_mpole_sys *s, const double *pot, double *de, double *vir)
 {
     const int n = s->n;
     double *fc = (double *)calloc(3 * (size_t)n, sizeof(double));
@@ -828,7 +828,7 @@ int orc_empole(orc_mpole_sys *s, int mode, orc_mpole_out *out)
     /* charge flux alters monopoles and valence charges, energy.cpp:241 */
     if (s->use_chgflx) {
         double *pdelta = (double *)malloc((size_t)n * sizeof(double));
-        charge_flux_delta(s, pdelta);
+        orc_charge_flux_delta(s, pdelta);
         for (int i = 0; i < n; i++) {
             pole[13 * i] = s->mono0[i] + pdelta[i];
             pval[i] = s->pval0[i] + pdelta[i];
@@ -910,7 +910,7 @@ int orc_empole(orc_mpole_sys *s, int mode, orc_mpole_out *out)
     }
     if (do_g && dem) {
         orc_torque(s, te, dem, do_v ? vloc : NULL);
-        if (cf) charge_flux_grad(s, pot, dem, do_v ? vloc : NULL);
+        if (cf) orc_charge_flux_grad(s, pot, dem, do_v ? vloc : NULL);
     }
     out->em = em;
     out->einter = einter;

@@ -75,6 +75,12 @@ typedef struct {
 /* empole_a<Mode, PenType, use_cf>: empole.cpp:96-357 */
 int orc_empole(orc_mpole_sys *s, int mode, orc_mpole_out *out);
 
+/* charge flux (alterchg.cpp:36-169, dcflux.cpp:31-225): the monopole change
+ * of every atom from its bonds and angles, and the gradient (+ virial, may
+ * be NULL) of pot[i] = dE/dq_i through those changes, added to de / vir */
+void orc_charge_flux_delta(const orc_mpole_sys *s, double *pdelta);
+void orc_charge_flux_grad(const orc_mpole_sys *s, const double *pot, double *de, double *vir);
+
 /* minimum image of one displacement, image.cpp:37-124 */
 void orc_image(const orc_box *b, double *xr, double *yr, double *zr);
 

@@ -1026,10 +1026,24 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
     c.g11 = offsets(n, s->np11); c.g12 = offsets(n, s->np12);
     c.g13 = offsets(n, s->np13); c.g14 = offsets(n, s->np14);
 
-    /* global multipoles */
+    /* global multipoles; charge flux alters the monopoles (and so the
+     * valence charges: total minus core) before the frames, energy.cpp:241 */
     double *pole = (double *)malloc(13 * (size_t)n * sizeof(double));
     double *rp13 = (double *)malloc(13 * (size_t)n * sizeof(double));
     memcpy(pole, mp->pole, 13 * (size_t)n * sizeof(double));
+    if (mp->use_chgflx) {
+        double *pdelta = (double *)malloc((size_t)n * sizeof(double));
+        orc_charge_flux_delta(mp, pdelta);
+        for (int i = 0; i < n; i++) pole[13 * i] = mp->mono0[i] + pdelta[i];
+        free(pdelta);
+    }
+    /* charge-flux gradient (with charge penetration, as empole.cpp:66-80
+     * selects it): pot_m[i] = dE_m/dq_i, pot_p[i] = dE_p/dq_i, each turned
+     * into forces through the monopole changes */
+    const int cf = c.cp && mp->use_chgflx && want_grad;
+    double *pot_m = cf ? (double *)calloc((size_t)n, sizeof(double)) : NULL;
+    double *pot_p = cf ? (double *)calloc((size_t)n, sizeof(double)) : NULL;
+    static const double e0[10] = {1.0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     orc_chkpole(mp, pole);
     orc_rotpole(mp, pole, rp13);
     c.rp = (double *)malloc(10 * (size_t)n * sizeof(double));
@@ -1083,6 +1097,12 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
                 gpair_eval(R, A[part], B[part], q->r, want_grad, &g);
                 em_real += g.e;
                 epair += g.e;
+                if (cf) {
+                    /* the pair energy is linear in each valence charge */
+                    gpair gq;
+                    if (A[part] == vi) { gpair_eval(R, e0, B[part], q->r, 0, &gq); pot_m[q->i] += gq.e; }
+                    if (B[part] == vk) { gpair_eval(R, A[part], e0, q->r, 0, &gq); pot_m[q->k] += gq.e; }
+                }
                 if (c.ewald && same) {
                     double R0[6];
                     gpair g0;
@@ -1165,6 +1185,8 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
                 site_energy(m, &pm[i], g, tq);
                 for (int a = 0; a < 3; a++) { dem[3 * i + a] += f * g[a]; te[3 * i + a] += f * tq[a]; }
             }
+            /* (f/2) sum M.phi and the self energy in the charge of i */
+            if (cf) pot_m[i] += f * pm[i].phi - 2.0 * f * c.a / SQRTPI * m[0];
         }
     }
     o->em = em_real + em_recip + em_self;
@@ -1175,6 +1197,7 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
     o->em_self = em_self;
     if (want_grad && (o->dem || want_vir)) {
         orc_torque(mp, te, dem, want_vir ? vm : NULL);
+        if (cf) orc_charge_flux_grad(mp, pot_m, dem, want_vir ? vm : NULL);
         if (o->dem) memcpy(o->dem, dem, 3 * (size_t)n * sizeof(double));
     }
 
@@ -1233,9 +1256,11 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
                         damped_series(&c, q, sc, dmpk, 5, R);
                         gpair_eval(R, di_, vk, q->r, 1, &g1);
                         for (int a = 0; a < 3; a++) { frc[a] += g1.frc[a]; tk[a] += g1.trqk[a]; }
+                        if (cf) { gpair gq; gpair_eval(R, di_, e0, q->r, 0, &gq); pot_p[k] += 0.5 * f * gq.e; }
                         damped_series(&c, q, sc, dmpi, 5, R);
                         gpair_eval(R, vi, dk_, q->r, 1, &g2);
                         for (int a = 0; a < 3; a++) { frc[a] += g2.frc[a]; ti[a] += g2.trqi[a]; }
+                        if (cf) { gpair gq; gpair_eval(R, e0, dk_, q->r, 0, &gq); pot_p[i] += 0.5 * f * gq.e; }
                     }
                     pol_series(&c, q, q->w, R);
                     gpair_eval(R, mdi, mpk, q->r, 1, &g1);
@@ -1315,6 +1340,7 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
                     }
                     site_energy(c.rp + 10 * i, &ps, g, tq);
                     for (int a = 0; a < 3; a++) { dep[3 * i + a] += 0.5 * f * g[a]; te[3 * i + a] += 0.5 * f * tq[a]; }
+                    if (cf) pot_p[i] += 0.5 * f * ps.phi;
                     if (want_vir) {
                         /* strain terms of the cross energies: (1/2)[C(mu_d + mu_p in phi_M)
                          * + C(M in phi_(mu_d + mu_p)) + C(mu_d in phi_p) + C(mu_p in phi_d)] */
@@ -1334,6 +1360,7 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
                 free(md); free(mq); free(pd); free(pq);
             }
             orc_torque(mp, te, dep, want_vir ? vp : NULL);
+            if (cf) orc_charge_flux_grad(mp, pot_p, dep, want_vir ? vp : NULL);
             if (o->dep) memcpy(o->dep, dep, n3 * sizeof(double));
         }
         free(Ed); free(Ep);
@@ -1343,7 +1370,7 @@ int orc_elec(orc_elec_sys *s, int want_grad, orc_elec_out *o)
 
     if (want_vir && o->vir_m) memcpy(o->vir_m, vm, sizeof vm);
     if (want_vir && o->vir_p) memcpy(o->vir_p, vp, sizeof vp);
-    free(L); free(te); free(dem); free(dep); free(pole); free(rp13); free(c.rp); free(pm);
+    free(L); free(te); free(dem); free(dep); free(pole); free(rp13); free(c.rp); free(pm); free(pot_m); free(pot_p);
     free(c.o12); free(c.o13); free(c.o14); free(c.o15); free(c.g11); free(c.g12); free(c.g13); free(c.g14);
     if (c.ewald) pme_free(&c);
     return 0;

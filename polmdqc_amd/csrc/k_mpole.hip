@@ -153,8 +153,9 @@ __global__ void k_rotpole(RotArgs a)
     if (a.cf) {
         // alterchg.cpp:50-53: pole[0] = mono0 + delta, pval = pval0 + delta
         // (the caller passes mono0 in pole[0])
-        p[0] += a.pdelta[i];
-        val += a.pdelta[i];
+        const double dq = (double)((const long long *)a.pdelta)[i] * (1.0 / PMQ_FX_CF);
+        p[0] += dq;
+        val += dq;
     }
     const int frame = a.polaxe[i];
     const int iz = a.zaxis[i] - 1, ix = a.xaxis[i] - 1, ky = a.yaxis[i], iy = abs(ky) - 1;
@@ -338,6 +339,7 @@ struct MpoleArgs {
     double off2, f, aewald;
     double *dem, *te, *pot, *aem;
     long long *demq, *teq;   // mixed precision: 64-bit fixed-point gradient and torque (else null)
+    long long *potq;         // mixed precision: fixed-point dE/dq of the scaled pairs (else null)
     const unsigned long long *xmask; // per tile: [64,128) row masks of the in-cutoff pairs with an active atom,
                                      // scaled pairs left out
     void *part;              // per tile partial sums (2 sides x PMQ_TILE_NV x 64 of R)
@@ -584,7 +586,15 @@ __global__ __launch_bounds__(64) void k_mpole_excl(MpoleArgs a)
         grad_add(a.te, a.teq, 3 * q.i + 2, o.tiz);
         grad_add(a.te, a.teq, 3 * q.k + 0, o.tkx); grad_add(a.te, a.teq, 3 * q.k + 1, o.tky);
         grad_add(a.te, a.teq, 3 * q.k + 2, o.tkz);
-        if (CF) { atomicAdd(&a.pot[q.i], (double)o.phii); atomicAdd(&a.pot[q.k], (double)o.phik); }
+        if (CF) {
+            if (a.potq) {
+                fx_add(a.potq, q.i, (double)o.phii, PMQ_FX_GRAD);
+                fx_add(a.potq, q.k, (double)o.phik, PMQ_FX_GRAD);
+            } else {
+                atomicAdd(&a.pot[q.i], (double)o.phii);
+                atomicAdd(&a.pot[q.k], (double)o.phik);
+            }
+        }
         if (VIR) {
             Vir6 v;
             v.pair(xr, yr, zr, o.fx, o.fy, o.fz);
@@ -606,9 +616,9 @@ static int launch_mpole_t(pmq_ctx *c, const MpoleArgs &a)
             if (qi) hipLaunchKernelGGL((k_mpole_tiles<R, PEN, CF_, ANA_, GRAD, VIR, EW_, true>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a); \
             else hipLaunchKernelGGL((k_mpole_tiles<R, PEN, CF_, ANA_, GRAD, VIR, EW_, false>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a); \
         } while (0)
-        if (a.demq) { // mixed precision: ENERGY, GRADIENT and VIRIAL (no analysis or charge flux)
-            if (ew) PMQ_MT(float, false, false, true);
-            else PMQ_MT(float, false, false, false);
+        if (a.demq) { // mixed precision: ENERGY, GRADIENT and VIRIAL (no analysis)
+            if (ew) PMQ_MT(float, CF, false, true);
+            else PMQ_MT(float, CF, false, false);
         } else {
             if (ew) PMQ_MT(double, CF, ANA, true);
             else PMQ_MT(double, CF, ANA, false);
@@ -629,7 +639,7 @@ static int launch_mpole_t(pmq_ctx *c, const MpoleArgs &a)
             r.off[q] = r.off[3 + q] = q;
             r.fxscale[q] = r.fxscale[3 + q] = PMQ_FX_GRAD;
         }
-        if (GRAD && CF && !mixed) r.out[6] = a.pot;
+        if (GRAD && CF) r.out[6] = a.pot;
         if (ANA && !mixed) r.out[7] = a.aem;
         for (int q = 6; q < 8; q++) { r.caller_order[q] = 1; r.stride[q] = 1; r.off[q] = 0; r.fxscale[q] = PMQ_FX_GRAD; }
         if ((rc = tile_reduce(c, r, 8, mixed, mixed))) return rc;
@@ -669,6 +679,7 @@ int launch_mpole_direct(pmq_ctx *c, int mode, double off2, bool mixed)
     a.dem = c->dem.p; a.te = c->te.p; a.pot = c->pot.p; a.aem = c->aem.p;
     a.demq = mixed ? c->fx_grad.p : nullptr;
     a.teq = mixed ? c->fx_grad.p + 3 * (size_t)c->n : nullptr;
+    a.potq = mixed ? c->fx_pot.p : nullptr;
     a.xmask = c->xmask.p;
     a.part = c->tpart.p;
     a.tscal = c->tscal.p;
@@ -891,12 +902,18 @@ struct CfArgs {
     const double *x, *y, *z;
     const int *ibnd, *iang, *balist;
     const double *bl, *bflx, *anat, *aflx, *abflx;
-    double *pdelta;
-    const double *pot;
-    double *de;
+    double *pdelta;              // 64-bit fixed point (PMQ_FX_CF): the sum does not depend on the order
+    const double *pot[2];        // dE/dq of the terms (multipoles, polarization; null: skipped)
+    double *de[2];               // their gradients (caller order)
+    long long *deq[2];           // mixed form: fixed-point gradients instead (else null)
     Totals *tot;
     int vir;
 };
+
+__device__ __forceinline__ void cf_shift(const CfArgs &a, int i, double v)
+{
+    fx_add((long long *)a.pdelta, i, v, PMQ_FX_CF);
+}
 
 __global__ void k_cf_delta(CfArgs a)
 {
@@ -906,8 +923,8 @@ __global__ void k_cf_delta(CfArgs a)
         const int ia = a.ibnd[2 * t], ib = a.ibnd[2 * t + 1];
         const double d[3] = {a.x[ia] - a.x[ib], a.y[ia] - a.y[ib], a.z[ia] - a.z[ib]};
         const double dq = a.bflx[t] * (sqrt(dot3(d, d)) - a.bl[t]);
-        atomicAdd(&a.pdelta[ia], -dq);
-        atomicAdd(&a.pdelta[ib], dq);
+        cf_shift(a, ia, -dq);
+        cf_shift(a, ib, dq);
     } else if (t < a.nbond + a.nangle) {
         const int g = t - a.nbond;
         const int ia = a.iang[3 * g], ib = a.iang[3 * g + 1], ic = a.iang[3 * g + 2];
@@ -920,18 +937,18 @@ __global__ void k_cf_delta(CfArgs a)
             const double dth = (radian * acos(cs) - a.anat[g]) / radian;
             const double dq1 = a.abflx[2 * g] * (rcb - a.bl[a.balist[2 * g + 1]]) + a.aflx[2 * g] * dth;
             const double dq2 = a.abflx[2 * g + 1] * (rab - a.bl[a.balist[2 * g]]) + a.aflx[2 * g + 1] * dth;
-            atomicAdd(&a.pdelta[ia], dq1);
-            atomicAdd(&a.pdelta[ib], -(dq1 + dq2));
-            atomicAdd(&a.pdelta[ic], dq2);
+            cf_shift(a, ia, dq1);
+            cf_shift(a, ib, -(dq1 + dq2));
+            cf_shift(a, ic, dq2);
         }
     }
 }
 
-__device__ __forceinline__ void cf_add(const CfArgs &a, int i, const double *f, double *W)
+__device__ __forceinline__ void cf_add(const CfArgs &a, int set, int i, const double *f, double *W)
 {
-    atomicAdd(&a.de[3 * i], f[0]);
-    atomicAdd(&a.de[3 * i + 1], f[1]);
-    atomicAdd(&a.de[3 * i + 2], f[2]);
+    grad_add(a.de[set], a.deq[set], 3 * i, f[0]);
+    grad_add(a.de[set], a.deq[set], 3 * i + 1, f[1]);
+    grad_add(a.de[set], a.deq[set], 3 * i + 2, f[2]);
     // dcflux.cpp:204-223 builds the virial from absolute coordinates
     const double xi = a.x[i], yi = a.y[i], zi = a.z[i];
     W[0] += xi * f[0]; W[1] += yi * f[0]; W[2] += zi * f[0];
@@ -942,39 +959,44 @@ __global__ __launch_bounds__(128) void k_cf_grad(CfArgs a)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double W[6] = {0, 0, 0, 0, 0, 0};
-    if (t < a.nbond) {
-        const int ia = a.ibnd[2 * t], ib = a.ibnd[2 * t + 1];
-        const double d[3] = {a.x[ia] - a.x[ib], a.y[ia] - a.y[ib], a.z[ia] - a.z[ib]};
-        const double pb = a.bflx[t] * (a.pot[ib] - a.pot[ia]) / sqrt(dot3(d, d));
-        const double fa[3] = {pb * d[0], pb * d[1], pb * d[2]};
-        const double fb[3] = {-fa[0], -fa[1], -fa[2]};
-        cf_add(a, ia, fa, W);
-        cf_add(a, ib, fb, W);
-    } else if (t < a.nbond + a.nangle) {
-        const int g = t - a.nbond;
-        const int ia = a.iang[3 * g], ib = a.iang[3 * g + 1], ic = a.iang[3 * g + 2];
-        const double ab[3] = {a.x[ia] - a.x[ib], a.y[ia] - a.y[ib], a.z[ia] - a.z[ib]};
-        const double cb[3] = {a.x[ic] - a.x[ib], a.y[ic] - a.y[ib], a.z[ic] - a.z[ib]};
-        const double rab2 = dot3(ab, ab), rcb2 = dot3(cb, cb);
-        const double rab = sqrt(rab2), rcb = sqrt(rcb2);
-        if (rab > 0.0 && rcb > 0.0) {
-            const double dpa = a.pot[ia] - a.pot[ib], dpc = a.pot[ic] - a.pot[ib];
-            const double pb1 = dpa * a.abflx[2 * g], pb2 = dpc * a.abflx[2 * g + 1];
-            const double rabc = rab * rcb, dp = dot3(ab, cb);
-            const double dra3 = dp / (rab2 * rab * rcb), drc3 = dp / (rab * rcb2 * rcb);
-            const double fterm = -rabc / sqrt(rab2 * rcb2 - dp * dp) * (dpa * a.aflx[2 * g] + dpc * a.aflx[2 * g + 1]);
-            double fa[3], fc[3], fb[3];
-            for (int j = 0; j < 3; j++) {
-                const double fa1 = pb2 * ab[j] / rab, fc1 = pb1 * cb[j] / rcb;
-                const double fa2 = fterm * (cb[j] / rabc - ab[j] * dra3);
-                const double fc2 = fterm * (ab[j] / rabc - cb[j] * drc3);
-                fa[j] = fa1 + fa2;
-                fc[j] = fc1 + fc2;
-                fb[j] = -fa1 - fc1 - fa2 - fc2;
+    for (int set = 0; set < 2; set++) {
+        const double *pot = a.pot[set];
+        if (!pot) continue;
+        if (t < a.nbond) {
+            const int ia = a.ibnd[2 * t], ib = a.ibnd[2 * t + 1];
+            const double d[3] = {a.x[ia] - a.x[ib], a.y[ia] - a.y[ib], a.z[ia] - a.z[ib]};
+            const double pb = a.bflx[t] * (pot[ib] - pot[ia]) / sqrt(dot3(d, d));
+            const double fa[3] = {pb * d[0], pb * d[1], pb * d[2]};
+            const double fb[3] = {-fa[0], -fa[1], -fa[2]};
+            cf_add(a, set, ia, fa, W);
+            cf_add(a, set, ib, fb, W);
+        } else if (t < a.nbond + a.nangle) {
+            const int g = t - a.nbond;
+            const int ia = a.iang[3 * g], ib = a.iang[3 * g + 1], ic = a.iang[3 * g + 2];
+            const double ab[3] = {a.x[ia] - a.x[ib], a.y[ia] - a.y[ib], a.z[ia] - a.z[ib]};
+            const double cb[3] = {a.x[ic] - a.x[ib], a.y[ic] - a.y[ib], a.z[ic] - a.z[ib]};
+            const double rab2 = dot3(ab, ab), rcb2 = dot3(cb, cb);
+            const double rab = sqrt(rab2), rcb = sqrt(rcb2);
+            if (rab > 0.0 && rcb > 0.0) {
+                const double dpa = pot[ia] - pot[ib], dpc = pot[ic] - pot[ib];
+                const double pb1 = dpa * a.abflx[2 * g], pb2 = dpc * a.abflx[2 * g + 1];
+                const double rabc = rab * rcb, dp = dot3(ab, cb);
+                const double dra3 = dp / (rab2 * rab * rcb), drc3 = dp / (rab * rcb2 * rcb);
+                const double fterm =
+                    -rabc / sqrt(rab2 * rcb2 - dp * dp) * (dpa * a.aflx[2 * g] + dpc * a.aflx[2 * g + 1]);
+                double fa[3], fc[3], fb[3];
+                for (int j = 0; j < 3; j++) {
+                    const double fa1 = pb2 * ab[j] / rab, fc1 = pb1 * cb[j] / rcb;
+                    const double fa2 = fterm * (cb[j] / rabc - ab[j] * dra3);
+                    const double fc2 = fterm * (ab[j] / rabc - cb[j] * drc3);
+                    fa[j] = fa1 + fa2;
+                    fc[j] = fc1 + fc2;
+                    fb[j] = -fa1 - fc1 - fa2 - fc2;
+                }
+                cf_add(a, set, ia, fa, W);
+                cf_add(a, set, ib, fb, W);
+                cf_add(a, set, ic, fc, W);
             }
-            cf_add(a, ia, fa, W);
-            cf_add(a, ib, fb, W);
-            cf_add(a, ic, fc, W);
         }
     }
     if (a.vir) {
@@ -1005,8 +1027,10 @@ static CfArgs cf_args(pmq_ctx *c, int mode)
     a.ibnd = c->ibnd.p; a.iang = c->iang.p; a.balist = c->balist.p;
     a.bl = c->bl.p; a.bflx = c->bflx.p; a.anat = c->anat.p; a.aflx = c->aflx.p; a.abflx = c->abflx.p;
     a.pdelta = c->pdelta.p;
-    a.pot = c->pot.p;
-    a.de = c->dem.p;
+    a.pot[0] = a.pot[1] = nullptr;
+    a.de[0] = c->dem.p;
+    a.de[1] = c->dep.p;
+    a.deq[0] = a.deq[1] = nullptr;
     a.tot = c->totals.p;
     a.vir = mode == PMQ_VIRIAL;
     return a;
@@ -1022,11 +1046,26 @@ int launch_chgflx_delta(pmq_ctx *c)
     return PMQ_OK;
 }
 
-int launch_chgflx_grad(pmq_ctx *c, int mode)
+int launch_chgflx_grad(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed)
 {
     const int nt = c->nbond + c->nangle;
-    if (nt > 0) hipLaunchKernelGGL(k_cf_grad, dim3((nt + 127) / 128), dim3(128), 0, c->stream, cf_args(c, mode));
+    const size_t n3 = 3 * (size_t)c->n;
+    int rc;
+    // the scaled pairs' fixed-point dE/dq first (mixed form)
+    if (mixed && mpole && (rc = launch_fx_flush(c, c->fx_pot.p, c->pot.p, c->n, PMQ_FX_GRAD))) return rc;
+    if (mixed && polar && (rc = launch_fx_flush(c, c->fx_pot.p + c->n, c->potp.p, c->n, PMQ_FX_GRAD))) return rc;
+    CfArgs a = cf_args(c, mode);
+    if (mpole) a.pot[0] = c->pot.p;
+    if (polar) a.pot[1] = c->potp.p;
+    if (mixed) {
+        a.deq[0] = c->fx_grad.p;
+        a.deq[1] = c->fx_grad.p + n3;
+    }
+    if (nt > 0 && (mpole || polar))
+        hipLaunchKernelGGL(k_cf_grad, dim3((nt + 127) / 128), dim3(128), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
+    if (mixed && mpole && (rc = launch_fx_flush(c, c->fx_grad.p, c->dem.p, n3, PMQ_FX_GRAD))) return rc;
+    if (mixed && polar && (rc = launch_fx_flush(c, c->fx_grad.p + n3, c->dep.p, n3, PMQ_FX_GRAD))) return rc;
     return PMQ_OK;
 }
 

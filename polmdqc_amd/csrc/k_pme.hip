@@ -573,6 +573,7 @@ struct MpolePostArgs {
     double f, aewald;
     int grad, polar;
     double *dem, *te;        // caller order
+    double *pot;             // charge flux: += dE/dq (reciprocal potential and self term), caller order, or null
     double *cphi;            // 10 * npad sorted (phi, g, H)
     double *fieldd, *fieldp; // 3 * npad sorted (+= reciprocal + self field)
     Totals *tot;
@@ -658,6 +659,8 @@ __global__ __launch_bounds__(256) void k_pme_mpole_post(MpolePostArgs a)
                                   -(m[3] * gr[0] - m[1] * gr[2]) - 2.0 * (QH[2][0] - QH[0][2]),
                                   -(m[1] * gr[1] - m[2] * gr[0]) - 2.0 * (QH[0][1] - QH[1][0])};
             for (int x = 0; x < 3; x++) a.te[3 * i + x] += a.f * tq[x];
+            // (f/2) M.phi and the self energy in the (total) charge of i
+            if (a.pot) a.pot[i] += a.f * phi - 2.0 * a.f * al / 1.77245385090551602730 * m[0];
         }
     }
     if (a.aem && s < a.n) {
@@ -755,6 +758,7 @@ struct PolGradPostArgs {
     const double *frac;          // n x 2 x 20: potentials of mu_d, mu_p
     double f, self;
     double *dep, *tep;           // caller order
+    double *potp;                // charge flux: += (f/2)(phi_d + phi_p) per atom (caller order), or null
     int vir;                     // the strain terms of the cross energies into tot
     Totals *tot;
 };
@@ -830,6 +834,8 @@ __device__ __forceinline__ void polgrad_site(const PolGradPostArgs &a, int s, Vi
                           m[3] * E[0] - m[1] * E[2] - 2.0 * (QH[2][0] - QH[0][2]),
                           m[1] * E[1] - m[2] * E[0] - 2.0 * (QH[0][1] - QH[1][0])};
     for (int x = 0; x < 3; x++) a.tep[3 * i + x] += hf * tq[x];
+    // the reciprocal cross energy (f/2) M.(phi_d + phi_p) in the charge of i
+    if (a.potp) a.potp[i] += hf * phs;
     if (a.vir) {
         const double gM[3] = {a.cphi[np + s], a.cphi[2 * np + s], a.cphi[3 * np + s]};
         const double gp[3] = {g[0] - gd[0], g[1] - gd[1], g[2] - gd[2]};
@@ -1192,6 +1198,7 @@ int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar)
     args.grad = mpole && (mode == PMQ_GRADIENT || mode == PMQ_VIRIAL);
     args.polar = polar;
     args.dem = c->dem.p; args.te = c->te.p;
+    args.pot = args.grad && c->prm.pentype != PMQ_PEN_NONE && c->prm.use_chgflx ? c->pot.p : nullptr;
     args.cphi = c->cphi.p; args.fieldd = c->pfield.p; args.fieldp = c->pfield.p + 3 * (size_t)c->npad;
     args.tot = mpole ? c->totals.p : nullptr;
     args.vir = mode == PMQ_VIRIAL ? 1 : 0;
@@ -1404,6 +1411,7 @@ int pme_polar_grad(pmq_ctx *c, bool vir)
     args.self = 4.0 / 3.0 * c->prm.aewald * c->prm.aewald * c->prm.aewald / 1.77245385090551602730;
     args.dep = c->dep.p;
     args.tep = c->tep.p;
+    args.potp = c->prm.pentype != PMQ_PEN_NONE && c->prm.use_chgflx ? c->potp.p : nullptr;
     args.vir = vir ? 1 : 0;
     args.tot = c->totals.p;
     {

@@ -64,6 +64,8 @@ struct PolArgs {
     long long *dstq;     // mixed precision: fixed-point fields (else null)
     void *part;          // per tile partial sums (2 sides x PMQ_TILE_NV x 64 of the pair arithmetic type)
     long long *depq, *tepq; // mixed precision: fixed-point gradient and torque (else null)
+    double *potp;        // charge flux (gradient kernels): dE_pol/dq per atom, caller order (else null)
+    long long *potpq;    // mixed precision: its fixed-point accumulator for the scaled pairs (else null)
     int vir;             // gradient kernels: also the pair virial (tile scalars 1..9; totals for scaled pairs)
     double *tscal;
     size_t tstride;
@@ -337,11 +339,14 @@ __device__ __forceinline__ void pol_pair_grad(const T *R, const Mp<T> &mi, const
 // multipole of k through Rt plus its valence multipole through Xk (and the
 // reverse through Xi); the dipole-dipole term uses Rm.  qi/qk: total charges
 // (mi/mk carry the valence charges)
+// With charge flux (PI/PK not null) also the potential of the induced
+// dipoles at each site, the derivative in its valence charge (which enters
+// both the total and the valence multipole), in *PI, *PK.
 template <class T>
 __device__ __forceinline__ void pol_pair_grad_cp(const T *Rt, const T *Xi, const T *Xk, const T *Rm, T qi, T qk,
                                                  const Mp<T> &mi, const Mp<T> &mk, const T *udi, const T *upi,
                                                  const T *udk, const T *upk, T xr, T yr, T zr, T sdp, T sd, T sp,
-                                                 T su, T *F, T *Ti, T *Tk)
+                                                 T su, T *F, T *Ti, T *Tk, T *PI = nullptr, T *PK = nullptr)
 {
     PairOut<T> o;
     Mp<T> ti = mi, tk = mk;
@@ -353,15 +358,19 @@ __device__ __forceinline__ void pol_pair_grad_cp(const T *Rt, const T *Xi, const
         pair_series<T, true>(Rt, ui, tk, xr, yr, zr, o);
         F[0] += w * o.fx; F[1] += w * o.fy; F[2] += w * o.fz;
         Tk[0] += w * o.tkx; Tk[1] += w * o.tky; Tk[2] += w * o.tkz;
+        if (PK) *PK += w * o.phik;
         pair_series<T, true>(Xk, ui, mk, xr, yr, zr, o);
         F[0] += w * o.fx; F[1] += w * o.fy; F[2] += w * o.fz;
         Tk[0] += w * o.tkx; Tk[1] += w * o.tky; Tk[2] += w * o.tkz;
+        if (PK) *PK += w * o.phik;
         pair_series<T, true>(Rt, ti, uk, xr, yr, zr, o);
         F[0] += w * o.fx; F[1] += w * o.fy; F[2] += w * o.fz;
         Ti[0] += w * o.tix; Ti[1] += w * o.tiy; Ti[2] += w * o.tiz;
+        if (PI) *PI += w * o.phii;
         pair_series<T, true>(Xi, mi, uk, xr, yr, zr, o);
         F[0] += w * o.fx; F[1] += w * o.fy; F[2] += w * o.fz;
         Ti[0] += w * o.tix; Ti[1] += w * o.tiy; Ti[2] += w * o.tiz;
+        if (PI) *PI += w * o.phii;
     };
     perm(sdp, udi[0] + upi[0], udi[1] + upi[1], udi[2] + upi[2], udk[0] + upk[0], udk[1] + upk[1], udk[2] + upk[2]);
     perm(sp, udi[0], udi[1], udi[2], udk[0], udk[1], udk[2]); // mu_d with the p-scaled permanent field
@@ -374,14 +383,15 @@ __device__ __forceinline__ void pol_pair_grad_cp(const T *Rt, const T *Xi, const
     }
 }
 
-template <bool EW, class T, bool CPD = false>
+template <bool EW, class T, bool CPD = false, bool CF = false>
 __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
 {
     TILE_PROLOGUE
     // side 0: block I, side 1: block J; compacted pair schedule (tile_pairs.h)
+    constexpr int NA = CF ? 7 : 6; // gradient, torque (and dE/dq with charge flux)
     __shared__ double bx[2][64], by[2][64], bz[2][64];
     __shared__ T bm[2][10][64], bu[2][6][64], bpd[2][64], bth[2][64];
-    __shared__ double acc[2][6][64];
+    __shared__ double acc[2][NA][64];
     __shared__ unsigned short plist[64 * 64];
     const double hf = 0.5 * a.f;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -397,7 +407,7 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
             // Thole: thole and pdamp; HIPPO: total charge and valence exponent
             bth[side][lane] = CPD ? a.scp[2 * np + s] : a.spol[np + s];
             bpd[side][lane] = CPD ? a.scp[np + s] : a.spol[2 * np + s];
-            for (int c = 0; c < 6; c++) acc[side][c][lane] = 0.0;
+            for (int c = 0; c < NA; c++) acc[side][c][lane] = 0.0;
         }
         const int npair = tile_pair_list(a.mask[(size_t)t * 128 + lane], plist, lane);
         __syncthreads();
@@ -418,12 +428,12 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
             mk.qyz = bm[1][8][jl]; mk.qzz = bm[1][9][jl];
             const T ui[6] = {bu[0][0][il], bu[0][1][il], bu[0][2][il], bu[0][3][il], bu[0][4][il], bu[0][5][il]};
             const T udk[3] = {bu[1][0][jl], bu[1][1][jl], bu[1][2][jl]}, upk[3] = {bu[1][3][jl], bu[1][4][jl], bu[1][5][jl]};
-            T f3[3] = {0, 0, 0}, ti[3] = {0, 0, 0}, tk[3] = {0, 0, 0};
+            T f3[3] = {0, 0, 0}, ti[3] = {0, 0, 0}, tk[3] = {0, 0, 0}, pi = T(0), pk = T(0);
             if (CPD) {
                 T Rt[6], Xi[6], Xk[6], Rm[6];
                 hippo_pol_series<T>(r2, bpd[0][il], bpd[1][jl], EW, T(a.aewald), T(1), Rt, Xi, Xk, Rm);
                 pol_pair_grad_cp<T>(Rt, Xi, Xk, Rm, bth[0][il], bth[1][jl], mi, mk, ui, ui + 3, udk, upk, xr, yr, zr,
-                                    T(1), T(0), T(0), T(1), f3, ti, tk);
+                                    T(1), T(0), T(0), T(1), f3, ti, tk, CF ? &pi : nullptr, CF ? &pk : nullptr);
             } else {
                 T lam[6], R[6];
                 thole_lambda<T>(sqrt(r2), bpd[0][il], bpd[1][jl], bth[0][il], bth[1][jl], lam);
@@ -436,13 +446,17 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
                 atomicAdd(&acc[1][c][jl], -(double)f3[c]);
                 atomicAdd(&acc[1][3 + c][jl], (double)tk[c]);
             }
+            if (CF) {
+                atomicAdd(&acc[0][NA - 1][il], (double)pi);
+                atomicAdd(&acc[1][NA - 1][jl], (double)pk);
+            }
             if (a.vir) vir.pair(xd, yd, zd, hf * (double)f3[0], hf * (double)f3[1], hf * (double)f3[2]);
         }
         __syncthreads();
         // partial sums: side 0 the i atoms, side 1 the j atoms
         T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
         T *Q = P + PMQ_TILE_NV * 64;
-        for (int c = 0; c < 6; c++) {
+        for (int c = 0; c < NA; c++) {
             P[c * 64 + lane] = T(hf * acc[0][c][lane]);
             Q[c * 64 + lane] = T(hf * acc[1][c][lane]);
         }
@@ -470,7 +484,8 @@ __global__ __launch_bounds__(64) void k_pol_grad_excl(PolArgs a)
         udi[c] = a.src[c * np + si]; upi[c] = a.src[(3 + c) * np + si];
         udk[c] = a.src[c * np + sk]; upk[c] = a.src[(3 + c) * np + sk];
     }
-    double F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0}, Tk[3] = {0, 0, 0};
+    double F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0}, Tk[3] = {0, 0, 0}, pi = 0.0, pk = 0.0;
+    double *PI = a.potp ? &pi : nullptr, *PK = a.potp ? &pk : nullptr;
     // the whole pair (the tiles leave the scaled pairs out): the series at
     // scale s is linear in s, B - rr + s dmp rr, so the damped part takes the
     // scales as weights and the Ewald part B - rr weight 1
@@ -480,7 +495,7 @@ __global__ __launch_bounds__(64) void k_pol_grad_excl(PolArgs a)
         double Rt[6], Xi[6], Xk[6], Rm[6];
         hippo_pol_series<double>(r2, a.scp[np + si], a.scp[np + sk], false, 0.0, 1.0, Rt, Xi, Xk, Rm);
         pol_pair_grad_cp<double>(Rt, Xi, Xk, Rm, a.scp[2 * np + si], a.scp[2 * np + sk], mi, mk, udi, upi, udk, upk,
-                                 xr, yr, zr, 0.0, q.d, q.p, q.u, F, Ti, Tk);
+                                 xr, yr, zr, 0.0, q.d, q.p, q.u, F, Ti, Tk, PI, PK);
     } else {
         double lam[6], R[6];
         thole_lambda<double>(sqrt(r2), a.spol[2 * np + si], a.spol[2 * np + sk], a.spol[np + si], a.spol[np + sk],
@@ -495,7 +510,7 @@ __global__ __launch_bounds__(64) void k_pol_grad_excl(PolArgs a)
         const double Z[6] = {0, 0, 0, 0, 0, 0};
         if (a.cpd)
             pol_pair_grad_cp<double>(R0, Z, Z, R0, a.scp[2 * np + si], a.scp[2 * np + sk], mi, mk, udi, upi, udk, upk,
-                                     xr, yr, zr, 0.0, 1.0, 1.0, 1.0, F, Ti, Tk);
+                                     xr, yr, zr, 0.0, 1.0, 1.0, 1.0, F, Ti, Tk, PI, PK);
         else
             pol_pair_grad<double>(R0, mi, mk, udi, upi, udk, upk, xr, yr, zr, 0.0, 1.0, 1.0, 1.0, F, Ti, Tk);
     }
@@ -505,6 +520,15 @@ __global__ __launch_bounds__(64) void k_pol_grad_excl(PolArgs a)
         grad_add(a.dep, a.depq, 3 * q.k + c, -hf * F[c]);
         grad_add(a.tep, a.tepq, 3 * q.i + c, hf * Ti[c]);
         grad_add(a.tep, a.tepq, 3 * q.k + c, hf * Tk[c]);
+    }
+    if (a.potp) {
+        if (a.potpq) {
+            fx_add(a.potpq, q.i, hf * pi, PMQ_FX_GRAD);
+            fx_add(a.potpq, q.k, hf * pk, PMQ_FX_GRAD);
+        } else {
+            atomicAdd(&a.potp[q.i], hf * pi);
+            atomicAdd(&a.potp[q.k], hf * pk);
+        }
     }
     if (a.vir) {
         Vir6 v;
@@ -659,6 +683,7 @@ static PolArgs pol_args(pmq_ctx *c, double off2)
     a.mask = c->xmask.p; // the scaled pairs are done whole by the exclusion kernels
     a.src = nullptr; a.dst = nullptr; a.dep = c->dep.p; a.tep = c->tep.p;
     a.dstq = nullptr; a.depq = nullptr; a.tepq = nullptr;
+    a.potp = nullptr; a.potpq = nullptr;
     a.part = c->tpart.p;
     a.vir = 0;
     a.tscal = c->tscal.p;
@@ -809,22 +834,31 @@ int polar_gradient(pmq_ctx *c, double off2, bool mixed, bool vir)
         a.depq = c->fx_grad.p;
         a.tepq = c->fx_grad.p + 3 * (size_t)c->n;
     }
+    // charge flux (with the charge-penetration damping): the induced dipoles'
+    // potential at every site as well
+    const bool cf = a.cpd && c->prm.use_chgflx;
+    if (cf) {
+        a.potp = c->potp.p;
+        a.potpq = mixed ? c->fx_pot.p + c->n : nullptr;
+    }
     {
         ProfScope ps_(c, "pol_grad_tiles");
         const dim3 g(tile_grid(c));
 #define PMQ_GRAD(EW, T, CPD) hipLaunchKernelGGL((k_pol_grad_tiles<EW, T, CPD>), g, dim3(64), 0, c->stream, a)
+#define PMQ_GRADCF(EW, T) hipLaunchKernelGGL((k_pol_grad_tiles<EW, T, true, true>), g, dim3(64), 0, c->stream, a)
         const int sel = (mixed ? 4 : 0) + (a.aewald > 0 ? 2 : 0) + (a.cpd ? 1 : 0);
         switch (sel) {
         case 0: PMQ_GRAD(false, double, false); break;
-        case 1: PMQ_GRAD(false, double, true); break;
+        case 1: if (cf) PMQ_GRADCF(false, double); else PMQ_GRAD(false, double, true); break;
         case 2: PMQ_GRAD(true, double, false); break;
-        case 3: PMQ_GRAD(true, double, true); break;
+        case 3: if (cf) PMQ_GRADCF(true, double); else PMQ_GRAD(true, double, true); break;
         case 4: PMQ_GRAD(false, float, false); break;
-        case 5: PMQ_GRAD(false, float, true); break;
+        case 5: if (cf) PMQ_GRADCF(false, float); else PMQ_GRAD(false, float, true); break;
         case 6: PMQ_GRAD(true, float, false); break;
-        default: PMQ_GRAD(true, float, true); break;
+        default: if (cf) PMQ_GRADCF(true, float); else PMQ_GRAD(true, float, true); break;
         }
 #undef PMQ_GRAD
+#undef PMQ_GRADCF
     }
     PMQ_HIP(hipGetLastError());
     TileReduceArgs r{};
@@ -835,8 +869,12 @@ int polar_gradient(pmq_ctx *c, double off2, bool mixed, bool vir)
         r.off[q] = r.off[3 + q] = q;
         r.fxscale[q] = r.fxscale[3 + q] = PMQ_FX_GRAD;
     }
+    if (cf) {
+        r.out[6] = c->potp.p;
+        r.caller_order[6] = 1; r.stride[6] = 1; r.off[6] = 0; r.fxscale[6] = PMQ_FX_GRAD;
+    }
     int rc;
-    if ((rc = tile_reduce(c, r, 6, mixed, mixed))) return rc;
+    if ((rc = tile_reduce(c, r, cf ? 8 : 6, mixed, mixed))) return rc;
     if (vir && (rc = tile_virial(c))) return rc;
     if (a.nexcl) hipLaunchKernelGGL(k_pol_grad_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());

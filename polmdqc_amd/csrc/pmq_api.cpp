@@ -418,6 +418,8 @@ int pmq_set_topology(pmq_ctx *c, const pmq_topology *t)
     const long long npairs = (long long)nblk * (nblk + 1) / 2;
     PMQ_HIP(c->x.alloc(n)); PMQ_HIP(c->y.alloc(n)); PMQ_HIP(c->z.alloc(n));
     PMQ_HIP(c->pdelta.alloc(n));
+    PMQ_HIP(c->potp.alloc(n));
+    PMQ_HIP(c->fx_pot.alloc(2 * (size_t)n));
     PMQ_HIP(c->rpole13.alloc(13 * (size_t)n));
     PMQ_HIP(c->perm.alloc(npad)); PMQ_HIP(c->iperm.alloc(n));
     PMQ_HIP(c->sx.alloc(npad)); PMQ_HIP(c->sy.alloc(npad)); PMQ_HIP(c->sz.alloc(npad));
@@ -668,10 +670,6 @@ int pmq_set_params(pmq_ctx *c, const pmq_params *p)
         PMQ_HIP(c->dedsp.alloc(3 * (size_t)n));
         PMQ_HIP(c->ter.alloc(3 * (size_t)n));
     }
-    if (p->use_ewald && p->use_chgflx) {
-        c->err = "pmq_set_params: charge flux with Ewald summation is not built";
-        return PMQ_ERR_UNSUPPORTED;
-    }
     c->prm = *p;
     c->prm.pole = nullptr;
     c->prm.pcore = c->prm.pval0 = c->prm.palpha = nullptr;
@@ -740,8 +738,8 @@ static int check_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
     const bool mixed = precision == PMQ_MIXED;
     // charge flux moves only the monopoles, which the multipole and
     // polarization terms see
-    if (mixed && (mode == PMQ_ANALYSIS || (c->prm.use_chgflx && (terms & (PMQ_TERM_MPOLE | PMQ_TERM_POLAR))))) {
-        c->err = "pmq_compute: the mixed form computes energy, gradient and virial without charge flux";
+    if (mixed && mode == PMQ_ANALYSIS) {
+        c->err = "pmq_compute: the mixed form computes energy, gradient and virial";
         return PMQ_ERR_UNSUPPORTED;
     }
     if (terms & ~(unsigned)(PMQ_TERM_MPOLE | PMQ_TERM_POLAR | PMQ_TERM_REPEL | PMQ_TERM_DISP | PMQ_TERM_VDW |
@@ -824,6 +822,10 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
             if (do_polar) {
                 add(c->dep.p, 3 * (size_t)n);
                 add(c->tep.p, 3 * (size_t)n);
+            }
+            if (c->prm.use_chgflx) {
+                add(c->potp.p, n);
+                if (mixed) add((double *)c->fx_pot.p, 2 * (size_t)n); // all-zero bits: 0
             }
             if (do_rep) {
                 add(c->der.p, 3 * (size_t)n);
@@ -909,7 +911,10 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
         } else if (grad && do_mpole) {
             if ((rc = launch_torque(c, mode, c->te.p, c->dem.p, mixed))) return rc;
         }
-        if (grad && do_mpole && cf && (rc = launch_chgflx_grad(c, mode))) return rc;
+        // charge flux: the chain term of both terms' gradients through the
+        // monopole shifts (dcflux.cpp:31-225), from the dE/dq of each
+        if (grad && cf && (do_mpole || do_polar) && (rc = launch_chgflx_grad(c, mode, do_mpole, do_polar, mixed)))
+            return rc;
     }
     c->last_mode = mode;
     c->last_terms = terms;

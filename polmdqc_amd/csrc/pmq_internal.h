@@ -128,6 +128,7 @@ struct DBuf {
 // (range 2^31), 2^40 per field unit (range 2^23).
 constexpr double PMQ_FX_GRAD = 4294967296.0;
 constexpr double PMQ_FX_FIELD = 1099511627776.0;
+constexpr double PMQ_FX_CF = 1125899906842624.0; // 2^50 per charge: the charge-flux monopole shifts
 
 __device__ __forceinline__ void fx_add(long long *q, size_t i, double v, double scale)
 {
@@ -272,7 +273,9 @@ struct pmq_ctx {
     // ---- device: per eval
     pmq::DBuf<double> x, y, z;
     pmq::DevBox box{};
-    pmq::DBuf<double> pdelta;        // charge-flux monopole shift
+    pmq::DBuf<double> pdelta;        // charge-flux monopole shift, as 64-bit fixed point (PMQ_FX_CF)
+    pmq::DBuf<double> potp;          // charge flux: dE_pol/dq per atom (caller order)
+    pmq::DBuf<long long> fx_pot;     // mixed form: 2n fixed-point dE/dq (multipoles, polarization) of the scaled pairs
     pmq::DBuf<double> rpole13;       // 13n global-frame multipoles, caller order
     // sorted SoA
     pmq::DBuf<int> perm, iperm;      // sorted -> original, original -> sorted
@@ -491,7 +494,9 @@ int polar_prepare(pmq_ctx *c, double off2, bool mixed, bool safe);
 int polar_solve(pmq_ctx *c, bool safe, int *iters_out);
 int polar_gradient(pmq_ctx *c, double off2, bool mixed, bool vir);
 int polar_finish(pmq_ctx *c, double *ud_out, double *up_out, double *aep = nullptr);
-int launch_chgflx_grad(pmq_ctx *c, int mode);
+// the charge-flux chain term of the multipole (pot -> dem) and polarization
+// (potp -> dep) gradients; mixed: fixed point, flushed
+int launch_chgflx_grad(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed);
 int launch_zero(pmq_ctx *c, double *p, size_t n);
 int tile_structure(pmq_ctx *c, double off2, bool safe);
 int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe);

@@ -155,8 +155,6 @@ def test_charge_transfer(engine, case, tmp_path):
     print(f"[gpu] charge transfer {case}", flush=True)
     s = system("hippo/water21.xyz") if case == "dimer" else hippo_box(tmp_path)
     s["use_ctrn"] = 1
-    if case == "box":
-        s["use_chgflx"] = 0  # charge flux with Ewald is not built (and charge transfer does not see it)
     o = oracle.vdwct(s)
     assert abs(o["ect"]) > 1e-3
     engine.set_system(s)
