@@ -616,9 +616,9 @@ static int launch_mpole_t(pmq_ctx *c, const MpoleArgs &a)
             if (qi) hipLaunchKernelGGL((k_mpole_tiles<R, PEN, CF_, ANA_, GRAD, VIR, EW_, true>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a); \
             else hipLaunchKernelGGL((k_mpole_tiles<R, PEN, CF_, ANA_, GRAD, VIR, EW_, false>), dim3(grid), dim3(PMQ_BLOCK), 0, c->stream, a); \
         } while (0)
-        if (a.demq) { // mixed precision: ENERGY, GRADIENT and VIRIAL (no analysis)
-            if (ew) PMQ_MT(float, CF, false, true);
-            else PMQ_MT(float, CF, false, false);
+        if (a.demq) { // mixed precision: FP32 pair terms, fixed-point sums
+            if (ew) PMQ_MT(float, CF, ANA, true);
+            else PMQ_MT(float, CF, ANA, false);
         } else {
             if (ew) PMQ_MT(double, CF, ANA, true);
             else PMQ_MT(double, CF, ANA, false);
@@ -640,7 +640,7 @@ static int launch_mpole_t(pmq_ctx *c, const MpoleArgs &a)
             r.fxscale[q] = r.fxscale[3 + q] = PMQ_FX_GRAD;
         }
         if (GRAD && CF) r.out[6] = a.pot;
-        if (ANA && !mixed) r.out[7] = a.aem;
+        if (ANA) r.out[7] = a.aem;
         for (int q = 6; q < 8; q++) { r.caller_order[q] = 1; r.stride[q] = 1; r.off[q] = 0; r.fxscale[q] = PMQ_FX_GRAD; }
         if ((rc = tile_reduce(c, r, 8, mixed, mixed))) return rc;
     }

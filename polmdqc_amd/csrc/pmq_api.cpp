@@ -738,10 +738,6 @@ static int check_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
     const bool mixed = precision == PMQ_MIXED;
     // charge flux moves only the monopoles, which the multipole and
     // polarization terms see
-    if (mixed && mode == PMQ_ANALYSIS) {
-        c->err = "pmq_compute: the mixed form computes energy, gradient and virial";
-        return PMQ_ERR_UNSUPPORTED;
-    }
     if (terms & ~(unsigned)(PMQ_TERM_MPOLE | PMQ_TERM_POLAR | PMQ_TERM_REPEL | PMQ_TERM_DISP | PMQ_TERM_VDW |
                             PMQ_TERM_CTRN)) {
         c->err = "pmq_compute: unknown term";

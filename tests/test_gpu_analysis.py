@@ -1,7 +1,8 @@
 """Analysis (nem, aem, einter; nep, aep) on the Ewald and polarization
 paths on the GPU against the oracle (tests/test_oracle_analysis.py pins the
 oracle's definitions).  Bars: nem and nep exact; energies, einter and the
-per-atom partitions to 1e-9 relative (FP64).
+per-atom partitions to 1e-9 relative (FP64), to 1e-5 of |em| in the mixed
+form.
 
 Cases: the 648-atom AMOEBA water box (PME 24^3); the alanine tetrapeptide
 in a periodic box (intramolecular pairs beyond 1-4 are unscaled, so einter
@@ -68,6 +69,11 @@ def test_ewald_multipole_analysis(engine, name):
     assert abs(g["aem"].sum() - g["em"]) <= 1e-10 * abs(g["em"])
     if name == "alatet_box":
         assert abs(g["em"] - g["einter"]) > 1.0  # the intramolecular part is not empty
+    # the mixed form: the same counts, partitions and einter to 1e-5 of |em|
+    m = engine.eval(L.ANALYSIS, L.TERM_MPOLE, L.MIXED)
+    assert m["nem"] == o["nem"]
+    assert np.max(np.abs(m["aem"] - o["aem"])) <= 1e-5 * abs(o["em"])
+    assert abs(m["einter"] - o["einter"]) <= 1e-5 * abs(o["em"])
 
 
 @pytest.mark.parametrize("periodic", [True, False])

@@ -69,10 +69,20 @@ def test_mixed_gradient_is_bit_reproducible(engine):
     assert np.array_equal(a["uind"], b["uind"])
 
 
-def test_mixed_rejects_unsupported_modes(engine):
+def test_mixed_analysis_equals_fp64_analysis(engine):
+    """ANALYSIS in the mixed form: the same interaction counts as FP64 (an
+    FP32 pair energy is zero only where the FP64 one is), the per-atom
+    partitions and einter to 1e-5 of |em| and |ep|, and the partition adds up
+    to em to 1e-7 relative."""
     from polmdqc_amd import _lib as L
-    from polmdqc_amd.engine import PmqError
 
     engine.set_system(box216())
-    with pytest.raises(PmqError):
-        engine.compute(L.ANALYSIS, L.TERM_MPOLE, L.MIXED)
+    terms = L.TERM_MPOLE | L.TERM_POLAR
+    d = engine.eval(L.ANALYSIS, terms, L.PLAIN_FP64)
+    m = engine.eval(L.ANALYSIS, terms, L.MIXED)
+    assert m["nem"] == d["nem"] and m["nep"] == d["nep"]
+    assert np.max(np.abs(m["aem"] - d["aem"])) <= 1e-5 * abs(d["em"])
+    assert np.max(np.abs(m["aep"] - d["aep"])) <= 1e-5 * abs(d["ep"])
+    assert abs(m["einter"] - d["einter"]) <= 1e-5 * abs(d["em"])
+    # the partition sums FP32 pair halves in fixed point, em the FP32 pair energies in FP64
+    assert abs(m["aem"].sum() - m["em"]) <= 1e-7 * abs(m["em"])
