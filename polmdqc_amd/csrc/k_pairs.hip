@@ -69,6 +69,24 @@ struct MaskArgs {
 
 // BK 0: no box or a rectangular one (one image per tile where it fits, else
 // image()'s per-axis wrapping); BK 1: other cells (image() per pair)
+// Transpose of the 64 x 64 bit matrix whose row l is lane l's word (bit c =
+// column c): lane c receives column c.  Six butterfly stages; stage j swaps,
+// between lanes l and l ^ j, the columns whose index has bit j set in the
+// upper lane with those whose index has it clear in the lower (each stage
+// exchanges one bit of the row index with the same bit of the column index).
+__device__ __forceinline__ unsigned long long bit_transpose64(unsigned long long x, int lane)
+{
+    const unsigned long long M[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                                     0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const int j = 32 >> k;
+        const unsigned long long o = __shfl_xor(x, j, 64);
+        x = (lane & j) ? ((x & ~M[k]) | ((o & ~M[k]) >> j)) : ((x & M[k]) | ((o & M[k]) << j));
+    }
+    return x;
+}
+
 template <int BK>
 __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
 {
@@ -149,22 +167,10 @@ __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
             }
         }
         // column mask of lane c: bit l set where lane l's row mask has bit c
-        unsigned long long col = 0ull, ncol = 0ull;
+        const unsigned long long col = bit_transpose64(m, lane);
         if (a.nmask) {
-            for (int c = 0; c < 64; c++) {
-                const unsigned long long b = __ballot((m >> c) & 1ull), nb = __ballot((nm >> c) & 1ull);
-                if (lane == c) {
-                    col = b;
-                    ncol = nb;
-                }
-            }
             a.nmask[(size_t)t * 128 + lane] = nm;
-            a.nmask[(size_t)t * 128 + 64 + lane] = ncol;
-        } else {
-            for (int c = 0; c < 64; c++) {
-                const unsigned long long b = __ballot((m >> c) & 1ull);
-                if (lane == c) col = b;
-            }
+            a.nmask[(size_t)t * 128 + 64 + lane] = bit_transpose64(nm, lane);
         }
         a.mask[(size_t)t * 128 + lane] = m;
         a.mask[(size_t)t * 128 + 64 + lane] = col;
@@ -210,58 +216,83 @@ __device__ __forceinline__ unsigned long long side_mask(const unsigned long long
 
 // Row layout: with NM a row's entries within the preconditioner cutoff come
 // first (nmask; their count in rownear), then the others.  soff and nsoff
-// are each tile's offsets within the two parts.  Tiles are taken U at a time
-// with every load issued unconditionally (indices clamped to the last tile
-// of the list), so the loads of a group are in flight together.
+// are each tile's offsets within the two parts.  A block's tile list is
+// split over RC_W waves: each counts its share first, the counts of the
+// waves before it give its starting offsets, and then it writes its tiles'
+// offsets.  Tiles are taken U at a time with every load issued
+// unconditionally (indices clamped to the last tile of the share), so the
+// loads of a group are in flight together.
+constexpr int RC_W = 4;
+
 template <bool NM>
-__global__ __launch_bounds__(64) void k_row_counts(int nblk, const int2 *__restrict__ tiles,
-                                                   const unsigned long long *__restrict__ mask,
-                                                   const unsigned long long *__restrict__ nmask,
-                                                   const int *__restrict__ adj, const int *__restrict__ adjn,
-                                                   int *__restrict__ soff, int *__restrict__ nsoff,
-                                                   int *__restrict__ rowcnt, int *__restrict__ rownear)
+__global__ __launch_bounds__(64 * RC_W) void k_row_counts(int nblk, const int2 *__restrict__ tiles,
+                                                          const unsigned long long *__restrict__ mask,
+                                                          const unsigned long long *__restrict__ nmask,
+                                                          const int *__restrict__ adj, const int *__restrict__ adjn,
+                                                          int *__restrict__ soff, int *__restrict__ nsoff,
+                                                          int *__restrict__ rowcnt, int *__restrict__ rownear)
 {
     constexpr int U = 8;
-    const int I = blockIdx.x, lane = threadIdx.x;
+    __shared__ int part[2][RC_W][64];
+    const int I = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int na = adjn[I];
     const int *lst = adj + (size_t)I * nblk;
+    const int per = (na + RC_W - 1) / RC_W;
+    const int b0 = min(na, w * per), b1 = min(na, b0 + per);
     int run = 0, nrun = 0;
-    for (int s0 = 0; s0 < na; s0 += U) {
-        int tt[U];
-        int2 ij[U];
-        unsigned long long m[U], nm[U];
-#pragma unroll
-        for (int k = 0; k < U; k++) tt[k] = lst[min(s0 + k, na - 1)];
-#pragma unroll
-        for (int k = 0; k < U; k++) ij[k] = tiles[tt[k]];
-#pragma unroll
-        for (int k = 0; k < U; k++) {
-            m[k] = side_mask(mask, tt[k], ij[k], I, lane);
-            nm[k] = NM ? side_mask(nmask, tt[k], ij[k], I, lane) : 0ull;
-        }
-#pragma unroll
-        for (int k = 0; k < U; k++) {
-            if (s0 + k >= na) break;
-            const size_t t = (size_t)tt[k];
-            const int2 IJ = ij[k];
-            if (IJ.x == IJ.y) {
-                soff[t * 128 + lane] = run;
-                soff[t * 128 + 64 + lane] = run;
-                if (NM) {
-                    nsoff[t * 128 + lane] = nrun;
-                    nsoff[t * 128 + 64 + lane] = nrun;
-                }
-            } else {
-                soff[t * 128 + (IJ.x == I ? 0 : 64) + lane] = run;
-                if (NM) nsoff[t * 128 + (IJ.x == I ? 0 : 64) + lane] = nrun;
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) {
+            // the offsets where this wave's share starts: the counts of the waves before it
+            part[0][w][lane] = run;
+            part[1][w][lane] = nrun;
+            __syncthreads();
+            run = nrun = 0;
+            for (int q = 0; q < w; q++) {
+                run += part[0][q][lane];
+                nrun += part[1][q][lane];
             }
-            run += __popcll(m[k] & ~nm[k]);
-            nrun += __popcll(nm[k]);
+        }
+        for (int s0 = b0; s0 < b1; s0 += U) {
+            int tt[U];
+            int2 ij[U];
+            unsigned long long m[U], nm[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) tt[k] = lst[min(s0 + k, b1 - 1)];
+#pragma unroll
+            for (int k = 0; k < U; k++) ij[k] = tiles[tt[k]];
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                m[k] = side_mask(mask, tt[k], ij[k], I, lane);
+                nm[k] = NM ? side_mask(nmask, tt[k], ij[k], I, lane) : 0ull;
+            }
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                if (s0 + k >= b1) break;
+                if (pass == 1) {
+                    const size_t t = (size_t)tt[k];
+                    const int2 IJ = ij[k];
+                    if (IJ.x == IJ.y) {
+                        soff[t * 128 + lane] = run;
+                        soff[t * 128 + 64 + lane] = run;
+                        if (NM) {
+                            nsoff[t * 128 + lane] = nrun;
+                            nsoff[t * 128 + 64 + lane] = nrun;
+                        }
+                    } else {
+                        soff[t * 128 + (IJ.x == I ? 0 : 64) + lane] = run;
+                        if (NM) nsoff[t * 128 + (IJ.x == I ? 0 : 64) + lane] = nrun;
+                    }
+                }
+                run += __popcll(m[k] & ~nm[k]);
+                nrun += __popcll(nm[k]);
+            }
         }
     }
-    rowcnt[I * 64 + lane] = run + nrun;
-    if (NM) rownear[I * 64 + lane] = nrun;
-    if (I == 0 && lane == 0) rowcnt[nblk * 64] = 0; // the scan's last element becomes the entry total
+    if (w == RC_W - 1) {
+        rowcnt[I * 64 + lane] = run + nrun;
+        if (NM) rownear[I * 64 + lane] = nrun;
+    }
+    if (I == 0 && threadIdx.x == 0) rowcnt[nblk * 64] = 0; // the scan's last element becomes the entry total
 }
 
 // V: stored type of the tensor (double, or float in the mixed form); T: the
@@ -640,10 +671,10 @@ int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe)
     PMQ_HIP(c->rowptr.alloc(np + 1));
     PMQ_HIP(c->rownear.alloc(np));
     if (c->prec_on)
-        hipLaunchKernelGGL(k_row_counts<true>, dim3(nblk), dim3(64), 0, st, nblk, c->tiles.p, c->tmask.p, c->nmask.p,
+        hipLaunchKernelGGL(k_row_counts<true>, dim3(nblk), dim3(64 * RC_W), 0, st, nblk, c->tiles.p, c->tmask.p, c->nmask.p,
                            c->adj.p, c->adjn.p, c->soff.p, c->nsoff.p, c->rowcnt.p, c->rownear.p);
     else
-        hipLaunchKernelGGL(k_row_counts<false>, dim3(nblk), dim3(64), 0, st, nblk, c->tiles.p, c->tmask.p, nullptr,
+        hipLaunchKernelGGL(k_row_counts<false>, dim3(nblk), dim3(64 * RC_W), 0, st, nblk, c->tiles.p, c->tmask.p, nullptr,
                            c->adj.p, c->adjn.p, c->soff.p, nullptr, c->rowcnt.p, nullptr);
     PMQ_HIP(hipGetLastError());
     size_t bytes = 0;
