@@ -133,7 +133,7 @@ __global__ __launch_bounds__(64) void k_disp_tiles(HippoArgs a)
     __shared__ double bx[2][64], by[2][64], bz[2][64];
     __shared__ T bc[2][64], ba[2][64];
     __shared__ double acc[2][3][64];
-    __shared__ unsigned short plist[64 * 64];
+    __shared__ unsigned short ring[TILE_RING];
     const int lane = threadIdx.x, np = a.npad;
     const int ntiles = *a.ntiles;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -147,13 +147,12 @@ __global__ __launch_bounds__(64) void k_disp_tiles(HippoArgs a)
             ba[side][lane] = a.shp[np + s];
             acc[side][0][lane] = acc[side][1][lane] = acc[side][2][lane] = 0.0;
         }
-        const int npair = tile_pair_list(a.mask[(size_t)t * 128 + lane], plist, lane);
+        TilePairStream ps(a.mask[(size_t)t * 128 + lane]);
         __syncthreads();
         double e = 0.0;
         Vir6 vir;
-        for (int base = 0; base < npair; base += 64) {
-            if (base + lane >= npair) break;
-            const int ent = plist[base + lane];
+        for (int cnt, ent; (cnt = ps.next(ring, lane, ent)) > 0;) {
+            if (lane >= cnt) break;
             const int il = ent >> 8, jl = ent & 63;
             double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
             image<double>(a.box, xd, yd, zd);
@@ -263,7 +262,7 @@ __global__ __launch_bounds__(64) void k_rep_tiles(HippoArgs a)
     __shared__ double bx[2][64], by[2][64], bz[2][64];
     __shared__ T bm[2][10][64], bsz[2][64], bdm[2][64];
     __shared__ double acc[2][6][64];
-    __shared__ unsigned short plist[64 * 64];
+    __shared__ unsigned short ring[TILE_RING];
     const int lane = threadIdx.x, np = a.npad;
     const int ntiles = *a.ntiles;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -281,13 +280,12 @@ __global__ __launch_bounds__(64) void k_rep_tiles(HippoArgs a)
             bdm[side][lane] = a.shp[3 * np + s];
             for (int c = 0; c < 6; c++) acc[side][c][lane] = 0.0;
         }
-        const int npair = tile_pair_list(a.mask[(size_t)t * 128 + lane], plist, lane);
+        TilePairStream ps(a.mask[(size_t)t * 128 + lane]);
         __syncthreads();
         double e = 0.0;
         Vir6 vir;
-        for (int base = 0; base < npair; base += 64) {
-            if (base + lane >= npair) break;
-            const int ent = plist[base + lane];
+        for (int cnt, ent; (cnt = ps.next(ring, lane, ent)) > 0;) {
+            if (lane >= cnt) break;
             const int il = ent >> 8, jl = ent & 63;
             double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
             image<double>(a.box, xd, yd, zd);
@@ -381,7 +379,7 @@ __global__ __launch_bounds__(64) void k_ct_tiles(HippoArgs a)
     __shared__ T bc[2][64], ba[2][64];
     __shared__ int bu[2][64];
     __shared__ double acc[2][3][64];
-    __shared__ unsigned short plist[64 * 64];
+    __shared__ unsigned short ring[TILE_RING];
     const int lane = threadIdx.x, np = a.npad;
     const int ntiles = *a.ntiles;
     const T f = T(a.felec);
@@ -397,13 +395,12 @@ __global__ __launch_bounds__(64) void k_ct_tiles(HippoArgs a)
             bu[side][lane] = a.shp[7 * np + s] != 0.0;
             acc[side][0][lane] = acc[side][1][lane] = acc[side][2][lane] = 0.0;
         }
-        const int npair = tile_pair_list(a.mask[(size_t)t * 128 + lane], plist, lane);
+        TilePairStream ps(a.mask[(size_t)t * 128 + lane]);
         __syncthreads();
         double e = 0.0;
         Vir6 vir;
-        for (int base = 0; base < npair; base += 64) {
-            if (base + lane >= npair) break;
-            const int ent = plist[base + lane];
+        for (int cnt, ent; (cnt = ps.next(ring, lane, ent)) > 0;) {
+            if (lane >= cnt) break;
             const int il = ent >> 8, jl = ent & 63;
             double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
             image<double>(a.box, xd, yd, zd);

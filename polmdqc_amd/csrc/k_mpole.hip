@@ -365,8 +365,9 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
     __shared__ int bmol[2][PMQ_BLOCK];
     // FP64 accumulators whatever R is (ds_add_f64 measured faster than
     // ds_add_f32 here)
-    __shared__ double acc[2][8][PMQ_BLOCK]; // f(3), t(3), pot, aem
-    __shared__ unsigned short plist[PMQ_BLOCK * PMQ_BLOCK];
+    constexpr int NACC = ANA ? 8 : CF ? 7 : 6;
+    __shared__ double acc[2][NACC][PMQ_BLOCK]; // f(3), t(3), pot, aem
+    __shared__ unsigned short ring[TILE_RING];
     const int lane = threadIdx.x;
     const int ntiles = *a.ntiles;
     const int np = a.npad;
@@ -383,18 +384,17 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
                 bc[side][0][lane] = a.scp[s]; bc[side][1][lane] = a.scp[np + s]; bc[side][2][lane] = a.scp[2 * np + s];
             }
             if (ANA) bmol[side][lane] = a.smol[s];
-            for (int c = 0; c < 8; c++) acc[side][c][lane] = 0.0;
+            for (int c = 0; c < NACC; c++) acc[side][c][lane] = 0.0;
         }
-        const int npair = tile_pair_list(a.xmask[(size_t)t * 128 + 64 + lane], plist, lane);
+        TilePairStream ps(a.xmask[(size_t)t * 128 + 64 + lane]);
         __syncthreads();
 
         double e = 0; // energy sums stay FP64 whatever R is
         R ein = 0;
         Vir6 vir; // FP64 whatever R is
         unsigned nem = 0;
-        for (int base = 0; base < npair; base += PMQ_BLOCK) {
-            if (base + lane >= npair) break;
-            const int ent = plist[base + lane];
+        for (int cnt, ent; (cnt = ps.next(ring, lane, ent)) > 0;) {
+            if (lane >= cnt) break;
             const int il = ent >> 8, jl = ent & 63;
             double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
             image<double>(a.box, xd, yd, zd);

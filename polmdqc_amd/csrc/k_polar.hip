@@ -392,7 +392,7 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
     __shared__ double bx[2][64], by[2][64], bz[2][64];
     __shared__ T bm[2][10][64], bu[2][6][64], bpd[2][64], bth[2][64];
     __shared__ double acc[2][NA][64];
-    __shared__ unsigned short plist[64 * 64];
+    __shared__ unsigned short ring[TILE_RING];
     const double hf = 0.5 * a.f;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int2 IJ = a.tiles[t];
@@ -409,11 +409,10 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
             bpd[side][lane] = CPD ? a.scp[np + s] : a.spol[2 * np + s];
             for (int c = 0; c < NA; c++) acc[side][c][lane] = 0.0;
         }
-        const int npair = tile_pair_list(a.mask[(size_t)t * 128 + lane], plist, lane);
+        TilePairStream ps(a.mask[(size_t)t * 128 + lane]);
         __syncthreads();
-        for (int base = 0; base < npair; base += 64) {
-            if (base + lane >= npair) break;
-            const int ent = plist[base + lane];
+        for (int cnt, ent; (cnt = ps.next(ring, lane, ent)) > 0;) {
+            if (lane >= cnt) break;
             const int il = ent >> 8, jl = ent & 63;
             double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
             image<double>(a.box, xd, yd, zd);

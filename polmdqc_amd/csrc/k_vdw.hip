@@ -89,7 +89,7 @@ __global__ __launch_bounds__(64) void k_vdw_tiles(VdwArgs a)
     __shared__ double bx[2][64], by[2][64], bz[2][64];
     __shared__ int bc[2][64], bu[2][64];
     __shared__ double acc[2][3][64];
-    __shared__ unsigned short plist[64 * 64];
+    __shared__ unsigned short ring[TILE_RING];
     const int lane = threadIdx.x, nv = a.nv, nv2 = nv * nv;
     const int ntiles = *a.ntiles;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -104,13 +104,12 @@ __global__ __launch_bounds__(64) void k_vdw_tiles(VdwArgs a)
             bu[side][lane] = i >= 0 ? a.vuse[i] : 0;
             acc[side][0][lane] = acc[side][1][lane] = acc[side][2][lane] = 0.0;
         }
-        const int npair = tile_pair_list(a.mask[(size_t)t * 128 + lane], plist, lane);
+        TilePairStream ps(a.mask[(size_t)t * 128 + lane]);
         __syncthreads();
         double e = 0.0;
         Vir6 vir;
-        for (int base = 0; base < npair; base += 64) {
-            if (base + lane >= npair) break;
-            const int ent = plist[base + lane];
+        for (int cnt, ent; (cnt = ps.next(ring, lane, ent)) > 0;) {
+            if (lane >= cnt) break;
             const int il = ent >> 8, jl = ent & 63;
             double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
             image<double>(a.box, xd, yd, zd);

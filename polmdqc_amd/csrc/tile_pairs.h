@@ -3,13 +3,26 @@
 // The tile masks give every lane (an atom of block I) the set of atoms of
 // block J it pairs with.  Looping over each lane's own mask leaves lanes idle
 // until the fullest row is done (about a fifth of the lanes do work on a
-// water box).  Instead the wave writes the tile's pairs to LDS once and then
-// takes them 64 at a time, one pair per lane.  The list is ordered by rank:
-// the first pair of every row, then the second, ..., and each row hands out
-// its partners starting at its own lane position, so the 64 pairs of a chunk
-// mostly have distinct i and distinct j (few same-address LDS atomics when the
-// chunk accumulates into both atoms).  The order depends only on the masks,
-// and one wave issues every LDS atomic, so sums are reproducible run to run.
+// water box).  Instead the wave hands the tile's pairs out 64 at a time, one
+// pair per lane, in rank order: the first pair of every row, then the
+// second, ..., and each row hands out its partners starting at its own lane
+// position, so the 64 pairs of a batch mostly have distinct i and distinct j
+// (few same-address LDS atomics when the batch accumulates into both atoms).
+// The order depends only on the masks, and one wave issues every LDS atomic,
+// so sums are reproducible run to run.
+//
+// Two forms of the same schedule.  tile_pair_list writes the whole list
+// (up to 4096 entries, 8 KB of LDS) before the first batch.
+// TilePairStream produces the pairs one rank at a time (at most 64 entries)
+// into a ring of 128 LDS entries and hands out a batch whenever 64 are
+// waiting, or at the end: 256 bytes of LDS, so more tile waves share a CU.
+// Batches are consecutive runs of the same rank order either way, so the
+// sums do not depend on the form.  Measured on a 12,288-atom water box: the
+// ring helps the kernels with heavy pair bodies and large LDS footprints
+// (multipole energy and polarization gradient tiles, 2-3% each) and slows
+// the light ones (pair-matrix fill +23%, permanent field +5%: the list
+// generation interleaved with the batches costs more than the occupancy
+// gains), so those keep the whole list.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,5 +50,46 @@ __device__ __forceinline__ int tile_pair_list(unsigned long long m, unsigned sho
     }
     return base;
 }
+
+constexpr int TILE_RING = 128; // LDS entries of the pair ring (two batches)
+
+struct TilePairStream {
+    unsigned long long m; // this lane's partners not yet produced
+    int made, taken;      // pairs produced and handed out so far (wave-uniform)
+    bool more;            // some row still has partners
+
+    __device__ __forceinline__ explicit TilePairStream(unsigned long long mask)
+        : m(mask), made(0), taken(0), more(true) {}
+
+    // The next batch: returns its size (0 when the tile is done; below 64
+    // only for the last batch) and sets ent to this lane's entry, row << 8 |
+    // col, valid when lane < the size.  Called by the whole wave, the only
+    // wave of its workgroup (one-wave tile kernels).
+    __device__ __forceinline__ int next(unsigned short *ring, int lane, int &ent)
+    {
+        const unsigned long long below = (1ull << lane) - 1ull;
+        while (more && made - taken < 64) {
+            const bool has = m != 0ull;
+            const unsigned long long b = __ballot(has);
+            if (b == 0ull) {
+                more = false;
+                break;
+            }
+            if (has) {
+                // first set bit at or after position `lane`, cyclically
+                const unsigned long long rot = lane ? ((m >> lane) | (m << (64 - lane))) : m;
+                const int jj = (__ffsll((long long)rot) - 1 + lane) & 63;
+                m &= ~(1ull << jj);
+                ring[(made + __popcll(b & below)) & (TILE_RING - 1)] = (unsigned short)((lane << 8) | jj);
+            }
+            made += __popcll(b);
+        }
+        const int cnt = min(64, made - taken);
+        __syncthreads();
+        ent = ring[(taken + lane) & (TILE_RING - 1)];
+        taken += cnt;
+        return cnt;
+    }
+};
 
 } // namespace pmq
