@@ -1,11 +1,16 @@
 // k_neighbors.hip -- spatial sort and the 64x64 block-tile list.
 //
-// Every evaluation: (1) key each atom by the Morton index of its cell
+// Every evaluation: (1) key each atom by the Hilbert index of its cell
 // (cells about 4 A wide in the box's fractional frame, or in the bounding box
 // of a non-periodic system; with PME the grid's spreading bricks) with the
-// atom index in the low 32 bits, so the sort is total and deterministic; (2) radix-sort the keys; (3) gather the
-// permutation; (4) bound each 64-atom block; (5) keep every block pair
-// (I <= J) whose bounds come within the cutoff, in (I, J) order.
+// atom index in the low 32 bits, so the sort is total and deterministic; (2)
+// radix-sort the keys; (3) gather the permutation; (4) bound each 64-atom
+// block; (5) keep every block pair (I <= J) whose bounds come within the
+// cutoff, in (I, J) order.  Runs of cells along a Hilbert curve are compact
+// (consecutive cells share a face), so 64 consecutive atoms fill a small box:
+// on a 12,288-atom water box with 4.1 A bricks and a 7 A cutoff the blocks'
+// mean extent is 9.7 x 10.2 x 9.6 A and 4,503 block pairs are kept, against
+// 16.0 x 11.9 x 9.5 A and 7,675 pairs for a Morton order of the same bricks.
 //
 // A block pair is kept when the minimum-image gap between the two bounding
 // boxes is within the cutoff.  For orthogonal boxes and free clusters the
@@ -69,16 +74,6 @@ DevBox make_box(const pmq_box *b)
     return d;
 }
 
-__device__ __forceinline__ unsigned spread3(unsigned v)
-{
-    v &= 0x3ff;
-    v = (v | (v << 16)) & 0x030000ff;
-    v = (v | (v << 8)) & 0x0300f00f;
-    v = (v | (v << 4)) & 0x030c30c3;
-    v = (v | (v << 2)) & 0x09249249;
-    return v;
-}
-
 // bounding box of a free cluster (one workgroup)
 __global__ __launch_bounds__(1024) void k_bbox(int n, const double *__restrict__ x, const double *__restrict__ y,
                                                const double *__restrict__ z, double *__restrict__ out)
@@ -121,7 +116,7 @@ __device__ __forceinline__ void frac_of(const DevBox &b, const double *bbox, dou
     }
 }
 
-__global__ void k_cell_keys(int n, DevBox b, const double *__restrict__ bbox, int ncx, int ncy, int ncz,
+__global__ void k_cell_keys(int n, DevBox b, const double *__restrict__ bbox, int ncx, int ncy, int ncz, int bits,
                             const double *__restrict__ x, const double *__restrict__ y,
                             const double *__restrict__ z, unsigned long long *__restrict__ keys)
 {
@@ -132,21 +127,21 @@ __global__ void k_cell_keys(int n, DevBox b, const double *__restrict__ bbox, in
     const unsigned cx = min((int)(f[0] * ncx), ncx - 1);
     const unsigned cy = min((int)(f[1] * ncy), ncy - 1);
     const unsigned cz = min((int)(f[2] * ncz), ncz - 1);
-    const unsigned m = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
+    const unsigned m = hilbert3(cx, cy, cz, bits);
     keys[i] = ((unsigned long long)m << 32) | (unsigned)i;
 }
 
 // with PME on: the cells are the spreading bricks (home_brick, the same
 // arithmetic the spreading kernels use), so every brick's atoms end up
 // contiguous in the sorted order
-__global__ void k_brick_keys(int n, BrickGeom g, const double *__restrict__ x, const double *__restrict__ y,
+__global__ void k_brick_keys(int n, BrickGeom g, int bits, const double *__restrict__ x, const double *__restrict__ y,
                              const double *__restrict__ z, unsigned long long *__restrict__ keys)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int hb[3];
     home_brick(g.A, g.K, g.nb, x[i], y[i], z[i], hb);
-    keys[i] = ((unsigned long long)morton3(hb[0], hb[1], hb[2]) << 32) | (unsigned)i;
+    keys[i] = ((unsigned long long)hilbert3(hb[0], hb[1], hb[2], bits) << 32) | (unsigned)i;
 }
 
 // permutation, sorted positions and per-atom flags (bit0 valid, bit1 active)
@@ -326,6 +321,9 @@ int build_neighbors(pmq_ctx *c, double cutoff, const double *px, const double *p
         for (int a = 0; a < 3; a++) nc[a] = std::max(1, std::min(1024, (int)(len[a] / 4.0)));
     }
     const int tb = 256;
+    int mx = 1;
+    for (int a = 0; a < 3; a++) mx = std::max(mx, c->pme_cells ? c->pme_nb[a] : nc[a]);
+    const int bits = hilbert_bits(mx);
     if (c->pme_cells) {
         BrickGeom g;
         for (int a = 0; a < 3; a++) {
@@ -333,19 +331,16 @@ int build_neighbors(pmq_ctx *c, double cutoff, const double *px, const double *p
             g.K[a] = c->K[a];
             g.nb[a] = c->pme_nb[a];
         }
-        hipLaunchKernelGGL(k_brick_keys, dim3((n + tb - 1) / tb), dim3(tb), 0, st, n, g, px, py, pz, c->keys.p);
+        hipLaunchKernelGGL(k_brick_keys, dim3((n + tb - 1) / tb), dim3(tb), 0, st, n, g, bits, px, py, pz,
+                           c->keys.p);
     } else {
         hipLaunchKernelGGL(k_cell_keys, dim3((n + tb - 1) / tb), dim3(tb), 0, st, n, b, c->bbox.p, nc[0], nc[1],
-                           nc[2], px, py, pz, c->keys.p);
+                           nc[2], bits, px, py, pz, c->keys.p);
     }
     // The low 32 bits of a key hold the atom index and keys are written in
-    // index order, so a stable sort on the Morton bits alone gives the same
+    // index order, so a stable sort on the Hilbert bits alone gives the same
     // order as a sort on all 64: only bits [32, 32 + 3 b) are passed, b the
     // bits of the largest cell (or brick) index on any axis.
-    int mx = 1;
-    for (int a = 0; a < 3; a++) mx = std::max(mx, c->pme_cells ? c->pme_nb[a] : nc[a]);
-    int bits = 1;
-    while ((1 << bits) < mx) bits++;
     const unsigned end_bit = 32u + 3u * (unsigned)bits;
     size_t bytes = 0;
     PMQ_HIP(rocprim::radix_sort_keys(nullptr, bytes, c->keys.p, c->keys_sorted.p, (size_t)n, 32u, end_bit, st));

@@ -35,18 +35,46 @@ __device__ __forceinline__ double grid_coord(const double *A, double px, double 
 // spreading bricks: axis a is cut at brick_lo(b) = floor(b K / nb)
 __device__ __host__ __forceinline__ int brick_lo(int b, int K, int nb) { return (int)(((long long)b * K) / nb); }
 
-// Morton interleave of three 10-bit cell or brick indices
-__device__ __forceinline__ unsigned morton3(unsigned x, unsigned y, unsigned z)
+// Position of cell (x, y, z) along the 3-D Hilbert curve through a cube of
+// 2^bits cells a side (bits <= 10), by Skilling's transpose method (J.
+// Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707, 381
+// (2004)): undo the curve's rotations and reflections level by level, Gray
+// code the result, then interleave the three words' bits, x highest.
+// Cells next to each other on the curve share a face, so any run of cells
+// along it stays compact, which is what the atom sort's 64-atom blocks need
+// (a Morton order jumps across the box between its octants).
+__device__ __host__ __forceinline__ unsigned hilbert3(unsigned x, unsigned y, unsigned z, int bits)
 {
-    auto spread = [](unsigned v) {
-        v &= 0x3ff;
-        v = (v | (v << 16)) & 0x030000ff;
-        v = (v | (v << 8)) & 0x0300f00f;
-        v = (v | (v << 4)) & 0x030c30c3;
-        v = (v | (v << 2)) & 0x09249249;
-        return v;
-    };
-    return spread(x) | (spread(y) << 1) | (spread(z) << 2);
+    unsigned X[3] = {x, y, z};
+    for (unsigned q = 1u << (bits - 1); q > 1u; q >>= 1) {
+        const unsigned p = q - 1u;
+        for (int i = 0; i < 3; i++) {
+            if (X[i] & q) {
+                X[0] ^= p;
+            } else {
+                const unsigned t = (X[0] ^ X[i]) & p;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    X[1] ^= X[0];
+    X[2] ^= X[1];
+    unsigned t = 0;
+    for (unsigned q = 1u << (bits - 1); q > 1u; q >>= 1)
+        if (X[2] & q) t ^= q - 1u;
+    unsigned h = 0;
+    for (int b = bits - 1; b >= 0; b--)
+        for (int i = 0; i < 3; i++) h = (h << 1) | (((X[i] ^ t) >> b) & 1u);
+    return h;
+}
+
+// bits of the Hilbert cube holding n cells on the longest axis
+__host__ __forceinline__ int hilbert_bits(int n)
+{
+    int bits = 1;
+    while ((1 << bits) < n) bits++;
+    return bits;
 }
 
 // the grid's brick layout, handed to the atom sort

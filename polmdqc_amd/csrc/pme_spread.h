@@ -43,7 +43,7 @@ __device__ __forceinline__ void frac_mpole(const PmeGeom &g, const double *m, do
 // homed in brick h covers only bricks h-1 and h on each axis, and brick b
 // receives exactly the atoms homed in b and b+1.  Each grid point is summed
 // by one lane and stored once: no atomics and no zeroing of the grid.  With
-// PME on, the atom sort orders atoms by the Morton index of their home brick
+// PME on, the atom sort orders atoms by the Hilbert index of their home brick
 // (home_brick on the same coordinates), so each brick's atoms are one
 // contiguous range of the sorted arrays, [bstart, bend).
 struct BrickArgs {

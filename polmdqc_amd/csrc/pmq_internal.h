@@ -2,7 +2,7 @@
 //
 // Data layout in HBM (one context = one system on one GPU):
 //   * coordinates x,y,z are kept in the caller's atom order (double[n] each);
-//   * every eval sorts atoms spatially (cells in Morton order, stable by atom
+//   * every eval sorts atoms spatially (cells in Hilbert order, stable by atom
 //     index) and the pair kernels work on "sorted" SoA copies: positions,
 //     the 10 rotated multipole components, and per-atom flags, padded to a
 //     multiple of 64 (one wavefront per 64-atom block);
