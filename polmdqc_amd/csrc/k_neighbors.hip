@@ -283,7 +283,7 @@ __global__ void k_tile_flags(int nblk, long long npairs, DevBox b, double cut, c
     flag[t] = keep ? 1 : 0;
 }
 
-// the listed tiles in pair order; the last pair's thread also stores the
+// the listed tiles (tile_slot order); the last pair's thread also stores the
 // count
 __global__ void k_tile_write(int nblk, long long npairs, const int *__restrict__ flag, const int *__restrict__ scan,
                              int2 *__restrict__ tiles, int *__restrict__ count)
@@ -293,7 +293,7 @@ __global__ void k_tile_write(int nblk, long long npairs, const int *__restrict__
     if (t >= npairs || !flag[t]) return;
     int I, J;
     tri_index(t, nblk, I, J);
-    tiles[scan[t]] = make_int2(I, J);
+    tiles[tile_slot(I, J, nblk, scan[t])] = make_int2(I, J);
 }
 
 int build_neighbors(pmq_ctx *c, double cutoff, const double *px, const double *py, const double *pz,

@@ -190,16 +190,16 @@ __global__ __launch_bounds__(64) void k_block_adj(int nblk, int cap, const int *
     int count = 0;
     for (int base = 0; base < nblk; base += 64) {
         const int J = base + lane;
-        int f = 0;
-        long long tri = 0;
+        int f = 0, slot = 0;
         if (J < nblk) {
             const long long lo = min(I, J), hi = max(I, J);
-            tri = lo * nblk - lo * (lo - 1) / 2 + (hi - lo);
-            f = flag[tri] && scan[tri] < cap;
+            const long long tri = lo * nblk - lo * (lo - 1) / 2 + (hi - lo);
+            slot = tile_slot((int)lo, (int)hi, nblk, scan[tri]);
+            f = flag[tri] && slot < cap;
         }
         const unsigned long long b = __ballot(f != 0);
         const int pos = count + __popcll(b & ((1ull << lane) - 1ull));
-        if (f) adj[(size_t)I * nblk + pos] = scan[tri];
+        if (f) adj[(size_t)I * nblk + pos] = slot;
         count += __popcll(b);
     }
     if (lane == 0) adjn[I] = count;

@@ -29,6 +29,20 @@
 
 namespace pmq {
 
+// Slot of the kept block pair (I, J), I <= J, in the tile list: the nblk
+// diagonal tiles (always kept: a block is within the cutoff of itself) come
+// first, in block order, then the others in (I, J) order.  scan is the
+// pair's exclusive count of kept pairs in (I, J) order, which before an
+// off-diagonal pair of row I includes the I + 1 diagonals of rows 0..I.
+// The tile kernels take tiles in slot order, one wave each, and a diagonal
+// tile holds several times the pairs of the others (about 1,200 against 190
+// on a water box with 64-atom Hilbert blocks and a 7 A cutoff), so issuing
+// them first keeps them off the end of the launch.
+__host__ __device__ __forceinline__ int tile_slot(int I, int J, int nblk, int scan)
+{
+    return I == J ? I : nblk + scan - (I + 1);
+}
+
 // ---------------------------------------------------------------------------
 // periodic box as the kernels see it (lattice constants of lattice.cpp)
 struct DevBox {
