@@ -73,6 +73,7 @@ struct HippoArgs {
     int n, npad;
     const int2 *tiles;
     const int *ntiles;
+    TileItems items; // pieces of the tiles (tile_pairs.h)
     const double *sx, *sy, *sz, *smp, *shp;
     const int *perm, *iperm;
     DevBox box;
@@ -136,7 +137,11 @@ __global__ __launch_bounds__(64) void k_disp_tiles(HippoArgs a)
     __shared__ unsigned short ring[TILE_RING];
     const int lane = threadIdx.x, np = a.npad;
     const int ntiles = *a.ntiles;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int v = blockIdx.x, nitems = ntiles + *a.items.xcnt; v < nitems; v += gridDim.x) {
+        TileItem it;
+        if (!tile_item(v, ntiles, a.items, it)) continue;
+        const int u = it.u;
+        const int t = it.t;
         const int2 IJ = a.tiles[t];
         __syncthreads();
 #pragma unroll
@@ -151,7 +156,8 @@ __global__ __launch_bounds__(64) void k_disp_tiles(HippoArgs a)
         __syncthreads();
         double e = 0.0;
         Vir6 vir;
-        for (int cnt, ent; (cnt = ps.next(ring, lane, ent)) > 0;) {
+        for (int cnt, ent, b = 0; b < it.b1 && (cnt = ps.next(ring, lane, ent)) > 0; b++) {
+            if (b < it.b0) continue;
             if (lane >= cnt) break;
             const int il = ent >> 8, jl = ent & 63;
             double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
@@ -173,7 +179,7 @@ __global__ __launch_bounds__(64) void k_disp_tiles(HippoArgs a)
             if (a.vir) vir.pair(xd, yd, zd, -g * (double)xr, -g * (double)yr, -g * (double)zr);
         }
         __syncthreads();
-        T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
+        T *P = (T *)a.part + (size_t)u * 2 * PMQ_TILE_NV * 64;
         T *Q = P + PMQ_TILE_NV * 64;
         for (int c = 0; c < 3; c++) {
             P[c * 64 + lane] = T(acc[0][c][lane]);
@@ -182,8 +188,8 @@ __global__ __launch_bounds__(64) void k_disp_tiles(HippoArgs a)
         const double ew = wsum(e);
         if (a.vir) vir.wave_sum();
         if (lane == 0) {
-            a.tscal[t] = ew; // scalar 0 of tile t
-            if (a.vir) vir.store_tile(a.tscal, a.tstride, t);
+            a.tscal[u] = ew; // scalar 0 of item u
+            if (a.vir) vir.store_tile(a.tscal, a.tstride, u);
         }
     }
 }
@@ -265,7 +271,11 @@ __global__ __launch_bounds__(64) void k_rep_tiles(HippoArgs a)
     __shared__ unsigned short ring[TILE_RING];
     const int lane = threadIdx.x, np = a.npad;
     const int ntiles = *a.ntiles;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int v = blockIdx.x, nitems = ntiles + *a.items.xcnt; v < nitems; v += gridDim.x) {
+        TileItem it;
+        if (!tile_item(v, ntiles, a.items, it)) continue;
+        const int u = it.u;
+        const int t = it.t;
         const int2 IJ = a.tiles[t];
         __syncthreads();
 #pragma unroll
@@ -284,7 +294,8 @@ __global__ __launch_bounds__(64) void k_rep_tiles(HippoArgs a)
         __syncthreads();
         double e = 0.0;
         Vir6 vir;
-        for (int cnt, ent; (cnt = ps.next(ring, lane, ent)) > 0;) {
+        for (int cnt, ent, b = 0; b < it.b1 && (cnt = ps.next(ring, lane, ent)) > 0; b++) {
+            if (b < it.b0) continue;
             if (lane >= cnt) break;
             const int il = ent >> 8, jl = ent & 63;
             double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
@@ -311,7 +322,7 @@ __global__ __launch_bounds__(64) void k_rep_tiles(HippoArgs a)
             if (a.vir) vir.pair(xd, yd, zd, (double)f[0], (double)f[1], (double)f[2]);
         }
         __syncthreads();
-        T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
+        T *P = (T *)a.part + (size_t)u * 2 * PMQ_TILE_NV * 64;
         T *Q = P + PMQ_TILE_NV * 64;
         for (int c = 0; c < 6; c++) {
             P[c * 64 + lane] = T(acc[0][c][lane]);
@@ -320,8 +331,8 @@ __global__ __launch_bounds__(64) void k_rep_tiles(HippoArgs a)
         const double ew = wsum(e);
         if (a.vir) vir.wave_sum();
         if (lane == 0) {
-            a.tscal[t] = ew; // scalar 0 of tile t
-            if (a.vir) vir.store_tile(a.tscal, a.tstride, t);
+            a.tscal[u] = ew; // scalar 0 of item u
+            if (a.vir) vir.store_tile(a.tscal, a.tstride, u);
         }
     }
 }
@@ -383,7 +394,11 @@ __global__ __launch_bounds__(64) void k_ct_tiles(HippoArgs a)
     const int lane = threadIdx.x, np = a.npad;
     const int ntiles = *a.ntiles;
     const T f = T(a.felec);
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int v = blockIdx.x, nitems = ntiles + *a.items.xcnt; v < nitems; v += gridDim.x) {
+        TileItem it;
+        if (!tile_item(v, ntiles, a.items, it)) continue;
+        const int u = it.u;
+        const int t = it.t;
         const int2 IJ = a.tiles[t];
         __syncthreads();
 #pragma unroll
@@ -399,7 +414,8 @@ __global__ __launch_bounds__(64) void k_ct_tiles(HippoArgs a)
         __syncthreads();
         double e = 0.0;
         Vir6 vir;
-        for (int cnt, ent; (cnt = ps.next(ring, lane, ent)) > 0;) {
+        for (int cnt, ent, b = 0; b < it.b1 && (cnt = ps.next(ring, lane, ent)) > 0; b++) {
+            if (b < it.b0) continue;
             if (lane >= cnt) break;
             const int il = ent >> 8, jl = ent & 63;
             double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
@@ -421,7 +437,7 @@ __global__ __launch_bounds__(64) void k_ct_tiles(HippoArgs a)
             if (a.vir) vir.pair(xd, yd, zd, -g * xd, -g * yd, -g * zd);
         }
         __syncthreads();
-        T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
+        T *P = (T *)a.part + (size_t)u * 2 * PMQ_TILE_NV * 64;
         T *Q = P + PMQ_TILE_NV * 64;
         for (int c = 0; c < 3; c++) {
             P[c * 64 + lane] = T(acc[0][c][lane]);
@@ -430,8 +446,8 @@ __global__ __launch_bounds__(64) void k_ct_tiles(HippoArgs a)
         const double ew = wsum(e);
         if (a.vir) vir.wave_sum();
         if (lane == 0) {
-            a.tscal[t] = ew; // scalar 0 of tile t
-            if (a.vir) vir.store_tile(a.tscal, a.tstride, t);
+            a.tscal[u] = ew; // scalar 0 of item u
+            if (a.vir) vir.store_tile(a.tscal, a.tstride, u);
         }
     }
 }
@@ -509,7 +525,8 @@ static HippoArgs hippo_args(pmq_ctx *c, double off2)
     a.de = nullptr; a.te = nullptr; a.deq = nullptr; a.teq = nullptr;
     a.tot = c->totals.p;
     a.vir = 0;
-    a.tstride = (size_t)c->tile_cap;
+    a.tstride = item_stride(c);
+    a.items = tile_items(c);
     return a;
 }
 
@@ -524,7 +541,7 @@ int launch_disp(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir)
     a.deq = mixed ? c->fx_grad.p : nullptr;
     {
         ProfScope ps_(c, "disp_tiles");
-        const dim3 g(c->tgrid);
+        const dim3 g(c->igrid);
         if (mixed) {
             if (ew) hipLaunchKernelGGL((k_disp_tiles<true, float>), g, dim3(64), 0, c->stream, a);
             else hipLaunchKernelGGL((k_disp_tiles<false, float>), g, dim3(64), 0, c->stream, a);
@@ -566,7 +583,7 @@ int launch_rep(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir)
     a.teq = mixed ? c->fx_grad.p + 3 * (size_t)c->n : nullptr;
     {
         ProfScope ps_(c, "rep_tiles");
-        const dim3 g(c->tgrid);
+        const dim3 g(c->igrid);
         if (mixed) hipLaunchKernelGGL((k_rep_tiles<float>), g, dim3(64), 0, c->stream, a);
         else hipLaunchKernelGGL((k_rep_tiles<double>), g, dim3(64), 0, c->stream, a);
     }
@@ -607,8 +624,8 @@ int launch_ct(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir)
     a.deq = mixed ? c->fx_grad.p : nullptr;
     {
         ProfScope ps_(c, "ct_tiles");
-        if (mixed) hipLaunchKernelGGL((k_ct_tiles<float>), dim3(c->tgrid), dim3(64), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_ct_tiles<double>), dim3(c->tgrid), dim3(64), 0, c->stream, a);
+        if (mixed) hipLaunchKernelGGL((k_ct_tiles<float>), dim3(c->igrid), dim3(64), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_ct_tiles<double>), dim3(c->igrid), dim3(64), 0, c->stream, a);
     }
     PMQ_HIP(hipGetLastError());
     int rc;

@@ -333,6 +333,7 @@ struct MpoleArgs {
     int n, npad;
     const int2 *tiles;
     const int *ntiles;
+    TileItems items; // pieces of the tiles (tile_pairs.h)
     const double *sx, *sy, *sz, *smp, *scp;
     const int *sflag, *smol, *perm, *iperm;
     DevBox box;
@@ -372,7 +373,11 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
     const int ntiles = *a.ntiles;
     const int np = a.npad;
 
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int v = blockIdx.x, nitems = ntiles + *a.items.xcnt; v < nitems; v += gridDim.x) {
+        TileItem it;
+        if (!tile_item(v, ntiles, a.items, it)) continue;
+        const int u = it.u;
+        const int t = it.t;
         const int2 IJ = a.tiles[t];
         __syncthreads();
 #pragma unroll
@@ -393,7 +398,8 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
         R ein = 0;
         Vir6 vir; // FP64 whatever R is
         unsigned nem = 0;
-        for (int cnt, ent; (cnt = ps.next(ring, lane, ent)) > 0;) {
+        for (int cnt, ent, b = 0; b < it.b1 && (cnt = ps.next(ring, lane, ent)) > 0; b++) {
+            if (b < it.b0) continue;
             if (lane >= cnt) break;
             const int il = ent >> 8, jl = ent & 63;
             double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
@@ -484,7 +490,7 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
         __syncthreads();
         // partial sums of this tile: side 0 = the i atoms, side 1 = the j atoms
         if (GRAD || ANA) {
-            R *P = (R *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
+            R *P = (R *)a.part + (size_t)u * 2 * PMQ_TILE_NV * 64;
             R *Q = P + PMQ_TILE_NV * 64;
             if (GRAD)
                 for (int c = 0; c < 6 + (CF ? 1 : 0); c++) {
@@ -502,11 +508,11 @@ __global__ __launch_bounds__(64) void k_mpole_tiles(MpoleArgs a)
         if (VIR) vir.wave_sum();
         if (ANA) { win = wave_sum((double)ein); nem = wave_sum(nem); }
         if (lane == 0) {
-            double *S = a.tscal + t;
+            double *S = a.tscal + u;
             const size_t ts = a.tstride;
             S[0] = ew;
             if (VIR || ANA) { // the full set is summed when either is on
-                vir.store_tile(a.tscal, ts, t);
+                vir.store_tile(a.tscal, ts, u);
                 S[10 * ts] = win; S[11 * ts] = (double)nem;
             }
         }
@@ -606,7 +612,7 @@ __global__ __launch_bounds__(64) void k_mpole_excl(MpoleArgs a)
 template <int PEN, bool CF, bool ANA, bool GRAD, bool VIR>
 static int launch_mpole_t(pmq_ctx *c, const MpoleArgs &a)
 {
-    const int grid = c->tgrid;
+    const int grid = c->igrid;
     {
         ProfScope ps_(c, "mpole_tiles");
         const bool ew = a.aewald > 0.0;
@@ -683,7 +689,8 @@ int launch_mpole_direct(pmq_ctx *c, int mode, double off2, bool mixed)
     a.xmask = c->xmask.p;
     a.part = c->tpart.p;
     a.tscal = c->tscal.p;
-    a.tstride = (size_t)c->tile_cap;
+    a.tstride = item_stride(c);
+    a.items = tile_items(c);
     a.tot = c->totals.p;
     a.excl = c->excl.p;
     a.nexcl = (int)c->h_excl.size();

@@ -284,12 +284,15 @@ __global__ void k_tile_flags(int nblk, long long npairs, DevBox b, double cut, c
 }
 
 // the listed tiles (tile_slot order); the last pair's thread also stores the
-// count
+// count and clears the count of further tile pieces (tile_pairs.h)
 __global__ void k_tile_write(int nblk, long long npairs, const int *__restrict__ flag, const int *__restrict__ scan,
                              int2 *__restrict__ tiles, int *__restrict__ count)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == npairs - 1) count[0] = scan[t] + flag[t];
+    if (t == npairs - 1) {
+        count[0] = scan[t] + flag[t];
+        count[1] = 0; // the tile masks count the further pieces of split tiles here
+    }
     if (t >= npairs || !flag[t]) return;
     int I, J;
     tri_index(t, nblk, I, J);

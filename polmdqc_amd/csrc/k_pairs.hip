@@ -65,6 +65,8 @@ struct MaskArgs {
     unsigned long long *nmask; // 128 per tile: the pairs within it (row, column masks), or null
     const double *blk;         // block bounds (PMQ_BLK_STRIDE per block)
     unsigned long long *xmask; // 128 per tile: copies of the in-cutoff and active rows (scaled pairs cleared later)
+    int *tpc;                  // per tile: piece code of its in-cutoff pair count (tile_pairs.h)
+    int *xlist, *xcnt;         // further pieces of the split tiles (tile | piece << 28), their count
 };
 
 // BK 0: no box or a rectangular one (one image per tile where it fits, else
@@ -177,6 +179,17 @@ __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
         a.mmask[(size_t)t * 64 + lane] = mm;
         a.xmask[(size_t)t * 128 + lane] = m;
         a.xmask[(size_t)t * 128 + 64 + lane] = mm;
+        int npair = __popcll(m);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) npair += __shfl_xor(npair, o, 64);
+        if (lane == 0) {
+            const int code = tile_piece_code(npair), np = code & 15;
+            a.tpc[t] = code;
+            if (np > 1) {
+                const int k = atomicAdd(a.xcnt, np - 1);
+                for (int p = 1; p < np; p++) a.xlist[k + p - 1] = t | (p << 28);
+            }
+        }
     }
 }
 
@@ -301,6 +314,7 @@ template <class V>
 struct FillArgs {
     const int2 *tiles;
     const int *ntiles;
+    TileItems items;
     const double *sx, *sy, *sz, *spol;
     const double *scp; // sorted core charge, valence exponent (HIPPO damping)
     DevBox box;
@@ -350,7 +364,10 @@ __global__ __launch_bounds__(64) void k_pair_fill(FillArgs<V> a)
         a.st->nnz = total;
         if ((size_t)total > a.cap) a.st->overflow |= 2;
     }
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int v = blockIdx.x, nitems = ntiles + *a.items.xcnt; v < nitems; v += gridDim.x) {
+        TileItem it;
+        if (!tile_item(v, ntiles, a.items, it)) continue;
+        const int t = it.t;
         const int2 IJ = a.tiles[t];
         const bool diag = IJ.x == IJ.y;
         const unsigned long long rm = a.mask[(size_t)t * 128 + lane], cm = a.mask[(size_t)t * 128 + 64 + lane];
@@ -371,9 +388,9 @@ __global__ __launch_bounds__(64) void k_pair_fill(FillArgs<V> a)
             bbase[side][lane] = a.rowptr[s] + (a.nmask ? a.rownear[s] : 0) + a.soff[so];
             bnbase[side][lane] = a.nmask ? a.rowptr[s] + a.nsoff[so] : 0;
         }
-        const int npair = tile_pair_list(rm, plist, lane);
+        const int npair = min(tile_pair_list(rm, plist, lane), it.b1 * 64);
         __syncthreads();
-        for (int base = 0; base < npair; base += 64) {
+        for (int base = it.b0 * 64; base < npair; base += 64) {
             if (base + lane >= npair) break;
             const int ent = plist[base + lane];
             const int il = ent >> 8, jl = ent & 63;
@@ -537,7 +554,7 @@ static int pair_fill(pmq_ctx *c, int tgrid)
     const int np = c->npad;
     hipStream_t st = c->stream;
     FillArgs<V> fa;
-    fa.tiles = c->tiles.p; fa.ntiles = c->ntiles_eff.p;
+    fa.tiles = c->tiles.p; fa.ntiles = c->ntiles_eff.p; fa.items = tile_items(c);
     fa.sx = c->sx.p; fa.sy = c->sy.p; fa.sz = c->sz.p; fa.spol = c->spol.p;
     fa.scp = c->scp.p;
     fa.box = c->box;
@@ -636,9 +653,12 @@ int tile_structure(pmq_ctx *c, double off2, bool safe)
     }
     PMQ_HIP(c->adj.alloc((size_t)nblk * nblk));
     PMQ_HIP(c->adjn.alloc(nblk));
-    PMQ_HIP(c->tpart.alloc(nt * 2 * PMQ_TILE_NV * 64));
-    PMQ_HIP(c->tscal.alloc(nt * PMQ_TILE_NS));
+    PMQ_HIP(c->tpart.alloc(PMQ_TILE_PIECES * nt * 2 * PMQ_TILE_NV * 64));
+    PMQ_HIP(c->tscal.alloc(PMQ_TILE_PIECES * nt * PMQ_TILE_NS));
+    PMQ_HIP(c->tpc.alloc(nt));
+    PMQ_HIP(c->xlist.alloc((PMQ_TILE_PIECES - 1) * nt));
     c->tgrid = (int)std::max<size_t>(1, std::min<size_t>(nt, 65536));
+    c->igrid = (int)std::max<size_t>(1, std::min<size_t>(nt + nt / 2, 262144));
 
     MaskArgs ma;
     ma.tiles = c->tiles.p; ma.ntiles = c->ntiles_d.p; ma.cap = c->tile_cap; ma.ntiles_eff = c->ntiles_eff.p;
@@ -649,6 +669,9 @@ int tile_structure(pmq_ctx *c, double off2, bool safe)
     ma.nmask = c->prec_on ? c->nmask.p : nullptr;
     ma.blk = c->blk.p;
     ma.xmask = c->xmask.p;
+    ma.tpc = c->tpc.p;
+    ma.xlist = c->xlist.p;
+    ma.xcnt = c->ntiles_d.p + 1;
     if (c->box.kind == PMQ_BOX_NONE || c->box.kind == PMQ_BOX_ORTHO)
         hipLaunchKernelGGL(k_tile_masks<0>, dim3(c->tgrid), dim3(64), 0, st, ma);
     else
@@ -697,8 +720,8 @@ int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe)
     if (c->prec_on) PMQ_HIP(c->tval_s.alloc(2 * c->tcap));
 
     c->tmixed = mixed;
-    if (mixed) return pair_fill<float, float>(c, c->tgrid);
-    return pair_fill<double, double>(c, c->tgrid);
+    if (mixed) return pair_fill<float, float>(c, c->igrid);
+    return pair_fill<double, double>(c, c->igrid);
 }
 
 // ---------------------------------------------------------------------------
@@ -709,13 +732,15 @@ int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe)
 // workgroup of RED_W waves per block: the block's tile list is staged in LDS,
 // wave w sums tiles w, w+RED_W, ... (independent loads, several in flight),
 // and the RED_W partial sums are added in wave order.
-constexpr int RED_MAXT = 1024; // tiles per block staged at a time
-constexpr int RED_W = 8;       // waves per block
+constexpr int RED_MAXT = 1024;                     // LDS entries (tile pieces) per round
+constexpr int RED_TPR = RED_MAXT / PMQ_TILE_PIECES; // tiles per round
+constexpr int RED_W = 8;                            // waves per block
 
 template <class P, int NV, bool FX>
 __global__ __launch_bounds__(64 * RED_W) void k_tile_reduce(TileReduceArgs a)
 {
     __shared__ int lt[RED_MAXT];
+    __shared__ int nent;
     __shared__ unsigned long long red[RED_W][NV][64]; // fixed-point sums, or the bits of double sums
     const int I = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int si = I * 64 + lane;
@@ -728,23 +753,48 @@ __global__ __launch_bounds__(64 * RED_W) void k_tile_reduce(TileReduceArgs a)
 #pragma unroll
     for (int c = 0; c < NV; c++) want[c] = a.out[c] != nullptr;
     const int na = a.adjn[I];
-    for (int base = 0; base < na; base += RED_MAXT) {
-        const int cnt = min(RED_MAXT, na - base);
+    const int ntiles = *a.ntiles;
+    for (int base = 0; base < na; base += RED_TPR) {
+        const int cnt = min(RED_TPR, na - base);
         __syncthreads();
-        // entry = partial-slot index (tile * 2 + side), bit 30 set: a diagonal tile (both sides)
-        for (int s = tid; s < cnt; s += 64 * RED_W) {
-            const int t = a.adj[(size_t)I * a.nblk + base + s];
-            const int2 IJ = a.tiles[t];
-            lt[s] = IJ.x == IJ.y ? ((t * 2) | (1 << 30)) : (t * 2 + (IJ.x == I ? 0 : 1));
+        // one entry per piece of each tile, tiles in list order and a tile's
+        // pieces together (tiles without pairs have none): the partial slot
+        // (item * 2 + side; piece p of tile t is item p ntiles + t), bit 30
+        // set for a diagonal tile (both sides).  Wave 0 lays them out with a
+        // prefix sum of the piece counts.
+        if (wv == 0) {
+            int m = 0;
+            for (int c0 = 0; c0 < cnt; c0 += 64) {
+                const int s = c0 + lane;
+                int np = 0, slot0 = 0, two = 0;
+                if (s < cnt) {
+                    const int t = a.adj[(size_t)I * a.nblk + base + s];
+                    const int2 IJ = a.tiles[t];
+                    np = a.tpc[t] & 15;
+                    two = IJ.x == IJ.y;
+                    slot0 = t * 2 + (two || IJ.x == I ? 0 : 1);
+                }
+                int inc = np;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int y = __shfl_up(inc, o, 64);
+                    if (lane >= o) inc += y;
+                }
+                const int off = m + inc - np;
+                for (int p = 0; p < np; p++) lt[off + p] = (slot0 + 2 * p * ntiles) | (two << 30);
+                m += __shfl(inc, 63, 64);
+            }
+            if (lane == 0) nent = m;
         }
         __syncthreads();
+        const int total = nent;
         // loads are unconditional (a diagonal tile's second side is read from
         // the next slot, any other tile's "second side" is its own slot again
         // and dropped), so the loads of several tiles are in flight together;
         // components nobody wants are read and never stored
         constexpr int RU = sizeof(P) == 4 ? 4 : 2; // tiles in flight per wave
 #pragma unroll RU
-        for (int s = wv; s < cnt; s += RED_W) {
+        for (int s = wv; s < total; s += RED_W) {
             const int e = lt[s];
             const int slot = e & ~(1 << 30);
             const bool two = (e >> 30) & 1;
@@ -796,19 +846,24 @@ __global__ __launch_bounds__(64 * RED_W) void k_tile_reduce(TileReduceArgs a)
 // virial (1..9), the inter-molecular energy (10) and the pair count (11)
 constexpr int SCAL_TB = 1024;
 
-// tile scalars k0..k1-1 of every tile into the totals (0 energy, 1..9 virial,
-// 10 inter-molecular energy, 11 pair count)
-__global__ __launch_bounds__(SCAL_TB) void k_tile_scalars(const int *ntiles_p, int which, int k0, int k1,
-                                                          const double *__restrict__ tscal, size_t stride,
-                                                          Totals *tot)
+// tile-item scalars k0..k1-1 of every item into the totals (0 energy, 1..9
+// virial, 10 inter-molecular energy, 11 pair count); items past their
+// tile's piece count wrote nothing and are skipped
+__global__ __launch_bounds__(SCAL_TB) void k_tile_scalars(const int *ntiles_p, const int *__restrict__ tpc, int which,
+                                                          int k0, int k1, const double *__restrict__ tscal,
+                                                          size_t stride, Totals *tot)
 {
     __shared__ double red[SCAL_TB];
     const int ntiles = *ntiles_p;
     for (int k = k0; k < k1; k++) {
-        // each thread's tiles in order (the unrolled loads go out together)
+        // each thread's tiles in order, a tile's pieces together (item p
+        // ntiles + t is piece p of tile t)
         double v = 0.0;
-#pragma unroll 8
-        for (int t = threadIdx.x; t < ntiles; t += SCAL_TB) v += tscal[(size_t)k * stride + t];
+#pragma unroll 4
+        for (int t = threadIdx.x; t < ntiles; t += SCAL_TB) {
+            const int np = tpc[t] & 15;
+            for (int p = 0; p < np; p++) v += tscal[(size_t)k * stride + (size_t)p * ntiles + t];
+        }
         red[threadIdx.x] = v;
         __syncthreads();
         for (int w = SCAL_TB / 2; w > 0; w >>= 1) {
@@ -830,6 +885,7 @@ int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fi
 {
     a.nblk = c->nblk; a.n = c->n;
     a.tiles = c->tiles.p; a.adj = c->adj.p; a.adjn = c->adjn.p; a.perm = c->perm.p;
+    a.tpc = c->tpc.p; a.ntiles = c->ntiles_eff.p;
     a.part = c->tpart.p;
     const dim3 g(c->nblk), b(64 * RED_W);
 #define PMQ_RED(P, NV, FX) hipLaunchKernelGGL((k_tile_reduce<P, NV, FX>), g, b, 0, c->stream, a)
@@ -849,16 +905,16 @@ int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fi
 
 int tile_scalars(pmq_ctx *c, int which, bool full)
 {
-    hipLaunchKernelGGL(k_tile_scalars, dim3(1), dim3(SCAL_TB), 0, c->stream, c->ntiles_eff.p, which, 0,
-                       full ? PMQ_TILE_NS : 1, c->tscal.p, (size_t)c->tile_cap, c->totals.p);
+    hipLaunchKernelGGL(k_tile_scalars, dim3(1), dim3(SCAL_TB), 0, c->stream, c->ntiles_eff.p, c->tpc.p, which, 0,
+                       full ? PMQ_TILE_NS : 1, c->tscal.p, item_stride(c), c->totals.p);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }
 
 int tile_virial(pmq_ctx *c)
 {
-    hipLaunchKernelGGL(k_tile_scalars, dim3(1), dim3(SCAL_TB), 0, c->stream, c->ntiles_eff.p, 0, 1, 10, c->tscal.p,
-                       (size_t)c->tile_cap, c->totals.p);
+    hipLaunchKernelGGL(k_tile_scalars, dim3(1), dim3(SCAL_TB), 0, c->stream, c->ntiles_eff.p, c->tpc.p, 0, 1, 10,
+                       c->tscal.p, item_stride(c), c->totals.p);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }

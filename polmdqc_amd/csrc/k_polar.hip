@@ -49,6 +49,7 @@ struct PolArgs {
     int n, npad;
     const int2 *tiles;
     const int *ntiles;
+    TileItems items; // pieces of the tiles (tile_pairs.h)
     const double *sx, *sy, *sz, *smp, *spol;
     const double *scp;   // sorted core charge, valence exponent (charge penetration)
     int cpd;             // HIPPO damping (charge penetration) instead of Thole
@@ -183,7 +184,11 @@ __global__ __launch_bounds__(64) void k_pol_field_tiles(PolArgs a)
     __shared__ T bm[2][10][64], bpd[2][64], bth[2][64];
     __shared__ double acc[2][3][64];
     __shared__ unsigned short plist[64 * 64];
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int v = blockIdx.x, nitems = ntiles + *a.items.xcnt; v < nitems; v += gridDim.x) {
+        TileItem it;
+        if (!tile_item(v, ntiles, a.items, it)) continue;
+        const int u = it.u;
+        const int t = it.t;
         const int2 IJ = a.tiles[t];
         __syncthreads();
 #pragma unroll
@@ -196,9 +201,9 @@ __global__ __launch_bounds__(64) void k_pol_field_tiles(PolArgs a)
             bpd[side][lane] = CPD ? a.scp[np + s] : a.spol[2 * np + s];
             acc[side][0][lane] = acc[side][1][lane] = acc[side][2][lane] = 0.0;
         }
-        const int npair = tile_pair_list(a.mask[(size_t)t * 128 + lane], plist, lane);
+        const int npair = min(tile_pair_list(a.mask[(size_t)t * 128 + lane], plist, lane), it.b1 * 64);
         __syncthreads();
-        for (int base = 0; base < npair; base += 64) {
+        for (int base = it.b0 * 64; base < npair; base += 64) {
             if (base + lane >= npair) break;
             const int ent = plist[base + lane];
             const int il = ent >> 8, jl = ent & 63;
@@ -237,7 +242,7 @@ __global__ __launch_bounds__(64) void k_pol_field_tiles(PolArgs a)
         }
         __syncthreads();
         // partial sums: side 0 the i atoms, side 1 the j atoms
-        T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
+        T *P = (T *)a.part + (size_t)u * 2 * PMQ_TILE_NV * 64;
         T *Q = P + PMQ_TILE_NV * 64;
         for (int c = 0; c < 3; c++) {
             P[c * 64 + lane] = T(acc[0][c][lane]);
@@ -394,7 +399,11 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
     __shared__ double acc[2][NA][64];
     __shared__ unsigned short ring[TILE_RING];
     const double hf = 0.5 * a.f;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int v = blockIdx.x, nitems = ntiles + *a.items.xcnt; v < nitems; v += gridDim.x) {
+        TileItem it;
+        if (!tile_item(v, ntiles, a.items, it)) continue;
+        const int u = it.u;
+        const int t = it.t;
         const int2 IJ = a.tiles[t];
         Vir6 vir;
         __syncthreads();
@@ -411,7 +420,8 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
         }
         TilePairStream ps(a.mask[(size_t)t * 128 + lane]);
         __syncthreads();
-        for (int cnt, ent; (cnt = ps.next(ring, lane, ent)) > 0;) {
+        for (int cnt, ent, b = 0; b < it.b1 && (cnt = ps.next(ring, lane, ent)) > 0; b++) {
+            if (b < it.b0) continue;
             if (lane >= cnt) break;
             const int il = ent >> 8, jl = ent & 63;
             double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
@@ -453,7 +463,7 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
         }
         __syncthreads();
         // partial sums: side 0 the i atoms, side 1 the j atoms
-        T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
+        T *P = (T *)a.part + (size_t)u * 2 * PMQ_TILE_NV * 64;
         T *Q = P + PMQ_TILE_NV * 64;
         for (int c = 0; c < NA; c++) {
             P[c * 64 + lane] = T(hf * acc[0][c][lane]);
@@ -461,7 +471,7 @@ __global__ __launch_bounds__(64) void k_pol_grad_tiles(PolArgs a)
         }
         if (a.vir) {
             vir.wave_sum();
-            if (lane == 0) vir.store_tile(a.tscal, a.tstride, t);
+            if (lane == 0) vir.store_tile(a.tscal, a.tstride, u);
         }
     }
 }
@@ -686,12 +696,13 @@ static PolArgs pol_args(pmq_ctx *c, double off2)
     a.part = c->tpart.p;
     a.vir = 0;
     a.tscal = c->tscal.p;
-    a.tstride = (size_t)c->tile_cap;
+    a.tstride = item_stride(c);
+    a.items = tile_items(c);
     a.tot = c->totals.p;
     return a;
 }
 
-static int tile_grid(pmq_ctx *c) { return c->tgrid; }
+static int tile_grid(pmq_ctx *c) { return c->igrid; }
 
 // dst = T src (real space, from the pair matrix) + reciprocal and self
 // field; skip: the solve's converged flag for the iterations of a

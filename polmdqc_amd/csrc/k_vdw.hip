@@ -27,6 +27,7 @@ struct VdwArgs {
     int n, npad, nv;
     const int2 *tiles;
     const int *ntiles;
+    TileItems items; // pieces of the tiles (tile_pairs.h)
     const double *sx, *sy, *sz;          // sorted reduced sites
     const int *perm, *iperm;
     const int *vcls, *vuse;              // caller order: class, reduced site active
@@ -92,7 +93,11 @@ __global__ __launch_bounds__(64) void k_vdw_tiles(VdwArgs a)
     __shared__ unsigned short ring[TILE_RING];
     const int lane = threadIdx.x, nv = a.nv, nv2 = nv * nv;
     const int ntiles = *a.ntiles;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int v = blockIdx.x, nitems = ntiles + *a.items.xcnt; v < nitems; v += gridDim.x) {
+        TileItem it;
+        if (!tile_item(v, ntiles, a.items, it)) continue;
+        const int u = it.u;
+        const int t = it.t;
         const int2 IJ = a.tiles[t];
         __syncthreads();
 #pragma unroll
@@ -108,7 +113,8 @@ __global__ __launch_bounds__(64) void k_vdw_tiles(VdwArgs a)
         __syncthreads();
         double e = 0.0;
         Vir6 vir;
-        for (int cnt, ent; (cnt = ps.next(ring, lane, ent)) > 0;) {
+        for (int cnt, ent, b = 0; b < it.b1 && (cnt = ps.next(ring, lane, ent)) > 0; b++) {
+            if (b < it.b0) continue;
             if (lane >= cnt) break;
             const int il = ent >> 8, jl = ent & 63;
             double xd = bx[1][jl] - bx[0][il], yd = by[1][jl] - by[0][il], zd = bz[1][jl] - bz[0][il];
@@ -132,7 +138,7 @@ __global__ __launch_bounds__(64) void k_vdw_tiles(VdwArgs a)
             if (a.vir) vir.pair(xd, yd, zd, -g * xd, -g * yd, -g * zd);
         }
         __syncthreads();
-        T *P = (T *)a.part + (size_t)t * 2 * PMQ_TILE_NV * 64;
+        T *P = (T *)a.part + (size_t)u * 2 * PMQ_TILE_NV * 64;
         T *Q = P + PMQ_TILE_NV * 64;
         for (int c = 0; c < 3; c++) {
             P[c * 64 + lane] = T(acc[0][c][lane]);
@@ -142,8 +148,8 @@ __global__ __launch_bounds__(64) void k_vdw_tiles(VdwArgs a)
         for (int o = 32; o > 0; o >>= 1) ew += __shfl_xor(ew, o, 64);
         if (a.vir) vir.wave_sum();
         if (lane == 0) {
-            a.tscal[t] = ew; // scalar 0 of tile t
-            if (a.vir) vir.store_tile(a.tscal, a.tstride, t);
+            a.tscal[u] = ew; // scalar 0 of item u
+            if (a.vir) vir.store_tile(a.tscal, a.tstride, u);
         }
     }
 }
@@ -237,7 +243,8 @@ int vdw_pass(pmq_ctx *c, bool grad, bool mixed, bool vir, bool safe)
     a.mask = c->xmask.p;
     a.part = c->tpart.p;
     a.tscal = c->tscal.p;
-    a.tstride = (size_t)c->tile_cap;
+    a.tstride = item_stride(c);
+    a.items = tile_items(c);
     a.excl = c->excl.p;
     a.nexcl = (int)c->h_excl.size();
     a.de = c->gvr.p;
@@ -246,8 +253,8 @@ int vdw_pass(pmq_ctx *c, bool grad, bool mixed, bool vir, bool safe)
     a.vir = vir ? 1 : 0;
     {
         ProfScope ps_(c, "vdw_tiles");
-        if (mixed) hipLaunchKernelGGL(k_vdw_tiles<float>, dim3(c->tgrid), dim3(64), 0, c->stream, a);
-        else hipLaunchKernelGGL(k_vdw_tiles<double>, dim3(c->tgrid), dim3(64), 0, c->stream, a);
+        if (mixed) hipLaunchKernelGGL(k_vdw_tiles<float>, dim3(c->igrid), dim3(64), 0, c->stream, a);
+        else hipLaunchKernelGGL(k_vdw_tiles<double>, dim3(c->igrid), dim3(64), 0, c->stream, a);
     }
     PMQ_HIP(hipGetLastError());
     if (grad) {

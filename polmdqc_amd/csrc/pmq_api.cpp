@@ -223,7 +223,7 @@ int pmq_create(const pmq_config *cfg, pmq_ctx **out)
         delete c;
         return PMQ_ERR_HIP;
     }
-    if (c->totals.alloc(1) != hipSuccess || c->bbox.alloc(6) != hipSuccess || c->ntiles_d.alloc(1) != hipSuccess ||
+    if (c->totals.alloc(1) != hipSuccess || c->bbox.alloc(6) != hipSuccess || c->ntiles_d.alloc(2) != hipSuccess ||
         c->ntiles_eff.alloc(1) != hipSuccess || c->status.alloc(1) != hipSuccess ||
         hipHostMalloc((void **)&c->h_status, sizeof(HostStatus), hipHostMallocDefault) != hipSuccess) {
         delete c;

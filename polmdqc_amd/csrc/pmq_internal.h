@@ -191,7 +191,7 @@ struct Vir6 {
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) *v[q] += __shfl_xor(*v[q], o, 64);
     }
-    // tile scalars 1..9 (row-major 3x3) of tile t, scalar q at tscal[q * ts + t]
+    // tile-item scalars 1..9 (row-major 3x3) of item t, scalar q at tscal[q * ts + t]
     __device__ __forceinline__ void store_tile(double *tscal, size_t ts, int t) const
     {
         double *S = tscal + t;
@@ -200,18 +200,24 @@ struct Vir6 {
     }
 };
 
-// per-tile partial sums of the pair kernels (k_pairs.hip): values per lane
-// and side, scalars per tile (0 energy, 1..9 virial, 10 inter-molecular
-// energy, 11 pair count), stored scalar-major (scalar q of tile t at
-// q * tile capacity + t) so each scalar's sum reads one contiguous array
+// per-item partial sums of the pair kernels (k_pairs.hip): values per lane
+// and side, scalars per item (0 energy, 1..9 virial, 10 inter-molecular
+// energy, 11 pair count), stored scalar-major (scalar q of item u at
+// q * item capacity + u) so each scalar's sum reads one contiguous array.
+// An item is one piece of a tile (tile_pairs.h TileItem): a tile with many
+// pairs is split into up to PMQ_TILE_PIECES pieces of about
+// PMQ_PIECE_BATCHES batches of 64 pairs, each run by its own wave.
 constexpr int PMQ_TILE_NV = 8;
 constexpr int PMQ_TILE_NS = 12;
+constexpr int PMQ_TILE_PIECES = 4;
+constexpr int PMQ_PIECE_BATCHES = 6;
 constexpr int PMQ_SHP_ROWS = 8; // rows of the sorted per-atom HIPPO, charge-transfer and activity table
 
 struct TileReduceArgs {
     int nblk, n;
     const int2 *tiles;
     const int *adj, *adjn, *perm;
+    const int *tpc, *ntiles; // piece code of each tile (k_tile_masks), tile count
     const void *part;
     double *out[8];       // per value: destination (or null)
     double *out2[8];      // second destination of the same value (or null)
@@ -304,7 +310,7 @@ struct pmq_ctx {
                                      // cell), then lowest and highest stored coordinate xyz
     pmq::DBuf<int> tileflag, tilescan;
     pmq::DBuf<int2> tiles;
-    pmq::DBuf<int> ntiles_d;
+    pmq::DBuf<int> ntiles_d;                     // [0] tiles listed, [1] further pieces (xlist entries)
     pmq::DBuf<double> bbox;          // 6: min xyz, max xyz (non-periodic systems)
     pmq::DBuf<unsigned char> scratch_sort;
     size_t scratch_sort_bytes = 0;
@@ -343,9 +349,12 @@ struct pmq_ctx {
     pmq::DBuf<unsigned long long> tmask;         // 128 per tile: 64 row masks, 64 column masks
     pmq::DBuf<unsigned long long> xmask;         // 128 per tile: tmask rows and mmask rows without the scaled pairs
     pmq::DBuf<unsigned long long> mmask;         // 64 per tile: row masks of pairs with an active atom
-    pmq::DBuf<double> tpart;                     // per tile: 2 sides x PMQ_TILE_NV x 64 partial sums
-    pmq::DBuf<double> tscal;                     // per tile: PMQ_TILE_NS scalars (energy, virial, ...)
-    int tgrid = 1;                               // workgroups of the tile kernels
+    pmq::DBuf<double> tpart;                     // per tile item: 2 sides x PMQ_TILE_NV x 64 partial sums
+    pmq::DBuf<double> tscal;                     // per tile item: PMQ_TILE_NS scalars (energy, virial, ...)
+    pmq::DBuf<int> tpc;                          // per tile: piece count | batches per piece << 4
+    pmq::DBuf<int> xlist;                        // further pieces of the split tiles (tile | piece << 28)
+    int tgrid = 1;                               // workgroups of the per-tile kernels (masks, parameters)
+    int igrid = 1;                               // workgroups of the per-item pair kernels
     int tile_cap = 0;                            // tile capacity of the tile buffers (0 = not sized yet)
     pmq::DBuf<int> ntiles_eff;                   // min(tile count, tile_cap): what the tile kernels loop over
     pmq::DBuf<pmq::RunStatus> status;
@@ -487,6 +496,8 @@ int launch_torque_pair(pmq_ctx *c, int mode0, const double *te0, double *de0, in
                        double *de1, bool mixed);
 constexpr int PMQ_ZERO_MAX = 16;
 int launch_zero_many(pmq_ctx *c, int k, double *const *p, const size_t *n);
+// scalar stride of the per-item tile scalars (tscal)
+inline size_t item_stride(const pmq_ctx *c) { return (size_t)PMQ_TILE_PIECES * (size_t)c->tile_cap; }
 int pme_setup(pmq_ctx *c);
 int fft_own_setup(pmq_ctx *c);
 bool fft_own_fits(const pmq_ctx *c, bool fp32);

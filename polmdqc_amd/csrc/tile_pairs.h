@@ -27,6 +27,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "pmq_internal.h"
+
 namespace pmq {
 
 // rank-ordered pair list of mask rows m (lane = row); entries row << 8 | col;
@@ -91,5 +93,62 @@ struct TilePairStream {
         return cnt;
     }
 };
+
+// Work items of the pair kernels: pieces of tiles.  A tile's piece count np
+// and batches per piece bpp come from its in-cutoff pair count (k_tile_masks:
+// np = 0 for a tile without pairs, at most PMQ_TILE_PIECES); piece p takes
+// batches [p bpp, (p + 1) bpp) of the tile's rank-ordered schedule, whatever
+// subset of those pairs the kernel's own mask keeps.  Items [0, ntiles) are
+// the first pieces of the tiles in slot order (diagonal tiles leading), then
+// come the further pieces of the heavy tiles, listed by k_tile_masks
+// (xlist, *xcnt entries of tile | piece << 28, in no fixed order: each piece
+// writes only its own partial slots, so the order of the items does not
+// reach the sums).  Piece p of tile t writes partial slot / scalar index
+// p ntiles + t.  One wave per tile made the launch wait on the densest tiles
+// (a diagonal tile holds about 1,200 pairs, the mean tile about 190), each
+// one wave's serial chain of batches while the other SIMDs had drained;
+// pieces of at most about six batches bound that chain.
+struct TileItems {
+    const int *tpc;   // per tile: np | bpp << 4
+    const int *xlist; // further pieces
+    const int *xcnt;  // their count
+};
+
+struct TileItem {
+    int t, p;   // tile and piece
+    int u;      // partial slot / scalar index: p ntiles + t
+    int b0, b1; // the piece's batches of the tile's schedule
+};
+
+__device__ __forceinline__ bool tile_item(int u, int ntiles, const TileItems &ti, TileItem &it)
+{
+    if (u < ntiles) {
+        it.t = u;
+        it.p = 0;
+    } else {
+        const int e = ti.xlist[u - ntiles];
+        it.t = e & ((1 << 28) - 1);
+        it.p = e >> 28;
+    }
+    const int code = ti.tpc[it.t];
+    const int np = code & 15, bpp = code >> 4;
+    if (it.p >= np) return false;
+    it.u = it.p * ntiles + it.t;
+    it.b0 = it.p * bpp;
+    it.b1 = it.b0 + bpp;
+    return true;
+}
+
+// the item lists of a context's current tile structure
+inline TileItems tile_items(const pmq_ctx *c) { return TileItems{c->tpc.p, c->xlist.p, c->ntiles_d.p + 1}; }
+
+// piece code of a tile with npair pairs
+__device__ __forceinline__ int tile_piece_code(int npair)
+{
+    const int nb = (npair + 63) / 64;
+    if (nb == 0) return 0;
+    const int np = min(PMQ_TILE_PIECES, (nb + PMQ_PIECE_BATCHES - 1) / PMQ_PIECE_BATCHES);
+    return np | (((nb + np - 1) / np) << 4);
+}
 
 } // namespace pmq
