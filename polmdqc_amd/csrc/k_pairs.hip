@@ -67,6 +67,8 @@ struct MaskArgs {
     unsigned long long *xmask; // 128 per tile: copies of the in-cutoff and active rows (scaled pairs cleared later)
     int *tpc;                  // per tile: piece code of its in-cutoff pair count (tile_pairs.h)
     int *xlist, *xcnt;         // further pieces of the split tiles (tile | piece << 28), their count
+    double *tscal;             // tile-item scalars: zeroed here for the tiles without pairs (no item of theirs runs)
+    size_t tstride;
 };
 
 // BK 0: no box or a rectangular one (one image per tile where it fits, else
@@ -183,8 +185,10 @@ __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) npair += __shfl_xor(npair, o, 64);
         if (lane == 0) {
-            const int code = tile_piece_code(npair), np = code & 15;
+            const int code = tile_piece_code(npair, ntiles <= PMQ_SPLIT_TILES ? PMQ_TILE_PIECES : 1), np = code & 15;
             a.tpc[t] = code;
+            if (np == 0)
+                for (int q = 0; q < PMQ_TILE_NS; q++) a.tscal[(size_t)q * a.tstride + t] = 0.0;
             if (np > 1) {
                 const int k = atomicAdd(a.xcnt, np - 1);
                 for (int p = 1; p < np; p++) a.xlist[k + p - 1] = t | (p << 28);
@@ -672,6 +676,8 @@ int tile_structure(pmq_ctx *c, double off2, bool safe)
     ma.tpc = c->tpc.p;
     ma.xlist = c->xlist.p;
     ma.xcnt = c->ntiles_d.p + 1;
+    ma.tscal = c->tscal.p;
+    ma.tstride = item_stride(c);
     if (c->box.kind == PMQ_BOX_NONE || c->box.kind == PMQ_BOX_ORTHO)
         hipLaunchKernelGGL(k_tile_masks<0>, dim3(c->tgrid), dim3(64), 0, st, ma);
     else
@@ -847,8 +853,9 @@ __global__ __launch_bounds__(64 * RED_W) void k_tile_reduce(TileReduceArgs a)
 constexpr int SCAL_TB = 1024;
 
 // tile-item scalars k0..k1-1 of every item into the totals (0 energy, 1..9
-// virial, 10 inter-molecular energy, 11 pair count); items past their
-// tile's piece count wrote nothing and are skipped
+// virial, 10 inter-molecular energy, 11 pair count): the first pieces of all
+// tiles (a tile without pairs has zeros there from the mask kernel), each
+// tile's further pieces after its first
 __global__ __launch_bounds__(SCAL_TB) void k_tile_scalars(const int *ntiles_p, const int *__restrict__ tpc, int which,
                                                           int k0, int k1, const double *__restrict__ tscal,
                                                           size_t stride, Totals *tot)
@@ -859,10 +866,12 @@ __global__ __launch_bounds__(SCAL_TB) void k_tile_scalars(const int *ntiles_p, c
         // each thread's tiles in order, a tile's pieces together (item p
         // ntiles + t is piece p of tile t)
         double v = 0.0;
-#pragma unroll 4
+        const double *S = tscal + (size_t)k * stride;
+#pragma unroll 8
         for (int t = threadIdx.x; t < ntiles; t += SCAL_TB) {
             const int np = tpc[t] & 15;
-            for (int p = 0; p < np; p++) v += tscal[(size_t)k * stride + (size_t)p * ntiles + t];
+            v += S[t];
+            for (int p = 1; p < np; p++) v += S[(size_t)p * ntiles + t];
         }
         red[threadIdx.x] = v;
         __syncthreads();

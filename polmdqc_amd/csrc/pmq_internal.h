@@ -211,6 +211,7 @@ constexpr int PMQ_TILE_NV = 8;
 constexpr int PMQ_TILE_NS = 12;
 constexpr int PMQ_TILE_PIECES = 4;
 constexpr int PMQ_PIECE_BATCHES = 6;
+constexpr int PMQ_SPLIT_TILES = 16384; // tile lists longer than this are not split
 constexpr int PMQ_SHP_ROWS = 8; // rows of the sorted per-atom HIPPO, charge-transfer and activity table
 
 struct TileReduceArgs {

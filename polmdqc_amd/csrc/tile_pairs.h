@@ -107,7 +107,11 @@ struct TilePairStream {
 // p ntiles + t.  One wave per tile made the launch wait on the densest tiles
 // (a diagonal tile holds about 1,200 pairs, the mean tile about 190), each
 // one wave's serial chain of batches while the other SIMDs had drained;
-// pieces of at most about six batches bound that chain.
+// pieces of at most about six batches bound that chain.  That tail matters
+// only while a launch has few tiles per SIMD: above PMQ_SPLIT_TILES tiles
+// no tile is split (measured: 4,503 tiles of a 12,288-atom box, the tile
+// kernels 20-35% faster split; 36,000 tiles of a 96,000-atom box, no gain,
+// and the extra partial slots cost the reduction 10%).
 struct TileItems {
     const int *tpc;   // per tile: np | bpp << 4
     const int *xlist; // further pieces
@@ -142,12 +146,12 @@ __device__ __forceinline__ bool tile_item(int u, int ntiles, const TileItems &ti
 // the item lists of a context's current tile structure
 inline TileItems tile_items(const pmq_ctx *c) { return TileItems{c->tpc.p, c->xlist.p, c->ntiles_d.p + 1}; }
 
-// piece code of a tile with npair pairs
-__device__ __forceinline__ int tile_piece_code(int npair)
+// piece code of a tile with npair pairs, at most maxp pieces
+__device__ __forceinline__ int tile_piece_code(int npair, int maxp)
 {
     const int nb = (npair + 63) / 64;
     if (nb == 0) return 0;
-    const int np = min(PMQ_TILE_PIECES, (nb + PMQ_PIECE_BATCHES - 1) / PMQ_PIECE_BATCHES);
+    const int np = min(maxp, (nb + PMQ_PIECE_BATCHES - 1) / PMQ_PIECE_BATCHES);
     return np | (((nb + np - 1) / np) << 4);
 }
 
