@@ -126,6 +126,13 @@ struct RotArgs {
     double *rpole13;                  // 13n caller order or null
     double *rotm;                     // 9n caller order: rows x, y, z of the local frame (or null)
     int *flip;                        // n: 1 where chkpole inverted the y-dependent components (or null)
+    // with the rotation, per sorted site (null: not wanted): the sorted
+    // polarization parameters (polarity, thole, pdamp; 3 * npad) and the
+    // permanent field zeroed (6 * npad)
+    const double *polarity, *thole, *pdamp;
+    double *spol, *pfield;
+    int nrot;                         // workgroups of the rotation; those past it clear the scaled pairs
+    XclearArgs xc;                    // from the tile masks, one thread each
 };
 
 __device__ __forceinline__ void dvec(const RotArgs &a, int from, int to, double *d)
@@ -139,9 +146,20 @@ __device__ __forceinline__ double dot3(const double *u, const double *v) { retur
 
 __global__ void k_rotpole(RotArgs a)
 {
+    if ((int)blockIdx.x >= a.nrot) {
+        xmask_clear_one(a.xc, ((int)blockIdx.x - a.nrot) * (int)blockDim.x + (int)threadIdx.x);
+        return;
+    }
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.npad) return;
     const int i = s < a.n ? a.perm[s] : -1;
+    if (a.spol) {
+        a.spol[s] = i >= 0 ? a.polarity[i] : 0.0;
+        a.spol[a.npad + s] = i >= 0 ? a.thole[i] : 0.0;
+        a.spol[2 * a.npad + s] = i >= 0 ? a.pdamp[i] : 0.0;
+    }
+    if (a.pfield)
+        for (int c = 0; c < 6; c++) a.pfield[(size_t)c * a.npad + s] = 0.0;
     if (i < 0) {
         for (int c = 0; c < 10; c++) a.smp[c * a.npad + s] = 0.0;
         for (int c = 0; c < 3; c++) a.scp[c * a.npad + s] = 0.0;
@@ -297,7 +315,7 @@ __global__ void k_rotpole(RotArgs a)
         }
 }
 
-int launch_rotpole(pmq_ctx *c, bool want_rpole13)
+int launch_rotpole(pmq_ctx *c, bool want_rpole13, bool polar_sites, bool clear_scaled)
 {
     RotArgs a;
     a.n = c->n;
@@ -314,8 +332,14 @@ int launch_rotpole(pmq_ctx *c, bool want_rpole13)
     a.rpole13 = want_rpole13 ? c->rpole13.p : nullptr;
     a.rotm = c->rotm.p;
     a.flip = c->flip.p;
+    a.polarity = c->polarity.p; a.thole = c->thole.p; a.pdamp = c->pdamp.p;
+    a.spol = polar_sites ? c->spol.p : nullptr;
+    a.pfield = polar_sites ? c->pfield.p : nullptr;
+    a.xc = xclear_args(c);
     const int tb = 128;
-    hipLaunchKernelGGL(k_rotpole, dim3((c->npad + tb - 1) / tb), dim3(tb), 0, c->stream, a);
+    a.nrot = (c->npad + tb - 1) / tb;
+    const int nx = clear_scaled ? (a.xc.nexcl + tb - 1) / tb : 0;
+    hipLaunchKernelGGL(k_rotpole, dim3(a.nrot + nx), dim3(tb), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }

@@ -133,33 +133,54 @@ __global__ void k_cell_keys(int n, DevBox b, const double *__restrict__ bbox, in
 
 // with PME on: the cells are the spreading bricks (home_brick, the same
 // arithmetic the spreading kernels use), so every brick's atoms end up
-// contiguous in the sorted order
+// contiguous in the sorted order; the bricks' sorted-atom ranges are
+// cleared here (empty) and set by k_gather
 __global__ void k_brick_keys(int n, BrickGeom g, int bits, const double *__restrict__ x, const double *__restrict__ y,
-                             const double *__restrict__ z, unsigned long long *__restrict__ keys)
+                             const double *__restrict__ z, unsigned long long *__restrict__ keys,
+                             int *__restrict__ bstart, int *__restrict__ bend)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nbricks = g.nb[0] * g.nb[1] * g.nb[2];
+    for (int q = i; q < nbricks; q += gridDim.x * blockDim.x) bstart[q] = bend[q] = 0;
     if (i >= n) return;
     int hb[3];
     home_brick(g.A, g.K, g.nb, x[i], y[i], z[i], hb);
     keys[i] = ((unsigned long long)hilbert3(hb[0], hb[1], hb[2], bits) << 32) | (unsigned)i;
 }
 
-// permutation, sorted positions and per-atom flags (bit0 valid, bit1 active)
+// permutation, sorted positions and per-atom flags (bit0 valid, bit1 active);
+// with PME bricks (bstart non-null) also the sorted-atom range [bstart,
+// bend) of every non-empty brick, from where the brick part of the sorted
+// keys changes (the brick's lexicographic index from the atom's home brick)
 __global__ void k_gather(int n, int npad, const unsigned long long *__restrict__ skeys, const double *__restrict__ x,
                          const double *__restrict__ y, const double *__restrict__ z, const int *__restrict__ usef,
                          const int *__restrict__ mol, int *__restrict__ perm, int *__restrict__ iperm,
                          double *__restrict__ sx, double *__restrict__ sy, double *__restrict__ sz,
-                         int *__restrict__ sflag, int *__restrict__ smol)
+                         int *__restrict__ sflag, int *__restrict__ smol, BrickGeom g, int *__restrict__ bstart,
+                         int *__restrict__ bend)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= npad) return;
     if (s < n) {
-        const int i = (int)(skeys[s] & 0xffffffffu);
+        const unsigned long long key = skeys[s];
+        const int i = (int)(key & 0xffffffffu);
         perm[s] = i;
         iperm[i] = s;
         sx[s] = x[i];
         sy[s] = y[i];
         sz[s] = z[i];
+        if (bstart) {
+            const unsigned h = (unsigned)(key >> 32);
+            const bool first = s == 0 || (unsigned)(skeys[s - 1] >> 32) != h;
+            const bool last = s == n - 1 || (unsigned)(skeys[s + 1] >> 32) != h;
+            if (first || last) {
+                int hb[3];
+                home_brick(g.A, g.K, g.nb, x[i], y[i], z[i], hb);
+                const int b = (hb[2] * g.nb[1] + hb[1]) * g.nb[0] + hb[0];
+                if (first) bstart[b] = s;
+                if (last) bend[b] = s + 1;
+            }
+        }
         sflag[s] = 1 | (usef[i] ? 2 : 0);
         smol[s] = mol[i];
     } else {
@@ -327,15 +348,22 @@ int build_neighbors(pmq_ctx *c, double cutoff, const double *px, const double *p
     int mx = 1;
     for (int a = 0; a < 3; a++) mx = std::max(mx, c->pme_cells ? c->pme_nb[a] : nc[a]);
     const int bits = hilbert_bits(mx);
+    BrickGeom g{};
+    int *bstart = nullptr, *bend = nullptr;
     if (c->pme_cells) {
-        BrickGeom g;
         for (int a = 0; a < 3; a++) {
             for (int q = 0; q < 3; q++) g.A[a][q] = c->pmeA[a][q];
             g.K[a] = c->K[a];
             g.nb[a] = c->pme_nb[a];
         }
+        bstart = c->pme_bstart.p;
+        bend = c->pme_bend.p;
+        if (!bstart || !bend) {
+            c->err = "build_neighbors: PME bricks not set up";
+            return PMQ_ERR_STATE;
+        }
         hipLaunchKernelGGL(k_brick_keys, dim3((n + tb - 1) / tb), dim3(tb), 0, st, n, g, bits, px, py, pz,
-                           c->keys.p);
+                           c->keys.p, bstart, bend);
     } else {
         hipLaunchKernelGGL(k_cell_keys, dim3((n + tb - 1) / tb), dim3(tb), 0, st, n, b, c->bbox.p, nc[0], nc[1],
                            nc[2], bits, px, py, pz, c->keys.p);
@@ -355,7 +383,8 @@ int build_neighbors(pmq_ctx *c, double cutoff, const double *px, const double *p
     PMQ_HIP(rocprim::radix_sort_keys(c->scratch_sort.p, bytes, c->keys.p, c->keys_sorted.p, (size_t)n, 32u, end_bit,
                                      st));
     hipLaunchKernelGGL(k_gather, dim3((npad + tb - 1) / tb), dim3(tb), 0, st, n, npad, c->keys_sorted.p, px, py, pz,
-                       use, c->molcule.p, c->perm.p, c->iperm.p, c->sx.p, c->sy.p, c->sz.p, c->sflag.p, c->smol.p);
+                       use, c->molcule.p, c->perm.p, c->iperm.p, c->sx.p, c->sy.p, c->sz.p, c->sflag.p, c->smol.p, g,
+                       bstart, bend);
     hipLaunchKernelGGL(k_block_bounds, dim3(nblk), dim3(PMQ_BLOCK), 0, st, n, nblk, b, c->bbox.p, c->sx.p, c->sy.p,
                        c->sz.p, c->blk.p);
     const long long npairs = (long long)nblk * (nblk + 1) / 2;

@@ -23,10 +23,10 @@
 // Build, all per evaluation on the library's stream:
 //   k_tile_masks   per (I,J) tile: 64 row masks (bit j of lane i: pair i,j
 //                  within the cutoff) and the transposed column masks, the
-//                  scaled-pair mask rows, and the tile count clamped to the
-//                  buffers' capacity
-//   k_block_adj    per block: the tiles it takes part in, ordered by the
-//                  other block's index
+//                  scaled-pair mask rows, the tile's pieces (tile_pairs.h)
+//                  and the tile count clamped to the buffers' capacity; in
+//                  the same launch, per block: the tiles it takes part in,
+//                  ordered by the other block's index
 //   k_row_counts   per block: each lane's entry offset within its row for
 //                  every tile of its list, and the row length
 //   (exclusive scan of row lengths -> row pointers)
@@ -91,13 +91,14 @@ __device__ __forceinline__ unsigned long long bit_transpose64(unsigned long long
     return x;
 }
 
+// the masks of tiles wg, wg + nwg, ... (one wave each)
 template <int BK>
-__global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
+__device__ __forceinline__ void tile_masks_body(const MaskArgs &a, int wg, int nwg)
 {
     const int lane = threadIdx.x;
     const int nlisted = *a.ntiles;
     const int ntiles = min(nlisted, a.cap);
-    if (blockIdx.x == 0 && lane == 0) {
+    if (wg == 0 && lane == 0) {
         *a.ntiles_eff = ntiles;
         a.st->ntiles = nlisted;
         if (nlisted > a.cap) a.st->overflow |= 1;
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
     __shared__ double jx[64], jy[64], jz[64];
     __shared__ int jflag[64];
     const double L[3] = {a.box.xbox, a.box.ybox, a.box.zbox};
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = wg; t < ntiles; t += nwg) {
         const int2 IJ = a.tiles[t];
         const int si = IJ.x * 64 + lane, sj = IJ.y * 64 + lane;
         // One image for the whole tile where every pair's offset v = xj - xi
@@ -199,11 +200,18 @@ __global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a)
 
 // tiles of block I in the order of the other block J (one wave per block);
 // tiles beyond the capacity are left out (the evaluation is then redone)
-__global__ __launch_bounds__(64) void k_block_adj(int nblk, int cap, const int *__restrict__ flag,
-                                                  const int *__restrict__ scan, int *__restrict__ adj,
-                                                  int *__restrict__ adjn)
+struct AdjArgs {
+    int nblk, cap;
+    const int *flag, *scan; // kept block pairs and their exclusive count, in (I, J) order
+    int *adj, *adjn;
+};
+
+__device__ __forceinline__ void block_adj_body(const AdjArgs &q, int I)
 {
-    const int I = blockIdx.x, lane = threadIdx.x;
+    const int nblk = q.nblk, cap = q.cap;
+    const int *flag = q.flag, *scan = q.scan;
+    int *adj = q.adj, *adjn = q.adjn;
+    const int lane = threadIdx.x;
     int count = 0;
     for (int base = 0; base < nblk; base += 64) {
         const int J = base + lane;
@@ -220,6 +228,15 @@ __global__ __launch_bounds__(64) void k_block_adj(int nblk, int cap, const int *
         count += __popcll(b);
     }
     if (lane == 0) adjn[I] = count;
+}
+
+// one launch: workgroups [0, nblk) the blocks' tile lists, the rest the
+// tile masks (independent: both need only the tile list)
+template <int BK>
+__global__ __launch_bounds__(64) void k_tile_masks(MaskArgs a, AdjArgs q)
+{
+    if ((int)blockIdx.x < q.nblk) block_adj_body(q, blockIdx.x);
+    else tile_masks_body<BK>(a, (int)blockIdx.x - q.nblk, (int)gridDim.x - q.nblk);
 }
 
 // the mask of lane `lane` of block I for tile t, in the orientation of I
@@ -598,43 +615,10 @@ static int pair_fill(pmq_ctx *c, int tgrid)
     return PMQ_OK;
 }
 
-// Rows of the tile masks without the scaled pairs (the 1-2..1-5 and
-// polarization-group pairs): [0,64) the in-cutoff rows, [64,128) the rows
-// with an active atom, written by k_tile_masks and cleared of the scaled
-// pairs here.  The pair kernels skip those pairs and the exclusion
-// kernels evaluate them whole, at their scales, in FP64: large intramolecular
-// pair terms are then never formed in one precision and cancelled in another.
-__global__ void k_xmask_clear(int nexcl, const ExclPair *__restrict__ excl, const int *__restrict__ iperm,
-                              const int2 *__restrict__ tiles, const int *__restrict__ adj,
-                              const int *__restrict__ adjn, int nblk, const int *ntiles,
-                              unsigned long long *__restrict__ xmask)
-{
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nexcl) return;
-    const int s1 = iperm[excl[q].i], s2 = iperm[excl[q].k];
-    const int a = min(s1, s2), b = max(s1, s2);
-    const int I = a >> 6, J = b >> 6;
-    // the tile (I, J) in I's list, which is ordered by the other block
-    const int *lst = adj + (size_t)I * nblk;
-    int lo = 0, hi = adjn[I];
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        const int2 IJ = tiles[lst[mid]];
-        const int other = IJ.x == I ? IJ.y : IJ.x;
-        if (other < J) lo = mid + 1; else hi = mid;
-    }
-    if (lo >= adjn[I]) return;
-    const int t = lst[lo];
-    const int2 IJ = tiles[t];
-    if ((IJ.x == I ? IJ.y : IJ.x) != J || t >= *ntiles) return; // beyond the cutoff: no tile holds it
-    // row: the atom of block IJ.x (the smaller one on a diagonal tile)
-    const int row = IJ.x == I ? (a & 63) : (b & 63), bit = IJ.x == I ? (b & 63) : (a & 63);
-    const unsigned long long clr = ~(1ull << bit);
-    atomicAnd(&xmask[(size_t)t * 128 + row], clr);
-    atomicAnd(&xmask[(size_t)t * 128 + 64 + row], clr);
-}
+// the scaled pairs cleared from the tile masks (tile_pairs.h), one thread each
+__global__ void k_xmask_clear(XclearArgs x) { xmask_clear_one(x, blockIdx.x * blockDim.x + threadIdx.x); }
 
-int tile_structure(pmq_ctx *c, double off2, bool safe)
+int tile_structure(pmq_ctx *c, double off2, bool safe, bool clear_scaled)
 {
     const int nblk = c->nblk;
     hipStream_t st = c->stream;
@@ -678,16 +662,14 @@ int tile_structure(pmq_ctx *c, double off2, bool safe)
     ma.xcnt = c->ntiles_d.p + 1;
     ma.tscal = c->tscal.p;
     ma.tstride = item_stride(c);
+    const AdjArgs q{nblk, c->tile_cap, c->tileflag.p, c->tilescan.p, c->adj.p, c->adjn.p};
     if (c->box.kind == PMQ_BOX_NONE || c->box.kind == PMQ_BOX_ORTHO)
-        hipLaunchKernelGGL(k_tile_masks<0>, dim3(c->tgrid), dim3(64), 0, st, ma);
+        hipLaunchKernelGGL(k_tile_masks<0>, dim3(nblk + c->tgrid), dim3(64), 0, st, ma, q);
     else
-        hipLaunchKernelGGL(k_tile_masks<1>, dim3(c->tgrid), dim3(64), 0, st, ma);
-    hipLaunchKernelGGL(k_block_adj, dim3(nblk), dim3(64), 0, st, nblk, c->tile_cap, c->tileflag.p, c->tilescan.p,
-                       c->adj.p, c->adjn.p);
+        hipLaunchKernelGGL(k_tile_masks<1>, dim3(nblk + c->tgrid), dim3(64), 0, st, ma, q);
     const int nexcl = (int)c->h_excl.size();
-    if (nexcl)
-        hipLaunchKernelGGL(k_xmask_clear, dim3((nexcl + 255) / 256), dim3(256), 0, st, nexcl, c->excl.p, c->iperm.p,
-                           c->tiles.p, c->adj.p, c->adjn.p, nblk, c->ntiles_eff.p, c->xmask.p);
+    if (nexcl && clear_scaled)
+        hipLaunchKernelGGL(k_xmask_clear, dim3((nexcl + 255) / 256), dim3(256), 0, st, xclear_args(c));
     PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }

@@ -283,28 +283,6 @@ __device__ __forceinline__ void pick(const T th[PMQ_MAX_ORDER][4], int k, T *out
         }
 }
 
-// sorted-atom range of every brick: the keys of the atom sort are
-// (Hilbert index of the home brick << 32) | atom, ascending
-__global__ void k_pme_brick_ranges(int n, int nb0, int nb1, int nb2, int bits, const unsigned long long *keys,
-                                   int *bstart, int *bend)
-{
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb0 * nb1 * nb2) return;
-    const unsigned m = hilbert3(b % nb0, (b / nb0) % nb1, b / (nb0 * nb1), bits);
-    const unsigned long long k0 = (unsigned long long)m << 32, k1 = (unsigned long long)(m + 1) << 32;
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (keys[mid] < k0) lo = mid + 1; else hi = mid;
-    }
-    bstart[b] = lo;
-    hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (keys[mid] < k1) lo = mid + 1; else hi = mid;
-    }
-    bend[b] = lo;
-}
 
 // the standalone spreading launch (body in pme_spread.h)
 template <int ORD, int MODE, class T, class GT = T>
@@ -1033,20 +1011,14 @@ static bool gather_planes(const pmq_ctx *c)
         }                                                                                                  \
     }
 
-// the sorted-atom range of every brick (once per evaluation; the atom sort
-// of this evaluation ordered the atoms by home brick)
+// the sorted-atom range of every brick comes with this evaluation's atom
+// sort (k_gather, when the sort followed the spreading bricks)
 static int pme_brick_ranges(pmq_ctx *c)
 {
     if (!c->pme_cells) {
         c->err = "PME: the atoms were not sorted by spreading brick";
         return PMQ_ERR_STATE;
     }
-    const int nbricks = c->pme_nb[0] * c->pme_nb[1] * c->pme_nb[2];
-    // the same curve as the atom sort's keys (build_neighbors)
-    const int bits = hilbert_bits(std::max(c->pme_nb[0], std::max(c->pme_nb[1], c->pme_nb[2])));
-    hipLaunchKernelGGL(k_pme_brick_ranges, dim3((nbricks + 255) / 256), dim3(256), 0, c->stream, c->n, c->pme_nb[0],
-                       c->pme_nb[1], c->pme_nb[2], bits, c->keys_sorted.p, c->pme_bstart.p, c->pme_bend.p);
-    PMQ_HIP(hipGetLastError());
     return PMQ_OK;
 }
 

@@ -546,18 +546,6 @@ __global__ __launch_bounds__(64) void k_pol_grad_excl(PolArgs a)
     }
 }
 
-// ---------------------------------------------------------------------------
-// sorted polarization parameters
-__global__ void k_pol_sort(int n, int npad, const int *perm, const double *pol, const double *thl, const double *pdm,
-                           double *spol)
-{
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= npad) return;
-    const int i = s < n ? perm[s] : -1;
-    spol[s] = i >= 0 ? pol[i] : 0.0;
-    spol[npad + s] = i >= 0 ? thl[i] : 0.0;
-    spol[2 * npad + s] = i >= 0 ? pdm[i] : 0.0;
-}
 
 // ---------------------------------------------------------------------------
 // preconditioned conjugate gradient on (1/alpha - T) mu = E, both sets at
@@ -719,13 +707,11 @@ static int mutual_field(pmq_ctx *c, const double *src, double *dst, const double
 
 int polar_prepare(pmq_ctx *c, double off2, bool mixed, bool safe)
 {
-    // sorted parameters, zeroed permanent field, real-space permanent field
+    // the real-space permanent field (the sorted parameters and the zeroed
+    // field come with the multipole rotation: launch_rotpole)
     const int np = c->npad;
-    hipLaunchKernelGGL(k_pol_sort, dim3((np + 255) / 256), dim3(256), 0, c->stream, c->n, np, c->perm.p,
-                       c->polarity.p, c->thole.p, c->pdamp.p, c->spol.p);
     int rc;
     if ((rc = pair_matrix_build(c, mixed, safe))) return rc;
-    if ((rc = launch_zero(c, c->pfield.p, 6 * (size_t)np))) return rc;
     PolArgs a = pol_args(c, off2);
     a.dst = c->pfield.p;
     a.dstq = mixed ? c->fx_field.p : nullptr;

@@ -856,9 +856,11 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
         {
             ProfScope ps(c, "neighbors");
             if ((rc = build_neighbors(c, cut))) return rc;
-            if ((rc = tile_structure(c, off2, safe))) return rc;
+            if ((rc = tile_structure(c, off2, safe, false))) return rc;
         }
-        if ((rc = launch_rotpole(c, true))) return rc;
+        // with the rotation: the sorted polarization parameters, the zeroed
+        // permanent field and the scaled pairs cleared from the tile masks
+        if ((rc = launch_rotpole(c, true, do_polar, true))) return rc;
         if ((do_rep || do_disp || do_ct) && (rc = hippo_sort(c))) return rc;
         if (do_rep) {
             const double rc2 = std::min(c->prm.repcut * c->prm.repcut, off2);

@@ -487,7 +487,10 @@ namespace pmq {
 // positions and active flags default to the atoms' (x, y, z, usef)
 int build_neighbors(pmq_ctx *c, double cutoff, const double *x = nullptr, const double *y = nullptr,
                     const double *z = nullptr, const int *use = nullptr);
-int launch_rotpole(pmq_ctx *c, bool want_rpole13);
+// polar_sites: also the sorted polarization parameters and the zeroed
+// permanent field; clear_scaled: also the scaled pairs cleared from the tile
+// masks (after tile_structure(..., false))
+int launch_rotpole(pmq_ctx *c, bool want_rpole13, bool polar_sites = false, bool clear_scaled = false);
 int launch_chgflx_delta(pmq_ctx *c);
 int launch_mpole_direct(pmq_ctx *c, int mode, double off2, bool mixed);
 int launch_fx_flush(pmq_ctx *c, long long *q, double *d, size_t n, double scale);
@@ -524,7 +527,9 @@ int polar_finish(pmq_ctx *c, double *ud_out, double *up_out, double *aep = nullp
 // (potp -> dep) gradients; mixed: fixed point, flushed
 int launch_chgflx_grad(pmq_ctx *c, int mode, bool mpole, bool polar, bool mixed);
 int launch_zero(pmq_ctx *c, double *p, size_t n);
-int tile_structure(pmq_ctx *c, double off2, bool safe);
+// clear_scaled false: the caller clears the scaled pairs from the tile masks
+// itself (with the multipole rotation, launch_rotpole)
+int tile_structure(pmq_ctx *c, double off2, bool safe, bool clear_scaled = true);
 int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe);
 int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fixed);
 int tile_scalars(pmq_ctx *c, int which, bool full);

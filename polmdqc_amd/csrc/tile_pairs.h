@@ -143,6 +143,56 @@ __device__ __forceinline__ bool tile_item(int u, int ntiles, const TileItems &ti
     return true;
 }
 
+// Rows of the tile masks without the scaled pairs (the 1-2..1-5 and
+// polarization-group pairs): [0,64) the in-cutoff rows, [64,128) the rows
+// with an active atom, written by k_tile_masks and cleared of the scaled
+// pairs after it (one thread per scaled pair).  The pair kernels skip those
+// pairs and the exclusion kernels evaluate them whole, at their scales, in
+// FP64: large intramolecular pair terms are then never formed in one
+// precision and cancelled in another.
+struct XclearArgs {
+    int nexcl;
+    const ExclPair *excl;
+    const int *iperm;
+    const int2 *tiles;
+    const int *adj, *adjn; // each block's tiles, ordered by the other block
+    int nblk;
+    const int *ntiles;
+    unsigned long long *xmask;
+};
+
+__device__ __forceinline__ void xmask_clear_one(const XclearArgs &x, int q)
+{
+    if (q >= x.nexcl) return;
+    const int s1 = x.iperm[x.excl[q].i], s2 = x.iperm[x.excl[q].k];
+    const int a = min(s1, s2), b = max(s1, s2);
+    const int I = a >> 6, J = b >> 6;
+    // the tile (I, J) in I's list, which is ordered by the other block
+    const int *lst = x.adj + (size_t)I * x.nblk;
+    int lo = 0, hi = x.adjn[I];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int2 IJ = x.tiles[lst[mid]];
+        const int other = IJ.x == I ? IJ.y : IJ.x;
+        if (other < J) lo = mid + 1; else hi = mid;
+    }
+    if (lo >= x.adjn[I]) return;
+    const int t = lst[lo];
+    const int2 IJ = x.tiles[t];
+    if ((IJ.x == I ? IJ.y : IJ.x) != J || t >= *x.ntiles) return; // beyond the cutoff: no tile holds it
+    // row: the atom of block IJ.x (the smaller one on a diagonal tile)
+    const int row = IJ.x == I ? (a & 63) : (b & 63), bit = IJ.x == I ? (b & 63) : (a & 63);
+    const unsigned long long clr = ~(1ull << bit);
+    atomicAnd(&x.xmask[(size_t)t * 128 + row], clr);
+    atomicAnd(&x.xmask[(size_t)t * 128 + 64 + row], clr);
+}
+
+inline XclearArgs xclear_args(const pmq_ctx *c)
+{
+    return XclearArgs{(int)c->h_excl.size(), c->excl.p, c->iperm.p, c->tiles.p, c->adj.p, c->adjn.p, c->nblk,
+                      c->ntiles_eff.p, c->xmask.p};
+}
+
 // the item lists of a context's current tile structure
 inline TileItems tile_items(const pmq_ctx *c) { return TileItems{c->tpc.p, c->xlist.p, c->ntiles_d.p + 1}; }
 
