@@ -550,6 +550,9 @@ int launch_disp(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir)
             else hipLaunchKernelGGL((k_disp_tiles<false, double>), g, dim3(64), 0, c->stream, a);
         }
     }
+    // the scaled pairs first: the reduction takes their fixed-point share
+    // (mixed form) into its sums, and the tile scalars in the same launch
+    if (a.nexcl) hipLaunchKernelGGL(k_disp_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     int rc;
     if (grad) {
@@ -560,15 +563,12 @@ int launch_disp(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir)
             r.stride[q] = 3;
             r.off[q] = q;
             r.fxscale[q] = PMQ_FX_GRAD;
+            r.fxq[q] = mixed ? c->fx_grad.p : nullptr;
         }
-        if ((rc = tile_reduce(c, r, 3, mixed, mixed))) return rc;
+        return tile_reduce(c, r, 3, mixed, mixed, 3, 0, vir ? 10 : 1);
     }
     if ((rc = tile_scalars(c, 3, false))) return rc;
     if (vir && (rc = tile_virial(c))) return rc;
-    if (a.nexcl) hipLaunchKernelGGL(k_disp_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
-    PMQ_HIP(hipGetLastError());
-    if (mixed && grad && (rc = launch_fx_flush(c, c->fx_grad.p, c->dedsp.p, 3 * (size_t)c->n, PMQ_FX_GRAD)))
-        return rc;
     return PMQ_OK;
 }
 
@@ -587,6 +587,9 @@ int launch_rep(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir)
         if (mixed) hipLaunchKernelGGL((k_rep_tiles<float>), g, dim3(64), 0, c->stream, a);
         else hipLaunchKernelGGL((k_rep_tiles<double>), g, dim3(64), 0, c->stream, a);
     }
+    // the scaled pairs first: the reduction takes their fixed-point share
+    // (mixed form) into its sums, and the tile scalars in the same launch
+    if (a.nexcl) hipLaunchKernelGGL(k_rep_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     int rc;
     if (grad) {
@@ -597,17 +600,13 @@ int launch_rep(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir)
             r.stride[q] = r.stride[3 + q] = 3;
             r.off[q] = r.off[3 + q] = q;
             r.fxscale[q] = r.fxscale[3 + q] = PMQ_FX_GRAD;
+            r.fxq[q] = mixed ? c->fx_grad.p : nullptr;
+            r.fxq[3 + q] = mixed ? c->fx_grad.p + 3 * (size_t)c->n : nullptr;
         }
-        if ((rc = tile_reduce(c, r, 6, mixed, mixed))) return rc;
+        return tile_reduce(c, r, 6, mixed, mixed, 2, 0, vir ? 10 : 1);
     }
     if ((rc = tile_scalars(c, 2, false))) return rc;
     if (vir && (rc = tile_virial(c))) return rc;
-    if (a.nexcl) hipLaunchKernelGGL(k_rep_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
-    PMQ_HIP(hipGetLastError());
-    if (mixed && grad) {
-        if ((rc = launch_fx_flush_pair(c, c->fx_grad.p, c->der.p, c->ter.p, 3 * (size_t)c->n, PMQ_FX_GRAD)))
-            return rc;
-    }
     return PMQ_OK;
 }
 
@@ -627,6 +626,9 @@ int launch_ct(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir)
         if (mixed) hipLaunchKernelGGL((k_ct_tiles<float>), dim3(c->igrid), dim3(64), 0, c->stream, a);
         else hipLaunchKernelGGL((k_ct_tiles<double>), dim3(c->igrid), dim3(64), 0, c->stream, a);
     }
+    // the scaled pairs first: the reduction takes their fixed-point share
+    // (mixed form) into its sums, and the tile scalars in the same launch
+    if (a.nexcl) hipLaunchKernelGGL(k_ct_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
     int rc;
     if (grad) {
@@ -637,15 +639,12 @@ int launch_ct(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir)
             r.stride[q] = 3;
             r.off[q] = q;
             r.fxscale[q] = PMQ_FX_GRAD;
+            r.fxq[q] = mixed ? c->fx_grad.p : nullptr;
         }
-        if ((rc = tile_reduce(c, r, 3, mixed, mixed))) return rc;
+        return tile_reduce(c, r, 3, mixed, mixed, 5, 0, vir ? 10 : 1); // energy component 5: ect
     }
     if ((rc = tile_scalars(c, 5, false))) return rc; // energy component 5: ect
     if (vir && (rc = tile_virial(c))) return rc;
-    if (a.nexcl) hipLaunchKernelGGL(k_ct_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
-    PMQ_HIP(hipGetLastError());
-    if (mixed && grad && (rc = launch_fx_flush(c, c->fx_grad.p, c->dect.p, 3 * (size_t)c->n, PMQ_FX_GRAD)))
-        return rc;
     return PMQ_OK;
 }
 

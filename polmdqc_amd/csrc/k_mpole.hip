@@ -657,12 +657,19 @@ static int launch_mpole_t(pmq_ctx *c, const MpoleArgs &a)
     }
     PMQ_HIP(hipGetLastError());
     const bool mixed = a.demq != nullptr;
-    int rc;
+    // the scaled pairs first: the reduction takes their fixed-point share of
+    // the gradient and torque (mixed form) into its sums, and the tile
+    // scalars in the same launch
+    if (a.nexcl > 0)
+        hipLaunchKernelGGL((k_mpole_excl<PEN, CF, ANA, GRAD, VIR>), dim3((a.nexcl + 63) / 64), dim3(64), 0,
+                           c->stream, a);
+    PMQ_HIP(hipGetLastError());
     if (GRAD || ANA) {
         TileReduceArgs r{};
         for (int q = 0; q < 3; q++) {
             if (GRAD) {
                 r.out[q] = a.dem; r.out[3 + q] = a.te;
+                r.fxq[q] = a.demq; r.fxq[3 + q] = a.teq;
             }
             r.caller_order[q] = r.caller_order[3 + q] = 1;
             r.stride[q] = r.stride[3 + q] = 3;
@@ -672,14 +679,9 @@ static int launch_mpole_t(pmq_ctx *c, const MpoleArgs &a)
         if (GRAD && CF) r.out[6] = a.pot;
         if (ANA) r.out[7] = a.aem;
         for (int q = 6; q < 8; q++) { r.caller_order[q] = 1; r.stride[q] = 1; r.off[q] = 0; r.fxscale[q] = PMQ_FX_GRAD; }
-        if ((rc = tile_reduce(c, r, 8, mixed, mixed))) return rc;
+        return tile_reduce(c, r, 8, mixed, mixed, 0, 0, (VIR || ANA) ? PMQ_TILE_NS : 1);
     }
-    if ((rc = tile_scalars(c, 0, VIR || ANA))) return rc;
-    if (a.nexcl > 0)
-        hipLaunchKernelGGL((k_mpole_excl<PEN, CF, ANA, GRAD, VIR>), dim3((a.nexcl + 63) / 64), dim3(64), 0,
-                           c->stream, a);
-    PMQ_HIP(hipGetLastError());
-    return PMQ_OK;
+    return tile_scalars(c, 0, VIR || ANA);
 }
 
 template <int PEN, bool CF>

@@ -713,6 +713,55 @@ int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe)
 }
 
 // ---------------------------------------------------------------------------
+// fixed-order sums of the per-tile scalars into the totals: scalar 0 into
+// energy component `which` (0 em, 1 ep, 2 er, 3 edsp, 4 ev, 5 ect), 1..9 the
+// virial, 10 the inter-molecular energy and 11 the pair count; the first
+// pieces of all tiles (a tile without pairs has zeros there from the mask
+// kernel), each tile's further pieces after its first.  One workgroup of TB
+// threads (red: TB doubles of LDS).
+constexpr int SCAL_TB = 1024;
+
+template <int TB>
+__device__ __forceinline__ void scalar_sums(int ntiles, const int *__restrict__ tpc, int which, int k0, int k1,
+                                            const double *__restrict__ tscal, size_t stride, Totals *tot, double *red)
+{
+    for (int k = k0; k < k1; k++) {
+        // each thread's tiles in order, a tile's pieces together (item p
+        // ntiles + t is piece p of tile t)
+        double v = 0.0;
+        const double *S = tscal + (size_t)k * stride;
+#pragma unroll 8
+        for (int t = threadIdx.x; t < ntiles; t += TB) {
+            const int np = tpc[t] & 15;
+            v += S[t];
+            for (int p = 1; p < np; p++) v += S[(size_t)p * ntiles + t];
+        }
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int w = TB / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            const double r = red[0];
+            if (k == 0) atomicAdd(&(&tot->em)[which], r);
+            else if (k <= 9) atomicAdd(&tot->vir[k - 1], r);
+            else if (k == 10) atomicAdd(&tot->einter, r);
+            else atomicAdd(&tot->nem, (unsigned long long)__double2ll_rn(r));
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(SCAL_TB) void k_tile_scalars(const int *ntiles_p, const int *__restrict__ tpc, int which,
+                                                          int k0, int k1, const double *__restrict__ tscal,
+                                                          size_t stride, Totals *tot)
+{
+    __shared__ double red[SCAL_TB];
+    scalar_sums<SCAL_TB>(*ntiles_p, tpc, which, k0, k1, tscal, stride, tot, red);
+}
+
+// ---------------------------------------------------------------------------
 // deterministic reduction of tile partials
 
 // per-atom sums of NV partials over the tiles of the atom's block; P is the
@@ -727,6 +776,11 @@ constexpr int RED_W = 8;                            // waves per block
 template <class P, int NV, bool FX>
 __global__ __launch_bounds__(64 * RED_W) void k_tile_reduce(TileReduceArgs a)
 {
+    if ((int)blockIdx.x == a.nblk) { // the scalar sums, when asked for
+        __shared__ double sred[64 * RED_W];
+        scalar_sums<64 * RED_W>(*a.ntiles, a.tpc, a.scal_which, a.scal_k0, a.scal_k1, a.tscal, a.tstride, a.tot, sred);
+        return;
+    }
     __shared__ int lt[RED_MAXT];
     __shared__ int nent;
     __shared__ unsigned long long red[RED_W][NV][64]; // fixed-point sums, or the bits of double sums
@@ -814,71 +868,41 @@ __global__ __launch_bounds__(64 * RED_W) void k_tile_reduce(TileReduceArgs a)
 #pragma unroll
     for (int c = 0; c < NV; c++) {
         if (!want[c]) continue;
-        double v;
+        const size_t idx = a.caller_order[c] ? (size_t)a.stride[c] * a.perm[si] + a.off[c] : (size_t)a.stride[c] * si + a.off[c];
+        double v, v2;
         if (FX) {
             long long sq = 0;
             for (int w = 0; w < RED_W; w++) sq += (long long)red[w][c][lane];
-            v = (double)sq / a.fxscale[c];
+            long long s1 = sq, s2 = sq;
+            if (a.fxq[c]) { s1 += a.fxq[c][idx]; a.fxq[c][idx] = 0; }
+            if (a.fxq2[c]) { s2 += a.fxq2[c][idx]; a.fxq2[c][idx] = 0; }
+            v = (double)s1 / a.fxscale[c];
+            v2 = (double)s2 / a.fxscale[c];
         } else {
             v = 0.0;
             for (int w = 0; w < RED_W; w++) v += __longlong_as_double((long long)red[w][c][lane]);
+            v2 = v;
         }
-        const size_t idx = a.caller_order[c] ? (size_t)a.stride[c] * a.perm[si] + a.off[c] : (size_t)a.stride[c] * si + a.off[c];
         a.out[c][idx] += v;
-        if (a.out2[c]) a.out2[c][idx] += v;
+        if (a.out2[c]) a.out2[c][idx] += v2;
     }
 }
 
-// fixed-order sums of the per-tile scalars into the totals: scalar 0 into
-// energy component `which` (0 em, 1 ep, 2 er, 3 edsp, 4 ev, 5 ect); with `full` also the
-// virial (1..9), the inter-molecular energy (10) and the pair count (11)
-constexpr int SCAL_TB = 1024;
-
-// tile-item scalars k0..k1-1 of every item into the totals (0 energy, 1..9
-// virial, 10 inter-molecular energy, 11 pair count): the first pieces of all
-// tiles (a tile without pairs has zeros there from the mask kernel), each
-// tile's further pieces after its first
-__global__ __launch_bounds__(SCAL_TB) void k_tile_scalars(const int *ntiles_p, const int *__restrict__ tpc, int which,
-                                                          int k0, int k1, const double *__restrict__ tscal,
-                                                          size_t stride, Totals *tot)
-{
-    __shared__ double red[SCAL_TB];
-    const int ntiles = *ntiles_p;
-    for (int k = k0; k < k1; k++) {
-        // each thread's tiles in order, a tile's pieces together (item p
-        // ntiles + t is piece p of tile t)
-        double v = 0.0;
-        const double *S = tscal + (size_t)k * stride;
-#pragma unroll 8
-        for (int t = threadIdx.x; t < ntiles; t += SCAL_TB) {
-            const int np = tpc[t] & 15;
-            v += S[t];
-            for (int p = 1; p < np; p++) v += S[(size_t)p * ntiles + t];
-        }
-        red[threadIdx.x] = v;
-        __syncthreads();
-        for (int w = SCAL_TB / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            const double r = red[0];
-            if (k == 0) atomicAdd(&(&tot->em)[which], r);
-            else if (k <= 9) atomicAdd(&tot->vir[k - 1], r);
-            else if (k == 10) atomicAdd(&tot->einter, r);
-            else atomicAdd(&tot->nem, (unsigned long long)__double2ll_rn(r));
-        }
-        __syncthreads();
-    }
-}
-
-int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fixed)
+int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fixed, int scal_which, int k0, int k1)
 {
     a.nblk = c->nblk; a.n = c->n;
     a.tiles = c->tiles.p; a.adj = c->adj.p; a.adjn = c->adjn.p; a.perm = c->perm.p;
     a.tpc = c->tpc.p; a.ntiles = c->ntiles_eff.p;
     a.part = c->tpart.p;
-    const dim3 g(c->nblk), b(64 * RED_W);
+    a.scal_which = scal_which; a.scal_k0 = k0; a.scal_k1 = k1;
+    a.tscal = c->tscal.p; a.tstride = item_stride(c); a.tot = c->totals.p;
+    if (!fixed)
+        for (int q = 0; q < 8; q++)
+            if (a.fxq[q] || a.fxq2[q]) {
+                c->err = "tile_reduce: fixed-point addends need the fixed-point sums";
+                return PMQ_ERR_STATE;
+            }
+    const dim3 g(c->nblk + (k1 > k0 ? 1 : 0)), b(64 * RED_W);
 #define PMQ_RED(P, NV, FX) hipLaunchKernelGGL((k_tile_reduce<P, NV, FX>), g, b, 0, c->stream, a)
     if (float_parts) {
         if (nv == 3) { if (fixed) PMQ_RED(float, 3, true); else PMQ_RED(float, 3, false); }

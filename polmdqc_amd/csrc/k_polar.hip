@@ -733,18 +733,20 @@ int polar_prepare(pmq_ctx *c, double off2, bool mixed, bool safe)
 #undef PMQ_FIELD
     }
     PMQ_HIP(hipGetLastError());
+    // the scaled pairs first: the reduction takes their fixed-point share of
+    // each set (mixed form) into its sums
+    if (a.nexcl) hipLaunchKernelGGL(k_pol_field_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
+    PMQ_HIP(hipGetLastError());
     TileReduceArgs r{};
     for (int q = 0; q < 3; q++) {
         r.out[q] = c->pfield.p + (size_t)q * np;
         r.out2[q] = c->pfield.p + (size_t)(3 + q) * np;
         r.stride[q] = 1;
         r.fxscale[q] = PMQ_FX_FIELD;
+        r.fxq[q] = mixed ? c->fx_field.p + (size_t)q * np : nullptr;
+        r.fxq2[q] = mixed ? c->fx_field.p + (size_t)(3 + q) * np : nullptr;
     }
-    if ((rc = tile_reduce(c, r, 3, mixed, mixed))) return rc;
-    if (a.nexcl) hipLaunchKernelGGL(k_pol_field_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
-    PMQ_HIP(hipGetLastError());
-    if (mixed && (rc = launch_fx_flush(c, c->fx_field.p, c->pfield.p, 6 * (size_t)np, PMQ_FX_FIELD))) return rc;
-    return PMQ_OK;
+    return tile_reduce(c, r, 3, mixed, mixed);
 }
 
 // safe: iterations go out one at a time, each followed by a host check of
@@ -869,11 +871,14 @@ int polar_gradient(pmq_ctx *c, double off2, bool mixed, bool vir)
         r.out[6] = c->potp.p;
         r.caller_order[6] = 1; r.stride[6] = 1; r.off[6] = 0; r.fxscale[6] = PMQ_FX_GRAD;
     }
-    int rc;
-    if ((rc = tile_reduce(c, r, cf ? 8 : 6, mixed, mixed))) return rc;
-    if (vir && (rc = tile_virial(c))) return rc;
+    for (int q = 0; q < 6; q++) r.fxq[q] = mixed ? c->fx_grad.p + (q < 3 ? 0 : 3 * (size_t)c->n) : nullptr;
+    // the scaled pairs first: the reduction takes their fixed-point share of
+    // the gradient and torque (mixed form) into its sums, and the virial's
+    // tile scalars in the same launch
     if (a.nexcl) hipLaunchKernelGGL(k_pol_grad_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
     PMQ_HIP(hipGetLastError());
+    int rc;
+    if ((rc = tile_reduce(c, r, cf ? 8 : 6, mixed, mixed, 0, vir ? 1 : 0, vir ? 10 : 0))) return rc;
     if (a.aewald > 0) return pme_polar_grad(c, vir);
     return PMQ_OK;
 }

@@ -257,6 +257,10 @@ int vdw_pass(pmq_ctx *c, bool grad, bool mixed, bool vir, bool safe)
         else hipLaunchKernelGGL(k_vdw_tiles<double>, dim3(c->igrid), dim3(64), 0, c->stream, a);
     }
     PMQ_HIP(hipGetLastError());
+    // the scaled pairs first: the reduction takes their fixed-point share
+    // (mixed form) into its sums, and the tile scalars in the same launch
+    if (a.nexcl) hipLaunchKernelGGL(k_vdw_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
+    PMQ_HIP(hipGetLastError());
     if (grad) {
         TileReduceArgs r{};
         for (int q = 0; q < 3; q++) {
@@ -265,15 +269,14 @@ int vdw_pass(pmq_ctx *c, bool grad, bool mixed, bool vir, bool safe)
             r.stride[q] = 3;
             r.off[q] = q;
             r.fxscale[q] = PMQ_FX_GRAD;
+            r.fxq[q] = mixed ? c->fx_grad.p : nullptr;
         }
-        if ((rc = tile_reduce(c, r, 3, mixed, mixed))) return rc;
+        if ((rc = tile_reduce(c, r, 3, mixed, mixed, 4, 0, vir ? 10 : 1))) return rc; // energy component 4: ev
+    } else {
+        if ((rc = tile_scalars(c, 4, false))) return rc; // energy component 4: ev
+        if (vir && (rc = tile_virial(c))) return rc;
     }
-    if ((rc = tile_scalars(c, 4, false))) return rc; // energy component 4: ev
-    if (vir && (rc = tile_virial(c))) return rc;
-    if (a.nexcl) hipLaunchKernelGGL(k_vdw_excl, dim3((a.nexcl + 63) / 64), dim3(64), 0, c->stream, a);
-    PMQ_HIP(hipGetLastError());
     if (grad) {
-        if (mixed && (rc = launch_fx_flush(c, c->fx_grad.p, c->gvr.p, 3 * (size_t)n, PMQ_FX_GRAD))) return rc;
         hipLaunchKernelGGL(k_vdw_split, dim3((n + tb - 1) / tb), dim3(tb), 0, c->stream, n, c->ired.p, c->kred.p,
                            c->vkid_off.p, c->vkid.p, c->gvr.p, c->dev.p);
         PMQ_HIP(hipGetLastError());

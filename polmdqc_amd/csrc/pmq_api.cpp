@@ -877,22 +877,13 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
         }
         if (do_polar && (rc = polar_prepare(c, off2, mixed, safe))) return rc;
         if (do_mpole && (rc = launch_mpole_direct(c, mode, off2, mixed))) return rc;
-        if (do_mpole && mixed && grad) {
-            if ((rc = launch_fx_flush_pair(c, c->fx_grad.p, c->dem.p, c->te.p, 3 * (size_t)n, PMQ_FX_GRAD))) return rc;
-        }
         if (pme) {
             if (ewald && (do_mpole || do_polar) && (rc = pme_mpole(c, mode, do_mpole, do_polar))) return rc;
             if (dewald && (rc = pme_disp(c, grad, !(ewald && (do_mpole || do_polar)), vir))) return rc;
         }
         if (do_polar) {
             if ((rc = polar_solve(c, safe, &c->pcg_iters))) return rc;
-            if (grad) {
-                if ((rc = polar_gradient(c, off2, mixed, vir))) return rc;
-                if (mixed) {
-                    if ((rc = launch_fx_flush_pair(c, c->fx_grad.p, c->dep.p, c->tep.p, 3 * (size_t)n, PMQ_FX_GRAD)))
-                        return rc;
-                }
-            }
+            if (grad && (rc = polar_gradient(c, off2, mixed, vir))) return rc;
             if ((rc = polar_finish(c, c->uind_out.p, c->uind_out.p + 3 * (size_t)n,
                                    mode == PMQ_ANALYSIS ? c->aep.p : nullptr)))
                 return rc;

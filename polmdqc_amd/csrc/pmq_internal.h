@@ -225,6 +225,16 @@ struct TileReduceArgs {
     int caller_order[8];  // index by caller order (perm) or by sorted order
     int stride[8], off[8];
     double fxscale[8];    // fixed-point scale of each value (mixed form)
+    // fixed-point accumulators (same index as out / out2) whose values join
+    // the sums and are cleared: the scaled-pair kernels' shares (mixed
+    // form; null: none)
+    long long *fxq[8], *fxq2[8];
+    // with k1 > k0 one more workgroup sums the tile-item scalars [k0, k1)
+    // into the totals (tile_scalars), beside the per-atom sums
+    int scal_which, scal_k0, scal_k1;
+    const double *tscal;
+    size_t tstride;
+    struct Totals *tot;
 };
 
 // energy / virial / count accumulators (one per context, in device memory)
@@ -531,7 +541,10 @@ int launch_zero(pmq_ctx *c, double *p, size_t n);
 // itself (with the multipole rotation, launch_rotpole)
 int tile_structure(pmq_ctx *c, double off2, bool safe, bool clear_scaled = true);
 int pair_matrix_build(pmq_ctx *c, bool mixed, bool safe);
-int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fixed);
+// scal_which, k0, k1: the tile-item scalars [k0, k1) summed in the same
+// launch (energy component scal_which for scalar 0), or none when k1 <= k0
+int tile_reduce(pmq_ctx *c, TileReduceArgs &a, int nv, bool float_parts, bool fixed, int scal_which = 0, int k0 = 0,
+                int k1 = 0);
 int tile_scalars(pmq_ctx *c, int which, bool full);
 int tile_virial(pmq_ctx *c); // tile scalars 1..9 (the pair virial) into the totals
 // skip: a device flag; when it is nonzero the launched kernels return at once
