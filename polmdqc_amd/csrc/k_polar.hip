@@ -796,6 +796,11 @@ int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
     while (!done && it < c->prm.politer) {
         const int nb = std::min(batch, c->prm.politer - it);
         for (int k = 0; k < nb; k++) {
+            // before the speculative last iteration of a batch, the test of
+            // the one before it, so that all of that iteration's launches
+            // return at once when it is not needed (otherwise only the
+            // launches after its first one would)
+            if (spec && k == nb - 1 && k > 0) hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(PCG_TB), 0, c->stream, a);
             a.iter = ++it;
             // the convergence test of the last iteration, the operator on z
             // ending in the step E1 (which records its block count), then E2
