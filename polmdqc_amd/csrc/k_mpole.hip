@@ -80,6 +80,25 @@ __global__ void k_zero_many(ZeroList z)
         d[i] = 0.0;
 }
 
+// the evaluation's status record (run counts and the PCG scalars) written
+// straight into the pinned host mirror, one launch instead of two copies
+__global__ void k_status_out(const RunStatus *__restrict__ st, const double *__restrict__ sc, HostStatus *h)
+{
+    const int t = threadIdx.x;
+    if (t == 0) h->st = *st;
+    if (t < 11) h->sc[t] = sc ? sc[t] : 0.0;
+}
+
+int launch_status_out(pmq_ctx *c)
+{
+    if (!c->h_status_dev) {
+        PMQ_HIP(hipHostGetDevicePointer((void **)&c->h_status_dev, c->h_status, 0));
+    }
+    hipLaunchKernelGGL(k_status_out, dim3(1), dim3(64), 0, c->stream, c->status.p, c->pcg.p, c->h_status_dev);
+    PMQ_HIP(hipGetLastError());
+    return PMQ_OK;
+}
+
 int launch_zero_many(pmq_ctx *c, int k, double *const *p, const size_t *n)
 {
     if (k <= 0) return PMQ_OK;

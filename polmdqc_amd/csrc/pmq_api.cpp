@@ -908,10 +908,9 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
     c->last_mode = mode;
     c->last_terms = terms;
     c->last_precision = precision;
-    // what the check needs, copied back as the evaluation's last step
-    PMQ_HIP(hipMemcpyAsync(&c->h_status->st, c->status.p, sizeof(RunStatus), hipMemcpyDeviceToHost, c->stream));
-    PMQ_HIP(hipMemcpyAsync(c->h_status->sc, c->pcg.p, sizeof(c->h_status->sc), hipMemcpyDeviceToHost, c->stream));
-    return PMQ_OK;
+    // what the check needs, written to the pinned host mirror as the
+    // evaluation's last step
+    return launch_status_out(c);
 }
 
 // one evaluation of the batched clusters: frames, the persistent cluster

@@ -370,6 +370,7 @@ struct pmq_ctx {
     pmq::DBuf<int> ntiles_eff;                   // min(tile count, tile_cap): what the tile kernels loop over
     pmq::DBuf<pmq::RunStatus> status;
     pmq::HostStatus *h_status = nullptr;         // pinned
+    pmq::HostStatus *h_status_dev = nullptr;     // its device address
     bool safe = true;                            // next evaluation sizes buffers and the PCG with host round trips
     bool pending = false;                        // an evaluation awaits its check (pmq_sync / pmq_fetch)
     bool pending_safe = false;                   // ... and it ran in safe mode
@@ -510,6 +511,8 @@ int launch_torque_pair(pmq_ctx *c, int mode0, const double *te0, double *de0, in
                        double *de1, bool mixed);
 constexpr int PMQ_ZERO_MAX = 16;
 int launch_zero_many(pmq_ctx *c, int k, double *const *p, const size_t *n);
+// the status record and PCG scalars into the pinned host mirror (h_status)
+int launch_status_out(pmq_ctx *c);
 // scalar stride of the per-item tile scalars (tscal)
 inline size_t item_stride(const pmq_ctx *c) { return (size_t)PMQ_TILE_PIECES * (size_t)c->tile_cap; }
 int pme_setup(pmq_ctx *c);
