@@ -868,7 +868,12 @@ static void precondition(ctx_t *c, const pairrec *L, long np, const double *poli
 }
 
 /* preconditioned conjugate gradient on (1/alpha - T) mu = E for both sets,
- * Jacobi preconditioner, stop when the rms residual (Debye) < poleps */
+ * stop when the rms residual (Debye) < poleps.  First guess: the direct
+ * induced dipoles a E; with the sparse preconditioner (pcgprec) those
+ * dipoles' near field added once, mu0 = a (E + T_s a E) with T_s the
+ * preconditioner's operator (pairs within usolvcut, no Ewald) -- one
+ * Jacobi step on the near field, which on the 12,288-atom water box saves
+ * one iteration (8 -> 7 to 1e-6 D); the GPU solver takes the same guess */
 static int solve_pcg(ctx_t *c, const pairrec *L, long np, const double *Ed, const double *Ep, double *ud, double *up)
 {
     const int n = c->n, n3 = 3 * n;
@@ -889,6 +894,17 @@ static int solve_pcg(ctx_t *c, const pairrec *L, long np, const double *Ed, cons
             ud[3 * i + a] = s->polarity[i] * Ed[3 * i + a];
             up[3 * i + a] = s->polarity[i] * Ep[3 * i + a];
         }
+    if (s->pcgprec) {
+        /* ... plus their near field: the preconditioner applied to E with
+         * a unit diagonal, a E + a T_s (a E) */
+        orc_elec_sys *sw = (orc_elec_sys *)s;
+        const double udiag = sw->udiag;
+        sw->udiag = 1.0;
+        precondition(c, L, np, poli, Ed, Ep, ud, up);
+        sw->udiag = udiag;
+        for (int j = 0; j < n3; j++)
+            if (s->polarity[j / 3] == 0.0) ud[j] = up[j] = 0.0;
+    }
     mutual_field(c, L, np, ud, up, Fd, Fp);
     for (int j = 0; j < n3; j++) {
         const int i = j / 3;

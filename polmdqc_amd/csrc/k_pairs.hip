@@ -520,7 +520,9 @@ __global__ __launch_bounds__(256) void k_pol_spmv(SpmvArgs a)
 // near part of every row, both dipole sets, G lanes per row, applied to
 // x = r (the initial residual: z = P r) or x = v (an iteration: the step E2,
 // z -= step P v; pcg_steps.h).  The columns' a x come packed with their
-// positions (pcg.pkw); the row's own a x is formed here.
+// positions (pcg.pkw); the row's own a x is formed here.  With E set: the
+// solve's first guess instead, u = a E + max(a, 1e-8) T_s (a E) (zero where
+// a = 0), into u and the packed source of the operator (pcg.pks).
 template <class V>
 struct PrecArgs {
     const int *rowptr, *rownear;
@@ -529,12 +531,40 @@ struct PrecArgs {
     const int *col;
     RowGeom geo;
     PcgVec pcg;
+    const double *E; // the first guess from this field (or null: the preconditioner step)
 };
 
 template <int G, int BK, class V>
 __global__ __launch_bounds__(PCG_TB) void k_pcg_prec(PrecArgs<V> a)
 {
     const PcgVec &pc = a.pcg;
+    if (a.E) { // the first guess (no PCG step)
+        const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+        const int row = tid / G, g = tid % G;
+        const int np = pc.npad;
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        if (row < np) {
+            const int e1 = (int)min((size_t)(a.rowptr[row] + a.rownear[row]), a.cap);
+            row_apply<G, PREC_U, BK, V>(a.col, a.ent_s, a.rowptr[row], e1, g, a.geo, a.geo.sx[row], a.geo.sy[row],
+                                        a.geo.sz[row], pc.pkw, acc);
+        }
+#pragma unroll
+        for (int c = 0; c < 6; c++)
+#pragma unroll
+            for (int off = G / 2; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+        if (row < np && g < 6) {
+            double v = acc[0];
+#pragma unroll
+            for (int c = 1; c < 6; c++)
+                if (g == c) v = acc[c];
+            const size_t i = (size_t)g * np + row;
+            const double pol = pc.spol[row];
+            const double u = pol != 0.0 ? pol * a.E[i] + fmax(1e-8, pol) * v : 0.0;
+            pc.u[i] = u;
+            pc.pks[(size_t)PMQ_PK * row + 3 + g] = u;
+        }
+        return;
+    }
     if (pc.iter > 0 && pc.sc[PCG_DONE] != 0.0) return;
     double step[2] = {0.0, 0.0};
     if (pc.iter > 0) pcg_e2_begin(pc, step);
@@ -1000,7 +1030,7 @@ static int spmv(pmq_ctx *c, double *dst, const double *skip, PcgVec *pcg, int ch
 }
 
 template <class V>
-static int precondition(pmq_ctx *c, const PcgVec &pcg)
+static int precondition(pmq_ctx *c, const PcgVec &pcg, const double *E)
 {
     constexpr int G = PREC_G;
     PrecArgs<V> a;
@@ -1008,6 +1038,7 @@ static int precondition(pmq_ctx *c, const PcgVec &pcg)
     a.ent_s = (const TEnt<V> *)c->tval_s.p;
     a.col = c->tcol.p; a.geo = row_geom(c);
     a.pcg = pcg;
+    a.E = E;
     const dim3 grid((unsigned)pair_matrix_prec_blocks(c));
     if (grid.x != (unsigned)pcg.n2) {
         c->err = "PCG: the preconditioner's partial count";
@@ -1024,14 +1055,14 @@ static int precondition(pmq_ctx *c, const PcgVec &pcg)
 
 int pair_matrix_prec_blocks(const pmq_ctx *c) { return (int)(((size_t)c->npad * PREC_G + PCG_TB - 1) / PCG_TB); }
 
-int pair_matrix_precondition(pmq_ctx *c, const PcgVec &pcg)
+int pair_matrix_precondition(pmq_ctx *c, const PcgVec &pcg, const double *E)
 {
     if (!c->prec_on) {
         c->err = "preconditioner applied without its matrix";
         return PMQ_ERR_STATE;
     }
-    if (c->tmixed) return precondition<float>(c, pcg);
-    return precondition<double>(c, pcg);
+    if (c->tmixed) return precondition<float>(c, pcg, E);
+    return precondition<double>(c, pcg, E);
 }
 
 int pair_matrix_apply(pmq_ctx *c, double *dst, const double *skip, PcgVec *pcg, int mode)

@@ -561,7 +561,8 @@ enum { PCG_APPLY_NONE = 0, PCG_APPLY_CHECK = 1, PCG_APPLY_E1 = 2 };
 int pair_matrix_apply(pmq_ctx *c, double *dst, const double *skip = nullptr, PcgVec *pcg = nullptr,
                       int mode = PCG_APPLY_NONE);
 int pair_matrix_prec_blocks(const pmq_ctx *c);
-int pair_matrix_precondition(pmq_ctx *c, const PcgVec &pcg);
+// E non-null: the solve's first guess a E + max(a, 1e-8) T_s (a E) instead
+int pair_matrix_precondition(pmq_ctx *c, const PcgVec &pcg, const double *E = nullptr);
 // device addresses of the batch scratch when it lives outside the context
 struct BatchScratch {
     Totals *tot;
