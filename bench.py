@@ -204,15 +204,18 @@ def timed(eng, L, terms, prec, steps, warmup):
 
 def bench_clusters(eng, L, steps, warmup) -> dict:
     """The batched workload: 512 non-periodic clusters of U{2..64} waters
-    (seed 512), no cutoff, mutual PCG to 1e-6 D per cluster; energy, forces and
-    parameter gradients of every cluster per evaluation (one graph launch),
-    both forms, with the autotuner's shape per size bucket; the same without
-    parameter gradients beside it."""
+    (seed 512), no cutoff, mutual PCG to 1e-6 D per cluster with the
+    polarizability preconditioner (pcgprec 0: on these clusters the sparse
+    one, pairs within usolvcut, takes 5.9 instead of 9.7 iterations but its
+    extra near-field pass per iteration costs more, 672 against 830 evals/s
+    mixed); energy, forces and parameter gradients of every cluster per
+    evaluation (one graph launch), both forms, with the autotuner's shape per
+    size bucket; the same without parameter gradients beside it."""
     from polmdqc_amd import water
 
     xyz, off = water.cluster_batch(512, 512, 2, 64)
     s = water.water_system(xyz, "water03")
-    s["pcgprec"] = 0  # the batched solver's diagonal preconditioner
+    s["pcgprec"] = 0  # the faster preconditioner on these clusters
     eng.set_system(s)
     eng.set_batch(off)
     terms = L.TERM_MPOLE | L.TERM_POLAR

@@ -171,7 +171,9 @@ typedef struct {
      * z = udiag a r + a T_s (a r), a = max(polarity, 1e-8) and T_s the
      * damped, u-scaled (w-scaled with charge penetration) real-space
      * dipole field tensor without Ewald of the pairs within usolvcut (taken
-     * no longer than the real-space cutoff); without it z = a r. */
+     * no longer than the real-space cutoff), and the solve starts from
+     * a (E + T_s a E) instead of a E; without it z = a r.  Batched clusters
+     * take either. */
     int pcgprec;
     double usolvcut, udiag;
     /* AMOEBA buffered 14-7 van der Waals (vdw.h, vdwpot.h, kvdw.cpp; the

@@ -15,9 +15,10 @@
 // The physics per pair is the one of the periodic path with no Ewald
 // screening and every pair inside the cutoff: permanent fields and the
 // dipole tensor with Thole damping at scale 1 plus (s - 1) lambda rr
-// corrections, PCG with the polarizability preconditioner and the test
-// debye * sqrt(max_set r.r / npolar) < poleps, E_pol = -(f/2) mu_d . E^p, and
-// the variational gradient.
+// corrections, PCG with the sparse preconditioner of the periodic path (the
+// pairs within usolvcut; with pcgprec off the polarizability alone) and its
+// first guess a (E + T_s a E), the test debye * sqrt(max_set r.r / npolar)
+// < poleps, E_pol = -(f/2) mu_d . E^p, and the variational gradient.
 #include <algorithm>
 #include <cstddef>
 #include <mutex>
@@ -43,6 +44,11 @@ struct BatchArgs {
     const int *useflag;   // per atom: 1 active (use), else 0
     double f, poleps;
     int politer;
+    // the sparse preconditioner (pcgprec): P x = udiag a x + max(a, 1e-8)
+    // T_s (a x), T_s the dipole field of the pairs within usolvcut; off:
+    // P x = a x
+    int prec;
+    double us2, udiag;
     int grad, do_mpole, do_polar;
     double *dem, *te, *dep, *tep;      // caller order (written)
     double *em_c, *ep_c, *it_c;        // per cluster
@@ -296,13 +302,15 @@ __global__ __launch_bounds__(ROWS * TPR) void k_batch_clusters(BatchArgs a)
             const double npol = npolar > 0 ? (double)npolar : 1.0;
             const double E6[6] = {Ed[0], Ed[1], Ed[2], Ep[0], Ep[1], Ep[2]};
             for (int c = 0; c < 6; c++) u[c] = alpha * E6[c];
-            // T v into tv (row-owned + u-scale corrections); v taken from vec
-            auto apply_T = [&](double *tv) {
+            // T v into tv (row-owned + u-scale corrections); v taken from vec;
+            // near: only the pairs within usolvcut (the preconditioner's T_s)
+            auto apply_T = [&](double *tv, bool near) {
                 double s[6] = {0, 0, 0, 0, 0, 0};
                 if (rowon)
                     for (int k = part; k < nc; k += TPR) {
                         if (k == row) continue;
                         const double xd = px[k] - xi, yd = py[k] - yi, zd = pz[k] - zi;
+                        if (near && xd * xd + yd * yd + zd * zd > a.us2) continue;
                         const T xr = T(xd), yr = T(yd), zr = T(zd);
                         const T r2 = xr * xr + yr * yr + zr * zr;
                         T lam[6], R[6];
@@ -325,6 +333,7 @@ __global__ __launch_bounds__(ROWS * TPR) void k_batch_clusters(BatchArgs a)
                     const int i = ex.i - o0, k = ex.k - o0;
                     const double xr = px[k] - px[i], yr = py[k] - py[i], zr = pz[k] - pz[i];
                     const double r2 = xr * xr + yr * yr + zr * zr;
+                    if (near && r2 > a.us2) continue;
                     double lam[6], R[6];
                     thole_lambda<double>(sqrt(r2), a.pdm[ex.i], a.pdm[ex.k], a.thl[ex.i], a.thl[ex.k], lam);
                     corr_series<double>(r2, ex.u, lam, R);
@@ -344,18 +353,40 @@ __global__ __launch_bounds__(ROWS * TPR) void k_batch_clusters(BatchArgs a)
                 __syncthreads();
                 for (int c = 0; c < 6; c++) tv[c] = rowon ? s[c] + acc.get(c, row, fx_field) : 0.0;
             };
-            // r = E + T u - u / alpha, z = alpha r, p = z
+            // z = P x for this row, x = the row's residual (both sets)
+            double tv[6];
+            auto precond = [&](const double *x, double *z) {
+                if (!a.prec) {
+                    for (int c = 0; c < 6; c++) z[c] = alpha * x[c];
+                    return;
+                }
+                __syncthreads();
+                if (own)
+                    for (int c = 0; c < 6; c++) vec[c][row] = alpha * x[c];
+                __syncthreads();
+                apply_T(tv, true);
+                for (int c = 0; c < 6; c++) z[c] = a.udiag * (alpha * x[c]) + fmax(1e-8, alpha) * tv[c];
+            };
+            // with the sparse preconditioner the first guess takes the near
+            // field of the direct dipoles once: u = a E + a T_s (a E)
+            if (a.prec) {
+                __syncthreads();
+                if (own)
+                    for (int c = 0; c < 6; c++) vec[c][row] = u[c];
+                __syncthreads();
+                apply_T(tv, true);
+                for (int c = 0; c < 6; c++) u[c] = alpha != 0.0 ? u[c] + fmax(1e-8, alpha) * tv[c] : 0.0;
+            }
+            // r = E + T u - u / alpha, z = P r, p = z
             __syncthreads();
             if (own)
                 for (int c = 0; c < 6; c++) vec[c][row] = u[c];
             __syncthreads();
-            double tv[6], r[6], z[6], p[6];
-            apply_T(tv);
-            for (int c = 0; c < 6; c++) {
-                r[c] = alpha != 0.0 ? E6[c] + tv[c] - u[c] / alpha : 0.0;
-                z[c] = alpha * r[c];
-                p[c] = z[c];
-            }
+            double r[6], z[6], p[6];
+            apply_T(tv, false);
+            for (int c = 0; c < 6; c++) r[c] = alpha != 0.0 ? E6[c] + tv[c] - u[c] / alpha : 0.0;
+            precond(r, z);
+            for (int c = 0; c < 6; c++) p[c] = z[c];
             double rz[2];
             block_sum2<NT>(own ? r[0] * z[0] + r[1] * z[1] + r[2] * z[2] : 0.0,
                            own ? r[3] * z[3] + r[4] * z[4] + r[5] * z[5] : 0.0, red, rz[0], rz[1]);
@@ -367,7 +398,7 @@ __global__ __launch_bounds__(ROWS * TPR) void k_batch_clusters(BatchArgs a)
                 if (own)
                     for (int c = 0; c < 6; c++) vec[c][row] = p[c];
                 __syncthreads();
-                apply_T(tv);
+                apply_T(tv, false);
                 double v[6];
                 for (int c = 0; c < 6; c++) v[c] = alpha != 0.0 ? p[c] / alpha - tv[c] : 0.0;
                 double pv[2];
@@ -378,8 +409,8 @@ __global__ __launch_bounds__(ROWS * TPR) void k_batch_clusters(BatchArgs a)
                     const double s = al[c / 3];
                     u[c] += s * p[c];
                     r[c] -= s * v[c];
-                    z[c] = alpha * r[c];
                 }
+                precond(r, z);
                 double rzn[2], rr[2];
                 block_sum2<NT>(own ? r[0] * z[0] + r[1] * z[1] + r[2] * z[2] : 0.0,
                                own ? r[3] * z[3] + r[4] * z[4] + r[5] * z[5] : 0.0, red, rzn[0], rzn[1]);
@@ -757,6 +788,9 @@ static BatchArgs batch_args(pmq_ctx *c, int mode, bool mpole, bool polar, const 
     a.f = c->prm.electric / c->prm.dielec;
     a.poleps = c->prm.poleps;
     a.politer = c->prm.politer;
+    a.prec = c->prm.pcgprec && c->prm.usolvcut > 0.0 ? 1 : 0;
+    a.us2 = c->prm.usolvcut * c->prm.usolvcut;
+    a.udiag = c->prm.udiag;
     a.grad = mode == PMQ_GRADIENT;
     a.do_mpole = mpole; a.do_polar = polar;
     a.dem = c->dem.p; a.te = c->te.p; a.dep = c->dep.p; a.tep = c->tep.p;

@@ -956,10 +956,6 @@ static int batch_check(pmq_ctx *c, int mode, unsigned terms)
         c->err = "pmq_compute: polarization requested but no polarizabilities were set";
         return PMQ_ERR_STATE;
     }
-    if ((terms & PMQ_TERM_POLAR) && c->prm.pcgprec) {
-        c->err = "pmq_compute: batched clusters solve with the diagonal preconditioner (set pcgprec = 0)";
-        return PMQ_ERR_UNSUPPORTED;
-    }
     return PMQ_OK;
 }
 

@@ -3,7 +3,9 @@
 Every cluster of the batch must equal that cluster solved alone: by the
 oracle (FP64, 1e-9 on energies, 1e-8 on gradients, PCG tightened to 1e-10 D
 so both solvers sit on the same solution; equal iteration counts at the
-production 1e-6 D), and by the one-system GPU path.  The mixed form keeps the
+production 1e-6 D), and by the one-system GPU path, with the sparse
+preconditioner (pairs within usolvcut, the default) and with the
+polarizability alone (pcgprec = 0).  The mixed form keeps the
 RMS force error below 1e-3 kcal/mol/A.  Graph replays, with eager work and
 synchronizations between them, must repeat the first (captured) result.
 """
@@ -25,25 +27,26 @@ def engine():
     e.close()
 
 
-def batch(nclusters, seed, max_mol=12):
+def batch(nclusters, seed, max_mol=12, prec=1):
     xyz, off = water.cluster_batch(nclusters, seed, 2, max_mol)
     s = water.water_system(xyz, "water03")
-    s["pcgprec"] = 0  # the batched solver's diagonal preconditioner
+    s["pcgprec"] = prec
     return s, off
 
 
 def sub(s, a, b):
     """The system of atoms a:b of a water system, as its own system."""
     t = water.water_system(np.stack([s["x"][a:b], s["y"][a:b], s["z"][a:b]], 1), "water03")
-    t["pcgprec"] = 0
+    t["pcgprec"] = s["pcgprec"]
     return t
 
 
-def test_batch_matches_oracle_per_cluster(engine):
+@pytest.mark.parametrize("prec", [1, 0])
+def test_batch_matches_oracle_per_cluster(engine, prec):
     from polmdqc_amd import _lib as L
 
-    print("[gpu] batch 12 clusters vs oracle", flush=True)
-    s, off = batch(12, 5)
+    print(f"[gpu] batch 12 clusters vs oracle, pcgprec {prec}", flush=True)
+    s, off = batch(12, 5, prec=prec)
     s["poleps"] = 1e-10
     engine.set_system(s)
     engine.set_batch(off)
@@ -61,10 +64,11 @@ def test_batch_matches_oracle_per_cluster(engine):
     assert abs(g["em"] - b["em"].sum()) <= 1e-12 * abs(g["em"])
 
 
-def test_batch_iterations_match_oracle(engine):
+@pytest.mark.parametrize("prec", [1, 0])
+def test_batch_iterations_match_oracle(engine, prec):
     from polmdqc_amd import _lib as L
 
-    s, off = batch(8, 11)
+    s, off = batch(8, 11, prec=prec)
     engine.set_system(s)
     engine.set_batch(off)
     engine.eval(L.ENERGY, L.TERM_MPOLE | L.TERM_POLAR)
@@ -74,10 +78,11 @@ def test_batch_iterations_match_oracle(engine):
         assert b["pcg_iters"][c] == o["iters"]
 
 
-def test_batch_equals_single_system_path(engine):
+@pytest.mark.parametrize("prec", [1, 0])
+def test_batch_equals_single_system_path(engine, prec):
     from polmdqc_amd import _lib as L
 
-    s, off = batch(6, 3)
+    s, off = batch(6, 3, prec=prec)
     engine.set_system(s)
     engine.set_batch(off)
     g = engine.eval(L.GRADIENT, L.TERM_MPOLE | L.TERM_POLAR)

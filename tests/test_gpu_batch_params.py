@@ -34,7 +34,7 @@ def engine():
 def batch(nclusters, seed, min_mol=2, max_mol=12):
     xyz, off = water.cluster_batch(nclusters, seed, min_mol, max_mol)
     s = water.water_system(xyz, "water03")
-    s["pcgprec"] = 0  # the batched solver's diagonal preconditioner
+    s["pcgprec"] = 0  # the polarizability preconditioner (the sparse one: test_gpu_batch.py)
     return s, off
 
 

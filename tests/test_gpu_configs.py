@@ -305,7 +305,7 @@ def test_amoeba12k_parameters_by_finite_differences(engine):
 def cluster512():
     xyz, off = water.cluster_batch(512, 512, 2, 64)
     s = water.water_system(xyz, "water03")
-    s["pcgprec"] = 0  # the batched solver's diagonal preconditioner
+    s["pcgprec"] = 0  # as benchmarked: the polarizability preconditioner (the sparse one: test_gpu_batch.py)
     return s, off
 
 
