@@ -520,8 +520,39 @@ struct GatherFracArgs {
     double *frac;          // n x nset x 20 fractional derivatives, sorted atom order
 };
 
+// Sums over the G lanes of a group of N values each, scattered: lane l
+// ends with the totals of components l, l + G, l + 2G, ... (< N) in
+// o[0], o[1], ...  Each stage halves the components a lane keeps and trades
+// the other half with the lane `h` away, G - 1 shuffles per G components
+// instead of log2(G) per component for a full all-reduce (20 values over
+// 32 lanes: 31 shuffles instead of 100).  A fixed order, so the sums are
+// reproducible.
+template <int G, int N>
+__device__ __forceinline__ void group_reduce_scatter(double *o, int l)
+{
+    constexpr int Q = (N + G - 1) / G;
+    double w[Q][G];
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+        for (int r = 0; r < G; r++) w[q][r] = q * G + r < N ? o[q * G + r] : 0.0;
+#pragma unroll
+    for (int h = G / 2; h > 0; h >>= 1) {
+        const bool hi = (l & h) != 0;
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+#pragma unroll
+            for (int k = 0; k < h; k++) {
+                const double keep = hi ? w[q][h + k] : w[q][k], send = hi ? w[q][k] : w[q][h + k];
+                w[q][k] = keep + __shfl_xor(send, h, 64);
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) o[q] = w[q][0];
+}
+
 // the 20 fractional potential derivatives of every atom in each of nset
-// grids; lane j < 20 of the group stores component j
+// grids; lane l of the group stores components l, l + G, l + 2G < 20
 template <int ORD, int GL, class T, class GT>
 __global__ __launch_bounds__(256) void k_pme_gather_frac(GatherFracArgs a)
 {
@@ -529,16 +560,10 @@ __global__ __launch_bounds__(256) void k_pme_gather_frac(GatherFracArgs a)
     for (int set = 0; set < a.nset; set++) {
         double fp[20];
         gather_share<ORD, G, 20, T, GT>(a.g, (const GT *)a.grid + set * a.gstride, tx, ty, ty1, tz, i0, ky, kz, on, fp);
-        group_reduce<G, 20>(fp);
-        // lane l of the group stores components l, l + G, l + 2G < 20
+        group_reduce_scatter<G, 20>(fp, l);
 #pragma unroll
-        for (int base = 0; base < 20; base += G) {
-            double v = 0.0;
-#pragma unroll
-            for (int j = base; j < base + G && j < 20; j++)
-                if (l == j - base) v = fp[j];
-            if (valid && base + l < 20) a.frac[((size_t)s * a.nset + set) * 20 + base + l] = v;
-        }
+        for (int q = 0; q * G < 20; q++)
+            if (valid && q * G + l < 20) a.frac[((size_t)s * a.nset + set) * 20 + q * G + l] = fp[q];
     }
 }
 
