@@ -520,6 +520,15 @@ struct GatherFracArgs {
     double *frac;          // n x nset x 20 fractional derivatives, sorted atom order
 };
 
+// this lane's share of the Cartesian potential gradient from its share of
+// the fractional derivatives (f[1..3]; the map of frac_to_cart)
+template <int G>
+__device__ __forceinline__ void frac_grad(const PmeGeom &g, const double *f, double *gc)
+{
+#pragma unroll
+    for (int x = 0; x < 3; x++) gc[x] = g.A[0][x] * f[1] + g.A[1][x] * f[2] + g.A[2][x] * f[3];
+}
+
 // Sums over the G lanes of a group of N values each, scattered: lane l
 // ends with the totals of components l, l + G, l + 2G, ... (< N) in
 // o[0], o[1], ...  Each stage halves the components a lane keeps and trades
@@ -721,18 +730,16 @@ __global__ __launch_bounds__(256) void k_pme_gather_field(GatherFieldArgs a)
         double fp0[4], fp1[4];
         gather_share<ORD, G, 4, T, GT>(a.g, (const GT *)a.grid, tx, ty, ty1, tz, i0, ky, kz, on, fp0);
         gather_share<ORD, G, 4, T, GT>(a.g, (const GT *)a.grid + a.gstride, tx, ty, ty1, tz, i0, ky, kz, on, fp1);
-        group_reduce<G, 4>(fp0);
-        group_reduce<G, 4>(fp1);
+        // the Cartesian gradient is linear in the fractional one, so each
+        // lane maps its share first and the group sums the six components
+        // scattered: lane c < 6 ends with component c (set c / 3, axis c % 3)
+        double gc[6];
+        frac_grad<G>(a.g, fp0, gc);
+        frac_grad<G>(a.g, fp1, gc + 3);
+        group_reduce_scatter<G, 6>(gc, l);
         double pv0 = 0.0, pv1 = 0.0;
         if (mine) {
-            double fp[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) fp[j] = l < 3 ? fp0[j] : fp1[j];
-            double phi, gr[3];
-            frac_to_cart<4>(a.g, fp, &phi, gr, nullptr, nullptr);
-            const int q = l % 3;
-            const double g = q == 0 ? gr[0] : q == 1 ? gr[1] : gr[2];
-            const double f = fr + (-g + a.self * in.z);
+            const double f = fr + (-gc[0] + a.self * in.z);
             (l < 3 ? pv0 : pv1) = pcg_e1_apply(a.pcg, s, l, f, in, beta);
         }
         pcg_e1_end(a.pcg, pv0, pv1);
@@ -741,12 +748,12 @@ __global__ __launch_bounds__(256) void k_pme_gather_field(GatherFieldArgs a)
     for (int set = 0; set < a.nset; set++) {
         double fp[4];
         gather_share<ORD, G, 4, T, GT>(a.g, (const GT *)a.grid + set * a.gstride, tx, ty, ty1, tz, i0, ky, kz, on, fp);
-        group_reduce<G, 4>(fp);
-        if (valid && l == 0) {
-            double phi, gr[3];
-            frac_to_cart<4>(a.g, fp, &phi, gr, nullptr, nullptr);
-            const size_t o = (size_t)set * 3 * a.npad;
-            for (int q = 0; q < 3; q++) a.field[o + q * a.npad + s] += -gr[q] + a.self * a.dip[o + q * a.npad + s];
+        double gc[3];
+        frac_grad<G>(a.g, fp, gc);
+        group_reduce_scatter<G, 3>(gc, l);
+        if (valid && l < 3) {
+            const size_t o = (size_t)set * 3 * a.npad + (size_t)l * a.npad + s;
+            a.field[o] += -gc[0] + a.self * a.dip[o];
         }
     }
 }
