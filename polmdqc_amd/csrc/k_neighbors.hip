@@ -4,9 +4,10 @@
 // (cells about 4 A wide in the box's fractional frame, or in the bounding box
 // of a non-periodic system; with PME the grid's spreading bricks) with the
 // atom index in the low 32 bits, so the sort is total and deterministic; (2)
-// radix-sort the keys; (3) gather the permutation; (4) bound each 64-atom
-// block; (5) keep every block pair (I <= J) whose bounds come within the
-// cutoff, in (I, J) order.  Runs of cells along a Hilbert curve are compact
+// radix-sort the keys; (3) gather the permutation and bound each 64-atom
+// block (one launch, a wave per block); (4) keep every block pair (I <= J)
+// whose bounds come within the cutoff (flags, a device scan, the list in
+// tile_slot order).  Runs of cells along a Hilbert curve are compact
 // (consecutive cells share a face), so 64 consecutive atoms fill a small box:
 // on a 12,288-atom water box with 4.1 A bricks and a 7 A cutoff the blocks'
 // mean extent is 9.7 x 10.2 x 9.6 A and 4,503 block pairs are kept, against
@@ -148,27 +149,50 @@ __global__ void k_brick_keys(int n, BrickGeom g, int bits, const double *__restr
     keys[i] = ((unsigned long long)hilbert3(hb[0], hb[1], hb[2], bits) << 32) | (unsigned)i;
 }
 
-// permutation, sorted positions and per-atom flags (bit0 valid, bit1 active);
-// with PME bricks (bstart non-null) also the sorted-atom range [bstart,
-// bend) of every non-empty brick, from where the brick part of the sorted
-// keys changes (the brick's lexicographic index from the atom's home brick)
-__global__ void k_gather(int n, int npad, const unsigned long long *__restrict__ skeys, const double *__restrict__ x,
-                         const double *__restrict__ y, const double *__restrict__ z, const int *__restrict__ usef,
-                         const int *__restrict__ mol, int *__restrict__ perm, int *__restrict__ iperm,
-                         double *__restrict__ sx, double *__restrict__ sy, double *__restrict__ sz,
-                         int *__restrict__ sflag, int *__restrict__ smol, BrickGeom g, int *__restrict__ bstart,
-                         int *__restrict__ bend)
+// wave (64-lane) minimum and maximum
+__device__ __forceinline__ double wave_min(double v)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= npad) return;
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// One wave per 64-atom block: the permutation, sorted positions and per-atom
+// flags (bit0 valid, bit1 active); with PME bricks (bstart non-null) the
+// sorted-atom range [bstart, bend) of every non-empty brick, from where the
+// brick part of the sorted keys changes (the brick's lexicographic index from
+// the atom's home brick); then the block's bounds: axis-aligned in the
+// (wrapped) fractional-cell frame, as a Cartesian center and half extent,
+// and the span of the stored (unwrapped) coordinates of every slot, padding
+// included, which the tile mask kernel's one-image test reads
+__global__ __launch_bounds__(PMQ_BLOCK) void k_gather(int n, int npad, const unsigned long long *__restrict__ skeys,
+                                                      const double *__restrict__ x, const double *__restrict__ y,
+                                                      const double *__restrict__ z, const int *__restrict__ usef,
+                                                      const int *__restrict__ mol, int *__restrict__ perm,
+                                                      int *__restrict__ iperm, double *__restrict__ sx,
+                                                      double *__restrict__ sy, double *__restrict__ sz,
+                                                      int *__restrict__ sflag, int *__restrict__ smol, BrickGeom g,
+                                                      int *__restrict__ bstart, int *__restrict__ bend, DevBox b,
+                                                      double *__restrict__ blk)
+{
+    const int I = blockIdx.x, lane = threadIdx.x;
+    const int s = I * PMQ_BLOCK + lane;
+    double q[3] = {0.0, 0.0, 0.0};
     if (s < n) {
         const unsigned long long key = skeys[s];
         const int i = (int)(key & 0xffffffffu);
         perm[s] = i;
         iperm[i] = s;
-        sx[s] = x[i];
-        sy[s] = y[i];
-        sz[s] = z[i];
+        q[0] = x[i];
+        q[1] = y[i];
+        q[2] = z[i];
+        sx[s] = q[0];
+        sy[s] = q[1];
+        sz[s] = q[2];
         if (bstart) {
             const unsigned h = (unsigned)(key >> 32);
             const bool first = s == 0 || (unsigned)(skeys[s - 1] >> 32) != h;
@@ -183,64 +207,38 @@ __global__ void k_gather(int n, int npad, const unsigned long long *__restrict__
         }
         sflag[s] = 1 | (usef[i] ? 2 : 0);
         smol[s] = mol[i];
-    } else {
+    } else if (s < npad) {
         perm[s] = -1;
         sx[s] = sy[s] = sz[s] = 0.0;
         sflag[s] = 0;
         smol[s] = -1;
     }
-}
-
-// axis-aligned bounds of each block in the (wrapped) fractional-cell frame,
-// expressed as a Cartesian center and half extent
-__global__ void k_block_bounds(int n, int nblk, DevBox b, const double *__restrict__ bbox,
-                               const double *__restrict__ sx, const double *__restrict__ sy,
-                               const double *__restrict__ sz, double *__restrict__ blk)
-{
-    const int I = blockIdx.x;
-    const int lane = threadIdx.x;
-    const int s = I * PMQ_BLOCK + lane;
-    __shared__ double lo[3][PMQ_BLOCK], hi[3][PMQ_BLOCK], rlo[3][PMQ_BLOCK], rhi[3][PMQ_BLOCK];
-    double p[3] = {0, 0, 0};
-    bool valid = s < n;
-    // span of the stored (unwrapped) coordinates of every slot of the block,
-    // padding included: the tile mask kernel's one-image test reads it
-    {
-        const double q[3] = {sx[s], sy[s], sz[s]};
-        for (int a = 0; a < 3; a++) rlo[a][lane] = rhi[a][lane] = q[a];
-    }
-    if (valid) {
-        p[0] = sx[s]; p[1] = sy[s]; p[2] = sz[s];
-        if (b.kind != PMQ_BOX_NONE) {
-            // wrap into the primary cell so the block is compact
-            double f[3];
-            for (int a = 0; a < 3; a++) {
-                f[a] = b.recip[a][0] * p[0] + b.recip[a][1] * p[1] + b.recip[a][2] * p[2];
-                f[a] -= floor(f[a]);
-            }
-            for (int a = 0; a < 3; a++) p[a] = b.lvec[a][0] * f[0] + b.lvec[a][1] * f[1] + b.lvec[a][2] * f[2];
+    // the block's bounds
+    const bool valid = s < n;
+    double p[3] = {q[0], q[1], q[2]};
+    if (valid && b.kind != PMQ_BOX_NONE) {
+        // wrap into the primary cell so the block is compact
+        double f[3];
+        for (int a = 0; a < 3; a++) {
+            f[a] = b.recip[a][0] * p[0] + b.recip[a][1] * p[1] + b.recip[a][2] * p[2];
+            f[a] -= floor(f[a]);
         }
+        for (int a = 0; a < 3; a++) p[a] = b.lvec[a][0] * f[0] + b.lvec[a][1] * f[1] + b.lvec[a][2] * f[2];
     }
+    double lo[3], hi[3], rlo[3], rhi[3];
     for (int a = 0; a < 3; a++) {
-        lo[a][lane] = valid ? p[a] : 1e300;
-        hi[a][lane] = valid ? p[a] : -1e300;
-    }
-    __syncthreads();
-    for (int w = PMQ_BLOCK / 2; w > 0; w >>= 1) {
-        if (lane < w)
-            for (int a = 0; a < 3; a++) {
-                lo[a][lane] = fmin(lo[a][lane], lo[a][lane + w]);
-                hi[a][lane] = fmax(hi[a][lane], hi[a][lane + w]);
-                rlo[a][lane] = fmin(rlo[a][lane], rlo[a][lane + w]);
-                rhi[a][lane] = fmax(rhi[a][lane], rhi[a][lane + w]);
-            }
-        __syncthreads();
+        lo[a] = wave_min(valid ? p[a] : 1e300);
+        hi[a] = wave_max(valid ? p[a] : -1e300);
+        rlo[a] = wave_min(q[a]);
+        rhi[a] = wave_max(q[a]);
     }
     if (lane < 3) {
-        blk[PMQ_BLK_STRIDE * I + lane] = 0.5 * (lo[lane][0] + hi[lane][0]);
-        blk[PMQ_BLK_STRIDE * I + 3 + lane] = 0.5 * (hi[lane][0] - lo[lane][0]);
-        blk[PMQ_BLK_STRIDE * I + 6 + lane] = rlo[lane][0];
-        blk[PMQ_BLK_STRIDE * I + 9 + lane] = rhi[lane][0];
+        const double l0 = lane == 0 ? lo[0] : lane == 1 ? lo[1] : lo[2];
+        const double h0 = lane == 0 ? hi[0] : lane == 1 ? hi[1] : hi[2];
+        blk[PMQ_BLK_STRIDE * I + lane] = 0.5 * (l0 + h0);
+        blk[PMQ_BLK_STRIDE * I + 3 + lane] = 0.5 * (h0 - l0);
+        blk[PMQ_BLK_STRIDE * I + 6 + lane] = lane == 0 ? rlo[0] : lane == 1 ? rlo[1] : rlo[2];
+        blk[PMQ_BLK_STRIDE * I + 9 + lane] = lane == 0 ? rhi[0] : lane == 1 ? rhi[1] : rhi[2];
     }
 }
 
@@ -257,11 +255,10 @@ __device__ __forceinline__ void tri_index(long long t, int nb, int &I, int &J)
     J = (int)(t - ((long long)i * nb - (long long)i * (i - 1) / 2)) + i;
 }
 
-__global__ void k_tile_flags(int nblk, long long npairs, DevBox b, double cut, const double *__restrict__ blk,
-                             int *__restrict__ flag)
+// whether block pair t (I <= J, row by row) comes within the cutoff
+__device__ __forceinline__ int tile_keep(long long t, int nblk, const DevBox &b, double cut,
+                                         const double *__restrict__ blk)
 {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npairs) return;
     int I, J;
     tri_index(t, nblk, I, J);
     bool keep = true;
@@ -301,7 +298,15 @@ __global__ void k_tile_flags(int nblk, long long npairs, DevBox b, double cut, c
         const double rj = sqrt(bj[3] * bj[3] + bj[4] * bj[4] + bj[5] * bj[5]);
         keep = sqrt(best) - ri - rj <= cut * (1.0 + 1e-12) + 1e-9;
     }
-    flag[t] = keep ? 1 : 0;
+    return keep ? 1 : 0;
+}
+
+__global__ void k_tile_flags(int nblk, long long npairs, DevBox b, double cut, const double *__restrict__ blk,
+                             int *__restrict__ flag)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs) return;
+    flag[t] = tile_keep(t, nblk, b, cut, blk);
 }
 
 // the listed tiles (tile_slot order); the last pair's thread also stores the
@@ -382,11 +387,13 @@ int build_neighbors(pmq_ctx *c, double cutoff, const double *px, const double *p
     bytes = c->scratch_sort_bytes;
     PMQ_HIP(rocprim::radix_sort_keys(c->scratch_sort.p, bytes, c->keys.p, c->keys_sorted.p, (size_t)n, 32u, end_bit,
                                      st));
-    hipLaunchKernelGGL(k_gather, dim3((npad + tb - 1) / tb), dim3(tb), 0, st, n, npad, c->keys_sorted.p, px, py, pz,
-                       use, c->molcule.p, c->perm.p, c->iperm.p, c->sx.p, c->sy.p, c->sz.p, c->sflag.p, c->smol.p, g,
-                       bstart, bend);
-    hipLaunchKernelGGL(k_block_bounds, dim3(nblk), dim3(PMQ_BLOCK), 0, st, n, nblk, b, c->bbox.p, c->sx.p, c->sy.p,
-                       c->sz.p, c->blk.p);
+    if (nblk * PMQ_BLOCK != npad) {
+        c->err = "build_neighbors: the padded atom count is not a whole number of blocks";
+        return PMQ_ERR_STATE;
+    }
+    hipLaunchKernelGGL(k_gather, dim3(nblk), dim3(PMQ_BLOCK), 0, st, n, npad, c->keys_sorted.p, px, py, pz, use,
+                       c->molcule.p, c->perm.p, c->iperm.p, c->sx.p, c->sy.p, c->sz.p, c->sflag.p, c->smol.p, g,
+                       bstart, bend, b, c->blk.p);
     const long long npairs = (long long)nblk * (nblk + 1) / 2;
     const int gb = (int)((npairs + tb - 1) / tb);
     hipLaunchKernelGGL(k_tile_flags, dim3(gb), dim3(tb), 0, st, nblk, npairs, b, cutoff, c->blk.p, c->tileflag.p);
