@@ -532,12 +532,53 @@ struct PrecArgs {
     RowGeom geo;
     PcgVec pcg;
     const double *E; // the first guess from this field (or null: the preconditioner step)
+    // The solve's reciprocal potential grids carried by linearity (PME):
+    // the workgroups past nrow fold the grid of this iteration's z (qg, the
+    // field gather's input, two sets) into those of the direction and the
+    // dipoles, gp = qg + beta gp and gu += step gp (at iteration 0, gu = qg:
+    // the grid of the first guess), the same steps E1 and E2 apply to p and
+    // u; at the end gu is the potential of the solved dipoles, with no
+    // spreading or transform of its own.  qg null: nothing carried.
+    const void *qg;
+    void *gp, *gu;
+    size_t gpts; // points of the two sets together
+    int g32;     // float grids
+    int nrow;    // workgroups of the rows
 };
+
+template <class GT>
+__device__ __forceinline__ void carry_grids(const PcgVec &pc, const void *qg_, void *gp_, void *gu_, size_t gpts,
+                                            int wg, int nwg)
+{
+    const GT *qg = (const GT *)qg_;
+    GT *gp = (GT *)gp_, *gu = (GT *)gu_;
+    const size_t half = gpts / 2;
+    const size_t first = (size_t)wg * PCG_TB + threadIdx.x, stride = (size_t)nwg * PCG_TB;
+    if (pc.iter == 0) {
+        for (size_t i = first; i < gpts; i += stride) gu[i] = qg[i];
+        return;
+    }
+    double step[2];
+    pcg_e2_begin(pc, step);
+    const double beta[2] = {pc.sc[PCG_BETA], pc.sc[PCG_BETA + 1]};
+    for (size_t i = first; i < gpts; i += stride) {
+        const int set = i < half ? 0 : 1;
+        const GT p = pc.iter == 1 ? qg[i] : GT((double)qg[i] + beta[set] * (double)gp[i]);
+        gp[i] = p;
+        gu[i] = GT((double)gu[i] + step[set] * (double)p);
+    }
+}
 
 template <int G, int BK, class V>
 __global__ __launch_bounds__(PCG_TB) void k_pcg_prec(PrecArgs<V> a)
 {
     const PcgVec &pc = a.pcg;
+    if (a.qg && (int)blockIdx.x >= a.nrow) {
+        if (pc.iter > 0 && pc.sc[PCG_DONE] != 0.0) return;
+        if (a.g32) carry_grids<float>(pc, a.qg, a.gp, a.gu, a.gpts, (int)blockIdx.x - a.nrow, (int)gridDim.x - a.nrow);
+        else carry_grids<double>(pc, a.qg, a.gp, a.gu, a.gpts, (int)blockIdx.x - a.nrow, (int)gridDim.x - a.nrow);
+        return;
+    }
     if (a.E) { // the first guess (no PCG step)
         const int tid = blockIdx.x * blockDim.x + threadIdx.x;
         const int row = tid / G, g = tid % G;
@@ -1030,7 +1071,7 @@ static int spmv(pmq_ctx *c, double *dst, const double *skip, PcgVec *pcg, int ch
 }
 
 template <class V>
-static int precondition(pmq_ctx *c, const PcgVec &pcg, const double *E)
+static int precondition(pmq_ctx *c, const PcgVec &pcg, const double *E, bool carry)
 {
     constexpr int G = PREC_G;
     PrecArgs<V> a;
@@ -1039,11 +1080,20 @@ static int precondition(pmq_ctx *c, const PcgVec &pcg, const double *E)
     a.col = c->tcol.p; a.geo = row_geom(c);
     a.pcg = pcg;
     a.E = E;
-    const dim3 grid((unsigned)pair_matrix_prec_blocks(c));
-    if (grid.x != (unsigned)pcg.n2) {
+    a.nrow = pair_matrix_prec_blocks(c);
+    if (a.nrow != pcg.n2) {
         c->err = "PCG: the preconditioner's partial count";
         return PMQ_ERR_STATE;
     }
+    void *qg = nullptr, *gp = nullptr, *gu = nullptr;
+    size_t gpts = 0;
+    bool g32 = false;
+    int ncarry = 0;
+    if (carry && !E && pme_carry_grids(c, &qg, &gp, &gu, &gpts, &g32))
+        ncarry = (int)std::min<size_t>((gpts + 4 * PCG_TB - 1) / (4 * PCG_TB), 4096);
+    a.qg = ncarry ? qg : nullptr;
+    a.gp = gp; a.gu = gu; a.gpts = gpts; a.g32 = g32 ? 1 : 0;
+    const dim3 grid((unsigned)(a.nrow + ncarry));
     switch (row_box(c)) {
     case ROW_BOX_NONE: hipLaunchKernelGGL((k_pcg_prec<G, ROW_BOX_NONE, V>), grid, dim3(PCG_TB), 0, c->stream, a); break;
     case ROW_BOX_ORTHO: hipLaunchKernelGGL((k_pcg_prec<G, ROW_BOX_ORTHO, V>), grid, dim3(PCG_TB), 0, c->stream, a); break;
@@ -1055,14 +1105,14 @@ static int precondition(pmq_ctx *c, const PcgVec &pcg, const double *E)
 
 int pair_matrix_prec_blocks(const pmq_ctx *c) { return (int)(((size_t)c->npad * PREC_G + PCG_TB - 1) / PCG_TB); }
 
-int pair_matrix_precondition(pmq_ctx *c, const PcgVec &pcg, const double *E)
+int pair_matrix_precondition(pmq_ctx *c, const PcgVec &pcg, const double *E, bool carry)
 {
     if (!c->prec_on) {
         c->err = "preconditioner applied without its matrix";
         return PMQ_ERR_STATE;
     }
-    if (c->tmixed) return precondition<float>(c, pcg, E);
-    return precondition<double>(c, pcg, E);
+    if (c->tmixed) return precondition<float>(c, pcg, E, carry);
+    return precondition<double>(c, pcg, E, carry);
 }
 
 int pair_matrix_apply(pmq_ctx *c, double *dst, const double *skip, PcgVec *pcg, int mode)

@@ -1107,12 +1107,12 @@ static int spread(pmq_ctx *c, int mode, const double *src, const double *charge,
 }
 
 // fractional potential derivatives of nset grids into pme_frac
-static int gather_frac(pmq_ctx *c, int nset)
+static int gather_frac(pmq_ctx *c, int nset, const void *grid = nullptr)
 {
     GatherFracArgs args;
     args.n = c->n; args.nset = nset; args.g = geom(c);
     args.sx = c->sx.p; args.sy = c->sy.p; args.sz = c->sz.p;
-    args.grid = c->qgrid.p;
+    args.grid = grid ? grid : c->qgrid.p;
     args.gstride = grid_size(c);
     args.frac = c->pme_frac.p;
     ProfScope ps_(c, "pme_gather");
@@ -1398,13 +1398,40 @@ int pme_disp(pmq_ctx *c, bool grad, bool need_bin, bool vir)
     return PMQ_OK;
 }
 
+// the reciprocal potential grids the solve carries (k_pcg_prec): p then u,
+// two sets each, in the element type of the field grids
+bool pme_carry_grids(pmq_ctx *c, void **qg, void **gp, void **gu, size_t *pts, bool *g32)
+{
+    if (!c->prm.use_ewald || !c->qgrid.p) return false;
+    const size_t gs = grid_size(c);
+    if (c->phi_acc.alloc(4 * gs) != hipSuccess) return false; // sized for FP64 grids
+    const bool f = grid32(c);
+    *qg = c->qgrid.p;
+    *gp = c->phi_acc.p;
+    *gu = f ? (void *)((float *)c->phi_acc.p + 2 * gs) : (void *)(c->phi_acc.p + 2 * gs);
+    *pts = 2 * gs;
+    *g32 = f;
+    return true;
+}
+
 int pme_polar_grad(pmq_ctx *c, bool vir)
 {
     int rc;
-    const size_t gs = grid_size(c);
-    if ((rc = spread(c, 1, c->uind.p, nullptr, 2))) return rc;
-    if ((rc = pme_potential(c, 2))) return rc;
-    if ((rc = gather_frac(c, 2))) return rc;
+    if (c->phi_carried) {
+        // the solve carried the potential grids of its dipoles (k_pcg_prec)
+        void *qg, *gp, *gu;
+        size_t pts;
+        bool g32;
+        if (!pme_carry_grids(c, &qg, &gp, &gu, &pts, &g32)) {
+            c->err = "PME: the carried potential grids are gone";
+            return PMQ_ERR_STATE;
+        }
+        if ((rc = gather_frac(c, 2, gu))) return rc;
+    } else {
+        if ((rc = spread(c, 1, c->uind.p, nullptr, 2))) return rc;
+        if ((rc = pme_potential(c, 2))) return rc;
+        if ((rc = gather_frac(c, 2))) return rc;
+    }
     PolGradPostArgs args;
     args.n = c->n; args.npad = c->npad; args.g = geom(c);
     args.smp = c->smp.p;

@@ -793,7 +793,15 @@ int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
     if (prec && (rc = pair_matrix_precondition(c, a, c->pfield.p))) return rc;
     if ((rc = mutual_field(c, c->uind.p, c->pvec.p))) return rc;
     hipLaunchKernelGGL(k_pcg_init, dim3(gb), dim3(PCG_TB), 0, c->stream, a, c->pfield.p, c->pvec.p);
-    if (prec && (rc = pair_matrix_precondition(c, a))) return rc;
+    // with PME and the sparse preconditioner, the preconditioner's launches
+    // also carry the reciprocal potential grids of p and u by linearity, so
+    // the gradient's reciprocal part needs no spreading or transform of u
+    void *qg_, *gp_, *gu_;
+    size_t pts_;
+    bool g32_;
+    const bool carry = prec && pme_carry_grids(c, &qg_, &gp_, &gu_, &pts_, &g32_);
+    c->phi_carried = false;
+    if (prec && (rc = pair_matrix_precondition(c, a, nullptr, carry))) return rc;
     PMQ_HIP(hipGetLastError());
     const bool spec = !safe && c->pcg_guess_iters > 0;
     int batch = spec ? std::min(c->pcg_guess_iters + 1, c->prm.politer) : 1;
@@ -813,7 +821,7 @@ int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
             // ending in the step E1 (which records its block count), then E2
             if ((rc = mutual_field(c, a.z, c->pvec.p, a.sc + PCG_DONE, &a))) return rc;
             if (prec) {
-                if ((rc = pair_matrix_precondition(c, a))) return rc;
+                if ((rc = pair_matrix_precondition(c, a, nullptr, carry))) return rc;
             } else {
                 hipLaunchKernelGGL(k_pcg_step, dim3(gb), dim3(PCG_TB), 0, c->stream, a);
             }
@@ -823,6 +831,7 @@ int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
         PMQ_HIP(hipGetLastError());
         if (spec) {
             *iters_out = 0; // known after the status check
+            c->phi_carried = carry;
             return PMQ_OK;
         }
         PMQ_HIP(hipMemcpyAsync(sc, c->pcg.p, sizeof sc, hipMemcpyDeviceToHost, c->stream));
@@ -832,6 +841,7 @@ int polar_solve(pmq_ctx *c, bool safe, int *iters_out)
     const int iters = done ? (int)sc[PCG_ITERS] : it;
     if (done) c->pcg_guess_iters = iters;
     *iters_out = done ? iters : -iters;
+    c->phi_carried = carry;
     return PMQ_OK;
 }
 

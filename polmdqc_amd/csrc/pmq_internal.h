@@ -438,6 +438,8 @@ struct pmq_ctx {
     pmq::DBuf<double> pme_vsrc;                  // VIRIAL: 10 x npad multipoles plus both induced-dipole sets
     double pme_dalpha = 0.0;                     // dispersion Ewald coefficient pme_gd was built for
     pmq::DBuf<double> qgrid;                     // 2 real grids
+    pmq::DBuf<double> phi_acc;                   // the solve's carried potential grids: p then u, 2 sets each
+    bool phi_carried = false;                    // phi_acc's u grids hold the potential of this solve's dipoles
     pmq::DBuf<double> cgrid;                     // 2 complex half grids (interleaved re, im)
     pmq::DBuf<double> pme_frac;                  // 2 x 20 fractional potential derivatives per atom
     int pme_nb[3] = {1, 1, 1};                   // spreading bricks per axis
@@ -561,8 +563,13 @@ enum { PCG_APPLY_NONE = 0, PCG_APPLY_CHECK = 1, PCG_APPLY_E1 = 2 };
 int pair_matrix_apply(pmq_ctx *c, double *dst, const double *skip = nullptr, PcgVec *pcg = nullptr,
                       int mode = PCG_APPLY_NONE);
 int pair_matrix_prec_blocks(const pmq_ctx *c);
-// E non-null: the solve's first guess a E + max(a, 1e-8) T_s (a E) instead
-int pair_matrix_precondition(pmq_ctx *c, const PcgVec &pcg, const double *E = nullptr);
+// E non-null: the solve's first guess a E + max(a, 1e-8) T_s (a E) instead;
+// carry: also fold the reciprocal potential grids of z into those of p and u
+// (pme_carry_grids)
+int pair_matrix_precondition(pmq_ctx *c, const PcgVec &pcg, const double *E = nullptr, bool carry = false);
+// the reciprocal potential grid of z (qgrid) and the carried grids of p and u
+// with their element type, when PME is set up (false otherwise)
+bool pme_carry_grids(pmq_ctx *c, void **qg, void **gp, void **gu, size_t *pts, bool *g32);
 // device addresses of the batch scratch when it lives outside the context
 struct BatchScratch {
     Totals *tot;
