@@ -410,12 +410,15 @@ PMQ_API int pmq_set_gather_lanes(pmq_ctx *ctx, int mode);
 PMQ_API int pmq_set_pair_form(pmq_ctx *ctx, int form);
 
 /* HIP graphs for the evaluations of one system (no counterpart in the
- * reference, which has no GPU path).  On (the default): once an evaluation
- * runs without host round trips, the second one with the same mode, terms,
- * precision, box and PCG batch is captured into a graph and later ones
- * replay it; any setter other than pmq_set_coords with an unchanged box,
- * and any evaluation that has to size buffers, starts over.  Off: every
- * evaluation is enqueued launch by launch.  Results are the same. */
+ * reference, which has no GPU path).  On: once an evaluation runs without
+ * host round trips, the second one with the same mode, terms, precision,
+ * box and PCG batch is captured into a graph and later ones replay it; any
+ * setter other than pmq_set_coords with an unchanged box, and any
+ * evaluation that has to size buffers, starts over.  Off (the default):
+ * every evaluation is enqueued launch by launch, which is faster once the
+ * permanent multipoles' reciprocal part runs on a second stream beside the
+ * pair kernels (a graph with that branch replays more slowly than the same
+ * launches enqueued on two streams).  Results are the same. */
 PMQ_API int pmq_set_graphs(pmq_ctx *ctx, int enable);
 
 #ifdef __cplusplus

@@ -62,7 +62,18 @@ static void destroy_plans(pmq_ctx *c)
     }
 }
 
-void pme_release(pmq_ctx *c) { destroy_plans(c); }
+void pme_release(pmq_ctx *c)
+{
+    destroy_plans(c);
+    if (c->pme_side) {
+        (void)hipStreamSynchronize(c->pme_side);
+        (void)hipStreamDestroy(c->pme_side);
+        c->pme_side = nullptr;
+    }
+    if (c->pme_fork) (void)hipEventDestroy(c->pme_fork);
+    if (c->pme_join) (void)hipEventDestroy(c->pme_join);
+    c->pme_fork = c->pme_join = nullptr;
+}
 
 // brick partition of one grid axis for spreading: bricks at least
 // max(4, order - 1) points wide
@@ -184,6 +195,10 @@ int pme_setup(pmq_ctx *c)
     const size_t nreal = (size_t)K[0] * K[1] * K[2], ncplx = (size_t)H0 * K[1] * K[2];
     PMQ_HIP(c->qgrid.alloc(2 * nreal));
     PMQ_HIP(c->cgrid.alloc(2 * 2 * ncplx));
+    // the second stream of pme_mpole_fields (never created under capture)
+    if (!c->pme_side) PMQ_HIP(hipStreamCreateWithFlags(&c->pme_side, hipStreamNonBlocking));
+    if (!c->pme_fork) PMQ_HIP(hipEventCreateWithFlags(&c->pme_fork, hipEventDisableTiming));
+    if (!c->pme_join) PMQ_HIP(hipEventCreateWithFlags(&c->pme_join, hipEventDisableTiming));
     if (!same_grid || !c->plan_f1) {
         destroy_plans(c);
         int rc;
@@ -1185,15 +1200,57 @@ static int choose_fft(pmq_ctx *c)
     return PMQ_OK;
 }
 
-int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar)
+// the permanent multipoles' fractional potential (spread, transform,
+// gather into pme_frac).  side: on pme_side, forked from the evaluation's
+// stream here and joined by pme_mpole, so that it runs beside the
+// real-space pair kernels launched in between (none of which touches the
+// grids or pme_frac); the FFT engine's stream follows
+int pme_mpole_fields(pmq_ctx *c, bool side)
 {
     int rc;
+    c->pme_forked = false;
     if ((rc = choose_fft(c))) return rc;
     if ((rc = pme_brick_ranges(c))) return rc;
     const double *charge = c->prm.pentype != PMQ_PEN_NONE ? c->scp.p + 2 * (size_t)c->npad : nullptr;
-    if ((rc = spread(c, 0, c->smp.p, charge, 1))) return rc;
-    if ((rc = pme_potential(c, 1))) return rc;
-    if ((rc = gather_frac(c, 1))) return rc;
+    side = side && !c->profiling;
+    hipStream_t main = c->stream;
+    side = side && c->pme_side && c->pme_fork && c->pme_join;
+    if (side) {
+        PMQ_HIP(hipEventRecord(c->pme_fork, main));
+        PMQ_HIP(hipStreamWaitEvent(c->pme_side, c->pme_fork, 0));
+        c->stream = c->pme_side;
+        if (c->fft_info && (rc = fft_check(c, rocfft_execution_info_set_stream((rocfft_execution_info)c->fft_info, c->pme_side), "stream"))) {
+            c->stream = main;
+            return rc;
+        }
+    }
+    if (!(rc = spread(c, 0, c->smp.p, charge, 1)) && !(rc = pme_potential(c, 1))) rc = gather_frac(c, 1);
+    if (side) {
+        c->stream = main;
+        if (c->fft_info) {
+            const int r2 = fft_check(c, rocfft_execution_info_set_stream((rocfft_execution_info)c->fft_info, main), "stream");
+            if (!rc) rc = r2;
+        }
+        const hipError_t e = hipEventRecord(c->pme_join, c->pme_side);
+        if (!rc && e != hipSuccess) {
+            c->err = hipGetErrorString(e);
+            rc = PMQ_ERR_HIP;
+        }
+        c->pme_forked = true;
+    }
+    return rc;
+}
+
+int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar)
+{
+    int rc;
+    if (c->pme_forked) {
+        c->pme_forked = false;
+        PMQ_HIP(hipStreamWaitEvent(c->stream, c->pme_join, 0));
+    } else if ((rc = pme_mpole_fields(c, false))) {
+        return rc;
+    }
+    const double *charge = c->prm.pentype != PMQ_PEN_NONE ? c->scp.p + 2 * (size_t)c->npad : nullptr;
     MpolePostArgs args;
     args.n = c->n; args.npad = c->npad; args.g = geom(c);
     args.smp = c->smp.p; args.charge = charge;

@@ -862,6 +862,9 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
         // permanent field and the scaled pairs cleared from the tile masks
         if ((rc = launch_rotpole(c, true, do_polar, true))) return rc;
         if ((do_rep || do_disp || do_ct) && (rc = hippo_sort(c))) return rc;
+        // the permanent reciprocal potential on a second stream, joined
+        // before its energy and fields are taken (pme_mpole)
+        if (pme && ewald && (do_mpole || do_polar) && (rc = pme_mpole_fields(c, true))) return rc;
         if (do_rep) {
             const double rc2 = std::min(c->prm.repcut * c->prm.repcut, off2);
             if ((rc = launch_rep(c, rc2, grad, mixed, vir))) return rc;

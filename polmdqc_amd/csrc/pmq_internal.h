@@ -406,6 +406,11 @@ struct pmq_ctx {
     bool b_merged[2] = {false, false};           // tuned: all clusters in one 256-row launch
     hipStream_t b_side[PMQ_BATCH_BUCKETS - 1] = {}; // the other buckets' streams (created by pmq_set_batch)
     hipEvent_t b_fork = nullptr, b_join[PMQ_BATCH_BUCKETS - 1] = {};
+    // the permanent multipoles' reciprocal potential beside the real-space
+    // pair kernels (default priority; created with the PME grids)
+    hipStream_t pme_side = nullptr;
+    hipEvent_t pme_fork = nullptr, pme_join = nullptr;
+    bool pme_forked = false; // pme_mpole_fields went out on pme_side
     bool b_pin[2][PMQ_BATCH_BUCKETS] = {{false, false, false}, {false, false, false}}; // pmq_set_batch_shape
     bool b_pgrad = false;                        // batched GRADIENT evaluations also give parameter gradients
     bool b_pg_valid = false;                     // pgo holds the last batched evaluation's
@@ -420,7 +425,7 @@ struct pmq_ctx {
         pmq::DevBox box;
     };
     unsigned long long epoch = 0;                // bumped by whatever changes launches or their arguments
-    bool graphs_on = true;                       // pmq_set_graphs
+    bool graphs_on = false;                      // pmq_set_graphs
     EvalKey e_last{}, e_key{};                   // last eager speculative evaluation; the captured one
     bool e_last_ok = false, e_nocapture = false; // e_last is set; capturing e_last failed
     void *e_graph = nullptr, *e_exec = nullptr;
@@ -530,6 +535,7 @@ int launch_ct(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir);
 // sites; run before the other terms build theirs
 int vdw_pass(pmq_ctx *c, bool grad, bool mixed, bool vir, bool safe);
 void pme_release(pmq_ctx *c);
+int pme_mpole_fields(pmq_ctx *c, bool side);
 int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar);
 // VIRIAL: the k-space sums of the multipole and polarization reciprocal
 // energies (after the solve; FP64 grids whatever the precision form)

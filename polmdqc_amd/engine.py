@@ -353,7 +353,8 @@ def set_pair_form(eng: Engine, form: int = PAIR_CARTESIAN):
 
 
 def set_graphs(eng: Engine, enable: bool = True):
-    """HIP graphs for repeated evaluations of one system (on by default)."""
+    """HIP graphs for repeated evaluations of one system (off by default:
+    plain enqueueing on two streams is faster)."""
     L = eng._L
     L.pmq_set_graphs.argtypes = [C.c_void_p, C.c_int]
     L.pmq_set_graphs.restype = C.c_int

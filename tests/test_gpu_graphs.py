@@ -1,4 +1,4 @@
-"""Evaluations of one system replayed from a HIP graph (pmq_set_graphs, on
+"""Evaluations of one system replayed from a HIP graph (pmq_set_graphs, off
 by default) against the same evaluations enqueued launch by launch.
 
 After the safe first evaluation, the second is enqueued plainly, the third
