@@ -1200,11 +1200,52 @@ static int choose_fft(pmq_ctx *c)
     return PMQ_OK;
 }
 
+// fn's launches on pme_side, forked from the evaluation's stream here and
+// joined by side_join, so that they run beside what the evaluation's stream
+// takes in between (which must not touch what fn reads or writes); the FFT
+// engine's stream follows.  Inline on the evaluation's stream when side is
+// false or profiling is on (the library's HIP-event timings assume one
+// stream).
+template <class F> static int side_launch(pmq_ctx *c, bool side, F &&fn)
+{
+    c->pme_forked = false;
+    side = side && !c->profiling && c->pme_side && c->pme_fork && c->pme_join;
+    if (!side) return fn();
+    int rc;
+    hipStream_t main = c->stream;
+    PMQ_HIP(hipEventRecord(c->pme_fork, main));
+    PMQ_HIP(hipStreamWaitEvent(c->pme_side, c->pme_fork, 0));
+    c->stream = c->pme_side;
+    rc = c->fft_info ? fft_check(c, rocfft_execution_info_set_stream((rocfft_execution_info)c->fft_info, c->pme_side),
+                                 "stream")
+                     : PMQ_OK;
+    if (!rc) rc = fn();
+    c->stream = main;
+    if (c->fft_info) {
+        const int r2 = fft_check(c, rocfft_execution_info_set_stream((rocfft_execution_info)c->fft_info, main), "stream");
+        if (!rc) rc = r2;
+    }
+    const hipError_t e = hipEventRecord(c->pme_join, c->pme_side);
+    if (!rc && e != hipSuccess) {
+        c->err = hipGetErrorString(e);
+        rc = PMQ_ERR_HIP;
+    }
+    c->pme_forked = true;
+    return rc;
+}
+
+// the evaluation's stream waits for the launches side_launch forked
+static int side_join(pmq_ctx *c)
+{
+    if (!c->pme_forked) return PMQ_OK;
+    c->pme_forked = false;
+    PMQ_HIP(hipStreamWaitEvent(c->stream, c->pme_join, 0));
+    return PMQ_OK;
+}
+
 // the permanent multipoles' fractional potential (spread, transform,
-// gather into pme_frac).  side: on pme_side, forked from the evaluation's
-// stream here and joined by pme_mpole, so that it runs beside the
-// real-space pair kernels launched in between (none of which touches the
-// grids or pme_frac); the FFT engine's stream follows
+// gather into pme_frac); side: beside the real-space pair kernels launched
+// between here and pme_mpole (none of them touches the grids or pme_frac)
 int pme_mpole_fields(pmq_ctx *c, bool side)
 {
     int rc;
@@ -1212,41 +1253,18 @@ int pme_mpole_fields(pmq_ctx *c, bool side)
     if ((rc = choose_fft(c))) return rc;
     if ((rc = pme_brick_ranges(c))) return rc;
     const double *charge = c->prm.pentype != PMQ_PEN_NONE ? c->scp.p + 2 * (size_t)c->npad : nullptr;
-    side = side && !c->profiling;
-    hipStream_t main = c->stream;
-    side = side && c->pme_side && c->pme_fork && c->pme_join;
-    if (side) {
-        PMQ_HIP(hipEventRecord(c->pme_fork, main));
-        PMQ_HIP(hipStreamWaitEvent(c->pme_side, c->pme_fork, 0));
-        c->stream = c->pme_side;
-        if (c->fft_info && (rc = fft_check(c, rocfft_execution_info_set_stream((rocfft_execution_info)c->fft_info, c->pme_side), "stream"))) {
-            c->stream = main;
-            return rc;
-        }
-    }
-    if (!(rc = spread(c, 0, c->smp.p, charge, 1)) && !(rc = pme_potential(c, 1))) rc = gather_frac(c, 1);
-    if (side) {
-        c->stream = main;
-        if (c->fft_info) {
-            const int r2 = fft_check(c, rocfft_execution_info_set_stream((rocfft_execution_info)c->fft_info, main), "stream");
-            if (!rc) rc = r2;
-        }
-        const hipError_t e = hipEventRecord(c->pme_join, c->pme_side);
-        if (!rc && e != hipSuccess) {
-            c->err = hipGetErrorString(e);
-            rc = PMQ_ERR_HIP;
-        }
-        c->pme_forked = true;
-    }
-    return rc;
+    return side_launch(c, side, [&]() {
+        int r;
+        if (!(r = spread(c, 0, c->smp.p, charge, 1)) && !(r = pme_potential(c, 1))) r = gather_frac(c, 1);
+        return r;
+    });
 }
 
 int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar)
 {
     int rc;
     if (c->pme_forked) {
-        c->pme_forked = false;
-        PMQ_HIP(hipStreamWaitEvent(c->stream, c->pme_join, 0));
+        if ((rc = side_join(c))) return rc;
     } else if ((rc = pme_mpole_fields(c, false))) {
         return rc;
     }
@@ -1471,23 +1489,37 @@ bool pme_carry_grids(pmq_ctx *c, void **qg, void **gp, void **gu, size_t *pts, b
     return true;
 }
 
+// the induced dipoles' fractional potential (gather into pme_frac, from
+// the grids the solve carried when it did, else spread and transform).
+// Measured on the second stream beside the real-space gradient tiles
+// (side = true): slower (957 -> 909 evals/s mixed on the 12,288-atom box),
+// the gather being too short to pay for the fork and join
+static int pme_polar_grad_fields(pmq_ctx *c, bool side)
+{
+    c->pme_forked = false;
+    void *qg, *gp, *gu = nullptr;
+    size_t pts;
+    bool g32;
+    if (c->phi_carried && !pme_carry_grids(c, &qg, &gp, &gu, &pts, &g32)) {
+        c->err = "PME: the carried potential grids are gone";
+        return PMQ_ERR_STATE;
+    }
+    return side_launch(c, side, [&]() {
+        // with carried grids, the potential of the solve's dipoles (k_pcg_prec)
+        if (gu) return gather_frac(c, 2, gu);
+        int r;
+        if (!(r = spread(c, 1, c->uind.p, nullptr, 2)) && !(r = pme_potential(c, 2))) r = gather_frac(c, 2);
+        return r;
+    });
+}
+
 int pme_polar_grad(pmq_ctx *c, bool vir)
 {
     int rc;
-    if (c->phi_carried) {
-        // the solve carried the potential grids of its dipoles (k_pcg_prec)
-        void *qg, *gp, *gu;
-        size_t pts;
-        bool g32;
-        if (!pme_carry_grids(c, &qg, &gp, &gu, &pts, &g32)) {
-            c->err = "PME: the carried potential grids are gone";
-            return PMQ_ERR_STATE;
-        }
-        if ((rc = gather_frac(c, 2, gu))) return rc;
-    } else {
-        if ((rc = spread(c, 1, c->uind.p, nullptr, 2))) return rc;
-        if ((rc = pme_potential(c, 2))) return rc;
-        if ((rc = gather_frac(c, 2))) return rc;
+    if (c->pme_forked) {
+        if ((rc = side_join(c))) return rc;
+    } else if ((rc = pme_polar_grad_fields(c, false))) {
+        return rc;
     }
     PolGradPostArgs args;
     args.n = c->n; args.npad = c->npad; args.g = geom(c);
