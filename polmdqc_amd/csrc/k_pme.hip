@@ -1244,30 +1244,34 @@ static int side_join(pmq_ctx *c)
 }
 
 // the permanent multipoles' fractional potential (spread, transform,
-// gather into pme_frac); side: beside the real-space pair kernels launched
-// between here and pme_mpole (none of them touches the grids or pme_frac)
-int pme_mpole_fields(pmq_ctx *c, bool side)
+// gather into pme_frac), and with disp the dispersion potential after it
+// (spread and transform, left in qgrid for pme_disp's gather); side: beside
+// the real-space pair kernels launched between here and pme_mpole (none of
+// them touches the grids or pme_frac)
+int pme_mpole_fields(pmq_ctx *c, bool side, bool disp)
 {
     int rc;
     c->pme_forked = false;
     if ((rc = choose_fft(c))) return rc;
     if ((rc = pme_brick_ranges(c))) return rc;
     const double *charge = c->prm.pentype != PMQ_PEN_NONE ? c->scp.p + 2 * (size_t)c->npad : nullptr;
-    return side_launch(c, side, [&]() {
+    rc = side_launch(c, side, [&]() {
         int r;
         if (!(r = spread(c, 0, c->smp.p, charge, 1)) && !(r = pme_potential(c, 1))) r = gather_frac(c, 1);
+        if (!r && disp && !(r = spread(c, 2, c->shp.p, nullptr, 1))) r = pme_potential(c, 1, c->pme_gd.p);
         return r;
     });
+    c->mpole_frac_ready = rc == PMQ_OK;
+    c->disp_pot_ready = rc == PMQ_OK && disp;
+    return rc;
 }
 
 int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar)
 {
     int rc;
-    if (c->pme_forked) {
-        if ((rc = side_join(c))) return rc;
-    } else if ((rc = pme_mpole_fields(c, false))) {
-        return rc;
-    }
+    if ((rc = side_join(c))) return rc;
+    if (!c->mpole_frac_ready && (rc = pme_mpole_fields(c, false))) return rc;
+    c->mpole_frac_ready = false;
     const double *charge = c->prm.pentype != PMQ_PEN_NONE ? c->scp.p + 2 * (size_t)c->npad : nullptr;
     MpolePostArgs args;
     args.n = c->n; args.npad = c->npad; args.g = geom(c);
@@ -1446,8 +1450,14 @@ int pme_disp(pmq_ctx *c, bool grad, bool need_bin, bool vir)
     int rc;
     if ((rc = choose_fft(c))) return rc;
     if (need_bin && (rc = pme_brick_ranges(c))) return rc;
-    if ((rc = spread(c, 2, c->shp.p, nullptr, 1))) return rc;
-    if ((rc = pme_potential(c, 1, c->pme_gd.p))) return rc;
+    if (c->disp_pot_ready) {
+        // computed after the multipoles' potential (pme_mpole_fields)
+        if ((rc = side_join(c))) return rc;
+        c->disp_pot_ready = false;
+    } else {
+        if ((rc = spread(c, 2, c->shp.p, nullptr, 1))) return rc;
+        if ((rc = pme_potential(c, 1, c->pme_gd.p))) return rc;
+    }
     DispPostArgs args;
     args.n = c->n; args.npad = c->npad; args.g = geom(c);
     args.sx = c->sx.p; args.sy = c->sy.p; args.sz = c->sz.p; args.cs = c->shp.p;

@@ -853,6 +853,7 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
         const bool pme = (ewald && (do_mpole || do_polar)) || dewald;
         if (pme && (rc = pme_setup(c))) return rc;
         c->pme_cells = pme;
+        c->mpole_frac_ready = c->disp_pot_ready = false;
         {
             ProfScope ps(c, "neighbors");
             if ((rc = build_neighbors(c, cut))) return rc;
@@ -864,7 +865,7 @@ static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, 
         if ((do_rep || do_disp || do_ct) && (rc = hippo_sort(c))) return rc;
         // the permanent reciprocal potential on a second stream, joined
         // before its energy and fields are taken (pme_mpole)
-        if (pme && ewald && (do_mpole || do_polar) && (rc = pme_mpole_fields(c, true))) return rc;
+        if (pme && ewald && (do_mpole || do_polar) && (rc = pme_mpole_fields(c, true, dewald))) return rc;
         if (do_rep) {
             const double rc2 = std::min(c->prm.repcut * c->prm.repcut, off2);
             if ((rc = launch_rep(c, rc2, grad, mixed, vir))) return rc;

@@ -410,7 +410,9 @@ struct pmq_ctx {
     // pair kernels (default priority; created with the PME grids)
     hipStream_t pme_side = nullptr;
     hipEvent_t pme_fork = nullptr, pme_join = nullptr;
-    bool pme_forked = false; // pme_mpole_fields went out on pme_side
+    bool pme_forked = false;       // launches went out on pme_side, not joined yet
+    bool mpole_frac_ready = false; // pme_mpole_fields ran in this evaluation
+    bool disp_pot_ready = false;   // ... and the dispersion potential after it
     bool b_pin[2][PMQ_BATCH_BUCKETS] = {{false, false, false}, {false, false, false}}; // pmq_set_batch_shape
     bool b_pgrad = false;                        // batched GRADIENT evaluations also give parameter gradients
     bool b_pg_valid = false;                     // pgo holds the last batched evaluation's
@@ -535,7 +537,7 @@ int launch_ct(pmq_ctx *c, double off2, bool grad, bool mixed, bool vir);
 // sites; run before the other terms build theirs
 int vdw_pass(pmq_ctx *c, bool grad, bool mixed, bool vir, bool safe);
 void pme_release(pmq_ctx *c);
-int pme_mpole_fields(pmq_ctx *c, bool side);
+int pme_mpole_fields(pmq_ctx *c, bool side, bool disp = false);
 int pme_mpole(pmq_ctx *c, int mode, bool mpole, bool polar);
 // VIRIAL: the k-space sums of the multipole and polarization reciprocal
 // energies (after the solve; FP64 grids whatever the precision form)
