@@ -785,10 +785,28 @@ static int check_compute(pmq_ctx *c, int mode, unsigned terms, int precision)
     return PMQ_OK;
 }
 
+static int compute_enqueue_body(pmq_ctx *c, int mode, unsigned terms, int precision, bool safe);
+
 // enqueue one evaluation; `safe` sizes the tile and pair-matrix buffers and
 // runs the PCG with host round trips, otherwise nothing waits for the device
-// and the run is checked afterwards (pmq_check)
+// and the run is checked afterwards (pmq_check).  Launches forked onto the
+// PME side stream are joined back into the context's stream also when the
+// evaluation stops early, so that waiting on that stream covers them.
 static int compute_enqueue(pmq_ctx *c, int mode, unsigned terms, int precision, bool safe)
+{
+    const int rc = compute_enqueue_body(c, mode, terms, precision, safe);
+    if (c->pme_forked) {
+        c->pme_forked = false;
+        const hipError_t e = hipStreamWaitEvent(c->stream, c->pme_join, 0);
+        if (!rc && e != hipSuccess) {
+            c->err = hipGetErrorString(e);
+            return PMQ_ERR_HIP;
+        }
+    }
+    return rc;
+}
+
+static int compute_enqueue_body(pmq_ctx *c, int mode, unsigned terms, int precision, bool safe)
 {
     const bool mixed = precision == PMQ_MIXED;
     const bool do_mpole = terms & PMQ_TERM_MPOLE, do_polar = terms & PMQ_TERM_POLAR;
