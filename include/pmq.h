@@ -392,7 +392,7 @@ PMQ_API int pmq_fft_choice(pmq_ctx *ctx, int *fp64_direct, int *fp32_grids, int 
 
 /* PME gather lanes.  No counterpart in the reference; for tuning and tests.
  * PMQ_GATHER_AUTO (the default) gives each atom 8 lanes, one stencil plane
- * each, from 32,768 atoms on and one lane per stencil row below; the other
+ * each, from 12,288 atoms on and one lane per stencil row below; the other
  * two values force one layout. */
 #define PMQ_GATHER_AUTO 0
 #define PMQ_GATHER_ROWS 1

@@ -1032,7 +1032,7 @@ static size_t grid_size(const pmq_ctx *c) { return (size_t)c->K[0] * c->K[1] * c
 // lanes per atom in the gathers: 8 (one stencil plane per lane, the weights
 // formed once per plane) when the system is large enough to fill the chip
 // that way, else one stencil row per lane (ORD^2 rows in 32 or 64 lanes)
-constexpr int PMQ_GATHER_PLANE_MIN = 32768;
+constexpr int PMQ_GATHER_PLANE_MIN = 12288;
 constexpr int gather_row_lanes(int ord) { return ord * ord <= 32 ? 32 : 64; }
 static dim3 lanes_grid(int n, int lanes) { return dim3(xcd_grid(((size_t)n * lanes + 255) / 256)); }
 
